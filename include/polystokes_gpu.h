@@ -1,0 +1,198 @@
+/*
+ * polystokes_gpu.h -- C ABI of the MI355X (gfx950) per-timestep hydrodynamic solve of PolyStokes.
+ *
+ * The library (polystokes_brownian_amd/libpsgpu.so) holds one simulation context per handle, keeps
+ * every per-step array resident in HBM, and runs hand-written HIP kernels for the FTS grand-mobility
+ * tiles, the saddle-point operator, MINRES, the Lanczos square root, the Verlet pairs, the forces and
+ * the integrator.  Every entry point returns 0 on success and a positive ps_err code otherwise; the
+ * message of the last failure is read with ps_last_error().  The library never aborts the process
+ * (the reference installs PetscAbortErrorHandler, src/init.cpp:367; this is a deliberate difference).
+ *
+ * Pointers are host pointers unless the name ends in _dev, in which case they are device pointers
+ * ordered on the context's stream.  One context must not be used from two host threads at once.
+ *
+ * Vector layouts follow the reference exactly (src/Stokes.cpp:67-98, src/mob.cpp:586-591):
+ *   mobility vector (n  = 3Nm + 11Nc):  [ f_m (3Nm) ; F_c (3Nc) ; T_c (3Nc) ; S_c (5Nc) ]
+ *   saddle vector   (ns = 6Nm + 17Nc):  [ f (n) ; u (3Nm + 6Nc) ]
+ */
+#ifndef POLYSTOKES_GPU_H
+#define POLYSTOKES_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int ps_err;
+enum {
+    PS_OK = 0,
+    PS_ERR_ARG = 1,      /* invalid argument or size */
+    PS_ERR_HIP = 2,      /* a HIP runtime call failed */
+    PS_ERR_STATE = 3,    /* call out of order (e.g. solve before mob) */
+    PS_ERR_NOCONV = 4,   /* a Krylov solve hit its iteration cap (result still written) */
+    PS_ERR_ALLOC = 5,    /* device allocation failed */
+    PS_ERR_NODEV = 6     /* no gfx950 device visible */
+};
+
+/* Arithmetic forms of the mobility apply. */
+enum {
+    PS_PREC_FP64 = 0,    /* tiles and applies in FP64 (parity form) */
+    PS_PREC_MIXED = 1    /* FP32 tiles/applies, FP64 recurrences, FP64 iterative refinement */
+};
+
+/* Excluded-volume force law, src/rhs.cpp:194-199 (POLYSTOKES_FORCE). */
+enum { PS_FORCE_HS = 0, PS_FORCE_HS_EXP = 1 };
+
+/*
+ * Parameters.  The first three groups are the reference's pybind11 constructor
+ * (src/bindings.cpp:363-387), particle_info (src/Stokes.cpp:171) and trap_info
+ * (src/Stokes.cpp:241), with the same meaning and defaults (ps_params_default).
+ */
+typedef struct ps_params {
+    /* PolyStokes(dt, samplerate, tmax, output_dir, mm_HI, chain_HI, fene, record_forces, tether,
+     * mono_ev, box, t, seed, restart) */
+    double dt;
+    int samplerate;
+    double tmax;
+    int mm_HI, chain_HI, fene, record_forces, tether, mono_ev;
+    int box_active;               /* 0: unbounded; 1: orthorhombic periodic box */
+    double box[3];
+    double t;
+    uint64_t seed;
+    double restart;
+    /* particle_info(kT, epsilon, Np, Nc, Nm, Npoly, Nmono_per_chain, beta, kbond, r0, Lmax, tau) */
+    double kT, epsilon;
+    int Np, Nc, Nm, Npoly, Nmono_per_chain;
+    double beta, kbond, r0, Lmax, tau;
+    /* trap_info(ktrap, tstart, trun, weaken_trap) */
+    double ktrap, tstart, trun, weaken_trap;
+    /* Saddle solve (KSPMINRES + PCJACOBI, src/init.cpp:40-64).  The reference hard-codes
+     * rtol = 1e-6 after KSPSetFromOptions; here it is a parameter (default 1e-6). */
+    double ksp_rtol, ksp_abstol;
+    int ksp_maxit;
+    /* Brownian square root M^{1/2} xi by Lanczos (replaces SLEPc MFNSolve, src/slip_vel.cpp:27-49):
+     * stop when the relative change of the estimate drops below sqrt_rtol, at sqrt_kmax vectors,
+     * or at an invariant subspace (beta_j <= 1e-12 beta_0, src/slip_vel.cpp:417). */
+    double sqrt_rtol;
+    int sqrt_kmax;
+    /* Environment knobs of the reference: POLYSTOKES_FORCE, HS_KPREF, HS_KMODE, HS_EXP_K
+     * (src/rhs.cpp:194, src/pair_interaction.cpp:108-110,227). */
+    int force_sel;
+    double hs_kpref;
+    int hs_kmode;
+    double hs_exp_k;
+    /* MI355X forms (no reference counterpart). */
+    int precision;                /* PS_PREC_FP64 or PS_PREC_MIXED */
+    double ir_rtol;               /* mixed form: FP64 refinement target on the saddle residual */
+    int ir_maxit;                 /* mixed form: cap on refinement sweeps */
+} ps_params;
+
+typedef struct ps_solve_info {
+    int iterations;      /* MINRES iterations (summed over refinement sweeps in the mixed form) */
+    int reason;          /* >0 converged (2 = rtol, 3 = abstol), <0 diverged (-3 = maxit), KSPConvergedReason signs */
+    double rnorm;        /* final preconditioned residual estimate */
+    double rnorm0;       /* initial preconditioned residual norm */
+    int refinements;     /* FP64 refinement sweeps (mixed form), 0 in the FP64 form */
+    double true_rel_residual; /* ||b - A x|| / ||b|| in FP64 (mixed form; -1 when not computed) */
+} ps_solve_info;
+
+typedef struct ps_sqrt_info {
+    int iterations;      /* Lanczos vectors used */
+    int nneg;            /* negative Ritz values floored to zero (src/slip_vel.cpp:450-459) */
+    double beta0;        /* ||xi|| */
+    double rel_change;   /* last relative change of the estimate (convergence measure) */
+} ps_sqrt_info;
+
+typedef struct ps_counters {
+    int64_t mob_calls, applies, minres_iterations, lanczos_iterations, saddle_solves, sqrt_calls;
+    int64_t nvlist;      /* Verlet pairs found by the last check_dist */
+    int64_t lanczos_nneg_events, lanczos_nneg_count;
+} ps_counters;
+
+/* Fill *p with the reference defaults (src/bindings.cpp:374-387; record_forces=true as bound). */
+void ps_params_default(ps_params* p);
+
+/* Create a context: alok_arrays + set_vars + init_solver + init_random (src/init.cpp:112-419),
+ * ~ PolyStokes ctor + particle_info + trap_info.  Allocates all device buffers. */
+ps_err ps_create(const ps_params* p, void** out_ctx);
+/* MATOP_DESTROY + cleanup (src/matfree_A.cpp:104-115, src/cleanup.cpp). */
+ps_err ps_destroy(void* ctx);
+const char* ps_last_error(const void* ctx);
+/* Device name and gfx arch of the context's GPU. */
+ps_err ps_device_info(void* ctx, char* name, int name_len, int* n_cu);
+
+/* initial_configuration (src/Stokes.cpp:254-317): x0 has 3Np entries; q0 has 4Nc entries or is NULL
+ * (identity quaternions).  Trap centres are the colloids' initial positions when restart == 0. */
+ps_err ps_initial_configuration(void* ctx, const double* x0, const double* q0);
+ps_err ps_set_positions(void* ctx, const double* x);
+/* Copy state back: x (3Np), q (4Nc), t; any pointer may be NULL. */
+ps_err ps_get_state(void* ctx, double* x, double* q, double* t);
+/* Copy per-step vectors back (any may be NULL): up, v_det_n, v_det_c, v_brown, udiff (3Nm+6Nc each),
+ * sb (5Nc), fext (3Nm+6Nc). */
+ps_err ps_get_vectors(void* ctx, double* up, double* v_det_n, double* v_det_c, double* v_brown,
+                      double* udiff, double* sb, double* fext);
+ps_err ps_get_counters(void* ctx, ps_counters* c);
+
+/* check_dist (src/check_dist.cpp:12-87): rebuild the Verlet list on the device with a cell list.
+ * Pairs (k, j), k < j, in the reference's enumeration (AA, AB, BB; src/init.cpp:225-291) enter
+ * vlist[k] iff dr < rverls[type] (float-rounded cutoffs, src/init.cpp:332).  Returns the count. */
+ps_err ps_check_dist(void* ctx, int64_t* n_pairs);
+/* Copy the Verlet list back as (row k, partner j, reference pair index) triples sorted by (k, j),
+ * i.e. exactly the order of vlist[k] in the reference.  cap is the capacity of each array. */
+ps_err ps_get_vlist(void* ctx, int32_t* rows, int32_t* partners, int64_t* pair_index, int64_t cap,
+                    int64_t* n_pairs);
+
+/* mob() (src/mob.cpp:635-1213): define the grand mobility M at the current positions.  The tiles
+ * are generated on the fly by every apply; this call snapshots the positions and the one 3x3 tile
+ * the reference carries over between pair loops (see ps_stale_tile). */
+ps_err ps_mob(void* ctx);
+/* The monomer-colloid translation block the reference writes for every AB pair: the mob_a left by
+ * the last processed AA pair of this mob() call (mobility_AB does not write mob_a,
+ * src/mob.cpp:407-453; initialize_rank2_array zeroes a copy, src/multi_arrays_initialize.cpp:5). */
+ps_err ps_stale_tile(void* ctx, double* tile9);
+
+/* u = M f (n entries), FP64 or the context's precision.  ~ the mobility block of MatMult(A). */
+ps_err ps_mob_mult(void* ctx, const double* f, double* u);
+ps_err ps_mob_mult_dev(void* ctx, const double* f_dev, double* u_dev);
+/* MATOP_MULT of the saddle operator A = [[M, B], [B^T, 0]], B = [-I_{3Nm+6Nc}; 0]
+ * (src/matfree_A.cpp:38-80, src/arrays.cpp:349-408): y = A x, ns entries. */
+ps_err ps_saddle_mult(void* ctx, const double* x, double* y);
+ps_err ps_saddle_mult_dev(void* ctx, const double* x_dev, double* y_dev);
+/* MATOP_GET_DIAGONAL with PCJacobiSetFixDiagonal (src/matfree_A.cpp:84-102, src/init.cpp:58-59):
+ * d = [diag(M) ; 1 ... 1]. */
+ps_err ps_saddle_diag(void* ctx, double* d);
+/* KSPSolve(ksp, b, x) with KSPMINRES + PCJACOBI (src/vel.cpp:41-51).  warm != 0 uses x as the
+ * initial guess (KSPSetInitialGuessNonzero).  Returns PS_ERR_NOCONV at the iteration cap. */
+ps_err ps_saddle_solve(void* ctx, const double* b, double* x, int warm, ps_solve_info* info);
+ps_err ps_saddle_solve_dev(void* ctx, const double* b_dev, double* x_dev, int warm, ps_solve_info* info);
+/* MFNSolve(mfn, xi, w) with FNSQRT on M (src/slip_vel.cpp:27-49): w = M^{1/2} xi by Lanczos with
+ * two-pass full reorthogonalisation and negative Ritz values floored (src/slip_vel.cpp:351-477). */
+ps_err ps_sqrt_apply(void* ctx, const double* xi, double* w, ps_sqrt_info* info);
+ps_err ps_sqrt_apply_dev(void* ctx, const double* xi_dev, double* w_dev, ps_sqrt_info* info);
+/* RHS(drift) (src/rhs.cpp:160-260): rebuild the saddle right-hand side on the device from the
+ * current Verlet list; rhs (ns entries) is copied back when non-NULL. */
+ps_err ps_rhs(void* ctx, int drift, double* rhs);
+/* PolyStokes::run() step loop (src/run.cpp:70-162) for nsteps steps, all device-resident.  Output
+ * writing is not performed. */
+ps_err ps_run(void* ctx, int nsteps);
+/* Kernel timing with HIP events on the context's stream (for roofline figures): between begin and
+ * end every monomer-monomer tile launch (the dominant kernel) and every whole mobility apply is
+ * bracketed by events; end synchronises and returns the summed durations. */
+typedef struct ps_profile {
+    int64_t aa_launches;  double aa_ms;      /* monomer-monomer tile kernel */
+    int64_t apply_launches; double apply_ms; /* whole apply (all tile kernels + finalize) */
+    int64_t aa_pairs;                        /* target x source monomer pairs per AA launch */
+} ps_profile;
+ps_err ps_profile_begin(void* ctx);
+ps_err ps_profile_end(void* ctx, ps_profile* out);
+
+/* Normal draws of the context's noise stream (std::mt19937_64 + std::normal_distribution<double>,
+ * src/init.cpp:105-110) -- the draws the next step would consume; advances the stream. */
+ps_err ps_draw_normals(void* ctx, double* out, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* POLYSTOKES_GPU_H */

@@ -1,0 +1,1270 @@
+// psoracle.cpp -- CPU FP64 restatement of PolyStokes' per-timestep hydrodynamic solve.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker the GPU library is compared against: only
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it, and never as the thing
+// measured or shipped.  The product path (polystokes_brownian_amd/libpsgpu.so) does not link it.
+//
+// Every function cites the reference file:line it restates.  Where the reference computes in long
+// double (boost::multi_array<long double>, src/multi_arrays.h:6-12) this restatement uses double;
+// the assembled entries differ by ~1 ulp, far inside the 1e-10 parity budget.  Compiled with
+// -ffp-contract=off so the pair geometry (check_dist) is bit-identical to the GPU kernels.
+//
+// Pinning (what checks each part, see tests/test_oracle.py):
+//   tiles      -- closed forms (numpy restatement of the tensor algebra from mobdat and src/mob.cpp),
+//                 symmetry, self constants, the reference fixture mob.csv/saddle.bin for layout.
+//   saddle     -- dense brute force A x; B = [-I; 0] placement.
+//   MINRES     -- the linear system itself (numpy.linalg.solve on the dense saddle matrix).
+//   Lanczos    -- numpy.linalg.eigh square root; invariant |M^1/2 xi|^2 = xi^T M xi.
+//   RNG        -- C++ standard known answer for mt19937_64; Python restatement of the polar method.
+//   pairs      -- brute-force O(N^2) enumeration in numpy with the same FP expression order.
+//   forces, integrator -- closed forms.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <random>
+#include <algorithm>
+#include <string>
+#include "../include/polystokes_gpu.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// Coefficients (src/Stokes.cpp:43-65).
+const double c1d2 = 1. / 2., c1d3 = 1. / 3., c1d4 = 1. / 4., c3d2 = 3. / 2., c3d4 = 3. / 4.,
+             c3d8 = 3. / 8., c5d6 = 5. / 6., c9d4 = 9. / 4., c9d5 = 9. / 5., c6d5 = 6. / 5.,
+             c9d8 = 9. / 8., c9d2 = 9. / 2., c9d10 = 9. / 10., c9d20 = 9. / 20., c9d32 = 9. / 32.,
+             c18d5 = 18. / 5., c36d5 = 36. / 5., c54d5 = 54. / 5.;
+
+// Levi-Civita and Kronecker tensors as set in mob() (src/mob.cpp:675-690).
+inline double EPS3(int i, int j, int k) {
+    if (i == j || j == k || i == k) return 0.0;
+    if ((i == 0 && j == 1 && k == 2) || (i == 1 && j == 2 && k == 0) || (i == 2 && j == 0 && k == 1))
+        return 1.0;
+    return -1.0;
+}
+inline double DEL(int i, int j) { return i == j ? 1.0 : 0.0; }
+
+// mesid (src/init.cpp:343-356).
+const int MESID[2][5] = {{0, 0, 0, 1, 1}, {2, 1, 2, 2, 2}};
+
+// kk = ndim - ii - jj with the cyclic fix-up used throughout src/mob.cpp (e.g. :186-189).
+inline int third(int a, int b) {
+    int k = 3 - a - b;
+    if (k == -1) k = 2;
+    if (k == 3) k = 0;
+    return k;
+}
+
+// ---------------------------------------------------------------- tiles
+struct Ee {
+    double e[3];
+    double ee[3][3];
+};
+inline void make_ee(const double e[3], Ee& t) {
+    for (int i = 0; i < 3; i++) t.e[i] = e[i];
+    for (int i = 0; i < 3; i++) {
+        t.ee[i][i] = e[i] * e[i];
+        for (int j = i + 1; j < 3; j++) {
+            t.ee[i][j] = e[i] * e[j];
+            t.ee[j][i] = t.ee[i][j];
+        }
+    }
+}
+
+// Monomer-monomer translation tile, mobility(..., AA=true) (src/mob.cpp:96-151).
+void tile_AA(double dr, double dr_inv, const double e[3], double beta, double a[3][3]) {
+    Ee t;
+    make_ee(e, t);
+    double dr_inv2 = dr_inv * dr_inv;
+    double dr_inv3 = dr_inv2 * dr_inv;
+    double x12a, y12a;
+    if (dr > 2.0 * beta) {
+        x12a = c3d8 * dr_inv - 2.0 * dr_inv3;
+        y12a = c3d4 * dr_inv - c1d2 * dr_inv3;
+    } else {
+        x12a = c3d8 * dr - 1.0;
+        y12a = 1.0 - c9d32 * dr;
+    }
+    for (int i = 0; i < 3; i++) {
+        a[i][i] = x12a * t.ee[i][i] + y12a * (1. - t.ee[i][i]);
+        for (int j = i + 1; j < 3; j++) {
+            a[i][j] = x12a * t.ee[i][j] - y12a * t.ee[i][j];
+            a[j][i] = a[i][j];
+        }
+    }
+}
+
+// gt (rank 3) and its flattening onto 3x5 (src/mob.cpp:198-227, :520-536).
+void gt_flat(const Ee& t, double x12g, double y12g, double out[3][5]) {
+    double gt[3][3][3];
+    for (int kk = 0; kk < 3; kk++)
+        for (int ii = 0; ii < 3; ii++)
+            for (int jj = 0; jj < 3; jj++)
+                gt[kk][ii][jj] = -x12g * (t.ee[ii][jj] - c1d3 * DEL(ii, jj)) * t.e[kk] +
+                                 y12g * (t.e[ii] * DEL(jj, kk) + t.e[jj] * DEL(ii, kk) -
+                                         2. * t.ee[ii][jj] * t.e[kk]);
+    for (int ii = 0; ii < 3; ii++) {
+        out[ii][0] = gt[ii][0][0] - gt[ii][2][2];
+        out[ii][1] = 2. * gt[ii][0][1];
+        out[ii][2] = 2. * gt[ii][0][2];
+        out[ii][3] = 2. * gt[ii][1][2];
+        out[ii][4] = gt[ii][1][1] - gt[ii][2][2];
+    }
+}
+
+// Rotation-force coupling b_ij = y12b eps_ijk e_k and its transpose (src/mob.cpp:183-196).
+void b_tiles(const Ee& t, double y12b, double b[3][3], double bt[3][3]) {
+    for (int ii = 0; ii < 3; ii++)
+        for (int jj = 0; jj < 3; jj++) {
+            int kk = third(ii, jj);
+            b[ii][jj] = y12b * EPS3(ii, jj, kk) * t.e[kk];
+            bt[jj][ii] = b[ii][jj];
+        }
+}
+
+// Monomer-colloid coupling, mobility_AB (src/mob.cpp:362-537).  mob_a is NOT written there
+// (:407-453 are commented out), so only b, bt, gt are produced.
+void tile_AB(double dr_inv, const double e[3], double beta2, double b[3][3], double bt[3][3],
+             double gt[3][5]) {
+    Ee t;
+    make_ee(e, t);
+    const double dr_inv2 = dr_inv * dr_inv;
+    const double dr_inv3 = dr_inv2 * dr_inv;
+    const double dr_inv4 = dr_inv3 * dr_inv;
+    const double y12b = -c3d4 * dr_inv2;
+    const double x12g = c9d4 * dr_inv2 - c9d20 * dr_inv4 * (3 + 5 * beta2);
+    const double y12g = c9d10 * dr_inv4 * (c1d2 + c5d6 * beta2);
+    b_tiles(t, y12b, b, bt);
+    gt_flat(t, x12g, y12g, gt);
+}
+
+struct BBTile {
+    double a[3][3], b[3][3], bt[3][3], c[3][3], gt[3][5], ht[3][5], m[5][5];
+};
+
+// Colloid-colloid FTS coupling, mobility(..., AB=false, AA=false) (src/mob.cpp:96-356).
+void tile_BB(double dr_inv, const double e[3], BBTile& T) {
+    Ee t;
+    make_ee(e, t);
+    double dr_inv2 = dr_inv * dr_inv;
+    double dr_inv3 = dr_inv2 * dr_inv;
+    double x12a = c3d2 * dr_inv - dr_inv3;
+    double y12a = c3d4 * dr_inv + c1d2 * dr_inv3;
+    for (int i = 0; i < 3; i++) {
+        T.a[i][i] = x12a * t.ee[i][i] + y12a * (1. - t.ee[i][i]);
+        for (int j = i + 1; j < 3; j++) {
+            T.a[i][j] = x12a * t.ee[i][j] - y12a * t.ee[i][j];
+            T.a[j][i] = T.a[i][j];
+        }
+    }
+    double dr_inv4 = dr_inv3 * dr_inv;
+    double dr_inv5 = dr_inv4 * dr_inv;
+    double y12b = -c3d4 * dr_inv2;
+    double x12g = c9d4 * dr_inv2 - c18d5 * dr_inv4;
+    double y12g = c6d5 * dr_inv4;
+    b_tiles(t, y12b, T.b, T.bt);
+    gt_flat(t, x12g, y12g, T.gt);
+    double x12c = c3d4 * dr_inv3;
+    double y12c = -c3d8 * dr_inv3;
+    double y12h = -c9d8 * dr_inv3;
+    double x12m = -c9d2 * dr_inv3 + c54d5 * dr_inv5;
+    double y12m = c9d4 * dr_inv3 - c36d5 * dr_inv5;
+    double z12m = c9d5 * dr_inv5;
+    // mob_c: note the '+' on the off-diagonal (src/mob.cpp:247-255).
+    for (int i = 0; i < 3; i++) {
+        T.c[i][i] = x12c * t.ee[i][i] + y12c * (1. - t.ee[i][i]);
+        for (int j = i + 1; j < 3; j++) {
+            T.c[i][j] = x12c * t.ee[i][j] + y12c * t.ee[i][j];
+            T.c[j][i] = T.c[i][j];
+        }
+    }
+    // ht (src/mob.cpp:257-276) and flattening (:314-319).
+    double ht[3][3][3];
+    for (int kk = 0; kk < 3; kk++)
+        for (int ii = 0; ii < 3; ii++)
+            for (int jj = 0; jj < 3; jj++) {
+                int ll = third(jj, kk), mm = third(ii, kk);
+                ht[kk][ii][jj] = y12h * (t.ee[ii][ll] * EPS3(jj, kk, ll) + t.ee[jj][mm] * EPS3(ii, kk, mm));
+            }
+    for (int ii = 0; ii < 3; ii++) {
+        T.ht[ii][0] = ht[ii][0][0] - ht[ii][2][2];
+        T.ht[ii][1] = 2. * ht[ii][0][1];
+        T.ht[ii][2] = 2. * ht[ii][0][2];
+        T.ht[ii][3] = 2. * ht[ii][1][2];
+        T.ht[ii][4] = ht[ii][1][1] - ht[ii][2][2];
+    }
+    // m (rank 4, src/mob.cpp:278-298) and the mesid flattening (:321-355).
+    static thread_local double m[3][3][3][3];
+    const auto& ee = t.ee;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++)
+                for (int l = 0; l < 3; l++)
+                    m[i][j][k][l] =
+                        c3d2 * x12m * (ee[i][j] - c1d3 * DEL(i, j)) * (ee[k][l] - c1d3 * DEL(k, l)) +
+                        c1d2 * y12m *
+                            (ee[i][k] * DEL(j, l) + ee[j][k] * DEL(i, l) + ee[i][l] * DEL(j, k) +
+                             ee[j][l] * DEL(i, k) - 4. * ee[i][j] * ee[k][l]) +
+                        c1d2 * z12m *
+                            (DEL(i, k) * DEL(j, l) + DEL(j, k) * DEL(i, l) - DEL(i, j) * DEL(k, l) +
+                             ee[i][j] * DEL(k, l) + ee[k][l] * DEL(i, j) - ee[i][k] * DEL(j, l) -
+                             ee[j][k] * DEL(i, l) - ee[i][l] * DEL(j, k) - ee[j][l] * DEL(i, k) +
+                             ee[i][j] * ee[k][l]);
+    for (int ii = 0; ii < 5; ii++) {
+        int a = MESID[0][ii], b = MESID[1][ii];
+        if (ii == 0 || ii == 4) {
+            T.m[ii][0] = m[a][a][0][0] - m[a][a][2][2] - (m[b][b][0][0] - m[b][b][2][2]);
+            T.m[ii][1] = 2. * (m[a][a][0][1] - m[b][b][0][1]);
+            T.m[ii][2] = 2. * (m[a][a][0][2] - m[b][b][0][2]);
+            T.m[ii][3] = 2. * (m[a][a][1][2] - m[b][b][1][2]);
+            T.m[ii][4] = m[a][a][1][1] - m[a][a][2][2] - (m[b][b][1][1] - m[b][b][2][2]);
+        } else {
+            T.m[ii][0] = 2. * (m[a][b][0][0] - m[a][b][2][2]);
+            T.m[ii][1] = 4. * (m[a][b][0][1]);
+            T.m[ii][2] = 4. * (m[a][b][0][2]);
+            T.m[ii][3] = 4. * (m[a][b][1][2]);
+            T.m[ii][4] = 2. * (m[a][b][1][1] - m[a][b][2][2]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- simulation state
+struct Sim {
+    ps_params p{};
+    int Np = 0, Nm = 0, Nc = 0;
+    int64_t n = 0, nu = 0, ns = 0, nm3 = 0, nc3 = 0, nm3nc3 = 0;
+    double L[3] = {0, 0, 0}, Linv[3] = {0, 0, 0};
+    float rverls[3], sigmas[3];
+    std::vector<int> chain_ids;
+    std::vector<int> bond0, bond1;
+    std::vector<double> x, xpos0, q, xpd;     // xpd: positions the last check_dist used
+    std::vector<double> x_n, up, udiff, v_det_n, v_det_c, v_brown, fext, drift_v, fb, sb;
+    // Verlet list (src/check_dist.cpp:59-62): per row k, partners j (increasing).
+    std::vector<std::vector<int>> vlist;
+    bool have_pd = false, have_mob = false;
+    // Mobility snapshot used by applies (positions of the last check_dist when mob() ran).
+    std::vector<double> xm;
+    double stale[3][3];        // mob_a carried across pair loops (see tile_AA / mob())
+    double stale_next[3][3];   // mob_a left after this mob() call (last BB tile or stale)
+    bool dense_mode = true;
+    std::vector<double> Mdense;   // n x n row-major when dense_mode
+    double ktrap_cur = 0.0;
+    double t = 0.0;
+    int64_t nsteps = 0;
+    std::mt19937_64 rng;
+    std::normal_distribution<double> nd{0.0, 1.0};
+    // diagnostics
+    int last_minres_its = 0, last_sqrt_its = 0, last_sqrt_nneg = 0;
+    int64_t minres_total = 0, applies = 0;
+    std::string err;
+};
+
+inline void min_image(const Sim& s, double& dx, double& dy, double& dz) {
+    // Box::minimum_image, orthorhombic path (src/box.cpp:40-62).
+    if (!s.p.box_active) return;
+    dx -= s.L[0] * std::nearbyint(dx * s.Linv[0]);
+    dy -= s.L[1] * std::nearbyint(dy * s.Linv[1]);
+    dz -= s.L[2] * std::nearbyint(dz * s.Linv[2]);
+}
+inline void wrap(const Sim& s, double& x, double& y, double& z) {
+    // Box::wrap (src/box.cpp:64-84).
+    if (!s.p.box_active) return;
+    x -= s.L[0] * std::nearbyint(x * s.Linv[0]);
+    y -= s.L[1] * std::nearbyint(y * s.Linv[1]);
+    z -= s.L[2] * std::nearbyint(z * s.Linv[2]);
+}
+
+// Pair geometry exactly as check_dist (src/check_dist.cpp:49-71): d = x_k - x_j, minimum image,
+// dr = sqrt(d.d), dr_inv = 1/dr, e = d * dr_inv.
+inline void pair_geom(const Sim& s, const double* xs, int k, int j, double e[3], double& dr,
+                      double& dr_inv) {
+    double dx = xs[3 * k] - xs[3 * j];
+    double dy = xs[3 * k + 1] - xs[3 * j + 1];
+    double dz = xs[3 * k + 2] - xs[3 * j + 2];
+    min_image(s, dx, dy, dz);
+    dr = std::sqrt(dx * dx + dy * dy + dz * dz);
+    dr_inv = 1 / dr;
+    e[0] = dx * dr_inv;
+    e[1] = dy * dr_inv;
+    e[2] = dz * dr_inv;
+}
+
+inline int pair_type(const Sim& s, int k, int j) {
+    if (k < s.Nm && j < s.Nm) return 0;
+    if (k < s.Nm) return 1;
+    return 2;
+}
+
+// Reference pair index of (k < j) in the enumeration of set_vars (src/init.cpp:225-262).
+int64_t pair_index(const Sim& s, int k, int j) {
+    int64_t Nm = s.Nm, Nc = s.Nc;
+    int64_t npAA = Nm * (Nm - 1) / 2, npAB = Nm * Nc;
+    if (j < s.Nm) return (int64_t)k * (2 * Nm - k - 1) / 2 + (j - k - 1);
+    if (k < s.Nm) return npAA + (int64_t)k * Nc + (j - s.Nm);
+    int64_t kc = k - s.Nm, jc = j - s.Nm;
+    return npAA + npAB + kc * (2 * Nc - kc - 1) / 2 + (jc - kc - 1);
+}
+
+bool same_chain(const Sim& s, int i, int j) { return s.chain_ids[i] == s.chain_ids[j]; }
+
+// ---------------------------------------------------------------- setup
+bool setup(Sim& s, const ps_params& p) {
+    s.p = p;
+    s.Np = p.Np;
+    s.Nm = p.Nm;
+    s.Nc = p.Nc;
+    if (s.Np != s.Nm + s.Nc || s.Np < 1 || s.Nm < 0 || s.Nc < 0) {
+        s.err = "Np must equal Nm + Nc";
+        return false;
+    }
+    if (!p.mm_HI) {
+        s.err = "the oracle restates the mm_HI=True (dense grand mobility) regime only";
+        return false;
+    }
+    s.nm3 = 3 * (int64_t)s.Nm;
+    s.nc3 = 3 * (int64_t)s.Nc;
+    s.nm3nc3 = s.nm3 + s.nc3;
+    s.n = s.nm3 + 11 * (int64_t)s.Nc;
+    s.nu = s.nm3 + 6 * (int64_t)s.Nc;
+    s.ns = s.n + s.nu;
+    for (int d = 0; d < 3; d++) {
+        s.L[d] = p.box_active ? p.box[d] : 0.0;
+        s.Linv[d] = p.box_active ? 1.0 / p.box[d] : 0.0;
+    }
+    // set_vars (src/init.cpp:205-212, :293-338).
+    s.chain_ids.assign(s.Nm, 0);
+    for (int ii = 0; ii < p.Npoly; ii++)
+        for (int jj = 0; jj < p.Nmono_per_chain; jj++) {
+            int kk = ii * p.Nmono_per_chain + jj;
+            if (kk < s.Nm) s.chain_ids[kk] = ii;
+        }
+    s.bond0.clear();
+    s.bond1.clear();
+    for (int ii = 0; ii < p.Npoly; ii++)
+        for (int jj = 0; jj < p.Nmono_per_chain - 1; jj++) {
+            int pidx = ii * p.Nmono_per_chain + jj;
+            s.bond0.push_back(pidx);
+            s.bond1.push_back(pidx + 1);
+        }
+    if (p.tether && s.Nc > 0) {
+        int cpc = p.Npoly / s.Nc;
+        for (int ii = 0; ii < p.Npoly; ii++) {
+            int host = s.Nm + (cpc > 0 ? ii / cpc : 0);
+            if (host >= s.Np) host = s.Np - 1;
+            s.bond0.push_back(ii * p.Nmono_per_chain);
+            s.bond1.push_back(host);
+        }
+    }
+    s.rverls[0] = (float)(0.05 + 2 * p.beta);
+    s.rverls[1] = (float)(1.5 + p.beta);
+    s.rverls[2] = (float)2.5;
+    s.sigmas[0] = (float)(2. * p.beta);
+    s.sigmas[1] = (float)(1. + p.beta);
+    s.sigmas[2] = (float)2.0;
+    s.x.assign(3 * s.Np, 0.0);
+    s.xpos0.assign(s.nc3, 0.0);
+    s.q.assign(4 * s.Nc, 0.0);
+    s.x_n.assign(3 * s.Np, 0.0);
+    for (auto* v : {&s.up, &s.udiff, &s.v_det_n, &s.v_det_c, &s.v_brown, &s.fext, &s.drift_v})
+        v->assign(s.nu, 0.0);
+    s.fb.assign(s.n, 0.0);
+    s.sb.assign(5 * s.Nc, 0.0);
+    s.vlist.assign(std::max(s.Np - 1, 0), {});
+    // fill_self leaves mob_a = I (src/mob.cpp:65-86, :568).
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) s.stale[i][j] = s.stale_next[i][j] = (i == j) ? 1.0 : 0.0;
+    s.ktrap_cur = p.ktrap;
+    // Ctor (src/Stokes.cpp:21-25): t = restart; nsteps = floor((tmax - restart)/dt).
+    s.t = p.restart;
+    s.nsteps = (int64_t)std::floor((p.tmax - p.restart) / p.dt);
+    if (s.nsteps < 0) s.nsteps = 0;
+    // init_random (src/init.cpp:105-110, :413-414).
+    uint64_t eff = p.seed + (uint64_t)std::llround(p.restart / p.dt);
+    s.rng.seed(eff);
+    s.nd = std::normal_distribution<double>(0.0, 1.0);
+    s.dense_mode = (s.n <= 8192);
+    return true;
+}
+
+// ---------------------------------------------------------------- check_dist
+void check_dist(Sim& s) {
+    // src/check_dist.cpp:12-87 (serial path): all stored pairs in enumeration order.
+    for (auto& v : s.vlist) v.clear();
+    const double* xs = s.x.data();
+    int Np = s.Np;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int k = 0; k < Np - 1; k++) {
+        for (int j = k + 1; j < Np; j++) {
+            double e[3], dr, dr_inv;
+            pair_geom(s, xs, k, j, e, dr, dr_inv);
+            double rverl = s.rverls[pair_type(s, k, j)];
+            if (dr < rverl) s.vlist[k].push_back(j);
+        }
+    }
+    s.xpd = s.x;
+    s.have_pd = true;
+}
+
+// ---------------------------------------------------------------- mob()
+// The 3x3 block every AB pair writes into M^mc / M^cm (src/mob.cpp:926-958) is the global
+// mob_a, which mobility_AB never writes: it holds the tile of the last AA pair processed in this
+// mob() call (src/mob.cpp:845-897), or, if none was processed, whatever the previous call left
+// (the last BB tile, src/mob.cpp:1026-1029) -- initially the identity from fill_self.
+void compute_stale(Sim& s) {
+    const double* xs = s.xm.data();
+    bool found = false;
+    int fi = -1, fj = -1;
+    // Last processed AA pair in (i < j) lexicographic order, skipping same-chain when chain_HI.
+    for (int i = s.Nm - 2; i >= 0 && !found; i--) {
+        for (int j = s.Nm - 1; j > i; j--) {
+            if (s.p.chain_HI && same_chain(s, i, j)) continue;
+            fi = i;
+            fj = j;
+            found = true;
+            break;
+        }
+    }
+    if (found) {
+        double e[3], dr, dr_inv;
+        pair_geom(s, xs, fi, fj, e, dr, dr_inv);
+        tile_AA(dr, dr_inv, e, s.p.beta, s.stale);
+    } else {
+        std::memcpy(s.stale, s.stale_next, sizeof(s.stale));
+    }
+    if (s.Nc >= 2) {
+        double e[3], dr, dr_inv;
+        pair_geom(s, xs, s.Np - 2, s.Np - 1, e, dr, dr_inv);
+        BBTile T;
+        tile_BB(dr_inv, e, T);
+        std::memcpy(s.stale_next, T.a, sizeof(T.a));
+    } else {
+        std::memcpy(s.stale_next, s.stale, sizeof(s.stale));
+    }
+}
+
+// Generic emission of every entry of M in the placement of mob()/fill_self; E(row, col, val).
+// Pairs are visited in the reference order so a dense INSERT reproduces mob() exactly.
+template <class E>
+void emit_self(const Sim& s, E&& emit) {
+    // fill_self (src/mob.cpp:539-633).
+    double binv = 1.0 / s.p.beta;
+    for (int k = 0; k < s.Nm; k++)
+        for (int i = 0; i < 3; i++) emit(3 * k + i, 3 * k + i, binv * 1.0);
+    for (int k = s.Nm; k < s.Np; k++) {
+        int64_t ph1 = 3 * (int64_t)k, ph2 = ph1 + s.nc3, ph3 = s.nu + 5 * (int64_t)(k - s.Nm);
+        for (int i = 0; i < 3; i++) {
+            emit(ph1 + i, ph1 + i, 1.0);
+            emit(ph2 + i, ph2 + i, 3. / 4.);
+        }
+        for (int i = 0; i < 5; i++)
+            for (int j = 0; j < 5; j++) {
+                double v = (i == j) ? 9. / 5. : ((i == 0 && j == 4) || (i == 4 && j == 0)) ? 9. / 10. : 0.0;
+                emit(ph3 + i, ph3 + j, v);
+            }
+    }
+}
+
+template <class E>
+void emit_AA_row(const Sim& s, int i, E&& emit) {
+    // AA pairs (i, j > i), src/mob.cpp:843-897.
+    const double* xs = s.xm.data();
+    for (int j = i + 1; j < s.Nm; j++) {
+        if (s.p.chain_HI && same_chain(s, i, j)) continue;
+        double e[3], dr, dr_inv, a[3][3];
+        pair_geom(s, xs, i, j, e, dr, dr_inv);
+        tile_AA(dr, dr_inv, e, s.p.beta, a);
+        int64_t ph1 = 3 * (int64_t)i, ph2 = 3 * (int64_t)j;
+        for (int ii = 0; ii < 3; ii++)
+            for (int jj = 0; jj < 3; jj++) {
+                emit(ph1 + ii, ph2 + jj, a[ii][jj]);
+                emit(ph2 + ii, ph1 + jj, a[jj][ii]);
+            }
+    }
+}
+
+template <class E>
+void emit_AB_row(const Sim& s, int mono, E&& emit) {
+    // AB pairs (monomer, colloid), src/mob.cpp:900-974.
+    const double* xs = s.xm.data();
+    for (int c = s.Nm; c < s.Np; c++) {
+        double e[3], dr, dr_inv, b[3][3], bt[3][3], gt[3][5];
+        pair_geom(s, xs, mono, c, e, dr, dr_inv);
+        tile_AB(dr_inv, e, s.p.beta * s.p.beta, b, bt, gt);
+        int64_t ph1 = 3 * (int64_t)mono, ph2 = 3 * (int64_t)c, ph4 = ph2 + s.nc3,
+                ph6 = 5 * (int64_t)(c - s.Nm) + s.nu;
+        for (int ii = 0; ii < 3; ii++) {
+            for (int jj = 0; jj < 3; jj++) {
+                emit(ph1 + ii, ph2 + jj, s.stale[ii][jj]);
+                emit(ph4 + ii, ph1 + jj, b[ii][jj]);
+                emit(ph1 + ii, ph4 + jj, bt[ii][jj]);
+                emit(ph2 + ii, ph1 + jj, s.stale[jj][ii]);
+            }
+            for (int jj = 0; jj < 5; jj++) {
+                emit(ph1 + ii, ph6 + jj, gt[ii][jj]);
+                emit(ph6 + jj, ph1 + ii, gt[ii][jj]);
+            }
+        }
+    }
+}
+
+template <class E>
+void emit_BB_pair(const Sim& s, int c1, int c2, E&& emit) {
+    // BB pairs (c1 < c2), src/mob.cpp:1026-1144.
+    const double* xs = s.xm.data();
+    double e[3], dr, dr_inv;
+    pair_geom(s, xs, c1, c2, e, dr, dr_inv);
+    BBTile T;
+    tile_BB(dr_inv, e, T);
+    int64_t ph5 = s.nu + 5 * (int64_t)(c1 - s.Nm), ph6 = s.nu + 5 * (int64_t)(c2 - s.Nm);
+    int64_t ph1 = 3 * (int64_t)c1, ph2 = 3 * (int64_t)c2, ph3 = ph1 + s.nc3, ph4 = ph2 + s.nc3;
+    for (int ii = 0; ii < 3; ii++) {
+        for (int jj = 0; jj < 3; jj++) {
+            emit(ph1 + ii, ph2 + jj, T.a[ii][jj]);
+            emit(ph3 + ii, ph2 + jj, T.b[ii][jj]);
+            emit(ph2 + ii, ph3 + jj, T.bt[ii][jj]);
+            emit(ph3 + ii, ph4 + jj, T.c[ii][jj]);
+            emit(ph2 + ii, ph1 + jj, T.a[jj][ii]);
+            emit(ph4 + ii, ph1 + jj, -T.b[jj][ii]);
+            emit(ph1 + ii, ph4 + jj, -T.bt[jj][ii]);
+            emit(ph4 + ii, ph3 + jj, T.c[jj][ii]);
+        }
+        for (int jj = 0; jj < 5; jj++) {
+            double v = T.gt[ii][jj];
+            emit(ph1 + ii, ph6 + jj, v);
+            emit(ph6 + jj, ph1 + ii, v);
+            emit(ph2 + ii, ph5 + jj, -v);   // 2-1 gt flips sign
+            emit(ph5 + jj, ph2 + ii, -v);
+            double h = T.ht[ii][jj];
+            emit(ph3 + ii, ph6 + jj, h);
+            emit(ph6 + jj, ph3 + ii, h);
+            emit(ph4 + ii, ph5 + jj, h);    // 2-1 ht does not
+            emit(ph5 + jj, ph4 + ii, h);
+        }
+    }
+    for (int ii = 0; ii < 5; ii++)
+        for (int jj = 0; jj < 5; jj++) {
+            emit(ph5 + ii, ph6 + jj, T.m[ii][jj]);
+            emit(ph6 + ii, ph5 + jj, T.m[jj][ii]);
+        }
+}
+
+void mob(Sim& s) {
+    s.xm = s.xpd;
+    compute_stale(s);
+    if (s.dense_mode) {
+        // Dense INSERT assembly of the n x n grand mobility (src/mob.cpp:635-1213); entries never
+        // written (same-chain pairs under chain_HI) stay 0 as in the MatZeroEntries'd A.
+        s.Mdense.assign((size_t)s.n * s.n, 0.0);
+        double* M = s.Mdense.data();
+        int64_t n = s.n;
+        auto ins = [M, n](int64_t r, int64_t c, double v) { M[r * n + c] = v; };
+        emit_self(s, ins);
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int i = 0; i < s.Nm - 1; i++) emit_AA_row(s, i, ins);
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int m = 0; m < s.Nm; m++) emit_AB_row(s, m, ins);
+        for (int c1 = s.Nm; c1 < s.Np; c1++)
+            for (int c2 = c1 + 1; c2 < s.Np; c2++) emit_BB_pair(s, c1, c2, ins);
+    }
+    s.have_mob = true;
+}
+
+// y = M x (n entries).
+void mob_apply(Sim& s, const double* x, double* y) {
+    int64_t n = s.n;
+    s.applies++;
+    if (s.dense_mode) {
+        const double* M = s.Mdense.data();
+#pragma omp parallel for schedule(static)
+        for (int64_t r = 0; r < n; r++) {
+            double acc = 0.0;
+            const double* row = M + r * n;
+            for (int64_t c = 0; c < n; c++) acc += row[c] * x[c];
+            y[r] = acc;
+        }
+        return;
+    }
+    // Matrix-free: entries emitted on the fly, accumulated per thread, reduced in thread order.
+    int nth = 1;
+#ifdef _OPENMP
+    nth = omp_get_max_threads();
+#endif
+    std::vector<std::vector<double>> part(nth, std::vector<double>(n, 0.0));
+#pragma omp parallel
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double* yp = part[tid].data();
+        auto acc = [yp, x](int64_t r, int64_t c, double v) { yp[r] += v * x[c]; };
+        if (tid == 0) emit_self(s, acc);
+#pragma omp for schedule(dynamic, 4)
+        for (int i = 0; i < s.Nm - 1; i++) emit_AA_row(s, i, acc);
+#pragma omp for schedule(dynamic, 4)
+        for (int m = 0; m < s.Nm; m++) emit_AB_row(s, m, acc);
+#pragma omp for schedule(dynamic, 1)
+        for (int c1 = s.Nm; c1 < s.Np; c1++)
+            for (int c2 = c1 + 1; c2 < s.Np; c2++) emit_BB_pair(s, c1, c2, acc);
+    }
+    for (int64_t r = 0; r < n; r++) {
+        double a = 0.0;
+        for (int t = 0; t < nth; t++) a += part[t][r];
+        y[r] = a;
+    }
+}
+
+// Saddle operator A = [[M, B], [B^T, 0]], B = [-I_nu; 0] (src/arrays.cpp:349-408).
+void saddle_apply(Sim& s, const double* x, double* y) {
+    mob_apply(s, x, y);
+    for (int64_t i = 0; i < s.nu; i++) y[i] -= x[s.n + i];
+    for (int64_t i = 0; i < s.nu; i++) y[s.n + i] = -x[i];
+}
+
+// Jacobi diagonal with fix-diagonal (src/matfree_A.cpp:84-102, src/init.cpp:58-59).
+void saddle_diag(const Sim& s, double* d) {
+    for (int64_t i = 0; i < s.nm3; i++) d[i] = 1.0 / s.p.beta;
+    for (int c = 0; c < s.Nc; c++) {
+        for (int i = 0; i < 3; i++) d[s.nm3 + 3 * c + i] = 1.0;
+        for (int i = 0; i < 3; i++) d[s.nm3 + s.nc3 + 3 * c + i] = 3. / 4.;
+        for (int i = 0; i < 5; i++) d[s.nu + 5 * c + i] = 9. / 5.;
+    }
+    for (int64_t i = 0; i < s.nu; i++) d[s.n + i] = 1.0;   // zero diagonal fixed to 1
+}
+
+inline double dot(const double* a, const double* b, int64_t n) {
+    double acc = 0.0;
+    for (int64_t i = 0; i < n; i++) acc += a[i] * b[i];
+    return acc;
+}
+
+// Preconditioned MINRES (Paige & Saunders 1975; the algorithm of PETSc KSPMINRES), Jacobi
+// preconditioner, ||b||_{P^-1}-relative stopping test (src/init.cpp:40-64, src/vel.cpp:41-51).
+int minres(Sim& s, const double* b, double* x, bool warm, ps_solve_info* info) {
+    int64_t N = s.ns;
+    std::vector<double> dinv(N), r1(N), r2(N), y(N), v(N), w(N, 0.0), w1(N, 0.0), w2(N, 0.0);
+    saddle_diag(s, dinv.data());
+    for (auto& d : dinv) d = 1.0 / d;
+    if (warm) {
+        saddle_apply(s, x, y.data());
+        for (int64_t i = 0; i < N; i++) r1[i] = b[i] - y[i];
+    } else {
+        for (int64_t i = 0; i < N; i++) {
+            x[i] = 0.0;
+            r1[i] = b[i];
+        }
+    }
+    for (int64_t i = 0; i < N; i++) y[i] = dinv[i] * r1[i];
+    double beta1 = dot(r1.data(), y.data(), N);
+    if (info) { info->iterations = 0; info->refinements = 0; info->true_rel_residual = -1.0; }
+    if (beta1 <= 0.0) {
+        if (info) { info->reason = 3; info->rnorm = 0.0; info->rnorm0 = 0.0; }
+        return 0;
+    }
+    beta1 = std::sqrt(beta1);
+    // Reference for the relative test: ||b|| in the same (P^-1) norm.
+    double bnorm = 0.0;
+    for (int64_t i = 0; i < N; i++) bnorm += b[i] * dinv[i] * b[i];
+    bnorm = std::sqrt(bnorm);
+    double tol = std::max(s.p.ksp_rtol * bnorm, s.p.ksp_abstol);
+    r2 = r1;
+    double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, phibar = beta1;
+    double cs = -1.0, sn = 0.0;
+    int its = 0, reason = -3;
+    if (phibar <= tol) reason = 2;
+    while (reason < 0 && its < s.p.ksp_maxit) {
+        its++;
+        double sc = 1.0 / beta;
+        for (int64_t i = 0; i < N; i++) v[i] = sc * y[i];
+        saddle_apply(s, v.data(), y.data());
+        if (its >= 2) {
+            double f = beta / oldb;
+            for (int64_t i = 0; i < N; i++) y[i] -= f * r1[i];
+        }
+        double alfa = dot(v.data(), y.data(), N);
+        double f2 = alfa / beta;
+        for (int64_t i = 0; i < N; i++) y[i] -= f2 * r2[i];
+        std::swap(r1, r2);
+        r2 = y;
+        for (int64_t i = 0; i < N; i++) y[i] = dinv[i] * r2[i];
+        oldb = beta;
+        double bb = dot(r2.data(), y.data(), N);
+        beta = std::sqrt(std::max(bb, 0.0));
+        double oldeps = epsln;
+        double delta = cs * dbar + sn * alfa;
+        double gbar = sn * dbar - cs * alfa;
+        epsln = sn * beta;
+        dbar = -cs * beta;
+        double gamma = std::sqrt(gbar * gbar + beta * beta);
+        gamma = std::max(gamma, 1e-300);
+        cs = gbar / gamma;
+        sn = beta / gamma;
+        double phi = cs * phibar;
+        phibar = sn * phibar;
+        double denom = 1.0 / gamma;
+        std::swap(w1, w2);   // w1 <- old w2
+        std::swap(w2, w);    // w2 <- old w; w <- old w1 (overwritten below)
+        for (int64_t i = 0; i < N; i++) {
+            w[i] = (v[i] - oldeps * w1[i] - delta * w2[i]) * denom;
+            x[i] += phi * w[i];
+        }
+        if (phibar <= tol) reason = (phibar <= s.p.ksp_abstol) ? 3 : 2;
+        if (beta == 0.0 && reason < 0) reason = 2;
+    }
+    s.last_minres_its = its;
+    s.minres_total += its;
+    if (info) {
+        info->iterations = its;
+        info->reason = reason;
+        info->rnorm = phibar;
+        info->rnorm0 = beta1;
+    }
+    return reason > 0 ? 0 : PS_ERR_NOCONV;
+}
+
+// Symmetric tridiagonal eigendecomposition by implicit QL with Wilkinson-type shifts
+// (EISPACK tql2; Bowdler, Martin, Reinsch & Wilkinson 1968).  d: diagonal (m), e: off-diagonal
+// (m-1).  On return d holds eigenvalues, Z (m x m, column-major) eigenvectors.
+bool tql2(int m, std::vector<double>& d, std::vector<double> e, std::vector<double>& Z) {
+    Z.assign((size_t)m * m, 0.0);
+    for (int i = 0; i < m; i++) Z[i + (size_t)i * m] = 1.0;
+    e.resize(m, 0.0);
+    for (int l = 0; l < m; l++) {
+        int iter = 0, mm;
+        do {
+            for (mm = l; mm < m - 1; mm++) {
+                double dd = std::fabs(d[mm]) + std::fabs(d[mm + 1]);
+                if (std::fabs(e[mm]) <= 1e-300 + 2.2e-16 * dd) break;
+            }
+            if (mm != l) {
+                if (iter++ == 200) return false;
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = std::hypot(g, 1.0);
+                g = d[mm] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
+                double s = 1.0, c = 1.0, p = 0.0;
+                int i;
+                bool underflow = false;
+                for (i = mm - 1; i >= l; i--) {
+                    double f = s * e[i], b = c * e[i];
+                    r = std::hypot(f, g);
+                    e[i + 1] = r;
+                    if (r == 0.0) {
+                        d[i + 1] -= p;
+                        e[mm] = 0.0;
+                        underflow = true;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * b;
+                    p = s * r;
+                    d[i + 1] = g + p;
+                    g = c * r - b;
+                    for (int k = 0; k < m; k++) {
+                        double zf = Z[k + (size_t)(i + 1) * m];
+                        Z[k + (size_t)(i + 1) * m] = s * Z[k + (size_t)i * m] + c * zf;
+                        Z[k + (size_t)i * m] = c * Z[k + (size_t)i * m] - s * zf;
+                    }
+                }
+                if (underflow) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[mm] = 0.0;
+            }
+        } while (mm != l);
+    }
+    return true;
+}
+
+// g = beta0 * U diag(sqrt(max(theta, 0))) U^T e1 (src/slip_vel.cpp:427-459).
+int tridiag_sqrt_weights(int m, const double* alpha, const double* betad, double beta0, double* g) {
+    std::vector<double> d(alpha, alpha + m), e(betad, betad + std::max(m - 1, 0)), Z;
+    tql2(m, d, e, Z);
+    int nneg = 0;
+    for (int i = 0; i < m; i++) {
+        double gi = 0.0;
+        for (int jj = 0; jj < m; jj++) {
+            double th = d[jj];
+            if (th < 0.0) th = 0.0;
+            gi += Z[i + (size_t)jj * m] * std::sqrt(th) * Z[0 + (size_t)jj * m];
+        }
+        g[i] = beta0 * gi;
+    }
+    for (int jj = 0; jj < m; jj++)
+        if (d[jj] < 0.0) nneg++;
+    return nneg;
+}
+
+// w = M^{1/2} xi by Lanczos, restating schur_sqrt_lanczos (src/slip_vel.cpp:351-477) on the full
+// grand mobility (the mm_HI operator of MFNSolve, src/slip_vel.cpp:27-49), with an added
+// relative-change stopping test at sqrt_rtol.
+void sqrt_apply(Sim& s, const double* b, double* out, ps_sqrt_info* info) {
+    int64_t n = s.n;
+    double beta0 = std::sqrt(dot(b, b, n));
+    for (int64_t i = 0; i < n; i++) out[i] = 0.0;
+    if (info) { info->iterations = 0; info->nneg = 0; info->beta0 = beta0; info->rel_change = 0.0; }
+    if (beta0 == 0.0) return;
+    int kmax = s.p.sqrt_kmax;
+    if (kmax > n) kmax = (int)n;
+    if (kmax < 1) kmax = 1;
+    std::vector<std::vector<double>> Q;
+    std::vector<double> alpha, betad, w(n), gprev, g;
+    Q.emplace_back(n);
+    for (int64_t i = 0; i < n; i++) Q[0][i] = b[i] * (1.0 / beta0);
+    int m = 0;
+    double relc = 0.0;
+    for (int j = 0; j < kmax; j++) {
+        const std::vector<double>& qj = Q[j];
+        mob_apply(s, qj.data(), w.data());
+        if (j > 0) {
+            const double bj1 = betad[j - 1];
+            for (int64_t i = 0; i < n; i++) w[i] -= bj1 * Q[j - 1][i];
+        }
+        double aj = dot(qj.data(), w.data(), n);
+        alpha.push_back(aj);
+        for (int64_t i = 0; i < n; i++) w[i] -= aj * qj[i];
+        for (int pass = 0; pass < 2; pass++)
+            for (int i = 0; i <= j; i++) {
+                double h = dot(Q[i].data(), w.data(), n);
+                for (int64_t k = 0; k < n; k++) w[k] -= h * Q[i][k];
+            }
+        m = j + 1;
+        double bj = std::sqrt(dot(w.data(), w.data(), n));
+        if (s.p.sqrt_rtol > 0.0) {
+            g.assign(m, 0.0);
+            tridiag_sqrt_weights(m, alpha.data(), betad.data(), beta0, g.data());
+            if (m >= 2) {
+                double dn = 0.0, gn = 0.0;
+                for (int i = 0; i < m; i++) {
+                    double gp = (i < m - 1) ? gprev[i] : 0.0;
+                    dn += (g[i] - gp) * (g[i] - gp);
+                    gn += g[i] * g[i];
+                }
+                relc = std::sqrt(dn / gn);
+                if (relc <= s.p.sqrt_rtol) break;
+            }
+            gprev = g;
+        }
+        if (bj <= 1e-12 * beta0 || j == kmax - 1) break;
+        betad.push_back(bj);
+        Q.emplace_back(n);
+        for (int64_t i = 0; i < n; i++) Q[j + 1][i] = w[i] * (1.0 / bj);
+    }
+    g.assign(m, 0.0);
+    int nneg = tridiag_sqrt_weights(m, alpha.data(), betad.data(), beta0, g.data());
+    for (int i = 0; i < m; i++)
+        for (int64_t k = 0; k < n; k++) out[k] += g[i] * Q[i][k];
+    s.last_sqrt_its = m;
+    s.last_sqrt_nneg = nneg;
+    if (info) { info->iterations = m; info->nneg = nneg; info->rel_change = relc; }
+}
+
+// ---------------------------------------------------------------- forces (src/rhs.cpp)
+inline double hs_force_mag(double dr, double sig, double k) {
+    // src/pair_interaction.h:319-324.
+    const double HS_REG = 0.01;
+    double sig_eff = sig * (1.0 + HS_REG);
+    if (dr >= sig_eff) return 0.0;
+    return k * (sig_eff - dr);
+}
+inline double hs_exp_force_mag(double dr, double sig, double k) {
+    // src/pair_interaction.h:326-331.
+    const double HS_REG = 0.01;
+    double sig_eff = sig * (1.0 + HS_REG);
+    double ex = std::exp(-k * (dr - sig_eff));
+    return ex / (1 - ex);
+}
+
+void rhs_build(Sim& s, bool drift, double* rhs) {
+    // RHS (src/rhs.cpp:160-260).
+    for (int64_t i = 0; i < s.ns; i++) rhs[i] = 0.0;
+    if (drift) {
+        for (int64_t i = 0; i < s.nu; i++) rhs[s.n + i] = s.fext[i];
+        return;
+    }
+    std::vector<double> fint(s.nm3nc3, 0.0);
+    // pair_interaction_hs / _hs_exp over the Verlet list (src/pair_interaction.cpp:83-150, 209-266).
+    double kk_hs = (s.p.hs_kmode == 1) ? s.p.hs_kpref / (s.p.dt * s.p.beta) : s.p.hs_kpref / s.p.dt;
+    const double* xs = s.xpd.data();
+    for (int k = 0; k < s.Np - 1; k++) {
+        for (int j : s.vlist[k]) {
+            int type = pair_type(s, k, j);
+            if (type == 0 && !s.p.mono_ev) continue;
+            double e[3], dr, dr_inv;
+            pair_geom(s, xs, k, j, e, dr, dr_inv);
+            double sig = s.sigmas[type];
+            double fmag = (s.p.force_sel == PS_FORCE_HS_EXP) ? hs_exp_force_mag(dr, sig, s.p.hs_exp_k)
+                                                             : hs_force_mag(dr, sig, kk_hs);
+            if (fmag == 0.0) continue;
+            double fx = fmag * e[0], fy = fmag * e[1], fz = fmag * e[2];
+            fint[3 * k] -= fx;
+            fint[3 * k + 1] -= fy;
+            fint[3 * k + 2] -= fz;
+            fint[3 * j] += fx;
+            fint[3 * j + 1] += fy;
+            fint[3 * j + 2] += fz;
+        }
+    }
+    // bond_forces (src/rhs.cpp:102-158).
+    for (size_t ii = 0; ii < s.bond0.size(); ii++) {
+        int kk = s.bond0[ii], jj = s.bond1[ii];
+        double r[4], f[3];
+        r[0] = s.x[3 * kk] - s.x[3 * jj];
+        r[1] = s.x[3 * kk + 1] - s.x[3 * jj + 1];
+        r[2] = s.x[3 * kk + 2] - s.x[3 * jj + 2];
+        min_image(s, r[0], r[1], r[2]);
+        r[3] = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        for (int d = 0; d < 3; d++) f[d] = -s.p.kbond * (1 - (s.p.r0 / r[3])) * r[d];
+        if (s.p.fene) {
+            double ext2 = 1 / (1 - (r[3] / s.p.Lmax) * (r[3] / s.p.Lmax));
+            for (int d = 0; d < 3; d++) f[d] *= ext2;
+        }
+        for (int d = 0; d < 3; d++) {
+            fint[3 * kk + d] -= f[d];
+            fint[3 * jj + d] += f[d];
+        }
+    }
+    // trapping_forces, static trap (src/rhs.cpp:46-100).
+    if (!(std::fabs(s.ktrap_cur) < 1e-12)) {
+        if (s.t > s.p.tstart + s.p.trun && s.p.weaken_trap >= 0) s.ktrap_cur = s.p.weaken_trap;
+        for (int i = s.Nm; i < s.Np; i++) {
+            int i3 = 3 * i, ic3 = 3 * (i - s.Nm);
+            double dx = s.x[i3] - s.xpos0[ic3];
+            double dy = s.x[i3 + 1] - s.xpos0[ic3 + 1];
+            double dz = s.x[i3 + 2] - s.xpos0[ic3 + 2];
+            min_image(s, dx, dy, dz);
+            fint[i3] += s.ktrap_cur * dx;
+            fint[i3 + 1] += s.ktrap_cur * dy;
+            fint[i3 + 2] += s.ktrap_cur * dz;
+        }
+    }
+    for (int64_t i = 0; i < s.nm3nc3; i++) rhs[s.n + i] = fint[i];
+}
+
+// ---------------------------------------------------------------- integrator
+void step(Sim& s) {
+    // src/traj.cpp:9-35.
+    int64_t n3 = 3 * (int64_t)s.Np;
+    double dt = s.p.dt, eps = 1.0e-3;
+    if (s.p.kT > 0) {
+        double drift_scale = s.p.kT / (2.0 * eps);
+        for (int64_t i = 0; i < n3; i++) {
+            s.x[i] += eps * s.fext[i];
+            s.x[i] += drift_scale * s.udiff[i] * dt;
+        }
+    }
+    for (int64_t i = 0; i < n3; i++) s.x[i] += s.up[i] * dt;
+    for (int64_t k3 = 0; k3 < n3; k3 += 3) wrap(s, s.x[k3], s.x[k3 + 1], s.x[k3 + 2]);
+}
+
+void step_from(Sim& s, const std::vector<double>& xb) {
+    // src/traj.cpp:37-62.
+    int64_t n3 = 3 * (int64_t)s.Np;
+    double dt = s.p.dt, eps = 1.0e-3;
+    if (s.p.kT > 0) {
+        double drift_scale = s.p.kT / (2.0 * eps);
+        for (int64_t i = 0; i < n3; i++) s.x[i] = xb[i] + drift_scale * s.udiff[i] * dt + s.up[i] * dt;
+    } else {
+        for (int64_t i = 0; i < n3; i++) s.x[i] = xb[i] + s.up[i] * dt;
+    }
+    for (int64_t k3 = 0; k3 < n3; k3 += 3) wrap(s, s.x[k3], s.x[k3 + 1], s.x[k3 + 2]);
+}
+
+void step_quaternion(Sim& s) {
+    // src/traj.cpp:64-94, src/quat.cpp:4-54.
+    double dt = s.p.dt;
+    for (int i = 0; i < s.Nc; i++) {
+        double* q = &s.q[4 * i];
+        const double* om = &s.up[3 * i + s.nm3nc3];
+        double w1 = q[0], x1 = q[1], y1 = q[2], z1 = q[3];
+        double w2 = 0.0, x2 = om[0], y2 = om[1], z2 = om[2];
+        double dq[4] = {w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                        w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2};
+        double qn[4];
+        for (int k = 0; k < 4; k++) qn[k] = q[k] + 0.5 * dq[k] * dt;
+        double norm = std::sqrt(qn[0] * qn[0] + qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]);
+        for (int k = 0; k < 4; k++) q[k] = qn[k] / norm;
+    }
+}
+
+// One iteration of the run() loop (src/run.cpp:70-162), output writing omitted.
+int run_step(Sim& s) {
+    int64_t n = s.n, nu = s.nu, ns = s.ns;
+    std::vector<double> rhs(ns), X(ns, 0.0), Xdet(ns, 0.0);
+    static thread_local std::vector<double> Xd;
+    if ((int64_t)Xd.size() != ns) Xd.assign(ns, 0.0);
+    int rc = 0;
+    s.t += s.p.dt;
+    s.x_n = s.x;
+    // predictor
+    check_dist(s);
+    rhs_build(s, false, rhs.data());
+    mob(s);
+    rc |= minres(s, rhs.data(), Xdet.data(), false, nullptr);   // solve_deterministic_vel
+    for (int64_t i = 0; i < nu; i++) s.v_det_n[i] = Xdet[n + i];
+    if (s.p.kT > 0) {
+        // solve_slip_vel (src/slip_vel.cpp:19-79)
+        for (int64_t i = 0; i < n; i++) s.fb[i] = s.nd(s.rng);
+        std::vector<double> W(n);
+        sqrt_apply(s, s.fb.data(), W.data(), nullptr);
+        double scale = std::sqrt(2 * s.p.kT / s.p.dt);
+        for (int64_t i = 0; i < n; i++) rhs[i] += scale * W[i];
+        // new_vel (src/vel.cpp:68-94)
+        rc |= minres(s, rhs.data(), X.data(), false, nullptr);
+        for (int64_t i = 0; i < nu; i++) s.up[i] = X[n + i];
+        if (s.p.record_forces)
+            for (int64_t i = 0; i < nu; i++) s.fext[i] = X[i];
+        for (int64_t k = 0; k < nu; k++) s.v_brown[k] = s.up[k] - s.v_det_n[k];
+        // drift (src/drift.cpp:17-62)
+        double eps = 1.0e-3;
+        for (int64_t i = 0; i < nu; i++) s.fext[i] = s.nd(s.rng);
+        int64_t n3 = 3 * (int64_t)s.Np;
+        for (int64_t i = 0; i < n3; i++) s.x[i] += eps * s.fext[i];
+        check_dist(s);
+        mob(s);
+        rhs_build(s, true, rhs.data());
+        rc |= minres(s, rhs.data(), Xd.data(), false, nullptr);
+        for (int64_t i = 0; i < nu; i++) s.drift_v[i] = Xd[n + i];
+        for (int i = 0; i < 5 * s.Nc; i++) s.sb[i] = Xd[nu + i];
+        for (int64_t i = 0; i < n3; i++) s.x[i] -= 2.0 * eps * s.fext[i];
+        check_dist(s);
+        mob(s);
+        rhs_build(s, true, rhs.data());
+        rc |= minres(s, rhs.data(), Xd.data(), true, nullptr);
+        for (int64_t i = 0; i < nu; i++) s.udiff[i] = Xd[n + i] - s.drift_v[i];
+        for (int i = 0; i < 5 * s.Nc; i++) s.sb[i] = (s.sb[i] - Xd[nu + i]) / (2 * eps);
+    } else {
+        rc |= minres(s, rhs.data(), X.data(), false, nullptr);
+        for (int64_t i = 0; i < nu; i++) s.up[i] = X[n + i];
+        if (s.p.record_forces)
+            for (int64_t i = 0; i < nu; i++) s.fext[i] = X[i];
+        std::fill(s.v_brown.begin(), s.v_brown.end(), 0.0);
+    }
+    step(s);
+    // corrector
+    check_dist(s);
+    rhs_build(s, false, rhs.data());
+    mob(s);
+    rc |= minres(s, rhs.data(), Xdet.data(), false, nullptr);
+    for (int64_t i = 0; i < nu; i++) s.v_det_c[i] = Xdet[n + i];
+    for (int64_t k = 0; k < nu; k++) s.up[k] = 0.5 * (s.v_det_n[k] + s.v_det_c[k]) + s.v_brown[k];
+    step_from(s, s.x_n);
+    step_quaternion(s);
+    return rc;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- C ABI (ctypes)
+extern "C" {
+
+void or_params_default(ps_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->samplerate = 1;
+    p->mm_HI = 1;
+    p->chain_HI = 0;
+    p->fene = 1;
+    p->record_forces = 1;
+    p->seed = 12345;
+    p->ksp_rtol = 1e-6;
+    p->ksp_abstol = 1e-50;
+    p->ksp_maxit = 10000;
+    p->sqrt_rtol = 1e-8;
+    p->sqrt_kmax = 64;
+    p->hs_kpref = 1.0;
+    p->hs_exp_k = 100.0;
+    p->weaken_trap = -1;
+    p->ir_rtol = 1e-6;
+    p->ir_maxit = 20;
+}
+
+void* or_create(const ps_params* p, char* err, int errlen) {
+    Sim* s = new Sim();
+    if (!setup(*s, *p)) {
+        if (err && errlen > 0) std::snprintf(err, errlen, "%s", s->err.c_str());
+        delete s;
+        return nullptr;
+    }
+    return s;
+}
+void or_destroy(void* h) { delete (Sim*)h; }
+void or_set_dense(void* h, int dense) { ((Sim*)h)->dense_mode = dense != 0; }
+int or_sizes(void* h, int64_t* n, int64_t* nu, int64_t* ns) {
+    Sim* s = (Sim*)h;
+    *n = s->n;
+    *nu = s->nu;
+    *ns = s->ns;
+    return 0;
+}
+void or_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+int or_get_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+int or_initial_configuration(void* h, const double* x0, const double* q0) {
+    // src/Stokes.cpp:254-317.
+    Sim* s = (Sim*)h;
+    std::memcpy(s->x.data(), x0, sizeof(double) * 3 * s->Np);
+    if (s->p.restart == 0.0)
+        for (int64_t ii = 0; ii < s->nc3; ii++) s->xpos0[ii] = s->x[ii + s->nm3];
+    for (int i = 0; i < s->Nc; i++) {
+        for (int k = 0; k < 4; k++) s->q[4 * i + k] = q0 ? q0[4 * i + k] : (k == 0 ? 1.0 : 0.0);
+    }
+    return 0;
+}
+int or_set_positions(void* h, const double* x) {
+    Sim* s = (Sim*)h;
+    std::memcpy(s->x.data(), x, sizeof(double) * 3 * s->Np);
+    return 0;
+}
+int or_get_state(void* h, double* x, double* q, double* t) {
+    Sim* s = (Sim*)h;
+    if (x) std::memcpy(x, s->x.data(), sizeof(double) * 3 * s->Np);
+    if (q) std::memcpy(q, s->q.data(), sizeof(double) * 4 * s->Nc);
+    if (t) *t = s->t;
+    return 0;
+}
+int or_get_vectors(void* h, double* up, double* vdn, double* vdc, double* vb, double* ud, double* sb,
+                   double* fext) {
+    Sim* s = (Sim*)h;
+    size_t b = sizeof(double) * s->nu;
+    if (up) std::memcpy(up, s->up.data(), b);
+    if (vdn) std::memcpy(vdn, s->v_det_n.data(), b);
+    if (vdc) std::memcpy(vdc, s->v_det_c.data(), b);
+    if (vb) std::memcpy(vb, s->v_brown.data(), b);
+    if (ud) std::memcpy(ud, s->udiff.data(), b);
+    if (sb) std::memcpy(sb, s->sb.data(), sizeof(double) * 5 * s->Nc);
+    if (fext) std::memcpy(fext, s->fext.data(), b);
+    return 0;
+}
+int64_t or_check_dist(void* h) {
+    Sim* s = (Sim*)h;
+    check_dist(*s);
+    int64_t c = 0;
+    for (auto& v : s->vlist) c += (int64_t)v.size();
+    return c;
+}
+int64_t or_get_vlist(void* h, int32_t* rows, int32_t* partners, int64_t* pidx, int64_t cap) {
+    Sim* s = (Sim*)h;
+    int64_t c = 0;
+    for (int k = 0; k < (int)s->vlist.size(); k++)
+        for (int j : s->vlist[k]) {
+            if (c < cap) {
+                rows[c] = k;
+                partners[c] = j;
+                pidx[c] = pair_index(*s, k, j);
+            }
+            c++;
+        }
+    return c;
+}
+int or_mob(void* h) {
+    Sim* s = (Sim*)h;
+    if (!s->have_pd) return PS_ERR_STATE;
+    mob(*s);
+    return 0;
+}
+int or_stale_tile(void* h, double* t9) {
+    Sim* s = (Sim*)h;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) t9[3 * i + j] = s->stale[i][j];
+    return 0;
+}
+int or_mob_dense(void* h, double* M) {
+    // Dense n x n grand mobility as assembled by mob() (row-major).
+    Sim* s = (Sim*)h;
+    if (!s->have_mob) return PS_ERR_STATE;
+    int64_t n = s->n;
+    std::fill(M, M + n * n, 0.0);
+    auto ins = [M, n](int64_t r, int64_t c, double v) { M[r * n + c] = v; };
+    emit_self(*s, ins);
+    for (int i = 0; i < s->Nm - 1; i++) emit_AA_row(*s, i, ins);
+    for (int m = 0; m < s->Nm; m++) emit_AB_row(*s, m, ins);
+    for (int c1 = s->Nm; c1 < s->Np; c1++)
+        for (int c2 = c1 + 1; c2 < s->Np; c2++) emit_BB_pair(*s, c1, c2, ins);
+    return 0;
+}
+int or_mob_mult(void* h, const double* f, double* u) {
+    Sim* s = (Sim*)h;
+    if (!s->have_mob) return PS_ERR_STATE;
+    mob_apply(*s, f, u);
+    return 0;
+}
+int or_saddle_mult(void* h, const double* x, double* y) {
+    Sim* s = (Sim*)h;
+    if (!s->have_mob) return PS_ERR_STATE;
+    saddle_apply(*s, x, y);
+    return 0;
+}
+int or_saddle_diag(void* h, double* d) {
+    saddle_diag(*(Sim*)h, d);
+    return 0;
+}
+int or_saddle_solve(void* h, const double* b, double* x, int warm, ps_solve_info* info) {
+    Sim* s = (Sim*)h;
+    if (!s->have_mob) return PS_ERR_STATE;
+    return minres(*s, b, x, warm != 0, info);
+}
+int or_sqrt_apply(void* h, const double* xi, double* w, ps_sqrt_info* info) {
+    Sim* s = (Sim*)h;
+    if (!s->have_mob) return PS_ERR_STATE;
+    sqrt_apply(*s, xi, w, info);
+    return 0;
+}
+int or_rhs(void* h, int drift, double* rhs) {
+    Sim* s = (Sim*)h;
+    if (!s->have_pd) return PS_ERR_STATE;
+    rhs_build(*s, drift != 0, rhs);
+    return 0;
+}
+int or_run(void* h, int nsteps) {
+    Sim* s = (Sim*)h;
+    int rc = 0;
+    for (int i = 0; i < nsteps; i++) rc |= run_step(*s);
+    return rc;
+}
+int or_draw_normals(void* h, double* out, int64_t n) {
+    Sim* s = (Sim*)h;
+    for (int64_t i = 0; i < n; i++) out[i] = s->nd(s->rng);
+    return 0;
+}
+int or_tridiag_sqrt(int m, const double* alpha, const double* betad, double beta0, double* g) {
+    return tridiag_sqrt_weights(m, alpha, betad, beta0, g);
+}
+int64_t or_counters(void* h, int64_t* applies, int64_t* minres_total, int* last_minres, int* last_sqrt) {
+    Sim* s = (Sim*)h;
+    if (applies) *applies = s->applies;
+    if (minres_total) *minres_total = s->minres_total;
+    if (last_minres) *last_minres = s->last_minres_its;
+    if (last_sqrt) *last_sqrt = s->last_sqrt_its;
+    return 0;
+}
+// Raw generators for the known-answer tests.
+uint64_t or_mt19937_64_nth(uint64_t seed, int64_t nth) {
+    std::mt19937_64 g(seed);
+    uint64_t v = 0;
+    for (int64_t i = 0; i < nth; i++) v = g();
+    return v;
+}
+void or_normals(uint64_t seed, int64_t n, double* out) {
+    std::mt19937_64 g(seed);
+    std::normal_distribution<double> d(0.0, 1.0);
+    for (int64_t i = 0; i < n; i++) out[i] = d(g);
+}
+
+}  // extern "C"

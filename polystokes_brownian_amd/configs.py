@@ -1,0 +1,184 @@
+"""Synthetic workloads in the reference's units.
+
+Unit mapping (one decision, used by every workload): the reference's length unit is the probe
+(colloid) radius and its mobility unit sets 6*pi*eta = 1 (src/mob.cpp:65-86, :572-583).  The
+workloads are specified with bead radius a = 1 and probe radius 5a, so
+    beta = 0.2,  lengths x 0.2,  spring constants x 25 (energy / length^2),
+    time x beta^3 = 0.008 (tau is the bead's Stokes time 6*pi*eta*a^3/kT),
+so dt = 1e-3 tau -> 8e-6, kbond = 100 -> 2500, r0 = 2.1a -> 0.42, ktrap = 500 -> 12500,
+Lmax = 3a -> 0.6, box 60a -> 12, 150a -> 30, 320a -> 64, 550a -> 110.
+
+Initial configurations follow the reference's generator conventions (seeded uniform probe centres,
+isotropic random-walk chains with minimum-image overlap rejection; test_sim_dumbbells_thermal.py
+:93-174): bead-bead >= 2*beta, bead-probe >= 1+beta, probe-probe >= 2.
+"""
+import math
+
+import numpy as np
+
+BETA = 0.2
+LSCALE = BETA
+KSCALE = 1.0 / BETA ** 2
+TSCALE = BETA ** 3
+
+
+def _min_image(d, L):
+    return d - L * np.rint(d / L)
+
+
+def _wrap(x, L):
+    return x - L * np.rint(x / L)
+
+
+def make_configuration(Nc, Npoly, Nmono, L, r0, seed, probe_positions=None, beta=BETA, max_tries=2000):
+    """Positions (3*Np,) for Nm = Npoly*Nmono monomers followed by Nc probes, wrapped in [-L/2, L/2)."""
+    rng = np.random.default_rng(seed)
+    Nm = Npoly * Nmono
+    x = np.zeros((Nm + Nc, 3))
+    # probes
+    if probe_positions is not None:
+        probes = np.asarray(probe_positions, dtype=float).reshape(Nc, 3)
+    else:
+        probes = np.zeros((Nc, 3))
+        k = 0
+        tries = 0
+        while k < Nc:
+            c = rng.uniform(-L / 2, L / 2, 3)
+            if k == 0 or np.all(np.linalg.norm(_min_image(probes[:k] - c, L), axis=1) >= 2.0 + 0.05):
+                probes[k] = c
+                k += 1
+            tries += 1
+            if tries > 1000 * max(Nc, 1):
+                raise RuntimeError("could not place probes")
+    x[Nm:] = probes
+    # cell grid for fast overlap tests of monomers
+    cell = 1.0 + beta
+    ncell = max(1, int(L // cell))
+    grid = {}
+
+    def key(p):
+        return tuple(((np.floor((_wrap(p, L) + L / 2) / (L / ncell))).astype(int)) % ncell)
+
+    def ok(p, skip=-1):
+        dprobe = np.linalg.norm(_min_image(probes - p, L), axis=1) if Nc else np.zeros(0)
+        if np.any(dprobe < 1.0 + beta):
+            return False
+        kx, ky, kz = key(p)
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for j in grid.get(((kx + dx) % ncell, (ky + dy) % ncell, (kz + dz) % ncell), ()):
+                        if j == skip:
+                            continue
+                        if np.linalg.norm(_min_image(x[j] - p, L)) < 2 * beta:
+                            return False
+        return True
+
+    def add(i, p):
+        x[i] = p
+        grid.setdefault(key(p), []).append(i)
+
+    def remove(i):
+        grid[key(x[i])].remove(i)
+
+    for c in range(Npoly):
+        for attempt in range(max_tries):
+            start = rng.uniform(-L / 2, L / 2, 3)
+            if not ok(start):
+                continue
+            placed = [c * Nmono]
+            add(c * Nmono, start)
+            good = True
+            for m in range(1, Nmono):
+                i = c * Nmono + m
+                prev = x[i - 1]
+                for _ in range(200):
+                    v = rng.normal(size=3)
+                    v /= np.linalg.norm(v)
+                    p = prev + r0 * v
+                    if ok(p, skip=i - 1):
+                        add(i, p)
+                        placed.append(i)
+                        break
+                else:
+                    good = False
+                    break
+            if good:
+                break
+            for i in placed:
+                remove(i)
+        else:
+            raise RuntimeError("could not place chain %d" % c)
+    return _wrap(x, L).reshape(-1)
+
+
+def workload(name):
+    """Parameter dict + generator arguments for a named workload."""
+    base = dict(
+        mm_HI=1, fene=0, record_forces=1, tether=0, mono_ev=0, seed=12345, restart=0.0,
+        kT=1.0, epsilon=1.0, beta=BETA, kbond=100.0 * KSCALE, r0=2.1 * LSCALE, Lmax=3.0 * LSCALE,
+        tau=1000.0, ktrap=500.0 * KSCALE, tstart=0.0, trun=1e9, weaken_trap=-1.0,
+        dt=1e-3 * TSCALE, samplerate=1 << 30,
+    )
+    if name == "c1":
+        d = dict(base, Nc=2, Npoly=11, Nmono=20, L=60 * LSCALE, chain_HI=1, ksp_rtol=1e-10,
+                 sqrt_rtol=1e-8, sqrt_kmax=64, steps=100, probes=[[-3.0, 0, 0], [3.0, 0, 0]], ic_seed=42)
+    elif name == "c2":
+        d = dict(base, Nc=20, Npoly=99, Nmono=20, L=150 * LSCALE, chain_HI=0, fene=1, ksp_rtol=1e-10,
+                 sqrt_rtol=1e-8, sqrt_kmax=64, steps=200, probes=None, ic_seed=42)
+    elif name == "c3":
+        d = dict(base, Nc=200, Npoly=990, Nmono=20, L=320 * LSCALE, chain_HI=0, ksp_rtol=1e-6,
+                 sqrt_rtol=1e-6, sqrt_kmax=32, steps=2, probes=None, ic_seed=42)
+    elif name == "c5":
+        d = dict(base, Nc=1000, Npoly=4950, Nmono=20, L=550 * LSCALE, chain_HI=0, ksp_rtol=1e-6,
+                 sqrt_rtol=0.0, sqrt_kmax=40, steps=1, probes=None, ic_seed=42)
+    elif name == "c5dense":
+        d = dict(base, Nc=80, Npoly=396, Nmono=20, L=None, chain_HI=0, ksp_rtol=1e-6,
+                 sqrt_rtol=1e-6, sqrt_kmax=32, steps=1, probes=None, ic_seed=7)
+        # same density as the 100k case
+        d["L"] = 550 * LSCALE * (8000 / 100000) ** (1 / 3)
+    else:
+        raise KeyError(name)
+    d["Nm"] = d["Npoly"] * d["Nmono"]
+    d["Np"] = d["Nm"] + d["Nc"]
+    return d
+
+
+def small(Nc, Npoly, Nmono, L, seed=1, chain_HI=0, **over):
+    """A small workload for parity tests."""
+    d = workload("c1")
+    d.update(dict(Nc=Nc, Npoly=Npoly, Nmono=Nmono, L=L, chain_HI=chain_HI, probes=None, ic_seed=seed))
+    d.update(over)
+    d["Nm"] = d["Npoly"] * d["Nmono"]
+    d["Np"] = d["Nm"] + d["Nc"]
+    return d
+
+
+def params_dict(w):
+    """Split a workload into the ps_params fields."""
+    keys = ["mm_HI", "chain_HI", "fene", "record_forces", "tether", "mono_ev", "seed", "restart", "kT",
+            "epsilon", "beta", "kbond", "r0", "Lmax", "tau", "ktrap", "tstart", "trun", "weaken_trap", "dt",
+            "samplerate", "Nc", "Nm", "Np", "Npoly", "ksp_rtol", "sqrt_rtol", "sqrt_kmax"]
+    d = {k: w[k] for k in keys if k in w}
+    d["Nmono_per_chain"] = w["Nmono"]
+    d["box"] = [w["L"]] * 3 if w.get("L") else None
+    d["tmax"] = w["dt"] * w.get("steps", 1)
+    return d
+
+
+def positions(w):
+    return make_configuration(w["Nc"], w["Npoly"], w["Nmono"], w["L"], w["r0"], w["ic_seed"],
+                              probe_positions=w.get("probes"))
+
+
+def replica_sizes(R=256, seed=1000, lo=100, hi=3000):
+    """Bead counts of the batched-replica workload: N_r = round(exp(U(ln lo, ln hi))), 10% probes,
+    the rest 20-bead chains."""
+    out = []
+    for r in range(R):
+        rng = np.random.default_rng(seed + r)
+        N = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        Nc = max(1, int(round(0.1 * N)))
+        Npoly = max(1, (N - Nc) // 20)
+        out.append((Nc, Npoly, 20))
+    return out

@@ -1,0 +1,675 @@
+// capi.hip -- the C ABI (include/polystokes_gpu.h), the integrator kernels and the run() loop.
+//
+// The step loop restates PolyStokes::run (src/run.cpp:70-162) with every array resident in HBM:
+// predictor (check_dist, RHS, mob, deterministic solve), Brownian slip (M^{1/2} xi, second solve),
+// RFD thermal drift (two probe solves at x +/- eps W, src/drift.cpp:17-62), Euler probe step
+// (src/traj.cpp:9-35), corrector (check_dist, RHS, mob, solve), trapezoidal update from x_n
+// (src/traj.cpp:37-62) and the quaternion update (src/traj.cpp:64-94, src/quat.cpp:40-54).
+// The noise stream is the reference's: std::mt19937_64 seeded with seed + llround(restart/dt) and
+// std::normal_distribution<double>, n draws for the slip then 3Nm+6Nc for the RFD per step
+// (src/slip_vel.cpp:11-17, src/drift.cpp:9-15), drawn on the host into pinned memory.
+#include "ps_common.h"
+
+#include <algorithm>
+#include <cmath>
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int TB = PS_BLOCK;
+inline unsigned eb(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + TB - 1) / TB); }
+
+// step(): undo the RFD probe, add the drift and the velocity, wrap (src/traj.cpp:9-35).
+__global__ void k_step(SysConst s, double* __restrict__ x, const double* __restrict__ fext,
+                       const double* __restrict__ udiff, const double* __restrict__ up, double dt, double kT) {
+    const int p = blockIdx.x * TB + threadIdx.x;
+    if (p >= s.Np) return;
+    const double eps = 1.0e-3;
+    double v[3];
+    for (int d = 0; d < 3; d++) {
+        const int64_t i = 3 * (int64_t)p + d;
+        double xi = x[i];
+        if (kT > 0) {
+            const double drift_scale = kT / (2.0 * eps);
+            xi += eps * fext[i];
+            xi += drift_scale * udiff[i] * dt;
+        }
+        xi += up[i] * dt;
+        v[d] = xi;
+    }
+    if (s.box_active)
+        for (int d = 0; d < 3; d++) v[d] -= s.L[d] * __builtin_rint(v[d] * s.Linv[d]);
+    for (int d = 0; d < 3; d++) x[3 * (int64_t)p + d] = v[d];
+}
+
+// step_from(x_base) (src/traj.cpp:37-62).
+__global__ void k_step_from(SysConst s, double* __restrict__ x, const double* __restrict__ xb,
+                            const double* __restrict__ udiff, const double* __restrict__ up, double dt, double kT) {
+    const int p = blockIdx.x * TB + threadIdx.x;
+    if (p >= s.Np) return;
+    const double eps = 1.0e-3;
+    double v[3];
+    for (int d = 0; d < 3; d++) {
+        const int64_t i = 3 * (int64_t)p + d;
+        if (kT > 0) {
+            const double drift_scale = kT / (2.0 * eps);
+            v[d] = xb[i] + drift_scale * udiff[i] * dt + up[i] * dt;
+        } else {
+            v[d] = xb[i] + up[i] * dt;
+        }
+    }
+    if (s.box_active)
+        for (int d = 0; d < 3; d++) v[d] -= s.L[d] * __builtin_rint(v[d] * s.Linv[d]);
+    for (int d = 0; d < 3; d++) x[3 * (int64_t)p + d] = v[d];
+}
+
+// x += a * w over the first n3 entries (the RFD probe displacements, src/drift.cpp:33,48).
+__global__ void k_probe(double* __restrict__ x, const double* __restrict__ w, double a, int64_t n3) {
+    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i < n3) x[i] += a * w[i];
+}
+
+// step_quaternion (src/traj.cpp:64-94, src/quat.cpp:4-54): q <- normalize(q + dt/2 q (x) (0, omega)).
+__global__ void k_quat(int Nc, int64_t nm3nc3, double* __restrict__ q, const double* __restrict__ up, double dt) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= Nc) return;
+    double* qi = q + 4 * (int64_t)i;
+    const double* om = up + nm3nc3 + 3 * (int64_t)i;
+    const double w1 = qi[0], x1 = qi[1], y1 = qi[2], z1 = qi[3];
+    const double w2 = 0.0, x2 = om[0], y2 = om[1], z2 = om[2];
+    const double dq[4] = {w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                          w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2};
+    double qn[4];
+    for (int k = 0; k < 4; k++) qn[k] = qi[k] + 0.5 * dq[k] * dt;
+    const double norm = __builtin_sqrt(qn[0] * qn[0] + qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]);
+    for (int k = 0; k < 4; k++) qi[k] = qn[k] / norm;
+}
+
+// out[i] = X[off + i]
+__global__ void k_extract(double* __restrict__ out, const double* __restrict__ X, int64_t off, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i < n) out[i] = X[off + i];
+}
+// c = a - b
+__global__ void k_sub(double* __restrict__ c, const double* __restrict__ a, const double* __restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i < n) c[i] = a[i] - b[i];
+}
+// up = 0.5 (v_det_n + v_det_c) + v_brown
+__global__ void k_trapz(double* __restrict__ up, const double* __restrict__ a, const double* __restrict__ b,
+                        const double* __restrict__ vb, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i < n) up[i] = 0.5 * (a[i] + b[i]) + vb[i];
+}
+// rhs[0:n] += a * w
+__global__ void k_add_slip(double* __restrict__ rhs, const double* __restrict__ w, double a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i < n) rhs[i] += a * w[i];
+}
+// sb = X[nu : nu+nc5] (first probe) / (sb - X[...]) / (2 eps) (second probe)
+__global__ void k_sb(double* __restrict__ sb, const double* __restrict__ X, int64_t nu, int64_t nc5, int second) {
+    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i >= nc5) return;
+    const double eps = 1.0e-3;
+    if (second) sb[i] = (sb[i] - X[nu + i]) / (2 * eps);
+    else sb[i] = X[nu + i];
+}
+
+ps_err draw_noise(Ctx* c, double* dst_dev, int64_t n) {
+    // The reference's host stream, drawn in order into pinned memory, copied to HBM.
+    PS_HIP(c, hipStreamSynchronize(c->stream));   // the pinned buffer may still be in flight
+    for (int64_t i = 0; i < n; i++) c->noise_h[i] = c->nd(c->rng);
+    PS_HIP(c, hipMemcpyAsync(dst_dev, c->noise_h, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    return PS_OK;
+}
+
+ps_err solve(Ctx* c, double* X, int warm) {
+    ps_solve_info info{};
+    ps_err r = saddle_solve_dev(c, c->rhs, X, warm, &info);
+    if (r == PS_ERR_NOCONV) r = PS_OK;   // the reference prints and continues (src/vel.cpp:45-51)
+    return r;
+}
+
+// One iteration of the run() loop.
+ps_err one_step(Ctx* c) {
+    const int64_t n = c->n, nu = c->nu, n3 = 3 * (int64_t)c->Np, nc5 = 5 * (int64_t)c->Nc;
+    hipStream_t st = c->stream;
+    const double dt = c->p.dt, kT = c->p.kT, eps = 1.0e-3;
+    c->t += dt;
+    PS_TRY(vec_copy(c, c->x_n, c->x, n3));
+    // ---- predictor
+    PS_TRY(check_dist_dev(c));
+    PS_TRY(rhs_dev(c, 0));
+    PS_TRY(mob_prepare(c));
+    PS_TRY(solve(c, c->Xdet, 0));
+    k_extract<<<eb(nu), TB, 0, st>>>(c->v_det_n, c->Xdet, n, nu);
+    PS_LAUNCH(c);
+    if (kT > 0) {
+        // solve_slip_vel: W = sqrt(M) xi, rhs[0:n] += sqrt(2kT/dt) W
+        PS_TRY(draw_noise(c, c->xi, n));
+        PS_TRY(sqrt_apply_dev(c, c->xi, c->W, nullptr));
+        k_add_slip<<<eb(n), TB, 0, st>>>(c->rhs, c->W, std::sqrt(2 * kT / dt), n);
+        PS_LAUNCH(c);
+        // new_vel
+        PS_TRY(solve(c, c->X, 0));
+        k_extract<<<eb(nu), TB, 0, st>>>(c->up, c->X, n, nu);
+        PS_LAUNCH(c);
+        if (c->p.record_forces) {
+            k_extract<<<eb(nu), TB, 0, st>>>(c->fext, c->X, 0, nu);
+            PS_LAUNCH(c);
+        }
+        k_sub<<<eb(nu), TB, 0, st>>>(c->v_brown, c->up, c->v_det_n, nu);
+        PS_LAUNCH(c);
+        // drift(): RFD probes at x +/- eps W
+        PS_TRY(draw_noise(c, c->fext, nu));
+        k_probe<<<eb(n3), TB, 0, st>>>(c->x, c->fext, eps, n3);
+        PS_LAUNCH(c);
+        PS_TRY(check_dist_dev(c));
+        PS_TRY(mob_prepare(c));
+        PS_TRY(rhs_dev(c, 1));
+        PS_TRY(solve(c, c->Xd, 0));
+        k_extract<<<eb(nu), TB, 0, st>>>(c->drift_v, c->Xd, n, nu);
+        PS_LAUNCH(c);
+        if (nc5 > 0) {
+            k_sb<<<eb(nc5), TB, 0, st>>>(c->sb, c->Xd, nu, nc5, 0);
+            PS_LAUNCH(c);
+        }
+        k_probe<<<eb(n3), TB, 0, st>>>(c->x, c->fext, -2.0 * eps, n3);
+        PS_LAUNCH(c);
+        PS_TRY(check_dist_dev(c));
+        PS_TRY(mob_prepare(c));
+        PS_TRY(rhs_dev(c, 1));
+        PS_TRY(solve(c, c->Xd, 1));
+        k_extract<<<eb(nu), TB, 0, st>>>(c->udiff, c->Xd, n, nu);
+        PS_LAUNCH(c);
+        k_sub<<<eb(nu), TB, 0, st>>>(c->udiff, c->udiff, c->drift_v, nu);
+        PS_LAUNCH(c);
+        if (nc5 > 0) {
+            k_sb<<<eb(nc5), TB, 0, st>>>(c->sb, c->Xd, nu, nc5, 1);
+            PS_LAUNCH(c);
+        }
+    } else {
+        PS_TRY(solve(c, c->X, 0));
+        k_extract<<<eb(nu), TB, 0, st>>>(c->up, c->X, n, nu);
+        PS_LAUNCH(c);
+        if (c->p.record_forces) {
+            k_extract<<<eb(nu), TB, 0, st>>>(c->fext, c->X, 0, nu);
+            PS_LAUNCH(c);
+        }
+        PS_HIP(c, hipMemsetAsync(c->v_brown, 0, sizeof(double) * (size_t)nu, st));
+    }
+    k_step<<<eb(c->Np), TB, 0, st>>>(c->sc, c->x, c->fext, c->udiff, c->up, dt, kT);
+    PS_LAUNCH(c);
+    // ---- corrector
+    PS_TRY(check_dist_dev(c));
+    PS_TRY(rhs_dev(c, 0));
+    PS_TRY(mob_prepare(c));
+    PS_TRY(solve(c, c->Xdet, 0));
+    k_extract<<<eb(nu), TB, 0, st>>>(c->v_det_c, c->Xdet, n, nu);
+    PS_LAUNCH(c);
+    k_trapz<<<eb(nu), TB, 0, st>>>(c->up, c->v_det_n, c->v_det_c, c->v_brown, nu);
+    PS_LAUNCH(c);
+    k_step_from<<<eb(c->Np), TB, 0, st>>>(c->sc, c->x, c->x_n, c->udiff, c->up, dt, kT);
+    PS_LAUNCH(c);
+    if (c->Nc > 0) {
+        k_quat<<<eb(c->Nc), TB, 0, st>>>(c->Nc, 3 * (int64_t)c->Np, c->q, c->up, dt);
+        PS_LAUNCH(c);
+    }
+    c->have_mob = true;
+    return PS_OK;
+}
+
+Ctx* as_ctx(void* h) { return static_cast<Ctx*>(h); }
+
+constexpr int EV_POOL = 4096;   // event pairs held before a harvest
+
+ps_err fail(Ctx* c, ps_err code, const std::string& msg) {
+    if (c) c->set_error(msg);
+    return code;
+}
+
+template <class F>
+ps_err with_io(Ctx* c, int64_t na, const double* in, int64_t nb, double* out, F&& fn) {
+    // host in -> device a; fn(a, b); device b -> host out
+    PS_HIP(c, hipMemcpyAsync(c->io_a, in, sizeof(double) * (size_t)na, hipMemcpyHostToDevice, c->stream));
+    PS_TRY(fn(c->io_a, c->io_b));
+    PS_HIP(c, hipMemcpyAsync(out, c->io_b, sizeof(double) * (size_t)nb, hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+
+void destroy_ctx(Ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    c->ev_pool.clear();
+    apply_free(c);
+    krylov_free(c);
+    pairs_free(c);
+    for (void* p : {(void*)c->x, (void*)c->xpd, (void*)c->xm, (void*)c->xpos0, (void*)c->q, (void*)c->x_n,
+                    (void*)c->chain, (void*)c->up, (void*)c->udiff, (void*)c->v_det_n, (void*)c->v_det_c,
+                    (void*)c->v_brown, (void*)c->fext, (void*)c->drift_v, (void*)c->sb, (void*)c->rhs,
+                    (void*)c->X, (void*)c->Xdet, (void*)c->Xd, (void*)c->W, (void*)c->xi, (void*)c->stale,
+                    (void*)c->io_a, (void*)c->io_b})
+        if (p) (void)hipFree(p);
+    if (c->noise_h) (void)hipHostFree(c->noise_h);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+ps_err Ctx::prof_harvest() {
+    Ctx* c = this;
+    if (ev_used == 0) return PS_OK;
+    PS_HIP(c, hipStreamSynchronize(stream));
+    for (int k = 0; k < ev_used; k++) {
+        float ms = 0.f;
+        PS_HIP(c, hipEventElapsedTime(&ms, ev_pool[2 * k], ev_pool[2 * k + 1]));
+        if (ev_kind[k] == 0) {
+            prof.aa_launches++;
+            prof.aa_ms += ms;
+        } else {
+            prof.apply_launches++;
+            prof.apply_ms += ms;
+        }
+    }
+    ev_used = 0;
+    return PS_OK;
+}
+
+ps_err Ctx::prof_record(int kind, hipEvent_t* a, hipEvent_t* b) {
+    Ctx* c = this;
+    if (ev_pool.empty()) {
+        ev_pool.resize(2 * EV_POOL);
+        ev_kind.resize(EV_POOL);
+        for (auto& e : ev_pool) PS_HIP(c, hipEventCreate(&e));
+    }
+    if (ev_used >= EV_POOL) PS_TRY(prof_harvest());
+    ev_kind[ev_used] = kind;
+    *a = ev_pool[2 * ev_used];
+    *b = ev_pool[2 * ev_used + 1];
+    ev_used++;
+    return PS_OK;
+}
+
+extern "C" {
+
+ps_err ps_profile_begin(void* h) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    PS_TRY(c->prof_harvest());
+    c->prof = ps_profile{};
+    c->prof.aa_pairs = (int64_t)c->Nm * c->Nm;
+    c->prof_on = true;
+    return PS_OK;
+}
+
+ps_err ps_profile_end(void* h, ps_profile* out) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    PS_TRY(c->prof_harvest());
+    c->prof_on = false;
+    if (out) *out = c->prof;
+    return PS_OK;
+}
+
+void ps_params_default(ps_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    // PolyStokes ctor defaults as bound (src/bindings.cpp:374-387)
+    p->samplerate = 1;
+    p->mm_HI = 1;
+    p->chain_HI = 0;
+    p->fene = 1;
+    p->record_forces = 1;
+    p->tether = 0;
+    p->mono_ev = 0;
+    p->box_active = 0;
+    p->t = 0.0;
+    p->seed = 12345;
+    p->restart = 0.0;
+    p->weaken_trap = -1;
+    // KSP: rtol 1e-6 (src/init.cpp:64), PETSc default abstol / maxit
+    p->ksp_rtol = 1e-6;
+    p->ksp_abstol = 1e-50;
+    p->ksp_maxit = 10000;
+    p->sqrt_rtol = 1e-8;
+    p->sqrt_kmax = 64;
+    p->force_sel = PS_FORCE_HS;
+    p->hs_kpref = 1.0;
+    p->hs_kmode = 0;
+    p->hs_exp_k = 100.0;
+    p->precision = PS_PREC_FP64;
+    p->ir_rtol = 1e-6;
+    p->ir_maxit = 20;
+}
+
+const char* ps_last_error(const void* ctx) {
+    if (!ctx) return g_create_error.c_str();
+    return static_cast<const Ctx*>(ctx)->err.c_str();
+}
+
+ps_err ps_create(const ps_params* p, void** out) {
+    if (!p || !out) return PS_ERR_ARG;
+    *out = nullptr;
+    g_create_error.clear();
+    if (p->Np != p->Nm + p->Nc || p->Np < 2 || p->Nm < 0 || p->Nc < 0) {
+        g_create_error = "Np must equal Nm + Nc and be at least 2";
+        return PS_ERR_ARG;
+    }
+    if (!p->mm_HI) {
+        g_create_error = "only the mm_HI=True (full grand mobility) regime is implemented";
+        return PS_ERR_ARG;
+    }
+    if (p->dt <= 0 || p->beta <= 0) {
+        g_create_error = "dt and beta must be positive";
+        return PS_ERR_ARG;
+    }
+    if (p->box_active && (p->box[0] <= 0 || p->box[1] <= 0 || p->box[2] <= 0)) {
+        g_create_error = "Box lengths must be positive";   // Box::configure (src/box.cpp:10-14)
+        return PS_ERR_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_create_error = "no HIP device visible";
+        return PS_ERR_NODEV;
+    }
+    Ctx* c = new Ctx();
+    c->p = *p;
+    c->Np = p->Np;
+    c->Nm = p->Nm;
+    c->Nc = p->Nc;
+    c->n = 3 * (int64_t)c->Nm + 11 * (int64_t)c->Nc;
+    c->nu = 3 * (int64_t)c->Nm + 6 * (int64_t)c->Nc;
+    c->ns = c->n + c->nu;
+    SysConst& s = c->sc;
+    s.Np = c->Np;
+    s.Nm = c->Nm;
+    s.Nc = c->Nc;
+    s.chain_HI = p->chain_HI;
+    s.box_active = p->box_active;
+    for (int d = 0; d < 3; d++) {
+        s.L[d] = p->box_active ? p->box[d] : 0.0;
+        s.Linv[d] = p->box_active ? 1.0 / p->box[d] : 0.0;
+    }
+    s.beta = p->beta;
+    s.beta2 = p->beta * p->beta;
+    s.beta_inv = 1.0 / p->beta;
+    s.two_beta = 2.0 * p->beta;
+    s.n = c->n;
+    s.nu = c->nu;
+    s.ns = c->ns;
+    s.nm3 = 3 * (int64_t)c->Nm;
+    s.nc3 = 3 * (int64_t)c->Nc;
+    auto bail = [&](ps_err code) {
+        g_create_error = c->err;
+        destroy_ctx(c);
+        return code;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess) return bail(PS_ERR_HIP);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        c->err = "hipStreamCreate failed";
+        return bail(PS_ERR_HIP);
+    }
+    const size_t n3 = 3 * (size_t)c->Np;
+    auto dalloc = [&](double** ptr, size_t cnt) -> bool {
+        return hipMalloc(ptr, sizeof(double) * std::max<size_t>(cnt, 1)) == hipSuccess;
+    };
+    bool ok = dalloc(&c->x, n3) && dalloc(&c->xpd, n3) && dalloc(&c->xm, n3) && dalloc(&c->xpos0, 3 * (size_t)c->Nc) &&
+              dalloc(&c->q, 4 * (size_t)c->Nc) && dalloc(&c->x_n, n3) && dalloc(&c->up, c->nu) &&
+              dalloc(&c->udiff, c->nu) && dalloc(&c->v_det_n, c->nu) && dalloc(&c->v_det_c, c->nu) &&
+              dalloc(&c->v_brown, c->nu) && dalloc(&c->fext, c->nu) && dalloc(&c->drift_v, c->nu) &&
+              dalloc(&c->sb, 5 * (size_t)c->Nc) && dalloc(&c->rhs, c->ns) && dalloc(&c->X, c->ns) &&
+              dalloc(&c->Xdet, c->ns) && dalloc(&c->Xd, c->ns) && dalloc(&c->W, c->n) && dalloc(&c->xi, c->n) &&
+              dalloc(&c->stale, 18) && dalloc(&c->io_a, c->ns) && dalloc(&c->io_b, c->ns);
+    if (!ok) {
+        c->err = "device allocation failed";
+        return bail(PS_ERR_ALLOC);
+    }
+    if (hipHostMalloc(&c->noise_h, sizeof(double) * (size_t)(c->n + c->nu), hipHostMallocDefault) != hipSuccess) {
+        c->err = "pinned allocation failed";
+        return bail(PS_ERR_ALLOC);
+    }
+    if (hipMalloc(&c->chain, sizeof(int) * std::max(c->Nm, 1)) != hipSuccess) return bail(PS_ERR_ALLOC);
+    // chain ids (src/init.cpp:205-212); same-chain pairs skipped under chain_HI
+    std::vector<int> chain(std::max(c->Nm, 1), 0);
+    for (int ii = 0; ii < p->Npoly; ii++)
+        for (int jj = 0; jj < p->Nmono_per_chain; jj++) {
+            const int kk = ii * p->Nmono_per_chain + jj;
+            if (kk < c->Nm) chain[kk] = ii;
+        }
+    if (hipMemcpy(c->chain, chain.data(), sizeof(int) * chain.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(PS_ERR_HIP);
+    // last processed AA pair of mob()'s monomer loop (src/mob.cpp:845-897)
+    c->stale_i = c->stale_j = -1;
+    for (int i = c->Nm - 2; i >= 0 && c->stale_i < 0; i--)
+        for (int j = c->Nm - 1; j > i; j--) {
+            if (p->chain_HI && chain[i] == chain[j]) continue;
+            c->stale_i = i;
+            c->stale_j = j;
+            break;
+        }
+    // mob_a after fill_self is the identity (src/mob.cpp:65-86, :568)
+    double st0[18] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (hipMemcpy(c->stale, st0, sizeof(st0), hipMemcpyHostToDevice) != hipSuccess) return bail(PS_ERR_HIP);
+    c->ktrap_cur = p->ktrap;
+    c->t = p->restart;   // src/Stokes.cpp:21
+    c->rng.seed(p->seed + (uint64_t)std::llround(p->restart / p->dt));
+    c->nd = std::normal_distribution<double>(0.0, 1.0);
+    if (apply_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
+    if (krylov_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
+    if (pairs_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
+    for (double* v : {c->up, c->udiff, c->v_det_n, c->v_det_c, c->v_brown, c->fext, c->drift_v})
+        (void)hipMemsetAsync(v, 0, sizeof(double) * (size_t)c->nu, c->stream);
+    for (double* v : {c->X, c->Xdet, c->Xd})
+        (void)hipMemsetAsync(v, 0, sizeof(double) * (size_t)c->ns, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(PS_ERR_HIP);
+    *out = c;
+    return PS_OK;
+}
+
+ps_err ps_destroy(void* h) {
+    destroy_ctx(as_ctx(h));
+    return PS_OK;
+}
+
+ps_err ps_device_info(void* h, char* name, int name_len, int* n_cu) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    hipDeviceProp_t prop;
+    PS_HIP(c, hipGetDeviceProperties(&prop, c->device));
+    if (name && name_len > 0) std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    if (n_cu) *n_cu = prop.multiProcessorCount;
+    return PS_OK;
+}
+
+ps_err ps_initial_configuration(void* h, const double* x0, const double* q0) {
+    Ctx* c = as_ctx(h);
+    if (!c || !x0) return PS_ERR_ARG;
+    const size_t n3 = 3 * (size_t)c->Np;
+    PS_HIP(c, hipMemcpy(c->x, x0, sizeof(double) * n3, hipMemcpyHostToDevice));
+    // trap centres = initial colloid positions on a fresh start (src/Stokes.cpp:279-284)
+    if (c->p.restart == 0.0 && c->Nc > 0)
+        PS_HIP(c, hipMemcpy(c->xpos0, x0 + 3 * (size_t)c->Nm, sizeof(double) * 3 * c->Nc, hipMemcpyHostToDevice));
+    else if (c->Nc > 0)
+        PS_HIP(c, hipMemset(c->xpos0, 0, sizeof(double) * 3 * c->Nc));
+    std::vector<double> q(4 * (size_t)std::max(c->Nc, 1), 0.0);
+    for (int i = 0; i < c->Nc; i++)
+        for (int k = 0; k < 4; k++) q[4 * i + k] = q0 ? q0[4 * i + k] : (k == 0 ? 1.0 : 0.0);
+    if (c->Nc > 0) PS_HIP(c, hipMemcpy(c->q, q.data(), sizeof(double) * 4 * c->Nc, hipMemcpyHostToDevice));
+    c->have_x = true;
+    return PS_OK;
+}
+
+ps_err ps_set_positions(void* h, const double* x) {
+    Ctx* c = as_ctx(h);
+    if (!c || !x) return PS_ERR_ARG;
+    PS_HIP(c, hipMemcpy(c->x, x, sizeof(double) * 3 * (size_t)c->Np, hipMemcpyHostToDevice));
+    c->have_x = true;
+    return PS_OK;
+}
+
+ps_err ps_get_state(void* h, double* x, double* q, double* t) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    if (x) PS_HIP(c, hipMemcpy(x, c->x, sizeof(double) * 3 * (size_t)c->Np, hipMemcpyDeviceToHost));
+    if (q && c->Nc > 0) PS_HIP(c, hipMemcpy(q, c->q, sizeof(double) * 4 * c->Nc, hipMemcpyDeviceToHost));
+    if (t) *t = c->t;
+    return PS_OK;
+}
+
+ps_err ps_get_vectors(void* h, double* up, double* vdn, double* vdc, double* vb, double* ud, double* sb, double* fext) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    const size_t b = sizeof(double) * (size_t)c->nu;
+    if (up) PS_HIP(c, hipMemcpy(up, c->up, b, hipMemcpyDeviceToHost));
+    if (vdn) PS_HIP(c, hipMemcpy(vdn, c->v_det_n, b, hipMemcpyDeviceToHost));
+    if (vdc) PS_HIP(c, hipMemcpy(vdc, c->v_det_c, b, hipMemcpyDeviceToHost));
+    if (vb) PS_HIP(c, hipMemcpy(vb, c->v_brown, b, hipMemcpyDeviceToHost));
+    if (ud) PS_HIP(c, hipMemcpy(ud, c->udiff, b, hipMemcpyDeviceToHost));
+    if (sb && c->Nc > 0) PS_HIP(c, hipMemcpy(sb, c->sb, sizeof(double) * 5 * c->Nc, hipMemcpyDeviceToHost));
+    if (fext) PS_HIP(c, hipMemcpy(fext, c->fext, b, hipMemcpyDeviceToHost));
+    return PS_OK;
+}
+
+ps_err ps_get_counters(void* h, ps_counters* out) {
+    Ctx* c = as_ctx(h);
+    if (!c || !out) return PS_ERR_ARG;
+    *out = c->cnt;
+    return PS_OK;
+}
+
+ps_err ps_check_dist(void* h, int64_t* n_pairs) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    if (!c->have_x) return fail(c, PS_ERR_STATE, "check_dist before initial_configuration");
+    PS_TRY(check_dist_dev(c));
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    if (n_pairs) *n_pairs = c->nvlist;
+    return PS_OK;
+}
+
+ps_err ps_get_vlist(void* h, int32_t* rows, int32_t* partners, int64_t* pidx, int64_t cap, int64_t* n_pairs) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    if (!c->have_pd) return fail(c, PS_ERR_STATE, "get_vlist before check_dist");
+    return vlist_copy(c, rows, partners, pidx, cap, n_pairs);
+}
+
+ps_err ps_mob(void* h) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    if (!c->have_pd) return fail(c, PS_ERR_STATE, "mob before check_dist (mob reads the pair geometry)");
+    PS_TRY(mob_prepare(c));
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    c->have_mob = true;
+    return PS_OK;
+}
+
+ps_err ps_stale_tile(void* h, double* t9) {
+    Ctx* c = as_ctx(h);
+    if (!c || !t9) return PS_ERR_ARG;
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    PS_HIP(c, hipMemcpy(t9, c->stale, sizeof(double) * 9, hipMemcpyDeviceToHost));
+    return PS_OK;
+}
+
+ps_err ps_mob_mult_dev(void* h, const double* f, double* u) {
+    Ctx* c = as_ctx(h);
+    if (!c || !f || !u) return PS_ERR_ARG;
+    if (!c->have_mob) return fail(c, PS_ERR_STATE, "apply before mob");
+    return mob_apply(c, f, u, c->p.precision);
+}
+ps_err ps_mob_mult(void* h, const double* f, double* u) {
+    Ctx* c = as_ctx(h);
+    if (!c || !f || !u) return PS_ERR_ARG;
+    if (!c->have_mob) return fail(c, PS_ERR_STATE, "apply before mob");
+    return with_io(c, c->n, f, c->n, u, [&](double* a, double* b) { return mob_apply(c, a, b, c->p.precision); });
+}
+ps_err ps_saddle_mult_dev(void* h, const double* x, double* y) {
+    Ctx* c = as_ctx(h);
+    if (!c || !x || !y) return PS_ERR_ARG;
+    if (!c->have_mob) return fail(c, PS_ERR_STATE, "apply before mob");
+    return saddle_apply(c, x, y, c->p.precision);
+}
+ps_err ps_saddle_mult(void* h, const double* x, double* y) {
+    Ctx* c = as_ctx(h);
+    if (!c || !x || !y) return PS_ERR_ARG;
+    if (!c->have_mob) return fail(c, PS_ERR_STATE, "apply before mob");
+    return with_io(c, c->ns, x, c->ns, y, [&](double* a, double* b) { return saddle_apply(c, a, b, c->p.precision); });
+}
+ps_err ps_saddle_diag(void* h, double* d) {
+    Ctx* c = as_ctx(h);
+    if (!c || !d) return PS_ERR_ARG;
+    PS_TRY(saddle_diag_dev(c, c->io_b));
+    PS_HIP(c, hipMemcpyAsync(d, c->io_b, sizeof(double) * (size_t)c->ns, hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+ps_err ps_saddle_solve_dev(void* h, const double* b, double* x, int warm, ps_solve_info* info) {
+    Ctx* c = as_ctx(h);
+    if (!c || !b || !x) return PS_ERR_ARG;
+    if (!c->have_mob) return fail(c, PS_ERR_STATE, "solve before mob");
+    ps_err r = saddle_solve_dev(c, b, x, warm, info);
+    if (r == PS_ERR_NOCONV) c->set_error("MINRES reached its iteration cap");
+    return r;
+}
+ps_err ps_saddle_solve(void* h, const double* b, double* x, int warm, ps_solve_info* info) {
+    Ctx* c = as_ctx(h);
+    if (!c || !b || !x) return PS_ERR_ARG;
+    if (!c->have_mob) return fail(c, PS_ERR_STATE, "solve before mob");
+    PS_HIP(c, hipMemcpyAsync(c->io_a, b, sizeof(double) * (size_t)c->ns, hipMemcpyHostToDevice, c->stream));
+    if (warm) PS_HIP(c, hipMemcpyAsync(c->io_b, x, sizeof(double) * (size_t)c->ns, hipMemcpyHostToDevice, c->stream));
+    ps_err r = saddle_solve_dev(c, c->io_a, c->io_b, warm, info);
+    if (r != PS_OK && r != PS_ERR_NOCONV) return r;
+    PS_HIP(c, hipMemcpyAsync(x, c->io_b, sizeof(double) * (size_t)c->ns, hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    if (r == PS_ERR_NOCONV) c->set_error("MINRES reached its iteration cap");
+    return r;
+}
+ps_err ps_sqrt_apply_dev(void* h, const double* xi, double* w, ps_sqrt_info* info) {
+    Ctx* c = as_ctx(h);
+    if (!c || !xi || !w) return PS_ERR_ARG;
+    if (!c->have_mob) return fail(c, PS_ERR_STATE, "sqrt before mob");
+    return sqrt_apply_dev(c, xi, w, info);
+}
+ps_err ps_sqrt_apply(void* h, const double* xi, double* w, ps_sqrt_info* info) {
+    Ctx* c = as_ctx(h);
+    if (!c || !xi || !w) return PS_ERR_ARG;
+    if (!c->have_mob) return fail(c, PS_ERR_STATE, "sqrt before mob");
+    return with_io(c, c->n, xi, c->n, w, [&](double* a, double* b) { return sqrt_apply_dev(c, a, b, info); });
+}
+ps_err ps_rhs(void* h, int drift, double* rhs) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    if (!c->have_pd) return fail(c, PS_ERR_STATE, "RHS before check_dist");
+    PS_TRY(rhs_dev(c, drift));
+    if (rhs) {
+        PS_HIP(c, hipMemcpyAsync(rhs, c->rhs, sizeof(double) * (size_t)c->ns, hipMemcpyDeviceToHost, c->stream));
+    }
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+ps_err ps_run(void* h, int nsteps) {
+    Ctx* c = as_ctx(h);
+    if (!c || nsteps < 0) return PS_ERR_ARG;
+    if (!c->have_x) return fail(c, PS_ERR_STATE, "run before initial_configuration");
+    for (int i = 0; i < nsteps; i++) PS_TRY(one_step(c));
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+ps_err ps_draw_normals(void* h, double* out, int64_t n) {
+    Ctx* c = as_ctx(h);
+    if (!c || (!out && n > 0) || n < 0) return PS_ERR_ARG;
+    for (int64_t i = 0; i < n; i++) out[i] = c->nd(c->rng);
+    return PS_OK;
+}
+
+}  // extern "C"

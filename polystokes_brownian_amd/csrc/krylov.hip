@@ -1,0 +1,726 @@
+// krylov.hip -- MINRES (saddle solve) and Lanczos (M^{1/2} xi) on the device, FP64 recurrences.
+//
+// MINRES (replaces PETSc KSPSolve with KSPMINRES + PCJACOBI, src/init.cpp:40-64, src/vel.cpp:7-66):
+// Paige & Saunders' preconditioned MINRES with the Jacobi diagonal of ArrowheadGetDiagonal +
+// PCJacobiSetFixDiagonal.  The scalar recurrences never leave the GPU: every vector kernel reduces
+// the previous kernel's block partials itself (same order in every block, so deterministic) and
+// the Givens update is recomputed redundantly per block from a double-buffered state (read slot A,
+// block 0 writes slot B).  Once converged the kernels turn into no-ops, so the host only polls the
+// state every few iterations and the result equals stopping exactly at convergence.
+//
+// Lanczos (replaces SLEPc MFNSolve/FNSQRT on the mobility, src/slip_vel.cpp:27-49, following
+// schur_sqrt_lanczos src/slip_vel.cpp:351-477): FP64 basis in HBM, two full reorthogonalisation
+// passes done as classical Gram-Schmidt (one block GEMV each) instead of the reference's sequential
+// modified Gram-Schmidt, tridiagonal eigendecomposition on the host (m <= 64), negative Ritz
+// values floored, relative-change stopping test.
+#include "ps_common.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace {
+
+constexpr int VB = PS_BLOCK;   // threads per vector block
+constexpr int VPT = 4;         // elements per thread
+constexpr int VCHUNK = VB * VPT;
+
+inline int nblocks(int64_t n) { return (int)std::max<int64_t>(1, (n + VCHUNK - 1) / VCHUNK); }
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Deterministic block sum (fixed tree); result broadcast to all threads.
+__device__ __forceinline__ double block_reduce(double v, double* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    return r;
+}
+
+// Sum of nb partials, identical in every block.
+__device__ __forceinline__ double sum_partials(const double* __restrict__ red, int nb, double* sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nb; i += VB) v += red[i];
+    return block_reduce(v, sh);
+}
+
+// ---------------------------------------------------------------------------- generic vector ops
+__global__ void k_axpby(double* __restrict__ y, double a, const double* __restrict__ x, double b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + b * y[i];
+}
+__global__ void k_copy(double* __restrict__ d, const double* __restrict__ s, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+__global__ void k_dot_part(const double* __restrict__ a, const double* __restrict__ b, int64_t n,
+                           double* __restrict__ red) {
+    __shared__ double sh[4];
+    const int64_t base = (int64_t)blockIdx.x * VCHUNK;
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < VPT; k++) {
+        const int64_t i = base + k * VB + threadIdx.x;
+        if (i < n) v += a[i] * b[i];
+    }
+    v = block_reduce(v, sh);
+    if (threadIdx.x == 0) red[blockIdx.x] = v;
+}
+__global__ void k_sum_to(const double* __restrict__ red, int nb, double* __restrict__ out) {
+    __shared__ double sh[4];
+    const double v = sum_partials(red, nb, sh);
+    if (threadIdx.x == 0) *out = v;
+}
+
+// ---------------------------------------------------------------------------- MINRES kernels
+// init: r1 = b - (warm ? y : 0), x = warm ? x : 0, y = dinv r1; partials r1.y and b.dinv.b.
+__global__ void k_mr_init(int64_t n, const double* __restrict__ b, double* __restrict__ x,
+                          double* __restrict__ r1, double* __restrict__ r2, double* __restrict__ y,
+                          const double* __restrict__ dinv, int warm, double* __restrict__ red) {
+    __shared__ double sh[4];
+    const int64_t base = (int64_t)blockIdx.x * VCHUNK;
+    double v1 = 0.0, v2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < VPT; k++) {
+        const int64_t i = base + k * VB + threadIdx.x;
+        if (i < n) {
+            const double bi = b[i];
+            double ri;
+            if (warm) {
+                ri = bi - y[i];
+            } else {
+                ri = bi;
+                x[i] = 0.0;
+            }
+            const double zi = dinv[i] * ri;
+            r1[i] = ri;
+            r2[i] = ri;
+            y[i] = zi;
+            v1 += ri * zi;
+            v2 += bi * dinv[i] * bi;
+        }
+    }
+    v1 = block_reduce(v1, sh);
+    v2 = block_reduce(v2, sh);
+    if (threadIdx.x == 0) {
+        red[blockIdx.x] = v1;
+        red[gridDim.x + blockIdx.x] = v2;
+    }
+}
+
+__global__ void k_mr_start(const double* __restrict__ red, int nb, double rtol, double abstol,
+                           MinresState* __restrict__ st) {
+    __shared__ double sh[4];
+    const double b1 = sum_partials(red, nb, sh);
+    const double bn = sum_partials(red + nb, nb, sh);
+    if (threadIdx.x == 0) {
+        MinresState s{};
+        s.beta1 = b1 > 0.0 ? sqrt(b1) : 0.0;
+        s.tol = fmax(rtol * sqrt(fmax(bn, 0.0)), abstol);
+        s.oldb = 0.0;
+        s.beta = s.beta1;
+        s.dbar = 0.0;
+        s.epsln = 0.0;
+        s.phibar = s.beta1;
+        s.cs = -1.0;
+        s.sn = 0.0;
+        s.its = 0;
+        s.done = 0;
+        s.reason = 0;
+        if (b1 <= 0.0) {
+            s.done = 1;
+            s.reason = 3;
+        } else if (s.phibar <= s.tol) {
+            s.done = 1;
+            s.reason = 2;
+        }
+        *st = s;
+    }
+}
+
+__global__ void k_mr_v0(int64_t n, const MinresState* __restrict__ st, const double* __restrict__ y,
+                        double* __restrict__ v) {
+    if (st->done) return;
+    const double sc = 1.0 / st->beta;
+    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
+    if (i < n) v[i] = sc * y[i];
+}
+
+// a: y -= (beta/oldb) r1 (from the 2nd iteration); partial v.y
+__global__ void k_mr_a(int64_t n, const MinresState* __restrict__ st, double* __restrict__ y,
+                       const double* __restrict__ r1, const double* __restrict__ v, double* __restrict__ red) {
+    __shared__ double sh[4];
+    if (st->done) return;
+    const bool sub = st->its >= 1;
+    const double f = sub ? st->beta / st->oldb : 0.0;
+    const int64_t base = (int64_t)blockIdx.x * VCHUNK;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < VPT; k++) {
+        const int64_t i = base + k * VB + threadIdx.x;
+        if (i < n) {
+            double yi = y[i];
+            if (sub) yi -= f * r1[i];
+            y[i] = yi;
+            acc += v[i] * yi;
+        }
+    }
+    acc = block_reduce(acc, sh);
+    if (threadIdx.x == 0) red[blockIdx.x] = acc;
+}
+
+// b: alfa = sum(red); y -= (alfa/beta) r2; z = dinv y (into zbuf); partial y.z
+__global__ void k_mr_b(int64_t n, const MinresState* __restrict__ st, const double* __restrict__ red, int nb,
+                       double* __restrict__ y, const double* __restrict__ r2, const double* __restrict__ dinv,
+                       double* __restrict__ zbuf, double* __restrict__ red2) {
+    __shared__ double sh[4];
+    if (st->done) return;
+    const double alfa = sum_partials(red, nb, sh);
+    const double f = alfa / st->beta;
+    const int64_t base = (int64_t)blockIdx.x * VCHUNK;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < VPT; k++) {
+        const int64_t i = base + k * VB + threadIdx.x;
+        if (i < n) {
+            const double yi = y[i] - f * r2[i];
+            const double zi = dinv[i] * yi;
+            y[i] = yi;
+            zbuf[i] = zi;
+            acc += yi * zi;
+        }
+    }
+    acc = block_reduce(acc, sh);
+    if (threadIdx.x == 0) red2[blockIdx.x] = acc;
+}
+
+// c: scalar Givens update (redundant per block), w rotation, x += phi w, v <- z / beta_new.
+__global__ void k_mr_c(int64_t n, const MinresState* __restrict__ st, MinresState* __restrict__ st_next,
+                       const double* __restrict__ red_a, const double* __restrict__ red_b, int nb,
+                       double* __restrict__ v, const double* __restrict__ z, const double* __restrict__ w_old,
+                       const double* __restrict__ w2_old, double* __restrict__ w_new, double* __restrict__ x,
+                       int maxit, double abstol) {
+    __shared__ double sh[4];
+    if (st->done) return;
+    const double alfa = sum_partials(red_a, nb, sh);
+    const double bb = sum_partials(red_b, nb, sh);
+    MinresState s = *st;
+    s.alfa = alfa;
+    s.oldb = s.beta;
+    s.beta = sqrt(fmax(bb, 0.0));
+    const double oldeps = s.epsln;
+    const double delta = s.cs * s.dbar + s.sn * alfa;
+    const double gbar = s.sn * s.dbar - s.cs * alfa;
+    s.epsln = s.sn * s.beta;
+    s.dbar = -s.cs * s.beta;
+    double gamma = sqrt(gbar * gbar + s.beta * s.beta);
+    gamma = fmax(gamma, 1e-300);
+    s.cs = gbar / gamma;
+    s.sn = s.beta / gamma;
+    const double phi = s.cs * s.phibar;
+    s.phibar = s.sn * s.phibar;
+    s.oldeps = oldeps;
+    s.delta = delta;
+    s.gamma = gamma;
+    s.phi = phi;
+    s.its += 1;
+    if (s.phibar <= s.tol) {
+        s.done = 1;
+        s.reason = (s.phibar <= abstol) ? 3 : 2;
+    } else if (s.beta == 0.0) {
+        s.done = 1;
+        s.reason = 2;
+    } else if (s.its >= maxit) {
+        s.done = 1;
+        s.reason = -3;
+    }
+    const double denom = 1.0 / gamma;
+    const double vs = (s.beta > 0.0) ? 1.0 / s.beta : 0.0;
+    const int64_t base = (int64_t)blockIdx.x * VCHUNK;
+#pragma unroll
+    for (int k = 0; k < VPT; k++) {
+        const int64_t i = base + k * VB + threadIdx.x;
+        if (i < n) {
+            // w1_new = w2_old (held in w_old's rotation), w2_new = w_old
+            const double wn = (v[i] - oldeps * w2_old[i] - delta * w_old[i]) * denom;
+            w_new[i] = wn;
+            x[i] += phi * wn;
+            v[i] = vs * z[i];
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *st_next = s;
+}
+
+__global__ void k_state_copy(const MinresState* __restrict__ a, MinresState* __restrict__ b) {
+    if (threadIdx.x == 0) *b = *a;
+}
+
+__global__ void k_diag(SysConst s, double* __restrict__ d) {
+    const int64_t r = (int64_t)blockIdx.x * VB + threadIdx.x;
+    if (r >= s.ns) return;
+    double v;
+    if (r < s.nm3) v = s.beta_inv;
+    else if (r < s.nm3 + s.nc3) v = 1.0;
+    else if (r < s.nu) v = 3. / 4.;
+    else if (r < s.n) v = 9. / 5.;
+    else v = 1.0;   // zero constraint diagonal fixed to 1 (PCJacobiSetFixDiagonal)
+    d[r] = v;
+}
+__global__ void k_recip(double* __restrict__ d, int64_t n) {
+    const int64_t r = (int64_t)blockIdx.x * VB + threadIdx.x;
+    if (r < n) d[r] = 1.0 / d[r];
+}
+
+// ---------------------------------------------------------------------------- Lanczos kernels
+// w -= bprev * qprev (if qprev); partial qj.w
+__global__ void k_lz_a(int64_t n, double* __restrict__ w, const double* __restrict__ qprev, double bprev,
+                       const double* __restrict__ qj, double* __restrict__ red) {
+    __shared__ double sh[4];
+    const int64_t base = (int64_t)blockIdx.x * VCHUNK;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < VPT; k++) {
+        const int64_t i = base + k * VB + threadIdx.x;
+        if (i < n) {
+            double wi = w[i];
+            if (qprev) wi -= bprev * qprev[i];
+            w[i] = wi;
+            acc += qj[i] * wi;
+        }
+    }
+    acc = block_reduce(acc, sh);
+    if (threadIdx.x == 0) red[blockIdx.x] = acc;
+}
+// alpha = sum(red) -> lz[j]; w -= alpha qj
+__global__ void k_lz_b(int64_t n, double* __restrict__ w, const double* __restrict__ qj,
+                       const double* __restrict__ red, int nb, double* __restrict__ alpha_out) {
+    __shared__ double sh[4];
+    const double a = sum_partials(red, nb, sh);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *alpha_out = a;
+    const int64_t base = (int64_t)blockIdx.x * VCHUNK;
+#pragma unroll
+    for (int k = 0; k < VPT; k++) {
+        const int64_t i = base + k * VB + threadIdx.x;
+        if (i < n) w[i] -= a * qj[i];
+    }
+}
+// partials of Q_i . w for i < m  -> red[i * nb + block]
+__global__ void k_lz_dots(int64_t n, const double* __restrict__ Q, int m, const double* __restrict__ w,
+                          double* __restrict__ red) {
+    __shared__ double sh[4];
+    const int64_t base = (int64_t)blockIdx.x * VCHUNK;
+    double wl[VPT];
+#pragma unroll
+    for (int k = 0; k < VPT; k++) {
+        const int64_t i = base + k * VB + threadIdx.x;
+        wl[k] = (i < n) ? w[i] : 0.0;
+    }
+    for (int q = 0; q < m; q++) {
+        const double* Qi = Q + (int64_t)q * n;
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < VPT; k++) {
+            const int64_t i = base + k * VB + threadIdx.x;
+            if (i < n) acc += Qi[i] * wl[k];
+        }
+        acc = block_reduce(acc, sh);
+        if (threadIdx.x == 0) red[(int64_t)q * gridDim.x + blockIdx.x] = acc;
+    }
+}
+// h_i = sum partials; w -= sum_i h_i Q_i
+__global__ void k_lz_sub(int64_t n, const double* __restrict__ Q, int m, double* __restrict__ w,
+                         const double* __restrict__ red, int nb) {
+    __shared__ double sh[4];
+    __shared__ double h[64];
+    for (int q = 0; q < m; q++) {
+        const double v = sum_partials(red + (int64_t)q * nb, nb, sh);
+        if (threadIdx.x == 0) h[q] = v;
+    }
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * VCHUNK;
+#pragma unroll
+    for (int k = 0; k < VPT; k++) {
+        const int64_t i = base + k * VB + threadIdx.x;
+        if (i < n) {
+            double wi = w[i];
+            for (int q = 0; q < m; q++) wi -= h[q] * Q[(int64_t)q * n + i];
+            w[i] = wi;
+        }
+    }
+}
+__global__ void k_scale(int64_t n, double* __restrict__ y, const double* __restrict__ x, double a) {
+    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
+    if (i < n) y[i] = a * x[i];
+}
+// out = sum_i g_i Q_i
+__global__ void k_lz_combine(int64_t n, const double* __restrict__ Q, int m, const double* __restrict__ g,
+                             double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int q = 0; q < m; q++) acc += g[q] * Q[(int64_t)q * n + i];
+    out[i] = acc;
+}
+
+inline unsigned eblocks(int64_t n) { return (unsigned)((n + VB - 1) / VB); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------- host side
+ps_err krylov_alloc(Ctx* c) {
+    const int64_t ns = c->ns, n = c->n;
+    PS_HIP(c, hipMalloc(&c->kr, sizeof(double) * 9 * (size_t)ns));
+    PS_HIP(c, hipMalloc(&c->dinv, sizeof(double) * (size_t)ns));
+    c->red_cap = std::max(8192, 2 * nblocks(ns) * 66);
+    PS_HIP(c, hipMalloc(&c->red, sizeof(double) * (size_t)c->red_cap));
+    PS_HIP(c, hipMalloc(&c->mstate, sizeof(MinresState) * 2));
+    PS_HIP(c, hipHostMalloc(&c->mstate_h, sizeof(MinresState) * 2, hipHostMallocDefault));
+    c->kmax = std::max(1, std::min<int>(c->p.sqrt_kmax > 0 ? c->p.sqrt_kmax : 64, 64));
+    if ((int64_t)c->kmax > n) c->kmax = (int)std::max<int64_t>(1, n);
+    PS_HIP(c, hipMalloc(&c->Q, sizeof(double) * (size_t)c->kmax * (size_t)n));
+    PS_HIP(c, hipMalloc(&c->lz, sizeof(double) * (3 * 64 + 8)));
+    PS_HIP(c, hipHostMalloc(&c->lz_h, sizeof(double) * (3 * 64 + 8), hipHostMallocDefault));
+    PS_HIP(c, hipMalloc(&c->gw, sizeof(double) * 64));
+    PS_HIP(c, hipMalloc(&c->ir_r, sizeof(double) * (size_t)ns));
+    PS_HIP(c, hipMalloc(&c->ir_d, sizeof(double) * (size_t)ns));
+    PS_TRY(saddle_diag_dev(c, c->dinv));
+    k_recip<<<eblocks(ns), VB, 0, c->stream>>>(c->dinv, ns);
+    PS_LAUNCH(c);
+    return PS_OK;
+}
+
+void krylov_free(Ctx* c) {
+    for (void* p : {(void*)c->kr, (void*)c->dinv, (void*)c->red, (void*)c->mstate, (void*)c->Q, (void*)c->lz,
+                    (void*)c->gw, (void*)c->ir_r, (void*)c->ir_d})
+        if (p) (void)hipFree(p);
+    if (c->mstate_h) (void)hipHostFree(c->mstate_h);
+    if (c->lz_h) (void)hipHostFree(c->lz_h);
+    c->kr = c->dinv = c->red = c->Q = c->lz = c->gw = c->ir_r = c->ir_d = nullptr;
+    c->mstate = nullptr;
+    c->mstate_h = nullptr;
+    c->lz_h = nullptr;
+}
+
+ps_err saddle_diag_dev(Ctx* c, double* d) {
+    k_diag<<<eblocks(c->ns), VB, 0, c->stream>>>(c->sc, d);
+    PS_LAUNCH(c);
+    return PS_OK;
+}
+
+ps_err vec_axpby(Ctx* c, double* y, double a, const double* x, double b, int64_t n) {
+    if (n <= 0) return PS_OK;
+    k_axpby<<<eblocks(n), VB, 0, c->stream>>>(y, a, x, b, n);
+    PS_LAUNCH(c);
+    return PS_OK;
+}
+ps_err vec_copy(Ctx* c, double* d, const double* s, int64_t n) {
+    if (n <= 0) return PS_OK;
+    PS_HIP(c, hipMemcpyAsync(d, s, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
+    return PS_OK;
+}
+ps_err vec_dot(Ctx* c, const double* a, const double* b, int64_t n, double* out) {
+    const int nb = nblocks(n);
+    k_dot_part<<<nb, VB, 0, c->stream>>>(a, b, n, c->red);
+    PS_LAUNCH(c);
+    k_sum_to<<<1, VB, 0, c->stream>>>(c->red, nb, c->lz + 3 * 64);
+    PS_LAUNCH(c);
+    PS_HIP(c, hipMemcpyAsync(c->lz_h + 3 * 64, c->lz + 3 * 64, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    *out = c->lz_h[3 * 64];
+    return PS_OK;
+}
+
+// Preconditioned MINRES on the saddle operator with the apply precision `prec`.
+static ps_err minres_dev(Ctx* c, const double* b, double* x, int warm, double rtol, int maxit, int prec,
+                         ps_solve_info* info) {
+    const int64_t N = c->ns;
+    const int nb = nblocks(N);
+    double* R1 = c->kr;
+    double* R2 = c->kr + N;
+    double* Y = c->kr + 2 * N;
+    double* V = c->kr + 3 * N;
+    double* Wb = c->kr + 4 * N;
+    double* W1 = c->kr + 5 * N;
+    double* W2 = c->kr + 6 * N;
+    double* red_a = c->red;
+    double* red_b = c->red + nb;
+    hipStream_t st = c->stream;
+    MinresState* sA = c->mstate;
+    MinresState* sB = c->mstate + 1;
+    if (warm) PS_TRY(saddle_apply(c, x, Y, prec));
+    k_mr_init<<<nb, VB, 0, st>>>(N, b, x, R1, R2, Y, c->dinv, warm, c->red);
+    PS_LAUNCH(c);
+    k_mr_start<<<1, VB, 0, st>>>(c->red, nb, rtol, c->p.ksp_abstol, sA);
+    PS_LAUNCH(c);
+    PS_HIP(c, hipMemsetAsync(Wb, 0, sizeof(double) * 3 * (size_t)N, st));   // w, w1, w2 = 0
+    k_mr_v0<<<eblocks(N), VB, 0, st>>>(N, sA, Y, V);
+    PS_LAUNCH(c);
+    // r2 = r1 was written by init; rotation state (r1, r2, y) and (w, w1, w2)
+    double *r1 = R1, *r2 = R2, *y = Y;
+    double *w = Wb, *w1 = W1, *w2 = W2;
+    int it = 0;
+    const int poll = 4;
+    bool done = false;
+    int reason = 0;
+    int its = 0;
+    double phibar = 0.0, beta1 = 0.0;
+    PS_HIP(c, hipMemcpyAsync(c->mstate_h, sA, sizeof(MinresState), hipMemcpyDeviceToHost, st));
+    PS_HIP(c, hipStreamSynchronize(st));
+    if (c->mstate_h[0].done) {
+        done = true;
+        reason = c->mstate_h[0].reason;
+        phibar = c->mstate_h[0].phibar;
+        beta1 = c->mstate_h[0].beta1;
+    }
+    while (!done) {
+        // y = A v
+        PS_TRY(saddle_apply(c, V, y, prec));
+        k_mr_a<<<nb, VB, 0, st>>>(N, sA, y, r1, V, red_a);
+        PS_LAUNCH(c);
+        // z into r1's buffer (old r1 is dead after k_mr_a)
+        k_mr_b<<<nb, VB, 0, st>>>(N, sA, red_a, nb, y, r2, c->dinv, r1, red_b);
+        PS_LAUNCH(c);
+        // rotation: r1 <- r2, r2 <- y, y <- z(r1 buffer)
+        double* z = r1;
+        r1 = r2;
+        r2 = y;
+        y = z;
+        // w_new written into w1's buffer; w1 <- w2, w2 <- w
+        k_mr_c<<<nb, VB, 0, st>>>(N, sA, sB, red_a, red_b, nb, V, y, w, w2, w1, x, maxit, c->p.ksp_abstol);
+        PS_LAUNCH(c);
+        double* wn = w1;
+        w1 = w2;
+        w2 = w;
+        w = wn;
+        // the next iteration reads sB: copy it into sA so kernels keep a fixed read slot
+        k_state_copy<<<1, 64, 0, st>>>(sB, sA);
+        PS_LAUNCH(c);
+        it++;
+        if (it % poll == 0 || it >= maxit) {
+            PS_HIP(c, hipMemcpyAsync(c->mstate_h, sA, sizeof(MinresState), hipMemcpyDeviceToHost, st));
+            PS_HIP(c, hipStreamSynchronize(st));
+            if (c->mstate_h[0].done || it >= maxit) {
+                done = true;
+                reason = c->mstate_h[0].done ? c->mstate_h[0].reason : -3;
+                its = c->mstate_h[0].its;
+                phibar = c->mstate_h[0].phibar;
+                beta1 = c->mstate_h[0].beta1;
+            }
+        }
+    }
+    c->last_minres_its = its;
+    c->cnt.minres_iterations += its;
+    c->cnt.saddle_solves++;
+    if (info) {
+        info->iterations = its;
+        info->reason = reason;
+        info->rnorm = phibar;
+        info->rnorm0 = beta1;
+        info->refinements = 0;
+        info->true_rel_residual = -1.0;
+    }
+    return reason > 0 ? PS_OK : PS_ERR_NOCONV;
+}
+
+ps_err saddle_solve_dev(Ctx* c, const double* b, double* x, int warm, ps_solve_info* info) {
+    if (c->p.precision != PS_PREC_MIXED) return minres_dev(c, b, x, warm, c->p.ksp_rtol, c->p.ksp_maxit, PS_PREC_FP64, info);
+    // Mixed form: FP32-apply MINRES corrections, FP64 residual r = b - A x (iterative refinement).
+    const int64_t N = c->ns;
+    double bn2 = 0.0;
+    PS_TRY(vec_dot(c, b, b, N, &bn2));
+    const double bn = std::sqrt(bn2);
+    if (!warm) PS_HIP(c, hipMemsetAsync(x, 0, sizeof(double) * (size_t)N, c->stream));
+    PS_TRY(saddle_apply(c, x, c->ir_r, PS_PREC_FP64));
+    PS_TRY(vec_axpby(c, c->ir_r, 1.0, b, -1.0, N));   // r = b - A x
+    double rn2 = 0.0;
+    PS_TRY(vec_dot(c, c->ir_r, c->ir_r, N, &rn2));
+    double rel = bn > 0 ? std::sqrt(rn2) / bn : 0.0;
+    int sweeps = 0, its = 0;
+    ps_err rc = PS_OK;
+    const double inner = std::max(c->p.ksp_rtol, 1e-6);
+    while (rel > c->p.ir_rtol && sweeps < std::max(1, c->p.ir_maxit)) {
+        ps_solve_info ii{};
+        ps_err r = minres_dev(c, c->ir_r, c->ir_d, 0, inner, c->p.ksp_maxit, PS_PREC_MIXED, &ii);
+        its += ii.iterations;
+        if (r != PS_OK && r != PS_ERR_NOCONV) return r;
+        PS_TRY(vec_axpby(c, x, 1.0, c->ir_d, 1.0, N));
+        PS_TRY(saddle_apply(c, x, c->ir_r, PS_PREC_FP64));
+        PS_TRY(vec_axpby(c, c->ir_r, 1.0, b, -1.0, N));
+        PS_TRY(vec_dot(c, c->ir_r, c->ir_r, N, &rn2));
+        rel = bn > 0 ? std::sqrt(rn2) / bn : 0.0;
+        sweeps++;
+    }
+    if (rel > c->p.ir_rtol) rc = PS_ERR_NOCONV;
+    c->last_minres_its = its;
+    if (info) {
+        info->iterations = its;
+        info->reason = rc == PS_OK ? 2 : -3;
+        info->rnorm = rel * bn;
+        info->rnorm0 = bn;
+        info->refinements = sweeps;
+        info->true_rel_residual = rel;
+    }
+    return rc;
+}
+
+// w = M^{1/2} xi.
+ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* out, ps_sqrt_info* info) {
+    const int64_t n = c->n;
+    const int nb = nblocks(n);
+    hipStream_t st = c->stream;
+    double b2 = 0.0;
+    PS_TRY(vec_dot(c, xi, xi, n, &b2));
+    const double beta0 = std::sqrt(b2);
+    PS_HIP(c, hipMemsetAsync(out, 0, sizeof(double) * (size_t)n, st));
+    if (info) {
+        info->iterations = 0;
+        info->nneg = 0;
+        info->beta0 = beta0;
+        info->rel_change = 0.0;
+    }
+    c->cnt.sqrt_calls++;
+    if (beta0 == 0.0) return PS_OK;
+    const int kmax = c->kmax;
+    const int prec = c->p.precision;
+    double* w = c->kr;   // MINRES scratch is free during the square root
+    std::vector<double> alpha, betad, g, gprev;
+    k_scale<<<eblocks(n), VB, 0, st>>>(n, c->Q, xi, 1.0 / beta0);
+    PS_LAUNCH(c);
+    int m = 0;
+    double relc = 0.0;
+    for (int j = 0; j < kmax; j++) {
+        const double* qj = c->Q + (int64_t)j * n;
+        PS_TRY(mob_apply(c, qj, w, prec));
+        k_lz_a<<<nb, VB, 0, st>>>(n, w, j > 0 ? c->Q + (int64_t)(j - 1) * n : nullptr,
+                                  j > 0 ? betad[j - 1] : 0.0, qj, c->red);
+        PS_LAUNCH(c);
+        k_lz_b<<<nb, VB, 0, st>>>(n, w, qj, c->red, nb, c->lz + j);
+        PS_LAUNCH(c);
+        for (int pass = 0; pass < 2; pass++) {
+            k_lz_dots<<<nb, VB, 0, st>>>(n, c->Q, j + 1, w, c->red);
+            PS_LAUNCH(c);
+            k_lz_sub<<<nb, VB, 0, st>>>(n, c->Q, j + 1, w, c->red, nb);
+            PS_LAUNCH(c);
+        }
+        k_dot_part<<<nb, VB, 0, st>>>(w, w, n, c->red);
+        PS_LAUNCH(c);
+        k_sum_to<<<1, VB, 0, st>>>(c->red, nb, c->lz + 64 + j);
+        PS_LAUNCH(c);
+        PS_HIP(c, hipMemcpyAsync(c->lz_h + j, c->lz + j, sizeof(double), hipMemcpyDeviceToHost, st));
+        PS_HIP(c, hipMemcpyAsync(c->lz_h + 64 + j, c->lz + 64 + j, sizeof(double), hipMemcpyDeviceToHost, st));
+        PS_HIP(c, hipStreamSynchronize(st));
+        alpha.push_back(c->lz_h[j]);
+        const double bj = std::sqrt(std::max(c->lz_h[64 + j], 0.0));
+        m = j + 1;
+        c->cnt.lanczos_iterations++;
+        bool stop = false;
+        if (c->p.sqrt_rtol > 0.0) {
+            g.assign(m, 0.0);
+            tridiag_sqrt_weights(m, alpha.data(), betad.data(), beta0, g.data());
+            if (m >= 2) {
+                double dn = 0.0, gn = 0.0;
+                for (int i = 0; i < m; i++) {
+                    const double gp = (i < m - 1) ? gprev[i] : 0.0;
+                    dn += (g[i] - gp) * (g[i] - gp);
+                    gn += g[i] * g[i];
+                }
+                relc = std::sqrt(dn / gn);
+                if (relc <= c->p.sqrt_rtol) stop = true;
+            }
+            gprev = g;
+        }
+        if (stop) break;
+        if (bj <= 1e-12 * beta0 || j == kmax - 1) break;
+        betad.push_back(bj);
+        k_scale<<<eblocks(n), VB, 0, st>>>(n, c->Q + (int64_t)(j + 1) * n, w, 1.0 / bj);
+        PS_LAUNCH(c);
+    }
+    g.assign(m, 0.0);
+    const int nneg = tridiag_sqrt_weights(m, alpha.data(), betad.data(), beta0, g.data());
+    if (nneg > 0) {
+        c->cnt.lanczos_nneg_events++;
+        c->cnt.lanczos_nneg_count += nneg;
+    }
+    PS_HIP(c, hipMemcpyAsync(c->gw, g.data(), sizeof(double) * m, hipMemcpyHostToDevice, st));
+    k_lz_combine<<<eblocks(n), VB, 0, st>>>(n, c->Q, m, c->gw, out);
+    PS_LAUNCH(c);
+    PS_HIP(c, hipStreamSynchronize(st));   // g lives on the host stack
+    if (info) {
+        info->iterations = m;
+        info->nneg = nneg;
+        info->rel_change = relc;
+    }
+    return PS_OK;
+}
+
+// ---------------------------------------------------------------------------- tridiagonal sqrt
+// Implicit QL with shifts on the symmetric tridiagonal T (EISPACK tql2, Bowdler, Martin, Reinsch &
+// Wilkinson 1968), eigenvectors accumulated; then g = beta0 U diag(sqrt(max(theta,0))) U^T e1
+// (src/slip_vel.cpp:427-459).  Returns the number of negative Ritz values.
+int tridiag_sqrt_weights(int m, const double* alpha, const double* betad, double beta0, double* g) {
+    std::vector<double> d(alpha, alpha + m), e(std::max(m, 1), 0.0), Z((size_t)m * m, 0.0);
+    for (int i = 0; i + 1 < m; i++) e[i] = betad[i];
+    for (int i = 0; i < m; i++) Z[i + (size_t)i * m] = 1.0;
+    for (int l = 0; l < m; l++) {
+        int iter = 0, mm;
+        do {
+            for (mm = l; mm < m - 1; mm++) {
+                const double dd = std::fabs(d[mm]) + std::fabs(d[mm + 1]);
+                if (std::fabs(e[mm]) <= 1e-300 + 2.2e-16 * dd) break;
+            }
+            if (mm != l) {
+                if (iter++ == 200) break;
+                double gg = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = std::hypot(gg, 1.0);
+                gg = d[mm] - d[l] + e[l] / (gg + (gg >= 0 ? std::fabs(r) : -std::fabs(r)));
+                double s = 1.0, cc = 1.0, p = 0.0;
+                bool underflow = false;
+                for (int i = mm - 1; i >= l; i--) {
+                    const double f = s * e[i], b = cc * e[i];
+                    r = std::hypot(f, gg);
+                    e[i + 1] = r;
+                    if (r == 0.0) {
+                        d[i + 1] -= p;
+                        e[mm] = 0.0;
+                        underflow = true;
+                        break;
+                    }
+                    s = f / r;
+                    cc = gg / r;
+                    gg = d[i + 1] - p;
+                    r = (d[i] - gg) * s + 2.0 * cc * b;
+                    p = s * r;
+                    d[i + 1] = gg + p;
+                    gg = cc * r - b;
+                    for (int k = 0; k < m; k++) {
+                        const double zf = Z[k + (size_t)(i + 1) * m];
+                        Z[k + (size_t)(i + 1) * m] = s * Z[k + (size_t)i * m] + cc * zf;
+                        Z[k + (size_t)i * m] = cc * Z[k + (size_t)i * m] - s * zf;
+                    }
+                }
+                if (underflow) continue;
+                d[l] -= p;
+                e[l] = gg;
+                e[mm] = 0.0;
+            }
+        } while (mm != l);
+    }
+    int nneg = 0;
+    for (int i = 0; i < m; i++) {
+        double gi = 0.0;
+        for (int jj = 0; jj < m; jj++) {
+            double th = d[jj];
+            if (th < 0.0) th = 0.0;
+            gi += Z[i + (size_t)jj * m] * std::sqrt(th) * Z[0 + (size_t)jj * m];
+        }
+        g[i] = beta0 * gi;
+    }
+    for (int jj = 0; jj < m; jj++)
+        if (d[jj] < 0.0) nneg++;
+    return nneg;
+}

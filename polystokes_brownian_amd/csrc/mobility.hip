@@ -1,0 +1,688 @@
+// mobility.hip -- u = M f with the FTS grand-mobility tiles generated on the fly (gfx950).
+//
+// M is never stored.  One apply launches four independent kernels whose partial sums are combined
+// by a deterministic finalize (no atomics, so every apply is bit-reproducible):
+//   k_aa       monomer rows x monomer sources: the 3x3 tile of mobility(..., AA=true)
+//              (src/mob.cpp:96-151), source positions/forces staged through LDS in 256-bead tiles,
+//              grid = (target blocks, source splits) to fill the 256 CUs;
+//   k_ab_mono  monomer rows x colloid sources: rotation and stresslet couplings of mobility_AB
+//              (src/mob.cpp:362-537) plus the carried-over translation tile;
+//   k_ab_coll  colloid rows x monomer sources (the transposed AB blocks, src/mob.cpp:926-972);
+//   k_bb       colloid rows x colloid sources: the full 11x11 FTS tile (src/mob.cpp:133-356,
+//              placement :1026-1144), always FP64;
+//   k_finalize self blocks (fill_self, src/mob.cpp:539-633) + the partial sums, and for the saddle
+//              operator the constraint coupling B = [-I; 0] (src/arrays.cpp:349-408).
+// The tile algebra is contracted analytically: the AA tile x ee + y (I - ee) applied to f is
+// y f + (x - y) (e.f) e; the flattened stresslet couplings act through the traceless symmetric
+// matrix S^ = [[S0,S1,S2],[S1,S4,S3],[S2,S3,-S0-S4]] (the inverse of the flattening at
+// src/mob.cpp:220-227), so no 3x5 or 5x5 block is ever materialised for AB pairs.
+#include "ps_common.h"
+
+namespace {
+
+constexpr double C1D2 = 1. / 2., C1D3 = 1. / 3., C3D2 = 3. / 2., C3D4 = 3. / 4., C3D8 = 3. / 8.,
+                 C5D6 = 5. / 6., C9D4 = 9. / 4., C9D5 = 9. / 5., C6D5 = 6. / 5., C9D8 = 9. / 8.,
+                 C9D2 = 9. / 2., C9D10 = 9. / 10., C9D20 = 9. / 20., C9D32 = 9. / 32.,
+                 C18D5 = 18. / 5., C36D5 = 36. / 5., C54D5 = 54. / 5.;
+
+template <class T>
+__device__ __forceinline__ void min_img(const SysConst& s, T& dx, T& dy, T& dz) {
+    dx -= (T)s.L[0] * ps_rint<T>(dx * (T)s.Linv[0]);
+    dy -= (T)s.L[1] * ps_rint<T>(dy * (T)s.Linv[1]);
+    dz -= (T)s.L[2] * ps_rint<T>(dz * (T)s.Linv[2]);
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block-wide sum of NV values per thread (256 threads = 4 waves); result valid in thread 0.
+template <class T, int NV>
+__device__ __forceinline__ void block_sum(T (&v)[NV], T (*sh)[NV]) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) sh[wid][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) v[k] = sh[0][k] + sh[1][k] + sh[2][k] + sh[3][k];
+}
+
+// ---------------------------------------------------------------------------- AA (monomer rows)
+template <class T, bool BOX, bool CHAIN>
+__global__ __launch_bounds__(PS_BLOCK) void k_aa(SysConst s, const T* __restrict__ xm,
+                                                 const double* __restrict__ f,
+                                                 const int* __restrict__ chain, int per_split,
+                                                 T* __restrict__ part) {
+    __shared__ T sx[PS_BLOCK], sy[PS_BLOCK], sz[PS_BLOCK], sfx[PS_BLOCK], sfy[PS_BLOCK], sfz[PS_BLOCK];
+    __shared__ int sc[PS_BLOCK];
+    const int Nm = s.Nm;
+    const int i = blockIdx.x * PS_BLOCK + threadIdx.x;
+    const bool act = i < Nm;
+    T xi = 0, yi = 0, zi = 0;
+    int ci = -1;
+    if (act) {
+        xi = xm[3 * i];
+        yi = xm[3 * i + 1];
+        zi = xm[3 * i + 2];
+        if (CHAIN) ci = chain[i];
+    }
+    const T tb = (T)s.two_beta;
+    T ux = 0, uy = 0, uz = 0;
+    const int s0 = blockIdx.y * per_split;
+    const int s1 = min(Nm, s0 + per_split);
+    for (int t0 = s0; t0 < s1; t0 += PS_BLOCK) {
+        __syncthreads();
+        const int j = t0 + threadIdx.x;
+        if (j < s1) {
+            sx[threadIdx.x] = xm[3 * j];
+            sy[threadIdx.x] = xm[3 * j + 1];
+            sz[threadIdx.x] = xm[3 * j + 2];
+            sfx[threadIdx.x] = (T)f[3 * j];
+            sfy[threadIdx.x] = (T)f[3 * j + 1];
+            sfz[threadIdx.x] = (T)f[3 * j + 2];
+            if (CHAIN) sc[threadIdx.x] = chain[j];
+        }
+        __syncthreads();
+        const int cnt = min(PS_BLOCK, s1 - t0);
+#pragma unroll 4
+        for (int k = 0; k < cnt; k++) {
+            T dx = xi - sx[k], dy = yi - sy[k], dz = zi - sz[k];
+            if (BOX) min_img<T>(s, dx, dy, dz);
+            T r2 = dx * dx + dy * dy + dz * dz;
+            bool skip = (t0 + k == i);
+            if (CHAIN) skip |= (sc[k] == ci);
+            r2 = skip ? (T)1 : r2;
+            const T rinv = ps_rsqrt(r2);
+            const T rinv2 = rinv * rinv;
+            const T rinv3 = rinv2 * rinv;
+            const T r = r2 * rinv;
+            const bool far = r > tb;
+            // mobility(..., AA=true): x12a, y12a (src/mob.cpp:122-131)
+            T xa = far ? (T)C3D8 * rinv - (T)2 * rinv3 : (T)C3D8 * r - (T)1;
+            T ya = far ? (T)C3D4 * rinv - (T)C1D2 * rinv3 : (T)1 - (T)C9D32 * r;
+            T cc = (xa - ya) * rinv2;
+            ya = skip ? (T)0 : ya;
+            cc = skip ? (T)0 : cc;
+            const T fx = sfx[k], fy = sfy[k], fz = sfz[k];
+            const T cs = cc * (dx * fx + dy * fy + dz * fz);
+            ux += ya * fx + cs * dx;
+            uy += ya * fy + cs * dy;
+            uz += ya * fz + cs * dz;
+        }
+    }
+    if (act) {
+        part[((int64_t)blockIdx.y * 3 + 0) * Nm + i] = ux;
+        part[((int64_t)blockIdx.y * 3 + 1) * Nm + i] = uy;
+        part[((int64_t)blockIdx.y * 3 + 2) * Nm + i] = uz;
+    }
+}
+
+// ---------------------------------------------------------------------------- AB, monomer rows
+// u_m += stale . sum_c F_c + y12b (e x T_c) - x12g (e.S^e) e + 2 y12g (S^e - (e.S^e) e)
+template <class T, bool BOX>
+__global__ __launch_bounds__(PS_BLOCK) void k_ab_mono(SysConst s, const T* __restrict__ xm,
+                                                      const double* __restrict__ f,
+                                                      const double* __restrict__ stale,
+                                                      T* __restrict__ part) {
+    __shared__ T cx[PS_BLOCK], cy[PS_BLOCK], cz[PS_BLOCK];
+    __shared__ T cv[11][PS_BLOCK];
+    const int Nm = s.Nm, Nc = s.Nc;
+    const int i = blockIdx.x * PS_BLOCK + threadIdx.x;
+    const bool act = i < Nm;
+    T xi = 0, yi = 0, zi = 0;
+    if (act) {
+        xi = xm[3 * i];
+        yi = xm[3 * i + 1];
+        zi = xm[3 * i + 2];
+    }
+    const T gA = (T)(C9D20 * (3 + 5 * s.beta2));
+    const T gB = (T)(C9D10 * (C1D2 + C5D6 * s.beta2));
+    T ux = 0, uy = 0, uz = 0, Fx = 0, Fy = 0, Fz = 0;
+    const int64_t nm3 = s.nm3, nc3 = s.nc3, nu = s.nu;
+    for (int c0 = 0; c0 < Nc; c0 += PS_BLOCK) {
+        __syncthreads();
+        const int c = c0 + threadIdx.x;
+        if (c < Nc) {
+            const int g = Nm + c;
+            cx[threadIdx.x] = xm[3 * g];
+            cy[threadIdx.x] = xm[3 * g + 1];
+            cz[threadIdx.x] = xm[3 * g + 2];
+            for (int k = 0; k < 3; k++) cv[k][threadIdx.x] = (T)f[nm3 + 3 * c + k];
+            for (int k = 0; k < 3; k++) cv[3 + k][threadIdx.x] = (T)f[nm3 + nc3 + 3 * c + k];
+            for (int k = 0; k < 5; k++) cv[6 + k][threadIdx.x] = (T)f[nu + 5 * c + k];
+        }
+        __syncthreads();
+        const int cnt = min(PS_BLOCK, Nc - c0);
+        for (int k = 0; k < cnt; k++) {
+            T dx = xi - cx[k], dy = yi - cy[k], dz = zi - cz[k];
+            if (BOX) min_img<T>(s, dx, dy, dz);
+            const T r2 = dx * dx + dy * dy + dz * dz;
+            const T rinv = ps_rsqrt(r2);
+            const T ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
+            const T rinv2 = rinv * rinv, rinv4 = rinv2 * rinv2;
+            const T y12b = -(T)C3D4 * rinv2;
+            const T x12g = (T)C9D4 * rinv2 - gA * rinv4;
+            const T y12g = gB * rinv4;
+            const T Tx = cv[3][k], Ty = cv[4][k], Tz = cv[5][k];
+            const T S0 = cv[6][k], S1 = cv[7][k], S2 = cv[8][k], S3 = cv[9][k], S4 = cv[10][k];
+            // S^ e
+            const T sex = S0 * ex + S1 * ey + S2 * ez;
+            const T sey = S1 * ex + S4 * ey + S3 * ez;
+            const T sez = S2 * ex + S3 * ey - (S0 + S4) * ez;
+            const T ese = ex * sex + ey * sey + ez * sez;
+            const T a = -x12g * ese - (T)2 * y12g * ese;
+            const T b2 = (T)2 * y12g;
+            ux += y12b * (ey * Tz - ez * Ty) + a * ex + b2 * sex;
+            uy += y12b * (ez * Tx - ex * Tz) + a * ey + b2 * sey;
+            uz += y12b * (ex * Ty - ey * Tx) + a * ez + b2 * sez;
+            Fx += cv[0][k];
+            Fy += cv[1][k];
+            Fz += cv[2][k];
+        }
+    }
+    if (act) {
+        // the translation block every AB pair carries (stale tile), applied to sum_c F_c
+        ux += (T)stale[0] * Fx + (T)stale[1] * Fy + (T)stale[2] * Fz;
+        uy += (T)stale[3] * Fx + (T)stale[4] * Fy + (T)stale[5] * Fz;
+        uz += (T)stale[6] * Fx + (T)stale[7] * Fy + (T)stale[8] * Fz;
+        part[0 * (int64_t)Nm + i] = ux;
+        part[1 * (int64_t)Nm + i] = uy;
+        part[2 * (int64_t)Nm + i] = uz;
+    }
+}
+
+// ---------------------------------------------------------------------------- AB, colloid rows
+// Omega_c += y12b (f_m x e);  E_c += flatten(G), G_ij = sum_k f_k gt_kij;  also sum_m f_m.
+template <class T, bool BOX>
+__global__ __launch_bounds__(PS_BLOCK) void k_ab_coll(SysConst s, const T* __restrict__ xm,
+                                                      const double* __restrict__ f, int per_chunk,
+                                                      T* __restrict__ part_c, T* __restrict__ part_fs) {
+    __shared__ T sh[4][11];
+    const int Nm = s.Nm, Nc = s.Nc;
+    const int c = blockIdx.x, ch = blockIdx.y;
+    const int g = Nm + c;
+    const T xc = xm[3 * g], yc = xm[3 * g + 1], zc = xm[3 * g + 2];
+    const T gA = (T)(C9D20 * (3 + 5 * s.beta2));
+    const T gB = (T)(C9D10 * (C1D2 + C5D6 * s.beta2));
+    T acc[11];
+#pragma unroll
+    for (int k = 0; k < 11; k++) acc[k] = 0;
+    T fsx = 0, fsy = 0, fsz = 0;
+    const int m0 = ch * per_chunk, m1 = min(Nm, m0 + per_chunk);
+    for (int m = m0 + threadIdx.x; m < m1; m += PS_BLOCK) {
+        T dx = xm[3 * m] - xc, dy = xm[3 * m + 1] - yc, dz = xm[3 * m + 2] - zc;
+        if (BOX) min_img<T>(s, dx, dy, dz);
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        const T rinv = ps_rsqrt(r2);
+        const T ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
+        const T rinv2 = rinv * rinv, rinv4 = rinv2 * rinv2;
+        const T y12b = -(T)C3D4 * rinv2;
+        const T x12g = (T)C9D4 * rinv2 - gA * rinv4;
+        const T y12g = gB * rinv4;
+        const T fx = (T)f[3 * m], fy = (T)f[3 * m + 1], fz = (T)f[3 * m + 2];
+        fsx += fx;
+        fsy += fy;
+        fsz += fz;
+        // rotation rows: y12b (f x e)
+        acc[3] += y12b * (fy * ez - fz * ey);
+        acc[4] += y12b * (fz * ex - fx * ez);
+        acc[5] += y12b * (fx * ey - fy * ex);
+        // stresslet rows
+        const T ef = ex * fx + ey * fy + ez * fz;
+        const T p = -x12g * ef - (T)2 * y12g * ef;   // coefficient of e_i e_j
+        // G_ij = p e_i e_j + y12g (e_i f_j + e_j f_i) + (x12g ef / 3) delta_ij
+        const T G00 = p * ex * ex + (T)2 * y12g * ex * fx;
+        const T G11 = p * ey * ey + (T)2 * y12g * ey * fy;
+        const T G22 = p * ez * ez + (T)2 * y12g * ez * fz;
+        const T G01 = p * ex * ey + y12g * (ex * fy + ey * fx);
+        const T G02 = p * ex * ez + y12g * (ex * fz + ez * fx);
+        const T G12 = p * ey * ez + y12g * (ey * fz + ez * fy);
+        acc[6] += G00 - G22;
+        acc[7] += (T)2 * G01;
+        acc[8] += (T)2 * G02;
+        acc[9] += (T)2 * G12;
+        acc[10] += G11 - G22;
+    }
+    acc[0] = fsx;
+    acc[1] = fsy;
+    acc[2] = fsz;
+    block_sum<T, 11>(acc, sh);
+    if (threadIdx.x == 0) {
+        T* out = part_c + ((int64_t)ch * Nc + c) * 12;
+        out[0] = 0;
+        out[1] = 0;
+        out[2] = 0;
+#pragma unroll
+        for (int k = 3; k < 11; k++) out[k] = acc[k];
+        if (c == 0) {
+            part_fs[3 * ch] = acc[0];
+            part_fs[3 * ch + 1] = acc[1];
+            part_fs[3 * ch + 2] = acc[2];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------- BB (FP64)
+__device__ __forceinline__ double eps3(int i, int j, int k) {
+    if (i == j || j == k || i == k) return 0.0;
+    if ((i == 0 && j == 1 && k == 2) || (i == 1 && j == 2 && k == 0) || (i == 2 && j == 0 && k == 1))
+        return 1.0;
+    return -1.0;
+}
+__device__ __forceinline__ double del(int i, int j) { return i == j ? 1.0 : 0.0; }
+__device__ __forceinline__ int third(int a, int b) {
+    int k = 3 - a - b;
+    if (k == -1) k = 2;
+    if (k == 3) k = 0;
+    return k;
+}
+
+struct BBT {
+    double a[3][3], b[3][3], bt[3][3], c[3][3], gt[3][5], ht[3][5], m[5][5];
+};
+
+// The full colloid-colloid FTS tile of mobility(..., AB=false, AA=false), src/mob.cpp:96-356.
+__device__ void tile_bb(double dr_inv, const double e[3], BBT& T) {
+    double ee[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) ee[i][j] = e[i] * e[j];
+    const double dr_inv2 = dr_inv * dr_inv, dr_inv3 = dr_inv2 * dr_inv, dr_inv4 = dr_inv3 * dr_inv,
+                 dr_inv5 = dr_inv4 * dr_inv;
+    const double x12a = C3D2 * dr_inv - dr_inv3, y12a = C3D4 * dr_inv + C1D2 * dr_inv3;
+    const double y12b = -C3D4 * dr_inv2;
+    const double x12g = C9D4 * dr_inv2 - C18D5 * dr_inv4, y12g = C6D5 * dr_inv4;
+    const double x12c = C3D4 * dr_inv3, y12c = -C3D8 * dr_inv3;
+    const double y12h = -C9D8 * dr_inv3;
+    const double x12m = -C9D2 * dr_inv3 + C54D5 * dr_inv5, y12m = C9D4 * dr_inv3 - C36D5 * dr_inv5,
+                 z12m = C9D5 * dr_inv5;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            T.a[i][j] = (i == j) ? x12a * ee[i][i] + y12a * (1. - ee[i][i]) : x12a * ee[i][j] - y12a * ee[i][j];
+            T.c[i][j] = (i == j) ? x12c * ee[i][i] + y12c * (1. - ee[i][i]) : x12c * ee[i][j] + y12c * ee[i][j];
+            const int k = third(i, j);
+            T.b[i][j] = y12b * eps3(i, j, k) * e[k];
+        }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) T.bt[j][i] = T.b[i][j];
+    double gt[3][3][3], ht[3][3][3];
+    for (int k = 0; k < 3; k++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                gt[k][i][j] = -x12g * (ee[i][j] - C1D3 * del(i, j)) * e[k] +
+                              y12g * (e[i] * del(j, k) + e[j] * del(i, k) - 2. * ee[i][j] * e[k]);
+                const int ll = third(j, k), mm = third(i, k);
+                ht[k][i][j] = y12h * (ee[i][ll] * eps3(j, k, ll) + ee[j][mm] * eps3(i, k, mm));
+            }
+    for (int i = 0; i < 3; i++) {
+        T.gt[i][0] = gt[i][0][0] - gt[i][2][2];
+        T.gt[i][1] = 2. * gt[i][0][1];
+        T.gt[i][2] = 2. * gt[i][0][2];
+        T.gt[i][3] = 2. * gt[i][1][2];
+        T.gt[i][4] = gt[i][1][1] - gt[i][2][2];
+        T.ht[i][0] = ht[i][0][0] - ht[i][2][2];
+        T.ht[i][1] = 2. * ht[i][0][1];
+        T.ht[i][2] = 2. * ht[i][0][2];
+        T.ht[i][3] = 2. * ht[i][1][2];
+        T.ht[i][4] = ht[i][1][1] - ht[i][2][2];
+    }
+    // m_ijkl contracted through mesid (src/mob.cpp:278-355).
+    const int MA[5] = {0, 0, 0, 1, 1}, MB[5] = {2, 1, 2, 2, 2};
+    auto mt = [&](int i, int j, int k, int l) {
+        return C3D2 * x12m * (ee[i][j] - C1D3 * del(i, j)) * (ee[k][l] - C1D3 * del(k, l)) +
+               C1D2 * y12m *
+                   (ee[i][k] * del(j, l) + ee[j][k] * del(i, l) + ee[i][l] * del(j, k) +
+                    ee[j][l] * del(i, k) - 4. * ee[i][j] * ee[k][l]) +
+               C1D2 * z12m *
+                   (del(i, k) * del(j, l) + del(j, k) * del(i, l) - del(i, j) * del(k, l) +
+                    ee[i][j] * del(k, l) + ee[k][l] * del(i, j) - ee[i][k] * del(j, l) -
+                    ee[j][k] * del(i, l) - ee[i][l] * del(j, k) - ee[j][l] * del(i, k) + ee[i][j] * ee[k][l]);
+    };
+    for (int r = 0; r < 5; r++) {
+        const int a = MA[r], b = MB[r];
+        if (r == 0 || r == 4) {
+            T.m[r][0] = mt(a, a, 0, 0) - mt(a, a, 2, 2) - (mt(b, b, 0, 0) - mt(b, b, 2, 2));
+            T.m[r][1] = 2. * (mt(a, a, 0, 1) - mt(b, b, 0, 1));
+            T.m[r][2] = 2. * (mt(a, a, 0, 2) - mt(b, b, 0, 2));
+            T.m[r][3] = 2. * (mt(a, a, 1, 2) - mt(b, b, 1, 2));
+            T.m[r][4] = mt(a, a, 1, 1) - mt(a, a, 2, 2) - (mt(b, b, 1, 1) - mt(b, b, 2, 2));
+        } else {
+            T.m[r][0] = 2. * (mt(a, b, 0, 0) - mt(a, b, 2, 2));
+            T.m[r][1] = 4. * mt(a, b, 0, 1);
+            T.m[r][2] = 4. * mt(a, b, 0, 2);
+            T.m[r][3] = 4. * mt(a, b, 1, 2);
+            T.m[r][4] = 2. * (mt(a, b, 1, 1) - mt(a, b, 2, 2));
+        }
+    }
+}
+
+__device__ __forceinline__ void pair_geom_dev(const SysConst& s, const double* xs, int k, int j, double e[3],
+                                              double& dr, double& dr_inv) {
+#pragma clang fp contract(off)
+    double dx = xs[3 * k] - xs[3 * j];
+    double dy = xs[3 * k + 1] - xs[3 * j + 1];
+    double dz = xs[3 * k + 2] - xs[3 * j + 2];
+    if (s.box_active) {
+        dx -= s.L[0] * __builtin_rint(dx * s.Linv[0]);
+        dy -= s.L[1] * __builtin_rint(dy * s.Linv[1]);
+        dz -= s.L[2] * __builtin_rint(dz * s.Linv[2]);
+    }
+    dr = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+    dr_inv = 1 / dr;
+    e[0] = dx * dr_inv;
+    e[1] = dy * dr_inv;
+    e[2] = dz * dr_inv;
+}
+
+// Colloid rows from colloid sources: one block per colloid, threads over partners.
+__global__ __launch_bounds__(PS_BLOCK) void k_bb(SysConst s, const double* __restrict__ xm,
+                                                 const double* __restrict__ f, double* __restrict__ bb) {
+    __shared__ double sh[4][11];
+    const int Nm = s.Nm, Nc = s.Nc;
+    const int c = blockIdx.x;
+    double acc[11];
+    for (int k = 0; k < 11; k++) acc[k] = 0.0;
+    const int64_t nm3 = s.nm3, nc3 = s.nc3, nu = s.nu;
+    for (int o = threadIdx.x; o < Nc; o += PS_BLOCK) {
+        if (o == c) continue;
+        const bool role1 = c < o;   // pair (min, max); c is particle "1" when it is the smaller index
+        const int c1 = role1 ? c : o, c2 = role1 ? o : c;
+        double e[3], dr, dr_inv;
+        pair_geom_dev(s, xm, Nm + c1, Nm + c2, e, dr, dr_inv);
+        BBT T;
+        tile_bb(dr_inv, e, T);
+        double Fo[3], To[3], So[5];
+        for (int k = 0; k < 3; k++) Fo[k] = f[nm3 + 3 * o + k];
+        for (int k = 0; k < 3; k++) To[k] = f[nm3 + nc3 + 3 * o + k];
+        for (int k = 0; k < 5; k++) So[k] = f[nu + 5 * o + k];
+        if (role1) {
+            for (int i = 0; i < 3; i++) {
+                double u = 0.0, w = 0.0;
+                for (int j = 0; j < 3; j++) {
+                    u += T.a[i][j] * Fo[j] - T.bt[j][i] * To[j];
+                    w += T.b[i][j] * Fo[j] + T.c[i][j] * To[j];
+                }
+                for (int j = 0; j < 5; j++) {
+                    u += T.gt[i][j] * So[j];
+                    w += T.ht[i][j] * So[j];
+                }
+                acc[i] += u;
+                acc[3 + i] += w;
+            }
+            for (int j = 0; j < 5; j++) {
+                double v = 0.0;
+                for (int i = 0; i < 3; i++) v += -T.gt[i][j] * Fo[i] + T.ht[i][j] * To[i];
+                for (int k = 0; k < 5; k++) v += T.m[j][k] * So[k];
+                acc[6 + j] += v;
+            }
+        } else {
+            for (int i = 0; i < 3; i++) {
+                double u = 0.0, w = 0.0;
+                for (int j = 0; j < 3; j++) {
+                    u += T.a[j][i] * Fo[j] + T.bt[i][j] * To[j];
+                    w += -T.b[j][i] * Fo[j] + T.c[j][i] * To[j];
+                }
+                for (int j = 0; j < 5; j++) {
+                    u += -T.gt[i][j] * So[j];
+                    w += T.ht[i][j] * So[j];
+                }
+                acc[i] += u;
+                acc[3 + i] += w;
+            }
+            for (int j = 0; j < 5; j++) {
+                double v = 0.0;
+                for (int i = 0; i < 3; i++) v += T.gt[i][j] * Fo[i] + T.ht[i][j] * To[i];
+                for (int k = 0; k < 5; k++) v += T.m[k][j] * So[k];
+                acc[6 + j] += v;
+            }
+        }
+    }
+    block_sum<double, 11>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 11; k++) bb[(int64_t)c * 11 + k] = acc[k];
+}
+
+// ---------------------------------------------------------------------------- finalize
+// mode 0: u = M f;  mode 1: saddle y = A x with x = [f ; lambda] (lambda = f + n).
+template <class T>
+__global__ __launch_bounds__(PS_BLOCK) void k_finalize(SysConst s, const double* __restrict__ f,
+                                                       double* __restrict__ u, const T* __restrict__ part_aa,
+                                                       int nsplit, const T* __restrict__ part_abm,
+                                                       const T* __restrict__ part_c,
+                                                       const T* __restrict__ part_fs, int nchunk,
+                                                       const double* __restrict__ bb,
+                                                       const double* __restrict__ stale, int saddle) {
+    const int64_t r = (int64_t)blockIdx.x * PS_BLOCK + threadIdx.x;
+    const int64_t n = s.n, nm3 = s.nm3, nc3 = s.nc3, nu = s.nu;
+    const int Nm = s.Nm, Nc = s.Nc;
+    if (r >= n) return;
+    double v;
+    if (r < nm3) {
+        const int i = (int)(r / 3), comp = (int)(r - 3 * (int64_t)i);
+        double acc = 0.0;
+        for (int sp = 0; sp < nsplit; sp++) acc += (double)part_aa[((int64_t)sp * 3 + comp) * Nm + i];
+        acc += (double)part_abm[(int64_t)comp * Nm + i];
+        v = s.beta_inv * f[r] + acc;
+    } else if (r < nm3 + nc3) {
+        const int64_t q = r - nm3;
+        const int c = (int)(q / 3), comp = (int)(q - 3 * (int64_t)c);
+        double fs[3] = {0.0, 0.0, 0.0};
+        double acc = 0.0;
+        for (int ch = 0; ch < nchunk; ch++) {
+            fs[0] += (double)part_fs[3 * ch];
+            fs[1] += (double)part_fs[3 * ch + 1];
+            fs[2] += (double)part_fs[3 * ch + 2];
+        }
+        for (int ch = 0; ch < nchunk; ch++) acc += (double)part_c[((int64_t)ch * Nc + c) * 12 + comp];
+        // transposed carried tile: U_c[comp] += sum_jj stale[jj][comp] * sum_m f_m[jj]
+        acc += stale[0 * 3 + comp] * fs[0] + stale[1 * 3 + comp] * fs[1] + stale[2 * 3 + comp] * fs[2];
+        v = 1.0 * f[r] + acc + bb[(int64_t)c * 11 + comp];
+    } else if (r < nu) {
+        const int64_t q = r - nm3 - nc3;
+        const int c = (int)(q / 3), comp = (int)(q - 3 * (int64_t)c);
+        double acc = 0.0;
+        for (int ch = 0; ch < nchunk; ch++) acc += (double)part_c[((int64_t)ch * Nc + c) * 12 + 3 + comp];
+        v = (3. / 4.) * f[r] + acc + bb[(int64_t)c * 11 + 3 + comp];
+    } else {
+        const int64_t q = r - nu;
+        const int c = (int)(q / 5), comp = (int)(q - 5 * (int64_t)c);
+        const double* S = f + nu + 5 * (int64_t)c;
+        double self = (9. / 5.) * S[comp];
+        if (comp == 0) self += (9. / 10.) * S[4];
+        if (comp == 4) self += (9. / 10.) * S[0];
+        double acc = 0.0;
+        for (int ch = 0; ch < nchunk; ch++) acc += (double)part_c[((int64_t)ch * Nc + c) * 12 + 6 + comp];
+        v = self + acc + bb[(int64_t)c * 11 + 6 + comp];
+    }
+    if (saddle) {
+        if (r < nu) {
+            v -= f[n + r];
+            u[n + r] = -f[r];
+        }
+    }
+    u[r] = v;
+}
+
+// ---------------------------------------------------------------------------- carried tile
+// stale[0..8]: the mob_a the AB pairs of this mob() call see; stale[9..17]: what is left for the
+// next call (the last BB pair's mob_a, or this call's tile when there is no BB pair).
+__global__ void k_stale(SysConst s, const double* __restrict__ xm, int si, int sj, double* __restrict__ st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double cur[9];
+    if (si >= 0) {
+        double e[3], dr, dr_inv;
+        pair_geom_dev(s, xm, si, sj, e, dr, dr_inv);
+        const double dr_inv2 = dr_inv * dr_inv, dr_inv3 = dr_inv2 * dr_inv;
+        double x12a, y12a;
+        if (dr > 2.0 * s.beta) {
+            x12a = C3D8 * dr_inv - 2.0 * dr_inv3;
+            y12a = C3D4 * dr_inv - C1D2 * dr_inv3;
+        } else {
+            x12a = C3D8 * dr - 1.0;
+            y12a = 1.0 - C9D32 * dr;
+        }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const double ee = e[i] * e[j];
+                cur[3 * i + j] = (i == j) ? x12a * ee + y12a * (1. - ee) : x12a * ee - y12a * ee;
+            }
+    } else {
+        for (int k = 0; k < 9; k++) cur[k] = st[9 + k];
+    }
+    for (int k = 0; k < 9; k++) st[k] = cur[k];
+    if (s.Nc >= 2) {
+        double e[3], dr, dr_inv;
+        pair_geom_dev(s, xm, s.Np - 2, s.Np - 1, e, dr, dr_inv);
+        const double dr_inv2 = dr_inv * dr_inv, dr_inv3 = dr_inv2 * dr_inv;
+        const double x12a = C3D2 * dr_inv - dr_inv3, y12a = C3D4 * dr_inv + C1D2 * dr_inv3;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const double ee = e[i] * e[j];
+                st[9 + 3 * i + j] = (i == j) ? x12a * ee + y12a * (1. - ee) : x12a * ee - y12a * ee;
+            }
+    } else {
+        for (int k = 0; k < 9; k++) st[9 + k] = cur[k];
+    }
+}
+
+__global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * PS_BLOCK + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+template <class T>
+ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double* u, int saddle) {
+    const SysConst& s = c->sc;
+    hipStream_t st = c->stream;
+    hipEvent_t ea0 = nullptr, ea1 = nullptr, ek0 = nullptr, ek1 = nullptr;
+    if (c->prof_on) {
+        PS_TRY(c->prof_record(1, &ea0, &ea1));
+        PS_HIP(c, hipEventRecord(ea0, st));
+    }
+    if (s.Nm > 0) {
+        dim3 g1((s.Nm + PS_BLOCK - 1) / PS_BLOCK, aw.nsplit_aa);
+        if (c->prof_on) {
+            PS_TRY(c->prof_record(0, &ek0, &ek1));
+            PS_HIP(c, hipEventRecord(ek0, st));
+        }
+        const bool box = s.box_active, ch = s.chain_HI;
+        if (box && ch)
+            k_aa<T, true, true><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
+        else if (box)
+            k_aa<T, true, false><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
+        else if (ch)
+            k_aa<T, false, true><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
+        else
+            k_aa<T, false, false><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
+        PS_LAUNCH(c);
+        if (c->prof_on) PS_HIP(c, hipEventRecord(ek1, st));
+        dim3 g2((s.Nm + PS_BLOCK - 1) / PS_BLOCK);
+        if (box)
+            k_ab_mono<T, true><<<g2, PS_BLOCK, 0, st>>>(s, xm, f, c->stale, (T*)aw.part_abm);
+        else
+            k_ab_mono<T, false><<<g2, PS_BLOCK, 0, st>>>(s, xm, f, c->stale, (T*)aw.part_abm);
+        PS_LAUNCH(c);
+    }
+    if (s.Nc > 0) {
+        if (s.Nm > 0) {
+            dim3 g3(s.Nc, aw.nchunk_c);
+            if (s.box_active)
+                k_ab_coll<T, true><<<g3, PS_BLOCK, 0, st>>>(s, xm, f, aw.per_chunk_c, (T*)aw.part_c, (T*)aw.part_fs);
+            else
+                k_ab_coll<T, false><<<g3, PS_BLOCK, 0, st>>>(s, xm, f, aw.per_chunk_c, (T*)aw.part_c, (T*)aw.part_fs);
+            PS_LAUNCH(c);
+        }
+        k_bb<<<s.Nc, PS_BLOCK, 0, st>>>(s, c->xm, f, aw.bb);
+        PS_LAUNCH(c);
+    }
+    dim3 g4((unsigned)((s.n + PS_BLOCK - 1) / PS_BLOCK));
+    k_finalize<T><<<g4, PS_BLOCK, 0, st>>>(s, f, u, (const T*)aw.part_aa, s.Nm > 0 ? aw.nsplit_aa : 0,
+                                           (const T*)aw.part_abm, (const T*)aw.part_c, (const T*)aw.part_fs,
+                                           (s.Nm > 0 && s.Nc > 0) ? aw.nchunk_c : 0, aw.bb, c->stale, saddle);
+    PS_LAUNCH(c);
+    if (c->prof_on) PS_HIP(c, hipEventRecord(ea1, st));
+    c->cnt.applies++;
+    return PS_OK;
+}
+
+ps_err alloc_work(Ctx* c, ApplyWork& aw, size_t elt) {
+    const SysConst& s = c->sc;
+    const int ntb = (s.Nm + PS_BLOCK - 1) / PS_BLOCK;
+    const int target = 4 * (c->n_cu > 0 ? c->n_cu : 256);
+    int nsplit = ntb > 0 ? (target + ntb - 1) / ntb : 1;
+    nsplit = std::max(1, std::min(nsplit, std::max(1, s.Nm / 64)));
+    aw.nsplit_aa = nsplit;
+    aw.per_split_aa = s.Nm > 0 ? (s.Nm + nsplit - 1) / nsplit : 0;
+    aw.nsplit_aa = s.Nm > 0 ? (s.Nm + aw.per_split_aa - 1) / aw.per_split_aa : 1;
+    int nchunk = s.Nc > 0 ? (2 * (c->n_cu > 0 ? c->n_cu : 256) + s.Nc - 1) / s.Nc : 1;
+    nchunk = std::max(1, std::min(nchunk, std::max(1, s.Nm / PS_BLOCK)));
+    aw.per_chunk_c = s.Nm > 0 ? (s.Nm + nchunk - 1) / nchunk : 0;
+    aw.nchunk_c = s.Nm > 0 ? (s.Nm + aw.per_chunk_c - 1) / aw.per_chunk_c : 1;
+    aw.bytes_aa = (size_t)aw.nsplit_aa * 3 * std::max(s.Nm, 1) * elt;
+    aw.bytes_abm = (size_t)3 * std::max(s.Nm, 1) * elt;
+    aw.bytes_c = (size_t)aw.nchunk_c * std::max(s.Nc, 1) * 12 * elt;
+    aw.bytes_fs = (size_t)aw.nchunk_c * 3 * elt;
+    PS_HIP(c, hipMalloc(&aw.part_aa, aw.bytes_aa));
+    PS_HIP(c, hipMalloc(&aw.part_abm, aw.bytes_abm));
+    PS_HIP(c, hipMalloc(&aw.part_c, aw.bytes_c));
+    PS_HIP(c, hipMalloc(&aw.part_fs, aw.bytes_fs));
+    PS_HIP(c, hipMalloc(&aw.bb, sizeof(double) * 11 * std::max(s.Nc, 1)));
+    PS_HIP(c, hipMemsetAsync(aw.part_c, 0, aw.bytes_c, c->stream));
+    PS_HIP(c, hipMemsetAsync(aw.part_fs, 0, aw.bytes_fs, c->stream));
+    PS_HIP(c, hipMemsetAsync(aw.bb, 0, sizeof(double) * 11 * std::max(s.Nc, 1), c->stream));
+    return PS_OK;
+}
+
+void free_work(ApplyWork& aw) {
+    for (void* p : {aw.part_aa, aw.part_abm, aw.part_c, aw.part_fs, (void*)aw.bb})
+        if (p) (void)hipFree(p);
+    aw = ApplyWork();
+}
+
+}  // namespace
+
+ps_err apply_alloc(Ctx* c) {
+    PS_TRY(alloc_work(c, c->aw64, sizeof(double)));
+    if (c->p.precision == PS_PREC_MIXED) {
+        PS_TRY(alloc_work(c, c->aw32, sizeof(float)));
+        PS_HIP(c, hipMalloc(&c->xm32, sizeof(float) * 3 * (size_t)c->Np));
+    }
+    return PS_OK;
+}
+
+void apply_free(Ctx* c) {
+    free_work(c->aw64);
+    free_work(c->aw32);
+    if (c->xm32) (void)hipFree(c->xm32);
+    c->xm32 = nullptr;
+}
+
+// mob(): snapshot the positions of the last check_dist and compute the carried translation tile.
+ps_err mob_prepare(Ctx* c) {
+    PS_HIP(c, hipMemcpyAsync(c->xm, c->xpd, sizeof(double) * 3 * (size_t)c->Np, hipMemcpyDeviceToDevice,
+                             c->stream));
+    k_stale<<<1, 64, 0, c->stream>>>(c->sc, c->xm, c->stale_i, c->stale_j, c->stale);
+    PS_LAUNCH(c);
+    if (c->xm32) {
+        const int64_t n3 = 3 * (int64_t)c->Np;
+        k_to_f32<<<(unsigned)((n3 + PS_BLOCK - 1) / PS_BLOCK), PS_BLOCK, 0, c->stream>>>(c->xm, c->xm32, n3);
+        PS_LAUNCH(c);
+    }
+    c->cnt.mob_calls++;
+    return PS_OK;
+}
+
+ps_err mob_apply(Ctx* c, const double* f, double* u, int prec) {
+    if (prec == PS_PREC_MIXED && c->xm32) return launch_apply<float>(c, c->xm32, c->aw32, f, u, 0);
+    return launch_apply<double>(c, c->xm, c->aw64, f, u, 0);
+}
+
+ps_err saddle_apply(Ctx* c, const double* x, double* y, int prec) {
+    if (prec == PS_PREC_MIXED && c->xm32) return launch_apply<float>(c, c->xm32, c->aw32, x, y, 1);
+    return launch_apply<double>(c, c->xm, c->aw64, x, y, 1);
+}

@@ -1,0 +1,499 @@
+// pairs.hip -- check_dist (Verlet pairs) and RHS (forces) on the device.
+//
+// check_dist (src/check_dist.cpp:12-87) visits all Np(Np-1)/2 pairs; here a periodic cell list of
+// edge >= max(rverls) yields the candidates, and every candidate is tested with exactly the
+// reference's FP expression sequence (d = x_k - x_j with k < j, minimum image by nearbyint,
+// dr = sqrt(dx*dx + dy*dy + dz*dz) without contraction, dr < (float)rverl), so the pair set is
+// bit-identical to the O(N^2) scan.  The list is kept in both directions (row p holds every partner
+// q != p) sorted by partner, so forces reduce per particle with no atomics; the reference's
+// vlist[k] is the q > k half, in the same order (pair index order, src/init.cpp:225-262).
+// Without a periodic box, or with fewer than 3 cells per axis, the candidates are all particles.
+#include "ps_common.h"
+
+#include <algorithm>
+#include <cmath>
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int TB = PS_BLOCK;
+
+struct PairConst {
+    SysConst s;
+    double rverl[3];
+    double sig[3];
+    int ncell[3];
+    double cell_size[3], cell_org[3];
+    int mode;   // 0: all-pairs candidates, 1: periodic cell list
+};
+
+__device__ __forceinline__ int ptype(int Nm, int k, int j) {
+    if (k < Nm && j < Nm) return 0;
+    if (k < Nm) return 1;
+    return 2;
+}
+
+// The reference pair geometry with k < j (src/check_dist.cpp:49-71).
+__device__ __forceinline__ void geom(const SysConst& s, const double* xs, int k, int j, double& dx, double& dy,
+                                     double& dz, double& dr) {
+    dx = xs[3 * k] - xs[3 * j];
+    dy = xs[3 * k + 1] - xs[3 * j + 1];
+    dz = xs[3 * k + 2] - xs[3 * j + 2];
+    if (s.box_active) {
+        dx -= s.L[0] * __builtin_rint(dx * s.Linv[0]);
+        dy -= s.L[1] * __builtin_rint(dy * s.Linv[1]);
+        dz -= s.L[2] * __builtin_rint(dz * s.Linv[2]);
+    }
+    dr = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+__device__ __forceinline__ bool within(const PairConst& pc, const double* xs, int p, int q) {
+    const int k = min(p, q), j = max(p, q);
+    double dx, dy, dz, dr;
+    geom(pc.s, xs, k, j, dx, dy, dz, dr);
+    return dr < pc.rverl[ptype(pc.s.Nm, k, j)];
+}
+
+__device__ __forceinline__ int cell_coord(const PairConst& pc, double x, int d) {
+    // wrap into [org, org + L) for binning only; the pair test uses the raw positions
+    const double L = pc.s.L[d];
+    double xw = x - L * floor((x - pc.cell_org[d]) / L);
+    int ci = (int)floor((xw - pc.cell_org[d]) / pc.cell_size[d]);
+    if (ci < 0) ci = 0;
+    if (ci >= pc.ncell[d]) ci = pc.ncell[d] - 1;
+    return ci;
+}
+
+__global__ void k_bin(PairConst pc, const double* __restrict__ xs, int* __restrict__ cell_of,
+                      int* __restrict__ cell_count) {
+    const int p = blockIdx.x * TB + threadIdx.x;
+    if (p >= pc.s.Np) return;
+    const int cx = cell_coord(pc, xs[3 * p], 0), cy = cell_coord(pc, xs[3 * p + 1], 1),
+              cz = cell_coord(pc, xs[3 * p + 2], 2);
+    const int cell = cx + pc.ncell[0] * (cy + pc.ncell[1] * cz);
+    cell_of[p] = cell;
+    atomicAdd(&cell_count[cell], 1);
+}
+
+__global__ void k_scatter(int Np, const int* __restrict__ cell_of, const int* __restrict__ cell_start,
+                          int* __restrict__ cursor, int* __restrict__ perm) {
+    const int p = blockIdx.x * TB + threadIdx.x;
+    if (p >= Np) return;
+    const int c = cell_of[p];
+    perm[cell_start[c] + atomicAdd(&cursor[c], 1)] = p;
+}
+
+// Single-block exclusive scan of n ints: out[0..n] (out[n] = total).
+__global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ in, int* __restrict__ out, int n) {
+    __shared__ int sh[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int v = (i < n) ? in[i] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int t = (threadIdx.x >= off) ? sh[threadIdx.x - off] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < n) out[i] = carry + sh[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += sh[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+template <class F>
+__device__ __forceinline__ void for_candidates(const PairConst& pc, const double* xs, const int* cell_of,
+                                               const int* cell_start, const int* perm, int p, F&& fn) {
+    if (pc.mode == 0) {
+        for (int q = 0; q < pc.s.Np; q++)
+            if (q != p) fn(q);
+        return;
+    }
+    const int c = cell_of[p];
+    const int nx = pc.ncell[0], ny = pc.ncell[1], nz = pc.ncell[2];
+    const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+    for (int oz = -1; oz <= 1; oz++)
+        for (int oy = -1; oy <= 1; oy++)
+            for (int ox = -1; ox <= 1; ox++) {
+                const int ax = (cx + ox + nx) % nx, ay = (cy + oy + ny) % ny, az = (cz + oz + nz) % nz;
+                const int cc = ax + nx * (ay + ny * az);
+                for (int t = cell_start[cc]; t < cell_start[cc + 1]; t++) {
+                    const int q = perm[t];
+                    if (q != p) fn(q);
+                }
+            }
+}
+
+__global__ void k_nb_count(PairConst pc, const double* __restrict__ xs, const int* __restrict__ cell_of,
+                           const int* __restrict__ cell_start, const int* __restrict__ perm,
+                           int* __restrict__ count) {
+    const int p = blockIdx.x * TB + threadIdx.x;
+    if (p >= pc.s.Np) return;
+    int cnt = 0;
+    for_candidates(pc, xs, cell_of, cell_start, perm, p, [&](int q) {
+        if (within(pc, xs, p, q)) cnt++;
+    });
+    count[p] = cnt;
+}
+
+__global__ void k_nb_fill(PairConst pc, const double* __restrict__ xs, const int* __restrict__ cell_of,
+                          const int* __restrict__ cell_start, const int* __restrict__ perm,
+                          const int* __restrict__ off, int* __restrict__ part) {
+    const int p = blockIdx.x * TB + threadIdx.x;
+    if (p >= pc.s.Np) return;
+    const int o = off[p];
+    int cnt = 0;
+    for_candidates(pc, xs, cell_of, cell_start, perm, p, [&](int q) {
+        if (within(pc, xs, p, q)) part[o + cnt++] = q;
+    });
+    // insertion sort (rows are short) -> reference order (increasing partner / pair index)
+    for (int a = 1; a < cnt; a++) {
+        const int v = part[o + a];
+        int b = a - 1;
+        while (b >= 0 && part[o + b] > v) {
+            part[o + b + 1] = part[o + b];
+            b--;
+        }
+        part[o + b + 1] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------- forces
+struct ForceConst {
+    SysConst s;
+    double sig[3];
+    int mono_ev, force_sel, fene;
+    double k_hs, hs_exp_k, kbond, r0, Lmax, ktrap;
+    int trap_on;
+};
+
+// rhs = [0 (n) ; fint (3Np) ; 0 (3Nc)] with fint = pairs + bonds + trap (src/rhs.cpp:160-260).
+__global__ void k_forces(ForceConst fc, const double* __restrict__ xpd, const double* __restrict__ x,
+                         const double* __restrict__ xpos0, const int* __restrict__ off,
+                         const int* __restrict__ part, const int* __restrict__ boff,
+                         const int* __restrict__ bond_k, const int* __restrict__ bond_j,
+                         double* __restrict__ rhs) {
+    const int p = blockIdx.x * TB + threadIdx.x;
+    const SysConst& s = fc.s;
+    if (p >= s.Np) return;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    // excluded volume over the Verlet partners (pair_interaction_hs / _hs_exp)
+    for (int t = off[p]; t < off[p + 1]; t++) {
+        const int q = part[t];
+        const int k = min(p, q), j = max(p, q);
+        const int type = ptype(s.Nm, k, j);
+        if (type == 0 && !fc.mono_ev) continue;
+        double dx, dy, dz, dr;
+        geom(s, xpd, k, j, dx, dy, dz, dr);
+        const double dr_inv = 1 / dr;
+        const double ex = dx * dr_inv, ey = dy * dr_inv, ez = dz * dr_inv;
+        const double sig_eff = fc.sig[type] * (1.0 + 0.01);
+        double fmag;
+        if (fc.force_sel == PS_FORCE_HS_EXP) {
+            const double ex_ = exp(-fc.hs_exp_k * (dr - sig_eff));
+            fmag = ex_ / (1 - ex_);
+        } else {
+            fmag = (dr >= sig_eff) ? 0.0 : fc.k_hs * (sig_eff - dr);
+        }
+        if (fmag == 0.0) continue;
+        const double gx = fmag * ex, gy = fmag * ey, gz = fmag * ez;
+        if (p == k) {
+            fx -= gx;
+            fy -= gy;
+            fz -= gz;
+        } else {
+            fx += gx;
+            fy += gy;
+            fz += gz;
+        }
+    }
+    // bonds (bond_forces, src/rhs.cpp:102-158)
+    for (int t = boff[p]; t < boff[p + 1]; t++) {
+        const int k = bond_k[t], j = bond_j[t];
+        double r0_ = x[3 * k] - x[3 * j], r1_ = x[3 * k + 1] - x[3 * j + 1], r2_ = x[3 * k + 2] - x[3 * j + 2];
+        if (s.box_active) {
+            r0_ -= s.L[0] * __builtin_rint(r0_ * s.Linv[0]);
+            r1_ -= s.L[1] * __builtin_rint(r1_ * s.Linv[1]);
+            r2_ -= s.L[2] * __builtin_rint(r2_ * s.Linv[2]);
+        }
+        const double r = __builtin_sqrt(r0_ * r0_ + r1_ * r1_ + r2_ * r2_);
+        double f0 = -fc.kbond * (1 - (fc.r0 / r)) * r0_;
+        double f1 = -fc.kbond * (1 - (fc.r0 / r)) * r1_;
+        double f2 = -fc.kbond * (1 - (fc.r0 / r)) * r2_;
+        if (fc.fene) {
+            const double ext2 = 1 / (1 - (r / fc.Lmax) * (r / fc.Lmax));
+            f0 *= ext2;
+            f1 *= ext2;
+            f2 *= ext2;
+        }
+        if (p == k) {
+            fx -= f0;
+            fy -= f1;
+            fz -= f2;
+        } else {
+            fx += f0;
+            fy += f1;
+            fz += f2;
+        }
+    }
+    // static harmonic trap on the colloids (trapping_forces, src/rhs.cpp:46-100)
+    if (fc.trap_on && p >= s.Nm) {
+        const int ic3 = 3 * (p - s.Nm);
+        double dx = x[3 * p] - xpos0[ic3], dy = x[3 * p + 1] - xpos0[ic3 + 1], dz = x[3 * p + 2] - xpos0[ic3 + 2];
+        if (s.box_active) {
+            dx -= s.L[0] * __builtin_rint(dx * s.Linv[0]);
+            dy -= s.L[1] * __builtin_rint(dy * s.Linv[1]);
+            dz -= s.L[2] * __builtin_rint(dz * s.Linv[2]);
+        }
+        fx += fc.ktrap * dx;
+        fy += fc.ktrap * dy;
+        fz += fc.ktrap * dz;
+    }
+    rhs[s.n + 3 * (int64_t)p] = fx;
+    rhs[s.n + 3 * (int64_t)p + 1] = fy;
+    rhs[s.n + 3 * (int64_t)p + 2] = fz;
+}
+
+__global__ void k_rhs_clear(SysConst s, double* __restrict__ rhs) {
+    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i >= s.ns) return;
+    if (i < s.n || i >= s.n + 3 * (int64_t)s.Np) rhs[i] = 0.0;
+}
+__global__ void k_rhs_drift(SysConst s, const double* __restrict__ fext, double* __restrict__ rhs) {
+    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i >= s.ns) return;
+    rhs[i] = (i < s.n) ? 0.0 : fext[i - s.n];
+}
+
+PairConst make_pc(Ctx* c) {
+    PairConst pc{};
+    pc.s = c->sc;
+    const float rv[3] = {(float)(0.05 + 2 * c->p.beta), (float)(1.5 + c->p.beta), (float)2.5};
+    const float sg[3] = {(float)(2. * c->p.beta), (float)(1. + c->p.beta), (float)2.0};
+    for (int k = 0; k < 3; k++) {
+        pc.rverl[k] = rv[k];
+        pc.sig[k] = sg[k];
+        pc.ncell[k] = c->ncell[k];
+        pc.cell_size[k] = c->cell_size[k];
+        pc.cell_org[k] = c->cell_org[k];
+    }
+    pc.mode = c->cell_mode;
+    return pc;
+}
+
+}  // namespace
+
+ps_err pairs_alloc(Ctx* c) {
+    const int Np = c->Np;
+    // Cell grid: edge >= the largest Verlet cutoff (the BB one, 2.5 colloid radii).
+    const double rmax = std::max({(double)(float)(0.05 + 2 * c->p.beta), (double)(float)(1.5 + c->p.beta), 2.5});
+    c->cell_mode = 0;
+    int ncells = 1;
+    if (c->p.box_active) {
+        int nc[3];
+        bool ok = true;
+        for (int d = 0; d < 3; d++) {
+            nc[d] = (int)std::floor(c->p.box[d] / (rmax * (1.0 + 1e-9)));
+            if (nc[d] < 3) ok = false;
+        }
+        if (ok) {
+            c->cell_mode = 1;
+            for (int d = 0; d < 3; d++) {
+                c->ncell[d] = nc[d];
+                c->cell_size[d] = c->p.box[d] / nc[d];
+                c->cell_org[d] = -0.5 * c->p.box[d];
+            }
+            ncells = nc[0] * nc[1] * nc[2];
+        }
+    }
+    if (c->cell_mode == 0) {
+        for (int d = 0; d < 3; d++) {
+            c->ncell[d] = 1;
+            c->cell_size[d] = 1.0;
+            c->cell_org[d] = 0.0;
+        }
+    }
+    PS_HIP(c, hipMalloc(&c->cell_of, sizeof(int) * std::max(Np, 1)));
+    PS_HIP(c, hipMalloc(&c->cell_start, sizeof(int) * (ncells + 1)));
+    PS_HIP(c, hipMalloc(&c->cell_count, sizeof(int) * ncells));
+    PS_HIP(c, hipMalloc(&c->cell_perm, sizeof(int) * std::max(Np, 1)));
+    PS_HIP(c, hipMalloc(&c->nb_count, sizeof(int) * std::max(Np, 1)));
+    PS_HIP(c, hipMalloc(&c->nb_off, sizeof(int) * (Np + 1)));
+    c->nb_cap = std::max<int64_t>(1 << 16, 64 * (int64_t)Np);
+    PS_HIP(c, hipMalloc(&c->nb_part, sizeof(int) * (size_t)c->nb_cap));
+    PS_HIP(c, hipMalloc(&c->dflag, sizeof(int) * 4));
+    PS_HIP(c, hipHostMalloc(&c->hflag, sizeof(int) * 4, hipHostMallocDefault));
+    // per-particle bond lists (CSR)
+    std::vector<std::vector<std::pair<int, int>>> per(Np);
+    std::vector<int> b0, b1;
+    const ps_params& p = c->p;
+    for (int ii = 0; ii < p.Npoly; ii++)
+        for (int jj = 0; jj < p.Nmono_per_chain - 1; jj++) {
+            const int k = ii * p.Nmono_per_chain + jj;
+            if (k + 1 < Np) {
+                b0.push_back(k);
+                b1.push_back(k + 1);
+            }
+        }
+    if (p.tether && c->Nc > 0) {
+        const int cpc = p.Npoly / c->Nc;
+        for (int ii = 0; ii < p.Npoly; ii++) {
+            int host = c->Nm + (cpc > 0 ? ii / cpc : 0);
+            if (host >= Np) host = Np - 1;
+            b0.push_back(ii * p.Nmono_per_chain);
+            b1.push_back(host);
+        }
+    }
+    c->nbonds = (int)b0.size();
+    for (size_t b = 0; b < b0.size(); b++) {
+        per[b0[b]].push_back({b0[b], b1[b]});
+        per[b1[b]].push_back({b0[b], b1[b]});
+    }
+    std::vector<int> boff(Np + 1, 0), bk, bj;
+    for (int q = 0; q < Np; q++) {
+        boff[q + 1] = boff[q] + (int)per[q].size();
+        for (auto& pr : per[q]) {
+            bk.push_back(pr.first);
+            bj.push_back(pr.second);
+        }
+    }
+    const size_t nb = std::max<size_t>(bk.size(), 1);
+    PS_HIP(c, hipMalloc(&c->bonds, sizeof(int) * (Np + 1 + 2 * nb)));
+    PS_HIP(c, hipMemcpy(c->bonds, boff.data(), sizeof(int) * (Np + 1), hipMemcpyHostToDevice));
+    if (!bk.empty()) {
+        PS_HIP(c, hipMemcpy(c->bonds + Np + 1, bk.data(), sizeof(int) * bk.size(), hipMemcpyHostToDevice));
+        PS_HIP(c, hipMemcpy(c->bonds + Np + 1 + nb, bj.data(), sizeof(int) * bj.size(), hipMemcpyHostToDevice));
+    }
+    c->bond_stride = nb;
+    return PS_OK;
+}
+
+void pairs_free(Ctx* c) {
+    for (void* p : {(void*)c->cell_of, (void*)c->cell_start, (void*)c->cell_count, (void*)c->cell_perm,
+                    (void*)c->nb_count, (void*)c->nb_off, (void*)c->nb_part, (void*)c->dflag, (void*)c->bonds})
+        if (p) (void)hipFree(p);
+    if (c->hflag) (void)hipHostFree(c->hflag);
+    c->cell_of = c->cell_start = c->cell_count = c->cell_perm = c->nb_count = c->nb_off = c->nb_part = nullptr;
+    c->dflag = c->bonds = nullptr;
+    c->hflag = nullptr;
+}
+
+ps_err check_dist_dev(Ctx* c) {
+    const int Np = c->Np;
+    hipStream_t st = c->stream;
+    PairConst pc = make_pc(c);
+    // geometry of this check_dist = current positions
+    PS_HIP(c, hipMemcpyAsync(c->xpd, c->x, sizeof(double) * 3 * (size_t)Np, hipMemcpyDeviceToDevice, st));
+    const unsigned gp = (unsigned)((Np + TB - 1) / TB);
+    if (c->cell_mode == 1) {
+        const int ncells = c->ncell[0] * c->ncell[1] * c->ncell[2];
+        PS_HIP(c, hipMemsetAsync(c->cell_count, 0, sizeof(int) * ncells, st));
+        k_bin<<<gp, TB, 0, st>>>(pc, c->xpd, c->cell_of, c->cell_count);
+        PS_LAUNCH(c);
+        k_scan<<<1, 1024, 0, st>>>(c->cell_count, c->cell_start, ncells);
+        PS_LAUNCH(c);
+        PS_HIP(c, hipMemsetAsync(c->cell_count, 0, sizeof(int) * ncells, st));
+        k_scatter<<<gp, TB, 0, st>>>(Np, c->cell_of, c->cell_start, c->cell_count, c->cell_perm);
+        PS_LAUNCH(c);
+    }
+    k_nb_count<<<gp, TB, 0, st>>>(pc, c->xpd, c->cell_of, c->cell_start, c->cell_perm, c->nb_count);
+    PS_LAUNCH(c);
+    k_scan<<<1, 1024, 0, st>>>(c->nb_count, c->nb_off, Np);
+    PS_LAUNCH(c);
+    PS_HIP(c, hipMemcpyAsync(c->hflag, c->nb_off + Np, sizeof(int), hipMemcpyDeviceToHost, st));
+    PS_HIP(c, hipStreamSynchronize(st));
+    const int64_t total = c->hflag[0];
+    if (total > c->nb_cap) {
+        PS_HIP(c, hipFree(c->nb_part));
+        c->nb_cap = total + total / 4 + 1024;
+        PS_HIP(c, hipMalloc(&c->nb_part, sizeof(int) * (size_t)c->nb_cap));
+    }
+    k_nb_fill<<<gp, TB, 0, st>>>(pc, c->xpd, c->cell_of, c->cell_start, c->cell_perm, c->nb_off, c->nb_part);
+    PS_LAUNCH(c);
+    c->nvlist = total / 2;
+    c->cnt.nvlist = c->nvlist;
+    c->have_pd = true;
+    return PS_OK;
+}
+
+ps_err rhs_dev(Ctx* c, int drift) {
+    hipStream_t st = c->stream;
+    const SysConst& s = c->sc;
+    const unsigned ge = (unsigned)((s.ns + TB - 1) / TB);
+    if (drift) {
+        k_rhs_drift<<<ge, TB, 0, st>>>(s, c->fext, c->rhs);
+        PS_LAUNCH(c);
+        return PS_OK;
+    }
+    k_rhs_clear<<<ge, TB, 0, st>>>(s, c->rhs);
+    PS_LAUNCH(c);
+    ForceConst fc{};
+    fc.s = s;
+    const float sg[3] = {(float)(2. * c->p.beta), (float)(1. + c->p.beta), (float)2.0};
+    for (int k = 0; k < 3; k++) fc.sig[k] = sg[k];
+    fc.mono_ev = c->p.mono_ev;
+    fc.force_sel = c->p.force_sel;
+    fc.fene = c->p.fene;
+    fc.k_hs = (c->p.hs_kmode == 1) ? c->p.hs_kpref / (c->p.dt * c->p.beta) : c->p.hs_kpref / c->p.dt;
+    fc.hs_exp_k = c->p.hs_exp_k;
+    fc.kbond = c->p.kbond;
+    fc.r0 = c->p.r0;
+    fc.Lmax = c->p.Lmax;
+    // trapping_forces: |ktrap| < 1e-12 disables; weaken_trap replaces ktrap permanently after tstop
+    fc.trap_on = !(std::fabs(c->ktrap_cur) < 1e-12);
+    if (fc.trap_on && c->t > c->p.tstart + c->p.trun && c->p.weaken_trap >= 0) c->ktrap_cur = c->p.weaken_trap;
+    fc.ktrap = c->ktrap_cur;
+    const int Np = c->Np;
+    const size_t nb = c->bond_stride;
+    const int* boff = c->bonds;
+    const int* bk = c->bonds + Np + 1;
+    const int* bj = c->bonds + Np + 1 + nb;
+    k_forces<<<(unsigned)((Np + TB - 1) / TB), TB, 0, st>>>(fc, c->xpd, c->x, c->xpos0, c->nb_off, c->nb_part,
+                                                           boff, bk, bj, c->rhs);
+    PS_LAUNCH(c);
+    return PS_OK;
+}
+
+// Copy back the reference-ordered Verlet list (q > k half of each row).
+ps_err vlist_copy(Ctx* c, int32_t* rows, int32_t* partners, int64_t* pidx, int64_t cap, int64_t* n_pairs) {
+    const int Np = c->Np;
+    std::vector<int> off(Np + 1);
+    PS_HIP(c, hipMemcpyAsync(off.data(), c->nb_off, sizeof(int) * (Np + 1), hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<int> part(std::max(off[Np], 1));
+    if (off[Np] > 0)
+        PS_HIP(c, hipMemcpy(part.data(), c->nb_part, sizeof(int) * off[Np], hipMemcpyDeviceToHost));
+    const int64_t Nm = c->Nm, Nc = c->Nc;
+    const int64_t npAA = Nm * (Nm - 1) / 2, npAB = Nm * Nc;
+    int64_t cnt = 0;
+    for (int k = 0; k < Np; k++)
+        for (int t = off[k]; t < off[k + 1]; t++) {
+            const int j = part[t];
+            if (j <= k) continue;
+            if (cnt < cap) {
+                if (rows) rows[cnt] = k;
+                if (partners) partners[cnt] = j;
+                if (pidx) {
+                    int64_t v;
+                    if (j < Nm) v = (int64_t)k * (2 * Nm - k - 1) / 2 + (j - k - 1);
+                    else if (k < Nm) v = npAA + (int64_t)k * Nc + (j - Nm);
+                    else {
+                        const int64_t kc = k - Nm, jc = j - Nm;
+                        v = npAA + npAB + kc * (2 * Nc - kc - 1) / 2 + (jc - kc - 1);
+                    }
+                    pidx[cnt] = v;
+                }
+            }
+            cnt++;
+        }
+    if (n_pairs) *n_pairs = cnt;
+    return PS_OK;
+}

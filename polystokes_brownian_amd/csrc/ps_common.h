@@ -1,0 +1,184 @@
+// ps_common.h -- context, error plumbing and device helpers shared by the gfx950 translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+#include "../../include/polystokes_gpu.h"
+
+#define PS_BLOCK 256
+
+// Return-on-error plumbing: every HIP call is checked; the message lands in ctx->err.
+#define PS_HIP(ctx, call)                                                                         \
+    do {                                                                                          \
+        hipError_t _e = (call);                                                                   \
+        if (_e != hipSuccess) {                                                                   \
+            (ctx)->set_error(std::string(#call) + ": " + hipGetErrorString(_e));                  \
+            return PS_ERR_HIP;                                                                    \
+        }                                                                                         \
+    } while (0)
+#define PS_TRY(expr)                                                                              \
+    do {                                                                                          \
+        ps_err _r = (expr);                                                                       \
+        if (_r != PS_OK) return _r;                                                               \
+    } while (0)
+#define PS_LAUNCH(ctx)                                                                            \
+    do {                                                                                          \
+        hipError_t _e = hipGetLastError();                                                        \
+        if (_e != hipSuccess) {                                                                   \
+            (ctx)->set_error(std::string("kernel launch: ") + hipGetErrorString(_e));             \
+            return PS_ERR_HIP;                                                                    \
+        }                                                                                         \
+    } while (0)
+
+// Constants of one system, passed by value to the kernels.
+struct SysConst {
+    int Np, Nm, Nc;
+    int chain_HI;
+    int box_active;
+    double L[3], Linv[3];
+    double beta, beta2, beta_inv, two_beta;
+    int64_t n, nu, ns, nm3, nc3;
+};
+
+// Device-resident MINRES scalars (one per solve).
+struct MinresState {
+    double beta1, tol, oldb, beta, dbar, epsln, phibar, cs, sn;
+    double alfa, oldeps, delta, gamma, phi;
+    int its, done, reason, pad;
+};
+
+// Apply workspace (partial sums) sized at create time.
+struct ApplyWork {
+    int nsplit_aa = 1, nchunk_c = 1;
+    int per_split_aa = 0, per_chunk_c = 0;
+    void* part_aa = nullptr;     // [nsplit][3][Nm]  (T)
+    void* part_abm = nullptr;    // [3][Nm]          (T)
+    void* part_c = nullptr;      // [nchunk][Nc][12] (T) (11 rows + monomer-force sum slot reuse)
+    void* part_fs = nullptr;     // [nchunk][3]      (T) sum of monomer forces per chunk
+    double* bb = nullptr;        // [Nc][11]
+    size_t bytes_aa = 0, bytes_abm = 0, bytes_c = 0, bytes_fs = 0;
+};
+
+struct Ctx {
+    ps_params p{};
+    SysConst sc{};
+    int Np = 0, Nm = 0, Nc = 0;
+    int64_t n = 0, nu = 0, ns = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int n_cu = 0;
+    std::string err;
+    void set_error(const std::string& m) { err = m; }
+
+    // ---- positions and per-step vectors (device)
+    double *x = nullptr, *xpd = nullptr, *xm = nullptr, *xpos0 = nullptr, *q = nullptr, *x_n = nullptr;
+    float* xm32 = nullptr;          // FP32 snapshot of xm for the mixed form
+    int* chain = nullptr;
+    double *up = nullptr, *udiff = nullptr, *v_det_n = nullptr, *v_det_c = nullptr, *v_brown = nullptr;
+    double *fext = nullptr, *drift_v = nullptr, *sb = nullptr;
+    double *rhs = nullptr, *X = nullptr, *Xdet = nullptr, *Xd = nullptr, *W = nullptr, *xi = nullptr;
+    double* stale = nullptr;        // [18]: stale tile (9) + tile carried to the next mob() (9)
+    double* trap = nullptr;         // [2]: current ktrap, time t (device copies for the force kernel)
+    int* bonds = nullptr;           // CSR: offsets [Np+1], then endpoints k [bond_stride], j [bond_stride]
+    int nbonds = 0;
+    size_t bond_stride = 1;
+    // ---- host mirrors
+    int stale_i = -1, stale_j = -1; // last processed AA pair (fixed by chain ids)
+    double ktrap_cur = 0.0, t = 0.0;
+    bool have_pd = false, have_mob = false, have_x = false;
+    // ---- apply
+    ApplyWork aw64, aw32;
+    // ---- Krylov work
+    double *kr = nullptr;           // 9 * ns scratch vectors for MINRES
+    double *dinv = nullptr;         // ns
+    double *red = nullptr;          // reduction partials
+    int red_cap = 0;
+    MinresState* mstate = nullptr;  // device
+    MinresState* mstate_h = nullptr;// pinned host
+    double* Q = nullptr;            // Lanczos basis kmax x n
+    double* lz = nullptr;           // Lanczos scalars on device (alpha, beta, h) [3 * kmax + 8]
+    double* lz_h = nullptr;         // pinned host mirror
+    double* gw = nullptr;           // recombination weights (device) kmax
+    double *ir_r = nullptr, *ir_d = nullptr;   // refinement residual / correction (ns)
+    int kmax = 0;
+    // ---- Verlet list (check_dist)
+    int ncell[3] = {0, 0, 0};
+    int cell_mode = 0;              // 0: brute force; 1: periodic cell list
+    double cell_size[3] = {0, 0, 0}, cell_org[3] = {0, 0, 0};
+    int* cell_of = nullptr;         // [Np]
+    int* cell_start = nullptr;      // [ncells+1]
+    int* cell_count = nullptr;      // [ncells]
+    int* cell_perm = nullptr;       // [Np]
+    int* nb_count = nullptr;        // [Np] pairs per row
+    int* nb_off = nullptr;          // [Np+1]
+    int* nb_part = nullptr;         // partners (sorted per row)
+    int64_t nb_cap = 0;
+    int64_t nvlist = 0;
+    int* dflag = nullptr;           // overflow flag etc. [4]
+    int* hflag = nullptr;           // pinned
+    // ---- host RNG (std::mt19937_64 + normal_distribution, src/init.cpp:105-110)
+    std::mt19937_64 rng;
+    std::normal_distribution<double> nd{0.0, 1.0};
+    double* noise_h = nullptr;      // pinned, n + nu
+    // ---- event timing (ps_profile_begin/end)
+    bool prof_on = false;
+    std::vector<hipEvent_t> ev_pool;
+    int ev_used = 0;
+    std::vector<int> ev_kind;       // 0: AA kernel, 1: whole apply (pairs of events)
+    ps_profile prof{};
+    ps_err prof_record(int kind, hipEvent_t* a, hipEvent_t* b);   // reserve a start/stop pair
+    ps_err prof_harvest();
+    // ---- counters
+    ps_counters cnt{};
+    int last_minres_its = 0;
+    // ---- scratch used by the host-pointer entry points
+    double* io_a = nullptr;
+    double* io_b = nullptr;
+};
+
+// ----------------------------------------------------------------------------- device helpers
+__device__ __forceinline__ double ps_rsqrt(double x) {
+    // Hardware reciprocal square root refined by two Newton-Raphson steps (full FP64 accuracy).
+    double y = __builtin_amdgcn_rsq(x);
+    double h = x * y;
+    double r = __builtin_fma(-h, y, 1.0);
+    y = __builtin_fma(0.5 * y, r, y);
+    h = x * y;
+    r = __builtin_fma(-h, y, 1.0);
+    y = __builtin_fma(0.5 * y, r, y);
+    return y;
+}
+__device__ __forceinline__ float ps_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+
+template <class T>
+__device__ __forceinline__ T ps_rint(T v);
+template <>
+__device__ __forceinline__ double ps_rint<double>(double v) { return __builtin_rint(v); }
+template <>
+__device__ __forceinline__ float ps_rint<float>(float v) { return __builtin_rintf(v); }
+
+// Host entry points implemented in the translation units.
+ps_err mob_prepare(Ctx* c);                                        // stale tile + snapshots
+ps_err mob_apply(Ctx* c, const double* f, double* u, int prec);    // u = M f (device pointers)
+ps_err saddle_apply(Ctx* c, const double* x, double* y, int prec); // y = A x (device pointers)
+ps_err apply_alloc(Ctx* c);
+void apply_free(Ctx* c);
+ps_err krylov_alloc(Ctx* c);
+void krylov_free(Ctx* c);
+ps_err saddle_diag_dev(Ctx* c, double* d);
+ps_err saddle_solve_dev(Ctx* c, const double* b, double* x, int warm, ps_solve_info* info);
+ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* w, ps_sqrt_info* info);
+ps_err pairs_alloc(Ctx* c);
+void pairs_free(Ctx* c);
+ps_err check_dist_dev(Ctx* c);
+ps_err rhs_dev(Ctx* c, int drift);
+ps_err vlist_copy(Ctx* c, int32_t* rows, int32_t* partners, int64_t* pidx, int64_t cap, int64_t* n_pairs);
+ps_err run_steps(Ctx* c, int nsteps);
+ps_err vec_axpby(Ctx* c, double* y, double a, const double* x, double b, int64_t n);  // y = a x + b y
+ps_err vec_copy(Ctx* c, double* dst, const double* src, int64_t n);
+ps_err vec_dot(Ctx* c, const double* a, const double* b, int64_t n, double* out);      // sync
+int tridiag_sqrt_weights(int m, const double* alpha, const double* betad, double beta0, double* g);

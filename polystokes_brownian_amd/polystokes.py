@@ -1,0 +1,318 @@
+"""Python front end of libpsgpu.so: the reference's pybind11 `PolyStokes` / `Box` API
+(src/bindings.cpp:359-437) over the C ABI, plus `Context`, a thin wrapper of every entry point.
+
+The HIP library is the only compute path: if libpsgpu.so is missing or no GPU is visible, every
+call raises -- there is no CPU fallback.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+from ._abi import (ERR_NAMES, PS_ERR_NOCONV, PS_OK, PS_PREC_FP64, PS_PREC_MIXED, PsCounters, PsParams,
+                   PsProfile, PsSolveInfo, PsSqrtInfo, params_from_dict)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpsgpu.so")
+_lib = None
+
+
+class PsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libpsgpu.so (built by __graft_entry__.build() / `make -C polystokes_brownian_amd/csrc`)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `make -C polystokes_brownian_amd/csrc`")
+        L = C.CDLL(LIB_PATH)
+        dp, vp, ip = C.POINTER(C.c_double), C.c_void_p, C.c_int
+        L.ps_params_default.argtypes = [C.POINTER(PsParams)]
+        L.ps_params_default.restype = None
+        L.ps_create.argtypes = [C.POINTER(PsParams), C.POINTER(vp)]
+        L.ps_destroy.argtypes = [vp]
+        L.ps_last_error.argtypes = [vp]
+        L.ps_last_error.restype = C.c_char_p
+        L.ps_device_info.argtypes = [vp, C.c_char_p, ip, C.POINTER(ip)]
+        L.ps_initial_configuration.argtypes = [vp, dp, dp]
+        L.ps_set_positions.argtypes = [vp, dp]
+        L.ps_get_state.argtypes = [vp, dp, dp, dp]
+        L.ps_get_vectors.argtypes = [vp] + [dp] * 7
+        L.ps_get_counters.argtypes = [vp, C.POINTER(PsCounters)]
+        L.ps_check_dist.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.ps_get_vlist.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                   C.c_int64, C.POINTER(C.c_int64)]
+        L.ps_mob.argtypes = [vp]
+        L.ps_stale_tile.argtypes = [vp, dp]
+        for name in ("ps_mob_mult", "ps_mob_mult_dev", "ps_saddle_mult", "ps_saddle_mult_dev"):
+            getattr(L, name).argtypes = [vp, C.c_void_p, C.c_void_p]
+        L.ps_saddle_diag.argtypes = [vp, dp]
+        L.ps_saddle_solve.argtypes = [vp, C.c_void_p, C.c_void_p, ip, C.POINTER(PsSolveInfo)]
+        L.ps_saddle_solve_dev.argtypes = [vp, C.c_void_p, C.c_void_p, ip, C.POINTER(PsSolveInfo)]
+        L.ps_sqrt_apply.argtypes = [vp, C.c_void_p, C.c_void_p, C.POINTER(PsSqrtInfo)]
+        L.ps_sqrt_apply_dev.argtypes = [vp, C.c_void_p, C.c_void_p, C.POINTER(PsSqrtInfo)]
+        L.ps_rhs.argtypes = [vp, ip, dp]
+        L.ps_run.argtypes = [vp, ip]
+        L.ps_draw_normals.argtypes = [vp, dp, C.c_int64]
+        L.ps_profile_begin.argtypes = [vp]
+        L.ps_profile_end.argtypes = [vp, C.POINTER(PsProfile)]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def default_params(**kw):
+    p = PsParams()
+    lib().ps_params_default(C.byref(p))
+    return params_from_dict(kw, p)
+
+
+class Context:
+    """One device-resident simulation: a 1:1 wrapper of the C ABI (host numpy arrays in and out)."""
+
+    def __init__(self, params):
+        if isinstance(params, dict):
+            params = default_params(**params)
+        self.params = params
+        h = C.c_void_p()
+        rc = lib().ps_create(C.byref(params), C.byref(h))
+        if rc != PS_OK:
+            raise PsError(rc, lib().ps_last_error(None).decode())
+        self.h = h
+        self.Np, self.Nm, self.Nc = params.Np, params.Nm, params.Nc
+        self.n = 3 * self.Nm + 11 * self.Nc
+        self.nu = 3 * self.Nm + 6 * self.Nc
+        self.ns = self.n + self.nu
+
+    def _check(self, rc, allow_noconv=False):
+        if rc == PS_OK or (allow_noconv and rc == PS_ERR_NOCONV):
+            return rc
+        raise PsError(rc, lib().ps_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ps_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def device_info(self):
+        name = C.create_string_buffer(256)
+        ncu = C.c_int()
+        self._check(lib().ps_device_info(self.h, name, 256, C.byref(ncu)))
+        return name.value.decode(), ncu.value
+
+    def initial_configuration(self, x0, q0=None):
+        x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1)
+        if x0.size != 3 * self.Np:
+            raise ValueError("init_x0 must hold 3*Np positions")
+        q = None if q0 is None else _p(np.ascontiguousarray(q0, dtype=np.float64).reshape(-1))
+        self._check(lib().ps_initial_configuration(self.h, _p(x0), q))
+
+    def set_positions(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        self._check(lib().ps_set_positions(self.h, _p(x)))
+
+    def state(self):
+        x = np.zeros(3 * self.Np)
+        q = np.zeros(max(4 * self.Nc, 1))
+        t = np.zeros(1)
+        self._check(lib().ps_get_state(self.h, _p(x), _p(q), _p(t)))
+        return x, q[: 4 * self.Nc], float(t[0])
+
+    def vectors(self):
+        out = {k: np.zeros(self.nu) for k in ("up", "v_det_n", "v_det_c", "v_brown", "udiff", "fext")}
+        out["sb"] = np.zeros(max(5 * self.Nc, 1))
+        self._check(lib().ps_get_vectors(self.h, _p(out["up"]), _p(out["v_det_n"]), _p(out["v_det_c"]),
+                                         _p(out["v_brown"]), _p(out["udiff"]), _p(out["sb"]), _p(out["fext"])))
+        out["sb"] = out["sb"][: 5 * self.Nc]
+        return out
+
+    def counters(self):
+        c = PsCounters()
+        self._check(lib().ps_get_counters(self.h, C.byref(c)))
+        return {k: getattr(c, k) for k, _ in PsCounters._fields_}
+
+    def check_dist(self):
+        n = C.c_int64()
+        self._check(lib().ps_check_dist(self.h, C.byref(n)))
+        return n.value
+
+    def vlist(self):
+        n = C.c_int64()
+        self._check(lib().ps_get_vlist(self.h, None, None, None, 0, C.byref(n)))
+        cnt = n.value
+        rows = np.zeros(max(cnt, 1), np.int32)
+        parts = np.zeros(max(cnt, 1), np.int32)
+        pidx = np.zeros(max(cnt, 1), np.int64)
+        self._check(lib().ps_get_vlist(self.h, rows.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       parts.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       pidx.ctypes.data_as(C.POINTER(C.c_int64)), cnt, C.byref(n)))
+        return rows[:cnt], parts[:cnt], pidx[:cnt]
+
+    def mob(self):
+        self._check(lib().ps_mob(self.h))
+
+    def stale_tile(self):
+        t = np.zeros(9)
+        self._check(lib().ps_stale_tile(self.h, _p(t)))
+        return t.reshape(3, 3)
+
+    def mob_mult(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        u = np.zeros(self.n)
+        self._check(lib().ps_mob_mult(self.h, f.ctypes.data, u.ctypes.data))
+        return u
+
+    def saddle_mult(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.ns)
+        self._check(lib().ps_saddle_mult(self.h, x.ctypes.data, y.ctypes.data))
+        return y
+
+    def saddle_diag(self):
+        d = np.zeros(self.ns)
+        self._check(lib().ps_saddle_diag(self.h, _p(d)))
+        return d
+
+    def saddle_solve(self, b, x0=None):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.ns) if x0 is None else np.array(x0, dtype=np.float64)
+        info = PsSolveInfo()
+        rc = self._check(lib().ps_saddle_solve(self.h, b.ctypes.data, x.ctypes.data, int(x0 is not None),
+                                               C.byref(info)), allow_noconv=True)
+        return x, info, rc
+
+    def sqrt_apply(self, xi):
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        w = np.zeros(self.n)
+        info = PsSqrtInfo()
+        self._check(lib().ps_sqrt_apply(self.h, xi.ctypes.data, w.ctypes.data, C.byref(info)))
+        return w, info
+
+    def rhs(self, drift=False):
+        r = np.zeros(self.ns)
+        self._check(lib().ps_rhs(self.h, int(drift), _p(r)))
+        return r
+
+    def run(self, nsteps):
+        self._check(lib().ps_run(self.h, int(nsteps)))
+
+    def draw_normals(self, n):
+        out = np.zeros(n)
+        self._check(lib().ps_draw_normals(self.h, _p(out), n))
+        return out
+
+    def profile_begin(self):
+        self._check(lib().ps_profile_begin(self.h))
+
+    def profile_end(self):
+        p = PsProfile()
+        self._check(lib().ps_profile_end(self.h, C.byref(p)))
+        return {k: getattr(p, k) for k, _ in PsProfile._fields_}
+
+
+class Box:
+    """Periodic box helper with the reference's semantics (src/box.cpp, src/bindings.cpp:418-436)."""
+
+    def __init__(self):
+        self._L = None
+
+    def configure(self, lengths):
+        lengths = list(lengths)
+        if not lengths:
+            self._L = None
+            return
+        if len(lengths) != 3:
+            raise ValueError("Box::configure expects an empty list (no PBC) or exactly 3 lengths [Lx, Ly, Lz]")
+        if not all(v > 0.0 for v in lengths):
+            raise ValueError("Box lengths must be positive")
+        self._L = [float(v) for v in lengths]
+
+    def active(self):
+        return self._L is not None
+
+    def minimum_image(self, dx, dy, dz):
+        if self._L is None:
+            return dx, dy, dz
+        out = []
+        for v, L in zip((dx, dy, dz), self._L):
+            out.append(v - L * np.rint(v * (1.0 / L)))
+        return tuple(float(v) for v in out)
+
+    def wrap(self, x, y, z):
+        return self.minimum_image(x, y, z)
+
+
+class PolyStokes:
+    """The reference's Python class (src/bindings.cpp:363-414), driving the MI355X library.
+
+    Same constructor keywords and defaults, same particle_info / trap_info / initial_configuration /
+    run sequence.  Two keyword-only additions select the MI355X forms and solver tolerances through
+    set_solver_options(); by default the reference's settings apply (MINRES rtol 1e-6, FP64)."""
+
+    def __init__(self, dt, samplerate, tmax, output_dir, mm_HI=True, chain_HI=False, fene=True,
+                 record_forces=True, tether=False, mono_ev=False, box=None, t=0.0, seed=12345, restart=0.0):
+        self._p = default_params()
+        b = Box()
+        b.configure([] if box is None else box)
+        self._pd = dict(dt=dt, samplerate=samplerate, tmax=tmax, mm_HI=int(mm_HI), chain_HI=int(chain_HI),
+                        fene=int(fene), record_forces=int(record_forces), tether=int(tether),
+                        mono_ev=int(mono_ev), box=None if box is None else list(box), t=t, seed=seed,
+                        restart=restart)
+        self.output_dir = output_dir
+        self._x0 = None
+        self._q0 = None
+        self._ctx = None
+        self.warn_neg_eig = True
+        # nsteps = floor((tmax - restart)/dt) (src/Stokes.cpp:25)
+        self.nsteps = max(0, int(math.floor((tmax - restart) / dt)))
+
+    def particle_info(self, kT, epsilon, Np, Nc, Nm, Npoly, Nmono_per_chain, beta, kbond, r0, Lmax, tau):
+        self._pd.update(kT=kT, epsilon=epsilon, Np=Np, Nc=Nc, Nm=Nm, Npoly=Npoly,
+                        Nmono_per_chain=Nmono_per_chain, beta=beta, kbond=kbond, r0=r0, Lmax=Lmax, tau=tau)
+
+    def trap_info(self, ktrap, tstart, trun, weaken_trap=-1):
+        self._pd.update(ktrap=ktrap, tstart=tstart, trun=trun, weaken_trap=weaken_trap)
+
+    def set_solver_options(self, precision=None, ksp_rtol=None, sqrt_rtol=None, sqrt_kmax=None, ir_rtol=None):
+        for k, v in dict(ksp_rtol=ksp_rtol, sqrt_rtol=sqrt_rtol, sqrt_kmax=sqrt_kmax, ir_rtol=ir_rtol).items():
+            if v is not None:
+                self._pd[k] = v
+        if precision is not None:
+            self._pd["precision"] = {"fp64": PS_PREC_FP64, "mixed": PS_PREC_MIXED}.get(precision, precision)
+
+    def set_warn_neg_eig(self, on=True):
+        self.warn_neg_eig = bool(on)
+
+    def initial_configuration(self, init_x0, init_q=None):
+        self._x0 = np.ascontiguousarray(init_x0, dtype=np.float64).reshape(-1)
+        self._q0 = None if init_q is None else np.ascontiguousarray(init_q, dtype=np.float64).reshape(-1)
+        if self._q0 is not None and self._q0.size != 4 * self._pd.get("Nc", 0):
+            self._q0 = None   # src/Stokes.cpp:298-301: size mismatch falls back to identity
+
+    def _context(self):
+        if self._ctx is None:
+            if self._x0 is None:
+                raise RuntimeError("initial_configuration must be called before run()")
+            self._ctx = Context(params_from_dict(self._pd, default_params()))
+            self._ctx.initial_configuration(self._x0, self._q0)
+        return self._ctx
+
+    def run(self):
+        """PolyStokes::run (src/run.cpp:55-175): nsteps steps on the device; returns final positions."""
+        ctx = self._context()
+        ctx.run(self.nsteps)
+        return ctx.state()[0]
+
+    @property
+    def context(self):
+        return self._context()

@@ -1,0 +1,98 @@
+"""CPU checks of the C ABI library: it loads, exports every entry point the header declares, fills
+the reference defaults, and fails loudly (no CPU fallback) when no GPU is visible."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from polystokes_brownian_amd import _abi
+from polystokes_brownian_amd import polystokes as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "polystokes_gpu.h")
+
+
+def declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ps_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_header_symbols_exported():
+    names = declared()
+    assert "ps_saddle_solve" in names and "ps_sqrt_apply" in names and "ps_mob_mult" in names
+    out = subprocess.run(["nm", "-D", "--defined-only", P.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (ps_[a-z_0-9]+)", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    lib = P.lib()
+    for n in names:
+        assert getattr(lib, n) is not None
+
+
+def test_struct_sizes_match_header():
+    # ctypes mirror of ps_params / ps_solve_info has the C layout (checked by a tiny C program)
+    code = r'''
+    #include <stdio.h>
+    #include <stddef.h>
+    #include "polystokes_gpu.h"
+    int main(){ printf("%zu %zu %zu %zu %zu %zu\n", sizeof(ps_params), sizeof(ps_solve_info), sizeof(ps_sqrt_info),
+                       sizeof(ps_counters), sizeof(ps_profile), offsetof(ps_params, ir_maxit)); return 0; }'''
+    tmp = os.path.join(os.environ.get("TMPDIR", "/tmp"), "ps_sizes_check")
+    with open(tmp + ".c", "w") as fh:
+        fh.write(code)
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", tmp, tmp + ".c"], check=True)
+    got = list(map(int, subprocess.run([tmp], capture_output=True, text=True, check=True).stdout.split()))
+    want = [C.sizeof(_abi.PsParams), C.sizeof(_abi.PsSolveInfo), C.sizeof(_abi.PsSqrtInfo),
+            C.sizeof(_abi.PsCounters), C.sizeof(_abi.PsProfile), _abi.PsParams.ir_maxit.offset]
+    assert got == want
+
+
+def test_reference_defaults():
+    p = P.default_params()
+    # src/bindings.cpp:374-387 and src/init.cpp:64
+    assert (p.mm_HI, p.chain_HI, p.fene, p.record_forces, p.tether, p.mono_ev) == (1, 0, 1, 1, 0, 0)
+    assert p.seed == 12345 and p.restart == 0.0 and p.box_active == 0
+    assert p.ksp_rtol == 1e-6 and p.ksp_abstol == 1e-50 and p.ksp_maxit == 10000
+    assert p.weaken_trap == -1 and p.hs_kpref == 1.0 and p.hs_exp_k == 100.0
+
+
+def _gpu_visible():
+    try:
+        import torch
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False
+
+
+@pytest.mark.skipif(_gpu_visible(), reason="checks the no-GPU error path")
+def test_no_gpu_fails_loudly():
+    p = P.default_params(Np=10, Nm=8, Nc=2, Npoly=2, Nmono_per_chain=4, beta=0.2, dt=1e-3)
+    with pytest.raises(P.PsError) as ei:
+        P.Context(p)
+    assert ei.value.code == _abi.PS_ERR_NODEV
+
+
+def test_argument_errors_before_device():
+    p = P.default_params(Np=10, Nm=7, Nc=2, beta=0.2, dt=1e-3)
+    with pytest.raises(P.PsError) as ei:
+        P.Context(p)
+    assert ei.value.code == _abi.PS_ERR_ARG
+    p = P.default_params(Np=10, Nm=8, Nc=2, beta=0.2, dt=1e-3, mm_HI=0)
+    with pytest.raises(P.PsError) as ei:
+        P.Context(p)
+    assert "mm_HI" in str(ei.value)
+
+
+def test_polystokes_api_surface():
+    sim = P.PolyStokes(1e-3, 10, 0.1, "out", mm_HI=True, chain_HI=True, box=[10, 10, 10], seed=7)
+    assert sim.nsteps == 100
+    sim.particle_info(1.0, 1.0, 12, 2, 10, 2, 5, 0.2, 100.0, 0.42, 0.6, 1000)
+    sim.trap_info(500.0, 0.0, 1.0)
+    b = P.Box()
+    b.configure([10.0, 10.0, 10.0])
+    assert b.active() and b.minimum_image(6.0, -6.0, 4.0) == (-4.0, 4.0, 4.0)
+    with pytest.raises(ValueError):
+        b.configure([1.0, 2.0])
