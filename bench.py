@@ -67,6 +67,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-applies", type=int, default=3)
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--solver", default="minres", choices=["minres", "block"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the solve is not sharded across GPUs",
@@ -78,13 +79,14 @@ def main():
 
     w = configs.workload(args.workload)
     pd = configs.params_dict(w)
+    pd["solver"] = 1 if args.solver == "block" else 0
     t0 = time.perf_counter()
     x0 = configs.positions(w)
     log(f"workload {args.workload}: Np={w['Np']} (Nc={w['Nc']}, Nm={w['Nm']}), configuration {time.perf_counter() - t0:.1f}s")
     Np, Nm = w["Np"], w["Nm"]
 
     # ---- accuracy reference: one FP64 step at tight tolerances
-    ref = make_ctx(Context, pd, x0, ksp_rtol=1e-10, sqrt_rtol=pd["sqrt_rtol"])
+    ref = make_ctx(Context, pd, x0, ksp_rtol=1e-10, sqrt_rtol=pd["sqrt_rtol"], solver=0)
     name, ncu = ref.device_info()
     log(f"device {name} ({ncu} CUs)")
     t0 = time.perf_counter()
@@ -145,7 +147,8 @@ def main():
         "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32+fp64", "data": "synthetic",
         "config": {"workload": f"{args.workload}: {Np} beads ({w['Nc']} probes + {w['Npoly']} chains x 20), "
-                               f"box {w['L']:.1f}, full mm-HI, MINRES rtol {pd['ksp_rtol']}, Lanczos m<={pd['sqrt_kmax']}",
+                               f"box {w['L']:.1f}, full mm-HI, saddle solver {args.solver} (rtol {pd['ksp_rtol']}), "
+                               f"Lanczos m<={pd['sqrt_kmax']}",
                    "global_batch": Np, "seq_len": 1, "parallelism": "single-gpu"},
         "mixed_vel_rel_err": head["vel_rel_err"],
         "fp64_value": fp64["value"] if fp64 else None,

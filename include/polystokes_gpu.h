@@ -86,7 +86,16 @@ typedef struct ps_params {
     int precision;                /* PS_PREC_FP64 or PS_PREC_MIXED */
     double ir_rtol;               /* mixed form: FP64 refinement target on the saddle residual */
     int ir_maxit;                 /* mixed form: cap on refinement sweeps */
+    int solver;                   /* PS_SOLVER_MINRES (reference algorithm) or PS_SOLVER_BLOCK */
 } ps_params;
+
+/* Saddle-solve algorithms.  Both solve the same system A x = b.
+ *   PS_SOLVER_MINRES: preconditioned MINRES on the full saddle system, as KSPSolve does.
+ *   PS_SOLVER_BLOCK:  exact block elimination of the constraint B = [-I; 0]: f_u = -b_u',
+ *                     M_ss f_s = b_s - M_su f_u (MINRES on the 5Nc x 5Nc stresslet block),
+ *                     u = (M f)_u - b_u -- two mobility applies per solve instead of one per
+ *                     MINRES iteration. */
+enum { PS_SOLVER_MINRES = 0, PS_SOLVER_BLOCK = 1 };
 
 typedef struct ps_solve_info {
     int iterations;      /* MINRES iterations (summed over refinement sweeps in the mixed form) */

@@ -1081,6 +1081,7 @@ void or_params_default(ps_params* p) {
     p->weaken_trap = -1;
     p->ir_rtol = 1e-6;
     p->ir_maxit = 20;
+    p->solver = PS_SOLVER_MINRES;
 }
 
 void* or_create(const ps_params* p, char* err, int errlen) {

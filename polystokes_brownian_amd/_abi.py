@@ -3,6 +3,7 @@ import ctypes as C
 
 PS_OK, PS_ERR_ARG, PS_ERR_HIP, PS_ERR_STATE, PS_ERR_NOCONV, PS_ERR_ALLOC, PS_ERR_NODEV = range(7)
 PS_PREC_FP64, PS_PREC_MIXED = 0, 1
+PS_SOLVER_MINRES, PS_SOLVER_BLOCK = 0, 1
 PS_FORCE_HS, PS_FORCE_HS_EXP = 0, 1
 
 ERR_NAMES = {
@@ -30,6 +31,7 @@ class PsParams(C.Structure):
         ("sqrt_rtol", C.c_double), ("sqrt_kmax", C.c_int),
         ("force_sel", C.c_int), ("hs_kpref", C.c_double), ("hs_kmode", C.c_int), ("hs_exp_k", C.c_double),
         ("precision", C.c_int), ("ir_rtol", C.c_double), ("ir_maxit", C.c_int),
+        ("solver", C.c_int),
     ]
 
 

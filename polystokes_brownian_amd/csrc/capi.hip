@@ -345,6 +345,7 @@ void ps_params_default(ps_params* p) {
     p->precision = PS_PREC_FP64;
     p->ir_rtol = 1e-6;
     p->ir_maxit = 20;
+    p->solver = PS_SOLVER_MINRES;
 }
 
 const char* ps_last_error(const void* ctx) {

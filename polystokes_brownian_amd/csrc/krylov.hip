@@ -391,13 +391,23 @@ ps_err krylov_alloc(Ctx* c) {
     PS_TRY(saddle_diag_dev(c, c->dinv));
     k_recip<<<eblocks(ns), VB, 0, c->stream>>>(c->dinv, ns);
     PS_LAUNCH(c);
+    if (c->p.solver == PS_SOLVER_BLOCK) {
+        const size_t n5 = 5 * (size_t)std::max(c->Nc, 1);
+        PS_HIP(c, hipMalloc(&c->Mss, sizeof(double) * n5 * n5));
+        PS_HIP(c, hipMalloc(&c->mss_dinv, sizeof(double) * n5));
+        PS_HIP(c, hipMalloc(&c->bs_y, sizeof(double) * (size_t)n));
+        PS_HIP(c, hipMalloc(&c->bs_r, sizeof(double) * n5));
+        PS_HIP(c, hipMalloc(&c->bs_x, sizeof(double) * (size_t)ns));
+    }
     return PS_OK;
 }
 
 void krylov_free(Ctx* c) {
     for (void* p : {(void*)c->kr, (void*)c->dinv, (void*)c->red, (void*)c->mstate, (void*)c->Q, (void*)c->lz,
-                    (void*)c->gw, (void*)c->ir_r, (void*)c->ir_d})
+                    (void*)c->gw, (void*)c->ir_r, (void*)c->ir_d, (void*)c->Mss, (void*)c->mss_dinv,
+                    (void*)c->bs_y, (void*)c->bs_r, (void*)c->bs_x})
         if (p) (void)hipFree(p);
+    c->Mss = c->mss_dinv = c->bs_y = c->bs_r = c->bs_x = nullptr;
     if (c->mstate_h) (void)hipHostFree(c->mstate_h);
     if (c->lz_h) (void)hipHostFree(c->lz_h);
     c->kr = c->dinv = c->red = c->Q = c->lz = c->gw = c->ir_r = c->ir_d = nullptr;
@@ -435,10 +445,10 @@ ps_err vec_dot(Ctx* c, const double* a, const double* b, int64_t n, double* out)
     return PS_OK;
 }
 
-// Preconditioned MINRES on the saddle operator with the apply precision `prec`.
-static ps_err minres_dev(Ctx* c, const double* b, double* x, int warm, double rtol, int maxit, int prec,
-                         ps_solve_info* info) {
-    const int64_t N = c->ns;
+// Preconditioned MINRES for a symmetric operator `op` of size N with Jacobi inverse diagonal dinv.
+template <class Op>
+static ps_err minres_core(Ctx* c, int64_t N, const double* dinv, Op&& op, const double* b, double* x, int warm,
+                          double rtol, int maxit, ps_solve_info* info) {
     const int nb = nblocks(N);
     double* R1 = c->kr;
     double* R2 = c->kr + N;
@@ -452,8 +462,8 @@ static ps_err minres_dev(Ctx* c, const double* b, double* x, int warm, double rt
     hipStream_t st = c->stream;
     MinresState* sA = c->mstate;
     MinresState* sB = c->mstate + 1;
-    if (warm) PS_TRY(saddle_apply(c, x, Y, prec));
-    k_mr_init<<<nb, VB, 0, st>>>(N, b, x, R1, R2, Y, c->dinv, warm, c->red);
+    if (warm) PS_TRY(op(x, Y));
+    k_mr_init<<<nb, VB, 0, st>>>(N, b, x, R1, R2, Y, dinv, warm, c->red);
     PS_LAUNCH(c);
     k_mr_start<<<1, VB, 0, st>>>(c->red, nb, rtol, c->p.ksp_abstol, sA);
     PS_LAUNCH(c);
@@ -479,11 +489,11 @@ static ps_err minres_dev(Ctx* c, const double* b, double* x, int warm, double rt
     }
     while (!done) {
         // y = A v
-        PS_TRY(saddle_apply(c, V, y, prec));
+        PS_TRY(op(V, y));
         k_mr_a<<<nb, VB, 0, st>>>(N, sA, y, r1, V, red_a);
         PS_LAUNCH(c);
         // z into r1's buffer (old r1 is dead after k_mr_a)
-        k_mr_b<<<nb, VB, 0, st>>>(N, sA, red_a, nb, y, r2, c->dinv, r1, red_b);
+        k_mr_b<<<nb, VB, 0, st>>>(N, sA, red_a, nb, y, r2, dinv, r1, red_b);
         PS_LAUNCH(c);
         // rotation: r1 <- r2, r2 <- y, y <- z(r1 buffer)
         double* z = r1;
@@ -513,9 +523,6 @@ static ps_err minres_dev(Ctx* c, const double* b, double* x, int warm, double rt
             }
         }
     }
-    c->last_minres_its = its;
-    c->cnt.minres_iterations += its;
-    c->cnt.saddle_solves++;
     if (info) {
         info->iterations = its;
         info->reason = reason;
@@ -527,27 +534,135 @@ static ps_err minres_dev(Ctx* c, const double* b, double* x, int warm, double rt
     return reason > 0 ? PS_OK : PS_ERR_NOCONV;
 }
 
+// Preconditioned MINRES on the saddle operator with the apply precision `prec`.
+static ps_err minres_dev(Ctx* c, const double* b, double* x, int warm, double rtol, int maxit, int prec,
+                         ps_solve_info* info) {
+    ps_solve_info ii{};
+    auto op = [c, prec](const double* in, double* out) { return saddle_apply(c, in, out, prec); };
+    ps_err r = minres_core(c, c->ns, c->dinv, op, b, x, warm, rtol, maxit, &ii);
+    c->last_minres_its = ii.iterations;
+    c->cnt.minres_iterations += ii.iterations;
+    c->cnt.saddle_solves++;
+    if (info) *info = ii;
+    return r;
+}
+
+// y = M_ss x (dense 5Nc x 5Nc, one wave per row).
+__global__ __launch_bounds__(VB) void k_gemv(int64_t n, const double* __restrict__ A, const double* __restrict__ x,
+                                             double* __restrict__ y) {
+    const int64_t row = (int64_t)blockIdx.x * (VB / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const double* a = A + row * n;
+    double acc = 0.0;
+    for (int64_t j = lane; j < n; j += 64) acc += a[j] * x[j];
+    acc = wave_sum(acc);
+    if (lane == 0) y[row] = acc;
+}
+__global__ void k_mss_dinv(int64_t n, const double* __restrict__ A, double* __restrict__ d) {
+    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
+    if (i < n) d[i] = 1.0 / A[i * n + i];
+}
+// x = [-b_lambda (nu) ; 0 (5Nc)]  (f_u from the constraint rows B^T f = b_lambda)
+__global__ void k_bs_init(double* __restrict__ x, const double* __restrict__ b, int64_t n, int64_t nu) {
+    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
+    if (i >= n) return;
+    x[i] = (i < nu) ? -b[n + i] : 0.0;
+}
+// rs = b_s - y_s
+__global__ void k_bs_rs(double* __restrict__ rs, const double* __restrict__ b, const double* __restrict__ y,
+                        int64_t nu, int64_t n5) {
+    const int64_t k = (int64_t)blockIdx.x * VB + threadIdx.x;
+    if (k < n5) rs[k] = b[nu + k] - y[nu + k];
+}
+// u = (M f)_u - b_u
+__global__ void k_bs_u(double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ b,
+                       int64_t n, int64_t nu) {
+    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
+    if (i < nu) x[n + i] = y[i] - b[i];
+}
+
+// Exact block elimination of A x = b with B = [-I_nu; 0]:
+//   f_u = -b[n:n+nu];  M_ss f_s = b_s - (M [f_u; 0])_s;  u = (M f)_u - b_u.
+static ps_err block_solve_once(Ctx* c, const double* b, double* x, int prec, int* mss_its) {
+    const int64_t n = c->n, nu = c->nu, n5 = 5 * (int64_t)c->Nc;
+    hipStream_t st = c->stream;
+    k_bs_init<<<eblocks(n), VB, 0, st>>>(x, b, n, nu);
+    PS_LAUNCH(c);
+    if (n5 > 0) {
+        PS_TRY(mob_apply(c, x, c->bs_y, prec));
+        k_bs_rs<<<eblocks(n5), VB, 0, st>>>(c->bs_r, b, c->bs_y, nu, n5);
+        PS_LAUNCH(c);
+        auto op = [c, n5](const double* in, double* out) -> ps_err {
+            k_gemv<<<(unsigned)((n5 + 3) / 4), VB, 0, c->stream>>>(n5, c->Mss, in, out);
+            PS_LAUNCH(c);
+            return PS_OK;
+        };
+        ps_solve_info si{};
+        ps_err r = minres_core(c, n5, c->mss_dinv, op, c->bs_r, x + nu, 0, 1e-13, 10000, &si);
+        if (r != PS_OK && r != PS_ERR_NOCONV) return r;
+        if (mss_its) *mss_its += si.iterations;
+    }
+    PS_TRY(mob_apply(c, x, c->bs_y, prec));
+    k_bs_u<<<eblocks(nu), VB, 0, st>>>(x, c->bs_y, b, n, nu);
+    PS_LAUNCH(c);
+    c->cnt.saddle_solves++;
+    return PS_OK;
+}
+
+ps_err block_prepare(Ctx* c) {
+    const int64_t n5 = 5 * (int64_t)c->Nc;
+    if (!c->Mss || n5 == 0) return PS_OK;
+    k_mss_dinv<<<eblocks(n5), VB, 0, c->stream>>>(n5, c->Mss, c->mss_dinv);
+    PS_LAUNCH(c);
+    return PS_OK;
+}
+
 ps_err saddle_solve_dev(Ctx* c, const double* b, double* x, int warm, ps_solve_info* info) {
-    if (c->p.precision != PS_PREC_MIXED) return minres_dev(c, b, x, warm, c->p.ksp_rtol, c->p.ksp_maxit, PS_PREC_FP64, info);
-    // Mixed form: FP32-apply MINRES corrections, FP64 residual r = b - A x (iterative refinement).
+    const bool block = c->p.solver == PS_SOLVER_BLOCK;
+    if (c->p.precision != PS_PREC_MIXED) {
+        if (!block) return minres_dev(c, b, x, warm, c->p.ksp_rtol, c->p.ksp_maxit, PS_PREC_FP64, info);
+        int its = 0;
+        PS_TRY(block_solve_once(c, b, x, PS_PREC_FP64, &its));
+        c->last_block_its = its;
+        if (info) {
+            *info = ps_solve_info{};
+            info->iterations = its;
+            info->reason = 2;
+            info->true_rel_residual = -1.0;
+        }
+        return PS_OK;
+    }
+    // Mixed form: FP32-apply corrections, FP64 residual r = b - A x (iterative refinement).
     const int64_t N = c->ns;
     double bn2 = 0.0;
     PS_TRY(vec_dot(c, b, b, N, &bn2));
     const double bn = std::sqrt(bn2);
-    if (!warm) PS_HIP(c, hipMemsetAsync(x, 0, sizeof(double) * (size_t)N, c->stream));
+    int sweeps = 0, its = 0;
+    double rel;
+    double rn2 = 0.0;
+    if (block && !warm) {
+        // first sweep straight from b
+        PS_TRY(block_solve_once(c, b, x, PS_PREC_MIXED, &its));
+        sweeps = 1;
+    } else if (!warm) {
+        PS_HIP(c, hipMemsetAsync(x, 0, sizeof(double) * (size_t)N, c->stream));
+    }
     PS_TRY(saddle_apply(c, x, c->ir_r, PS_PREC_FP64));
     PS_TRY(vec_axpby(c, c->ir_r, 1.0, b, -1.0, N));   // r = b - A x
-    double rn2 = 0.0;
     PS_TRY(vec_dot(c, c->ir_r, c->ir_r, N, &rn2));
-    double rel = bn > 0 ? std::sqrt(rn2) / bn : 0.0;
-    int sweeps = 0, its = 0;
+    rel = bn > 0 ? std::sqrt(rn2) / bn : 0.0;
     ps_err rc = PS_OK;
     const double inner = std::max(c->p.ksp_rtol, 1e-6);
     while (rel > c->p.ir_rtol && sweeps < std::max(1, c->p.ir_maxit)) {
-        ps_solve_info ii{};
-        ps_err r = minres_dev(c, c->ir_r, c->ir_d, 0, inner, c->p.ksp_maxit, PS_PREC_MIXED, &ii);
-        its += ii.iterations;
-        if (r != PS_OK && r != PS_ERR_NOCONV) return r;
+        if (block) {
+            PS_TRY(block_solve_once(c, c->ir_r, c->ir_d, PS_PREC_MIXED, &its));
+        } else {
+            ps_solve_info ii{};
+            ps_err r = minres_dev(c, c->ir_r, c->ir_d, 0, inner, c->p.ksp_maxit, PS_PREC_MIXED, &ii);
+            its += ii.iterations;
+            if (r != PS_OK && r != PS_ERR_NOCONV) return r;
+        }
         PS_TRY(vec_axpby(c, x, 1.0, c->ir_d, 1.0, N));
         PS_TRY(saddle_apply(c, x, c->ir_r, PS_PREC_FP64));
         PS_TRY(vec_axpby(c, c->ir_r, 1.0, b, -1.0, N));

@@ -550,6 +550,33 @@ __global__ void k_stale(SysConst s, const double* __restrict__ xm, int si, int s
     }
 }
 
+// Dense stresslet-stresslet block M_ss (5Nc x 5Nc, row-major): self blocks (fill_self) and the
+// m tiles of every colloid pair in the placement of src/mob.cpp:1129-1143.
+__global__ __launch_bounds__(PS_BLOCK) void k_mss(SysConst s, const double* __restrict__ xm,
+                                                  double* __restrict__ Mss) {
+    const int Nm = s.Nm, Nc = s.Nc;
+    const int c = blockIdx.x;
+    const int64_t n5 = 5 * (int64_t)Nc;
+    for (int o = threadIdx.x; o < Nc; o += PS_BLOCK) {
+        double blk[5][5];
+        if (o == c) {
+            for (int i = 0; i < 5; i++)
+                for (int j = 0; j < 5; j++)
+                    blk[i][j] = (i == j) ? 9. / 5. : ((i == 0 && j == 4) || (i == 4 && j == 0)) ? 9. / 10. : 0.0;
+        } else {
+            const bool role1 = c < o;
+            double e[3], dr, dr_inv;
+            pair_geom_dev(s, xm, Nm + (role1 ? c : o), Nm + (role1 ? o : c), e, dr, dr_inv);
+            BBT T;
+            tile_bb(dr_inv, e, T);
+            for (int i = 0; i < 5; i++)
+                for (int j = 0; j < 5; j++) blk[i][j] = role1 ? T.m[i][j] : T.m[j][i];
+        }
+        for (int i = 0; i < 5; i++)
+            for (int j = 0; j < 5; j++) Mss[(5 * (int64_t)c + i) * n5 + 5 * (int64_t)o + j] = blk[i][j];
+    }
+}
+
 __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * PS_BLOCK + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
@@ -672,6 +699,11 @@ ps_err mob_prepare(Ctx* c) {
         const int64_t n3 = 3 * (int64_t)c->Np;
         k_to_f32<<<(unsigned)((n3 + PS_BLOCK - 1) / PS_BLOCK), PS_BLOCK, 0, c->stream>>>(c->xm, c->xm32, n3);
         PS_LAUNCH(c);
+    }
+    if (c->Mss && c->Nc > 0) {
+        k_mss<<<c->Nc, PS_BLOCK, 0, c->stream>>>(c->sc, c->xm, c->Mss);
+        PS_LAUNCH(c);
+        PS_TRY(block_prepare(c));
     }
     c->cnt.mob_calls++;
     return PS_OK;

@@ -105,6 +105,12 @@ struct Ctx {
     double* gw = nullptr;           // recombination weights (device) kmax
     double *ir_r = nullptr, *ir_d = nullptr;   // refinement residual / correction (ns)
     int kmax = 0;
+    // ---- block-elimination solver (PS_SOLVER_BLOCK)
+    double* Mss = nullptr;          // 5Nc x 5Nc stresslet block, rebuilt by mob()
+    double* mss_dinv = nullptr;     // 1 / diag(M_ss)
+    double* bs_y = nullptr;         // n
+    double* bs_r = nullptr;         // 5Nc
+    double* bs_x = nullptr;         // ns (correction of a refinement sweep)
     // ---- Verlet list (check_dist)
     int ncell[3] = {0, 0, 0};
     int cell_mode = 0;              // 0: brute force; 1: periodic cell list
@@ -135,6 +141,7 @@ struct Ctx {
     // ---- counters
     ps_counters cnt{};
     int last_minres_its = 0;
+    int last_block_its = 0;
     // ---- scratch used by the host-pointer entry points
     double* io_a = nullptr;
     double* io_b = nullptr;
@@ -171,6 +178,7 @@ ps_err krylov_alloc(Ctx* c);
 void krylov_free(Ctx* c);
 ps_err saddle_diag_dev(Ctx* c, double* d);
 ps_err saddle_solve_dev(Ctx* c, const double* b, double* x, int warm, ps_solve_info* info);
+ps_err block_prepare(Ctx* c);   // after mob(): M_ss Jacobi diagonal
 ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* w, ps_sqrt_info* info);
 ps_err pairs_alloc(Ctx* c);
 void pairs_free(Ctx* c);

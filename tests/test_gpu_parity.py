@@ -192,6 +192,54 @@ def test_five_steps_c1():
     assert g.check_dist() == o.check_dist()
 
 
+@pytest.mark.parametrize("name", ["c1", "small_chainHI", "small_dense"])
+def test_block_solver_matches_direct_solve(name):
+    """PS_SOLVER_BLOCK solves the same saddle system exactly (two applies + the M_ss solve)."""
+    g, o, _ = build(name, solver=1)
+    g.check_dist(); o.check_dist()
+    g.mob(); o.mob()
+    b = o.rhs()
+    b[: o.n] += np.random.default_rng(3).normal(size=o.n)
+    A = np.zeros((o.ns, o.ns))
+    A[: o.n, : o.n] = o.mob_dense()
+    A[: o.nu, o.n:] = -np.eye(o.nu)
+    A[o.n:, : o.nu] = -np.eye(o.nu)
+    xd = np.linalg.solve(A, b)
+    xg, info, _ = g.saddle_solve(b)
+    print("case", name, "block M_ss its", info.iterations, "err", rel(xg, xd), flush=True)
+    assert rel(xg, xd) < 1e-10
+    assert rel(A @ xg, b) < 1e-11
+
+
+def test_block_solver_step_matches_minres_step():
+    g, o, x0 = build("c1", ksp_rtol=1e-12, sqrt_rtol=1e-10, solver=1)
+    g.run(1)
+    o.run(1)
+    xg, _, _ = g.state()
+    xo, _, _ = o.state()
+    assert rel(xg - x0, xo - x0) < 1e-9
+    vg, vo = g.vectors(), o.vectors()
+    assert rel(vg["up"], vo["up"]) < 1e-9
+
+
+def test_mixed_block_velocity_error():
+    w = configs.small(4, 10, 20, 8.0, seed=9)
+    pd = configs.params_dict(w)
+    x0 = configs.positions(w)
+    g64 = Context(dict(pd, ksp_rtol=1e-12))
+    gb = Context(dict(pd, precision=1, solver=1, ir_rtol=1e-10))
+    for g in (g64, gb):
+        g.initial_configuration(x0)
+        g.check_dist()
+        g.mob()
+    b = g64.rhs()
+    x64, _, _ = g64.saddle_solve(b)
+    xb, info, _ = gb.saddle_solve(b)
+    err = rel(xb[g64.n:], x64[g64.n:])
+    print("mixed block velocity rel err", err, "sweeps", info.refinements, flush=True)
+    assert err < 1e-9 and info.refinements <= 4
+
+
 def test_mixed_precision_velocity_error():
     w = configs.small(4, 10, 20, 8.0, seed=9)
     pd = configs.params_dict(w)
