@@ -1,19 +1,25 @@
 """Benchmark: bead-steps/s of the full per-timestep hydrodynamic solve on one MI355X.
 
-Workload (default): 20,000 beads = 200 probes (radius 5a) + 990 chains x 20 beads (radius a),
-periodic box 320a, harmonic bonds, full monomer-monomer HI, kT = 1 (reference units: beta = 0.2,
-lengths x 0.2).  A step is one iteration of PolyStokes::run: predictor solve, Brownian slip by the
-Lanczos square root, the two RFD probe solves, the corrector solve and the integrator.  Inputs are
-synthetic (seeded), resident in HBM before timing.
+Workload (default c3): 20,000 beads = 200 probes (radius 5a) + 990 chains x 20 beads (radius a),
+periodic box 320a, harmonic bonds, full monomer-monomer HI, kT = 1, in the reference's units
+(beta = 0.2, lengths x 0.2, time x beta^3).  A step is one iteration of PolyStokes::run: predictor
+solve, Brownian slip by the Lanczos square root (m <= 32, rtol 1e-6), the two RFD probe solves,
+the corrector solve and the integrator.  Inputs are synthetic (seeded), resident in HBM before the
+timed region; the timed region is exactly `steps` full steps.
 
-The headline value is the mixed-precision form (FP32 tiles, FP64 MINRES/Lanczos recurrences and
-FP64 iterative refinement); the plain FP64 form is timed beside it, and both are checked against an
-FP64 run at tight tolerances (relative velocity error).  The roofline entry is for the dominant
-kernel (the monomer-monomer tile kernel), timed live with HIP events on the library's stream.  The
-CPU baseline is the FP64 oracle on the host cores, timed on a bounded sample (a few matrix-free
-applies) and scaled by the applies per step the FP64 form performs.
+Forms timed:
+  mixed/block  (headline)  FP32 tiles, FP64 Krylov recurrences, block-eliminated saddle solve with
+                           FP64 iterative refinement of the saddle residual to 1e-7;
+  fp64/block               the same algorithm in FP64 (the FP64 GPU form the ratio is against);
+  fp64/minres              the reference algorithm (Jacobi MINRES, rtol 1e-6) in FP64, 1 step.
+Each form's velocity `up` after one step from the same state and noise is compared with the FP64
+form of the same algorithm (relative error).  The roofline entry is the dominant kernel (the
+monomer-monomer tile kernel) of the headline form, timed live with HIP events bracketing every
+launch on the library's stream.  The CPU baseline is the FP64 oracle (the reference algorithm,
+matrix-free applies, OpenMP on the host cores) timed on a bounded sample of applies and scaled by
+the applies per step of the corresponding GPU form.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3] [--no-cpu] [--no-minres]
 """
 import argparse
 import json
@@ -41,14 +47,18 @@ def make_ctx(Context, pd, x0, **over):
     return c
 
 
-def timed_run(ctx, steps, warmup):
-    if warmup:
-        ctx.run(warmup)
+def timed_run(ctx, steps, warmup, label):
+    for k in range(warmup):
+        t0 = time.perf_counter()
+        ctx.run(1)
+        log(f"{label}: warmup step {k + 1}/{warmup} {time.perf_counter() - t0:.2f}s")
     c0 = ctx.counters()
     ctx.state()   # synchronise
     ctx.profile_begin()
     t0 = time.perf_counter()
-    ctx.run(steps)
+    for k in range(steps):
+        ctx.run(1)   # ps_run synchronises the library's stream at its end
+        log(f"{label}: timed step {k + 1}/{steps} at {time.perf_counter() - t0:.2f}s")
     ctx.state()
     dt = time.perf_counter() - t0
     prof = ctx.profile_end()
@@ -58,16 +68,27 @@ def timed_run(ctx, steps, warmup):
     return dt, prof, per
 
 
+def one_step_up(Context, pd, x0, **over):
+    ctx = make_ctx(Context, pd, x0, **over)
+    ctx.run(1)
+    up = ctx.vectors()["up"]
+    ctx.close()
+    return up
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-applies", type=int, default=3)
-    ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--solver", default="minres", choices=["minres", "block"])
+    ap.add_argument("--no-minres", action="store_true", help="skip timing the reference MINRES algorithm")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the solve is not sharded across GPUs",
@@ -79,43 +100,52 @@ def main():
 
     w = configs.workload(args.workload)
     pd = configs.params_dict(w)
-    pd["solver"] = 1 if args.solver == "block" else 0
     t0 = time.perf_counter()
     x0 = configs.positions(w)
-    log(f"workload {args.workload}: Np={w['Np']} (Nc={w['Nc']}, Nm={w['Nm']}), configuration {time.perf_counter() - t0:.1f}s")
     Np, Nm = w["Np"], w["Nm"]
+    log(f"workload {args.workload}: Np={Np} (Nc={w['Nc']}, Nm={Nm}), configuration {time.perf_counter() - t0:.1f}s")
 
-    # ---- accuracy reference: one FP64 step at tight tolerances
-    ref = make_ctx(Context, pd, x0, ksp_rtol=1e-10, sqrt_rtol=pd["sqrt_rtol"], solver=0)
-    name, ncu = ref.device_info()
-    log(f"device {name} ({ncu} CUs)")
+    forms = {
+        "mixed/block": dict(precision=1, solver=1, ir_rtol=1e-7),
+        "fp64/block": dict(precision=0, solver=1),
+        "fp64/minres": dict(precision=0, solver=0),
+    }
+    if args.no_minres:
+        forms.pop("fp64/minres")
+
+    # accuracy: one step of each form from the same state and noise, against the FP64 form of the
+    # same saddle algorithm
     t0 = time.perf_counter()
-    ref.run(1)
-    up_ref = ref.vectors()["up"]
-    log(f"reference FP64 step (rtol 1e-10): {time.perf_counter() - t0:.2f}s")
-    ref.close()
+    up_fp64_block = one_step_up(Context, pd, x0, **forms["fp64/block"])
+    log(f"fp64/block accuracy step {time.perf_counter() - t0:.2f}s")
+    up_mixed = one_step_up(Context, pd, x0, **forms["mixed/block"])
+    errs = {"mixed/block": rel(up_mixed, up_fp64_block), "fp64/block": 0.0}
+    log(f"mixed/block vs fp64/block velocity rel err {errs['mixed/block']:.3e}")
 
     results = {}
-    forms = [("mixed", 1)] + ([] if args.skip_fp64 else [("fp64", 0)])
-    for form, prec in forms:
-        ctx = make_ctx(Context, pd, x0, precision=prec, ir_rtol=1e-7 if prec else pd["ksp_rtol"])
-        ctx.run(1)
-        err = float(np.linalg.norm(ctx.vectors()["up"] - up_ref) / np.linalg.norm(up_ref))
+    for form, over in forms.items():
+        ctx = make_ctx(Context, pd, x0, **over)
+        if form == "fp64/minres":
+            name, ncu = ctx.device_info()
+            dt, prof, per = timed_run(ctx, 1, 0, form)
+            up = ctx.vectors()["up"]
+            errs[form] = rel(up, up_fp64_block)
+            steps = 1
+        else:
+            dt, prof, per = timed_run(ctx, args.steps, args.warmup, form)
+            steps = args.steps
         ctx.close()
-        ctx = make_ctx(Context, pd, x0, precision=prec, ir_rtol=1e-7 if prec else pd["ksp_rtol"])
-        dt, prof, per = timed_run(ctx, args.steps, args.warmup)
-        ctx.close()
-        bs = Np * args.steps / dt
         aa_avg = prof["aa_ms"] / max(prof["aa_launches"], 1) * 1e-3
-        flops = 18.0 * Nm * Nm   # 2 * nnz of the monomer-monomer block (3Nm)^2
-        results[form] = dict(value=bs, ms_per_step=1e3 * dt / args.steps, vel_rel_err=err, per_step=per,
-                             aa_ms_avg=aa_avg * 1e3, aa_launches=prof["aa_launches"],
+        flops = 18.0 * Nm * Nm   # 2 * nnz of the monomer-monomer block (3Nm)^2 per launch
+        results[form] = dict(value=Np * steps / dt, ms_per_step=1e3 * dt / steps, vel_rel_err=errs.get(form),
+                             per_step=per, aa_ms_avg=aa_avg * 1e3, aa_launches=prof["aa_launches"],
                              apply_ms_avg=prof["apply_ms"] / max(prof["apply_launches"], 1),
                              achieved=flops / aa_avg if aa_avg > 0 else 0.0)
-        log(f"{form}: {bs:.4g} bead-steps/s, {1e3 * dt / args.steps:.1f} ms/step, vel rel err {err:.2e}, "
+        log(f"{form}: {results[form]['value']:.4g} bead-steps/s, {results[form]['ms_per_step']:.1f} ms/step, "
             f"per step {per}, k_aa avg {aa_avg * 1e3:.3f} ms, apply avg {results[form]['apply_ms_avg']:.3f} ms")
 
     cpu = None
+    cpu_block = None
     if not args.no_cpu:
         from oracle.pyoracle import Oracle, get_threads
         o = Oracle(pd, dense=False)
@@ -128,18 +158,21 @@ def main():
         for _ in range(args.cpu_applies):
             o.mob_mult(f)
         t_apply = (time.perf_counter() - t0) / args.cpu_applies
-        applies = results.get("fp64", results["mixed"])["per_step"]["applies"]
-        t_step = t_apply * applies
-        cpu = {"value": Np / t_step, "unit": "bead-steps/s", "cores": get_threads(), "kind": "port",
+        ref_form = "fp64/minres" if "fp64/minres" in results else "fp64/block"
+        applies = results[ref_form]["per_step"]["applies"]
+        cpu = {"value": Np / (t_apply * applies), "unit": "bead-steps/s", "cores": get_threads(), "kind": "port",
                "sample": f"{args.cpu_applies} FP64 matrix-free mobility applies of the oracle at N={Np} "
-                         f"({t_apply:.3f} s each), scaled by the {applies:.0f} applies per step of the FP64 form"}
+                         f"({t_apply:.3f} s each on {get_threads()} threads), scaled by the {applies:.0f} applies "
+                         f"per step of the {ref_form} GPU form (the reference algorithm)"}
+        ab = results["fp64/block"]["per_step"]["applies"]
+        cpu_block = {"value": Np / (t_apply * ab), "applies_per_step": ab}
         log(f"cpu baseline {cpu}")
 
-    head = results["mixed"]
-    fp64 = results.get("fp64")
+    head = results["mixed/block"]
+    f64 = results["fp64/block"]
     roof = {"bound": "mfma", "achieved": head["achieved"] / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
             "frac": head["achieved"] / PEAK["fp32"], "traffic": None,
-            "kernel": "k_aa<float> (monomer-monomer tiles, VALU-bound)",
+            "kernel": "k_aa<float> (monomer-monomer tiles; VALU-bound, packed FP32 disabled)",
             "flops_per_launch": 18.0 * Nm * Nm, "avg_launch_ms": head["aa_ms_avg"]}
     out = {
         "metric": "bead-steps/s at N=20k: mixed-prec vs FP64 GPU at 1e-6 rel err; % roofline",
@@ -147,20 +180,20 @@ def main():
         "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32+fp64", "data": "synthetic",
         "config": {"workload": f"{args.workload}: {Np} beads ({w['Nc']} probes + {w['Npoly']} chains x 20), "
-                               f"box {w['L']:.1f}, full mm-HI, saddle solver {args.solver} (rtol {pd['ksp_rtol']}), "
-                               f"Lanczos m<={pd['sqrt_kmax']}",
+                               f"box {w['L']:.1f}, full mm-HI, mixed precision, block-eliminated saddle solve "
+                               f"with FP64 refinement, Lanczos m<={pd['sqrt_kmax']} rtol {pd['sqrt_rtol']}",
                    "global_batch": Np, "seq_len": 1, "parallelism": "single-gpu"},
-        "mixed_vel_rel_err": head["vel_rel_err"],
-        "fp64_value": fp64["value"] if fp64 else None,
-        "fp64_vel_rel_err": fp64["vel_rel_err"] if fp64 else None,
-        "mixed_over_fp64": head["value"] / fp64["value"] if fp64 else None,
-        "per_step": head["per_step"],
+        "vel_rel_err_vs_fp64": head["vel_rel_err"],
+        "fp64_value": f64["value"],
+        "mixed_over_fp64": head["value"] / f64["value"],
+        "forms": {k: {kk: v[kk] for kk in ("value", "ms_per_step", "vel_rel_err", "per_step", "aa_ms_avg",
+                                             "apply_ms_avg")} for k, v in results.items()},
         "roofline": roof,
+        "roofline_fp64": {"achieved": f64["achieved"] / 1e12, "peak": PEAK["fp64"] / 1e12,
+                          "frac": f64["achieved"] / PEAK["fp64"], "avg_launch_ms": f64["aa_ms_avg"]},
         "cpu_baseline": cpu,
+        "cpu_baseline_block_algorithm": cpu_block,
     }
-    if fp64:
-        out["roofline_fp64"] = {"achieved": fp64["achieved"] / 1e12, "peak": PEAK["fp64"] / 1e12,
-                                "frac": fp64["achieved"] / PEAK["fp64"], "avg_launch_ms": fp64["aa_ms_avg"]}
     print(json.dumps(out), flush=True)
 
 

@@ -72,7 +72,10 @@ __global__ __launch_bounds__(PS_BLOCK) void k_aa(SysConst s, const T* __restrict
         zi = xm[3 * i + 2];
         if (CHAIN) ci = chain[i];
     }
-    const T tb = (T)s.two_beta;
+    // mobility(..., AA=true), src/mob.cpp:122-131, applied as y f + (x - y)/r^2 (d.f) d:
+    //   far  (r > 2 beta): y = r^-1 (3/4 - 1/2 r^-2),  (x - y)/r^2 = r^-3 (-3/8 - 3/2 r^-2)
+    //   near (r <= 2 beta): y = 1 - 9/32 r,          (x - y)/r^2 = (21/32 r - 2) r^-2
+    const T tb2 = (T)(s.two_beta * s.two_beta);
     T ux = 0, uy = 0, uz = 0;
     const int s0 = blockIdx.y * per_split;
     const int s1 = min(Nm, s0 + per_split);
@@ -90,30 +93,55 @@ __global__ __launch_bounds__(PS_BLOCK) void k_aa(SysConst s, const T* __restrict
         }
         __syncthreads();
         const int cnt = min(PS_BLOCK, s1 - t0);
+        // Source tiles are aligned to 256, so only the tile holding this block's own beads can
+        // contain i == j: the masked loop runs there (and under chain_HI), the lean loop elsewhere.
+        const bool masked = CHAIN || (t0 == (int)(blockIdx.x * PS_BLOCK));
+        if (masked) {
+            for (int k = 0; k < cnt; k++) {
+                T dx = xi - sx[k], dy = yi - sy[k], dz = zi - sz[k];
+                if (BOX) min_img<T>(s, dx, dy, dz);
+                T r2 = dx * dx + dy * dy + dz * dz;
+                bool skip = (t0 + k == i);
+                if (CHAIN) skip |= (sc[k] == ci);
+                r2 = skip ? (T)1 : r2;
+                const T rinv = ps_rsqrt(r2);
+                const T rinv2 = rinv * rinv;
+                T ya = rinv * ((T)C3D4 - (T)C1D2 * rinv2);
+                T cc = rinv * rinv2 * (-(T)C3D8 - (T)C3D2 * rinv2);
+                if (r2 <= tb2) {
+                    const T r = r2 * rinv;
+                    ya = (T)1 - (T)C9D32 * r;
+                    cc = ((T)(21. / 32.) * r - (T)2) * rinv2;
+                }
+                ya = skip ? (T)0 : ya;
+                cc = skip ? (T)0 : cc;
+                const T fx = sfx[k], fy = sfy[k], fz = sfz[k];
+                const T cs = cc * (dx * fx + dy * fy + dz * fz);
+                ux += ya * fx + cs * dx;
+                uy += ya * fy + cs * dy;
+                uz += ya * fz + cs * dz;
+            }
+        } else {
 #pragma unroll 4
-        for (int k = 0; k < cnt; k++) {
-            T dx = xi - sx[k], dy = yi - sy[k], dz = zi - sz[k];
-            if (BOX) min_img<T>(s, dx, dy, dz);
-            T r2 = dx * dx + dy * dy + dz * dz;
-            bool skip = (t0 + k == i);
-            if (CHAIN) skip |= (sc[k] == ci);
-            r2 = skip ? (T)1 : r2;
-            const T rinv = ps_rsqrt(r2);
-            const T rinv2 = rinv * rinv;
-            const T rinv3 = rinv2 * rinv;
-            const T r = r2 * rinv;
-            const bool far = r > tb;
-            // mobility(..., AA=true): x12a, y12a (src/mob.cpp:122-131)
-            T xa = far ? (T)C3D8 * rinv - (T)2 * rinv3 : (T)C3D8 * r - (T)1;
-            T ya = far ? (T)C3D4 * rinv - (T)C1D2 * rinv3 : (T)1 - (T)C9D32 * r;
-            T cc = (xa - ya) * rinv2;
-            ya = skip ? (T)0 : ya;
-            cc = skip ? (T)0 : cc;
-            const T fx = sfx[k], fy = sfy[k], fz = sfz[k];
-            const T cs = cc * (dx * fx + dy * fy + dz * fz);
-            ux += ya * fx + cs * dx;
-            uy += ya * fy + cs * dy;
-            uz += ya * fz + cs * dz;
+            for (int k = 0; k < cnt; k++) {
+                T dx = xi - sx[k], dy = yi - sy[k], dz = zi - sz[k];
+                if (BOX) min_img<T>(s, dx, dy, dz);
+                const T r2 = dx * dx + dy * dy + dz * dz;
+                const T rinv = ps_rsqrt(r2);
+                const T rinv2 = rinv * rinv;
+                T ya = rinv * ((T)C3D4 - (T)C1D2 * rinv2);
+                T cc = rinv * rinv2 * (-(T)C3D8 - (T)C3D2 * rinv2);
+                if (__builtin_expect(r2 <= tb2, 0)) {
+                    const T r = r2 * rinv;
+                    ya = (T)1 - (T)C9D32 * r;
+                    cc = ((T)(21. / 32.) * r - (T)2) * rinv2;
+                }
+                const T fx = sfx[k], fy = sfy[k], fz = sfz[k];
+                const T cs = cc * (dx * fx + dy * fy + dz * fz);
+                ux += ya * fx + cs * dx;
+                uy += ya * fy + cs * dy;
+                uz += ya * fz + cs * dz;
+            }
         }
     }
     if (act) {
@@ -644,7 +672,9 @@ ps_err alloc_work(Ctx* c, ApplyWork& aw, size_t elt) {
     int nsplit = ntb > 0 ? (target + ntb - 1) / ntb : 1;
     nsplit = std::max(1, std::min(nsplit, std::max(1, s.Nm / 64)));
     aw.nsplit_aa = nsplit;
+    // source splits are whole 256-bead tiles (k_aa relies on tile-aligned sources)
     aw.per_split_aa = s.Nm > 0 ? (s.Nm + nsplit - 1) / nsplit : 0;
+    aw.per_split_aa = ((aw.per_split_aa + PS_BLOCK - 1) / PS_BLOCK) * PS_BLOCK;
     aw.nsplit_aa = s.Nm > 0 ? (s.Nm + aw.per_split_aa - 1) / aw.per_split_aa : 1;
     int nchunk = s.Nc > 0 ? (2 * (c->n_cu > 0 ? c->n_cu : 256) + s.Nc - 1) / s.Nc : 1;
     nchunk = std::max(1, std::min(nchunk, std::max(1, s.Nm / PS_BLOCK)));

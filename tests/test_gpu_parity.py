@@ -144,9 +144,11 @@ def test_saddle_solve(name, tol, same_its):
         assert abs(ig.iterations - io.iterations) <= 2
         assert rel(xg, xo) < tol
     assert rel(xg, xd) < tol and rel(xo, xd) < tol
-    # warm start from the solution converges at once to the same answer
+    # warm start from the solution: converges at once (well-conditioned) to the same answer
     xw, iw, _ = g.saddle_solve(b, x0=xg)
-    assert iw.iterations <= 2 and rel(xw, xd) < tol
+    assert rel(xw, xd) < tol
+    if same_its:
+        assert iw.iterations <= 2
 
 
 @pytest.mark.parametrize("name", ["c1", "small_dense"])
@@ -176,8 +178,11 @@ def test_one_step(name):
     # displacement over the step, the quantity the tolerance is stated for
     x0 = build(name)[2]
     assert rel(xg - x0, xo - x0) < 1e-10
-    for k in ("up", "v_det_n", "v_det_c", "v_brown", "udiff"):
+    for k in ("up", "v_det_n", "v_det_c", "v_brown"):
         assert rel(vg[k], vo[k]) < 1e-10, k
+    # udiff and sb are differences of two probe solves 2*eps = 2e-3 apart (src/drift.cpp:57-58): each
+    # solve is pinned at rtol 1e-12, so their difference is pinned only to ~1e-12 / eps
+    assert rel(vg["udiff"], vo["udiff"]) < 1e-8
     assert rel(vg["sb"], vo["sb"]) < 1e-8
     assert rel(qg, qo) < 1e-12
 
@@ -222,38 +227,45 @@ def test_block_solver_step_matches_minres_step():
     assert rel(vg["up"], vo["up"]) < 1e-9
 
 
+def _direct(o, b):
+    A = np.zeros((o.ns, o.ns))
+    A[: o.n, : o.n] = o.mob_dense()
+    A[: o.nu, o.n:] = -np.eye(o.nu)
+    A[o.n:, : o.nu] = -np.eye(o.nu)
+    return np.linalg.solve(A, b)
+
+
+def _mixed_case(**over):
+    w = configs.small(4, 10, 20, 8.0, seed=9)
+    pd = configs.params_dict(w)
+    x0 = configs.positions(w)
+    g = Context(dict(pd, precision=1, **over))
+    g.initial_configuration(x0)
+    g.check_dist()
+    g.mob()
+    o = _oracle()(pd)
+    o.initial_configuration(x0)
+    o.check_dist()
+    o.mob()
+    b = o.rhs()
+    b[: o.n] += np.random.default_rng(5).normal(size=o.n)
+    return g, o, b, _direct(o, b)
+
+
 def test_mixed_block_velocity_error():
-    w = configs.small(4, 10, 20, 8.0, seed=9)
-    pd = configs.params_dict(w)
-    x0 = configs.positions(w)
-    g64 = Context(dict(pd, ksp_rtol=1e-12))
-    gb = Context(dict(pd, precision=1, solver=1, ir_rtol=1e-10))
-    for g in (g64, gb):
-        g.initial_configuration(x0)
-        g.check_dist()
-        g.mob()
-    b = g64.rhs()
-    x64, _, _ = g64.saddle_solve(b)
-    xb, info, _ = gb.saddle_solve(b)
-    err = rel(xb[g64.n:], x64[g64.n:])
-    print("mixed block velocity rel err", err, "sweeps", info.refinements, flush=True)
-    assert err < 1e-9 and info.refinements <= 4
+    """FP32 tiles + block elimination + FP64 refinement against the dense FP64 direct solve."""
+    g, o, b, xd = _mixed_case(solver=1, ir_rtol=1e-10)
+    x, info, _ = g.saddle_solve(b)
+    err = rel(x[o.n:], xd[o.n:])
+    print("mixed block velocity rel err", err, "sweeps", info.refinements, "true res", info.true_rel_residual, flush=True)
+    assert info.true_rel_residual <= 1e-10
+    assert err < 1e-8 and info.refinements <= 6
 
 
-def test_mixed_precision_velocity_error():
-    w = configs.small(4, 10, 20, 8.0, seed=9)
-    pd = configs.params_dict(w)
-    x0 = configs.positions(w)
-    g64 = Context(dict(pd, ksp_rtol=1e-12))
-    g32 = Context(dict(pd, precision=1, ksp_rtol=1e-6, ir_rtol=1e-8))
-    for g in (g64, g32):
-        g.initial_configuration(x0)
-        g.check_dist()
-        g.mob()
-    b = g64.rhs()
-    x64, _, _ = g64.saddle_solve(b)
-    x32, info, _ = g32.saddle_solve(b)
-    nu = g64.nu
-    err = rel(x32[g64.n:], x64[g64.n:])
-    print("mixed velocity rel err", err, "sweeps", info.refinements, "its", info.iterations, flush=True)
+def test_mixed_minres_velocity_error():
+    """FP32-apply MINRES corrections + FP64 refinement against the dense FP64 direct solve."""
+    g, o, b, xd = _mixed_case(solver=0, ksp_rtol=1e-6, ir_rtol=1e-9, ir_maxit=30)
+    x, info, _ = g.saddle_solve(b)
+    err = rel(x[o.n:], xd[o.n:])
+    print("mixed minres velocity rel err", err, "sweeps", info.refinements, "its", info.iterations, flush=True)
     assert err < 1e-6
