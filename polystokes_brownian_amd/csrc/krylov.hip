@@ -590,7 +590,8 @@ static ps_err block_solve_once(Ctx* c, const double* b, double* x, int prec, int
     k_bs_init<<<eblocks(n), VB, 0, st>>>(x, b, n, nu);
     PS_LAUNCH(c);
     if (n5 > 0) {
-        PS_TRY(mob_apply(c, x, c->bs_y, prec));
+        // only the 5Nc stresslet rows of M [f_u; 0] are needed: colloid-row kernels, O(Nm Nc)
+        PS_TRY(mob_apply_stresslet_rows(c, x, c->bs_y));
         k_bs_rs<<<eblocks(n5), VB, 0, st>>>(c->bs_r, b, c->bs_y, nu, n5);
         PS_LAUNCH(c);
         auto op = [c, n5](const double* in, double* out) -> ps_err {
@@ -599,7 +600,7 @@ static ps_err block_solve_once(Ctx* c, const double* b, double* x, int prec, int
             return PS_OK;
         };
         ps_solve_info si{};
-        ps_err r = minres_core(c, n5, c->mss_dinv, op, c->bs_r, x + nu, 0, 1e-13, 10000, &si);
+        ps_err r = minres_core(c, n5, c->mss_dinv, op, c->bs_r, x + nu, 0, 1e-12, 10000, &si);
         if (r != PS_OK && r != PS_ERR_NOCONV) return r;
         if (mss_its) *mss_its += si.iterations;
     }
@@ -620,7 +621,9 @@ ps_err block_prepare(Ctx* c) {
 
 ps_err saddle_solve_dev(Ctx* c, const double* b, double* x, int warm, ps_solve_info* info) {
     const bool block = c->p.solver == PS_SOLVER_BLOCK;
-    if (c->p.precision != PS_PREC_MIXED) {
+    // Block elimination costs one full apply per solve; that apply stays FP64 in every form, since
+    // an FP32 apply followed by an FP64 residual would cost more than the FP64 apply itself.
+    if (c->p.precision != PS_PREC_MIXED || block) {
         if (!block) return minres_dev(c, b, x, warm, c->p.ksp_rtol, c->p.ksp_maxit, PS_PREC_FP64, info);
         int its = 0;
         PS_TRY(block_solve_once(c, b, x, PS_PREC_FP64, &its));
@@ -685,7 +688,36 @@ ps_err saddle_solve_dev(Ctx* c, const double* b, double* x, int warm, ps_solve_i
     return rc;
 }
 
-// w = M^{1/2} xi.
+// q_{j+1} = w / sqrt(norm2) with the norm read on the device (no host round trip per iteration)
+__global__ void k_lz_next(int64_t n, double* __restrict__ q, const double* __restrict__ w,
+                          const double* __restrict__ norm2) {
+    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
+    if (i < n) q[i] = w[i] * (1.0 / sqrt(fmax(*norm2, 0.0)));
+}
+__global__ void k_lz_a_dev(int64_t n, double* __restrict__ w, const double* __restrict__ qprev,
+                           const double* __restrict__ bprev_norm2, const double* __restrict__ qj,
+                           double* __restrict__ red) {
+    __shared__ double sh[4];
+    const double bprev = qprev ? sqrt(fmax(*bprev_norm2, 0.0)) : 0.0;
+    const int64_t base = (int64_t)blockIdx.x * VCHUNK;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < VPT; k++) {
+        const int64_t i = base + k * VB + threadIdx.x;
+        if (i < n) {
+            double wi = w[i];
+            if (qprev) wi -= bprev * qprev[i];
+            w[i] = wi;
+            acc += qj[i] * wi;
+        }
+    }
+    acc = block_reduce(acc, sh);
+    if (threadIdx.x == 0) red[blockIdx.x] = acc;
+}
+
+// w = M^{1/2} xi.  The device runs the Lanczos recurrence ahead; the host reads alpha/beta every
+// LZ_POLL iterations and finds the first m where the stopping test holds (vectors built past it are
+// never used, so the result equals stopping at m).
 ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* out, ps_sqrt_info* info) {
     const int64_t n = c->n;
     const int nb = nblocks(n);
@@ -704,58 +736,70 @@ ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* out, ps_sqrt_info* info)
     if (beta0 == 0.0) return PS_OK;
     const int kmax = c->kmax;
     const int prec = c->p.precision;
+    const int LZ_POLL = 8;
     double* w = c->kr;   // MINRES scratch is free during the square root
     std::vector<double> alpha, betad, g, gprev;
     k_scale<<<eblocks(n), VB, 0, st>>>(n, c->Q, xi, 1.0 / beta0);
     PS_LAUNCH(c);
     int m = 0;
     double relc = 0.0;
-    for (int j = 0; j < kmax; j++) {
-        const double* qj = c->Q + (int64_t)j * n;
-        PS_TRY(mob_apply(c, qj, w, prec));
-        k_lz_a<<<nb, VB, 0, st>>>(n, w, j > 0 ? c->Q + (int64_t)(j - 1) * n : nullptr,
-                                  j > 0 ? betad[j - 1] : 0.0, qj, c->red);
-        PS_LAUNCH(c);
-        k_lz_b<<<nb, VB, 0, st>>>(n, w, qj, c->red, nb, c->lz + j);
-        PS_LAUNCH(c);
-        for (int pass = 0; pass < 2; pass++) {
-            k_lz_dots<<<nb, VB, 0, st>>>(n, c->Q, j + 1, w, c->red);
+    bool stop = false;
+    int j0 = 0;
+    while (!stop && j0 < kmax) {
+        const int j1 = std::min(kmax, j0 + LZ_POLL);   // iterations [j0, j1) on the device
+        for (int j = j0; j < j1; j++) {
+            const double* qj = c->Q + (int64_t)j * n;
+            PS_TRY(mob_apply(c, qj, w, prec));
+            k_lz_a_dev<<<nb, VB, 0, st>>>(n, w, j > 0 ? c->Q + (int64_t)(j - 1) * n : nullptr,
+                                          c->lz + 64 + (j > 0 ? j - 1 : 0), qj, c->red);
             PS_LAUNCH(c);
-            k_lz_sub<<<nb, VB, 0, st>>>(n, c->Q, j + 1, w, c->red, nb);
+            k_lz_b<<<nb, VB, 0, st>>>(n, w, qj, c->red, nb, c->lz + j);
             PS_LAUNCH(c);
-        }
-        k_dot_part<<<nb, VB, 0, st>>>(w, w, n, c->red);
-        PS_LAUNCH(c);
-        k_sum_to<<<1, VB, 0, st>>>(c->red, nb, c->lz + 64 + j);
-        PS_LAUNCH(c);
-        PS_HIP(c, hipMemcpyAsync(c->lz_h + j, c->lz + j, sizeof(double), hipMemcpyDeviceToHost, st));
-        PS_HIP(c, hipMemcpyAsync(c->lz_h + 64 + j, c->lz + 64 + j, sizeof(double), hipMemcpyDeviceToHost, st));
-        PS_HIP(c, hipStreamSynchronize(st));
-        alpha.push_back(c->lz_h[j]);
-        const double bj = std::sqrt(std::max(c->lz_h[64 + j], 0.0));
-        m = j + 1;
-        c->cnt.lanczos_iterations++;
-        bool stop = false;
-        if (c->p.sqrt_rtol > 0.0) {
-            g.assign(m, 0.0);
-            tridiag_sqrt_weights(m, alpha.data(), betad.data(), beta0, g.data());
-            if (m >= 2) {
-                double dn = 0.0, gn = 0.0;
-                for (int i = 0; i < m; i++) {
-                    const double gp = (i < m - 1) ? gprev[i] : 0.0;
-                    dn += (g[i] - gp) * (g[i] - gp);
-                    gn += g[i] * g[i];
-                }
-                relc = std::sqrt(dn / gn);
-                if (relc <= c->p.sqrt_rtol) stop = true;
+            for (int pass = 0; pass < 2; pass++) {
+                k_lz_dots<<<nb, VB, 0, st>>>(n, c->Q, j + 1, w, c->red);
+                PS_LAUNCH(c);
+                k_lz_sub<<<nb, VB, 0, st>>>(n, c->Q, j + 1, w, c->red, nb);
+                PS_LAUNCH(c);
             }
-            gprev = g;
+            k_dot_part<<<nb, VB, 0, st>>>(w, w, n, c->red);
+            PS_LAUNCH(c);
+            k_sum_to<<<1, VB, 0, st>>>(c->red, nb, c->lz + 64 + j);
+            PS_LAUNCH(c);
+            if (j + 1 < kmax) {
+                k_lz_next<<<eblocks(n), VB, 0, st>>>(n, c->Q + (int64_t)(j + 1) * n, w, c->lz + 64 + j);
+                PS_LAUNCH(c);
+            }
         }
-        if (stop) break;
-        if (bj <= 1e-12 * beta0 || j == kmax - 1) break;
-        betad.push_back(bj);
-        k_scale<<<eblocks(n), VB, 0, st>>>(n, c->Q + (int64_t)(j + 1) * n, w, 1.0 / bj);
-        PS_LAUNCH(c);
+        PS_HIP(c, hipMemcpyAsync(c->lz_h, c->lz, sizeof(double) * 128, hipMemcpyDeviceToHost, st));
+        PS_HIP(c, hipStreamSynchronize(st));
+        for (int j = j0; j < j1 && !stop; j++) {
+            alpha.push_back(c->lz_h[j]);
+            const double bj = std::sqrt(std::max(c->lz_h[64 + j], 0.0));
+            m = j + 1;
+            c->cnt.lanczos_iterations++;
+            if (c->p.sqrt_rtol > 0.0) {
+                g.assign(m, 0.0);
+                tridiag_sqrt_weights(m, alpha.data(), betad.data(), beta0, g.data());
+                if (m >= 2) {
+                    double dn = 0.0, gn = 0.0;
+                    for (int i = 0; i < m; i++) {
+                        const double gp = (i < m - 1) ? gprev[i] : 0.0;
+                        dn += (g[i] - gp) * (g[i] - gp);
+                        gn += g[i] * g[i];
+                    }
+                    relc = std::sqrt(dn / gn);
+                    if (relc <= c->p.sqrt_rtol) stop = true;
+                }
+                gprev = g;
+            }
+            if (stop) break;
+            if (bj <= 1e-12 * beta0 || j == kmax - 1) {
+                stop = true;
+                break;
+            }
+            betad.push_back(bj);
+        }
+        j0 = j1;
     }
     g.assign(m, 0.0);
     const int nneg = tridiag_sqrt_weights(m, alpha.data(), betad.data(), beta0, g.data());

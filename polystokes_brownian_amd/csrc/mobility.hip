@@ -476,7 +476,8 @@ __global__ __launch_bounds__(PS_BLOCK) void k_bb(SysConst s, const double* __res
 }
 
 // ---------------------------------------------------------------------------- finalize
-// mode 0: u = M f;  mode 1: saddle y = A x with x = [f ; lambda] (lambda = f + n).
+// mode 0: u = M f;  mode 1: saddle y = A x with x = [f ; lambda] (lambda = f + n);
+// rows start at r_begin (the stresslet-rows-only apply starts at nu).
 template <class T>
 __global__ __launch_bounds__(PS_BLOCK) void k_finalize(SysConst s, const double* __restrict__ f,
                                                        double* __restrict__ u, const T* __restrict__ part_aa,
@@ -484,8 +485,9 @@ __global__ __launch_bounds__(PS_BLOCK) void k_finalize(SysConst s, const double*
                                                        const T* __restrict__ part_c,
                                                        const T* __restrict__ part_fs, int nchunk,
                                                        const double* __restrict__ bb,
-                                                       const double* __restrict__ stale, int saddle) {
-    const int64_t r = (int64_t)blockIdx.x * PS_BLOCK + threadIdx.x;
+                                                       const double* __restrict__ stale, int saddle,
+                                                       int64_t r_begin) {
+    const int64_t r = r_begin + (int64_t)blockIdx.x * PS_BLOCK + threadIdx.x;
     const int64_t n = s.n, nm3 = s.nm3, nc3 = s.nc3, nu = s.nu;
     const int Nm = s.Nm, Nc = s.Nc;
     if (r >= n) return;
@@ -610,16 +612,18 @@ __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, in
     if (i < n) b[i] = (float)a[i];
 }
 
+// saddle: 0 = u = M f, 1 = saddle operator, 2 = only the stresslet rows [nu, n) of M f.
 template <class T>
 ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double* u, int saddle) {
     const SysConst& s = c->sc;
     hipStream_t st = c->stream;
     hipEvent_t ea0 = nullptr, ea1 = nullptr, ek0 = nullptr, ek1 = nullptr;
-    if (c->prof_on) {
+    const bool srows = saddle == 2;
+    if (c->prof_on && !srows) {
         PS_TRY(c->prof_record(1, &ea0, &ea1));
         PS_HIP(c, hipEventRecord(ea0, st));
     }
-    if (s.Nm > 0) {
+    if (s.Nm > 0 && !srows) {
         dim3 g1((s.Nm + PS_BLOCK - 1) / PS_BLOCK, aw.nsplit_aa);
         if (c->prof_on) {
             PS_TRY(c->prof_record(0, &ek0, &ek1));
@@ -655,11 +659,15 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
         k_bb<<<s.Nc, PS_BLOCK, 0, st>>>(s, c->xm, f, aw.bb);
         PS_LAUNCH(c);
     }
-    dim3 g4((unsigned)((s.n + PS_BLOCK - 1) / PS_BLOCK));
+    const int64_t r_begin = srows ? s.nu : 0;
+    if (s.n - r_begin <= 0) return PS_OK;
+    dim3 g4((unsigned)((s.n - r_begin + PS_BLOCK - 1) / PS_BLOCK));
     k_finalize<T><<<g4, PS_BLOCK, 0, st>>>(s, f, u, (const T*)aw.part_aa, s.Nm > 0 ? aw.nsplit_aa : 0,
                                            (const T*)aw.part_abm, (const T*)aw.part_c, (const T*)aw.part_fs,
-                                           (s.Nm > 0 && s.Nc > 0) ? aw.nchunk_c : 0, aw.bb, c->stale, saddle);
+                                           (s.Nm > 0 && s.Nc > 0) ? aw.nchunk_c : 0, aw.bb, c->stale,
+                                           srows ? 0 : saddle, r_begin);
     PS_LAUNCH(c);
+    if (srows) return PS_OK;
     if (c->prof_on) PS_HIP(c, hipEventRecord(ea1, st));
     c->cnt.applies++;
     return PS_OK;
@@ -742,6 +750,10 @@ ps_err mob_prepare(Ctx* c) {
 ps_err mob_apply(Ctx* c, const double* f, double* u, int prec) {
     if (prec == PS_PREC_MIXED && c->xm32) return launch_apply<float>(c, c->xm32, c->aw32, f, u, 0);
     return launch_apply<double>(c, c->xm, c->aw64, f, u, 0);
+}
+
+ps_err mob_apply_stresslet_rows(Ctx* c, const double* f, double* u) {
+    return launch_apply<double>(c, c->xm, c->aw64, f, u, 2);
 }
 
 ps_err saddle_apply(Ctx* c, const double* x, double* y, int prec) {

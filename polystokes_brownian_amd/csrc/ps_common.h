@@ -172,6 +172,7 @@ __device__ __forceinline__ float ps_rint<float>(float v) { return __builtin_rint
 ps_err mob_prepare(Ctx* c);                                        // stale tile + snapshots
 ps_err mob_apply(Ctx* c, const double* f, double* u, int prec);    // u = M f (device pointers)
 ps_err saddle_apply(Ctx* c, const double* x, double* y, int prec); // y = A x (device pointers)
+ps_err mob_apply_stresslet_rows(Ctx* c, const double* f, double* u); // u[nu:n] = (M f)[nu:n], FP64
 ps_err apply_alloc(Ctx* c);
 void apply_free(Ctx* c);
 ps_err krylov_alloc(Ctx* c);

@@ -253,13 +253,18 @@ def _mixed_case(**over):
 
 
 def test_mixed_block_velocity_error():
-    """FP32 tiles + block elimination + FP64 refinement against the dense FP64 direct solve."""
-    g, o, b, xd = _mixed_case(solver=1, ir_rtol=1e-10)
+    """Mixed form with block elimination: the solve's one apply stays FP64, so the saddle solution
+    matches the dense FP64 direct solve to FP64 accuracy; the FP32 tiles serve the square root."""
+    g, o, b, xd = _mixed_case(solver=1)
     x, info, _ = g.saddle_solve(b)
     err = rel(x[o.n:], xd[o.n:])
-    print("mixed block velocity rel err", err, "sweeps", info.refinements, "true res", info.true_rel_residual, flush=True)
-    assert info.true_rel_residual <= 1e-10
-    assert err < 1e-8 and info.refinements <= 6
+    print("mixed block velocity rel err", err, flush=True)
+    assert err < 1e-10
+    xi = np.random.default_rng(6).normal(size=o.n)
+    w32, i32 = g.sqrt_apply(xi)
+    w64, i64 = o.sqrt_apply(xi)
+    print("mixed sqrt rel err", rel(w32, w64), i32.iterations, i64.iterations, flush=True)
+    assert rel(w32, w64) < 1e-5
 
 
 def test_mixed_minres_velocity_error():
