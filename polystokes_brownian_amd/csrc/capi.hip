@@ -12,6 +12,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <memory>
+#include <mutex>
+#include <thread>
 
 #pragma clang fp contract(off)
 
@@ -116,13 +120,63 @@ __global__ void k_sb(double* __restrict__ sb, const double* __restrict__ X, int6
     else sb[i] = X[nu + i];
 }
 
-ps_err draw_noise(Ctx* c, double* dst_dev, int64_t n) {
-    // The reference's host stream, drawn in order into pinned memory, copied to HBM.
-    PS_HIP(c, hipStreamSynchronize(c->stream));   // the pinned buffer may still be in flight
-    for (int64_t i = 0; i < n; i++) c->noise_h[i] = c->nd(c->rng);
-    PS_HIP(c, hipMemcpyAsync(dst_dev, c->noise_h, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream));
-    return PS_OK;
-}
+// Noise producer for ps_run: a host thread draws each step's n + (3Nm+6Nc) normals from the
+// reference's stream, in the reference's order (slip first, then RFD), into one of two pinned slots
+// while the GPU works on the previous step.  It draws exactly nsteps blocks, so the stream position
+// after ps_run(k) equals k sequential steps.  A slot is refilled only after the copies that read it
+// have completed (event).
+struct NoisePipe {
+    Ctx* c = nullptr;
+    int64_t len = 0;
+    int nsteps = 0;
+    double* slot[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    std::mutex mu;
+    std::condition_variable cv;
+    int filled = 0, consumed = 0;
+    bool abort = false;
+    std::thread th;
+
+    void produce() {
+        for (int s = 0; s < nsteps; s++) {
+            const int k = s & 1;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return abort || consumed >= s - 1; });
+                if (abort) return;
+            }
+            if (s >= 2) (void)hipEventSynchronize(done[k]);
+            for (int64_t i = 0; i < len; i++) slot[k][i] = c->nd(c->rng);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                filled = s + 1;
+            }
+            cv.notify_all();
+        }
+    }
+    double* take(int s) {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return filled > s; });
+        return slot[s & 1];
+    }
+    ps_err release(int s) {
+        PS_HIP(c, hipEventRecord(done[s & 1], c->stream));
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            consumed = s + 1;
+        }
+        cv.notify_all();
+        return PS_OK;
+    }
+    void stop() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            abort = true;
+        }
+        cv.notify_all();
+        if (th.joinable()) th.join();
+    }
+};
 
 ps_err solve(Ctx* c, double* X, int warm) {
     ps_solve_info info{};
@@ -131,8 +185,9 @@ ps_err solve(Ctx* c, double* X, int warm) {
     return r;
 }
 
-// One iteration of the run() loop.
-ps_err one_step(Ctx* c) {
+// One iteration of the run() loop; `pipe` supplies this step's noise (step index s), or is null
+// when kT <= 0.
+ps_err one_step(Ctx* c, NoisePipe* pipe, int s) {
     const int64_t n = c->n, nu = c->nu, n3 = 3 * (int64_t)c->Np, nc5 = 5 * (int64_t)c->Nc;
     hipStream_t st = c->stream;
     const double dt = c->p.dt, kT = c->p.kT, eps = 1.0e-3;
@@ -147,7 +202,8 @@ ps_err one_step(Ctx* c) {
     PS_LAUNCH(c);
     if (kT > 0) {
         // solve_slip_vel: W = sqrt(M) xi, rhs[0:n] += sqrt(2kT/dt) W
-        PS_TRY(draw_noise(c, c->xi, n));
+        double* noise = pipe->take(s);
+        PS_HIP(c, hipMemcpyAsync(c->xi, noise, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, st));
         PS_TRY(sqrt_apply_dev(c, c->xi, c->W, nullptr));
         k_add_slip<<<eb(n), TB, 0, st>>>(c->rhs, c->W, std::sqrt(2 * kT / dt), n);
         PS_LAUNCH(c);
@@ -162,7 +218,8 @@ ps_err one_step(Ctx* c) {
         k_sub<<<eb(nu), TB, 0, st>>>(c->v_brown, c->up, c->v_det_n, nu);
         PS_LAUNCH(c);
         // drift(): RFD probes at x +/- eps W
-        PS_TRY(draw_noise(c, c->fext, nu));
+        PS_HIP(c, hipMemcpyAsync(c->fext, noise + n, sizeof(double) * (size_t)nu, hipMemcpyHostToDevice, st));
+        PS_TRY(pipe->release(s));
         k_probe<<<eb(n3), TB, 0, st>>>(c->x, c->fext, eps, n3);
         PS_LAUNCH(c);
         PS_TRY(check_dist_dev(c));
@@ -254,6 +311,8 @@ void destroy_ctx(Ctx* c) {
                     (void*)c->io_a, (void*)c->io_b})
         if (p) (void)hipFree(p);
     if (c->noise_h) (void)hipHostFree(c->noise_h);
+    for (hipEvent_t e : c->noise_ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -432,7 +491,12 @@ ps_err ps_create(const ps_params* p, void** out) {
         c->err = "device allocation failed";
         return bail(PS_ERR_ALLOC);
     }
-    if (hipHostMalloc(&c->noise_h, sizeof(double) * (size_t)(c->n + c->nu), hipHostMallocDefault) != hipSuccess) {
+    if (hipEventCreateWithFlags(&c->noise_ev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->noise_ev[1], hipEventDisableTiming) != hipSuccess) {
+        c->err = "event creation failed";
+        return bail(PS_ERR_HIP);
+    }
+    if (hipHostMalloc(&c->noise_h, sizeof(double) * 2 * (size_t)(c->n + c->nu), hipHostMallocDefault) != hipSuccess) {
         c->err = "pinned allocation failed";
         return bail(PS_ERR_ALLOC);
     }
@@ -662,7 +726,23 @@ ps_err ps_run(void* h, int nsteps) {
     Ctx* c = as_ctx(h);
     if (!c || nsteps < 0) return PS_ERR_ARG;
     if (!c->have_x) return fail(c, PS_ERR_STATE, "run before initial_configuration");
-    for (int i = 0; i < nsteps; i++) PS_TRY(one_step(c));
+    if (nsteps == 0) return PS_OK;
+    std::unique_ptr<NoisePipe> pipe;
+    if (c->p.kT > 0) {
+        pipe.reset(new NoisePipe());
+        pipe->c = c;
+        pipe->len = c->n + c->nu;
+        pipe->nsteps = nsteps;
+        pipe->slot[0] = c->noise_h;
+        pipe->slot[1] = c->noise_h + pipe->len;
+        pipe->done[0] = c->noise_ev[0];
+        pipe->done[1] = c->noise_ev[1];
+        pipe->th = std::thread([p = pipe.get()] { p->produce(); });
+    }
+    ps_err rc = PS_OK;
+    for (int i = 0; i < nsteps && rc == PS_OK; i++) rc = one_step(c, pipe.get(), i);
+    if (pipe) pipe->stop();
+    if (rc != PS_OK) return rc;
     PS_HIP(c, hipStreamSynchronize(c->stream));
     return PS_OK;
 }

@@ -206,7 +206,11 @@ __global__ void k_mr_c(int64_t n, const MinresState* __restrict__ st, MinresStat
                        const double* __restrict__ w2_old, double* __restrict__ w_new, double* __restrict__ x,
                        int maxit, double abstol) {
     __shared__ double sh[4];
-    if (st->done) return;
+    if (st->done) {
+        // the host alternates the read/write slots every iteration: carry the final state over
+        if (blockIdx.x == 0 && threadIdx.x == 0) *st_next = *st;
+        return;
+    }
     const double alfa = sum_partials(red_a, nb, sh);
     const double bb = sum_partials(red_b, nb, sh);
     MinresState s = *st;
@@ -256,9 +260,6 @@ __global__ void k_mr_c(int64_t n, const MinresState* __restrict__ st, MinresStat
     if (blockIdx.x == 0 && threadIdx.x == 0) *st_next = s;
 }
 
-__global__ void k_state_copy(const MinresState* __restrict__ a, MinresState* __restrict__ b) {
-    if (threadIdx.x == 0) *b = *a;
-}
 
 __global__ void k_diag(SysConst s, double* __restrict__ d) {
     const int64_t r = (int64_t)blockIdx.x * VB + threadIdx.x;
@@ -507,9 +508,8 @@ static ps_err minres_core(Ctx* c, int64_t N, const double* dinv, Op&& op, const 
         w1 = w2;
         w2 = w;
         w = wn;
-        // the next iteration reads sB: copy it into sA so kernels keep a fixed read slot
-        k_state_copy<<<1, 64, 0, st>>>(sB, sA);
-        PS_LAUNCH(c);
+        // the next iteration reads the state k_mr_c just wrote
+        std::swap(sA, sB);
         it++;
         if (it % poll == 0 || it >= maxit) {
             PS_HIP(c, hipMemcpyAsync(c->mstate_h, sA, sizeof(MinresState), hipMemcpyDeviceToHost, st));

@@ -156,11 +156,14 @@ __global__ __launch_bounds__(PS_BLOCK) void k_aa(SysConst s, const T* __restrict
 template <class T, bool BOX>
 __global__ __launch_bounds__(PS_BLOCK) void k_ab_mono(SysConst s, const T* __restrict__ xm,
                                                       const double* __restrict__ f,
-                                                      const double* __restrict__ stale,
+                                                      const double* __restrict__ stale, int per_cchunk,
                                                       T* __restrict__ part) {
     __shared__ T cx[PS_BLOCK], cy[PS_BLOCK], cz[PS_BLOCK];
     __shared__ T cv[11][PS_BLOCK];
-    const int Nm = s.Nm, Nc = s.Nc;
+    const int Nm = s.Nm;
+    // colloid sources [cb, ce) of this block's chunk (blockIdx.y)
+    const int cb = blockIdx.y * per_cchunk;
+    const int ce = min(s.Nc, cb + per_cchunk);
     const int i = blockIdx.x * PS_BLOCK + threadIdx.x;
     const bool act = i < Nm;
     T xi = 0, yi = 0, zi = 0;
@@ -173,10 +176,10 @@ __global__ __launch_bounds__(PS_BLOCK) void k_ab_mono(SysConst s, const T* __res
     const T gB = (T)(C9D10 * (C1D2 + C5D6 * s.beta2));
     T ux = 0, uy = 0, uz = 0, Fx = 0, Fy = 0, Fz = 0;
     const int64_t nm3 = s.nm3, nc3 = s.nc3, nu = s.nu;
-    for (int c0 = 0; c0 < Nc; c0 += PS_BLOCK) {
+    for (int c0 = cb; c0 < ce; c0 += PS_BLOCK) {
         __syncthreads();
         const int c = c0 + threadIdx.x;
-        if (c < Nc) {
+        if (c < ce) {
             const int g = Nm + c;
             cx[threadIdx.x] = xm[3 * g];
             cy[threadIdx.x] = xm[3 * g + 1];
@@ -186,7 +189,7 @@ __global__ __launch_bounds__(PS_BLOCK) void k_ab_mono(SysConst s, const T* __res
             for (int k = 0; k < 5; k++) cv[6 + k][threadIdx.x] = (T)f[nu + 5 * c + k];
         }
         __syncthreads();
-        const int cnt = min(PS_BLOCK, Nc - c0);
+        const int cnt = min(PS_BLOCK, ce - c0);
         for (int k = 0; k < cnt; k++) {
             T dx = xi - cx[k], dy = yi - cy[k], dz = zi - cz[k];
             if (BOX) min_img<T>(s, dx, dy, dz);
@@ -219,9 +222,9 @@ __global__ __launch_bounds__(PS_BLOCK) void k_ab_mono(SysConst s, const T* __res
         ux += (T)stale[0] * Fx + (T)stale[1] * Fy + (T)stale[2] * Fz;
         uy += (T)stale[3] * Fx + (T)stale[4] * Fy + (T)stale[5] * Fz;
         uz += (T)stale[6] * Fx + (T)stale[7] * Fy + (T)stale[8] * Fz;
-        part[0 * (int64_t)Nm + i] = ux;
-        part[1 * (int64_t)Nm + i] = uy;
-        part[2 * (int64_t)Nm + i] = uz;
+        part[((int64_t)blockIdx.y * 3 + 0) * Nm + i] = ux;
+        part[((int64_t)blockIdx.y * 3 + 1) * Nm + i] = uy;
+        part[((int64_t)blockIdx.y * 3 + 2) * Nm + i] = uz;
     }
 }
 
@@ -481,7 +484,7 @@ __global__ __launch_bounds__(PS_BLOCK) void k_bb(SysConst s, const double* __res
 template <class T>
 __global__ __launch_bounds__(PS_BLOCK) void k_finalize(SysConst s, const double* __restrict__ f,
                                                        double* __restrict__ u, const T* __restrict__ part_aa,
-                                                       int nsplit, const T* __restrict__ part_abm,
+                                                       int nsplit, const T* __restrict__ part_abm, int nchunk_m,
                                                        const T* __restrict__ part_c,
                                                        const T* __restrict__ part_fs, int nchunk,
                                                        const double* __restrict__ bb,
@@ -496,7 +499,7 @@ __global__ __launch_bounds__(PS_BLOCK) void k_finalize(SysConst s, const double*
         const int i = (int)(r / 3), comp = (int)(r - 3 * (int64_t)i);
         double acc = 0.0;
         for (int sp = 0; sp < nsplit; sp++) acc += (double)part_aa[((int64_t)sp * 3 + comp) * Nm + i];
-        acc += (double)part_abm[(int64_t)comp * Nm + i];
+        for (int cm = 0; cm < nchunk_m; cm++) acc += (double)part_abm[((int64_t)cm * 3 + comp) * Nm + i];
         v = s.beta_inv * f[r] + acc;
     } else if (r < nm3 + nc3) {
         const int64_t q = r - nm3;
@@ -640,11 +643,11 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
             k_aa<T, false, false><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
         PS_LAUNCH(c);
         if (c->prof_on) PS_HIP(c, hipEventRecord(ek1, st));
-        dim3 g2((s.Nm + PS_BLOCK - 1) / PS_BLOCK);
+        dim3 g2((s.Nm + PS_BLOCK - 1) / PS_BLOCK, aw.nchunk_m);
         if (box)
-            k_ab_mono<T, true><<<g2, PS_BLOCK, 0, st>>>(s, xm, f, c->stale, (T*)aw.part_abm);
+            k_ab_mono<T, true><<<g2, PS_BLOCK, 0, st>>>(s, xm, f, c->stale, aw.per_chunk_m, (T*)aw.part_abm);
         else
-            k_ab_mono<T, false><<<g2, PS_BLOCK, 0, st>>>(s, xm, f, c->stale, (T*)aw.part_abm);
+            k_ab_mono<T, false><<<g2, PS_BLOCK, 0, st>>>(s, xm, f, c->stale, aw.per_chunk_m, (T*)aw.part_abm);
         PS_LAUNCH(c);
     }
     if (s.Nc > 0) {
@@ -663,7 +666,8 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
     if (s.n - r_begin <= 0) return PS_OK;
     dim3 g4((unsigned)((s.n - r_begin + PS_BLOCK - 1) / PS_BLOCK));
     k_finalize<T><<<g4, PS_BLOCK, 0, st>>>(s, f, u, (const T*)aw.part_aa, s.Nm > 0 ? aw.nsplit_aa : 0,
-                                           (const T*)aw.part_abm, (const T*)aw.part_c, (const T*)aw.part_fs,
+                                           (const T*)aw.part_abm, s.Nm > 0 ? aw.nchunk_m : 0,
+                                           (const T*)aw.part_c, (const T*)aw.part_fs,
                                            (s.Nm > 0 && s.Nc > 0) ? aw.nchunk_c : 0, aw.bb, c->stale,
                                            srows ? 0 : saddle, r_begin);
     PS_LAUNCH(c);
@@ -689,7 +693,11 @@ ps_err alloc_work(Ctx* c, ApplyWork& aw, size_t elt) {
     aw.per_chunk_c = s.Nm > 0 ? (s.Nm + nchunk - 1) / nchunk : 0;
     aw.nchunk_c = s.Nm > 0 ? (s.Nm + aw.per_chunk_c - 1) / aw.per_chunk_c : 1;
     aw.bytes_aa = (size_t)aw.nsplit_aa * 3 * std::max(s.Nm, 1) * elt;
-    aw.bytes_abm = (size_t)3 * std::max(s.Nm, 1) * elt;
+    // monomer rows from colloid sources: chunks of >= 32 colloids on blockIdx.y
+    int nchm = std::max(1, std::min((s.Nc + 31) / 32, ntb > 0 ? (target + ntb - 1) / ntb : 1));
+    aw.per_chunk_m = s.Nc > 0 ? (s.Nc + nchm - 1) / nchm : 1;
+    aw.nchunk_m = s.Nc > 0 ? (s.Nc + aw.per_chunk_m - 1) / aw.per_chunk_m : 1;
+    aw.bytes_abm = (size_t)aw.nchunk_m * 3 * std::max(s.Nm, 1) * elt;
     aw.bytes_c = (size_t)aw.nchunk_c * std::max(s.Nc, 1) * 12 * elt;
     aw.bytes_fs = (size_t)aw.nchunk_c * 3 * elt;
     PS_HIP(c, hipMalloc(&aw.part_aa, aw.bytes_aa));

@@ -53,10 +53,10 @@ struct MinresState {
 
 // Apply workspace (partial sums) sized at create time.
 struct ApplyWork {
-    int nsplit_aa = 1, nchunk_c = 1;
-    int per_split_aa = 0, per_chunk_c = 0;
+    int nsplit_aa = 1, nchunk_c = 1, nchunk_m = 1;
+    int per_split_aa = 0, per_chunk_c = 0, per_chunk_m = 0;
     void* part_aa = nullptr;     // [nsplit][3][Nm]  (T)
-    void* part_abm = nullptr;    // [3][Nm]          (T)
+    void* part_abm = nullptr;    // [nchunk_m][3][Nm] (T), colloid-source chunks
     void* part_c = nullptr;      // [nchunk][Nc][12] (T) (11 rows + monomer-force sum slot reuse)
     void* part_fs = nullptr;     // [nchunk][3]      (T) sum of monomer forces per chunk
     double* bb = nullptr;        // [Nc][11]
@@ -129,7 +129,8 @@ struct Ctx {
     // ---- host RNG (std::mt19937_64 + normal_distribution, src/init.cpp:105-110)
     std::mt19937_64 rng;
     std::normal_distribution<double> nd{0.0, 1.0};
-    double* noise_h = nullptr;      // pinned, n + nu
+    double* noise_h = nullptr;      // pinned, 2 slots of n + nu (ps_run's noise producer)
+    hipEvent_t noise_ev[2] = {nullptr, nullptr};
     // ---- event timing (ps_profile_begin/end)
     bool prof_on = false;
     std::vector<hipEvent_t> ev_pool;
