@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <vector>
 #include <random>
 #include <algorithm>
@@ -643,14 +644,14 @@ inline double dot(const double* a, const double* b, int64_t n) {
 }
 
 // Preconditioned MINRES (Paige & Saunders 1975; the algorithm of PETSc KSPMINRES), Jacobi
-// preconditioner, ||b||_{P^-1}-relative stopping test (src/init.cpp:40-64, src/vel.cpp:41-51).
-int minres(Sim& s, const double* b, double* x, bool warm, ps_solve_info* info) {
-    int64_t N = s.ns;
-    std::vector<double> dinv(N), r1(N), r2(N), y(N), v(N), w(N, 0.0), w1(N, 0.0), w2(N, 0.0);
-    saddle_diag(s, dinv.data());
-    for (auto& d : dinv) d = 1.0 / d;
+// preconditioner, ||b||_{P^-1}-relative stopping test (src/init.cpp:40-64, src/vel.cpp:41-51), for a
+// symmetric operator `op` of size N with inverse diagonal `dinv`.
+int minres_op(int64_t N, const std::function<void(const double*, double*)>& op, const double* dinv_in,
+              const double* b, double* x, bool warm, double rtol, double abstol, int maxit, ps_solve_info* info) {
+    std::vector<double> dinv(dinv_in, dinv_in + N);
+    std::vector<double> r1(N), r2(N), y(N), v(N), w(N, 0.0), w1(N, 0.0), w2(N, 0.0);
     if (warm) {
-        saddle_apply(s, x, y.data());
+        op(x, y.data());
         for (int64_t i = 0; i < N; i++) r1[i] = b[i] - y[i];
     } else {
         for (int64_t i = 0; i < N; i++) {
@@ -670,17 +671,17 @@ int minres(Sim& s, const double* b, double* x, bool warm, ps_solve_info* info) {
     double bnorm = 0.0;
     for (int64_t i = 0; i < N; i++) bnorm += b[i] * dinv[i] * b[i];
     bnorm = std::sqrt(bnorm);
-    double tol = std::max(s.p.ksp_rtol * bnorm, s.p.ksp_abstol);
+    double tol = std::max(rtol * bnorm, abstol);
     r2 = r1;
     double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, phibar = beta1;
     double cs = -1.0, sn = 0.0;
     int its = 0, reason = -3;
     if (phibar <= tol) reason = 2;
-    while (reason < 0 && its < s.p.ksp_maxit) {
+    while (reason < 0 && its < maxit) {
         its++;
         double sc = 1.0 / beta;
         for (int64_t i = 0; i < N; i++) v[i] = sc * y[i];
-        saddle_apply(s, v.data(), y.data());
+        op(v.data(), y.data());
         if (its >= 2) {
             double f = beta / oldb;
             for (int64_t i = 0; i < N; i++) y[i] -= f * r1[i];
@@ -712,11 +713,9 @@ int minres(Sim& s, const double* b, double* x, bool warm, ps_solve_info* info) {
             w[i] = (v[i] - oldeps * w1[i] - delta * w2[i]) * denom;
             x[i] += phi * w[i];
         }
-        if (phibar <= tol) reason = (phibar <= s.p.ksp_abstol) ? 3 : 2;
+        if (phibar <= tol) reason = (phibar <= abstol) ? 3 : 2;
         if (beta == 0.0 && reason < 0) reason = 2;
     }
-    s.last_minres_its = its;
-    s.minres_total += its;
     if (info) {
         info->iterations = its;
         info->reason = reason;
@@ -724,6 +723,130 @@ int minres(Sim& s, const double* b, double* x, bool warm, ps_solve_info* info) {
         info->rnorm0 = beta1;
     }
     return reason > 0 ? 0 : PS_ERR_NOCONV;
+}
+
+// MINRES on the saddle system (the reference's only solver).
+int minres(Sim& s, const double* b, double* x, bool warm, ps_solve_info* info) {
+    std::vector<double> dinv(s.ns);
+    saddle_diag(s, dinv.data());
+    for (auto& d : dinv) d = 1.0 / d;
+    auto op = [&s](const double* in, double* out) { saddle_apply(s, in, out); };
+    ps_solve_info ii{};
+    int rc = minres_op(s.ns, op, dinv.data(), b, x, warm, s.p.ksp_rtol, s.p.ksp_abstol, s.p.ksp_maxit, &ii);
+    s.last_minres_its = ii.iterations;
+    s.minres_total += ii.iterations;
+    if (info) *info = ii;
+    return rc;
+}
+
+// Stresslet rows of y = M x: y[nu + k] for k < 5 Nc (the rows the block elimination needs).
+void mob_apply_stresslet_rows(Sim& s, const double* x, double* ys) {
+    const int64_t n = s.n, nu = s.nu, n5 = n - nu;
+    std::fill(ys, ys + n5, 0.0);
+    if (n5 == 0) return;
+    if (s.dense_mode) {
+        const double* M = s.Mdense.data();
+#pragma omp parallel for schedule(static)
+        for (int64_t k = 0; k < n5; k++) {
+            const double* row = M + (nu + k) * n;
+            double acc = 0.0;
+            for (int64_t c = 0; c < n; c++) acc += row[c] * x[c];
+            ys[k] = acc;
+        }
+        return;
+    }
+    // Matrix-free: only the entries of stresslet rows (self, AB and BB emitters; AA rows are
+    // monomer rows).  Each thread accumulates a private copy, reduced in thread order.
+    int nth = 1;
+#ifdef _OPENMP
+    nth = omp_get_max_threads();
+#endif
+    std::vector<std::vector<double>> part(nth, std::vector<double>(n5, 0.0));
+#pragma omp parallel
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double* yp = part[tid].data();
+        auto acc = [yp, x, nu](int64_t r, int64_t c, double v) {
+            if (r >= nu) yp[r - nu] += v * x[c];
+        };
+        if (tid == 0) emit_self(s, acc);
+#pragma omp for schedule(dynamic, 4)
+        for (int m = 0; m < s.Nm; m++) emit_AB_row(s, m, acc);
+#pragma omp for schedule(dynamic, 1)
+        for (int c1 = s.Nm; c1 < s.Np; c1++)
+            for (int c2 = c1 + 1; c2 < s.Np; c2++) emit_BB_pair(s, c1, c2, acc);
+    }
+    for (int64_t k = 0; k < n5; k++) {
+        double a = 0.0;
+        for (int t = 0; t < nth; t++) a += part[t][k];
+        ys[k] = a;
+    }
+}
+
+// Dense stresslet block M_ss (5Nc x 5Nc, row-major): rows and columns nu..n-1 of M.
+void mss_dense(const Sim& s, std::vector<double>& Mss) {
+    const int64_t nu = s.nu, n5 = s.n - s.nu;
+    Mss.assign((size_t)(n5 * n5), 0.0);
+    double* A = Mss.data();
+    auto ins = [A, nu, n5](int64_t r, int64_t c, double v) {
+        if (r >= nu && c >= nu) A[(r - nu) * n5 + (c - nu)] = v;
+    };
+    emit_self(s, ins);
+    for (int c1 = s.Nm; c1 < s.Np; c1++)
+        for (int c2 = c1 + 1; c2 < s.Np; c2++) emit_BB_pair(s, c1, c2, ins);
+}
+
+// Exact block elimination of the saddle system [[M, B], [B^T, 0]] [f; u] = [b_f; b_l] with
+// B = [-I_nu; 0] (the constraint rows of src/arrays.cpp:349-408 pin f_u):
+//   f_u = -b_l;  M_ss f_s = b_s - (M [f_u; 0])_s  (Jacobi MINRES on the dense M_ss, rtol 1e-12);
+//   u = (M f)_u - b_u.
+// It solves the same linear system as the reference's MINRES; the velocities agree to the
+// solvers' tolerances, not bit for bit.
+int block_solve(Sim& s, const double* b, double* x, ps_solve_info* info) {
+    const int64_t n = s.n, nu = s.nu, n5 = n - nu;
+    for (int64_t i = 0; i < n; i++) x[i] = (i < nu) ? -b[n + i] : 0.0;
+    int its = 0, rc = 0;
+    if (n5 > 0) {
+        std::vector<double> ys(n5), rs(n5), Mss, dinv(n5);
+        mob_apply_stresslet_rows(s, x, ys.data());
+        for (int64_t k = 0; k < n5; k++) rs[k] = b[nu + k] - ys[k];
+        mss_dense(s, Mss);
+        for (int64_t k = 0; k < n5; k++) dinv[k] = 1.0 / Mss[(size_t)(k * n5 + k)];
+        const double* A = Mss.data();
+        auto op = [A, n5](const double* in, double* out) {
+#pragma omp parallel for schedule(static)
+            for (int64_t r = 0; r < n5; r++) {
+                const double* row = A + r * n5;
+                double acc = 0.0;
+                for (int64_t c = 0; c < n5; c++) acc += row[c] * in[c];
+                out[r] = acc;
+            }
+        };
+        ps_solve_info si{};
+        rc = minres_op(n5, op, dinv.data(), rs.data(), x + nu, false, 1e-12, s.p.ksp_abstol, 10000, &si);
+        its = si.iterations;
+    }
+    std::vector<double> y(n);
+    mob_apply(s, x, y.data());
+    for (int64_t i = 0; i < nu; i++) x[n + i] = y[i] - b[i];
+    s.last_minres_its = its;
+    s.minres_total += its;
+    if (info) {
+        *info = ps_solve_info{};
+        info->iterations = its;
+        info->reason = rc == 0 ? 2 : -3;
+        info->true_rel_residual = -1.0;
+    }
+    return rc;
+}
+
+// The saddle solve of the configured solver (PS_SOLVER_MINRES: the reference's; PS_SOLVER_BLOCK).
+int saddle_solve(Sim& s, const double* b, double* x, bool warm, ps_solve_info* info) {
+    if (s.p.solver == PS_SOLVER_BLOCK) return block_solve(s, b, x, info);
+    return minres(s, b, x, warm, info);
 }
 
 // Symmetric tridiagonal eigendecomposition by implicit QL with Wilkinson-type shifts
@@ -1005,7 +1128,7 @@ int run_step(Sim& s) {
     check_dist(s);
     rhs_build(s, false, rhs.data());
     mob(s);
-    rc |= minres(s, rhs.data(), Xdet.data(), false, nullptr);   // solve_deterministic_vel
+    rc |= saddle_solve(s, rhs.data(), Xdet.data(), false, nullptr);   // solve_deterministic_vel
     for (int64_t i = 0; i < nu; i++) s.v_det_n[i] = Xdet[n + i];
     if (s.p.kT > 0) {
         // solve_slip_vel (src/slip_vel.cpp:19-79)
@@ -1015,7 +1138,7 @@ int run_step(Sim& s) {
         double scale = std::sqrt(2 * s.p.kT / s.p.dt);
         for (int64_t i = 0; i < n; i++) rhs[i] += scale * W[i];
         // new_vel (src/vel.cpp:68-94)
-        rc |= minres(s, rhs.data(), X.data(), false, nullptr);
+        rc |= saddle_solve(s, rhs.data(), X.data(), false, nullptr);
         for (int64_t i = 0; i < nu; i++) s.up[i] = X[n + i];
         if (s.p.record_forces)
             for (int64_t i = 0; i < nu; i++) s.fext[i] = X[i];
@@ -1028,18 +1151,18 @@ int run_step(Sim& s) {
         check_dist(s);
         mob(s);
         rhs_build(s, true, rhs.data());
-        rc |= minres(s, rhs.data(), Xd.data(), false, nullptr);
+        rc |= saddle_solve(s, rhs.data(), Xd.data(), false, nullptr);
         for (int64_t i = 0; i < nu; i++) s.drift_v[i] = Xd[n + i];
         for (int i = 0; i < 5 * s.Nc; i++) s.sb[i] = Xd[nu + i];
         for (int64_t i = 0; i < n3; i++) s.x[i] -= 2.0 * eps * s.fext[i];
         check_dist(s);
         mob(s);
         rhs_build(s, true, rhs.data());
-        rc |= minres(s, rhs.data(), Xd.data(), true, nullptr);
+        rc |= saddle_solve(s, rhs.data(), Xd.data(), true, nullptr);
         for (int64_t i = 0; i < nu; i++) s.udiff[i] = Xd[n + i] - s.drift_v[i];
         for (int i = 0; i < 5 * s.Nc; i++) s.sb[i] = (s.sb[i] - Xd[nu + i]) / (2 * eps);
     } else {
-        rc |= minres(s, rhs.data(), X.data(), false, nullptr);
+        rc |= saddle_solve(s, rhs.data(), X.data(), false, nullptr);
         for (int64_t i = 0; i < nu; i++) s.up[i] = X[n + i];
         if (s.p.record_forces)
             for (int64_t i = 0; i < nu; i++) s.fext[i] = X[i];
@@ -1050,7 +1173,7 @@ int run_step(Sim& s) {
     check_dist(s);
     rhs_build(s, false, rhs.data());
     mob(s);
-    rc |= minres(s, rhs.data(), Xdet.data(), false, nullptr);
+    rc |= saddle_solve(s, rhs.data(), Xdet.data(), false, nullptr);
     for (int64_t i = 0; i < nu; i++) s.v_det_c[i] = Xdet[n + i];
     for (int64_t k = 0; k < nu; k++) s.up[k] = 0.5 * (s.v_det_n[k] + s.v_det_c[k]) + s.v_brown[k];
     step_from(s, s.x_n);
@@ -1219,7 +1342,7 @@ int or_saddle_diag(void* h, double* d) {
 int or_saddle_solve(void* h, const double* b, double* x, int warm, ps_solve_info* info) {
     Sim* s = (Sim*)h;
     if (!s->have_mob) return PS_ERR_STATE;
-    return minres(*s, b, x, warm != 0, info);
+    return saddle_solve(*s, b, x, warm != 0, info);
 }
 int or_sqrt_apply(void* h, const double* xi, double* w, ps_sqrt_info* info) {
     Sim* s = (Sim*)h;

@@ -421,6 +421,39 @@ def test_saddle_apply_and_minres():
     np.testing.assert_allclose(xs[: o.nu], -b[o.n:], rtol=1e-9, atol=1e-9)
 
 
+@pytest.mark.parametrize("dense", [1, 0])
+def test_block_elimination_solve_against_direct(dense):
+    """The block-elimination solver (f_u = -b_l, M_ss solve, u from M f) solves the same saddle
+    system: checked against numpy's dense LU in both the dense and the matrix-free oracle modes,
+    on a box dense enough that Jacobi MINRES needs hundreds of iterations."""
+    from oracle.pyoracle import lib
+    o, x0, pd = make(4, 6, 10, 4.5, seed=5, solver=1)
+    lib().or_set_dense(o.h, dense)
+    o.check_dist()
+    o.mob()
+    A = dense_saddle(o)
+    b = o.rhs()
+    b[: o.n] += np.random.default_rng(3).normal(size=o.n)
+    xs, info, rc = o.saddle_solve(b)
+    assert rc == 0 and info.reason == 2
+    assert info.iterations < 5 * 4 * 5     # the M_ss system is small and well conditioned
+    xd = np.linalg.solve(A, b)
+    assert rel(xs, xd) < 1e-10
+    assert rel(A @ xs, b) < 1e-11
+    np.testing.assert_array_equal(xs[: o.nu], -b[o.n:])
+
+
+def test_block_elimination_step_matches_minres_step():
+    o1, x0, _ = make(2, 5, 10, 6.0, seed=8, ksp_rtol=1e-12, sqrt_rtol=1e-10)
+    o2, _, _ = make(2, 5, 10, 6.0, seed=8, ksp_rtol=1e-12, sqrt_rtol=1e-10, solver=1)
+    o1.run(1)
+    o2.run(1)
+    x1, _, _ = o1.state()
+    x2, _, _ = o2.state()
+    assert rel(x2 - x0, x1 - x0) < 1e-9
+    assert rel(o2.vectors()["up"], o1.vectors()["up"]) < 1e-9
+
+
 def test_minres_iteration_cap_reports_failure():
     o, _, _ = make(2, 5, 10, 6.0, seed=8, ksp_rtol=1e-14, ksp_maxit=3)
     o.check_dist()
