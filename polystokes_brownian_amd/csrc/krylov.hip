@@ -278,80 +278,51 @@ __global__ void k_recip(double* __restrict__ d, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------- Lanczos kernels
-// w -= bprev * qprev (if qprev); partial qj.w
-__global__ void k_lz_a(int64_t n, double* __restrict__ w, const double* __restrict__ qprev, double bprev,
-                       const double* __restrict__ qj, double* __restrict__ red) {
+// Classical Gram-Schmidt pass against Q_0..Q_{m-1}.  Dots: block (chunk, q) writes the partial of
+// Q_q . w over its chunk of LZ_CH elements to red[q * nchunk + chunk].
+constexpr int LZ_CH = VB * 8;
+__global__ __launch_bounds__(VB) void k_lz_dots(int64_t n, const double* __restrict__ Q,
+                                                const double* __restrict__ w, double* __restrict__ red) {
     __shared__ double sh[4];
-    const int64_t base = (int64_t)blockIdx.x * VCHUNK;
+    const int q = blockIdx.y;
+    const int64_t base = (int64_t)blockIdx.x * LZ_CH;
+    const double* Qq = Q + (int64_t)q * n;
     double acc = 0.0;
 #pragma unroll
-    for (int k = 0; k < VPT; k++) {
+    for (int k = 0; k < LZ_CH / VB; k++) {
         const int64_t i = base + k * VB + threadIdx.x;
-        if (i < n) {
-            double wi = w[i];
-            if (qprev) wi -= bprev * qprev[i];
-            w[i] = wi;
-            acc += qj[i] * wi;
-        }
+        if (i < n) acc += Qq[i] * w[i];
     }
     acc = block_reduce(acc, sh);
-    if (threadIdx.x == 0) red[blockIdx.x] = acc;
+    if (threadIdx.x == 0) red[(int64_t)q * gridDim.x + blockIdx.x] = acc;
 }
-// alpha = sum(red) -> lz[j]; w -= alpha qj
-__global__ void k_lz_b(int64_t n, double* __restrict__ w, const double* __restrict__ qj,
-                       const double* __restrict__ red, int nb, double* __restrict__ alpha_out) {
-    __shared__ double sh[4];
-    const double a = sum_partials(red, nb, sh);
-    if (blockIdx.x == 0 && threadIdx.x == 0) *alpha_out = a;
-    const int64_t base = (int64_t)blockIdx.x * VCHUNK;
-#pragma unroll
-    for (int k = 0; k < VPT; k++) {
-        const int64_t i = base + k * VB + threadIdx.x;
-        if (i < n) w[i] -= a * qj[i];
-    }
-}
-// partials of Q_i . w for i < m  -> red[i * nb + block]
-__global__ void k_lz_dots(int64_t n, const double* __restrict__ Q, int m, const double* __restrict__ w,
-                          double* __restrict__ red) {
-    __shared__ double sh[4];
-    const int64_t base = (int64_t)blockIdx.x * VCHUNK;
-    double wl[VPT];
-#pragma unroll
-    for (int k = 0; k < VPT; k++) {
-        const int64_t i = base + k * VB + threadIdx.x;
-        wl[k] = (i < n) ? w[i] : 0.0;
-    }
-    for (int q = 0; q < m; q++) {
-        const double* Qi = Q + (int64_t)q * n;
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < VPT; k++) {
-            const int64_t i = base + k * VB + threadIdx.x;
-            if (i < n) acc += Qi[i] * wl[k];
-        }
-        acc = block_reduce(acc, sh);
-        if (threadIdx.x == 0) red[(int64_t)q * gridDim.x + blockIdx.x] = acc;
-    }
-}
-// h_i = sum partials; w -= sum_i h_i Q_i
-__global__ void k_lz_sub(int64_t n, const double* __restrict__ Q, int m, double* __restrict__ w,
-                         const double* __restrict__ red, int nb) {
+// h_q = sum of the nchunk partials of q (fixed order, every block alike); w -= sum_q h_q Q_q.
+// Pass 0 stores h_{m-1} = q_j . (M q_j) as the Lanczos alpha_j; pass 1 leaves per-block partials
+// of |w|^2 in red2.
+__global__ __launch_bounds__(VB) void k_lz_sub(int64_t n, const double* __restrict__ Q, int m,
+                                               double* __restrict__ w, const double* __restrict__ red,
+                                               int nchunk, double* __restrict__ alpha_out,
+                                               double* __restrict__ red2) {
     __shared__ double sh[4];
     __shared__ double h[64];
-    for (int q = 0; q < m; q++) {
-        const double v = sum_partials(red + (int64_t)q * nb, nb, sh);
-        if (threadIdx.x == 0) h[q] = v;
+    if (threadIdx.x < m) {
+        const double* r = red + (int64_t)threadIdx.x * nchunk;
+        double v = 0.0;
+        for (int c = 0; c < nchunk; c++) v += r[c];
+        h[threadIdx.x] = v;
     }
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * VCHUNK;
-#pragma unroll
-    for (int k = 0; k < VPT; k++) {
-        const int64_t i = base + k * VB + threadIdx.x;
-        if (i < n) {
-            double wi = w[i];
-            for (int q = 0; q < m; q++) wi -= h[q] * Q[(int64_t)q * n + i];
-            w[i] = wi;
-        }
+    if (alpha_out && blockIdx.x == 0 && threadIdx.x == 0) *alpha_out = h[m - 1];
+    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
+    double wi = 0.0;
+    if (i < n) {
+        wi = w[i];
+        for (int q = 0; q < m; q++) wi -= h[q] * Q[(int64_t)q * n + i];
+        w[i] = wi;
+    }
+    if (red2) {
+        const double acc = block_reduce(wi * wi, sh);
+        if (threadIdx.x == 0) red2[blockIdx.x] = acc;
     }
 }
 __global__ void k_scale(int64_t n, double* __restrict__ y, const double* __restrict__ x, double a) {
@@ -547,18 +518,204 @@ static ps_err minres_dev(Ctx* c, const double* b, double* x, int warm, double rt
     return r;
 }
 
-// y = M_ss x (dense 5Nc x 5Nc, one wave per row).
-__global__ __launch_bounds__(VB) void k_gemv(int64_t n, const double* __restrict__ A, const double* __restrict__ x,
-                                             double* __restrict__ y) {
+// ---------------------------------------------------------------------------- small dense MINRES
+// The stresslet system M_ss f_s = r (5Nc unknowns, dense, FP64) is solved by the same preconditioned
+// MINRES recurrence with two launches per iteration: a multi-block GEMV and one single-block kernel
+// that does all the vector work and the scalar recurrence (block-wide reductions, fixed order).
+// Vectors: r1, r2, y, v, wa (= w_{k-1}), wb (= w_{k-2}); the state lives in one MinresState.
+constexpr int MS_T = 1024;   // threads of the single-block kernels
+
+__device__ __forceinline__ double block_reduce_1024(double v, double* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+#pragma unroll
+    for (int k = 0; k < MS_T / 64; k++) r += sh[k];
+    return r;
+}
+
+__global__ __launch_bounds__(MS_T) void k_ms_init(int64_t N, const double* __restrict__ b, double* __restrict__ x,
+                                                  const double* __restrict__ dinv, double* __restrict__ r1,
+                                                  double* __restrict__ r2, double* __restrict__ y,
+                                                  double* __restrict__ v, double* __restrict__ wa,
+                                                  double* __restrict__ wb, double rtol, double abstol,
+                                                  MinresState* __restrict__ st) {
+    __shared__ double sh[MS_T / 64];
+    double a1 = 0.0, a2 = 0.0;
+    for (int64_t i = threadIdx.x; i < N; i += MS_T) {
+        const double bi = b[i], zi = dinv[i] * bi;
+        x[i] = 0.0;
+        r1[i] = bi;
+        r2[i] = bi;
+        y[i] = zi;
+        wa[i] = 0.0;
+        wb[i] = 0.0;
+        a1 += bi * zi;
+        a2 += bi * dinv[i] * bi;
+    }
+    const double b1 = block_reduce_1024(a1, sh);
+    const double bn = block_reduce_1024(a2, sh);
+    MinresState s{};
+    s.beta1 = b1 > 0.0 ? sqrt(b1) : 0.0;
+    s.tol = fmax(rtol * sqrt(fmax(bn, 0.0)), abstol);
+    s.beta = s.beta1;
+    s.phibar = s.beta1;
+    s.cs = -1.0;
+    if (b1 <= 0.0) {
+        s.done = 1;
+        s.reason = 3;
+    } else if (s.phibar <= s.tol) {
+        s.done = 1;
+        s.reason = 2;
+    }
+    if (!s.done) {
+        const double sc = 1.0 / s.beta;
+        for (int64_t i = threadIdx.x; i < N; i += MS_T) v[i] = sc * y[i];
+    }
+    if (threadIdx.x == 0) *st = s;
+}
+
+// y = A v, one wave per row, four independent accumulators per lane.
+__global__ __launch_bounds__(VB) void k_ms_gemv(int64_t N, const double* __restrict__ A, const double* __restrict__ v,
+                                                double* __restrict__ y, const MinresState* __restrict__ st) {
+    if (st->done) return;
     const int64_t row = (int64_t)blockIdx.x * (VB / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (row >= n) return;
-    const double* a = A + row * n;
-    double acc = 0.0;
-    for (int64_t j = lane; j < n; j += 64) acc += a[j] * x[j];
-    acc = wave_sum(acc);
+    if (row >= N) return;
+    const double* a = A + row * N;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int64_t j = lane;
+    for (; j + 192 < N; j += 256) {
+        s0 += a[j] * v[j];
+        s1 += a[j + 64] * v[j + 64];
+        s2 += a[j + 128] * v[j + 128];
+        s3 += a[j + 192] * v[j + 192];
+    }
+    for (; j < N; j += 64) s0 += a[j] * v[j];
+    const double acc = wave_sum((s0 + s1) + (s2 + s3));
     if (lane == 0) y[row] = acc;
 }
+
+// One MINRES iteration after y = A v (Paige & Saunders; the same recurrence as k_mr_a/b/c).
+__global__ __launch_bounds__(MS_T) void k_ms_update(int64_t N, const double* __restrict__ dinv,
+                                                    double* __restrict__ r1, double* __restrict__ r2,
+                                                    double* __restrict__ y, double* __restrict__ v,
+                                                    double* __restrict__ wa, double* __restrict__ wb,
+                                                    double* __restrict__ x, int maxit, double abstol,
+                                                    MinresState* __restrict__ st) {
+    __shared__ double sh[MS_T / 64];
+    MinresState s = *st;
+    if (s.done) return;
+    // y -= (beta/oldb) r1 (from the 2nd iteration); alfa = v . y
+    const bool sub = s.its >= 1;
+    const double f = sub ? s.beta / s.oldb : 0.0;
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < N; i += MS_T) {
+        double yi = y[i];
+        if (sub) yi -= f * r1[i];
+        y[i] = yi;
+        acc += v[i] * yi;
+    }
+    const double alfa = block_reduce_1024(acc, sh);
+    // y -= (alfa/beta) r2; r1 <- r2; r2 <- y; z = dinv y (kept in y); beta_new^2 = r2 . z
+    const double f2 = alfa / s.beta;
+    acc = 0.0;
+    for (int64_t i = threadIdx.x; i < N; i += MS_T) {
+        const double yi = y[i] - f2 * r2[i];
+        const double zi = dinv[i] * yi;
+        r1[i] = r2[i];
+        r2[i] = yi;
+        y[i] = zi;
+        acc += yi * zi;
+    }
+    const double bb = block_reduce_1024(acc, sh);
+    s.alfa = alfa;
+    s.oldb = s.beta;
+    s.beta = sqrt(fmax(bb, 0.0));
+    const double oldeps = s.epsln;
+    const double delta = s.cs * s.dbar + s.sn * alfa;
+    const double gbar = s.sn * s.dbar - s.cs * alfa;
+    s.epsln = s.sn * s.beta;
+    s.dbar = -s.cs * s.beta;
+    const double gamma = fmax(sqrt(gbar * gbar + s.beta * s.beta), 1e-300);
+    s.cs = gbar / gamma;
+    s.sn = s.beta / gamma;
+    const double phi = s.cs * s.phibar;
+    s.phibar = s.sn * s.phibar;
+    s.oldeps = oldeps;
+    s.delta = delta;
+    s.gamma = gamma;
+    s.phi = phi;
+    s.its += 1;
+    if (s.phibar <= s.tol) {
+        s.done = 1;
+        s.reason = (s.phibar <= abstol) ? 3 : 2;
+    } else if (s.beta == 0.0) {
+        s.done = 1;
+        s.reason = 2;
+    } else if (s.its >= maxit) {
+        s.done = 1;
+        s.reason = -3;
+    }
+    // w_k = (v - oldeps w_{k-2} - delta w_{k-1}) / gamma; x += phi w_k; v <- z / beta_new
+    const double denom = 1.0 / gamma;
+    const double vs = (s.beta > 0.0) ? 1.0 / s.beta : 0.0;
+    for (int64_t i = threadIdx.x; i < N; i += MS_T) {
+        const double w1 = wa[i];
+        const double wn = (v[i] - oldeps * wb[i] - delta * w1) * denom;
+        wb[i] = w1;
+        wa[i] = wn;
+        x[i] += phi * wn;
+        v[i] = vs * y[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *st = s;
+}
+
+// Solve A x = b for the small dense SPD-like block A (N x N, row-major) by the recurrence above,
+// polling the state every MS_POLL iterations (the kernels are no-ops once it has converged).
+static ps_err minres_small(Ctx* c, int64_t N, const double* A, const double* dinv, const double* b, double* x,
+                           double rtol, int maxit, ps_solve_info* info) {
+    hipStream_t st = c->stream;
+    double* r1 = c->kr;
+    double* r2 = c->kr + N;
+    double* y = c->kr + 2 * N;
+    double* v = c->kr + 3 * N;
+    double* wa = c->kr + 4 * N;
+    double* wb = c->kr + 5 * N;
+    MinresState* sd = c->mstate;
+    k_ms_init<<<1, MS_T, 0, st>>>(N, b, x, dinv, r1, r2, y, v, wa, wb, rtol, c->p.ksp_abstol, sd);
+    PS_LAUNCH(c);
+    const int MS_POLL = 8;
+    const unsigned gg = (unsigned)((N + 3) / 4);
+    int it = 0;
+    for (;;) {
+        PS_HIP(c, hipMemcpyAsync(c->mstate_h, sd, sizeof(MinresState), hipMemcpyDeviceToHost, st));
+        PS_HIP(c, hipStreamSynchronize(st));
+        if (c->mstate_h[0].done || it >= maxit) break;
+        for (int k = 0; k < MS_POLL && it < maxit; k++, it++) {
+            k_ms_gemv<<<gg, VB, 0, st>>>(N, A, v, y, sd);
+            PS_LAUNCH(c);
+            k_ms_update<<<1, MS_T, 0, st>>>(N, dinv, r1, r2, y, v, wa, wb, x, maxit, c->p.ksp_abstol, sd);
+            PS_LAUNCH(c);
+        }
+    }
+    const MinresState& s = c->mstate_h[0];
+    const int reason = s.done ? s.reason : -3;
+    if (info) {
+        *info = ps_solve_info{};
+        info->iterations = s.its;
+        info->reason = reason;
+        info->rnorm = s.phibar;
+        info->rnorm0 = s.beta1;
+        info->true_rel_residual = -1.0;
+    }
+    return reason > 0 ? PS_OK : PS_ERR_NOCONV;
+}
+
 __global__ void k_mss_dinv(int64_t n, const double* __restrict__ A, double* __restrict__ d) {
     const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
     if (i < n) d[i] = 1.0 / A[i * n + i];
@@ -594,13 +751,8 @@ static ps_err block_solve_once(Ctx* c, const double* b, double* x, int prec, int
         PS_TRY(mob_apply_stresslet_rows(c, x, c->bs_y));
         k_bs_rs<<<eblocks(n5), VB, 0, st>>>(c->bs_r, b, c->bs_y, nu, n5);
         PS_LAUNCH(c);
-        auto op = [c, n5](const double* in, double* out) -> ps_err {
-            k_gemv<<<(unsigned)((n5 + 3) / 4), VB, 0, c->stream>>>(n5, c->Mss, in, out);
-            PS_LAUNCH(c);
-            return PS_OK;
-        };
         ps_solve_info si{};
-        ps_err r = minres_core(c, n5, c->mss_dinv, op, c->bs_r, x + nu, 0, 1e-12, 10000, &si);
+        ps_err r = minres_small(c, n5, c->Mss, c->mss_dinv, c->bs_r, x + nu, 1e-12, 10000, &si);
         if (r != PS_OK && r != PS_ERR_NOCONV) return r;
         if (mss_its) *mss_its += si.iterations;
     }
@@ -688,39 +840,27 @@ ps_err saddle_solve_dev(Ctx* c, const double* b, double* x, int warm, ps_solve_i
     return rc;
 }
 
-// q_{j+1} = w / sqrt(norm2) with the norm read on the device (no host round trip per iteration)
-__global__ void k_lz_next(int64_t n, double* __restrict__ q, const double* __restrict__ w,
-                          const double* __restrict__ norm2) {
-    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
-    if (i < n) q[i] = w[i] * (1.0 / sqrt(fmax(*norm2, 0.0)));
-}
-__global__ void k_lz_a_dev(int64_t n, double* __restrict__ w, const double* __restrict__ qprev,
-                           const double* __restrict__ bprev_norm2, const double* __restrict__ qj,
-                           double* __restrict__ red) {
+// beta_j^2 = sum of the |w|^2 partials (every block alike; block 0 stores it); q_{j+1} = w / beta_j
+// when q is given.  No host round trip per iteration.
+__global__ __launch_bounds__(VB) void k_lz_next(int64_t n, double* __restrict__ q, const double* __restrict__ w,
+                                                const double* __restrict__ red2, int nb2,
+                                                double* __restrict__ norm2_out) {
     __shared__ double sh[4];
-    const double bprev = qprev ? sqrt(fmax(*bprev_norm2, 0.0)) : 0.0;
-    const int64_t base = (int64_t)blockIdx.x * VCHUNK;
-    double acc = 0.0;
-#pragma unroll
-    for (int k = 0; k < VPT; k++) {
-        const int64_t i = base + k * VB + threadIdx.x;
-        if (i < n) {
-            double wi = w[i];
-            if (qprev) wi -= bprev * qprev[i];
-            w[i] = wi;
-            acc += qj[i] * wi;
-        }
-    }
-    acc = block_reduce(acc, sh);
-    if (threadIdx.x == 0) red[blockIdx.x] = acc;
+    const double b2 = sum_partials(red2, nb2, sh);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *norm2_out = b2;
+    if (!q) return;
+    const double sc = 1.0 / sqrt(fmax(b2, 0.0));
+    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
+    if (i < n) q[i] = w[i] * sc;
 }
-
 // w = M^{1/2} xi.  The device runs the Lanczos recurrence ahead; the host reads alpha/beta every
 // LZ_POLL iterations and finds the first m where the stopping test holds (vectors built past it are
 // never used, so the result equals stopping at m).
 ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* out, ps_sqrt_info* info) {
     const int64_t n = c->n;
-    const int nb = nblocks(n);
+    const unsigned nch = (unsigned)((n + LZ_CH - 1) / LZ_CH);
+    const unsigned ne = eblocks(n);
+    double* red2 = c->red + (size_t)nch * 64;   // after the dot partials of up to 64 vectors
     hipStream_t st = c->stream;
     double b2 = 0.0;
     PS_TRY(vec_dot(c, xi, xi, n, &b2));
@@ -748,27 +888,20 @@ ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* out, ps_sqrt_info* info)
     while (!stop && j0 < kmax) {
         const int j1 = std::min(kmax, j0 + LZ_POLL);   // iterations [j0, j1) on the device
         for (int j = j0; j < j1; j++) {
+            // w = M q_j; two classical Gram-Schmidt passes against q_0..q_j (the first one also
+            // yields alpha_j = q_j . M q_j); beta_j = |w|; q_{j+1} = w / beta_j.
             const double* qj = c->Q + (int64_t)j * n;
             PS_TRY(mob_apply(c, qj, w, prec));
-            k_lz_a_dev<<<nb, VB, 0, st>>>(n, w, j > 0 ? c->Q + (int64_t)(j - 1) * n : nullptr,
-                                          c->lz + 64 + (j > 0 ? j - 1 : 0), qj, c->red);
-            PS_LAUNCH(c);
-            k_lz_b<<<nb, VB, 0, st>>>(n, w, qj, c->red, nb, c->lz + j);
-            PS_LAUNCH(c);
             for (int pass = 0; pass < 2; pass++) {
-                k_lz_dots<<<nb, VB, 0, st>>>(n, c->Q, j + 1, w, c->red);
+                k_lz_dots<<<dim3(nch, j + 1), VB, 0, st>>>(n, c->Q, w, c->red);
                 PS_LAUNCH(c);
-                k_lz_sub<<<nb, VB, 0, st>>>(n, c->Q, j + 1, w, c->red, nb);
-                PS_LAUNCH(c);
-            }
-            k_dot_part<<<nb, VB, 0, st>>>(w, w, n, c->red);
-            PS_LAUNCH(c);
-            k_sum_to<<<1, VB, 0, st>>>(c->red, nb, c->lz + 64 + j);
-            PS_LAUNCH(c);
-            if (j + 1 < kmax) {
-                k_lz_next<<<eblocks(n), VB, 0, st>>>(n, c->Q + (int64_t)(j + 1) * n, w, c->lz + 64 + j);
+                k_lz_sub<<<ne, VB, 0, st>>>(n, c->Q, j + 1, w, c->red, nch, pass == 0 ? c->lz + j : nullptr,
+                                            pass == 1 ? red2 : nullptr);
                 PS_LAUNCH(c);
             }
+            k_lz_next<<<ne, VB, 0, st>>>(n, j + 1 < kmax ? c->Q + (int64_t)(j + 1) * n : nullptr, w, red2,
+                                         (int)ne, c->lz + 64 + j);
+            PS_LAUNCH(c);
         }
         PS_HIP(c, hipMemcpyAsync(c->lz_h, c->lz, sizeof(double) * 128, hipMemcpyDeviceToHost, st));
         PS_HIP(c, hipStreamSynchronize(st));

@@ -48,13 +48,6 @@ __device__ __forceinline__ void geom(const SysConst& s, const double* xs, int k,
     dr = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
 }
 
-__device__ __forceinline__ bool within(const PairConst& pc, const double* xs, int p, int q) {
-    const int k = min(p, q), j = max(p, q);
-    double dx, dy, dz, dr;
-    geom(pc.s, xs, k, j, dx, dy, dz, dr);
-    return dr < pc.rverl[ptype(pc.s.Nm, k, j)];
-}
-
 __device__ __forceinline__ int cell_coord(const PairConst& pc, double x, int d) {
     // wrap into [org, org + L) for binning only; the pair test uses the raw positions
     const double L = pc.s.L[d];
@@ -76,93 +69,157 @@ __global__ void k_bin(PairConst pc, const double* __restrict__ xs, int* __restri
     atomicAdd(&cell_count[cell], 1);
 }
 
-__global__ void k_scatter(int Np, const int* __restrict__ cell_of, const int* __restrict__ cell_start,
-                          int* __restrict__ cursor, int* __restrict__ perm) {
+// Scatter particles into cell order: perm[slot] = p and a cell-ordered copy of the positions, so a
+// cell's candidates are read contiguously.  The in-cell order depends on atomics; rows are sorted by
+// partner afterwards, so the pair lists do not.
+__global__ void k_scatter(int Np, const double* __restrict__ xs, const int* __restrict__ cell_of,
+                          const int* __restrict__ cell_start, int* __restrict__ cursor, int* __restrict__ perm,
+                          double* __restrict__ xsort) {
     const int p = blockIdx.x * TB + threadIdx.x;
     if (p >= Np) return;
     const int c = cell_of[p];
-    perm[cell_start[c] + atomicAdd(&cursor[c], 1)] = p;
+    const int slot = cell_start[c] + atomicAdd(&cursor[c], 1);
+    perm[slot] = p;
+    xsort[3 * slot] = xs[3 * p];
+    xsort[3 * slot + 1] = xs[3 * p + 1];
+    xsort[3 * slot + 2] = xs[3 * p + 2];
 }
 
-// Single-block exclusive scan of n ints: out[0..n] (out[n] = total).
+// Single-block exclusive scan of n ints: out[0..n] with out[n] = total, and out[n + 1] = max(in).
+// Each of the 1024 threads scans a contiguous chunk, so there is one block-wide scan in total.
 __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ in, int* __restrict__ out, int n) {
     __shared__ int sh[1024];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
+    __shared__ int shm[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b = min(t * per, n), e = min(b + per, n);
+    int s = 0, mx = 0;
+    for (int i = b; i < e; i++) {
+        const int v = in[i];
+        s += v;
+        mx = max(mx, v);
+    }
+    sh[t] = s;
+    shm[t] = mx;
     __syncthreads();
-    for (int base = 0; base < n; base += 1024) {
-        const int i = base + threadIdx.x;
-        const int v = (i < n) ? in[i] : 0;
-        sh[threadIdx.x] = v;
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int a = (t >= off) ? sh[t - off] : 0;
+        const int m = (t >= off) ? shm[t - off] : 0;
         __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            const int t = (threadIdx.x >= off) ? sh[threadIdx.x - off] : 0;
-            __syncthreads();
-            sh[threadIdx.x] += t;
-            __syncthreads();
-        }
-        if (i < n) out[i] = carry + sh[threadIdx.x] - v;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry += sh[1023];
+        sh[t] += a;
+        shm[t] = max(shm[t], m);
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[n] = carry;
+    int run = sh[t] - s;
+    for (int i = b; i < e; i++) {
+        const int v = in[i];
+        out[i] = run;
+        run += v;
+    }
+    if (t == 1023) {
+        out[n] = sh[1023];
+        out[n + 1] = shm[1023];
+    }
 }
 
+// A group of GL lanes serves one particle: in cell mode lane l < 27 walks stencil cell l (the 27
+// cells are distinct when every axis has >= 3 cells); otherwise the lanes stride over all particles.
+constexpr int GL = 32;
+
 template <class F>
-__device__ __forceinline__ void for_candidates(const PairConst& pc, const double* xs, const int* cell_of,
-                                               const int* cell_start, const int* perm, int p, F&& fn) {
+__device__ __forceinline__ void visit_candidates(const PairConst& pc, const double* xs, const int* cell_of,
+                                                 const int* cell_start, const int* perm, const double* xsort,
+                                                 int p, int lane, F&& fn) {
     if (pc.mode == 0) {
-        for (int q = 0; q < pc.s.Np; q++)
-            if (q != p) fn(q);
+        for (int q = lane; q < pc.s.Np; q += GL)
+            if (q != p) fn(q, xs + 3 * q);
         return;
     }
+    if (lane >= 27) return;
     const int c = cell_of[p];
     const int nx = pc.ncell[0], ny = pc.ncell[1], nz = pc.ncell[2];
     const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-    for (int oz = -1; oz <= 1; oz++)
-        for (int oy = -1; oy <= 1; oy++)
-            for (int ox = -1; ox <= 1; ox++) {
-                const int ax = (cx + ox + nx) % nx, ay = (cy + oy + ny) % ny, az = (cz + oz + nz) % nz;
-                const int cc = ax + nx * (ay + ny * az);
-                for (int t = cell_start[cc]; t < cell_start[cc + 1]; t++) {
-                    const int q = perm[t];
-                    if (q != p) fn(q);
-                }
-            }
+    const int ox = lane % 3 - 1, oy = (lane / 3) % 3 - 1, oz = lane / 9 - 1;
+    const int ax = (cx + ox + nx) % nx, ay = (cy + oy + ny) % ny, az = (cz + oz + nz) % nz;
+    const int cc = ax + nx * (ay + ny * az);
+    const int t1 = cell_start[cc + 1];
+    for (int t = cell_start[cc]; t < t1; t++) {
+        const int q = perm[t];
+        if (q != p) fn(q, xsort + 3 * t);
+    }
 }
 
-__global__ void k_nb_count(PairConst pc, const double* __restrict__ xs, const int* __restrict__ cell_of,
-                           const int* __restrict__ cell_start, const int* __restrict__ perm,
-                           int* __restrict__ count) {
-    const int p = blockIdx.x * TB + threadIdx.x;
-    if (p >= pc.s.Np) return;
+// The reference's pair test for particles p, q with positions xp, xq (the pair's k = min(p, q)).
+__device__ __forceinline__ bool within_pq(const PairConst& pc, int p, const double* xp, int q, const double* xq) {
+    const bool pk = p < q;
+    const double* xk = pk ? xp : xq;
+    const double* xj = pk ? xq : xp;
+    double dx = xk[0] - xj[0];
+    double dy = xk[1] - xj[1];
+    double dz = xk[2] - xj[2];
+    if (pc.s.box_active) {
+        dx -= pc.s.L[0] * __builtin_rint(dx * pc.s.Linv[0]);
+        dy -= pc.s.L[1] * __builtin_rint(dy * pc.s.Linv[1]);
+        dz -= pc.s.L[2] * __builtin_rint(dz * pc.s.Linv[2]);
+    }
+    const double dr = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+    return dr < pc.rverl[ptype(pc.s.Nm, pk ? p : q, pk ? q : p)];
+}
+
+__device__ __forceinline__ int group_sum(int v) {
+    for (int off = GL / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, GL);
+    return v;
+}
+
+__global__ __launch_bounds__(TB) void k_nb_count(PairConst pc, const double* __restrict__ xs,
+                                                 const int* __restrict__ cell_of, const int* __restrict__ cell_start,
+                                                 const int* __restrict__ perm, const double* __restrict__ xsort,
+                                                 int* __restrict__ count) {
+    const int p = (blockIdx.x * TB + threadIdx.x) / GL;
+    const int lane = threadIdx.x % GL;
+    if (p >= pc.s.Np) return;   // whole groups leave together
+    const double xp[3] = {xs[3 * p], xs[3 * p + 1], xs[3 * p + 2]};
     int cnt = 0;
-    for_candidates(pc, xs, cell_of, cell_start, perm, p, [&](int q) {
-        if (within(pc, xs, p, q)) cnt++;
+    visit_candidates(pc, xs, cell_of, cell_start, perm, xsort, p, lane, [&](int q, const double* xq) {
+        if (within_pq(pc, p, xp, q, xq)) cnt++;
     });
-    count[p] = cnt;
+    cnt = group_sum(cnt);
+    if (lane == 0) count[p] = cnt;
 }
 
-__global__ void k_nb_fill(PairConst pc, const double* __restrict__ xs, const int* __restrict__ cell_of,
-                          const int* __restrict__ cell_start, const int* __restrict__ perm,
-                          const int* __restrict__ off, int* __restrict__ part) {
-    const int p = blockIdx.x * TB + threadIdx.x;
+// Fill row p with its partners in increasing order.  The hits are collected in LDS (cap entries per
+// group, cap >= the longest row), then each entry goes to its rank in the row (partners are distinct).
+__global__ __launch_bounds__(TB) void k_nb_fill(PairConst pc, const double* __restrict__ xs,
+                                                const int* __restrict__ cell_of, const int* __restrict__ cell_start,
+                                                const int* __restrict__ perm, const double* __restrict__ xsort,
+                                                const int* __restrict__ off, int cap, int* __restrict__ part) {
+    extern __shared__ int lds[];
+    const int g = threadIdx.x / GL;
+    const int ngroups = blockDim.x / GL;
+    int* cursor = lds;                       // [ngroups]
+    int* buf = lds + ngroups + g * cap;      // [cap] for this group
+    const int p = (blockIdx.x * blockDim.x + threadIdx.x) / GL;
+    const int lane = threadIdx.x % GL;
+    if (lane == 0) cursor[g] = 0;
+    __syncthreads();
+    if (p < pc.s.Np) {
+        const double xp[3] = {xs[3 * p], xs[3 * p + 1], xs[3 * p + 2]};
+        visit_candidates(pc, xs, cell_of, cell_start, perm, xsort, p, lane, [&](int q, const double* xq) {
+            if (within_pq(pc, p, xp, q, xq)) {
+                const int k = atomicAdd(&cursor[g], 1);
+                if (k < cap) buf[k] = q;
+            }
+        });
+    }
+    __syncthreads();
     if (p >= pc.s.Np) return;
     const int o = off[p];
-    int cnt = 0;
-    for_candidates(pc, xs, cell_of, cell_start, perm, p, [&](int q) {
-        if (within(pc, xs, p, q)) part[o + cnt++] = q;
-    });
-    // insertion sort (rows are short) -> reference order (increasing partner / pair index)
-    for (int a = 1; a < cnt; a++) {
-        const int v = part[o + a];
-        int b = a - 1;
-        while (b >= 0 && part[o + b] > v) {
-            part[o + b + 1] = part[o + b];
-            b--;
-        }
-        part[o + b + 1] = v;
+    const int n = min(cursor[g], cap);
+    for (int e = lane; e < n; e += GL) {
+        const int v = buf[e];
+        int rank = 0;
+        for (int a = 0; a < n; a++) rank += buf[a] < v;
+        part[o + rank] = v;
     }
 }
 
@@ -322,11 +379,12 @@ ps_err pairs_alloc(Ctx* c) {
         }
     }
     PS_HIP(c, hipMalloc(&c->cell_of, sizeof(int) * std::max(Np, 1)));
-    PS_HIP(c, hipMalloc(&c->cell_start, sizeof(int) * (ncells + 1)));
+    PS_HIP(c, hipMalloc(&c->cell_start, sizeof(int) * (ncells + 2)));
     PS_HIP(c, hipMalloc(&c->cell_count, sizeof(int) * ncells));
     PS_HIP(c, hipMalloc(&c->cell_perm, sizeof(int) * std::max(Np, 1)));
+    PS_HIP(c, hipMalloc(&c->cell_xs, sizeof(double) * 3 * std::max(Np, 1)));
     PS_HIP(c, hipMalloc(&c->nb_count, sizeof(int) * std::max(Np, 1)));
-    PS_HIP(c, hipMalloc(&c->nb_off, sizeof(int) * (Np + 1)));
+    PS_HIP(c, hipMalloc(&c->nb_off, sizeof(int) * (Np + 2)));
     c->nb_cap = std::max<int64_t>(1 << 16, 64 * (int64_t)Np);
     PS_HIP(c, hipMalloc(&c->nb_part, sizeof(int) * (size_t)c->nb_cap));
     PS_HIP(c, hipMalloc(&c->dflag, sizeof(int) * 4));
@@ -378,8 +436,10 @@ ps_err pairs_alloc(Ctx* c) {
 
 void pairs_free(Ctx* c) {
     for (void* p : {(void*)c->cell_of, (void*)c->cell_start, (void*)c->cell_count, (void*)c->cell_perm,
-                    (void*)c->nb_count, (void*)c->nb_off, (void*)c->nb_part, (void*)c->dflag, (void*)c->bonds})
+                    (void*)c->cell_xs, (void*)c->nb_count, (void*)c->nb_off, (void*)c->nb_part, (void*)c->dflag,
+                    (void*)c->bonds})
         if (p) (void)hipFree(p);
+    c->cell_xs = nullptr;
     if (c->hflag) (void)hipHostFree(c->hflag);
     c->cell_of = c->cell_start = c->cell_count = c->cell_perm = c->nb_count = c->nb_off = c->nb_part = nullptr;
     c->dflag = c->bonds = nullptr;
@@ -393,6 +453,7 @@ ps_err check_dist_dev(Ctx* c) {
     // geometry of this check_dist = current positions
     PS_HIP(c, hipMemcpyAsync(c->xpd, c->x, sizeof(double) * 3 * (size_t)Np, hipMemcpyDeviceToDevice, st));
     const unsigned gp = (unsigned)((Np + TB - 1) / TB);
+    const double* xsort = c->xpd;
     if (c->cell_mode == 1) {
         const int ncells = c->ncell[0] * c->ncell[1] * c->ncell[2];
         PS_HIP(c, hipMemsetAsync(c->cell_count, 0, sizeof(int) * ncells, st));
@@ -401,22 +462,37 @@ ps_err check_dist_dev(Ctx* c) {
         k_scan<<<1, 1024, 0, st>>>(c->cell_count, c->cell_start, ncells);
         PS_LAUNCH(c);
         PS_HIP(c, hipMemsetAsync(c->cell_count, 0, sizeof(int) * ncells, st));
-        k_scatter<<<gp, TB, 0, st>>>(Np, c->cell_of, c->cell_start, c->cell_count, c->cell_perm);
+        k_scatter<<<gp, TB, 0, st>>>(Np, c->xpd, c->cell_of, c->cell_start, c->cell_count, c->cell_perm, c->cell_xs);
         PS_LAUNCH(c);
+        xsort = c->cell_xs;
     }
-    k_nb_count<<<gp, TB, 0, st>>>(pc, c->xpd, c->cell_of, c->cell_start, c->cell_perm, c->nb_count);
+    const unsigned gg = (unsigned)(((int64_t)Np * GL + TB - 1) / TB);
+    k_nb_count<<<gg, TB, 0, st>>>(pc, c->xpd, c->cell_of, c->cell_start, c->cell_perm, xsort, c->nb_count);
     PS_LAUNCH(c);
     k_scan<<<1, 1024, 0, st>>>(c->nb_count, c->nb_off, Np);
     PS_LAUNCH(c);
-    PS_HIP(c, hipMemcpyAsync(c->hflag, c->nb_off + Np, sizeof(int), hipMemcpyDeviceToHost, st));
+    PS_HIP(c, hipMemcpyAsync(c->hflag, c->nb_off + Np, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
     PS_HIP(c, hipStreamSynchronize(st));
     const int64_t total = c->hflag[0];
+    const int longest = std::max(c->hflag[1], 1);
     if (total > c->nb_cap) {
         PS_HIP(c, hipFree(c->nb_part));
         c->nb_cap = total + total / 4 + 1024;
         PS_HIP(c, hipMalloc(&c->nb_part, sizeof(int) * (size_t)c->nb_cap));
     }
-    k_nb_fill<<<gp, TB, 0, st>>>(pc, c->xpd, c->cell_of, c->cell_start, c->cell_perm, c->nb_off, c->nb_part);
+    // LDS per block: a cursor and `longest` entries per group; 8 groups per block when that fits in
+    // 64 KiB, else one group per block.
+    int gpb = TB / GL;
+    if ((size_t)gpb * (longest + 1) * sizeof(int) > 65536) gpb = 1;
+    if ((size_t)gpb * (longest + 1) * sizeof(int) > 65536) {
+        c->set_error("check_dist: a particle has " + std::to_string(longest) +
+                     " Verlet partners, more than one LDS row holds");
+        return PS_ERR_ARG;
+    }
+    const size_t lds = sizeof(int) * (size_t)gpb * (longest + 1);
+    k_nb_fill<<<(unsigned)((Np + gpb - 1) / gpb), gpb * GL, lds, st>>>(pc, c->xpd, c->cell_of, c->cell_start,
+                                                                        c->cell_perm, xsort, c->nb_off, longest,
+                                                                        c->nb_part);
     PS_LAUNCH(c);
     c->nvlist = total / 2;
     c->cnt.nvlist = c->nvlist;

@@ -116,11 +116,12 @@ struct Ctx {
     int cell_mode = 0;              // 0: brute force; 1: periodic cell list
     double cell_size[3] = {0, 0, 0}, cell_org[3] = {0, 0, 0};
     int* cell_of = nullptr;         // [Np]
-    int* cell_start = nullptr;      // [ncells+1]
+    int* cell_start = nullptr;      // [ncells+2]: exclusive scan, total, max
     int* cell_count = nullptr;      // [ncells]
     int* cell_perm = nullptr;       // [Np]
+    double* cell_xs = nullptr;      // [3 Np] positions in cell order
     int* nb_count = nullptr;        // [Np] pairs per row
-    int* nb_off = nullptr;          // [Np+1]
+    int* nb_off = nullptr;          // [Np+2]: row offsets, total, longest row
     int* nb_part = nullptr;         // partners (sorted per row)
     int64_t nb_cap = 0;
     int64_t nvlist = 0;
