@@ -87,7 +87,14 @@ typedef struct ps_params {
     double ir_rtol;               /* mixed form: FP64 refinement target on the saddle residual */
     int ir_maxit;                 /* mixed form: cap on refinement sweeps */
     int solver;                   /* PS_SOLVER_MINRES (reference algorithm) or PS_SOLVER_BLOCK */
+    int aa_kernel;                /* monomer-monomer kernel: PS_AA_AUTO, PS_AA_ORDERED, PS_AA_SYMMETRIC */
 } ps_params;
+
+/* Monomer-monomer tile kernel.  ORDERED evaluates every ordered pair (target-stationary, source
+ * splits); SYMMETRIC evaluates every unordered pair once and applies the tile both ways (cubic or
+ * no box, chain_HI off; otherwise ORDERED is used).  AUTO picks SYMMETRIC when its tile-pair grid
+ * fills the device.  Both give the same operator to rounding. */
+enum { PS_AA_AUTO = 0, PS_AA_ORDERED = 1, PS_AA_SYMMETRIC = 2 };
 
 /* Saddle-solve algorithms.  Both solve the same system A x = b.
  *   PS_SOLVER_MINRES: preconditioned MINRES on the full saddle system, as KSPSolve does.

@@ -1205,6 +1205,7 @@ void or_params_default(ps_params* p) {
     p->ir_rtol = 1e-6;
     p->ir_maxit = 20;
     p->solver = PS_SOLVER_MINRES;
+    p->aa_kernel = PS_AA_AUTO;
 }
 
 void* or_create(const ps_params* p, char* err, int errlen) {

@@ -405,6 +405,7 @@ void ps_params_default(ps_params* p) {
     p->ir_rtol = 1e-6;
     p->ir_maxit = 20;
     p->solver = PS_SOLVER_MINRES;
+    p->aa_kernel = PS_AA_AUTO;
 }
 
 const char* ps_last_error(const void* ctx) {
@@ -430,6 +431,11 @@ ps_err ps_create(const ps_params* p, void** out) {
     }
     if (p->box_active && (p->box[0] <= 0 || p->box[1] <= 0 || p->box[2] <= 0)) {
         g_create_error = "Box lengths must be positive";   // Box::configure (src/box.cpp:10-14)
+        return PS_ERR_ARG;
+    }
+    if (p->precision < PS_PREC_FP64 || p->precision > PS_PREC_MIXED || p->solver < PS_SOLVER_MINRES ||
+        p->solver > PS_SOLVER_BLOCK || p->aa_kernel < PS_AA_AUTO || p->aa_kernel > PS_AA_SYMMETRIC) {
+        g_create_error = "precision, solver or aa_kernel out of range";
         return PS_ERR_ARG;
     }
     int ndev = 0;

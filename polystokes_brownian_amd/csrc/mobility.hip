@@ -151,6 +151,222 @@ __global__ __launch_bounds__(PS_BLOCK) void k_aa(SysConst s, const T* __restrict
     }
 }
 
+// ---------------------------------------------------------------------------- AA, symmetric form
+// The AA tile is symmetric in the pair (the same e e^T for d and -d), so one evaluation of the
+// pair geometry serves both u_i += A f_j and u_j += A f_i.  Block (I, J), I < J, of the
+// upper triangle of 256-bead tiles: lane t of wave w owns target i = 256 I + 64 w + t in registers;
+// the source tile J sits in LDS in four windows of 64, each stored twice (entries 128 q + m and
+// 128 q + m + 64 hold source 64 q + m), so at step k lane t reads entry 128 q + t + k: a systolic
+// sweep in which every lane meets a different source, with no index arithmetic.  The u_j
+// contributions accumulate in registers that rotate with the sources and are added to an LDS
+// accumulator per source at the end of each window; the waves visit the windows in staggered
+// phases (window (w + p) mod 4 in phase p, a barrier between phases), so no two waves touch one
+// window at once and every accumulator sums in a fixed order.
+// Diagonal blocks (I == J) run the ordered loop.  Output: part[(J 3 + c) Nm + i] is the
+// contribution of source tile J to target i, written exactly once per (J, i).
+//
+// Positions are in box units when the box is active (cubic box, x / L), so the minimum image is
+// d - rint(d) and L folds into the constants:
+//   far:  ya = a (c1 - c2 a^2),  cs = a^3 (c3 + c4 a^2)   with a = |d_s|^-1, u += ya f + cs (d_s.f) d_s
+//   near: r = L |d_s|^2 a:  ya = 1 - 9/32 r,  cs = (21/32 r - 2) a^2
+// c1 = 3/(4L), c2 = 1/(2L^3), c3 = -3/(8L), c4 = -3/(2L^3) (L = 1 without a box).
+struct AAConst {
+    double c1, c2, c3, c4, L, tb2s, clamp;
+};
+
+template <class T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+    using type = float4;
+};
+template <>
+struct Vec4<double> {
+    using type = double4;
+};
+
+template <class T>
+__device__ __forceinline__ T ps_fma(T a, T b, T c);
+template <>
+__device__ __forceinline__ float ps_fma<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <>
+__device__ __forceinline__ double ps_fma<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// lane t receives the value of lane (t + 1) mod 64 (DPP wave_rol:1; a double moves as two halves)
+__device__ __forceinline__ float wave_rol1(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x134, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double wave_rol1(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), 0x134, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x134, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <class T, bool BOX, bool PAD>
+__global__ __launch_bounds__(PS_BLOCK) void k_aa_sym(int Nm, const int2* __restrict__ pairs, AAConst k,
+                                                     const T* __restrict__ xs, const double* __restrict__ f,
+                                                     T* __restrict__ part) {
+    using V = typename Vec4<T>::type;
+    __shared__ V S[512], F[512];
+    __shared__ T A[3][PS_BLOCK];
+    const int2 pr = pairs[blockIdx.x];
+    const int I = pr.x, J = pr.y;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    {
+        const int j = J * PS_BLOCK + t;
+        V sv = {0, 0, 0, 0}, fv = {0, 0, 0, 0};
+        if (!PAD || j < Nm) {
+            sv.x = xs[4 * j];
+            sv.y = xs[4 * j + 1];
+            sv.z = xs[4 * j + 2];
+            fv.x = (T)f[3 * j];
+            fv.y = (T)f[3 * j + 1];
+            fv.z = (T)f[3 * j + 2];
+        }
+        const int e = (w << 7) + l;
+        S[e] = sv;
+        S[e + 64] = sv;
+        F[e] = fv;
+        F[e + 64] = fv;
+#pragma unroll
+        for (int c = 0; c < 3; c++) A[c][t] = 0;
+    }
+    const int i = I * PS_BLOCK + t;
+    T xi = 0, yi = 0, zi = 0, fxi = 0, fyi = 0, fzi = 0;
+    if (!PAD || i < Nm) {
+        xi = xs[4 * i];
+        yi = xs[4 * i + 1];
+        zi = xs[4 * i + 2];
+        fxi = (T)f[3 * i];
+        fyi = (T)f[3 * i + 1];
+        fzi = (T)f[3 * i + 2];
+    }
+    const T c1 = (T)k.c1, c2 = (T)k.c2, c3 = (T)k.c3, c4 = (T)k.c4, Lb = (T)k.L, tb2s = (T)k.tb2s;
+    T ux = 0, uy = 0, uz = 0;
+    __syncthreads();
+    if (I == J) {
+        // ordered loop over the 256 sources of the own tile, skipping i itself (and ghosts)
+        for (int q = 0; q < 4; q++) {
+            for (int m = 0; m < 64; m++) {
+                const int jt = (q << 6) + m;
+                const V xj = S[(q << 7) + m];
+                const V fj = F[(q << 7) + m];
+                T dx = xi - xj.x, dy = yi - xj.y, dz = zi - xj.z;
+                if (BOX) {
+                    dx -= ps_rint<T>(dx);
+                    dy -= ps_rint<T>(dy);
+                    dz -= ps_rint<T>(dz);
+                }
+                T r2 = dx * dx + dy * dy + dz * dz;
+                bool skip = jt == t;
+                if (PAD) skip |= (J * PS_BLOCK + jt >= Nm);
+                r2 = skip ? (T)1 : r2;
+                const T a = ps_rsqrt(r2);
+                const T a2 = a * a;
+                T ya = a * (c1 - c2 * a2);
+                T cs = a * a2 * (c3 + c4 * a2);
+                if (r2 <= tb2s) {
+                    const T r = Lb * r2 * a;
+                    ya = (T)1 - (T)C9D32 * r;
+                    cs = ((T)(21. / 32.) * r - (T)2) * a2;
+                }
+                ya = skip ? (T)0 : ya;
+                cs = skip ? (T)0 : cs;
+                const T ci = cs * (dx * fj.x + dy * fj.y + dz * fj.z);
+                ux = ps_fma(ya, fj.x, ux);
+                uy = ps_fma(ya, fj.y, uy);
+                uz = ps_fma(ya, fj.z, uz);
+                ux = ps_fma(ci, dx, ux);
+                uy = ps_fma(ci, dy, uy);
+                uz = ps_fma(ci, dz, uz);
+            }
+        }
+        if (!PAD || i < Nm) {
+            part[((int64_t)J * 3 + 0) * Nm + i] = ux;
+            part[((int64_t)J * 3 + 1) * Nm + i] = uy;
+            part[((int64_t)J * 3 + 2) * Nm + i] = uz;
+        }
+        return;
+    }
+    const T clamp = (T)k.clamp;
+    // The u_j partial sums travel with their sources in registers: after step m lane t holds the
+    // sum for source (t + m) mod 64 of the window, and a wave rotation (DPP wave_rol:1, lane t
+    // reads lane t + 1) hands it on, so step m + 1 adds to the right source; after 64 steps every
+    // lane has met every source.  (LDS atomics for the same sums measured several times slower.)
+    constexpr int UNR = sizeof(T) == sizeof(float) ? 8 : 4;
+    for (int p = 0; p < 4; p++) {
+        const int q = (w + p) & 3;
+        const int base = (q << 7) + l;
+        const V* Sq = S + base;
+        const V* Fq = F + base;
+        T Rx = 0, Ry = 0, Rz = 0;
+#pragma unroll UNR
+        for (int m = 0; m < 64; m++) {
+            const V xj = Sq[m];
+            const V fj = Fq[m];
+            T dx = xi - xj.x, dy = yi - xj.y, dz = zi - xj.z;
+            if (BOX) {
+                dx -= ps_rint<T>(dx);
+                dy -= ps_rint<T>(dy);
+                dz -= ps_rint<T>(dz);
+            }
+            T r2 = ps_fma(dz, dz, ps_fma(dy, dy, dx * dx));
+            if (PAD) r2 = r2 > clamp ? r2 : clamp;   // ghost beads (f = 0) stay finite
+            const T a = ps_rsqrt(r2);
+            const T a2 = a * a;
+            T ya = a * ps_fma(-c2, a2, c1);
+            T cs = (a * a2) * ps_fma(c4, a2, c3);
+            if (__builtin_expect(r2 <= tb2s, 0)) {
+                const T r = Lb * r2 * a;
+                ya = (T)1 - (T)C9D32 * r;
+                cs = ((T)(21. / 32.) * r - (T)2) * a2;
+            }
+            // u_i += ya f_j + cs (d.f_j) d
+            const T ci = cs * ps_fma(dz, fj.z, ps_fma(dy, fj.y, dx * fj.x));
+            ux = ps_fma(ya, fj.x, ux);
+            uy = ps_fma(ya, fj.y, uy);
+            uz = ps_fma(ya, fj.z, uz);
+            ux = ps_fma(ci, dx, ux);
+            uy = ps_fma(ci, dy, uy);
+            uz = ps_fma(ci, dz, uz);
+            // u_j += ya f_i + cs (d.f_i) d
+            const T cj = cs * ps_fma(dz, fzi, ps_fma(dy, fyi, dx * fxi));
+            Rx = wave_rol1(Rx) + ps_fma(cj, dx, ya * fxi);
+            Ry = wave_rol1(Ry) + ps_fma(cj, dy, ya * fyi);
+            Rz = wave_rol1(Rz) + ps_fma(cj, dz, ya * fzi);
+        }
+        // lane t now holds the complete sum of the wave for source (t + 63) mod 64; the waves
+        // visit a window one at a time, so a plain LDS update is race-free and ordered
+        const int e = (q << 6) + ((l + 63) & 63);
+        A[0][e] += Rx;
+        A[1][e] += Ry;
+        A[2][e] += Rz;
+        __syncthreads();
+    }
+    if (!PAD || i < Nm) {
+        part[((int64_t)J * 3 + 0) * Nm + i] = ux;
+        part[((int64_t)J * 3 + 1) * Nm + i] = uy;
+        part[((int64_t)J * 3 + 2) * Nm + i] = uz;
+    }
+    const int j = J * PS_BLOCK + t;
+    if (!PAD || j < Nm) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) part[((int64_t)I * 3 + c) * Nm + j] = A[c][t];
+    }
+}
+
+// Scaled monomer positions for k_aa_sym: [Nm][4] (x, y, z, 0) * inv_L.
+template <class T>
+__global__ void k_aa_pos(int Nm, const double* __restrict__ xm, double inv_L, T* __restrict__ xs) {
+    const int i = blockIdx.x * PS_BLOCK + threadIdx.x;
+    if (i >= Nm) return;
+    xs[4 * i] = (T)(xm[3 * i] * inv_L);
+    xs[4 * i + 1] = (T)(xm[3 * i + 1] * inv_L);
+    xs[4 * i + 2] = (T)(xm[3 * i + 2] * inv_L);
+    xs[4 * i + 3] = (T)0;
+}
+
 // ---------------------------------------------------------------------------- AB, monomer rows
 // u_m += stale . sum_c F_c + y12b (e x T_c) - x12g (e.S^e) e + 2 y12g (S^e - (e.S^e) e)
 template <class T, bool BOX>
@@ -633,7 +849,33 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
             PS_HIP(c, hipEventRecord(ek0, st));
         }
         const bool box = s.box_active, ch = s.chain_HI;
-        if (box && ch)
+        if (aw.sym) {
+            AAConst k{};
+            const double L = box ? s.L[0] : 1.0;
+            k.c1 = 3.0 / (4.0 * L);
+            k.c2 = 1.0 / (2.0 * L * L * L);
+            k.c3 = -3.0 / (8.0 * L);
+            k.c4 = -3.0 / (2.0 * L * L * L);
+            k.L = L;
+            k.tb2s = s.two_beta * s.two_beta / (L * L);
+            k.clamp = sizeof(T) == sizeof(float) ? 1e-30 : 1e-200;
+            const T* xs = (const T*)aw.xs;
+            if (aw.npair_full > 0) {
+                if (box)
+                    k_aa_sym<T, true, false><<<aw.npair_full, PS_BLOCK, 0, st>>>(s.Nm, aw.pairs, k, xs, f, (T*)aw.part_aa);
+                else
+                    k_aa_sym<T, false, false><<<aw.npair_full, PS_BLOCK, 0, st>>>(s.Nm, aw.pairs, k, xs, f, (T*)aw.part_aa);
+                PS_LAUNCH(c);
+            }
+            if (aw.npair_pad > 0) {
+                const int2* pp = aw.pairs + aw.npair_full;
+                if (box)
+                    k_aa_sym<T, true, true><<<aw.npair_pad, PS_BLOCK, 0, st>>>(s.Nm, pp, k, xs, f, (T*)aw.part_aa);
+                else
+                    k_aa_sym<T, false, true><<<aw.npair_pad, PS_BLOCK, 0, st>>>(s.Nm, pp, k, xs, f, (T*)aw.part_aa);
+                PS_LAUNCH(c);
+            }
+        } else if (box && ch)
             k_aa<T, true, true><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
         else if (box)
             k_aa<T, true, false><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
@@ -692,6 +934,31 @@ ps_err alloc_work(Ctx* c, ApplyWork& aw, size_t elt) {
     nchunk = std::max(1, std::min(nchunk, std::max(1, s.Nm / PS_BLOCK)));
     aw.per_chunk_c = s.Nm > 0 ? (s.Nm + nchunk - 1) / nchunk : 0;
     aw.nchunk_c = s.Nm > 0 ? (s.Nm + aw.per_chunk_c - 1) / aw.per_chunk_c : 1;
+    // symmetric AA kernel: cubic box (or none), no chain_HI, and (AUTO) at least one tile pair per CU
+    const bool cubic = !s.box_active || (s.L[0] == s.L[1] && s.L[1] == s.L[2]);
+    const int npairs_all = ntb * (ntb + 1) / 2;
+    const int ncu = c->n_cu > 0 ? c->n_cu : 256;
+    const int sel = c->p.aa_kernel;
+    aw.sym = s.Nm > 0 && cubic && !s.chain_HI &&
+             (sel == PS_AA_SYMMETRIC || (sel == PS_AA_AUTO && npairs_all >= ncu));
+    if (aw.sym) {
+        aw.ntile = ntb;
+        aw.nsplit_aa = ntb;
+        aw.inv_L = s.box_active ? 1.0 / s.L[0] : 1.0;
+        const bool partial = (s.Nm % PS_BLOCK) != 0;
+        std::vector<int2> full, pad;
+        for (int I = 0; I < ntb; I++)
+            for (int J = I; J < ntb; J++) {
+                const bool last = partial && (J == ntb - 1);
+                (last ? pad : full).push_back(make_int2(I, J));
+            }
+        aw.npair_full = (int)full.size();
+        aw.npair_pad = (int)pad.size();
+        full.insert(full.end(), pad.begin(), pad.end());
+        PS_HIP(c, hipMalloc(&aw.pairs, sizeof(int2) * full.size()));
+        PS_HIP(c, hipMemcpy(aw.pairs, full.data(), sizeof(int2) * full.size(), hipMemcpyHostToDevice));
+        PS_HIP(c, hipMalloc(&aw.xs, 4 * elt * (size_t)s.Nm));
+    }
     aw.bytes_aa = (size_t)aw.nsplit_aa * 3 * std::max(s.Nm, 1) * elt;
     // monomer rows from colloid sources: chunks of >= 32 colloids on blockIdx.y
     int nchm = std::max(1, std::min((s.Nc + 31) / 32, ntb > 0 ? (target + ntb - 1) / ntb : 1));
@@ -712,7 +979,7 @@ ps_err alloc_work(Ctx* c, ApplyWork& aw, size_t elt) {
 }
 
 void free_work(ApplyWork& aw) {
-    for (void* p : {aw.part_aa, aw.part_abm, aw.part_c, aw.part_fs, (void*)aw.bb})
+    for (void* p : {aw.part_aa, aw.part_abm, aw.part_c, aw.part_fs, (void*)aw.bb, (void*)aw.pairs, aw.xs})
         if (p) (void)hipFree(p);
     aw = ApplyWork();
 }
@@ -744,6 +1011,15 @@ ps_err mob_prepare(Ctx* c) {
     if (c->xm32) {
         const int64_t n3 = 3 * (int64_t)c->Np;
         k_to_f32<<<(unsigned)((n3 + PS_BLOCK - 1) / PS_BLOCK), PS_BLOCK, 0, c->stream>>>(c->xm, c->xm32, n3);
+        PS_LAUNCH(c);
+    }
+    const unsigned gm = (unsigned)((c->Nm + PS_BLOCK - 1) / PS_BLOCK);
+    if (c->aw64.sym) {
+        k_aa_pos<double><<<gm, PS_BLOCK, 0, c->stream>>>(c->Nm, c->xm, c->aw64.inv_L, (double*)c->aw64.xs);
+        PS_LAUNCH(c);
+    }
+    if (c->aw32.sym) {
+        k_aa_pos<float><<<gm, PS_BLOCK, 0, c->stream>>>(c->Nm, c->xm, c->aw32.inv_L, (float*)c->aw32.xs);
         PS_LAUNCH(c);
     }
     if (c->Mss && c->Nc > 0) {

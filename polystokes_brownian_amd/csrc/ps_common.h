@@ -53,6 +53,13 @@ struct MinresState {
 
 // Apply workspace (partial sums) sized at create time.
 struct ApplyWork {
+    // symmetric AA kernel: 256-bead tiles, upper-triangle tile pairs (full tiles first, then the
+    // pairs that involve the partial last tile), scaled positions [Nm][4]
+    bool sym = false;
+    int ntile = 0, npair_full = 0, npair_pad = 0;
+    int2* pairs = nullptr;
+    void* xs = nullptr;
+    double inv_L = 1.0;
     int nsplit_aa = 1, nchunk_c = 1, nchunk_m = 1;
     int per_split_aa = 0, per_chunk_c = 0, per_chunk_m = 0;
     void* part_aa = nullptr;     // [nsplit][3][Nm]  (T)

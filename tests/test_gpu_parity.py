@@ -111,6 +111,56 @@ def test_mobility_apply(name):
     np.testing.assert_array_equal(g.saddle_diag(), o.saddle_diag())
 
 
+SYM_CASES = {
+    # one partial tile (diagonal block only)
+    "c1": lambda: configs.workload("c1"),
+    # 3 tiles, the last one partial (off-diagonal blocks with and without ghost beads)
+    "box_700": lambda: configs.small(3, 35, 20, 14.0, seed=12),
+    # exactly 4 full tiles, no ghost beads
+    "box_1024": lambda: configs.small(2, 64, 16, 16.0, seed=13),
+    "nobox_700": lambda: dict(configs.small(3, 35, 20, 14.0, seed=14), L=None),
+}
+
+
+def _sym_build(name, **over):
+    w = SYM_CASES[name]()
+    if w.get("L") is None:
+        x0 = configs.positions(dict(w, L=14.0))
+    else:
+        x0 = configs.positions(w)
+    pd = configs.params_dict(w)
+    pd.update(over)
+    return pd, x0
+
+
+@pytest.mark.parametrize("name", list(SYM_CASES))
+def test_symmetric_aa_kernel(name):
+    """The symmetric monomer-monomer kernel (one evaluation per unordered pair, applied both ways)
+    against the oracle and against the ordered kernel; FP64 to 1e-12, FP32 tiles to 1e-5."""
+    pd, x0 = _sym_build(name)
+    o = _oracle()(pd)
+    o.initial_configuration(x0)
+    o.check_dist()
+    o.mob()
+    rng = np.random.default_rng(21)
+    f = rng.normal(size=o.n)
+    uo = o.mob_mult(f)
+    us = {}
+    for kern in (1, 2):
+        for prec in (0, 1):
+            g = Context(dict(pd, aa_kernel=kern, precision=prec))
+            g.initial_configuration(x0)
+            g.check_dist()
+            g.mob()
+            u = g.mob_mult(f)
+            print("case", name, "kernel", kern, "precision", prec, "rel", rel(u, uo), flush=True)
+            assert rel(u, uo) < (1e-12 if prec == 0 else 1e-5)
+            assert np.array_equal(u, g.mob_mult(f))   # deterministic
+            us[(kern, prec)] = u
+            g.close()
+    assert rel(us[(2, 0)], us[(1, 0)]) < 1e-13
+
+
 @pytest.mark.parametrize("name", ["c1", "small_chainHI"])
 def test_rhs(name):
     g, o, _ = build(name)
