@@ -48,19 +48,21 @@ def make_ctx(Context, pd, x0, **over):
 
 
 def timed_run(ctx, steps, warmup, label):
-    for k in range(warmup):
+    """W warmup steps, then exactly `steps` steps in one run() call (as PolyStokes.run drives them:
+    the noise of step k+1 is drawn on the host while the GPU works on step k)."""
+    if warmup > 0:
         t0 = time.perf_counter()
-        ctx.run(1)
-        log(f"{label}: warmup step {k + 1}/{warmup} {time.perf_counter() - t0:.2f}s")
+        ctx.run(warmup)
+        log(f"{label}: {warmup} warmup step(s) {time.perf_counter() - t0:.2f}s")
     c0 = ctx.counters()
     ctx.state()   # synchronise
     ctx.profile_begin()
+    log(f"{label}: timing {steps} step(s)")
     t0 = time.perf_counter()
-    for k in range(steps):
-        ctx.run(1)   # ps_run synchronises the library's stream at its end
-        log(f"{label}: timed step {k + 1}/{steps} at {time.perf_counter() - t0:.2f}s")
+    ctx.run(steps)   # ps_run synchronises the library's stream at its end
     ctx.state()
     dt = time.perf_counter() - t0
+    log(f"{label}: {steps} timed step(s) {dt:.3f}s")
     prof = ctx.profile_end()
     c1 = ctx.counters()
     per = {k: (c1[k] - c0[k]) / steps for k in ("applies", "minres_iterations", "lanczos_iterations",
@@ -83,12 +85,14 @@ def rel(a, b):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-applies", type=int, default=3)
     ap.add_argument("--no-minres", action="store_true", help="skip timing the reference MINRES algorithm")
+    ap.add_argument("--profile-form", default=None,
+                    help="only run warmup + timed steps of this form (for a profiler), no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the solve is not sharded across GPUs",
@@ -112,6 +116,13 @@ def main():
     }
     if args.no_minres:
         forms.pop("fp64/minres")
+    if args.profile_form:
+        ctx = make_ctx(Context, pd, x0, **forms[args.profile_form])
+        dt, prof, per = timed_run(ctx, args.steps, args.warmup, args.profile_form)
+        log(f"{args.profile_form}: {Np * args.steps / dt:.4g} bead-steps/s, {1e3 * dt / args.steps:.2f} ms/step, "
+            f"per step {per}")
+        ctx.close()
+        return
 
     # accuracy: one step of each form from the same state and noise, against the FP64 form of the
     # same saddle algorithm

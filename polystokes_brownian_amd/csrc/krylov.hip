@@ -357,6 +357,7 @@ ps_err krylov_alloc(Ctx* c) {
     PS_HIP(c, hipMalloc(&c->Q, sizeof(double) * (size_t)c->kmax * (size_t)n));
     PS_HIP(c, hipMalloc(&c->lz, sizeof(double) * (3 * 64 + 8)));
     PS_HIP(c, hipHostMalloc(&c->lz_h, sizeof(double) * (3 * 64 + 8), hipHostMallocDefault));
+    for (int k = 0; k < 2; k++) PS_HIP(c, hipEventCreateWithFlags(&c->lz_ev[k], hipEventDisableTiming));
     PS_HIP(c, hipMalloc(&c->gw, sizeof(double) * 64));
     PS_HIP(c, hipMalloc(&c->ir_r, sizeof(double) * (size_t)ns));
     PS_HIP(c, hipMalloc(&c->ir_d, sizeof(double) * (size_t)ns));
@@ -366,22 +367,28 @@ ps_err krylov_alloc(Ctx* c) {
     if (c->p.solver == PS_SOLVER_BLOCK) {
         const size_t n5 = 5 * (size_t)std::max(c->Nc, 1);
         PS_HIP(c, hipMalloc(&c->Mss, sizeof(double) * n5 * n5));
-        PS_HIP(c, hipMalloc(&c->mss_dinv, sizeof(double) * n5));
         PS_HIP(c, hipMalloc(&c->bs_y, sizeof(double) * (size_t)n));
         PS_HIP(c, hipMalloc(&c->bs_r, sizeof(double) * n5));
         PS_HIP(c, hipMalloc(&c->bs_x, sizeof(double) * (size_t)ns));
+        PS_HIP(c, hipMalloc(&c->ms_x, sizeof(double) * n5));
     }
     return PS_OK;
 }
 
 void krylov_free(Ctx* c) {
     for (void* p : {(void*)c->kr, (void*)c->dinv, (void*)c->red, (void*)c->mstate, (void*)c->Q, (void*)c->lz,
-                    (void*)c->gw, (void*)c->ir_r, (void*)c->ir_d, (void*)c->Mss, (void*)c->mss_dinv,
-                    (void*)c->bs_y, (void*)c->bs_r, (void*)c->bs_x})
+                    (void*)c->gw, (void*)c->ir_r, (void*)c->ir_d, (void*)c->Mss,
+                    (void*)c->bs_y, (void*)c->bs_r, (void*)c->bs_x, (void*)c->ms_x})
         if (p) (void)hipFree(p);
-    c->Mss = c->mss_dinv = c->bs_y = c->bs_r = c->bs_x = nullptr;
+    if (c->ms_exec) (void)hipGraphExecDestroy(c->ms_exec);
+    c->ms_exec = nullptr;
+    c->ms_x = nullptr;
+    c->Mss = c->bs_y = c->bs_r = c->bs_x = nullptr;
     if (c->mstate_h) (void)hipHostFree(c->mstate_h);
     if (c->lz_h) (void)hipHostFree(c->lz_h);
+    for (int k = 0; k < 2; k++)
+        if (c->lz_ev[k]) (void)hipEventDestroy(c->lz_ev[k]);
+    c->lz_ev[0] = c->lz_ev[1] = nullptr;
     c->kr = c->dinv = c->red = c->Q = c->lz = c->gw = c->ir_r = c->ir_d = nullptr;
     c->mstate = nullptr;
     c->mstate_h = nullptr;
@@ -523,7 +530,19 @@ static ps_err minres_dev(Ctx* c, const double* b, double* x, int warm, double rt
 // MINRES recurrence with two launches per iteration: a multi-block GEMV and one single-block kernel
 // that does all the vector work and the scalar recurrence (block-wide reductions, fixed order).
 // Vectors: r1, r2, y, v, wa (= w_{k-1}), wb (= w_{k-2}); the state lives in one MinresState.
+// Preconditioner: block Jacobi with the exact inverse of the 5x5 diagonal blocks.  Every diagonal
+// block of M_ss is the stresslet self block of fill_self (src/mob.cpp:600-612), 9/5 I with 9/10 at
+// (0,4) and (4,0), so its inverse couples entries 0 and 4 of a colloid by [[a, b], [b, a]],
+// a = (9/5)/((9/5)^2 - (9/10)^2), b = -(9/10)/((9/5)^2 - (9/10)^2), and scales entries 1..3 by 5/9.
 constexpr int MS_T = 1024;   // threads of the single-block kernels
+
+__device__ __forceinline__ double ms_prec(const double* __restrict__ v, int64_t i) {
+    const double a = 1.8 / 2.43, b = -0.9 / 2.43;
+    const int r = (int)(i % 5);
+    if (r == 0) return a * v[i] + b * v[i + 4];
+    if (r == 4) return b * v[i - 4] + a * v[i];
+    return v[i] * (5.0 / 9.0);
+}
 
 __device__ __forceinline__ double block_reduce_1024(double v, double* sh) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -538,15 +557,14 @@ __device__ __forceinline__ double block_reduce_1024(double v, double* sh) {
 }
 
 __global__ __launch_bounds__(MS_T) void k_ms_init(int64_t N, const double* __restrict__ b, double* __restrict__ x,
-                                                  const double* __restrict__ dinv, double* __restrict__ r1,
-                                                  double* __restrict__ r2, double* __restrict__ y,
-                                                  double* __restrict__ v, double* __restrict__ wa,
-                                                  double* __restrict__ wb, double rtol, double abstol,
-                                                  MinresState* __restrict__ st) {
+                                                  double* __restrict__ r1, double* __restrict__ r2,
+                                                  double* __restrict__ y, double* __restrict__ v,
+                                                  double* __restrict__ wa, double* __restrict__ wb, double rtol,
+                                                  double abstol, MinresState* __restrict__ st) {
     __shared__ double sh[MS_T / 64];
-    double a1 = 0.0, a2 = 0.0;
+    double a1 = 0.0;
     for (int64_t i = threadIdx.x; i < N; i += MS_T) {
-        const double bi = b[i], zi = dinv[i] * bi;
+        const double bi = b[i], zi = ms_prec(b, i);
         x[i] = 0.0;
         r1[i] = bi;
         r2[i] = bi;
@@ -554,10 +572,10 @@ __global__ __launch_bounds__(MS_T) void k_ms_init(int64_t N, const double* __res
         wa[i] = 0.0;
         wb[i] = 0.0;
         a1 += bi * zi;
-        a2 += bi * dinv[i] * bi;
     }
+    // cold start: r1 = b, so r1 . P^-1 r1 is also the b-norm of the stopping test
     const double b1 = block_reduce_1024(a1, sh);
-    const double bn = block_reduce_1024(a2, sh);
+    const double bn = b1;
     MinresState s{};
     s.beta1 = b1 > 0.0 ? sqrt(b1) : 0.0;
     s.tol = fmax(rtol * sqrt(fmax(bn, 0.0)), abstol);
@@ -599,39 +617,59 @@ __global__ __launch_bounds__(VB) void k_ms_gemv(int64_t N, const double* __restr
     if (lane == 0) y[row] = acc;
 }
 
-// One MINRES iteration after y = A v (Paige & Saunders; the same recurrence as k_mr_a/b/c).
-__global__ __launch_bounds__(MS_T) void k_ms_update(int64_t N, const double* __restrict__ dinv,
-                                                    double* __restrict__ r1, double* __restrict__ r2,
-                                                    double* __restrict__ y, double* __restrict__ v,
-                                                    double* __restrict__ wa, double* __restrict__ wb,
-                                                    double* __restrict__ x, int maxit, double abstol,
-                                                    MinresState* __restrict__ st) {
-    __shared__ double sh[MS_T / 64];
-    MinresState s = *st;
-    if (s.done) return;
+// One MINRES iteration per launch (Paige & Saunders; the recurrence of k_mr_a/b/c).  The O(N)
+// vector work is small, so every block does it redundantly over the whole vectors (same loops, same
+// reduction tree, hence the same scalars in every block), keeps v_{k+1} in LDS, writes its own row
+// range of the next vector set, and computes its rows of y_{k+1} = A v_{k+1}.  The vectors and the
+// state are double-buffered by iteration parity (`cur` is read, `nxt` written), so no block reads a
+// value another block of the same launch writes.
+struct MsSet {
+    double *r1, *r2, *y, *v, *wa, *wb;   // wa = w_{k-1}, wb = w_{k-2}; y = A v
+};
+
+__global__ __launch_bounds__(VB) void k_ms_iter(int64_t N, const double* __restrict__ A, MsSet cur, MsSet nxt,
+                                                double* __restrict__ x, int rows,
+                                                const MinresState* __restrict__ st_cur,
+                                                MinresState* __restrict__ st_nxt, int maxit, double abstol) {
+    extern __shared__ double V[];   // [N]: z, then v_{k+1}
+    __shared__ double sh[4];
+    MinresState s = *st_cur;
+    if (s.done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *st_nxt = s;
+        return;
+    }
     // y -= (beta/oldb) r1 (from the 2nd iteration); alfa = v . y
     const bool sub = s.its >= 1;
     const double f = sub ? s.beta / s.oldb : 0.0;
     double acc = 0.0;
-    for (int64_t i = threadIdx.x; i < N; i += MS_T) {
-        double yi = y[i];
-        if (sub) yi -= f * r1[i];
-        y[i] = yi;
-        acc += v[i] * yi;
+    for (int64_t i = threadIdx.x; i < N; i += VB) {
+        double ya = cur.y[i];
+        if (sub) ya -= f * cur.r1[i];
+        acc += cur.v[i] * ya;
     }
-    const double alfa = block_reduce_1024(acc, sh);
-    // y -= (alfa/beta) r2; r1 <- r2; r2 <- y; z = dinv y (kept in y); beta_new^2 = r2 . z
+    const double alfa = block_reduce(acc, sh);
+    // r2_new = y - (alfa/beta) r2; z = P^-1 r2_new per colloid block; beta_new^2 = r2_new . z
     const double f2 = alfa / s.beta;
+    const double pa = 1.8 / 2.43, pb = -0.9 / 2.43;
     acc = 0.0;
-    for (int64_t i = threadIdx.x; i < N; i += MS_T) {
-        const double yi = y[i] - f2 * r2[i];
-        const double zi = dinv[i] * yi;
-        r1[i] = r2[i];
-        r2[i] = yi;
-        y[i] = zi;
-        acc += yi * zi;
+    for (int64_t cb = threadIdx.x; cb < N / 5; cb += VB) {
+        double r[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            const int64_t i = 5 * cb + k;
+            double ya = cur.y[i];
+            if (sub) ya -= f * cur.r1[i];
+            r[k] = ya - f2 * cur.r2[i];
+        }
+        const double z[5] = {pa * r[0] + pb * r[4], r[1] * (5.0 / 9.0), r[2] * (5.0 / 9.0), r[3] * (5.0 / 9.0),
+                             pb * r[0] + pa * r[4]};
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            V[5 * cb + k] = z[k];
+            acc += r[k] * z[k];
+        }
     }
-    const double bb = block_reduce_1024(acc, sh);
+    const double bb = block_reduce(acc, sh);
     s.alfa = alfa;
     s.oldb = s.beta;
     s.beta = sqrt(fmax(bb, 0.0));
@@ -660,49 +698,111 @@ __global__ __launch_bounds__(MS_T) void k_ms_update(int64_t N, const double* __r
         s.done = 1;
         s.reason = -3;
     }
-    // w_k = (v - oldeps w_{k-2} - delta w_{k-1}) / gamma; x += phi w_k; v <- z / beta_new
     const double denom = 1.0 / gamma;
     const double vs = (s.beta > 0.0) ? 1.0 / s.beta : 0.0;
-    for (int64_t i = threadIdx.x; i < N; i += MS_T) {
-        const double w1 = wa[i];
-        const double wn = (v[i] - oldeps * wb[i] - delta * w1) * denom;
-        wb[i] = w1;
-        wa[i] = wn;
-        x[i] += phi * wn;
-        v[i] = vs * y[i];
-    }
+    // V holds z; each thread scales the entries it wrote (colloid blocks cb = threadIdx.x + k VB)
+    for (int64_t cb = threadIdx.x; cb < N / 5; cb += VB)
+#pragma unroll
+        for (int k = 0; k < 5; k++) V[5 * cb + k] *= vs;
     __syncthreads();
-    if (threadIdx.x == 0) *st = s;
+    // own rows: w_k = (v - oldeps w_{k-2} - delta w_{k-1}) / gamma, x += phi w_k, shift the vectors
+    const int64_t rb = (int64_t)blockIdx.x * rows, re = min(N, rb + rows);
+    for (int64_t i = rb + threadIdx.x; i < re; i += VB) {
+        double ya = cur.y[i];
+        if (sub) ya -= f * cur.r1[i];
+        const double w1 = cur.wa[i];
+        const double wn = (cur.v[i] - oldeps * cur.wb[i] - delta * w1) * denom;
+        nxt.wb[i] = w1;
+        nxt.wa[i] = wn;
+        x[i] += phi * wn;
+        nxt.r1[i] = cur.r2[i];
+        nxt.r2[i] = ya - f2 * cur.r2[i];
+        nxt.v[i] = V[i];
+    }
+    // y_{k+1} = A v_{k+1} for the own rows, one wave per row
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int64_t row = rb + wid; row < re; row += VB / 64) {
+        const double* a = A + row * N;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int64_t j = lane;
+        for (; j + 192 < N; j += 256) {
+            s0 += a[j] * V[j];
+            s1 += a[j + 64] * V[j + 64];
+            s2 += a[j + 128] * V[j + 128];
+            s3 += a[j + 192] * V[j + 192];
+        }
+        for (; j < N; j += 64) s0 += a[j] * V[j];
+        const double t = wave_sum((s0 + s1) + (s2 + s3));
+        if (lane == 0) nxt.y[row] = t;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *st_nxt = s;
 }
 
 // Solve A x = b for the small dense SPD-like block A (N x N, row-major) by the recurrence above,
 // polling the state every MS_POLL iterations (the kernels are no-ops once it has converged).
-static ps_err minres_small(Ctx* c, int64_t N, const double* A, const double* dinv, const double* b, double* x,
-                           double rtol, int maxit, ps_solve_info* info) {
+// The polling group (MS_POLL launches of k_ms_iter, parities 0, 1, 0, ...) is captured once as a
+// HIP graph on fixed buffers and replayed: the iterations are a few microseconds each, so separate
+// launches would leave the device waiting on the host.  The solution accumulates in c->ms_x.
+static ps_err minres_small(Ctx* c, int64_t N, const double* A, const double* b, double* x, double rtol, int maxit,
+                           ps_solve_info* info) {
     hipStream_t st = c->stream;
-    double* r1 = c->kr;
-    double* r2 = c->kr + N;
-    double* y = c->kr + 2 * N;
-    double* v = c->kr + 3 * N;
-    double* wa = c->kr + 4 * N;
-    double* wb = c->kr + 5 * N;
-    MinresState* sd = c->mstate;
-    k_ms_init<<<1, MS_T, 0, st>>>(N, b, x, dinv, r1, r2, y, v, wa, wb, rtol, c->p.ksp_abstol, sd);
+    if (N > 8192) {   // v_{k+1} of k_ms_iter lives in 64 KiB of LDS
+        c->set_error("block solver: 5 Nc = " + std::to_string(N) + " stresslet unknowns exceed 8192");
+        return PS_ERR_ARG;
+    }
+    constexpr int MS_POLL = 8;   // even: a group leaves the parity where it found it
+    MsSet set[2];
+    for (int k = 0; k < 2; k++) {
+        double* base = c->kr + (size_t)k * 6 * N;
+        set[k] = MsSet{base, base + N, base + 2 * N, base + 3 * N, base + 4 * N, base + 5 * N};
+    }
+    MinresState* sd = c->mstate;   // two slots, by parity
+    double* xs = c->ms_x;
+    const int rows = (int)std::max<int64_t>(8, (N + 255) / 256);
+    const unsigned nb = (unsigned)((N + rows - 1) / rows);
+    const size_t lds = sizeof(double) * (size_t)N;
+    auto launch_group = [&]() -> ps_err {
+        for (int k = 0; k < MS_POLL; k++) {
+            const int p = k & 1;
+            k_ms_iter<<<nb, VB, lds, st>>>(N, A, set[p], set[p ^ 1], xs, rows, sd + p, sd + (p ^ 1), maxit,
+                                           c->p.ksp_abstol);
+            PS_LAUNCH(c);
+        }
+        return PS_OK;
+    };
+    if (!c->ms_exec) {
+        hipGraph_t gr = nullptr;
+        PS_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const ps_err r = launch_group();
+        const hipError_t e = hipStreamEndCapture(st, &gr);
+        if (r != PS_OK) return r;
+        if (e != hipSuccess) {
+            c->set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            return PS_ERR_HIP;
+        }
+        const hipError_t e2 = hipGraphInstantiate(&c->ms_exec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e2 != hipSuccess) {
+            c->ms_exec = nullptr;
+            c->set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+            return PS_ERR_HIP;
+        }
+    }
+    k_ms_init<<<1, MS_T, 0, st>>>(N, b, xs, set[0].r1, set[0].r2, set[0].y, set[0].v, set[0].wa, set[0].wb, rtol,
+                                   c->p.ksp_abstol, sd);
     PS_LAUNCH(c);
-    const int MS_POLL = 8;
-    const unsigned gg = (unsigned)((N + 3) / 4);
+    k_ms_gemv<<<(unsigned)((N + 3) / 4), VB, 0, st>>>(N, A, set[0].v, set[0].y, sd);
+    PS_LAUNCH(c);
     int it = 0;
     for (;;) {
+        // after a whole number of groups the current state is in slot 0
         PS_HIP(c, hipMemcpyAsync(c->mstate_h, sd, sizeof(MinresState), hipMemcpyDeviceToHost, st));
         PS_HIP(c, hipStreamSynchronize(st));
         if (c->mstate_h[0].done || it >= maxit) break;
-        for (int k = 0; k < MS_POLL && it < maxit; k++, it++) {
-            k_ms_gemv<<<gg, VB, 0, st>>>(N, A, v, y, sd);
-            PS_LAUNCH(c);
-            k_ms_update<<<1, MS_T, 0, st>>>(N, dinv, r1, r2, y, v, wa, wb, x, maxit, c->p.ksp_abstol, sd);
-            PS_LAUNCH(c);
-        }
+        PS_HIP(c, hipGraphLaunch(c->ms_exec, st));
+        it += MS_POLL;
     }
+    PS_HIP(c, hipMemcpyAsync(x, xs, sizeof(double) * (size_t)N, hipMemcpyDeviceToDevice, st));
     const MinresState& s = c->mstate_h[0];
     const int reason = s.done ? s.reason : -3;
     if (info) {
@@ -716,10 +816,6 @@ static ps_err minres_small(Ctx* c, int64_t N, const double* A, const double* din
     return reason > 0 ? PS_OK : PS_ERR_NOCONV;
 }
 
-__global__ void k_mss_dinv(int64_t n, const double* __restrict__ A, double* __restrict__ d) {
-    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
-    if (i < n) d[i] = 1.0 / A[i * n + i];
-}
 // x = [-b_lambda (nu) ; 0 (5Nc)]  (f_u from the constraint rows B^T f = b_lambda)
 __global__ void k_bs_init(double* __restrict__ x, const double* __restrict__ b, int64_t n, int64_t nu) {
     const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
@@ -752,7 +848,7 @@ static ps_err block_solve_once(Ctx* c, const double* b, double* x, int prec, int
         k_bs_rs<<<eblocks(n5), VB, 0, st>>>(c->bs_r, b, c->bs_y, nu, n5);
         PS_LAUNCH(c);
         ps_solve_info si{};
-        ps_err r = minres_small(c, n5, c->Mss, c->mss_dinv, c->bs_r, x + nu, 1e-12, 10000, &si);
+        ps_err r = minres_small(c, n5, c->Mss, c->bs_r, x + nu, 1e-12, 10000, &si);
         if (r != PS_OK && r != PS_ERR_NOCONV) return r;
         if (mss_its) *mss_its += si.iterations;
     }
@@ -763,13 +859,6 @@ static ps_err block_solve_once(Ctx* c, const double* b, double* x, int prec, int
     return PS_OK;
 }
 
-ps_err block_prepare(Ctx* c) {
-    const int64_t n5 = 5 * (int64_t)c->Nc;
-    if (!c->Mss || n5 == 0) return PS_OK;
-    k_mss_dinv<<<eblocks(n5), VB, 0, c->stream>>>(n5, c->Mss, c->mss_dinv);
-    PS_LAUNCH(c);
-    return PS_OK;
-}
 
 ps_err saddle_solve_dev(Ctx* c, const double* b, double* x, int warm, ps_solve_info* info) {
     const bool block = c->p.solver == PS_SOLVER_BLOCK;
@@ -881,13 +970,12 @@ ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* out, ps_sqrt_info* info)
     std::vector<double> alpha, betad, g, gprev;
     k_scale<<<eblocks(n), VB, 0, st>>>(n, c->Q, xi, 1.0 / beta0);
     PS_LAUNCH(c);
-    int m = 0;
-    double relc = 0.0;
-    bool stop = false;
-    int j0 = 0;
-    while (!stop && j0 < kmax) {
-        const int j1 = std::min(kmax, j0 + LZ_POLL);   // iterations [j0, j1) on the device
-        for (int j = j0; j < j1; j++) {
+    // Iterations run in groups of LZ_POLL; group g + 1 is enqueued before the host reads group g
+    // and evaluates the stopping test, so the host's tridiagonal work overlaps the device.  If the
+    // test stops inside group g, the vectors group g + 1 builds are never used (the result equals
+    // stopping at m).
+    auto enqueue_group = [&](int g0, int g1, hipEvent_t ev) -> ps_err {
+        for (int j = g0; j < g1; j++) {
             // w = M q_j; two classical Gram-Schmidt passes against q_0..q_j (the first one also
             // yields alpha_j = q_j . M q_j); beta_j = |w|; q_{j+1} = w / beta_j.
             const double* qj = c->Q + (int64_t)j * n;
@@ -903,8 +991,27 @@ ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* out, ps_sqrt_info* info)
                                          (int)ne, c->lz + 64 + j);
             PS_LAUNCH(c);
         }
-        PS_HIP(c, hipMemcpyAsync(c->lz_h, c->lz, sizeof(double) * 128, hipMemcpyDeviceToHost, st));
-        PS_HIP(c, hipStreamSynchronize(st));
+        // this group's alpha and beta^2 only: the slices of the groups never overlap
+        PS_HIP(c, hipMemcpyAsync(c->lz_h + g0, c->lz + g0, sizeof(double) * (g1 - g0), hipMemcpyDeviceToHost, st));
+        PS_HIP(c, hipMemcpyAsync(c->lz_h + 64 + g0, c->lz + 64 + g0, sizeof(double) * (g1 - g0),
+                                 hipMemcpyDeviceToHost, st));
+        PS_HIP(c, hipEventRecord(ev, st));
+        return PS_OK;
+    };
+    int m = 0, last_nneg = 0;
+    bool have_g = false;
+    double relc = 0.0;
+    bool stop = false;
+    // groups of LZ_POLL, halving towards kmax so that the group whose test runs with nothing left to
+    // overlap is a single iteration
+    auto group_end = [&](int g0) { return g0 + std::max(1, std::min(LZ_POLL, (kmax - g0) / 2)); };
+    int j0 = 0, j1 = group_end(0), grp = 0;
+    PS_TRY(enqueue_group(j0, j1, c->lz_ev[0]));
+    while (!stop) {
+        const int n0 = j1, n1 = n0 < kmax ? group_end(n0) : n0;
+        const bool more = n0 < kmax;
+        if (more) PS_TRY(enqueue_group(n0, n1, c->lz_ev[(grp + 1) & 1]));
+        PS_HIP(c, hipEventSynchronize(c->lz_ev[grp & 1]));
         for (int j = j0; j < j1 && !stop; j++) {
             alpha.push_back(c->lz_h[j]);
             const double bj = std::sqrt(std::max(c->lz_h[64 + j], 0.0));
@@ -912,7 +1019,8 @@ ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* out, ps_sqrt_info* info)
             c->cnt.lanczos_iterations++;
             if (c->p.sqrt_rtol > 0.0) {
                 g.assign(m, 0.0);
-                tridiag_sqrt_weights(m, alpha.data(), betad.data(), beta0, g.data());
+                last_nneg = tridiag_sqrt_weights(m, alpha.data(), betad.data(), beta0, g.data());
+                have_g = true;
                 if (m >= 2) {
                     double dn = 0.0, gn = 0.0;
                     for (int i = 0; i < m; i++) {
@@ -932,10 +1040,17 @@ ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* out, ps_sqrt_info* info)
             }
             betad.push_back(bj);
         }
-        j0 = j1;
+        if (!more) break;
+        j0 = n0;
+        j1 = n1;
+        grp++;
     }
-    g.assign(m, 0.0);
-    const int nneg = tridiag_sqrt_weights(m, alpha.data(), betad.data(), beta0, g.data());
+    // the stopping test already produced the weights of the final m
+    if (!(have_g && (int)g.size() == m)) {
+        g.assign(m, 0.0);
+        last_nneg = tridiag_sqrt_weights(m, alpha.data(), betad.data(), beta0, g.data());
+    }
+    const int nneg = last_nneg;
     if (nneg > 0) {
         c->cnt.lanczos_nneg_events++;
         c->cnt.lanczos_nneg_count += nneg;
@@ -970,13 +1085,13 @@ int tridiag_sqrt_weights(int m, const double* alpha, const double* betad, double
             if (mm != l) {
                 if (iter++ == 200) break;
                 double gg = (d[l + 1] - d[l]) / (2.0 * e[l]);
-                double r = std::hypot(gg, 1.0);
+                double r = std::sqrt(gg * gg + 1.0);
                 gg = d[mm] - d[l] + e[l] / (gg + (gg >= 0 ? std::fabs(r) : -std::fabs(r)));
                 double s = 1.0, cc = 1.0, p = 0.0;
                 bool underflow = false;
                 for (int i = mm - 1; i >= l; i--) {
                     const double f = s * e[i], b = cc * e[i];
-                    r = std::hypot(f, gg);
+                    r = std::sqrt(f * f + gg * gg);
                     e[i + 1] = r;
                     if (r == 0.0) {
                         d[i + 1] -= p;

@@ -1025,7 +1025,6 @@ ps_err mob_prepare(Ctx* c) {
     if (c->Mss && c->Nc > 0) {
         k_mss<<<c->Nc, PS_BLOCK, 0, c->stream>>>(c->sc, c->xm, c->Mss);
         PS_LAUNCH(c);
-        PS_TRY(block_prepare(c));
     }
     c->cnt.mob_calls++;
     return PS_OK;

@@ -109,15 +109,17 @@ struct Ctx {
     double* Q = nullptr;            // Lanczos basis kmax x n
     double* lz = nullptr;           // Lanczos scalars on device (alpha, beta, h) [3 * kmax + 8]
     double* lz_h = nullptr;         // pinned host mirror
+    hipEvent_t lz_ev[2] = {nullptr, nullptr};   // Lanczos polling groups (alternating)
     double* gw = nullptr;           // recombination weights (device) kmax
     double *ir_r = nullptr, *ir_d = nullptr;   // refinement residual / correction (ns)
     int kmax = 0;
     // ---- block-elimination solver (PS_SOLVER_BLOCK)
     double* Mss = nullptr;          // 5Nc x 5Nc stresslet block, rebuilt by mob()
-    double* mss_dinv = nullptr;     // 1 / diag(M_ss)
     double* bs_y = nullptr;         // n
     double* bs_r = nullptr;         // 5Nc
     double* bs_x = nullptr;         // ns (correction of a refinement sweep)
+    double* ms_x = nullptr;         // 5Nc: solution of the M_ss solve (fixed address for the graph)
+    hipGraphExec_t ms_exec = nullptr;   // MS_POLL iterations of the M_ss solve, captured once
     // ---- Verlet list (check_dist)
     int ncell[3] = {0, 0, 0};
     int cell_mode = 0;              // 0: brute force; 1: periodic cell list
@@ -188,7 +190,6 @@ ps_err krylov_alloc(Ctx* c);
 void krylov_free(Ctx* c);
 ps_err saddle_diag_dev(Ctx* c, double* d);
 ps_err saddle_solve_dev(Ctx* c, const double* b, double* x, int warm, ps_solve_info* info);
-ps_err block_prepare(Ctx* c);   // after mob(): M_ss Jacobi diagonal
 ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* w, ps_sqrt_info* info);
 ps_err pairs_alloc(Ctx* c);
 void pairs_free(Ctx* c);
