@@ -341,12 +341,6 @@ ps_err Ctx::prof_harvest() {
 }
 
 ps_err Ctx::prof_record(int kind, hipEvent_t* a, hipEvent_t* b) {
-    Ctx* c = this;
-    if (ev_pool.empty()) {
-        ev_pool.resize(2 * EV_POOL);
-        ev_kind.resize(EV_POOL);
-        for (auto& e : ev_pool) PS_HIP(c, hipEventCreate(&e));
-    }
     if (ev_used >= EV_POOL) PS_TRY(prof_harvest());
     ev_kind[ev_used] = kind;
     *a = ev_pool[2 * ev_used];
@@ -361,6 +355,11 @@ ps_err ps_profile_begin(void* h) {
     Ctx* c = as_ctx(h);
     if (!c) return PS_ERR_ARG;
     PS_TRY(c->prof_harvest());
+    if (c->ev_pool.empty()) {   // created here, outside any timed region
+        c->ev_pool.resize(2 * EV_POOL);
+        c->ev_kind.resize(EV_POOL);
+        for (auto& e : c->ev_pool) PS_HIP(c, hipEventCreate(&e));
+    }
     c->prof = ps_profile{};
     c->prof.aa_pairs = (int64_t)c->Nm * c->Nm;
     c->prof_on = true;

@@ -308,6 +308,7 @@ __global__ __launch_bounds__(VB) void k_lz_sub(int64_t n, const double* __restri
     if (threadIdx.x < m) {
         const double* r = red + (int64_t)threadIdx.x * nchunk;
         double v = 0.0;
+#pragma unroll 8
         for (int c = 0; c < nchunk; c++) v += r[c];
         h[threadIdx.x] = v;
     }

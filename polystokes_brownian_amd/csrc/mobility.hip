@@ -203,15 +203,20 @@ __device__ __forceinline__ double wave_rol1(double v) {
     return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
 
+template <class T>
+struct AASmem {
+    typename Vec4<T>::type S[512], F[512];
+    T A[3][PS_BLOCK];
+};
+
+// One tile pair (I, J); PAD: the pair involves the partial last tile (ghost beads beyond Nm).
 template <class T, bool BOX, bool PAD>
-__global__ __launch_bounds__(PS_BLOCK) void k_aa_sym(int Nm, const int2* __restrict__ pairs, AAConst k,
-                                                     const T* __restrict__ xs, const double* __restrict__ f,
-                                                     T* __restrict__ part) {
+__device__ __forceinline__ void aa_sym_tile(int Nm, int I, int J, const AAConst& k, const T* __restrict__ xs,
+                                            const double* __restrict__ f, T* __restrict__ part, AASmem<T>& sm) {
     using V = typename Vec4<T>::type;
-    __shared__ V S[512], F[512];
-    __shared__ T A[3][PS_BLOCK];
-    const int2 pr = pairs[blockIdx.x];
-    const int I = pr.x, J = pr.y;
+    V* S = sm.S;
+    V* F = sm.F;
+    T(*A)[PS_BLOCK] = sm.A;
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     {
         const int j = J * PS_BLOCK + t;
@@ -354,6 +359,20 @@ __global__ __launch_bounds__(PS_BLOCK) void k_aa_sym(int Nm, const int2* __restr
 #pragma unroll
         for (int c = 0; c < 3; c++) part[((int64_t)I * 3 + c) * Nm + j] = A[c][t];
     }
+}
+
+// The pairs that involve the partial last tile come first (npad of them), so their blocks start
+// early and overlap the rest of the launch instead of forming a tail.
+template <class T, bool BOX>
+__global__ __launch_bounds__(PS_BLOCK) void k_aa_sym(int Nm, const int2* __restrict__ pairs, int npad, AAConst k,
+                                                     const T* __restrict__ xs, const double* __restrict__ f,
+                                                     T* __restrict__ part) {
+    __shared__ AASmem<T> sm;
+    const int2 pr = pairs[blockIdx.x];
+    if ((int)blockIdx.x < npad)
+        aa_sym_tile<T, BOX, true>(Nm, pr.x, pr.y, k, xs, f, part, sm);
+    else
+        aa_sym_tile<T, BOX, false>(Nm, pr.x, pr.y, k, xs, f, part, sm);
 }
 
 // Scaled monomer positions for k_aa_sym: [Nm][4] (x, y, z, 0) * inv_L.
@@ -714,6 +733,8 @@ __global__ __launch_bounds__(PS_BLOCK) void k_finalize(SysConst s, const double*
     if (r < nm3) {
         const int i = (int)(r / 3), comp = (int)(r - 3 * (int64_t)i);
         double acc = 0.0;
+        // the loads are independent: unrolled so that several are in flight (the sum order is fixed)
+#pragma unroll 8
         for (int sp = 0; sp < nsplit; sp++) acc += (double)part_aa[((int64_t)sp * 3 + comp) * Nm + i];
         for (int cm = 0; cm < nchunk_m; cm++) acc += (double)part_abm[((int64_t)cm * 3 + comp) * Nm + i];
         v = s.beta_inv * f[r] + acc;
@@ -860,21 +881,12 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
             k.tb2s = s.two_beta * s.two_beta / (L * L);
             k.clamp = sizeof(T) == sizeof(float) ? 1e-30 : 1e-200;
             const T* xs = (const T*)aw.xs;
-            if (aw.npair_full > 0) {
-                if (box)
-                    k_aa_sym<T, true, false><<<aw.npair_full, PS_BLOCK, 0, st>>>(s.Nm, aw.pairs, k, xs, f, (T*)aw.part_aa);
-                else
-                    k_aa_sym<T, false, false><<<aw.npair_full, PS_BLOCK, 0, st>>>(s.Nm, aw.pairs, k, xs, f, (T*)aw.part_aa);
-                PS_LAUNCH(c);
-            }
-            if (aw.npair_pad > 0) {
-                const int2* pp = aw.pairs + aw.npair_full;
-                if (box)
-                    k_aa_sym<T, true, true><<<aw.npair_pad, PS_BLOCK, 0, st>>>(s.Nm, pp, k, xs, f, (T*)aw.part_aa);
-                else
-                    k_aa_sym<T, false, true><<<aw.npair_pad, PS_BLOCK, 0, st>>>(s.Nm, pp, k, xs, f, (T*)aw.part_aa);
-                PS_LAUNCH(c);
-            }
+            const int nblk = aw.npair_full + aw.npair_pad;
+            if (box)
+                k_aa_sym<T, true><<<nblk, PS_BLOCK, 0, st>>>(s.Nm, aw.pairs, aw.npair_pad, k, xs, f, (T*)aw.part_aa);
+            else
+                k_aa_sym<T, false><<<nblk, PS_BLOCK, 0, st>>>(s.Nm, aw.pairs, aw.npair_pad, k, xs, f, (T*)aw.part_aa);
+            PS_LAUNCH(c);
         } else if (box && ch)
             k_aa<T, true, true><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
         else if (box)
@@ -954,9 +966,9 @@ ps_err alloc_work(Ctx* c, ApplyWork& aw, size_t elt) {
             }
         aw.npair_full = (int)full.size();
         aw.npair_pad = (int)pad.size();
-        full.insert(full.end(), pad.begin(), pad.end());
-        PS_HIP(c, hipMalloc(&aw.pairs, sizeof(int2) * full.size()));
-        PS_HIP(c, hipMemcpy(aw.pairs, full.data(), sizeof(int2) * full.size(), hipMemcpyHostToDevice));
+        pad.insert(pad.end(), full.begin(), full.end());   // ghost-bead pairs first
+        PS_HIP(c, hipMalloc(&aw.pairs, sizeof(int2) * pad.size()));
+        PS_HIP(c, hipMemcpy(aw.pairs, pad.data(), sizeof(int2) * pad.size(), hipMemcpyHostToDevice));
         PS_HIP(c, hipMalloc(&aw.xs, 4 * elt * (size_t)s.Nm));
     }
     aw.bytes_aa = (size_t)aw.nsplit_aa * 3 * std::max(s.Nm, 1) * elt;

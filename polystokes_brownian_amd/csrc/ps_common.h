@@ -53,8 +53,8 @@ struct MinresState {
 
 // Apply workspace (partial sums) sized at create time.
 struct ApplyWork {
-    // symmetric AA kernel: 256-bead tiles, upper-triangle tile pairs (full tiles first, then the
-    // pairs that involve the partial last tile), scaled positions [Nm][4]
+    // symmetric AA kernel: 256-bead tiles, upper-triangle tile pairs (the pairs that involve the
+    // partial last tile first, then the full ones), scaled positions [Nm][4]
     bool sym = false;
     int ntile = 0, npair_full = 0, npair_pad = 0;
     int2* pairs = nullptr;
