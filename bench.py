@@ -8,16 +8,18 @@ the corrector solve and the integrator.  Inputs are synthetic (seeded), resident
 timed region; the timed region is exactly `steps` full steps.
 
 Forms timed:
-  mixed/block  (headline)  FP32 tiles, FP64 Krylov recurrences, block-eliminated saddle solve with
-                           FP64 iterative refinement of the saddle residual to 1e-7;
+  mixed/block  (headline)  FP32 tiles for the 32 applies of the Lanczos square root, FP64 Lanczos
+                           recurrences, and the saddle solves by exact block elimination in FP64;
   fp64/block               the same algorithm in FP64 (the FP64 GPU form the ratio is against);
   fp64/minres              the reference algorithm (Jacobi MINRES, rtol 1e-6) in FP64, 1 step.
 Each form's velocity `up` after one step from the same state and noise is compared with the FP64
 form of the same algorithm (relative error).  The roofline entry is the dominant kernel (the
-monomer-monomer tile kernel) of the headline form, timed live with HIP events bracketing every
-launch on the library's stream.  The CPU baseline is the FP64 oracle (the reference algorithm,
-matrix-free applies, OpenMP on the host cores) timed on a bounded sample of applies and scaled by
-the applies per step of the corresponding GPU form.
+symmetric monomer-monomer tile kernel) of the headline form: its launches record their own start
+and stop events on the library's stream (hipExtLaunchKernel) over the timed region.  Its HBM bytes
+per launch come from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a plain 256 MiB copy
+(--calibrate-copy), as constants below.  The CPU baseline is the FP64 oracle (matrix-free applies,
+OpenMP on the host cores) timed on a bounded sample of applies and scaled by the applies per step
+of the reference algorithm on the GPU (and, separately, of the block algorithm).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3] [--no-cpu] [--no-minres]
 """
@@ -33,6 +35,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}
+# FP32 VALU peak without packed (dual-issue) FP32 instructions, which this build does not emit:
+# 256 CUs x 64 FMA lanes x 2 flop x 2.4 GHz.
+PEAK_FP32_VALU = 78.6e12
+# HBM bytes per launch of the symmetric monomer-monomer kernel at N = 20,000 (19,800 monomers), from
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE of `bench.py --profile-form <form> --steps 2
+# --warmup 0 --no-cpu --calibrate-copy 256`, each scaled by what the counter reads for the 256 MiB
+# calibration copy (FETCH_SIZE 0.50x, WRITE_SIZE 1.00x of the bytes moved); see the summary in
+# profiles/k_aa_sym_hbm_traffic_n20000.md.  FP32: 3483 KiB fetched, 18099 KiB written per launch.
+TRAFFIC_K_AA = {"fp32": (3483.0 / 0.50 + 18099.0 / 1.00) * 1024, "fp64": (4722.6 / 0.50 + 36197.3 / 1.00) * 1024}
 
 
 def log(*a):
@@ -91,6 +102,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-applies", type=int, default=3)
     ap.add_argument("--no-minres", action="store_true", help="skip timing the reference MINRES algorithm")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device (a plain copy of known size, to scale HBM counters)")
     ap.add_argument("--profile-form", default=None,
                     help="only run warmup + timed steps of this form (for a profiler), no JSON line")
     args = ap.parse_args()
@@ -101,6 +114,18 @@ def main():
 
     from polystokes_brownian_amd import configs
     from polystokes_brownian_amd.polystokes import Context
+
+    if args.calibrate_copy > 0:
+        import torch
+        nel = args.calibrate_copy * (1 << 20) // 8
+        src = torch.randn(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {nel * 8} bytes read and written per dispatch, 2 dispatches")
+        del src, dst
 
     w = configs.workload(args.workload)
     pd = configs.params_dict(w)
@@ -174,25 +199,33 @@ def main():
         cpu = {"value": Np / (t_apply * applies), "unit": "bead-steps/s", "cores": get_threads(), "kind": "port",
                "sample": f"{args.cpu_applies} FP64 matrix-free mobility applies of the oracle at N={Np} "
                          f"({t_apply:.3f} s each on {get_threads()} threads), scaled by the {applies:.0f} applies "
-                         f"per step of the {ref_form} GPU form (the reference algorithm)"}
+                         f"per step of the {ref_form} GPU form"
+                         + (" (the reference algorithm: Jacobi MINRES on the saddle system)"
+                            if ref_form == "fp64/minres" else " (block-eliminated saddle solves)")}
         ab = results["fp64/block"]["per_step"]["applies"]
-        cpu_block = {"value": Np / (t_apply * ab), "applies_per_step": ab}
+        cpu_block = {"value": Np / (t_apply * ab), "unit": "bead-steps/s", "applies_per_step": ab,
+                     "sample": "the same applies, scaled by the applies per step of the fp64/block GPU form"}
         log(f"cpu baseline {cpu}")
 
     head = results["mixed/block"]
     f64 = results["fp64/block"]
+    traffic = TRAFFIC_K_AA["fp32"] if args.workload == "c3" else None
     roof = {"bound": "mfma", "achieved": head["achieved"] / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
-            "frac": head["achieved"] / PEAK["fp32"], "traffic": None,
-            "kernel": "k_aa<float> (monomer-monomer tiles; VALU-bound, packed FP32 disabled)",
-            "flops_per_launch": 18.0 * Nm * Nm, "avg_launch_ms": head["aa_ms_avg"]}
+            "frac": head["achieved"] / PEAK["fp32"], "traffic": traffic,
+            "kernel": "k_aa_sym<float> (monomer-monomer tiles; compute-bound on the VALU: packed FP32 and "
+                      "MFMA do not apply to one-vector tile generation)",
+            "flops_per_launch": 18.0 * Nm * Nm, "avg_launch_ms": head["aa_ms_avg"],
+            "frac_of_fp32_valu_peak": head["achieved"] / PEAK_FP32_VALU,
+            "traffic_gbs": (traffic / (head["aa_ms_avg"] * 1e-3) / 1e9) if traffic else None}
     out = {
         "metric": "bead-steps/s at N=20k: mixed-prec vs FP64 GPU at 1e-6 rel err; % roofline",
         "value": head["value"], "unit": "bead-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32+fp64", "data": "synthetic",
         "config": {"workload": f"{args.workload}: {Np} beads ({w['Nc']} probes + {w['Npoly']} chains x 20), "
-                               f"box {w['L']:.1f}, full mm-HI, mixed precision, block-eliminated saddle solve "
-                               f"with FP64 refinement, Lanczos m<={pd['sqrt_kmax']} rtol {pd['sqrt_rtol']}",
+                               f"box {w['L']:.1f}, full mm-HI, mixed precision (FP32 tiles in the Lanczos "
+                               f"square root), exact FP64 block-eliminated saddle solves, Lanczos "
+                               f"m<={pd['sqrt_kmax']} rtol {pd['sqrt_rtol']}",
                    "global_batch": Np, "seq_len": 1, "parallelism": "single-gpu"},
         "vel_rel_err_vs_fp64": head["vel_rel_err"],
         "fp64_value": f64["value"],
@@ -201,7 +234,8 @@ def main():
                                              "apply_ms_avg")} for k, v in results.items()},
         "roofline": roof,
         "roofline_fp64": {"achieved": f64["achieved"] / 1e12, "peak": PEAK["fp64"] / 1e12,
-                          "frac": f64["achieved"] / PEAK["fp64"], "avg_launch_ms": f64["aa_ms_avg"]},
+                          "frac": f64["achieved"] / PEAK["fp64"], "avg_launch_ms": f64["aa_ms_avg"],
+                          "traffic": TRAFFIC_K_AA["fp64"] if args.workload == "c3" else None},
         "cpu_baseline": cpu,
         "cpu_baseline_block_algorithm": cpu_block,
     }

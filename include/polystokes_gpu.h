@@ -194,7 +194,10 @@ ps_err ps_rhs(void* ctx, int drift, double* rhs);
 ps_err ps_run(void* ctx, int nsteps);
 /* Kernel timing with HIP events on the context's stream (for roofline figures): between begin and
  * end every monomer-monomer tile launch (the dominant kernel) and every whole mobility apply is
- * bracketed by events; end synchronises and returns the summed durations. */
+ * timed; end synchronises and returns the summed durations.  The symmetric tile kernel's events are
+ * recorded by its own dispatch (hipExtLaunchKernel), so they measure the kernel alone; under
+ * rocprofv3 (ROCPROF_OUTPUT_PATH set), whose dispatch interception crashes together with such
+ * events, it is bracketed by event records like the other kernels (a few microseconds longer). */
 typedef struct ps_profile {
     int64_t aa_launches;  double aa_ms;      /* monomer-monomer tile kernel */
     int64_t apply_launches; double apply_ms; /* whole apply (all tile kernels + finalize) */

@@ -11,6 +11,7 @@
 #include "ps_common.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <condition_variable>
 #include <memory>
@@ -362,6 +363,7 @@ ps_err ps_profile_begin(void* h) {
     }
     c->prof = ps_profile{};
     c->prof.aa_pairs = (int64_t)c->Nm * c->Nm;
+    c->prof_dispatch_events = std::getenv("ROCPROF_OUTPUT_PATH") == nullptr;
     c->prof_on = true;
     return PS_OK;
 }

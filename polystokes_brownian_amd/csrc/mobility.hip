@@ -18,6 +18,8 @@
 // src/mob.cpp:220-227), so no 3x5 or 5x5 block is ever materialised for AB pairs.
 #include "ps_common.h"
 
+#include <hip/hip_ext.h>
+
 namespace {
 
 constexpr double C1D2 = 1. / 2., C1D3 = 1. / 3., C3D2 = 3. / 2., C3D4 = 3. / 4., C3D8 = 3. / 8.,
@@ -865,10 +867,12 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
     }
     if (s.Nm > 0 && !srows) {
         dim3 g1((s.Nm + PS_BLOCK - 1) / PS_BLOCK, aw.nsplit_aa);
-        if (c->prof_on) {
-            PS_TRY(c->prof_record(0, &ek0, &ek1));
-            PS_HIP(c, hipEventRecord(ek0, st));
-        }
+        // profiling: the symmetric kernel's own dispatch records its start/stop (hipExtLaunchKernel);
+        // the ordered kernel, and every kernel under rocprofv3 (whose dispatch interception crashes
+        // the process together with hipExtLaunchKernel's events), is bracketed by event records
+        const bool dispatch_ev = aw.sym && c->prof_dispatch_events;
+        if (c->prof_on) PS_TRY(c->prof_record(0, &ek0, &ek1));
+        if (c->prof_on && !dispatch_ev) PS_HIP(c, hipEventRecord(ek0, st));
         const bool box = s.box_active, ch = s.chain_HI;
         if (aw.sym) {
             AAConst k{};
@@ -882,10 +886,16 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
             k.clamp = sizeof(T) == sizeof(float) ? 1e-30 : 1e-200;
             const T* xs = (const T*)aw.xs;
             const int nblk = aw.npair_full + aw.npair_pad;
+            const int2* pairs = aw.pairs;
+            const int npad = aw.npair_pad;
+            T* part = (T*)aw.part_aa;
+            hipEvent_t e0 = dispatch_ev ? ek0 : nullptr, e1 = dispatch_ev ? ek1 : nullptr;
             if (box)
-                k_aa_sym<T, true><<<nblk, PS_BLOCK, 0, st>>>(s.Nm, aw.pairs, aw.npair_pad, k, xs, f, (T*)aw.part_aa);
+                hipExtLaunchKernelGGL(k_aa_sym<T, true>, dim3(nblk), dim3(PS_BLOCK), 0, st, e0, e1, 0, s.Nm, pairs,
+                                      npad, k, xs, f, part);
             else
-                k_aa_sym<T, false><<<nblk, PS_BLOCK, 0, st>>>(s.Nm, aw.pairs, aw.npair_pad, k, xs, f, (T*)aw.part_aa);
+                hipExtLaunchKernelGGL(k_aa_sym<T, false>, dim3(nblk), dim3(PS_BLOCK), 0, st, e0, e1, 0, s.Nm,
+                                      pairs, npad, k, xs, f, part);
             PS_LAUNCH(c);
         } else if (box && ch)
             k_aa<T, true, true><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
@@ -896,7 +906,7 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
         else
             k_aa<T, false, false><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
         PS_LAUNCH(c);
-        if (c->prof_on) PS_HIP(c, hipEventRecord(ek1, st));
+        if (c->prof_on && !dispatch_ev) PS_HIP(c, hipEventRecord(ek1, st));
         dim3 g2((s.Nm + PS_BLOCK - 1) / PS_BLOCK, aw.nchunk_m);
         if (box)
             k_ab_mono<T, true><<<g2, PS_BLOCK, 0, st>>>(s, xm, f, c->stale, aw.per_chunk_m, (T*)aw.part_abm);

@@ -143,6 +143,7 @@ struct Ctx {
     hipEvent_t noise_ev[2] = {nullptr, nullptr};
     // ---- event timing (ps_profile_begin/end)
     bool prof_on = false;
+    bool prof_dispatch_events = true;   // false under rocprofv3 (ROCPROF_OUTPUT_PATH set)
     std::vector<hipEvent_t> ev_pool;
     int ev_used = 0;
     std::vector<int> ev_kind;       // 0: AA kernel, 1: whole apply (pairs of events)

@@ -283,12 +283,20 @@ class PolyStokes:
     def trap_info(self, ktrap, tstart, trun, weaken_trap=-1):
         self._pd.update(ktrap=ktrap, tstart=tstart, trun=trun, weaken_trap=weaken_trap)
 
-    def set_solver_options(self, precision=None, ksp_rtol=None, sqrt_rtol=None, sqrt_kmax=None, ir_rtol=None):
+    def set_solver_options(self, precision=None, ksp_rtol=None, sqrt_rtol=None, sqrt_kmax=None, ir_rtol=None,
+                           solver=None, aa_kernel=None):
+        """MI355X forms (no reference counterpart): precision "fp64" | "mixed"; solver "minres" (the
+        reference's KSPMINRES + PCJACOBI) | "block" (exact block elimination); aa_kernel "auto" |
+        "ordered" | "symmetric"; tolerances as in ps_params."""
         for k, v in dict(ksp_rtol=ksp_rtol, sqrt_rtol=sqrt_rtol, sqrt_kmax=sqrt_kmax, ir_rtol=ir_rtol).items():
             if v is not None:
                 self._pd[k] = v
         if precision is not None:
             self._pd["precision"] = {"fp64": PS_PREC_FP64, "mixed": PS_PREC_MIXED}.get(precision, precision)
+        if solver is not None:
+            self._pd["solver"] = {"minres": 0, "block": 1}.get(solver, solver)
+        if aa_kernel is not None:
+            self._pd["aa_kernel"] = {"auto": 0, "ordered": 1, "symmetric": 2}.get(aa_kernel, aa_kernel)
 
     def set_warn_neg_eig(self, on=True):
         self.warn_neg_eig = bool(on)
