@@ -93,6 +93,53 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+PEAK_MFMA_16BIT = 2.5e15   # dense BF16 / FP16 MFMA, MI355X
+HBM_PEAK = 8.0e12
+
+
+def dense_bench(args, configs, Context):
+    """Stored-dense M (16-bit, MFMA operand order) applied to 16 vectors at once: HBM GB/s of the
+    apply kernel (it reads the stored M once per call), MFMA utilisation, and the error of M X
+    against the FP64 on-the-fly apply."""
+    wl = args.workload if args.workload != "c3" else "c5dense"
+    w = configs.workload(wl)
+    pd = configs.params_dict(w)
+    x0 = configs.positions(w)
+    ctx = make_ctx(Context, pd, x0)
+    ctx.check_dist()
+    ctx.mob()
+    n = ctx.n
+    t0 = time.perf_counter()
+    ctx.dense_build(args.dense)
+    ctx.state()
+    t_build = time.perf_counter() - t0
+    log(f"dense {args.dense}: n = {n}, build {t_build * 1e3:.1f} ms (incl. allocation)")
+    X = np.random.default_rng(7).normal(size=(16, n))
+    Y = ctx.dense_mult(X)   # warmup
+    reps = max(args.steps, 5)
+    ctx.profile_begin()
+    for _ in range(reps):
+        Y = ctx.dense_mult(X)
+    prof = ctx.profile_end()
+    t = prof["dense_ms"] / max(prof["dense_launches"], 1) * 1e-3
+    ref = np.stack([ctx.mob_mult(X[j]) for j in range(16)])
+    err = rel(Y, ref)
+    col_err = max(rel(Y[j], ref[j]) for j in range(16))
+    npad = (n + 31) // 32 * 32
+    bytes_m = npad * npad * 2
+    flops = 2.0 * n * n * 16
+    out = {"metric": "stored-dense 16-bit MFMA multi-vector apply (16 vectors)", "value": bytes_m / t / 1e9,
+           "unit": "GB/s", "higher_is_better": True, "n_gpus": 1, "steps": reps, "warmup": 1,
+           "ms_per_step": t * 1e3, "dtype": args.dense + " storage, fp32 accumulation", "data": "synthetic",
+           "config": {"workload": f"{wl}: {w['Np']} beads, n = {n}, 16 vectors"},
+           "roofline": {"bound": "hbm", "achieved": bytes_m / t / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": bytes_m / t / HBM_PEAK, "traffic": bytes_m},
+           "mfma_tflops": flops / t / 1e12, "mfma_utilisation": flops / t / PEAK_MFMA_16BIT,
+           "rel_err_vs_fp64_apply": err, "worst_vector_rel_err": col_err, "build_ms": t_build * 1e3}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +151,9 @@ def main():
     ap.add_argument("--no-minres", action="store_true", help="skip timing the reference MINRES algorithm")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (a plain copy of known size, to scale HBM counters)")
+    ap.add_argument("--dense", default=None, metavar="STORAGE", choices=["bf16", "f16"],
+                    help="instead: the stored-dense MFMA multi-vector apply (16 vectors) at --workload "
+                         "(default c5dense, N = 8,000) with STORAGE; prints its own JSON line")
     ap.add_argument("--profile-form", default=None,
                     help="only run warmup + timed steps of this form (for a profiler), no JSON line")
     args = ap.parse_args()
@@ -126,6 +176,9 @@ def main():
         torch.cuda.synchronize()
         log(f"calibration copy: {nel * 8} bytes read and written per dispatch, 2 dispatches")
         del src, dst
+
+    if args.dense:
+        return dense_bench(args, configs, Context)
 
     w = configs.workload(args.workload)
     pd = configs.params_dict(w)

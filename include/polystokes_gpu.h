@@ -202,9 +202,21 @@ typedef struct ps_profile {
     int64_t aa_launches;  double aa_ms;      /* monomer-monomer tile kernel */
     int64_t apply_launches; double apply_ms; /* whole apply (all tile kernels + finalize) */
     int64_t aa_pairs;                        /* target x source monomer pairs per AA launch */
+    int64_t dense_launches; double dense_ms; /* stored-dense multi-vector apply kernel */
 } ps_profile;
 ps_err ps_profile_begin(void* ctx);
 ps_err ps_profile_end(void* ctx, ps_profile* out);
+/* Stored-dense grand mobility on MFMA (an MI355X form).  The reference's counterpart is the dense
+ * assembly of mob() (MatSetValues INSERT into A, src/mob.cpp:635-1213) followed by PETSc's dense
+ * MatMult.  ps_dense_build stores the M of the last ps_mob, rounded once to 16 bits, in the
+ * operand order of v_mfma_f32_16x16x32_{bf16,f16} (n rounded up to 32 in both dimensions:
+ * 2 bytes per entry).  ps_dense_mult applies it to nrhs <= 16 vectors at once with FP32
+ * accumulation: X and Y hold nrhs vectors of length n one after the other (X[j n + i]).  One call
+ * reads the stored M once, whatever nrhs.  Timing: ps_profile's dense_* fields. */
+enum { PS_DENSE_BF16 = 1, PS_DENSE_F16 = 2 };
+ps_err ps_dense_build(void* ctx, int storage);
+ps_err ps_dense_mult(void* ctx, const double* X, double* Y, int nrhs);        /* host pointers */
+ps_err ps_dense_mult_dev(void* ctx, const double* X, double* Y, int nrhs);    /* device pointers */
 
 /* Normal draws of the context's noise stream (std::mt19937_64 + std::normal_distribution<double>,
  * src/init.cpp:105-110) -- the draws the next step would consume; advances the stream. */

@@ -75,6 +75,10 @@ def sizes(p):
     return n, nu, n + nu
 
 
+PS_DENSE_BF16, PS_DENSE_F16 = 1, 2
+
+
 class PsProfile(C.Structure):
     _fields_ = [("aa_launches", C.c_int64), ("aa_ms", C.c_double), ("apply_launches", C.c_int64),
-                ("apply_ms", C.c_double), ("aa_pairs", C.c_int64)]
+                ("apply_ms", C.c_double), ("aa_pairs", C.c_int64), ("dense_launches", C.c_int64),
+                ("dense_ms", C.c_double)]

@@ -305,6 +305,8 @@ void destroy_ctx(Ctx* c) {
     apply_free(c);
     krylov_free(c);
     pairs_free(c);
+    dense_free(c);
+    if (c->dense_io) (void)hipFree(c->dense_io);
     for (void* p : {(void*)c->x, (void*)c->xpd, (void*)c->xm, (void*)c->xpos0, (void*)c->q, (void*)c->x_n,
                     (void*)c->chain, (void*)c->up, (void*)c->udiff, (void*)c->v_det_n, (void*)c->v_det_c,
                     (void*)c->v_brown, (void*)c->fext, (void*)c->drift_v, (void*)c->sb, (void*)c->rhs,
@@ -332,9 +334,12 @@ ps_err Ctx::prof_harvest() {
         if (ev_kind[k] == 0) {
             prof.aa_launches++;
             prof.aa_ms += ms;
-        } else {
+        } else if (ev_kind[k] == 1) {
             prof.apply_launches++;
             prof.apply_ms += ms;
+        } else {
+            prof.dense_launches++;
+            prof.dense_ms += ms;
         }
     }
     ev_used = 0;
@@ -664,6 +669,30 @@ ps_err ps_mob_mult(void* h, const double* f, double* u) {
     if (!c || !f || !u) return PS_ERR_ARG;
     if (!c->have_mob) return fail(c, PS_ERR_STATE, "apply before mob");
     return with_io(c, c->n, f, c->n, u, [&](double* a, double* b) { return mob_apply(c, a, b, c->p.precision); });
+}
+ps_err ps_dense_build(void* h, int storage) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    return dense_build(c, storage);
+}
+ps_err ps_dense_mult_dev(void* h, const double* X, double* Y, int nrhs) {
+    Ctx* c = as_ctx(h);
+    if (!c || !X || !Y) return PS_ERR_ARG;
+    return dense_mult_dev(c, X, Y, nrhs);
+}
+ps_err ps_dense_mult(void* h, const double* X, double* Y, int nrhs) {
+    Ctx* c = as_ctx(h);
+    if (!c || !X || !Y) return PS_ERR_ARG;
+    if (nrhs < 1 || nrhs > 16) return fail(c, PS_ERR_ARG, "dense_mult: 1 <= nrhs <= 16");
+    const size_t len = (size_t)nrhs * (size_t)c->n;
+    if (!c->dense_io) PS_HIP(c, hipMalloc(&c->dense_io, sizeof(double) * 2 * 16 * (size_t)c->n));
+    double* dx = c->dense_io;
+    double* dy = c->dense_io + 16 * (size_t)c->n;
+    PS_HIP(c, hipMemcpyAsync(dx, X, sizeof(double) * len, hipMemcpyHostToDevice, c->stream));
+    PS_TRY(dense_mult_dev(c, dx, dy, nrhs));
+    PS_HIP(c, hipMemcpyAsync(Y, dy, sizeof(double) * len, hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    return PS_OK;
 }
 ps_err ps_saddle_mult_dev(void* h, const double* x, double* y) {
     Ctx* c = as_ctx(h);

@@ -146,7 +146,7 @@ struct Ctx {
     bool prof_dispatch_events = true;   // false under rocprofv3 (ROCPROF_OUTPUT_PATH set)
     std::vector<hipEvent_t> ev_pool;
     int ev_used = 0;
-    std::vector<int> ev_kind;       // 0: AA kernel, 1: whole apply (pairs of events)
+    std::vector<int> ev_kind;       // 0: AA kernel, 1: whole apply, 2: dense apply (pairs of events)
     ps_profile prof{};
     ps_err prof_record(int kind, hipEvent_t* a, hipEvent_t* b);   // reserve a start/stop pair
     ps_err prof_harvest();
@@ -157,6 +157,12 @@ struct Ctx {
     // ---- scratch used by the host-pointer entry points
     double* io_a = nullptr;
     double* io_b = nullptr;
+    // ---- stored-dense M (dense.hip)
+    void* dense_M = nullptr;        // np x np 16-bit, MFMA operand order
+    void* dense_X = nullptr;        // vectors in B-operand order
+    double* dense_io = nullptr;     // 2 x 16 x n, host-pointer entry point
+    size_t dense_bytes = 0;
+    int dense_storage = 0;
 };
 
 // ----------------------------------------------------------------------------- device helpers
@@ -198,6 +204,9 @@ ps_err check_dist_dev(Ctx* c);
 ps_err rhs_dev(Ctx* c, int drift);
 ps_err vlist_copy(Ctx* c, int32_t* rows, int32_t* partners, int64_t* pidx, int64_t cap, int64_t* n_pairs);
 ps_err run_steps(Ctx* c, int nsteps);
+ps_err dense_build(Ctx* c, int storage);
+ps_err dense_mult_dev(Ctx* c, const double* X, double* Y, int nrhs);   // Y_j = M X_j, vector-major
+void dense_free(Ctx* c);
 ps_err vec_axpby(Ctx* c, double* y, double a, const double* x, double b, int64_t n);  // y = a x + b y
 ps_err vec_copy(Ctx* c, double* dst, const double* src, int64_t n);
 ps_err vec_dot(Ctx* c, const double* a, const double* b, int64_t n, double* out);      // sync

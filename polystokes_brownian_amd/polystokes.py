@@ -61,6 +61,9 @@ def lib():
         L.ps_draw_normals.argtypes = [vp, dp, C.c_int64]
         L.ps_profile_begin.argtypes = [vp]
         L.ps_profile_end.argtypes = [vp, C.POINTER(PsProfile)]
+        L.ps_dense_build.argtypes = [vp, ip]
+        L.ps_dense_mult.argtypes = [vp, C.c_void_p, C.c_void_p, ip]
+        L.ps_dense_mult_dev.argtypes = [vp, C.c_void_p, C.c_void_p, ip]
         _lib = L
     return _lib
 
@@ -210,6 +213,17 @@ class Context:
         out = np.zeros(n)
         self._check(lib().ps_draw_normals(self.h, _p(out), n))
         return out
+
+    def dense_build(self, storage="bf16"):
+        """Store the M of the last mob() in 16 bits in MFMA operand order (ps_dense_build)."""
+        self._check(lib().ps_dense_build(self.h, {"bf16": 1, "f16": 2}.get(storage, storage)))
+
+    def dense_mult(self, X):
+        """Y = M X for X of shape (nrhs, n), nrhs <= 16, on the stored-dense M (ps_dense_mult)."""
+        X = np.ascontiguousarray(np.atleast_2d(X), dtype=np.float64)
+        Y = np.zeros_like(X)
+        self._check(lib().ps_dense_mult(self.h, X.ctypes.data, Y.ctypes.data, X.shape[0]))
+        return Y
 
     def profile_begin(self):
         self._check(lib().ps_profile_begin(self.h))

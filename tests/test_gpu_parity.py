@@ -324,3 +324,66 @@ def test_mixed_minres_velocity_error():
     err = rel(x[o.n:], xd[o.n:])
     print("mixed minres velocity rel err", err, "sweeps", info.refinements, "its", info.iterations, flush=True)
     assert err < 1e-6
+
+
+def _bf16_round(v):
+    """Round float64 -> float32 -> bfloat16 (round to nearest even), returned as float64."""
+    f = np.asarray(v, dtype=np.float32).copy()
+    b = f.view(np.uint32).astype(np.uint64)
+    b = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+    return b.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _h_round(v, storage):
+    if storage == "bf16":
+        return _bf16_round(v)
+    return np.asarray(v, dtype=np.float32).astype(np.float16).astype(np.float64)
+
+
+@pytest.mark.parametrize("name", ["c1", "box_700"])
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_dense_mfma_apply(name, storage):
+    """Stored-dense M on MFMA: columns read back through the apply (exact for unit vectors) equal the
+    oracle's FP64 entries rounded to the storage type (to one 16-bit ulp where the two FP64 values
+    straddle a rounding boundary), and M X for 16 random vectors matches the FP64 product."""
+    pd, x0 = _sym_build(name)
+    o = _oracle()(pd)
+    o.initial_configuration(x0)
+    o.check_dist()
+    o.mob()
+    M = o.mob_dense()
+    g = Context(pd)
+    g.initial_configuration(x0)
+    g.check_dist()
+    g.mob()
+    with pytest.raises(Exception):
+        g.dense_mult(np.zeros((1, g.n)))          # before dense_build
+    g.dense_build(storage)
+    n = g.n
+    # columns: the first monomer ones, every colloid force/torque/stresslet column of colloid 0, the last
+    nu = o.nu
+    cols =[0, 1, 2, 3, 3 * pd["Nm"], 3 * pd["Nm"] + 1, 3 * pd["Nm"] + 3 * pd["Nc"],
+            3 * pd["Nm"] + 3 * pd["Nc"] + 2, nu, nu + 1, nu + 4, n - 1, n // 2, n // 3, 7, n - 5]
+    E = np.zeros((16, n))
+    for k, cidx in enumerate(cols):
+        E[k, cidx] = 1.0
+    got = g.dense_mult(E)
+    want = _h_round(M[:, cols].T, storage)
+    ulp = 2.0 ** (-7 if storage == "bf16" else -10)
+    diff = np.abs(got - want)
+    tol = ulp * np.abs(want) + 1e-7 * np.abs(M).max()
+    print("case", name, storage, "exact fraction", np.mean(diff == 0), "max rel", (diff / (np.abs(want) + 1e-30)).max(),
+          flush=True)
+    assert np.all(diff <= tol)
+    assert np.mean(diff == 0) > 0.995
+    X = np.random.default_rng(31).normal(size=(16, n))
+    Y = g.dense_mult(X)
+    err = np.linalg.norm(Y - X @ M.T) / np.linalg.norm(X @ M.T)
+    print("case", name, storage, "M X rel err", err, flush=True)
+    assert err < (2e-2 if storage == "bf16" else 3e-3)
+    # a smaller batch gives the same vectors
+    Y3 = g.dense_mult(X[:3])
+    np.testing.assert_array_equal(Y3, Y[:3])
+    with pytest.raises(Exception):
+        g.dense_mult(np.zeros((17, n)))
+    g.close()
