@@ -283,6 +283,8 @@ def main():
         "vel_rel_err_vs_fp64": head["vel_rel_err"],
         "fp64_value": f64["value"],
         "mixed_over_fp64": head["value"] / f64["value"],
+        "mixed_over_fp64_reference_algorithm": (head["value"] / results["fp64/minres"]["value"]
+                                                if "fp64/minres" in results else None),
         "forms": {k: {kk: v[kk] for kk in ("value", "ms_per_step", "vel_rel_err", "per_step", "aa_ms_avg",
                                              "apply_ms_avg")} for k, v in results.items()},
         "roofline": roof,
