@@ -218,6 +218,16 @@ ps_err ps_dense_build(void* ctx, int storage);
 ps_err ps_dense_mult(void* ctx, const double* X, double* Y, int nrhs);        /* host pointers */
 ps_err ps_dense_mult_dev(void* ctx, const double* X, double* Y, int nrhs);    /* device pointers */
 
+/* Output frames (src/write_configuration.cpp:10-166): config_<t>_.txt (3Np positions),
+ * quats_<t>_.txt (4Nc), force_<t>_.txt (fext: 3Np forces then 3Nc torques; only with
+ * record_forces) and brownian_stresslet_<t>_.txt (5Nc), "%.10f" tab-separated, one particle per
+ * line, <t> printed by a default std::ostream.  ps_run_output is run() with the reference's
+ * schedule (src/run.cpp:148-160): frames after loop index i == 0 and every i % samplerate == 0.
+ * ps_format_time writes the <t> text of a file name. */
+ps_err ps_write_outputs(void* ctx, const char* output_dir);
+ps_err ps_run_output(void* ctx, int nsteps, const char* output_dir);
+ps_err ps_format_time(double t, char* buf, int len);
+
 /* Normal draws of the context's noise stream (std::mt19937_64 + std::normal_distribution<double>,
  * src/init.cpp:105-110) -- the draws the next step would consume; advances the stream. */
 ps_err ps_draw_normals(void* ctx, double* out, int64_t n);

@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <cstdio>
+#include <sstream>
 #include <cmath>
 #include <condition_variable>
 #include <memory>
@@ -782,6 +784,76 @@ ps_err ps_run(void* h, int nsteps) {
     PS_HIP(c, hipStreamSynchronize(c->stream));
     return PS_OK;
 }
+// File name of a frame: output_dir + "/" + prefix + t + "_.txt" with t printed by a default
+// std::ostream (6 significant digits, %g style), as src/write_configuration.cpp:18-19 does.
+static std::string frame_name(const char* dir, const char* prefix, double t) {
+    std::ostringstream oss;
+    oss << dir << "/" << prefix << t << "_.txt";
+    return oss.str();
+}
+
+// rows x cols values, "%.10f" separated by tabs, one row per line (src/write_configuration.cpp).
+static bool write_table(const std::string& name, const double* v, int64_t rows, int cols) {
+    FILE* f = std::fopen(name.c_str(), "w");
+    if (!f) return false;
+    for (int64_t i = 0; i < rows; i++)
+        for (int k = 0; k < cols; k++) std::fprintf(f, k + 1 < cols ? "%.10f\t" : "%.10f\n", v[i * cols + k]);
+    return std::fclose(f) == 0;
+}
+
+ps_err ps_format_time(double t, char* buf, int len) {
+    if (!buf || len <= 0) return PS_ERR_ARG;
+    std::ostringstream oss;
+    oss << t;
+    const std::string s = oss.str();
+    if ((int)s.size() + 1 > len) return PS_ERR_ARG;
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+    return PS_OK;
+}
+
+ps_err ps_write_outputs(void* h, const char* output_dir) {
+    Ctx* c = as_ctx(h);
+    if (!c || !output_dir) return PS_ERR_ARG;
+    if (!c->have_x) return fail(c, PS_ERR_STATE, "write_outputs before initial_configuration");
+    const int64_t Np = c->Np, Nc = c->Nc;
+    std::vector<double> x(3 * Np), q(4 * std::max<int64_t>(Nc, 1)), fx(c->nu), sb(5 * std::max<int64_t>(Nc, 1));
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    PS_HIP(c, hipMemcpy(x.data(), c->x, sizeof(double) * 3 * Np, hipMemcpyDeviceToHost));
+    if (Nc > 0) {
+        PS_HIP(c, hipMemcpy(q.data(), c->q, sizeof(double) * 4 * Nc, hipMemcpyDeviceToHost));
+        PS_HIP(c, hipMemcpy(sb.data(), c->sb, sizeof(double) * 5 * Nc, hipMemcpyDeviceToHost));
+    }
+    PS_HIP(c, hipMemcpy(fx.data(), c->fext, sizeof(double) * c->nu, hipMemcpyDeviceToHost));
+    const double t = c->t;
+    bool ok = write_table(frame_name(output_dir, "config_", t), x.data(), Np, 3);                 // :10-40
+    ok = ok && write_table(frame_name(output_dir, "quats_", t), q.data(), Nc, 4);                  // :42-73
+    if (c->p.record_forces) {                                                                      // :75-120
+        // monomer + colloid forces (3 Np), then the colloid torques (3 Nc, from offset 3 Np)
+        ok = ok && write_table(frame_name(output_dir, "force_", t), fx.data(), Np + Nc, 3);
+    }
+    ok = ok && write_table(frame_name(output_dir, "brownian_stresslet_", t), sb.data(), Nc, 5);    // :123-165
+    if (!ok) return fail(c, PS_ERR_ARG, std::string("cannot write the frame files in ") + output_dir);
+    return PS_OK;
+}
+
+// run() with the reference's output schedule (src/run.cpp:148-160): the frame files are written after
+// loop index i when i == 0 or i % samplerate == 0.  Runs the steps between writes in one ps_run each.
+ps_err ps_run_output(void* h, int nsteps, const char* output_dir) {
+    Ctx* c = as_ctx(h);
+    if (!c || nsteps < 0 || !output_dir) return PS_ERR_ARG;
+    const int sr = std::max(1, c->p.samplerate);
+    int i = 0;   // next loop index
+    while (i < nsteps) {
+        // steps up to and including the next index that writes
+        const int next = (i == 0) ? 0 : ((i + sr - 1) / sr) * sr;
+        const int last = std::min(next, nsteps - 1);
+        PS_TRY(ps_run(h, last - i + 1));
+        if (last == next) PS_TRY(ps_write_outputs(h, output_dir));
+        i = last + 1;
+    }
+    return PS_OK;
+}
+
 ps_err ps_draw_normals(void* h, double* out, int64_t n) {
     Ctx* c = as_ctx(h);
     if (!c || (!out && n > 0) || n < 0) return PS_ERR_ARG;

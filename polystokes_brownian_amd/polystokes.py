@@ -61,6 +61,9 @@ def lib():
         L.ps_draw_normals.argtypes = [vp, dp, C.c_int64]
         L.ps_profile_begin.argtypes = [vp]
         L.ps_profile_end.argtypes = [vp, C.POINTER(PsProfile)]
+        L.ps_write_outputs.argtypes = [vp, C.c_char_p]
+        L.ps_run_output.argtypes = [vp, ip, C.c_char_p]
+        L.ps_format_time.argtypes = [C.c_double, C.c_char_p, ip]
         L.ps_dense_build.argtypes = [vp, ip]
         L.ps_dense_mult.argtypes = [vp, C.c_void_p, C.c_void_p, ip]
         L.ps_dense_mult_dev.argtypes = [vp, C.c_void_p, C.c_void_p, ip]
@@ -214,6 +217,14 @@ class Context:
         self._check(lib().ps_draw_normals(self.h, _p(out), n))
         return out
 
+    def write_outputs(self, output_dir):
+        """Frame files of the current state (ps_write_outputs, src/write_configuration.cpp)."""
+        self._check(lib().ps_write_outputs(self.h, os.fsencode(output_dir)))
+
+    def run_output(self, nsteps, output_dir):
+        """nsteps steps with the reference's output schedule (ps_run_output, src/run.cpp:148-160)."""
+        self._check(lib().ps_run_output(self.h, int(nsteps), os.fsencode(output_dir)))
+
     def dense_build(self, storage="bf16"):
         """Store the M of the last mob() in 16 bits in MFMA operand order (ps_dense_build)."""
         self._check(lib().ps_dense_build(self.h, {"bf16": 1, "f16": 2}.get(storage, storage)))
@@ -330,9 +341,15 @@ class PolyStokes:
         return self._ctx
 
     def run(self):
-        """PolyStokes::run (src/run.cpp:55-175): nsteps steps on the device; returns final positions."""
+        """PolyStokes::run (src/run.cpp:55-175): nsteps steps on the device, writing the frame files
+        (config_/quats_/force_/brownian_stresslet_<t>_.txt) to output_dir on the reference's
+        schedule when output_dir is given; returns the final positions."""
         ctx = self._context()
-        ctx.run(self.nsteps)
+        if self.output_dir:
+            os.makedirs(self.output_dir, exist_ok=True)
+            ctx.run_output(self.nsteps, self.output_dir)
+        else:
+            ctx.run(self.nsteps)
         return ctx.state()[0]
 
     @property

@@ -228,7 +228,7 @@ def test_one_step(name):
     # displacement over the step, the quantity the tolerance is stated for
     x0 = build(name)[2]
     assert rel(xg - x0, xo - x0) < 1e-10
-    for k in ("up", "v_det_n", "v_det_c", "v_brown"):
+    for k in ("up", "v_det_n", "v_det_c", "v_brown", "fext"):
         assert rel(vg[k], vo[k]) < 1e-10, k
     # udiff and sb are differences of two probe solves 2*eps = 2e-3 apart (src/drift.cpp:57-58): each
     # solve is pinned at rtol 1e-12, so their difference is pinned only to ~1e-12 / eps
@@ -387,3 +387,58 @@ def test_dense_mfma_apply(name, storage):
     with pytest.raises(Exception):
         g.dense_mult(np.zeros((17, n)))
     g.close()
+
+
+def test_frame_outputs_and_restart(tmp_path):
+    """ps_run_output writes frames after loop indices 0, samplerate, 2 samplerate, ... with the
+    positions, quaternions and stresslets of those steps (against the oracle stepped to the same
+    times), and a restart from the last written frame (restart = its time: seed offset, trap kept at
+    the origin, quaternions restored) continues like the oracle restarted from the same files."""
+    from polystokes_brownian_amd.frames import format_time, read_frames, read_quaternions, read_trajectory
+    w = configs.workload("c1")
+    pd = configs.params_dict(w)
+    pd.update(samplerate=2, ksp_rtol=1e-12, sqrt_rtol=1e-10)
+    x0 = configs.positions(w)
+    out = tmp_path / "frames"
+    out.mkdir()
+    g = Context(pd)
+    g.initial_configuration(x0)
+    g.run_output(5, str(out))
+    o = _oracle()(pd)
+    o.initial_configuration(x0)
+    times, frames = read_trajectory(str(out))
+    qt, quats = read_quaternions(str(out))
+    st, stress = read_frames(str(out), "brownian_stresslet_")
+    ft, forces = read_frames(str(out), "force_")
+    assert len(times) == 3 and len(qt) == 3 and len(st) == 3 and len(ft) == 3
+    names = set(p.name for p in out.iterdir())
+    done = 0
+    want = set()
+    for k, steps in enumerate((1, 3, 5)):   # steps completed after loop indices 0, 2, 4
+        o.run(steps - done)
+        done = steps
+        xo, qo, to = o.state()
+        tt = format_time(to)
+        want |= {f"{pre}{tt}_.txt" for pre in ("config_", "quats_", "force_", "brownian_stresslet_")}
+        assert times[k] == float(tt)
+        np.testing.assert_allclose(frames[k], xo.reshape(-1, 3), rtol=0, atol=1e-9)
+        np.testing.assert_allclose(quats[k], qo.reshape(-1, 4), rtol=0, atol=1e-9)
+        vo = o.vectors()
+        np.testing.assert_allclose(stress[k].ravel(), vo["sb"], rtol=1e-7, atol=1e-8 * max(1.0, np.abs(vo["sb"]).max()))
+        np.testing.assert_allclose(forces[k].ravel(), vo["fext"], rtol=1e-7, atol=1e-8 * max(1.0, np.abs(vo["fext"]).max()))
+    assert names == want
+    # restart from the last frame
+    t_last = float(times[-1])
+    pd2 = dict(pd, restart=t_last)
+    xl, ql = frames[-1].ravel(), quats[-1].ravel()
+    g2 = Context(pd2)
+    g2.initial_configuration(xl, ql)
+    g2.run(2)
+    o2 = _oracle()(pd2)
+    o2.initial_configuration(xl, ql)
+    o2.run(2)
+    xg, qg, tg = g2.state()
+    xo, qo, to = o2.state()
+    assert tg == to
+    assert rel(xg - xl, xo - xl) < 1e-9
+    assert rel(qg, qo) < 1e-12

@@ -96,3 +96,15 @@ def test_polystokes_api_surface():
     assert b.active() and b.minimum_image(6.0, -6.0, 4.0) == (-4.0, 4.0, 4.0)
     with pytest.raises(ValueError):
         b.configure([1.0, 2.0])
+
+
+def test_frame_file_time_format():
+    """Frame names use a default std::ostream's formatting of t (src/write_configuration.cpp:18-19),
+    i.e. C's %g with 6 significant digits."""
+    from polystokes_brownian_amd.frames import format_time, frame_path
+    cases = {0.001: "0.001", 1e-05: "1e-05", 0.123456789: "0.123457", 100.0: "100", 1234567.0: "1.23457e+06",
+             0.0025: "0.0025", 0.0: "0", 8e-06: "8e-06", 3 * 8e-06: "2.4e-05", 0.1 + 0.2: "0.3"}
+    for t, want in cases.items():
+        assert format_time(t) == want, t
+        assert format_time(t) == "%g" % t
+    assert frame_path("out", "config_", 0.001) == os.path.join("out", "config_0.001_.txt")
