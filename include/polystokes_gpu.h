@@ -228,6 +228,27 @@ ps_err ps_write_outputs(void* ctx, const char* output_dir);
 ps_err ps_run_output(void* ctx, int nsteps, const char* output_dir);
 ps_err ps_format_time(double t, char* buf, int len);
 
+/* Near-field lubrication between colloids (an addition: the reference distributes the two-sphere
+ * tables lubdat/r2babc.dat, r2bgh.dat, r2bm.dat but no source reads them; src/saddle.cpp:53 leaves
+ * it as a TODO).  Each table value is the exact two-sphere resistance minus the inverse of the
+ * far-field two-sphere mobility, 6 pi eta = 1 and radius 1, Kim & Karrila's convention (e from the
+ * sphere whose F, T, S is given to its partner; r2bm holds M11 + M12).  Each table is passed as its
+ * rows, r first (which 0: r2babc, 11 columns; 1: r2bgh, 7; 2: r2bm, 4) and interpolated by monotone
+ * cubic Hermite (Fritsch-Carlson) in r; below the first node the first row is used.  ps_lub_build
+ * lists the colloid pairs with 0 < r < rcut (rcut at most the last node) at the positions of the
+ * last ps_mob.  ps_lub_apply maps uw = [U_c (3Nc); W_c (3Nc)] and a strain einf shared by all colloids
+ * (5 values flattened as the reference's EV, src/mob.cpp:300-305; NULL for none) to
+ * fts = [F_c (3Nc); T_c (3Nc); S_c (5Nc, flattened as SV)] in FP64: (U - U_inf, W - W_inf, -E_inf)
+ * -> (F, T, S) exerted on the fluid.  ps_lub_interp / ps_lub_slopes evaluate the interpolant and
+ * its node slopes on the host without a context (ncols - 1 outputs). */
+enum { PS_LUB_ABC = 0, PS_LUB_GH = 1, PS_LUB_M = 2 };
+ps_err ps_lub_set_table(void* ctx, int which, const double* rows, int nrows, int ncols);
+ps_err ps_lub_build(void* ctx, double rcut, int64_t* npairs);
+ps_err ps_lub_apply(void* ctx, const double* uw, const double* einf, double* fts);    /* host pointers */
+ps_err ps_lub_apply_dev(void* ctx, const double* uw, const double* einf, double* fts);/* device pointers */
+ps_err ps_lub_interp(const double* rows, int nrows, int ncols, double r, double* out);
+ps_err ps_lub_slopes(const double* rows, int nrows, int ncols, double* slopes);
+
 /* Normal draws of the context's noise stream (std::mt19937_64 + std::normal_distribution<double>,
  * src/init.cpp:105-110) -- the draws the next step would consume; advances the stream. */
 ps_err ps_draw_normals(void* ctx, double* out, int64_t n);

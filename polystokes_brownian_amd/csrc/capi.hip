@@ -308,6 +308,7 @@ void destroy_ctx(Ctx* c) {
     krylov_free(c);
     pairs_free(c);
     dense_free(c);
+    lub_free(c);
     if (c->dense_io) (void)hipFree(c->dense_io);
     for (void* p : {(void*)c->x, (void*)c->xpd, (void*)c->xm, (void*)c->xpos0, (void*)c->q, (void*)c->x_n,
                     (void*)c->chain, (void*)c->up, (void*)c->udiff, (void*)c->v_det_n, (void*)c->v_det_c,
@@ -695,6 +696,39 @@ ps_err ps_dense_mult(void* h, const double* X, double* Y, int nrhs) {
     PS_HIP(c, hipMemcpyAsync(Y, dy, sizeof(double) * len, hipMemcpyDeviceToHost, c->stream));
     PS_HIP(c, hipStreamSynchronize(c->stream));
     return PS_OK;
+}
+ps_err ps_lub_set_table(void* h, int which, const double* rows, int nrows, int ncols) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    return lub_set_table(c, which, rows, nrows, ncols);
+}
+ps_err ps_lub_build(void* h, double rcut, int64_t* npairs) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    return lub_build(c, rcut, npairs);
+}
+ps_err ps_lub_apply_dev(void* h, const double* uw, const double* einf, double* fts) {
+    Ctx* c = as_ctx(h);
+    if (!c || !uw || !fts) return PS_ERR_ARG;
+    return lub_apply_dev(c, uw, einf, fts);
+}
+ps_err ps_lub_apply(void* h, const double* uw, const double* einf, double* fts) {
+    Ctx* c = as_ctx(h);
+    if (!c || !uw || !fts) return PS_ERR_ARG;
+    const int64_t Nc = c->Nc;
+    if (Nc == 0) return PS_OK;
+    // io_a holds [uw (6Nc); E (5)], io_b the 11Nc result (both are ns >= 17 Nc long)
+    if (einf)
+        PS_HIP(c, hipMemcpyAsync(c->io_a + 6 * Nc, einf, sizeof(double) * 5, hipMemcpyHostToDevice, c->stream));
+    return with_io(c, 6 * Nc, uw, 11 * Nc, fts, [&](const double* a, double* b) {
+        return lub_apply_dev(c, a, einf ? a + 6 * Nc : nullptr, b);
+    });
+}
+ps_err ps_lub_interp(const double* rows, int nrows, int ncols, double r, double* out) {
+    return lub_interp_host(rows, nrows, ncols, r, out);
+}
+ps_err ps_lub_slopes(const double* rows, int nrows, int ncols, double* slopes) {
+    return lub_slopes_host(rows, nrows, ncols, slopes);
 }
 ps_err ps_saddle_mult_dev(void* h, const double* x, double* y) {
     Ctx* c = as_ctx(h);

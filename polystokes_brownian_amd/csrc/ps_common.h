@@ -163,6 +163,15 @@ struct Ctx {
     double* dense_io = nullptr;     // 2 x 16 x n, host-pointer entry point
     size_t dense_bytes = 0;
     int dense_storage = 0;
+    // ---- lubrication (lub.hip)
+    double* lub_tab[3] = {nullptr, nullptr, nullptr};   // r[n], f[n k], slopes[n k] per table
+    int lub_rows[3] = {0, 0, 0};
+    double lub_rlast[3] = {0, 0, 0};   // last node of each table
+    int* lub_off = nullptr;         // [Nc+1] partner offsets
+    int* lub_part = nullptr;        // partners (both directions)
+    int lub_cap = 0;
+    double lub_rcut = 0.0;
+    bool lub_built = false;
 };
 
 // ----------------------------------------------------------------------------- device helpers
@@ -207,6 +216,12 @@ ps_err run_steps(Ctx* c, int nsteps);
 ps_err dense_build(Ctx* c, int storage);
 ps_err dense_mult_dev(Ctx* c, const double* X, double* Y, int nrhs);   // Y_j = M X_j, vector-major
 void dense_free(Ctx* c);
+ps_err lub_set_table(Ctx* c, int which, const double* rows, int nrows, int ncols);
+ps_err lub_build(Ctx* c, double rcut, int64_t* npairs);
+ps_err lub_apply_dev(Ctx* c, const double* uw, const double* einf, double* fts);
+void lub_free(Ctx* c);
+ps_err lub_interp_host(const double* rows, int nrows, int ncols, double r, double* out);
+ps_err lub_slopes_host(const double* rows, int nrows, int ncols, double* slopes);
 ps_err vec_axpby(Ctx* c, double* y, double a, const double* x, double b, int64_t n);  // y = a x + b y
 ps_err vec_copy(Ctx* c, double* dst, const double* src, int64_t n);
 ps_err vec_dot(Ctx* c, const double* a, const double* b, int64_t n, double* out);      // sync

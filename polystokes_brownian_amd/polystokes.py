@@ -67,6 +67,12 @@ def lib():
         L.ps_dense_build.argtypes = [vp, ip]
         L.ps_dense_mult.argtypes = [vp, C.c_void_p, C.c_void_p, ip]
         L.ps_dense_mult_dev.argtypes = [vp, C.c_void_p, C.c_void_p, ip]
+        L.ps_lub_set_table.argtypes = [vp, ip, dp, ip, ip]
+        L.ps_lub_build.argtypes = [vp, C.c_double, C.POINTER(C.c_int64)]
+        for name in ("ps_lub_apply", "ps_lub_apply_dev"):
+            getattr(L, name).argtypes = [vp, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ps_lub_interp.argtypes = [dp, ip, ip, C.c_double, dp]
+        L.ps_lub_slopes.argtypes = [dp, ip, ip, dp]
         _lib = L
     return _lib
 
@@ -235,6 +241,35 @@ class Context:
         Y = np.zeros_like(X)
         self._check(lib().ps_dense_mult(self.h, X.ctypes.data, Y.ctypes.data, X.shape[0]))
         return Y
+
+    def lub_set_tables(self, tables=None):
+        """Load the three two-sphere tables (rows with r first; default: the bundled lubdat files)."""
+        from .lubrication import load_tables
+        tables = load_tables() if tables is None else tables
+        for which, rows in enumerate(tables):
+            rows = np.ascontiguousarray(rows, dtype=np.float64)
+            self._check(lib().ps_lub_set_table(self.h, which, _p(rows), rows.shape[0], rows.shape[1]))
+
+    def lub_build(self, rcut=4.0):
+        """Colloid pairs closer than rcut at the last mob() positions; returns the pair count."""
+        n = C.c_int64()
+        self._check(lib().ps_lub_build(self.h, float(rcut), C.byref(n)))
+        return n.value
+
+    def lub_apply(self, uw, einf=None):
+        """[F_c; T_c; S_c] (11 Nc) of the pair lubrication for uw = [U_c; W_c] (6 Nc) and strain einf (5)."""
+        uw = np.ascontiguousarray(uw, dtype=np.float64)
+        if uw.shape != (6 * self.Nc,):
+            raise ValueError(f"uw must have length 6 Nc = {6 * self.Nc}")
+        out = np.zeros(11 * self.Nc)
+        e = None
+        if einf is not None:
+            e = np.ascontiguousarray(einf, dtype=np.float64)
+            if e.shape != (5,):
+                raise ValueError("einf must have 5 entries")
+        self._check(lib().ps_lub_apply(self.h, uw.ctypes.data, None if e is None else e.ctypes.data,
+                                       out.ctypes.data))
+        return out
 
     def profile_begin(self):
         self._check(lib().ps_profile_begin(self.h))
