@@ -249,6 +249,17 @@ ps_err ps_lub_apply_dev(void* ctx, const double* uw, const double* einf, double*
 ps_err ps_lub_interp(const double* rows, int nrows, int ncols, double r, double* out);
 ps_err ps_lub_slopes(const double* rows, int nrows, int ncols, double* slopes);
 
+/* The mm_HI=False (arrowhead) regime, at most one colloid (src/matfree_A.cpp:1-16): ps_mob builds
+ * M^cm; ps_sync_mcc is sync_mcc_schur_correction (src/slip_vel.cpp:81-203), the eigenvalue floor of
+ * the colloid Schur complement that ps_run applies after the predictor's and the corrector's mob
+ * (src/run.cpp:92, :129).  ps_get_schur returns the floored M^cc (11 x 11, row-major), its
+ * eigenvectors (columns of q) and the floored spectrum; ps_get_cell_list the monomer cell list of
+ * the last WCA force evaluation (mono_ev): the cell of each monomer, the grid dimensions and
+ * geom = [cell size (3), origin (3), periodic] (src/cell_list.cpp:23-96). */
+ps_err ps_sync_mcc(void* ctx);
+ps_err ps_get_schur(void* ctx, double* mcc, double* q, double* lam);
+ps_err ps_get_cell_list(void* ctx, int32_t* cell_of, int32_t* dims, double* geom);
+
 /* Normal draws of the context's noise stream (std::mt19937_64 + std::normal_distribution<double>,
  * src/init.cpp:105-110) -- the draws the next step would consume; advances the stream. */
 ps_err ps_draw_normals(void* ctx, double* out, int64_t n);

@@ -253,6 +253,19 @@ struct Sim {
     double stale_next[3][3];   // mob_a left after this mob() call (last BB tile or stale)
     bool dense_mode = true;
     std::vector<double> Mdense;   // n x n row-major when dense_mode
+    // ---- mm_HI == false (arrowhead) regime, src/matfree_A.cpp, src/slip_vel.cpp:81-349
+    bool arrow = false;
+    int64_t nc11 = 0;
+    float rcuts[3] = {0, 0, 0};
+    std::vector<double> Mcm;            // M^cm: nc11 x nm3, row-major (colloid DOF, monomer DOF)
+    std::vector<double> Mcc_base, Mcc;  // colloid self block, and its eigenvalue-floored copy
+    std::vector<double> schurQ, schurLam;   // S = Q diag(lam) Q^T after the floor (columns of Q)
+    std::vector<double> jdiag;          // Jacobi diagonal fixed at the first saddle solve
+    // monomer cell list (src/cell_list.cpp:23-96)
+    int cl_nc[3] = {1, 1, 1};
+    double cl_size[3] = {1, 1, 1}, cl_org[3] = {0, 0, 0};
+    bool cl_periodic = false;
+    std::vector<int> cl_head, cl_next;
     double ktrap_cur = 0.0;
     double t = 0.0;
     int64_t nsteps = 0;
@@ -303,7 +316,8 @@ inline int pair_type(const Sim& s, int k, int j) {
 // Reference pair index of (k < j) in the enumeration of set_vars (src/init.cpp:225-262).
 int64_t pair_index(const Sim& s, int k, int j) {
     int64_t Nm = s.Nm, Nc = s.Nc;
-    int64_t npAA = Nm * (Nm - 1) / 2, npAB = Nm * Nc;
+    // with mm_HI off only AB then BB pairs are stored (src/init.cpp:263-290)
+    int64_t npAA = s.arrow ? 0 : Nm * (Nm - 1) / 2, npAB = Nm * Nc;
     if (j < s.Nm) return (int64_t)k * (2 * Nm - k - 1) / 2 + (j - k - 1);
     if (k < s.Nm) return npAA + (int64_t)k * Nc + (j - s.Nm);
     int64_t kc = k - s.Nm, jc = j - s.Nm;
@@ -322,8 +336,11 @@ bool setup(Sim& s, const ps_params& p) {
         s.err = "Np must equal Nm + Nc";
         return false;
     }
-    if (!p.mm_HI) {
-        s.err = "the oracle restates the mm_HI=True (dense grand mobility) regime only";
+    s.arrow = !p.mm_HI;
+    if (s.arrow && s.Nc > 1) {
+        // The arrowhead shell has no colloid-colloid block (src/matfree_A.cpp:13-16, :124-127);
+        // with Nc > 1 the reference's mob() would MatSetValues into the shell.
+        s.err = "mm_HI=False supports at most one colloid (the reference's arrowhead operator)";
         return false;
     }
     s.nm3 = 3 * (int64_t)s.Nm;
@@ -366,6 +383,33 @@ bool setup(Sim& s, const ps_params& p) {
     s.sigmas[0] = (float)(2. * p.beta);
     s.sigmas[1] = (float)(1. + p.beta);
     s.sigmas[2] = (float)2.0;
+    {
+        // rcuts = 2^(1/6) sigmas, stored as float (src/init.cpp:330-338)
+        const double pref = std::pow(2, 1.0 / 6.0);
+        for (int k = 0; k < 3; k++) s.rcuts[k] = (float)(pref * (double)s.sigmas[k]);
+    }
+    s.nc11 = 11 * (int64_t)s.Nc;
+    if (s.arrow) {
+        // fill_self into Mcc_base (src/mob.cpp:575-627), colloid-local index = global DOF - nm3;
+        // Mcc_block starts as a copy (src/mob.cpp:629-632).
+        const int64_t m = s.nc11;
+        s.Mcc_base.assign((size_t)(m * m), 0.0);
+        for (int c = 0; c < s.Nc; c++) {
+            for (int i = 0; i < 3; i++) {
+                int64_t t = 3 * c + i, r = s.nc3 + 3 * c + i;
+                s.Mcc_base[(size_t)(t * m + t)] = 1.0;
+                s.Mcc_base[(size_t)(r * m + r)] = 3. / 4.;
+            }
+            int64_t s0 = 6 * (int64_t)s.Nc + 5 * c;
+            for (int i = 0; i < 5; i++)
+                for (int j = 0; j < 5; j++) {
+                    double v = (i == j) ? 9. / 5. : ((i == 0 && j == 4) || (i == 4 && j == 0)) ? 9. / 10. : 0.0;
+                    s.Mcc_base[(size_t)((s0 + i) * m + s0 + j)] = v;
+                }
+        }
+        s.Mcc = s.Mcc_base;
+        s.Mcm.assign((size_t)(m * s.nm3), 0.0);
+    }
     s.x.assign(3 * s.Np, 0.0);
     s.xpos0.assign(s.nc3, 0.0);
     s.q.assign(4 * s.Nc, 0.0);
@@ -387,19 +431,21 @@ bool setup(Sim& s, const ps_params& p) {
     uint64_t eff = p.seed + (uint64_t)std::llround(p.restart / p.dt);
     s.rng.seed(eff);
     s.nd = std::normal_distribution<double>(0.0, 1.0);
-    s.dense_mode = (s.n <= 8192);
+    s.dense_mode = !s.arrow && (s.n <= 8192);
     return true;
 }
 
 // ---------------------------------------------------------------- check_dist
 void check_dist(Sim& s) {
-    // src/check_dist.cpp:12-87 (serial path): all stored pairs in enumeration order.
+    // src/check_dist.cpp:12-87 (serial path): all stored pairs in enumeration order.  With
+    // mm_HI off only AB and BB pairs are stored (src/init.cpp:263-290): monomer-monomer pairs
+    // reach the forces through the monomer cell list instead.
     for (auto& v : s.vlist) v.clear();
     const double* xs = s.x.data();
     int Np = s.Np;
 #pragma omp parallel for schedule(dynamic, 16)
     for (int k = 0; k < Np - 1; k++) {
-        for (int j = k + 1; j < Np; j++) {
+        for (int j = (s.arrow && k < s.Nm) ? std::max(k + 1, s.Nm) : k + 1; j < Np; j++) {
             double e[3], dr, dr_inv;
             pair_geom(s, xs, k, j, e, dr, dr_inv);
             double rverl = s.rverls[pair_type(s, k, j)];
@@ -553,8 +599,115 @@ void emit_BB_pair(const Sim& s, int c1, int c2, E&& emit) {
         }
 }
 
+// M^cm of the arrowhead regime (src/mob.cpp:975-1023): for every monomer-colloid pair, the colloid's
+// rotation rows take mobility_AB's b (lb[ii][jj]), its stresslet rows the flattened gt (lgt[ii][jj]
+// at stresslet row jj, monomer DOF ii); the translation rows stay 0 because mobility_AB never
+// writes its translation tile (src/mob.cpp:407-453 are commented out) and la is value-initialised.
+void build_mcm(Sim& s) {
+    const int64_t nm3 = s.nm3;
+    std::fill(s.Mcm.begin(), s.Mcm.end(), 0.0);
+    const double* xs = s.xm.data();
+    for (int k = 0; k < s.Nm; k++)
+        for (int c = s.Nm; c < s.Np; c++) {
+            double e[3], dr, dr_inv, b[3][3], bt[3][3], gt[3][5];
+            pair_geom(s, xs, k, c, e, dr, dr_inv);
+            tile_AB(dr_inv, e, s.p.beta * s.p.beta, b, bt, gt);
+            const int64_t ci = c - s.Nm, col = 3 * (int64_t)k;
+            const int64_t rrow = s.nc3 + 3 * ci, srow = 6 * (int64_t)s.Nc + 5 * ci;
+            for (int ii = 0; ii < 3; ii++) {
+                for (int jj = 0; jj < 3; jj++) s.Mcm[(size_t)((rrow + ii) * nm3 + col + jj)] = b[ii][jj];
+                for (int jj = 0; jj < 5; jj++) s.Mcm[(size_t)((srow + jj) * nm3 + col + ii)] = gt[ii][jj];
+            }
+        }
+}
+
+// Cyclic Jacobi eigendecomposition of a symmetric m x m matrix (row-major A, destroyed): on return
+// lam holds the eigenvalues and V (row-major) the eigenvectors as columns.  Rotation angles by
+// Rutishauser's formulas; sweeps until the off-diagonal mass is below the rounding floor.
+void jacobi_eig(int m, std::vector<double> A, std::vector<double>& lam, std::vector<double>& V) {
+    V.assign((size_t)m * m, 0.0);
+    for (int i = 0; i < m; i++) V[(size_t)i * m + i] = 1.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0, dia = 0.0;
+        for (int p = 0; p < m; p++) {
+            dia += A[(size_t)p * m + p] * A[(size_t)p * m + p];
+            for (int q = p + 1; q < m; q++) off += A[(size_t)p * m + q] * A[(size_t)p * m + q];
+        }
+        if (off <= 1e-36 * dia || off == 0.0) break;
+        for (int p = 0; p < m - 1; p++)
+            for (int q = p + 1; q < m; q++) {
+                const double apq = A[(size_t)p * m + q];
+                if (apq == 0.0) continue;
+                const double theta = (A[(size_t)q * m + q] - A[(size_t)p * m + p]) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < m; k++) {   // A <- A J (columns p, q)
+                    const double akp = A[(size_t)k * m + p], akq = A[(size_t)k * m + q];
+                    A[(size_t)k * m + p] = c * akp - sn * akq;
+                    A[(size_t)k * m + q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < m; k++) {   // A <- J^T A (rows p, q)
+                    const double apk = A[(size_t)p * m + k], aqk = A[(size_t)q * m + k];
+                    A[(size_t)p * m + k] = c * apk - sn * aqk;
+                    A[(size_t)q * m + k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < m; k++) {   // V <- V J
+                    const double vkp = V[(size_t)k * m + p], vkq = V[(size_t)k * m + q];
+                    V[(size_t)k * m + p] = c * vkp - sn * vkq;
+                    V[(size_t)k * m + q] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    lam.resize(m);
+    for (int i = 0; i < m; i++) lam[i] = A[(size_t)i * m + i];
+}
+
+// sync_mcc_schur_correction (src/slip_vel.cpp:81-203): S = Mcc_base - beta M^cm M^cm^T; every
+// eigenvalue below 0.1 is raised to 0.1 and the raise Q diag(delta) Q^T is added to a fresh copy
+// of Mcc_base.  Q and the floored spectrum are kept for the noise.  The reference's syev and this
+// Jacobi give eigenvectors that may differ in sign or within degenerate eigenspaces; the floored
+// matrix and its square root do not depend on that choice.  nc11 > 64 (the Lanczos branch) is
+// unreachable with at most one colloid.
+void schur_sync(Sim& s) {
+    if (!s.arrow || s.nc11 == 0 || s.nc11 > 64) return;
+    const int m = (int)s.nc11;
+    const int64_t nm3 = s.nm3;
+    std::vector<double> S(s.Mcc_base);
+    for (int a = 0; a < m; a++)
+        for (int b = 0; b < m; b++) {
+            double acc = 0.0;
+            const double* ra = &s.Mcm[(size_t)a * nm3];
+            const double* rb = &s.Mcm[(size_t)b * nm3];
+            for (int64_t k = 0; k < nm3; k++) acc += ra[k] * rb[k];
+            S[(size_t)a * m + b] -= s.p.beta * acc;
+        }
+    std::vector<double> lam, Q;
+    jacobi_eig(m, S, lam, Q);
+    const double eps = 0.1;
+    std::vector<double> delta(m);
+    for (int j = 0; j < m; j++) delta[j] = (lam[j] < eps) ? (eps - lam[j]) : 0.0;
+    s.Mcc = s.Mcc_base;
+    for (int a = 0; a < m; a++)
+        for (int b = 0; b < m; b++) {
+            double corr = 0.0;
+            for (int j = 0; j < m; j++) {
+                if (delta[j] == 0.0) continue;
+                corr += Q[(size_t)a * m + j] * delta[j] * Q[(size_t)b * m + j];
+            }
+            s.Mcc[(size_t)a * m + b] += corr;
+        }
+    s.schurQ = Q;
+    s.schurLam.resize(m);
+    for (int j = 0; j < m; j++) s.schurLam[j] = lam[j] + delta[j];
+}
+
 void mob(Sim& s) {
     s.xm = s.xpd;
+    if (s.arrow) {
+        build_mcm(s);
+        s.have_mob = true;
+        return;
+    }
     compute_stale(s);
     if (s.dense_mode) {
         // Dense INSERT assembly of the n x n grand mobility (src/mob.cpp:635-1213); entries never
@@ -578,6 +731,26 @@ void mob(Sim& s) {
 void mob_apply(Sim& s, const double* x, double* y) {
     int64_t n = s.n;
     s.applies++;
+    if (s.arrow) {
+        // ArrowheadMult's mobility part (src/matfree_A.cpp:54-57): y_m = M^mc x_c + beta_inv x_m,
+        // y_c = M^cm x_m + M^cc x_c with the eigenvalue-floored M^cc.
+        const int64_t nm3 = s.nm3, m = s.nc11;
+        const double binv = 1.0 / s.p.beta;
+        const double* xc = x + nm3;
+        for (int64_t i = 0; i < nm3; i++) {
+            double acc = 0.0;
+            for (int64_t r = 0; r < m; r++) acc += s.Mcm[(size_t)(r * nm3 + i)] * xc[r];
+            y[i] = acc + binv * x[i];
+        }
+        for (int64_t r = 0; r < m; r++) {
+            double a1 = 0.0, a2 = 0.0;
+            const double* row = &s.Mcm[(size_t)(r * nm3)];
+            for (int64_t k = 0; k < nm3; k++) a1 += row[k] * x[k];
+            for (int64_t q = 0; q < m; q++) a2 += s.Mcc[(size_t)(r * m + q)] * xc[q];
+            y[nm3 + r] = a1 + a2;
+        }
+        return;
+    }
     if (s.dense_mode) {
         const double* M = s.Mdense.data();
 #pragma omp parallel for schedule(static)
@@ -628,6 +801,13 @@ void saddle_apply(Sim& s, const double* x, double* y) {
 
 // Jacobi diagonal with fix-diagonal (src/matfree_A.cpp:84-102, src/init.cpp:58-59).
 void saddle_diag(const Sim& s, double* d) {
+    if (s.arrow) {
+        // ArrowheadGetDiagonal: [beta_inv; diag(M^cc); 0 -> 1]
+        for (int64_t i = 0; i < s.nm3; i++) d[i] = 1.0 / s.p.beta;
+        for (int64_t r = 0; r < s.nc11; r++) d[s.nm3 + r] = s.Mcc[(size_t)(r * s.nc11 + r)];
+        for (int64_t i = 0; i < s.nu; i++) d[s.n + i] = 1.0;
+        return;
+    }
     for (int64_t i = 0; i < s.nm3; i++) d[i] = 1.0 / s.p.beta;
     for (int c = 0; c < s.Nc; c++) {
         for (int i = 0; i < 3; i++) d[s.nm3 + 3 * c + i] = 1.0;
@@ -725,10 +905,21 @@ int minres_op(int64_t N, const std::function<void(const double*, double*)>& op, 
     return reason > 0 ? 0 : PS_ERR_NOCONV;
 }
 
-// MINRES on the saddle system (the reference's only solver).
+// MINRES on the saddle system (the reference's only solver).  With mm_HI off the operator is a
+// MatShell whose state never changes, so PETSc's PCJacobi takes the diagonal at the first KSPSolve
+// and keeps it ("identical preconditioner since operator is unchanged"): the diagonal of M^cc at
+// that solve is frozen for the rest of the run.  With mm_HI the diagonal is constant.
 int minres(Sim& s, const double* b, double* x, bool warm, ps_solve_info* info) {
     std::vector<double> dinv(s.ns);
-    saddle_diag(s, dinv.data());
+    if (s.arrow) {
+        if (s.jdiag.empty()) {
+            s.jdiag.resize(s.ns);
+            saddle_diag(s, s.jdiag.data());
+        }
+        dinv = s.jdiag;
+    } else {
+        saddle_diag(s, dinv.data());
+    }
     for (auto& d : dinv) d = 1.0 / d;
     auto op = [&s](const double* in, double* out) { saddle_apply(s, in, out); };
     ps_solve_info ii{};
@@ -744,6 +935,13 @@ void mob_apply_stresslet_rows(Sim& s, const double* x, double* ys) {
     const int64_t n = s.n, nu = s.nu, n5 = n - nu;
     std::fill(ys, ys + n5, 0.0);
     if (n5 == 0) return;
+    if (s.arrow) {
+        std::vector<double> y(n);
+        mob_apply(s, x, y.data());
+        s.applies--;
+        for (int64_t k = 0; k < n5; k++) ys[k] = y[nu + k];
+        return;
+    }
     if (s.dense_mode) {
         const double* M = s.Mdense.data();
 #pragma omp parallel for schedule(static)
@@ -790,6 +988,12 @@ void mob_apply_stresslet_rows(Sim& s, const double* x, double* ys) {
 void mss_dense(const Sim& s, std::vector<double>& Mss) {
     const int64_t nu = s.nu, n5 = s.n - s.nu;
     Mss.assign((size_t)(n5 * n5), 0.0);
+    if (s.arrow) {   // the stresslet block of the floored M^cc
+        const int64_t off = s.nc11 - n5;
+        for (int64_t a = 0; a < n5; a++)
+            for (int64_t b = 0; b < n5; b++) Mss[(size_t)(a * n5 + b)] = s.Mcc[(size_t)((off + a) * s.nc11 + off + b)];
+        return;
+    }
     double* A = Mss.data();
     auto ins = [A, nu, n5](int64_t r, int64_t c, double v) {
         if (r >= nu && c >= nu) A[(r - nu) * n5 + (c - nu)] = v;
@@ -928,6 +1132,39 @@ int tridiag_sqrt_weights(int m, const double* alpha, const double* betad, double
 // relative-change stopping test at sqrt_rtol.
 void sqrt_apply(Sim& s, const double* b, double* out, ps_sqrt_info* info) {
     int64_t n = s.n;
+    if (s.arrow) {
+        // build_slip_vel_schur, dense branch (src/slip_vel.cpp:205-349): block-Cholesky factor of
+        // [[beta_inv I, M^mc], [M^cm, M^cc]]: u_m = sqrt(beta_inv) xi_m,
+        // u_c = sqrt(beta) M^cm xi_m + Q diag(sqrt(lam)) Q^T xi_c with the floored Schur spectrum
+        // of the last sync (taken now if none has run).
+        const int64_t nm3 = s.nm3, m = s.nc11;
+        if (m > 0 && (int64_t)s.schurQ.size() != m * m) schur_sync(s);
+        std::vector<double> uc(m, 0.0), yv(m, 0.0);
+        for (int64_t r = 0; r < m; r++) {
+            double acc = 0.0;
+            const double* row = &s.Mcm[(size_t)(r * nm3)];
+            for (int64_t k = 0; k < nm3; k++) acc += row[k] * b[k];
+            uc[r] = acc * std::sqrt(s.p.beta);
+        }
+        const double* xc = b + nm3;
+        for (int64_t j = 0; j < m; j++) {
+            double yj = 0.0;
+            for (int64_t i = 0; i < m; i++) yj += s.schurQ[(size_t)(i * m + j)] * xc[i];
+            yv[j] = std::sqrt(s.schurLam[j]) * yj;
+        }
+        for (int64_t i = 0; i < m; i++) {
+            double si = 0.0;
+            for (int64_t j = 0; j < m; j++) si += s.schurQ[(size_t)(i * m + j)] * yv[j];
+            uc[i] += si;
+        }
+        const double cm = std::sqrt(1.0 / s.p.beta);
+        for (int64_t i = 0; i < nm3; i++) out[i] = cm * b[i];
+        for (int64_t r = 0; r < m; r++) out[nm3 + r] = uc[r];
+        s.last_sqrt_its = 0;
+        s.last_sqrt_nneg = 0;
+        if (info) { info->iterations = 0; info->nneg = 0; info->beta0 = std::sqrt(dot(b, b, n)); info->rel_change = 0.0; }
+        return;
+    }
     double beta0 = std::sqrt(dot(b, b, n));
     for (int64_t i = 0; i < n; i++) out[i] = 0.0;
     if (info) { info->iterations = 0; info->nneg = 0; info->beta0 = beta0; info->rel_change = 0.0; }
@@ -1003,6 +1240,121 @@ inline double hs_exp_force_mag(double dr, double sig, double k) {
     return ex / (1 - ex);
 }
 
+// ---------------------------------------------------------------- monomer cell list (mm_HI off)
+// build_monomer_cell_list (src/cell_list.cpp:23-96): cell side from the AA Verlet cutoff; a
+// periodic grid when every axis has >= 3 cells, else one cell; unbounded: the monomers' bounding
+// box.  Monomers are prepended to their cell's chain (descending index order within a cell).
+void build_monomer_cell_list(Sim& s) {
+    const double rcell = s.rverls[0];
+    if (s.p.box_active) {
+        int nc_min = 1 << 30;
+        for (int d = 0; d < 3; d++) {
+            s.cl_nc[d] = (int)std::floor(s.L[d] / rcell);
+            nc_min = std::min(nc_min, s.cl_nc[d]);
+        }
+        if (nc_min >= 3) {
+            for (int d = 0; d < 3; d++) {
+                s.cl_size[d] = s.L[d] / s.cl_nc[d];
+                s.cl_org[d] = -0.5 * s.L[d];
+            }
+            s.cl_periodic = true;
+        } else {
+            for (int d = 0; d < 3; d++) {
+                s.cl_nc[d] = 1;
+                s.cl_size[d] = (s.L[d] > 0.0) ? s.L[d] : 1.0;
+                s.cl_org[d] = -0.5 * s.L[d];
+            }
+            s.cl_periodic = false;
+        }
+    } else {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int i = 0; i < s.Nm; i++)
+            for (int d = 0; d < 3; d++) {
+                lo[d] = std::min(lo[d], s.x[3 * i + d]);
+                hi[d] = std::max(hi[d], s.x[3 * i + d]);
+            }
+        if (s.Nm == 0)
+            for (int d = 0; d < 3; d++) lo[d] = hi[d] = 0.0;
+        for (int d = 0; d < 3; d++) {
+            double ext = hi[d] - lo[d];
+            s.cl_nc[d] = std::max(1, (int)std::floor(ext / rcell));
+            s.cl_size[d] = (ext > 0.0) ? ext / s.cl_nc[d] : rcell;
+            s.cl_org[d] = lo[d];
+        }
+        s.cl_periodic = false;
+    }
+    const int64_t ncells = (int64_t)s.cl_nc[0] * s.cl_nc[1] * s.cl_nc[2];
+    s.cl_head.assign((size_t)ncells, -1);
+    s.cl_next.assign(s.Nm, -1);
+    for (int i = 0; i < s.Nm; i++) {
+        int c[3];
+        for (int d = 0; d < 3; d++) {
+            int ci = (int)std::floor((s.x[3 * i + d] - s.cl_org[d]) / s.cl_size[d]);
+            if (ci < 0) ci = 0;
+            if (ci >= s.cl_nc[d]) ci = s.cl_nc[d] - 1;
+            c[d] = ci;
+        }
+        int64_t cell = (int64_t)c[0] + (int64_t)s.cl_nc[0] * (c[1] + (int64_t)s.cl_nc[1] * c[2]);
+        s.cl_next[i] = s.cl_head[cell];
+        s.cl_head[cell] = i;
+    }
+}
+
+// WCA force magnitude (src/pair_interaction.h:11-18): 24 eps/r (2 (sig/r)^12 - (sig/r)^6), r < rcut.
+inline double wca_force_mag(double dr, double dr_inv, double sig, double rcut, double eps) {
+    if (dr >= rcut) return 0.0;
+    double sr = sig * dr_inv, s2 = sr * sr, s6 = s2 * s2 * s2, s12 = s6 * s6;
+    return 24.0 * eps * dr_inv * (2.0 * s12 - s6);
+}
+
+// monomer_wca (src/cell_list.cpp:98-172): each unordered monomer pair once -- the home cell (j
+// after i in the chain) and the 13 forward neighbour cells -- in the reference's visiting order.
+void monomer_wca(Sim& s, double* f) {
+    static const int OFF[13][3] = {{1, 0, 0},  {-1, 1, 0}, {0, 1, 0},  {1, 1, 0},  {-1, -1, 1},
+                                   {0, -1, 1}, {1, -1, 1}, {-1, 0, 1}, {0, 0, 1},  {1, 0, 1},
+                                   {-1, 1, 1}, {0, 1, 1},  {1, 1, 1}};
+    const double sig = s.sigmas[0], rcut = s.rcuts[0], rcut2 = rcut * rcut, eps = s.p.epsilon;
+    auto interact = [&](int i, int j) {
+        double dx = s.x[3 * i] - s.x[3 * j], dy = s.x[3 * i + 1] - s.x[3 * j + 1], dz = s.x[3 * i + 2] - s.x[3 * j + 2];
+        min_image(s, dx, dy, dz);
+        double dr2 = dx * dx + dy * dy + dz * dz;
+        if (dr2 >= rcut2 || dr2 == 0.0) return;
+        double dr = std::sqrt(dr2), dr_inv = 1.0 / dr;
+        double fmag = wca_force_mag(dr, dr_inv, sig, rcut, eps);
+        if (fmag == 0.0) return;
+        double fx = fmag * dx * dr_inv, fy = fmag * dy * dr_inv, fz = fmag * dz * dr_inv;
+        f[3 * i] -= fx;
+        f[3 * i + 1] -= fy;
+        f[3 * i + 2] -= fz;
+        f[3 * j] += fx;
+        f[3 * j + 1] += fy;
+        f[3 * j + 2] += fz;
+    };
+    const int* nc = s.cl_nc;
+    for (int cz = 0; cz < nc[2]; cz++)
+        for (int cy = 0; cy < nc[1]; cy++)
+            for (int cx = 0; cx < nc[0]; cx++) {
+                int64_t home = (int64_t)cx + (int64_t)nc[0] * (cy + (int64_t)nc[1] * cz);
+                int hi = s.cl_head[home];
+                if (hi < 0) continue;
+                for (int i = hi; i != -1; i = s.cl_next[i])
+                    for (int j = s.cl_next[i]; j != -1; j = s.cl_next[j]) interact(i, j);
+                for (int o = 0; o < 13; o++) {
+                    int nx = cx + OFF[o][0], ny = cy + OFF[o][1], nz = cz + OFF[o][2];
+                    if (s.cl_periodic) {
+                        nx = (nx % nc[0] + nc[0]) % nc[0];
+                        ny = (ny % nc[1] + nc[1]) % nc[1];
+                        nz = (nz % nc[2] + nc[2]) % nc[2];
+                    } else if (nx < 0 || nx >= nc[0] || ny < 0 || ny >= nc[1] || nz < 0 || nz >= nc[2]) {
+                        continue;
+                    }
+                    int64_t nb = (int64_t)nx + (int64_t)nc[0] * (ny + (int64_t)nc[1] * nz);
+                    for (int i = hi; i != -1; i = s.cl_next[i])
+                        for (int j = s.cl_head[nb]; j != -1; j = s.cl_next[j]) interact(i, j);
+                }
+            }
+}
+
 void rhs_build(Sim& s, bool drift, double* rhs) {
     // RHS (src/rhs.cpp:160-260).
     for (int64_t i = 0; i < s.ns; i++) rhs[i] = 0.0;
@@ -1032,6 +1384,12 @@ void rhs_build(Sim& s, bool drift, double* rhs) {
             fint[3 * j + 1] += fy;
             fint[3 * j + 2] += fz;
         }
+    }
+    // monomer-monomer WCA through the cell list, mm_HI off (src/rhs.cpp:200-205; the cell list is
+    // built from the same positions just before, src/run.cpp:88-90, :124-126).
+    if (s.arrow && s.p.mono_ev) {
+        build_monomer_cell_list(s);
+        monomer_wca(s, fint.data());
     }
     // bond_forces (src/rhs.cpp:102-158).
     for (size_t ii = 0; ii < s.bond0.size(); ii++) {
@@ -1128,6 +1486,7 @@ int run_step(Sim& s) {
     check_dist(s);
     rhs_build(s, false, rhs.data());
     mob(s);
+    schur_sync(s);   // sync_mcc_schur_correction (src/run.cpp:92), a no-op with mm_HI
     rc |= saddle_solve(s, rhs.data(), Xdet.data(), false, nullptr);   // solve_deterministic_vel
     for (int64_t i = 0; i < nu; i++) s.v_det_n[i] = Xdet[n + i];
     if (s.p.kT > 0) {
@@ -1173,6 +1532,7 @@ int run_step(Sim& s) {
     check_dist(s);
     rhs_build(s, false, rhs.data());
     mob(s);
+    schur_sync(s);   // src/run.cpp:129; the drift's two mob() calls above are not followed by one
     rc |= saddle_solve(s, rhs.data(), Xdet.data(), false, nullptr);
     for (int64_t i = 0; i < nu; i++) s.v_det_c[i] = Xdet[n + i];
     for (int64_t k = 0; k < nu; k++) s.up[k] = 0.5 * (s.v_det_n[k] + s.v_det_c[k]) + s.v_brown[k];
@@ -1316,12 +1676,60 @@ int or_mob_dense(void* h, double* M) {
     if (!s->have_mob) return PS_ERR_STATE;
     int64_t n = s->n;
     std::fill(M, M + n * n, 0.0);
+    if (s->arrow) {
+        // the arrowhead grand mobility [[beta_inv I, M^mc], [M^cm, M^cc]] (src/matfree_A.cpp:3-11)
+        const int64_t nm3 = s->nm3, m = s->nc11;
+        for (int64_t i = 0; i < nm3; i++) M[i * n + i] = 1.0 / s->p.beta;
+        for (int64_t r = 0; r < m; r++) {
+            for (int64_t k = 0; k < nm3; k++) {
+                M[(nm3 + r) * n + k] = s->Mcm[(size_t)(r * nm3 + k)];
+                M[k * n + nm3 + r] = s->Mcm[(size_t)(r * nm3 + k)];
+            }
+            for (int64_t q = 0; q < m; q++) M[(nm3 + r) * n + nm3 + q] = s->Mcc[(size_t)(r * m + q)];
+        }
+        return 0;
+    }
     auto ins = [M, n](int64_t r, int64_t c, double v) { M[r * n + c] = v; };
     emit_self(*s, ins);
     for (int i = 0; i < s->Nm - 1; i++) emit_AA_row(*s, i, ins);
     for (int m = 0; m < s->Nm; m++) emit_AB_row(*s, m, ins);
     for (int c1 = s->Nm; c1 < s->Np; c1++)
         for (int c2 = c1 + 1; c2 < s->Np; c2++) emit_BB_pair(*s, c1, c2, ins);
+    return 0;
+}
+// Arrowhead regime accessors: the Schur sync, the floored M^cc / its eigenvectors and spectrum
+// (returns nc11), and the monomer cell list of the last force evaluation with mono_ev.
+int or_schur_sync(void* h) {
+    Sim* s = (Sim*)h;
+    if (!s->arrow) return PS_ERR_STATE;
+    if (!s->have_mob) return PS_ERR_STATE;
+    schur_sync(*s);
+    return 0;
+}
+int64_t or_schur(void* h, double* mcc, double* q, double* lam) {
+    Sim* s = (Sim*)h;
+    if (!s->arrow) return -1;
+    const int64_t m = s->nc11;
+    if (mcc) std::copy(s->Mcc.begin(), s->Mcc.end(), mcc);
+    if (q && (int64_t)s->schurQ.size() == m * m) std::copy(s->schurQ.begin(), s->schurQ.end(), q);
+    if (lam && (int64_t)s->schurLam.size() == m) std::copy(s->schurLam.begin(), s->schurLam.end(), lam);
+    return m;
+}
+int or_cell_list(void* h, int32_t* cell_of, int32_t* dims, double* geom) {
+    Sim* s = (Sim*)h;
+    if (!s->arrow) return PS_ERR_STATE;
+    build_monomer_cell_list(*s);
+    if (dims) for (int d = 0; d < 3; d++) dims[d] = s->cl_nc[d];
+    if (geom) {
+        for (int d = 0; d < 3; d++) {
+            geom[d] = s->cl_size[d];
+            geom[3 + d] = s->cl_org[d];
+        }
+        geom[6] = s->cl_periodic ? 1.0 : 0.0;
+    }
+    if (cell_of)
+        for (size_t c = 0; c < s->cl_head.size(); c++)
+            for (int i = s->cl_head[c]; i != -1; i = s->cl_next[i]) cell_of[i] = (int32_t)c;
     return 0;
 }
 int or_mob_mult(void* h, const double* f, double* u) {

@@ -57,6 +57,10 @@ def lib():
         L.or_mt19937_64_nth.restype = C.c_uint64
         L.or_mt19937_64_nth.argtypes = [C.c_uint64, C.c_int64]
         L.or_normals.argtypes = [C.c_uint64, C.c_int64, dp]
+        L.or_schur_sync.argtypes = [vp]
+        L.or_schur.restype = C.c_int64
+        L.or_schur.argtypes = [vp, dp, dp, dp]
+        L.or_cell_list.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), dp]
         _lib = L
     return _lib
 
@@ -146,6 +150,30 @@ class Oracle:
         M = np.zeros((self.n, self.n))
         lib().or_mob_dense(self.h, _p(M))
         return M
+
+    def schur_sync(self):
+        """sync_mcc_schur_correction of the mm_HI=False regime."""
+        if lib().or_schur_sync(self.h):
+            raise RuntimeError("schur_sync needs mm_HI=False and a mob()")
+
+    def schur(self):
+        """(floored M^cc, eigenvectors Q (columns), floored spectrum) of the last sync."""
+        m = 11 * self.Nc
+        mcc, q, lam = np.zeros((m, m)), np.zeros((m, m)), np.zeros(m)
+        if lib().or_schur(self.h, _p(mcc), _p(q), _p(lam)) < 0:
+            raise RuntimeError("schur needs mm_HI=False")
+        return mcc, q, lam
+
+    def cell_list(self):
+        """Monomer cell list at the current positions: (cell of each monomer, dims, size, origin,
+        periodic)."""
+        cell = np.zeros(self.Nm, dtype=np.int32)
+        dims = np.zeros(3, dtype=np.int32)
+        geom = np.zeros(7)
+        if lib().or_cell_list(self.h, cell.ctypes.data_as(C.POINTER(C.c_int32)),
+                              dims.ctypes.data_as(C.POINTER(C.c_int32)), _p(geom)):
+            raise RuntimeError("cell_list needs mm_HI=False")
+        return cell, dims, geom[:3].copy(), geom[3:6].copy(), bool(geom[6])
 
     def mob_mult(self, f):
         f = np.ascontiguousarray(f, dtype=np.float64)

@@ -137,6 +137,20 @@ def workload(name):
                  sqrt_rtol=1e-6, sqrt_kmax=32, steps=1, probes=None, ic_seed=7)
         # same density as the 100k case
         d["L"] = 550 * LSCALE * (8000 / 100000) ** (1 / 3)
+    elif name in ("arrow10k", "arrow_thermal"):
+        # monomer-monomer HI off (the arrowhead operator), one trapped colloid at the origin and
+        # free dumbbells, in the reference's own units.  arrow10k is the case of the reference's
+        # published serial profile (docs/profiling_baseline_serial.md:6-16, scratch_profile.py):
+        # 5000 dumbbells, beta 0.001, dt 1e-5, box 10, kbond 1, 300 steps.  arrow_thermal follows
+        # test_sim_dumbbells_thermal.py:455-520 (beta 0.1, dt 1e-3) with fewer dumbbells.
+        beta = 0.001 if name == "arrow10k" else 0.1
+        nd = 5000 if name == "arrow10k" else 500
+        r0 = 2.0 ** (1.0 / 6.0) * (2.0 * beta) + (0.03 if name == "arrow10k" else 0.0)
+        d = dict(base, mm_HI=0, chain_HI=0, record_forces=0, Nc=1, Npoly=nd, Nmono=2, L=10.0, beta=beta,
+                 dt=1e-5 if name == "arrow10k" else 1e-3, kbond=1.0, r0=r0, Lmax=1.5 * r0, tau=1000.0,
+                 epsilon=5.0 if name == "arrow10k" else 1.0, ktrap=1.0, ksp_rtol=1e-6, sqrt_rtol=1e-6,
+                 sqrt_kmax=32, steps=300, probes=None, ic_seed=2024, dumbbells=True,
+                 min_sep=2.0 ** (1.0 / 6.0) * (2.0 * beta))
     else:
         raise KeyError(name)
     d["Nm"] = d["Npoly"] * d["Nmono"]
@@ -166,7 +180,61 @@ def params_dict(w):
     return d
 
 
+def dumbbell_configuration(n_dumbbell, r0, kbond, kT, L, seed, clearance=1.5, min_sep=0.0, max_tries=1000,
+                           min_bond=0.0):
+    """Positions (3*(2 n_dumbbell + 1),) of free dumbbells around one colloid at the origin, drawn as
+    the reference's thermal-dumbbell generator draws them (test_sim_dumbbells_thermal.py:93-174):
+    centre uniform in the box, bond vector r0 u + N(0, kT/kbond) per component with u isotropic,
+    beads wrapped into the box, rejected when closer than `clearance` to the colloid or closer than
+    `min_sep` (minimum image) to a bead of an earlier dumbbell.  `min_bond` (not in the reference's
+    generator) also rejects bond vectors shorter than it, e.g. inside the monomer WCA core."""
+    rng = np.random.default_rng(seed)
+    Lv = np.full(3, float(L))
+    sigma = math.sqrt(kT / kbond)
+    beads = np.zeros((2 * n_dumbbell, 3))
+    ncell = max(1, int(L // min_sep)) if min_sep > 0 else 1
+    grid = {}
+
+    def key(p):
+        return tuple((np.floor((p + Lv / 2) / (Lv / ncell)).astype(int)) % ncell)
+
+    def clear(p):
+        if min_sep <= 0:
+            return True
+        kx, ky, kz = key(p)
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for q in grid.get(((kx + dx) % ncell, (ky + dy) % ncell, (kz + dz) % ncell), ()):
+                        if np.linalg.norm(_min_image(q - p, Lv)) < min_sep:
+                            return False
+        return True
+
+    for dd in range(n_dumbbell):
+        for _ in range(max_tries):
+            c = rng.uniform(-Lv / 2, Lv / 2)
+            u = rng.normal(size=3)
+            v = r0 * u / np.linalg.norm(u) + rng.normal(scale=sigma, size=3)
+            if np.linalg.norm(v) < min_bond:
+                continue
+            b0, b1 = _wrap(c - 0.5 * v, Lv), _wrap(c + 0.5 * v, Lv)
+            if np.linalg.norm(b0) < clearance or np.linalg.norm(b1) < clearance:
+                continue
+            if not (clear(b0) and clear(b1)):
+                continue
+            beads[2 * dd], beads[2 * dd + 1] = b0, b1
+            for b in (b0, b1):
+                grid.setdefault(key(b), []).append(b)
+            break
+        else:
+            raise RuntimeError("could not place dumbbell %d" % dd)
+    return np.concatenate([beads.reshape(-1), np.zeros(3)])
+
+
 def positions(w):
+    if w.get("dumbbells"):
+        return dumbbell_configuration(w["Npoly"], w["r0"], w["kbond"], w["kT"], w["L"], w["ic_seed"],
+                                      min_sep=w.get("min_sep", 0.0))
     return make_configuration(w["Nc"], w["Npoly"], w["Nmono"], w["L"], w["r0"], w["ic_seed"],
                               probe_positions=w.get("probes"))
 

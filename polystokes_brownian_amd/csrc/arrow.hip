@@ -1,0 +1,669 @@
+// arrow.hip -- the mm_HI == false regime: the O(N) "arrowhead" grand mobility
+//     M = [[ beta_inv I , M^mc ],
+//          [ M^cm       , M^cc ]]          (src/matfree_A.cpp:1-16, :38-80)
+// for monomers without monomer-monomer hydrodynamics and at most one colloid, its eigenvalue-floored
+// colloid block (src/slip_vel.cpp:81-203), its block-Cholesky noise (src/slip_vel.cpp:205-349), the
+// Verlet list of monomer-colloid pairs only (src/init.cpp:263-290) and the monomer cell list of the
+// WCA forces (src/cell_list.cpp:23-172).
+//
+// HBM layout: M^cm is stored as its 8 non-zero rows (colloid rotation 3, stresslet 5) over the 3 Nm
+// monomer DOFs, row-major [8][3Nm], so that a thread per monomer DOF reads every row coalesced.  The
+// three translation rows are zero by construction: mobility_AB never writes its translation tile
+// (src/mob.cpp:407-453 are commented out), so the reference fills them from a value-initialised
+// buffer.  M^cc, its eigenvectors and floored spectrum live in `ar_mcc` [121 + 121 + 11].
+//
+// Every operation is a DOF-parallel pass plus one small finishing block: the apply and the noise
+// are two launches each, the Schur floor a Gram reduction plus one workgroup that runs a parallel
+// Jacobi eigensolver on the 11 x 11 Schur complement -- no host round trip.
+#include "ps_common.h"
+#include "tiles.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace {
+
+constexpr int AB = 256;             // threads per DOF-parallel block
+constexpr int NR = 8;               // stored rows of M^cm (rotation 3 + stresslet 5)
+constexpr int NSYM = NR * (NR + 1) / 2;
+
+unsigned dof_blocks(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + AB - 1) / AB); }
+
+// ------------------------------------------------------------------------------------ Verlet list
+// Monomer k pairs with the colloid when their separation is below the AB Verlet cutoff (a float,
+// compared after promotion, src/check_dist.cpp:49-62).
+__global__ void k_ar_flag(SysConst s, const double* __restrict__ xpd, double rverl_ab, int* __restrict__ flag) {
+    const int k = blockIdx.x * AB + threadIdx.x;
+    if (k >= s.Nm) return;
+    int f = 0;
+    if (s.Nc == 1) {
+        double e[3], dr, dr_inv;
+        pair_geom_dev(s, xpd, k, s.Nm, e, dr, dr_inv);
+        f = dr < rverl_ab;
+    }
+    flag[k] = f;
+}
+// Rows: a monomer's single partner is the colloid; the colloid's row lists its monomers ascending.
+__global__ void k_ar_counts(int Nm, int Nc, const int* __restrict__ flag, const int* __restrict__ pos,
+                            int* __restrict__ nb_count) {
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i < Nm) nb_count[i] = flag[i];
+    else if (i < Nm + Nc) nb_count[i] = pos[Nm];
+}
+__global__ void k_ar_fill(int Nm, const int* __restrict__ flag, const int* __restrict__ pos,
+                          const int* __restrict__ nb_off, int* __restrict__ part) {
+    const int k = blockIdx.x * AB + threadIdx.x;
+    if (k >= Nm || !flag[k]) return;
+    part[nb_off[k]] = Nm;
+    part[nb_off[Nm] + pos[k]] = k;
+}
+
+// ------------------------------------------------------------------------------------ M^cm
+// Column block of monomer k: rotation rows take mobility_AB's b, stresslet rows its flattened gt
+// (placement of src/mob.cpp:1013-1020).
+__global__ void k_ar_mcm(SysConst s, const double* __restrict__ xm, double beta2, double* __restrict__ mcm) {
+    const int k = blockIdx.x * AB + threadIdx.x;
+    if (k >= s.Nm) return;
+    const int64_t nm3 = s.nm3;
+    double e[3], dr, dr_inv, b[3][3], bt[3][3], gt[3][5];
+    pair_geom_dev(s, xm, k, s.Nm, e, dr, dr_inv);
+    tile_ab_dev(dr_inv, e, beta2, b, bt, gt);
+    for (int ii = 0; ii < 3; ii++) {
+        for (int jj = 0; jj < 3; jj++) mcm[ii * nm3 + 3 * k + jj] = b[ii][jj];
+        for (int jj = 0; jj < 5; jj++) mcm[(3 + jj) * nm3 + 3 * k + ii] = gt[ii][jj];
+    }
+}
+
+// Block reduction of NV doubles per thread into part[blockIdx.x * NV ...] (fixed order).
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* __restrict__ part) {
+    __shared__ double sh[AB / 64][NV];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        double a = v[q];
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+        if (lane == 0) sh[wv][q] = a;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < NV; q += AB) {
+        double a = 0.0;
+        for (int w = 0; w < AB / 64; w++) a += sh[w][q];
+        part[(size_t)blockIdx.x * NV + q] = a;
+    }
+}
+
+// Gram partials of the stored rows: P = M8 M8^T, upper triangle (NSYM entries) per block.
+__global__ __launch_bounds__(AB) void k_ar_gram(int64_t nm3, const double* __restrict__ mcm, double* __restrict__ part) {
+    double acc[NSYM];
+#pragma unroll
+    for (int q = 0; q < NSYM; q++) acc[q] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * AB + threadIdx.x; i < nm3; i += (int64_t)gridDim.x * AB) {
+        double v[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) v[r] = mcm[r * nm3 + i];
+        int q = 0;
+#pragma unroll
+        for (int a = 0; a < NR; a++)
+#pragma unroll
+            for (int b = a; b < NR; b++) acc[q++] += v[a] * v[b];
+    }
+    block_reduce_store<NSYM>(acc, part);
+}
+
+// The colloid self block (fill_self, src/mob.cpp:575-627) for one colloid: [F 3 | T 3 | S 5].
+__device__ __forceinline__ double mcc_base(int a, int b) {
+    if (a != b) return ((a == 6 && b == 10) || (a == 10 && b == 6)) ? 9. / 10. : 0.0;
+    return a < 3 ? 1.0 : a < 6 ? 3. / 4. : 9. / 5.;
+}
+
+// sync_mcc_schur_correction (src/slip_vel.cpp:81-203) in one workgroup: S = M^cc_base - beta P;
+// eigen-decomposition by a parallel cyclic Jacobi method (Rutishauser's rotation formulas; the
+// round-robin ordering rotates 6 disjoint index pairs of the 12-padded matrix at once); eigenvalues
+// below 0.1 raised to 0.1; M^cc = base + Q diag(delta) Q^T.  Writes M^cc, Q (row-major, eigenvectors
+// as columns), the floored spectrum, and the 5 x 5 stresslet block of M^cc for the block solver.
+constexpr int JN = 12;
+__global__ __launch_bounds__(128) void k_ar_schur(const double* __restrict__ part, int nb, double beta,
+                                                  double* __restrict__ out, double* __restrict__ mss) {
+    __shared__ double A[JN][JN + 1], V[JN][JN + 1];
+    __shared__ double P[NSYM];
+    __shared__ double cs[JN / 2], sn[JN / 2];
+    __shared__ int pp[JN / 2], qq[JN / 2];
+    __shared__ int done;
+    const int t = threadIdx.x;
+    if (t < NSYM) {
+        double a = 0.0;
+        for (int b = 0; b < nb; b++) a += part[(size_t)b * NSYM + t];
+        P[t] = a;
+    }
+    __syncthreads();
+    for (int q = t; q < JN * JN; q += blockDim.x) {
+        const int a = q / JN, b = q % JN;
+        double v = 0.0;
+        if (a < 11 && b < 11) {
+            v = mcc_base(a, b);
+            if (a >= 3 && b >= 3) {
+                const int i = min(a, b) - 3, j = max(a, b) - 3;   // upper-triangle index of (i, j)
+                v -= beta * P[i * NR - i * (i - 1) / 2 + (j - i)];
+            }
+        }
+        A[a][b] = v;
+        V[a][b] = (a == b) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int sweep = 0; sweep < 60; sweep++) {
+        if (t == 0) {
+            double off = 0.0, dia = 0.0;
+            for (int a = 0; a < 11; a++) {
+                dia += A[a][a] * A[a][a];
+                for (int b = a + 1; b < 11; b++) off += A[a][b] * A[a][b];
+            }
+            done = (off <= 1e-36 * dia || off == 0.0);
+        }
+        __syncthreads();
+        if (done) break;
+        for (int round = 0; round < JN - 1; round++) {
+            if (t < JN / 2) {
+                // circle method: seat 0 fixed, seats 1..11 rotate by `round`
+                auto seat = [round](int j) { return j == 0 ? 0 : 1 + (j - 1 + round) % (JN - 1); };
+                int p = seat(t), q = seat(JN - 1 - t);
+                if (p > q) { const int w = p; p = q; q = w; }
+                const double apq = A[p][q];
+                double c = 1.0, s = 0.0;
+                if (apq != 0.0) {
+                    const double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
+                    const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                    c = 1.0 / sqrt(tt * tt + 1.0);
+                    s = tt * c;
+                }
+                cs[t] = c;
+                sn[t] = s;
+                pp[t] = p;
+                qq[t] = q;
+            }
+            __syncthreads();
+            if (t < (JN / 2) * JN) {   // columns p, q of A and V (A J, V J); disjoint across pairs
+                const int k = t / JN, j = t % JN, p = pp[k], q = qq[k];
+                const double c = cs[k], s = sn[k];
+                const double ajp = A[j][p], ajq = A[j][q], vjp = V[j][p], vjq = V[j][q];
+                A[j][p] = c * ajp - s * ajq;
+                A[j][q] = s * ajp + c * ajq;
+                V[j][p] = c * vjp - s * vjq;
+                V[j][q] = s * vjp + c * vjq;
+            }
+            __syncthreads();
+            if (t < (JN / 2) * JN) {   // rows p, q of A (J^T A)
+                const int k = t / JN, j = t % JN, p = pp[k], q = qq[k];
+                const double c = cs[k], s = sn[k];
+                const double apj = A[p][j], aqj = A[q][j];
+                A[p][j] = c * apj - s * aqj;
+                A[q][j] = s * apj + c * aqj;
+            }
+            __syncthreads();
+        }
+    }
+    __shared__ double delta[11];
+    if (t < 11) {
+        const double lam = A[t][t];
+        delta[t] = lam < 0.1 ? 0.1 - lam : 0.0;
+        out[242 + t] = lam + delta[t];
+    }
+    __syncthreads();
+    for (int q = t; q < 121; q += blockDim.x) {
+        const int a = q / 11, b = q % 11;
+        double corr = 0.0;
+        for (int j = 0; j < 11; j++)
+            if (delta[j] != 0.0) corr += V[a][j] * delta[j] * V[b][j];
+        const double m = mcc_base(a, b) + corr;
+        out[q] = m;
+        out[121 + q] = V[a][b];
+        if (mss && a >= 6 && b >= 6) mss[(a - 6) * 5 + (b - 6)] = m;
+    }
+}
+
+// ------------------------------------------------------------------------------------ apply
+// Monomer rows of y = M x (saddle: + B terms) and the per-block partials of M^cm x_m:
+//   y_m = beta_inv x_m + M^mc x_c  [- l_m; and y_l,m = -x_m]   (src/matfree_A.cpp:54-78)
+// xr: the colloid's rotation and stresslet entries of x (x + nm3 + 3), null without a colloid.
+__global__ __launch_bounds__(AB) void k_ar_apply(int64_t nm3, int64_t n, double binv, const double* __restrict__ mcm,
+                                                 const double* __restrict__ x, const double* __restrict__ xr,
+                                                 double* __restrict__ y, double* __restrict__ part, int saddle) {
+    const int64_t i = (int64_t)blockIdx.x * AB + threadIdx.x;
+    double p[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) p[r] = 0.0;
+    if (i < nm3) {
+        const double xi = x[i];
+        double acc = 0.0;
+        if (xr) {
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const double v = mcm[r * nm3 + i];
+                acc += v * xr[r];
+                p[r] = v * xi;
+            }
+        }
+        double yi = acc + binv * xi;
+        if (saddle) {
+            yi -= x[n + i];
+            y[n + i] = -xi;
+        }
+        y[i] = yi;
+    }
+    block_reduce_store<NR>(p, part);
+}
+// Colloid rows: y_c = M^cm x_m + M^cc x_c (+ B terms).
+__global__ void k_ar_apply_fin(int64_t nm3, int64_t n, int64_t nu, const double* __restrict__ part, int nb,
+                               const double* __restrict__ mcc, const double* __restrict__ x, double* __restrict__ y,
+                               int saddle) {
+    const int r = threadIdx.x;
+    if (r >= 11) return;
+    double a1 = 0.0;
+    if (r >= 3)
+        for (int b = 0; b < nb; b++) a1 += part[(size_t)b * NR + (r - 3)];
+    double a2 = 0.0;
+    for (int q = 0; q < 11; q++) a2 += mcc[r * 11 + q] * x[nm3 + q];
+    double yr = a1 + a2;
+    if (saddle && nm3 + r < nu) {
+        yr -= x[n + nm3 + r];
+        y[n + nm3 + r] = -x[nm3 + r];
+    }
+    y[nm3 + r] = yr;
+}
+
+// ------------------------------------------------------------------------------------ noise
+// build_slip_vel_schur: u_m = sqrt(beta_inv) xi_m, partials of M^cm xi_m.
+__global__ __launch_bounds__(AB) void k_ar_noise(int64_t nm3, double sq_binv, const double* __restrict__ mcm,
+                                                 const double* __restrict__ xi, double* __restrict__ out,
+                                                 double* __restrict__ part) {
+    const int64_t i = (int64_t)blockIdx.x * AB + threadIdx.x;
+    double p[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) p[r] = 0.0;
+    if (i < nm3) {
+        const double v = xi[i];
+#pragma unroll
+        for (int r = 0; r < NR; r++) p[r] = mcm[r * nm3 + i] * v;
+        out[i] = sq_binv * v;
+    }
+    block_reduce_store<NR>(p, part);
+}
+// u_c = sqrt(beta) M^cm xi_m + Q diag(sqrt(lam)) Q^T xi_c.
+__global__ void k_ar_noise_fin(int64_t nm3, const double* __restrict__ part, int nb, double sq_beta,
+                               const double* __restrict__ mcc_q_lam, const double* __restrict__ xi,
+                               double* __restrict__ out) {
+    __shared__ double yv[11];
+    const int r = threadIdx.x;
+    const double* Q = mcc_q_lam + 121;
+    const double* lam = mcc_q_lam + 242;
+    if (r < 11) {
+        double yj = 0.0;
+        for (int i = 0; i < 11; i++) yj += Q[i * 11 + r] * xi[nm3 + i];
+        yv[r] = sqrt(lam[r]) * yj;
+    }
+    __syncthreads();
+    if (r >= 11) return;
+    double uc = 0.0;
+    if (r >= 3)
+        for (int b = 0; b < nb; b++) uc += part[(size_t)b * NR + (r - 3)];
+    uc *= sq_beta;
+    double si = 0.0;
+    for (int j = 0; j < 11; j++) si += Q[r * 11 + j] * yv[j];
+    out[nm3 + r] = uc + si;
+}
+
+// ------------------------------------------------------------------------------------ diagonal
+// ArrowheadGetDiagonal + fix-diagonal (src/matfree_A.cpp:84-102): [beta_inv; diag(M^cc); 1]; as
+// the inverse when `inverse` is set.
+__global__ void k_ar_diag(int64_t nm3, int64_t n, int64_t ns, double beta, const double* __restrict__ mcc,
+                          double* __restrict__ d, int inverse) {
+    const int64_t i = (int64_t)blockIdx.x * AB + threadIdx.x;
+    if (i >= ns) return;
+    double v = (i < nm3) ? 1.0 / beta : (i < n) ? mcc[(i - nm3) * 12] : 1.0;
+    d[i] = inverse ? 1.0 / v : v;
+}
+
+// ------------------------------------------------------------------------------------ WCA cell list
+struct WcaGrid {
+    int nc[3];
+    double size[3], org[3];
+    int periodic;
+};
+__device__ __forceinline__ int wca_cell_coord(const WcaGrid& g, double x, int d) {
+    int ci = (int)floor((x - g.org[d]) / g.size[d]);
+    if (ci < 0) ci = 0;
+    if (ci >= g.nc[d]) ci = g.nc[d] - 1;
+    return ci;
+}
+__global__ void k_wc_bin(int Nm, WcaGrid g, const double* __restrict__ x, int* __restrict__ cell_of,
+                         int* __restrict__ count) {
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= Nm) return;
+    const int cx = wca_cell_coord(g, x[3 * i], 0), cy = wca_cell_coord(g, x[3 * i + 1], 1),
+              cz = wca_cell_coord(g, x[3 * i + 2], 2);
+    const int c = cx + g.nc[0] * (cy + g.nc[1] * cz);
+    cell_of[i] = c;
+    atomicAdd(&count[c], 1);
+}
+__global__ void k_wc_scatter(int Nm, const int* __restrict__ cell_of, const int* __restrict__ start,
+                             int* __restrict__ cursor, int* __restrict__ mem) {
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= Nm) return;
+    const int c = cell_of[i];
+    mem[start[c] + atomicAdd(&cursor[c], 1)] = i;
+}
+// Members of each cell in ascending index (a deterministic summation order).
+__global__ void k_wc_sort(int ncells, const int* __restrict__ start, int* __restrict__ mem) {
+    const int c = blockIdx.x * AB + threadIdx.x;
+    if (c >= ncells) return;
+    const int a = start[c], b = start[c + 1];
+    for (int i = a + 1; i < b; i++) {
+        const int v = mem[i];
+        int j = i - 1;
+        while (j >= a && mem[j] > v) {
+            mem[j + 1] = mem[j];
+            j--;
+        }
+        mem[j + 1] = v;
+    }
+}
+// Force on monomer i from every monomer j within the WCA cutoff in the 27 surrounding cells
+// (monomer_wca, src/cell_list.cpp:98-172, gathered per monomer; the reference visits each pair once
+// and applies it to both).  Added to the force slot of the RHS, which holds minus the force.
+__global__ void k_wc_force(SysConst s, WcaGrid g, const double* __restrict__ x, const int* __restrict__ cell_of,
+                           const int* __restrict__ start, const int* __restrict__ mem, double sig, double rcut,
+                           double eps, double* __restrict__ rhs) {
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= s.Nm) return;
+    const int c = cell_of[i];
+    const int cx = c % g.nc[0], cy = (c / g.nc[0]) % g.nc[1], cz = c / (g.nc[0] * g.nc[1]);
+    const double rcut2 = rcut * rcut;
+    const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int oz = -1; oz <= 1; oz++)
+        for (int oy = -1; oy <= 1; oy++)
+            for (int ox = -1; ox <= 1; ox++) {
+                int nx = cx + ox, ny = cy + oy, nz = cz + oz;
+                if (g.periodic) {
+                    nx = (nx % g.nc[0] + g.nc[0]) % g.nc[0];
+                    ny = (ny % g.nc[1] + g.nc[1]) % g.nc[1];
+                    nz = (nz % g.nc[2] + g.nc[2]) % g.nc[2];
+                } else if (nx < 0 || nx >= g.nc[0] || ny < 0 || ny >= g.nc[1] || nz < 0 || nz >= g.nc[2]) {
+                    continue;
+                }
+                const int nb = nx + g.nc[0] * (ny + g.nc[1] * nz);
+                for (int m = start[nb]; m < start[nb + 1]; m++) {
+                    const int j = mem[m];
+                    if (j == i) continue;
+                    double dx = xi0 - x[3 * j], dy = xi1 - x[3 * j + 1], dz = xi2 - x[3 * j + 2];
+                    if (s.box_active) {
+                        dx -= s.L[0] * rint(dx * s.Linv[0]);
+                        dy -= s.L[1] * rint(dy * s.Linv[1]);
+                        dz -= s.L[2] * rint(dz * s.Linv[2]);
+                    }
+                    const double dr2 = dx * dx + dy * dy + dz * dz;
+                    if (dr2 >= rcut2 || dr2 == 0.0) continue;
+                    const double dr = sqrt(dr2), dr_inv = 1.0 / dr;
+                    if (dr >= rcut) continue;
+                    const double sr = sig * dr_inv, s2 = sr * sr, s6 = s2 * s2 * s2, s12 = s6 * s6;
+                    const double fmag = 24.0 * eps * dr_inv * (2.0 * s12 - s6);
+                    if (fmag == 0.0) continue;
+                    f0 += fmag * dx * dr_inv;
+                    f1 += fmag * dy * dr_inv;
+                    f2 += fmag * dz * dr_inv;
+                }
+            }
+    rhs[s.n + 3 * i] -= f0;
+    rhs[s.n + 3 * i + 1] -= f1;
+    rhs[s.n + 3 * i + 2] -= f2;
+}
+// Bounding box of the monomers (the unbounded cell grid), one block.
+__global__ void k_wc_bbox(int Nm, const double* __restrict__ x, double* __restrict__ out) {
+    __shared__ double lo[3][256], hi[3][256];
+    const int t = threadIdx.x;
+    double l[3] = {1e300, 1e300, 1e300}, h[3] = {-1e300, -1e300, -1e300};
+    for (int i = t; i < Nm; i += blockDim.x)
+        for (int d = 0; d < 3; d++) {
+            l[d] = fmin(l[d], x[3 * i + d]);
+            h[d] = fmax(h[d], x[3 * i + d]);
+        }
+    for (int d = 0; d < 3; d++) {
+        lo[d][t] = l[d];
+        hi[d][t] = h[d];
+    }
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w)
+            for (int d = 0; d < 3; d++) {
+                lo[d][t] = fmin(lo[d][t], lo[d][t + w]);
+                hi[d][t] = fmax(hi[d][t], hi[d][t + w]);
+            }
+        __syncthreads();
+    }
+    if (t < 3) {
+        out[t] = lo[t][0];
+        out[3 + t] = hi[t][0];
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------- host side
+ps_err arrow_alloc(Ctx* c) {
+    const int64_t nm3 = c->sc.nm3;
+    const int nb = (int)dof_blocks(nm3);
+    c->ar_nb = nb;
+    PS_HIP(c, hipMalloc(&c->ar_mcm, sizeof(double) * (size_t)std::max<int64_t>(NR * nm3, 1)));
+    PS_HIP(c, hipMemset(c->ar_mcm, 0, sizeof(double) * (size_t)std::max<int64_t>(NR * nm3, 1)));
+    PS_HIP(c, hipMalloc(&c->ar_part, sizeof(double) * (size_t)nb * NSYM));
+    PS_HIP(c, hipMalloc(&c->ar_mcc, sizeof(double) * 256));
+    PS_HIP(c, hipMalloc(&c->ar_flag, sizeof(int) * (size_t)(c->Nm + 2)));
+    PS_HIP(c, hipMalloc(&c->ar_pos, sizeof(int) * (size_t)(c->Nm + 2)));
+    PS_HIP(c, hipHostMalloc(&c->ar_host, sizeof(double) * 8, hipHostMallocDefault));
+    // M^cc starts as the colloid self block, Q as the identity (src/mob.cpp:629-632)
+    double h[256] = {};
+    for (int a = 0; a < 11; a++) {
+        h[a * 11 + a] = a < 3 ? 1.0 : a < 6 ? 0.75 : 1.8;
+        h[121 + a * 11 + a] = 1.0;
+    }
+    h[6 * 11 + 10] = h[10 * 11 + 6] = 0.9;
+    PS_HIP(c, hipMemcpy(c->ar_mcc, h, sizeof(double) * 256, hipMemcpyHostToDevice));
+    if (c->Mss) {
+        double m5[25] = {};
+        for (int a = 0; a < 5; a++)
+            for (int b = 0; b < 5; b++) m5[a * 5 + b] = h[(6 + a) * 11 + 6 + b];
+        PS_HIP(c, hipMemcpy(c->Mss, m5, sizeof(m5), hipMemcpyHostToDevice));
+    }
+    return PS_OK;
+}
+
+void arrow_free(Ctx* c) {
+    for (void* p : {(void*)c->ar_mcm, (void*)c->ar_part, (void*)c->ar_mcc, (void*)c->ar_flag, (void*)c->ar_pos,
+                    (void*)c->wc_cell, (void*)c->wc_count, (void*)c->wc_start, (void*)c->wc_mem,
+                    (void*)c->ar_bbox})
+        if (p) (void)hipFree(p);
+    if (c->ar_host) (void)hipHostFree(c->ar_host);
+    c->ar_mcm = c->ar_part = c->ar_mcc = c->ar_bbox = c->ar_host = nullptr;
+    c->ar_flag = c->ar_pos = c->wc_cell = c->wc_count = c->wc_start = c->wc_mem = nullptr;
+}
+
+ps_err arrow_check_dist(Ctx* c) {
+    hipStream_t st = c->stream;
+    const int Nm = c->Nm, Np = c->Np;
+    PS_HIP(c, hipMemcpyAsync(c->xpd, c->x, sizeof(double) * 3 * (size_t)Np, hipMemcpyDeviceToDevice, st));
+    const double rverl_ab = (double)(float)(1.5 + c->p.beta);
+    if (Nm > 0) {
+        k_ar_flag<<<dof_blocks(Nm), AB, 0, st>>>(c->sc, c->xpd, rverl_ab, c->ar_flag);
+        PS_LAUNCH(c);
+        PS_TRY(scan_ints(c, c->ar_flag, c->ar_pos, Nm));
+    } else {
+        PS_HIP(c, hipMemsetAsync(c->ar_pos, 0, sizeof(int) * 2, st));
+    }
+    k_ar_counts<<<dof_blocks(Np), AB, 0, st>>>(Nm, c->Nc, c->ar_flag, c->ar_pos, c->nb_count);
+    PS_LAUNCH(c);
+    PS_TRY(scan_ints(c, c->nb_count, c->nb_off, Np));
+    PS_HIP(c, hipMemcpyAsync(c->hflag, c->nb_off + Np, sizeof(int), hipMemcpyDeviceToHost, st));
+    PS_HIP(c, hipStreamSynchronize(st));
+    const int64_t total = c->hflag[0];
+    if (total > c->nb_cap) {
+        PS_HIP(c, hipFree(c->nb_part));
+        c->nb_cap = total + total / 4 + 1024;
+        PS_HIP(c, hipMalloc(&c->nb_part, sizeof(int) * (size_t)c->nb_cap));
+    }
+    if (Nm > 0) {
+        k_ar_fill<<<dof_blocks(Nm), AB, 0, st>>>(Nm, c->ar_flag, c->ar_pos, c->nb_off, c->nb_part);
+        PS_LAUNCH(c);
+    }
+    c->nvlist = total / 2;
+    c->cnt.nvlist = c->nvlist;
+    c->have_pd = true;
+    return PS_OK;
+}
+
+ps_err arrow_mob(Ctx* c) {
+    hipStream_t st = c->stream;
+    PS_HIP(c, hipMemcpyAsync(c->xm, c->xpd, sizeof(double) * 3 * (size_t)c->Np, hipMemcpyDeviceToDevice, st));
+    if (c->Nm > 0 && c->Nc == 1) {
+        k_ar_mcm<<<dof_blocks(c->Nm), AB, 0, st>>>(c->sc, c->xm, c->p.beta * c->p.beta, c->ar_mcm);
+        PS_LAUNCH(c);
+    }
+    c->have_mob = true;
+    return PS_OK;
+}
+
+ps_err arrow_schur(Ctx* c) {
+    if (c->Nc != 1) return PS_OK;
+    hipStream_t st = c->stream;
+    const int64_t nm3 = c->sc.nm3;
+    const int nb = std::min(c->ar_nb, 1024);
+    if (nm3 > 0) {
+        k_ar_gram<<<nb, AB, 0, st>>>(nm3, c->ar_mcm, c->ar_part);
+        PS_LAUNCH(c);
+    }
+    k_ar_schur<<<1, 128, 0, st>>>(c->ar_part, nm3 > 0 ? nb : 0, c->p.beta, c->ar_mcc, c->Mss);
+    PS_LAUNCH(c);
+    c->ar_synced = true;
+    return PS_OK;
+}
+
+ps_err arrow_apply(Ctx* c, const double* x, double* y, int saddle) {
+    hipStream_t st = c->stream;
+    const int64_t nm3 = c->sc.nm3, n = c->n, nu = c->nu;
+    const unsigned nb = dof_blocks(nm3);
+    if (nm3 > 0) {
+        k_ar_apply<<<nb, AB, 0, st>>>(nm3, n, 1.0 / c->p.beta, c->ar_mcm, x, c->Nc == 1 ? x + nm3 + 3 : nullptr, y,
+                                      c->ar_part, saddle);
+        PS_LAUNCH(c);
+    }
+    if (c->Nc == 1) {
+        k_ar_apply_fin<<<1, 64, 0, st>>>(nm3, n, nu, c->ar_part, nm3 > 0 ? (int)nb : 0, c->ar_mcc, x, y, saddle);
+        PS_LAUNCH(c);
+    }
+    c->cnt.applies++;
+    return PS_OK;
+}
+
+ps_err arrow_noise(Ctx* c, const double* xi, double* out) {
+    hipStream_t st = c->stream;
+    const int64_t nm3 = c->sc.nm3;
+    const unsigned nb = dof_blocks(nm3);
+    if (c->Nc == 1 && !c->ar_synced) PS_TRY(arrow_schur(c));
+    if (nm3 > 0) {
+        k_ar_noise<<<nb, AB, 0, st>>>(nm3, std::sqrt(1.0 / c->p.beta), c->ar_mcm, xi, out, c->ar_part);
+        PS_LAUNCH(c);
+    }
+    if (c->Nc == 1) {
+        k_ar_noise_fin<<<1, 64, 0, st>>>(nm3, c->ar_part, nm3 > 0 ? (int)nb : 0, std::sqrt(c->p.beta), c->ar_mcc, xi,
+                                         out);
+        PS_LAUNCH(c);
+    }
+    c->cnt.sqrt_calls++;
+    return PS_OK;
+}
+
+ps_err arrow_diag(Ctx* c, double* d, int inverse) {
+    k_ar_diag<<<dof_blocks(c->ns), AB, 0, c->stream>>>(c->sc.nm3, c->n, c->ns, c->p.beta, c->ar_mcc, d, inverse);
+    PS_LAUNCH(c);
+    return PS_OK;
+}
+
+// The monomer cell list of the WCA forces: the reference's grid (src/cell_list.cpp:23-96) with cells
+// of the AA Verlet cutoff, periodic when every axis has >= 3 cells, else one cell; unbounded: the
+// monomers' bounding box.
+ps_err arrow_wca(Ctx* c) {
+    hipStream_t st = c->stream;
+    const int Nm = c->Nm;
+    if (Nm == 0) return PS_OK;
+    const double rcell = (double)(float)(0.05 + 2 * c->p.beta);
+    WcaGrid g{};
+    if (c->sc.box_active) {
+        int nmin = 1 << 30;
+        for (int d = 0; d < 3; d++) {
+            g.nc[d] = (int)std::floor(c->sc.L[d] / rcell);
+            nmin = std::min(nmin, g.nc[d]);
+        }
+        if (nmin >= 3) {
+            for (int d = 0; d < 3; d++) {
+                g.size[d] = c->sc.L[d] / g.nc[d];
+                g.org[d] = -0.5 * c->sc.L[d];
+            }
+            g.periodic = 1;
+        } else {
+            for (int d = 0; d < 3; d++) {
+                g.nc[d] = 1;
+                g.size[d] = c->sc.L[d] > 0.0 ? c->sc.L[d] : 1.0;
+                g.org[d] = -0.5 * c->sc.L[d];
+            }
+        }
+    } else {
+        if (!c->ar_bbox) PS_HIP(c, hipMalloc(&c->ar_bbox, sizeof(double) * 6));
+        k_wc_bbox<<<1, 256, 0, st>>>(Nm, c->x, c->ar_bbox);
+        PS_LAUNCH(c);
+        PS_HIP(c, hipMemcpyAsync(c->ar_host, c->ar_bbox, sizeof(double) * 6, hipMemcpyDeviceToHost, st));
+        PS_HIP(c, hipStreamSynchronize(st));
+        for (int d = 0; d < 3; d++) {
+            const double ext = c->ar_host[3 + d] - c->ar_host[d];
+            g.nc[d] = std::max(1, (int)std::floor(ext / rcell));
+            g.size[d] = ext > 0.0 ? ext / g.nc[d] : rcell;
+            g.org[d] = c->ar_host[d];
+        }
+    }
+    const int64_t ncells = (int64_t)g.nc[0] * g.nc[1] * g.nc[2];
+    if (ncells > (int64_t)1 << 30) {
+        c->set_error("WCA cell list: more than 2^30 cells");
+        return PS_ERR_ARG;
+    }
+    if (ncells > c->wc_cap) {
+        for (int** p : {&c->wc_count, &c->wc_start})
+            if (*p) (void)hipFree(*p);
+        c->wc_cap = ncells;
+        PS_HIP(c, hipMalloc(&c->wc_count, sizeof(int) * (size_t)ncells));
+        PS_HIP(c, hipMalloc(&c->wc_start, sizeof(int) * (size_t)(ncells + 2)));
+    }
+    if (!c->wc_cell) {
+        PS_HIP(c, hipMalloc(&c->wc_cell, sizeof(int) * (size_t)Nm));
+        PS_HIP(c, hipMalloc(&c->wc_mem, sizeof(int) * (size_t)Nm));
+    }
+    for (int d = 0; d < 3; d++) {
+        c->wc_dims[d] = g.nc[d];
+        c->wc_geom[d] = g.size[d];
+        c->wc_geom[3 + d] = g.org[d];
+    }
+    c->wc_geom[6] = g.periodic;
+    PS_HIP(c, hipMemsetAsync(c->wc_count, 0, sizeof(int) * (size_t)ncells, st));
+    k_wc_bin<<<dof_blocks(Nm), AB, 0, st>>>(Nm, g, c->x, c->wc_cell, c->wc_count);
+    PS_LAUNCH(c);
+    PS_TRY(scan_ints(c, c->wc_count, c->wc_start, (int)ncells));
+    PS_HIP(c, hipMemsetAsync(c->wc_count, 0, sizeof(int) * (size_t)ncells, st));
+    k_wc_scatter<<<dof_blocks(Nm), AB, 0, st>>>(Nm, c->wc_cell, c->wc_start, c->wc_count, c->wc_mem);
+    PS_LAUNCH(c);
+    k_wc_sort<<<dof_blocks(ncells), AB, 0, st>>>((int)ncells, c->wc_start, c->wc_mem);
+    PS_LAUNCH(c);
+    const double sig = (double)(float)(2. * c->p.beta);
+    const double rcut = (double)(float)(std::pow(2, 1.0 / 6.0) * sig);
+    k_wc_force<<<dof_blocks(Nm), AB, 0, st>>>(c->sc, g, c->x, c->wc_cell, c->wc_start, c->wc_mem, sig, rcut,
+                                              c->p.epsilon, c->rhs);
+    PS_LAUNCH(c);
+    c->wc_built = true;
+    return PS_OK;
+}

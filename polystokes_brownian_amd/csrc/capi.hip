@@ -200,6 +200,7 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s) {
     PS_TRY(check_dist_dev(c));
     PS_TRY(rhs_dev(c, 0));
     PS_TRY(mob_prepare(c));
+    if (c->arrow) PS_TRY(arrow_schur(c));   // sync_mcc_schur_correction (src/run.cpp:92)
     PS_TRY(solve(c, c->Xdet, 0));
     k_extract<<<eb(nu), TB, 0, st>>>(c->v_det_n, c->Xdet, n, nu);
     PS_LAUNCH(c);
@@ -265,6 +266,7 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s) {
     PS_TRY(check_dist_dev(c));
     PS_TRY(rhs_dev(c, 0));
     PS_TRY(mob_prepare(c));
+    if (c->arrow) PS_TRY(arrow_schur(c));   // src/run.cpp:129 (the drift's mob() calls take none)
     PS_TRY(solve(c, c->Xdet, 0));
     k_extract<<<eb(nu), TB, 0, st>>>(c->v_det_c, c->Xdet, n, nu);
     PS_LAUNCH(c);
@@ -307,6 +309,7 @@ void destroy_ctx(Ctx* c) {
     apply_free(c);
     krylov_free(c);
     pairs_free(c);
+    arrow_free(c);
     dense_free(c);
     lub_free(c);
     if (c->dense_io) (void)hipFree(c->dense_io);
@@ -430,8 +433,9 @@ ps_err ps_create(const ps_params* p, void** out) {
         g_create_error = "Np must equal Nm + Nc and be at least 2";
         return PS_ERR_ARG;
     }
-    if (!p->mm_HI) {
-        g_create_error = "only the mm_HI=True (full grand mobility) regime is implemented";
+    if (!p->mm_HI && p->Nc > 1) {
+        // the arrowhead operator has no colloid-colloid block (src/matfree_A.cpp:13-16, :124-127)
+        g_create_error = "mm_HI=False supports at most one colloid (the reference's arrowhead operator)";
         return PS_ERR_ARG;
     }
     if (p->dt <= 0 || p->beta <= 0) {
@@ -457,6 +461,7 @@ ps_err ps_create(const ps_params* p, void** out) {
     c->Np = p->Np;
     c->Nm = p->Nm;
     c->Nc = p->Nc;
+    c->arrow = !p->mm_HI;
     c->n = 3 * (int64_t)c->Nm + 11 * (int64_t)c->Nc;
     c->nu = 3 * (int64_t)c->Nm + 6 * (int64_t)c->Nc;
     c->ns = c->n + c->nu;
@@ -544,6 +549,7 @@ ps_err ps_create(const ps_params* p, void** out) {
     if (apply_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
     if (krylov_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
     if (pairs_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
+    if (c->arrow && arrow_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
     for (double* v : {c->up, c->udiff, c->v_det_n, c->v_det_c, c->v_brown, c->fext, c->drift_v})
         (void)hipMemsetAsync(v, 0, sizeof(double) * (size_t)c->nu, c->stream);
     for (double* v : {c->X, c->Xdet, c->Xd})
@@ -673,9 +679,41 @@ ps_err ps_mob_mult(void* h, const double* f, double* u) {
     if (!c->have_mob) return fail(c, PS_ERR_STATE, "apply before mob");
     return with_io(c, c->n, f, c->n, u, [&](double* a, double* b) { return mob_apply(c, a, b, c->p.precision); });
 }
+ps_err ps_sync_mcc(void* h) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    if (!c->arrow) return fail(c, PS_ERR_STATE, "sync_mcc: the mm_HI=False regime only");
+    if (!c->have_mob) return fail(c, PS_ERR_STATE, "sync_mcc before mob");
+    return arrow_schur(c);
+}
+ps_err ps_get_schur(void* h, double* mcc, double* q, double* lam) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    if (!c->arrow || c->Nc != 1) return fail(c, PS_ERR_STATE, "get_schur: mm_HI=False with one colloid only");
+    double buf[253];
+    PS_HIP(c, hipMemcpyAsync(buf, c->ar_mcc, sizeof(buf), hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    if (mcc) std::copy(buf, buf + 121, mcc);
+    if (q) std::copy(buf + 121, buf + 242, q);
+    if (lam) std::copy(buf + 242, buf + 253, lam);
+    return PS_OK;
+}
+ps_err ps_get_cell_list(void* h, int32_t* cell_of, int32_t* dims, double* geom) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    if (!c->arrow || !c->wc_built) return fail(c, PS_ERR_STATE, "get_cell_list: no monomer cell list built yet");
+    if (cell_of && c->Nm > 0) {
+        PS_HIP(c, hipMemcpyAsync(cell_of, c->wc_cell, sizeof(int) * (size_t)c->Nm, hipMemcpyDeviceToHost, c->stream));
+        PS_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    if (dims) std::copy(c->wc_dims, c->wc_dims + 3, dims);
+    if (geom) std::copy(c->wc_geom, c->wc_geom + 7, geom);
+    return PS_OK;
+}
 ps_err ps_dense_build(void* h, int storage) {
     Ctx* c = as_ctx(h);
     if (!c) return PS_ERR_ARG;
+    if (c->arrow) return fail(c, PS_ERR_STATE, "dense_build: the arrowhead mobility is O(N); nothing to store");
     return dense_build(c, storage);
 }
 ps_err ps_dense_mult_dev(void* h, const double* X, double* Y, int nrhs) {

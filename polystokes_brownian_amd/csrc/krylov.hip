@@ -362,9 +362,11 @@ ps_err krylov_alloc(Ctx* c) {
     PS_HIP(c, hipMalloc(&c->gw, sizeof(double) * 64));
     PS_HIP(c, hipMalloc(&c->ir_r, sizeof(double) * (size_t)ns));
     PS_HIP(c, hipMalloc(&c->ir_d, sizeof(double) * (size_t)ns));
-    PS_TRY(saddle_diag_dev(c, c->dinv));
-    k_recip<<<eblocks(ns), VB, 0, c->stream>>>(c->dinv, ns);
-    PS_LAUNCH(c);
+    if (!c->arrow) {   // the arrowhead diagonal is taken at the first MINRES solve (arrow.hip)
+        PS_TRY(saddle_diag_dev(c, c->dinv));
+        k_recip<<<eblocks(ns), VB, 0, c->stream>>>(c->dinv, ns);
+        PS_LAUNCH(c);
+    }
     if (c->p.solver == PS_SOLVER_BLOCK) {
         const size_t n5 = 5 * (size_t)std::max(c->Nc, 1);
         PS_HIP(c, hipMalloc(&c->Mss, sizeof(double) * n5 * n5));
@@ -397,6 +399,7 @@ void krylov_free(Ctx* c) {
 }
 
 ps_err saddle_diag_dev(Ctx* c, double* d) {
+    if (c->arrow) return arrow_diag(c, d, 0);
     k_diag<<<eblocks(c->ns), VB, 0, c->stream>>>(c->sc, d);
     PS_LAUNCH(c);
     return PS_OK;
@@ -517,6 +520,12 @@ static ps_err minres_core(Ctx* c, int64_t N, const double* dinv, Op&& op, const 
 static ps_err minres_dev(Ctx* c, const double* b, double* x, int warm, double rtol, int maxit, int prec,
                          ps_solve_info* info) {
     ps_solve_info ii{};
+    if (c->arrow && !c->ar_diag_frozen) {
+        // mm_HI off: the operator is a PETSc MatShell whose state never changes, so PCJacobi takes the
+        // diagonal at the first KSPSolve and keeps it for the run (PCSetUp skips an unchanged operator)
+        PS_TRY(arrow_diag(c, c->dinv, 1));
+        c->ar_diag_frozen = true;
+    }
     auto op = [c, prec](const double* in, double* out) { return saddle_apply(c, in, out, prec); };
     ps_err r = minres_core(c, c->ns, c->dinv, op, b, x, warm, rtol, maxit, &ii);
     c->last_minres_its = ii.iterations;
@@ -948,6 +957,10 @@ __global__ __launch_bounds__(VB) void k_lz_next(int64_t n, double* __restrict__ 
 // never used, so the result equals stopping at m).
 ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* out, ps_sqrt_info* info) {
     const int64_t n = c->n;
+    if (c->arrow) {
+        if (info) *info = ps_sqrt_info{};
+        return arrow_noise(c, xi, out);
+    }
     const unsigned nch = (unsigned)((n + LZ_CH - 1) / LZ_CH);
     const unsigned ne = eblocks(n);
     double* red2 = c->red + (size_t)nch * 64;   // after the dot partials of up to 64 vectors

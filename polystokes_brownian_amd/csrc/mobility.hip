@@ -914,6 +914,10 @@ void apply_free(Ctx* c) {
 
 // mob(): snapshot the positions of the last check_dist and compute the carried translation tile.
 ps_err mob_prepare(Ctx* c) {
+    if (c->arrow) {
+        c->cnt.mob_calls++;
+        return arrow_mob(c);
+    }
     PS_HIP(c, hipMemcpyAsync(c->xm, c->xpd, sizeof(double) * 3 * (size_t)c->Np, hipMemcpyDeviceToDevice,
                              c->stream));
     k_stale<<<1, 64, 0, c->stream>>>(c->sc, c->xm, c->stale_i, c->stale_j, c->stale);
@@ -941,15 +945,18 @@ ps_err mob_prepare(Ctx* c) {
 }
 
 ps_err mob_apply(Ctx* c, const double* f, double* u, int prec) {
+    if (c->arrow) return arrow_apply(c, f, u, 0);
     if (prec == PS_PREC_MIXED && c->xm32) return launch_apply<float>(c, c->xm32, c->aw32, f, u, 0);
     return launch_apply<double>(c, c->xm, c->aw64, f, u, 0);
 }
 
 ps_err mob_apply_stresslet_rows(Ctx* c, const double* f, double* u) {
+    if (c->arrow) return arrow_apply(c, f, u, 0);   // every row: O(N) anyway
     return launch_apply<double>(c, c->xm, c->aw64, f, u, 2);
 }
 
 ps_err saddle_apply(Ctx* c, const double* x, double* y, int prec) {
+    if (c->arrow) return arrow_apply(c, x, y, 1);
     if (prec == PS_PREC_MIXED && c->xm32) return launch_apply<float>(c, c->xm32, c->aw32, x, y, 1);
     return launch_apply<double>(c, c->xm, c->aw64, x, y, 1);
 }

@@ -446,7 +446,14 @@ void pairs_free(Ctx* c) {
     c->hflag = nullptr;
 }
 
+ps_err scan_ints(Ctx* c, const int* in, int* out, int n) {
+    k_scan<<<1, 1024, 0, c->stream>>>(in, out, n);
+    PS_LAUNCH(c);
+    return PS_OK;
+}
+
 ps_err check_dist_dev(Ctx* c) {
+    if (c->arrow) return arrow_check_dist(c);
     const int Np = c->Np;
     hipStream_t st = c->stream;
     PairConst pc = make_pc(c);
@@ -535,6 +542,8 @@ ps_err rhs_dev(Ctx* c, int drift) {
     k_forces<<<(unsigned)((Np + TB - 1) / TB), TB, 0, st>>>(fc, c->xpd, c->x, c->xpos0, c->nb_off, c->nb_part,
                                                            boff, bk, bj, c->rhs);
     PS_LAUNCH(c);
+    // mm_HI off: monomer-monomer excluded volume through the monomer cell list (src/rhs.cpp:200-205)
+    if (c->arrow && c->p.mono_ev) PS_TRY(arrow_wca(c));
     return PS_OK;
 }
 
@@ -548,7 +557,8 @@ ps_err vlist_copy(Ctx* c, int32_t* rows, int32_t* partners, int64_t* pidx, int64
     if (off[Np] > 0)
         PS_HIP(c, hipMemcpy(part.data(), c->nb_part, sizeof(int) * off[Np], hipMemcpyDeviceToHost));
     const int64_t Nm = c->Nm, Nc = c->Nc;
-    const int64_t npAA = Nm * (Nm - 1) / 2, npAB = Nm * Nc;
+    // with mm_HI off no monomer-monomer pair is stored: AB pairs are numbered from 0 (src/init.cpp:263-290)
+    const int64_t npAA = c->arrow ? 0 : Nm * (Nm - 1) / 2, npAB = Nm * Nc;
     int64_t cnt = 0;
     for (int k = 0; k < Np; k++)
         for (int t = off[k]; t < off[k + 1]; t++) {

@@ -163,6 +163,26 @@ struct Ctx {
     double* dense_io = nullptr;     // 2 x 16 x n, host-pointer entry point
     size_t dense_bytes = 0;
     int dense_storage = 0;
+    // ---- mm_HI == false regime (arrow.hip)
+    bool arrow = false;
+    double* ar_mcm = nullptr;       // [8][3Nm]: the rotation and stresslet rows of M^cm
+    double* ar_part = nullptr;      // per-block partials
+    double* ar_mcc = nullptr;       // floored M^cc (121), eigenvectors Q (121), floored spectrum (11)
+    double* ar_bbox = nullptr;      // monomer bounding box (unbounded WCA grid)
+    double* ar_host = nullptr;      // pinned
+    int* ar_flag = nullptr;         // [Nm+2] monomer-colloid Verlet flags
+    int* ar_pos = nullptr;          // [Nm+2] their exclusive scan
+    int ar_nb = 0;
+    bool ar_synced = false;         // a Schur floor has run
+    bool ar_diag_frozen = false;    // the Jacobi diagonal is fixed at the first MINRES solve
+    int* wc_cell = nullptr;         // WCA cell list: cell of each monomer
+    int* wc_count = nullptr;
+    int* wc_start = nullptr;        // [ncells+2]
+    int* wc_mem = nullptr;          // monomers by cell, ascending within a cell
+    int64_t wc_cap = 0;
+    int wc_dims[3] = {0, 0, 0};
+    double wc_geom[7] = {0, 0, 0, 0, 0, 0, 0};   // size (3), origin (3), periodic
+    bool wc_built = false;
     // ---- lubrication (lub.hip)
     double* lub_tab[3] = {nullptr, nullptr, nullptr};   // r[n], f[n k], slopes[n k] per table
     int lub_rows[3] = {0, 0, 0};
@@ -216,6 +236,16 @@ ps_err run_steps(Ctx* c, int nsteps);
 ps_err dense_build(Ctx* c, int storage);
 ps_err dense_mult_dev(Ctx* c, const double* X, double* Y, int nrhs);   // Y_j = M X_j, vector-major
 void dense_free(Ctx* c);
+ps_err arrow_alloc(Ctx* c);
+void arrow_free(Ctx* c);
+ps_err arrow_check_dist(Ctx* c);                                   // Verlet list, AB + BB pairs only
+ps_err arrow_mob(Ctx* c);                                          // snapshot + M^cm
+ps_err arrow_schur(Ctx* c);                                        // sync_mcc_schur_correction
+ps_err arrow_apply(Ctx* c, const double* x, double* y, int saddle);
+ps_err arrow_noise(Ctx* c, const double* xi, double* out);         // block-Cholesky slip noise
+ps_err arrow_diag(Ctx* c, double* d, int inverse);
+ps_err arrow_wca(Ctx* c);                                          // monomer WCA into the RHS
+ps_err scan_ints(Ctx* c, const int* in, int* out, int n);          // out[n] = total, out[n+1] = max
 ps_err lub_set_table(Ctx* c, int which, const double* rows, int nrows, int ncols);
 ps_err lub_build(Ctx* c, double rcut, int64_t* npairs);
 ps_err lub_apply_dev(Ctx* c, const double* uw, const double* einf, double* fts);

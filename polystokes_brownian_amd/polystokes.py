@@ -71,6 +71,9 @@ def lib():
         L.ps_lub_build.argtypes = [vp, C.c_double, C.POINTER(C.c_int64)]
         for name in ("ps_lub_apply", "ps_lub_apply_dev"):
             getattr(L, name).argtypes = [vp, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ps_sync_mcc.argtypes = [vp]
+        L.ps_get_schur.argtypes = [vp, dp, dp, dp]
+        L.ps_get_cell_list.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), dp]
         L.ps_lub_interp.argtypes = [dp, ip, ip, C.c_double, dp]
         L.ps_lub_slopes.argtypes = [dp, ip, ip, dp]
         _lib = L
@@ -241,6 +244,26 @@ class Context:
         Y = np.zeros_like(X)
         self._check(lib().ps_dense_mult(self.h, X.ctypes.data, Y.ctypes.data, X.shape[0]))
         return Y
+
+    def sync_mcc(self):
+        """sync_mcc_schur_correction of the mm_HI=False regime (ps_sync_mcc)."""
+        self._check(lib().ps_sync_mcc(self.h))
+
+    def schur(self):
+        """(floored M^cc, eigenvectors Q as columns, floored spectrum) of the last ps_sync_mcc."""
+        mcc, q, lam = np.zeros((11, 11)), np.zeros((11, 11)), np.zeros(11)
+        self._check(lib().ps_get_schur(self.h, _p(mcc), _p(q), _p(lam)))
+        return mcc, q, lam
+
+    def cell_list(self):
+        """Monomer cell list of the last WCA evaluation: (cell of each monomer, dims, size, origin,
+        periodic)."""
+        cell = np.zeros(max(self.Nm, 1), dtype=np.int32)
+        dims = np.zeros(3, dtype=np.int32)
+        geom = np.zeros(7)
+        self._check(lib().ps_get_cell_list(self.h, cell.ctypes.data_as(C.POINTER(C.c_int32)),
+                                           dims.ctypes.data_as(C.POINTER(C.c_int32)), _p(geom)))
+        return cell[: self.Nm], dims, geom[:3].copy(), geom[3:6].copy(), bool(geom[6])
 
     def lub_set_tables(self, tables=None):
         """Load the three two-sphere tables (rows with r first; default: the bundled lubdat files)."""

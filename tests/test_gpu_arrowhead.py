@@ -1,0 +1,169 @@
+"""GPU parity of the mm_HI=False (arrowhead) regime: the HIP path (csrc/arrow.hip, through the C ABI)
+against the oracle on the same inputs.
+
+Tolerances: Verlet pairs and the monomer cell list bit-exact; the floored M^cc, applies, noise and
+forces 1e-12 relative; solves and steps 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+from polystokes_brownian_amd import configs
+from polystokes_brownian_amd.polystokes import Context
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300)
+
+
+def arrow_pair(nd=200, beta=0.1, L=6.0, seed=5, mono_ev=0, clearance=1.5, cloud=False, **over):
+    from oracle.pyoracle import Oracle
+    w = configs.workload("arrow_thermal")
+    w.update(dict(Npoly=nd, Nm=2 * nd, Np=2 * nd + 1, beta=beta, L=L, ic_seed=seed, mono_ev=mono_ev))
+    if mono_ev:
+        # the WCA core is stiff against a monomer mobility of 1/beta: at dt = 1e-3 one Brownian step is
+        # comparable to sigma and a pair inside the core is thrown across the box, so the WCA cases
+        # step at dt = 1e-5
+        w["dt"] = 1e-5
+    # positions are drawn in a box large enough to clear the colloid (a box below 3 cells per axis
+    # is then applied as is: minimum images and one WCA cell)
+    gen_L = L if (L and L >= 3.0) else (6.0 if L else 10.0)
+    # with mono_ev, bonds start outside the WCA core (a bonded pair deep inside it blows up at once)
+    wca = 2 ** (1 / 6) * 2 * beta if mono_ev else 0.0
+    x0 = configs.dumbbell_configuration(nd, w["r0"], w["kbond"], w["kT"], gen_L, seed, clearance=clearance,
+                                        min_sep=wca, min_bond=wca)
+    if cloud:
+        # a dense monomer cloud beside the colloid: many monomer pairs inside the WCA cutoff
+        x = x0.reshape(-1, 3)
+        x[: nd] = np.random.default_rng(seed).uniform(-1.0, 1.0, size=(nd, 3)) + np.array([2.2, 0, 0])
+        x0 = x.reshape(-1)
+    if L is None:
+        w["L"] = None
+    pd = configs.params_dict(w)
+    pd.update(over)
+    g = Context(pd)
+    g.initial_configuration(x0)
+    o = Oracle(pd)
+    o.initial_configuration(x0)
+    return g, o, x0, pd
+
+
+CASES = {
+    # name: kwargs -- "floor" cases put monomers against the colloid with a large beta so that the
+    # Schur complement has eigenvalues below 0.1
+    "thermal": dict(nd=300, beta=0.1, L=6.0),
+    "floor": dict(nd=120, beta=0.5, L=4.0, clearance=1.05),
+    "nobox": dict(nd=150, beta=0.1, L=None),
+    "small_beta": dict(nd=500, beta=0.001, L=10.0, dt=1e-5),   # the published case's beta and dt
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_arrow_vlist_bit_exact(name):
+    g, o, _, _ = arrow_pair(**CASES[name])
+    n_g = g.check_dist()
+    n_o = o.check_dist()
+    assert n_g == n_o
+    for a, b in zip(g.vlist(), o.vlist()):
+        np.testing.assert_array_equal(a, b)
+    g.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_arrow_schur_apply_noise(name):
+    g, o, _, _ = arrow_pair(**CASES[name])
+    for h in (g, o):
+        h.check_dist()
+        h.mob()
+    M0 = o.mob_dense()                      # M^cc is still the self block here
+    Mcm, base = M0[-11:, :-11], M0[-11:, -11:]
+    floor_acts = np.linalg.eigvalsh(base - CASES[name]["beta"] * Mcm @ Mcm.T).min() < 0.1
+    assert floor_acts == (name == "floor")
+    g.sync_mcc()
+    o.schur_sync()
+    mg, qg, lg = g.schur()
+    mo, qo, lo = o.schur()
+    assert rel(mg, mo) < 1e-12
+    assert rel(np.sort(lg), np.sort(lo)) < 1e-12
+    # the floored Schur complement's square root does not depend on the eigenvector choice
+    np.testing.assert_allclose(qg @ np.diag(np.sqrt(lg)) @ qg.T, qo @ np.diag(np.sqrt(lo)) @ qo.T,
+                               atol=1e-12 * np.sqrt(lo.max()))
+    rng = np.random.default_rng(7)
+    n, ns = o.n, o.ns
+    for _ in range(2):
+        f = rng.normal(size=n)
+        assert rel(g.mob_mult(f), o.mob_mult(f)) < 1e-12
+        xs = rng.normal(size=ns)
+        assert rel(g.saddle_mult(xs), o.saddle_mult(xs)) < 1e-12
+        xi = rng.normal(size=n)
+        assert rel(g.sqrt_apply(xi)[0], o.sqrt_apply(xi)[0]) < 1e-12
+    assert rel(g.saddle_diag(), o.saddle_diag()) < 1e-12
+    g.close()
+
+
+@pytest.mark.parametrize("name", ["thermal", "floor"])
+@pytest.mark.parametrize("solver", [0, 1])
+def test_arrow_saddle_solve(name, solver):
+    g, o, _, _ = arrow_pair(solver=solver, ksp_rtol=1e-12, **CASES[name])
+    for h in (g, o):
+        h.check_dist()
+        h.mob()
+    g.sync_mcc()
+    o.schur_sync()
+    b = np.random.default_rng(3).normal(size=o.ns)
+    xg, ig, rg = g.saddle_solve(b)
+    xo, io, ro = o.saddle_solve(b)
+    assert rel(xg, xo) < 1e-10
+    if solver == 0:
+        assert abs(ig.iterations - io.iterations) <= 1
+    g.close()
+
+
+@pytest.mark.parametrize("L", [6.0, None, 0.6])
+def test_arrow_wca_forces_and_cell_list(L):
+    g, o, _, _ = arrow_pair(nd=400, beta=0.1, L=6.0, mono_ev=1, cloud=True)
+    if L != 6.0:
+        g.close()
+        g, o, _, _ = arrow_pair(nd=400, beta=0.1, L=L, mono_ev=1, cloud=True)
+    for h in (g, o):
+        h.check_dist()
+    rg, ro = g.rhs(), o.rhs()
+    assert rel(rg, ro) < 1e-12
+    cg, dg, sg, og, pg = g.cell_list()
+    co, do, so, oo, po = o.cell_list()
+    np.testing.assert_array_equal(dg, do)
+    np.testing.assert_array_equal(sg, so)
+    np.testing.assert_array_equal(og, oo)
+    assert pg == po
+    np.testing.assert_array_equal(cg, co)
+    g.close()
+
+
+# The floor case steps with the exact block solver only: it is ill-conditioned (beta 0.5, monomers
+# pressed into the colloid), and with MINRES the 1e-12 residual times the conditioning and the RFD's
+# 1/eps reaches 1e-9 in the steps; its MINRES solve is checked alone in test_arrow_saddle_solve.
+@pytest.mark.parametrize("name,mono_ev,solver", [("thermal", 0, 0), ("thermal", 1, 0), ("thermal", 0, 1),
+                                                  ("floor", 0, 1), ("nobox", 1, 0), ("small_beta", 0, 1)])
+def test_arrow_steps(name, mono_ev, solver):
+    # MINRES to 1e-12: the RFD drift is the difference of two solves at positions 2 eps apart scaled by
+    # kT / (2 eps), so at the workloads' 1e-6 two implementations' rounding-level differences in the
+    # unconverged iterates reach the positions amplified by 1 / eps (the step tests of the full
+    # mobility solve tightly for the same reason)
+    kw = dict(CASES[name], mono_ev=mono_ev)
+    g, o, _, _ = arrow_pair(solver=solver, ksp_rtol=1e-12, **kw)
+    g.run(3)
+    o.run(3)
+    tol = 1e-10
+    xg, qg, tg = g.state()
+    xo, qo, to = o.state()
+    assert tg == to
+    assert rel(xg, xo) < tol, (name, rel(xg, xo))
+    assert rel(qg, qo) < tol
+    vg, vo = g.vectors(), o.vectors()
+    for k in ("up", "v_det_n", "v_det_c", "v_brown"):
+        assert rel(vg[k], vo[k]) < tol, (k, rel(vg[k], vo[k]))
+    # udiff is the raw difference of the two RFD solves (before the kT / (2 eps) scaling): with MINRES
+    # its relative error is the solves' 1e-12 residual amplified by 1 / eps
+    assert rel(vg["udiff"], vo["udiff"]) < (tol if solver == 1 else 1e-8)
+    g.close()
