@@ -140,6 +140,78 @@ def dense_bench(args, configs, Context):
     print(json.dumps(out), flush=True)
 
 
+# The reference's only published throughput: mm_HI=False, Nm = 10,000 monomers in 5000 dumbbells + 1
+# trapped colloid, beta 0.001, dt 1e-5, box 10^3, 300 steps, serial CPU.  Best figure: MINRES with a
+# single-threaded OpenBLAS, 23.6 s / 300 steps (docs/profiling_baseline_serial.md:109-112, :128).
+ARROW_PUBLISHED = 10001 * 300 / 23.6
+
+
+def arrow_bench(args, configs, Context, steps):
+    """The mm_HI=False (arrowhead) regime on the published case (workload arrow10k).  Forms: the
+    reference algorithm (Jacobi MINRES rtol 1e-6, diagonal frozen at the first solve) and exact block
+    elimination; both FP64.  Roofline: the DOF pass of the arrowhead apply (HBM-bound), its launches
+    timed by their own dispatch events; algorithmic bytes per launch = the 8 stored rows of M^cm, the
+    monomer input and output (saddle applies also read the multiplier rows and write the constraint
+    rows).  CPU baseline: the oracle on one host core over a bounded number of steps."""
+    w = configs.workload("arrow10k")
+    pd = configs.params_dict(w)
+    x0 = configs.positions(w)
+    Np, nm3 = w["Np"], 3 * w["Nm"]
+    forms = {"fp64/block": dict(solver=1), "fp64/minres": dict(solver=0)}
+    res = {}
+    for form, over in forms.items():
+        ctx = make_ctx(Context, pd, x0, **over)
+        dt, prof, per = timed_run(ctx, steps, args.warmup, "arrow " + form)
+        ctx.close()
+        avg = prof["arrow_ms"] / max(prof["arrow_launches"], 1) * 1e-3
+        res[form] = dict(value=Np * steps / dt, ms_per_step=1e3 * dt / steps, per_step=per,
+                         apply_pass_ms_avg=avg * 1e3, apply_pass_launches=prof["arrow_launches"])
+        log(f"arrow {form}: {res[form]['value']:.4g} bead-steps/s, {res[form]['ms_per_step']:.3f} ms/step, "
+            f"per step {per}, apply pass avg {avg * 1e6:.2f} us over {prof['arrow_launches']} launches")
+    up_b = one_step_up(Context, pd, x0, **forms["fp64/block"])
+    up_m = one_step_up(Context, pd, x0, **forms["fp64/minres"])
+    head = res["fp64/block"]
+    # the block form's applies are mobility applies: 8 rows + x_m + y_m doubles per monomer DOF
+    bytes_launch = 8.0 * (8 + 1 + 1) * nm3
+    achieved = bytes_launch / (head["apply_pass_ms_avg"] * 1e-3) if head["apply_pass_ms_avg"] > 0 else 0.0
+    cpu = None
+    if not args.no_cpu:
+        from oracle.pyoracle import Oracle, get_threads, set_threads
+        nthr = get_threads()
+        set_threads(1)
+        o = Oracle(pd)
+        o.initial_configuration(x0)
+        ncpu = args.arrow_cpu_steps
+        t0 = time.perf_counter()
+        o.run(ncpu)
+        t_cpu = (time.perf_counter() - t0) / ncpu
+        set_threads(nthr)
+        cpu = {"value": Np / t_cpu, "unit": "bead-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{ncpu} steps of the oracle's arrowhead regime (the reference algorithm, Jacobi MINRES "
+                         f"rtol 1e-6) on the same configuration, one host core, {t_cpu * 1e3:.1f} ms/step"}
+        log(f"arrow cpu baseline {cpu}")
+    return {
+        "metric": "bead-steps/s, mm_HI=False arrowhead, the reference's published case",
+        "value": head["value"], "unit": "bead-steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": head["value"] / ARROW_PUBLISHED,
+        "baseline_published": {"value": ARROW_PUBLISHED, "unit": "bead-steps/s",
+                               "source": "docs/profiling_baseline_serial.md:109-112 (MINRES + OpenBLAS, 1 core)"},
+        "dtype": "fp64", "data": "synthetic",
+        "config": {"workload": f"arrow10k: {w['Nm']} monomers in {w['Npoly']} dumbbells + 1 trapped colloid, "
+                               f"beta {w['beta']}, dt {w['dt']}, box {w['L']}, kbond {w['kbond']}, kT {w['kT']}, "
+                               f"mm_HI off; exact FP64 block-eliminated saddle solves",
+                   "global_batch": Np, "seq_len": 1, "parallelism": "single-gpu"},
+        "reference_algorithm": dict(res["fp64/minres"], vs_baseline=res["fp64/minres"]["value"] / ARROW_PUBLISHED),
+        "block_vs_minres_velocity_rel_diff": rel(up_b, up_m),
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k_ar_apply (the DOF pass of the arrowhead apply)", "bytes_per_launch": bytes_launch,
+                     "avg_launch_ms": head["apply_pass_ms_avg"]},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +228,12 @@ def main():
                          "(default c5dense, N = 8,000) with STORAGE; prints its own JSON line")
     ap.add_argument("--profile-form", default=None,
                     help="only run warmup + timed steps of this form (for a profiler), no JSON line")
+    ap.add_argument("--arrow", action="store_true",
+                    help="instead: the mm_HI=False arrowhead regime on the reference's published case; prints "
+                         "its own JSON line")
+    ap.add_argument("--arrow-steps", type=int, default=300, help="timed steps of the arrowhead case (300 as published)")
+    ap.add_argument("--arrow-cpu-steps", type=int, default=20, help="oracle steps of the arrowhead CPU baseline")
+    ap.add_argument("--no-arrow", action="store_true", help="omit the arrowhead case from the default line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the solve is not sharded across GPUs",
@@ -179,6 +257,9 @@ def main():
 
     if args.dense:
         return dense_bench(args, configs, Context)
+    if args.arrow:
+        print(json.dumps(arrow_bench(args, configs, Context, args.arrow_steps)), flush=True)
+        return
 
     w = configs.workload(args.workload)
     pd = configs.params_dict(w)
@@ -294,6 +375,8 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baseline_block_algorithm": cpu_block,
     }
+    if not args.no_arrow:
+        out["arrowhead_published_case"] = arrow_bench(args, configs, Context, args.arrow_steps)
     print(json.dumps(out), flush=True)
 
 

@@ -18,6 +18,8 @@
 #include "ps_common.h"
 #include "tiles.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cmath>
 
@@ -31,25 +33,36 @@ unsigned dof_blocks(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + AB -
 
 // ------------------------------------------------------------------------------------ Verlet list
 // Monomer k pairs with the colloid when their separation is below the AB Verlet cutoff (a float,
-// compared after promotion, src/check_dist.cpp:49-62).
-__global__ void k_ar_flag(SysConst s, const double* __restrict__ xpd, double rverl_ab, int* __restrict__ flag) {
+// compared after promotion, src/check_dist.cpp:49-62).  The same pass snapshots the positions.
+__global__ void k_ar_flag(SysConst s, const double* __restrict__ x, double* __restrict__ xpd, double rverl_ab,
+                          int* __restrict__ flag) {
     const int k = blockIdx.x * AB + threadIdx.x;
+    if (k >= s.Np) return;
+    xpd[3 * k] = x[3 * k];
+    xpd[3 * k + 1] = x[3 * k + 1];
+    xpd[3 * k + 2] = x[3 * k + 2];
     if (k >= s.Nm) return;
     int f = 0;
     if (s.Nc == 1) {
         double e[3], dr, dr_inv;
-        pair_geom_dev(s, xpd, k, s.Nm, e, dr, dr_inv);
+        pair_geom_dev(s, x, k, s.Nm, e, dr, dr_inv);
         f = dr < rverl_ab;
     }
     flag[k] = f;
 }
-// Rows: a monomer's single partner is the colloid; the colloid's row lists its monomers ascending.
-__global__ void k_ar_counts(int Nm, int Nc, const int* __restrict__ flag, const int* __restrict__ pos,
-                            int* __restrict__ nb_count) {
-    const int i = blockIdx.x * AB + threadIdx.x;
-    if (i < Nm) nb_count[i] = flag[i];
-    else if (i < Nm + Nc) nb_count[i] = pos[Nm];
+// Row offsets: a monomer row holds its flag (the colloid as partner), the colloid row all flagged
+// monomers, so nb_off[k] = pos[k] up to the colloid and 2 * total after it; nb_off[Np + 1] is the
+// longest row.
+__global__ void k_ar_offsets(int Nm, int Nc, const int* __restrict__ pos, int* __restrict__ nb_off) {
+    const int k = blockIdx.x * AB + threadIdx.x;
+    const int Np = Nm + Nc;
+    if (k > Np + 1) return;
+    const int total = Nm > 0 ? pos[Nm] : 0;
+    if (k <= Nm) nb_off[k] = Nm > 0 ? pos[k] : 0;
+    else if (k <= Np) nb_off[k] = 2 * total;
+    else nb_off[k] = Nc == 1 ? (total > 0 ? total : 1) : 0;
 }
+// Rows: a monomer's single partner is the colloid; the colloid's row lists its monomers ascending.
 __global__ void k_ar_fill(int Nm, const int* __restrict__ flag, const int* __restrict__ pos,
                           const int* __restrict__ nb_off, int* __restrict__ part) {
     const int k = blockIdx.x * AB + threadIdx.x;
@@ -60,13 +73,18 @@ __global__ void k_ar_fill(int Nm, const int* __restrict__ flag, const int* __res
 
 // ------------------------------------------------------------------------------------ M^cm
 // Column block of monomer k: rotation rows take mobility_AB's b, stresslet rows its flattened gt
-// (placement of src/mob.cpp:1013-1020).
-__global__ void k_ar_mcm(SysConst s, const double* __restrict__ xm, double beta2, double* __restrict__ mcm) {
+// (placement of src/mob.cpp:1013-1020).  The same pass snapshots the positions of this mob().
+__global__ void k_ar_mcm(SysConst s, const double* __restrict__ xpd, double* __restrict__ xm, double beta2,
+                         double* __restrict__ mcm) {
     const int k = blockIdx.x * AB + threadIdx.x;
-    if (k >= s.Nm) return;
+    if (k >= s.Np) return;
+    xm[3 * k] = xpd[3 * k];
+    xm[3 * k + 1] = xpd[3 * k + 1];
+    xm[3 * k + 2] = xpd[3 * k + 2];
+    if (k >= s.Nm || s.Nc != 1) return;
     const int64_t nm3 = s.nm3;
     double e[3], dr, dr_inv, b[3][3], bt[3][3], gt[3][5];
-    pair_geom_dev(s, xm, k, s.Nm, e, dr, dr_inv);
+    pair_geom_dev(s, xpd, k, s.Nm, e, dr, dr_inv);
     tile_ab_dev(dr_inv, e, beta2, b, bt, gt);
     for (int ii = 0; ii < 3; ii++) {
         for (int jj = 0; jj < 3; jj++) mcm[ii * nm3 + 3 * k + jj] = b[ii][jj];
@@ -91,6 +109,31 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* __re
         for (int w = 0; w < AB / 64; w++) a += sh[w][q];
         part[(size_t)blockIdx.x * NV + q] = a;
     }
+}
+
+// After every block has stored its partials: true in the one block that arrives last (which then
+// resets the counter for the next launch).  Partials must be stored before the call.
+__device__ __forceinline__ bool last_block(int* __restrict__ counter) {
+    __shared__ int is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        is_last = atomicAdd(counter, 1) == (int)gridDim.x - 1;
+        if (is_last) *counter = 0;
+    }
+    __syncthreads();
+    return is_last;
+}
+// In the last block: the NR row sums of all blocks' partials, 32 lanes per row, combined in a fixed
+// order (deterministic), into rs[NR] (LDS).  Partials are read past the per-CU cache.
+__device__ __forceinline__ void sum_rows(const double* __restrict__ part, int nb, double* rs) {
+    const int r = threadIdx.x >> 5, l = threadIdx.x & 31;
+    double a = 0.0;
+    if (r < NR)
+        for (int b = l; b < nb; b += 32) a += ((const volatile double*)part)[(size_t)b * NR + r];
+    for (int off = 16; off > 0; off >>= 1) a += __shfl_down(a, off, 32);
+    if (r < NR && l == 0) rs[r] = a;
+    __syncthreads();
 }
 
 // Gram partials of the stored rows: P = M8 M8^T, upper triangle (NSYM entries) per block.
@@ -130,12 +173,16 @@ __global__ __launch_bounds__(128) void k_ar_schur(const double* __restrict__ par
     __shared__ double cs[JN / 2], sn[JN / 2];
     __shared__ int pp[JN / 2], qq[JN / 2];
     __shared__ int done;
+    __shared__ double Pp[3][NSYM];
     const int t = threadIdx.x;
-    if (t < NSYM) {
+    if (t < 3 * NSYM) {   // three lanes per entry, independent loads, combined in a fixed order
+        const int q = t % NSYM, g = t / NSYM;
         double a = 0.0;
-        for (int b = 0; b < nb; b++) a += part[(size_t)b * NSYM + t];
-        P[t] = a;
+        for (int b = g; b < nb; b += 3) a += part[(size_t)b * NSYM + q];
+        Pp[g][q] = a;
     }
+    __syncthreads();
+    if (t < NSYM) P[t] = (Pp[0][t] + Pp[1][t]) + Pp[2][t];
     __syncthreads();
     for (int q = t; q < JN * JN; q += blockDim.x) {
         const int a = q / JN, b = q % JN;
@@ -152,13 +199,23 @@ __global__ __launch_bounds__(128) void k_ar_schur(const double* __restrict__ par
     }
     __syncthreads();
     for (int sweep = 0; sweep < 60; sweep++) {
-        if (t == 0) {
+        if (t < 64) {   // off-diagonal and diagonal mass by one wave: lane t < 55 takes pair t
             double off = 0.0, dia = 0.0;
-            for (int a = 0; a < 11; a++) {
-                dia += A[a][a] * A[a][a];
-                for (int b = a + 1; b < 11; b++) off += A[a][b] * A[a][b];
+            if (t < 55) {
+                int a = 0, rem = t;
+                while (rem >= 10 - a) {
+                    rem -= 10 - a;
+                    a++;
+                }
+                const double v = A[a][a + 1 + rem];
+                off = v * v;
             }
-            done = (off <= 1e-36 * dia || off == 0.0);
+            if (t < 11) dia = A[t][t] * A[t][t];
+            for (int o = 32; o > 0; o >>= 1) {
+                off += __shfl_down(off, o, 64);
+                dia += __shfl_down(dia, o, 64);
+            }
+            if (t == 0) done = (off <= 1e-36 * dia || off == 0.0);
         }
         __syncthreads();
         if (done) break;
@@ -222,12 +279,17 @@ __global__ __launch_bounds__(128) void k_ar_schur(const double* __restrict__ par
 }
 
 // ------------------------------------------------------------------------------------ apply
-// Monomer rows of y = M x (saddle: + B terms) and the per-block partials of M^cm x_m:
-//   y_m = beta_inv x_m + M^mc x_c  [- l_m; and y_l,m = -x_m]   (src/matfree_A.cpp:54-78)
-// xr: the colloid's rotation and stresslet entries of x (x + nm3 + 3), null without a colloid.
-__global__ __launch_bounds__(AB) void k_ar_apply(int64_t nm3, int64_t n, double binv, const double* __restrict__ mcm,
+// y = M x, or the saddle A x, in one launch (src/matfree_A.cpp:54-78):
+//   monomer rows  y_m = beta_inv x_m + M^mc x_c  [saddle: - l_m, and y_l,m = -x_m]
+//   colloid rows  y_c = M^cm x_m + M^cc x_c      [saddle: - l_c, and y_l,c = -x_c on the F, T rows]
+// Each block stores its partials of M^cm x_m; the last block to finish sums them (fixed order) and
+// writes the colloid rows.  xr: the colloid's rotation and stresslet entries of x (x + nm3 + 3),
+// null without a colloid (then no colloid rows).
+__global__ __launch_bounds__(AB) void k_ar_apply(int64_t nm3, int64_t n, int64_t nu, double binv,
+                                                 const double* __restrict__ mcm, const double* __restrict__ mcc,
                                                  const double* __restrict__ x, const double* __restrict__ xr,
-                                                 double* __restrict__ y, double* __restrict__ part, int saddle) {
+                                                 double* __restrict__ y, double* __restrict__ part,
+                                                 int* __restrict__ counter, int saddle) {
     const int64_t i = (int64_t)blockIdx.x * AB + threadIdx.x;
     double p[NR];
 #pragma unroll
@@ -250,20 +312,29 @@ __global__ __launch_bounds__(AB) void k_ar_apply(int64_t nm3, int64_t n, double 
         }
         y[i] = yi;
     }
+    if (!xr) return;
     block_reduce_store<NR>(p, part);
-}
-// Colloid rows: y_c = M^cm x_m + M^cc x_c (+ B terms).
-__global__ void k_ar_apply_fin(int64_t nm3, int64_t n, int64_t nu, const double* __restrict__ part, int nb,
-                               const double* __restrict__ mcc, const double* __restrict__ x, double* __restrict__ y,
-                               int saddle) {
+    if (!last_block(counter)) return;
+    __shared__ double rs[NR];
+    sum_rows(part, gridDim.x, rs);
     const int r = threadIdx.x;
     if (r >= 11) return;
-    double a1 = 0.0;
-    if (r >= 3)
-        for (int b = 0; b < nb; b++) a1 += part[(size_t)b * NR + (r - 3)];
     double a2 = 0.0;
     for (int q = 0; q < 11; q++) a2 += mcc[r * 11 + q] * x[nm3 + q];
-    double yr = a1 + a2;
+    double yr = (r >= 3 ? rs[r - 3] : 0.0) + a2;
+    if (saddle && nm3 + r < nu) {
+        yr -= x[n + nm3 + r];
+        y[n + nm3 + r] = -x[nm3 + r];
+    }
+    y[nm3 + r] = yr;
+}
+// The colloid rows alone (no monomers): y_c = M^cc x_c (+ B terms).
+__global__ void k_ar_apply_c(int64_t nm3, int64_t n, int64_t nu, const double* __restrict__ mcc,
+                             const double* __restrict__ x, double* __restrict__ y, int saddle) {
+    const int r = threadIdx.x;
+    if (r >= 11) return;
+    double yr = 0.0;
+    for (int q = 0; q < 11; q++) yr += mcc[r * 11 + q] * x[nm3 + q];
     if (saddle && nm3 + r < nu) {
         yr -= x[n + nm3 + r];
         y[n + nm3 + r] = -x[nm3 + r];
@@ -271,11 +342,41 @@ __global__ void k_ar_apply_fin(int64_t nm3, int64_t n, int64_t nu, const double*
     y[nm3 + r] = yr;
 }
 
+// The 5 x 5 stresslet system of the block elimination, M_ss f_s = b_s - y_s, by Cholesky (M_ss is
+// the stresslet block of the floored M^cc: SPD).
+__global__ void k_ar_mss_solve(const double* __restrict__ mss, const double* __restrict__ b,
+                               const double* __restrict__ y, int64_t nu, double* __restrict__ x) {
+    if (threadIdx.x != 0) return;
+    double Lm[5][5] = {}, r[5];
+    for (int k = 0; k < 5; k++) r[k] = b[nu + k] - y[nu + k];
+    for (int j = 0; j < 5; j++) {
+        double d = mss[j * 5 + j];
+        for (int k = 0; k < j; k++) d -= Lm[j][k] * Lm[j][k];
+        Lm[j][j] = sqrt(d);
+        for (int i = j + 1; i < 5; i++) {
+            double v = mss[i * 5 + j];
+            for (int k = 0; k < j; k++) v -= Lm[i][k] * Lm[j][k];
+            Lm[i][j] = v / Lm[j][j];
+        }
+    }
+    for (int i = 0; i < 5; i++) {   // L z = r
+        for (int k = 0; k < i; k++) r[i] -= Lm[i][k] * r[k];
+        r[i] /= Lm[i][i];
+    }
+    for (int i = 4; i >= 0; i--) {   // L^T f = z
+        for (int k = i + 1; k < 5; k++) r[i] -= Lm[k][i] * r[k];
+        r[i] /= Lm[i][i];
+    }
+    for (int k = 0; k < 5; k++) x[nu + k] = r[k];
+}
+
 // ------------------------------------------------------------------------------------ noise
-// build_slip_vel_schur: u_m = sqrt(beta_inv) xi_m, partials of M^cm xi_m.
-__global__ __launch_bounds__(AB) void k_ar_noise(int64_t nm3, double sq_binv, const double* __restrict__ mcm,
+// build_slip_vel_schur in one launch: u_m = sqrt(beta_inv) xi_m; the last block adds
+// u_c = sqrt(beta) M^cm xi_m + Q diag(sqrt(lam)) Q^T xi_c.
+__global__ __launch_bounds__(AB) void k_ar_noise(int64_t nm3, double sq_binv, double sq_beta,
+                                                 const double* __restrict__ mcm, const double* __restrict__ mcc_q_lam,
                                                  const double* __restrict__ xi, double* __restrict__ out,
-                                                 double* __restrict__ part) {
+                                                 double* __restrict__ part, int* __restrict__ counter, int colloid) {
     const int64_t i = (int64_t)blockIdx.x * AB + threadIdx.x;
     double p[NR];
 #pragma unroll
@@ -286,27 +387,23 @@ __global__ __launch_bounds__(AB) void k_ar_noise(int64_t nm3, double sq_binv, co
         for (int r = 0; r < NR; r++) p[r] = mcm[r * nm3 + i] * v;
         out[i] = sq_binv * v;
     }
+    if (!colloid) return;
     block_reduce_store<NR>(p, part);
-}
-// u_c = sqrt(beta) M^cm xi_m + Q diag(sqrt(lam)) Q^T xi_c.
-__global__ void k_ar_noise_fin(int64_t nm3, const double* __restrict__ part, int nb, double sq_beta,
-                               const double* __restrict__ mcc_q_lam, const double* __restrict__ xi,
-                               double* __restrict__ out) {
+    if (!last_block(counter)) return;
+    __shared__ double rs[NR];
     __shared__ double yv[11];
-    const int r = threadIdx.x;
+    sum_rows(part, gridDim.x, rs);
     const double* Q = mcc_q_lam + 121;
     const double* lam = mcc_q_lam + 242;
+    const int r = threadIdx.x;
     if (r < 11) {
         double yj = 0.0;
-        for (int i = 0; i < 11; i++) yj += Q[i * 11 + r] * xi[nm3 + i];
+        for (int k = 0; k < 11; k++) yj += Q[k * 11 + r] * xi[nm3 + k];
         yv[r] = sqrt(lam[r]) * yj;
     }
     __syncthreads();
     if (r >= 11) return;
-    double uc = 0.0;
-    if (r >= 3)
-        for (int b = 0; b < nb; b++) uc += part[(size_t)b * NR + (r - 3)];
-    uc *= sq_beta;
+    const double uc = (r >= 3 ? rs[r - 3] : 0.0) * sq_beta;
     double si = 0.0;
     for (int j = 0; j < 11; j++) si += Q[r * 11 + j] * yv[j];
     out[nm3 + r] = uc + si;
@@ -460,6 +557,15 @@ ps_err arrow_alloc(Ctx* c) {
     PS_HIP(c, hipMalloc(&c->ar_flag, sizeof(int) * (size_t)(c->Nm + 2)));
     PS_HIP(c, hipMalloc(&c->ar_pos, sizeof(int) * (size_t)(c->Nm + 2)));
     PS_HIP(c, hipHostMalloc(&c->ar_host, sizeof(double) * 8, hipHostMallocDefault));
+    PS_HIP(c, hipMalloc(&c->ar_counter, sizeof(int) * 2));
+    PS_HIP(c, hipMemset(c->ar_counter, 0, sizeof(int) * 2));
+    // Verlet partners: at most 2 Nm entries (each monomer with the colloid, both directions)
+    if (c->nb_cap < 2 * (int64_t)c->Nm + 2) {
+        if (c->nb_part) (void)hipFree(c->nb_part);
+        c->nb_part = nullptr;
+        c->nb_cap = 2 * (int64_t)c->Nm + 2;
+        PS_HIP(c, hipMalloc(&c->nb_part, sizeof(int) * (size_t)c->nb_cap));
+    }
     // M^cc starts as the colloid self block, Q as the identity (src/mob.cpp:629-632)
     double h[256] = {};
     for (int a = 0; a < 11; a++) {
@@ -479,54 +585,47 @@ ps_err arrow_alloc(Ctx* c) {
 
 void arrow_free(Ctx* c) {
     for (void* p : {(void*)c->ar_mcm, (void*)c->ar_part, (void*)c->ar_mcc, (void*)c->ar_flag, (void*)c->ar_pos,
+                    (void*)c->ar_counter,
                     (void*)c->wc_cell, (void*)c->wc_count, (void*)c->wc_start, (void*)c->wc_mem,
                     (void*)c->ar_bbox})
         if (p) (void)hipFree(p);
     if (c->ar_host) (void)hipHostFree(c->ar_host);
     c->ar_mcm = c->ar_part = c->ar_mcc = c->ar_bbox = c->ar_host = nullptr;
-    c->ar_flag = c->ar_pos = c->wc_cell = c->wc_count = c->wc_start = c->wc_mem = nullptr;
+    c->ar_flag = c->ar_pos = c->wc_cell = c->wc_count = c->wc_start = c->wc_mem = c->ar_counter = nullptr;
 }
 
+// The Verlet list with no host round trip: a monomer has at most one partner (the colloid), so the
+// partner array's capacity is fixed at allocation (2 Nm); the pair count is read only by
+// ps_check_dist.
 ps_err arrow_check_dist(Ctx* c) {
     hipStream_t st = c->stream;
     const int Nm = c->Nm, Np = c->Np;
-    PS_HIP(c, hipMemcpyAsync(c->xpd, c->x, sizeof(double) * 3 * (size_t)Np, hipMemcpyDeviceToDevice, st));
     const double rverl_ab = (double)(float)(1.5 + c->p.beta);
-    if (Nm > 0) {
-        k_ar_flag<<<dof_blocks(Nm), AB, 0, st>>>(c->sc, c->xpd, rverl_ab, c->ar_flag);
-        PS_LAUNCH(c);
-        PS_TRY(scan_ints(c, c->ar_flag, c->ar_pos, Nm));
-    } else {
-        PS_HIP(c, hipMemsetAsync(c->ar_pos, 0, sizeof(int) * 2, st));
-    }
-    k_ar_counts<<<dof_blocks(Np), AB, 0, st>>>(Nm, c->Nc, c->ar_flag, c->ar_pos, c->nb_count);
+    k_ar_flag<<<dof_blocks(Np), AB, 0, st>>>(c->sc, c->x, c->xpd, rverl_ab, c->ar_flag);
     PS_LAUNCH(c);
-    PS_TRY(scan_ints(c, c->nb_count, c->nb_off, Np));
-    PS_HIP(c, hipMemcpyAsync(c->hflag, c->nb_off + Np, sizeof(int), hipMemcpyDeviceToHost, st));
-    PS_HIP(c, hipStreamSynchronize(st));
-    const int64_t total = c->hflag[0];
-    if (total > c->nb_cap) {
-        PS_HIP(c, hipFree(c->nb_part));
-        c->nb_cap = total + total / 4 + 1024;
-        PS_HIP(c, hipMalloc(&c->nb_part, sizeof(int) * (size_t)c->nb_cap));
-    }
-    if (Nm > 0) {
+    if (Nm > 0) PS_TRY(scan_ints(c, c->ar_flag, c->ar_pos, Nm));
+    k_ar_offsets<<<dof_blocks(Np + 2), AB, 0, st>>>(Nm, c->Nc, c->ar_pos, c->nb_off);
+    PS_LAUNCH(c);
+    if (Nm > 0 && c->Nc == 1) {
         k_ar_fill<<<dof_blocks(Nm), AB, 0, st>>>(Nm, c->ar_flag, c->ar_pos, c->nb_off, c->nb_part);
         PS_LAUNCH(c);
     }
-    c->nvlist = total / 2;
-    c->cnt.nvlist = c->nvlist;
+    c->nvlist = -1;   // counted on demand (arrow_count_pairs)
     c->have_pd = true;
     return PS_OK;
 }
 
+ps_err arrow_count_pairs(Ctx* c) {
+    PS_HIP(c, hipMemcpyAsync(c->hflag, c->nb_off + c->Np, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    c->nvlist = c->hflag[0] / 2;
+    c->cnt.nvlist = c->nvlist;
+    return PS_OK;
+}
+
 ps_err arrow_mob(Ctx* c) {
-    hipStream_t st = c->stream;
-    PS_HIP(c, hipMemcpyAsync(c->xm, c->xpd, sizeof(double) * 3 * (size_t)c->Np, hipMemcpyDeviceToDevice, st));
-    if (c->Nm > 0 && c->Nc == 1) {
-        k_ar_mcm<<<dof_blocks(c->Nm), AB, 0, st>>>(c->sc, c->xm, c->p.beta * c->p.beta, c->ar_mcm);
-        PS_LAUNCH(c);
-    }
+    k_ar_mcm<<<dof_blocks(c->Np), AB, 0, c->stream>>>(c->sc, c->xpd, c->xm, c->p.beta * c->p.beta, c->ar_mcm);
+    PS_LAUNCH(c);
     c->have_mob = true;
     return PS_OK;
 }
@@ -535,7 +634,7 @@ ps_err arrow_schur(Ctx* c) {
     if (c->Nc != 1) return PS_OK;
     hipStream_t st = c->stream;
     const int64_t nm3 = c->sc.nm3;
-    const int nb = std::min(c->ar_nb, 1024);
+    const int nb = std::min(c->ar_nb, 64);   // grid-stride: few partials for the one-block finish
     if (nm3 > 0) {
         k_ar_gram<<<nb, AB, 0, st>>>(nm3, c->ar_mcm, c->ar_part);
         PS_LAUNCH(c);
@@ -551,32 +650,39 @@ ps_err arrow_apply(Ctx* c, const double* x, double* y, int saddle) {
     const int64_t nm3 = c->sc.nm3, n = c->n, nu = c->nu;
     const unsigned nb = dof_blocks(nm3);
     if (nm3 > 0) {
-        k_ar_apply<<<nb, AB, 0, st>>>(nm3, n, 1.0 / c->p.beta, c->ar_mcm, x, c->Nc == 1 ? x + nm3 + 3 : nullptr, y,
-                                      c->ar_part, saddle);
+        // timing (ps_profile_begin/end): the launch records its own start/stop events at dispatch, or
+        // is bracketed by event records under rocprofv3
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (c->prof_on) PS_TRY(c->prof_record(3, &e0, &e1));
+        const bool dispatch_ev = c->prof_on && c->prof_dispatch_events;
+        if (c->prof_on && !dispatch_ev) PS_HIP(c, hipEventRecord(e0, st));
+        hipExtLaunchKernelGGL(k_ar_apply, dim3(nb), dim3(AB), 0, st, dispatch_ev ? e0 : nullptr,
+                              dispatch_ev ? e1 : nullptr, 0, nm3, n, nu, 1.0 / c->p.beta, (const double*)c->ar_mcm,
+                              (const double*)c->ar_mcc, x, c->Nc == 1 ? x + nm3 + 3 : (const double*)nullptr, y,
+                              c->ar_part, c->ar_counter, saddle);
         PS_LAUNCH(c);
-    }
-    if (c->Nc == 1) {
-        k_ar_apply_fin<<<1, 64, 0, st>>>(nm3, n, nu, c->ar_part, nm3 > 0 ? (int)nb : 0, c->ar_mcc, x, y, saddle);
+        if (c->prof_on && !dispatch_ev) PS_HIP(c, hipEventRecord(e1, st));
+    } else if (c->Nc == 1) {
+        k_ar_apply_c<<<1, 64, 0, st>>>(nm3, n, nu, c->ar_mcc, x, y, saddle);
         PS_LAUNCH(c);
     }
     c->cnt.applies++;
     return PS_OK;
 }
 
+ps_err arrow_mss_solve(Ctx* c, const double* b, const double* y, double* x) {
+    k_ar_mss_solve<<<1, 64, 0, c->stream>>>(c->Mss, b, y, c->nu, x);
+    PS_LAUNCH(c);
+    return PS_OK;
+}
+
 ps_err arrow_noise(Ctx* c, const double* xi, double* out) {
     hipStream_t st = c->stream;
     const int64_t nm3 = c->sc.nm3;
-    const unsigned nb = dof_blocks(nm3);
     if (c->Nc == 1 && !c->ar_synced) PS_TRY(arrow_schur(c));
-    if (nm3 > 0) {
-        k_ar_noise<<<nb, AB, 0, st>>>(nm3, std::sqrt(1.0 / c->p.beta), c->ar_mcm, xi, out, c->ar_part);
-        PS_LAUNCH(c);
-    }
-    if (c->Nc == 1) {
-        k_ar_noise_fin<<<1, 64, 0, st>>>(nm3, c->ar_part, nm3 > 0 ? (int)nb : 0, std::sqrt(c->p.beta), c->ar_mcc, xi,
-                                         out);
-        PS_LAUNCH(c);
-    }
+    k_ar_noise<<<dof_blocks(nm3), AB, 0, st>>>(nm3, std::sqrt(1.0 / c->p.beta), std::sqrt(c->p.beta), c->ar_mcm,
+                                               c->ar_mcc, xi, out, c->ar_part, c->ar_counter + 1, c->Nc == 1);
+    PS_LAUNCH(c);
     c->cnt.sqrt_calls++;
     return PS_OK;
 }

@@ -343,9 +343,12 @@ ps_err Ctx::prof_harvest() {
         } else if (ev_kind[k] == 1) {
             prof.apply_launches++;
             prof.apply_ms += ms;
-        } else {
+        } else if (ev_kind[k] == 2) {
             prof.dense_launches++;
             prof.dense_ms += ms;
+        } else {
+            prof.arrow_launches++;
+            prof.arrow_ms += ms;
         }
     }
     ev_used = 0;
@@ -637,6 +640,7 @@ ps_err ps_check_dist(void* h, int64_t* n_pairs) {
     if (!c) return PS_ERR_ARG;
     if (!c->have_x) return fail(c, PS_ERR_STATE, "check_dist before initial_configuration");
     PS_TRY(check_dist_dev(c));
+    if (c->arrow) PS_TRY(arrow_count_pairs(c));
     PS_HIP(c, hipStreamSynchronize(c->stream));
     if (n_pairs) *n_pairs = c->nvlist;
     return PS_OK;

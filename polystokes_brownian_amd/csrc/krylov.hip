@@ -852,7 +852,11 @@ static ps_err block_solve_once(Ctx* c, const double* b, double* x, int prec, int
     hipStream_t st = c->stream;
     k_bs_init<<<eblocks(n), VB, 0, st>>>(x, b, n, nu);
     PS_LAUNCH(c);
-    if (n5 > 0) {
+    if (n5 > 0 && c->arrow) {
+        // the arrowhead's stresslet block is the 5 x 5 floored M^cc block: one apply, then Cholesky
+        PS_TRY(mob_apply_stresslet_rows(c, x, c->bs_y));
+        PS_TRY(arrow_mss_solve(c, b, c->bs_y, x));
+    } else if (n5 > 0) {
         // only the 5Nc stresslet rows of M [f_u; 0] are needed: colloid-row kernels, O(Nm Nc)
         PS_TRY(mob_apply_stresslet_rows(c, x, c->bs_y));
         k_bs_rs<<<eblocks(n5), VB, 0, st>>>(c->bs_r, b, c->bs_y, nu, n5);

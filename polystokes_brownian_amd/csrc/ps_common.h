@@ -146,7 +146,7 @@ struct Ctx {
     bool prof_dispatch_events = true;   // false under rocprofv3 (ROCPROF_OUTPUT_PATH set)
     std::vector<hipEvent_t> ev_pool;
     int ev_used = 0;
-    std::vector<int> ev_kind;       // 0: AA kernel, 1: whole apply, 2: dense apply (pairs of events)
+    std::vector<int> ev_kind;       // 0: AA kernel, 1: whole apply, 2: dense apply, 3: arrowhead apply pass
     ps_profile prof{};
     ps_err prof_record(int kind, hipEvent_t* a, hipEvent_t* b);   // reserve a start/stop pair
     ps_err prof_harvest();
@@ -172,6 +172,7 @@ struct Ctx {
     double* ar_host = nullptr;      // pinned
     int* ar_flag = nullptr;         // [Nm+2] monomer-colloid Verlet flags
     int* ar_pos = nullptr;          // [Nm+2] their exclusive scan
+    int* ar_counter = nullptr;      // [2] last-block counters (apply, noise), reset by their last block
     int ar_nb = 0;
     bool ar_synced = false;         // a Schur floor has run
     bool ar_diag_frozen = false;    // the Jacobi diagonal is fixed at the first MINRES solve
@@ -239,6 +240,8 @@ void dense_free(Ctx* c);
 ps_err arrow_alloc(Ctx* c);
 void arrow_free(Ctx* c);
 ps_err arrow_check_dist(Ctx* c);                                   // Verlet list, AB + BB pairs only
+ps_err arrow_count_pairs(Ctx* c);                                  // host copy of the pair count (syncs)
+ps_err arrow_mss_solve(Ctx* c, const double* b, const double* y, double* x);   // 5 x 5 stresslet system
 ps_err arrow_mob(Ctx* c);                                          // snapshot + M^cm
 ps_err arrow_schur(Ctx* c);                                        // sync_mcc_schur_correction
 ps_err arrow_apply(Ctx* c, const double* x, double* y, int saddle);
