@@ -264,6 +264,11 @@ ps_err ps_get_cell_list(void* ctx, int32_t* cell_of, int32_t* dims, double* geom
 /* Normal draws of the context's noise stream (std::mt19937_64 + std::normal_distribution<double>,
  * src/init.cpp:105-110) -- the draws the next step would consume; advances the stream. */
 ps_err ps_draw_normals(void* ctx, double* out, int64_t n);
+/* The same stream without a context (seeded with `seed`, drawn in fills of `chunk` values): the host
+ * generator the library uses, for checking it against std::normal_distribution.  `margin` > 0
+ * replaces the number of polar-method attempts a fill first takes per pair still missing (1.35 by
+ * default; acceptance is pi/4), so a small value makes each fill take several further ranges. */
+ps_err ps_normal_stream(uint64_t seed, double* out, int64_t n, int64_t chunk, double margin);
 
 #ifdef __cplusplus
 }

@@ -149,7 +149,7 @@ struct NoisePipe {
                 if (abort) return;
             }
             if (s >= 2) (void)hipEventSynchronize(done[k]);
-            for (int64_t i = 0; i < len; i++) slot[k][i] = c->nd(c->rng);
+            c->normals.fill(slot[k], len);
             {
                 std::lock_guard<std::mutex> lk(mu);
                 filled = s + 1;
@@ -547,8 +547,7 @@ ps_err ps_create(const ps_params* p, void** out) {
     if (hipMemcpy(c->stale, st0, sizeof(st0), hipMemcpyHostToDevice) != hipSuccess) return bail(PS_ERR_HIP);
     c->ktrap_cur = p->ktrap;
     c->t = p->restart;   // src/Stokes.cpp:21
-    c->rng.seed(p->seed + (uint64_t)std::llround(p->restart / p->dt));
-    c->nd = std::normal_distribution<double>(0.0, 1.0);
+    c->normals.seed(p->seed + (uint64_t)std::llround(p->restart / p->dt));
     if (apply_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
     if (krylov_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
     if (pairs_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
@@ -877,6 +876,16 @@ static bool write_table(const std::string& name, const double* v, int64_t rows, 
     return std::fclose(f) == 0;
 }
 
+ps_err ps_normal_stream(uint64_t seed, double* out, int64_t n, int64_t chunk, double margin) {
+    if (!out || n < 0 || !(margin >= 0.0)) return PS_ERR_ARG;
+    NormalStream g;
+    g.seed(seed);
+    if (margin > 0) g.set_margin(margin);
+    if (chunk <= 0) chunk = n;
+    for (int64_t i = 0; i < n; i += chunk) g.fill(out + i, std::min(chunk, n - i));
+    return PS_OK;
+}
+
 ps_err ps_format_time(double t, char* buf, int len) {
     if (!buf || len <= 0) return PS_ERR_ARG;
     std::ostringstream oss;
@@ -933,7 +942,7 @@ ps_err ps_run_output(void* h, int nsteps, const char* output_dir) {
 ps_err ps_draw_normals(void* h, double* out, int64_t n) {
     Ctx* c = as_ctx(h);
     if (!c || (!out && n > 0) || n < 0) return PS_ERR_ARG;
-    for (int64_t i = 0; i < n; i++) out[i] = c->nd(c->rng);
+    c->normals.fill(out, n);
     return PS_OK;
 }
 

@@ -1,5 +1,6 @@
 // ps_common.h -- context, error plumbing and device helpers shared by the gfx950 translation units.
 #pragma once
+#include "normals.h"
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
@@ -136,9 +137,8 @@ struct Ctx {
     int64_t nvlist = 0;
     int* dflag = nullptr;           // overflow flag etc. [4]
     int* hflag = nullptr;           // pinned
-    // ---- host RNG (std::mt19937_64 + normal_distribution, src/init.cpp:105-110)
-    std::mt19937_64 rng;
-    std::normal_distribution<double> nd{0.0, 1.0};
+    // ---- host RNG: the stream of std::mt19937_64 + normal_distribution (src/init.cpp:105-110)
+    NormalStream normals;
     double* noise_h = nullptr;      // pinned, 2 slots of n + nu (ps_run's noise producer)
     hipEvent_t noise_ev[2] = {nullptr, nullptr};
     // ---- event timing (ps_profile_begin/end)

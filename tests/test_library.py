@@ -5,6 +5,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from polystokes_brownian_amd import _abi
@@ -108,3 +109,23 @@ def test_frame_file_time_format():
         assert format_time(t) == want, t
         assert format_time(t) == "%g" % t
     assert frame_path("out", "config_", 0.001) == os.path.join("out", "config_0.001_.txt")
+
+
+@pytest.mark.parametrize("chunk", [0, 1, 7, 4096, 60001, 250000])
+def test_normal_stream_is_std_normal_distribution(chunk):
+    """The library's bulk noise generator reproduces std::mt19937_64 + std::normal_distribution bit for
+    bit (the oracle calls the standard library itself), across fill boundaries where the polar
+    method's cached second value carries over, and with the transform split over host threads."""
+    import ctypes as C
+    from oracle.pyoracle import normals
+    from polystokes_brownian_amd.polystokes import lib
+    L = lib()
+    L.ps_normal_stream.argtypes = [C.c_uint64, C.POINTER(C.c_double), C.c_int64, C.c_int64, C.c_double]
+    n = 600001
+    for seed in (12345, 2 ** 63 + 17):
+        want = normals(seed, n)
+        # margin 0.3: each fill first takes too few attempts and extends over several further ranges
+        for margin in (0.0, 0.3):
+            got = np.zeros(n)
+            assert L.ps_normal_stream(seed, got.ctypes.data_as(C.POINTER(C.c_double)), n, chunk, margin) == 0
+            np.testing.assert_array_equal(got, want)
