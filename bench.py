@@ -158,6 +158,13 @@ def arrow_bench(args, configs, Context, steps):
     x0 = configs.positions(w)
     Np, nm3 = w["Np"], 3 * w["Nm"]
     forms = {"fp64/block": dict(solver=1), "fp64/minres": dict(solver=0)}
+    if args.profile_form:
+        ctx = make_ctx(Context, pd, x0, **forms[args.profile_form])
+        dt, prof, per = timed_run(ctx, steps, args.warmup, "arrow " + args.profile_form)
+        ctx.close()
+        log(f"arrow {args.profile_form}: {Np * steps / dt:.4g} bead-steps/s, {1e3 * dt / steps:.3f} ms/step, "
+            f"per step {per}")
+        return None
     res = {}
     for form, over in forms.items():
         ctx = make_ctx(Context, pd, x0, **over)
@@ -258,7 +265,9 @@ def main():
     if args.dense:
         return dense_bench(args, configs, Context)
     if args.arrow:
-        print(json.dumps(arrow_bench(args, configs, Context, args.arrow_steps)), flush=True)
+        line = arrow_bench(args, configs, Context, args.arrow_steps)
+        if line is not None:
+            print(json.dumps(line), flush=True)
         return
 
     w = configs.workload(args.workload)
