@@ -204,6 +204,8 @@ typedef struct ps_profile {
     int64_t aa_pairs;                        /* target x source monomer pairs per AA launch */
     int64_t dense_launches; double dense_ms; /* stored-dense multi-vector apply kernel */
     int64_t arrow_launches; double arrow_ms; /* mm_HI=False: the DOF pass of the arrowhead apply */
+    int64_t arsolve_launches; double arsolve_ms; /* mm_HI=False: a block-eliminated saddle solve, first
+                                                    dispatch to last (its two DOF passes) */
 } ps_profile;
 ps_err ps_profile_begin(void* ctx);
 ps_err ps_profile_end(void* ctx, ps_profile* out);

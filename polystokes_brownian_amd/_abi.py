@@ -81,4 +81,5 @@ PS_DENSE_BF16, PS_DENSE_F16 = 1, 2
 class PsProfile(C.Structure):
     _fields_ = [("aa_launches", C.c_int64), ("aa_ms", C.c_double), ("apply_launches", C.c_int64),
                 ("apply_ms", C.c_double), ("aa_pairs", C.c_int64), ("dense_launches", C.c_int64),
-                ("dense_ms", C.c_double), ("arrow_launches", C.c_int64), ("arrow_ms", C.c_double)]
+                ("dense_ms", C.c_double), ("arrow_launches", C.c_int64), ("arrow_ms", C.c_double),
+                ("arsolve_launches", C.c_int64), ("arsolve_ms", C.c_double)]

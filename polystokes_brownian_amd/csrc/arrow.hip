@@ -33,63 +33,135 @@ unsigned dof_blocks(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + AB -
 
 // ------------------------------------------------------------------------------------ Verlet list
 // Monomer k pairs with the colloid when their separation is below the AB Verlet cutoff (a float,
-// compared after promotion, src/check_dist.cpp:49-62).  The same pass snapshots the positions.
-__global__ void k_ar_flag(SysConst s, const double* __restrict__ x, double* __restrict__ xpd, double rverl_ab,
-                          int* __restrict__ flag) {
-    const int k = blockIdx.x * AB + threadIdx.x;
-    if (k >= s.Np) return;
-    xpd[3 * k] = x[3 * k];
-    xpd[3 * k + 1] = x[3 * k + 1];
-    xpd[3 * k + 2] = x[3 * k + 2];
-    if (k >= s.Nm) return;
+// compared after promotion, src/check_dist.cpp:49-62).  Rows (CSR, both directions): a monomer's
+// single partner is the colloid; the colloid's row lists its monomers ascending; nb_off[k] = pos[k]
+// (the exclusive scan of the flags) up to the colloid, 2 * total after it, and nb_off[Np + 1] = the
+// longest row.  Two launches, no host round trip:
+//   pass 1: flag, the flag's exclusive scan within the block (ballot + popcount, then the 4 wave
+//           totals), the block's count; the last block to finish scans the block counts and writes
+//           the colloid rows' offsets.  The same pass snapshots the positions.
+//   pass 2: every monomer's offset = its block's offset + its scan within the block, and the rows.
+__device__ __forceinline__ bool last_block(int* __restrict__ counter);
+__global__ __launch_bounds__(AB) void k_ar_vl1(SysConst s, const double* __restrict__ x, double* __restrict__ xpd,
+                                               double rverl_ab, int* __restrict__ flag, int* __restrict__ lpos,
+                                               int* __restrict__ bscan, int* __restrict__ nb_off,
+                                               int* __restrict__ counter) {
+    __shared__ int wsum[AB / 64];
+    __shared__ int csum[AB / 64];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int k = blockIdx.x * AB + t;
+    if (k < s.Np) {
+        xpd[3 * k] = x[3 * k];
+        xpd[3 * k + 1] = x[3 * k + 1];
+        xpd[3 * k + 2] = x[3 * k + 2];
+    }
     int f = 0;
-    if (s.Nc == 1) {
+    if (k < s.Nm && s.Nc == 1) {
         double e[3], dr, dr_inv;
         pair_geom_dev(s, x, k, s.Nm, e, dr, dr_inv);
         f = dr < rverl_ab;
     }
-    flag[k] = f;
+    const unsigned long long m = __ballot(f);
+    const int below = __popcll(m & ((1ull << lane) - 1ull));   // flags of lower lanes
+    if (lane == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    int base = 0, cnt = 0;
+    for (int q = 0; q < AB / 64; q++) {
+        base += q < w ? wsum[q] : 0;
+        cnt += wsum[q];
+    }
+    if (k < s.Nm) {
+        flag[k] = f;
+        lpos[k] = base + below;
+    }
+    if (t == 0) bscan[blockIdx.x] = cnt;
+    if (!last_block(counter)) return;
+    // exclusive scan of the nb block counts: each thread sums a contiguous chunk, the chunk sums are
+    // scanned by shuffles within a wave and across the waves
+    const int nb = gridDim.x, per = (nb + AB - 1) / AB;
+    const int b0 = min(t * per, nb), b1 = min(b0 + per, nb);
+    int cs = 0;
+    for (int b = b0; b < b1; b++) cs += bscan[b];
+    int inc = cs;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) csum[w] = inc;
+    __syncthreads();
+    int cbase = 0, total = 0;
+    for (int q = 0; q < AB / 64; q++) {
+        cbase += q < w ? csum[q] : 0;
+        total += csum[q];
+    }
+    int run = cbase + inc - cs;
+    for (int b = b0; b < b1; b++) {
+        const int v = bscan[b];
+        bscan[nb + b] = run;
+        run += v;
+    }
+    if (t == 0) {
+        nb_off[s.Nm] = total;
+        for (int q = s.Nm + 1; q <= s.Np; q++) nb_off[q] = 2 * total;
+        nb_off[s.Np + 1] = s.Nc == 1 ? (total > 0 ? total : 1) : 0;
+    }
 }
-// Row offsets: a monomer row holds its flag (the colloid as partner), the colloid row all flagged
-// monomers, so nb_off[k] = pos[k] up to the colloid and 2 * total after it; nb_off[Np + 1] is the
-// longest row.
-__global__ void k_ar_offsets(int Nm, int Nc, const int* __restrict__ pos, int* __restrict__ nb_off) {
+__global__ __launch_bounds__(AB) void k_ar_vl2(int Nm, int nb1, const int* __restrict__ flag,
+                                               const int* __restrict__ lpos, const int* __restrict__ bscan,
+                                               int* __restrict__ nb_off, int* __restrict__ part) {
     const int k = blockIdx.x * AB + threadIdx.x;
-    const int Np = Nm + Nc;
-    if (k > Np + 1) return;
-    const int total = Nm > 0 ? pos[Nm] : 0;
-    if (k <= Nm) nb_off[k] = Nm > 0 ? pos[k] : 0;
-    else if (k <= Np) nb_off[k] = 2 * total;
-    else nb_off[k] = Nc == 1 ? (total > 0 ? total : 1) : 0;
-}
-// Rows: a monomer's single partner is the colloid; the colloid's row lists its monomers ascending.
-__global__ void k_ar_fill(int Nm, const int* __restrict__ flag, const int* __restrict__ pos,
-                          const int* __restrict__ nb_off, int* __restrict__ part) {
-    const int k = blockIdx.x * AB + threadIdx.x;
-    if (k >= Nm || !flag[k]) return;
-    part[nb_off[k]] = Nm;
-    part[nb_off[Nm] + pos[k]] = k;
+    if (k >= Nm) return;
+    const int pos = bscan[nb1 + blockIdx.x] + lpos[k];   // pass 1 had nb1 blocks of the same size
+    nb_off[k] = pos;
+    if (flag[k]) {
+        part[pos] = Nm;                 // the monomer's row: the colloid
+        part[nb_off[Nm] + pos] = k;     // the colloid's row, ascending
+    }
 }
 
 // ------------------------------------------------------------------------------------ M^cm
 // Column block of monomer k: rotation rows take mobility_AB's b, stresslet rows its flattened gt
-// (placement of src/mob.cpp:1013-1020).  The same pass snapshots the positions of this mob().
-__global__ void k_ar_mcm(SysConst s, const double* __restrict__ xpd, double* __restrict__ xm, double beta2,
-                         double* __restrict__ mcm) {
+// (placement of src/mob.cpp:1013-1020).  The same pass snapshots the positions of this mob().  With
+// `gram`, the block also stores its partials of the Gram block M8 M8^T (upper triangle, NSYM per
+// block) for the Schur floor that follows.
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* __restrict__ part);
+__global__ __launch_bounds__(AB) void k_ar_mcm(SysConst s, const double* __restrict__ xpd, double* __restrict__ xm,
+                                               double beta2, double* __restrict__ mcm, double* __restrict__ gram) {
     const int k = blockIdx.x * AB + threadIdx.x;
-    if (k >= s.Np) return;
-    xm[3 * k] = xpd[3 * k];
-    xm[3 * k + 1] = xpd[3 * k + 1];
-    xm[3 * k + 2] = xpd[3 * k + 2];
-    if (k >= s.Nm || s.Nc != 1) return;
-    const int64_t nm3 = s.nm3;
-    double e[3], dr, dr_inv, b[3][3], bt[3][3], gt[3][5];
-    pair_geom_dev(s, xpd, k, s.Nm, e, dr, dr_inv);
-    tile_ab_dev(dr_inv, e, beta2, b, bt, gt);
-    for (int ii = 0; ii < 3; ii++) {
-        for (int jj = 0; jj < 3; jj++) mcm[ii * nm3 + 3 * k + jj] = b[ii][jj];
-        for (int jj = 0; jj < 5; jj++) mcm[(3 + jj) * nm3 + 3 * k + ii] = gt[ii][jj];
+    double acc[NSYM];
+#pragma unroll
+    for (int q = 0; q < NSYM; q++) acc[q] = 0.0;
+    if (k < s.Np) {
+        xm[3 * k] = xpd[3 * k];
+        xm[3 * k + 1] = xpd[3 * k + 1];
+        xm[3 * k + 2] = xpd[3 * k + 2];
     }
+    if (k < s.Nm && s.Nc == 1) {
+        const int64_t nm3 = s.nm3;
+        double e[3], dr, dr_inv, b[3][3], bt[3][3], gt[3][5];
+        pair_geom_dev(s, xpd, k, s.Nm, e, dr, dr_inv);
+        tile_ab_dev(dr_inv, e, beta2, b, bt, gt);
+        for (int ii = 0; ii < 3; ii++) {
+            for (int jj = 0; jj < 3; jj++) mcm[ii * nm3 + 3 * k + jj] = b[ii][jj];
+            for (int jj = 0; jj < 5; jj++) mcm[(3 + jj) * nm3 + 3 * k + ii] = gt[ii][jj];
+        }
+        if (gram) {
+            // DOF 3k + d holds row r's value b[r][d] (r < 3) or gt[d][r - 3] (r >= 3)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                double v[NR];
+#pragma unroll
+                for (int r = 0; r < NR; r++) v[r] = r < 3 ? b[r][d] : gt[d][r - 3];
+                int q = 0;
+#pragma unroll
+                for (int a = 0; a < NR; a++)
+#pragma unroll
+                    for (int c = a; c < NR; c++) acc[q++] += v[a] * v[c];
+            }
+        }
+    }
+    if (gram) block_reduce_store<NSYM>(acc, gram);   // uniform across the block
 }
 
 // Block reduction of NV doubles per thread into part[blockIdx.x * NV ...] (fixed order).
@@ -115,22 +187,23 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* __re
 // resets the counter for the next launch).  Partials must be stored before the call.
 __device__ __forceinline__ bool last_block(int* __restrict__ counter) {
     __shared__ int is_last;
-    __threadfence();
+    __threadfence();   // release: this block's partials are visible device-wide before it counts
     __syncthreads();
     if (threadIdx.x == 0) {
         is_last = atomicAdd(counter, 1) == (int)gridDim.x - 1;
         if (is_last) *counter = 0;
     }
     __syncthreads();
+    if (is_last) __threadfence();   // acquire: later loads see every block's partials
     return is_last;
 }
 // In the last block: the NR row sums of all blocks' partials, 32 lanes per row, combined in a fixed
-// order (deterministic), into rs[NR] (LDS).  Partials are read past the per-CU cache.
-__device__ __forceinline__ void sum_rows(const double* __restrict__ part, int nb, double* rs) {
+// order (deterministic), into rs[NR] (LDS).  Ordinary loads after last_block's acquire fence.
+__device__ __forceinline__ void sum_rows(const double* part, int nb, double* rs) {
     const int r = threadIdx.x >> 5, l = threadIdx.x & 31;
     double a = 0.0;
     if (r < NR)
-        for (int b = l; b < nb; b += 32) a += ((const volatile double*)part)[(size_t)b * NR + r];
+        for (int b = l; b < nb; b += 32) a += part[(size_t)b * NR + r];
     for (int off = 16; off > 0; off >>= 1) a += __shfl_down(a, off, 32);
     if (r < NR && l == 0) rs[r] = a;
     __syncthreads();
@@ -160,31 +233,94 @@ __device__ __forceinline__ double mcc_base(int a, int b) {
     return a < 3 ? 1.0 : a < 6 ? 3. / 4. : 9. / 5.;
 }
 
-// sync_mcc_schur_correction (src/slip_vel.cpp:81-203) in one workgroup: S = M^cc_base - beta P;
-// eigen-decomposition by a parallel cyclic Jacobi method (Rutishauser's rotation formulas; the
-// round-robin ordering rotates 6 disjoint index pairs of the 12-padded matrix at once); eigenvalues
-// below 0.1 raised to 0.1; M^cc = base + Q diag(delta) Q^T.  Writes M^cc, Q (row-major, eigenvectors
-// as columns), the floored spectrum, and the 5 x 5 stresslet block of M^cc for the block solver.
+// Cholesky factor of the 5 x 5 stresslet block m (SPD: the stresslet block of the floored M^cc):
+// L row-major, lower triangle, with the reciprocals of its diagonal on the diagonal.
+__host__ __device__ inline void mss_factor(const double* m, double* L) {
+    for (int q = 0; q < 25; q++) L[q] = 0.0;
+    for (int j = 0; j < 5; j++) {
+        double d = m[j * 5 + j];
+        for (int k = 0; k < j; k++) d -= L[j * 5 + k] * L[j * 5 + k];
+        const double ljj = sqrt(d);
+        L[j * 5 + j] = 1.0 / ljj;
+        for (int i = j + 1; i < 5; i++) {
+            double v = m[i * 5 + j];
+            for (int k = 0; k < j; k++) v -= L[i * 5 + k] * L[j * 5 + k];
+            L[i * 5 + j] = v / ljj;
+        }
+    }
+}
+// Solves L L^T f = r in place with a factor from mss_factor.
+__device__ __forceinline__ void mss_factor_solve(const double* L, double (&r)[5]) {
+    for (int i = 0; i < 5; i++) {
+        for (int k = 0; k < i; k++) r[i] -= L[i * 5 + k] * r[k];
+        r[i] *= L[i * 5 + i];
+    }
+    for (int i = 4; i >= 0; i--) {
+        for (int k = i + 1; k < 5; k++) r[i] -= L[k * 5 + i] * r[k];
+        r[i] *= L[i * 5 + i];
+    }
+}
+
+// sync_mcc_schur_correction (src/slip_vel.cpp:81-203) in one wave: S = M^cc_base - beta P, its
+// eigen-decomposition, eigenvalues below 0.1 raised to 0.1, and M^cc = base + Q diag(delta) Q^T.
+// Writes M^cc, Q (row-major, eigenvectors as columns), the floored spectrum, and the 5 x 5 stresslet
+// block of M^cc for the block solver.
+//
+// Eigensolver: the parallel cyclic Jacobi method on the 12-padded matrix, warm-started from the
+// previous sync's eigenvectors Q0 (S moves little between syncs, so Q0^T S Q0 is nearly diagonal and
+// a few sweeps converge it); Q0 is first re-orthonormalised by one Newton-Schulz step,
+// Q0 <- (3/2) Q0 - (1/2) Q0 Q0^T Q0, so rounding does not accumulate over syncs.  Each round of the
+// round-robin ordering rotates 6 disjoint index pairs at once (Rutishauser's angle formulas) as one
+// two-sided update A' = J^T A J: each of 36 lanes updates the 2 x 2 block of A on one pair's rows and
+// another's columns (B' = J1^T B J2); the rotated pairs' off-diagonal entries are set to zero and
+// their diagonal entries become a_pp - t a_pq and a_qq + t a_pq.  Sweeps stop when the off-diagonal
+// mass is below 1e-30 of the diagonal's.  The floored matrix and the square root Q diag(sqrt(lam)) Q^T do not depend on the
+// eigenvector choice, so the warm start changes them only by rounding.
 constexpr int JN = 12;
-__global__ __launch_bounds__(128) void k_ar_schur(const double* __restrict__ part, int nb, double beta,
-                                                  double* __restrict__ out, double* __restrict__ mss) {
-    __shared__ double A[JN][JN + 1], V[JN][JN + 1];
-    __shared__ double P[NSYM];
-    __shared__ double cs[JN / 2], sn[JN / 2];
-    __shared__ int pp[JN / 2], qq[JN / 2];
-    __shared__ int done;
-    __shared__ double Pp[3][NSYM];
-    const int t = threadIdx.x;
-    if (t < 3 * NSYM) {   // three lanes per entry, independent loads, combined in a fixed order
-        const int q = t % NSYM, g = t / NSYM;
-        double a = 0.0;
-        for (int b = g; b < nb; b += 3) a += part[(size_t)b * NSYM + q];
-        Pp[g][q] = a;
+// Round-robin ordering (circle method) of round r: seat 0 holds index 0, seat j >= 1 index
+// 1 + (j - 1 + r) mod 11; pair k joins seats k and 11 - k.
+__device__ __forceinline__ int jseat(int j, int r) { return j == 0 ? 0 : 1 + (j - 1 + r) % (JN - 1); }
+// the indices p < q of pair k in round r
+__device__ __forceinline__ void jpair_of(int k, int r, int& p, int& q) {
+    const int a = jseat(k, r), b = jseat(JN - 1 - k, r);
+    p = min(a, b);
+    q = max(a, b);
+}
+// C = X^T Y (tx) or X Y (!tx) for 12 x 12 LDS matrices; each lane computes entries t, t + 64, t + 128.
+template <bool TX>
+__device__ __forceinline__ void jgemm(const double (*X)[JN + 1], const double (*Y)[JN + 1], double (*C)[JN + 1]) {
+    double r[3];
+#pragma unroll
+    for (int h = 0; h < 3; h++) {
+        const int e = threadIdx.x + 64 * h;
+        r[h] = 0.0;
+        if (e < JN * JN) {
+            const int a = e / JN, b = e % JN;
+#pragma unroll
+            for (int k = 0; k < JN; k++) r[h] += (TX ? X[k][a] : X[a][k]) * Y[k][b];
+        }
     }
     __syncthreads();
-    if (t < NSYM) P[t] = (Pp[0][t] + Pp[1][t]) + Pp[2][t];
+#pragma unroll
+    for (int h = 0; h < 3; h++) {
+        const int e = threadIdx.x + 64 * h;
+        if (e < JN * JN) C[e / JN][e % JN] = r[h];
+    }
     __syncthreads();
-    for (int q = t; q < JN * JN; q += blockDim.x) {
+}
+__global__ __launch_bounds__(64) void k_ar_schur(const double* __restrict__ part, int nb, double beta,
+                                                 double* __restrict__ out, double* __restrict__ mss) {
+    __shared__ double A[JN][JN + 1], V[JN][JN + 1], Q0[JN][JN + 1], W[JN][JN + 1];
+    __shared__ double P[NSYM];
+    const int t = threadIdx.x;
+    if (t < NSYM) {
+        double a = 0.0;
+#pragma unroll 8
+        for (int b = 0; b < nb; b++) a += part[(size_t)b * NSYM + t];
+        P[t] = a;
+    }
+    __syncthreads();
+    for (int q = t; q < JN * JN; q += 64) {
         const int a = q / JN, b = q % JN;
         double v = 0.0;
         if (a < 11 && b < 11) {
@@ -194,71 +330,116 @@ __global__ __launch_bounds__(128) void k_ar_schur(const double* __restrict__ par
                 v -= beta * P[i * NR - i * (i - 1) / 2 + (j - i)];
             }
         }
-        A[a][b] = v;
-        V[a][b] = (a == b) ? 1.0 : 0.0;
+        A[a][b] = v;   // S
+        Q0[a][b] = (a < 11 && b < 11) ? out[121 + a * 11 + b] : (a == b ? 1.0 : 0.0);
     }
     __syncthreads();
+    // Newton-Schulz: Q0 <- 1.5 Q0 - 0.5 Q0 (Q0^T Q0)
+    jgemm<true>(Q0, Q0, W);    // W = Q0^T Q0
+    jgemm<false>(Q0, W, V);    // V = Q0 W
+    for (int q = t; q < JN * JN; q += 64) Q0[q / JN][q % JN] = 1.5 * Q0[q / JN][q % JN] - 0.5 * V[q / JN][q % JN];
+    __syncthreads();
+    jgemm<false>(A, Q0, W);    // W = S Q0
+    jgemm<true>(Q0, W, V);     // V = Q0^T S Q0
+    for (int q = t; q < JN * JN; q += 64) {
+        const int a = q / JN, b = q % JN;
+        A[a][b] = 0.5 * (V[a][b] + V[b][a]);
+    }
+    __syncthreads();
+    for (int q = t; q < JN * JN; q += 64) V[q / JN][q % JN] = (q / JN == q % JN) ? 1.0 : 0.0;
+    __syncthreads();
     for (int sweep = 0; sweep < 60; sweep++) {
-        if (t < 64) {   // off-diagonal and diagonal mass by one wave: lane t < 55 takes pair t
-            double off = 0.0, dia = 0.0;
-            if (t < 55) {
-                int a = 0, rem = t;
-                while (rem >= 10 - a) {
-                    rem -= 10 - a;
-                    a++;
-                }
-                const double v = A[a][a + 1 + rem];
-                off = v * v;
+        // off-diagonal and diagonal mass: lane t takes upper entries t and t + 64 (66 in all)
+        double off = 0.0, dia = 0.0;
+        for (int u = t; u < JN * (JN - 1) / 2; u += 64) {
+            int a = 0, rem = u;
+            while (rem >= JN - 1 - a) {
+                rem -= JN - 1 - a;
+                a++;
             }
-            if (t < 11) dia = A[t][t] * A[t][t];
-            for (int o = 32; o > 0; o >>= 1) {
-                off += __shfl_down(off, o, 64);
-                dia += __shfl_down(dia, o, 64);
-            }
-            if (t == 0) done = (off <= 1e-36 * dia || off == 0.0);
+            const double v = A[a][a + 1 + rem];
+            off += v * v;
         }
-        __syncthreads();
-        if (done) break;
+        if (t < JN) dia = A[t][t] * A[t][t];
+        for (int o = 32; o > 0; o >>= 1) {
+            off += __shfl_xor(off, o, 64);
+            dia += __shfl_xor(dia, o, 64);
+        }
+        if (off <= 1e-30 * dia || off == 0.0) break;   // the same value in every lane
         for (int round = 0; round < JN - 1; round++) {
+            // Lane t < 36 owns the 2 x 2 block of A on rows of pair t / 6 and columns of pair t % 6
+            // (B' = J1^T B J2); V's rows are updated pair by pair (72 tasks: row a, pair k);
+            // lanes 0..5 compute the 6 rotations, which the others take by shuffle.
+            const int k1 = min(t / 6, JN / 2 - 1), k2 = t % 6;
+            int p1, q1, p2, q2;
+            jpair_of(k1, round, p1, q1);
+            jpair_of(k2, round, p2, q2);
+            double b00 = 0.0, b01 = 0.0, b10 = 0.0, b11 = 0.0;
+            if (t < 36) {
+                b00 = A[p1][p2];
+                b01 = A[p1][q2];
+                b10 = A[q1][p2];
+                b11 = A[q1][q2];
+            }
+            int va[2], vk[2], vp[2], vq[2];
+            double v0[2], v1[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int task = min(t + 64 * h, 71);
+                va[h] = task / 6;
+                vk[h] = task % 6;
+                jpair_of(vk[h], round, vp[h], vq[h]);
+                v0[h] = V[va[h]][vp[h]];
+                v1[h] = V[va[h]][vq[h]];
+            }
+            double c = 1.0, sv = 0.0, tt = 0.0;
             if (t < JN / 2) {
-                // circle method: seat 0 fixed, seats 1..11 rotate by `round`
-                auto seat = [round](int j) { return j == 0 ? 0 : 1 + (j - 1 + round) % (JN - 1); };
-                int p = seat(t), q = seat(JN - 1 - t);
-                if (p > q) { const int w = p; p = q; q = w; }
+                int p, q;
+                jpair_of(t, round, p, q);
                 const double apq = A[p][q];
-                double c = 1.0, s = 0.0;
                 if (apq != 0.0) {
                     const double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
-                    const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                    tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
                     c = 1.0 / sqrt(tt * tt + 1.0);
-                    s = tt * c;
+                    sv = tt * c;
                 }
-                cs[t] = c;
-                sn[t] = s;
-                pp[t] = p;
-                qq[t] = q;
             }
-            __syncthreads();
-            if (t < (JN / 2) * JN) {   // columns p, q of A and V (A J, V J); disjoint across pairs
-                const int k = t / JN, j = t % JN, p = pp[k], q = qq[k];
-                const double c = cs[k], s = sn[k];
-                const double ajp = A[j][p], ajq = A[j][q], vjp = V[j][p], vjq = V[j][q];
-                A[j][p] = c * ajp - s * ajq;
-                A[j][q] = s * ajp + c * ajq;
-                V[j][p] = c * vjp - s * vjq;
-                V[j][q] = s * vjp + c * vjq;
+            const double c1 = __shfl(c, k1, 64), s1 = __shfl(sv, k1, 64), t1 = __shfl(tt, k1, 64);
+            const double c2 = __shfl(c, k2, 64), s2 = __shfl(sv, k2, 64);
+            double cvv[2], svv[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                cvv[h] = __shfl(c, vk[h], 64);
+                svv[h] = __shfl(sv, vk[h], 64);
             }
-            __syncthreads();
-            if (t < (JN / 2) * JN) {   // rows p, q of A (J^T A)
-                const int k = t / JN, j = t % JN, p = pp[k], q = qq[k];
-                const double c = cs[k], s = sn[k];
-                const double apj = A[p][j], aqj = A[q][j];
-                A[p][j] = c * apj - s * aqj;
-                A[q][j] = s * apj + c * aqj;
+            __syncthreads();   // every old value read
+            if (t < 36) {
+                if (k1 == k2) {
+                    // the rotated pair: b00 = a_pp, b01 = a_pq, b11 = a_qq
+                    A[p1][p1] = b00 - t1 * b01;
+                    A[q1][q1] = b11 + t1 * b01;
+                    A[p1][q1] = 0.0;
+                    A[q1][p1] = 0.0;
+                } else {
+                    // M = B J2 (columns p2, q2), then J1^T M (rows p1, q1)
+                    const double m00 = b00 * c2 - b01 * s2, m01 = b00 * s2 + b01 * c2;
+                    const double m10 = b10 * c2 - b11 * s2, m11 = b10 * s2 + b11 * c2;
+                    A[p1][p2] = c1 * m00 - s1 * m10;
+                    A[p1][q2] = c1 * m01 - s1 * m11;
+                    A[q1][p2] = s1 * m00 + c1 * m10;
+                    A[q1][q2] = s1 * m01 + c1 * m11;
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                if (t + 64 * h >= 72) break;
+                V[va[h]][vp[h]] = cvv[h] * v0[h] - svv[h] * v1[h];
+                V[va[h]][vq[h]] = svv[h] * v0[h] + cvv[h] * v1[h];
             }
             __syncthreads();
         }
     }
+    jgemm<false>(Q0, V, W);    // Q = Q0 V
     __shared__ double delta[11];
     if (t < 11) {
         const double lam = A[t][t];
@@ -266,15 +447,25 @@ __global__ __launch_bounds__(128) void k_ar_schur(const double* __restrict__ par
         out[242 + t] = lam + delta[t];
     }
     __syncthreads();
-    for (int q = t; q < 121; q += blockDim.x) {
+    __shared__ double m5[25];
+    for (int q = t; q < 121; q += 64) {
         const int a = q / 11, b = q % 11;
         double corr = 0.0;
         for (int j = 0; j < 11; j++)
-            if (delta[j] != 0.0) corr += V[a][j] * delta[j] * V[b][j];
+            if (delta[j] != 0.0) corr += W[a][j] * delta[j] * W[b][j];
         const double m = mcc_base(a, b) + corr;
         out[q] = m;
-        out[121 + q] = V[a][b];
-        if (mss && a >= 6 && b >= 6) mss[(a - 6) * 5 + (b - 6)] = m;
+        out[121 + q] = W[a][b];
+        if (a >= 6 && b >= 6) {
+            m5[(a - 6) * 5 + (b - 6)] = m;
+            if (mss) mss[(a - 6) * 5 + (b - 6)] = m;
+        }
+    }
+    __syncthreads();
+    if (t == 0) {   // the block solver's factor of the stresslet block, at out + 256
+        double L[25];
+        mss_factor(m5, L);
+        for (int q = 0; q < 25; q++) out[256 + q] = L[q];
     }
 }
 
@@ -342,13 +533,10 @@ __global__ void k_ar_apply_c(int64_t nm3, int64_t n, int64_t nu, const double* _
     y[nm3 + r] = yr;
 }
 
-// The 5 x 5 stresslet system of the block elimination, M_ss f_s = b_s - y_s, by Cholesky (M_ss is
-// the stresslet block of the floored M^cc: SPD).
-__global__ void k_ar_mss_solve(const double* __restrict__ mss, const double* __restrict__ b,
-                               const double* __restrict__ y, int64_t nu, double* __restrict__ x) {
-    if (threadIdx.x != 0) return;
-    double Lm[5][5] = {}, r[5];
-    for (int k = 0; k < 5; k++) r[k] = b[nu + k] - y[nu + k];
+// Solves M_ss f = r in place for the 5 x 5 stresslet block (SPD: the stresslet block of the floored
+// M^cc) by Cholesky, in one thread.
+__device__ __forceinline__ void mss_cholesky_solve(const double* __restrict__ mss, double (&r)[5]) {
+    double Lm[5][5] = {};
     for (int j = 0; j < 5; j++) {
         double d = mss[j * 5 + j];
         for (int k = 0; k < j; k++) d -= Lm[j][k] * Lm[j][k];
@@ -367,7 +555,88 @@ __global__ void k_ar_mss_solve(const double* __restrict__ mss, const double* __r
         for (int k = i + 1; k < 5; k++) r[i] -= Lm[k][i] * r[k];
         r[i] /= Lm[i][i];
     }
+}
+
+// The 5 x 5 stresslet system of the block elimination, M_ss f_s = b_s - y_s.
+__global__ void k_ar_mss_solve(const double* __restrict__ mss, const double* __restrict__ b,
+                               const double* __restrict__ y, int64_t nu, double* __restrict__ x) {
+    if (threadIdx.x != 0) return;
+    double r[5];
+    for (int k = 0; k < 5; k++) r[k] = b[nu + k] - y[nu + k];
+    mss_cholesky_solve(mss, r);
     for (int k = 0; k < 5; k++) x[nu + k] = r[k];
+}
+
+// ------------------------------------------------------------------------------------ block solve
+// The saddle system [[M, B], [B^T, 0]] [f; u] = [b_f; b_l] with B = [-I_nu; 0] by exact block
+// elimination (f_u = -b_l; M_ss f_s = b_s - (M [f_u; 0])_s; u = (M f)_u - b_u) for one colloid,
+// in two DOF passes.  The only sum over monomers is R = M^cm f_m (8 rows), which both the stresslet
+// system and the colloid's velocity rows need, so:
+//   pass 1: f_m = -b_l,m and the partials of R; the last block to finish sums them (fixed order),
+//           sets the colloid's F, T from b_l, solves M_ss f_s = b_s - R_s - M^cc_{s,FT} f_FT by
+//           Cholesky and writes the colloid's velocity rows u_c = R + M^cc f_c - b_u,c;
+//   pass 2: the monomer velocity rows u_m = beta_inv f_m + M^mc f_c - b_u,m.
+// `uout` (optional) receives a copy of u.  Same arithmetic as two applies of k_ar_apply around the
+// 5 x 5 solve, with the reduction done once.
+__global__ __launch_bounds__(AB) void k_ar_bs1(int64_t nm3, int64_t n, const double* __restrict__ mcm,
+                                               const double* __restrict__ mcc, const double* __restrict__ b,
+                                               double* __restrict__ x,
+                                               double* __restrict__ uout, double* __restrict__ part,
+                                               int* __restrict__ counter) {
+    const int64_t i = (int64_t)blockIdx.x * AB + threadIdx.x;
+    double p[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) p[r] = 0.0;
+    if (i < nm3) {
+        const double f = -b[n + i];
+        x[i] = f;
+#pragma unroll
+        for (int r = 0; r < NR; r++) p[r] = mcm[r * nm3 + i] * f;
+    }
+    block_reduce_store<NR>(p, part);
+    if (!last_block(counter)) return;
+    // the last block: stage M^cc, the factor and the colloid's entries of b in LDS, sum the partials
+    __shared__ double rs[NR];
+    __shared__ double fc[11], bc[11], mc[121], lf[25];
+    const int t = threadIdx.x;
+    if (t < 121) mc[t] = mcc[t];
+    else if (t < 146) lf[t - 121] = mcc[256 + (t - 121)];
+    else if (t < 157) bc[t - 146] = b[nm3 + (t - 146)];
+    else if (t < 163) fc[t - 157] = -b[n + nm3 + (t - 157)];
+    sum_rows(part, gridDim.x, rs);   // ends with a barrier
+    if (t == 0) {
+        double r[5];
+        for (int k = 0; k < 5; k++) {
+            double a2 = 0.0;
+            for (int q = 0; q < 6; q++) a2 += mc[(6 + k) * 11 + q] * fc[q];
+            r[k] = bc[6 + k] - (rs[3 + k] + a2);
+        }
+        mss_factor_solve(lf, r);
+        for (int k = 0; k < 5; k++) fc[6 + k] = r[k];
+    }
+    __syncthreads();
+    if (t < 11) x[nm3 + t] = fc[t];
+    if (t < 6) {
+        double a2 = 0.0;
+        for (int q = 0; q < 11; q++) a2 += mc[t * 11 + q] * fc[q];
+        const double u = ((t >= 3 ? rs[t - 3] : 0.0) + a2) - bc[t];
+        x[n + nm3 + t] = u;
+        if (uout) uout[nm3 + t] = u;
+    }
+}
+__global__ __launch_bounds__(AB) void k_ar_bs2(int64_t nm3, int64_t n, double binv, const double* __restrict__ mcm,
+                                               const double* __restrict__ b, double* __restrict__ x,
+                                               double* __restrict__ uout) {
+    const int64_t i = (int64_t)blockIdx.x * AB + threadIdx.x;
+    if (i >= nm3) return;
+    const double* xr = x + nm3 + 3;   // the colloid's torque and stresslet entries of f
+    const double f = x[i];
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < NR; r++) acc += mcm[r * nm3 + i] * xr[r];
+    const double u = (acc + binv * f) - b[i];
+    x[n + i] = u;
+    if (uout) uout[i] = u;
 }
 
 // ------------------------------------------------------------------------------------ noise
@@ -553,12 +822,14 @@ ps_err arrow_alloc(Ctx* c) {
     PS_HIP(c, hipMalloc(&c->ar_mcm, sizeof(double) * (size_t)std::max<int64_t>(NR * nm3, 1)));
     PS_HIP(c, hipMemset(c->ar_mcm, 0, sizeof(double) * (size_t)std::max<int64_t>(NR * nm3, 1)));
     PS_HIP(c, hipMalloc(&c->ar_part, sizeof(double) * (size_t)nb * NSYM));
-    PS_HIP(c, hipMalloc(&c->ar_mcc, sizeof(double) * 256));
+    PS_HIP(c, hipMalloc(&c->ar_gpart, sizeof(double) * (size_t)dof_blocks(c->Np) * NSYM));
+    PS_HIP(c, hipMalloc(&c->ar_mcc, sizeof(double) * 512));
     PS_HIP(c, hipMalloc(&c->ar_flag, sizeof(int) * (size_t)(c->Nm + 2)));
     PS_HIP(c, hipMalloc(&c->ar_pos, sizeof(int) * (size_t)(c->Nm + 2)));
     PS_HIP(c, hipHostMalloc(&c->ar_host, sizeof(double) * 8, hipHostMallocDefault));
-    PS_HIP(c, hipMalloc(&c->ar_counter, sizeof(int) * 2));
-    PS_HIP(c, hipMemset(c->ar_counter, 0, sizeof(int) * 2));
+    PS_HIP(c, hipMalloc(&c->ar_bscan, sizeof(int) * 2 * (size_t)dof_blocks(c->Np)));
+    PS_HIP(c, hipMalloc(&c->ar_counter, sizeof(int) * 3));
+    PS_HIP(c, hipMemset(c->ar_counter, 0, sizeof(int) * 3));
     // Verlet partners: at most 2 Nm entries (each monomer with the colloid, both directions)
     if (c->nb_cap < 2 * (int64_t)c->Nm + 2) {
         if (c->nb_part) (void)hipFree(c->nb_part);
@@ -567,13 +838,19 @@ ps_err arrow_alloc(Ctx* c) {
         PS_HIP(c, hipMalloc(&c->nb_part, sizeof(int) * (size_t)c->nb_cap));
     }
     // M^cc starts as the colloid self block, Q as the identity (src/mob.cpp:629-632)
-    double h[256] = {};
+    double h[512] = {};
     for (int a = 0; a < 11; a++) {
         h[a * 11 + a] = a < 3 ? 1.0 : a < 6 ? 0.75 : 1.8;
         h[121 + a * 11 + a] = 1.0;
     }
     h[6 * 11 + 10] = h[10 * 11 + 6] = 0.9;
-    PS_HIP(c, hipMemcpy(c->ar_mcc, h, sizeof(double) * 256, hipMemcpyHostToDevice));
+    {
+        double m5[25];
+        for (int a = 0; a < 5; a++)
+            for (int b = 0; b < 5; b++) m5[a * 5 + b] = h[(6 + a) * 11 + 6 + b];
+        mss_factor(m5, h + 256);
+    }
+    PS_HIP(c, hipMemcpy(c->ar_mcc, h, sizeof(double) * 512, hipMemcpyHostToDevice));
     if (c->Mss) {
         double m5[25] = {};
         for (int a = 0; a < 5; a++)
@@ -585,13 +862,13 @@ ps_err arrow_alloc(Ctx* c) {
 
 void arrow_free(Ctx* c) {
     for (void* p : {(void*)c->ar_mcm, (void*)c->ar_part, (void*)c->ar_mcc, (void*)c->ar_flag, (void*)c->ar_pos,
-                    (void*)c->ar_counter,
+                    (void*)c->ar_bscan, (void*)c->ar_counter, (void*)c->ar_gpart,
                     (void*)c->wc_cell, (void*)c->wc_count, (void*)c->wc_start, (void*)c->wc_mem,
                     (void*)c->ar_bbox})
         if (p) (void)hipFree(p);
     if (c->ar_host) (void)hipHostFree(c->ar_host);
-    c->ar_mcm = c->ar_part = c->ar_mcc = c->ar_bbox = c->ar_host = nullptr;
-    c->ar_flag = c->ar_pos = c->wc_cell = c->wc_count = c->wc_start = c->wc_mem = c->ar_counter = nullptr;
+    c->ar_mcm = c->ar_part = c->ar_gpart = c->ar_mcc = c->ar_bbox = c->ar_host = nullptr;
+    c->ar_flag = c->ar_pos = c->ar_bscan = c->wc_cell = c->wc_count = c->wc_start = c->wc_mem = c->ar_counter = nullptr;
 }
 
 // The Verlet list with no host round trip: a monomer has at most one partner (the colloid), so the
@@ -601,13 +878,13 @@ ps_err arrow_check_dist(Ctx* c) {
     hipStream_t st = c->stream;
     const int Nm = c->Nm, Np = c->Np;
     const double rverl_ab = (double)(float)(1.5 + c->p.beta);
-    k_ar_flag<<<dof_blocks(Np), AB, 0, st>>>(c->sc, c->x, c->xpd, rverl_ab, c->ar_flag);
+    const unsigned nb1 = dof_blocks(Np);
+    k_ar_vl1<<<nb1, AB, 0, st>>>(c->sc, c->x, c->xpd, rverl_ab, c->ar_flag, c->ar_pos, c->ar_bscan, c->nb_off,
+                                 c->ar_counter + 2);
     PS_LAUNCH(c);
-    if (Nm > 0) PS_TRY(scan_ints(c, c->ar_flag, c->ar_pos, Nm));
-    k_ar_offsets<<<dof_blocks(Np + 2), AB, 0, st>>>(Nm, c->Nc, c->ar_pos, c->nb_off);
-    PS_LAUNCH(c);
-    if (Nm > 0 && c->Nc == 1) {
-        k_ar_fill<<<dof_blocks(Nm), AB, 0, st>>>(Nm, c->ar_flag, c->ar_pos, c->nb_off, c->nb_part);
+    if (Nm > 0) {
+        k_ar_vl2<<<dof_blocks(Nm), AB, 0, st>>>(Nm, (int)nb1, c->ar_flag, c->ar_pos, c->ar_bscan, c->nb_off,
+                                                c->nb_part);
         PS_LAUNCH(c);
     }
     c->nvlist = -1;   // counted on demand (arrow_count_pairs)
@@ -623,9 +900,15 @@ ps_err arrow_count_pairs(Ctx* c) {
     return PS_OK;
 }
 
+// M^cm of this mob(); when a Schur floor follows (ar_gram_next), the same launch stores the Gram
+// partials it needs.
 ps_err arrow_mob(Ctx* c) {
-    k_ar_mcm<<<dof_blocks(c->Np), AB, 0, c->stream>>>(c->sc, c->xpd, c->xm, c->p.beta * c->p.beta, c->ar_mcm);
+    const bool gram = c->ar_gram_next && c->Nc == 1 && c->Nm > 0;
+    k_ar_mcm<<<dof_blocks(c->Np), AB, 0, c->stream>>>(c->sc, c->xpd, c->xm, c->p.beta * c->p.beta, c->ar_mcm,
+                                                      gram ? c->ar_gpart : nullptr);
     PS_LAUNCH(c);
+    c->ar_gram_next = false;
+    c->ar_gram_ready = gram;
     c->have_mob = true;
     return PS_OK;
 }
@@ -634,13 +917,19 @@ ps_err arrow_schur(Ctx* c) {
     if (c->Nc != 1) return PS_OK;
     hipStream_t st = c->stream;
     const int64_t nm3 = c->sc.nm3;
-    const int nb = std::min(c->ar_nb, 64);   // grid-stride: few partials for the one-block finish
-    if (nm3 > 0) {
-        k_ar_gram<<<nb, AB, 0, st>>>(nm3, c->ar_mcm, c->ar_part);
+    if (c->ar_gram_ready) {   // the Gram partials of the current M^cm, from arrow_mob
+        k_ar_schur<<<1, 64, 0, st>>>(c->ar_gpart, (int)dof_blocks(c->Np), c->p.beta, c->ar_mcc, c->Mss);
+        PS_LAUNCH(c);
+    } else {
+        const int nb = std::min(c->ar_nb, 64);   // grid-stride: few partials for the one-block finish
+        if (nm3 > 0) {
+            k_ar_gram<<<nb, AB, 0, st>>>(nm3, c->ar_mcm, c->ar_part);
+            PS_LAUNCH(c);
+        }
+        k_ar_schur<<<1, 64, 0, st>>>(c->ar_part, nm3 > 0 ? nb : 0, c->p.beta, c->ar_mcc, c->Mss);
         PS_LAUNCH(c);
     }
-    k_ar_schur<<<1, 128, 0, st>>>(c->ar_part, nm3 > 0 ? nb : 0, c->p.beta, c->ar_mcc, c->Mss);
-    PS_LAUNCH(c);
+    c->ar_gram_ready = false;
     c->ar_synced = true;
     return PS_OK;
 }
@@ -667,6 +956,28 @@ ps_err arrow_apply(Ctx* c, const double* x, double* y, int saddle) {
         PS_LAUNCH(c);
     }
     c->cnt.applies++;
+    return PS_OK;
+}
+
+// One block-eliminated saddle solve (Nc == 1, Nm > 0): k_ar_bs1 + k_ar_bs2.  Timing kind 4 spans the
+// first dispatch to the second's end.
+ps_err arrow_block_solve(Ctx* c, const double* b, double* x, double* uout) {
+    hipStream_t st = c->stream;
+    const int64_t nm3 = c->sc.nm3, n = c->n;
+    const unsigned nb = dof_blocks(nm3);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->prof_on) PS_TRY(c->prof_record(4, &e0, &e1));
+    const bool dispatch_ev = c->prof_on && c->prof_dispatch_events;
+    if (c->prof_on && !dispatch_ev) PS_HIP(c, hipEventRecord(e0, st));
+    hipExtLaunchKernelGGL(k_ar_bs1, dim3(nb), dim3(AB), 0, st, dispatch_ev ? e0 : nullptr, nullptr, 0, nm3, n,
+                          (const double*)c->ar_mcm, (const double*)c->ar_mcc, b, x, uout, c->ar_part,
+                          c->ar_counter);
+    PS_LAUNCH(c);
+    hipExtLaunchKernelGGL(k_ar_bs2, dim3(nb), dim3(AB), 0, st, nullptr, dispatch_ev ? e1 : nullptr, 0, nm3, n,
+                          1.0 / c->p.beta, (const double*)c->ar_mcm, b, x, uout);
+    PS_LAUNCH(c);
+    if (c->prof_on && !dispatch_ev) PS_HIP(c, hipEventRecord(e1, st));
+    c->cnt.applies += 2;
     return PS_OK;
 }
 

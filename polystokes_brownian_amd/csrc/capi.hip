@@ -181,9 +181,10 @@ struct NoisePipe {
     }
 };
 
-ps_err solve(Ctx* c, double* X, int warm) {
+// The step's saddle solve of rhs into X; `uout` (optional) receives its velocity rows.
+ps_err solve(Ctx* c, double* X, int warm, double* uout = nullptr) {
     ps_solve_info info{};
-    ps_err r = saddle_solve_dev(c, c->rhs, X, warm, &info);
+    ps_err r = saddle_solve_dev(c, c->rhs, X, warm, &info, uout);
     if (r == PS_ERR_NOCONV) r = PS_OK;   // the reference prints and continues (src/vel.cpp:45-51)
     return r;
 }
@@ -199,11 +200,10 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s) {
     // ---- predictor
     PS_TRY(check_dist_dev(c));
     PS_TRY(rhs_dev(c, 0));
+    c->ar_gram_next = c->arrow;
     PS_TRY(mob_prepare(c));
     if (c->arrow) PS_TRY(arrow_schur(c));   // sync_mcc_schur_correction (src/run.cpp:92)
-    PS_TRY(solve(c, c->Xdet, 0));
-    k_extract<<<eb(nu), TB, 0, st>>>(c->v_det_n, c->Xdet, n, nu);
-    PS_LAUNCH(c);
+    PS_TRY(solve(c, c->Xdet, 0, c->v_det_n));
     if (kT > 0) {
         // solve_slip_vel: W = sqrt(M) xi, rhs[0:n] += sqrt(2kT/dt) W
         double* noise = pipe->take(s);
@@ -212,9 +212,7 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s) {
         k_add_slip<<<eb(n), TB, 0, st>>>(c->rhs, c->W, std::sqrt(2 * kT / dt), n);
         PS_LAUNCH(c);
         // new_vel
-        PS_TRY(solve(c, c->X, 0));
-        k_extract<<<eb(nu), TB, 0, st>>>(c->up, c->X, n, nu);
-        PS_LAUNCH(c);
+        PS_TRY(solve(c, c->X, 0, c->up));
         if (c->p.record_forces) {
             k_extract<<<eb(nu), TB, 0, st>>>(c->fext, c->X, 0, nu);
             PS_LAUNCH(c);
@@ -229,9 +227,7 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s) {
         PS_TRY(check_dist_dev(c));
         PS_TRY(mob_prepare(c));
         PS_TRY(rhs_dev(c, 1));
-        PS_TRY(solve(c, c->Xd, 0));
-        k_extract<<<eb(nu), TB, 0, st>>>(c->drift_v, c->Xd, n, nu);
-        PS_LAUNCH(c);
+        PS_TRY(solve(c, c->Xd, 0, c->drift_v));
         if (nc5 > 0) {
             k_sb<<<eb(nc5), TB, 0, st>>>(c->sb, c->Xd, nu, nc5, 0);
             PS_LAUNCH(c);
@@ -241,9 +237,7 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s) {
         PS_TRY(check_dist_dev(c));
         PS_TRY(mob_prepare(c));
         PS_TRY(rhs_dev(c, 1));
-        PS_TRY(solve(c, c->Xd, 1));
-        k_extract<<<eb(nu), TB, 0, st>>>(c->udiff, c->Xd, n, nu);
-        PS_LAUNCH(c);
+        PS_TRY(solve(c, c->Xd, 1, c->udiff));
         k_sub<<<eb(nu), TB, 0, st>>>(c->udiff, c->udiff, c->drift_v, nu);
         PS_LAUNCH(c);
         if (nc5 > 0) {
@@ -251,9 +245,7 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s) {
             PS_LAUNCH(c);
         }
     } else {
-        PS_TRY(solve(c, c->X, 0));
-        k_extract<<<eb(nu), TB, 0, st>>>(c->up, c->X, n, nu);
-        PS_LAUNCH(c);
+        PS_TRY(solve(c, c->X, 0, c->up));
         if (c->p.record_forces) {
             k_extract<<<eb(nu), TB, 0, st>>>(c->fext, c->X, 0, nu);
             PS_LAUNCH(c);
@@ -265,11 +257,10 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s) {
     // ---- corrector
     PS_TRY(check_dist_dev(c));
     PS_TRY(rhs_dev(c, 0));
+    c->ar_gram_next = c->arrow;
     PS_TRY(mob_prepare(c));
     if (c->arrow) PS_TRY(arrow_schur(c));   // src/run.cpp:129 (the drift's mob() calls take none)
-    PS_TRY(solve(c, c->Xdet, 0));
-    k_extract<<<eb(nu), TB, 0, st>>>(c->v_det_c, c->Xdet, n, nu);
-    PS_LAUNCH(c);
+    PS_TRY(solve(c, c->Xdet, 0, c->v_det_c));
     k_trapz<<<eb(nu), TB, 0, st>>>(c->up, c->v_det_n, c->v_det_c, c->v_brown, nu);
     PS_LAUNCH(c);
     k_step_from<<<eb(c->Np), TB, 0, st>>>(c->sc, c->x, c->x_n, c->udiff, c->up, dt, kT);
@@ -346,9 +337,12 @@ ps_err Ctx::prof_harvest() {
         } else if (ev_kind[k] == 2) {
             prof.dense_launches++;
             prof.dense_ms += ms;
-        } else {
+        } else if (ev_kind[k] == 3) {
             prof.arrow_launches++;
             prof.arrow_ms += ms;
+        } else {
+            prof.arsolve_launches++;
+            prof.arsolve_ms += ms;
         }
     }
     ev_used = 0;

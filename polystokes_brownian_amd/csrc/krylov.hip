@@ -847,9 +847,18 @@ __global__ void k_bs_u(double* __restrict__ x, const double* __restrict__ y, con
 
 // Exact block elimination of A x = b with B = [-I_nu; 0]:
 //   f_u = -b[n:n+nu];  M_ss f_s = b_s - (M [f_u; 0])_s;  u = (M f)_u - b_u.
-static ps_err block_solve_once(Ctx* c, const double* b, double* x, int prec, int* mss_its) {
+// With one colloid and monomers, the arrowhead regime runs the whole elimination in two DOF passes
+// (arrow_block_solve), which also write u to `uout` when given (*u_done is then set).
+static ps_err block_solve_once(Ctx* c, const double* b, double* x, int prec, int* mss_its,
+                               double* uout = nullptr, bool* u_done = nullptr) {
     const int64_t n = c->n, nu = c->nu, n5 = 5 * (int64_t)c->Nc;
     hipStream_t st = c->stream;
+    if (c->arrow && c->Nc == 1 && c->sc.nm3 > 0) {
+        PS_TRY(arrow_block_solve(c, b, x, uout));
+        if (u_done) *u_done = uout != nullptr;
+        c->cnt.saddle_solves++;
+        return PS_OK;
+    }
     k_bs_init<<<eblocks(n), VB, 0, st>>>(x, b, n, nu);
     PS_LAUNCH(c);
     if (n5 > 0 && c->arrow) {
@@ -874,14 +883,28 @@ static ps_err block_solve_once(Ctx* c, const double* b, double* x, int prec, int
 }
 
 
-ps_err saddle_solve_dev(Ctx* c, const double* b, double* x, int warm, ps_solve_info* info) {
+static ps_err saddle_solve_into(Ctx* c, const double* b, double* x, int warm, ps_solve_info* info, double* uout,
+                                bool* u_done);
+// The saddle solve; `uout` (optional) receives u = x[n : n + nu].
+ps_err saddle_solve_dev(Ctx* c, const double* b, double* x, int warm, ps_solve_info* info, double* uout) {
+    bool u_done = false;
+    ps_err rc = saddle_solve_into(c, b, x, warm, info, uout, &u_done);
+    if (uout && !u_done && (rc == PS_OK || rc == PS_ERR_NOCONV)) {
+        const ps_err r2 = vec_copy(c, uout, x + c->n, c->nu);
+        if (r2 != PS_OK) return r2;
+    }
+    return rc;
+}
+
+static ps_err saddle_solve_into(Ctx* c, const double* b, double* x, int warm, ps_solve_info* info, double* uout,
+                                bool* u_done) {
     const bool block = c->p.solver == PS_SOLVER_BLOCK;
     // Block elimination costs one full apply per solve; that apply stays FP64 in every form, since
     // an FP32 apply followed by an FP64 residual would cost more than the FP64 apply itself.
     if (c->p.precision != PS_PREC_MIXED || block) {
         if (!block) return minres_dev(c, b, x, warm, c->p.ksp_rtol, c->p.ksp_maxit, PS_PREC_FP64, info);
         int its = 0;
-        PS_TRY(block_solve_once(c, b, x, PS_PREC_FP64, &its));
+        PS_TRY(block_solve_once(c, b, x, PS_PREC_FP64, &its, uout, u_done));
         c->last_block_its = its;
         if (info) {
             *info = ps_solve_info{};

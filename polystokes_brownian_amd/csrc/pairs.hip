@@ -86,11 +86,11 @@ __global__ void k_scatter(int Np, const double* __restrict__ xs, const int* __re
 }
 
 // Single-block exclusive scan of n ints: out[0..n] with out[n] = total, and out[n + 1] = max(in).
-// Each of the 1024 threads scans a contiguous chunk, so there is one block-wide scan in total.
+// Each of the 1024 threads scans a contiguous chunk; the chunk sums are scanned within each wave by
+// shuffles and across the 16 waves through LDS (one barrier).
 __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ in, int* __restrict__ out, int n) {
-    __shared__ int sh[1024];
-    __shared__ int shm[1024];
-    const int t = threadIdx.x;
+    __shared__ int wsum[16], wmax[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int per = (n + 1023) / 1024;
     const int b = min(t * per, n), e = min(b + per, n);
     int s = 0, mx = 0;
@@ -99,26 +99,28 @@ __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ in, int* 
         s += v;
         mx = max(mx, v);
     }
-    sh[t] = s;
-    shm[t] = mx;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int a = (t >= off) ? sh[t - off] : 0;
-        const int m = (t >= off) ? shm[t - off] : 0;
-        __syncthreads();
-        sh[t] += a;
-        shm[t] = max(shm[t], m);
-        __syncthreads();
+    int inc = s;   // inclusive scan of the chunk sums within the wave
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
     }
-    int run = sh[t] - s;
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+    if (lane == 63) wsum[w] = inc;
+    if (lane == 0) wmax[w] = mx;
+    __syncthreads();
+    int base = 0;
+    for (int k = 0; k < w; k++) base += wsum[k];
+    int run = base + inc - s;
     for (int i = b; i < e; i++) {
         const int v = in[i];
         out[i] = run;
         run += v;
     }
     if (t == 1023) {
-        out[n] = sh[1023];
-        out[n + 1] = shm[1023];
+        out[n] = base + inc;
+        int m = 0;
+        for (int k = 0; k < 16; k++) m = max(m, wmax[k]);
+        out[n + 1] = m;
     }
 }
 
@@ -232,47 +234,45 @@ struct ForceConst {
     int trap_on;
 };
 
-// rhs = [0 (n) ; fint (3Np) ; 0 (3Nc)] with fint = pairs + bonds + trap (src/rhs.cpp:160-260).
-__global__ void k_forces(ForceConst fc, const double* __restrict__ xpd, const double* __restrict__ x,
-                         const double* __restrict__ xpos0, const int* __restrict__ off,
-                         const int* __restrict__ part, const int* __restrict__ boff,
-                         const int* __restrict__ bond_k, const int* __restrict__ bond_j,
-                         double* __restrict__ rhs) {
-    const int p = blockIdx.x * TB + threadIdx.x;
+// Excluded-volume force of the Verlet pair (p, q) on p (pair_interaction_hs / _hs_exp), added to f.
+__device__ __forceinline__ void pair_force_add(const ForceConst& fc, const double* __restrict__ xpd, int p, int q,
+                                               double& fx, double& fy, double& fz) {
     const SysConst& s = fc.s;
-    if (p >= s.Np) return;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    // excluded volume over the Verlet partners (pair_interaction_hs / _hs_exp)
-    for (int t = off[p]; t < off[p + 1]; t++) {
-        const int q = part[t];
-        const int k = min(p, q), j = max(p, q);
-        const int type = ptype(s.Nm, k, j);
-        if (type == 0 && !fc.mono_ev) continue;
-        double dx, dy, dz, dr;
-        geom(s, xpd, k, j, dx, dy, dz, dr);
-        const double dr_inv = 1 / dr;
-        const double ex = dx * dr_inv, ey = dy * dr_inv, ez = dz * dr_inv;
-        const double sig_eff = fc.sig[type] * (1.0 + 0.01);
-        double fmag;
-        if (fc.force_sel == PS_FORCE_HS_EXP) {
-            const double ex_ = exp(-fc.hs_exp_k * (dr - sig_eff));
-            fmag = ex_ / (1 - ex_);
-        } else {
-            fmag = (dr >= sig_eff) ? 0.0 : fc.k_hs * (sig_eff - dr);
-        }
-        if (fmag == 0.0) continue;
-        const double gx = fmag * ex, gy = fmag * ey, gz = fmag * ez;
-        if (p == k) {
-            fx -= gx;
-            fy -= gy;
-            fz -= gz;
-        } else {
-            fx += gx;
-            fy += gy;
-            fz += gz;
-        }
+    const int k = min(p, q), j = max(p, q);
+    const int type = ptype(s.Nm, k, j);
+    if (type == 0 && !fc.mono_ev) return;
+    double dx, dy, dz, dr;
+    geom(s, xpd, k, j, dx, dy, dz, dr);
+    const double dr_inv = 1 / dr;
+    const double ex = dx * dr_inv, ey = dy * dr_inv, ez = dz * dr_inv;
+    const double sig_eff = fc.sig[type] * (1.0 + 0.01);
+    double fmag;
+    if (fc.force_sel == PS_FORCE_HS_EXP) {
+        const double ex_ = exp(-fc.hs_exp_k * (dr - sig_eff));
+        fmag = ex_ / (1 - ex_);
+    } else {
+        fmag = (dr >= sig_eff) ? 0.0 : fc.k_hs * (sig_eff - dr);
     }
-    // bonds (bond_forces, src/rhs.cpp:102-158)
+    if (fmag == 0.0) return;
+    const double gx = fmag * ex, gy = fmag * ey, gz = fmag * ez;
+    if (p == k) {
+        fx -= gx;
+        fy -= gy;
+        fz -= gz;
+    } else {
+        fx += gx;
+        fy += gy;
+        fz += gz;
+    }
+}
+
+// Bonds (bond_forces, src/rhs.cpp:102-158) and the static harmonic trap on the colloids
+// (trapping_forces, src/rhs.cpp:46-100) of particle p, added to f.
+__device__ __forceinline__ void bond_trap_add(const ForceConst& fc, const double* __restrict__ x,
+                                              const double* __restrict__ xpos0, const int* __restrict__ boff,
+                                              const int* __restrict__ bond_k, const int* __restrict__ bond_j, int p,
+                                              double& fx, double& fy, double& fz) {
+    const SysConst& s = fc.s;
     for (int t = boff[p]; t < boff[p + 1]; t++) {
         const int k = bond_k[t], j = bond_j[t];
         double r0_ = x[3 * k] - x[3 * j], r1_ = x[3 * k + 1] - x[3 * j + 1], r2_ = x[3 * k + 2] - x[3 * j + 2];
@@ -301,7 +301,6 @@ __global__ void k_forces(ForceConst fc, const double* __restrict__ xpd, const do
             fz += f2;
         }
     }
-    // static harmonic trap on the colloids (trapping_forces, src/rhs.cpp:46-100)
     if (fc.trap_on && p >= s.Nm) {
         const int ic3 = 3 * (p - s.Nm);
         double dx = x[3 * p] - xpos0[ic3], dy = x[3 * p + 1] - xpos0[ic3 + 1], dz = x[3 * p + 2] - xpos0[ic3 + 2];
@@ -314,6 +313,40 @@ __global__ void k_forces(ForceConst fc, const double* __restrict__ xpd, const do
         fy += fc.ktrap * dy;
         fz += fc.ktrap * dz;
     }
+}
+
+// rhs = [0 (n) ; fint (3Np) ; 0 (3Nc)] with fint = pairs + bonds + trap (src/rhs.cpp:160-260).
+// Blocks below `mono_blocks` give each monomer a thread; the blocks after them give each colloid a
+// wave, whose lanes stride over its partner list (a colloid can have thousands of Verlet partners),
+// combined by a fixed shuffle tree, so the result is deterministic.
+constexpr int FW = TB / 64;   // colloids per colloid block
+__global__ __launch_bounds__(TB) void k_forces(ForceConst fc, const double* __restrict__ xpd,
+                                               const double* __restrict__ x, const double* __restrict__ xpos0,
+                                               const int* __restrict__ off, const int* __restrict__ part,
+                                               const int* __restrict__ boff, const int* __restrict__ bond_k,
+                                               const int* __restrict__ bond_j, int mono_blocks,
+                                               double* __restrict__ rhs) {
+    const SysConst& s = fc.s;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    int p;
+    if ((int)blockIdx.x < mono_blocks) {
+        p = blockIdx.x * TB + threadIdx.x;
+        if (p >= s.Nm) return;
+        for (int t = off[p]; t < off[p + 1]; t++) pair_force_add(fc, xpd, p, part[t], fx, fy, fz);
+    } else {
+        const int lane = threadIdx.x & 63;
+        const int ci = ((int)blockIdx.x - mono_blocks) * FW + (int)(threadIdx.x >> 6);
+        if (ci >= s.Nc) return;   // whole wave
+        p = s.Nm + ci;
+        for (int t = off[p] + lane; t < off[p + 1]; t += 64) pair_force_add(fc, xpd, p, part[t], fx, fy, fz);
+        for (int o = 32; o > 0; o >>= 1) {
+            fx += __shfl_down(fx, o, 64);
+            fy += __shfl_down(fy, o, 64);
+            fz += __shfl_down(fz, o, 64);
+        }
+        if (lane != 0) return;
+    }
+    bond_trap_add(fc, x, xpos0, boff, bond_k, bond_j, p, fx, fy, fz);
     rhs[s.n + 3 * (int64_t)p] = fx;
     rhs[s.n + 3 * (int64_t)p + 1] = fy;
     rhs[s.n + 3 * (int64_t)p + 2] = fz;
@@ -539,8 +572,11 @@ ps_err rhs_dev(Ctx* c, int drift) {
     const int* boff = c->bonds;
     const int* bk = c->bonds + Np + 1;
     const int* bj = c->bonds + Np + 1 + nb;
-    k_forces<<<(unsigned)((Np + TB - 1) / TB), TB, 0, st>>>(fc, c->xpd, c->x, c->xpos0, c->nb_off, c->nb_part,
-                                                           boff, bk, bj, c->rhs);
+    const int mono_blocks = (c->Nm + TB - 1) / TB, coll_blocks = (c->Nc + FW - 1) / FW;
+    if (mono_blocks + coll_blocks > 0) {
+        k_forces<<<(unsigned)(mono_blocks + coll_blocks), TB, 0, st>>>(fc, c->xpd, c->x, c->xpos0, c->nb_off,
+                                                                       c->nb_part, boff, bk, bj, mono_blocks, c->rhs);
+    }
     PS_LAUNCH(c);
     // mm_HI off: monomer-monomer excluded volume through the monomer cell list (src/rhs.cpp:200-205)
     if (c->arrow && c->p.mono_ev) PS_TRY(arrow_wca(c));

@@ -167,12 +167,18 @@ struct Ctx {
     bool arrow = false;
     double* ar_mcm = nullptr;       // [8][3Nm]: the rotation and stresslet rows of M^cm
     double* ar_part = nullptr;      // per-block partials
-    double* ar_mcc = nullptr;       // floored M^cc (121), eigenvectors Q (121), floored spectrum (11)
+    double* ar_gpart = nullptr;     // per-block Gram partials of the last M^cm (for the Schur floor)
+    bool ar_gram_next = false;      // the next mob() is followed by a Schur floor: store its Gram partials
+    bool ar_gram_ready = false;     // ar_gpart holds the current M^cm's Gram partials
+    double* ar_mcc = nullptr;       // floored M^cc (121), eigenvectors Q (121), floored spectrum (11);
+                                    // at 256: Cholesky factor of its stresslet block (25, 1/diag)
     double* ar_bbox = nullptr;      // monomer bounding box (unbounded WCA grid)
     double* ar_host = nullptr;      // pinned
     int* ar_flag = nullptr;         // [Nm+2] monomer-colloid Verlet flags
-    int* ar_pos = nullptr;          // [Nm+2] their exclusive scan
-    int* ar_counter = nullptr;      // [2] last-block counters (apply, noise), reset by their last block
+    int* ar_pos = nullptr;          // [Nm+2] their exclusive scan within each block
+    int* ar_bscan = nullptr;        // [2 nb]: per-block flag counts, then their exclusive scan
+    int* ar_counter = nullptr;      // [3] last-block counters (apply, noise, Verlet list), reset by their
+                                    // last block
     int ar_nb = 0;
     bool ar_synced = false;         // a Schur floor has run
     bool ar_diag_frozen = false;    // the Jacobi diagonal is fixed at the first MINRES solve
@@ -226,7 +232,8 @@ void apply_free(Ctx* c);
 ps_err krylov_alloc(Ctx* c);
 void krylov_free(Ctx* c);
 ps_err saddle_diag_dev(Ctx* c, double* d);
-ps_err saddle_solve_dev(Ctx* c, const double* b, double* x, int warm, ps_solve_info* info);
+ps_err saddle_solve_dev(Ctx* c, const double* b, double* x, int warm, ps_solve_info* info,
+                        double* uout = nullptr);
 ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* w, ps_sqrt_info* info);
 ps_err pairs_alloc(Ctx* c);
 void pairs_free(Ctx* c);
@@ -241,6 +248,7 @@ ps_err arrow_alloc(Ctx* c);
 void arrow_free(Ctx* c);
 ps_err arrow_check_dist(Ctx* c);                                   // Verlet list, AB + BB pairs only
 ps_err arrow_count_pairs(Ctx* c);                                  // host copy of the pair count (syncs)
+ps_err arrow_block_solve(Ctx* c, const double* b, double* x, double* uout);
 ps_err arrow_mss_solve(Ctx* c, const double* b, const double* y, double* x);   // 5 x 5 stresslet system
 ps_err arrow_mob(Ctx* c);                                          // snapshot + M^cm
 ps_err arrow_schur(Ctx* c);                                        // sync_mcc_schur_correction

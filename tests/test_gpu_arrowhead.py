@@ -59,9 +59,13 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("name", list(CASES))
+# "large": 66 000 monomers, past the single-launch list (65 536), so the multi-launch path is checked
+VLIST_CASES = dict(CASES, large=dict(nd=33000, beta=0.1, L=40.0))
+
+
+@pytest.mark.parametrize("name", list(VLIST_CASES))
 def test_arrow_vlist_bit_exact(name):
-    g, o, _, _ = arrow_pair(**CASES[name])
+    g, o, _, _ = arrow_pair(**VLIST_CASES[name])
     n_g = g.check_dist()
     n_o = o.check_dist()
     assert n_g == n_o
