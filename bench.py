@@ -58,23 +58,26 @@ def make_ctx(Context, pd, x0, **over):
     return c
 
 
-def timed_run(ctx, steps, warmup, label):
+def timed_run(ctx, steps, warmup, label, profile=True):
     """W warmup steps, then exactly `steps` steps in one run() call (as PolyStokes.run drives them:
-    the noise of step k+1 is drawn on the host while the GPU works on step k)."""
+    the noise of step k+1 is drawn on the host while the GPU works on step k).  With `profile`, the
+    library times the kernels it reports (ps_profile) -- which also makes ps_run launch every kernel
+    itself rather than replay a step graph."""
     if warmup > 0:
         t0 = time.perf_counter()
         ctx.run(warmup)
         log(f"{label}: {warmup} warmup step(s) {time.perf_counter() - t0:.2f}s")
     c0 = ctx.counters()
     ctx.state()   # synchronise
-    ctx.profile_begin()
+    if profile:
+        ctx.profile_begin()
     log(f"{label}: timing {steps} step(s)")
     t0 = time.perf_counter()
     ctx.run(steps)   # ps_run synchronises the library's stream at its end
     ctx.state()
     dt = time.perf_counter() - t0
     log(f"{label}: {steps} timed step(s) {dt:.3f}s")
-    prof = ctx.profile_end()
+    prof = ctx.profile_end() if profile else None
     c1 = ctx.counters()
     per = {k: (c1[k] - c0[k]) / steps for k in ("applies", "minres_iterations", "lanczos_iterations",
                                                  "saddle_solves", "mob_calls")}
@@ -149,10 +152,12 @@ ARROW_PUBLISHED = 10001 * 300 / 23.6
 def arrow_bench(args, configs, Context, steps):
     """The mm_HI=False (arrowhead) regime on the published case (workload arrow10k).  Forms: the
     reference algorithm (Jacobi MINRES rtol 1e-6, diagonal frozen at the first solve) and exact block
-    elimination; both FP64.  Roofline: the DOF pass of the arrowhead apply (HBM-bound), its launches
-    timed by their own dispatch events; algorithmic bytes per launch = the 8 stored rows of M^cm, the
-    monomer input and output (saddle applies also read the multiplier rows and write the constraint
-    rows).  CPU baseline: the oracle on one host core over a bounded number of steps."""
+    elimination; both FP64.  The timed steps run as ps_run drives them (the block form replays each
+    step as a HIP graph); a further profiled run times the dominant kernels by their own dispatch
+    events.  Roofline (HBM-bound): for the block form one block-eliminated solve, k_ar_bs1 + k_ar_bs2,
+    per solve 8 B x 22 x 3Nm (each pass reads the 8 stored rows of M^cm; pass 1 reads b_l and writes
+    f_m, pass 2 reads f_m and b_u and writes u_m and its copy); for MINRES the apply's DOF pass,
+    8 B x 10 x 3Nm.  CPU baseline: the oracle on one host core over a bounded number of steps."""
     w = configs.workload("arrow10k")
     pd = configs.params_dict(w)
     x0 = configs.positions(w)
@@ -160,7 +165,7 @@ def arrow_bench(args, configs, Context, steps):
     forms = {"fp64/block": dict(solver=1), "fp64/minres": dict(solver=0)}
     if args.profile_form:
         ctx = make_ctx(Context, pd, x0, **forms[args.profile_form])
-        dt, prof, per = timed_run(ctx, steps, args.warmup, "arrow " + args.profile_form)
+        dt, _, per = timed_run(ctx, steps, args.warmup, "arrow " + args.profile_form, profile=False)
         ctx.close()
         log(f"arrow {args.profile_form}: {Np * steps / dt:.4g} bead-steps/s, {1e3 * dt / steps:.3f} ms/step, "
             f"per step {per}")
@@ -168,19 +173,23 @@ def arrow_bench(args, configs, Context, steps):
     res = {}
     for form, over in forms.items():
         ctx = make_ctx(Context, pd, x0, **over)
-        dt, prof, per = timed_run(ctx, steps, args.warmup, "arrow " + form)
+        dt, _, per = timed_run(ctx, steps, args.warmup, "arrow " + form, profile=False)
+        _, prof, _ = timed_run(ctx, min(steps, 50), 0, "arrow " + form + " kernel timing", profile=True)
         ctx.close()
-        avg = prof["arrow_ms"] / max(prof["arrow_launches"], 1) * 1e-3
+        if form == "fp64/block":
+            kms, kn = prof["arsolve_ms"], prof["arsolve_launches"]
+        else:
+            kms, kn = prof["arrow_ms"], prof["arrow_launches"]
+        avg = kms / max(kn, 1) * 1e-3
         res[form] = dict(value=Np * steps / dt, ms_per_step=1e3 * dt / steps, per_step=per,
-                         apply_pass_ms_avg=avg * 1e3, apply_pass_launches=prof["arrow_launches"])
+                         kernel_ms_avg=avg * 1e3, kernel_launches=kn)
         log(f"arrow {form}: {res[form]['value']:.4g} bead-steps/s, {res[form]['ms_per_step']:.3f} ms/step, "
-            f"per step {per}, apply pass avg {avg * 1e6:.2f} us over {prof['arrow_launches']} launches")
+            f"per step {per}, dominant kernel avg {avg * 1e6:.2f} us over {kn} launches")
     up_b = one_step_up(Context, pd, x0, **forms["fp64/block"])
     up_m = one_step_up(Context, pd, x0, **forms["fp64/minres"])
     head = res["fp64/block"]
-    # the block form's applies are mobility applies: 8 rows + x_m + y_m doubles per monomer DOF
-    bytes_launch = 8.0 * (8 + 1 + 1) * nm3
-    achieved = bytes_launch / (head["apply_pass_ms_avg"] * 1e-3) if head["apply_pass_ms_avg"] > 0 else 0.0
+    bytes_launch = 8.0 * 22 * nm3   # one block-eliminated solve: both passes
+    achieved = bytes_launch / (head["kernel_ms_avg"] * 1e-3) if head["kernel_ms_avg"] > 0 else 0.0
     cpu = None
     if not args.no_cpu:
         from oracle.pyoracle import Oracle, get_threads, set_threads
@@ -213,8 +222,8 @@ def arrow_bench(args, configs, Context, steps):
         "block_vs_minres_velocity_rel_diff": rel(up_b, up_m),
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_ar_apply (the DOF pass of the arrowhead apply)", "bytes_per_launch": bytes_launch,
-                     "avg_launch_ms": head["apply_pass_ms_avg"]},
+                     "kernel": "k_ar_bs1 + k_ar_bs2 (one block-eliminated saddle solve, first dispatch to last)",
+                     "bytes_per_launch": bytes_launch, "avg_launch_ms": head["kernel_ms_avg"]},
         "cpu_baseline": cpu,
     }
 

@@ -88,6 +88,10 @@ typedef struct ps_params {
     int ir_maxit;                 /* mixed form: cap on refinement sweeps */
     int solver;                   /* PS_SOLVER_MINRES (reference algorithm) or PS_SOLVER_BLOCK */
     int aa_kernel;                /* monomer-monomer kernel: PS_AA_AUTO, PS_AA_ORDERED, PS_AA_SYMMETRIC */
+    int graph;                    /* ps_run: 1 (default) replays each step as a HIP graph where the step
+                                     has no host round trip (mm_HI=False, one colloid, PS_SOLVER_BLOCK,
+                                     kT > 0, no mono_ev, no record_forces, profiling off); 0 launches
+                                     every kernel */
 } ps_params;
 
 /* Monomer-monomer tile kernel.  ORDERED evaluates every ordered pair (target-stationary, source

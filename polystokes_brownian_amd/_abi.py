@@ -31,7 +31,7 @@ class PsParams(C.Structure):
         ("sqrt_rtol", C.c_double), ("sqrt_kmax", C.c_int),
         ("force_sel", C.c_int), ("hs_kpref", C.c_double), ("hs_kmode", C.c_int), ("hs_exp_k", C.c_double),
         ("precision", C.c_int), ("ir_rtol", C.c_double), ("ir_maxit", C.c_int),
-        ("solver", C.c_int), ("aa_kernel", C.c_int),
+        ("solver", C.c_int), ("aa_kernel", C.c_int), ("graph", C.c_int),
     ]
 
 

@@ -30,6 +30,10 @@ constexpr int NR = 8;               // stored rows of M^cm (rotation 3 + stressl
 constexpr int NSYM = NR * (NR + 1) / 2;
 
 unsigned dof_blocks(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + AB - 1) / AB); }
+// The passes that end in a grid-wide reduction (apply, block solve, noise) give each thread ARD DOFs:
+// fewer blocks store partials and take the finishing counter, which shortens the last-block chain.
+constexpr int ARD = 4;
+unsigned red_blocks(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + AB * ARD - 1) / (AB * ARD)); }
 
 // ------------------------------------------------------------------------------------ Verlet list
 // Monomer k pairs with the colloid when their separation is below the AB Verlet cutoff (a float,
@@ -169,12 +173,14 @@ template <int NV>
 __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* __restrict__ part) {
     __shared__ double sh[AB / 64][NV];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // offsets outermost: the NV shuffle chains are independent and overlap
 #pragma unroll
-    for (int q = 0; q < NV; q++) {
-        double a = v[q];
-        for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
-        if (lane == 0) sh[wv][q] = a;
-    }
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int q = 0; q < NV; q++) v[q] += __shfl_down(v[q], off, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < NV; q++) sh[wv][q] = v[q];
     __syncthreads();
     for (int q = threadIdx.x; q < NV; q += AB) {
         double a = 0.0;
@@ -263,22 +269,30 @@ __device__ __forceinline__ void mss_factor_solve(const double* L, double (&r)[5]
 
 // sync_mcc_schur_correction (src/slip_vel.cpp:81-203) in one wave: S = M^cc_base - beta P, its
 // eigen-decomposition, eigenvalues below 0.1 raised to 0.1, and M^cc = base + Q diag(delta) Q^T.
-// Writes M^cc, Q (row-major, eigenvectors as columns), the floored spectrum, and the 5 x 5 stresslet
-// block of M^cc for the block solver.
+// Writes M^cc, Q (row-major, eigenvectors as columns), the floored spectrum, the 5 x 5 stresslet
+// block of M^cc (mss) and its Cholesky factor (at out + 256) for the block solver, and at out + 300
+// a flag: 1 when Q and the spectrum were left from an earlier sync (lazy mode, below).
 //
-// Eigensolver: the parallel cyclic Jacobi method on the 12-padded matrix, warm-started from the
-// previous sync's eigenvectors Q0 (S moves little between syncs, so Q0^T S Q0 is nearly diagonal and
-// a few sweeps converge it); Q0 is first re-orthonormalised by one Newton-Schulz step,
-// Q0 <- (3/2) Q0 - (1/2) Q0 Q0^T Q0, so rounding does not accumulate over syncs.  Each round of the
-// round-robin ordering rotates 6 disjoint index pairs at once (Rutishauser's angle formulas) as one
-// two-sided update A' = J^T A J: each of 36 lanes updates the 2 x 2 block of A on one pair's rows and
-// another's columns (B' = J1^T B J2); the rotated pairs' off-diagonal entries are set to zero and
-// their diagonal entries become a_pp - t a_pq and a_qq + t a_pq.  Sweeps stop when the off-diagonal
-// mass is below 1e-30 of the diagonal's.  The floored matrix and the square root Q diag(sqrt(lam)) Q^T do not depend on the
-// eigenvector choice, so the warm start changes them only by rounding.
-constexpr int JN = 12;
+// Only the rotation and stresslet rows (3..10) of S differ from the base: its translation block is
+// the identity and decoupled (the stored rows of M^cm have no translation rows).  So the translation
+// eigenpairs are (1, e_0..e_2) and the eigensolver works on the 8 x 8 block: the parallel cyclic
+// Jacobi method, warm-started from the previous sync's eigenvectors Q0 (S moves little between
+// syncs, so Q0^T S Q0 is nearly diagonal and a few sweeps converge it); Q0 is first
+// re-orthonormalised by one Newton-Schulz step, Q0 <- (3/2) Q0 - (1/2) Q0 Q0^T Q0, so rounding does
+// not accumulate over syncs.  Each round of the round-robin ordering rotates 4 disjoint index pairs
+// at once (Rutishauser's angle formulas) as one two-sided update A' = J^T A J: each of 16 lanes
+// updates the 2 x 2 block of A on one pair's rows and another's columns (B' = J1^T B J2); the rotated
+// pairs' off-diagonal entries are set to zero and their diagonal entries become a_pp - t a_pq and
+// a_qq + t a_pq.  Sweeps stop when the off-diagonal mass is below 1e-30 of the diagonal's.  The
+// floored matrix and the square root Q diag(sqrt(lam)) Q^T do not depend on the eigenvector choice,
+// so the warm start changes them only by rounding.
+//
+// Lazy mode (a sync whose Q and spectrum nothing reads before the next sync): when Gershgorin's discs
+// put every eigenvalue of S above 0.1, no eigenvalue is floored and M^cc is the base block exactly,
+// so the eigensolver is skipped.
+constexpr int JN = 8;
 // Round-robin ordering (circle method) of round r: seat 0 holds index 0, seat j >= 1 index
-// 1 + (j - 1 + r) mod 11; pair k joins seats k and 11 - k.
+// 1 + (j - 1 + r) mod (JN - 1); pair k joins seats k and JN - 1 - k.
 __device__ __forceinline__ int jseat(int j, int r) { return j == 0 ? 0 : 1 + (j - 1 + r) % (JN - 1); }
 // the indices p < q of pair k in round r
 __device__ __forceinline__ void jpair_of(int k, int r, int& p, int& q) {
@@ -286,134 +300,118 @@ __device__ __forceinline__ void jpair_of(int k, int r, int& p, int& q) {
     p = min(a, b);
     q = max(a, b);
 }
-// C = X^T Y (tx) or X Y (!tx) for 12 x 12 LDS matrices; each lane computes entries t, t + 64, t + 128.
+// C = X^T Y (tx) or X Y (!tx) for JN x JN LDS matrices; lane t computes entry t.
 template <bool TX>
 __device__ __forceinline__ void jgemm(const double (*X)[JN + 1], const double (*Y)[JN + 1], double (*C)[JN + 1]) {
-    double r[3];
+    const int t = threadIdx.x, a = t / JN, b = t % JN;
+    double r = 0.0;
 #pragma unroll
-    for (int h = 0; h < 3; h++) {
-        const int e = threadIdx.x + 64 * h;
-        r[h] = 0.0;
-        if (e < JN * JN) {
-            const int a = e / JN, b = e % JN;
-#pragma unroll
-            for (int k = 0; k < JN; k++) r[h] += (TX ? X[k][a] : X[a][k]) * Y[k][b];
-        }
-    }
+    for (int k = 0; k < JN; k++) r += (TX ? X[k][a] : X[a][k]) * Y[k][b];
     __syncthreads();
-#pragma unroll
-    for (int h = 0; h < 3; h++) {
-        const int e = threadIdx.x + 64 * h;
-        if (e < JN * JN) C[e / JN][e % JN] = r[h];
-    }
+    C[a][b] = r;
     __syncthreads();
 }
 __global__ __launch_bounds__(64) void k_ar_schur(const double* __restrict__ part, int nb, double beta,
-                                                 double* __restrict__ out, double* __restrict__ mss) {
+                                                 double* __restrict__ out, double* __restrict__ mss, int lazy) {
+    static_assert(JN * JN == 64, "one lane per entry of the 8 x 8 block");
     __shared__ double A[JN][JN + 1], V[JN][JN + 1], Q0[JN][JN + 1], W[JN][JN + 1];
-    __shared__ double P[NSYM];
-    const int t = threadIdx.x;
+    __shared__ double P[NSYM], m5[25], lam8[JN], delta[JN];
+    __shared__ int skip;
+    const int t = threadIdx.x, a = t / JN, b = t % JN;
     if (t < NSYM) {
-        double a = 0.0;
-#pragma unroll 8
-        for (int b = 0; b < nb; b++) a += part[(size_t)b * NSYM + t];
-        P[t] = a;
+        double acc = 0.0;
+#pragma unroll 16
+        for (int q = 0; q < nb; q++) acc += part[(size_t)q * NSYM + t];
+        P[t] = acc;
     }
     __syncthreads();
-    for (int q = t; q < JN * JN; q += 64) {
-        const int a = q / JN, b = q % JN;
-        double v = 0.0;
-        if (a < 11 && b < 11) {
-            v = mcc_base(a, b);
-            if (a >= 3 && b >= 3) {
-                const int i = min(a, b) - 3, j = max(a, b) - 3;   // upper-triangle index of (i, j)
-                v -= beta * P[i * NR - i * (i - 1) / 2 + (j - i)];
-            }
+    {   // S on rows / columns 3..10
+        const int i = min(a, b), j = max(a, b);   // upper-triangle index of (i, j)
+        A[a][b] = mcc_base(a + 3, b + 3) - beta * P[i * NR - i * (i - 1) / 2 + (j - i)];
+        Q0[a][b] = out[121 + (a + 3) * 11 + (b + 3)];
+    }
+    __syncthreads();
+    if (lazy) {
+        if (t < JN) {   // Gershgorin: row t's disc lies above 0.1
+            double r = 0.0;
+            for (int k = 0; k < JN; k++)
+                if (k != t) r += fabs(A[t][k]);
+            W[0][t] = (A[t][t] - r > 0.1) ? 1.0 : 0.0;
         }
-        A[a][b] = v;   // S
-        Q0[a][b] = (a < 11 && b < 11) ? out[121 + a * 11 + b] : (a == b ? 1.0 : 0.0);
+        __syncthreads();
+        if (t == 0) {
+            int all = 1;
+            for (int k = 0; k < JN; k++) all &= W[0][k] != 0.0;
+            skip = all;
+        }
+        __syncthreads();
+        if (skip) {
+            for (int q = t; q < 121; q += 64) out[q] = mcc_base(q / 11, q % 11);
+            if (t < 25) {
+                const double m = mcc_base(6 + t / 5, 6 + t % 5);
+                if (mss) mss[t] = m;
+                out[256 + t] = out[320 + t];   // the base block's factor, stored at allocation
+            }
+            if (t == 0) out[300] = 1.0;
+            return;
+        }
     }
-    __syncthreads();
     // Newton-Schulz: Q0 <- 1.5 Q0 - 0.5 Q0 (Q0^T Q0)
     jgemm<true>(Q0, Q0, W);    // W = Q0^T Q0
     jgemm<false>(Q0, W, V);    // V = Q0 W
-    for (int q = t; q < JN * JN; q += 64) Q0[q / JN][q % JN] = 1.5 * Q0[q / JN][q % JN] - 0.5 * V[q / JN][q % JN];
+    Q0[a][b] = 1.5 * Q0[a][b] - 0.5 * V[a][b];
     __syncthreads();
     jgemm<false>(A, Q0, W);    // W = S Q0
     jgemm<true>(Q0, W, V);     // V = Q0^T S Q0
-    for (int q = t; q < JN * JN; q += 64) {
-        const int a = q / JN, b = q % JN;
-        A[a][b] = 0.5 * (V[a][b] + V[b][a]);
-    }
+    A[a][b] = 0.5 * (V[a][b] + V[b][a]);
     __syncthreads();
-    for (int q = t; q < JN * JN; q += 64) V[q / JN][q % JN] = (q / JN == q % JN) ? 1.0 : 0.0;
+    V[a][b] = (a == b) ? 1.0 : 0.0;
     __syncthreads();
     for (int sweep = 0; sweep < 60; sweep++) {
-        // off-diagonal and diagonal mass: lane t takes upper entries t and t + 64 (66 in all)
-        double off = 0.0, dia = 0.0;
-        for (int u = t; u < JN * (JN - 1) / 2; u += 64) {
-            int a = 0, rem = u;
-            while (rem >= JN - 1 - a) {
-                rem -= JN - 1 - a;
-                a++;
-            }
-            const double v = A[a][a + 1 + rem];
-            off += v * v;
-        }
-        if (t < JN) dia = A[t][t] * A[t][t];
+        // off-diagonal and diagonal mass (lane t: entry (a, b))
+        const double v = A[a][b];
+        double off = (a < b) ? v * v : 0.0, dia = (a == b) ? v * v : 0.0;
         for (int o = 32; o > 0; o >>= 1) {
             off += __shfl_xor(off, o, 64);
             dia += __shfl_xor(dia, o, 64);
         }
         if (off <= 1e-30 * dia || off == 0.0) break;   // the same value in every lane
         for (int round = 0; round < JN - 1; round++) {
-            // Lane t < 36 owns the 2 x 2 block of A on rows of pair t / 6 and columns of pair t % 6
-            // (B' = J1^T B J2); V's rows are updated pair by pair (72 tasks: row a, pair k);
-            // lanes 0..5 compute the 6 rotations, which the others take by shuffle.
-            const int k1 = min(t / 6, JN / 2 - 1), k2 = t % 6;
+            // Lane t < 16 owns the 2 x 2 block of A on rows of pair t / 4 and columns of pair t % 4
+            // (B' = J1^T B J2); lane t < 32 updates row t / 4 of V on pair t % 4; lanes 0..3 compute
+            // the 4 rotations, which the others take by shuffle.
+            const int k1 = min(t / (JN / 2), JN / 2 - 1), k2 = t % (JN / 2);
             int p1, q1, p2, q2;
             jpair_of(k1, round, p1, q1);
             jpair_of(k2, round, p2, q2);
             double b00 = 0.0, b01 = 0.0, b10 = 0.0, b11 = 0.0;
-            if (t < 36) {
+            if (t < 16) {
                 b00 = A[p1][p2];
                 b01 = A[p1][q2];
                 b10 = A[q1][p2];
                 b11 = A[q1][q2];
             }
-            int va[2], vk[2], vp[2], vq[2];
-            double v0[2], v1[2];
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int task = min(t + 64 * h, 71);
-                va[h] = task / 6;
-                vk[h] = task % 6;
-                jpair_of(vk[h], round, vp[h], vq[h]);
-                v0[h] = V[va[h]][vp[h]];
-                v1[h] = V[va[h]][vq[h]];
-            }
+            const int va = min(t / (JN / 2), JN - 1);   // V task: row va, pair k2
+            const double v0 = V[va][p2], v1 = V[va][q2];
             double c = 1.0, sv = 0.0, tt = 0.0;
             if (t < JN / 2) {
                 int p, q;
                 jpair_of(t, round, p, q);
                 const double apq = A[p][q];
                 if (apq != 0.0) {
-                    const double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
-                    tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                    // t = sgn(d) h / (|d| + sqrt(d^2 + h^2)), h = 2 a_pq, d = a_qq - a_pp: Rutishauser's
+                    // tan of the rotation angle, the smaller root
+                    const double d = A[q][q] - A[p][p], h = 2.0 * apq;
+                    const double rr = sqrt(d * d + h * h);
+                    tt = (d >= 0 ? h : -h) / (fabs(d) + rr);
                     c = 1.0 / sqrt(tt * tt + 1.0);
                     sv = tt * c;
                 }
             }
             const double c1 = __shfl(c, k1, 64), s1 = __shfl(sv, k1, 64), t1 = __shfl(tt, k1, 64);
             const double c2 = __shfl(c, k2, 64), s2 = __shfl(sv, k2, 64);
-            double cvv[2], svv[2];
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                cvv[h] = __shfl(c, vk[h], 64);
-                svv[h] = __shfl(sv, vk[h], 64);
-            }
             __syncthreads();   // every old value read
-            if (t < 36) {
+            if (t < 16) {
                 if (k1 == k2) {
                     // the rotated pair: b00 = a_pp, b01 = a_pq, b11 = a_qq
                     A[p1][p1] = b00 - t1 * b01;
@@ -430,42 +428,41 @@ __global__ __launch_bounds__(64) void k_ar_schur(const double* __restrict__ part
                     A[q1][q2] = s1 * m01 + c1 * m11;
                 }
             }
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                if (t + 64 * h >= 72) break;
-                V[va[h]][vp[h]] = cvv[h] * v0[h] - svv[h] * v1[h];
-                V[va[h]][vq[h]] = svv[h] * v0[h] + cvv[h] * v1[h];
+            if (t < 32) {
+                V[va][p2] = c2 * v0 - s2 * v1;
+                V[va][q2] = s2 * v0 + c2 * v1;
             }
             __syncthreads();
         }
     }
-    jgemm<false>(Q0, V, W);    // Q = Q0 V
-    __shared__ double delta[11];
-    if (t < 11) {
-        const double lam = A[t][t];
-        delta[t] = lam < 0.1 ? 0.1 - lam : 0.0;
-        out[242 + t] = lam + delta[t];
+    jgemm<false>(Q0, V, W);    // Q = Q0 V (the 8 x 8 block)
+    if (t < JN) {
+        const double l = A[t][t];
+        delta[t] = l < 0.1 ? 0.1 - l : 0.0;
+        lam8[t] = l + delta[t];
     }
     __syncthreads();
-    __shared__ double m5[25];
     for (int q = t; q < 121; q += 64) {
-        const int a = q / 11, b = q % 11;
+        const int i = q / 11, j = q % 11;
         double corr = 0.0;
-        for (int j = 0; j < 11; j++)
-            if (delta[j] != 0.0) corr += W[a][j] * delta[j] * W[b][j];
-        const double m = mcc_base(a, b) + corr;
+        if (i >= 3 && j >= 3)
+            for (int k = 0; k < JN; k++)
+                if (delta[k] != 0.0) corr += W[i - 3][k] * delta[k] * W[j - 3][k];
+        const double m = mcc_base(i, j) + corr;
         out[q] = m;
-        out[121 + q] = W[a][b];
-        if (a >= 6 && b >= 6) {
-            m5[(a - 6) * 5 + (b - 6)] = m;
-            if (mss) mss[(a - 6) * 5 + (b - 6)] = m;
+        out[121 + q] = (i >= 3 && j >= 3) ? W[i - 3][j - 3] : (i == j ? 1.0 : 0.0);
+        if (i >= 6 && j >= 6) {
+            m5[(i - 6) * 5 + (j - 6)] = m;
+            if (mss) mss[(i - 6) * 5 + (j - 6)] = m;
         }
     }
+    if (t < 11) out[242 + t] = t < 3 ? 1.0 : lam8[t - 3];
     __syncthreads();
     if (t == 0) {   // the block solver's factor of the stresslet block, at out + 256
         double L[25];
         mss_factor(m5, L);
         for (int q = 0; q < 25; q++) out[256 + q] = L[q];
+        out[300] = 0.0;
     }
 }
 
@@ -481,27 +478,30 @@ __global__ __launch_bounds__(AB) void k_ar_apply(int64_t nm3, int64_t n, int64_t
                                                  const double* __restrict__ x, const double* __restrict__ xr,
                                                  double* __restrict__ y, double* __restrict__ part,
                                                  int* __restrict__ counter, int saddle) {
-    const int64_t i = (int64_t)blockIdx.x * AB + threadIdx.x;
     double p[NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) p[r] = 0.0;
-    if (i < nm3) {
-        const double xi = x[i];
-        double acc = 0.0;
-        if (xr) {
 #pragma unroll
-            for (int r = 0; r < NR; r++) {
-                const double v = mcm[r * nm3 + i];
-                acc += v * xr[r];
-                p[r] = v * xi;
+    for (int d = 0; d < ARD; d++) {
+        const int64_t i = ((int64_t)blockIdx.x * ARD + d) * AB + threadIdx.x;
+        if (i < nm3) {
+            const double xi = x[i];
+            double acc = 0.0;
+            if (xr) {
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    const double v = mcm[r * nm3 + i];
+                    acc += v * xr[r];
+                    p[r] += v * xi;
+                }
             }
+            double yi = acc + binv * xi;
+            if (saddle) {
+                yi -= x[n + i];
+                y[n + i] = -xi;
+            }
+            y[i] = yi;
         }
-        double yi = acc + binv * xi;
-        if (saddle) {
-            yi -= x[n + i];
-            y[n + i] = -xi;
-        }
-        y[i] = yi;
     }
     if (!xr) return;
     block_reduce_store<NR>(p, part);
@@ -583,15 +583,18 @@ __global__ __launch_bounds__(AB) void k_ar_bs1(int64_t nm3, int64_t n, const dou
                                                double* __restrict__ x,
                                                double* __restrict__ uout, double* __restrict__ part,
                                                int* __restrict__ counter) {
-    const int64_t i = (int64_t)blockIdx.x * AB + threadIdx.x;
     double p[NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) p[r] = 0.0;
-    if (i < nm3) {
-        const double f = -b[n + i];
-        x[i] = f;
 #pragma unroll
-        for (int r = 0; r < NR; r++) p[r] = mcm[r * nm3 + i] * f;
+    for (int d = 0; d < ARD; d++) {
+        const int64_t i = ((int64_t)blockIdx.x * ARD + d) * AB + threadIdx.x;
+        if (i < nm3) {
+            const double f = -b[n + i];
+            x[i] = f;
+#pragma unroll
+            for (int r = 0; r < NR; r++) p[r] += mcm[r * nm3 + i] * f;
+        }
     }
     block_reduce_store<NR>(p, part);
     if (!last_block(counter)) return;
@@ -646,15 +649,18 @@ __global__ __launch_bounds__(AB) void k_ar_noise(int64_t nm3, double sq_binv, do
                                                  const double* __restrict__ mcm, const double* __restrict__ mcc_q_lam,
                                                  const double* __restrict__ xi, double* __restrict__ out,
                                                  double* __restrict__ part, int* __restrict__ counter, int colloid) {
-    const int64_t i = (int64_t)blockIdx.x * AB + threadIdx.x;
     double p[NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) p[r] = 0.0;
-    if (i < nm3) {
-        const double v = xi[i];
 #pragma unroll
-        for (int r = 0; r < NR; r++) p[r] = mcm[r * nm3 + i] * v;
-        out[i] = sq_binv * v;
+    for (int d = 0; d < ARD; d++) {
+        const int64_t i = ((int64_t)blockIdx.x * ARD + d) * AB + threadIdx.x;
+        if (i < nm3) {
+            const double v = xi[i];
+#pragma unroll
+            for (int r = 0; r < NR; r++) p[r] += mcm[r * nm3 + i] * v;
+            out[i] = sq_binv * v;
+        }
     }
     if (!colloid) return;
     block_reduce_store<NR>(p, part);
@@ -849,6 +855,7 @@ ps_err arrow_alloc(Ctx* c) {
         for (int a = 0; a < 5; a++)
             for (int b = 0; b < 5; b++) m5[a * 5 + b] = h[(6 + a) * 11 + 6 + b];
         mss_factor(m5, h + 256);
+        mss_factor(m5, h + 320);   // kept: the factor of the base block, for lazy syncs
     }
     PS_HIP(c, hipMemcpy(c->ar_mcc, h, sizeof(double) * 512, hipMemcpyHostToDevice));
     if (c->Mss) {
@@ -913,31 +920,45 @@ ps_err arrow_mob(Ctx* c) {
     return PS_OK;
 }
 
-ps_err arrow_schur(Ctx* c) {
+// The Schur floor of the current M^cm.  `lazy`: the caller reads only the floored M^cc before the
+// next sync (k_ar_schur may then skip the eigensolver and flag Q as left over; ps_get_schur redoes
+// the sync from the same Gram partials).
+ps_err arrow_schur(Ctx* c, int lazy) {
     if (c->Nc != 1) return PS_OK;
     hipStream_t st = c->stream;
     const int64_t nm3 = c->sc.nm3;
     if (c->ar_gram_ready) {   // the Gram partials of the current M^cm, from arrow_mob
-        k_ar_schur<<<1, 64, 0, st>>>(c->ar_gpart, (int)dof_blocks(c->Np), c->p.beta, c->ar_mcc, c->Mss);
-        PS_LAUNCH(c);
+        c->ar_last_part = c->ar_gpart;
+        c->ar_last_nb = (int)dof_blocks(c->Np);
     } else {
         const int nb = std::min(c->ar_nb, 64);   // grid-stride: few partials for the one-block finish
         if (nm3 > 0) {
             k_ar_gram<<<nb, AB, 0, st>>>(nm3, c->ar_mcm, c->ar_part);
             PS_LAUNCH(c);
         }
-        k_ar_schur<<<1, 64, 0, st>>>(c->ar_part, nm3 > 0 ? nb : 0, c->p.beta, c->ar_mcc, c->Mss);
-        PS_LAUNCH(c);
+        c->ar_last_part = c->ar_part;
+        c->ar_last_nb = nm3 > 0 ? nb : 0;
+        lazy = 0;   // ar_part is reused by the applies: no later redo from it
     }
+    k_ar_schur<<<1, 64, 0, st>>>(c->ar_last_part, c->ar_last_nb, c->p.beta, c->ar_mcc, c->Mss, lazy);
+    PS_LAUNCH(c);
     c->ar_gram_ready = false;
     c->ar_synced = true;
+    return PS_OK;
+}
+
+// The last Schur floor again, in full, from its Gram partials (after a lazy one skipped Q).
+ps_err arrow_schur_redo(Ctx* c) {
+    if (!c->ar_last_part) return PS_OK;
+    k_ar_schur<<<1, 64, 0, c->stream>>>(c->ar_last_part, c->ar_last_nb, c->p.beta, c->ar_mcc, c->Mss, 0);
+    PS_LAUNCH(c);
     return PS_OK;
 }
 
 ps_err arrow_apply(Ctx* c, const double* x, double* y, int saddle) {
     hipStream_t st = c->stream;
     const int64_t nm3 = c->sc.nm3, n = c->n, nu = c->nu;
-    const unsigned nb = dof_blocks(nm3);
+    const unsigned nb = red_blocks(nm3);
     if (nm3 > 0) {
         // timing (ps_profile_begin/end): the launch records its own start/stop events at dispatch, or
         // is bracketed by event records under rocprofv3
@@ -964,18 +985,24 @@ ps_err arrow_apply(Ctx* c, const double* x, double* y, int saddle) {
 ps_err arrow_block_solve(Ctx* c, const double* b, double* x, double* uout) {
     hipStream_t st = c->stream;
     const int64_t nm3 = c->sc.nm3, n = c->n;
-    const unsigned nb = dof_blocks(nm3);
+    const unsigned nb1 = red_blocks(nm3), nb2 = dof_blocks(nm3);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->prof_on) PS_TRY(c->prof_record(4, &e0, &e1));
     const bool dispatch_ev = c->prof_on && c->prof_dispatch_events;
     if (c->prof_on && !dispatch_ev) PS_HIP(c, hipEventRecord(e0, st));
-    hipExtLaunchKernelGGL(k_ar_bs1, dim3(nb), dim3(AB), 0, st, dispatch_ev ? e0 : nullptr, nullptr, 0, nm3, n,
-                          (const double*)c->ar_mcm, (const double*)c->ar_mcc, b, x, uout, c->ar_part,
-                          c->ar_counter);
-    PS_LAUNCH(c);
-    hipExtLaunchKernelGGL(k_ar_bs2, dim3(nb), dim3(AB), 0, st, nullptr, dispatch_ev ? e1 : nullptr, 0, nm3, n,
-                          1.0 / c->p.beta, (const double*)c->ar_mcm, b, x, uout);
-    PS_LAUNCH(c);
+    if (dispatch_ev) {
+        hipExtLaunchKernelGGL(k_ar_bs1, dim3(nb1), dim3(AB), 0, st, e0, nullptr, 0, nm3, n, (const double*)c->ar_mcm,
+                              (const double*)c->ar_mcc, b, x, uout, c->ar_part, c->ar_counter);
+        PS_LAUNCH(c);
+        hipExtLaunchKernelGGL(k_ar_bs2, dim3(nb2), dim3(AB), 0, st, nullptr, e1, 0, nm3, n, 1.0 / c->p.beta,
+                              (const double*)c->ar_mcm, b, x, uout);
+        PS_LAUNCH(c);
+    } else {   // plain launches (also what a step graph captures)
+        k_ar_bs1<<<nb1, AB, 0, st>>>(nm3, n, c->ar_mcm, c->ar_mcc, b, x, uout, c->ar_part, c->ar_counter);
+        PS_LAUNCH(c);
+        k_ar_bs2<<<nb2, AB, 0, st>>>(nm3, n, 1.0 / c->p.beta, c->ar_mcm, b, x, uout);
+        PS_LAUNCH(c);
+    }
     if (c->prof_on && !dispatch_ev) PS_HIP(c, hipEventRecord(e1, st));
     c->cnt.applies += 2;
     return PS_OK;
@@ -990,8 +1017,8 @@ ps_err arrow_mss_solve(Ctx* c, const double* b, const double* y, double* x) {
 ps_err arrow_noise(Ctx* c, const double* xi, double* out) {
     hipStream_t st = c->stream;
     const int64_t nm3 = c->sc.nm3;
-    if (c->Nc == 1 && !c->ar_synced) PS_TRY(arrow_schur(c));
-    k_ar_noise<<<dof_blocks(nm3), AB, 0, st>>>(nm3, std::sqrt(1.0 / c->p.beta), std::sqrt(c->p.beta), c->ar_mcm,
+    if (c->Nc == 1 && !c->ar_synced) PS_TRY(arrow_schur(c, 0));
+    k_ar_noise<<<red_blocks(nm3), AB, 0, st>>>(nm3, std::sqrt(1.0 / c->p.beta), std::sqrt(c->p.beta), c->ar_mcm,
                                                c->ar_mcc, xi, out, c->ar_part, c->ar_counter + 1, c->Nc == 1);
     PS_LAUNCH(c);
     c->cnt.sqrt_calls++;

@@ -190,8 +190,9 @@ ps_err solve(Ctx* c, double* X, int warm, double* uout = nullptr) {
 }
 
 // One iteration of the run() loop; `pipe` supplies this step's noise (step index s), or is null
-// when kT <= 0.
-ps_err one_step(Ctx* c, NoisePipe* pipe, int s) {
+// when kT <= 0.  With `noise_fixed` (graph capture) the noise is read from that pinned slot and the
+// pipe is not touched.
+ps_err one_step(Ctx* c, NoisePipe* pipe, int s, double* noise_fixed = nullptr) {
     const int64_t n = c->n, nu = c->nu, n3 = 3 * (int64_t)c->Np, nc5 = 5 * (int64_t)c->Nc;
     hipStream_t st = c->stream;
     const double dt = c->p.dt, kT = c->p.kT, eps = 1.0e-3;
@@ -202,11 +203,11 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s) {
     PS_TRY(rhs_dev(c, 0));
     c->ar_gram_next = c->arrow;
     PS_TRY(mob_prepare(c));
-    if (c->arrow) PS_TRY(arrow_schur(c));   // sync_mcc_schur_correction (src/run.cpp:92)
+    if (c->arrow) PS_TRY(arrow_schur(c, 0));   // sync_mcc_schur_correction (src/run.cpp:92)
     PS_TRY(solve(c, c->Xdet, 0, c->v_det_n));
     if (kT > 0) {
         // solve_slip_vel: W = sqrt(M) xi, rhs[0:n] += sqrt(2kT/dt) W
-        double* noise = pipe->take(s);
+        double* noise = noise_fixed ? noise_fixed : pipe->take(s);
         PS_HIP(c, hipMemcpyAsync(c->xi, noise, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, st));
         PS_TRY(sqrt_apply_dev(c, c->xi, c->W, nullptr));
         k_add_slip<<<eb(n), TB, 0, st>>>(c->rhs, c->W, std::sqrt(2 * kT / dt), n);
@@ -221,7 +222,7 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s) {
         PS_LAUNCH(c);
         // drift(): RFD probes at x +/- eps W
         PS_HIP(c, hipMemcpyAsync(c->fext, noise + n, sizeof(double) * (size_t)nu, hipMemcpyHostToDevice, st));
-        PS_TRY(pipe->release(s));
+        if (!noise_fixed) PS_TRY(pipe->release(s));
         k_probe<<<eb(n3), TB, 0, st>>>(c->x, c->fext, eps, n3);
         PS_LAUNCH(c);
         PS_TRY(check_dist_dev(c));
@@ -259,7 +260,8 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s) {
     PS_TRY(rhs_dev(c, 0));
     c->ar_gram_next = c->arrow;
     PS_TRY(mob_prepare(c));
-    if (c->arrow) PS_TRY(arrow_schur(c));   // src/run.cpp:129 (the drift's mob() calls take none)
+    // src/run.cpp:129 (the drift's mob() calls take none); lazy: only the corrector's solve reads it
+    if (c->arrow) PS_TRY(arrow_schur(c, 1));
     PS_TRY(solve(c, c->Xdet, 0, c->v_det_c));
     k_trapz<<<eb(nu), TB, 0, st>>>(c->up, c->v_det_n, c->v_det_c, c->v_brown, nu);
     PS_LAUNCH(c);
@@ -281,6 +283,77 @@ ps_err fail(Ctx* c, ps_err code, const std::string& msg) {
     if (c) c->set_error(msg);
     return code;
 }
+
+// A step with no host round trip can be replayed: the arrowhead regime with block solves, where every
+// launch's arguments are fixed for the run (no Krylov polling, no WCA grid from the host).
+bool step_graphable(const Ctx* c) {
+    return c->p.graph && c->arrow && c->Nc == 1 && c->Nm > 0 && c->p.solver == PS_SOLVER_BLOCK && c->p.kT > 0 &&
+           !c->p.mono_ev && !c->p.record_forces && !c->prof_on;
+}
+
+void step_graphs_reset(Ctx* c) {
+    for (auto& g : c->step_graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+// The step as a HIP graph reading its noise from pinned slot `slot` (captured, not run): the host's
+// t and counters are restored, and the counter increments of one step kept for the replays.  The
+// launch sequence of a step does not depend on its index, only on the slot's address.  Captured
+// before the noise producer starts, so no other thread makes HIP calls during a capture.
+ps_err capture_step(Ctx* c, int k, double* slot) {
+    const double t0 = c->t;
+    const ps_counters cnt0 = c->cnt;
+    PS_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const ps_err r = one_step(c, nullptr, k, slot);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->t = t0;
+    if (r != PS_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        c->cnt = cnt0;
+        return r;
+    }
+    if (e != hipSuccess) return fail(c, PS_ERR_HIP, std::string("step capture: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&c->step_graph[k], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        c->step_graph[k] = nullptr;
+        c->cnt = cnt0;
+        return fail(c, PS_ERR_HIP, std::string("step graph: ") + hipGetErrorString(ei));
+    }
+    const int64_t* a = &c->cnt.mob_calls;
+    const int64_t* b = &cnt0.mob_calls;
+    int64_t* d = &c->step_graph_delta.mob_calls;
+    for (size_t q = 0; q < sizeof(ps_counters) / sizeof(int64_t); q++) d[q] = a[q] - b[q];
+    c->cnt = cnt0;
+    return PS_OK;
+}
+
+// One step by replaying the graph of its noise slot; false in *eager when the step had to run
+// eagerly: a step that weakens the trap (ktrap is a kernel argument), after which the run continues
+// without graphs.
+ps_err graph_step(Ctx* c, NoisePipe* pipe, int s, bool* eager) {
+    const int k = s & 1;
+    const bool trap_on = !(std::fabs(c->ktrap_cur) < 1e-12);
+    if (trap_on && c->p.weaken_trap >= 0 && c->ktrap_cur != c->p.weaken_trap &&
+        c->t + c->p.dt > c->p.tstart + c->p.trun) {
+        *eager = true;
+        return one_step(c, pipe, s);
+    }
+    (void)pipe->take(s);   // the slot is filled (the graph reads it)
+    PS_HIP(c, hipGraphLaunch(c->step_graph[k], c->stream));
+    PS_TRY(pipe->release(s));
+    c->t += c->p.dt;
+    int64_t* a = &c->cnt.mob_calls;
+    const int64_t* d = &c->step_graph_delta.mob_calls;
+    for (size_t q = 0; q < sizeof(ps_counters) / sizeof(int64_t); q++)
+        if (&a[q] != &c->cnt.nvlist) a[q] += d[q];
+    return PS_OK;
+}
+
 
 template <class F>
 ps_err with_io(Ctx* c, int64_t na, const double* in, int64_t nb, double* out, F&& fn) {
@@ -313,6 +386,7 @@ void destroy_ctx(Ctx* c) {
     if (c->noise_h) (void)hipHostFree(c->noise_h);
     for (hipEvent_t e : c->noise_ev)
         if (e) (void)hipEventDestroy(e);
+    step_graphs_reset(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -415,6 +489,7 @@ void ps_params_default(ps_params* p) {
     p->ir_maxit = 20;
     p->solver = PS_SOLVER_MINRES;
     p->aa_kernel = PS_AA_AUTO;
+    p->graph = 1;
 }
 
 const char* ps_last_error(const void* ctx) {
@@ -681,15 +756,20 @@ ps_err ps_sync_mcc(void* h) {
     if (!c) return PS_ERR_ARG;
     if (!c->arrow) return fail(c, PS_ERR_STATE, "sync_mcc: the mm_HI=False regime only");
     if (!c->have_mob) return fail(c, PS_ERR_STATE, "sync_mcc before mob");
-    return arrow_schur(c);
+    return arrow_schur(c, 0);
 }
 ps_err ps_get_schur(void* h, double* mcc, double* q, double* lam) {
     Ctx* c = as_ctx(h);
     if (!c) return PS_ERR_ARG;
     if (!c->arrow || c->Nc != 1) return fail(c, PS_ERR_STATE, "get_schur: mm_HI=False with one colloid only");
-    double buf[253];
+    double buf[301];
     PS_HIP(c, hipMemcpyAsync(buf, c->ar_mcc, sizeof(buf), hipMemcpyDeviceToHost, c->stream));
     PS_HIP(c, hipStreamSynchronize(c->stream));
+    if (buf[300] != 0.0) {   // a lazy sync left Q and the spectrum of an earlier one: redo it in full
+        PS_TRY(arrow_schur_redo(c));
+        PS_HIP(c, hipMemcpyAsync(buf, c->ar_mcc, sizeof(buf), hipMemcpyDeviceToHost, c->stream));
+        PS_HIP(c, hipStreamSynchronize(c->stream));
+    }
     if (mcc) std::copy(buf, buf + 121, mcc);
     if (q) std::copy(buf + 121, buf + 242, q);
     if (lam) std::copy(buf + 242, buf + 253, lam);
@@ -844,11 +924,25 @@ ps_err ps_run(void* h, int nsteps) {
         pipe->slot[1] = c->noise_h + pipe->len;
         pipe->done[0] = c->noise_ev[0];
         pipe->done[1] = c->noise_ev[1];
-        pipe->th = std::thread([p = pipe.get()] { p->produce(); });
     }
     ps_err rc = PS_OK;
-    for (int i = 0; i < nsteps && rc == PS_OK; i++) rc = one_step(c, pipe.get(), i);
+    bool graph = pipe && step_graphable(c) && nsteps >= 3;   // two captures pay off from a few steps
+    if (graph) {   // both slots' graphs, before the producer thread makes any HIP call
+        rc = capture_step(c, 0, pipe->slot[0]);
+        if (rc == PS_OK) rc = capture_step(c, 1, pipe->slot[1]);
+        if (rc != PS_OK) {
+            step_graphs_reset(c);
+            return rc;
+        }
+    }
+    if (pipe) pipe->th = std::thread([p = pipe.get()] { p->produce(); });
+    for (int i = 0; i < nsteps && rc == PS_OK; i++) {
+        bool eager = false;
+        rc = graph ? graph_step(c, pipe.get(), i, &eager) : one_step(c, pipe.get(), i);
+        if (eager) graph = false;
+    }
     if (pipe) pipe->stop();
+    step_graphs_reset(c);   // the graphs read this run's noise slots
     if (rc != PS_OK) return rc;
     PS_HIP(c, hipStreamSynchronize(c->stream));
     return PS_OK;

@@ -327,6 +327,12 @@ __global__ __launch_bounds__(TB) void k_forces(ForceConst fc, const double* __re
                                                const int* __restrict__ bond_j, int mono_blocks,
                                                double* __restrict__ rhs) {
     const SysConst& s = fc.s;
+    {   // the rows this launch does not compute: rhs[0:n] and rhs[n + 3Np:] are zero
+        const int64_t n3 = 3 * (int64_t)s.Np, nclear = s.ns - n3;
+        const int64_t stride = (int64_t)gridDim.x * TB;
+        for (int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x; i < nclear; i += stride)
+            rhs[i < s.n ? i : i + n3] = 0.0;
+    }
     double fx = 0.0, fy = 0.0, fz = 0.0;
     int p;
     if ((int)blockIdx.x < mono_blocks) {
@@ -352,11 +358,6 @@ __global__ __launch_bounds__(TB) void k_forces(ForceConst fc, const double* __re
     rhs[s.n + 3 * (int64_t)p + 2] = fz;
 }
 
-__global__ void k_rhs_clear(SysConst s, double* __restrict__ rhs) {
-    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
-    if (i >= s.ns) return;
-    if (i < s.n || i >= s.n + 3 * (int64_t)s.Np) rhs[i] = 0.0;
-}
 __global__ void k_rhs_drift(SysConst s, const double* __restrict__ fext, double* __restrict__ rhs) {
     const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
     if (i >= s.ns) return;
@@ -549,8 +550,6 @@ ps_err rhs_dev(Ctx* c, int drift) {
         PS_LAUNCH(c);
         return PS_OK;
     }
-    k_rhs_clear<<<ge, TB, 0, st>>>(s, c->rhs);
-    PS_LAUNCH(c);
     ForceConst fc{};
     fc.s = s;
     const float sg[3] = {(float)(2. * c->p.beta), (float)(1. + c->p.beta), (float)2.0};
@@ -573,10 +572,9 @@ ps_err rhs_dev(Ctx* c, int drift) {
     const int* bk = c->bonds + Np + 1;
     const int* bj = c->bonds + Np + 1 + nb;
     const int mono_blocks = (c->Nm + TB - 1) / TB, coll_blocks = (c->Nc + FW - 1) / FW;
-    if (mono_blocks + coll_blocks > 0) {
-        k_forces<<<(unsigned)(mono_blocks + coll_blocks), TB, 0, st>>>(fc, c->xpd, c->x, c->xpos0, c->nb_off,
-                                                                       c->nb_part, boff, bk, bj, mono_blocks, c->rhs);
-    }
+    k_forces<<<(unsigned)std::max(1, mono_blocks + coll_blocks), TB, 0, st>>>(fc, c->xpd, c->x, c->xpos0, c->nb_off,
+                                                                             c->nb_part, boff, bk, bj, mono_blocks,
+                                                                             c->rhs);
     PS_LAUNCH(c);
     // mm_HI off: monomer-monomer excluded volume through the monomer cell list (src/rhs.cpp:200-205)
     if (c->arrow && c->p.mono_ev) PS_TRY(arrow_wca(c));

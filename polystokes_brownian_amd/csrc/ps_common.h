@@ -170,6 +170,10 @@ struct Ctx {
     double* ar_gpart = nullptr;     // per-block Gram partials of the last M^cm (for the Schur floor)
     bool ar_gram_next = false;      // the next mob() is followed by a Schur floor: store its Gram partials
     bool ar_gram_ready = false;     // ar_gpart holds the current M^cm's Gram partials
+    hipGraphExec_t step_graph[2] = {nullptr, nullptr};   // ps_run's step replays, one per noise slot
+    ps_counters step_graph_delta{};                      // counter increments of one step
+    const double* ar_last_part = nullptr;   // the Gram partials of the last Schur floor, and their count
+    int ar_last_nb = 0;
     double* ar_mcc = nullptr;       // floored M^cc (121), eigenvectors Q (121), floored spectrum (11);
                                     // at 256: Cholesky factor of its stresslet block (25, 1/diag)
     double* ar_bbox = nullptr;      // monomer bounding box (unbounded WCA grid)
@@ -251,7 +255,8 @@ ps_err arrow_count_pairs(Ctx* c);                                  // host copy 
 ps_err arrow_block_solve(Ctx* c, const double* b, double* x, double* uout);
 ps_err arrow_mss_solve(Ctx* c, const double* b, const double* y, double* x);   // 5 x 5 stresslet system
 ps_err arrow_mob(Ctx* c);                                          // snapshot + M^cm
-ps_err arrow_schur(Ctx* c);                                        // sync_mcc_schur_correction
+ps_err arrow_schur(Ctx* c, int lazy);                              // sync_mcc_schur_correction
+ps_err arrow_schur_redo(Ctx* c);                                   // the last one again, in full
 ps_err arrow_apply(Ctx* c, const double* x, double* y, int saddle);
 ps_err arrow_noise(Ctx* c, const double* xi, double* out);         // block-Cholesky slip noise
 ps_err arrow_diag(Ctx* c, double* d, int inverse);

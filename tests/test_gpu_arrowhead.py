@@ -170,4 +170,41 @@ def test_arrow_steps(name, mono_ev, solver):
     # udiff is the raw difference of the two RFD solves (before the kT / (2 eps) scaling): with MINRES
     # its relative error is the solves' 1e-12 residual amplified by 1 / eps
     assert rel(vg["udiff"], vo["udiff"]) < (tol if solver == 1 else 1e-8)
+    # the corrector's Schur floor: where no eigenvalue is floored the library skips its eigensolver
+    # during run() and ps_get_schur redoes it from the same Gram partials
+    mg, qg, lg = g.schur()
+    mo, qo, lo = o.schur()
+    assert rel(mg, mo) < tol
+    assert rel(np.sort(lg), np.sort(lo)) < tol
+    np.testing.assert_allclose(qg @ np.diag(np.sqrt(lg)) @ qg.T, qo @ np.diag(np.sqrt(lo)) @ qo.T,
+                               atol=tol * np.sqrt(lo.max()))
     g.close()
+
+
+@pytest.mark.parametrize("weaken", [False, True])
+def test_arrow_step_graph_replays_the_eager_step(weaken):
+    """ps_run replays the arrowhead step (block solves) as a HIP graph per noise slot: 7 steps give each
+    slot's graph several replays, and the result equals launching every kernel, bit for bit, with the
+    same counters.  With `weaken`, the trap weakens at step 3: ktrap is a kernel argument, so that step
+    and the rest of the run launch their kernels eagerly."""
+    over = dict(solver=1, ksp_rtol=1e-12)
+    if weaken:
+        over.update(tstart=0.0, trun=3.5 * configs.workload("arrow_thermal")["dt"], weaken_trap=0.25)
+    ga, o, x0, pd = arrow_pair(graph=1, **CASES["thermal"], **over)
+    gb = Context(dict(pd, graph=0))
+    gb.initial_configuration(x0)
+    ga.run(7)
+    gb.run(7)
+    xa, qa, ta = ga.state()
+    xb, qb, tb = gb.state()
+    assert ta == tb
+    np.testing.assert_array_equal(xa, xb)
+    np.testing.assert_array_equal(qa, qb)
+    for k in ("up", "v_det_n", "v_det_c", "v_brown", "udiff"):
+        np.testing.assert_array_equal(ga.vectors()[k], gb.vectors()[k])
+    assert ga.counters() == gb.counters()
+    # and against the oracle
+    o.run(7)
+    assert rel(xa, o.state()[0]) < 1e-10
+    ga.close()
+    gb.close()
