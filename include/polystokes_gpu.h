@@ -276,6 +276,38 @@ ps_err ps_draw_normals(void* ctx, double* out, int64_t n);
  * default; acceptance is pi/4), so a small value makes each fill take several further ranges. */
 ps_err ps_normal_stream(uint64_t seed, double* out, int64_t n, int64_t chunk, double margin);
 
+/* Batched replicas (an MI355X form with no reference counterpart): R independent systems, each the
+ * reference's simulation of its own probes and chains (unbounded, full monomer hydrodynamics) with
+ * its own noise seed, advanced together by one persistent kernel per step (one workgroup per
+ * replica; the step of src/run.cpp:70-162 with block-eliminated saddle solves and a Lanczos square
+ * root of sqrt_kmax iterations).  Replica r has Nc[r] colloids and Npoly[r] chains of Nmono[r]
+ * monomers (monomers first, chain by chain, then the colloids, as in a single context); the shared
+ * parameters come from *p (mm_HI = 1 and no box required; precision PS_PREC_MIXED runs the square
+ * root's monomer tiles in FP32).  Each replica's noise is its own std::mt19937_64 +
+ * normal_distribution stream seeded seeds[r] + llround(restart / dt), drawn in a single context's
+ * order, so every replica reproduces a single-context run. */
+ps_err ps_batch_create(const ps_params* p, int R, const int* Nc, const int* Npoly, const int* Nmono,
+                       const uint64_t* seeds, void** out_batch);
+ps_err ps_batch_destroy(void* batch);
+const char* ps_batch_last_error(const void* batch);
+/* out6 = {Np, Nc, Nm, n, nu, ns} of replica r. */
+ps_err ps_batch_sizes(void* batch, int r, int64_t* out6);
+/* Initial configuration of replica r: 3Np positions, 4Nc quaternions (null: identity); the trap
+ * centres are the colloids' positions on a fresh start. */
+ps_err ps_batch_set_state(void* batch, int r, const double* x0, const double* q0);
+ps_err ps_batch_get_state(void* batch, int r, double* x, double* q, double* t);
+ps_err ps_batch_get_vectors(void* batch, int r, double* up, double* v_det_n, double* v_det_c, double* v_brown,
+                            double* udiff);
+ps_err ps_batch_run(void* batch, int nsteps);
+/* Per replica (4 R ints): stresslet MINRES iterations of the last step, Lanczos iterations,
+ * negative Ritz values floored, 0. */
+ps_err ps_batch_stats(void* batch, int* out);
+/* One operation on every replica at its current positions, vectors concatenated over the
+ * replicas: 0 / 1: y = M f (FP64 / FP32 monomer tiles; n each), 2: the saddle solve (ns each),
+ * 3: W = M^{1/2} xi (n each), 4: mob() (no vectors; run it before 0-3), 5: the force rhs (ns
+ * each, no input).  iters (R, may be null): op 2 MINRES, op 3 Lanczos iterations. */
+ps_err ps_batch_op(void* batch, int op, const double* in, double* out, int* iters);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,8 +30,11 @@ def _wrap(x, L):
     return x - L * np.rint(x / L)
 
 
-def make_configuration(Nc, Npoly, Nmono, L, r0, seed, probe_positions=None, beta=BETA, max_tries=2000):
-    """Positions (3*Np,) for Nm = Npoly*Nmono monomers followed by Nc probes, wrapped in [-L/2, L/2)."""
+def make_configuration(Nc, Npoly, Nmono, L, r0, seed, probe_positions=None, beta=BETA, max_tries=2000,
+                       unwrap=False):
+    """Positions (3*Np,) for Nm = Npoly*Nmono monomers followed by Nc probes, wrapped in [-L/2, L/2).
+    With `unwrap`, chains are returned as the continuous walks they were grown as (for unbounded
+    systems): the overlap tests used minimum images, and unbounded distances are never shorter."""
     rng = np.random.default_rng(seed)
     Nm = Npoly * Nmono
     x = np.zeros((Nm + Nc, 3))
@@ -109,6 +112,8 @@ def make_configuration(Nc, Npoly, Nmono, L, r0, seed, probe_positions=None, beta
                 remove(i)
         else:
             raise RuntimeError("could not place chain %d" % c)
+    if unwrap:
+        return x.reshape(-1).copy()
     return _wrap(x, L).reshape(-1)
 
 
@@ -250,3 +255,21 @@ def replica_sizes(R=256, seed=1000, lo=100, hi=3000):
         Npoly = max(1, (N - Nc) // 20)
         out.append((Nc, Npoly, 20))
     return out
+
+
+def replica_workload(R=256, seed=1000, lo=100, hi=3000, sizes=None):
+    """The batched-replica workload: R unbounded systems with the C3 parameters (harmonic bonds, full
+    monomer hydrodynamics, no chain_HI), sizes from replica_sizes, each at C3's bead density (box
+    side 320a (N_r / 20 000)^(1/3) for the placement), positions and noise seeded seed + r.  Lanczos
+    runs 32 iterations.  Returns (params dict, list of replica dicts)."""
+    w = workload("c3")
+    w.update(L=None, sqrt_kmax=32, sqrt_rtol=0.0)
+    params = params_dict(w)
+    reps = []
+    for r, (Nc, Npoly, Nmono) in enumerate(sizes if sizes is not None else replica_sizes(R, seed, lo, hi)):
+        N = Nc + Npoly * Nmono
+        L = 320 * LSCALE * (N / 20000.0) ** (1.0 / 3.0)
+        x0 = make_configuration(Nc, Npoly, Nmono, L, w["r0"], seed + r, unwrap=True)
+        reps.append(dict(Nc=Nc, Npoly=Npoly, Nmono=Nmono, seed=seed + r, x0=x0))
+    return params, reps
+

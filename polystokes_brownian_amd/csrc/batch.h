@@ -1,0 +1,49 @@
+// batch.h -- layout shared by the batched-replica kernels (batch.hip) and their host side.
+#pragma once
+#include <cstdint>
+
+#include "../../include/polystokes_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+constexpr int BATCH_THREADS = 1024;   // one workgroup per replica
+constexpr int BATCH_KMAX = 48;        // Lanczos iterations held on chip (LDS tridiagonal and U)
+
+// Per-replica arrays in the FP64 pool (BO_NZ indexes the noise buffer instead).
+enum BatchOff {
+    BO_X, BO_XN, BO_XPD, BO_XM,         // positions, x_n, check_dist snapshot, mob() snapshot (3Np)
+    BO_Q, BO_XP0,                       // quaternions (4Nc), trap centres (3Nc)
+    BO_RHS, BO_XS, BO_XDET, BO_XD,      // saddle vectors (ns)
+    BO_UP, BO_VDN, BO_VDC, BO_VB, BO_UD, BO_DV,   // velocities (nu): up, v_det_n, v_det_c, v_brown, udiff, drift
+    BO_W, BO_Y, BO_LW,                  // n: square-root output, apply output, Lanczos work
+    BO_QB,                              // Lanczos basis (kmax n)
+    BO_SB,                              // Brownian stresslet (5Nc)
+    BO_M5,                              // stresslet MINRES work (9 x 5Nc)
+    BO_MSS,                             // dense stresslet block (25 Nc^2)
+    BO_ST,                              // carried tile (18)
+    BO_NZ,                              // this step's noise (n + nu), in the noise buffer
+    BO_NOFF
+};
+
+struct BRep {
+    int Np, Nm, Nc, Nmono;   // monomers in Nm / Nmono chains of Nmono consecutive beads, then colloids
+    int si, sj;              // mob()'s last monomer-monomer pair (the carried tile), -1 without one
+    int64_t n, nu, ns, nm3, nc3;
+    int64_t off[BO_NOFF];
+};
+
+struct BConst {
+    double beta, beta2, beta_inv, two_beta, dt, kT;
+    double kbond, r0, Lmax, ktrap, k_hs, hs_exp_k;
+    double sig[3], rverl[3];
+    int fene, force_sel, mono_ev, chain_HI, trap_on;
+    int kmax;      // Lanczos iterations (an invariant subspace ends it earlier)
+    int mixed;     // FP32 monomer tiles in the square root's applies
+    double ms_rtol;
+    int ms_maxit;
+};
+
+ps_err batch_launch_step(hipStream_t st, const BConst& B, const BRep* reps, const int* order, int R, double* pool,
+                         const double* noise, double ktrap, int* stats);
+ps_err batch_launch_op(hipStream_t st, const BConst& B, const BRep* reps, int R, double* pool, int op,
+                       const double* in, double* out, const int64_t* io_off, int* stats);

@@ -1,0 +1,1296 @@
+// batch.hip -- batched replicas: many independent PolyStokes systems (uneven bead counts, each with
+// its own noise seed) advanced by one persistent kernel per step.
+//
+// Each replica is an unbounded system of probes and chains with full monomer hydrodynamics, run
+// exactly as a single context runs it (the reference's run() step, src/run.cpp:70-162: predictor,
+// slip noise, RFD drift, corrector) with block-eliminated saddle solves and the Lanczos square
+// root.  One workgroup of 1024 threads owns one replica for the whole step: every phase (forces,
+// the carried tile and the stresslet block of mob(), the grand-mobility applies, the stresslet
+// MINRES, the Lanczos recurrence and its tridiagonal square root, the RFD probes and the
+// integrator) is a workgroup-scope function separated by barriers, so a replica needs no other
+// workgroup and no host round trip.  Workgroups take replicas largest first (the host orders
+// them), so the longest replicas start first.
+//
+// The noise is drawn on the host from each replica's own stream (std::mt19937_64 + normal
+// distribution, seeded as a single context is), so every replica reproduces an independent
+// single-system run.  Lanczos runs a fixed number of iterations (sqrt_kmax; an invariant subspace
+// ends it early), with two-pass classical Gram-Schmidt reorthogonalisation.
+//
+// Layout: one FP64 pool; each replica's arrays sit at its own offsets (BRep::off).  Positions are
+// interleaved xyz; saddle vectors [f (n) ; u (nu)] as in a single context.
+#include "ps_common.h"
+#include "tiles.h"
+#include "batch.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int BW = BATCH_THREADS;   // threads per replica workgroup
+constexpr int NW = BW / 64;
+
+__device__ __forceinline__ double wsum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+// Workgroup sum; every thread gets the same value (fixed order).  red: LDS, NW doubles.
+__device__ double wg_sum(double v, double* red) {
+    v = wsum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < NW; k++) s += red[k];
+    return s;
+}
+__device__ double wg_dot(const double* a, const double* b, int64_t n, double* red) {
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += BW) s += a[i] * b[i];
+    return wg_sum(s, red);
+}
+
+// ---------------------------------------------------------------------------------- geometry
+// Unbounded systems: d = x_k - x_j, dr, e = d / dr (the reference's pair_geom without a box).
+__device__ __forceinline__ void bgeom(const double* xs, int k, int j, double e[3], double& dr, double& dr_inv) {
+    const double dx = xs[3 * k] - xs[3 * j], dy = xs[3 * k + 1] - xs[3 * j + 1], dz = xs[3 * k + 2] - xs[3 * j + 2];
+    dr = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+    dr_inv = 1 / dr;
+    e[0] = dx * dr_inv;
+    e[1] = dy * dr_inv;
+    e[2] = dz * dr_inv;
+}
+__device__ __forceinline__ int btype(int Nm, int k, int j) { return (j < Nm) ? 0 : (k < Nm ? 1 : 2); }
+
+// ---------------------------------------------------------------------------------- forces
+// rhs = [0 (n) ; fint (3Np) ; 0 (3Nc)]: excluded volume over the pairs a Verlet list would hold
+// (dr below the pair type's cutoff, at the snapshot positions), bonds along each chain, the static
+// trap on the colloids (src/rhs.cpp:160-260, src/pair_interaction.h:13-72).
+__device__ void wg_forces(const BConst& B, const BRep& R, const double* xpd, const double* x, const double* xpos0,
+                          double ktrap, double* rhs) {
+    const int Np = R.Np, Nm = R.Nm;
+    for (int64_t i = threadIdx.x; i < R.ns; i += BW)
+        if (i < R.n || i >= R.n + 3 * (int64_t)Np) rhs[i] = 0.0;
+    for (int p = threadIdx.x; p < Np; p += BW) {
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int q = 0; q < Np; q++) {
+            if (q == p) continue;
+            const int k = min(p, q), j = max(p, q);
+            const int type = btype(Nm, k, j);
+            if (type == 0 && !B.mono_ev) continue;
+            double e[3], dr, dr_inv;
+            bgeom(xpd, k, j, e, dr, dr_inv);
+            if (!(dr < B.rverl[type])) continue;
+            const double sig_eff = B.sig[type] * (1.0 + 0.01);
+            double fmag;
+            if (B.force_sel == PS_FORCE_HS_EXP) {
+                const double ex_ = exp(-B.hs_exp_k * (dr - sig_eff));
+                fmag = ex_ / (1 - ex_);
+            } else {
+                fmag = (dr >= sig_eff) ? 0.0 : B.k_hs * (sig_eff - dr);
+            }
+            if (fmag == 0.0) continue;
+            const double gx = fmag * e[0], gy = fmag * e[1], gz = fmag * e[2];
+            if (p == k) {
+                fx -= gx;
+                fy -= gy;
+                fz -= gz;
+            } else {
+                fx += gx;
+                fy += gy;
+                fz += gz;
+            }
+        }
+        if (p < Nm) {   // bonds (bond_forces, src/rhs.cpp:102-158): p - 1 and p + 1 in the chain
+            const int c0 = (p / R.Nmono) * R.Nmono;
+            for (int side = 0; side < 2; side++) {
+                const int k = side == 0 ? p - 1 : p, j = side == 0 ? p : p + 1;
+                if (k < c0 || j >= c0 + R.Nmono) continue;
+                const double r0_ = x[3 * k] - x[3 * j], r1_ = x[3 * k + 1] - x[3 * j + 1], r2_ = x[3 * k + 2] - x[3 * j + 2];
+                const double r = __builtin_sqrt(r0_ * r0_ + r1_ * r1_ + r2_ * r2_);
+                double f0 = -B.kbond * (1 - (B.r0 / r)) * r0_;
+                double f1 = -B.kbond * (1 - (B.r0 / r)) * r1_;
+                double f2 = -B.kbond * (1 - (B.r0 / r)) * r2_;
+                if (B.fene) {
+                    const double ext2 = 1 / (1 - (r / B.Lmax) * (r / B.Lmax));
+                    f0 *= ext2;
+                    f1 *= ext2;
+                    f2 *= ext2;
+                }
+                if (p == k) {
+                    fx -= f0;
+                    fy -= f1;
+                    fz -= f2;
+                } else {
+                    fx += f0;
+                    fy += f1;
+                    fz += f2;
+                }
+            }
+        } else if (B.trap_on) {   // trapping_forces (src/rhs.cpp:46-100)
+            const int c = p - Nm;
+            fx += ktrap * (x[3 * p] - xpos0[3 * c]);
+            fy += ktrap * (x[3 * p + 1] - xpos0[3 * c + 1]);
+            fz += ktrap * (x[3 * p + 2] - xpos0[3 * c + 2]);
+        }
+        rhs[R.n + 3 * (int64_t)p] = fx;
+        rhs[R.n + 3 * (int64_t)p + 1] = fy;
+        rhs[R.n + 3 * (int64_t)p + 2] = fz;
+    }
+}
+
+// ---------------------------------------------------------------------------------- mob()
+// The carried tile (the mob_a every AB pair sees, src/mob.cpp:926-958): the tile of the last AA
+// pair of this mob() (si, sj), else the previous call's leftover; st[9..17] keeps what the next call
+// inherits (the last BB pair's tile, or this one).  Thread 0.
+__device__ void b_stale(const BConst& B, const BRep& R, const double* xm, double* st) {
+    double cur[9];
+    if (R.si >= 0) {
+        double e[3], dr, dr_inv;
+        bgeom(xm, R.si, R.sj, e, dr, dr_inv);
+        const double dr_inv2 = dr_inv * dr_inv, dr_inv3 = dr_inv2 * dr_inv;
+        double x12a, y12a;
+        if (dr > B.two_beta) {
+            x12a = C3D8 * dr_inv - 2.0 * dr_inv3;
+            y12a = C3D4 * dr_inv - C1D2 * dr_inv3;
+        } else {
+            x12a = C3D8 * dr - 1.0;
+            y12a = 1.0 - C9D32 * dr;
+        }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const double ee = e[i] * e[j];
+                cur[3 * i + j] = (i == j) ? x12a * ee + y12a * (1. - ee) : x12a * ee - y12a * ee;
+            }
+    } else {
+        for (int k = 0; k < 9; k++) cur[k] = st[9 + k];
+    }
+    for (int k = 0; k < 9; k++) st[k] = cur[k];
+    if (R.Nc >= 2) {
+        double e[3], dr, dr_inv;
+        bgeom(xm, R.Np - 2, R.Np - 1, e, dr, dr_inv);
+        const double dr_inv2 = dr_inv * dr_inv, dr_inv3 = dr_inv2 * dr_inv;
+        const double x12a = C3D2 * dr_inv - dr_inv3, y12a = C3D4 * dr_inv + C1D2 * dr_inv3;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const double ee = e[i] * e[j];
+                st[9 + 3 * i + j] = (i == j) ? x12a * ee + y12a * (1. - ee) : x12a * ee - y12a * ee;
+            }
+    } else {
+        for (int k = 0; k < 9; k++) st[9 + k] = cur[k];
+    }
+}
+// mob(): snapshot xm = xpd, the carried tile, and the dense stresslet block M_ss (5Nc x 5Nc):
+// self blocks and every colloid pair's m tile (placement of src/mob.cpp:1129-1143).
+__device__ void wg_mob(const BConst& B, const BRep& R, const double* xpd, double* xm, double* st, double* Mss) {
+    for (int64_t i = threadIdx.x; i < 3 * (int64_t)R.Np; i += BW) xm[i] = xpd[i];
+    __syncthreads();
+    if (threadIdx.x == 0) b_stale(B, R, xm, st);
+    const int Nc = R.Nc, Nm = R.Nm;
+    const int64_t n5 = 5 * (int64_t)Nc;
+    for (int64_t pr = threadIdx.x; pr < (int64_t)Nc * Nc; pr += BW) {
+        const int c = (int)(pr / Nc), o = (int)(pr % Nc);
+        double blk[5][5];
+        if (o == c) {
+            for (int i = 0; i < 5; i++)
+                for (int j = 0; j < 5; j++)
+                    blk[i][j] = (i == j) ? 9. / 5. : ((i == 0 && j == 4) || (i == 4 && j == 0)) ? 9. / 10. : 0.0;
+        } else {
+            const bool role1 = c < o;
+            double e[3], dr, dr_inv;
+            bgeom(xm, Nm + (role1 ? c : o), Nm + (role1 ? o : c), e, dr, dr_inv);
+            BBT T;
+            tile_bb(dr_inv, e, T);
+            for (int i = 0; i < 5; i++)
+                for (int j = 0; j < 5; j++) blk[i][j] = role1 ? T.m[i][j] : T.m[j][i];
+        }
+        for (int i = 0; i < 5; i++)
+            for (int j = 0; j < 5; j++) Mss[(5 * (int64_t)c + i) * n5 + 5 * (int64_t)o + j] = blk[i][j];
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------- apply
+// u = M f for the grand mobility of src/mob.cpp:635-1213 (the layout of a single context; the
+// reference's quirks: the carried translation tile, the '+' of mob_c, the 2-1 gt sign).  T is the
+// arithmetic of the monomer-monomer and monomer-colloid tiles (float in the mixed form's square
+// root); colloid-colloid tiles are FP64.  `srows`: only the stresslet rows (nu..n).  sh: LDS,
+// 6 BW doubles (staged monomer sources).
+template <class T>
+__device__ void wg_apply(const BConst& B, const BRep& R, const double* xm, const double* f, const double* st,
+                         double* u, bool srows, double* sh) {
+    const int Nm = R.Nm, Nc = R.Nc, t = threadIdx.x;
+    const int64_t nm3 = R.nm3, nc3 = R.nc3, nu = R.nu;
+    const T tb2 = (T)(B.two_beta * B.two_beta);
+    const T gA = (T)(C9D20 * (3 + 5 * B.beta2));
+    const T gB = (T)(C9D10 * (C1D2 + C5D6 * B.beta2));
+    if (!srows) {
+        for (int i0 = 0; i0 < Nm; i0 += BW) {
+            const int i = i0 + t;
+            const bool act = i < Nm;
+            T xi = 0, yi = 0, zi = 0;
+            int ci = -1;
+            if (act) {
+                xi = (T)xm[3 * i];
+                yi = (T)xm[3 * i + 1];
+                zi = (T)xm[3 * i + 2];
+                ci = i / R.Nmono;
+            }
+            T ux = 0, uy = 0, uz = 0;
+            // monomer sources (mobility(..., AA=true), src/mob.cpp:122-131): y f + (x - y)/r^2 (d.f) d
+            for (int j0 = 0; j0 < Nm; j0 += BW) {
+                __syncthreads();
+                const int j = j0 + t;
+                if (j < Nm) {
+                    sh[t] = xm[3 * j];
+                    sh[BW + t] = xm[3 * j + 1];
+                    sh[2 * BW + t] = xm[3 * j + 2];
+                    sh[3 * BW + t] = f[3 * j];
+                    sh[4 * BW + t] = f[3 * j + 1];
+                    sh[5 * BW + t] = f[3 * j + 2];
+                }
+                __syncthreads();
+                if (!act) continue;
+                const int cnt = min(BW, Nm - j0);
+                for (int k = 0; k < cnt; k++) {
+                    const int jj = j0 + k;
+                    if (jj == i) continue;
+                    if (B.chain_HI && jj / R.Nmono == ci) continue;
+                    const T dx = xi - (T)sh[k], dy = yi - (T)sh[BW + k], dz = zi - (T)sh[2 * BW + k];
+                    const T r2 = dx * dx + dy * dy + dz * dz;
+                    const T rinv = ps_rsqrt(r2);
+                    const T rinv2 = rinv * rinv;
+                    T ya = rinv * ((T)C3D4 - (T)C1D2 * rinv2);
+                    T cc = rinv * rinv2 * (-(T)C3D8 - (T)C3D2 * rinv2);
+                    if (r2 <= tb2) {
+                        const T r = r2 * rinv;
+                        ya = (T)1 - (T)C9D32 * r;
+                        cc = ((T)(21. / 32.) * r - (T)2) * rinv2;
+                    }
+                    const T fx = (T)sh[3 * BW + k], fy = (T)sh[4 * BW + k], fz = (T)sh[5 * BW + k];
+                    const T cs = cc * (dx * fx + dy * fy + dz * fz);
+                    ux += ya * fx + cs * dx;
+                    uy += ya * fy + cs * dy;
+                    uz += ya * fz + cs * dz;
+                }
+            }
+            if (!act) continue;
+            // colloid sources (mobility_AB, src/mob.cpp:362-537): rotlet and stresslet couplings,
+            // and the carried tile applied to the sum of colloid forces
+            T vx = 0, vy = 0, vz = 0, Fx = 0, Fy = 0, Fz = 0;
+            for (int c = 0; c < Nc; c++) {
+                const int g = Nm + c;
+                const T dx = xi - (T)xm[3 * g], dy = yi - (T)xm[3 * g + 1], dz = zi - (T)xm[3 * g + 2];
+                const T r2 = dx * dx + dy * dy + dz * dz;
+                const T rinv = ps_rsqrt(r2);
+                const T ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
+                const T rinv2 = rinv * rinv, rinv4 = rinv2 * rinv2;
+                const T y12b = -(T)C3D4 * rinv2;
+                const T x12g = (T)C9D4 * rinv2 - gA * rinv4;
+                const T y12g = gB * rinv4;
+                const T Tx = (T)f[nm3 + nc3 + 3 * c], Ty = (T)f[nm3 + nc3 + 3 * c + 1], Tz = (T)f[nm3 + nc3 + 3 * c + 2];
+                const double* S = f + nu + 5 * (int64_t)c;
+                const T S0 = (T)S[0], S1 = (T)S[1], S2 = (T)S[2], S3 = (T)S[3], S4 = (T)S[4];
+                const T sex = S0 * ex + S1 * ey + S2 * ez;
+                const T sey = S1 * ex + S4 * ey + S3 * ez;
+                const T sez = S2 * ex + S3 * ey - (S0 + S4) * ez;
+                const T ese = ex * sex + ey * sey + ez * sez;
+                const T a = -x12g * ese - (T)2 * y12g * ese;
+                const T b2 = (T)2 * y12g;
+                vx += y12b * (ey * Tz - ez * Ty) + a * ex + b2 * sex;
+                vy += y12b * (ez * Tx - ex * Tz) + a * ey + b2 * sey;
+                vz += y12b * (ex * Ty - ey * Tx) + a * ez + b2 * sez;
+                Fx += (T)f[nm3 + 3 * c];
+                Fy += (T)f[nm3 + 3 * c + 1];
+                Fz += (T)f[nm3 + 3 * c + 2];
+            }
+            vx += (T)st[0] * Fx + (T)st[1] * Fy + (T)st[2] * Fz;
+            vy += (T)st[3] * Fx + (T)st[4] * Fy + (T)st[5] * Fz;
+            vz += (T)st[6] * Fx + (T)st[7] * Fy + (T)st[8] * Fz;
+            u[3 * i] = B.beta_inv * f[3 * i] + ((double)ux + (double)vx);
+            u[3 * i + 1] = B.beta_inv * f[3 * i + 1] + ((double)uy + (double)vy);
+            u[3 * i + 2] = B.beta_inv * f[3 * i + 2] + ((double)uz + (double)vz);
+        }
+    }
+    // colloid rows: monomer sources (AB: rotation and stresslet rows, the sum of monomer forces for
+    // the transposed carried tile) and colloid sources (the full FP64 FTS tile)
+    for (int c = t; c < Nc; c += BW) {
+        const int g = Nm + c;
+        const T xc = (T)xm[3 * g], yc = (T)xm[3 * g + 1], zc = (T)xm[3 * g + 2];
+        T acc[11];
+        for (int k = 0; k < 11; k++) acc[k] = 0;
+        T fsx = 0, fsy = 0, fsz = 0;
+        for (int m = 0; m < Nm; m++) {
+            const T dx = (T)xm[3 * m] - xc, dy = (T)xm[3 * m + 1] - yc, dz = (T)xm[3 * m + 2] - zc;
+            const T r2 = dx * dx + dy * dy + dz * dz;
+            const T rinv = ps_rsqrt(r2);
+            const T ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
+            const T rinv2 = rinv * rinv, rinv4 = rinv2 * rinv2;
+            const T y12b = -(T)C3D4 * rinv2;
+            const T x12g = (T)C9D4 * rinv2 - gA * rinv4;
+            const T y12g = gB * rinv4;
+            const T fx = (T)f[3 * m], fy = (T)f[3 * m + 1], fz = (T)f[3 * m + 2];
+            fsx += fx;
+            fsy += fy;
+            fsz += fz;
+            acc[3] += y12b * (fy * ez - fz * ey);
+            acc[4] += y12b * (fz * ex - fx * ez);
+            acc[5] += y12b * (fx * ey - fy * ex);
+            const T ef = ex * fx + ey * fy + ez * fz;
+            const T p = -x12g * ef - (T)2 * y12g * ef;
+            const T G00 = p * ex * ex + (T)2 * y12g * ex * fx;
+            const T G11 = p * ey * ey + (T)2 * y12g * ey * fy;
+            const T G22 = p * ez * ez + (T)2 * y12g * ez * fz;
+            const T G01 = p * ex * ey + y12g * (ex * fy + ey * fx);
+            const T G02 = p * ex * ez + y12g * (ex * fz + ez * fx);
+            const T G12 = p * ey * ez + y12g * (ey * fz + ez * fy);
+            acc[6] += G00 - G22;
+            acc[7] += (T)2 * G01;
+            acc[8] += (T)2 * G02;
+            acc[9] += (T)2 * G12;
+            acc[10] += G11 - G22;
+        }
+        double bb[11];
+        for (int k = 0; k < 11; k++) bb[k] = 0.0;
+        for (int o = 0; o < Nc; o++) {
+            if (o == c) continue;
+            const bool role1 = c < o;
+            const int c1 = role1 ? c : o, c2 = role1 ? o : c;
+            double e[3], dr, dr_inv;
+            bgeom(xm, Nm + c1, Nm + c2, e, dr, dr_inv);
+            BBT Tt;
+            tile_bb(dr_inv, e, Tt);
+            double Fo[3], To[3], So[5];
+            for (int k = 0; k < 3; k++) Fo[k] = f[nm3 + 3 * o + k];
+            for (int k = 0; k < 3; k++) To[k] = f[nm3 + nc3 + 3 * o + k];
+            for (int k = 0; k < 5; k++) So[k] = f[nu + 5 * o + k];
+            if (role1) {
+                for (int i = 0; i < 3; i++) {
+                    double uu = 0.0, w = 0.0;
+                    for (int j = 0; j < 3; j++) {
+                        uu += Tt.a[i][j] * Fo[j] - Tt.bt[j][i] * To[j];
+                        w += Tt.b[i][j] * Fo[j] + Tt.c[i][j] * To[j];
+                    }
+                    for (int j = 0; j < 5; j++) {
+                        uu += Tt.gt[i][j] * So[j];
+                        w += Tt.ht[i][j] * So[j];
+                    }
+                    bb[i] += uu;
+                    bb[3 + i] += w;
+                }
+                for (int j = 0; j < 5; j++) {
+                    double v = 0.0;
+                    for (int i = 0; i < 3; i++) v += -Tt.gt[i][j] * Fo[i] + Tt.ht[i][j] * To[i];
+                    for (int k = 0; k < 5; k++) v += Tt.m[j][k] * So[k];
+                    bb[6 + j] += v;
+                }
+            } else {
+                for (int i = 0; i < 3; i++) {
+                    double uu = 0.0, w = 0.0;
+                    for (int j = 0; j < 3; j++) {
+                        uu += Tt.a[j][i] * Fo[j] + Tt.bt[i][j] * To[j];
+                        w += -Tt.b[j][i] * Fo[j] + Tt.c[j][i] * To[j];
+                    }
+                    for (int j = 0; j < 5; j++) {
+                        uu += -Tt.gt[i][j] * So[j];
+                        w += Tt.ht[i][j] * So[j];
+                    }
+                    bb[i] += uu;
+                    bb[3 + i] += w;
+                }
+                for (int j = 0; j < 5; j++) {
+                    double v = 0.0;
+                    for (int i = 0; i < 3; i++) v += Tt.gt[i][j] * Fo[i] + Tt.ht[i][j] * To[i];
+                    for (int k = 0; k < 5; k++) v += Tt.m[k][j] * So[k];
+                    bb[6 + j] += v;
+                }
+            }
+        }
+        const double fs[3] = {(double)fsx, (double)fsy, (double)fsz};
+        if (!srows) {
+            for (int comp = 0; comp < 3; comp++) {
+                const int64_t r = nm3 + 3 * (int64_t)c + comp;
+                const double carried = st[0 * 3 + comp] * fs[0] + st[1 * 3 + comp] * fs[1] + st[2 * 3 + comp] * fs[2];
+                u[r] = 1.0 * f[r] + carried + bb[comp];
+            }
+            for (int comp = 0; comp < 3; comp++) {
+                const int64_t r = nm3 + nc3 + 3 * (int64_t)c + comp;
+                u[r] = (3. / 4.) * f[r] + (double)acc[3 + comp] + bb[3 + comp];
+            }
+        }
+        const double* S = f + nu + 5 * (int64_t)c;
+        for (int comp = 0; comp < 5; comp++) {
+            double self = (9. / 5.) * S[comp];
+            if (comp == 0) self += (9. / 10.) * S[4];
+            if (comp == 4) self += (9. / 10.) * S[0];
+            u[nu + 5 * (int64_t)c + comp] = self + (double)acc[6 + comp] + bb[6 + comp];
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------- solves
+// MINRES (Paige & Saunders) on M_ss x = r, preconditioned by the inverse of the stresslet self block
+// (every diagonal 5 x 5 block of M_ss), to rtol in the preconditioned residual norm.  Work: 8
+// vectors of n5 at w.  Returns the iterations.
+__device__ __forceinline__ void self_inv5(const double* v, double* out) {
+    // inverse of [[9/5, 0, 0, 0, 9/10], [0, 9/5, ...], ..., [9/10, 0, 0, 0, 9/5]]
+    const double a = 9. / 5., b = 9. / 10., det = a * a - b * b;
+    out[0] = (a * v[0] - b * v[4]) / det;
+    out[4] = (a * v[4] - b * v[0]) / det;
+    out[1] = v[1] / a;
+    out[2] = v[2] / a;
+    out[3] = v[3] / a;
+}
+__device__ int wg_minres_ss(const BConst& B, int64_t n5, const double* Mss, const double* rhs, double* x, double* w,
+                            double* red) {
+    double *r1 = w, *r2 = w + n5, *y = w + 2 * n5, *v = w + 3 * n5, *ww = w + 4 * n5, *w1 = w + 5 * n5,
+           *w2 = w + 6 * n5;
+    const int t = threadIdx.x;
+    for (int64_t i = t; i < n5; i += BW) {
+        x[i] = 0.0;
+        r1[i] = rhs[i];
+        r2[i] = rhs[i];
+        ww[i] = 0.0;
+        w2[i] = 0.0;
+    }
+    __syncthreads();
+    for (int64_t b = t; b < n5 / 5; b += BW) self_inv5(r1 + 5 * b, y + 5 * b);
+    __syncthreads();
+    const double beta1 = sqrt(fmax(wg_dot(r1, y, n5, red), 0.0));
+    if (beta1 == 0.0) return 0;
+    double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, phibar = beta1, cs = -1.0, sn = 0.0;
+    int its = 0;
+    for (int itn = 1; itn <= B.ms_maxit; itn++) {
+        its = itn;
+        const double s = 1.0 / beta;
+        for (int64_t i = t; i < n5; i += BW) v[i] = s * y[i];
+        __syncthreads();
+        // y = M_ss v: one wave per row, lanes over columns
+        for (int64_t row = t >> 6; row < n5; row += NW) {
+            double acc = 0.0;
+            const double* mr = Mss + row * n5;
+            for (int64_t k = t & 63; k < n5; k += 64) acc += mr[k] * v[k];
+            acc = wsum(acc);
+            if ((t & 63) == 0) y[row] = acc;
+        }
+        __syncthreads();
+        if (itn >= 2)
+            for (int64_t i = t; i < n5; i += BW) y[i] -= (beta / oldb) * r1[i];
+        __syncthreads();
+        const double alfa = wg_dot(v, y, n5, red);
+        for (int64_t i = t; i < n5; i += BW) {
+            y[i] -= (alfa / beta) * r2[i];
+            r1[i] = r2[i];
+            r2[i] = y[i];
+        }
+        __syncthreads();
+        for (int64_t b = t; b < n5 / 5; b += BW) self_inv5(r2 + 5 * b, y + 5 * b);
+        __syncthreads();
+        oldb = beta;
+        beta = sqrt(fmax(wg_dot(r2, y, n5, red), 0.0));
+        const double oldeps = epsln;
+        const double delta = cs * dbar + sn * alfa;
+        const double gbar = sn * dbar - cs * alfa;
+        epsln = sn * beta;
+        dbar = -cs * beta;
+        double gamma = sqrt(gbar * gbar + beta * beta);
+        gamma = fmax(gamma, 2.2e-16);
+        cs = gbar / gamma;
+        sn = beta / gamma;
+        const double phi = cs * phibar;
+        phibar = sn * phibar;
+        const double denom = 1.0 / gamma;
+        for (int64_t i = t; i < n5; i += BW) {
+            const double a1 = w2[i];
+            w1[i] = a1;
+            w2[i] = ww[i];
+            ww[i] = (v[i] - oldeps * a1 - delta * ww[i]) * denom;
+            x[i] += phi * ww[i];
+        }
+        __syncthreads();
+        if (phibar <= B.ms_rtol * beta1 || beta == 0.0) break;
+    }
+    return its;
+}
+
+// The saddle system [[M, B], [B^T, 0]] [f; u] = b with B = [-I_nu; 0] by exact block elimination:
+// f_u = -b_l;  M_ss f_s = b_s - (M [f_u; 0])_s;  u = (M f)_u - b_u.  `uout` gets u too.
+__device__ int wg_block_solve(const BConst& B, const BRep& R, double* pool, const double* xm, const double* st,
+                              const double* b, double* x, double* uout, double* sh, double* red) {
+    const int64_t n = R.n, nu = R.nu, n5 = 5 * (int64_t)R.Nc;
+    double* y = pool + R.off[BO_Y];
+    double* w5 = pool + R.off[BO_M5];
+    for (int64_t i = threadIdx.x; i < n; i += BW) x[i] = (i < nu) ? -b[n + i] : 0.0;
+    __syncthreads();
+    int its = 0;
+    if (n5 > 0) {
+        wg_apply<double>(B, R, xm, x, st, y, true, sh);
+        double* rs = w5 + 8 * n5;
+        for (int64_t k = threadIdx.x; k < n5; k += BW) rs[k] = b[nu + k] - y[nu + k];
+        __syncthreads();
+        its = wg_minres_ss(B, n5, pool + R.off[BO_MSS], rs, x + nu, w5, red);
+    }
+    wg_apply<double>(B, R, xm, x, st, y, false, sh);
+    for (int64_t i = threadIdx.x; i < nu; i += BW) {
+        const double u = y[i] - b[i];
+        x[n + i] = u;
+        if (uout) uout[i] = u;
+    }
+    __syncthreads();
+    return its;
+}
+
+// ---------------------------------------------------------------------------------- square root
+// tql2 (implicit QL on a symmetric tridiagonal, eigenvectors in Z, column-major m x m): thread 0.
+__device__ bool b_tql2(int m, double* d, double* e, double* Z) {
+    for (int i = 0; i < m * m; i++) Z[i] = 0.0;
+    for (int i = 0; i < m; i++) Z[i + i * m] = 1.0;
+    for (int l = 0; l < m; l++) {
+        int iter = 0, mm;
+        do {
+            for (mm = l; mm < m - 1; mm++) {
+                const double dd = fabs(d[mm]) + fabs(d[mm + 1]);
+                if (fabs(e[mm]) <= 1e-300 + 2.2e-16 * dd) break;
+            }
+            if (mm != l) {
+                if (iter++ == 200) return false;
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = hypot(g, 1.0);
+                g = d[mm] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
+                double s = 1.0, c = 1.0, p = 0.0;
+                bool underflow = false;
+                for (int i = mm - 1; i >= l; i--) {
+                    const double f = s * e[i], bb = c * e[i];
+                    r = hypot(f, g);
+                    e[i + 1] = r;
+                    if (r == 0.0) {
+                        d[i + 1] -= p;
+                        e[mm] = 0.0;
+                        underflow = true;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * bb;
+                    p = s * r;
+                    d[i + 1] = g + p;
+                    g = c * r - bb;
+                    for (int k = 0; k < m; k++) {
+                        const double zf = Z[k + (i + 1) * m];
+                        Z[k + (i + 1) * m] = s * Z[k + i * m] + c * zf;
+                        Z[k + i * m] = c * Z[k + i * m] - s * zf;
+                    }
+                }
+                if (underflow) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[mm] = 0.0;
+            }
+        } while (mm != l);
+    }
+    return true;
+}
+
+// out = M^{1/2} xi (n) by Lanczos (schur_sqrt_lanczos, src/slip_vel.cpp:351-477, on the full grand
+// mobility): kmax iterations, or fewer on an invariant subspace (beta_j <= 1e-12 beta_0); two-pass
+// classical Gram-Schmidt against the stored basis; T_m = U diag(theta) U^T by QL; negative Ritz
+// values floored at 0; out = beta_0 Q U diag(sqrt(theta+)) U^T e_1.  Applies in T.  Returns m.
+template <class T>
+__device__ int wg_sqrt(const BConst& B, const BRep& R, double* pool, const double* xm, const double* st,
+                       const double* xi, double* out, double* sh, double* red, double* lz, int* nneg_out) {
+    const int64_t n = R.n;
+    double* Q = pool + R.off[BO_QB];
+    double* w = pool + R.off[BO_LW];
+    double* alpha = lz;              // [BATCH_KMAX]
+    double* betad = lz + BATCH_KMAX;  // [BATCH_KMAX]
+    double* h = lz + 2 * BATCH_KMAX;  // [BATCH_KMAX]
+    double* g = lz + 3 * BATCH_KMAX;  // [BATCH_KMAX]
+    double* Z = lz + 4 * BATCH_KMAX;  // [BATCH_KMAX^2]
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const double beta0 = sqrt(wg_dot(xi, xi, n, red));
+    for (int64_t i = t; i < n; i += BW) out[i] = 0.0;
+    if (nneg_out && t == 0) *nneg_out = 0;
+    __syncthreads();
+    if (beta0 == 0.0) return 0;
+    int kmax = B.kmax;
+    if (kmax > n) kmax = (int)n;
+    if (kmax < 1) kmax = 1;
+    for (int64_t i = t; i < n; i += BW) Q[i] = xi[i] * (1.0 / beta0);
+    __syncthreads();
+    int m = 0;
+    for (int j = 0; j < kmax; j++) {
+        const double* qj = Q + (int64_t)j * n;
+        wg_apply<T>(B, R, xm, qj, st, w, false, sh);
+        if (j > 0) {
+            const double bj1 = betad[j - 1];
+            const double* qp = Q + (int64_t)(j - 1) * n;
+            for (int64_t i = t; i < n; i += BW) w[i] -= bj1 * qp[i];
+            __syncthreads();
+        }
+        const double aj = wg_dot(qj, w, n, red);
+        if (t == 0) alpha[j] = aj;
+        for (int64_t i = t; i < n; i += BW) w[i] -= aj * qj[i];
+        __syncthreads();
+        for (int pass = 0; pass < 2; pass++) {
+            // h_i = Q_i . w for i <= j: wave wv takes i = wv, wv + NW, ...
+            for (int i = wv; i <= j; i += NW) {
+                const double* qi = Q + (int64_t)i * n;
+                double s = 0.0;
+                for (int64_t k = lane; k < n; k += 64) s += qi[k] * w[k];
+                s = wsum(s);
+                if (lane == 0) h[i] = s;
+            }
+            __syncthreads();
+            for (int64_t k = t; k < n; k += BW) {
+                double acc = w[k];
+                for (int i = 0; i <= j; i++) acc -= h[i] * Q[(int64_t)i * n + k];
+                w[k] = acc;
+            }
+            __syncthreads();
+        }
+        m = j + 1;
+        const double bj = sqrt(wg_dot(w, w, n, red));
+        if (bj <= 1e-12 * beta0 || j == kmax - 1) break;
+        if (t == 0) betad[j] = bj;
+        double* qn = Q + (int64_t)(j + 1) * n;
+        for (int64_t i = t; i < n; i += BW) qn[i] = w[i] * (1.0 / bj);
+        __syncthreads();
+    }
+    if (t == 0) {
+        // g = beta0 U diag(sqrt(max(theta, 0))) U^T e1 (src/slip_vel.cpp:427-459)
+        double dd[BATCH_KMAX], ee[BATCH_KMAX];
+        for (int i = 0; i < m; i++) dd[i] = alpha[i];
+        for (int i = 0; i < m; i++) ee[i] = (i < m - 1) ? betad[i] : 0.0;
+        b_tql2(m, dd, ee, Z);
+        int nneg = 0;
+        for (int i = 0; i < m; i++) {
+            double gi = 0.0;
+            for (int jj = 0; jj < m; jj++) {
+                double th = dd[jj];
+                if (th < 0.0) th = 0.0;
+                gi += Z[i + jj * m] * sqrt(th) * Z[0 + jj * m];
+            }
+            g[i] = beta0 * gi;
+        }
+        for (int jj = 0; jj < m; jj++)
+            if (dd[jj] < 0.0) nneg++;
+        if (nneg_out) *nneg_out = nneg;
+    }
+    __syncthreads();
+    for (int64_t k = t; k < n; k += BW) {
+        double acc = 0.0;
+        for (int i = 0; i < m; i++) acc += g[i] * Q[(int64_t)i * n + k];
+        out[k] = acc;
+    }
+    __syncthreads();
+    return m;
+}
+
+// ---------------------------------------------------------------------------------- the step
+struct BStepArgs {
+    BConst B;
+    const BRep* reps;
+    const int* order;      // workgroup -> replica, largest first
+    double* pool;
+    const double* noise;   // device copy of this step's noise: replica r at reps[r].off[BO_NZ] - noise base
+    double ktrap;
+    int* stats;            // per replica: [minres iterations, lanczos m, negative Ritz values, 0]
+};
+
+__global__ __launch_bounds__(BW) void k_batch_step(BStepArgs a) {
+    __shared__ double sh[6 * BW];
+    __shared__ double red[NW];
+    __shared__ double lz[4 * BATCH_KMAX + BATCH_KMAX * BATCH_KMAX];
+    __shared__ int nneg_sh;
+    const BConst& B = a.B;
+    const int r = a.order[blockIdx.x];
+    const BRep R = a.reps[r];
+    double* P = a.pool;
+    const int64_t n = R.n, nu = R.nu, n3 = 3 * (int64_t)R.Np, nc5 = 5 * (int64_t)R.Nc;
+    double *x = P + R.off[BO_X], *xn = P + R.off[BO_XN], *xpd = P + R.off[BO_XPD], *xm = P + R.off[BO_XM];
+    double *rhs = P + R.off[BO_RHS], *X = P + R.off[BO_XS], *Xdet = P + R.off[BO_XDET], *Xd = P + R.off[BO_XD];
+    double *up = P + R.off[BO_UP], *vdn = P + R.off[BO_VDN], *vdc = P + R.off[BO_VDC], *vb = P + R.off[BO_VB];
+    double *ud = P + R.off[BO_UD], *dv = P + R.off[BO_DV], *W = P + R.off[BO_W], *sb = P + R.off[BO_SB];
+    double *st = P + R.off[BO_ST], *Mss = P + R.off[BO_MSS], *q = P + R.off[BO_Q];
+    const double* xpos0 = P + R.off[BO_XP0];
+    const double* noise = a.noise + R.off[BO_NZ];   // n slip normals, then nu RFD normals
+    const int t = threadIdx.x;
+    const double dt = B.dt, kT = B.kT, eps = 1.0e-3;
+    int its = 0;
+    auto snapshot = [&]() {
+        for (int64_t i = t; i < n3; i += BW) xpd[i] = x[i];
+        __syncthreads();
+    };
+    auto solve = [&](const double* b, double* Xo, double* uo) {
+        its += wg_block_solve(B, R, P, xm, st, b, Xo, uo, sh, red);
+    };
+    for (int64_t i = t; i < n3; i += BW) xn[i] = x[i];
+    // ---- predictor
+    snapshot();
+    wg_forces(B, R, xpd, x, xpos0, a.ktrap, rhs);
+    __syncthreads();
+    wg_mob(B, R, xpd, xm, st, Mss);
+    solve(rhs, Xdet, vdn);
+    int lm = 0;
+    if (kT > 0) {
+        lm = B.mixed ? wg_sqrt<float>(B, R, P, xm, st, noise, W, sh, red, lz, &nneg_sh)
+                     : wg_sqrt<double>(B, R, P, xm, st, noise, W, sh, red, lz, &nneg_sh);
+        const double sc = sqrt(2 * kT / dt);
+        for (int64_t i = t; i < n; i += BW) rhs[i] += sc * W[i];
+        __syncthreads();
+        solve(rhs, X, up);
+        for (int64_t i = t; i < nu; i += BW) vb[i] = up[i] - vdn[i];
+        // drift(): RFD probes at x +/- eps W (src/drift.cpp:17-62)
+        const double* fext = noise + n;
+        for (int64_t i = t; i < n3; i += BW) x[i] += eps * fext[i];
+        __syncthreads();
+        snapshot();
+        wg_mob(B, R, xpd, xm, st, Mss);
+        for (int64_t i = t; i < R.ns; i += BW) rhs[i] = (i < n) ? 0.0 : fext[i - n];
+        __syncthreads();
+        solve(rhs, Xd, dv);
+        for (int64_t i = t; i < nc5; i += BW) sb[i] = Xd[nu + i];
+        for (int64_t i = t; i < n3; i += BW) x[i] += -2.0 * eps * fext[i];
+        __syncthreads();
+        snapshot();
+        wg_mob(B, R, xpd, xm, st, Mss);
+        for (int64_t i = t; i < R.ns; i += BW) rhs[i] = (i < n) ? 0.0 : fext[i - n];
+        __syncthreads();
+        solve(rhs, Xd, ud);
+        for (int64_t i = t; i < nu; i += BW) ud[i] = ud[i] - dv[i];
+        for (int64_t i = t; i < nc5; i += BW) sb[i] = (sb[i] - Xd[nu + i]) / (2 * eps);
+        __syncthreads();
+        // step() (src/traj.cpp:9-35): undo the probe, add the drift and the velocity
+        const double drift_scale = kT / (2.0 * eps);
+        for (int64_t i = t; i < n3; i += BW) {
+            double v = x[i];
+            v += eps * fext[i];
+            v += drift_scale * ud[i] * dt;
+            v += up[i] * dt;
+            x[i] = v;
+        }
+    } else {
+        solve(rhs, X, up);
+        for (int64_t i = t; i < nu; i += BW) vb[i] = 0.0;
+        for (int64_t i = t; i < n3; i += BW) x[i] = x[i] + up[i] * dt;
+    }
+    __syncthreads();
+    // ---- corrector
+    snapshot();
+    wg_forces(B, R, xpd, x, xpos0, a.ktrap, rhs);
+    __syncthreads();
+    wg_mob(B, R, xpd, xm, st, Mss);
+    solve(rhs, Xdet, vdc);
+    for (int64_t i = t; i < nu; i += BW) up[i] = 0.5 * (vdn[i] + vdc[i]) + vb[i];
+    __syncthreads();
+    for (int64_t i = t; i < n3; i += BW)
+        x[i] = (kT > 0) ? xn[i] + (kT / (2.0 * eps)) * ud[i] * dt + up[i] * dt : xn[i] + up[i] * dt;
+    // step_quaternion (src/traj.cpp:64-94)
+    for (int c = t; c < R.Nc; c += BW) {
+        double* qi = q + 4 * (int64_t)c;
+        const double* om = up + R.nm3 + R.nc3 + 3 * (int64_t)c;
+        const double w1 = qi[0], x1 = qi[1], y1 = qi[2], z1 = qi[3];
+        const double w2 = 0.0, x2 = om[0], y2 = om[1], z2 = om[2];
+        const double dq[4] = {w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                              w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2};
+        double qn[4];
+        for (int k = 0; k < 4; k++) qn[k] = qi[k] + 0.5 * dq[k] * dt;
+        const double norm = __builtin_sqrt(qn[0] * qn[0] + qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]);
+        for (int k = 0; k < 4; k++) qi[k] = qn[k] / norm;
+    }
+    if (t == 0) {
+        a.stats[4 * r] = its;
+        a.stats[4 * r + 1] = lm;
+        a.stats[4 * r + 2] = kT > 0 ? nneg_sh : 0;
+        a.stats[4 * r + 3] = 0;
+    }
+}
+
+// Single-operation kernels (one replica per workgroup, for the checks against the oracle):
+// op 0: y = M f (FP64), op 1: y = M f with float tiles, op 2: saddle solve b -> x,
+// op 3: W = M^{1/2} xi (FP64 Lanczos), op 4: mob() at the current positions (snapshot + tiles),
+// op 5: rhs of the forces at the current positions.
+__global__ __launch_bounds__(BW) void k_batch_op(BConst B, const BRep* reps, double* pool, int op, const double* in,
+                                                 double* out, const int64_t* io_off, int* stats) {
+    __shared__ double sh[6 * BW];
+    __shared__ double red[NW];
+    __shared__ double lz[4 * BATCH_KMAX + BATCH_KMAX * BATCH_KMAX];
+    __shared__ int nneg_sh;
+    const int r = blockIdx.x;
+    const BRep R = reps[r];
+    double* P = pool;
+    const double* inr = in + io_off[r];
+    double* outr = out + io_off[r];
+    double *x = P + R.off[BO_X], *xpd = P + R.off[BO_XPD], *xm = P + R.off[BO_XM], *st = P + R.off[BO_ST];
+    int s0 = 0;
+    if (op == 0) {
+        wg_apply<double>(B, R, xm, inr, st, outr, false, sh);
+    } else if (op == 1) {
+        wg_apply<float>(B, R, xm, inr, st, outr, false, sh);
+    } else if (op == 2) {
+        s0 = wg_block_solve(B, R, P, xm, st, inr, outr, nullptr, sh, red);
+    } else if (op == 3) {
+        s0 = wg_sqrt<double>(B, R, P, xm, st, inr, outr, sh, red, lz, &nneg_sh);
+    } else if (op == 4) {
+        for (int64_t i = threadIdx.x; i < 3 * (int64_t)R.Np; i += BW) xpd[i] = x[i];
+        __syncthreads();
+        wg_mob(B, R, xpd, xm, st, P + R.off[BO_MSS]);
+    } else if (op == 5) {
+        for (int64_t i = threadIdx.x; i < 3 * (int64_t)R.Np; i += BW) xpd[i] = x[i];
+        __syncthreads();
+        wg_forces(B, R, xpd, x, P + R.off[BO_XP0], B.ktrap, outr);
+    }
+    if (threadIdx.x == 0 && stats) stats[r] = s0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------- host
+ps_err batch_launch_step(hipStream_t st, const BConst& B, const BRep* reps, const int* order, int R, double* pool,
+                         const double* noise, double ktrap, int* stats) {
+    BStepArgs a{B, reps, order, pool, noise, ktrap, stats};
+    k_batch_step<<<R, BW, 0, st>>>(a);
+    return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_HIP;
+}
+ps_err batch_launch_op(hipStream_t st, const BConst& B, const BRep* reps, int R, double* pool, int op,
+                       const double* in, double* out, const int64_t* io_off, int* stats) {
+    k_batch_op<<<R, BW, 0, st>>>(B, reps, pool, op, in, out, io_off, stats);
+    return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_HIP;
+}
+
+// ---------------------------------------------------------------------------------- C ABI
+#include "normals.h"
+
+#include <algorithm>
+#include <cmath>
+#include <condition_variable>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_batch_create_error;
+
+struct Batch {
+    int R = 0;
+    ps_params p{};
+    BConst B{};
+    std::vector<BRep> reps;
+    std::vector<int> order;
+    BRep* d_reps = nullptr;
+    int* d_order = nullptr;
+    int* d_stats = nullptr;
+    double* pool = nullptr;
+    int64_t pool_n = 0;
+    double* noise_d = nullptr;
+    double* noise_h[2] = {nullptr, nullptr};
+    int64_t noise_n = 0;
+    hipEvent_t copied[2] = {nullptr, nullptr};
+    std::vector<std::unique_ptr<NormalStream>> rng;
+    hipStream_t stream = nullptr;
+    double t = 0.0, ktrap_cur = 0.0;
+    std::string err;
+    int nthreads = 1;
+
+    void set_error(const std::string& m) { err = m; }
+    // draw this step's noise of every replica (n slip normals, then nu RFD normals) into slot k,
+    // replicas split over host threads; each replica's stream advances as a single context's does
+    void fill(int k) {
+        auto work = [this, k](int w) {
+            for (int r = w; r < R; r += nthreads)
+                rng[r]->fill(noise_h[k] + reps[r].off[BO_NZ], reps[r].n + reps[r].nu);
+        };
+        if (nthreads <= 1) {
+            work(0);
+            return;
+        }
+        std::vector<std::thread> th;
+        for (int w = 1; w < nthreads; w++) th.emplace_back(work, w);
+        work(0);
+        for (auto& x : th) x.join();
+    }
+};
+
+#define BPS_HIP(b, call)                                                                          \
+    do {                                                                                          \
+        hipError_t _e = (call);                                                                   \
+        if (_e != hipSuccess) {                                                                   \
+            (b)->set_error(std::string(#call) + ": " + hipGetErrorString(_e));                    \
+            return PS_ERR_HIP;                                                                    \
+        }                                                                                         \
+    } while (0)
+
+void batch_free(Batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (void* q : {(void*)b->d_reps, (void*)b->d_order, (void*)b->d_stats, (void*)b->pool, (void*)b->noise_d})
+        if (q) (void)hipFree(q);
+    for (double* h : b->noise_h)
+        if (h) (void)hipHostFree(h);
+    for (hipEvent_t e : b->copied)
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int64_t pad8(int64_t v) { return (v + 7) / 8 * 8; }
+
+}  // namespace
+
+extern "C" {
+
+ps_err ps_batch_create(const ps_params* p, int R, const int* Nc, const int* Npoly, const int* Nmono,
+                       const uint64_t* seeds, void** out) {
+    if (!p || !out || R <= 0 || !Nc || !Npoly || !Nmono || !seeds) return PS_ERR_ARG;
+    *out = nullptr;
+    auto bad = [](const std::string& m) {
+        g_batch_create_error = m;
+        return PS_ERR_ARG;
+    };
+    if (!p->mm_HI) return bad("batched replicas: full monomer hydrodynamics (mm_HI) only");
+    if (p->box_active) return bad("batched replicas: unbounded systems only (no periodic box)");
+    if (p->sqrt_kmax < 1 || p->sqrt_kmax > BATCH_KMAX) return bad("batched replicas: sqrt_kmax must be in 1..48");
+    Batch* b = new Batch();
+    b->R = R;
+    b->p = *p;
+    BConst& B = b->B;
+    B.beta = p->beta;
+    B.beta2 = p->beta * p->beta;
+    B.beta_inv = 1.0 / p->beta;
+    B.two_beta = 2.0 * p->beta;
+    B.dt = p->dt;
+    B.kT = p->kT;
+    B.kbond = p->kbond;
+    B.r0 = p->r0;
+    B.Lmax = p->Lmax;
+    B.ktrap = p->ktrap;
+    B.k_hs = (p->hs_kmode == 1) ? p->hs_kpref / (p->dt * p->beta) : p->hs_kpref / p->dt;
+    B.hs_exp_k = p->hs_exp_k;
+    const float sg[3] = {(float)(2. * p->beta), (float)(1. + p->beta), (float)2.0};
+    const float rv[3] = {(float)(0.05 + 2. * p->beta), (float)(1.5 + p->beta), (float)2.5};
+    for (int k = 0; k < 3; k++) {
+        B.sig[k] = sg[k];
+        B.rverl[k] = rv[k];
+    }
+    B.fene = p->fene;
+    B.force_sel = p->force_sel;
+    B.mono_ev = p->mono_ev;
+    B.chain_HI = p->chain_HI;
+    B.trap_on = !(std::fabs(p->ktrap) < 1e-12);
+    B.kmax = p->sqrt_kmax;
+    B.mixed = p->precision == PS_PREC_MIXED;
+    B.ms_rtol = 1e-12;
+    B.ms_maxit = 10000;
+    b->ktrap_cur = p->ktrap;
+    b->t = p->restart;
+    // per-replica sizes and pool offsets
+    int64_t pos = 0, npos = 0;
+    b->reps.resize(R);
+    for (int r = 0; r < R; r++) {
+        if (Nc[r] < 0 || Npoly[r] < 0 || Nmono[r] < 0 || (Npoly[r] > 0 && Nmono[r] < 1)) {
+            batch_free(b);
+            return bad("batched replicas: negative sizes");
+        }
+        BRep& Q = b->reps[r];
+        Q.Nm = Npoly[r] * Nmono[r];
+        Q.Nc = Nc[r];
+        Q.Np = Q.Nm + Q.Nc;
+        Q.Nmono = std::max(Nmono[r], 1);
+        if (Q.Np < 2) {
+            batch_free(b);
+            return bad("batched replicas: every replica needs at least two beads");
+        }
+        Q.nm3 = 3 * (int64_t)Q.Nm;
+        Q.nc3 = 3 * (int64_t)Q.Nc;
+        Q.n = 3 * (int64_t)Q.Nm + 11 * (int64_t)Q.Nc;
+        Q.nu = 3 * (int64_t)Q.Nm + 6 * (int64_t)Q.Nc;
+        Q.ns = Q.n + Q.nu;
+        // mob()'s last monomer-monomer pair (src/mob.cpp:845-897)
+        Q.si = Q.sj = -1;
+        for (int i = Q.Nm - 2; i >= 0 && Q.si < 0; i--)
+            for (int j = Q.Nm - 1; j > i; j--) {
+                if (p->chain_HI && i / Q.Nmono == j / Q.Nmono) continue;
+                Q.si = i;
+                Q.sj = j;
+                break;
+            }
+        const int64_t n3 = 3 * (int64_t)Q.Np, n5 = 5 * (int64_t)Q.Nc;
+        const int64_t sizes[BO_NOFF] = {n3, n3, n3, n3, 4 * (int64_t)Q.Nc, 3 * (int64_t)Q.Nc, Q.ns, Q.ns, Q.ns, Q.ns,
+                                        Q.nu, Q.nu, Q.nu, Q.nu, Q.nu, Q.nu, Q.n, Q.n, Q.n, (int64_t)p->sqrt_kmax * Q.n,
+                                        n5, 9 * n5, n5 * n5, 18, 0};
+        for (int k = 0; k < BO_NOFF; k++) {
+            if (k == BO_NZ) continue;
+            Q.off[k] = pos;
+            pos += pad8(std::max<int64_t>(sizes[k], 1));
+        }
+        Q.off[BO_NZ] = npos;
+        npos += pad8(Q.n + Q.nu);
+    }
+    b->pool_n = pos;
+    b->noise_n = npos;
+    // workgroups take replicas by decreasing work (the applies grow as Np^2)
+    b->order.resize(R);
+    std::iota(b->order.begin(), b->order.end(), 0);
+    std::stable_sort(b->order.begin(), b->order.end(),
+                     [&](int a, int c) { return b->reps[a].Np > b->reps[c].Np; });
+    auto fail_hip = [&](hipError_t e) {
+        g_batch_create_error = std::string("batched replicas: ") + hipGetErrorString(e);
+        batch_free(b);
+        return PS_ERR_HIP;
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return fail_hip(e);
+    if ((e = hipMalloc(&b->pool, sizeof(double) * (size_t)b->pool_n)) != hipSuccess) return fail_hip(e);
+    if ((e = hipMemset(b->pool, 0, sizeof(double) * (size_t)b->pool_n)) != hipSuccess) return fail_hip(e);
+    if ((e = hipMalloc(&b->noise_d, sizeof(double) * (size_t)std::max<int64_t>(b->noise_n, 1))) != hipSuccess)
+        return fail_hip(e);
+    for (int k = 0; k < 2; k++) {
+        if ((e = hipHostMalloc(&b->noise_h[k], sizeof(double) * (size_t)std::max<int64_t>(b->noise_n, 1),
+                               hipHostMallocDefault)) != hipSuccess)
+            return fail_hip(e);
+        if ((e = hipEventCreateWithFlags(&b->copied[k], hipEventDisableTiming)) != hipSuccess) return fail_hip(e);
+    }
+    if ((e = hipMalloc(&b->d_reps, sizeof(BRep) * R)) != hipSuccess) return fail_hip(e);
+    if ((e = hipMalloc(&b->d_order, sizeof(int) * R)) != hipSuccess) return fail_hip(e);
+    if ((e = hipMalloc(&b->d_stats, sizeof(int) * 4 * R)) != hipSuccess) return fail_hip(e);
+    if ((e = hipMemcpy(b->d_reps, b->reps.data(), sizeof(BRep) * R, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail_hip(e);
+    if ((e = hipMemcpy(b->d_order, b->order.data(), sizeof(int) * R, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail_hip(e);
+    if ((e = hipMemset(b->d_stats, 0, sizeof(int) * 4 * R)) != hipSuccess) return fail_hip(e);
+    // initial state: identity quaternions, the carried tile after fill_self (the identity)
+    for (int r = 0; r < R; r++) {
+        const BRep& Q = b->reps[r];
+        std::vector<double> q(4 * (size_t)Q.Nc, 0.0);
+        for (int i = 0; i < Q.Nc; i++) q[4 * i] = 1.0;
+        const double st0[18] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 1, 0, 0, 0, 1, 0, 0, 0, 1};
+        if (Q.Nc > 0 &&
+            (e = hipMemcpy(b->pool + Q.off[BO_Q], q.data(), sizeof(double) * q.size(), hipMemcpyHostToDevice)) !=
+                hipSuccess)
+            return fail_hip(e);
+        if ((e = hipMemcpy(b->pool + Q.off[BO_ST], st0, sizeof(st0), hipMemcpyHostToDevice)) != hipSuccess)
+            return fail_hip(e);
+    }
+    // each replica's noise stream, seeded as a single context's (seed + llround(restart / dt))
+    b->rng.resize(R);
+    for (int r = 0; r < R; r++) {
+        b->rng[r].reset(new NormalStream());
+        b->rng[r]->set_max_parts(1);
+        b->rng[r]->seed(seeds[r] + (uint64_t)std::llround(p->restart / p->dt));
+    }
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    b->nthreads = (int)std::min<unsigned>(16, std::max(1u, hw / 2));
+    *out = b;
+    return PS_OK;
+}
+
+const char* ps_batch_last_error(const void* h) {
+    return h ? static_cast<const Batch*>(h)->err.c_str() : g_batch_create_error.c_str();
+}
+
+ps_err ps_batch_destroy(void* h) {
+    batch_free(static_cast<Batch*>(h));
+    return PS_OK;
+}
+
+ps_err ps_batch_sizes(void* h, int r, int64_t* out6) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || !out6 || r < 0 || r >= b->R) return PS_ERR_ARG;
+    const BRep& Q = b->reps[r];
+    const int64_t v[6] = {Q.Np, Q.Nc, Q.Nm, Q.n, Q.nu, Q.ns};
+    std::copy(v, v + 6, out6);
+    return PS_OK;
+}
+
+ps_err ps_batch_set_state(void* h, int r, const double* x0, const double* q0) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || !x0 || r < 0 || r >= b->R) return PS_ERR_ARG;
+    const BRep& Q = b->reps[r];
+    BPS_HIP(b, hipStreamSynchronize(b->stream));
+    BPS_HIP(b, hipMemcpy(b->pool + Q.off[BO_X], x0, sizeof(double) * 3 * (size_t)Q.Np, hipMemcpyHostToDevice));
+    if (Q.Nc > 0) {
+        // trap centres = initial colloid positions on a fresh start (src/Stokes.cpp:279-284)
+        std::vector<double> c0(3 * (size_t)Q.Nc, 0.0);
+        if (b->p.restart == 0.0) std::copy(x0 + 3 * (size_t)Q.Nm, x0 + 3 * (size_t)Q.Np, c0.begin());
+        BPS_HIP(b, hipMemcpy(b->pool + Q.off[BO_XP0], c0.data(), sizeof(double) * c0.size(), hipMemcpyHostToDevice));
+        std::vector<double> q(4 * (size_t)Q.Nc, 0.0);
+        for (int i = 0; i < Q.Nc; i++)
+            for (int k = 0; k < 4; k++) q[4 * i + k] = q0 ? q0[4 * i + k] : (k == 0 ? 1.0 : 0.0);
+        BPS_HIP(b, hipMemcpy(b->pool + Q.off[BO_Q], q.data(), sizeof(double) * q.size(), hipMemcpyHostToDevice));
+    }
+    return PS_OK;
+}
+
+ps_err ps_batch_get_state(void* h, int r, double* x, double* q, double* t) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || r < 0 || r >= b->R) return PS_ERR_ARG;
+    const BRep& Q = b->reps[r];
+    BPS_HIP(b, hipStreamSynchronize(b->stream));
+    if (x) BPS_HIP(b, hipMemcpy(x, b->pool + Q.off[BO_X], sizeof(double) * 3 * (size_t)Q.Np, hipMemcpyDeviceToHost));
+    if (q && Q.Nc > 0)
+        BPS_HIP(b, hipMemcpy(q, b->pool + Q.off[BO_Q], sizeof(double) * 4 * (size_t)Q.Nc, hipMemcpyDeviceToHost));
+    if (t) *t = b->t;
+    return PS_OK;
+}
+
+// The last step's velocity vectors of replica r (nu each; any may be null).
+ps_err ps_batch_get_vectors(void* h, int r, double* up, double* v_det_n, double* v_det_c, double* v_brown,
+                            double* udiff) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || r < 0 || r >= b->R) return PS_ERR_ARG;
+    const BRep& Q = b->reps[r];
+    BPS_HIP(b, hipStreamSynchronize(b->stream));
+    const std::pair<double*, int> outs[5] = {{up, BO_UP}, {v_det_n, BO_VDN}, {v_det_c, BO_VDC}, {v_brown, BO_VB},
+                                             {udiff, BO_UD}};
+    for (const auto& o : outs)
+        if (o.first)
+            BPS_HIP(b, hipMemcpy(o.first, b->pool + Q.off[o.second], sizeof(double) * (size_t)Q.nu,
+                                 hipMemcpyDeviceToHost));
+    return PS_OK;
+}
+
+// nsteps of every replica: one persistent kernel per step; a host thread draws step k + 1's noise
+// while the GPU runs step k (two pinned slots; a slot is refilled only after its copy completed).
+ps_err ps_batch_run(void* h, int nsteps) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || nsteps < 0) return PS_ERR_ARG;
+    if (nsteps == 0) return PS_OK;
+    const bool noisy = b->p.kT > 0;
+    std::mutex mu;
+    std::condition_variable cv;
+    int filled = 0, consumed = 0;
+    bool abort = false;
+    std::thread producer;
+    if (noisy) {
+        producer = std::thread([&] {
+            for (int s = 0; s < nsteps; s++) {
+                const int k = s & 1;
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    cv.wait(lk, [&] { return abort || consumed >= s - 1; });
+                    if (abort) return;
+                }
+                if (s >= 2) (void)hipEventSynchronize(b->copied[k]);
+                b->fill(k);
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    filled = s + 1;
+                }
+                cv.notify_all();
+            }
+        });
+    }
+    ps_err rc = PS_OK;
+    for (int s = 0; s < nsteps && rc == PS_OK; s++) {
+        const int k = s & 1;
+        b->t += b->p.dt;
+        // trapping_forces: weaken_trap replaces ktrap permanently after tstart + trun
+        if (b->B.trap_on && b->t > b->p.tstart + b->p.trun && b->p.weaken_trap >= 0) b->ktrap_cur = b->p.weaken_trap;
+        if (noisy) {
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return filled > s; });
+            }
+            hipError_t e = hipMemcpyAsync(b->noise_d, b->noise_h[k], sizeof(double) * (size_t)b->noise_n,
+                                          hipMemcpyHostToDevice, b->stream);
+            if (e == hipSuccess) e = hipEventRecord(b->copied[k], b->stream);
+            if (e != hipSuccess) {
+                b->set_error(std::string("noise copy: ") + hipGetErrorString(e));
+                rc = PS_ERR_HIP;
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                consumed = s + 1;
+            }
+            cv.notify_all();
+            if (rc != PS_OK) break;
+        }
+        rc = batch_launch_step(b->stream, b->B, b->d_reps, b->d_order, b->R, b->pool, b->noise_d, b->ktrap_cur,
+                               b->d_stats);
+        if (rc != PS_OK) b->set_error("batched step: kernel launch failed");
+    }
+    if (noisy) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            abort = true;
+        }
+        cv.notify_all();
+        producer.join();
+    }
+    if (rc != PS_OK) return rc;
+    BPS_HIP(b, hipStreamSynchronize(b->stream));
+    return PS_OK;
+}
+
+// Per replica: [stresslet MINRES iterations of the last step, Lanczos iterations, negative Ritz
+// values, 0] (4 R ints).
+ps_err ps_batch_stats(void* h, int* out) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || !out) return PS_ERR_ARG;
+    BPS_HIP(b, hipStreamSynchronize(b->stream));
+    BPS_HIP(b, hipMemcpy(out, b->d_stats, sizeof(int) * 4 * (size_t)b->R, hipMemcpyDeviceToHost));
+    return PS_OK;
+}
+
+// One operation on every replica at its current positions (for checks), inputs and outputs
+// concatenated over the replicas: op 0 / 1 y = M f (FP64 / float monomer tiles; n each, after
+// op 4), op 2 the saddle solve (b, x: ns each, after op 4), op 3 W = M^{1/2} xi (n each, FP64,
+// after op 4), op 4 mob() at the current positions (no input or output), op 5 the force rhs (ns).
+// iters (R ints, may be null): op 2 stresslet MINRES iterations, op 3 Lanczos iterations.
+ps_err ps_batch_op(void* h, int op, const double* in, double* out, int* iters) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || op < 0 || op > 5) return PS_ERR_ARG;
+    std::vector<int64_t> io(b->R, 0);
+    int64_t tot = 0;
+    for (int r = 0; r < b->R; r++) {
+        io[r] = tot;
+        const BRep& Q = b->reps[r];
+        tot += (op == 2 || op == 5) ? Q.ns : (op == 4 ? 0 : Q.n);
+    }
+    if (op != 4 && (!out || (op != 5 && !in))) return PS_ERR_ARG;
+    double *din = nullptr, *dout = nullptr;
+    int64_t* dio = nullptr;
+    int* dit = nullptr;
+    auto cleanup = [&]() {
+        for (void* q : {(void*)din, (void*)dout, (void*)dio, (void*)dit})
+            if (q) (void)hipFree(q);
+    };
+    ps_err rc = PS_OK;
+    do {
+        const size_t bytes = sizeof(double) * (size_t)std::max<int64_t>(tot, 1);
+        if (hipMalloc(&din, bytes) != hipSuccess || hipMalloc(&dout, bytes) != hipSuccess ||
+            hipMalloc(&dio, sizeof(int64_t) * b->R) != hipSuccess || hipMalloc(&dit, sizeof(int) * b->R) != hipSuccess) {
+            b->set_error("batch op: allocation failed");
+            rc = PS_ERR_HIP;
+            break;
+        }
+        if ((in && tot > 0 && hipMemcpy(din, in, bytes, hipMemcpyHostToDevice) != hipSuccess) ||
+            hipMemcpy(dio, io.data(), sizeof(int64_t) * b->R, hipMemcpyHostToDevice) != hipSuccess) {
+            b->set_error("batch op: copy failed");
+            rc = PS_ERR_HIP;
+            break;
+        }
+        BConst B = b->B;
+        B.ktrap = b->ktrap_cur;
+        rc = batch_launch_op(b->stream, B, b->d_reps, b->R, b->pool, op, din, dout, dio, dit);
+        if (rc != PS_OK) {
+            b->set_error("batch op: kernel launch failed");
+            break;
+        }
+        if (hipStreamSynchronize(b->stream) != hipSuccess ||
+            (out && tot > 0 && hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess) ||
+            (iters && hipMemcpy(iters, dit, sizeof(int) * b->R, hipMemcpyDeviceToHost) != hipSuccess)) {
+            b->set_error("batch op: execution failed");
+            rc = PS_ERR_HIP;
+        }
+    } while (false);
+    cleanup();
+    return rc;
+}
+
+}  // extern "C"

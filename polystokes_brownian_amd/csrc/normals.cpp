@@ -172,7 +172,7 @@ void NormalStream::fill(double* out, int64_t n) {
     if (need == 0) return;
     const int64_t np = (need + 1) / 2;   // pairs; with an odd count the last keeps its x value
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const int maxparts = (int)std::min<unsigned>(8, std::max(1u, hw / 2));
+    const int maxparts = (int)std::min<unsigned>((unsigned)max_parts_, std::max(1u, hw / 2));
     auto parts_for = [&](int64_t attempts) {
         return (int)std::max<int64_t>(1, std::min<int64_t>(maxparts, attempts / ATTEMPTS_PER_PART_MIN));
     };

@@ -36,10 +36,13 @@ public:
     void fill(double* out, int64_t n);
     // attempts a fill first takes per pair still missing (any value > 0 gives the same stream)
     void set_margin(double m) { margin_ = m; }
+    // cap on the host threads a fill uses (1: the caller's thread only)
+    void set_max_parts(int m) { max_parts_ = m < 1 ? 1 : m; }
 
 private:
     uint64_t mt_[312] = {};        // MT state; always at the end of a block (all 312 words used)
     double margin_ = 1.35;
+    int max_parts_ = 8;
     bool saved_ok_ = false;
     double saved_ = 0.0;
     std::vector<uint64_t> buf_;    // buffered draws [head_, tail_)

@@ -76,6 +76,18 @@ def lib():
         L.ps_get_cell_list.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), dp]
         L.ps_lub_interp.argtypes = [dp, ip, ip, C.c_double, dp]
         L.ps_lub_slopes.argtypes = [dp, ip, ip, dp]
+        L.ps_batch_create.argtypes = [C.POINTER(PsParams), ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip),
+                                      C.POINTER(C.c_uint64), C.POINTER(vp)]
+        L.ps_batch_destroy.argtypes = [vp]
+        L.ps_batch_last_error.argtypes = [vp]
+        L.ps_batch_last_error.restype = C.c_char_p
+        L.ps_batch_sizes.argtypes = [vp, ip, C.POINTER(C.c_int64)]
+        L.ps_batch_set_state.argtypes = [vp, ip, dp, dp]
+        L.ps_batch_get_state.argtypes = [vp, ip, dp, dp, dp]
+        L.ps_batch_get_vectors.argtypes = [vp, ip, dp, dp, dp, dp, dp]
+        L.ps_batch_run.argtypes = [vp, ip]
+        L.ps_batch_stats.argtypes = [vp, C.POINTER(ip)]
+        L.ps_batch_op.argtypes = [vp, ip, dp, dp, C.POINTER(ip)]
         _lib = L
     return _lib
 
@@ -413,3 +425,90 @@ class PolyStokes:
     @property
     def context(self):
         return self._context()
+
+
+class Batch:
+    """Batched replicas (ps_batch_*): R independent systems sharing the parameters `params` (a dict
+    of ps_params fields; unbounded, full monomer hydrodynamics), each given by a dict with Nc,
+    Npoly, Nmono, seed and, optionally, x0 / q0.  Each replica reproduces a single Context run with
+    its own seed."""
+
+    def __init__(self, params, replicas):
+        p = default_params()
+        params_from_dict(params, p)
+        self.R = len(replicas)
+        ia = lambda key: (C.c_int * self.R)(*[int(r[key]) for r in replicas])
+        seeds = (C.c_uint64 * self.R)(*[int(r["seed"]) for r in replicas])
+        h = C.c_void_p()
+        rc = lib().ps_batch_create(C.byref(p), self.R, ia("Nc"), ia("Npoly"), ia("Nmono"), seeds, C.byref(h))
+        if rc != 0:
+            raise PsError(rc, lib().ps_batch_last_error(None).decode())
+        self.h = h
+        self.sizes = []
+        for r in range(self.R):
+            v = (C.c_int64 * 6)()
+            self._check(lib().ps_batch_sizes(self.h, r, v))
+            self.sizes.append(dict(zip(("Np", "Nc", "Nm", "n", "nu", "ns"), list(v))))
+        for r, rep in enumerate(replicas):
+            if rep.get("x0") is not None:
+                self.set_state(r, rep["x0"], rep.get("q0"))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise PsError(rc, lib().ps_batch_last_error(self.h).decode())
+
+    def set_state(self, r, x0, q0=None):
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        q0 = None if q0 is None else np.ascontiguousarray(q0, dtype=np.float64)
+        self._check(lib().ps_batch_set_state(self.h, r, _p(x0), None if q0 is None else _p(q0)))
+
+    def state(self, r):
+        s = self.sizes[r]
+        x, q, t = np.zeros(3 * s["Np"]), np.zeros(max(4 * s["Nc"], 1)), C.c_double()
+        self._check(lib().ps_batch_get_state(self.h, r, _p(x), _p(q), C.byref(t)))
+        return x, q[: 4 * s["Nc"]], t.value
+
+    def vectors(self, r):
+        nu = self.sizes[r]["nu"]
+        out = {k: np.zeros(nu) for k in ("up", "v_det_n", "v_det_c", "v_brown", "udiff")}
+        self._check(lib().ps_batch_get_vectors(self.h, r, *[_p(out[k]) for k in
+                                                             ("up", "v_det_n", "v_det_c", "v_brown", "udiff")]))
+        return out
+
+    def run(self, nsteps):
+        self._check(lib().ps_batch_run(self.h, int(nsteps)))
+
+    def stats(self):
+        out = (C.c_int * (4 * self.R))()
+        self._check(lib().ps_batch_stats(self.h, out))
+        return np.array(list(out), dtype=np.int64).reshape(self.R, 4)
+
+    def op(self, op, inputs=None):
+        """ps_batch_op: a list of per-replica input vectors in, per-replica outputs (and the
+        iteration counts) out."""
+        key = "ns" if op in (2, 5) else "n"
+        lens = [s[key] for s in self.sizes]
+        offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        tot = int(offs[-1])
+        xin = np.zeros(max(tot, 1))
+        if inputs is not None:
+            for r, v in enumerate(inputs):
+                xin[offs[r]:offs[r + 1]] = v
+        xout = np.zeros(max(tot, 1))
+        its = (C.c_int * self.R)()
+        self._check(lib().ps_batch_op(self.h, int(op), _p(xin), _p(xout), its))
+        if op == 4:
+            return None, list(its)
+        return [xout[offs[r]:offs[r + 1]].copy() for r in range(self.R)], list(its)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ps_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

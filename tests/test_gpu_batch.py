@@ -1,0 +1,95 @@
+"""GPU parity of the batched replicas (csrc/batch.hip, through the C ABI): each replica of a batch
+against the oracle run as a single system with the same parameters, configuration and seed.
+
+Tolerances: applies and forces 1e-12 relative; solves, the Lanczos square root (same fixed number
+of iterations) and 3-step trajectories 1e-10; the FP32-tile apply against FP64 1e-5.
+"""
+import numpy as np
+import pytest
+
+from polystokes_brownian_amd import configs
+from polystokes_brownian_amd.polystokes import Batch
+
+pytestmark = pytest.mark.gpu
+
+# uneven small replicas (Nc, Npoly, Nmono): one colloid-only, one chain-only, a single chain
+SIZES = [(3, 4, 20), (1, 2, 20), (6, 9, 20), (2, 1, 20), (4, 0, 20), (0, 3, 20)]
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300)
+
+
+def make(sizes=SIZES, **over):
+    params, reps = configs.replica_workload(sizes=sizes, seed=2000)
+    params.update(sqrt_kmax=12, **over)
+    return params, reps
+
+
+def oracle_for(params, rep):
+    from oracle.pyoracle import Oracle
+    pd = dict(params, Nc=rep["Nc"], Npoly=rep["Npoly"], Nmono_per_chain=rep["Nmono"], Nm=rep["Npoly"] * rep["Nmono"],
+              Np=rep["Npoly"] * rep["Nmono"] + rep["Nc"], seed=rep["seed"], solver=1)
+    o = Oracle(pd)
+    o.initial_configuration(rep["x0"])
+    return o
+
+
+def test_batch_operations_match_the_oracle():
+    params, reps = make()
+    b = Batch(params, reps)
+    rng = np.random.default_rng(11)
+    b.op(4)   # mob() at the initial positions
+    fs = [rng.normal(size=s["n"]) for s in b.sizes]
+    bs = [rng.normal(size=s["ns"]) for s in b.sizes]
+    xis = [rng.normal(size=s["n"]) for s in b.sizes]
+    y64, _ = b.op(0, fs)
+    y32, _ = b.op(1, fs)
+    xs, _ = b.op(2, bs)
+    ws, lz = b.op(3, xis)
+    rh, _ = b.op(5)
+    for r, rep in enumerate(reps):
+        o = oracle_for(params, rep)
+        o.check_dist()
+        o.mob()
+        assert rel(y64[r], o.mob_mult(fs[r])) < 1e-12, r
+        assert rel(y32[r], y64[r]) < 1e-5, r
+        assert rel(xs[r], o.saddle_solve(bs[r])[0]) < 1e-10, r
+        assert rel(ws[r], o.sqrt_apply(xis[r])[0]) < 1e-10, r
+        assert rel(rh[r], o.rhs()) < 1e-12, r
+        assert lz[r] == min(12, b.sizes[r]["n"]) or lz[r] < 12
+    b.close()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batch_steps_match_single_system_runs(mixed):
+    """Three steps of every replica against the oracle's single-system run (FP64 form at 1e-10); the
+    mixed form's FP32 square-root tiles stay within 1e-5 of it."""
+    params, reps = make()
+    b = Batch(params, reps)
+    b.run(3)
+    bm = None
+    if mixed:
+        bm = Batch(dict(params, precision=1), reps)
+        bm.run(3)
+    for r, rep in enumerate(reps):
+        o = oracle_for(params, rep)
+        o.run(3)
+        xg, qg, tg = b.state(r)
+        xo, qo, to = o.state()
+        assert abs(tg - to) < 1e-15
+        assert rel(xg, xo) < 1e-10, (r, rel(xg, xo))
+        if rep["Nc"]:
+            assert rel(qg, qo) < 1e-10
+        vg, vo = b.vectors(r), o.vectors()
+        for k in ("up", "v_det_n", "v_det_c", "v_brown", "udiff"):
+            assert rel(vg[k], vo[k]) < 1e-10, (r, k, rel(vg[k], vo[k]))
+        if mixed:
+            xm, _, _ = bm.state(r)
+            # the error relative to the displacement of the three steps
+            assert np.linalg.norm(xm - xo) / np.linalg.norm(xo - rep["x0"]) < 1e-4, r
+    st = b.stats()
+    assert (st[:, 1] > 0).all()
+    b.close()
+    if bm:
+        bm.close()
