@@ -228,6 +228,91 @@ def arrow_bench(args, configs, Context, steps):
     }
 
 
+def batch_bench(args, configs):
+    """Batched replicas (256 unbounded systems, N_r log-uniform in [100, 3000], 10% probes, 20-bead
+    chains, C3 parameters, seeds 1000 + r): one persistent kernel per step.  Forms: mixed (FP32
+    monomer tiles in the 32-iteration Lanczos square root, FP64 solves and recurrences; headline) and
+    FP64.  value = aggregate bead-steps/s = sum_r N_r x steps / wall.  Accuracy: one step of each
+    form from the same state and noise, per-replica relative velocity error of mixed against FP64.
+    Roofline (VALU-bound tiles): algorithmic work per step = sum_r 37 x 2 n_r^2 flop (5 full
+    applies of the saddle solves + 32 Lanczos applies; the stresslet-row applies are not counted),
+    over the step time.  CPU baseline: the oracle (one host core) stepping a sample of replicas,
+    scaled to the batch by sum N_r^2."""
+    from polystokes_brownian_amd.polystokes import Batch
+    t0 = time.perf_counter()
+    params, reps = configs.replica_workload(R=args.batch_replicas)
+    N = np.array([r["Nc"] + r["Npoly"] * r["Nmono"] for r in reps], dtype=np.int64)
+    log(f"batch: {len(reps)} replicas, {N.sum()} beads (max {N.max()}), configurations {time.perf_counter() - t0:.1f}s")
+    forms = {"mixed": dict(precision=1), "fp64": dict(precision=0)}
+    steps = args.batch_steps
+    res = {}
+    ups = {}
+    for form, over in forms.items():
+        b = Batch(dict(params, **over), reps)
+        t1 = time.perf_counter()
+        b.run(1)   # accuracy step (also the warmup)
+        log(f"batch {form}: first step {time.perf_counter() - t1:.3f}s")
+        ups[form] = [b.vectors(r)["up"] for r in range(b.R)]
+        t1 = time.perf_counter()
+        b.run(steps)
+        dt = time.perf_counter() - t1
+        st = b.stats()
+        b.close()
+        res[form] = dict(value=float(N.sum()) * steps / dt, ms_per_step=1e3 * dt / steps,
+                         lanczos_iterations_max=int(st[:, 1].max()),
+                         minres_iterations_max=int(st[:, 0].max()))
+        log(f"batch {form}: {res[form]['value']:.4g} bead-steps/s, {res[form]['ms_per_step']:.2f} ms/step")
+    errs = np.array([rel(a, c) for a, c in zip(ups["mixed"], ups["fp64"])])
+    n = np.array([3 * (r["Npoly"] * r["Nmono"]) + 11 * r["Nc"] for r in reps], dtype=np.float64)
+    flops = float(np.sum(37.0 * 2.0 * n * n))
+    head = res["mixed"]
+    achieved = flops / (head["ms_per_step"] * 1e-3)
+    cpu = None
+    if not args.no_cpu:
+        from oracle.pyoracle import Oracle, get_threads, set_threads
+        nthr = get_threads()
+        set_threads(1)
+        order = np.argsort(N)
+        sample = [int(order[k]) for k in np.linspace(0, len(order) - 1, 6).astype(int)[:5]]
+        t_cpu = 0.0
+        for r in sample:
+            rep = reps[r]
+            pd = dict(params, Nc=rep["Nc"], Npoly=rep["Npoly"], Nmono_per_chain=rep["Nmono"],
+                      Nm=rep["Npoly"] * rep["Nmono"], Np=int(N[r]), seed=rep["seed"], solver=1)
+            o = Oracle(pd)
+            o.initial_configuration(rep["x0"])
+            t1 = time.perf_counter()
+            o.run(1)
+            t_cpu += time.perf_counter() - t1
+        set_threads(nthr)
+        scale = float(np.sum(N.astype(float) ** 2)) / float(np.sum(N[sample].astype(float) ** 2))
+        t_step = t_cpu * scale
+        cpu = {"value": float(N.sum()) / t_step, "unit": "bead-steps/s", "cores": 1, "kind": "port",
+               "sample": f"one FP64 step of the oracle (block solves, the same Lanczos) for {len(sample)} replicas "
+                         f"(N = {', '.join(str(int(N[r])) for r in sample)}) on one host core, {t_cpu:.2f} s, scaled to "
+                         f"the 256 replicas by sum N_r^2 ({t_step:.1f} s per batch step)"}
+        log(f"batch cpu baseline {cpu}")
+    return {
+        "metric": "aggregate bead-steps/s, 256 batched replicas (FP32 square-root tiles, FP64 recurrences)",
+        "value": head["value"], "unit": "bead-steps/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32+fp64", "data": "synthetic",
+        "config": {"workload": f"batched replicas: {len(reps)} unbounded systems, {int(N.sum())} beads, N_r log-uniform "
+                               f"in [100, 3000] ({int(N.min())}..{int(N.max())}), 10% probes, 20-bead chains, C3 "
+                               f"parameters, seeds 1000 + r, Lanczos m = 32, block-eliminated solves",
+                   "global_batch": int(N.sum()), "seq_len": 1, "parallelism": "single-gpu"},
+        "fp64_value": res["fp64"]["value"], "fp64_ms_per_step": res["fp64"]["ms_per_step"],
+        "mixed_over_fp64": head["value"] / res["fp64"]["value"],
+        "per_replica_vel_rel_err_vs_fp64": {"max": float(errs.max()), "median": float(np.median(errs))},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK["fp32"], "traffic": None,
+                     "kernel": "k_batch_step (the whole step of every replica; VALU-bound tile generation)",
+                     "flops_per_launch": flops, "avg_launch_ms": head["ms_per_step"],
+                     "frac_of_fp32_valu_peak": achieved / PEAK_FP32_VALU},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -250,6 +335,10 @@ def main():
     ap.add_argument("--arrow-steps", type=int, default=300, help="timed steps of the arrowhead case (300 as published)")
     ap.add_argument("--arrow-cpu-steps", type=int, default=20, help="oracle steps of the arrowhead CPU baseline")
     ap.add_argument("--no-arrow", action="store_true", help="omit the arrowhead case from the default line")
+    ap.add_argument("--batch", action="store_true",
+                    help="instead: the batched-replica workload (256 replicas); prints its own JSON line")
+    ap.add_argument("--batch-replicas", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the solve is not sharded across GPUs",
@@ -273,6 +362,9 @@ def main():
 
     if args.dense:
         return dense_bench(args, configs, Context)
+    if args.batch:
+        print(json.dumps(batch_bench(args, configs)), flush=True)
+        return
     if args.arrow:
         line = arrow_bench(args, configs, Context, args.arrow_steps)
         if line is not None:

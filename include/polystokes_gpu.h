@@ -299,8 +299,9 @@ ps_err ps_batch_get_state(void* batch, int r, double* x, double* q, double* t);
 ps_err ps_batch_get_vectors(void* batch, int r, double* up, double* v_det_n, double* v_det_c, double* v_brown,
                             double* udiff);
 ps_err ps_batch_run(void* batch, int nsteps);
-/* Per replica (4 R ints): stresslet MINRES iterations of the last step, Lanczos iterations,
- * negative Ritz values floored, 0. */
+/* Per replica (4 R ints) for the last step: stresslet MINRES iterations, Lanczos iterations,
+ * negative Ritz values floored, saddle solves whose stresslet MINRES stopped at its cap (1000
+ * iterations; the solve continues unconverged, as the reference continues after KSPSolve's cap). */
 ps_err ps_batch_stats(void* batch, int* out);
 /* One operation on every replica at its current positions, vectors concatenated over the
  * replicas: 0 / 1: y = M f (FP64 / FP32 monomer tiles; n each), 2: the saddle solve (ns each),

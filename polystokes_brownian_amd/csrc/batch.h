@@ -20,6 +20,9 @@ enum BatchOff {
     BO_SB,                              // Brownian stresslet (5Nc)
     BO_M5,                              // stresslet MINRES work (9 x 5Nc)
     BO_MSS,                             // dense stresslet block (25 Nc^2)
+    BO_MCC,                             // dense colloid-colloid block, zero diagonal blocks (121 Nc^2)
+    BO_FC,                              // an apply's colloid sources gathered [F T S] per colloid (11Nc)
+    BO_BBV,                             // their colloid-colloid contribution (11Nc)
     BO_ST,                              // carried tile (18)
     BO_NZ,                              // this step's noise (n + nu), in the noise buffer
     BO_NOFF
@@ -40,7 +43,8 @@ struct BConst {
     int kmax;      // Lanczos iterations (an invariant subspace ends it earlier)
     int mixed;     // FP32 monomer tiles in the square root's applies
     double ms_rtol;
-    int ms_maxit;
+    int ms_maxit;   // cap on stresslet MINRES iterations (past it the solve continues unconverged, as the
+                    // reference does after KSPSolve's cap, and the replica's step counts it)
 };
 
 ps_err batch_launch_step(hipStream_t st, const BConst& B, const BRep* reps, const int* order, int R, double* pool,

@@ -179,32 +179,70 @@ __device__ void b_stale(const BConst& B, const BRep& R, const double* xm, double
         for (int k = 0; k < 9; k++) st[9 + k] = cur[k];
     }
 }
-// mob(): snapshot xm = xpd, the carried tile, and the dense stresslet block M_ss (5Nc x 5Nc):
-// self blocks and every colloid pair's m tile (placement of src/mob.cpp:1129-1143).
-__device__ void wg_mob(const BConst& B, const BRep& R, const double* xpd, double* xm, double* st, double* Mss) {
+// mob(): snapshot xm = xpd, the carried tile, and the dense colloid-colloid blocks: M_cc (11Nc x 11Nc,
+// the pair tiles of every ordered colloid pair, zero diagonal blocks) and the stresslet block M_ss
+// (5Nc x 5Nc: self blocks and the pairs' m tiles, placement of src/mob.cpp:1129-1143).  One FTS tile
+// per unordered pair; its two orientations are the role expressions of a colloid pair in
+// mobility(..., AB=false, AA=false) (src/mob.cpp:1042-1143): for target c1 < source c2
+//   U: [a F, -bt^T T, gt S]   W: [b F, c T, ht S]   S: [-gt^T F, ht^T T, m S]
+// and for target c2 > source c1
+//   U: [a^T F, bt T, -gt S]   W: [-b^T F, c^T T, ht S]   S: [gt^T F, ht^T T, m^T S].
+__device__ void wg_mob(const BConst& B, const BRep& R, double* P, const double* xpd, double* xm, double* st) {
     for (int64_t i = threadIdx.x; i < 3 * (int64_t)R.Np; i += BW) xm[i] = xpd[i];
     __syncthreads();
     if (threadIdx.x == 0) b_stale(B, R, xm, st);
     const int Nc = R.Nc, Nm = R.Nm;
-    const int64_t n5 = 5 * (int64_t)Nc;
+    const int64_t n5 = 5 * (int64_t)Nc, n11 = 11 * (int64_t)Nc;
+    double* Mss = P + R.off[BO_MSS];
+    double* Mcc = P + R.off[BO_MCC];
+    for (int64_t idx = threadIdx.x; idx < 121 * (int64_t)Nc; idx += BW) {
+        const int64_t c = idx / 121, q = idx % 121;
+        Mcc[(11 * c + q / 11) * n11 + 11 * c + q % 11] = 0.0;
+    }
+    for (int64_t idx = threadIdx.x; idx < 25 * (int64_t)Nc; idx += BW) {
+        const int64_t c = idx / 25;
+        const int i = (int)(idx % 25) / 5, j = (int)(idx % 5);
+        Mss[(5 * c + i) * n5 + 5 * c + j] =
+            (i == j) ? 9. / 5. : ((i == 0 && j == 4) || (i == 4 && j == 0)) ? 9. / 10. : 0.0;
+    }
     for (int64_t pr = threadIdx.x; pr < (int64_t)Nc * Nc; pr += BW) {
-        const int c = (int)(pr / Nc), o = (int)(pr % Nc);
-        double blk[5][5];
-        if (o == c) {
-            for (int i = 0; i < 5; i++)
-                for (int j = 0; j < 5; j++)
-                    blk[i][j] = (i == j) ? 9. / 5. : ((i == 0 && j == 4) || (i == 4 && j == 0)) ? 9. / 10. : 0.0;
-        } else {
-            const bool role1 = c < o;
-            double e[3], dr, dr_inv;
-            bgeom(xm, Nm + (role1 ? c : o), Nm + (role1 ? o : c), e, dr, dr_inv);
-            BBT T;
-            tile_bb(dr_inv, e, T);
-            for (int i = 0; i < 5; i++)
-                for (int j = 0; j < 5; j++) blk[i][j] = role1 ? T.m[i][j] : T.m[j][i];
+        const int c1 = (int)(pr / Nc), c2 = (int)(pr % Nc);
+        if (c1 >= c2) continue;
+        double e[3], dr, dr_inv;
+        bgeom(xm, Nm + c1, Nm + c2, e, dr, dr_inv);
+        BBT T;
+        tile_bb(dr_inv, e, T);
+        double* r1 = Mcc + (11 * (int64_t)c1) * n11 + 11 * (int64_t)c2;   // target c1, source c2
+        double* r2 = Mcc + (11 * (int64_t)c2) * n11 + 11 * (int64_t)c1;   // target c2, source c1
+        for (int i = 0; i < 3; i++) {
+            for (int j = 0; j < 3; j++) {
+                r1[i * n11 + j] = T.a[i][j];
+                r1[i * n11 + 3 + j] = -T.bt[j][i];
+                r1[(3 + i) * n11 + j] = T.b[i][j];
+                r1[(3 + i) * n11 + 3 + j] = T.c[i][j];
+                r2[i * n11 + j] = T.a[j][i];
+                r2[i * n11 + 3 + j] = T.bt[i][j];
+                r2[(3 + i) * n11 + j] = -T.b[j][i];
+                r2[(3 + i) * n11 + 3 + j] = T.c[j][i];
+            }
+            for (int j = 0; j < 5; j++) {
+                r1[i * n11 + 6 + j] = T.gt[i][j];
+                r1[(3 + i) * n11 + 6 + j] = T.ht[i][j];
+                r1[(6 + j) * n11 + i] = -T.gt[i][j];
+                r1[(6 + j) * n11 + 3 + i] = T.ht[i][j];
+                r2[i * n11 + 6 + j] = -T.gt[i][j];
+                r2[(3 + i) * n11 + 6 + j] = T.ht[i][j];
+                r2[(6 + j) * n11 + i] = T.gt[i][j];
+                r2[(6 + j) * n11 + 3 + i] = T.ht[i][j];
+            }
         }
-        for (int i = 0; i < 5; i++)
-            for (int j = 0; j < 5; j++) Mss[(5 * (int64_t)c + i) * n5 + 5 * (int64_t)o + j] = blk[i][j];
+        for (int j = 0; j < 5; j++)
+            for (int k = 0; k < 5; k++) {
+                r1[(6 + j) * n11 + 6 + k] = T.m[j][k];
+                r2[(6 + j) * n11 + 6 + k] = T.m[k][j];
+                Mss[(5 * (int64_t)c1 + j) * n5 + 5 * (int64_t)c2 + k] = T.m[j][k];
+                Mss[(5 * (int64_t)c2 + j) * n5 + 5 * (int64_t)c1 + k] = T.m[k][j];
+            }
     }
     __syncthreads();
 }
@@ -216,8 +254,8 @@ __device__ void wg_mob(const BConst& B, const BRep& R, const double* xpd, double
 // root); colloid-colloid tiles are FP64.  `srows`: only the stresslet rows (nu..n).  sh: LDS,
 // 6 BW doubles (staged monomer sources).
 template <class T>
-__device__ void wg_apply(const BConst& B, const BRep& R, const double* xm, const double* f, const double* st,
-                         double* u, bool srows, double* sh) {
+__device__ void wg_apply(const BConst& B, const BRep& R, double* pool, const double* xm, const double* f,
+                         const double* st, double* u, bool srows, double* sh) {
     const int Nm = R.Nm, Nc = R.Nc, t = threadIdx.x;
     const int64_t nm3 = R.nm3, nc3 = R.nc3, nu = R.nu;
     const T tb2 = (T)(B.two_beta * B.two_beta);
@@ -311,15 +349,36 @@ __device__ void wg_apply(const BConst& B, const BRep& R, const double* xm, const
             u[3 * i + 2] = B.beta_inv * f[3 * i + 2] + ((double)uz + (double)vz);
         }
     }
-    // colloid rows: monomer sources (AB: rotation and stresslet rows, the sum of monomer forces for
-    // the transposed carried tile) and colloid sources (the full FP64 FTS tile)
-    for (int c = t; c < Nc; c += BW) {
+    // colloid rows.  Colloid sources: the stored M_cc against f's colloid entries gathered as
+    // [F T S] per colloid (one wave per row).  Monomer sources (AB: rotation and stresslet rows, and
+    // the sum of monomer forces for the transposed carried tile): one wave per colloid, lanes over
+    // the monomers.
+    const int64_t n11 = 11 * (int64_t)Nc;
+    double* fc = pool + R.off[BO_FC];
+    double* bbv = pool + R.off[BO_BBV];
+    const double* Mcc = pool + R.off[BO_MCC];
+    const int lane = t & 63, wv = t >> 6;
+    for (int64_t idx = t; idx < n11; idx += BW) {
+        const int64_t o = idx / 11;
+        const int k = (int)(idx % 11);
+        fc[idx] = k < 3 ? f[nm3 + 3 * o + k] : k < 6 ? f[nm3 + nc3 + 3 * o + (k - 3)] : f[nu + 5 * o + (k - 6)];
+    }
+    __syncthreads();
+    for (int64_t row = wv; row < n11; row += NW) {
+        const double* mr = Mcc + row * n11;
+        double acc = 0.0;
+        for (int64_t k = lane; k < n11; k += 64) acc += mr[k] * fc[k];
+        acc = wsum(acc);
+        if (lane == 0) bbv[row] = acc;
+    }
+    __syncthreads();
+    for (int c = wv; c < Nc; c += NW) {
         const int g = Nm + c;
         const T xc = (T)xm[3 * g], yc = (T)xm[3 * g + 1], zc = (T)xm[3 * g + 2];
         T acc[11];
         for (int k = 0; k < 11; k++) acc[k] = 0;
         T fsx = 0, fsy = 0, fsz = 0;
-        for (int m = 0; m < Nm; m++) {
+        for (int m = lane; m < Nm; m += 64) {
             const T dx = (T)xm[3 * m] - xc, dy = (T)xm[3 * m + 1] - yc, dz = (T)xm[3 * m + 2] - zc;
             const T r2 = dx * dx + dy * dy + dz * dz;
             const T rinv = ps_rsqrt(r2);
@@ -349,63 +408,11 @@ __device__ void wg_apply(const BConst& B, const BRep& R, const double* xm, const
             acc[9] += (T)2 * G12;
             acc[10] += G11 - G22;
         }
-        double bb[11];
-        for (int k = 0; k < 11; k++) bb[k] = 0.0;
-        for (int o = 0; o < Nc; o++) {
-            if (o == c) continue;
-            const bool role1 = c < o;
-            const int c1 = role1 ? c : o, c2 = role1 ? o : c;
-            double e[3], dr, dr_inv;
-            bgeom(xm, Nm + c1, Nm + c2, e, dr, dr_inv);
-            BBT Tt;
-            tile_bb(dr_inv, e, Tt);
-            double Fo[3], To[3], So[5];
-            for (int k = 0; k < 3; k++) Fo[k] = f[nm3 + 3 * o + k];
-            for (int k = 0; k < 3; k++) To[k] = f[nm3 + nc3 + 3 * o + k];
-            for (int k = 0; k < 5; k++) So[k] = f[nu + 5 * o + k];
-            if (role1) {
-                for (int i = 0; i < 3; i++) {
-                    double uu = 0.0, w = 0.0;
-                    for (int j = 0; j < 3; j++) {
-                        uu += Tt.a[i][j] * Fo[j] - Tt.bt[j][i] * To[j];
-                        w += Tt.b[i][j] * Fo[j] + Tt.c[i][j] * To[j];
-                    }
-                    for (int j = 0; j < 5; j++) {
-                        uu += Tt.gt[i][j] * So[j];
-                        w += Tt.ht[i][j] * So[j];
-                    }
-                    bb[i] += uu;
-                    bb[3 + i] += w;
-                }
-                for (int j = 0; j < 5; j++) {
-                    double v = 0.0;
-                    for (int i = 0; i < 3; i++) v += -Tt.gt[i][j] * Fo[i] + Tt.ht[i][j] * To[i];
-                    for (int k = 0; k < 5; k++) v += Tt.m[j][k] * So[k];
-                    bb[6 + j] += v;
-                }
-            } else {
-                for (int i = 0; i < 3; i++) {
-                    double uu = 0.0, w = 0.0;
-                    for (int j = 0; j < 3; j++) {
-                        uu += Tt.a[j][i] * Fo[j] + Tt.bt[i][j] * To[j];
-                        w += -Tt.b[j][i] * Fo[j] + Tt.c[j][i] * To[j];
-                    }
-                    for (int j = 0; j < 5; j++) {
-                        uu += -Tt.gt[i][j] * So[j];
-                        w += Tt.ht[i][j] * So[j];
-                    }
-                    bb[i] += uu;
-                    bb[3 + i] += w;
-                }
-                for (int j = 0; j < 5; j++) {
-                    double v = 0.0;
-                    for (int i = 0; i < 3; i++) v += Tt.gt[i][j] * Fo[i] + Tt.ht[i][j] * To[i];
-                    for (int k = 0; k < 5; k++) v += Tt.m[k][j] * So[k];
-                    bb[6 + j] += v;
-                }
-            }
-        }
-        const double fs[3] = {(double)fsx, (double)fsy, (double)fsz};
+        double av[11];
+        for (int k = 3; k < 11; k++) av[k] = wsum((double)acc[k]);
+        const double fs[3] = {wsum((double)fsx), wsum((double)fsy), wsum((double)fsz)};
+        if (lane != 0) continue;
+        const double* bb = bbv + 11 * (int64_t)c;
         if (!srows) {
             for (int comp = 0; comp < 3; comp++) {
                 const int64_t r = nm3 + 3 * (int64_t)c + comp;
@@ -414,7 +421,7 @@ __device__ void wg_apply(const BConst& B, const BRep& R, const double* xm, const
             }
             for (int comp = 0; comp < 3; comp++) {
                 const int64_t r = nm3 + nc3 + 3 * (int64_t)c + comp;
-                u[r] = (3. / 4.) * f[r] + (double)acc[3 + comp] + bb[3 + comp];
+                u[r] = (3. / 4.) * f[r] + av[3 + comp] + bb[3 + comp];
             }
         }
         const double* S = f + nu + 5 * (int64_t)c;
@@ -422,7 +429,7 @@ __device__ void wg_apply(const BConst& B, const BRep& R, const double* xm, const
             double self = (9. / 5.) * S[comp];
             if (comp == 0) self += (9. / 10.) * S[4];
             if (comp == 4) self += (9. / 10.) * S[0];
-            u[nu + 5 * (int64_t)c + comp] = self + (double)acc[6 + comp] + bb[6 + comp];
+            u[nu + 5 * (int64_t)c + comp] = self + av[6 + comp] + bb[6 + comp];
         }
     }
     __syncthreads();
@@ -524,13 +531,13 @@ __device__ int wg_block_solve(const BConst& B, const BRep& R, double* pool, cons
     __syncthreads();
     int its = 0;
     if (n5 > 0) {
-        wg_apply<double>(B, R, xm, x, st, y, true, sh);
+        wg_apply<double>(B, R, pool, xm, x, st, y, true, sh);
         double* rs = w5 + 8 * n5;
         for (int64_t k = threadIdx.x; k < n5; k += BW) rs[k] = b[nu + k] - y[nu + k];
         __syncthreads();
         its = wg_minres_ss(B, n5, pool + R.off[BO_MSS], rs, x + nu, w5, red);
     }
-    wg_apply<double>(B, R, xm, x, st, y, false, sh);
+    wg_apply<double>(B, R, pool, xm, x, st, y, false, sh);
     for (int64_t i = threadIdx.x; i < nu; i += BW) {
         const double u = y[i] - b[i];
         x[n + i] = u;
@@ -621,7 +628,7 @@ __device__ int wg_sqrt(const BConst& B, const BRep& R, double* pool, const doubl
     int m = 0;
     for (int j = 0; j < kmax; j++) {
         const double* qj = Q + (int64_t)j * n;
-        wg_apply<T>(B, R, xm, qj, st, w, false, sh);
+        wg_apply<T>(B, R, pool, xm, qj, st, w, false, sh);
         if (j > 0) {
             const double bj1 = betad[j - 1];
             const double* qp = Q + (int64_t)(j - 1) * n;
@@ -712,25 +719,27 @@ __global__ __launch_bounds__(BW) void k_batch_step(BStepArgs a) {
     double *rhs = P + R.off[BO_RHS], *X = P + R.off[BO_XS], *Xdet = P + R.off[BO_XDET], *Xd = P + R.off[BO_XD];
     double *up = P + R.off[BO_UP], *vdn = P + R.off[BO_VDN], *vdc = P + R.off[BO_VDC], *vb = P + R.off[BO_VB];
     double *ud = P + R.off[BO_UD], *dv = P + R.off[BO_DV], *W = P + R.off[BO_W], *sb = P + R.off[BO_SB];
-    double *st = P + R.off[BO_ST], *Mss = P + R.off[BO_MSS], *q = P + R.off[BO_Q];
+    double *st = P + R.off[BO_ST], *q = P + R.off[BO_Q];
     const double* xpos0 = P + R.off[BO_XP0];
     const double* noise = a.noise + R.off[BO_NZ];   // n slip normals, then nu RFD normals
     const int t = threadIdx.x;
     const double dt = B.dt, kT = B.kT, eps = 1.0e-3;
-    int its = 0;
+    int its = 0, unconverged = 0;
     auto snapshot = [&]() {
         for (int64_t i = t; i < n3; i += BW) xpd[i] = x[i];
         __syncthreads();
     };
     auto solve = [&](const double* b, double* Xo, double* uo) {
-        its += wg_block_solve(B, R, P, xm, st, b, Xo, uo, sh, red);
+        const int k = wg_block_solve(B, R, P, xm, st, b, Xo, uo, sh, red);
+        its += k;
+        unconverged += k >= B.ms_maxit;
     };
     for (int64_t i = t; i < n3; i += BW) xn[i] = x[i];
     // ---- predictor
     snapshot();
     wg_forces(B, R, xpd, x, xpos0, a.ktrap, rhs);
     __syncthreads();
-    wg_mob(B, R, xpd, xm, st, Mss);
+    wg_mob(B, R, P, xpd, xm, st);
     solve(rhs, Xdet, vdn);
     int lm = 0;
     if (kT > 0) {
@@ -746,7 +755,7 @@ __global__ __launch_bounds__(BW) void k_batch_step(BStepArgs a) {
         for (int64_t i = t; i < n3; i += BW) x[i] += eps * fext[i];
         __syncthreads();
         snapshot();
-        wg_mob(B, R, xpd, xm, st, Mss);
+        wg_mob(B, R, P, xpd, xm, st);
         for (int64_t i = t; i < R.ns; i += BW) rhs[i] = (i < n) ? 0.0 : fext[i - n];
         __syncthreads();
         solve(rhs, Xd, dv);
@@ -754,7 +763,7 @@ __global__ __launch_bounds__(BW) void k_batch_step(BStepArgs a) {
         for (int64_t i = t; i < n3; i += BW) x[i] += -2.0 * eps * fext[i];
         __syncthreads();
         snapshot();
-        wg_mob(B, R, xpd, xm, st, Mss);
+        wg_mob(B, R, P, xpd, xm, st);
         for (int64_t i = t; i < R.ns; i += BW) rhs[i] = (i < n) ? 0.0 : fext[i - n];
         __syncthreads();
         solve(rhs, Xd, ud);
@@ -780,7 +789,7 @@ __global__ __launch_bounds__(BW) void k_batch_step(BStepArgs a) {
     snapshot();
     wg_forces(B, R, xpd, x, xpos0, a.ktrap, rhs);
     __syncthreads();
-    wg_mob(B, R, xpd, xm, st, Mss);
+    wg_mob(B, R, P, xpd, xm, st);
     solve(rhs, Xdet, vdc);
     for (int64_t i = t; i < nu; i += BW) up[i] = 0.5 * (vdn[i] + vdc[i]) + vb[i];
     __syncthreads();
@@ -803,7 +812,7 @@ __global__ __launch_bounds__(BW) void k_batch_step(BStepArgs a) {
         a.stats[4 * r] = its;
         a.stats[4 * r + 1] = lm;
         a.stats[4 * r + 2] = kT > 0 ? nneg_sh : 0;
-        a.stats[4 * r + 3] = 0;
+        a.stats[4 * r + 3] = unconverged;
     }
 }
 
@@ -825,9 +834,9 @@ __global__ __launch_bounds__(BW) void k_batch_op(BConst B, const BRep* reps, dou
     double *x = P + R.off[BO_X], *xpd = P + R.off[BO_XPD], *xm = P + R.off[BO_XM], *st = P + R.off[BO_ST];
     int s0 = 0;
     if (op == 0) {
-        wg_apply<double>(B, R, xm, inr, st, outr, false, sh);
+        wg_apply<double>(B, R, P, xm, inr, st, outr, false, sh);
     } else if (op == 1) {
-        wg_apply<float>(B, R, xm, inr, st, outr, false, sh);
+        wg_apply<float>(B, R, P, xm, inr, st, outr, false, sh);
     } else if (op == 2) {
         s0 = wg_block_solve(B, R, P, xm, st, inr, outr, nullptr, sh, red);
     } else if (op == 3) {
@@ -835,7 +844,7 @@ __global__ __launch_bounds__(BW) void k_batch_op(BConst B, const BRep* reps, dou
     } else if (op == 4) {
         for (int64_t i = threadIdx.x; i < 3 * (int64_t)R.Np; i += BW) xpd[i] = x[i];
         __syncthreads();
-        wg_mob(B, R, xpd, xm, st, P + R.off[BO_MSS]);
+        wg_mob(B, R, P, xpd, xm, st);
     } else if (op == 5) {
         for (int64_t i = threadIdx.x; i < 3 * (int64_t)R.Np; i += BW) xpd[i] = x[i];
         __syncthreads();
@@ -985,7 +994,7 @@ ps_err ps_batch_create(const ps_params* p, int R, const int* Nc, const int* Npol
     B.kmax = p->sqrt_kmax;
     B.mixed = p->precision == PS_PREC_MIXED;
     B.ms_rtol = 1e-12;
-    B.ms_maxit = 10000;
+    B.ms_maxit = 1000;
     b->ktrap_cur = p->ktrap;
     b->t = p->restart;
     // per-replica sizes and pool offsets
@@ -1020,9 +1029,10 @@ ps_err ps_batch_create(const ps_params* p, int R, const int* Nc, const int* Npol
                 break;
             }
         const int64_t n3 = 3 * (int64_t)Q.Np, n5 = 5 * (int64_t)Q.Nc;
+        const int64_t n11 = 11 * (int64_t)Q.Nc;
         const int64_t sizes[BO_NOFF] = {n3, n3, n3, n3, 4 * (int64_t)Q.Nc, 3 * (int64_t)Q.Nc, Q.ns, Q.ns, Q.ns, Q.ns,
                                         Q.nu, Q.nu, Q.nu, Q.nu, Q.nu, Q.nu, Q.n, Q.n, Q.n, (int64_t)p->sqrt_kmax * Q.n,
-                                        n5, 9 * n5, n5 * n5, 18, 0};
+                                        n5, 9 * n5, n5 * n5, n11 * n11, n11, n11, 18, 0};
         for (int k = 0; k < BO_NOFF; k++) {
             if (k == BO_NZ) continue;
             Q.off[k] = pos;
@@ -1228,7 +1238,7 @@ ps_err ps_batch_run(void* h, int nsteps) {
 }
 
 // Per replica: [stresslet MINRES iterations of the last step, Lanczos iterations, negative Ritz
-// values, 0] (4 R ints).
+// values, solves whose stresslet MINRES stopped at its cap] (4 R ints).
 ps_err ps_batch_stats(void* h, int* out) {
     Batch* b = static_cast<Batch*>(h);
     if (!b || !out) return PS_ERR_ARG;
