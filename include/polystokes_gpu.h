@@ -308,6 +308,15 @@ ps_err ps_batch_stats(void* batch, int* out);
  * 3: W = M^{1/2} xi (n each), 4: mob() (no vectors; run it before 0-3), 5: the force rhs (ns
  * each, no input).  iters (R, may be null): op 2 MINRES, op 3 Lanczos iterations. */
 ps_err ps_batch_op(void* batch, int op, const double* in, double* out, int* iters);
+/* The last step's wall-clock time of each replica in microseconds and the team size (workgroups)
+ * it ran with; either output (R entries) may be null.  No reference counterpart: the reference
+ * runs one system per process. */
+ps_err ps_batch_times(void* batch, double* us, int* team);
+/* Step schedule: replicas far above the mean work run on a team of up to team_max (1..16)
+ * workgroups, each rank with at least min_blocks (>= 1) work units (blocks of 64 monomer rows or
+ * 16 colloid rows); with autotune
+ * the packing is redone before every step from the measured per-replica times. */
+ps_err ps_batch_config(void* batch, int team_max, int min_blocks, int autotune);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,7 @@
 
 constexpr int BATCH_THREADS = 1024;   // one workgroup per replica
 constexpr int BATCH_KMAX = 48;        // Lanczos iterations held on chip (LDS tridiagonal and U)
+constexpr int BATCH_TEAM_MAX = 16;    // workgroups sharing one replica's applies
 
 // Per-replica arrays in the FP64 pool (BO_NZ indexes the noise buffer instead).
 enum BatchOff {
@@ -20,9 +21,8 @@ enum BatchOff {
     BO_SB,                              // Brownian stresslet (5Nc)
     BO_M5,                              // stresslet MINRES work (9 x 5Nc)
     BO_MSS,                             // dense stresslet block (25 Nc^2)
-    BO_MCC,                             // dense colloid-colloid block, zero diagonal blocks (121 Nc^2)
-    BO_FC,                              // an apply's colloid sources gathered [F T S] per colloid (11Nc)
-    BO_BBV,                             // their colloid-colloid contribution (11Nc)
+    BO_MCC,                             // dense colloid-colloid block in the saddle vector's order
+                                        // (rows and columns: F 3Nc, T 3Nc, S 5Nc), zero diagonal blocks
     BO_ST,                              // carried tile (18)
     BO_NZ,                              // this step's noise (n + nu), in the noise buffer
     BO_NOFF
@@ -47,7 +47,31 @@ struct BConst {
                     // reference does after KSPSolve's cap, and the replica's step counts it)
 };
 
-ps_err batch_launch_step(hipStream_t st, const BConst& B, const BRep* reps, const int* order, int R, double* pool,
-                         const double* noise, double ktrap, int* stats);
+// Fork-join block of a team (one per team, 64 bytes): the leader publishes an operation and bumps
+// seq; each member runs its slice and bumps done; err stops every later wait of the team.
+struct BTeamCtl {
+    int seq, done, err, op;
+    const double* in;
+    double* out;
+    const double* mat;
+    int64_t n;
+    int64_t pad[3];
+};
+
+// Step schedule (device ints): per workgroup w, sched[5w..5w+4] = {team or -1, rank in the team,
+// team size, first and end index of its solo replicas in sched[solo_off..]}; the replica of team k
+// is sched[team_off + k].  A team's workgroups run that replica first (rank 0 leads), then each
+// runs its own solo replicas.
+struct BSchedDev {
+    const int* sched;
+    int team_off, solo_off;
+    BTeamCtl* ctl;
+    int* err;          // set when a team wait timed out
+    double* times;     // per replica: wall-clock ticks of its last step (leader's view)
+};
+
+ps_err batch_launch_step(hipStream_t st, const BConst& B, const BRep* reps, const BSchedDev& sd, int nwg,
+                         bool cooperative, double* pool, const double* noise, double ktrap, int* stats);
+int batch_step_occupancy();   // co-resident step workgroups per CU
 ps_err batch_launch_op(hipStream_t st, const BConst& B, const BRep* reps, int R, double* pool, int op,
-                       const double* in, double* out, const int64_t* io_off, int* stats);
+                       const double* in, double* out, const int64_t* io_off, int* stats, int* err);

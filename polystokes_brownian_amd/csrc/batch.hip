@@ -49,6 +49,58 @@ __device__ double wg_dot(const double* a, const double* b, int64_t n, double* re
     return wg_sum(s, red);
 }
 
+// ---------------------------------------------------------------------------------- teams
+// A replica far above the mean work is run by a team of workgroups: the leader (rank 0) runs the
+// step and forks each apply, stresslet GEMV and mob() tile pass to the members through its
+// BTeamCtl; every rank runs the same slice function on its share and the leader joins.  The step
+// kernel is a cooperative launch, so all workgroups are co-resident and a member always reaches
+// its next wait.  Every wait is bounded (wall clock): a timeout sets the team's err, which ends
+// every later wait of the team at once, and the batch's error flag, which the host reports.
+enum TeamOp { TOP_EXIT = 1, TOP_APPLY_D, TOP_APPLY_DS, TOP_APPLY_F, TOP_GEMV, TOP_MOB };
+constexpr uint64_t TEAM_WAIT_TICKS = 1000000000ull;   // 10 s of the 100 MHz wall clock
+
+struct TeamCtx {
+    BTeamCtl* ctl;   // null for a workgroup alone
+    int* gerr;
+    int rank, g, nfork;
+};
+
+__device__ bool team_wait_ge(int* p, int target, int* err) {
+    const uint64_t t0 = wall_clock64();
+    for (;;) {
+        if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+        if (wall_clock64() - t0 > TEAM_WAIT_TICKS) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(8);
+    }
+}
+// Leader: publish op after every thread's earlier stores (release), then bump seq.
+__device__ void team_fork(TeamCtx& tc, int op, const double* in, double* out, const double* mat, int64_t n) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        BTeamCtl* c = tc.ctl;
+        c->op = op;
+        c->in = in;
+        c->out = out;
+        c->mat = mat;
+        c->n = n;
+        __threadfence();
+        __hip_atomic_fetch_add(&c->seq, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (op != TOP_EXIT) tc.nfork++;
+}
+// Leader: wait for every member's slice of the last fork (acquire).
+__device__ void team_join(TeamCtx& tc) {
+    if (threadIdx.x == 0 && !team_wait_ge(&tc.ctl->done, tc.nfork * (tc.g - 1), &tc.ctl->err))
+        __hip_atomic_store(tc.gerr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    __threadfence();
+}
+
 // ---------------------------------------------------------------------------------- geometry
 // Unbounded systems: d = x_k - x_j, dr, e = d / dr (the reference's pair_geom without a box).
 __device__ __forceinline__ void bgeom(const double* xs, int k, int j, double e[3], double& dr, double& dr_inv) {
@@ -179,91 +231,112 @@ __device__ void b_stale(const BConst& B, const BRep& R, const double* xm, double
         for (int k = 0; k < 9; k++) st[9 + k] = cur[k];
     }
 }
-// mob(): snapshot xm = xpd, the carried tile, and the dense colloid-colloid blocks: M_cc (11Nc x 11Nc,
-// the pair tiles of every ordered colloid pair, zero diagonal blocks) and the stresslet block M_ss
-// (5Nc x 5Nc: self blocks and the pairs' m tiles, placement of src/mob.cpp:1129-1143).  One FTS tile
-// per unordered pair; its two orientations are the role expressions of a colloid pair in
-// mobility(..., AB=false, AA=false) (src/mob.cpp:1042-1143): for target c1 < source c2
+// Row (and column) of colloid c's component q (F 0..2, T 3..5, S 6..10) in M_cc: the saddle
+// vector's own order, so M_cc's columns meet f[nm3 .. n) as stored.
+__device__ __forceinline__ int64_t cc_ix(int Nc, int c, int q) {
+    return q < 3 ? 3 * (int64_t)c + q : q < 6 ? 3 * (int64_t)Nc + 3 * c + (q - 3) : 6 * (int64_t)Nc + 5 * c + (q - 6);
+}
+// mob()'s dense colloid-colloid blocks: M_cc (11Nc x 11Nc, the pair tiles of every ordered colloid
+// pair, zero diagonal blocks) and the stresslet block M_ss (5Nc x 5Nc: self blocks and the pairs'
+// m tiles, placement of src/mob.cpp:1129-1143).  One FTS tile per unordered pair; its two
+// orientations are the role expressions of a colloid pair in mobility(..., AB=false, AA=false)
+// (src/mob.cpp:1042-1143): for target c1 < source c2
 //   U: [a F, -bt^T T, gt S]   W: [b F, c T, ht S]   S: [-gt^T F, ht^T T, m S]
 // and for target c2 > source c1
 //   U: [a^T F, bt T, -gt S]   W: [-b^T F, c^T T, ht S]   S: [gt^T F, ht^T T, m^T S].
-__device__ void wg_mob(const BConst& B, const BRep& R, double* P, const double* xpd, double* xm, double* st) {
-    for (int64_t i = threadIdx.x; i < 3 * (int64_t)R.Np; i += BW) xm[i] = xpd[i];
-    __syncthreads();
-    if (threadIdx.x == 0) b_stale(B, R, xm, st);
+// This slice: unordered pairs p = rank BW + t, rank BW + t + g BW, ... (row-major over c1 < c2).
+__device__ void mob_slice(const BRep& R, double* P, int rank, int g) {
     const int Nc = R.Nc, Nm = R.Nm;
     const int64_t n5 = 5 * (int64_t)Nc, n11 = 11 * (int64_t)Nc;
+    const double* xm = P + R.off[BO_XM];
     double* Mss = P + R.off[BO_MSS];
     double* Mcc = P + R.off[BO_MCC];
-    for (int64_t idx = threadIdx.x; idx < 121 * (int64_t)Nc; idx += BW) {
-        const int64_t c = idx / 121, q = idx % 121;
-        Mcc[(11 * c + q / 11) * n11 + 11 * c + q % 11] = 0.0;
-    }
-    for (int64_t idx = threadIdx.x; idx < 25 * (int64_t)Nc; idx += BW) {
-        const int64_t c = idx / 25;
-        const int i = (int)(idx % 25) / 5, j = (int)(idx % 5);
-        Mss[(5 * c + i) * n5 + 5 * c + j] =
-            (i == j) ? 9. / 5. : ((i == 0 && j == 4) || (i == 4 && j == 0)) ? 9. / 10. : 0.0;
-    }
-    for (int64_t pr = threadIdx.x; pr < (int64_t)Nc * Nc; pr += BW) {
-        const int c1 = (int)(pr / Nc), c2 = (int)(pr % Nc);
-        if (c1 >= c2) continue;
+    const int64_t npair = (int64_t)Nc * (Nc - 1) / 2;
+    auto first = [Nc](int64_t c) { return c * (2 * (int64_t)Nc - c - 1) / 2; };   // pairs before row c
+    for (int64_t p = (int64_t)rank * BW + threadIdx.x; p < npair; p += (int64_t)g * BW) {
+        const double m2 = 2.0 * Nc - 1.0;
+        int64_t c1 = (int64_t)((m2 - __builtin_sqrt(fmax(m2 * m2 - 8.0 * (double)p, 0.0))) * 0.5);
+        if (c1 < 0) c1 = 0;
+        while (c1 > 0 && first(c1) > p) c1--;
+        while (first(c1 + 1) <= p) c1++;
+        const int64_t c2 = c1 + 1 + (p - first(c1));
         double e[3], dr, dr_inv;
-        bgeom(xm, Nm + c1, Nm + c2, e, dr, dr_inv);
+        bgeom(xm, Nm + (int)c1, Nm + (int)c2, e, dr, dr_inv);
         BBT T;
         tile_bb(dr_inv, e, T);
-        double* r1 = Mcc + (11 * (int64_t)c1) * n11 + 11 * (int64_t)c2;   // target c1, source c2
-        double* r2 = Mcc + (11 * (int64_t)c2) * n11 + 11 * (int64_t)c1;   // target c2, source c1
+        // put(target, source, i, j, v): M_cc entry of target component i, source component j
+        auto put = [&](int64_t ct, int64_t cs, int i, int j, double v) {
+            Mcc[cc_ix(Nc, (int)ct, i) * n11 + cc_ix(Nc, (int)cs, j)] = v;
+        };
         for (int i = 0; i < 3; i++) {
             for (int j = 0; j < 3; j++) {
-                r1[i * n11 + j] = T.a[i][j];
-                r1[i * n11 + 3 + j] = -T.bt[j][i];
-                r1[(3 + i) * n11 + j] = T.b[i][j];
-                r1[(3 + i) * n11 + 3 + j] = T.c[i][j];
-                r2[i * n11 + j] = T.a[j][i];
-                r2[i * n11 + 3 + j] = T.bt[i][j];
-                r2[(3 + i) * n11 + j] = -T.b[j][i];
-                r2[(3 + i) * n11 + 3 + j] = T.c[j][i];
+                put(c1, c2, i, j, T.a[i][j]);
+                put(c1, c2, i, 3 + j, -T.bt[j][i]);
+                put(c1, c2, 3 + i, j, T.b[i][j]);
+                put(c1, c2, 3 + i, 3 + j, T.c[i][j]);
+                put(c2, c1, i, j, T.a[j][i]);
+                put(c2, c1, i, 3 + j, T.bt[i][j]);
+                put(c2, c1, 3 + i, j, -T.b[j][i]);
+                put(c2, c1, 3 + i, 3 + j, T.c[j][i]);
             }
             for (int j = 0; j < 5; j++) {
-                r1[i * n11 + 6 + j] = T.gt[i][j];
-                r1[(3 + i) * n11 + 6 + j] = T.ht[i][j];
-                r1[(6 + j) * n11 + i] = -T.gt[i][j];
-                r1[(6 + j) * n11 + 3 + i] = T.ht[i][j];
-                r2[i * n11 + 6 + j] = -T.gt[i][j];
-                r2[(3 + i) * n11 + 6 + j] = T.ht[i][j];
-                r2[(6 + j) * n11 + i] = T.gt[i][j];
-                r2[(6 + j) * n11 + 3 + i] = T.ht[i][j];
+                put(c1, c2, i, 6 + j, T.gt[i][j]);
+                put(c1, c2, 3 + i, 6 + j, T.ht[i][j]);
+                put(c1, c2, 6 + j, i, -T.gt[i][j]);
+                put(c1, c2, 6 + j, 3 + i, T.ht[i][j]);
+                put(c2, c1, i, 6 + j, -T.gt[i][j]);
+                put(c2, c1, 3 + i, 6 + j, T.ht[i][j]);
+                put(c2, c1, 6 + j, i, T.gt[i][j]);
+                put(c2, c1, 6 + j, 3 + i, T.ht[i][j]);
             }
         }
         for (int j = 0; j < 5; j++)
             for (int k = 0; k < 5; k++) {
-                r1[(6 + j) * n11 + 6 + k] = T.m[j][k];
-                r2[(6 + j) * n11 + 6 + k] = T.m[k][j];
-                Mss[(5 * (int64_t)c1 + j) * n5 + 5 * (int64_t)c2 + k] = T.m[j][k];
-                Mss[(5 * (int64_t)c2 + j) * n5 + 5 * (int64_t)c1 + k] = T.m[k][j];
+                put(c1, c2, 6 + j, 6 + k, T.m[j][k]);
+                put(c2, c1, 6 + j, 6 + k, T.m[k][j]);
+                Mss[(5 * c1 + j) * n5 + 5 * c2 + k] = T.m[j][k];
+                Mss[(5 * c2 + j) * n5 + 5 * c1 + k] = T.m[k][j];
             }
     }
-    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------------- apply
 // u = M f for the grand mobility of src/mob.cpp:635-1213 (the layout of a single context; the
 // reference's quirks: the carried translation tile, the '+' of mob_c, the 2-1 gt sign).  T is the
 // arithmetic of the monomer-monomer and monomer-colloid tiles (float in the mixed form's square
-// root); colloid-colloid tiles are FP64.  `srows`: only the stresslet rows (nu..n).  sh: LDS,
-// 6 BW doubles (staged monomer sources).
+// root); colloid-colloid tiles are FP64.  `srows`: only the stresslet rows (nu..n).
+//
+// This slice of a team of g: monomer rows in blocks of 64 targets b = rank, rank + g, ... (the
+// lanes take the targets, wave w the sources w, w + 16, ... of each staged tile of 1024, the 16
+// partial sums added in a fixed order), then the rows of colloids c = rank, rank + g, ... (one
+// wave per colloid: the monomer sources over the lanes, then the stored M_cc rows against the
+// colloid part of f).  stage: LDS, 6 BW doubles.
 template <class T>
-__device__ void wg_apply(const BConst& B, const BRep& R, double* pool, const double* xm, const double* f,
-                         const double* st, double* u, bool srows, double* sh) {
-    const int Nm = R.Nm, Nc = R.Nc, t = threadIdx.x;
-    const int64_t nm3 = R.nm3, nc3 = R.nc3, nu = R.nu;
+__device__ __forceinline__ void apply_slice(const BConst& B, const BRep& R, const double* pool, const double* f,
+                                            double* u, bool srows, double* stage, int rank, int g) {
+    const double* xm = pool + R.off[BO_XM];
+    const double* st = pool + R.off[BO_ST];
+    const int Nm = R.Nm, Nc = R.Nc, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int64_t nm3 = R.nm3, nu = R.nu;
     const T tb2 = (T)(B.two_beta * B.two_beta);
     const T gA = (T)(C9D20 * (3 + 5 * B.beta2));
     const T gB = (T)(C9D10 * (C1D2 + C5D6 * B.beta2));
-    if (!srows) {
-        for (int i0 = 0; i0 < Nm; i0 += BW) {
-            const int i = i0 + t;
+    constexpr int SW = sizeof(T) == 8 ? 6 : 8;   // staged source stride (x y z fx fy fz [pad])
+    T* sh = reinterpret_cast<T*>(stage);
+    if (!srows && Nm > 0) {
+        // the carried tile applied to the sum of colloid forces (the same in every monomer row)
+        double F0 = 0.0, F1 = 0.0, F2 = 0.0;
+        for (int c = 0; c < Nc; c++) {
+            F0 += f[nm3 + 3 * c];
+            F1 += f[nm3 + 3 * c + 1];
+            F2 += f[nm3 + 3 * c + 2];
+        }
+        const double car0 = st[0] * F0 + st[1] * F1 + st[2] * F2;
+        const double car1 = st[3] * F0 + st[4] * F1 + st[5] * F2;
+        const double car2 = st[6] * F0 + st[7] * F1 + st[8] * F2;
+        const int nblk = (Nm + 63) >> 6;
+        for (int b = rank; b < nblk; b += g) {
+            const int i = (b << 6) + lane;
             const bool act = i < Nm;
             T xi = 0, yi = 0, zi = 0;
             int ci = -1;
@@ -276,24 +349,26 @@ __device__ void wg_apply(const BConst& B, const BRep& R, double* pool, const dou
             T ux = 0, uy = 0, uz = 0;
             // monomer sources (mobility(..., AA=true), src/mob.cpp:122-131): y f + (x - y)/r^2 (d.f) d
             for (int j0 = 0; j0 < Nm; j0 += BW) {
+                const int cnt = min(BW, Nm - j0);
                 __syncthreads();
-                const int j = j0 + t;
-                if (j < Nm) {
-                    sh[t] = xm[3 * j];
-                    sh[BW + t] = xm[3 * j + 1];
-                    sh[2 * BW + t] = xm[3 * j + 2];
-                    sh[3 * BW + t] = f[3 * j];
-                    sh[4 * BW + t] = f[3 * j + 1];
-                    sh[5 * BW + t] = f[3 * j + 2];
+                if (t < cnt) {
+                    const int j = j0 + t;
+                    T* d = sh + SW * t;
+                    d[0] = (T)xm[3 * j];
+                    d[1] = (T)xm[3 * j + 1];
+                    d[2] = (T)xm[3 * j + 2];
+                    d[3] = (T)f[3 * j];
+                    d[4] = (T)f[3 * j + 1];
+                    d[5] = (T)f[3 * j + 2];
                 }
                 __syncthreads();
                 if (!act) continue;
-                const int cnt = min(BW, Nm - j0);
-                for (int k = 0; k < cnt; k++) {
+                for (int k = wv; k < cnt; k += NW) {
                     const int jj = j0 + k;
                     if (jj == i) continue;
                     if (B.chain_HI && jj / R.Nmono == ci) continue;
-                    const T dx = xi - (T)sh[k], dy = yi - (T)sh[BW + k], dz = zi - (T)sh[2 * BW + k];
+                    const T* sk = sh + SW * k;
+                    const T dx = xi - sk[0], dy = yi - sk[1], dz = zi - sk[2];
                     const T r2 = dx * dx + dy * dy + dz * dz;
                     const T rinv = ps_rsqrt(r2);
                     const T rinv2 = rinv * rinv;
@@ -304,77 +379,72 @@ __device__ void wg_apply(const BConst& B, const BRep& R, double* pool, const dou
                         ya = (T)1 - (T)C9D32 * r;
                         cc = ((T)(21. / 32.) * r - (T)2) * rinv2;
                     }
-                    const T fx = (T)sh[3 * BW + k], fy = (T)sh[4 * BW + k], fz = (T)sh[5 * BW + k];
+                    const T fx = sk[3], fy = sk[4], fz = sk[5];
                     const T cs = cc * (dx * fx + dy * fy + dz * fz);
                     ux += ya * fx + cs * dx;
                     uy += ya * fy + cs * dy;
                     uz += ya * fz + cs * dz;
                 }
             }
-            if (!act) continue;
             // colloid sources (mobility_AB, src/mob.cpp:362-537): rotlet and stresslet couplings,
-            // and the carried tile applied to the sum of colloid forces
-            T vx = 0, vy = 0, vz = 0, Fx = 0, Fy = 0, Fz = 0;
-            for (int c = 0; c < Nc; c++) {
-                const int g = Nm + c;
-                const T dx = xi - (T)xm[3 * g], dy = yi - (T)xm[3 * g + 1], dz = zi - (T)xm[3 * g + 2];
-                const T r2 = dx * dx + dy * dy + dz * dz;
-                const T rinv = ps_rsqrt(r2);
-                const T ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
-                const T rinv2 = rinv * rinv, rinv4 = rinv2 * rinv2;
-                const T y12b = -(T)C3D4 * rinv2;
-                const T x12g = (T)C9D4 * rinv2 - gA * rinv4;
-                const T y12g = gB * rinv4;
-                const T Tx = (T)f[nm3 + nc3 + 3 * c], Ty = (T)f[nm3 + nc3 + 3 * c + 1], Tz = (T)f[nm3 + nc3 + 3 * c + 2];
-                const double* S = f + nu + 5 * (int64_t)c;
-                const T S0 = (T)S[0], S1 = (T)S[1], S2 = (T)S[2], S3 = (T)S[3], S4 = (T)S[4];
-                const T sex = S0 * ex + S1 * ey + S2 * ez;
-                const T sey = S1 * ex + S4 * ey + S3 * ez;
-                const T sez = S2 * ex + S3 * ey - (S0 + S4) * ez;
-                const T ese = ex * sex + ey * sey + ez * sez;
-                const T a = -x12g * ese - (T)2 * y12g * ese;
-                const T b2 = (T)2 * y12g;
-                vx += y12b * (ey * Tz - ez * Ty) + a * ex + b2 * sex;
-                vy += y12b * (ez * Tx - ex * Tz) + a * ey + b2 * sey;
-                vz += y12b * (ex * Ty - ey * Tx) + a * ez + b2 * sez;
-                Fx += (T)f[nm3 + 3 * c];
-                Fy += (T)f[nm3 + 3 * c + 1];
-                Fz += (T)f[nm3 + 3 * c + 2];
+            // wave w taking colloids w, w + 16, ...
+            T vx = 0, vy = 0, vz = 0;
+            if (act)
+                for (int c = wv; c < Nc; c += NW) {
+                    const int gc = Nm + c;
+                    const T dx = xi - (T)xm[3 * gc], dy = yi - (T)xm[3 * gc + 1], dz = zi - (T)xm[3 * gc + 2];
+                    const T r2 = dx * dx + dy * dy + dz * dz;
+                    const T rinv = ps_rsqrt(r2);
+                    const T ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
+                    const T rinv2 = rinv * rinv, rinv4 = rinv2 * rinv2;
+                    const T y12b = -(T)C3D4 * rinv2;
+                    const T x12g = (T)C9D4 * rinv2 - gA * rinv4;
+                    const T y12g = gB * rinv4;
+                    const int64_t tq = nm3 + 3 * (int64_t)Nc + 3 * c;
+                    const T Tx = (T)f[tq], Ty = (T)f[tq + 1], Tz = (T)f[tq + 2];
+                    const double* S = f + nu + 5 * (int64_t)c;
+                    const T S0 = (T)S[0], S1 = (T)S[1], S2 = (T)S[2], S3 = (T)S[3], S4 = (T)S[4];
+                    const T sex = S0 * ex + S1 * ey + S2 * ez;
+                    const T sey = S1 * ex + S4 * ey + S3 * ez;
+                    const T sez = S2 * ex + S3 * ey - (S0 + S4) * ez;
+                    const T ese = ex * sex + ey * sey + ez * sez;
+                    const T a = -x12g * ese - (T)2 * y12g * ese;
+                    const T b2 = (T)2 * y12g;
+                    vx += y12b * (ey * Tz - ez * Ty) + a * ex + b2 * sex;
+                    vy += y12b * (ez * Tx - ex * Tz) + a * ey + b2 * sey;
+                    vz += y12b * (ex * Ty - ey * Tx) + a * ez + b2 * sez;
+                }
+            // the 16 partial sums of each target, added in wave order
+            __syncthreads();
+            double* pr = stage;
+            pr[(0 * NW + wv) * 64 + lane] = (double)ux + (double)vx;
+            pr[(1 * NW + wv) * 64 + lane] = (double)uy + (double)vy;
+            pr[(2 * NW + wv) * 64 + lane] = (double)uz + (double)vz;
+            __syncthreads();
+            if (wv == 0 && act) {
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+                for (int w = 0; w < NW; w++) {
+                    s0 += pr[(0 * NW + w) * 64 + lane];
+                    s1 += pr[(1 * NW + w) * 64 + lane];
+                    s2 += pr[(2 * NW + w) * 64 + lane];
+                }
+                u[3 * i] = B.beta_inv * f[3 * i] + (s0 + car0);
+                u[3 * i + 1] = B.beta_inv * f[3 * i + 1] + (s1 + car1);
+                u[3 * i + 2] = B.beta_inv * f[3 * i + 2] + (s2 + car2);
             }
-            vx += (T)st[0] * Fx + (T)st[1] * Fy + (T)st[2] * Fz;
-            vy += (T)st[3] * Fx + (T)st[4] * Fy + (T)st[5] * Fz;
-            vz += (T)st[6] * Fx + (T)st[7] * Fy + (T)st[8] * Fz;
-            u[3 * i] = B.beta_inv * f[3 * i] + ((double)ux + (double)vx);
-            u[3 * i + 1] = B.beta_inv * f[3 * i + 1] + ((double)uy + (double)vy);
-            u[3 * i + 2] = B.beta_inv * f[3 * i + 2] + ((double)uz + (double)vz);
         }
     }
-    // colloid rows.  Colloid sources: the stored M_cc against f's colloid entries gathered as
-    // [F T S] per colloid (one wave per row).  Monomer sources (AB: rotation and stresslet rows, and
-    // the sum of monomer forces for the transposed carried tile): one wave per colloid, lanes over
-    // the monomers.
+    // colloid rows.  Monomer sources (AB: rotation and stresslet rows, and the sum of monomer forces
+    // for the transposed carried tile) over the lanes; colloid sources: the stored M_cc rows of the
+    // colloid against f[nm3 .. n), which is [F; T; S] in M_cc's column order.
     const int64_t n11 = 11 * (int64_t)Nc;
-    double* fc = pool + R.off[BO_FC];
-    double* bbv = pool + R.off[BO_BBV];
     const double* Mcc = pool + R.off[BO_MCC];
-    const int lane = t & 63, wv = t >> 6;
-    for (int64_t idx = t; idx < n11; idx += BW) {
-        const int64_t o = idx / 11;
-        const int k = (int)(idx % 11);
-        fc[idx] = k < 3 ? f[nm3 + 3 * o + k] : k < 6 ? f[nm3 + nc3 + 3 * o + (k - 3)] : f[nu + 5 * o + (k - 6)];
-    }
-    __syncthreads();
-    for (int64_t row = wv; row < n11; row += NW) {
-        const double* mr = Mcc + row * n11;
-        double acc = 0.0;
-        for (int64_t k = lane; k < n11; k += 64) acc += mr[k] * fc[k];
-        acc = wsum(acc);
-        if (lane == 0) bbv[row] = acc;
-    }
-    __syncthreads();
-    for (int c = wv; c < Nc; c += NW) {
-        const int g = Nm + c;
-        const T xc = (T)xm[3 * g], yc = (T)xm[3 * g + 1], zc = (T)xm[3 * g + 2];
+    const double* fcol = f + nm3;
+    for (int kc = wv;; kc += NW) {
+        const int c = rank + kc * g;
+        if (c >= Nc) break;
+        const int gc = Nm + c;
+        const T xc = (T)xm[3 * gc], yc = (T)xm[3 * gc + 1], zc = (T)xm[3 * gc + 2];
         T acc[11];
         for (int k = 0; k < 11; k++) acc[k] = 0;
         T fsx = 0, fsy = 0, fsz = 0;
@@ -411,17 +481,39 @@ __device__ void wg_apply(const BConst& B, const BRep& R, double* pool, const dou
         double av[11];
         for (int k = 3; k < 11; k++) av[k] = wsum((double)acc[k]);
         const double fs[3] = {wsum((double)fsx), wsum((double)fsy), wsum((double)fsz)};
+        double gm[11];
+        if (srows) {
+            const double* mr[5];
+            double ga[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int q = 0; q < 5; q++) mr[q] = Mcc + cc_ix(Nc, c, 6 + q) * n11;
+            for (int64_t k = lane; k < n11; k += 64) {
+                const double fv = fcol[k];
+                for (int q = 0; q < 5; q++) ga[q] += mr[q][k] * fv;
+            }
+            for (int q = 0; q < 5; q++) gm[6 + q] = wsum(ga[q]);
+        } else {
+            const double* mr[11];
+            double ga[11];
+            for (int q = 0; q < 11; q++) {
+                mr[q] = Mcc + cc_ix(Nc, c, q) * n11;
+                ga[q] = 0.0;
+            }
+            for (int64_t k = lane; k < n11; k += 64) {
+                const double fv = fcol[k];
+                for (int q = 0; q < 11; q++) ga[q] += mr[q][k] * fv;
+            }
+            for (int q = 0; q < 11; q++) gm[q] = wsum(ga[q]);
+        }
         if (lane != 0) continue;
-        const double* bb = bbv + 11 * (int64_t)c;
         if (!srows) {
             for (int comp = 0; comp < 3; comp++) {
                 const int64_t r = nm3 + 3 * (int64_t)c + comp;
                 const double carried = st[0 * 3 + comp] * fs[0] + st[1 * 3 + comp] * fs[1] + st[2 * 3 + comp] * fs[2];
-                u[r] = 1.0 * f[r] + carried + bb[comp];
+                u[r] = 1.0 * f[r] + carried + gm[comp];
             }
             for (int comp = 0; comp < 3; comp++) {
-                const int64_t r = nm3 + nc3 + 3 * (int64_t)c + comp;
-                u[r] = (3. / 4.) * f[r] + av[3 + comp] + bb[3 + comp];
+                const int64_t r = nm3 + 3 * (int64_t)Nc + 3 * (int64_t)c + comp;
+                u[r] = (3. / 4.) * f[r] + av[3 + comp] + gm[3 + comp];
             }
         }
         const double* S = f + nu + 5 * (int64_t)c;
@@ -429,10 +521,84 @@ __device__ void wg_apply(const BConst& B, const BRep& R, double* pool, const dou
             double self = (9. / 5.) * S[comp];
             if (comp == 0) self += (9. / 10.) * S[4];
             if (comp == 4) self += (9. / 10.) * S[0];
-            u[nu + 5 * (int64_t)c + comp] = self + av[6 + comp] + bb[6 + comp];
+            u[nu + 5 * (int64_t)c + comp] = self + av[6 + comp] + gm[6 + comp];
         }
     }
+}
+
+// y = A v (n x n, row-major): rows rank 16 + w, rank 16 + w + 16 g, ... one wave per row.
+__device__ __forceinline__ void gemv_slice(const double* A, const double* v, double* y, int64_t n, int rank, int g) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int64_t row = (int64_t)rank * NW + wv; row < n; row += (int64_t)g * NW) {
+        const double* mr = A + row * n;
+        double acc = 0.0;
+        for (int64_t k = lane; k < n; k += 64) acc += mr[k] * v[k];
+        acc = wsum(acc);
+        if (lane == 0) y[row] = acc;
+    }
+}
+
+// One team operation's slice (leaders, members and lone workgroups all run it; not inlined, so the
+// step kernel holds one copy of each slice).
+__device__ __noinline__ void run_slice(const BConst& B, const BRep& R, double* P, int op, const double* in,
+                                       double* out, const double* mat, int64_t n, int rank, int g) {
+    __shared__ double stage[6 * BW];
+    if (op == TOP_APPLY_D)
+        apply_slice<double>(B, R, P, in, out, false, stage, rank, g);
+    else if (op == TOP_APPLY_DS)
+        apply_slice<double>(B, R, P, in, out, true, stage, rank, g);
+    else if (op == TOP_APPLY_F)
+        apply_slice<float>(B, R, P, in, out, false, stage, rank, g);
+    else if (op == TOP_GEMV)
+        gemv_slice(mat, in, out, n, rank, g);
+    else if (op == TOP_MOB)
+        mob_slice(R, P, rank, g);
+}
+
+// One split operation on this workgroup (alone) or its team (as the leader).
+__device__ void team_op(const BConst& B, const BRep& R, double* P, TeamCtx& tc, int op, const double* in,
+                        double* out, const double* mat, int64_t n) {
+    if (tc.g > 1) {
+        team_fork(tc, op, in, out, mat, n);
+        run_slice(B, R, P, op, in, out, mat, n, 0, tc.g);
+        team_join(tc);
+    } else {
+        run_slice(B, R, P, op, in, out, mat, n, 0, 1);
+        __syncthreads();
+    }
+}
+
+template <class T>
+__device__ void wg_apply(const BConst& B, const BRep& R, double* P, const double* f, double* u, bool srows,
+                         TeamCtx& tc) {
+    team_op(B, R, P, tc, sizeof(T) == 4 ? TOP_APPLY_F : srows ? TOP_APPLY_DS : TOP_APPLY_D, f, u, nullptr, 0);
+}
+
+// mob(): snapshot xm = xpd, the carried tile, the self blocks of M_ss and zero diagonal blocks of
+// M_cc, then the pair tiles (split over the team).
+__device__ void wg_mob(const BConst& B, const BRep& R, double* P, const double* xpd, TeamCtx& tc) {
+    double* xm = P + R.off[BO_XM];
+    double* st = P + R.off[BO_ST];
+    for (int64_t i = threadIdx.x; i < 3 * (int64_t)R.Np; i += BW) xm[i] = xpd[i];
     __syncthreads();
+    if (threadIdx.x == 0) b_stale(B, R, xm, st);
+    const int Nc = R.Nc;
+    const int64_t n5 = 5 * (int64_t)Nc, n11 = 11 * (int64_t)Nc;
+    double* Mss = P + R.off[BO_MSS];
+    double* Mcc = P + R.off[BO_MCC];
+    for (int64_t idx = threadIdx.x; idx < 121 * (int64_t)Nc; idx += BW) {
+        const int c = (int)(idx / 121), q = (int)(idx % 121);
+        Mcc[cc_ix(Nc, c, q / 11) * n11 + cc_ix(Nc, c, q % 11)] = 0.0;
+    }
+    for (int64_t idx = threadIdx.x; idx < 25 * (int64_t)Nc; idx += BW) {
+        const int64_t c = idx / 25;
+        const int i = (int)(idx % 25) / 5, j = (int)(idx % 5);
+        Mss[(5 * c + i) * n5 + 5 * c + j] =
+            (i == j) ? 9. / 5. : ((i == 0 && j == 4) || (i == 4 && j == 0)) ? 9. / 10. : 0.0;
+    }
+    const int64_t npair = (int64_t)Nc * (Nc - 1) / 2;
+    TeamCtx solo{nullptr, tc.gerr, 0, 1, 0};
+    team_op(B, R, P, npair >= 2 * BW ? tc : solo, TOP_MOB, nullptr, nullptr, nullptr, 0);
 }
 
 // ---------------------------------------------------------------------------------- solves
@@ -448,8 +614,10 @@ __device__ __forceinline__ void self_inv5(const double* v, double* out) {
     out[2] = v[2] / a;
     out[3] = v[3] / a;
 }
-__device__ int wg_minres_ss(const BConst& B, int64_t n5, const double* Mss, const double* rhs, double* x, double* w,
-                            double* red) {
+__device__ int wg_minres_ss(const BConst& B, const BRep& R, double* P, TeamCtx& tc, int64_t n5, const double* Mss,
+                            const double* rhs, double* x, double* w, double* red) {
+    TeamCtx solo{nullptr, tc.gerr, 0, 1, 0};
+    TeamCtx& gtc = n5 >= 512 ? tc : solo;   // the GEMV splits over the team when it is large
     double *r1 = w, *r2 = w + n5, *y = w + 2 * n5, *v = w + 3 * n5, *ww = w + 4 * n5, *w1 = w + 5 * n5,
            *w2 = w + 6 * n5;
     const int t = threadIdx.x;
@@ -472,15 +640,7 @@ __device__ int wg_minres_ss(const BConst& B, int64_t n5, const double* Mss, cons
         const double s = 1.0 / beta;
         for (int64_t i = t; i < n5; i += BW) v[i] = s * y[i];
         __syncthreads();
-        // y = M_ss v: one wave per row, lanes over columns
-        for (int64_t row = t >> 6; row < n5; row += NW) {
-            double acc = 0.0;
-            const double* mr = Mss + row * n5;
-            for (int64_t k = t & 63; k < n5; k += 64) acc += mr[k] * v[k];
-            acc = wsum(acc);
-            if ((t & 63) == 0) y[row] = acc;
-        }
-        __syncthreads();
+        team_op(B, R, P, gtc, TOP_GEMV, v, y, Mss, n5);   // y = M_ss v
         if (itn >= 2)
             for (int64_t i = t; i < n5; i += BW) y[i] -= (beta / oldb) * r1[i];
         __syncthreads();
@@ -522,8 +682,8 @@ __device__ int wg_minres_ss(const BConst& B, int64_t n5, const double* Mss, cons
 
 // The saddle system [[M, B], [B^T, 0]] [f; u] = b with B = [-I_nu; 0] by exact block elimination:
 // f_u = -b_l;  M_ss f_s = b_s - (M [f_u; 0])_s;  u = (M f)_u - b_u.  `uout` gets u too.
-__device__ int wg_block_solve(const BConst& B, const BRep& R, double* pool, const double* xm, const double* st,
-                              const double* b, double* x, double* uout, double* sh, double* red) {
+__device__ int wg_block_solve(const BConst& B, const BRep& R, double* pool, const double* b, double* x, double* uout,
+                              double* red, TeamCtx& tc) {
     const int64_t n = R.n, nu = R.nu, n5 = 5 * (int64_t)R.Nc;
     double* y = pool + R.off[BO_Y];
     double* w5 = pool + R.off[BO_M5];
@@ -531,13 +691,13 @@ __device__ int wg_block_solve(const BConst& B, const BRep& R, double* pool, cons
     __syncthreads();
     int its = 0;
     if (n5 > 0) {
-        wg_apply<double>(B, R, pool, xm, x, st, y, true, sh);
+        wg_apply<double>(B, R, pool, x, y, true, tc);
         double* rs = w5 + 8 * n5;
         for (int64_t k = threadIdx.x; k < n5; k += BW) rs[k] = b[nu + k] - y[nu + k];
         __syncthreads();
-        its = wg_minres_ss(B, n5, pool + R.off[BO_MSS], rs, x + nu, w5, red);
+        its = wg_minres_ss(B, R, pool, tc, n5, pool + R.off[BO_MSS], rs, x + nu, w5, red);
     }
-    wg_apply<double>(B, R, pool, xm, x, st, y, false, sh);
+    wg_apply<double>(B, R, pool, x, y, false, tc);
     for (int64_t i = threadIdx.x; i < nu; i += BW) {
         const double u = y[i] - b[i];
         x[n + i] = u;
@@ -604,8 +764,8 @@ __device__ bool b_tql2(int m, double* d, double* e, double* Z) {
 // classical Gram-Schmidt against the stored basis; T_m = U diag(theta) U^T by QL; negative Ritz
 // values floored at 0; out = beta_0 Q U diag(sqrt(theta+)) U^T e_1.  Applies in T.  Returns m.
 template <class T>
-__device__ int wg_sqrt(const BConst& B, const BRep& R, double* pool, const double* xm, const double* st,
-                       const double* xi, double* out, double* sh, double* red, double* lz, int* nneg_out) {
+__device__ int wg_sqrt(const BConst& B, const BRep& R, double* pool, const double* xi, double* out, double* red,
+                       double* lz, int* nneg_out, TeamCtx& tc) {
     const int64_t n = R.n;
     double* Q = pool + R.off[BO_QB];
     double* w = pool + R.off[BO_LW];
@@ -628,7 +788,7 @@ __device__ int wg_sqrt(const BConst& B, const BRep& R, double* pool, const doubl
     int m = 0;
     for (int j = 0; j < kmax; j++) {
         const double* qj = Q + (int64_t)j * n;
-        wg_apply<T>(B, R, pool, xm, qj, st, w, false, sh);
+        wg_apply<T>(B, R, pool, qj, w, false, tc);
         if (j > 0) {
             const double bj1 = betad[j - 1];
             const double* qp = Q + (int64_t)(j - 1) * n;
@@ -698,28 +858,51 @@ __device__ int wg_sqrt(const BConst& B, const BRep& R, double* pool, const doubl
 struct BStepArgs {
     BConst B;
     const BRep* reps;
-    const int* order;      // workgroup -> replica, largest first
+    BSchedDev sd;
     double* pool;
     const double* noise;   // device copy of this step's noise: replica r at reps[r].off[BO_NZ] - noise base
     double ktrap;
-    int* stats;            // per replica: [minres iterations, lanczos m, negative Ritz values, 0]
+    int* stats;            // per replica: [minres iterations, lanczos m, negative Ritz values, unconverged]
 };
 
-__global__ __launch_bounds__(BW) void k_batch_step(BStepArgs a) {
-    __shared__ double sh[6 * BW];
-    __shared__ double red[NW];
-    __shared__ double lz[4 * BATCH_KMAX + BATCH_KMAX * BATCH_KMAX];
-    __shared__ int nneg_sh;
+// A team member: run the leader's forks on this rank's share until it exits.
+__device__ void team_member(const BStepArgs& a, const BRep& R, TeamCtx& tc, int* cmd) {
+    int last = 0;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const bool ok = team_wait_ge(&tc.ctl->seq, last + 1, &tc.ctl->err);
+            if (!ok) __hip_atomic_store(tc.gerr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            *cmd = ok;
+        }
+        __syncthreads();
+        const int ok = *cmd;
+        __syncthreads();
+        if (!ok) return;
+        __threadfence();   // acquire: the fork's arguments and inputs
+        last++;
+        const BTeamCtl* c = tc.ctl;
+        const int op = c->op;
+        if (op == TOP_EXIT) return;
+        run_slice(a.B, R, a.pool, op, c->in, c->out, c->mat, c->n, tc.rank, tc.g);
+        __threadfence();   // release this slice's outputs
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_fetch_add(&tc.ctl->done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// The step of replica r by this workgroup alone or as its team's leader.
+__device__ __forceinline__ void step_replica(const BStepArgs& a, int r, TeamCtx& tc, double* red, double* lz,
+                                             int& nneg_sh) {
     const BConst& B = a.B;
-    const int r = a.order[blockIdx.x];
     const BRep R = a.reps[r];
+    const uint64_t t_start = wall_clock64();
     double* P = a.pool;
     const int64_t n = R.n, nu = R.nu, n3 = 3 * (int64_t)R.Np, nc5 = 5 * (int64_t)R.Nc;
-    double *x = P + R.off[BO_X], *xn = P + R.off[BO_XN], *xpd = P + R.off[BO_XPD], *xm = P + R.off[BO_XM];
+    double *x = P + R.off[BO_X], *xn = P + R.off[BO_XN], *xpd = P + R.off[BO_XPD];
     double *rhs = P + R.off[BO_RHS], *X = P + R.off[BO_XS], *Xdet = P + R.off[BO_XDET], *Xd = P + R.off[BO_XD];
     double *up = P + R.off[BO_UP], *vdn = P + R.off[BO_VDN], *vdc = P + R.off[BO_VDC], *vb = P + R.off[BO_VB];
     double *ud = P + R.off[BO_UD], *dv = P + R.off[BO_DV], *W = P + R.off[BO_W], *sb = P + R.off[BO_SB];
-    double *st = P + R.off[BO_ST], *q = P + R.off[BO_Q];
+    double* q = P + R.off[BO_Q];
     const double* xpos0 = P + R.off[BO_XP0];
     const double* noise = a.noise + R.off[BO_NZ];   // n slip normals, then nu RFD normals
     const int t = threadIdx.x;
@@ -730,7 +913,7 @@ __global__ __launch_bounds__(BW) void k_batch_step(BStepArgs a) {
         __syncthreads();
     };
     auto solve = [&](const double* b, double* Xo, double* uo) {
-        const int k = wg_block_solve(B, R, P, xm, st, b, Xo, uo, sh, red);
+        const int k = wg_block_solve(B, R, P, b, Xo, uo, red, tc);
         its += k;
         unconverged += k >= B.ms_maxit;
     };
@@ -739,12 +922,12 @@ __global__ __launch_bounds__(BW) void k_batch_step(BStepArgs a) {
     snapshot();
     wg_forces(B, R, xpd, x, xpos0, a.ktrap, rhs);
     __syncthreads();
-    wg_mob(B, R, P, xpd, xm, st);
+    wg_mob(B, R, P, xpd, tc);
     solve(rhs, Xdet, vdn);
     int lm = 0;
     if (kT > 0) {
-        lm = B.mixed ? wg_sqrt<float>(B, R, P, xm, st, noise, W, sh, red, lz, &nneg_sh)
-                     : wg_sqrt<double>(B, R, P, xm, st, noise, W, sh, red, lz, &nneg_sh);
+        lm = B.mixed ? wg_sqrt<float>(B, R, P, noise, W, red, lz, &nneg_sh, tc)
+                     : wg_sqrt<double>(B, R, P, noise, W, red, lz, &nneg_sh, tc);
         const double sc = sqrt(2 * kT / dt);
         for (int64_t i = t; i < n; i += BW) rhs[i] += sc * W[i];
         __syncthreads();
@@ -755,7 +938,7 @@ __global__ __launch_bounds__(BW) void k_batch_step(BStepArgs a) {
         for (int64_t i = t; i < n3; i += BW) x[i] += eps * fext[i];
         __syncthreads();
         snapshot();
-        wg_mob(B, R, P, xpd, xm, st);
+        wg_mob(B, R, P, xpd, tc);
         for (int64_t i = t; i < R.ns; i += BW) rhs[i] = (i < n) ? 0.0 : fext[i - n];
         __syncthreads();
         solve(rhs, Xd, dv);
@@ -763,7 +946,7 @@ __global__ __launch_bounds__(BW) void k_batch_step(BStepArgs a) {
         for (int64_t i = t; i < n3; i += BW) x[i] += -2.0 * eps * fext[i];
         __syncthreads();
         snapshot();
-        wg_mob(B, R, P, xpd, xm, st);
+        wg_mob(B, R, P, xpd, tc);
         for (int64_t i = t; i < R.ns; i += BW) rhs[i] = (i < n) ? 0.0 : fext[i - n];
         __syncthreads();
         solve(rhs, Xd, ud);
@@ -789,7 +972,7 @@ __global__ __launch_bounds__(BW) void k_batch_step(BStepArgs a) {
     snapshot();
     wg_forces(B, R, xpd, x, xpos0, a.ktrap, rhs);
     __syncthreads();
-    wg_mob(B, R, P, xpd, xm, st);
+    wg_mob(B, R, P, xpd, tc);
     solve(rhs, Xdet, vdc);
     for (int64_t i = t; i < nu; i += BW) up[i] = 0.5 * (vdn[i] + vdc[i]) + vb[i];
     __syncthreads();
@@ -808,11 +991,43 @@ __global__ __launch_bounds__(BW) void k_batch_step(BStepArgs a) {
         const double norm = __builtin_sqrt(qn[0] * qn[0] + qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]);
         for (int k = 0; k < 4; k++) qi[k] = qn[k] / norm;
     }
+    __syncthreads();
     if (t == 0) {
         a.stats[4 * r] = its;
         a.stats[4 * r + 1] = lm;
         a.stats[4 * r + 2] = kT > 0 ? nneg_sh : 0;
         a.stats[4 * r + 3] = unconverged;
+        a.sd.times[r] = (double)(wall_clock64() - t_start);
+    }
+}
+
+// One step of every replica.  Workgroup w first takes its team role (sched[5w] >= 0: leader or
+// member of that team's replica), then its solo replicas in order.
+__global__ __launch_bounds__(BW) void k_batch_step(BStepArgs a) {
+    __shared__ double red[NW];
+    __shared__ double lz[4 * BATCH_KMAX + BATCH_KMAX * BATCH_KMAX];
+    __shared__ int nneg_sh, cmd_sh;
+    const int* wi = a.sd.sched + 5 * blockIdx.x;
+    const int team = wi[0];
+    for (int k = team >= 0 ? -1 : wi[3]; k < wi[4]; k = (k < 0) ? wi[3] : k + 1) {
+        TeamCtx tc{nullptr, a.sd.err, 0, 1, 0};
+        int r;
+        if (k < 0) {
+            tc.ctl = a.sd.ctl + team;
+            tc.rank = wi[1];
+            tc.g = wi[2];
+            r = a.sd.sched[a.sd.team_off + team];
+            if (tc.rank > 0) {
+                const BRep R = a.reps[r];
+                team_member(a, R, tc, &cmd_sh);
+                continue;
+            }
+        } else {
+            r = a.sd.sched[a.sd.solo_off + k];
+        }
+        step_replica(a, r, tc, red, lz, nneg_sh);
+        if (tc.g > 1) team_fork(tc, TOP_EXIT, nullptr, nullptr, nullptr, 0);
+        __syncthreads();
     }
 }
 
@@ -821,8 +1036,7 @@ __global__ __launch_bounds__(BW) void k_batch_step(BStepArgs a) {
 // op 3: W = M^{1/2} xi (FP64 Lanczos), op 4: mob() at the current positions (snapshot + tiles),
 // op 5: rhs of the forces at the current positions.
 __global__ __launch_bounds__(BW) void k_batch_op(BConst B, const BRep* reps, double* pool, int op, const double* in,
-                                                 double* out, const int64_t* io_off, int* stats) {
-    __shared__ double sh[6 * BW];
+                                                 double* out, const int64_t* io_off, int* stats, int* err) {
     __shared__ double red[NW];
     __shared__ double lz[4 * BATCH_KMAX + BATCH_KMAX * BATCH_KMAX];
     __shared__ int nneg_sh;
@@ -831,20 +1045,21 @@ __global__ __launch_bounds__(BW) void k_batch_op(BConst B, const BRep* reps, dou
     double* P = pool;
     const double* inr = in + io_off[r];
     double* outr = out + io_off[r];
-    double *x = P + R.off[BO_X], *xpd = P + R.off[BO_XPD], *xm = P + R.off[BO_XM], *st = P + R.off[BO_ST];
+    double *x = P + R.off[BO_X], *xpd = P + R.off[BO_XPD];
+    TeamCtx tc{nullptr, err, 0, 1, 0};
     int s0 = 0;
     if (op == 0) {
-        wg_apply<double>(B, R, P, xm, inr, st, outr, false, sh);
+        wg_apply<double>(B, R, P, inr, outr, false, tc);
     } else if (op == 1) {
-        wg_apply<float>(B, R, P, xm, inr, st, outr, false, sh);
+        wg_apply<float>(B, R, P, inr, outr, false, tc);
     } else if (op == 2) {
-        s0 = wg_block_solve(B, R, P, xm, st, inr, outr, nullptr, sh, red);
+        s0 = wg_block_solve(B, R, P, inr, outr, nullptr, red, tc);
     } else if (op == 3) {
-        s0 = wg_sqrt<double>(B, R, P, xm, st, inr, outr, sh, red, lz, &nneg_sh);
+        s0 = wg_sqrt<double>(B, R, P, inr, outr, red, lz, &nneg_sh, tc);
     } else if (op == 4) {
         for (int64_t i = threadIdx.x; i < 3 * (int64_t)R.Np; i += BW) xpd[i] = x[i];
         __syncthreads();
-        wg_mob(B, R, P, xpd, xm, st);
+        wg_mob(B, R, P, xpd, tc);
     } else if (op == 5) {
         for (int64_t i = threadIdx.x; i < 3 * (int64_t)R.Np; i += BW) xpd[i] = x[i];
         __syncthreads();
@@ -856,15 +1071,27 @@ __global__ __launch_bounds__(BW) void k_batch_op(BConst B, const BRep* reps, dou
 }  // namespace
 
 // ---------------------------------------------------------------------------------- host
-ps_err batch_launch_step(hipStream_t st, const BConst& B, const BRep* reps, const int* order, int R, double* pool,
-                         const double* noise, double ktrap, int* stats) {
-    BStepArgs a{B, reps, order, pool, noise, ktrap, stats};
-    k_batch_step<<<R, BW, 0, st>>>(a);
+ps_err batch_launch_step(hipStream_t st, const BConst& B, const BRep* reps, const BSchedDev& sd, int nwg,
+                         bool cooperative, double* pool, const double* noise, double ktrap, int* stats) {
+    BStepArgs a{B, reps, sd, pool, noise, ktrap, stats};
+    if (cooperative) {
+        void* args[] = {&a};
+        return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_batch_step), dim3(nwg), dim3(BW), args, 0,
+                                          st) == hipSuccess
+                   ? PS_OK
+                   : PS_ERR_HIP;
+    }
+    k_batch_step<<<nwg, BW, 0, st>>>(a);
     return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_HIP;
 }
+int batch_step_occupancy() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_step, BW, 0) != hipSuccess) return 0;
+    return nb;
+}
 ps_err batch_launch_op(hipStream_t st, const BConst& B, const BRep* reps, int R, double* pool, int op,
-                       const double* in, double* out, const int64_t* io_off, int* stats) {
-    k_batch_op<<<R, BW, 0, st>>>(B, reps, pool, op, in, out, io_off, stats);
+                       const double* in, double* out, const int64_t* io_off, int* stats, int* err) {
+    k_batch_op<<<R, BW, 0, st>>>(B, reps, pool, op, in, out, io_off, stats, err);
     return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_HIP;
 }
 
@@ -890,10 +1117,24 @@ struct Batch {
     ps_params p{};
     BConst B{};
     std::vector<BRep> reps;
-    std::vector<int> order;
     BRep* d_reps = nullptr;
-    int* d_order = nullptr;
     int* d_stats = nullptr;
+    // step schedule: teams for the replicas far above the mean work, longest-first packing of the
+    // rest; re-packed after every step from each replica's measured time (autotune)
+    int nwg_max = 1;             // co-resident step workgroups (CUs x occupancy)
+    bool coop = false;           // cooperative launches available (teams need them)
+    int team_max = BATCH_TEAM_MAX, min_blocks = 2, autotune = 1;
+    double tick_us = 0.01;       // wall-clock tick in microseconds
+    std::vector<double> cost;    // per replica: work estimate (microseconds on one workgroup)
+    std::vector<int> team_of;    // per replica: team size in the current schedule
+    std::vector<int> sched;      // 5 nwg ints, team replicas, solo replicas
+    int nwg = 1, nteams = 0, team_off = 0, solo_off = 0;
+    bool sched_dirty = true, resched = true;
+    int* d_sched = nullptr;
+    BTeamCtl* d_ctl = nullptr;
+    int* d_err = nullptr;
+    double* d_times = nullptr;
+    int64_t steps_run = 0;
     double* pool = nullptr;
     int64_t pool_n = 0;
     double* noise_d = nullptr;
@@ -937,7 +1178,8 @@ struct Batch {
 void batch_free(Batch* b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    for (void* q : {(void*)b->d_reps, (void*)b->d_order, (void*)b->d_stats, (void*)b->pool, (void*)b->noise_d})
+    for (void* q : {(void*)b->d_reps, (void*)b->d_stats, (void*)b->pool, (void*)b->noise_d, (void*)b->d_sched,
+                    (void*)b->d_ctl, (void*)b->d_err, (void*)b->d_times})
         if (q) (void)hipFree(q);
     for (double* h : b->noise_h)
         if (h) (void)hipHostFree(h);
@@ -948,6 +1190,93 @@ void batch_free(Batch* b) {
 }
 
 int64_t pad8(int64_t v) { return (v + 7) / 8 * 8; }
+
+constexpr double TEAM_EFF = 0.75;   // assumed parallel efficiency of a team (work per rank / time)
+
+// The step schedule from the cost estimates: for a target makespan M (the mean share of a
+// workgroup, raised by 5% steps until all teams fit in the co-resident workgroups), a replica whose
+// cost exceeds M gets a team of ceil(cost / (M eff)) workgroups (at most team_max, and at most one
+// rank per min_blocks work units -- blocks of 64 monomer rows or 16 colloid rows), then
+// every other replica goes, longest first, to the workgroup with the least estimated load.
+void build_schedule(Batch* b) {
+    const int R = b->R;
+    std::vector<int> ord(R);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return b->cost[x] > b->cost[y]; });
+    std::vector<int> g(R, 1);
+    double total = 0.0;
+    for (double c : b->cost) total += c;
+    const bool teams = b->coop && b->team_max > 1;
+    if (teams) {
+        // target makespan M: the mean share of a workgroup, raised until the teams fit
+        std::vector<int> cap(R);
+        for (int r = 0; r < R; r++) {
+            // work units of a rank: blocks of 64 monomer rows and groups of 16 colloid rows
+            const int units = (b->reps[r].Nm + 63) / 64 + (b->reps[r].Nc + 15) / 16;
+            cap[r] = std::min(b->team_max, std::max(1, units / std::max(1, b->min_blocks)));
+        }
+        double M = total / b->nwg_max;
+        for (int it = 0; it < 200; it++) {
+            int used = 0;
+            for (int r = 0; r < R; r++) {
+                g[r] = 1;
+                if (b->cost[r] > M) g[r] = std::max(1, std::min(cap[r], (int)std::ceil(b->cost[r] / (M * TEAM_EFF))));
+                if (g[r] > 1) used += g[r];
+            }
+            if (used <= b->nwg_max) break;
+            M *= 1.05;
+            if (it == 199) std::fill(g.begin(), g.end(), 1);
+        }
+    }
+    int nteam_wg = 0, nsolo = 0;
+    b->nteams = 0;
+    for (int r : ord) {
+        if (g[r] > 1) {
+            nteam_wg += g[r];
+            b->nteams++;
+        } else {
+            nsolo++;
+        }
+    }
+    b->nwg = std::max(1, std::min(b->nwg_max, nteam_wg + nsolo));
+    if (!teams) b->nwg = std::max(1, std::min(R, b->nwg_max));
+    b->team_of = g;
+    std::vector<int>& S = b->sched;
+    S.assign(5 * (size_t)b->nwg + R, 0);
+    b->team_off = 5 * b->nwg;
+    b->solo_off = b->team_off + b->nteams;
+    std::vector<double> load(b->nwg, 0.0);
+    int w = 0, k = 0;
+    for (int i = 0; i < b->nwg; i++) S[5 * i] = -1;
+    for (int r : ord) {
+        if (g[r] <= 1) continue;
+        S[b->team_off + k] = r;
+        for (int j = 0; j < g[r]; j++, w++) {
+            S[5 * w] = k;
+            S[5 * w + 1] = j;
+            S[5 * w + 2] = g[r];
+            load[w] = b->cost[r] / (g[r] * TEAM_EFF);
+        }
+        k++;
+    }
+    // solos: longest first onto the least-loaded workgroup (ties: lowest index)
+    std::vector<std::vector<int>> lists(b->nwg);
+    for (int r : ord) {
+        if (g[r] > 1) continue;
+        const int best = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        lists[best].push_back(r);
+        load[best] += b->cost[r];
+    }
+    int pos = 0;
+    for (int i = 0; i < b->nwg; i++) {
+        S[5 * i + 3] = pos;
+        for (int r : lists[i]) S[b->solo_off + pos++] = r;
+        S[5 * i + 4] = pos;
+    }
+    S.resize(b->solo_off + pos);
+    b->sched_dirty = true;
+    b->resched = false;
+}
 
 }  // namespace
 
@@ -1032,7 +1361,7 @@ ps_err ps_batch_create(const ps_params* p, int R, const int* Nc, const int* Npol
         const int64_t n11 = 11 * (int64_t)Q.Nc;
         const int64_t sizes[BO_NOFF] = {n3, n3, n3, n3, 4 * (int64_t)Q.Nc, 3 * (int64_t)Q.Nc, Q.ns, Q.ns, Q.ns, Q.ns,
                                         Q.nu, Q.nu, Q.nu, Q.nu, Q.nu, Q.nu, Q.n, Q.n, Q.n, (int64_t)p->sqrt_kmax * Q.n,
-                                        n5, 9 * n5, n5 * n5, n11 * n11, n11, n11, 18, 0};
+                                        n5, 9 * n5, n5 * n5, n11 * n11, 18, 0};
         for (int k = 0; k < BO_NOFF; k++) {
             if (k == BO_NZ) continue;
             Q.off[k] = pos;
@@ -1043,11 +1372,13 @@ ps_err ps_batch_create(const ps_params* p, int R, const int* Nc, const int* Npol
     }
     b->pool_n = pos;
     b->noise_n = npos;
-    // workgroups take replicas by decreasing work (the applies grow as Np^2)
-    b->order.resize(R);
-    std::iota(b->order.begin(), b->order.end(), 0);
-    std::stable_sort(b->order.begin(), b->order.end(),
-                     [&](int a, int c) { return b->reps[a].Np > b->reps[c].Np; });
+    // first cost estimate (the applies grow as Np^2, the stored colloid block as (11 Nc)^2);
+    // measured step times replace it after the first step
+    b->cost.resize(R);
+    for (int r = 0; r < R; r++) {
+        const double np = b->reps[r].Np, nc = b->reps[r].Nc;
+        b->cost[r] = 1e-3 * (np * np + 30.0 * nc * nc);
+    }
     auto fail_hip = [&](hipError_t e) {
         g_batch_create_error = std::string("batched replicas: ") + hipGetErrorString(e);
         batch_free(b);
@@ -1066,11 +1397,28 @@ ps_err ps_batch_create(const ps_params* p, int R, const int* Nc, const int* Npol
         if ((e = hipEventCreateWithFlags(&b->copied[k], hipEventDisableTiming)) != hipSuccess) return fail_hip(e);
     }
     if ((e = hipMalloc(&b->d_reps, sizeof(BRep) * R)) != hipSuccess) return fail_hip(e);
-    if ((e = hipMalloc(&b->d_order, sizeof(int) * R)) != hipSuccess) return fail_hip(e);
+    {
+        int dev = 0, ncu = 1, coop = 0, khz = 0;
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return fail_hip(e);
+        if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+            return fail_hip(e);
+        if ((e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev)) != hipSuccess) coop = 0;
+        if ((e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev)) != hipSuccess || khz <= 0)
+            khz = 100000;
+        const int occ = batch_step_occupancy();
+        b->nwg_max = std::max(1, ncu * std::max(1, occ));
+        b->coop = coop != 0 && occ >= 1;
+        b->tick_us = 1000.0 / khz;
+    }
+    if ((e = hipMalloc(&b->d_sched, sizeof(int) * (5 * (size_t)b->nwg_max + 2 * (size_t)R))) != hipSuccess)
+        return fail_hip(e);
+    if ((e = hipMalloc(&b->d_ctl, sizeof(BTeamCtl) * (size_t)b->nwg_max)) != hipSuccess) return fail_hip(e);
+    if ((e = hipMalloc(&b->d_err, sizeof(int))) != hipSuccess) return fail_hip(e);
+    if ((e = hipMemset(b->d_err, 0, sizeof(int))) != hipSuccess) return fail_hip(e);
+    if ((e = hipMalloc(&b->d_times, sizeof(double) * R)) != hipSuccess) return fail_hip(e);
+    if ((e = hipMemset(b->d_times, 0, sizeof(double) * R)) != hipSuccess) return fail_hip(e);
     if ((e = hipMalloc(&b->d_stats, sizeof(int) * 4 * R)) != hipSuccess) return fail_hip(e);
     if ((e = hipMemcpy(b->d_reps, b->reps.data(), sizeof(BRep) * R, hipMemcpyHostToDevice)) != hipSuccess)
-        return fail_hip(e);
-    if ((e = hipMemcpy(b->d_order, b->order.data(), sizeof(int) * R, hipMemcpyHostToDevice)) != hipSuccess)
         return fail_hip(e);
     if ((e = hipMemset(b->d_stats, 0, sizeof(int) * 4 * R)) != hipSuccess) return fail_hip(e);
     // initial state: identity quaternions, the carried tile after fill_self (the identity)
@@ -1164,6 +1512,42 @@ ps_err ps_batch_get_vectors(void* h, int r, double* up, double* v_det_n, double*
     return PS_OK;
 }
 
+namespace {
+// The team error flag of the steps so far (a team wait that timed out).
+ps_err batch_check_err(Batch* b) {
+    int err = 0;
+    BPS_HIP(b, hipMemcpy(&err, b->d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) {
+        b->set_error("batched step: a team wait timed out (results of this run are invalid)");
+        return PS_ERR_HIP;
+    }
+    return PS_OK;
+}
+// Before a step: with `measured` (a step has run) and autotune on, wait for it, turn its
+// per-replica times into cost estimates and re-pack; upload the schedule when it changed.
+ps_err batch_schedule_step(Batch* b, bool measured) {
+    if (measured && b->autotune) {
+        BPS_HIP(b, hipStreamSynchronize(b->stream));
+        std::vector<double> ticks(b->R);
+        BPS_HIP(b, hipMemcpy(ticks.data(), b->d_times, sizeof(double) * b->R, hipMemcpyDeviceToHost));
+        for (int r = 0; r < b->R; r++) {
+            const double us = ticks[r] * b->tick_us;
+            const int g = b->team_of.empty() ? 1 : b->team_of[r];
+            if (us > 0) b->cost[r] = us * (g > 1 ? g * TEAM_EFF : 1.0);
+        }
+        build_schedule(b);
+    } else if (b->resched) {
+        build_schedule(b);
+    }
+    if (b->sched_dirty) {
+        BPS_HIP(b, hipStreamSynchronize(b->stream));
+        BPS_HIP(b, hipMemcpy(b->d_sched, b->sched.data(), sizeof(int) * b->sched.size(), hipMemcpyHostToDevice));
+        b->sched_dirty = false;
+    }
+    return PS_OK;
+}
+}  // namespace
+
 // nsteps of every replica: one persistent kernel per step; a host thread draws step k + 1's noise
 // while the GPU runs step k (two pinned slots; a slot is refilled only after its copy completed).
 ps_err ps_batch_run(void* h, int nsteps) {
@@ -1220,9 +1604,21 @@ ps_err ps_batch_run(void* h, int nsteps) {
             cv.notify_all();
             if (rc != PS_OK) break;
         }
-        rc = batch_launch_step(b->stream, b->B, b->d_reps, b->d_order, b->R, b->pool, b->noise_d, b->ktrap_cur,
-                               b->d_stats);
+        rc = batch_schedule_step(b, s > 0 || b->steps_run > 0);
+        if (rc != PS_OK) break;
+        if (b->nteams > 0) {
+            hipError_t e = hipMemsetAsync(b->d_ctl, 0, sizeof(BTeamCtl) * (size_t)b->nteams, b->stream);
+            if (e != hipSuccess) {
+                b->set_error(std::string("team reset: ") + hipGetErrorString(e));
+                rc = PS_ERR_HIP;
+                break;
+            }
+        }
+        const BSchedDev sd{b->d_sched, b->team_off, b->solo_off, b->d_ctl, b->d_err, b->d_times};
+        rc = batch_launch_step(b->stream, b->B, b->d_reps, sd, b->nwg, b->nteams > 0, b->pool, b->noise_d,
+                               b->ktrap_cur, b->d_stats);
         if (rc != PS_OK) b->set_error("batched step: kernel launch failed");
+        b->steps_run++;
     }
     if (noisy) {
         {
@@ -1234,7 +1630,7 @@ ps_err ps_batch_run(void* h, int nsteps) {
     }
     if (rc != PS_OK) return rc;
     BPS_HIP(b, hipStreamSynchronize(b->stream));
-    return PS_OK;
+    return batch_check_err(b);
 }
 
 // Per replica: [stresslet MINRES iterations of the last step, Lanczos iterations, negative Ritz
@@ -1244,6 +1640,35 @@ ps_err ps_batch_stats(void* h, int* out) {
     if (!b || !out) return PS_ERR_ARG;
     BPS_HIP(b, hipStreamSynchronize(b->stream));
     BPS_HIP(b, hipMemcpy(out, b->d_stats, sizeof(int) * 4 * (size_t)b->R, hipMemcpyDeviceToHost));
+    return PS_OK;
+}
+
+// The last step's wall-clock time of each replica in microseconds (a team's replica: its leader's
+// view) and the team size it ran with (1: a workgroup alone); either output may be null.
+ps_err ps_batch_times(void* h, double* us, int* team) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b) return PS_ERR_ARG;
+    BPS_HIP(b, hipStreamSynchronize(b->stream));
+    if (us) {
+        std::vector<double> ticks(b->R);
+        BPS_HIP(b, hipMemcpy(ticks.data(), b->d_times, sizeof(double) * b->R, hipMemcpyDeviceToHost));
+        for (int r = 0; r < b->R; r++) us[r] = ticks[r] * b->tick_us;
+    }
+    if (team)
+        for (int r = 0; r < b->R; r++) team[r] = b->team_of.empty() ? 1 : b->team_of[r];
+    return PS_OK;
+}
+
+// Schedule knobs: team_max (1..16; 1 turns teams off), min_blocks (>= 1: blocks of 64 monomer rows
+// per team rank at least), autotune (0: keep the size-based cost model).  Takes effect at the next
+// step.
+ps_err ps_batch_config(void* h, int team_max, int min_blocks, int autotune) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || team_max < 1 || team_max > BATCH_TEAM_MAX || min_blocks < 1) return PS_ERR_ARG;
+    b->team_max = team_max;
+    b->min_blocks = min_blocks;
+    b->autotune = autotune != 0;
+    b->resched = true;   // re-pack before the next step
     return PS_OK;
 }
 
@@ -1287,7 +1712,7 @@ ps_err ps_batch_op(void* h, int op, const double* in, double* out, int* iters) {
         }
         BConst B = b->B;
         B.ktrap = b->ktrap_cur;
-        rc = batch_launch_op(b->stream, B, b->d_reps, b->R, b->pool, op, din, dout, dio, dit);
+        rc = batch_launch_op(b->stream, B, b->d_reps, b->R, b->pool, op, din, dout, dio, dit, b->d_err);
         if (rc != PS_OK) {
             b->set_error("batch op: kernel launch failed");
             break;

@@ -88,6 +88,8 @@ def lib():
         L.ps_batch_run.argtypes = [vp, ip]
         L.ps_batch_stats.argtypes = [vp, C.POINTER(ip)]
         L.ps_batch_op.argtypes = [vp, ip, dp, dp, C.POINTER(ip)]
+        L.ps_batch_times.argtypes = [vp, dp, C.POINTER(ip)]
+        L.ps_batch_config.argtypes = [vp, ip, ip, ip]
         _lib = L
     return _lib
 
@@ -482,6 +484,17 @@ class Batch:
         out = (C.c_int * (4 * self.R))()
         self._check(lib().ps_batch_stats(self.h, out))
         return np.array(list(out), dtype=np.int64).reshape(self.R, 4)
+
+    def times(self):
+        """ps_batch_times: (per-replica wall-clock microseconds of the last step, team sizes)."""
+        us = np.zeros(self.R)
+        team = (C.c_int * self.R)()
+        self._check(lib().ps_batch_times(self.h, _p(us), team))
+        return us, np.array(list(team), dtype=np.int64)
+
+    def config(self, team_max=16, min_blocks=2, autotune=True):
+        """ps_batch_config: the step schedule's team limits and autotuning."""
+        self._check(lib().ps_batch_config(self.h, int(team_max), int(min_blocks), int(bool(autotune))))
 
     def op(self, op, inputs=None):
         """ps_batch_op: a list of per-replica input vectors in, per-replica outputs (and the

@@ -93,3 +93,42 @@ def test_batch_steps_match_single_system_runs(mixed):
     b.close()
     if bm:
         bm.close()
+
+
+# a replica with enough colloids for the team-split stresslet GEMV (5Nc >= 512) and mob() tile pass
+TEAM_SIZES = [(110, 2, 20), (6, 9, 20), (3, 4, 20), (0, 3, 20)]
+
+
+def test_teams_are_bit_identical_to_lone_workgroups_and_match_the_oracle():
+    """Replicas split over teams of workgroups (forced on these small sizes) give the same bits as
+    the same batch run one workgroup per replica, and match single-system oracle runs."""
+    params, reps = make(TEAM_SIZES)
+    bt = Batch(params, reps)
+    bt.config(team_max=4, min_blocks=1, autotune=True)
+    bs = Batch(params, reps)
+    bs.config(team_max=1)
+    bt.run(1)
+    bt.run(1)   # the second step runs the autotuned schedule
+    bs.run(2)
+    _, team = bt.times()
+    assert team.max() > 1, team
+    for r, rep in enumerate(reps):
+        xt, qt, _ = bt.state(r)
+        xs, qs, _ = bs.state(r)
+        assert np.array_equal(xt, xs), r
+        assert np.array_equal(qt, qs), r
+        vt, vs = bt.vectors(r), bs.vectors(r)
+        for k in vt:
+            assert np.array_equal(vt[k], vs[k]), (r, k)
+    for r, rep in enumerate(reps):
+        o = oracle_for(params, rep)
+        o.run(2)
+        xo, qo, _ = o.state()
+        xg, qg, _ = bt.state(r)
+        assert rel(xg, xo) < 1e-10, (r, rel(xg, xo))
+        if rep["Nc"]:
+            assert rel(qg, qo) < 1e-10
+    us, _ = bt.times()
+    assert (us > 0).all()
+    bt.close()
+    bs.close()
