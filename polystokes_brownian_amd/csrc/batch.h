@@ -21,8 +21,6 @@ enum BatchOff {
     BO_SB,                              // Brownian stresslet (5Nc)
     BO_M5,                              // stresslet MINRES work (9 x 5Nc)
     BO_MSS,                             // dense stresslet block (25 Nc^2)
-    BO_MCC,                             // dense colloid-colloid block in the saddle vector's order
-                                        // (rows and columns: F 3Nc, T 3Nc, S 5Nc), zero diagonal blocks
     BO_ST,                              // carried tile (18)
     BO_NZ,                              // this step's noise (n + nu), in the noise buffer
     BO_NOFF

@@ -77,10 +77,16 @@ __device__ bool team_wait_ge(int* p, int target, int* err) {
         __builtin_amdgcn_s_sleep(8);
     }
 }
-// Leader: publish op after every thread's earlier stores (release), then bump seq.
-__device__ void team_fork(TeamCtx& tc, int op, const double* in, double* out, const double* mat, int64_t n) {
-    __threadfence();
+// Release point of a workgroup: every wave waits for its own memory operations, then the
+// workgroup meets; one thread then publishes with an agent-scope release (a single L2 write-back
+// for the workgroup instead of one per wave).
+__device__ __forceinline__ void wg_release_point() {
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+}
+// Leader: publish op after every thread's earlier stores, then bump seq (release).
+__device__ void team_fork(TeamCtx& tc, int op, const double* in, double* out, const double* mat, int64_t n) {
+    wg_release_point();
     if (threadIdx.x == 0) {
         BTeamCtl* c = tc.ctl;
         c->op = op;
@@ -88,17 +94,19 @@ __device__ void team_fork(TeamCtx& tc, int op, const double* in, double* out, co
         c->out = out;
         c->mat = mat;
         c->n = n;
-        __threadfence();
         __hip_atomic_fetch_add(&c->seq, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (op != TOP_EXIT) tc.nfork++;
 }
-// Leader: wait for every member's slice of the last fork (acquire).
+// Leader: wait for every member's slice of the last fork; one agent-scope acquire by the waiting
+// thread before the workgroup proceeds.
 __device__ void team_join(TeamCtx& tc) {
-    if (threadIdx.x == 0 && !team_wait_ge(&tc.ctl->done, tc.nfork * (tc.g - 1), &tc.ctl->err))
-        __hip_atomic_store(tc.gerr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) {
+        if (!team_wait_ge(&tc.ctl->done, tc.nfork * (tc.g - 1), &tc.ctl->err))
+            __hip_atomic_store(tc.gerr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
     __syncthreads();
-    __threadfence();
 }
 
 // ---------------------------------------------------------------------------------- geometry
@@ -231,26 +239,81 @@ __device__ void b_stale(const BConst& B, const BRep& R, const double* xm, double
         for (int k = 0; k < 9; k++) st[9 + k] = cur[k];
     }
 }
-// Row (and column) of colloid c's component q (F 0..2, T 3..5, S 6..10) in M_cc: the saddle
-// vector's own order, so M_cc's columns meet f[nm3 .. n) as stored.
-__device__ __forceinline__ int64_t cc_ix(int Nc, int c, int q) {
-    return q < 3 ? 3 * (int64_t)c + q : q < 6 ? 3 * (int64_t)Nc + 3 * c + (q - 3) : 6 * (int64_t)Nc + 5 * c + (q - 6);
+// Colloid-colloid coupling in contracted form: the FTS tile of mobility(..., AB=false, AA=false)
+// (src/mob.cpp:96-356) applied to one source's [F T S] without forming the 11 x 11 tile, for
+// e = (x_lo - x_hi) / r of the pair's lower and higher colloid index.  With S the symmetric
+// traceless matrix of the stresslet (S_22 = -S_0 - S_4), a = S e, s = e.S e and w = T x e, the
+// role expressions of src/mob.cpp:1042-1143 reduce to (sg = +1 when the target is the lower index,
+// -1 when it is the higher; flat(X) = [X00 - X22, 2X01, 2X02, 2X12, X11 - X22]):
+//   U += (x12a - y12a)(e.F) e + y12a F - y12b w + sg [(-x12g - 2 y12g) s e + 2 y12g a]
+//   W += y12b (F x e) + (x12c + y12c)(e.T) e + y12c (1 - 2 e_i^2) T_i + 2 y12h (e x a)
+//   Sr += -sg flat(p e e^T + y12g (e F^T + F e^T)) + flat(y12h (e w^T + w e^T))
+//         + flat(A e e^T + B (e a^T + a e^T) + z12m S),
+// p = -(x12g + 2 y12g)(e.F), A = s (3/2 x12m - 2 y12m + 1/2 z12m), B = y12m - z12m (the '+' of
+// mob_c is the (x12c + y12c) off-diagonal term).
+__device__ __forceinline__ void cc_mS(const double e[3], double ri3, double ri5, const double S[5], double out[5]) {
+    const double x12m = -C9D2 * ri3 + C54D5 * ri5, y12m = C9D4 * ri3 - C36D5 * ri5, z12m = C9D5 * ri5;
+    const double S22 = -S[0] - S[4];
+    const double a0 = S[0] * e[0] + S[1] * e[1] + S[2] * e[2];
+    const double a1 = S[1] * e[0] + S[4] * e[1] + S[3] * e[2];
+    const double a2 = S[2] * e[0] + S[3] * e[1] + S22 * e[2];
+    const double s = e[0] * a0 + e[1] * a1 + e[2] * a2;
+    const double A = s * (C3D2 * x12m - 2.0 * y12m + C1D2 * z12m), Bc = y12m - z12m;
+    out[0] = A * (e[0] * e[0] - e[2] * e[2]) + 2.0 * Bc * (e[0] * a0 - e[2] * a2) + z12m * (S[0] - S22);
+    out[1] = 2.0 * (A * e[0] * e[1] + Bc * (e[0] * a1 + e[1] * a0) + z12m * S[1]);
+    out[2] = 2.0 * (A * e[0] * e[2] + Bc * (e[0] * a2 + e[2] * a0) + z12m * S[2]);
+    out[3] = 2.0 * (A * e[1] * e[2] + Bc * (e[1] * a2 + e[2] * a1) + z12m * S[3]);
+    out[4] = A * (e[1] * e[1] - e[2] * e[2]) + 2.0 * Bc * (e[1] * a1 - e[2] * a2) + z12m * (S[4] - S22);
 }
-// mob()'s dense colloid-colloid blocks: M_cc (11Nc x 11Nc, the pair tiles of every ordered colloid
-// pair, zero diagonal blocks) and the stresslet block M_ss (5Nc x 5Nc: self blocks and the pairs'
-// m tiles, placement of src/mob.cpp:1129-1143).  One FTS tile per unordered pair; its two
-// orientations are the role expressions of a colloid pair in mobility(..., AB=false, AA=false)
-// (src/mob.cpp:1042-1143): for target c1 < source c2
-//   U: [a F, -bt^T T, gt S]   W: [b F, c T, ht S]   S: [-gt^T F, ht^T T, m S]
-// and for target c2 > source c1
-//   U: [a^T F, bt T, -gt S]   W: [-b^T F, c^T T, ht S]   S: [gt^T F, ht^T T, m^T S].
+template <bool SROWS>
+__device__ __forceinline__ void cc_couple(const double e[3], double ri, double sg, const double F[3], const double Tq[3],
+                                          const double S[5], double U[3], double W[3], double Sr[5]) {
+    const double ri2 = ri * ri, ri3 = ri2 * ri, ri4 = ri2 * ri2, ri5 = ri4 * ri;
+    const double y12g = C6D5 * ri4, x12g = C9D4 * ri2 - C18D5 * ri4, y12h = -C9D8 * ri3, y12b = -C3D4 * ri2;
+    const double S22 = -S[0] - S[4];
+    const double a0 = S[0] * e[0] + S[1] * e[1] + S[2] * e[2];
+    const double a1 = S[1] * e[0] + S[4] * e[1] + S[3] * e[2];
+    const double a2 = S[2] * e[0] + S[3] * e[1] + S22 * e[2];
+    const double w0 = Tq[1] * e[2] - Tq[2] * e[1], w1 = Tq[2] * e[0] - Tq[0] * e[2], w2 = Tq[0] * e[1] - Tq[1] * e[0];
+    const double eF = e[0] * F[0] + e[1] * F[1] + e[2] * F[2];
+    if (!SROWS) {
+        const double x12a = C3D2 * ri - ri3, y12a = C3D4 * ri + C1D2 * ri3;
+        const double x12c = C3D4 * ri3, y12c = -C3D8 * ri3;
+        const double s = e[0] * a0 + e[1] * a1 + e[2] * a2;
+        const double gs = sg * (-x12g * s - 2.0 * y12g * s), g2 = sg * 2.0 * y12g;
+        const double ca = (x12a - y12a) * eF;
+        U[0] += ca * e[0] + y12a * F[0] - y12b * w0 + (gs * e[0] + g2 * a0);
+        U[1] += ca * e[1] + y12a * F[1] - y12b * w1 + (gs * e[1] + g2 * a1);
+        U[2] += ca * e[2] + y12a * F[2] - y12b * w2 + (gs * e[2] + g2 * a2);
+        const double eT = e[0] * Tq[0] + e[1] * Tq[1] + e[2] * Tq[2];
+        const double cc = (x12c + y12c) * eT, h2 = 2.0 * y12h;
+        W[0] += y12b * (F[1] * e[2] - F[2] * e[1]) + cc * e[0] + y12c * (1.0 - 2.0 * e[0] * e[0]) * Tq[0] +
+                h2 * (e[1] * a2 - e[2] * a1);
+        W[1] += y12b * (F[2] * e[0] - F[0] * e[2]) + cc * e[1] + y12c * (1.0 - 2.0 * e[1] * e[1]) * Tq[1] +
+                h2 * (e[2] * a0 - e[0] * a2);
+        W[2] += y12b * (F[0] * e[1] - F[1] * e[0]) + cc * e[2] + y12c * (1.0 - 2.0 * e[2] * e[2]) * Tq[2] +
+                h2 * (e[0] * a1 - e[1] * a0);
+    }
+    const double p = -(x12g + 2.0 * y12g) * eF, ms = -sg, h = 2.0 * y12h;
+    double m[5];
+    cc_mS(e, ri3, ri5, S, m);
+    Sr[0] += ms * (p * (e[0] * e[0] - e[2] * e[2]) + 2.0 * y12g * (e[0] * F[0] - e[2] * F[2])) +
+             h * (e[0] * w0 - e[2] * w2) + m[0];
+    Sr[1] += ms * (2.0 * (p * e[0] * e[1] + y12g * (e[0] * F[1] + e[1] * F[0]))) + h * (e[0] * w1 + e[1] * w0) + m[1];
+    Sr[2] += ms * (2.0 * (p * e[0] * e[2] + y12g * (e[0] * F[2] + e[2] * F[0]))) + h * (e[0] * w2 + e[2] * w0) + m[2];
+    Sr[3] += ms * (2.0 * (p * e[1] * e[2] + y12g * (e[1] * F[2] + e[2] * F[1]))) + h * (e[1] * w2 + e[2] * w1) + m[3];
+    Sr[4] += ms * (p * (e[1] * e[1] - e[2] * e[2]) + 2.0 * y12g * (e[1] * F[1] - e[2] * F[2])) +
+             h * (e[1] * w1 - e[2] * w2) + m[4];
+}
+
+// mob()'s stresslet block M_ss (5Nc x 5Nc: self blocks and the pairs' m tiles, placement of
+// src/mob.cpp:1129-1143), column q of a pair's m tile being m applied to the unit stresslet q.
 // This slice: unordered pairs p = rank BW + t, rank BW + t + g BW, ... (row-major over c1 < c2).
 __device__ void mob_slice(const BRep& R, double* P, int rank, int g) {
     const int Nc = R.Nc, Nm = R.Nm;
-    const int64_t n5 = 5 * (int64_t)Nc, n11 = 11 * (int64_t)Nc;
+    const int64_t n5 = 5 * (int64_t)Nc;
     const double* xm = P + R.off[BO_XM];
     double* Mss = P + R.off[BO_MSS];
-    double* Mcc = P + R.off[BO_MCC];
     const int64_t npair = (int64_t)Nc * (Nc - 1) / 2;
     auto first = [Nc](int64_t c) { return c * (2 * (int64_t)Nc - c - 1) / 2; };   // pairs before row c
     for (int64_t p = (int64_t)rank * BW + threadIdx.x; p < npair; p += (int64_t)g * BW) {
@@ -262,41 +325,16 @@ __device__ void mob_slice(const BRep& R, double* P, int rank, int g) {
         const int64_t c2 = c1 + 1 + (p - first(c1));
         double e[3], dr, dr_inv;
         bgeom(xm, Nm + (int)c1, Nm + (int)c2, e, dr, dr_inv);
-        BBT T;
-        tile_bb(dr_inv, e, T);
-        // put(target, source, i, j, v): M_cc entry of target component i, source component j
-        auto put = [&](int64_t ct, int64_t cs, int i, int j, double v) {
-            Mcc[cc_ix(Nc, (int)ct, i) * n11 + cc_ix(Nc, (int)cs, j)] = v;
-        };
-        for (int i = 0; i < 3; i++) {
-            for (int j = 0; j < 3; j++) {
-                put(c1, c2, i, j, T.a[i][j]);
-                put(c1, c2, i, 3 + j, -T.bt[j][i]);
-                put(c1, c2, 3 + i, j, T.b[i][j]);
-                put(c1, c2, 3 + i, 3 + j, T.c[i][j]);
-                put(c2, c1, i, j, T.a[j][i]);
-                put(c2, c1, i, 3 + j, T.bt[i][j]);
-                put(c2, c1, 3 + i, j, -T.b[j][i]);
-                put(c2, c1, 3 + i, 3 + j, T.c[j][i]);
-            }
+        const double ri3 = dr_inv * dr_inv * dr_inv, ri5 = ri3 * dr_inv * dr_inv;
+        for (int q = 0; q < 5; q++) {
+            double Sq[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, m[5];
+            Sq[q] = 1.0;
+            cc_mS(e, ri3, ri5, Sq, m);
             for (int j = 0; j < 5; j++) {
-                put(c1, c2, i, 6 + j, T.gt[i][j]);
-                put(c1, c2, 3 + i, 6 + j, T.ht[i][j]);
-                put(c1, c2, 6 + j, i, -T.gt[i][j]);
-                put(c1, c2, 6 + j, 3 + i, T.ht[i][j]);
-                put(c2, c1, i, 6 + j, -T.gt[i][j]);
-                put(c2, c1, 3 + i, 6 + j, T.ht[i][j]);
-                put(c2, c1, 6 + j, i, T.gt[i][j]);
-                put(c2, c1, 6 + j, 3 + i, T.ht[i][j]);
+                Mss[(5 * c1 + j) * n5 + 5 * c2 + q] = m[j];
+                Mss[(5 * c2 + j) * n5 + 5 * c1 + q] = m[j];
             }
         }
-        for (int j = 0; j < 5; j++)
-            for (int k = 0; k < 5; k++) {
-                put(c1, c2, 6 + j, 6 + k, T.m[j][k]);
-                put(c2, c1, 6 + j, 6 + k, T.m[k][j]);
-                Mss[(5 * c1 + j) * n5 + 5 * c2 + k] = T.m[j][k];
-                Mss[(5 * c2 + j) * n5 + 5 * c1 + k] = T.m[k][j];
-            }
     }
 }
 
@@ -309,8 +347,8 @@ __device__ void mob_slice(const BRep& R, double* P, int rank, int g) {
 // This slice of a team of g: monomer rows in blocks of 64 targets b = rank, rank + g, ... (the
 // lanes take the targets, wave w the sources w, w + 16, ... of each staged tile of 1024, the 16
 // partial sums added in a fixed order), then the rows of colloids c = rank, rank + g, ... (one
-// wave per colloid: the monomer sources over the lanes, then the stored M_cc rows against the
-// colloid part of f).  stage: LDS, 6 BW doubles.
+// wave per colloid: the monomer sources over the lanes, then the colloid sources over the lanes,
+// coupled in contracted form).  stage: LDS, 6 BW doubles.
 template <class T>
 __device__ __forceinline__ void apply_slice(const BConst& B, const BRep& R, const double* pool, const double* f,
                                             double* u, bool srows, double* stage, int rank, int g) {
@@ -435,11 +473,8 @@ __device__ __forceinline__ void apply_slice(const BConst& B, const BRep& R, cons
         }
     }
     // colloid rows.  Monomer sources (AB: rotation and stresslet rows, and the sum of monomer forces
-    // for the transposed carried tile) over the lanes; colloid sources: the stored M_cc rows of the
-    // colloid against f[nm3 .. n), which is [F; T; S] in M_cc's column order.
-    const int64_t n11 = 11 * (int64_t)Nc;
-    const double* Mcc = pool + R.off[BO_MCC];
-    const double* fcol = f + nm3;
+    // for the transposed carried tile) over the lanes, then the colloid sources over the lanes
+    // (the contracted colloid-colloid coupling, FP64).
     for (int kc = wv;; kc += NW) {
         const int c = rank + kc * g;
         if (c >= Nc) break;
@@ -482,27 +517,35 @@ __device__ __forceinline__ void apply_slice(const BConst& B, const BRep& R, cons
         for (int k = 3; k < 11; k++) av[k] = wsum((double)acc[k]);
         const double fs[3] = {wsum((double)fsx), wsum((double)fsy), wsum((double)fsz)};
         double gm[11];
-        if (srows) {
-            const double* mr[5];
-            double ga[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            for (int q = 0; q < 5; q++) mr[q] = Mcc + cc_ix(Nc, c, 6 + q) * n11;
-            for (int64_t k = lane; k < n11; k += 64) {
-                const double fv = fcol[k];
-                for (int q = 0; q < 5; q++) ga[q] += mr[q][k] * fv;
+        {
+            double U[3] = {0.0, 0.0, 0.0}, Wr[3] = {0.0, 0.0, 0.0}, Sr[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            const double xc0 = xm[3 * gc], xc1 = xm[3 * gc + 1], xc2 = xm[3 * gc + 2];
+            for (int s2 = lane; s2 < Nc; s2 += 64) {
+                if (s2 == c) continue;
+                const int gs = Nm + s2;
+                // e = (x_lo - x_hi) / r
+                const double sgn = (c < s2) ? 1.0 : -1.0;
+                const double dx = sgn * (xc0 - xm[3 * gs]), dy = sgn * (xc1 - xm[3 * gs + 1]),
+                             dz = sgn * (xc2 - xm[3 * gs + 2]);
+                const double dr = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+                const double ri = 1 / dr;
+                const double e[3] = {dx * ri, dy * ri, dz * ri};
+                const double* Fs = f + nm3 + 3 * (int64_t)s2;
+                const double* Ts = f + nm3 + 3 * (int64_t)Nc + 3 * (int64_t)s2;
+                const double* Ss = f + nu + 5 * (int64_t)s2;
+                const double Fv[3] = {Fs[0], Fs[1], Fs[2]}, Tv[3] = {Ts[0], Ts[1], Ts[2]};
+                const double Sv[5] = {Ss[0], Ss[1], Ss[2], Ss[3], Ss[4]};
+                if (srows)
+                    cc_couple<true>(e, ri, sgn, Fv, Tv, Sv, U, Wr, Sr);
+                else
+                    cc_couple<false>(e, ri, sgn, Fv, Tv, Sv, U, Wr, Sr);
             }
-            for (int q = 0; q < 5; q++) gm[6 + q] = wsum(ga[q]);
-        } else {
-            const double* mr[11];
-            double ga[11];
-            for (int q = 0; q < 11; q++) {
-                mr[q] = Mcc + cc_ix(Nc, c, q) * n11;
-                ga[q] = 0.0;
-            }
-            for (int64_t k = lane; k < n11; k += 64) {
-                const double fv = fcol[k];
-                for (int q = 0; q < 11; q++) ga[q] += mr[q][k] * fv;
-            }
-            for (int q = 0; q < 11; q++) gm[q] = wsum(ga[q]);
+            if (!srows)
+                for (int q = 0; q < 3; q++) {
+                    gm[q] = wsum(U[q]);
+                    gm[3 + q] = wsum(Wr[q]);
+                }
+            for (int q = 0; q < 5; q++) gm[6 + q] = wsum(Sr[q]);
         }
         if (lane != 0) continue;
         if (!srows) {
@@ -574,8 +617,8 @@ __device__ void wg_apply(const BConst& B, const BRep& R, double* P, const double
     team_op(B, R, P, tc, sizeof(T) == 4 ? TOP_APPLY_F : srows ? TOP_APPLY_DS : TOP_APPLY_D, f, u, nullptr, 0);
 }
 
-// mob(): snapshot xm = xpd, the carried tile, the self blocks of M_ss and zero diagonal blocks of
-// M_cc, then the pair tiles (split over the team).
+// mob(): snapshot xm = xpd, the carried tile, the self blocks of M_ss, then the pairs' m tiles
+// (split over the team).
 __device__ void wg_mob(const BConst& B, const BRep& R, double* P, const double* xpd, TeamCtx& tc) {
     double* xm = P + R.off[BO_XM];
     double* st = P + R.off[BO_ST];
@@ -583,13 +626,8 @@ __device__ void wg_mob(const BConst& B, const BRep& R, double* P, const double* 
     __syncthreads();
     if (threadIdx.x == 0) b_stale(B, R, xm, st);
     const int Nc = R.Nc;
-    const int64_t n5 = 5 * (int64_t)Nc, n11 = 11 * (int64_t)Nc;
+    const int64_t n5 = 5 * (int64_t)Nc;
     double* Mss = P + R.off[BO_MSS];
-    double* Mcc = P + R.off[BO_MCC];
-    for (int64_t idx = threadIdx.x; idx < 121 * (int64_t)Nc; idx += BW) {
-        const int c = (int)(idx / 121), q = (int)(idx % 121);
-        Mcc[cc_ix(Nc, c, q / 11) * n11 + cc_ix(Nc, c, q % 11)] = 0.0;
-    }
     for (int64_t idx = threadIdx.x; idx < 25 * (int64_t)Nc; idx += BW) {
         const int64_t c = idx / 25;
         const int i = (int)(idx % 25) / 5, j = (int)(idx % 5);
@@ -872,20 +910,19 @@ __device__ void team_member(const BStepArgs& a, const BRep& R, TeamCtx& tc, int*
         if (threadIdx.x == 0) {
             const bool ok = team_wait_ge(&tc.ctl->seq, last + 1, &tc.ctl->err);
             if (!ok) __hip_atomic_store(tc.gerr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the fork's arguments and inputs
             *cmd = ok;
         }
         __syncthreads();
         const int ok = *cmd;
         __syncthreads();
         if (!ok) return;
-        __threadfence();   // acquire: the fork's arguments and inputs
         last++;
         const BTeamCtl* c = tc.ctl;
         const int op = c->op;
         if (op == TOP_EXIT) return;
         run_slice(a.B, R, a.pool, op, c->in, c->out, c->mat, c->n, tc.rank, tc.g);
-        __threadfence();   // release this slice's outputs
-        __syncthreads();
+        wg_release_point();   // this slice's outputs
         if (threadIdx.x == 0) __hip_atomic_fetch_add(&tc.ctl->done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -1135,6 +1172,7 @@ struct Batch {
     int* d_err = nullptr;
     double* d_times = nullptr;
     int64_t steps_run = 0;
+    std::thread prefetch;        // draws the next run's first step of noise into slot 0
     double* pool = nullptr;
     int64_t pool_n = 0;
     double* noise_d = nullptr;
@@ -1177,6 +1215,7 @@ struct Batch {
 
 void batch_free(Batch* b) {
     if (!b) return;
+    if (b->prefetch.joinable()) b->prefetch.join();
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (void* q : {(void*)b->d_reps, (void*)b->d_stats, (void*)b->pool, (void*)b->noise_d, (void*)b->d_sched,
                     (void*)b->d_ctl, (void*)b->d_err, (void*)b->d_times})
@@ -1358,10 +1397,9 @@ ps_err ps_batch_create(const ps_params* p, int R, const int* Nc, const int* Npol
                 break;
             }
         const int64_t n3 = 3 * (int64_t)Q.Np, n5 = 5 * (int64_t)Q.Nc;
-        const int64_t n11 = 11 * (int64_t)Q.Nc;
         const int64_t sizes[BO_NOFF] = {n3, n3, n3, n3, 4 * (int64_t)Q.Nc, 3 * (int64_t)Q.Nc, Q.ns, Q.ns, Q.ns, Q.ns,
                                         Q.nu, Q.nu, Q.nu, Q.nu, Q.nu, Q.nu, Q.n, Q.n, Q.n, (int64_t)p->sqrt_kmax * Q.n,
-                                        n5, 9 * n5, n5 * n5, n11 * n11, 18, 0};
+                                        n5, 9 * n5, n5 * n5, 18, 0};
         for (int k = 0; k < BO_NOFF; k++) {
             if (k == BO_NZ) continue;
             Q.off[k] = pos;
@@ -1560,9 +1598,15 @@ ps_err ps_batch_run(void* h, int nsteps) {
     int filled = 0, consumed = 0;
     bool abort = false;
     std::thread producer;
+    int first = 0;   // steps whose noise is already drawn (the previous run's prefetch)
+    if (b->prefetch.joinable()) {
+        b->prefetch.join();
+        first = 1;
+        filled = 1;
+    }
     if (noisy) {
-        producer = std::thread([&] {
-            for (int s = 0; s < nsteps; s++) {
+        producer = std::thread([&, first] {
+            for (int s = first; s < nsteps; s++) {
                 const int k = s & 1;
                 {
                     std::unique_lock<std::mutex> lk(mu);
@@ -1630,6 +1674,8 @@ ps_err ps_batch_run(void* h, int nsteps) {
     }
     if (rc != PS_OK) return rc;
     BPS_HIP(b, hipStreamSynchronize(b->stream));
+    // draw the next run's first step of noise while the caller works (every copy has completed)
+    if (noisy) b->prefetch = std::thread([b] { b->fill(0); });
     return batch_check_err(b);
 }
 
