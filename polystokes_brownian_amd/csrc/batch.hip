@@ -28,6 +28,7 @@ namespace {
 
 constexpr int BW = BATCH_THREADS;   // threads per replica workgroup
 constexpr int NW = BW / 64;
+constexpr int BCCH = 256;   // colloids staged in LDS per pass of the monomer rows
 
 __device__ __forceinline__ double wsum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -56,7 +57,7 @@ __device__ double wg_dot(const double* a, const double* b, int64_t n, double* re
 // kernel is a cooperative launch, so all workgroups are co-resident and a member always reaches
 // its next wait.  Every wait is bounded (wall clock): a timeout sets the team's err, which ends
 // every later wait of the team at once, and the batch's error flag, which the host reports.
-enum TeamOp { TOP_EXIT = 1, TOP_APPLY_D, TOP_APPLY_DS, TOP_APPLY_F, TOP_GEMV, TOP_MOB };
+enum TeamOp { TOP_EXIT = 1, TOP_APPLY_D, TOP_APPLY_DS, TOP_APPLY_F, TOP_GEMV, TOP_MOB, TOP_ROWS_D, TOP_ROWS_F };
 constexpr uint64_t TEAM_WAIT_TICKS = 1000000000ull;   // 10 s of the 100 MHz wall clock
 
 struct TeamCtx {
@@ -339,29 +340,167 @@ __device__ void mob_slice(const BRep& R, double* P, int rank, int g) {
 }
 
 // ---------------------------------------------------------------------------------- apply
+// Fused multiply-add, written out where the tiles want it (this file is built without contraction,
+// which the pair geometry and the integrator need).
+__device__ __forceinline__ float b_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double b_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Wave rotation (DPP wave_rol:1): lane t receives lane t + 1's value.
+__device__ __forceinline__ float b_rol1(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x134, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double b_rol1(double v) {
+    const long long q = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(q & 0xffffffff), 0x134, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(q >> 32), 0x134, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// The AA tile's coefficients at r^2 (lengths in monomer radii): ya = 3/(4r) - 1/(2r^3) and
+// cc = -3/(8r^3) - 3/(2r^5), or the overlap forms 1 - 9r/32 and (21r/32 - 2)/r^2 for r <= 2 (the
+// overlap branch is taken only when some lane of the wave needs it).
+template <class T>
+__device__ __forceinline__ void aa_coef(T r2, T tb2, T& ya, T& cc) {
+    const T rinv = ps_rsqrt(r2);
+    const T rinv2 = rinv * rinv;
+    ya = rinv * b_fma(-(T)C1D2, rinv2, (T)C3D4);
+    cc = (rinv * rinv2) * b_fma(-(T)C3D2, rinv2, -(T)C3D8);
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(r2 <= tb2) != 0, 0)) {
+        if (r2 <= tb2) {
+            const T r = r2 * rinv;
+            ya = b_fma(-(T)C9D32, r, (T)1);
+            cc = b_fma((T)(21. / 32.), r, -(T)2) * rinv2;
+        }
+    }
+}
+
+// Monomer-monomer sums of u = M f (mobility(..., AA=true), src/mob.cpp:122-131: y f + (x - y)/r^2
+// (d.f) d) by 64 x 64 blocks, one block pair per wave (pairs rank 16 + w, + 16 g, ...):
+// part[(3K + c) Nm + i] = the sum over the sources of block K for target i.  The pair tile is
+// symmetric, so a pair I < J is evaluated once: lanes hold the targets of I, and at step m lane l
+// takes source (l + m) mod 64 of J from the wave's LDS window and adds the tile to u_i; the
+// source-side sum u_j travels with the sources by a wave rotation, so after 64 steps lane l holds
+// it for source (l + 63) mod 64.  Diagonal blocks (and every block pair when chain_HI skips same-
+// chain pairs) run one way.  Padding beads sit far apart with zero force.  stage: LDS, 16 windows
+// of 64 sources.
+template <class T>
+__device__ __forceinline__ void aa_pairs(const BConst& B, const BRep& R, const double* xm, const double* f,
+                                         double* part, double* stage, int rank, int g) {
+    const int Nm = R.Nm, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nblk = (Nm + 63) >> 6;
+    constexpr int SW = sizeof(T) == 8 ? 6 : 8;
+    T* win = reinterpret_cast<T*>(stage) + wv * 64 * SW;
+    const T tb2 = (T)(B.two_beta * B.two_beta), FAR = (T)1e10;
+    const bool chain = B.chain_HI != 0;
+    const int64_t nitem = chain ? (int64_t)nblk * nblk : (int64_t)nblk * (nblk + 1) / 2;
+    auto first = [nblk](int64_t I) { return I * nblk - I * (I - 1) / 2; };   // items before row I
+    for (int64_t it = (int64_t)rank * NW + wv; it < nitem; it += (int64_t)g * NW) {
+        int I, J;
+        if (chain) {
+            I = (int)(it / nblk);
+            J = (int)(it - (int64_t)I * nblk);
+        } else {
+            const double m2 = 2.0 * nblk + 1.0;
+            int64_t r = (int64_t)((m2 - __builtin_sqrt(fmax(m2 * m2 - 8.0 * (double)it, 0.0))) * 0.5);
+            if (r < 0) r = 0;
+            while (r > 0 && first(r) > it) r--;
+            while (r + 1 < nblk && first(r + 1) <= it) r++;
+            I = (int)r;
+            J = (int)(I + (it - first(r)));
+        }
+        const int i = (I << 6) + lane, j = (J << 6) + lane;
+        const bool ai = i < Nm, aj = j < Nm;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // the previous item's window reads
+        {
+            T* w = win + SW * lane;
+            w[0] = aj ? (T)xm[3 * j] : -FAR;
+            w[1] = aj ? (T)xm[3 * j + 1] : -FAR;
+            w[2] = aj ? (T)xm[3 * j + 2] : -FAR;
+            w[3] = aj ? (T)f[3 * j] : (T)0;
+            w[4] = aj ? (T)f[3 * j + 1] : (T)0;
+            w[5] = aj ? (T)f[3 * j + 2] : (T)0;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const T xi = ai ? (T)xm[3 * i] : FAR, yi = ai ? (T)xm[3 * i + 1] : FAR, zi = ai ? (T)xm[3 * i + 2] : FAR;
+        const T fxi = ai ? (T)f[3 * i] : (T)0, fyi = ai ? (T)f[3 * i + 1] : (T)0, fzi = ai ? (T)f[3 * i + 2] : (T)0;
+        T ux = 0, uy = 0, uz = 0;
+        if (I == J || chain) {
+            const int ci = i / R.Nmono;
+            for (int m = 0; m < 64; m++) {
+                const T* sj = win + SW * m;
+                const int jj = (J << 6) + m;
+                bool skip = jj == i || jj >= Nm;
+                if (chain) skip = skip || jj / R.Nmono == ci;
+                const T dx = xi - sj[0], dy = yi - sj[1], dz = zi - sj[2];
+                T r2 = b_fma(dz, dz, b_fma(dy, dy, dx * dx));
+                r2 = skip ? (T)1 : r2;
+                T ya, cc;
+                aa_coef(r2, tb2, ya, cc);
+                ya = skip ? (T)0 : ya;
+                cc = skip ? (T)0 : cc;
+                const T fx = sj[3], fy = sj[4], fz = sj[5];
+                const T cs = cc * b_fma(dz, fz, b_fma(dy, fy, dx * fx));
+                ux = b_fma(cs, dx, b_fma(ya, fx, ux));
+                uy = b_fma(cs, dy, b_fma(ya, fy, uy));
+                uz = b_fma(cs, dz, b_fma(ya, fz, uz));
+            }
+        } else {
+            T Rx = 0, Ry = 0, Rz = 0;
+            for (int m = 0; m < 64; m++) {
+                const T* sj = win + SW * ((lane + m) & 63);
+                const T dx = xi - sj[0], dy = yi - sj[1], dz = zi - sj[2];
+                const T r2 = b_fma(dz, dz, b_fma(dy, dy, dx * dx));
+                T ya, cc;
+                aa_coef(r2, tb2, ya, cc);
+                const T fx = sj[3], fy = sj[4], fz = sj[5];
+                const T cs = cc * b_fma(dz, fz, b_fma(dy, fy, dx * fx));
+                ux = b_fma(cs, dx, b_fma(ya, fx, ux));
+                uy = b_fma(cs, dy, b_fma(ya, fy, uy));
+                uz = b_fma(cs, dz, b_fma(ya, fz, uz));
+                // u_j += ya f_i + (d.f_i)/r^2 d (the same tile: d -> -d twice)
+                const T cj = cc * b_fma(dz, fzi, b_fma(dy, fyi, dx * fxi));
+                Rx = b_rol1(Rx) + b_fma(cj, dx, ya * fxi);
+                Ry = b_rol1(Ry) + b_fma(cj, dy, ya * fyi);
+                Rz = b_rol1(Rz) + b_fma(cj, dz, ya * fzi);
+            }
+            const int jo = (J << 6) + ((lane + 63) & 63);
+            if (jo < Nm) {
+                part[(3 * (int64_t)I + 0) * Nm + jo] = (double)Rx;
+                part[(3 * (int64_t)I + 1) * Nm + jo] = (double)Ry;
+                part[(3 * (int64_t)I + 2) * Nm + jo] = (double)Rz;
+            }
+        }
+        if (ai) {
+            part[(3 * (int64_t)J + 0) * Nm + i] = (double)ux;
+            part[(3 * (int64_t)J + 1) * Nm + i] = (double)uy;
+            part[(3 * (int64_t)J + 2) * Nm + i] = (double)uz;
+        }
+    }
+}
+
 // u = M f for the grand mobility of src/mob.cpp:635-1213 (the layout of a single context; the
 // reference's quirks: the carried translation tile, the '+' of mob_c, the 2-1 gt sign).  T is the
 // arithmetic of the monomer-monomer and monomer-colloid tiles (float in the mixed form's square
 // root); colloid-colloid tiles are FP64.  `srows`: only the stresslet rows (nu..n).
 //
-// This slice of a team of g: monomer rows in blocks of 64 targets b = rank, rank + g, ... (the
-// lanes take the targets, wave w the sources w, w + 16, ... of each staged tile of 1024, the 16
-// partial sums added in a fixed order), then the rows of colloids c = rank, rank + g, ... (one
-// wave per colloid: the monomer sources over the lanes, then the colloid sources over the lanes,
-// coupled in contracted form).  stage: LDS, 6 BW doubles.
+// Two passes (a team joins between them).  sel 0: the monomer-monomer block pairs (aa_pairs) and
+// the rows of colloids c = rank, rank + g, ... (one wave per colloid: the monomer sources from LDS
+// tiles over the lanes, then the colloid sources over the lanes, coupled in contracted form).
+// sel 1: the monomer rows in blocks of 64 targets b = rank, rank + g, ...: the colloid sources
+// (wave w taking colloids w, w + 16, ... from LDS), the partner blocks' partial sums, the carried
+// tile and the self term.  stage: LDS, 6 BW doubles; cstage: 11 BCCH doubles.
 template <class T>
-__device__ __forceinline__ void apply_slice(const BConst& B, const BRep& R, const double* pool, const double* f,
-                                            double* u, bool srows, double* stage, int rank, int g) {
+__device__ __forceinline__ void apply_slice(const BConst& B, const BRep& R, double* pool, const double* f, double* u,
+                                            bool srows, int sel, double* stage, double* cstage, int rank, int g) {
     const double* xm = pool + R.off[BO_XM];
     const double* st = pool + R.off[BO_ST];
     const int Nm = R.Nm, Nc = R.Nc, t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int64_t nm3 = R.nm3, nu = R.nu;
-    const T tb2 = (T)(B.two_beta * B.two_beta);
     const T gA = (T)(C9D20 * (3 + 5 * B.beta2));
     const T gB = (T)(C9D10 * (C1D2 + C5D6 * B.beta2));
-    constexpr int SW = sizeof(T) == 8 ? 6 : 8;   // staged source stride (x y z fx fy fz [pad])
-    T* sh = reinterpret_cast<T*>(stage);
-    if (!srows && Nm > 0) {
+    double* part = pool + R.off[BO_PART];
+    if (!srows && Nm > 0 && sel == 0) aa_pairs<T>(B, R, xm, f, part, stage, rank, g);
+    if (!srows && Nm > 0 && sel == 1) {
         // the carried tile applied to the sum of colloid forces (the same in every monomer row)
         double F0 = 0.0, F1 = 0.0, F2 = 0.0;
         for (int c = 0; c < Nc; c++) {
@@ -377,60 +516,30 @@ __device__ __forceinline__ void apply_slice(const BConst& B, const BRep& R, cons
             const int i = (b << 6) + lane;
             const bool act = i < Nm;
             T xi = 0, yi = 0, zi = 0;
-            int ci = -1;
             if (act) {
                 xi = (T)xm[3 * i];
                 yi = (T)xm[3 * i + 1];
                 zi = (T)xm[3 * i + 2];
-                ci = i / R.Nmono;
             }
-            T ux = 0, uy = 0, uz = 0;
-            // monomer sources (mobility(..., AA=true), src/mob.cpp:122-131): y f + (x - y)/r^2 (d.f) d
-            for (int j0 = 0; j0 < Nm; j0 += BW) {
-                const int cnt = min(BW, Nm - j0);
+            // colloid sources (mobility_AB, src/mob.cpp:362-537): rotlet and stresslet couplings,
+            // [x T S] of up to BCCH colloids staged in LDS per pass, wave w taking w, w + 16, ...
+            T vx = 0, vy = 0, vz = 0;
+            T* cs = reinterpret_cast<T*>(cstage);
+            for (int c0 = 0; c0 < Nc; c0 += BCCH) {
+                const int cn = min(BCCH, Nc - c0);
                 __syncthreads();
-                if (t < cnt) {
-                    const int j = j0 + t;
-                    T* d = sh + SW * t;
-                    d[0] = (T)xm[3 * j];
-                    d[1] = (T)xm[3 * j + 1];
-                    d[2] = (T)xm[3 * j + 2];
-                    d[3] = (T)f[3 * j];
-                    d[4] = (T)f[3 * j + 1];
-                    d[5] = (T)f[3 * j + 2];
+                for (int idx = t; idx < 11 * cn; idx += BW) {
+                    const int cc = idx / 11, q = idx - 11 * (idx / 11), c = c0 + cc;
+                    const double v = q < 3 ? xm[3 * (int64_t)(Nm + c) + q]
+                                           : q < 6 ? f[nm3 + 3 * (int64_t)Nc + 3 * c + (q - 3)]
+                                                   : f[nu + 5 * (int64_t)c + (q - 6)];
+                    cs[idx] = (T)v;
                 }
                 __syncthreads();
                 if (!act) continue;
-                for (int k = wv; k < cnt; k += NW) {
-                    const int jj = j0 + k;
-                    if (jj == i) continue;
-                    if (B.chain_HI && jj / R.Nmono == ci) continue;
-                    const T* sk = sh + SW * k;
-                    const T dx = xi - sk[0], dy = yi - sk[1], dz = zi - sk[2];
-                    const T r2 = dx * dx + dy * dy + dz * dz;
-                    const T rinv = ps_rsqrt(r2);
-                    const T rinv2 = rinv * rinv;
-                    T ya = rinv * ((T)C3D4 - (T)C1D2 * rinv2);
-                    T cc = rinv * rinv2 * (-(T)C3D8 - (T)C3D2 * rinv2);
-                    if (r2 <= tb2) {
-                        const T r = r2 * rinv;
-                        ya = (T)1 - (T)C9D32 * r;
-                        cc = ((T)(21. / 32.) * r - (T)2) * rinv2;
-                    }
-                    const T fx = sk[3], fy = sk[4], fz = sk[5];
-                    const T cs = cc * (dx * fx + dy * fy + dz * fz);
-                    ux += ya * fx + cs * dx;
-                    uy += ya * fy + cs * dy;
-                    uz += ya * fz + cs * dz;
-                }
-            }
-            // colloid sources (mobility_AB, src/mob.cpp:362-537): rotlet and stresslet couplings,
-            // wave w taking colloids w, w + 16, ...
-            T vx = 0, vy = 0, vz = 0;
-            if (act)
-                for (int c = wv; c < Nc; c += NW) {
-                    const int gc = Nm + c;
-                    const T dx = xi - (T)xm[3 * gc], dy = yi - (T)xm[3 * gc + 1], dz = zi - (T)xm[3 * gc + 2];
+                for (int cc = wv; cc < cn; cc += NW) {
+                    const T* cv = cs + 11 * cc;
+                    const T dx = xi - cv[0], dy = yi - cv[1], dz = zi - cv[2];
                     const T r2 = dx * dx + dy * dy + dz * dz;
                     const T rinv = ps_rsqrt(r2);
                     const T ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
@@ -438,10 +547,8 @@ __device__ __forceinline__ void apply_slice(const BConst& B, const BRep& R, cons
                     const T y12b = -(T)C3D4 * rinv2;
                     const T x12g = (T)C9D4 * rinv2 - gA * rinv4;
                     const T y12g = gB * rinv4;
-                    const int64_t tq = nm3 + 3 * (int64_t)Nc + 3 * c;
-                    const T Tx = (T)f[tq], Ty = (T)f[tq + 1], Tz = (T)f[tq + 2];
-                    const double* S = f + nu + 5 * (int64_t)c;
-                    const T S0 = (T)S[0], S1 = (T)S[1], S2 = (T)S[2], S3 = (T)S[3], S4 = (T)S[4];
+                    const T Tx = cv[3], Ty = cv[4], Tz = cv[5];
+                    const T S0 = cv[6], S1 = cv[7], S2 = cv[8], S3 = cv[9], S4 = cv[10];
                     const T sex = S0 * ex + S1 * ey + S2 * ez;
                     const T sey = S1 * ex + S4 * ey + S3 * ez;
                     const T sez = S2 * ex + S3 * ey - (S0 + S4) * ez;
@@ -452,12 +559,21 @@ __device__ __forceinline__ void apply_slice(const BConst& B, const BRep& R, cons
                     vy += y12b * (ez * Tx - ex * Tz) + a * ey + b2 * sey;
                     vz += y12b * (ex * Ty - ey * Tx) + a * ez + b2 * sez;
                 }
-            // the 16 partial sums of each target, added in wave order
+            }
+            // wave w adds the monomer partial sums of partner blocks w, w + 16, ...; then the 16
+            // sums of each target, added in wave order
+            double p0 = (double)vx, p1 = (double)vy, p2 = (double)vz;
+            if (act)
+                for (int K = wv; K < nblk; K += NW) {
+                    p0 += part[(3 * (int64_t)K + 0) * Nm + i];
+                    p1 += part[(3 * (int64_t)K + 1) * Nm + i];
+                    p2 += part[(3 * (int64_t)K + 2) * Nm + i];
+                }
             __syncthreads();
             double* pr = stage;
-            pr[(0 * NW + wv) * 64 + lane] = (double)ux + (double)vx;
-            pr[(1 * NW + wv) * 64 + lane] = (double)uy + (double)vy;
-            pr[(2 * NW + wv) * 64 + lane] = (double)uz + (double)vz;
+            pr[(0 * NW + wv) * 64 + lane] = p0;
+            pr[(1 * NW + wv) * 64 + lane] = p1;
+            pr[(2 * NW + wv) * 64 + lane] = p2;
             __syncthreads();
             if (wv == 0 && act) {
                 double s0 = 0.0, s1 = 0.0, s2 = 0.0;
@@ -472,47 +588,68 @@ __device__ __forceinline__ void apply_slice(const BConst& B, const BRep& R, cons
             }
         }
     }
-    // colloid rows.  Monomer sources (AB: rotation and stresslet rows, and the sum of monomer forces
-    // for the transposed carried tile) over the lanes, then the colloid sources over the lanes
-    // (the contracted colloid-colloid coupling, FP64).
-    for (int kc = wv;; kc += NW) {
+    if (sel == 1) return;
+    // colloid rows, 16 at a time (wave w: this rank's colloid r0 + w).  Monomer sources (AB: rotation
+    // and stresslet rows, and the sum of monomer forces for the transposed carried tile) from LDS
+    // tiles of 1024 monomers shared by the 16 waves, over the lanes; then the colloid sources over
+    // the lanes (the contracted colloid-colloid coupling, FP64).
+    const int ncol = Nc > rank ? (Nc - rank + g - 1) / g : 0;
+    T* sx = reinterpret_cast<T*>(stage);
+    for (int r0 = 0; r0 < ncol; r0 += NW) {
+        const int kc = r0 + wv;
+        const bool cact = kc < ncol;
         const int c = rank + kc * g;
-        if (c >= Nc) break;
-        const int gc = Nm + c;
+        const int gc = Nm + (cact ? c : 0);
         const T xc = (T)xm[3 * gc], yc = (T)xm[3 * gc + 1], zc = (T)xm[3 * gc + 2];
         T acc[11];
         for (int k = 0; k < 11; k++) acc[k] = 0;
         T fsx = 0, fsy = 0, fsz = 0;
-        for (int m = lane; m < Nm; m += 64) {
-            const T dx = (T)xm[3 * m] - xc, dy = (T)xm[3 * m + 1] - yc, dz = (T)xm[3 * m + 2] - zc;
-            const T r2 = dx * dx + dy * dy + dz * dz;
-            const T rinv = ps_rsqrt(r2);
-            const T ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
-            const T rinv2 = rinv * rinv, rinv4 = rinv2 * rinv2;
-            const T y12b = -(T)C3D4 * rinv2;
-            const T x12g = (T)C9D4 * rinv2 - gA * rinv4;
-            const T y12g = gB * rinv4;
-            const T fx = (T)f[3 * m], fy = (T)f[3 * m + 1], fz = (T)f[3 * m + 2];
-            fsx += fx;
-            fsy += fy;
-            fsz += fz;
-            acc[3] += y12b * (fy * ez - fz * ey);
-            acc[4] += y12b * (fz * ex - fx * ez);
-            acc[5] += y12b * (fx * ey - fy * ex);
-            const T ef = ex * fx + ey * fy + ez * fz;
-            const T p = -x12g * ef - (T)2 * y12g * ef;
-            const T G00 = p * ex * ex + (T)2 * y12g * ex * fx;
-            const T G11 = p * ey * ey + (T)2 * y12g * ey * fy;
-            const T G22 = p * ez * ez + (T)2 * y12g * ez * fz;
-            const T G01 = p * ex * ey + y12g * (ex * fy + ey * fx);
-            const T G02 = p * ex * ez + y12g * (ex * fz + ez * fx);
-            const T G12 = p * ey * ez + y12g * (ey * fz + ez * fy);
-            acc[6] += G00 - G22;
-            acc[7] += (T)2 * G01;
-            acc[8] += (T)2 * G02;
-            acc[9] += (T)2 * G12;
-            acc[10] += G11 - G22;
+        for (int m0 = 0; m0 < Nm; m0 += BW) {
+            const int mn = min(BW, Nm - m0);
+            __syncthreads();
+            if (t < mn) {
+                const int64_t m = m0 + t;
+                sx[t] = (T)xm[3 * m];
+                sx[BW + t] = (T)xm[3 * m + 1];
+                sx[2 * BW + t] = (T)xm[3 * m + 2];
+                sx[3 * BW + t] = (T)f[3 * m];
+                sx[4 * BW + t] = (T)f[3 * m + 1];
+                sx[5 * BW + t] = (T)f[3 * m + 2];
+            }
+            __syncthreads();
+            if (!cact) continue;
+            for (int mm = lane; mm < mn; mm += 64) {
+                const T dx = sx[mm] - xc, dy = sx[BW + mm] - yc, dz = sx[2 * BW + mm] - zc;
+                const T r2 = dx * dx + dy * dy + dz * dz;
+                const T rinv = ps_rsqrt(r2);
+                const T ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
+                const T rinv2 = rinv * rinv, rinv4 = rinv2 * rinv2;
+                const T y12b = -(T)C3D4 * rinv2;
+                const T x12g = (T)C9D4 * rinv2 - gA * rinv4;
+                const T y12g = gB * rinv4;
+                const T fx = sx[3 * BW + mm], fy = sx[4 * BW + mm], fz = sx[5 * BW + mm];
+                fsx += fx;
+                fsy += fy;
+                fsz += fz;
+                acc[3] += y12b * (fy * ez - fz * ey);
+                acc[4] += y12b * (fz * ex - fx * ez);
+                acc[5] += y12b * (fx * ey - fy * ex);
+                const T ef = ex * fx + ey * fy + ez * fz;
+                const T p = -x12g * ef - (T)2 * y12g * ef;
+                const T G00 = p * ex * ex + (T)2 * y12g * ex * fx;
+                const T G11 = p * ey * ey + (T)2 * y12g * ey * fy;
+                const T G22 = p * ez * ez + (T)2 * y12g * ez * fz;
+                const T G01 = p * ex * ey + y12g * (ex * fy + ey * fx);
+                const T G02 = p * ex * ez + y12g * (ex * fz + ez * fx);
+                const T G12 = p * ey * ez + y12g * (ey * fz + ez * fy);
+                acc[6] += G00 - G22;
+                acc[7] += (T)2 * G01;
+                acc[8] += (T)2 * G02;
+                acc[9] += (T)2 * G12;
+                acc[10] += G11 - G22;
+            }
         }
+        if (!cact) continue;
         double av[11];
         for (int k = 3; k < 11; k++) av[k] = wsum((double)acc[k]);
         const double fs[3] = {wsum((double)fsx), wsum((double)fsy), wsum((double)fsz)};
@@ -586,12 +723,17 @@ __device__ __forceinline__ void gemv_slice(const double* A, const double* v, dou
 __device__ __noinline__ void run_slice(const BConst& B, const BRep& R, double* P, int op, const double* in,
                                        double* out, const double* mat, int64_t n, int rank, int g) {
     __shared__ double stage[6 * BW];
+    __shared__ double cstage[11 * BCCH];
     if (op == TOP_APPLY_D)
-        apply_slice<double>(B, R, P, in, out, false, stage, rank, g);
+        apply_slice<double>(B, R, P, in, out, false, 0, stage, cstage, rank, g);
+    else if (op == TOP_ROWS_D)
+        apply_slice<double>(B, R, P, in, out, false, 1, stage, cstage, rank, g);
     else if (op == TOP_APPLY_DS)
-        apply_slice<double>(B, R, P, in, out, true, stage, rank, g);
+        apply_slice<double>(B, R, P, in, out, true, 0, stage, cstage, rank, g);
     else if (op == TOP_APPLY_F)
-        apply_slice<float>(B, R, P, in, out, false, stage, rank, g);
+        apply_slice<float>(B, R, P, in, out, false, 0, stage, cstage, rank, g);
+    else if (op == TOP_ROWS_F)
+        apply_slice<float>(B, R, P, in, out, false, 1, stage, cstage, rank, g);
     else if (op == TOP_GEMV)
         gemv_slice(mat, in, out, n, rank, g);
     else if (op == TOP_MOB)
@@ -614,7 +756,12 @@ __device__ void team_op(const BConst& B, const BRep& R, double* P, TeamCtx& tc, 
 template <class T>
 __device__ void wg_apply(const BConst& B, const BRep& R, double* P, const double* f, double* u, bool srows,
                          TeamCtx& tc) {
-    team_op(B, R, P, tc, sizeof(T) == 4 ? TOP_APPLY_F : srows ? TOP_APPLY_DS : TOP_APPLY_D, f, u, nullptr, 0);
+    if (srows) {
+        team_op(B, R, P, tc, TOP_APPLY_DS, f, u, nullptr, 0);
+        return;
+    }
+    team_op(B, R, P, tc, sizeof(T) == 4 ? TOP_APPLY_F : TOP_APPLY_D, f, u, nullptr, 0);
+    team_op(B, R, P, tc, sizeof(T) == 4 ? TOP_ROWS_F : TOP_ROWS_D, f, u, nullptr, 0);
 }
 
 // mob(): snapshot xm = xpd, the carried tile, the self blocks of M_ss, then the pairs' m tiles
@@ -1230,63 +1377,90 @@ void batch_free(Batch* b) {
 
 int64_t pad8(int64_t v) { return (v + 7) / 8 * 8; }
 
-constexpr double TEAM_EFF = 0.75;   // assumed parallel efficiency of a team (work per rank / time)
+// Team time model: a replica of lone-workgroup time t1 on a team of g takes
+// t1 (TEAM_SERIAL + (1 - TEAM_SERIAL) / g) TEAM_OVER (forces, Lanczos orthogonalisation and the
+// stresslet MINRES vector work stay on the leader; TEAM_OVER covers the fork-joins).
+constexpr double TEAM_SERIAL = 0.08, TEAM_OVER = 1.05;
+double team_time(double t1, int g) { return g <= 1 ? t1 : t1 * (TEAM_SERIAL + (1.0 - TEAM_SERIAL) / g) * TEAM_OVER; }
 
-// The step schedule from the cost estimates: for a target makespan M (the mean share of a
-// workgroup, raised by 5% steps until all teams fit in the co-resident workgroups), a replica whose
-// cost exceeds M gets a team of ceil(cost / (M eff)) workgroups (at most team_max, and at most one
-// rank per min_blocks work units -- blocks of 64 monomer rows or 16 colloid rows), then
-// every other replica goes, longest first, to the workgroup with the least estimated load.
+// The step schedule from the lone-workgroup time estimates (cost): the smallest makespan T (by
+// bisection) for which every replica with cost > T gets the smallest team (at most team_max, and at
+// most one rank per min_blocks work units -- blocks of 64 monomer rows or 16 colloid rows) that
+// finishes within T, the teams fit in the co-resident workgroups, and the other replicas, longest
+// first, each fit into the workgroup with the most time left before T (team workgroups after their
+// team phase).  Without cooperative launches every replica runs alone, packed the same way.
 void build_schedule(Batch* b) {
-    const int R = b->R;
+    const int R = b->R, W = b->nwg_max;
     std::vector<int> ord(R);
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return b->cost[x] > b->cost[y]; });
-    std::vector<int> g(R, 1);
-    double total = 0.0;
-    for (double c : b->cost) total += c;
     const bool teams = b->coop && b->team_max > 1;
-    if (teams) {
-        // target makespan M: the mean share of a workgroup, raised until the teams fit
-        std::vector<int> cap(R);
+    std::vector<int> cap(R, 1);
+    if (teams)
         for (int r = 0; r < R; r++) {
-            // work units of a rank: blocks of 64 monomer rows and groups of 16 colloid rows
             const int units = (b->reps[r].Nm + 63) / 64 + (b->reps[r].Nc + 15) / 16;
             cap[r] = std::min(b->team_max, std::max(1, units / std::max(1, b->min_blocks)));
         }
-        double M = total / b->nwg_max;
-        for (int it = 0; it < 200; it++) {
-            int used = 0;
-            for (int r = 0; r < R; r++) {
-                g[r] = 1;
-                if (b->cost[r] > M) g[r] = std::max(1, std::min(cap[r], (int)std::ceil(b->cost[r] / (M * TEAM_EFF))));
-                if (g[r] > 1) used += g[r];
-            }
-            if (used <= b->nwg_max) break;
-            M *= 1.05;
-            if (it == 199) std::fill(g.begin(), g.end(), 1);
+    std::vector<int> g(R, 1), gbest(R, 1);
+    std::vector<std::vector<int>> lists, lbest;
+    std::vector<int> team_wg;   // team index -> first workgroup
+    // feasibility of makespan T: fills g and lists (per workgroup: its solo replicas)
+    auto feasible = [&](double T) -> bool {
+        int used = 0;
+        std::vector<double> left;
+        for (int r : ord) {
+            g[r] = 1;
+            if (b->cost[r] <= T) continue;
+            int gr = 2;
+            while (gr <= cap[r] && team_time(b->cost[r], gr) > T) gr++;
+            if (gr > cap[r]) return false;
+            g[r] = gr;
+            for (int k = 0; k < gr; k++) left.push_back(T - team_time(b->cost[r], gr));
+            used += gr;
+            if (used > W) return false;
         }
-    }
-    int nteam_wg = 0, nsolo = 0;
-    b->nteams = 0;
-    for (int r : ord) {
-        if (g[r] > 1) {
-            nteam_wg += g[r];
-            b->nteams++;
-        } else {
-            nsolo++;
+        const int nteam_wg = used;
+        int nsolo = 0;
+        for (int r : ord) nsolo += g[r] == 1;
+        const int nw = std::max(1, std::min(W, nteam_wg + nsolo));
+        while ((int)left.size() < nw) left.push_back(T);
+        lists.assign(left.size(), {});
+        for (int r : ord) {
+            if (g[r] > 1) continue;
+            const int w = (int)(std::max_element(left.begin(), left.end()) - left.begin());
+            if (left[w] < b->cost[r] && !(left[w] == T && b->cost[r] <= T)) return false;
+            lists[w].push_back(r);
+            left[w] -= b->cost[r];
         }
+        return true;
+    };
+    double total = 0.0, lo = 0.0;
+    for (int r = 0; r < R; r++) {
+        total += b->cost[r];
+        lo = std::max(lo, team_time(b->cost[r], cap[r]));
     }
-    b->nwg = std::max(1, std::min(b->nwg_max, nteam_wg + nsolo));
-    if (!teams) b->nwg = std::max(1, std::min(R, b->nwg_max));
+    lo = std::max(lo, total / W);
+    double hi = std::max(lo, 1e-30) * 2.0;
+    int guard = 0;
+    while (!feasible(hi) && guard++ < 60) hi *= 2.0;
+    for (int it = 0; it < 40; it++) {
+        const double mid = 0.5 * (lo + hi);
+        if (feasible(mid))
+            hi = mid;
+        else
+            lo = mid;
+    }
+    feasible(hi);
     b->team_of = g;
+    b->nteams = 0;
+    for (int r : ord) b->nteams += g[r] > 1;
+    b->nwg = (int)lists.size();
     std::vector<int>& S = b->sched;
     S.assign(5 * (size_t)b->nwg + R, 0);
     b->team_off = 5 * b->nwg;
     b->solo_off = b->team_off + b->nteams;
-    std::vector<double> load(b->nwg, 0.0);
-    int w = 0, k = 0;
     for (int i = 0; i < b->nwg; i++) S[5 * i] = -1;
+    int w = 0, k = 0;
     for (int r : ord) {
         if (g[r] <= 1) continue;
         S[b->team_off + k] = r;
@@ -1294,17 +1468,8 @@ void build_schedule(Batch* b) {
             S[5 * w] = k;
             S[5 * w + 1] = j;
             S[5 * w + 2] = g[r];
-            load[w] = b->cost[r] / (g[r] * TEAM_EFF);
         }
         k++;
-    }
-    // solos: longest first onto the least-loaded workgroup (ties: lowest index)
-    std::vector<std::vector<int>> lists(b->nwg);
-    for (int r : ord) {
-        if (g[r] > 1) continue;
-        const int best = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        lists[best].push_back(r);
-        load[best] += b->cost[r];
     }
     int pos = 0;
     for (int i = 0; i < b->nwg; i++) {
@@ -1316,7 +1481,6 @@ void build_schedule(Batch* b) {
     b->sched_dirty = true;
     b->resched = false;
 }
-
 }  // namespace
 
 extern "C" {
@@ -1399,7 +1563,7 @@ ps_err ps_batch_create(const ps_params* p, int R, const int* Nc, const int* Npol
         const int64_t n3 = 3 * (int64_t)Q.Np, n5 = 5 * (int64_t)Q.Nc;
         const int64_t sizes[BO_NOFF] = {n3, n3, n3, n3, 4 * (int64_t)Q.Nc, 3 * (int64_t)Q.Nc, Q.ns, Q.ns, Q.ns, Q.ns,
                                         Q.nu, Q.nu, Q.nu, Q.nu, Q.nu, Q.nu, Q.n, Q.n, Q.n, (int64_t)p->sqrt_kmax * Q.n,
-                                        n5, 9 * n5, n5 * n5, 18, 0};
+                                        n5, 9 * n5, n5 * n5, 3 * (int64_t)((Q.Nm + 63) / 64) * Q.Nm, 18, 0};
         for (int k = 0; k < BO_NOFF; k++) {
             if (k == BO_NZ) continue;
             Q.off[k] = pos;
@@ -1571,7 +1735,7 @@ ps_err batch_schedule_step(Batch* b, bool measured) {
         for (int r = 0; r < b->R; r++) {
             const double us = ticks[r] * b->tick_us;
             const int g = b->team_of.empty() ? 1 : b->team_of[r];
-            if (us > 0) b->cost[r] = us * (g > 1 ? g * TEAM_EFF : 1.0);
+            if (us > 0) b->cost[r] = us / team_time(1.0, g);   // the lone-workgroup time it implies
         }
         build_schedule(b);
     } else if (b->resched) {
