@@ -312,6 +312,10 @@ ps_err ps_batch_op(void* batch, int op, const double* in, double* out, int* iter
  * it ran with; either output (R entries) may be null.  No reference counterpart: the reference
  * runs one system per process. */
 ps_err ps_batch_times(void* batch, double* us, int* team);
+/* The last step's time of each replica per phase in microseconds (5 R: forces, mob(), saddle
+ * solves, square root, and within those the team leader's time waiting in joins).  No reference
+ * counterpart. */
+ps_err ps_batch_phase_times(void* batch, double* us5);
 /* Step schedule: replicas far above the mean work run on a team of up to team_max (1..16)
  * workgroups, each rank with at least min_blocks (>= 1) work units (blocks of 64 monomer rows or
  * 16 colloid rows); with autotune

@@ -67,6 +67,7 @@ struct BSchedDev {
     BTeamCtl* ctl;
     int* err;          // set when a team wait timed out
     double* times;     // per replica: wall-clock ticks of its last step (leader's view)
+    double* phases;    // per replica: ticks in forces, mob(), saddle solves, square root, team joins
 };
 
 ps_err batch_launch_step(hipStream_t st, const BConst& B, const BRep* reps, const BSchedDev& sd, int nwg,

@@ -29,6 +29,7 @@ namespace {
 constexpr int BW = BATCH_THREADS;   // threads per replica workgroup
 constexpr int NW = BW / 64;
 constexpr int BCCH = 256;   // colloids staged in LDS per pass of the monomer rows
+constexpr int FTILE = (4 * BATCH_KMAX + BATCH_KMAX * BATCH_KMAX) / 3;   // force partners per LDS tile
 
 __device__ __forceinline__ double wsum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -64,6 +65,7 @@ struct TeamCtx {
     BTeamCtl* ctl;   // null for a workgroup alone
     int* gerr;
     int rank, g, nfork;
+    uint64_t wait_ticks;   // the leader's thread 0: wall-clock ticks spent in joins
 };
 
 __device__ bool team_wait_ge(int* p, int target, int* err) {
@@ -103,9 +105,11 @@ __device__ void team_fork(TeamCtx& tc, int op, const double* in, double* out, co
 // thread before the workgroup proceeds.
 __device__ void team_join(TeamCtx& tc) {
     if (threadIdx.x == 0) {
+        const uint64_t t0 = wall_clock64();
         if (!team_wait_ge(&tc.ctl->done, tc.nfork * (tc.g - 1), &tc.ctl->err))
             __hip_atomic_store(tc.gerr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        tc.wait_ticks += wall_clock64() - t0;
     }
     __syncthreads();
 }
@@ -127,40 +131,66 @@ __device__ __forceinline__ int btype(int Nm, int k, int j) { return (j < Nm) ? 0
 // (dr below the pair type's cutoff, at the snapshot positions), bonds along each chain, the static
 // trap on the colloids (src/rhs.cpp:160-260, src/pair_interaction.h:13-72).
 __device__ void wg_forces(const BConst& B, const BRep& R, const double* xpd, const double* x, const double* xpos0,
-                          double ktrap, double* rhs) {
-    const int Np = R.Np, Nm = R.Nm;
-    for (int64_t i = threadIdx.x; i < R.ns; i += BW)
+                          double ktrap, double* rhs, double* tile) {
+    const int Np = R.Np, Nm = R.Nm, t = threadIdx.x;
+    for (int64_t i = t; i < R.ns; i += BW)
         if (i < R.n || i >= R.n + 3 * (int64_t)Np) rhs[i] = 0.0;
-    for (int p = threadIdx.x; p < Np; p += BW) {
+    for (int p0 = 0; p0 < Np; p0 += BW) {
+        const int p = p0 + t;
+        const bool act = p < Np;
         double fx = 0.0, fy = 0.0, fz = 0.0;
-        for (int q = 0; q < Np; q++) {
-            if (q == p) continue;
-            const int k = min(p, q), j = max(p, q);
-            const int type = btype(Nm, k, j);
-            if (type == 0 && !B.mono_ev) continue;
-            double e[3], dr, dr_inv;
-            bgeom(xpd, k, j, e, dr, dr_inv);
-            if (!(dr < B.rverl[type])) continue;
-            const double sig_eff = B.sig[type] * (1.0 + 0.01);
-            double fmag;
-            if (B.force_sel == PS_FORCE_HS_EXP) {
-                const double ex_ = exp(-B.hs_exp_k * (dr - sig_eff));
-                fmag = ex_ / (1 - ex_);
-            } else {
-                fmag = (dr >= sig_eff) ? 0.0 : B.k_hs * (sig_eff - dr);
-            }
-            if (fmag == 0.0) continue;
-            const double gx = fmag * e[0], gy = fmag * e[1], gz = fmag * e[2];
-            if (p == k) {
-                fx -= gx;
-                fy -= gy;
-                fz -= gz;
-            } else {
-                fx += gx;
-                fy += gy;
-                fz += gz;
+        const double xp0 = act ? xpd[3 * p] : 0.0, xp1 = act ? xpd[3 * p + 1] : 0.0, xp2 = act ? xpd[3 * p + 2] : 0.0;
+        // without monomer excluded volume a monomer's partners are the colloids (same order)
+        const int qs = (!B.mono_ev && p < Nm) ? Nm : 0;
+        // per pair type (registers, not an indexed kernel argument)
+        const double rv0 = B.rverl[0], rv1 = B.rverl[1], rv2 = B.rverl[2];
+        const double sg0 = B.sig[0], sg1 = B.sig[1], sg2 = B.sig[2];
+        // partners q in increasing order, their positions staged in LDS tiles of FTILE
+        for (int q0 = 0; q0 < Np; q0 += FTILE) {
+            const int qn = min(FTILE, Np - q0);
+            __syncthreads();
+            for (int k = t; k < 3 * qn; k += BW) tile[k] = xpd[3 * (int64_t)q0 + k];
+            __syncthreads();
+            if (!act) continue;
+            for (int q = max(q0, qs); q < q0 + qn; q++) {
+                if (q == p) continue;
+                const int k = min(p, q), j = max(p, q);
+                const int type = btype(Nm, k, j);
+                if (type == 0 && !B.mono_ev) continue;
+                const double* xq = tile + 3 * (q - q0);
+                // d = x_k - x_j as the reference's pair_geom
+                const double sgn = (p == k) ? 1.0 : -1.0;
+                const double dx = sgn * (xp0 - xq[0]), dy = sgn * (xp1 - xq[1]), dz = sgn * (xp2 - xq[2]);
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                // far pairs: r^2 above rverl^2 (1 + 1e-12) has sqrt(r^2) > rverl, so the exact
+                // test below would reject it too
+                const double rv = type == 0 ? rv0 : type == 1 ? rv1 : rv2;
+                if (r2 > rv * rv * (1.0 + 1e-12)) continue;
+                const double dr = __builtin_sqrt(r2), dr_inv = 1 / dr;
+                const double e[3] = {dx * dr_inv, dy * dr_inv, dz * dr_inv};
+                if (!(dr < rv)) continue;
+                const double sig_eff = (type == 0 ? sg0 : type == 1 ? sg1 : sg2) * (1.0 + 0.01);
+                double fmag;
+                if (B.force_sel == PS_FORCE_HS_EXP) {
+                    const double ex_ = exp(-B.hs_exp_k * (dr - sig_eff));
+                    fmag = ex_ / (1 - ex_);
+                } else {
+                    fmag = (dr >= sig_eff) ? 0.0 : B.k_hs * (sig_eff - dr);
+                }
+                if (fmag == 0.0) continue;
+                const double gx = fmag * e[0], gy = fmag * e[1], gz = fmag * e[2];
+                if (p == k) {
+                    fx -= gx;
+                    fy -= gy;
+                    fz -= gz;
+                } else {
+                    fx += gx;
+                    fy += gy;
+                    fz += gz;
+                }
             }
         }
+        if (!act) continue;
         if (p < Nm) {   // bonds (bond_forces, src/rhs.cpp:102-158): p - 1 and p + 1 in the chain
             const int c0 = (p / R.Nmono) * R.Nmono;
             for (int side = 0; side < 2; side++) {
@@ -501,12 +531,29 @@ __device__ __forceinline__ void apply_slice(const BConst& B, const BRep& R, doub
     double* part = pool + R.off[BO_PART];
     if (!srows && Nm > 0 && sel == 0) aa_pairs<T>(B, R, xm, f, part, stage, rank, g);
     if (!srows && Nm > 0 && sel == 1) {
-        // the carried tile applied to the sum of colloid forces (the same in every monomer row)
+        // the carried tile applied to the sum of colloid forces (the same in every monomer row):
+        // a workgroup sum in a fixed order
         double F0 = 0.0, F1 = 0.0, F2 = 0.0;
-        for (int c = 0; c < Nc; c++) {
+        for (int c = t; c < Nc; c += BW) {
             F0 += f[nm3 + 3 * c];
             F1 += f[nm3 + 3 * c + 1];
             F2 += f[nm3 + 3 * c + 2];
+        }
+        F0 = wsum(F0);
+        F1 = wsum(F1);
+        F2 = wsum(F2);
+        __syncthreads();
+        if (lane == 0) {
+            stage[wv] = F0;
+            stage[NW + wv] = F1;
+            stage[2 * NW + wv] = F2;
+        }
+        __syncthreads();
+        F0 = F1 = F2 = 0.0;
+        for (int w = 0; w < NW; w++) {
+            F0 += stage[w];
+            F1 += stage[NW + w];
+            F2 += stage[2 * NW + w];
         }
         const double car0 = st[0] * F0 + st[1] * F1 + st[2] * F2;
         const double car1 = st[3] * F0 + st[4] * F1 + st[5] * F2;
@@ -540,24 +587,24 @@ __device__ __forceinline__ void apply_slice(const BConst& B, const BRep& R, doub
                 for (int cc = wv; cc < cn; cc += NW) {
                     const T* cv = cs + 11 * cc;
                     const T dx = xi - cv[0], dy = yi - cv[1], dz = zi - cv[2];
-                    const T r2 = dx * dx + dy * dy + dz * dz;
+                    const T r2 = b_fma(dz, dz, b_fma(dy, dy, dx * dx));
                     const T rinv = ps_rsqrt(r2);
                     const T ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
                     const T rinv2 = rinv * rinv, rinv4 = rinv2 * rinv2;
                     const T y12b = -(T)C3D4 * rinv2;
-                    const T x12g = (T)C9D4 * rinv2 - gA * rinv4;
+                    const T x12g = b_fma(-gA, rinv4, (T)C9D4 * rinv2);
                     const T y12g = gB * rinv4;
                     const T Tx = cv[3], Ty = cv[4], Tz = cv[5];
                     const T S0 = cv[6], S1 = cv[7], S2 = cv[8], S3 = cv[9], S4 = cv[10];
-                    const T sex = S0 * ex + S1 * ey + S2 * ez;
-                    const T sey = S1 * ex + S4 * ey + S3 * ez;
-                    const T sez = S2 * ex + S3 * ey - (S0 + S4) * ez;
-                    const T ese = ex * sex + ey * sey + ez * sez;
-                    const T a = -x12g * ese - (T)2 * y12g * ese;
+                    const T sex = b_fma(S2, ez, b_fma(S1, ey, S0 * ex));
+                    const T sey = b_fma(S3, ez, b_fma(S4, ey, S1 * ex));
+                    const T sez = b_fma(-(S0 + S4), ez, b_fma(S3, ey, S2 * ex));
+                    const T ese = b_fma(ez, sez, b_fma(ey, sey, ex * sex));
+                    const T a = -b_fma((T)2, y12g, x12g) * ese;
                     const T b2 = (T)2 * y12g;
-                    vx += y12b * (ey * Tz - ez * Ty) + a * ex + b2 * sex;
-                    vy += y12b * (ez * Tx - ex * Tz) + a * ey + b2 * sey;
-                    vz += y12b * (ex * Ty - ey * Tx) + a * ez + b2 * sez;
+                    vx = b_fma(b2, sex, b_fma(a, ex, b_fma(y12b, b_fma(ey, Tz, -(ez * Ty)), vx)));
+                    vy = b_fma(b2, sey, b_fma(a, ey, b_fma(y12b, b_fma(ez, Tx, -(ex * Tz)), vy)));
+                    vz = b_fma(b2, sez, b_fma(a, ez, b_fma(y12b, b_fma(ex, Ty, -(ey * Tx)), vz)));
                 }
             }
             // wave w adds the monomer partial sums of partner blocks w, w + 16, ...; then the 16
@@ -620,32 +667,33 @@ __device__ __forceinline__ void apply_slice(const BConst& B, const BRep& R, doub
             if (!cact) continue;
             for (int mm = lane; mm < mn; mm += 64) {
                 const T dx = sx[mm] - xc, dy = sx[BW + mm] - yc, dz = sx[2 * BW + mm] - zc;
-                const T r2 = dx * dx + dy * dy + dz * dz;
+                const T r2 = b_fma(dz, dz, b_fma(dy, dy, dx * dx));
                 const T rinv = ps_rsqrt(r2);
                 const T ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
                 const T rinv2 = rinv * rinv, rinv4 = rinv2 * rinv2;
                 const T y12b = -(T)C3D4 * rinv2;
-                const T x12g = (T)C9D4 * rinv2 - gA * rinv4;
-                const T y12g = gB * rinv4;
+                const T x12g = b_fma(-gA, rinv4, (T)C9D4 * rinv2);
+                const T y12g = gB * rinv4, y2 = (T)2 * y12g;
                 const T fx = sx[3 * BW + mm], fy = sx[4 * BW + mm], fz = sx[5 * BW + mm];
                 fsx += fx;
                 fsy += fy;
                 fsz += fz;
-                acc[3] += y12b * (fy * ez - fz * ey);
-                acc[4] += y12b * (fz * ex - fx * ez);
-                acc[5] += y12b * (fx * ey - fy * ex);
-                const T ef = ex * fx + ey * fy + ez * fz;
-                const T p = -x12g * ef - (T)2 * y12g * ef;
-                const T G00 = p * ex * ex + (T)2 * y12g * ex * fx;
-                const T G11 = p * ey * ey + (T)2 * y12g * ey * fy;
-                const T G22 = p * ez * ez + (T)2 * y12g * ez * fz;
-                const T G01 = p * ex * ey + y12g * (ex * fy + ey * fx);
-                const T G02 = p * ex * ez + y12g * (ex * fz + ez * fx);
-                const T G12 = p * ey * ez + y12g * (ey * fz + ez * fy);
+                acc[3] = b_fma(y12b, b_fma(fy, ez, -(fz * ey)), acc[3]);
+                acc[4] = b_fma(y12b, b_fma(fz, ex, -(fx * ez)), acc[4]);
+                acc[5] = b_fma(y12b, b_fma(fx, ey, -(fy * ex)), acc[5]);
+                const T ef = b_fma(ez, fz, b_fma(ey, fy, ex * fx));
+                const T p = -b_fma((T)2, y12g, x12g) * ef;
+                const T pex = p * ex, pey = p * ey;
+                const T G00 = b_fma(pex, ex, y2 * (ex * fx));
+                const T G11 = b_fma(pey, ey, y2 * (ey * fy));
+                const T G22 = b_fma(p * ez, ez, y2 * (ez * fz));
+                const T G01 = b_fma(pex, ey, y12g * b_fma(ex, fy, ey * fx));
+                const T G02 = b_fma(pex, ez, y12g * b_fma(ex, fz, ez * fx));
+                const T G12 = b_fma(pey, ez, y12g * b_fma(ey, fz, ez * fy));
                 acc[6] += G00 - G22;
-                acc[7] += (T)2 * G01;
-                acc[8] += (T)2 * G02;
-                acc[9] += (T)2 * G12;
+                acc[7] = b_fma((T)2, G01, acc[7]);
+                acc[8] = b_fma((T)2, G02, acc[8]);
+                acc[9] = b_fma((T)2, G12, acc[9]);
                 acc[10] += G11 - G22;
             }
         }
@@ -782,7 +830,7 @@ __device__ void wg_mob(const BConst& B, const BRep& R, double* P, const double* 
             (i == j) ? 9. / 5. : ((i == 0 && j == 4) || (i == 4 && j == 0)) ? 9. / 10. : 0.0;
     }
     const int64_t npair = (int64_t)Nc * (Nc - 1) / 2;
-    TeamCtx solo{nullptr, tc.gerr, 0, 1, 0};
+    TeamCtx solo{nullptr, tc.gerr, 0, 1, 0, 0};
     team_op(B, R, P, npair >= 2 * BW ? tc : solo, TOP_MOB, nullptr, nullptr, nullptr, 0);
 }
 
@@ -801,7 +849,7 @@ __device__ __forceinline__ void self_inv5(const double* v, double* out) {
 }
 __device__ int wg_minres_ss(const BConst& B, const BRep& R, double* P, TeamCtx& tc, int64_t n5, const double* Mss,
                             const double* rhs, double* x, double* w, double* red) {
-    TeamCtx solo{nullptr, tc.gerr, 0, 1, 0};
+    TeamCtx solo{nullptr, tc.gerr, 0, 1, 0, 0};
     TeamCtx& gtc = n5 >= 512 ? tc : solo;   // the GEMV splits over the team when it is large
     double *r1 = w, *r2 = w + n5, *y = w + 2 * n5, *v = w + 3 * n5, *ww = w + 4 * n5, *w1 = w + 5 * n5,
            *w2 = w + 6 * n5;
@@ -893,10 +941,16 @@ __device__ int wg_block_solve(const BConst& B, const BRep& R, double* pool, cons
 }
 
 // ---------------------------------------------------------------------------------- square root
-// tql2 (implicit QL on a symmetric tridiagonal, eigenvectors in Z, column-major m x m): thread 0.
-__device__ bool b_tql2(int m, double* d, double* e, double* Z) {
-    for (int i = 0; i < m * m; i++) Z[i] = 0.0;
-    for (int i = 0; i < m; i++) Z[i + i * m] = 1.0;
+// tql2 (implicit QL on a symmetric tridiagonal, eigenvectors in Z, column-major m x m) on one wave:
+// every lane runs the same scalar recurrence on d and e (LDS; all lanes store identical values),
+// and lane k applies each plane rotation to row k of Z.
+__device__ bool b_tql2_wave(int m, double* d, double* e, double* Z) {
+    const int lane = threadIdx.x & 63;
+    auto wsync = []() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); };
+    for (int k = lane; k < m * m; k += 64) Z[k] = 0.0;
+    wsync();
+    if (lane < m) Z[lane + lane * m] = 1.0;
+    wsync();
     for (int l = 0; l < m; l++) {
         int iter = 0, mm;
         do {
@@ -914,11 +968,13 @@ __device__ bool b_tql2(int m, double* d, double* e, double* Z) {
                 for (int i = mm - 1; i >= l; i--) {
                     const double f = s * e[i], bb = c * e[i];
                     r = hypot(f, g);
+                    wsync();
                     e[i + 1] = r;
                     if (r == 0.0) {
                         d[i + 1] -= p;
                         e[mm] = 0.0;
                         underflow = true;
+                        wsync();
                         break;
                     }
                     s = f / r;
@@ -926,18 +982,22 @@ __device__ bool b_tql2(int m, double* d, double* e, double* Z) {
                     g = d[i + 1] - p;
                     r = (d[i] - g) * s + 2.0 * c * bb;
                     p = s * r;
+                    wsync();
                     d[i + 1] = g + p;
                     g = c * r - bb;
-                    for (int k = 0; k < m; k++) {
-                        const double zf = Z[k + (i + 1) * m];
-                        Z[k + (i + 1) * m] = s * Z[k + i * m] + c * zf;
-                        Z[k + i * m] = c * Z[k + i * m] - s * zf;
+                    if (lane < m) {
+                        const double zf = Z[lane + (i + 1) * m];
+                        Z[lane + (i + 1) * m] = s * Z[lane + i * m] + c * zf;
+                        Z[lane + i * m] = c * Z[lane + i * m] - s * zf;
                     }
+                    wsync();
                 }
                 if (underflow) continue;
+                wsync();
                 d[l] -= p;
                 e[l] = g;
                 e[mm] = 0.0;
+                wsync();
             }
         } while (mm != l);
     }
@@ -985,15 +1045,31 @@ __device__ int wg_sqrt(const BConst& B, const BRep& R, double* pool, const doubl
         for (int64_t i = t; i < n; i += BW) w[i] -= aj * qj[i];
         __syncthreads();
         for (int pass = 0; pass < 2; pass++) {
-            // h_i = Q_i . w for i <= j: wave wv takes i = wv, wv + NW, ...
-            for (int i = wv; i <= j; i += NW) {
-                const double* qi = Q + (int64_t)i * n;
-                double s = 0.0;
-                for (int64_t k = lane; k < n; k += 64) s += qi[k] * w[k];
-                s = wsum(s);
-                if (lane == 0) h[i] = s;
+            // h_i = Q_i . w for i <= j, 8 basis vectors at a time over every thread's elements; the
+            // per-wave sums meet in Z (free until the tridiagonal solve) and add in wave order
+            for (int i0 = 0; i0 <= j; i0 += 8) {
+                const int ni = min(8, j + 1 - i0);
+                double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                const double* q0 = Q + (int64_t)i0 * n;
+                for (int64_t k = t; k < n; k += BW) {
+                    const double wk = w[k];
+#pragma unroll
+                    for (int qq = 0; qq < 8; qq++)
+                        if (qq < ni) acc[qq] += q0[(int64_t)qq * n + k] * wk;
+                }
+#pragma unroll
+                for (int qq = 0; qq < 8; qq++) {
+                    const double v = wsum(acc[qq]);
+                    if (lane == 0) Z[wv * 8 + qq] = v;
+                }
+                __syncthreads();
+                if (t < ni) {
+                    double sum = 0.0;
+                    for (int ww = 0; ww < NW; ww++) sum += Z[ww * 8 + t];
+                    h[i0 + t] = sum;
+                }
+                __syncthreads();
             }
-            __syncthreads();
             for (int64_t k = t; k < n; k += BW) {
                 double acc = w[k];
                 for (int i = 0; i <= j; i++) acc -= h[i] * Q[(int64_t)i * n + k];
@@ -1009,25 +1085,32 @@ __device__ int wg_sqrt(const BConst& B, const BRep& R, double* pool, const doubl
         for (int64_t i = t; i < n; i += BW) qn[i] = w[i] * (1.0 / bj);
         __syncthreads();
     }
-    if (t == 0) {
-        // g = beta0 U diag(sqrt(max(theta, 0))) U^T e1 (src/slip_vel.cpp:427-459)
-        double dd[BATCH_KMAX], ee[BATCH_KMAX];
-        for (int i = 0; i < m; i++) dd[i] = alpha[i];
-        for (int i = 0; i < m; i++) ee[i] = (i < m - 1) ? betad[i] : 0.0;
-        b_tql2(m, dd, ee, Z);
-        int nneg = 0;
-        for (int i = 0; i < m; i++) {
-            double gi = 0.0;
+    __syncthreads();
+    if (wv == 0) {
+        // g = beta0 U diag(sqrt(max(theta, 0))) U^T e1 (src/slip_vel.cpp:427-459); the QL works on
+        // d (in h) and e (in g; dead before g is written)
+        double* dd = h;
+        double* ee = g;
+        for (int i = lane; i < m; i += 64) {
+            dd[i] = alpha[i];
+            ee[i] = (i < m - 1) ? betad[i] : 0.0;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        b_tql2_wave(m, dd, ee, Z);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        double gi = 0.0;
+        if (lane < m)
             for (int jj = 0; jj < m; jj++) {
                 double th = dd[jj];
                 if (th < 0.0) th = 0.0;
-                gi += Z[i + jj * m] * sqrt(th) * Z[0 + jj * m];
+                gi += Z[lane + jj * m] * sqrt(th) * Z[0 + jj * m];
             }
-            g[i] = beta0 * gi;
-        }
+        int nneg = 0;
         for (int jj = 0; jj < m; jj++)
             if (dd[jj] < 0.0) nneg++;
-        if (nneg_out) *nneg_out = nneg;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (lane < m) g[lane] = beta0 * gi;
+        if (lane == 0 && nneg_out) *nneg_out = nneg;
     }
     __syncthreads();
     for (int64_t k = t; k < n; k += BW) {
@@ -1092,6 +1175,15 @@ __device__ __forceinline__ void step_replica(const BStepArgs& a, int r, TeamCtx&
     const int t = threadIdx.x;
     const double dt = B.dt, kT = B.kT, eps = 1.0e-3;
     int its = 0, unconverged = 0;
+    // wall-clock ticks per phase (thread 0): forces, mob(), saddle solves, square root
+    uint64_t ph_last = t_start, ph[4] = {0, 0, 0, 0};
+    auto mark = [&](int k) {
+        if (t == 0) {
+            const uint64_t now = wall_clock64();
+            ph[k] += now - ph_last;
+            ph_last = now;
+        }
+    };
     auto snapshot = [&]() {
         for (int64_t i = t; i < n3; i += BW) xpd[i] = x[i];
         __syncthreads();
@@ -1100,18 +1192,22 @@ __device__ __forceinline__ void step_replica(const BStepArgs& a, int r, TeamCtx&
         const int k = wg_block_solve(B, R, P, b, Xo, uo, red, tc);
         its += k;
         unconverged += k >= B.ms_maxit;
+        mark(2);
     };
     for (int64_t i = t; i < n3; i += BW) xn[i] = x[i];
     // ---- predictor
     snapshot();
-    wg_forces(B, R, xpd, x, xpos0, a.ktrap, rhs);
+    wg_forces(B, R, xpd, x, xpos0, a.ktrap, rhs, lz);
     __syncthreads();
+    mark(0);
     wg_mob(B, R, P, xpd, tc);
+    mark(1);
     solve(rhs, Xdet, vdn);
     int lm = 0;
     if (kT > 0) {
         lm = B.mixed ? wg_sqrt<float>(B, R, P, noise, W, red, lz, &nneg_sh, tc)
                      : wg_sqrt<double>(B, R, P, noise, W, red, lz, &nneg_sh, tc);
+        mark(3);
         const double sc = sqrt(2 * kT / dt);
         for (int64_t i = t; i < n; i += BW) rhs[i] += sc * W[i];
         __syncthreads();
@@ -1123,6 +1219,7 @@ __device__ __forceinline__ void step_replica(const BStepArgs& a, int r, TeamCtx&
         __syncthreads();
         snapshot();
         wg_mob(B, R, P, xpd, tc);
+        mark(1);
         for (int64_t i = t; i < R.ns; i += BW) rhs[i] = (i < n) ? 0.0 : fext[i - n];
         __syncthreads();
         solve(rhs, Xd, dv);
@@ -1131,6 +1228,7 @@ __device__ __forceinline__ void step_replica(const BStepArgs& a, int r, TeamCtx&
         __syncthreads();
         snapshot();
         wg_mob(B, R, P, xpd, tc);
+        mark(1);
         for (int64_t i = t; i < R.ns; i += BW) rhs[i] = (i < n) ? 0.0 : fext[i - n];
         __syncthreads();
         solve(rhs, Xd, ud);
@@ -1154,9 +1252,11 @@ __device__ __forceinline__ void step_replica(const BStepArgs& a, int r, TeamCtx&
     __syncthreads();
     // ---- corrector
     snapshot();
-    wg_forces(B, R, xpd, x, xpos0, a.ktrap, rhs);
+    wg_forces(B, R, xpd, x, xpos0, a.ktrap, rhs, lz);
     __syncthreads();
+    mark(0);
     wg_mob(B, R, P, xpd, tc);
+    mark(1);
     solve(rhs, Xdet, vdc);
     for (int64_t i = t; i < nu; i += BW) up[i] = 0.5 * (vdn[i] + vdc[i]) + vb[i];
     __syncthreads();
@@ -1182,6 +1282,8 @@ __device__ __forceinline__ void step_replica(const BStepArgs& a, int r, TeamCtx&
         a.stats[4 * r + 2] = kT > 0 ? nneg_sh : 0;
         a.stats[4 * r + 3] = unconverged;
         a.sd.times[r] = (double)(wall_clock64() - t_start);
+        for (int k = 0; k < 4; k++) a.sd.phases[5 * r + k] = (double)ph[k];
+        a.sd.phases[5 * r + 4] = (double)tc.wait_ticks;
     }
 }
 
@@ -1194,7 +1296,7 @@ __global__ __launch_bounds__(BW) void k_batch_step(BStepArgs a) {
     const int* wi = a.sd.sched + 5 * blockIdx.x;
     const int team = wi[0];
     for (int k = team >= 0 ? -1 : wi[3]; k < wi[4]; k = (k < 0) ? wi[3] : k + 1) {
-        TeamCtx tc{nullptr, a.sd.err, 0, 1, 0};
+        TeamCtx tc{nullptr, a.sd.err, 0, 1, 0, 0};
         int r;
         if (k < 0) {
             tc.ctl = a.sd.ctl + team;
@@ -1230,7 +1332,7 @@ __global__ __launch_bounds__(BW) void k_batch_op(BConst B, const BRep* reps, dou
     const double* inr = in + io_off[r];
     double* outr = out + io_off[r];
     double *x = P + R.off[BO_X], *xpd = P + R.off[BO_XPD];
-    TeamCtx tc{nullptr, err, 0, 1, 0};
+    TeamCtx tc{nullptr, err, 0, 1, 0, 0};
     int s0 = 0;
     if (op == 0) {
         wg_apply<double>(B, R, P, inr, outr, false, tc);
@@ -1247,7 +1349,7 @@ __global__ __launch_bounds__(BW) void k_batch_op(BConst B, const BRep* reps, dou
     } else if (op == 5) {
         for (int64_t i = threadIdx.x; i < 3 * (int64_t)R.Np; i += BW) xpd[i] = x[i];
         __syncthreads();
-        wg_forces(B, R, xpd, x, P + R.off[BO_XP0], B.ktrap, outr);
+        wg_forces(B, R, xpd, x, P + R.off[BO_XP0], B.ktrap, outr, lz);
     }
     if (threadIdx.x == 0 && stats) stats[r] = s0;
 }
@@ -1318,6 +1420,7 @@ struct Batch {
     BTeamCtl* d_ctl = nullptr;
     int* d_err = nullptr;
     double* d_times = nullptr;
+    double* d_phases = nullptr;
     int64_t steps_run = 0;
     std::thread prefetch;        // draws the next run's first step of noise into slot 0
     double* pool = nullptr;
@@ -1365,7 +1468,7 @@ void batch_free(Batch* b) {
     if (b->prefetch.joinable()) b->prefetch.join();
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (void* q : {(void*)b->d_reps, (void*)b->d_stats, (void*)b->pool, (void*)b->noise_d, (void*)b->d_sched,
-                    (void*)b->d_ctl, (void*)b->d_err, (void*)b->d_times})
+                    (void*)b->d_ctl, (void*)b->d_err, (void*)b->d_times, (void*)b->d_phases})
         if (q) (void)hipFree(q);
     for (double* h : b->noise_h)
         if (h) (void)hipHostFree(h);
@@ -1619,6 +1722,8 @@ ps_err ps_batch_create(const ps_params* p, int R, const int* Nc, const int* Npol
     if ((e = hipMemset(b->d_err, 0, sizeof(int))) != hipSuccess) return fail_hip(e);
     if ((e = hipMalloc(&b->d_times, sizeof(double) * R)) != hipSuccess) return fail_hip(e);
     if ((e = hipMemset(b->d_times, 0, sizeof(double) * R)) != hipSuccess) return fail_hip(e);
+    if ((e = hipMalloc(&b->d_phases, sizeof(double) * 5 * (size_t)R)) != hipSuccess) return fail_hip(e);
+    if ((e = hipMemset(b->d_phases, 0, sizeof(double) * 5 * (size_t)R)) != hipSuccess) return fail_hip(e);
     if ((e = hipMalloc(&b->d_stats, sizeof(int) * 4 * R)) != hipSuccess) return fail_hip(e);
     if ((e = hipMemcpy(b->d_reps, b->reps.data(), sizeof(BRep) * R, hipMemcpyHostToDevice)) != hipSuccess)
         return fail_hip(e);
@@ -1822,7 +1927,7 @@ ps_err ps_batch_run(void* h, int nsteps) {
                 break;
             }
         }
-        const BSchedDev sd{b->d_sched, b->team_off, b->solo_off, b->d_ctl, b->d_err, b->d_times};
+        const BSchedDev sd{b->d_sched, b->team_off, b->solo_off, b->d_ctl, b->d_err, b->d_times, b->d_phases};
         rc = batch_launch_step(b->stream, b->B, b->d_reps, sd, b->nwg, b->nteams > 0, b->pool, b->noise_d,
                                b->ktrap_cur, b->d_stats);
         if (rc != PS_OK) b->set_error("batched step: kernel launch failed");
@@ -1866,6 +1971,18 @@ ps_err ps_batch_times(void* h, double* us, int* team) {
     }
     if (team)
         for (int r = 0; r < b->R; r++) team[r] = b->team_of.empty() ? 1 : b->team_of[r];
+    return PS_OK;
+}
+
+// The last step's time of each replica per phase, in microseconds (R x 5: forces, mob(), saddle
+// solves, square root -- vector work between phases counts toward the next phase -- and, within
+// those, the team leader's time waiting in joins).
+ps_err ps_batch_phase_times(void* h, double* us5) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || !us5) return PS_ERR_ARG;
+    BPS_HIP(b, hipStreamSynchronize(b->stream));
+    BPS_HIP(b, hipMemcpy(us5, b->d_phases, sizeof(double) * 5 * (size_t)b->R, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 5 * b->R; k++) us5[k] *= b->tick_us;
     return PS_OK;
 }
 

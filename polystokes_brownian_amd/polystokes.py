@@ -90,6 +90,7 @@ def lib():
         L.ps_batch_op.argtypes = [vp, ip, dp, dp, C.POINTER(ip)]
         L.ps_batch_times.argtypes = [vp, dp, C.POINTER(ip)]
         L.ps_batch_config.argtypes = [vp, ip, ip, ip]
+        L.ps_batch_phase_times.argtypes = [vp, dp]
         _lib = L
     return _lib
 
@@ -491,6 +492,13 @@ class Batch:
         team = (C.c_int * self.R)()
         self._check(lib().ps_batch_times(self.h, _p(us), team))
         return us, np.array(list(team), dtype=np.int64)
+
+    def phase_times(self):
+        """ps_batch_phase_times: (R, 5) microseconds in forces, mob(), saddle solves, square root,
+        and the team leader's time waiting in joins."""
+        out = np.zeros(5 * self.R)
+        self._check(lib().ps_batch_phase_times(self.h, _p(out)))
+        return out.reshape(self.R, 5)
 
     def config(self, team_max=16, min_blocks=2, autotune=True):
         """ps_batch_config: the step schedule's team limits and autotuning."""
