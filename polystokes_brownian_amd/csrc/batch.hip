@@ -29,7 +29,7 @@ namespace {
 constexpr int BW = BATCH_THREADS;   // threads per replica workgroup
 constexpr int NW = BW / 64;
 constexpr int BCCH = 256;   // colloids staged in LDS per pass of the monomer rows
-constexpr int FTILE = (4 * BATCH_KMAX + BATCH_KMAX * BATCH_KMAX) / 3;   // force partners per LDS tile
+constexpr int FTILE = 1024;   // force partners per LDS tile
 
 __device__ __forceinline__ double wsum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -130,8 +130,13 @@ __device__ __forceinline__ int btype(int Nm, int k, int j) { return (j < Nm) ? 0
 // rhs = [0 (n) ; fint (3Np) ; 0 (3Nc)]: excluded volume over the pairs a Verlet list would hold
 // (dr below the pair type's cutoff, at the snapshot positions), bonds along each chain, the static
 // trap on the colloids (src/rhs.cpp:160-260, src/pair_interaction.h:13-72).
-__device__ void wg_forces(const BConst& B, const BRep& R, const double* xpd, const double* x, const double* xpos0,
-                          double ktrap, double* rhs, double* tile) {
+// force partners' positions, one LDS tile (the force pass is not inlined into the step, so its
+// loops get registers of their own; the tile is referenced directly so it stays LDS)
+__shared__ double g_ftile[3 * FTILE];
+
+__device__ __noinline__ void wg_forces(const BConst& B, const BRep& R, const double* xpd, const double* x,
+                                       const double* xpos0, double ktrap, double* rhs) {
+    double* tile = g_ftile;
     const int Np = R.Np, Nm = R.Nm, t = threadIdx.x;
     for (int64_t i = t; i < R.ns; i += BW)
         if (i < R.n || i >= R.n + 3 * (int64_t)Np) rhs[i] = 0.0;
@@ -1197,7 +1202,7 @@ __device__ __forceinline__ void step_replica(const BStepArgs& a, int r, TeamCtx&
     for (int64_t i = t; i < n3; i += BW) xn[i] = x[i];
     // ---- predictor
     snapshot();
-    wg_forces(B, R, xpd, x, xpos0, a.ktrap, rhs, lz);
+    wg_forces(B, R, xpd, x, xpos0, a.ktrap, rhs);
     __syncthreads();
     mark(0);
     wg_mob(B, R, P, xpd, tc);
@@ -1252,7 +1257,7 @@ __device__ __forceinline__ void step_replica(const BStepArgs& a, int r, TeamCtx&
     __syncthreads();
     // ---- corrector
     snapshot();
-    wg_forces(B, R, xpd, x, xpos0, a.ktrap, rhs, lz);
+    wg_forces(B, R, xpd, x, xpos0, a.ktrap, rhs);
     __syncthreads();
     mark(0);
     wg_mob(B, R, P, xpd, tc);
@@ -1349,7 +1354,7 @@ __global__ __launch_bounds__(BW) void k_batch_op(BConst B, const BRep* reps, dou
     } else if (op == 5) {
         for (int64_t i = threadIdx.x; i < 3 * (int64_t)R.Np; i += BW) xpd[i] = x[i];
         __syncthreads();
-        wg_forces(B, R, xpd, x, P + R.off[BO_XP0], B.ktrap, outr, lz);
+        wg_forces(B, R, xpd, x, P + R.off[BO_XP0], B.ktrap, outr);
     }
     if (threadIdx.x == 0 && stats) stats[r] = s0;
 }
