@@ -20,7 +20,6 @@ enum BatchOff {
     BO_QB,                              // Lanczos basis (kmax n)
     BO_SB,                              // Brownian stresslet (5Nc)
     BO_M5,                              // stresslet MINRES work (9 x 5Nc)
-    BO_MSS,                             // dense stresslet block (25 Nc^2)
     BO_PART,                            // monomer-monomer partial sums per partner block (3 nblk Nm)
     BO_ST,                              // carried tile (18)
     BO_NZ,                              // this step's noise (n + nu), in the noise buffer

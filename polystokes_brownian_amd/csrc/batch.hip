@@ -58,7 +58,7 @@ __device__ double wg_dot(const double* a, const double* b, int64_t n, double* re
 // kernel is a cooperative launch, so all workgroups are co-resident and a member always reaches
 // its next wait.  Every wait is bounded (wall clock): a timeout sets the team's err, which ends
 // every later wait of the team at once, and the batch's error flag, which the host reports.
-enum TeamOp { TOP_EXIT = 1, TOP_APPLY_D, TOP_APPLY_DS, TOP_APPLY_F, TOP_GEMV, TOP_MOB, TOP_ROWS_D, TOP_ROWS_F };
+enum TeamOp { TOP_EXIT = 1, TOP_APPLY_D, TOP_APPLY_DS, TOP_APPLY_F, TOP_MSS, TOP_ROWS_D, TOP_ROWS_F };
 constexpr uint64_t TEAM_WAIT_TICKS = 1000000000ull;   // 10 s of the 100 MHz wall clock
 
 struct TeamCtx {
@@ -340,38 +340,6 @@ __device__ __forceinline__ void cc_couple(const double e[3], double ri, double s
     Sr[3] += ms * (2.0 * (p * e[1] * e[2] + y12g * (e[1] * F[2] + e[2] * F[1]))) + h * (e[1] * w2 + e[2] * w1) + m[3];
     Sr[4] += ms * (p * (e[1] * e[1] - e[2] * e[2]) + 2.0 * y12g * (e[1] * F[1] - e[2] * F[2])) +
              h * (e[1] * w1 - e[2] * w2) + m[4];
-}
-
-// mob()'s stresslet block M_ss (5Nc x 5Nc: self blocks and the pairs' m tiles, placement of
-// src/mob.cpp:1129-1143), column q of a pair's m tile being m applied to the unit stresslet q.
-// This slice: unordered pairs p = rank BW + t, rank BW + t + g BW, ... (row-major over c1 < c2).
-__device__ void mob_slice(const BRep& R, double* P, int rank, int g) {
-    const int Nc = R.Nc, Nm = R.Nm;
-    const int64_t n5 = 5 * (int64_t)Nc;
-    const double* xm = P + R.off[BO_XM];
-    double* Mss = P + R.off[BO_MSS];
-    const int64_t npair = (int64_t)Nc * (Nc - 1) / 2;
-    auto first = [Nc](int64_t c) { return c * (2 * (int64_t)Nc - c - 1) / 2; };   // pairs before row c
-    for (int64_t p = (int64_t)rank * BW + threadIdx.x; p < npair; p += (int64_t)g * BW) {
-        const double m2 = 2.0 * Nc - 1.0;
-        int64_t c1 = (int64_t)((m2 - __builtin_sqrt(fmax(m2 * m2 - 8.0 * (double)p, 0.0))) * 0.5);
-        if (c1 < 0) c1 = 0;
-        while (c1 > 0 && first(c1) > p) c1--;
-        while (first(c1 + 1) <= p) c1++;
-        const int64_t c2 = c1 + 1 + (p - first(c1));
-        double e[3], dr, dr_inv;
-        bgeom(xm, Nm + (int)c1, Nm + (int)c2, e, dr, dr_inv);
-        const double ri3 = dr_inv * dr_inv * dr_inv, ri5 = ri3 * dr_inv * dr_inv;
-        for (int q = 0; q < 5; q++) {
-            double Sq[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, m[5];
-            Sq[q] = 1.0;
-            cc_mS(e, ri3, ri5, Sq, m);
-            for (int j = 0; j < 5; j++) {
-                Mss[(5 * c1 + j) * n5 + 5 * c2 + q] = m[j];
-                Mss[(5 * c2 + j) * n5 + 5 * c1 + q] = m[j];
-            }
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------------- apply
@@ -759,15 +727,40 @@ __device__ __forceinline__ void apply_slice(const BConst& B, const BRep& R, doub
     }
 }
 
-// y = A v (n x n, row-major): rows rank 16 + w, rank 16 + w + 16 g, ... one wave per row.
-__device__ __forceinline__ void gemv_slice(const double* A, const double* v, double* y, int64_t n, int rank, int g) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int64_t row = (int64_t)rank * NW + wv; row < n; row += (int64_t)g * NW) {
-        const double* mr = A + row * n;
-        double acc = 0.0;
-        for (int64_t k = lane; k < n; k += 64) acc += mr[k] * v[k];
-        acc = wsum(acc);
-        if (lane == 0) y[row] = acc;
+// y = M_ss v without storing M_ss (5Nc x 5Nc: the stresslet self blocks and every colloid pair's
+// m tile, placement of src/mob.cpp:1129-1143): colloids c = rank 16 + w, + 16 g, ... one wave
+// per colloid, the source colloids over the lanes, m applied in contracted form (cc_mS, even in e).
+__device__ __forceinline__ void mss_slice(const BRep& R, const double* P, const double* v, double* y, int rank, int g) {
+    const int Nc = R.Nc, Nm = R.Nm, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const double* xm = P + R.off[BO_XM];
+    for (int c = rank * NW + wv; c < Nc; c += g * NW) {
+        const int gc = Nm + c;
+        const double xc0 = xm[3 * gc], xc1 = xm[3 * gc + 1], xc2 = xm[3 * gc + 2];
+        double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int s2 = lane; s2 < Nc; s2 += 64) {
+            if (s2 == c) continue;
+            const int gs = Nm + s2;
+            const double dx = xc0 - xm[3 * gs], dy = xc1 - xm[3 * gs + 1], dz = xc2 - xm[3 * gs + 2];
+            const double dr = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+            const double ri = 1 / dr;
+            const double e[3] = {dx * ri, dy * ri, dz * ri};
+            const double ri3 = ri * ri * ri, ri5 = ri3 * ri * ri;
+            const double* vs = v + 5 * (int64_t)s2;
+            const double Sv[5] = {vs[0], vs[1], vs[2], vs[3], vs[4]};
+            double m[5];
+            cc_mS(e, ri3, ri5, Sv, m);
+            for (int q = 0; q < 5; q++) acc[q] += m[q];
+        }
+        for (int q = 0; q < 5; q++) acc[q] = wsum(acc[q]);
+        if (lane == 0) {
+            const double* vc = v + 5 * (int64_t)c;
+            double* yc = y + 5 * (int64_t)c;
+            yc[0] = ((9. / 5.) * vc[0] + (9. / 10.) * vc[4]) + acc[0];
+            yc[1] = (9. / 5.) * vc[1] + acc[1];
+            yc[2] = (9. / 5.) * vc[2] + acc[2];
+            yc[3] = (9. / 5.) * vc[3] + acc[3];
+            yc[4] = ((9. / 5.) * vc[4] + (9. / 10.) * vc[0]) + acc[4];
+        }
     }
 }
 
@@ -787,10 +780,8 @@ __device__ __noinline__ void run_slice(const BConst& B, const BRep& R, double* P
         apply_slice<float>(B, R, P, in, out, false, 0, stage, cstage, rank, g);
     else if (op == TOP_ROWS_F)
         apply_slice<float>(B, R, P, in, out, false, 1, stage, cstage, rank, g);
-    else if (op == TOP_GEMV)
-        gemv_slice(mat, in, out, n, rank, g);
-    else if (op == TOP_MOB)
-        mob_slice(R, P, rank, g);
+    else if (op == TOP_MSS)
+        mss_slice(R, P, in, out, rank, g);
 }
 
 // One split operation on this workgroup (alone) or its team (as the leader).
@@ -817,26 +808,15 @@ __device__ void wg_apply(const BConst& B, const BRep& R, double* P, const double
     team_op(B, R, P, tc, sizeof(T) == 4 ? TOP_ROWS_F : TOP_ROWS_D, f, u, nullptr, 0);
 }
 
-// mob(): snapshot xm = xpd, the carried tile, the self blocks of M_ss, then the pairs' m tiles
-// (split over the team).
+// mob(): snapshot xm = xpd and the carried tile (every other block of the grand mobility is applied
+// from positions).
 __device__ void wg_mob(const BConst& B, const BRep& R, double* P, const double* xpd, TeamCtx& tc) {
     double* xm = P + R.off[BO_XM];
     double* st = P + R.off[BO_ST];
     for (int64_t i = threadIdx.x; i < 3 * (int64_t)R.Np; i += BW) xm[i] = xpd[i];
     __syncthreads();
     if (threadIdx.x == 0) b_stale(B, R, xm, st);
-    const int Nc = R.Nc;
-    const int64_t n5 = 5 * (int64_t)Nc;
-    double* Mss = P + R.off[BO_MSS];
-    for (int64_t idx = threadIdx.x; idx < 25 * (int64_t)Nc; idx += BW) {
-        const int64_t c = idx / 25;
-        const int i = (int)(idx % 25) / 5, j = (int)(idx % 5);
-        Mss[(5 * c + i) * n5 + 5 * c + j] =
-            (i == j) ? 9. / 5. : ((i == 0 && j == 4) || (i == 4 && j == 0)) ? 9. / 10. : 0.0;
-    }
-    const int64_t npair = (int64_t)Nc * (Nc - 1) / 2;
-    TeamCtx solo{nullptr, tc.gerr, 0, 1, 0, 0};
-    team_op(B, R, P, npair >= 2 * BW ? tc : solo, TOP_MOB, nullptr, nullptr, nullptr, 0);
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------------- solves
@@ -852,8 +832,8 @@ __device__ __forceinline__ void self_inv5(const double* v, double* out) {
     out[2] = v[2] / a;
     out[3] = v[3] / a;
 }
-__device__ int wg_minres_ss(const BConst& B, const BRep& R, double* P, TeamCtx& tc, int64_t n5, const double* Mss,
-                            const double* rhs, double* x, double* w, double* red) {
+__device__ int wg_minres_ss(const BConst& B, const BRep& R, double* P, TeamCtx& tc, int64_t n5, const double* rhs,
+                            double* x, double* w, double* red) {
     TeamCtx solo{nullptr, tc.gerr, 0, 1, 0, 0};
     TeamCtx& gtc = n5 >= 512 ? tc : solo;   // the GEMV splits over the team when it is large
     double *r1 = w, *r2 = w + n5, *y = w + 2 * n5, *v = w + 3 * n5, *ww = w + 4 * n5, *w1 = w + 5 * n5,
@@ -878,7 +858,7 @@ __device__ int wg_minres_ss(const BConst& B, const BRep& R, double* P, TeamCtx& 
         const double s = 1.0 / beta;
         for (int64_t i = t; i < n5; i += BW) v[i] = s * y[i];
         __syncthreads();
-        team_op(B, R, P, gtc, TOP_GEMV, v, y, Mss, n5);   // y = M_ss v
+        team_op(B, R, P, gtc, TOP_MSS, v, y, nullptr, n5);   // y = M_ss v
         if (itn >= 2)
             for (int64_t i = t; i < n5; i += BW) y[i] -= (beta / oldb) * r1[i];
         __syncthreads();
@@ -933,7 +913,7 @@ __device__ int wg_block_solve(const BConst& B, const BRep& R, double* pool, cons
         double* rs = w5 + 8 * n5;
         for (int64_t k = threadIdx.x; k < n5; k += BW) rs[k] = b[nu + k] - y[nu + k];
         __syncthreads();
-        its = wg_minres_ss(B, R, pool, tc, n5, pool + R.off[BO_MSS], rs, x + nu, w5, red);
+        its = wg_minres_ss(B, R, pool, tc, n5, rs, x + nu, w5, red);
     }
     wg_apply<double>(B, R, pool, x, y, false, tc);
     for (int64_t i = threadIdx.x; i < nu; i += BW) {
@@ -1671,7 +1651,7 @@ ps_err ps_batch_create(const ps_params* p, int R, const int* Nc, const int* Npol
         const int64_t n3 = 3 * (int64_t)Q.Np, n5 = 5 * (int64_t)Q.Nc;
         const int64_t sizes[BO_NOFF] = {n3, n3, n3, n3, 4 * (int64_t)Q.Nc, 3 * (int64_t)Q.Nc, Q.ns, Q.ns, Q.ns, Q.ns,
                                         Q.nu, Q.nu, Q.nu, Q.nu, Q.nu, Q.nu, Q.n, Q.n, Q.n, (int64_t)p->sqrt_kmax * Q.n,
-                                        n5, 9 * n5, n5 * n5, 3 * (int64_t)((Q.Nm + 63) / 64) * Q.Nm, 18, 0};
+                                        n5, 9 * n5, 3 * (int64_t)((Q.Nm + 63) / 64) * Q.Nm, 18, 0};
         for (int k = 0; k < BO_NOFF; k++) {
             if (k == BO_NZ) continue;
             Q.off[k] = pos;
