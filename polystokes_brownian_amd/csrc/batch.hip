@@ -397,8 +397,7 @@ __device__ __forceinline__ void aa_pairs(const BConst& B, const BRep& R, const d
     const bool chain = B.chain_HI != 0;
     const int64_t nitem = chain ? (int64_t)nblk * nblk : (int64_t)nblk * (nblk + 1) / 2;
     auto first = [nblk](int64_t I) { return I * nblk - I * (I - 1) / 2; };   // items before row I
-    for (int64_t it = (int64_t)rank * NW + wv; it < nitem; it += (int64_t)g * NW) {
-        int I, J;
+    auto decode = [&](int64_t it, int& I, int& J) {
         if (chain) {
             I = (int)(it / nblk);
             J = (int)(it - (int64_t)I * nblk);
@@ -411,21 +410,43 @@ __device__ __forceinline__ void aa_pairs(const BConst& B, const BRep& R, const d
             I = (int)r;
             J = (int)(I + (it - first(r)));
         }
+    };
+    // this lane's target and source beads of an item, loaded one item ahead (the loads of the next
+    // item are in flight while the current one runs)
+    double pt[6], ps[6];
+    auto load = [&](int I, int J) {
         const int i = (I << 6) + lane, j = (J << 6) + lane;
         const bool ai = i < Nm, aj = j < Nm;
+        for (int q = 0; q < 3; q++) {
+            pt[q] = ai ? xm[3 * (int64_t)i + q] : (double)FAR;
+            pt[3 + q] = ai ? f[3 * (int64_t)i + q] : 0.0;
+            ps[q] = aj ? xm[3 * (int64_t)j + q] : -(double)FAR;
+            ps[3 + q] = aj ? f[3 * (int64_t)j + q] : 0.0;
+        }
+    };
+    int64_t it = (int64_t)rank * NW + wv;
+    int In = 0, Jn = 0;
+    if (it < nitem) {
+        decode(it, In, Jn);
+        load(In, Jn);
+    }
+    for (; it < nitem; it += (int64_t)g * NW) {
+        const int I = In, J = Jn;
+        const int i = (I << 6) + lane;
+        const bool ai = i < Nm;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // the previous item's window reads
         {
             T* w = win + SW * lane;
-            w[0] = aj ? (T)xm[3 * j] : -FAR;
-            w[1] = aj ? (T)xm[3 * j + 1] : -FAR;
-            w[2] = aj ? (T)xm[3 * j + 2] : -FAR;
-            w[3] = aj ? (T)f[3 * j] : (T)0;
-            w[4] = aj ? (T)f[3 * j + 1] : (T)0;
-            w[5] = aj ? (T)f[3 * j + 2] : (T)0;
+            for (int q = 0; q < 6; q++) w[q] = (T)ps[q];
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const T xi = ai ? (T)xm[3 * i] : FAR, yi = ai ? (T)xm[3 * i + 1] : FAR, zi = ai ? (T)xm[3 * i + 2] : FAR;
-        const T fxi = ai ? (T)f[3 * i] : (T)0, fyi = ai ? (T)f[3 * i + 1] : (T)0, fzi = ai ? (T)f[3 * i + 2] : (T)0;
+        const T xi = (T)pt[0], yi = (T)pt[1], zi = (T)pt[2];
+        const T fxi = (T)pt[3], fyi = (T)pt[4], fzi = (T)pt[5];
+        const int64_t itn = it + (int64_t)g * NW;
+        if (itn < nitem) {
+            decode(itn, In, Jn);
+            load(In, Jn);
+        }
         T ux = 0, uy = 0, uz = 0;
         if (I == J || chain) {
             const int ci = i / R.Nmono;
