@@ -28,7 +28,7 @@ namespace {
 
 constexpr int BW = BATCH_THREADS;   // threads per replica workgroup
 constexpr int NW = BW / 64;
-constexpr int BCCH = 256;   // colloids staged in LDS per pass of the monomer rows
+constexpr int BCCH = 512;   // colloids staged in LDS per pass of the monomer rows
 constexpr int FTILE = 1024;   // force partners per LDS tile
 
 __device__ __forceinline__ double wsum(double v) {
@@ -568,15 +568,18 @@ __device__ __forceinline__ void apply_slice(const BConst& B, const BRep& R, doub
             T* cs = reinterpret_cast<T*>(cstage);
             for (int c0 = 0; c0 < Nc; c0 += BCCH) {
                 const int cn = min(BCCH, Nc - c0);
-                __syncthreads();
-                for (int idx = t; idx < 11 * cn; idx += BW) {
-                    const int cc = idx / 11, q = idx - 11 * (idx / 11), c = c0 + cc;
-                    const double v = q < 3 ? xm[3 * (int64_t)(Nm + c) + q]
-                                           : q < 6 ? f[nm3 + 3 * (int64_t)Nc + 3 * c + (q - 3)]
-                                                   : f[nu + 5 * (int64_t)c + (q - 6)];
-                    cs[idx] = (T)v;
-                }
-                __syncthreads();
+                // with every colloid in one chunk, the first block's staging serves the others
+                const bool stage_now = Nc > BCCH || b == rank;
+                if (stage_now) __syncthreads();
+                if (stage_now)
+                    for (int idx = t; idx < 11 * cn; idx += BW) {
+                        const int cc = idx / 11, q = idx - 11 * (idx / 11), c = c0 + cc;
+                        const double v = q < 3 ? xm[3 * (int64_t)(Nm + c) + q]
+                                               : q < 6 ? f[nm3 + 3 * (int64_t)Nc + 3 * c + (q - 3)]
+                                                       : f[nu + 5 * (int64_t)c + (q - 6)];
+                        cs[idx] = (T)v;
+                    }
+                if (stage_now) __syncthreads();
                 if (!act) continue;
                 for (int cc = wv; cc < cn; cc += NW) {
                     const T* cv = cs + 11 * cc;
