@@ -4,12 +4,13 @@
 // Each replica is an unbounded system of probes and chains with full monomer hydrodynamics, run
 // exactly as a single context runs it (the reference's run() step, src/run.cpp:70-162: predictor,
 // slip noise, RFD drift, corrector) with block-eliminated saddle solves and the Lanczos square
-// root.  One workgroup of 1024 threads owns one replica for the whole step: every phase (forces,
-// the carried tile and the stresslet block of mob(), the grand-mobility applies, the stresslet
-// MINRES, the Lanczos recurrence and its tridiagonal square root, the RFD probes and the
-// integrator) is a workgroup-scope function separated by barriers, so a replica needs no other
-// workgroup and no host round trip.  Workgroups take replicas largest first (the host orders
-// them), so the longest replicas start first.
+// root.  A workgroup of 1024 threads runs a replica's whole step (forces, mob()'s snapshot and
+// carried tile, the grand-mobility applies, the stresslet MINRES, the Lanczos recurrence and its
+// tridiagonal square root, the RFD probes and the integrator) with no host round trip; every
+// block of the mobility is applied from positions, none is stored.  Replicas far above the mean
+// work run on a team of workgroups that shares the applies, the stresslet products and the pair
+// passes; the host packs the other replicas into the remaining workgroups (makespan schedule,
+// re-tuned from measured per-replica times).
 //
 // The noise is drawn on the host from each replica's own stream (std::mt19937_64 + normal
 // distribution, seeded as a single context is), so every replica reproduces an independent
@@ -53,7 +54,7 @@ __device__ double wg_dot(const double* a, const double* b, int64_t n, double* re
 
 // ---------------------------------------------------------------------------------- teams
 // A replica far above the mean work is run by a team of workgroups: the leader (rank 0) runs the
-// step and forks each apply, stresslet GEMV and mob() tile pass to the members through its
+// step and forks each apply pass and stresslet product (M_ss v) to the members through its
 // BTeamCtl; every rank runs the same slice function on its share and the leader joins.  The step
 // kernel is a cooperative launch, so all workgroups are co-resident and a member always reaches
 // its next wait.  Every wait is bounded (wall clock): a timeout sets the team's err, which ends
@@ -859,7 +860,7 @@ __device__ __forceinline__ void self_inv5(const double* v, double* out) {
 __device__ int wg_minres_ss(const BConst& B, const BRep& R, double* P, TeamCtx& tc, int64_t n5, const double* rhs,
                             double* x, double* w, double* red) {
     TeamCtx solo{nullptr, tc.gerr, 0, 1, 0, 0};
-    TeamCtx& gtc = n5 >= 512 ? tc : solo;   // the GEMV splits over the team when it is large
+    TeamCtx& gtc = n5 >= 512 ? tc : solo;   // M_ss v splits over the team when it is large
     double *r1 = w, *r2 = w + n5, *y = w + 2 * n5, *v = w + 3 * n5, *ww = w + 4 * n5, *w1 = w + 5 * n5,
            *w2 = w + 6 * n5;
     const int t = threadIdx.x;
@@ -1686,7 +1687,7 @@ ps_err ps_batch_create(const ps_params* p, int R, const int* Nc, const int* Npol
     }
     b->pool_n = pos;
     b->noise_n = npos;
-    // first cost estimate (the applies grow as Np^2, the stored colloid block as (11 Nc)^2);
+    // first cost estimate (the monomer tiles grow as Np^2, the colloid pair couplings as Nc^2);
     // measured step times replace it after the first step
     b->cost.resize(R);
     for (int r = 0; r < R; r++) {
