@@ -96,6 +96,45 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+def bond_lengths(x, Nm, Nmono, L):
+    """Lengths of the chain bonds (consecutive monomers of each chain), minimum image in a box."""
+    xm = np.asarray(x[: 3 * Nm]).reshape(-1, Nmono, 3)
+    d = xm[:, 1:] - xm[:, :-1]
+    if L:
+        d -= L * np.rint(d / L)
+    return np.linalg.norm(d, axis=2).ravel()
+
+
+def precision_errors(Context, pd, x0, forms, w, steps=1, seed=7):
+    """The mixed form against the FP64 form of the same algorithm from the same state and noise:
+    the Lanczos square root M^{1/2} xi at the initial configuration (same xi), and the bond-length
+    distribution after `steps` steps (relative change of its mean and standard deviation, and the
+    largest change of one bond relative to r0), with the FP64 form's own distribution beside it.
+    One step: with the reference's carried monomer-colloid tile these systems leave the well-posed
+    regime within a few steps, after which two trajectories that differ by rounding part ways."""
+    ctxs = {f: make_ctx(Context, pd, x0, **forms[f]) for f in ("mixed/block", "fp64/block")}
+    xi = np.random.default_rng(seed).normal(size=ctxs["fp64/block"].n)
+    W, bl = {}, {}
+    for f, c in ctxs.items():
+        c.check_dist()
+        c.mob()
+        W[f] = c.sqrt_apply(xi)[0]
+        c.run(steps)
+        bl[f] = bond_lengths(c.state()[0], w["Nm"], w["Nmono"], w.get("L"))
+        c.close()
+    a, b = bl["mixed/block"], bl["fp64/block"]
+    return {
+        "sqrt_rel_err_vs_fp64": rel(W["mixed/block"], W["fp64/block"]),
+        "bond_length_drift_vs_fp64": {
+            "steps": steps, "bonds": int(b.size),
+            "fp64_mean": float(b.mean()), "fp64_std": float(b.std()), "fp64_max": float(b.max()),
+            "mean_rel": float(abs(a.mean() - b.mean()) / b.mean()),
+            "std_rel": float(abs(a.std() - b.std()) / b.std()),
+            "max_bond_change_over_r0": float(np.abs(a - b).max() / w["r0"]),
+        },
+    }
+
+
 PEAK_MFMA_16BIT = 2.5e15   # dense BF16 / FP16 MFMA, MI355X
 HBM_PEAK = 8.0e12
 
@@ -419,6 +458,9 @@ def main():
     up_mixed = one_step_up(Context, pd, x0, **forms["mixed/block"])
     errs = {"mixed/block": rel(up_mixed, up_fp64_block), "fp64/block": 0.0}
     log(f"mixed/block vs fp64/block velocity rel err {errs['mixed/block']:.3e}")
+    t0 = time.perf_counter()
+    perr = precision_errors(Context, pd, x0, forms, w)
+    log(f"mixed/block vs fp64/block: {perr} ({time.perf_counter() - t0:.1f}s)")
 
     results = {}
     for form, over in forms.items():
@@ -490,6 +532,7 @@ def main():
                                f"m<={pd['sqrt_kmax']} rtol {pd['sqrt_rtol']}",
                    "global_batch": Np, "seq_len": 1, "parallelism": "single-gpu"},
         "vel_rel_err_vs_fp64": head["vel_rel_err"],
+        **perr,
         "fp64_value": f64["value"],
         "mixed_over_fp64": head["value"] / f64["value"],
         "mixed_over_fp64_reference_algorithm": (head["value"] / results["fp64/minres"]["value"]
