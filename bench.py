@@ -278,7 +278,9 @@ def batch_bench(args, configs):
     systems leave the well-posed regime within two steps (overlapping probes, stresslet solves that
     stop at their cap), so later steps would time that breakdown rather than the step.  The first
     (untimed) step of each form is the accuracy step and tunes the schedule.  Accuracy: per-replica
-    relative velocity error of mixed against FP64 from the same state and noise.  Roofline
+    relative velocity error of mixed against FP64 from the same state and noise, the square root
+    with FP32 against FP64 tiles on the same xi, and the bond-length distribution after the first
+    step.  Roofline
     (VALU-bound tiles): algorithmic work per step = sum_r 37 x 2 n_r^2 flop (5 full applies of the
     saddle solves + 32 Lanczos applies; the stresslet-row applies are not counted), over the step
     time.  CPU baseline: the oracle (one host core) stepping a sample of replicas, scaled to the
@@ -292,12 +294,23 @@ def batch_bench(args, configs):
     steps = args.batch_steps
     res = {}
     ups = {}
+    sqrt_errs, bonds = None, {}
     for form, over in forms.items():
         b = Batch(dict(params, **over), reps)
+        if sqrt_errs is None:
+            # the square root on the same xi at the initial positions: FP32 against FP64 tiles
+            b.op(4)
+            rng = np.random.default_rng(7)
+            xis = [rng.normal(size=s_["n"]) for s_ in b.sizes]
+            w64, _ = b.op(3, xis)
+            w32, _ = b.op(6, xis)
+            sqrt_errs = np.array([rel(a, c) for a, c in zip(w32, w64)])
         t1 = time.perf_counter()
         b.run(1)   # accuracy step (also the warmup and the first measured schedule)
         log(f"batch {form}: first step {time.perf_counter() - t1:.3f}s")
         ups[form] = [b.vectors(r)["up"] for r in range(b.R)]
+        bonds[form] = [bond_lengths(b.state(r)[0], rep["Npoly"] * rep["Nmono"], rep["Nmono"], None)
+                       for r, rep in enumerate(reps)]
         dt, its, unconv = 0.0, 0, 0
         for k in range(steps):
             for r, rep in enumerate(reps):
@@ -318,6 +331,15 @@ def batch_bench(args, configs):
         log(f"batch {form}: {res[form]['value']:.4g} bead-steps/s, {res[form]['ms_per_step']:.2f} ms/step, "
             f"{res[form]['teams']} teams, replica ms max {res[form]['replica_ms_max']:.1f}")
     errs = np.array([rel(a, c) for a, c in zip(ups["mixed"], ups["fp64"])])
+    # bond lengths after the first step, every chain bond of every replica
+    ba = np.concatenate([v for v in bonds["mixed"] if v.size])
+    bb = np.concatenate([v for v in bonds["fp64"] if v.size])
+    r0 = float(params["r0"])
+    bond_drift = {"steps": 1, "bonds": int(bb.size), "fp64_mean": float(bb.mean()), "fp64_std": float(bb.std()),
+                  "mean_rel": float(abs(ba.mean() - bb.mean()) / bb.mean()),
+                  "std_rel": float(abs(ba.std() - bb.std()) / bb.std()),
+                  "max_bond_change_over_r0": float(np.abs(ba - bb).max() / r0)}
+    log(f"batch mixed vs fp64: sqrt {sqrt_errs.max():.3e} (max), bonds {bond_drift}")
     n = np.array([3 * (r["Npoly"] * r["Nmono"]) + 11 * r["Nc"] for r in reps], dtype=np.float64)
     flops = float(np.sum(37.0 * 2.0 * n * n))
     head = res["mixed"]
@@ -361,6 +383,8 @@ def batch_bench(args, configs):
         "schedule": {form: {k: v for k, v in r.items() if k not in ("value", "ms_per_step")} for form, r in res.items()},
         "timed_steps_start_from": "the initial configurations (restored outside the timed region)",
         "per_replica_vel_rel_err_vs_fp64": {"max": float(errs.max()), "median": float(np.median(errs))},
+        "per_replica_sqrt_rel_err_vs_fp64": {"max": float(sqrt_errs.max()), "median": float(np.median(sqrt_errs))},
+        "bond_length_drift_vs_fp64": bond_drift,
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK["fp32"], "traffic": None,
                      "kernel": "k_batch_step (the whole step of every replica; VALU-bound tile generation)",

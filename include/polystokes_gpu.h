@@ -305,8 +305,9 @@ ps_err ps_batch_run(void* batch, int nsteps);
 ps_err ps_batch_stats(void* batch, int* out);
 /* One operation on every replica at its current positions, vectors concatenated over the
  * replicas: 0 / 1: y = M f (FP64 / FP32 monomer tiles; n each), 2: the saddle solve (ns each),
- * 3: W = M^{1/2} xi (n each), 4: mob() (no vectors; run it before 0-3), 5: the force rhs (ns
- * each, no input).  iters (R, may be null): op 2 MINRES, op 3 Lanczos iterations. */
+ * 3 / 6: W = M^{1/2} xi (n each; FP64 / FP32 monomer tiles), 4: mob() (no vectors; run it before
+ * 0-3 and 6), 5: the force rhs (ns each, no input).  iters (R, may be null): op 2 MINRES, op 3 / 6
+ * Lanczos iterations. */
 ps_err ps_batch_op(void* batch, int op, const double* in, double* out, int* iters);
 /* The last step's wall-clock time of each replica in microseconds and the team size (workgroups)
  * it ran with; either output (R entries) may be null.  No reference counterpart: the reference

@@ -1352,6 +1352,8 @@ __global__ __launch_bounds__(BW) void k_batch_op(BConst B, const BRep* reps, dou
         s0 = wg_block_solve(B, R, P, inr, outr, nullptr, red, tc);
     } else if (op == 3) {
         s0 = wg_sqrt<double>(B, R, P, inr, outr, red, lz, &nneg_sh, tc);
+    } else if (op == 6) {
+        s0 = wg_sqrt<float>(B, R, P, inr, outr, red, lz, &nneg_sh, tc);
     } else if (op == 4) {
         for (int64_t i = threadIdx.x; i < 3 * (int64_t)R.Np; i += BW) xpd[i] = x[i];
         __syncthreads();
@@ -2011,12 +2013,13 @@ ps_err ps_batch_config(void* h, int team_max, int min_blocks, int autotune) {
 
 // One operation on every replica at its current positions (for checks), inputs and outputs
 // concatenated over the replicas: op 0 / 1 y = M f (FP64 / float monomer tiles; n each, after
-// op 4), op 2 the saddle solve (b, x: ns each, after op 4), op 3 W = M^{1/2} xi (n each, FP64,
-// after op 4), op 4 mob() at the current positions (no input or output), op 5 the force rhs (ns).
-// iters (R ints, may be null): op 2 stresslet MINRES iterations, op 3 Lanczos iterations.
+// op 4), op 2 the saddle solve (b, x: ns each, after op 4), op 3 / 6 W = M^{1/2} xi (n each, FP64 /
+// float monomer tiles, after op 4), op 4 mob() at the current positions (no input or output), op 5
+// the force rhs (ns).  iters (R ints, may be null): op 2 stresslet MINRES iterations, op 3 / 6
+// Lanczos iterations.
 ps_err ps_batch_op(void* h, int op, const double* in, double* out, int* iters) {
     Batch* b = static_cast<Batch*>(h);
-    if (!b || op < 0 || op > 5) return PS_ERR_ARG;
+    if (!b || op < 0 || op > 6) return PS_ERR_ARG;
     std::vector<int64_t> io(b->R, 0);
     int64_t tot = 0;
     for (int r = 0; r < b->R; r++) {

@@ -2,7 +2,8 @@
 against the oracle run as a single system with the same parameters, configuration and seed.
 
 Tolerances: applies and forces 1e-12 relative; solves, the Lanczos square root (same fixed number
-of iterations) and 3-step trajectories 1e-10; the FP32-tile apply against FP64 1e-5.
+of iterations) and 3-step trajectories 1e-10; the FP32-tile apply against FP64 1e-5, the FP32-tile
+square root against FP64 1e-4.
 """
 import numpy as np
 import pytest
@@ -47,6 +48,7 @@ def test_batch_operations_match_the_oracle():
     y32, _ = b.op(1, fs)
     xs, _ = b.op(2, bs)
     ws, lz = b.op(3, xis)
+    w32, _ = b.op(6, xis)
     rh, _ = b.op(5)
     for r, rep in enumerate(reps):
         o = oracle_for(params, rep)
@@ -56,6 +58,8 @@ def test_batch_operations_match_the_oracle():
         assert rel(y32[r], y64[r]) < 1e-5, r
         assert rel(xs[r], o.saddle_solve(bs[r])[0]) < 1e-10, r
         assert rel(ws[r], o.sqrt_apply(xis[r])[0]) < 1e-10, r
+        # the same Lanczos with FP32 monomer tiles (the mixed form's square root)
+        assert rel(w32[r], ws[r]) < 1e-4, (r, rel(w32[r], ws[r]))
         assert rel(rh[r], o.rhs()) < 1e-12, r
         assert lz[r] == min(12, b.sizes[r]["n"]) or lz[r] < 12
     b.close()
