@@ -1689,12 +1689,14 @@ ps_err ps_batch_create(const ps_params* p, int R, const int* Nc, const int* Npol
     }
     b->pool_n = pos;
     b->noise_n = npos;
-    // first cost estimate (the monomer tiles grow as Np^2, the colloid pair couplings as Nc^2);
-    // measured step times replace it after the first step
+    // first cost estimate: the monomer tiles grow as Np^2 and the colloid pair couplings as Nc^2,
+    // over a floor of vector work that every step pays (the Lanczos recurrence, the MINRES
+    // iterations; fitted to lone-workgroup step times at Np = 156 ... 2980); measured step times
+    // replace it after the first step
     b->cost.resize(R);
     for (int r = 0; r < R; r++) {
         const double np = b->reps[r].Np, nc = b->reps[r].Nc;
-        b->cost[r] = 1e-3 * (np * np + 30.0 * nc * nc);
+        b->cost[r] = 1e-3 * (np * np + 30.0 * nc * nc) + 0.5 * np + 400.0;
     }
     auto fail_hip = [&](hipError_t e) {
         g_batch_create_error = std::string("batched replicas: ") + hipGetErrorString(e);
