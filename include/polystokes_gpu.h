@@ -322,6 +322,11 @@ ps_err ps_batch_phase_times(void* batch, double* us5);
  * 16 colloid rows); with autotune
  * the packing is redone before every step from the measured per-replica times. */
 ps_err ps_batch_config(void* batch, int team_max, int min_blocks, int autotune);
+/* The step schedule's planner, without a GPU: lone-workgroup costs and team caps (>= 1) of R
+ * replicas on W co-resident workgroups.  Out (each may be null): the team size of each replica, the
+ * workgroup of its first rank, each workgroup's planned load (W) and the makespan target T.  No
+ * reference counterpart. */
+ps_err ps_batch_plan(int R, const double* cost, const int* cap, int W, int* team, int* wg, double* load, double* T);
 
 #ifdef __cplusplus
 }

@@ -1504,33 +1504,29 @@ double team_time(double t1, int g) { return g <= 1 ? t1 : t1 * (TEAM_SERIAL + (1
 // finishes within T, the teams fit in the co-resident workgroups, and the other replicas, longest
 // first, each fit into the workgroup with the most time left before T (team workgroups after their
 // team phase).  Without cooperative launches every replica runs alone, packed the same way.
-void build_schedule(Batch* b) {
-    const int R = b->R, W = b->nwg_max;
-    std::vector<int> ord(R);
+// The planner behind build_schedule (no GPU state): costs and team caps of R replicas, W
+// workgroups.  Returns the makespan target T; g = team size per replica, lists = the solo replicas
+// of each workgroup (team workgroups first, in decreasing cost of their replica), ord = replicas by
+// decreasing cost.
+double plan_schedule(const std::vector<double>& cost, const std::vector<int>& cap, int W, std::vector<int>& g,
+                     std::vector<std::vector<int>>& lists, std::vector<int>& ord) {
+    const int R = (int)cost.size();
+    ord.resize(R);
     std::iota(ord.begin(), ord.end(), 0);
-    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return b->cost[x] > b->cost[y]; });
-    const bool teams = b->coop && b->team_max > 1;
-    std::vector<int> cap(R, 1);
-    if (teams)
-        for (int r = 0; r < R; r++) {
-            const int units = (b->reps[r].Nm + 63) / 64 + (b->reps[r].Nc + 15) / 16;
-            cap[r] = std::min(b->team_max, std::max(1, units / std::max(1, b->min_blocks)));
-        }
-    std::vector<int> g(R, 1), gbest(R, 1);
-    std::vector<std::vector<int>> lists, lbest;
-    std::vector<int> team_wg;   // team index -> first workgroup
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    g.assign(R, 1);
     // feasibility of makespan T: fills g and lists (per workgroup: its solo replicas)
     auto feasible = [&](double T) -> bool {
         int used = 0;
         std::vector<double> left;
         for (int r : ord) {
             g[r] = 1;
-            if (b->cost[r] <= T) continue;
+            if (cost[r] <= T) continue;
             int gr = 2;
-            while (gr <= cap[r] && team_time(b->cost[r], gr) > T) gr++;
+            while (gr <= cap[r] && team_time(cost[r], gr) > T) gr++;
             if (gr > cap[r]) return false;
             g[r] = gr;
-            for (int k = 0; k < gr; k++) left.push_back(T - team_time(b->cost[r], gr));
+            for (int k = 0; k < gr; k++) left.push_back(T - team_time(cost[r], gr));
             used += gr;
             if (used > W) return false;
         }
@@ -1543,16 +1539,16 @@ void build_schedule(Batch* b) {
         for (int r : ord) {
             if (g[r] > 1) continue;
             const int w = (int)(std::max_element(left.begin(), left.end()) - left.begin());
-            if (left[w] < b->cost[r] && !(left[w] == T && b->cost[r] <= T)) return false;
+            if (left[w] < cost[r]) return false;
             lists[w].push_back(r);
-            left[w] -= b->cost[r];
+            left[w] -= cost[r];
         }
         return true;
     };
     double total = 0.0, lo = 0.0;
     for (int r = 0; r < R; r++) {
-        total += b->cost[r];
-        lo = std::max(lo, team_time(b->cost[r], cap[r]));
+        total += cost[r];
+        lo = std::max(lo, team_time(cost[r], cap[r]));
     }
     lo = std::max(lo, total / W);
     double hi = std::max(lo, 1e-30) * 2.0;
@@ -1566,6 +1562,21 @@ void build_schedule(Batch* b) {
             lo = mid;
     }
     feasible(hi);
+    return hi;
+}
+
+void build_schedule(Batch* b) {
+    const int R = b->R, W = b->nwg_max;
+    const bool teams = b->coop && b->team_max > 1;
+    std::vector<int> cap(R, 1);
+    if (teams)
+        for (int r = 0; r < R; r++) {
+            const int units = (b->reps[r].Nm + 63) / 64 + (b->reps[r].Nc + 15) / 16;
+            cap[r] = std::min(b->team_max, std::max(1, units / std::max(1, b->min_blocks)));
+        }
+    std::vector<int> g, ord;
+    std::vector<std::vector<int>> lists;
+    plan_schedule(b->cost, cap, W, g, lists, ord);
     b->team_of = g;
     b->nteams = 0;
     for (int r : ord) b->nteams += g[r] > 1;
@@ -2010,6 +2021,41 @@ ps_err ps_batch_config(void* h, int team_max, int min_blocks, int autotune) {
     b->min_blocks = min_blocks;
     b->autotune = autotune != 0;
     b->resched = true;   // re-pack before the next step
+    return PS_OK;
+}
+
+// The step schedule's planner without a GPU: lone-workgroup costs and team caps of R replicas on W
+// co-resident workgroups.  Out (each may be null): team size per replica (R), the workgroup of
+// each replica's first (or only) rank (R), each workgroup's planned load (W; unused ones 0) and
+// the makespan target T.
+ps_err ps_batch_plan(int R, const double* cost, const int* cap, int W, int* team, int* wg, double* load, double* T) {
+    if (R <= 0 || W <= 0 || !cost || !cap) return PS_ERR_ARG;
+    for (int r = 0; r < R; r++)
+        if (!(cost[r] >= 0.0) || cap[r] < 1) return PS_ERR_ARG;
+    std::vector<double> c(cost, cost + R);
+    std::vector<int> cp(cap, cap + R), g, ord;
+    std::vector<std::vector<int>> lists;
+    const double Tm = plan_schedule(c, cp, W, g, lists, ord);
+    std::vector<int> first(R, -1);
+    std::vector<double> ld(W, 0.0);
+    int w = 0;
+    for (int r : ord)
+        if (g[r] > 1) {
+            first[r] = w;
+            for (int j = 0; j < g[r] && w < W; j++, w++) ld[w] = team_time(c[r], g[r]);
+        }
+    for (int i = 0; i < (int)lists.size() && i < W; i++)
+        for (int r : lists[i]) {
+            first[r] = i;
+            ld[i] += c[r];
+        }
+    if (team)
+        for (int r = 0; r < R; r++) team[r] = g[r];
+    if (wg)
+        for (int r = 0; r < R; r++) wg[r] = first[r];
+    if (load)
+        for (int i = 0; i < W; i++) load[i] = ld[i];
+    if (T) *T = Tm;
     return PS_OK;
 }
 

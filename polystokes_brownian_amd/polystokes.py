@@ -91,6 +91,7 @@ def lib():
         L.ps_batch_times.argtypes = [vp, dp, C.POINTER(ip)]
         L.ps_batch_config.argtypes = [vp, ip, ip, ip]
         L.ps_batch_phase_times.argtypes = [vp, dp]
+        L.ps_batch_plan.argtypes = [ip, dp, C.POINTER(ip), ip, C.POINTER(ip), C.POINTER(ip), dp, dp]
         _lib = L
     return _lib
 
@@ -428,6 +429,20 @@ class PolyStokes:
     @property
     def context(self):
         return self._context()
+
+
+def batch_plan(cost, cap, W):
+    """ps_batch_plan: (team sizes, workgroup of each replica's first rank, per-workgroup loads,
+    makespan target) for lone-workgroup costs and team caps on W workgroups (host only)."""
+    cost = np.ascontiguousarray(cost, dtype=np.float64)
+    R = len(cost)
+    capa = (C.c_int * R)(*[int(c) for c in cap])
+    team, wg = (C.c_int * R)(), (C.c_int * R)()
+    load, T = np.zeros(W), np.zeros(1)
+    rc = lib().ps_batch_plan(R, _p(cost), capa, int(W), team, wg, _p(load), _p(T))
+    if rc != PS_OK:
+        raise PsError(rc, "ps_batch_plan")
+    return np.array(list(team)), np.array(list(wg)), load, float(T[0])
 
 
 class Batch:
