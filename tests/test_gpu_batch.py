@@ -136,3 +136,27 @@ def test_teams_are_bit_identical_to_lone_workgroups_and_match_the_oracle():
     assert (us > 0).all()
     bt.close()
     bs.close()
+
+
+def test_chain_hi_replicas_match_the_oracle():
+    """chain_HI (no hydrodynamics between beads of the same chain): the monomer blocks run one way
+    with the same-chain skip; the apply and two steps of every replica against the oracle."""
+    params, reps = make([(3, 4, 20), (2, 7, 20), (0, 2, 20)], chain_HI=1)
+    b = Batch(params, reps)
+    rng = np.random.default_rng(5)
+    b.op(4)
+    fs = [rng.normal(size=s["n"]) for s in b.sizes]
+    y64, _ = b.op(0, fs)
+    for r, rep in enumerate(reps):
+        o = oracle_for(params, rep)
+        o.check_dist()
+        o.mob()
+        assert rel(y64[r], o.mob_mult(fs[r])) < 1e-12, r
+    b.close()
+    b = Batch(params, reps)
+    b.run(2)
+    for r, rep in enumerate(reps):
+        o = oracle_for(params, rep)
+        o.run(2)
+        assert rel(b.state(r)[0], o.state()[0]) < 1e-10, r
+    b.close()
