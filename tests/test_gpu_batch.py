@@ -160,3 +160,17 @@ def test_chain_hi_replicas_match_the_oracle():
         o.run(2)
         assert rel(b.state(r)[0], o.state()[0]) < 1e-10, r
     b.close()
+
+
+def test_more_replicas_than_workgroups_match_the_oracle():
+    """300 replicas on at most 256 co-resident workgroups (one per CU): some workgroups run a list of
+    several replicas in turn.  Two steps of every replica against the oracle."""
+    params, reps = make([(1, 1, 8), (0, 1, 8), (2, 0, 8)] * 100)
+    assert len(reps) == 300
+    b = Batch(params, reps)
+    b.run(2)
+    for r, rep in enumerate(reps):
+        o = oracle_for(params, rep)
+        o.run(2)
+        assert rel(b.state(r)[0], o.state()[0]) < 1e-10, r
+    b.close()
