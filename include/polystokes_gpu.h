@@ -46,7 +46,7 @@ enum { PS_FORCE_HS = 0, PS_FORCE_HS_EXP = 1 };
 
 /*
  * Parameters.  The first three groups are the reference's pybind11 constructor
- * (src/bindings.cpp:363-387), particle_info (src/Stokes.cpp:171) and trap_info
+ * (src/bindings.cpp:44-69), particle_info (src/Stokes.cpp:171) and trap_info
  * (src/Stokes.cpp:241), with the same meaning and defaults (ps_params_default).
  */
 typedef struct ps_params {
@@ -130,7 +130,7 @@ typedef struct ps_counters {
     int64_t lanczos_nneg_events, lanczos_nneg_count;
 } ps_counters;
 
-/* Fill *p with the reference defaults (src/bindings.cpp:374-387; record_forces=true as bound). */
+/* Fill *p with the reference defaults (src/bindings.cpp:56-69; record_forces=true as bound). */
 void ps_params_default(ps_params* p);
 
 /* Create a context: alok_arrays + set_vars + init_solver + init_random (src/init.cpp:112-419),
@@ -313,6 +313,9 @@ ps_err ps_batch_op(void* batch, int op, const double* in, double* out, int* iter
  * it ran with; either output (R entries) may be null.  No reference counterpart: the reference
  * runs one system per process. */
 ps_err ps_batch_times(void* batch, double* us, int* team);
+/* Workgroups of the last step's launch (nwg_last) and the co-resident maximum the schedule packs
+ * replicas onto (nwg_max); either pointer may be null.  No reference counterpart. */
+ps_err ps_batch_workgroups(void* batch, int* nwg_last, int* nwg_max);
 /* The last step's time of each replica per phase in microseconds (5 R: forces, mob(), saddle
  * solves, square root, and within those the team leader's time waiting in joins).  No reference
  * counterpart. */

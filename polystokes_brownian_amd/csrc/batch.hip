@@ -1999,6 +1999,15 @@ ps_err ps_batch_times(void* h, double* us, int* team) {
     return PS_OK;
 }
 
+// Workgroups of the last step's launch and the most that can be co-resident.
+ps_err ps_batch_workgroups(void* h, int* nwg_last, int* nwg_max) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b) return PS_ERR_ARG;
+    if (nwg_last) *nwg_last = b->nwg;
+    if (nwg_max) *nwg_max = b->nwg_max;
+    return PS_OK;
+}
+
 // The last step's time of each replica per phase, in microseconds (R x 5: forces, mob(), saddle
 // solves, square root -- vector work between phases counts toward the next phase -- and, within
 // those, the team leader's time waiting in joins).

@@ -461,7 +461,7 @@ ps_err ps_profile_end(void* h, ps_profile* out) {
 
 void ps_params_default(ps_params* p) {
     std::memset(p, 0, sizeof(*p));
-    // PolyStokes ctor defaults as bound (src/bindings.cpp:374-387)
+    // PolyStokes ctor defaults as bound (src/bindings.cpp:56-69)
     p->samplerate = 1;
     p->mm_HI = 1;
     p->chain_HI = 0;

@@ -1,5 +1,5 @@
 """Python front end of libpsgpu.so: the reference's pybind11 `PolyStokes` / `Box` API
-(src/bindings.cpp:359-437) over the C ABI, plus `Context`, a thin wrapper of every entry point.
+(src/bindings.cpp:41-119) over the C ABI, plus `Context`, a thin wrapper of every entry point.
 
 The HIP library is the only compute path: if libpsgpu.so is missing or no GPU is visible, every
 call raises -- there is no CPU fallback.
@@ -91,6 +91,7 @@ def lib():
         L.ps_batch_times.argtypes = [vp, dp, C.POINTER(ip)]
         L.ps_batch_config.argtypes = [vp, ip, ip, ip]
         L.ps_batch_phase_times.argtypes = [vp, dp]
+        L.ps_batch_workgroups.argtypes = [vp, C.POINTER(ip), C.POINTER(ip)]
         L.ps_batch_plan.argtypes = [ip, dp, C.POINTER(ip), ip, C.POINTER(ip), C.POINTER(ip), dp, dp]
         _lib = L
     return _lib
@@ -98,6 +99,15 @@ def lib():
 
 def _p(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _vec(a, n, name):
+    """`a` as a contiguous float64 vector of exactly n entries; ValueError otherwise.  Every array the
+    C ABI reads or writes is checked here, before the call: the library trusts the sizes it is given."""
+    v = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    if v.size != n:
+        raise ValueError(f"{name} must have {n} entries, got {v.size}")
+    return v
 
 
 def default_params(**kw):
@@ -142,14 +152,12 @@ class Context:
         return name.value.decode(), ncu.value
 
     def initial_configuration(self, x0, q0=None):
-        x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1)
-        if x0.size != 3 * self.Np:
-            raise ValueError("init_x0 must hold 3*Np positions")
-        q = None if q0 is None else _p(np.ascontiguousarray(q0, dtype=np.float64).reshape(-1))
-        self._check(lib().ps_initial_configuration(self.h, _p(x0), q))
+        x0 = _vec(x0, 3 * self.Np, "init_x0 (3 Np positions)")
+        q = None if q0 is None else _vec(q0, 4 * self.Nc, "init_q (4 Nc quaternion entries)")
+        self._check(lib().ps_initial_configuration(self.h, _p(x0), None if q is None else _p(q)))
 
     def set_positions(self, x):
-        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        x = _vec(x, 3 * self.Np, "x (3 Np positions)")
         self._check(lib().ps_set_positions(self.h, _p(x)))
 
     def state(self):
@@ -198,13 +206,13 @@ class Context:
         return t.reshape(3, 3)
 
     def mob_mult(self, f):
-        f = np.ascontiguousarray(f, dtype=np.float64)
+        f = _vec(f, self.n, "f (n = 3 Nm + 11 Nc)")
         u = np.zeros(self.n)
         self._check(lib().ps_mob_mult(self.h, f.ctypes.data, u.ctypes.data))
         return u
 
     def saddle_mult(self, x):
-        x = np.ascontiguousarray(x, dtype=np.float64)
+        x = _vec(x, self.ns, "x (ns = 6 Nm + 17 Nc)")
         y = np.zeros(self.ns)
         self._check(lib().ps_saddle_mult(self.h, x.ctypes.data, y.ctypes.data))
         return y
@@ -215,15 +223,15 @@ class Context:
         return d
 
     def saddle_solve(self, b, x0=None):
-        b = np.ascontiguousarray(b, dtype=np.float64)
-        x = np.zeros(self.ns) if x0 is None else np.array(x0, dtype=np.float64)
+        b = _vec(b, self.ns, "b (ns = 6 Nm + 17 Nc)")
+        x = np.zeros(self.ns) if x0 is None else _vec(x0, self.ns, "x0 (ns = 6 Nm + 17 Nc)").copy()
         info = PsSolveInfo()
         rc = self._check(lib().ps_saddle_solve(self.h, b.ctypes.data, x.ctypes.data, int(x0 is not None),
                                                C.byref(info)), allow_noconv=True)
         return x, info, rc
 
     def sqrt_apply(self, xi):
-        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        xi = _vec(xi, self.n, "xi (n = 3 Nm + 11 Nc)")
         w = np.zeros(self.n)
         info = PsSqrtInfo()
         self._check(lib().ps_sqrt_apply(self.h, xi.ctypes.data, w.ctypes.data, C.byref(info)))
@@ -257,6 +265,8 @@ class Context:
     def dense_mult(self, X):
         """Y = M X for X of shape (nrhs, n), nrhs <= 16, on the stored-dense M (ps_dense_mult)."""
         X = np.ascontiguousarray(np.atleast_2d(X), dtype=np.float64)
+        if X.ndim != 2 or X.shape[1] != self.n or not 1 <= X.shape[0] <= 16:
+            raise ValueError(f"X must have shape (nrhs, n) with 1 <= nrhs <= 16 and n = {self.n}, got {X.shape}")
         Y = np.zeros_like(X)
         self._check(lib().ps_dense_mult(self.h, X.ctypes.data, Y.ctypes.data, X.shape[0]))
         return Y
@@ -297,15 +307,11 @@ class Context:
 
     def lub_apply(self, uw, einf=None):
         """[F_c; T_c; S_c] (11 Nc) of the pair lubrication for uw = [U_c; W_c] (6 Nc) and strain einf (5)."""
-        uw = np.ascontiguousarray(uw, dtype=np.float64)
-        if uw.shape != (6 * self.Nc,):
-            raise ValueError(f"uw must have length 6 Nc = {6 * self.Nc}")
+        uw = _vec(uw, 6 * self.Nc, "uw (6 Nc)")
         out = np.zeros(11 * self.Nc)
         e = None
         if einf is not None:
-            e = np.ascontiguousarray(einf, dtype=np.float64)
-            if e.shape != (5,):
-                raise ValueError("einf must have 5 entries")
+            e = _vec(einf, 5, "einf")
         self._check(lib().ps_lub_apply(self.h, uw.ctypes.data, None if e is None else e.ctypes.data,
                                        out.ctypes.data))
         return out
@@ -320,7 +326,7 @@ class Context:
 
 
 class Box:
-    """Periodic box helper with the reference's semantics (src/box.cpp, src/bindings.cpp:418-436)."""
+    """Periodic box helper with the reference's semantics (src/box.cpp, src/bindings.cpp:96-118)."""
 
     def __init__(self):
         self._L = None
@@ -352,7 +358,7 @@ class Box:
 
 
 class PolyStokes:
-    """The reference's Python class (src/bindings.cpp:363-414), driving the MI355X library.
+    """The reference's Python class (src/bindings.cpp:41-95), driving the MI355X library.
 
     Same constructor keywords and defaults, same particle_info / trap_info / initial_configuration /
     run sequence.  Two keyword-only additions select the MI355X forms and solver tolerances through
@@ -475,19 +481,25 @@ class Batch:
         if rc != 0:
             raise PsError(rc, lib().ps_batch_last_error(self.h).decode())
 
+    def _replica(self, r):
+        if not 0 <= int(r) < self.R:
+            raise ValueError(f"replica index {r} outside [0, {self.R})")
+        return self.sizes[int(r)]
+
     def set_state(self, r, x0, q0=None):
-        x0 = np.ascontiguousarray(x0, dtype=np.float64)
-        q0 = None if q0 is None else np.ascontiguousarray(q0, dtype=np.float64)
+        s = self._replica(r)
+        x0 = _vec(x0, 3 * s["Np"], f"x0 of replica {r} (3 Np)")
+        q0 = None if q0 is None else _vec(q0, 4 * s["Nc"], f"q0 of replica {r} (4 Nc)")
         self._check(lib().ps_batch_set_state(self.h, r, _p(x0), None if q0 is None else _p(q0)))
 
     def state(self, r):
-        s = self.sizes[r]
+        s = self._replica(r)
         x, q, t = np.zeros(3 * s["Np"]), np.zeros(max(4 * s["Nc"], 1)), C.c_double()
         self._check(lib().ps_batch_get_state(self.h, r, _p(x), _p(q), C.byref(t)))
         return x, q[: 4 * s["Nc"]], t.value
 
     def vectors(self, r):
-        nu = self.sizes[r]["nu"]
+        nu = self._replica(r)["nu"]
         out = {k: np.zeros(nu) for k in ("up", "v_det_n", "v_det_c", "v_brown", "udiff")}
         self._check(lib().ps_batch_get_vectors(self.h, r, *[_p(out[k]) for k in
                                                              ("up", "v_det_n", "v_det_c", "v_brown", "udiff")]))
@@ -508,6 +520,12 @@ class Batch:
         self._check(lib().ps_batch_times(self.h, _p(us), team))
         return us, np.array(list(team), dtype=np.int64)
 
+    def workgroups(self):
+        """ps_batch_workgroups: (workgroups of the last step's launch, co-resident maximum)."""
+        a, m = C.c_int(), C.c_int()
+        self._check(lib().ps_batch_workgroups(self.h, C.byref(a), C.byref(m)))
+        return a.value, m.value
+
     def phase_times(self):
         """ps_batch_phase_times: (R, 5) microseconds in forces, mob(), saddle solves, square root,
         and the team leader's time waiting in joins."""
@@ -526,10 +544,14 @@ class Batch:
         lens = [s[key] for s in self.sizes]
         offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
         tot = int(offs[-1])
+        if op not in (0, 1, 2, 3, 4, 5, 6):
+            raise ValueError(f"unknown batch op {op}")
         xin = np.zeros(max(tot, 1))
         if inputs is not None:
+            if len(inputs) != self.R:
+                raise ValueError(f"op {op} takes one input vector per replica ({self.R}), got {len(inputs)}")
             for r, v in enumerate(inputs):
-                xin[offs[r]:offs[r + 1]] = v
+                xin[offs[r]:offs[r + 1]] = _vec(v, lens[r], f"input of replica {r} ({key})")
         xout = np.zeros(max(tot, 1))
         its = (C.c_int * self.R)()
         self._check(lib().ps_batch_op(self.h, int(op), _p(xin), _p(xout), its))

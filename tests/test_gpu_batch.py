@@ -57,11 +57,12 @@ def test_batch_operations_match_the_oracle():
         assert rel(y64[r], o.mob_mult(fs[r])) < 1e-12, r
         assert rel(y32[r], y64[r]) < 1e-5, r
         assert rel(xs[r], o.saddle_solve(bs[r])[0]) < 1e-10, r
-        assert rel(ws[r], o.sqrt_apply(xis[r])[0]) < 1e-10, r
+        wo, io = o.sqrt_apply(xis[r])
+        assert rel(ws[r], wo) < 1e-10, r
+        assert lz[r] == io.iterations, (r, lz[r], io.iterations)
         # the same Lanczos with FP32 monomer tiles (the mixed form's square root)
         assert rel(w32[r], ws[r]) < 1e-4, (r, rel(w32[r], ws[r]))
         assert rel(rh[r], o.rhs()) < 1e-12, r
-        assert lz[r] == min(12, b.sizes[r]["n"]) or lz[r] < 12
     b.close()
 
 
@@ -169,6 +170,11 @@ def test_more_replicas_than_workgroups_match_the_oracle():
     assert len(reps) == 300
     b = Batch(params, reps)
     b.run(2)
+    nwg, nwg_max = b.workgroups()
+    print("workgroups", nwg, "co-resident maximum", nwg_max, flush=True)
+    if nwg_max >= len(reps):
+        pytest.skip(f"{nwg_max} co-resident workgroups: every replica has its own")
+    assert nwg <= nwg_max < len(reps)
     for r, rep in enumerate(reps):
         o = oracle_for(params, rep)
         o.run(2)

@@ -1,0 +1,268 @@
+"""GPU parity at the sizes of the named workloads, against the CPU FP64 oracle.
+
+  c2   2,000 beads (20 probes + 99 x 20), box 30, FENE bonds (Lmax 0.6), full monomer HI: the
+       Verlet list bit-exact at the start and at the GPU's own positions after 50, 100 and 200
+       steps; the RHS with FENE to 1e-12; one step to 1e-10.
+  c3   20,000 beads (200 probes + 990 x 20), box 64: several contexts made and destroyed in one
+       process (ordered and symmetric monomer kernels, FP64 and FP32 tiles, kernel timing on and
+       off), each apply against the oracle's matrix-free apply (FP64 1e-12, FP32 1e-5); one step of
+       the block-eliminated form against the oracle's step on the same state and noise (1e-10).
+  c4   256 replicas log-uniform in [100, 3000] beads: one batched step; the smallest, median and
+       largest replicas against single-system oracle steps (1e-10); every replica's mixed-form
+       velocity against its FP64 velocity (1e-6).
+  c5   100,000 beads: one FP64 (1e-12) and one FP32 (1e-5) apply against the oracle's matrix-free
+       apply; the 8,000-bead stored-dense 16-bit MFMA apply of 16 vectors against 16 oracle applies.
+  force laws: hs_exp (POLYSTOKES_FORCE=1) and tether bonds, RHS and one step against the oracle.
+
+The oracle is the checker only; every GPU call goes through the C ABI.
+"""
+import numpy as np
+import pytest
+
+from polystokes_brownian_amd import configs
+from polystokes_brownian_amd.polystokes import Batch, Context
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(pd, dense=None):
+    from oracle.pyoracle import Oracle
+    return Oracle(pd, dense=dense)
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+_CACHE = {}
+
+
+def _workload(name):
+    if name not in _CACHE:
+        w = configs.workload(name)
+        _CACHE[name] = (w, configs.params_dict(w), configs.positions(w))
+    return _CACHE[name]
+
+
+# ------------------------------------------------------------------------------------------- c2
+def _c2_pair(**over):
+    w, pd, x0 = _workload("c2")
+    pd = dict(pd, **over)
+    g = Context(pd)
+    g.initial_configuration(x0)
+    o = _oracle(pd, dense=False)
+    o.initial_configuration(x0)
+    return g, o, pd, x0
+
+
+def _same_vlist(g, o):
+    ng, no = g.check_dist(), o.check_dist()
+    assert ng == no, (ng, no)
+    for a, b in zip(g.vlist(), o.vlist()):
+        assert np.array_equal(a, b)
+    return ng
+
+
+def test_c2_verlet_list_bit_exact_along_a_gpu_trajectory():
+    g, o, pd, x0 = _c2_pair(solver=1)
+    say("c2: Np", g.Np, "FENE", pd["fene"], "Lmax", pd["Lmax"])
+    say("c2: initial pairs", _same_vlist(g, o))
+    done = 0
+    for target in (50, 100, 200):
+        g.run(target - done)
+        done = target
+        x, _, _ = g.state()
+        assert np.all(np.isfinite(x)), target
+        o.set_positions(x)
+        g.set_positions(x)
+        say("c2: after", target, "GPU steps, pairs", _same_vlist(g, o))
+    g.close()
+
+
+def test_c2_rhs_with_fene_and_one_step():
+    g, o, pd, x0 = _c2_pair(solver=1, ksp_rtol=1e-12, sqrt_rtol=1e-10)
+    g.check_dist()
+    o.check_dist()
+    rg, ro = g.rhs(), o.rhs()
+    say("c2: FENE rhs rel", rel(rg, ro))
+    assert rel(rg, ro) < 1e-12
+    g.run(1)
+    o.run(1)
+    xg, _, _ = g.state()
+    xo, _, _ = o.state()
+    vg, vo = g.vectors(), o.vectors()
+    say("c2: one step displacement rel", rel(xg - x0, xo - x0), "up rel", rel(vg["up"], vo["up"]))
+    assert rel(xg - x0, xo - x0) < 1e-10
+    for k in ("up", "v_det_n", "v_det_c", "v_brown"):
+        assert rel(vg[k], vo[k]) < 1e-10, k
+    g.close()
+
+
+# ------------------------------------------------------------------------------------------- c3
+def test_c3_applies_in_one_process_against_the_oracle():
+    """Contexts of every kernel and precision made and destroyed one after another in one process,
+    with timing on and off, each checked against the oracle's matrix-free apply on a force vector of
+    the full length n (3 Nm force rows followed by the 11 Nc colloid rows)."""
+    w, pd, x0 = _workload("c3")
+    o = _oracle(pd, dense=False)
+    o.initial_configuration(x0)
+    o.check_dist()
+    o.mob()
+    f = np.random.default_rng(20000).normal(size=o.n)
+    uo = o.mob_mult(f)
+    runs = [(1, 0, False), (2, 0, False), (2, 1, True), (1, 1, False), (2, 0, True), (0, 0, False),
+            (0, 1, True), (2, 1, False)]
+    us = {}
+    for kern, prec, prof in runs:
+        g = Context(dict(pd, aa_kernel=kern, precision=prec))
+        g.initial_configuration(x0)
+        g.check_dist()
+        g.mob()
+        if prof:
+            g.profile_begin()
+        u = g.mob_mult(f)
+        if prof:
+            g.profile_end()
+        e = rel(u, uo)
+        say("c3: kernel", kern, "precision", prec, "timing", prof, "rel vs oracle", e)
+        assert e < (1e-12 if prec == 0 else 1e-5), (kern, prec, prof, e)
+        if (kern, prec) in us:
+            assert np.array_equal(us[(kern, prec)], u)   # the same bits in a later context
+        us[(kern, prec)] = u
+        g.close()
+    assert rel(us[(2, 0)], us[(1, 0)]) < 1e-13
+
+
+def test_c3_block_step_against_the_oracle():
+    w, pd, x0 = _workload("c3")
+    pd = dict(pd, solver=1)
+    g = Context(pd)
+    g.initial_configuration(x0)
+    g.run(1)
+    o = _oracle(pd, dense=False)
+    o.initial_configuration(x0)
+    o.run(1)
+    xg, _, _ = g.state()
+    xo, _, _ = o.state()
+    vg, vo = g.vectors(), o.vectors()
+    e = {k: rel(vg[k], vo[k]) for k in ("up", "v_det_n", "v_det_c", "v_brown")}
+    say("c3: one block step, displacement rel", rel(xg - x0, xo - x0), e)
+    assert rel(xg - x0, xo - x0) < 1e-10
+    for k, v in e.items():
+        assert v < 1e-10, k
+    g.close()
+
+
+# ------------------------------------------------------------------------------------------- c4
+def test_c4_one_batched_step_against_single_system_runs():
+    params, reps = configs.replica_workload()
+    sizes = np.array([r["Nc"] + r["Npoly"] * r["Nmono"] for r in reps])
+    order = np.argsort(sizes, kind="stable")
+    pick = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    say("c4:", len(reps), "replicas,", int(sizes.sum()), "beads; checked replicas", pick, sizes[pick])
+    b = Batch(params, reps)
+    b.run(1)
+    for r in pick:
+        rep = reps[r]
+        pr = dict(params, Nc=rep["Nc"], Npoly=rep["Npoly"], Nmono_per_chain=rep["Nmono"],
+                  Nm=rep["Npoly"] * rep["Nmono"], Np=rep["Npoly"] * rep["Nmono"] + rep["Nc"], seed=rep["seed"], solver=1)
+        o = _oracle(pr)
+        o.initial_configuration(rep["x0"])
+        o.run(1)
+        xg, _, _ = b.state(r)
+        xo, _, _ = o.state()
+        vg, vo = b.vectors(r), o.vectors()
+        say("c4: replica", r, "Np", pr["Np"], "displacement rel", rel(xg - rep["x0"], xo - rep["x0"]),
+            "up rel", rel(vg["up"], vo["up"]))
+        assert rel(xg - rep["x0"], xo - rep["x0"]) < 1e-10, r
+        assert rel(vg["up"], vo["up"]) < 1e-10, r
+    up64 = [b.vectors(r)["up"] for r in range(len(reps))]
+    b.close()
+    bm = Batch(dict(params, precision=1), reps)
+    bm.run(1)
+    errs = np.array([rel(bm.vectors(r)["up"], up64[r]) for r in range(len(reps))])
+    bm.close()
+    say("c4: mixed-form velocity error vs FP64, max", errs.max(), "median", np.median(errs))
+    assert errs.max() < 1e-6
+
+
+# ------------------------------------------------------------------------------------------- c5
+def test_c5_applies_against_the_oracle():
+    w, pd, x0 = _workload("c5")
+    say("c5: Np", w["Np"])
+    o = _oracle(pd, dense=False)
+    o.initial_configuration(x0)
+    o.check_dist()
+    o.mob()
+    f = np.random.default_rng(100000).normal(size=o.n)
+    uo = o.mob_mult(f)
+    for prec, tol in ((0, 1e-12), (1, 1e-5)):
+        g = Context(dict(pd, precision=prec))
+        g.initial_configuration(x0)
+        g.check_dist()
+        g.mob()
+        e = rel(g.mob_mult(f), uo)
+        say("c5: precision", prec, "rel vs oracle", e)
+        assert e < tol
+        g.close()
+
+
+@pytest.mark.parametrize("storage,tol", [("bf16", 2e-2), ("f16", 3e-3)])
+def test_c5dense_mfma_apply_against_oracle_applies(storage, tol):
+    """The stored-dense 16-bit M applied to 16 vectors at once, against 16 FP64 oracle applies; the
+    error is the 16-bit rounding of M (bf16: 8 bits, f16: 11 bits of mantissa)."""
+    w, pd, x0 = _workload("c5dense")
+    o = _oracle(pd, dense=False)
+    o.initial_configuration(x0)
+    o.check_dist()
+    o.mob()
+    g = Context(pd)
+    g.initial_configuration(x0)
+    g.check_dist()
+    g.mob()
+    g.dense_build(storage)
+    X = np.random.default_rng(7).normal(size=(16, g.n))
+    Y = g.dense_mult(X)
+    Yo = np.stack([o.mob_mult(X[k]) for k in range(16)])
+    e = rel(Y, Yo)
+    per = [rel(Y[k], Yo[k]) for k in range(16)]
+    say("c5dense:", storage, "Np", w["Np"], "rel vs 16 oracle applies", e, "worst vector", max(per))
+    assert e < tol and max(per) < 2 * tol
+    g.close()
+
+
+# ------------------------------------------------------------------------------------ force laws
+def _small_pair(**over):
+    w = configs.small(4, 6, 10, 6.0, seed=17)
+    pd = configs.params_dict(w)
+    pd.update(ksp_rtol=1e-12, sqrt_rtol=1e-10, solver=1, **over)
+    x0 = configs.positions(w)
+    g = Context(pd)
+    g.initial_configuration(x0)
+    o = _oracle(pd)
+    o.initial_configuration(x0)
+    return g, o, x0
+
+
+@pytest.mark.parametrize("over", [dict(force_sel=1), dict(force_sel=1, hs_exp_k=40.0), dict(tether=1),
+                                  dict(tether=1, fene=1, Lmax=2.0)])
+def test_force_laws_rhs_and_one_step(over):
+    g, o, x0 = _small_pair(**over)
+    g.check_dist()
+    o.check_dist()
+    rg, ro = g.rhs(), o.rhs()
+    say("force law", over, "rhs rel", rel(rg, ro))
+    assert rel(rg, ro) < 1e-12
+    g.run(1)
+    o.run(1)
+    xg, _, _ = g.state()
+    xo, _, _ = o.state()
+    say("force law", over, "step displacement rel", rel(xg - x0, xo - x0))
+    assert rel(xg - x0, xo - x0) < 1e-10
+    assert rel(g.vectors()["up"], o.vectors()["up"]) < 1e-10
+    g.close()

@@ -53,7 +53,7 @@ def test_struct_sizes_match_header():
 
 def test_reference_defaults():
     p = P.default_params()
-    # src/bindings.cpp:374-387 and src/init.cpp:64
+    # src/bindings.cpp:56-69 and src/init.cpp:64
     assert (p.mm_HI, p.chain_HI, p.fene, p.record_forces, p.tether, p.mono_ev) == (1, 0, 1, 1, 0, 0)
     assert p.seed == 12345 and p.restart == 0.0 and p.box_active == 0
     assert p.ksp_rtol == 1e-6 and p.ksp_abstol == 1e-50 and p.ksp_maxit == 10000
