@@ -85,7 +85,8 @@ typedef struct ps_params {
     /* MI355X forms (no reference counterpart). */
     int precision;                /* PS_PREC_FP64 or PS_PREC_MIXED */
     double ir_rtol;               /* mixed form: FP64 refinement target on the saddle residual */
-    int ir_maxit;                 /* mixed form: cap on refinement sweeps */
+    int ir_maxit;                 /* mixed form: cap on refinement sweeps; with PS_SOLVER_BLOCK, 0 means
+                                     no refinement (the solve's one apply runs on FP32 tiles) */
     int solver;                   /* PS_SOLVER_MINRES (reference algorithm) or PS_SOLVER_BLOCK */
     int aa_kernel;                /* monomer-monomer kernel: PS_AA_AUTO, PS_AA_ORDERED, PS_AA_SYMMETRIC */
     int graph;                    /* ps_run: 1 (default) replays each step as a HIP graph where the step
@@ -128,6 +129,8 @@ typedef struct ps_counters {
     int64_t mob_calls, applies, minres_iterations, lanczos_iterations, saddle_solves, sqrt_calls;
     int64_t nvlist;      /* Verlet pairs found by the last check_dist */
     int64_t lanczos_nneg_events, lanczos_nneg_count;
+    int64_t stresslet_iterations;  /* PS_SOLVER_BLOCK: MINRES iterations of the 5Nc x 5Nc stresslet solves */
+    int64_t stresslet_capped;      /* PS_SOLVER_BLOCK: stresslet solves stopped at their cap (10000) */
 } ps_counters;
 
 /* Fill *p with the reference defaults (src/bindings.cpp:56-69; record_forces=true as bound). */

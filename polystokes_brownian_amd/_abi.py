@@ -47,7 +47,8 @@ class PsSqrtInfo(C.Structure):
 class PsCounters(C.Structure):
     _fields_ = [("mob_calls", C.c_int64), ("applies", C.c_int64), ("minres_iterations", C.c_int64),
                 ("lanczos_iterations", C.c_int64), ("saddle_solves", C.c_int64), ("sqrt_calls", C.c_int64),
-                ("nvlist", C.c_int64), ("lanczos_nneg_events", C.c_int64), ("lanczos_nneg_count", C.c_int64)]
+                ("nvlist", C.c_int64), ("lanczos_nneg_events", C.c_int64), ("lanczos_nneg_count", C.c_int64),
+                ("stresslet_iterations", C.c_int64), ("stresslet_capped", C.c_int64)]
 
 
 def params_from_dict(d, base=None):

@@ -22,6 +22,7 @@
 #include "ps_common.h"
 #include "tiles.h"
 #include "batch.h"
+#include "tql2.h"
 
 #pragma clang fp contract(off)
 
@@ -951,69 +952,6 @@ __device__ int wg_block_solve(const BConst& B, const BRep& R, double* pool, cons
 }
 
 // ---------------------------------------------------------------------------------- square root
-// tql2 (implicit QL on a symmetric tridiagonal, eigenvectors in Z, column-major m x m) on one wave:
-// every lane runs the same scalar recurrence on d and e (LDS; all lanes store identical values),
-// and lane k applies each plane rotation to row k of Z.
-__device__ bool b_tql2_wave(int m, double* d, double* e, double* Z) {
-    const int lane = threadIdx.x & 63;
-    auto wsync = []() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); };
-    for (int k = lane; k < m * m; k += 64) Z[k] = 0.0;
-    wsync();
-    if (lane < m) Z[lane + lane * m] = 1.0;
-    wsync();
-    for (int l = 0; l < m; l++) {
-        int iter = 0, mm;
-        do {
-            for (mm = l; mm < m - 1; mm++) {
-                const double dd = fabs(d[mm]) + fabs(d[mm + 1]);
-                if (fabs(e[mm]) <= 1e-300 + 2.2e-16 * dd) break;
-            }
-            if (mm != l) {
-                if (iter++ == 200) return false;
-                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-                double r = hypot(g, 1.0);
-                g = d[mm] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
-                double s = 1.0, c = 1.0, p = 0.0;
-                bool underflow = false;
-                for (int i = mm - 1; i >= l; i--) {
-                    const double f = s * e[i], bb = c * e[i];
-                    r = hypot(f, g);
-                    wsync();
-                    e[i + 1] = r;
-                    if (r == 0.0) {
-                        d[i + 1] -= p;
-                        e[mm] = 0.0;
-                        underflow = true;
-                        wsync();
-                        break;
-                    }
-                    s = f / r;
-                    c = g / r;
-                    g = d[i + 1] - p;
-                    r = (d[i] - g) * s + 2.0 * c * bb;
-                    p = s * r;
-                    wsync();
-                    d[i + 1] = g + p;
-                    g = c * r - bb;
-                    if (lane < m) {
-                        const double zf = Z[lane + (i + 1) * m];
-                        Z[lane + (i + 1) * m] = s * Z[lane + i * m] + c * zf;
-                        Z[lane + i * m] = c * Z[lane + i * m] - s * zf;
-                    }
-                    wsync();
-                }
-                if (underflow) continue;
-                wsync();
-                d[l] -= p;
-                e[l] = g;
-                e[mm] = 0.0;
-                wsync();
-            }
-        } while (mm != l);
-    }
-    return true;
-}
-
 // out = M^{1/2} xi (n) by Lanczos (schur_sqrt_lanczos, src/slip_vel.cpp:351-477, on the full grand
 // mobility): kmax iterations, or fewer on an invariant subspace (beta_j <= 1e-12 beta_0); two-pass
 // classical Gram-Schmidt against the stored basis; T_m = U diag(theta) U^T by QL; negative Ritz
@@ -1097,16 +1035,13 @@ __device__ int wg_sqrt(const BConst& B, const BRep& R, double* pool, const doubl
     }
     __syncthreads();
     if (wv == 0) {
-        // g = beta0 U diag(sqrt(max(theta, 0))) U^T e1 (src/slip_vel.cpp:427-459); the QL works on
-        // d (in h) and e (in g; dead before g is written)
+        // g = beta0 U diag(sqrt(max(theta, 0))) U^T e1 (src/slip_vel.cpp:427-459); the QL keeps d and e
+        // in the lanes' registers, the spectrum then goes to h for every lane to read
+        double dv = lane < m ? alpha[lane] : 0.0;
+        double ev = (lane < m - 1) ? betad[lane] : 0.0;
+        tql2_wave(m, dv, ev, Z);
         double* dd = h;
-        double* ee = g;
-        for (int i = lane; i < m; i += 64) {
-            dd[i] = alpha[i];
-            ee[i] = (i < m - 1) ? betad[i] : 0.0;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        b_tql2_wave(m, dd, ee, Z);
+        if (lane < m) dd[lane] = dv;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         double gi = 0.0;
         if (lane < m)

@@ -181,12 +181,17 @@ struct NoisePipe {
     }
 };
 
-// The step's saddle solve of rhs into X; `uout` (optional) receives its velocity rows.
+// The step's saddle solve of rhs into X; `uout` (optional) receives its velocity rows.  No solve
+// information is asked for, so a block-eliminated solve makes no host round trip.
 ps_err solve(Ctx* c, double* X, int warm, double* uout = nullptr) {
-    ps_solve_info info{};
-    ps_err r = saddle_solve_dev(c, c->rhs, X, warm, &info, uout);
+    ps_err r = saddle_solve_dev(c, c->rhs, X, warm, nullptr, uout);
     if (r == PS_ERR_NOCONV) r = PS_OK;   // the reference prints and continues (src/vel.cpp:45-51)
     return r;
+}
+
+__global__ void k_zero(double* __restrict__ a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i < n) a[i] = 0.0;
 }
 
 // One iteration of the run() loop; `pipe` supplies this step's noise (step index s), or is null
@@ -251,7 +256,8 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s, double* noise_fixed = nullptr) {
             k_extract<<<eb(nu), TB, 0, st>>>(c->fext, c->X, 0, nu);
             PS_LAUNCH(c);
         }
-        PS_HIP(c, hipMemsetAsync(c->v_brown, 0, sizeof(double) * (size_t)nu, st));
+        k_zero<<<eb(nu), TB, 0, st>>>(c->v_brown, nu);
+        PS_LAUNCH(c);
     }
     k_step<<<eb(c->Np), TB, 0, st>>>(c->sc, c->x, c->fext, c->udiff, c->up, dt, kT);
     PS_LAUNCH(c);
@@ -284,11 +290,32 @@ ps_err fail(Ctx* c, ps_err code, const std::string& msg) {
     return code;
 }
 
-// A step with no host round trip can be replayed: the arrowhead regime with block solves, where every
-// launch's arguments are fixed for the run (no Krylov polling, no WCA grid from the host).
+// A step with no host round trip can be replayed, since every launch's arguments are then fixed for
+// the run: block-eliminated solves (the stresslet MINRES runs to convergence inside one launch) and a
+// Lanczos square root whose stopping test runs on the device.  In the arrowhead regime: one colloid,
+// no WCA grid from the host.
 bool step_graphable(const Ctx* c) {
-    return c->p.graph && c->arrow && c->Nc == 1 && c->Nm > 0 && c->p.solver == PS_SOLVER_BLOCK && c->p.kT > 0 &&
-           !c->p.mono_ev && !c->p.record_forces && !c->prof_on;
+    if (!c->p.graph || c->p.solver != PS_SOLVER_BLOCK || !(c->p.kT > 0) || c->prof_on) return false;
+    if (c->arrow) return c->Nc == 1 && c->Nm > 0 && !c->p.mono_ev && !c->p.record_forces;
+    // FP64 refinement of the mixed form reads its residual norms on the host
+    return !(c->p.precision == PS_PREC_MIXED && c->p.ir_maxit > 0);
+}
+
+// Sticky device flags of the last run: a Verlet list over its fixed capacity, a grid-wide wait that
+// ran out of time.  Reads the device counters (a host round trip).
+ps_err check_device_flags(Ctx* c) {
+    PS_HIP(c, hipMemcpyAsync(c->dcnt_h, c->dcnt, sizeof(int64_t) * DC_N, hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->dcnt_h[DC_VLIST_OVERFLOW]) {
+        c->set_error("a Verlet list exceeded its fixed capacity (511 partners per particle, 64 per particle "
+                     "on average); the step results are not valid");
+        return PS_ERR_STATE;
+    }
+    if (c->dcnt_h[DC_BARRIER_TIMEOUT]) {
+        c->set_error("the stresslet MINRES grid barrier ran out of time; the step results are not valid");
+        return PS_ERR_HIP;
+    }
+    return PS_OK;
 }
 
 void step_graphs_reset(Ctx* c) {
@@ -387,6 +414,9 @@ void destroy_ctx(Ctx* c) {
     for (hipEvent_t e : c->noise_ev)
         if (e) (void)hipEventDestroy(e);
     step_graphs_reset(c);
+    if (c->side_fork) (void)hipEventDestroy(c->side_fork);
+    if (c->side_join) (void)hipEventDestroy(c->side_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -564,6 +594,12 @@ ps_err ps_create(const ps_params* p, void** out) {
     if (hipGetDevice(&c->device) != hipSuccess) return bail(PS_ERR_HIP);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->side_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->side_join, hipEventDisableTiming) != hipSuccess) {
+        c->err = "hipStreamCreate failed";
+        return bail(PS_ERR_HIP);
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         c->err = "hipStreamCreate failed";
         return bail(PS_ERR_HIP);
@@ -699,7 +735,17 @@ ps_err ps_get_vectors(void* h, double* up, double* vdn, double* vdc, double* vb,
 ps_err ps_get_counters(void* h, ps_counters* out) {
     Ctx* c = as_ctx(h);
     if (!c || !out) return PS_ERR_ARG;
+    // the counts the kernels decide (Lanczos iterations, negative Ritz values, stresslet iterations)
+    // accumulate on the device
+    PS_HIP(c, hipMemcpyAsync(c->dcnt_h, c->dcnt, sizeof(int64_t) * DC_N, hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    if (!c->arrow && c->have_pd) PS_TRY(vlist_count(c, nullptr));
     *out = c->cnt;
+    out->lanczos_iterations += c->dcnt_h[DC_LANCZOS_ITERS];
+    out->lanczos_nneg_events += c->dcnt_h[DC_NNEG_EVENTS];
+    out->lanczos_nneg_count += c->dcnt_h[DC_NNEG_COUNT];
+    out->stresslet_iterations = c->dcnt_h[DC_MSS_ITERS];
+    out->stresslet_capped = c->dcnt_h[DC_MSS_CAPPED];
     return PS_OK;
 }
 
@@ -708,10 +754,13 @@ ps_err ps_check_dist(void* h, int64_t* n_pairs) {
     if (!c) return PS_ERR_ARG;
     if (!c->have_x) return fail(c, PS_ERR_STATE, "check_dist before initial_configuration");
     PS_TRY(check_dist_dev(c));
-    if (c->arrow) PS_TRY(arrow_count_pairs(c));
-    PS_HIP(c, hipStreamSynchronize(c->stream));
-    if (n_pairs) *n_pairs = c->nvlist;
-    return PS_OK;
+    if (c->arrow) {
+        PS_TRY(arrow_count_pairs(c));
+        PS_HIP(c, hipStreamSynchronize(c->stream));
+        if (n_pairs) *n_pairs = c->nvlist;
+        return PS_OK;
+    }
+    return vlist_count(c, n_pairs);
 }
 
 ps_err ps_get_vlist(void* h, int32_t* rows, int32_t* partners, int64_t* pidx, int64_t cap, int64_t* n_pairs) {
@@ -944,8 +993,7 @@ ps_err ps_run(void* h, int nsteps) {
     if (pipe) pipe->stop();
     step_graphs_reset(c);   // the graphs read this run's noise slots
     if (rc != PS_OK) return rc;
-    PS_HIP(c, hipStreamSynchronize(c->stream));
-    return PS_OK;
+    return check_device_flags(c);
 }
 // File name of a frame: output_dir + "/" + prefix + t + "_.txt" with t printed by a default
 // std::ostream (6 significant digits, %g style), as src/write_configuration.cpp:18-19 does.

@@ -14,6 +14,7 @@
 // modified Gram-Schmidt, tridiagonal eigendecomposition on the host (m <= 64), negative Ritz
 // values floored, relative-change stopping test.
 #include "ps_common.h"
+#include "tql2.h"
 
 #include <algorithm>
 #include <cmath>
@@ -326,20 +327,6 @@ __global__ __launch_bounds__(VB) void k_lz_sub(int64_t n, const double* __restri
         if (threadIdx.x == 0) red2[blockIdx.x] = acc;
     }
 }
-__global__ void k_scale(int64_t n, double* __restrict__ y, const double* __restrict__ x, double a) {
-    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
-    if (i < n) y[i] = a * x[i];
-}
-// out = sum_i g_i Q_i
-__global__ void k_lz_combine(int64_t n, const double* __restrict__ Q, int m, const double* __restrict__ g,
-                             double* __restrict__ out) {
-    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
-    if (i >= n) return;
-    double acc = 0.0;
-    for (int q = 0; q < m; q++) acc += g[q] * Q[(int64_t)q * n + i];
-    out[i] = acc;
-}
-
 inline unsigned eblocks(int64_t n) { return (unsigned)((n + VB - 1) / VB); }
 
 }  // namespace
@@ -357,9 +344,16 @@ ps_err krylov_alloc(Ctx* c) {
     if ((int64_t)c->kmax > n) c->kmax = (int)std::max<int64_t>(1, n);
     PS_HIP(c, hipMalloc(&c->Q, sizeof(double) * (size_t)c->kmax * (size_t)n));
     PS_HIP(c, hipMalloc(&c->lz, sizeof(double) * (3 * 64 + 8)));
+    PS_HIP(c, hipMemset(c->lz, 0, sizeof(double) * (3 * 64 + 8)));
     PS_HIP(c, hipHostMalloc(&c->lz_h, sizeof(double) * (3 * 64 + 8), hipHostMallocDefault));
-    for (int k = 0; k < 2; k++) PS_HIP(c, hipEventCreateWithFlags(&c->lz_ev[k], hipEventDisableTiming));
-    PS_HIP(c, hipMalloc(&c->gw, sizeof(double) * 64));
+    PS_HIP(c, hipMalloc(&c->lzg, sizeof(double) * 65 * 64));
+    PS_HIP(c, hipMalloc(&c->lzinfo, sizeof(double) * 4));
+    PS_HIP(c, hipMemset(c->lzinfo, 0, sizeof(double) * 4));
+    PS_HIP(c, hipMalloc(&c->dcnt, sizeof(int64_t) * DC_N));
+    PS_HIP(c, hipMemset(c->dcnt, 0, sizeof(int64_t) * DC_N));
+    PS_HIP(c, hipHostMalloc(&c->dcnt_h, sizeof(int64_t) * DC_N, hipHostMallocDefault));
+    PS_HIP(c, hipMalloc(&c->gbar, sizeof(unsigned) * 4));
+    PS_HIP(c, hipMemset(c->gbar, 0, sizeof(unsigned) * 4));
     PS_HIP(c, hipMalloc(&c->ir_r, sizeof(double) * (size_t)ns));
     PS_HIP(c, hipMalloc(&c->ir_d, sizeof(double) * (size_t)ns));
     if (!c->arrow) {   // the arrowhead diagonal is taken at the first MINRES solve (arrow.hip)
@@ -372,27 +366,24 @@ ps_err krylov_alloc(Ctx* c) {
         PS_HIP(c, hipMalloc(&c->Mss, sizeof(double) * n5 * n5));
         PS_HIP(c, hipMalloc(&c->bs_y, sizeof(double) * (size_t)n));
         PS_HIP(c, hipMalloc(&c->bs_r, sizeof(double) * n5));
-        PS_HIP(c, hipMalloc(&c->bs_x, sizeof(double) * (size_t)ns));
-        PS_HIP(c, hipMalloc(&c->ms_x, sizeof(double) * n5));
     }
     return PS_OK;
 }
 
 void krylov_free(Ctx* c) {
     for (void* p : {(void*)c->kr, (void*)c->dinv, (void*)c->red, (void*)c->mstate, (void*)c->Q, (void*)c->lz,
-                    (void*)c->gw, (void*)c->ir_r, (void*)c->ir_d, (void*)c->Mss,
-                    (void*)c->bs_y, (void*)c->bs_r, (void*)c->bs_x, (void*)c->ms_x})
+                    (void*)c->lzg, (void*)c->lzinfo, (void*)c->dcnt, (void*)c->gbar, (void*)c->ir_r, (void*)c->ir_d,
+                    (void*)c->Mss, (void*)c->bs_y, (void*)c->bs_r})
         if (p) (void)hipFree(p);
-    if (c->ms_exec) (void)hipGraphExecDestroy(c->ms_exec);
-    c->ms_exec = nullptr;
-    c->ms_x = nullptr;
-    c->Mss = c->bs_y = c->bs_r = c->bs_x = nullptr;
+    c->Mss = c->bs_y = c->bs_r = nullptr;
+    c->lzg = c->lzinfo = nullptr;
+    c->dcnt = nullptr;
+    c->gbar = nullptr;
     if (c->mstate_h) (void)hipHostFree(c->mstate_h);
     if (c->lz_h) (void)hipHostFree(c->lz_h);
-    for (int k = 0; k < 2; k++)
-        if (c->lz_ev[k]) (void)hipEventDestroy(c->lz_ev[k]);
-    c->lz_ev[0] = c->lz_ev[1] = nullptr;
-    c->kr = c->dinv = c->red = c->Q = c->lz = c->gw = c->ir_r = c->ir_d = nullptr;
+    if (c->dcnt_h) (void)hipHostFree(c->dcnt_h);
+    c->dcnt_h = nullptr;
+    c->kr = c->dinv = c->red = c->Q = c->lz = c->ir_r = c->ir_d = nullptr;
     c->mstate = nullptr;
     c->mstate_h = nullptr;
     c->lz_h = nullptr;
@@ -537,199 +528,72 @@ static ps_err minres_dev(Ctx* c, const double* b, double* x, int warm, double rt
 
 // ---------------------------------------------------------------------------- small dense MINRES
 // The stresslet system M_ss f_s = r (5Nc unknowns, dense, FP64) is solved by the same preconditioned
-// MINRES recurrence with two launches per iteration: a multi-block GEMV and one single-block kernel
-// that does all the vector work and the scalar recurrence (block-wide reductions, fixed order).
-// Vectors: r1, r2, y, v, wa (= w_{k-1}), wb (= w_{k-2}); the state lives in one MinresState.
+// MINRES recurrence in one persistent launch: every block keeps an identical copy of the scalar
+// state, does the O(5Nc) vector work of an iteration redundantly over the whole vectors (same loops,
+// same reduction tree, hence the same scalars everywhere), keeps v_{k+1} in LDS, writes its own row
+// range of the next vector set and its rows of y_{k+1} = A v_{k+1}; a grid-wide barrier separates
+// the iterations.  The vectors are double-buffered by iteration parity, so no block reads in an
+// iteration what another block writes in it.  All blocks see the same state, so they leave the loop
+// at the same iteration: at convergence or at the cap, with no host round trip.
 // Preconditioner: block Jacobi with the exact inverse of the 5x5 diagonal blocks.  Every diagonal
 // block of M_ss is the stresslet self block of fill_self (src/mob.cpp:600-612), 9/5 I with 9/10 at
 // (0,4) and (4,0), so its inverse couples entries 0 and 4 of a colloid by [[a, b], [b, a]],
 // a = (9/5)/((9/5)^2 - (9/10)^2), b = -(9/10)/((9/5)^2 - (9/10)^2), and scales entries 1..3 by 5/9.
-constexpr int MS_T = 1024;   // threads of the single-block kernels
+constexpr uint64_t GRID_WAIT_TICKS = 1000000000ull;   // 10 s of the 100 MHz wall clock
 
-__device__ __forceinline__ double ms_prec(const double* __restrict__ v, int64_t i) {
-    const double a = 1.8 / 2.43, b = -0.9 / 2.43;
-    const int r = (int)(i % 5);
-    if (r == 0) return a * v[i] + b * v[i + 4];
-    if (r == 4) return b * v[i - 4] + a * v[i];
-    return v[i] * (5.0 / 9.0);
-}
-
-__device__ __forceinline__ double block_reduce_1024(double v, double* sh) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    v = wave_sum(v);
-    __syncthreads();
-    if (lane == 0) sh[wid] = v;
-    __syncthreads();
-    double r = 0.0;
-#pragma unroll
-    for (int k = 0; k < MS_T / 64; k++) r += sh[k];
-    return r;
-}
-
-__global__ __launch_bounds__(MS_T) void k_ms_init(int64_t N, const double* __restrict__ b, double* __restrict__ x,
-                                                  double* __restrict__ r1, double* __restrict__ r2,
-                                                  double* __restrict__ y, double* __restrict__ v,
-                                                  double* __restrict__ wa, double* __restrict__ wb, double rtol,
-                                                  double abstol, MinresState* __restrict__ st) {
-    __shared__ double sh[MS_T / 64];
-    double a1 = 0.0;
-    for (int64_t i = threadIdx.x; i < N; i += MS_T) {
-        const double bi = b[i], zi = ms_prec(b, i);
-        x[i] = 0.0;
-        r1[i] = bi;
-        r2[i] = bi;
-        y[i] = zi;
-        wa[i] = 0.0;
-        wb[i] = 0.0;
-        a1 += bi * zi;
-    }
-    // cold start: r1 = b, so r1 . P^-1 r1 is also the b-norm of the stopping test
-    const double b1 = block_reduce_1024(a1, sh);
-    const double bn = b1;
-    MinresState s{};
-    s.beta1 = b1 > 0.0 ? sqrt(b1) : 0.0;
-    s.tol = fmax(rtol * sqrt(fmax(bn, 0.0)), abstol);
-    s.beta = s.beta1;
-    s.phibar = s.beta1;
-    s.cs = -1.0;
-    if (b1 <= 0.0) {
-        s.done = 1;
-        s.reason = 3;
-    } else if (s.phibar <= s.tol) {
-        s.done = 1;
-        s.reason = 2;
-    }
-    if (!s.done) {
-        const double sc = 1.0 / s.beta;
-        for (int64_t i = threadIdx.x; i < N; i += MS_T) v[i] = sc * y[i];
-    }
-    if (threadIdx.x == 0) *st = s;
-}
-
-// y = A v, one wave per row, four independent accumulators per lane.
-__global__ __launch_bounds__(VB) void k_ms_gemv(int64_t N, const double* __restrict__ A, const double* __restrict__ v,
-                                                double* __restrict__ y, const MinresState* __restrict__ st) {
-    if (st->done) return;
-    const int64_t row = (int64_t)blockIdx.x * (VB / 64) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= N) return;
-    const double* a = A + row * N;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int64_t j = lane;
-    for (; j + 192 < N; j += 256) {
-        s0 += a[j] * v[j];
-        s1 += a[j + 64] * v[j + 64];
-        s2 += a[j + 128] * v[j + 128];
-        s3 += a[j + 192] * v[j + 192];
-    }
-    for (; j < N; j += 64) s0 += a[j] * v[j];
-    const double acc = wave_sum((s0 + s1) + (s2 + s3));
-    if (lane == 0) y[row] = acc;
-}
-
-// One MINRES iteration per launch (Paige & Saunders; the recurrence of k_mr_a/b/c).  The O(N)
-// vector work is small, so every block does it redundantly over the whole vectors (same loops, same
-// reduction tree, hence the same scalars in every block), keeps v_{k+1} in LDS, writes its own row
-// range of the next vector set, and computes its rows of y_{k+1} = A v_{k+1}.  The vectors and the
-// state are double-buffered by iteration parity (`cur` is read, `nxt` written), so no block reads a
-// value another block of the same launch writes.
 struct MsSet {
     double *r1, *r2, *y, *v, *wa, *wb;   // wa = w_{k-1}, wb = w_{k-2}; y = A v
 };
 
-__global__ __launch_bounds__(VB) void k_ms_iter(int64_t N, const double* __restrict__ A, MsSet cur, MsSet nxt,
-                                                double* __restrict__ x, int rows,
-                                                const MinresState* __restrict__ st_cur,
-                                                MinresState* __restrict__ st_nxt, int maxit, double abstol) {
-    extern __shared__ double V[];   // [N]: z, then v_{k+1}
-    __shared__ double sh[4];
-    MinresState s = *st_cur;
-    if (s.done) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) *st_nxt = s;
-        return;
-    }
-    // y -= (beta/oldb) r1 (from the 2nd iteration); alfa = v . y
-    const bool sub = s.its >= 1;
-    const double f = sub ? s.beta / s.oldb : 0.0;
-    double acc = 0.0;
-    for (int64_t i = threadIdx.x; i < N; i += VB) {
-        double ya = cur.y[i];
-        if (sub) ya -= f * cur.r1[i];
-        acc += cur.v[i] * ya;
-    }
-    const double alfa = block_reduce(acc, sh);
-    // r2_new = y - (alfa/beta) r2; z = P^-1 r2_new per colloid block; beta_new^2 = r2_new . z
-    const double f2 = alfa / s.beta;
-    const double pa = 1.8 / 2.43, pb = -0.9 / 2.43;
-    acc = 0.0;
-    for (int64_t cb = threadIdx.x; cb < N / 5; cb += VB) {
-        double r[5];
-#pragma unroll
-        for (int k = 0; k < 5; k++) {
-            const int64_t i = 5 * cb + k;
-            double ya = cur.y[i];
-            if (sub) ya -= f * cur.r1[i];
-            r[k] = ya - f2 * cur.r2[i];
-        }
-        const double z[5] = {pa * r[0] + pb * r[4], r[1] * (5.0 / 9.0), r[2] * (5.0 / 9.0), r[3] * (5.0 / 9.0),
-                             pb * r[0] + pa * r[4]};
-#pragma unroll
-        for (int k = 0; k < 5; k++) {
-            V[5 * cb + k] = z[k];
-            acc += r[k] * z[k];
-        }
-    }
-    const double bb = block_reduce(acc, sh);
-    s.alfa = alfa;
-    s.oldb = s.beta;
-    s.beta = sqrt(fmax(bb, 0.0));
-    const double oldeps = s.epsln;
-    const double delta = s.cs * s.dbar + s.sn * alfa;
-    const double gbar = s.sn * s.dbar - s.cs * alfa;
-    s.epsln = s.sn * s.beta;
-    s.dbar = -s.cs * s.beta;
-    const double gamma = fmax(sqrt(gbar * gbar + s.beta * s.beta), 1e-300);
-    s.cs = gbar / gamma;
-    s.sn = s.beta / gamma;
-    const double phi = s.cs * s.phibar;
-    s.phibar = s.sn * s.phibar;
-    s.oldeps = oldeps;
-    s.delta = delta;
-    s.gamma = gamma;
-    s.phi = phi;
-    s.its += 1;
-    if (s.phibar <= s.tol) {
-        s.done = 1;
-        s.reason = (s.phibar <= abstol) ? 3 : 2;
-    } else if (s.beta == 0.0) {
-        s.done = 1;
-        s.reason = 2;
-    } else if (s.its >= maxit) {
-        s.done = 1;
-        s.reason = -3;
-    }
-    const double denom = 1.0 / gamma;
-    const double vs = (s.beta > 0.0) ? 1.0 / s.beta : 0.0;
-    // V holds z; each thread scales the entries it wrote (colloid blocks cb = threadIdx.x + k VB)
-    for (int64_t cb = threadIdx.x; cb < N / 5; cb += VB)
-#pragma unroll
-        for (int k = 0; k < 5; k++) V[5 * cb + k] *= vs;
+// Barrier over the whole grid (every block co-resident: the grid is at most one block per CU).  Each
+// wave waits for its own stores, the block meets, one thread arrives with an agent-scope release and
+// waits for the generation to move on, bounded by the wall clock; a timeout sets the sticky error
+// word, which ends every other block's wait at once.  Returns false after a timeout.
+__device__ bool grid_sync(unsigned* gb, unsigned nblocks, unsigned& gen) {
+    __shared__ int ok_sh;
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    // own rows: w_k = (v - oldeps w_{k-2} - delta w_{k-1}) / gamma, x += phi w_k, shift the vectors
-    const int64_t rb = (int64_t)blockIdx.x * rows, re = min(N, rb + rows);
-    for (int64_t i = rb + threadIdx.x; i < re; i += VB) {
-        double ya = cur.y[i];
-        if (sub) ya -= f * cur.r1[i];
-        const double w1 = cur.wa[i];
-        const double wn = (cur.v[i] - oldeps * cur.wb[i] - delta * w1) * denom;
-        nxt.wb[i] = w1;
-        nxt.wa[i] = wn;
-        x[i] += phi * wn;
-        nxt.r1[i] = cur.r2[i];
-        nxt.r2[i] = ya - f2 * cur.r2[i];
-        nxt.v[i] = V[i];
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const unsigned target = gen + 1;
+        if (__hip_atomic_fetch_add(&gb[0], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == nblocks - 1) {
+            __hip_atomic_store(&gb[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&gb[1], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const uint64_t t0 = wall_clock64();
+            while ((int)(__hip_atomic_load(&gb[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+                if (__hip_atomic_load(&gb[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    ok = 0;
+                    break;
+                }
+                if (wall_clock64() - t0 > GRID_WAIT_TICKS) {
+                    __hip_atomic_store(&gb[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        ok_sh = ok;
     }
-    // y_{k+1} = A v_{k+1} for the own rows, one wave per row
+    __syncthreads();
+    gen++;
+    return ok_sh != 0;
+}
+
+__device__ __forceinline__ void ms_prec5(const double (&r)[5], double (&z)[5]) {
+    const double pa = 1.8 / 2.43, pb = -0.9 / 2.43;
+    z[0] = pa * r[0] + pb * r[4];
+    z[1] = r[1] * (5.0 / 9.0);
+    z[2] = r[2] * (5.0 / 9.0);
+    z[3] = r[3] * (5.0 / 9.0);
+    z[4] = pb * r[0] + pa * r[4];
+}
+
+// Rows [rb, re) of y = A V, one wave per row, four independent accumulators per lane.
+__device__ __forceinline__ void ms_gemv_rows(int64_t N, const double* __restrict__ A, const double* V, int64_t rb,
+                                             int64_t re, double* __restrict__ y) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int64_t row = rb + wid; row < re; row += VB / 64) {
         const double* a = A + row * N;
@@ -743,87 +607,203 @@ __global__ __launch_bounds__(VB) void k_ms_iter(int64_t N, const double* __restr
         }
         for (; j < N; j += 64) s0 += a[j] * V[j];
         const double t = wave_sum((s0 + s1) + (s2 + s3));
-        if (lane == 0) nxt.y[row] = t;
+        if (lane == 0) y[row] = t;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) *st_nxt = s;
 }
 
-// Solve A x = b for the small dense SPD-like block A (N x N, row-major) by the recurrence above,
-// polling the state every MS_POLL iterations (the kernels are no-ops once it has converged).
-// The polling group (MS_POLL launches of k_ms_iter, parities 0, 1, 0, ...) is captured once as a
-// HIP graph on fixed buffers and replayed: the iterations are a few microseconds each, so separate
-// launches would leave the device waiting on the host.  The solution accumulates in c->ms_x.
+__global__ __launch_bounds__(VB) void k_ms_solve(int64_t N, const double* __restrict__ A, const double* __restrict__ b,
+                                                 double* __restrict__ x, MsSet set0, MsSet set1, int rows,
+                                                 double rtol, double abstol, int maxit, unsigned* gb,
+                                                 MinresState* __restrict__ st_out, int64_t* __restrict__ dcnt) {
+    extern __shared__ double V[];   // [N]: z, then v_{k+1}
+    __shared__ double sh[4];
+    if (__hip_atomic_load(&gb[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    unsigned gen = __hip_atomic_load(&gb[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned nbk = gridDim.x;
+    const int64_t rb = (int64_t)blockIdx.x * rows, re = min(N, rb + rows);
+    const int64_t ncb = N / 5;
+    // start (cold): x = 0, r1 = r2 = b, z = P^-1 b, beta1^2 = b . z (also the b-norm of the test)
+    double acc = 0.0;
+    for (int64_t cb = threadIdx.x; cb < ncb; cb += VB) {
+        double r[5], z[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) r[k] = b[5 * cb + k];
+        ms_prec5(r, z);
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            V[5 * cb + k] = z[k];
+            acc += r[k] * z[k];
+        }
+    }
+    const double b1 = block_reduce(acc, sh);
+    MinresState s{};
+    s.beta1 = b1 > 0.0 ? sqrt(b1) : 0.0;
+    s.tol = fmax(rtol * sqrt(fmax(b1, 0.0)), abstol);
+    s.beta = s.beta1;
+    s.phibar = s.beta1;
+    s.cs = -1.0;
+    if (b1 <= 0.0) {
+        s.done = 1;
+        s.reason = 3;
+    } else if (s.phibar <= s.tol) {
+        s.done = 1;
+        s.reason = 2;
+    }
+    const double sc0 = s.done ? 0.0 : 1.0 / s.beta;
+    for (int64_t cb = threadIdx.x; cb < ncb; cb += VB)
+#pragma unroll
+        for (int k = 0; k < 5; k++) V[5 * cb + k] *= sc0;
+    __syncthreads();
+    for (int64_t i = rb + threadIdx.x; i < re; i += VB) {
+        x[i] = 0.0;
+        set0.r1[i] = b[i];
+        set0.r2[i] = b[i];
+        set0.wa[i] = 0.0;
+        set0.wb[i] = 0.0;
+        set0.v[i] = V[i];
+    }
+    bool ok = true;
+    if (!s.done) {
+        ms_gemv_rows(N, A, V, rb, re, set0.y);
+        ok = grid_sync(gb, nbk, gen);
+    }
+    int par = 0;
+    while (ok && !s.done) {
+        const MsSet cur = par ? set1 : set0;
+        const MsSet nxt = par ? set0 : set1;
+        // y -= (beta/oldb) r1 (from the 2nd iteration); alfa = v . y
+        const bool sub = s.its >= 1;
+        const double f = sub ? s.beta / s.oldb : 0.0;
+        acc = 0.0;
+        for (int64_t i = threadIdx.x; i < N; i += VB) {
+            double ya = cur.y[i];
+            if (sub) ya -= f * cur.r1[i];
+            acc += cur.v[i] * ya;
+        }
+        const double alfa = block_reduce(acc, sh);
+        // r2_new = y - (alfa/beta) r2; z = P^-1 r2_new per colloid block; beta_new^2 = r2_new . z
+        const double f2 = alfa / s.beta;
+        acc = 0.0;
+        for (int64_t cb = threadIdx.x; cb < ncb; cb += VB) {
+            double r[5], z[5];
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                const int64_t i = 5 * cb + k;
+                double ya = cur.y[i];
+                if (sub) ya -= f * cur.r1[i];
+                r[k] = ya - f2 * cur.r2[i];
+            }
+            ms_prec5(r, z);
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                V[5 * cb + k] = z[k];
+                acc += r[k] * z[k];
+            }
+        }
+        const double bb = block_reduce(acc, sh);
+        s.alfa = alfa;
+        s.oldb = s.beta;
+        s.beta = sqrt(fmax(bb, 0.0));
+        const double oldeps = s.epsln;
+        const double delta = s.cs * s.dbar + s.sn * alfa;
+        const double gbar = s.sn * s.dbar - s.cs * alfa;
+        s.epsln = s.sn * s.beta;
+        s.dbar = -s.cs * s.beta;
+        const double gamma = fmax(sqrt(gbar * gbar + s.beta * s.beta), 1e-300);
+        s.cs = gbar / gamma;
+        s.sn = s.beta / gamma;
+        const double phi = s.cs * s.phibar;
+        s.phibar = s.sn * s.phibar;
+        s.oldeps = oldeps;
+        s.delta = delta;
+        s.gamma = gamma;
+        s.phi = phi;
+        s.its += 1;
+        if (s.phibar <= s.tol) {
+            s.done = 1;
+            s.reason = (s.phibar <= abstol) ? 3 : 2;
+        } else if (s.beta == 0.0) {
+            s.done = 1;
+            s.reason = 2;
+        } else if (s.its >= maxit) {
+            s.done = 1;
+            s.reason = -3;
+        }
+        const double denom = 1.0 / gamma;
+        const double vs = (s.beta > 0.0) ? 1.0 / s.beta : 0.0;
+        // V holds z; each thread scales the entries it wrote
+        for (int64_t cb = threadIdx.x; cb < ncb; cb += VB)
+#pragma unroll
+            for (int k = 0; k < 5; k++) V[5 * cb + k] *= vs;
+        __syncthreads();
+        // own rows: w_k = (v - oldeps w_{k-2} - delta w_{k-1}) / gamma, x += phi w_k, shift the vectors
+        for (int64_t i = rb + threadIdx.x; i < re; i += VB) {
+            double ya = cur.y[i];
+            if (sub) ya -= f * cur.r1[i];
+            const double w1 = cur.wa[i];
+            const double wn = (cur.v[i] - oldeps * cur.wb[i] - delta * w1) * denom;
+            nxt.wb[i] = w1;
+            nxt.wa[i] = wn;
+            x[i] += phi * wn;
+            nxt.r1[i] = cur.r2[i];
+            nxt.r2[i] = ya - f2 * cur.r2[i];
+            nxt.v[i] = V[i];
+        }
+        par ^= 1;
+        if (s.done) break;
+        ms_gemv_rows(N, A, V, rb, re, nxt.y);
+        ok = grid_sync(gb, nbk, gen);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (!ok) {
+            s.done = 1;
+            s.reason = -5;   // KSP_DIVERGED_BREAKDOWN's sign: the wait ran out of time
+            dcnt[DC_BARRIER_TIMEOUT] = 1;
+        }
+        *st_out = s;
+        dcnt[DC_MSS_ITERS] += s.its;
+        if (s.reason < 0) dcnt[DC_MSS_CAPPED] += 1;
+    }
+}
+
+// Solve A x = b for the small dense block A (N x N, row-major) by the recurrence above in one launch.
+// With `info` the final state is read back (a host round trip); without, nothing is.
 static ps_err minres_small(Ctx* c, int64_t N, const double* A, const double* b, double* x, double rtol, int maxit,
                            ps_solve_info* info) {
     hipStream_t st = c->stream;
-    if (N > 8192) {   // v_{k+1} of k_ms_iter lives in 64 KiB of LDS
+    if (N > 8192 || N % 5 != 0) {   // v_{k+1} lives in 64 KiB of LDS; the rows come in colloid blocks
         c->set_error("block solver: 5 Nc = " + std::to_string(N) + " stresslet unknowns exceed 8192");
         return PS_ERR_ARG;
     }
-    constexpr int MS_POLL = 8;   // even: a group leaves the parity where it found it
     MsSet set[2];
     for (int k = 0; k < 2; k++) {
         double* base = c->kr + (size_t)k * 6 * N;
         set[k] = MsSet{base, base + N, base + 2 * N, base + 3 * N, base + 4 * N, base + 5 * N};
     }
-    MinresState* sd = c->mstate;   // two slots, by parity
-    double* xs = c->ms_x;
-    const int rows = (int)std::max<int64_t>(8, (N + 255) / 256);
+    // at most one block per CU, so the whole grid is co-resident for its barriers
+    const int ncu = std::max(1, c->n_cu);
+    int rows = (int)std::max<int64_t>(8, (N + 255) / 256);
+    while ((N + rows - 1) / rows > ncu) rows *= 2;
     const unsigned nb = (unsigned)((N + rows - 1) / rows);
     const size_t lds = sizeof(double) * (size_t)N;
-    auto launch_group = [&]() -> ps_err {
-        for (int k = 0; k < MS_POLL; k++) {
-            const int p = k & 1;
-            k_ms_iter<<<nb, VB, lds, st>>>(N, A, set[p], set[p ^ 1], xs, rows, sd + p, sd + (p ^ 1), maxit,
-                                           c->p.ksp_abstol);
-            PS_LAUNCH(c);
-        }
-        return PS_OK;
-    };
-    if (!c->ms_exec) {
-        hipGraph_t gr = nullptr;
-        PS_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        const ps_err r = launch_group();
-        const hipError_t e = hipStreamEndCapture(st, &gr);
-        if (r != PS_OK) return r;
-        if (e != hipSuccess) {
-            c->set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-            return PS_ERR_HIP;
-        }
-        const hipError_t e2 = hipGraphInstantiate(&c->ms_exec, gr, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(gr);
-        if (e2 != hipSuccess) {
-            c->ms_exec = nullptr;
-            c->set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
-            return PS_ERR_HIP;
-        }
-    }
-    k_ms_init<<<1, MS_T, 0, st>>>(N, b, xs, set[0].r1, set[0].r2, set[0].y, set[0].v, set[0].wa, set[0].wb, rtol,
-                                   c->p.ksp_abstol, sd);
+    k_ms_solve<<<nb, VB, lds, st>>>(N, A, b, x, set[0], set[1], rows, rtol, c->p.ksp_abstol, maxit, c->gbar,
+                                     c->mstate, c->dcnt);
     PS_LAUNCH(c);
-    k_ms_gemv<<<(unsigned)((N + 3) / 4), VB, 0, st>>>(N, A, set[0].v, set[0].y, sd);
-    PS_LAUNCH(c);
-    int it = 0;
-    for (;;) {
-        // after a whole number of groups the current state is in slot 0
-        PS_HIP(c, hipMemcpyAsync(c->mstate_h, sd, sizeof(MinresState), hipMemcpyDeviceToHost, st));
-        PS_HIP(c, hipStreamSynchronize(st));
-        if (c->mstate_h[0].done || it >= maxit) break;
-        PS_HIP(c, hipGraphLaunch(c->ms_exec, st));
-        it += MS_POLL;
-    }
-    PS_HIP(c, hipMemcpyAsync(x, xs, sizeof(double) * (size_t)N, hipMemcpyDeviceToDevice, st));
+    if (!info) return PS_OK;
+    PS_HIP(c, hipMemcpyAsync(c->mstate_h, c->mstate, sizeof(MinresState), hipMemcpyDeviceToHost, st));
+    PS_HIP(c, hipStreamSynchronize(st));
     const MinresState& s = c->mstate_h[0];
-    const int reason = s.done ? s.reason : -3;
-    if (info) {
-        *info = ps_solve_info{};
-        info->iterations = s.its;
-        info->reason = reason;
-        info->rnorm = s.phibar;
-        info->rnorm0 = s.beta1;
-        info->true_rel_residual = -1.0;
+    if (s.reason == -5) {
+        c->set_error("stresslet MINRES: a grid-wide wait ran out of time");
+        return PS_ERR_HIP;
     }
-    return reason > 0 ? PS_OK : PS_ERR_NOCONV;
+    *info = ps_solve_info{};
+    info->iterations = s.its;
+    info->reason = s.reason;
+    info->rnorm = s.phibar;
+    info->rnorm0 = s.beta1;
+    info->true_rel_residual = -1.0;
+    return s.reason > 0 ? PS_OK : PS_ERR_NOCONV;
 }
 
 // x = [-b_lambda (nu) ; 0 (5Nc)]  (f_u from the constraint rows B^T f = b_lambda)
@@ -849,7 +829,7 @@ __global__ void k_bs_u(double* __restrict__ x, const double* __restrict__ y, con
 //   f_u = -b[n:n+nu];  M_ss f_s = b_s - (M [f_u; 0])_s;  u = (M f)_u - b_u.
 // With one colloid and monomers, the arrowhead regime runs the whole elimination in two DOF passes
 // (arrow_block_solve), which also write u to `uout` when given (*u_done is then set).
-static ps_err block_solve_once(Ctx* c, const double* b, double* x, int prec, int* mss_its,
+static ps_err block_solve_once(Ctx* c, const double* b, double* x, int prec, ps_solve_info* mss_info,
                                double* uout = nullptr, bool* u_done = nullptr) {
     const int64_t n = c->n, nu = c->nu, n5 = 5 * (int64_t)c->Nc;
     hipStream_t st = c->stream;
@@ -871,9 +851,9 @@ static ps_err block_solve_once(Ctx* c, const double* b, double* x, int prec, int
         k_bs_rs<<<eblocks(n5), VB, 0, st>>>(c->bs_r, b, c->bs_y, nu, n5);
         PS_LAUNCH(c);
         ps_solve_info si{};
-        ps_err r = minres_small(c, n5, c->Mss, c->bs_r, x + nu, 1e-12, 10000, &si);
+        ps_err r = minres_small(c, n5, c->Mss, c->bs_r, x + nu, 1e-12, 10000, mss_info ? &si : nullptr);
         if (r != PS_OK && r != PS_ERR_NOCONV) return r;
-        if (mss_its) *mss_its += si.iterations;
+        if (mss_info) mss_info->iterations += si.iterations;
     }
     PS_TRY(mob_apply(c, x, c->bs_y, prec));
     k_bs_u<<<eblocks(nu), VB, 0, st>>>(x, c->bs_y, b, n, nu);
@@ -899,16 +879,18 @@ ps_err saddle_solve_dev(Ctx* c, const double* b, double* x, int warm, ps_solve_i
 static ps_err saddle_solve_into(Ctx* c, const double* b, double* x, int warm, ps_solve_info* info, double* uout,
                                 bool* u_done) {
     const bool block = c->p.solver == PS_SOLVER_BLOCK;
-    // Block elimination costs one full apply per solve; that apply stays FP64 in every form, since
-    // an FP32 apply followed by an FP64 residual would cost more than the FP64 apply itself.
-    if (c->p.precision != PS_PREC_MIXED || block) {
+    const bool mixed = c->p.precision == PS_PREC_MIXED;
+    // Block elimination costs one full apply per solve.  In the mixed form that apply runs on FP32
+    // tiles; without refinement (ir_maxit = 0) the solve is exact up to that apply's rounding, and
+    // with it every sweep adds an FP64 residual apply.
+    if (!mixed || (block && c->p.ir_maxit <= 0)) {
         if (!block) return minres_dev(c, b, x, warm, c->p.ksp_rtol, c->p.ksp_maxit, PS_PREC_FP64, info);
-        int its = 0;
-        PS_TRY(block_solve_once(c, b, x, PS_PREC_FP64, &its, uout, u_done));
-        c->last_block_its = its;
+        // the stresslet solve's iteration count is read back only when asked for (a host round trip)
+        ps_solve_info si{};
+        PS_TRY(block_solve_once(c, b, x, c->p.precision, info ? &si : nullptr, uout, u_done));
         if (info) {
             *info = ps_solve_info{};
-            info->iterations = its;
+            info->iterations = si.iterations;
             info->reason = 2;
             info->true_rel_residual = -1.0;
         }
@@ -922,9 +904,10 @@ static ps_err saddle_solve_into(Ctx* c, const double* b, double* x, int warm, ps
     int sweeps = 0, its = 0;
     double rel;
     double rn2 = 0.0;
+    ps_solve_info si{};
     if (block && !warm) {
         // first sweep straight from b
-        PS_TRY(block_solve_once(c, b, x, PS_PREC_MIXED, &its));
+        PS_TRY(block_solve_once(c, b, x, PS_PREC_MIXED, &si));
         sweeps = 1;
     } else if (!warm) {
         PS_HIP(c, hipMemsetAsync(x, 0, sizeof(double) * (size_t)N, c->stream));
@@ -939,7 +922,7 @@ static ps_err saddle_solve_into(Ctx* c, const double* b, double* x, int warm, ps
     const double inner = std::max(c->p.ksp_rtol, 1e-4);
     while (rel > c->p.ir_rtol && sweeps < std::max(1, c->p.ir_maxit)) {
         if (block) {
-            PS_TRY(block_solve_once(c, c->ir_r, c->ir_d, PS_PREC_MIXED, &its));
+            PS_TRY(block_solve_once(c, c->ir_r, c->ir_d, PS_PREC_MIXED, &si));
         } else {
             ps_solve_info ii{};
             ps_err r = minres_dev(c, c->ir_r, c->ir_d, 0, inner, c->p.ksp_maxit, PS_PREC_MIXED, &ii);
@@ -954,6 +937,7 @@ static ps_err saddle_solve_into(Ctx* c, const double* b, double* x, int warm, ps
         sweeps++;
     }
     if (rel > c->p.ir_rtol) rc = PS_ERR_NOCONV;
+    its += si.iterations;
     c->last_minres_its = its;
     if (info) {
         info->iterations = its;
@@ -979,9 +963,132 @@ __global__ __launch_bounds__(VB) void k_lz_next(int64_t n, double* __restrict__ 
     const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
     if (i < n) q[i] = w[i] * sc;
 }
-// w = M^{1/2} xi.  The device runs the Lanczos recurrence ahead; the host reads alpha/beta every
-// LZ_POLL iterations and finds the first m where the stopping test holds (vectors built past it are
-// never used, so the result equals stopping at m).
+
+// Lanczos scalars in c->lz: alpha_j at [j], beta_j^2 at [LZ_BETA2 + j], |xi|^2 at [LZ_XI2].
+constexpr int LZ_BETA2 = 64, LZ_XI2 = 3 * 64 + 1;
+constexpr int LZ_G = 64;   // row stride of the weights table c->lzg (one row per m)
+
+// q_0 = xi / |xi| (zero when xi = 0).
+__global__ void k_lz_start(int64_t n, const double* __restrict__ xi, double* __restrict__ q0,
+                           const double* __restrict__ lz) {
+    const double b2 = lz[LZ_XI2];
+    const double sc = b2 > 0.0 ? 1.0 / sqrt(b2) : 0.0;
+    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
+    if (i < n) q0[i] = xi[i] * sc;
+}
+
+// The number of Lanczos vectors before the stopping test: the first j with beta_j <= 1e-12 beta_0
+// (an invariant subspace, src/slip_vel.cpp:417) gives m = j + 1, else m = kmax.
+__device__ __forceinline__ int lz_mlim(const double* __restrict__ lz, int kmax, double beta0) {
+    for (int j = 0; j < kmax - 1; j++)
+        if (sqrt(fmax(lz[LZ_BETA2 + j], 0.0)) <= 1e-12 * beta0) return j + 1;
+    return kmax;
+}
+
+// The square-root weights g_m = beta0 U diag(sqrt(max(theta, 0))) U^T e1 of T_m = U diag(theta) U^T
+// (src/slip_vel.cpp:427-459) for m = blockIdx.x + 1, one wave each, all m in parallel: the
+// recurrence ran ahead to kmax, and the stopping test (k_lz_final) needs the weights of every m up
+// to the one where it stops.  Without the test only the last m is solved.
+__global__ __launch_bounds__(64) void k_lz_ritz(const double* __restrict__ lz, int kmax, int every_m,
+                                               double* __restrict__ G, double* __restrict__ negs) {
+    __shared__ double d[64], Z[64 * 64];
+    const int lane = threadIdx.x;
+    const int m = blockIdx.x + 1;
+    const double beta0 = sqrt(fmax(lz[LZ_XI2], 0.0));
+    if (!(beta0 > 0.0)) return;
+    const int mlim = lz_mlim(lz, kmax, beta0);
+    if (m > mlim || (!every_m && m != mlim)) return;
+    bool finite = true;
+    for (int i = 0; i < m; i++) {
+        finite = finite && isfinite(lz[i]);
+        if (i < m - 1) finite = finite && isfinite(lz[LZ_BETA2 + i]);
+    }
+    if (!finite) {   // a broken recurrence: no QL sweeps over NaNs, the weights say so instead
+        if (lane < m) G[(m - 1) * LZ_G + lane] = __builtin_nan("");
+        if (lane == 0) negs[m - 1] = 0.0;
+        return;
+    }
+    double dv = lane < m ? lz[lane] : 0.0;
+    double ev = (lane < m - 1) ? sqrt(fmax(lz[LZ_BETA2 + lane], 0.0)) : 0.0;
+    tql2_wave(m, dv, ev, Z);
+    if (lane < m) d[lane] = dv;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    double gi = 0.0;
+    if (lane < m)
+        for (int jj = 0; jj < m; jj++) {
+            double th = d[jj];
+            if (th < 0.0) th = 0.0;
+            gi += Z[lane + jj * m] * sqrt(th) * Z[0 + jj * m];
+        }
+    int nneg = 0;
+    for (int jj = 0; jj < m; jj++)
+        if (d[jj] < 0.0) nneg++;
+    if (lane < m) G[(m - 1) * LZ_G + lane] = beta0 * gi;
+    if (lane == 0) negs[m - 1] = (double)nneg;
+}
+
+// Stopping test and recombination.  m* is the first m >= 2 whose relative change
+// |g_m - [g_{m-1}; 0]| / |g_m| is at most rtol (rtol > 0), else the m of lz_mlim; out = Q g_{m*}.
+// Every block finds m* itself (wave 0, a few hundred flops); block 0 records it, the negative Ritz
+// values and the last relative change in info and adds them to the device counters.
+__global__ __launch_bounds__(VB) void k_lz_final(int64_t n, const double* __restrict__ Q, const double* __restrict__ lz,
+                                                 int kmax, double rtol, const double* __restrict__ G,
+                                                 const double* __restrict__ negs, double* __restrict__ out,
+                                                 double* __restrict__ info, int64_t* __restrict__ dcnt) {
+    __shared__ double sg[64];
+    __shared__ int sm;
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        const double beta0 = sqrt(fmax(lz[LZ_XI2], 0.0));
+        int mstar = 0;
+        double relc = 0.0;
+        if (beta0 > 0.0) {
+            const int mlim = lz_mlim(lz, kmax, beta0);
+            mstar = mlim;
+            if (rtol > 0.0)
+                for (int m = 2; m <= mlim; m++) {
+                    const double gi = lane < m ? G[(m - 1) * LZ_G + lane] : 0.0;
+                    const double gp = lane < m - 1 ? G[(m - 2) * LZ_G + lane] : 0.0;
+                    const double dn = wave_sum((gi - gp) * (gi - gp));
+                    const double gn = wave_sum(gi * gi);
+                    relc = sqrt(dn / gn);
+                    if (relc <= rtol) {
+                        mstar = m;
+                        break;
+                    }
+                }
+        }
+        sg[lane] = (lane < mstar) ? G[(mstar - 1) * LZ_G + lane] : 0.0;
+        if (lane == 0) {
+            sm = mstar;
+            if (blockIdx.x == 0) {
+                const double nneg = mstar > 0 ? negs[mstar - 1] : 0.0;
+                info[0] = (double)mstar;
+                info[1] = nneg;
+                info[2] = relc;
+                info[3] = beta0;
+                dcnt[DC_LANCZOS_ITERS] += mstar;
+                if (nneg > 0.0) {
+                    dcnt[DC_NNEG_EVENTS] += 1;
+                    dcnt[DC_NNEG_COUNT] += (int64_t)nneg;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int m = sm;
+    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int q = 0; q < m; q++) acc += sg[q] * Q[(int64_t)q * n + i];
+    out[i] = acc;
+}
+
+// w = M^{1/2} xi by Lanczos (schur_sqrt_lanczos, src/slip_vel.cpp:351-477, applied to the whole grand
+// mobility) with no host round trip: the recurrence runs kmax iterations on the device (two
+// classical Gram-Schmidt passes each), the weights of every m come from one-wave QL solves, and the
+// stopping test picks the m the sequential test would have stopped at, so the result equals
+// stopping there; vectors built past it are never used.  With `info` the outcome is read back.
 ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* out, ps_sqrt_info* info) {
     const int64_t n = c->n;
     if (c->arrow) {
@@ -990,187 +1097,49 @@ ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* out, ps_sqrt_info* info)
     }
     const unsigned nch = (unsigned)((n + LZ_CH - 1) / LZ_CH);
     const unsigned ne = eblocks(n);
+    const int nbd = nblocks(n);
     double* red2 = c->red + (size_t)nch * 64;   // after the dot partials of up to 64 vectors
     hipStream_t st = c->stream;
-    double b2 = 0.0;
-    PS_TRY(vec_dot(c, xi, xi, n, &b2));
-    const double beta0 = std::sqrt(b2);
-    PS_HIP(c, hipMemsetAsync(out, 0, sizeof(double) * (size_t)n, st));
-    if (info) {
-        info->iterations = 0;
-        info->nneg = 0;
-        info->beta0 = beta0;
-        info->rel_change = 0.0;
-    }
-    c->cnt.sqrt_calls++;
-    if (beta0 == 0.0) return PS_OK;
     const int kmax = c->kmax;
     const int prec = c->p.precision;
-    const int LZ_POLL = 8;
     double* w = c->kr;   // MINRES scratch is free during the square root
-    std::vector<double> alpha, betad, g, gprev;
-    k_scale<<<eblocks(n), VB, 0, st>>>(n, c->Q, xi, 1.0 / beta0);
+    c->cnt.sqrt_calls++;
+    k_dot_part<<<nbd, VB, 0, st>>>(xi, xi, n, c->red);
     PS_LAUNCH(c);
-    // Iterations run in groups of LZ_POLL; group g + 1 is enqueued before the host reads group g
-    // and evaluates the stopping test, so the host's tridiagonal work overlaps the device.  If the
-    // test stops inside group g, the vectors group g + 1 builds are never used (the result equals
-    // stopping at m).
-    auto enqueue_group = [&](int g0, int g1, hipEvent_t ev) -> ps_err {
-        for (int j = g0; j < g1; j++) {
-            // w = M q_j; two classical Gram-Schmidt passes against q_0..q_j (the first one also
-            // yields alpha_j = q_j . M q_j); beta_j = |w|; q_{j+1} = w / beta_j.
-            const double* qj = c->Q + (int64_t)j * n;
-            PS_TRY(mob_apply(c, qj, w, prec));
-            for (int pass = 0; pass < 2; pass++) {
-                k_lz_dots<<<dim3(nch, j + 1), VB, 0, st>>>(n, c->Q, w, c->red);
-                PS_LAUNCH(c);
-                k_lz_sub<<<ne, VB, 0, st>>>(n, c->Q, j + 1, w, c->red, nch, pass == 0 ? c->lz + j : nullptr,
-                                            pass == 1 ? red2 : nullptr);
-                PS_LAUNCH(c);
-            }
-            k_lz_next<<<ne, VB, 0, st>>>(n, j + 1 < kmax ? c->Q + (int64_t)(j + 1) * n : nullptr, w, red2,
-                                         (int)ne, c->lz + 64 + j);
+    k_sum_to<<<1, VB, 0, st>>>(c->red, nbd, c->lz + LZ_XI2);
+    PS_LAUNCH(c);
+    k_lz_start<<<ne, VB, 0, st>>>(n, xi, c->Q, c->lz);
+    PS_LAUNCH(c);
+    for (int j = 0; j < kmax; j++) {
+        // w = M q_j; two classical Gram-Schmidt passes against q_0..q_j (the first one also yields
+        // alpha_j = q_j . M q_j); beta_j = |w|; q_{j+1} = w / beta_j.
+        const double* qj = c->Q + (int64_t)j * n;
+        PS_TRY(mob_apply(c, qj, w, prec));
+        for (int pass = 0; pass < 2; pass++) {
+            k_lz_dots<<<dim3(nch, j + 1), VB, 0, st>>>(n, c->Q, w, c->red);
+            PS_LAUNCH(c);
+            k_lz_sub<<<ne, VB, 0, st>>>(n, c->Q, j + 1, w, c->red, nch, pass == 0 ? c->lz + j : nullptr,
+                                        pass == 1 ? red2 : nullptr);
             PS_LAUNCH(c);
         }
-        // this group's alpha and beta^2 only: the slices of the groups never overlap
-        PS_HIP(c, hipMemcpyAsync(c->lz_h + g0, c->lz + g0, sizeof(double) * (g1 - g0), hipMemcpyDeviceToHost, st));
-        PS_HIP(c, hipMemcpyAsync(c->lz_h + 64 + g0, c->lz + 64 + g0, sizeof(double) * (g1 - g0),
-                                 hipMemcpyDeviceToHost, st));
-        PS_HIP(c, hipEventRecord(ev, st));
-        return PS_OK;
-    };
-    int m = 0, last_nneg = 0;
-    bool have_g = false;
-    double relc = 0.0;
-    bool stop = false;
-    // groups of LZ_POLL, halving towards kmax so that the group whose test runs with nothing left to
-    // overlap is a single iteration
-    auto group_end = [&](int g0) { return g0 + std::max(1, std::min(LZ_POLL, (kmax - g0) / 2)); };
-    int j0 = 0, j1 = group_end(0), grp = 0;
-    PS_TRY(enqueue_group(j0, j1, c->lz_ev[0]));
-    while (!stop) {
-        const int n0 = j1, n1 = n0 < kmax ? group_end(n0) : n0;
-        const bool more = n0 < kmax;
-        if (more) PS_TRY(enqueue_group(n0, n1, c->lz_ev[(grp + 1) & 1]));
-        PS_HIP(c, hipEventSynchronize(c->lz_ev[grp & 1]));
-        for (int j = j0; j < j1 && !stop; j++) {
-            alpha.push_back(c->lz_h[j]);
-            const double bj = std::sqrt(std::max(c->lz_h[64 + j], 0.0));
-            m = j + 1;
-            c->cnt.lanczos_iterations++;
-            if (c->p.sqrt_rtol > 0.0) {
-                g.assign(m, 0.0);
-                last_nneg = tridiag_sqrt_weights(m, alpha.data(), betad.data(), beta0, g.data());
-                have_g = true;
-                if (m >= 2) {
-                    double dn = 0.0, gn = 0.0;
-                    for (int i = 0; i < m; i++) {
-                        const double gp = (i < m - 1) ? gprev[i] : 0.0;
-                        dn += (g[i] - gp) * (g[i] - gp);
-                        gn += g[i] * g[i];
-                    }
-                    relc = std::sqrt(dn / gn);
-                    if (relc <= c->p.sqrt_rtol) stop = true;
-                }
-                gprev = g;
-            }
-            if (stop) break;
-            if (bj <= 1e-12 * beta0 || j == kmax - 1) {
-                stop = true;
-                break;
-            }
-            betad.push_back(bj);
-        }
-        if (!more) break;
-        j0 = n0;
-        j1 = n1;
-        grp++;
+        k_lz_next<<<ne, VB, 0, st>>>(n, j + 1 < kmax ? c->Q + (int64_t)(j + 1) * n : nullptr, w, red2, (int)ne,
+                                     c->lz + LZ_BETA2 + j);
+        PS_LAUNCH(c);
     }
-    // the stopping test already produced the weights of the final m
-    if (!(have_g && (int)g.size() == m)) {
-        g.assign(m, 0.0);
-        last_nneg = tridiag_sqrt_weights(m, alpha.data(), betad.data(), beta0, g.data());
-    }
-    const int nneg = last_nneg;
-    if (nneg > 0) {
-        c->cnt.lanczos_nneg_events++;
-        c->cnt.lanczos_nneg_count += nneg;
-    }
-    PS_HIP(c, hipMemcpyAsync(c->gw, g.data(), sizeof(double) * m, hipMemcpyHostToDevice, st));
-    k_lz_combine<<<eblocks(n), VB, 0, st>>>(n, c->Q, m, c->gw, out);
+    const double rtol = c->p.sqrt_rtol;
+    double* negs = c->lzg + LZ_G * 64;
+    k_lz_ritz<<<kmax, 64, 0, st>>>(c->lz, kmax, rtol > 0.0 ? 1 : 0, c->lzg, negs);
     PS_LAUNCH(c);
-    PS_HIP(c, hipStreamSynchronize(st));   // g lives on the host stack
+    k_lz_final<<<ne, VB, 0, st>>>(n, c->Q, c->lz, kmax, rtol, c->lzg, negs, out, c->lzinfo, c->dcnt);
+    PS_LAUNCH(c);
     if (info) {
-        info->iterations = m;
-        info->nneg = nneg;
-        info->rel_change = relc;
+        double h[4];
+        PS_HIP(c, hipMemcpyAsync(h, c->lzinfo, sizeof(h), hipMemcpyDeviceToHost, st));
+        PS_HIP(c, hipStreamSynchronize(st));
+        info->iterations = (int)h[0];
+        info->nneg = (int)h[1];
+        info->rel_change = h[2];
+        info->beta0 = h[3];
     }
     return PS_OK;
-}
-
-// ---------------------------------------------------------------------------- tridiagonal sqrt
-// Implicit QL with shifts on the symmetric tridiagonal T (EISPACK tql2, Bowdler, Martin, Reinsch &
-// Wilkinson 1968), eigenvectors accumulated; then g = beta0 U diag(sqrt(max(theta,0))) U^T e1
-// (src/slip_vel.cpp:427-459).  Returns the number of negative Ritz values.
-int tridiag_sqrt_weights(int m, const double* alpha, const double* betad, double beta0, double* g) {
-    std::vector<double> d(alpha, alpha + m), e(std::max(m, 1), 0.0), Z((size_t)m * m, 0.0);
-    for (int i = 0; i + 1 < m; i++) e[i] = betad[i];
-    for (int i = 0; i < m; i++) Z[i + (size_t)i * m] = 1.0;
-    for (int l = 0; l < m; l++) {
-        int iter = 0, mm;
-        do {
-            for (mm = l; mm < m - 1; mm++) {
-                const double dd = std::fabs(d[mm]) + std::fabs(d[mm + 1]);
-                if (std::fabs(e[mm]) <= 1e-300 + 2.2e-16 * dd) break;
-            }
-            if (mm != l) {
-                if (iter++ == 200) break;
-                double gg = (d[l + 1] - d[l]) / (2.0 * e[l]);
-                double r = std::sqrt(gg * gg + 1.0);
-                gg = d[mm] - d[l] + e[l] / (gg + (gg >= 0 ? std::fabs(r) : -std::fabs(r)));
-                double s = 1.0, cc = 1.0, p = 0.0;
-                bool underflow = false;
-                for (int i = mm - 1; i >= l; i--) {
-                    const double f = s * e[i], b = cc * e[i];
-                    r = std::sqrt(f * f + gg * gg);
-                    e[i + 1] = r;
-                    if (r == 0.0) {
-                        d[i + 1] -= p;
-                        e[mm] = 0.0;
-                        underflow = true;
-                        break;
-                    }
-                    s = f / r;
-                    cc = gg / r;
-                    gg = d[i + 1] - p;
-                    r = (d[i] - gg) * s + 2.0 * cc * b;
-                    p = s * r;
-                    d[i + 1] = gg + p;
-                    gg = cc * r - b;
-                    for (int k = 0; k < m; k++) {
-                        const double zf = Z[k + (size_t)(i + 1) * m];
-                        Z[k + (size_t)(i + 1) * m] = s * Z[k + (size_t)i * m] + cc * zf;
-                        Z[k + (size_t)i * m] = cc * Z[k + (size_t)i * m] - s * zf;
-                    }
-                }
-                if (underflow) continue;
-                d[l] -= p;
-                e[l] = gg;
-                e[mm] = 0.0;
-            }
-        } while (mm != l);
-    }
-    int nneg = 0;
-    for (int i = 0; i < m; i++) {
-        double gi = 0.0;
-        for (int jj = 0; jj < m; jj++) {
-            double th = d[jj];
-            if (th < 0.0) th = 0.0;
-            gi += Z[i + (size_t)jj * m] * std::sqrt(th) * Z[0 + (size_t)jj * m];
-        }
-        g[i] = beta0 * gi;
-    }
-    for (int jj = 0; jj < m; jj++)
-        if (d[jj] < 0.0) nneg++;
-    return nneg;
 }

@@ -21,6 +21,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <type_traits>
+
 namespace {
 
 
@@ -165,11 +167,16 @@ __global__ __launch_bounds__(PS_BLOCK) void k_aa(SysConst s, const T* __restrict
 // Diagonal blocks (I == J) run the ordered loop.  Output: part[(J 3 + c) Nm + i] is the
 // contribution of source tile J to target i, written exactly once per (J, i).
 //
-// Positions are in box units when the box is active (cubic box, x / L), so the minimum image is
-// d - rint(d) and L folds into the constants:
-//   far:  ya = a (c1 - c2 a^2),  cs = a^3 (c3 + c4 a^2)   with a = |d_s|^-1, u += ya f + cs (d_s.f) d_s
-//   near: r = L |d_s|^2 a:  ya = 1 - 9/32 r,  cs = (21/32 r - 2) a^2
-// c1 = 3/(4L), c2 = 1/(2L^3), c3 = -3/(8L), c4 = -3/(2L^3) (L = 1 without a box).
+// Positions.  FP64, and FP32 without a box: x / L in the compute type (L = 1 without a box), so the
+// minimum image is d - rint(d).  FP32 in a cubic box: fixed point, x / L * 2^32 as a 32-bit
+// integer, so the wrap-around of the integer difference IS the minimum image (one integer
+// subtraction and one conversion per axis instead of three floating-point operations), and the
+// separation carries 32 bits of the box for every pair -- close pairs keep their relative
+// precision (6e-8) instead of losing it to the rounding of two large coordinates.
+// In either case the unit u of d (L, or L / 2^32) folds into the constants:
+//   far:  ya = a (c1 - c2 a^2),  cs = a^3 (c3 + c4 a^2)   with a = |d|^-1, u += ya f + cs (d.f) d
+//   near: r = u |d|^2 a:  ya = 1 - 9/32 r,  cs = (21/32 r - 2) a^2
+// c1 = 3/(4u), c2 = 1/(2u^3), c3 = -3/(8u), c4 = -3/(2u^3).
 struct AAConst {
     double c1, c2, c3, c4, L, tb2s, clamp;
 };
@@ -185,6 +192,14 @@ struct Vec4<double> {
     using type = double4;
 };
 
+// The stored position type of the symmetric kernel: FIX = FP32 in a box (fixed point).
+template <class T, bool BOX>
+struct AAPos {
+    static constexpr bool FIX = sizeof(T) == sizeof(float) && BOX;
+    using P = typename std::conditional<FIX, int, T>::type;
+    using V = typename std::conditional<FIX, int4, typename Vec4<T>::type>::type;
+};
+
 template <class T>
 __device__ __forceinline__ T ps_fma(T a, T b, T c);
 template <>
@@ -192,35 +207,56 @@ __device__ __forceinline__ float ps_fma<float>(float a, float b, float c) { retu
 template <>
 __device__ __forceinline__ double ps_fma<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-// lane t receives the value of lane (t + 1) mod 64 (DPP wave_rol:1; a double moves as two halves)
+// One separation component in the kernel's unit: the minimum image of x_i - x_j.
+template <class T, bool BOX, bool FIX, class P>
+__device__ __forceinline__ T aa_sep(P xi, P xj) {
+    if constexpr (FIX) {
+        return (T)(int)((unsigned)xi - (unsigned)xj);
+    } else {
+        T d = xi - xj;
+        if (BOX) d -= ps_rint<T>(d);
+        return d;
+    }
+}
+
+// lane t receives the value of lane (t + 1) mod 64 (DPP wave_rol:1; a double moves as two halves).
+// Every lane reads a valid lane, so no old value is needed.
 __device__ __forceinline__ float wave_rol1(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x134, 0xF, 0xF, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x134, 0xF, 0xF, false));
 }
 __device__ __forceinline__ double wave_rol1(double v) {
     const long long b = __builtin_bit_cast(long long, v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), 0x134, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x134, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), 0x134, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x134, 0xF, 0xF, false);
     return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
 
-template <class T>
+template <class T, bool BOX>
 struct AASmem {
-    typename Vec4<T>::type S[512], F[512];
+    typename AAPos<T, BOX>::V S[512];
+    typename Vec4<T>::type F[512];
     T A[3][PS_BLOCK];
 };
 
-// One tile pair (I, J); PAD: the pair involves the partial last tile (ghost beads beyond Nm).
+// One tile pair (I, J); PAD: the pair involves the partial last tile (ghost beads beyond Nm, at the
+// origin with f = 0).
 template <class T, bool BOX, bool PAD>
-__device__ __forceinline__ void aa_sym_tile(int Nm, int I, int J, const AAConst& k, const T* __restrict__ xs,
-                                            const double* __restrict__ f, T* __restrict__ part, AASmem<T>& sm) {
+__device__ __forceinline__ void aa_sym_tile(int Nm, int I, int J, const AAConst& k,
+                                            const typename AAPos<T, BOX>::P* __restrict__ xs,
+                                            const double* __restrict__ f, T* __restrict__ part, AASmem<T, BOX>& sm) {
+    using PT = AAPos<T, BOX>;
+    constexpr bool FIX = PT::FIX;
+    using P = typename PT::P;
+    using PV = typename PT::V;
     using V = typename Vec4<T>::type;
-    V* S = sm.S;
+    PV* S = sm.S;
     V* F = sm.F;
     T(*A)[PS_BLOCK] = sm.A;
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     {
         const int j = J * PS_BLOCK + t;
-        V sv = {0, 0, 0, 0}, fv = {0, 0, 0, 0};
+        PV sv = {0, 0, 0, 0};
+        V fv = {0, 0, 0, 0};
         if (!PAD || j < Nm) {
             sv.x = xs[4 * j];
             sv.y = xs[4 * j + 1];
@@ -238,7 +274,8 @@ __device__ __forceinline__ void aa_sym_tile(int Nm, int I, int J, const AAConst&
         for (int c = 0; c < 3; c++) A[c][t] = 0;
     }
     const int i = I * PS_BLOCK + t;
-    T xi = 0, yi = 0, zi = 0, fxi = 0, fyi = 0, fzi = 0;
+    P xi = 0, yi = 0, zi = 0;
+    T fxi = 0, fyi = 0, fzi = 0;
     if (!PAD || i < Nm) {
         xi = xs[4 * i];
         yi = xs[4 * i + 1];
@@ -255,14 +292,10 @@ __device__ __forceinline__ void aa_sym_tile(int Nm, int I, int J, const AAConst&
         for (int q = 0; q < 4; q++) {
             for (int m = 0; m < 64; m++) {
                 const int jt = (q << 6) + m;
-                const V xj = S[(q << 7) + m];
+                const PV xj = S[(q << 7) + m];
                 const V fj = F[(q << 7) + m];
-                T dx = xi - xj.x, dy = yi - xj.y, dz = zi - xj.z;
-                if (BOX) {
-                    dx -= ps_rint<T>(dx);
-                    dy -= ps_rint<T>(dy);
-                    dz -= ps_rint<T>(dz);
-                }
+                const T dx = aa_sep<T, BOX, FIX>(xi, xj.x), dy = aa_sep<T, BOX, FIX>(yi, xj.y),
+                        dz = aa_sep<T, BOX, FIX>(zi, xj.z);
                 T r2 = dx * dx + dy * dy + dz * dz;
                 bool skip = jt == t;
                 if (PAD) skip |= (J * PS_BLOCK + jt >= Nm);
@@ -295,27 +328,24 @@ __device__ __forceinline__ void aa_sym_tile(int Nm, int I, int J, const AAConst&
         return;
     }
     const T clamp = (T)k.clamp;
-    // The u_j partial sums travel with their sources in registers: after step m lane t holds the
-    // sum for source (t + m) mod 64 of the window, and a wave rotation (DPP wave_rol:1, lane t
-    // reads lane t + 1) hands it on, so step m + 1 adds to the right source; after 64 steps every
-    // lane has met every source.  (LDS atomics for the same sums measured several times slower.)
+    // The u_j partial sums travel with their sources in registers: at step m lane t holds the sum
+    // for source (t + m - 1) mod 64 of the window; a wave rotation (DPP wave_rol:1, lane t reads lane
+    // t + 1) hands it on to the lane that meets that source's successor, and the step's two
+    // contributions go into it with fused multiply-adds.  After 64 steps every lane has met every
+    // source.  (LDS atomics for the same sums measured several times slower.)
     constexpr int UNR = sizeof(T) == sizeof(float) ? 8 : 4;
     for (int p = 0; p < 4; p++) {
         const int q = (w + p) & 3;
         const int base = (q << 7) + l;
-        const V* Sq = S + base;
+        const PV* Sq = S + base;
         const V* Fq = F + base;
         T Rx = 0, Ry = 0, Rz = 0;
 #pragma unroll UNR
         for (int m = 0; m < 64; m++) {
-            const V xj = Sq[m];
+            const PV xj = Sq[m];
             const V fj = Fq[m];
-            T dx = xi - xj.x, dy = yi - xj.y, dz = zi - xj.z;
-            if (BOX) {
-                dx -= ps_rint<T>(dx);
-                dy -= ps_rint<T>(dy);
-                dz -= ps_rint<T>(dz);
-            }
+            const T dx = aa_sep<T, BOX, FIX>(xi, xj.x), dy = aa_sep<T, BOX, FIX>(yi, xj.y),
+                    dz = aa_sep<T, BOX, FIX>(zi, xj.z);
             T r2 = ps_fma(dz, dz, ps_fma(dy, dy, dx * dx));
             if (PAD) r2 = r2 > clamp ? r2 : clamp;   // ghost beads (f = 0) stay finite
             const T a = ps_rsqrt(r2);
@@ -337,9 +367,9 @@ __device__ __forceinline__ void aa_sym_tile(int Nm, int I, int J, const AAConst&
             uz = ps_fma(ci, dz, uz);
             // u_j += ya f_i + cs (d.f_i) d
             const T cj = cs * ps_fma(dz, fzi, ps_fma(dy, fyi, dx * fxi));
-            Rx = wave_rol1(Rx) + ps_fma(cj, dx, ya * fxi);
-            Ry = wave_rol1(Ry) + ps_fma(cj, dy, ya * fyi);
-            Rz = wave_rol1(Rz) + ps_fma(cj, dz, ya * fzi);
+            Rx = ps_fma(cj, dx, ps_fma(ya, fxi, wave_rol1(Rx)));
+            Ry = ps_fma(cj, dy, ps_fma(ya, fyi, wave_rol1(Ry)));
+            Rz = ps_fma(cj, dz, ps_fma(ya, fzi, wave_rol1(Rz)));
         }
         // lane t now holds the complete sum of the wave for source (t + 63) mod 64; the waves
         // visit a window one at a time, so a plain LDS update is race-free and ordered
@@ -365,9 +395,9 @@ __device__ __forceinline__ void aa_sym_tile(int Nm, int I, int J, const AAConst&
 // early and overlap the rest of the launch instead of forming a tail.
 template <class T, bool BOX>
 __global__ __launch_bounds__(PS_BLOCK) void k_aa_sym(int Nm, const int2* __restrict__ pairs, int npad, AAConst k,
-                                                     const T* __restrict__ xs, const double* __restrict__ f,
-                                                     T* __restrict__ part) {
-    __shared__ AASmem<T> sm;
+                                                     const typename AAPos<T, BOX>::P* __restrict__ xs,
+                                                     const double* __restrict__ f, T* __restrict__ part) {
+    __shared__ AASmem<T, BOX> sm;
     const int2 pr = pairs[blockIdx.x];
     if ((int)blockIdx.x < npad)
         aa_sym_tile<T, BOX, true>(Nm, pr.x, pr.y, k, xs, f, part, sm);
@@ -384,6 +414,21 @@ __global__ void k_aa_pos(int Nm, const double* __restrict__ xm, double inv_L, T*
     xs[4 * i + 1] = (T)(xm[3 * i + 1] * inv_L);
     xs[4 * i + 2] = (T)(xm[3 * i + 2] * inv_L);
     xs[4 * i + 3] = (T)0;
+}
+
+// Fixed-point monomer positions for the FP32 symmetric kernel in a box: [Nm][4] (x, y, z, 0) with
+// x / L wrapped into [-1/2, 1/2] and scaled by 2^32 (the ends meet: 2^31 wraps to -2^31).
+__global__ void k_aa_pos_fix(int Nm, const double* __restrict__ xm, double inv_L, int* __restrict__ xs) {
+    const int i = blockIdx.x * PS_BLOCK + threadIdx.x;
+    if (i >= Nm) return;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double t = xm[3 * i + d] * inv_L;
+        t -= __builtin_rint(t);
+        const long long v = __builtin_llrint(t * 4294967296.0);
+        xs[4 * i + d] = (int)(unsigned)(unsigned long long)v;
+    }
+    xs[4 * i + 3] = 0;
 }
 
 // ---------------------------------------------------------------------------- AB, monomer rows
@@ -753,6 +798,43 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
         PS_TRY(c->prof_record(1, &ea0, &ea1));
         PS_HIP(c, hipEventRecord(ea0, st));
     }
+    // The colloid-source rows of the monomers, the colloid rows and the colloid pairs do not depend
+    // on the monomer-monomer tile kernel: on the side stream they run beside it (and fill its tail);
+    // the finalize waits for both.
+    auto side_kernels = [&](hipStream_t ss, bool mono) -> ps_err {
+        if (mono) {
+            dim3 g2((s.Nm + PS_BLOCK - 1) / PS_BLOCK, aw.nchunk_m);
+            if (s.box_active)
+                k_ab_mono<T, true><<<g2, PS_BLOCK, 0, ss>>>(s, xm, f, c->stale, aw.per_chunk_m, (T*)aw.part_abm);
+            else
+                k_ab_mono<T, false><<<g2, PS_BLOCK, 0, ss>>>(s, xm, f, c->stale, aw.per_chunk_m, (T*)aw.part_abm);
+            PS_LAUNCH(c);
+        }
+        if (s.Nc > 0) {
+            if (s.Nm > 0) {
+                dim3 g3(s.Nc, aw.nchunk_c);
+                if (s.box_active)
+                    k_ab_coll<T, true><<<g3, PS_BLOCK, 0, ss>>>(s, xm, f, aw.per_chunk_c, (T*)aw.part_c,
+                                                                (T*)aw.part_fs);
+                else
+                    k_ab_coll<T, false><<<g3, PS_BLOCK, 0, ss>>>(s, xm, f, aw.per_chunk_c, (T*)aw.part_c,
+                                                                 (T*)aw.part_fs);
+                PS_LAUNCH(c);
+            }
+            k_bb<<<s.Nc, PS_BLOCK, 0, ss>>>(s, c->xm, f, aw.bb);
+            PS_LAUNCH(c);
+        }
+        return PS_OK;
+    };
+    const bool forked = s.Nm > 0 && !srows;
+    if (forked) {
+        PS_HIP(c, hipEventRecord(c->side_fork, st));
+        PS_HIP(c, hipStreamWaitEvent(c->side, c->side_fork, 0));
+        PS_TRY(side_kernels(c->side, true));
+        PS_HIP(c, hipEventRecord(c->side_join, c->side));
+    } else {
+        PS_TRY(side_kernels(st, false));
+    }
     if (s.Nm > 0 && !srows) {
         dim3 g1((s.Nm + PS_BLOCK - 1) / PS_BLOCK, aw.nsplit_aa);
         // profiling: the symmetric kernel's own dispatch records its start/stop (hipExtLaunchKernel);
@@ -764,26 +846,36 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
         const bool box = s.box_active, ch = s.chain_HI;
         if (aw.sym) {
             AAConst k{};
-            const double L = box ? s.L[0] : 1.0;
+            constexpr bool f32 = sizeof(T) == sizeof(float);
+            // the unit of the separations: box units, or box / 2^32 for the fixed-point FP32 form
+            const double L = box ? (f32 ? s.L[0] / 4294967296.0 : s.L[0]) : 1.0;
             k.c1 = 3.0 / (4.0 * L);
             k.c2 = 1.0 / (2.0 * L * L * L);
             k.c3 = -3.0 / (8.0 * L);
             k.c4 = -3.0 / (2.0 * L * L * L);
             k.L = L;
             k.tb2s = s.two_beta * s.two_beta / (L * L);
-            k.clamp = sizeof(T) == sizeof(float) ? 1e-30 : 1e-200;
-            const T* xs = (const T*)aw.xs;
+            // ghost beads sit at the origin with f = 0: the clamp keeps every coefficient finite
+            k.clamp = f32 ? (box ? 1.0 : 1e-30) : 1e-200;
+            const void* xs = aw.xs;
             const int nblk = aw.npair_full + aw.npair_pad;
             const int2* pairs = aw.pairs;
             const int npad = aw.npair_pad;
             T* part = (T*)aw.part_aa;
-            hipEvent_t e0 = dispatch_ev ? ek0 : nullptr, e1 = dispatch_ev ? ek1 : nullptr;
-            if (box)
-                hipExtLaunchKernelGGL(k_aa_sym<T, true>, dim3(nblk), dim3(PS_BLOCK), 0, st, e0, e1, 0, s.Nm, pairs,
-                                      npad, k, xs, f, part);
-            else
-                hipExtLaunchKernelGGL(k_aa_sym<T, false>, dim3(nblk), dim3(PS_BLOCK), 0, st, e0, e1, 0, s.Nm,
-                                      pairs, npad, k, xs, f, part);
+            using PB = const typename AAPos<T, true>::P*;
+            using PN = const typename AAPos<T, false>::P*;
+            if (dispatch_ev) {
+                if (box)
+                    hipExtLaunchKernelGGL(k_aa_sym<T, true>, dim3(nblk), dim3(PS_BLOCK), 0, st, ek0, ek1, 0, s.Nm,
+                                          pairs, npad, k, (PB)xs, f, part);
+                else
+                    hipExtLaunchKernelGGL(k_aa_sym<T, false>, dim3(nblk), dim3(PS_BLOCK), 0, st, ek0, ek1, 0, s.Nm,
+                                          pairs, npad, k, (PN)xs, f, part);
+            } else if (box) {
+                k_aa_sym<T, true><<<nblk, PS_BLOCK, 0, st>>>(s.Nm, pairs, npad, k, (PB)xs, f, part);
+            } else {
+                k_aa_sym<T, false><<<nblk, PS_BLOCK, 0, st>>>(s.Nm, pairs, npad, k, (PN)xs, f, part);
+            }
             PS_LAUNCH(c);
         } else if (box && ch)
             k_aa<T, true, true><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
@@ -795,25 +887,8 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
             k_aa<T, false, false><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
         PS_LAUNCH(c);
         if (c->prof_on && !dispatch_ev) PS_HIP(c, hipEventRecord(ek1, st));
-        dim3 g2((s.Nm + PS_BLOCK - 1) / PS_BLOCK, aw.nchunk_m);
-        if (box)
-            k_ab_mono<T, true><<<g2, PS_BLOCK, 0, st>>>(s, xm, f, c->stale, aw.per_chunk_m, (T*)aw.part_abm);
-        else
-            k_ab_mono<T, false><<<g2, PS_BLOCK, 0, st>>>(s, xm, f, c->stale, aw.per_chunk_m, (T*)aw.part_abm);
-        PS_LAUNCH(c);
     }
-    if (s.Nc > 0) {
-        if (s.Nm > 0) {
-            dim3 g3(s.Nc, aw.nchunk_c);
-            if (s.box_active)
-                k_ab_coll<T, true><<<g3, PS_BLOCK, 0, st>>>(s, xm, f, aw.per_chunk_c, (T*)aw.part_c, (T*)aw.part_fs);
-            else
-                k_ab_coll<T, false><<<g3, PS_BLOCK, 0, st>>>(s, xm, f, aw.per_chunk_c, (T*)aw.part_c, (T*)aw.part_fs);
-            PS_LAUNCH(c);
-        }
-        k_bb<<<s.Nc, PS_BLOCK, 0, st>>>(s, c->xm, f, aw.bb);
-        PS_LAUNCH(c);
-    }
+    if (forked) PS_HIP(c, hipStreamWaitEvent(st, c->side_join, 0));
     const int64_t r_begin = srows ? s.nu : 0;
     if (s.n - r_begin <= 0) return PS_OK;
     dim3 g4((unsigned)((s.n - r_begin + PS_BLOCK - 1) / PS_BLOCK));
@@ -932,7 +1007,10 @@ ps_err mob_prepare(Ctx* c) {
         k_aa_pos<double><<<gm, PS_BLOCK, 0, c->stream>>>(c->Nm, c->xm, c->aw64.inv_L, (double*)c->aw64.xs);
         PS_LAUNCH(c);
     }
-    if (c->aw32.sym) {
+    if (c->aw32.sym && c->sc.box_active) {
+        k_aa_pos_fix<<<gm, PS_BLOCK, 0, c->stream>>>(c->Nm, c->xm, c->aw32.inv_L, (int*)c->aw32.xs);
+        PS_LAUNCH(c);
+    } else if (c->aw32.sym) {
         k_aa_pos<float><<<gm, PS_BLOCK, 0, c->stream>>>(c->Nm, c->xm, c->aw32.inv_L, (float*)c->aw32.xs);
         PS_LAUNCH(c);
     }

@@ -194,7 +194,8 @@ __global__ __launch_bounds__(TB) void k_nb_count(PairConst pc, const double* __r
 __global__ __launch_bounds__(TB) void k_nb_fill(PairConst pc, const double* __restrict__ xs,
                                                 const int* __restrict__ cell_of, const int* __restrict__ cell_start,
                                                 const int* __restrict__ perm, const double* __restrict__ xsort,
-                                                const int* __restrict__ off, int cap, int* __restrict__ part) {
+                                                const int* __restrict__ off, int cap, int* __restrict__ part,
+                                                int64_t part_cap, int64_t* __restrict__ overflow) {
     extern __shared__ int lds[];
     const int g = threadIdx.x / GL;
     const int ngroups = blockDim.x / GL;
@@ -217,12 +218,19 @@ __global__ __launch_bounds__(TB) void k_nb_fill(PairConst pc, const double* __re
     if (p >= pc.s.Np) return;
     const int o = off[p];
     const int n = min(cursor[g], cap);
+    // a row longer than the group's LDS row or a list longer than `part` is cut and flagged
+    if (lane == 0 && (cursor[g] > cap || (int64_t)off[pc.s.Np] > part_cap)) *overflow = 1;
     for (int e = lane; e < n; e += GL) {
         const int v = buf[e];
         int rank = 0;
         for (int a = 0; a < n; a++) rank += buf[a] < v;
-        part[o + rank] = v;
+        if ((int64_t)o + rank < part_cap) part[o + rank] = v;
     }
+}
+
+__global__ void k_zero_int(int* __restrict__ a, int n) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i < n) a[i] = 0;
 }
 
 // ---------------------------------------------------------------------------- forces
@@ -486,6 +494,11 @@ ps_err scan_ints(Ctx* c, const int* in, int* out, int n) {
     return PS_OK;
 }
 
+// The Verlet list with no host round trip: every row holds at most VL_ROW_CAP partners and the
+// whole list at most nb_cap entries (64 per particle); a longer row or list sets the sticky
+// DC_VLIST_OVERFLOW flag, which the entry points report as an error.
+constexpr int VL_ROW_CAP = 511;
+
 ps_err check_dist_dev(Ctx* c) {
     if (c->arrow) return arrow_check_dist(c);
     const int Np = c->Np;
@@ -497,12 +510,15 @@ ps_err check_dist_dev(Ctx* c) {
     const double* xsort = c->xpd;
     if (c->cell_mode == 1) {
         const int ncells = c->ncell[0] * c->ncell[1] * c->ncell[2];
-        PS_HIP(c, hipMemsetAsync(c->cell_count, 0, sizeof(int) * ncells, st));
+        const unsigned gc = (unsigned)((ncells + TB - 1) / TB);
+        k_zero_int<<<gc, TB, 0, st>>>(c->cell_count, ncells);
+        PS_LAUNCH(c);
         k_bin<<<gp, TB, 0, st>>>(pc, c->xpd, c->cell_of, c->cell_count);
         PS_LAUNCH(c);
         k_scan<<<1, 1024, 0, st>>>(c->cell_count, c->cell_start, ncells);
         PS_LAUNCH(c);
-        PS_HIP(c, hipMemsetAsync(c->cell_count, 0, sizeof(int) * ncells, st));
+        k_zero_int<<<gc, TB, 0, st>>>(c->cell_count, ncells);
+        PS_LAUNCH(c);
         k_scatter<<<gp, TB, 0, st>>>(Np, c->xpd, c->cell_of, c->cell_start, c->cell_count, c->cell_perm, c->cell_xs);
         PS_LAUNCH(c);
         xsort = c->cell_xs;
@@ -512,32 +528,31 @@ ps_err check_dist_dev(Ctx* c) {
     PS_LAUNCH(c);
     k_scan<<<1, 1024, 0, st>>>(c->nb_count, c->nb_off, Np);
     PS_LAUNCH(c);
-    PS_HIP(c, hipMemcpyAsync(c->hflag, c->nb_off + Np, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-    PS_HIP(c, hipStreamSynchronize(st));
-    const int64_t total = c->hflag[0];
-    const int longest = std::max(c->hflag[1], 1);
-    if (total > c->nb_cap) {
-        PS_HIP(c, hipFree(c->nb_part));
-        c->nb_cap = total + total / 4 + 1024;
-        PS_HIP(c, hipMalloc(&c->nb_part, sizeof(int) * (size_t)c->nb_cap));
-    }
-    // LDS per block: a cursor and `longest` entries per group; 8 groups per block when that fits in
-    // 64 KiB, else one group per block.
-    int gpb = TB / GL;
-    if ((size_t)gpb * (longest + 1) * sizeof(int) > 65536) gpb = 1;
-    if ((size_t)gpb * (longest + 1) * sizeof(int) > 65536) {
-        c->set_error("check_dist: a particle has " + std::to_string(longest) +
-                     " Verlet partners, more than one LDS row holds");
-        return PS_ERR_ARG;
-    }
-    const size_t lds = sizeof(int) * (size_t)gpb * (longest + 1);
+    // LDS per block: a cursor and VL_ROW_CAP entries for each of its 8 groups
+    const int gpb = TB / GL;
+    const size_t lds = sizeof(int) * (size_t)gpb * (VL_ROW_CAP + 1);
     k_nb_fill<<<(unsigned)((Np + gpb - 1) / gpb), gpb * GL, lds, st>>>(pc, c->xpd, c->cell_of, c->cell_start,
-                                                                        c->cell_perm, xsort, c->nb_off, longest,
-                                                                        c->nb_part);
+                                                                        c->cell_perm, xsort, c->nb_off, VL_ROW_CAP,
+                                                                        c->nb_part, c->nb_cap,
+                                                                        c->dcnt + DC_VLIST_OVERFLOW);
     PS_LAUNCH(c);
-    c->nvlist = total / 2;
-    c->cnt.nvlist = c->nvlist;
     c->have_pd = true;
+    return PS_OK;
+}
+
+// The pair count of the last check_dist (a host round trip); an overflowed list is an error.
+ps_err vlist_count(Ctx* c, int64_t* n_pairs) {
+    PS_HIP(c, hipMemcpyAsync(c->hflag, c->nb_off + c->Np, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(c, hipMemcpyAsync(c->dcnt_h, c->dcnt, sizeof(int64_t) * DC_N, hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->dcnt_h[DC_VLIST_OVERFLOW]) {
+        c->set_error("check_dist: a Verlet row exceeded " + std::to_string(VL_ROW_CAP) +
+                     " partners or the list exceeded " + std::to_string(c->nb_cap) + " entries");
+        return PS_ERR_STATE;
+    }
+    c->nvlist = c->hflag[0] / 2;
+    c->cnt.nvlist = c->nvlist;
+    if (n_pairs) *n_pairs = c->nvlist;
     return PS_OK;
 }
 
@@ -584,6 +599,7 @@ ps_err rhs_dev(Ctx* c, int drift) {
 // Copy back the reference-ordered Verlet list (q > k half of each row).
 ps_err vlist_copy(Ctx* c, int32_t* rows, int32_t* partners, int64_t* pidx, int64_t cap, int64_t* n_pairs) {
     const int Np = c->Np;
+    if (!c->arrow) PS_TRY(vlist_count(c, nullptr));
     std::vector<int> off(Np + 1);
     PS_HIP(c, hipMemcpyAsync(off.data(), c->nb_off, sizeof(int) * (Np + 1), hipMemcpyDeviceToHost, c->stream));
     PS_HIP(c, hipStreamSynchronize(c->stream));

@@ -45,6 +45,19 @@ struct SysConst {
     int64_t n, nu, ns, nm3, nc3;
 };
 
+// Device-side counters and flags (Ctx::dcnt): accumulated by the kernels that decide them, so that
+// a step needs no host round trip; ps_get_counters adds them to the host-side counts.
+enum {
+    DC_LANCZOS_ITERS = 0,   // Lanczos vectors used by the square roots
+    DC_NNEG_EVENTS,         // square roots that floored a negative Ritz value
+    DC_NNEG_COUNT,          // negative Ritz values floored
+    DC_MSS_ITERS,           // stresslet MINRES iterations of the block solves
+    DC_MSS_CAPPED,          // stresslet solves that stopped at their iteration cap
+    DC_VLIST_OVERFLOW,      // a Verlet row or the whole list exceeded its fixed capacity (sticky flag)
+    DC_BARRIER_TIMEOUT,     // a grid-wide wait ran out of time (sticky flag)
+    DC_N = 8
+};
+
 // Device-resident MINRES scalars (one per solve).
 struct MinresState {
     double beta1, tol, oldb, beta, dbar, epsln, phibar, cs, sn;
@@ -77,6 +90,9 @@ struct Ctx {
     int Np = 0, Nm = 0, Nc = 0;
     int64_t n = 0, nu = 0, ns = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;     // second stream: the colloid-row and carried-tile kernels of an apply
+                                    // run on it beside the monomer-monomer tile kernel (default priority)
+    hipEvent_t side_fork = nullptr, side_join = nullptr;
     int device = 0;
     int n_cu = 0;
     std::string err;
@@ -108,19 +124,21 @@ struct Ctx {
     MinresState* mstate = nullptr;  // device
     MinresState* mstate_h = nullptr;// pinned host
     double* Q = nullptr;            // Lanczos basis kmax x n
-    double* lz = nullptr;           // Lanczos scalars on device (alpha, beta, h) [3 * kmax + 8]
+    double* lz = nullptr;           // Lanczos scalars on device: alpha [64], beta^2 [64], then scalar slots
     double* lz_h = nullptr;         // pinned host mirror
-    hipEvent_t lz_ev[2] = {nullptr, nullptr};   // Lanczos polling groups (alternating)
-    double* gw = nullptr;           // recombination weights (device) kmax
     double *ir_r = nullptr, *ir_d = nullptr;   // refinement residual / correction (ns)
     int kmax = 0;
+    double* lzg = nullptr;          // Lanczos square root weights of every m (64 x 64), then per m the
+                                    // negative Ritz value count (64)
+    double* lzinfo = nullptr;       // [4]: m, negative Ritz values, relative change, beta0 of the last one
+    // ---- device-side counters and flags (DC_*), read by ps_get_counters and after ps_run
+    int64_t* dcnt = nullptr;
+    int64_t* dcnt_h = nullptr;      // pinned
+    unsigned* gbar = nullptr;       // grid barrier of the persistent stresslet MINRES: count, generation, error
     // ---- block-elimination solver (PS_SOLVER_BLOCK)
     double* Mss = nullptr;          // 5Nc x 5Nc stresslet block, rebuilt by mob()
     double* bs_y = nullptr;         // n
     double* bs_r = nullptr;         // 5Nc
-    double* bs_x = nullptr;         // ns (correction of a refinement sweep)
-    double* ms_x = nullptr;         // 5Nc: solution of the M_ss solve (fixed address for the graph)
-    hipGraphExec_t ms_exec = nullptr;   // MS_POLL iterations of the M_ss solve, captured once
     // ---- Verlet list (check_dist)
     int ncell[3] = {0, 0, 0};
     int cell_mode = 0;              // 0: brute force; 1: periodic cell list
@@ -153,7 +171,6 @@ struct Ctx {
     // ---- counters
     ps_counters cnt{};
     int last_minres_its = 0;
-    int last_block_its = 0;
     // ---- scratch used by the host-pointer entry points
     double* io_a = nullptr;
     double* io_b = nullptr;
@@ -244,6 +261,7 @@ void pairs_free(Ctx* c);
 ps_err check_dist_dev(Ctx* c);
 ps_err rhs_dev(Ctx* c, int drift);
 ps_err vlist_copy(Ctx* c, int32_t* rows, int32_t* partners, int64_t* pidx, int64_t cap, int64_t* n_pairs);
+ps_err vlist_count(Ctx* c, int64_t* n_pairs);                      // pair count of the last check_dist (syncs)
 ps_err run_steps(Ctx* c, int nsteps);
 ps_err dense_build(Ctx* c, int storage);
 ps_err dense_mult_dev(Ctx* c, const double* X, double* Y, int nrhs);   // Y_j = M X_j, vector-major
@@ -271,4 +289,3 @@ ps_err lub_slopes_host(const double* rows, int nrows, int ncols, double* slopes)
 ps_err vec_axpby(Ctx* c, double* y, double a, const double* x, double b, int64_t n);  // y = a x + b y
 ps_err vec_copy(Ctx* c, double* dst, const double* src, int64_t n);
 ps_err vec_dot(Ctx* c, const double* a, const double* b, int64_t n, double* out);      // sync
-int tridiag_sqrt_weights(int m, const double* alpha, const double* betad, double beta0, double* g);

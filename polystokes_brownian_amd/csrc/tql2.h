@@ -1,0 +1,108 @@
+// tql2.h -- symmetric tridiagonal eigensolver on one wavefront (device code).
+//
+// Implicit QL with shifts (EISPACK tql2: Bowdler, Martin, Reinsch & Wilkinson, Numer. Math. 11
+// (1968) 293-306), eigenvectors accumulated.  It serves the Lanczos square root, whose tridiagonal
+// T_m = U diag(theta) U^T gives M^{1/2} xi ~ beta0 Q U diag(sqrt(max(theta, 0))) U^T e1
+// (src/slip_vel.cpp:427-459, where LAPACK dsyev plays this part).
+//
+// Layout: lane k holds d[k] and e[k] in registers and row k of Z (column-major m x m in LDS,
+// Z[k + col m]).  The QL sweep is one sequential chain of plane rotations; every lane runs it on
+// scalars read from the owning lanes (readlane, no LDS round trip and no fence per rotation) and
+// applies each rotation to its own row of Z.  A rotation's radius comes from a reciprocal square
+// root refined by two Newton steps (one chain of multiply-adds instead of a square root and two
+// divisions).  m <= 64.
+#pragma once
+#include <hip/hip_runtime.h>
+
+__device__ __forceinline__ double tql2_rl(double v, int lane) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// 1/sqrt(t) for t > 0 to FP64 accuracy: hardware estimate, two Newton-Raphson steps.
+__device__ __forceinline__ double tql2_rsqrt(double t) {
+    double y = __builtin_amdgcn_rsq(t);
+    double h = t * y;
+    double r = __builtin_fma(-h, y, 1.0);
+    y = __builtin_fma(0.5 * y, r, y);
+    h = t * y;
+    r = __builtin_fma(-h, y, 1.0);
+    y = __builtin_fma(0.5 * y, r, y);
+    return y;
+}
+
+// Eigenvalues of T (diagonal dv, off-diagonal ev of lane k; ev of lane m-1 unused) into dv,
+// eigenvectors into the columns of Z.  Returns false when an eigenvalue needs more than 200 QL
+// iterations (T is then left partly reduced).
+__device__ inline bool tql2_wave(int m, double& dv, double& ev, double* Z) {
+    const int lane = threadIdx.x & 63;
+    for (int k = lane; k < m * m; k += 64) Z[k] = 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane < m) Z[lane + lane * m] = 1.0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane >= m - 1) ev = 0.0;
+    for (int l = 0; l < m; l++) {
+        int iter = 0, mm;
+        do {
+            // mm: the first index >= l with a negligible e[mm], else m - 1 (all lanes test at once)
+            const double dnext = __shfl_down(dv, 1, 64);
+            const bool small = lane >= l && lane < m - 1 &&
+                               fabs(ev) <= 1e-300 + 2.2e-16 * (fabs(dv) + fabs(dnext));
+            const unsigned long long mask = __ballot(small);
+            mm = mask ? (int)__builtin_ctzll(mask) : m - 1;
+            if (mm != l) {
+                if (iter++ == 200) return false;
+                const double dl = tql2_rl(dv, l), el = tql2_rl(ev, l);
+                double g = (tql2_rl(dv, l + 1) - dl) / (2.0 * el);
+                double r = sqrt(g * g + 1.0);
+                g = tql2_rl(dv, mm) - dl + el / (g + (g >= 0 ? fabs(r) : -fabs(r)));
+                double s = 1.0, c = 1.0, p = 0.0;
+                bool underflow = false;
+                double dip1 = tql2_rl(dv, mm);   // d[i + 1] before this sweep changes it
+                for (int i = mm - 1; i >= l; i--) {
+                    const double ei = tql2_rl(ev, i), di = tql2_rl(dv, i);
+                    const double f = s * ei, b = c * ei;
+                    const double t = __builtin_fma(f, f, g * g);
+                    if (t == 0.0) {   // both zero: the rotation is the identity, deflate here
+                        if (lane == i + 1) {
+                            ev = 0.0;
+                            dv = dip1 - p;
+                        }
+                        if (lane == mm) ev = 0.0;
+                        underflow = true;
+                        break;
+                    }
+                    const double y = tql2_rsqrt(t);
+                    r = t * y;
+                    s = f * y;
+                    c = g * y;
+                    g = dip1 - p;
+                    const double rr = (di - g) * s + 2.0 * c * b;
+                    p = s * rr;
+                    if (lane == i + 1) {
+                        ev = r;
+                        dv = g + p;
+                    }
+                    g = c * rr - b;
+                    dip1 = di;
+                    if (lane < m) {
+                        const double zf = Z[lane + (i + 1) * m];
+                        const double zi = Z[lane + i * m];
+                        Z[lane + (i + 1) * m] = s * zi + c * zf;
+                        Z[lane + i * m] = c * zi - s * zf;
+                    }
+                }
+                if (underflow) continue;
+                if (lane == l) {
+                    dv -= p;
+                    ev = g;
+                }
+                if (lane == mm) ev = 0.0;
+            }
+        } while (mm != l);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    return true;
+}

@@ -302,13 +302,19 @@ def _mixed_case(**over):
 
 
 def test_mixed_block_velocity_error():
-    """Mixed form with block elimination: the solve's one apply stays FP64, so the saddle solution
-    matches the dense FP64 direct solve to FP64 accuracy; the FP32 tiles serve the square root."""
-    g, o, b, xd = _mixed_case(solver=1)
+    """Mixed form with block elimination: without refinement the solve's one apply runs on FP32 tiles
+    and the velocity carries their rounding (below 1e-6 against the dense FP64 direct solve); with
+    FP64 residual refinement it recovers FP64 accuracy.  The FP32 tiles also serve the square root."""
+    g, o, b, xd = _mixed_case(solver=1, ir_maxit=0)
     x, info, _ = g.saddle_solve(b)
     err = rel(x[o.n:], xd[o.n:])
-    print("mixed block velocity rel err", err, flush=True)
-    assert err < 1e-10
+    print("mixed block velocity rel err, no refinement", err, flush=True)
+    assert err < 1e-6
+    g2 = _mixed_case(solver=1, ir_maxit=10, ir_rtol=1e-13)[0]
+    x2, info2, _ = g2.saddle_solve(b)
+    err2 = rel(x2[o.n:], xd[o.n:])
+    print("mixed block velocity rel err, refined", err2, "sweeps", info2.refinements, flush=True)
+    assert err2 < 1e-10 and info2.refinements >= 1
     xi = np.random.default_rng(6).normal(size=o.n)
     w32, i32 = g.sqrt_apply(xi)
     w64, i64 = o.sqrt_apply(xi)
@@ -441,3 +447,48 @@ def test_frame_outputs_and_restart(tmp_path):
     assert tg == to
     assert rel(xg - xl, xo - xl) < 1e-9
     assert rel(qg, qo) < 1e-12
+
+
+@pytest.mark.parametrize("name,over", [("c1", dict()), ("box_1024", dict(aa_kernel=2, precision=1, ir_maxit=0)),
+                                       ("box_1024", dict(aa_kernel=1))])
+def test_mmhi_step_graph_replays_the_eager_step(name, over):
+    """Full monomer HI with block-eliminated solves: ps_run replays each step as a HIP graph (the
+    stresslet MINRES and the Lanczos stopping test run on the device).  Four replayed steps give the
+    same bits as four steps launched kernel by kernel, and the same device-side counts."""
+    if name in SYM_CASES:
+        pd, x0 = _sym_build(name)
+    else:
+        w = configs.workload(name)
+        pd, x0 = configs.params_dict(w), configs.positions(w)
+    pd = dict(pd, solver=1, ksp_rtol=1e-12, sqrt_rtol=1e-10, **over)
+    out = {}
+    for graph in (1, 0):
+        g = Context(dict(pd, graph=graph))
+        g.initial_configuration(x0)
+        g.run(4)
+        out[graph] = (g.state()[0], g.vectors(), g.counters())
+        g.close()
+    (x1, v1, c1), (x0_, v0, c0) = out[1], out[0]
+    print("case", name, over, "counters graph", c1, flush=True)
+    assert np.array_equal(x1, x0_)
+    for k in ("up", "udiff", "v_brown", "sb"):
+        assert np.array_equal(v1[k], v0[k]), k
+    for k in ("applies", "mob_calls", "saddle_solves", "sqrt_calls", "lanczos_iterations",
+              "stresslet_iterations", "lanczos_nneg_count", "nvlist"):
+        assert c1[k] == c0[k], k
+    assert c1["stresslet_iterations"] > 0 and c1["lanczos_iterations"] > 0
+
+
+def test_stresslet_solve_reports_iterations_and_counts():
+    """The persistent stresslet MINRES: the solve information read back equals what the device
+    counters accumulated, and the solution still matches the direct solve."""
+    g, o, _ = build("small_dense", solver=1)
+    g.check_dist(); o.check_dist()
+    g.mob(); o.mob()
+    b = o.rhs()
+    b[: o.n] += np.random.default_rng(8).normal(size=o.n)
+    c0 = g.counters()["stresslet_iterations"]
+    x, info, _ = g.saddle_solve(b)
+    c1 = g.counters()["stresslet_iterations"]
+    assert info.iterations > 0 and c1 - c0 == info.iterations
+    assert rel(x, _direct(o, b)) < 1e-10

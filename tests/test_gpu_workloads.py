@@ -139,23 +139,29 @@ def test_c3_applies_in_one_process_against_the_oracle():
 
 
 def test_c3_block_step_against_the_oracle():
+    """One step of the block-eliminated form from the same state and noise as the oracle's step: the
+    FP64 form to 1e-10, and the mixed form (every apply on FP32 tiles, no refinement) to 1e-6 in the
+    velocity, the quantity the mixed form's tolerance is stated for.  The Brownian part alone (a
+    smaller vector, so the same absolute error is a larger relative one) is held to 5e-6."""
     w, pd, x0 = _workload("c3")
     pd = dict(pd, solver=1)
-    g = Context(pd)
-    g.initial_configuration(x0)
-    g.run(1)
     o = _oracle(pd, dense=False)
     o.initial_configuration(x0)
     o.run(1)
-    xg, _, _ = g.state()
     xo, _, _ = o.state()
-    vg, vo = g.vectors(), o.vectors()
-    e = {k: rel(vg[k], vo[k]) for k in ("up", "v_det_n", "v_det_c", "v_brown")}
-    say("c3: one block step, displacement rel", rel(xg - x0, xo - x0), e)
-    assert rel(xg - x0, xo - x0) < 1e-10
-    for k, v in e.items():
-        assert v < 1e-10, k
-    g.close()
+    vo = o.vectors()
+    for prec, tol_x, tol_v in ((0, 1e-10, 1e-10), (1, 1e-6, 1e-6)):
+        g = Context(dict(pd, precision=prec, ir_maxit=0))
+        g.initial_configuration(x0)
+        g.run(1)
+        xg, _, _ = g.state()
+        vg = g.vectors()
+        e = {k: rel(vg[k], vo[k]) for k in ("up", "v_det_n", "v_det_c", "v_brown")}
+        say("c3: precision", prec, "one block step, displacement rel", rel(xg - x0, xo - x0), e)
+        assert rel(xg - x0, xo - x0) < tol_x
+        for k, v in e.items():
+            assert v < (5 * tol_v if (prec == 1 and k == "v_brown") else tol_v), (prec, k, v)
+        g.close()
 
 
 # ------------------------------------------------------------------------------------------- c4
