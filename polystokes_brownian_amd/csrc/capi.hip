@@ -414,9 +414,6 @@ void destroy_ctx(Ctx* c) {
     for (hipEvent_t e : c->noise_ev)
         if (e) (void)hipEventDestroy(e);
     step_graphs_reset(c);
-    if (c->side_fork) (void)hipEventDestroy(c->side_fork);
-    if (c->side_join) (void)hipEventDestroy(c->side_join);
-    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -594,12 +591,6 @@ ps_err ps_create(const ps_params* p, void** out) {
     if (hipGetDevice(&c->device) != hipSuccess) return bail(PS_ERR_HIP);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->side_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->side_join, hipEventDisableTiming) != hipSuccess) {
-        c->err = "hipStreamCreate failed";
-        return bail(PS_ERR_HIP);
-    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         c->err = "hipStreamCreate failed";
         return bail(PS_ERR_HIP);

@@ -766,6 +766,197 @@ __global__ __launch_bounds__(VB) void k_ms_solve(int64_t N, const double* __rest
     }
 }
 
+// The same recurrence for a small block (5Nc <= MS_LDS_N): every block keeps r1, r2 and v of the
+// whole vector in LDS (computed redundantly, so identical in every block) and its rows of A in
+// registers (rows x ceil(N / 256) <= 32 doubles a thread).  The only data crossing blocks is
+// y = A v: each block writes its rows and, after the barrier, reads the whole vector once.
+constexpr int MS_LDS_N = 1024, MS_MAXR = 8, MS_MAXK = 4;
+
+__global__ __launch_bounds__(VB) void k_ms_solve_lds(int N, const double* __restrict__ A, const double* __restrict__ b,
+                                                     double* __restrict__ x, double* __restrict__ ybuf, int rows,
+                                                     double rtol, double abstol, int maxit, unsigned* gb,
+                                                     MinresState* __restrict__ st_out, int64_t* __restrict__ dcnt) {
+    extern __shared__ double lds[];
+    double* r1 = lds;           // [N]
+    double* r2 = lds + N;       // [N]
+    double* v = lds + 2 * N;    // [N]
+    double* z = lds + 3 * N;    // [N]: r2_new, then z = P^-1 r2_new, then v_{k+1}
+    double* y = lds + 4 * N;    // [N]: A v_k, then y - f r1
+    __shared__ double sh[4];
+    __shared__ double shr[MS_MAXR][4];
+    if (__hip_atomic_load(&gb[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    unsigned gen = __hip_atomic_load(&gb[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned nbk = gridDim.x;
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const int rb = blockIdx.x * rows;
+    const int nr = min(rows, N - rb);
+    const int ncb = N / 5;
+    // this block's rows of A, kept in registers for the whole solve
+    double Ac[MS_MAXR][MS_MAXK];
+#pragma unroll
+    for (int q = 0; q < MS_MAXR; q++)
+#pragma unroll
+        for (int k = 0; k < MS_MAXK; k++) {
+            const int j = t + k * VB;
+            Ac[q][k] = (q < nr && j < N) ? A[(int64_t)(rb + q) * N + j] : 0.0;
+        }
+    // y_own = A v for this block's rows, into ybuf
+    auto gemv = [&](double* __restrict__ yout) {
+        double pq[MS_MAXR];
+#pragma unroll
+        for (int q = 0; q < MS_MAXR; q++) {
+            double a = 0.0;
+#pragma unroll
+            for (int k = 0; k < MS_MAXK; k++) {
+                const int j = t + k * VB;
+                if (j < N) a += Ac[q][k] * v[j];
+            }
+            pq[q] = wave_sum(a);
+        }
+        if (lane == 0)
+#pragma unroll
+            for (int q = 0; q < MS_MAXR; q++) shr[q][wid] = pq[q];
+        __syncthreads();
+        if (t < nr) yout[rb + t] = (shr[t][0] + shr[t][1]) + (shr[t][2] + shr[t][3]);
+    };
+    // start (cold): x = 0, r1 = r2 = b, z = P^-1 b, beta1^2 = b . z
+    double acc = 0.0;
+    for (int cb = t; cb < ncb; cb += VB) {
+        double r[5], zz[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) r[k] = b[5 * cb + k];
+        ms_prec5(r, zz);
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            r1[5 * cb + k] = r[k];
+            r2[5 * cb + k] = r[k];
+            z[5 * cb + k] = zz[k];
+            acc += r[k] * zz[k];
+        }
+    }
+    const double b1 = block_reduce(acc, sh);
+    MinresState s{};
+    s.beta1 = b1 > 0.0 ? sqrt(b1) : 0.0;
+    s.tol = fmax(rtol * sqrt(fmax(b1, 0.0)), abstol);
+    s.beta = s.beta1;
+    s.phibar = s.beta1;
+    s.cs = -1.0;
+    if (b1 <= 0.0) {
+        s.done = 1;
+        s.reason = 3;
+    } else if (s.phibar <= s.tol) {
+        s.done = 1;
+        s.reason = 2;
+    }
+    const double sc0 = s.done ? 0.0 : 1.0 / s.beta;
+    for (int i = t; i < N; i += VB) v[i] = z[i] * sc0;
+    // own rows: w_{k-1}, w_{k-2}, x (thread q holds row rb + q)
+    double wa = 0.0, wb = 0.0, xo = 0.0;
+    __syncthreads();
+    int par = 0;
+    bool ok = true;
+    if (!s.done) {
+        gemv(ybuf);
+        ok = grid_sync(gb, nbk, gen);
+    }
+    while (ok && !s.done) {
+        const double* yk = ybuf + (par ? N : 0);
+        double* ynext = ybuf + (par ? 0 : N);
+        // y -= (beta/oldb) r1 (from the 2nd iteration); alfa = v . y
+        const bool sub = s.its >= 1;
+        const double f = sub ? s.beta / s.oldb : 0.0;
+        acc = 0.0;
+        for (int i = t; i < N; i += VB) {
+            double ya = yk[i];
+            if (sub) ya -= f * r1[i];
+            y[i] = ya;
+            acc += v[i] * ya;
+        }
+        const double alfa = block_reduce(acc, sh);
+        // r2_new = y - (alfa/beta) r2; z = P^-1 r2_new per colloid block; beta_new^2 = r2_new . z
+        const double f2 = alfa / s.beta;
+        acc = 0.0;
+        for (int cb = t; cb < ncb; cb += VB) {
+            double r[5], zz[5];
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                const int i = 5 * cb + k;
+                r[k] = y[i] - f2 * r2[i];
+            }
+            ms_prec5(r, zz);
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                const int i = 5 * cb + k;
+                y[i] = r[k];   // r2_new (y' is dead once read)
+                z[i] = zz[k];
+                acc += r[k] * zz[k];
+            }
+        }
+        const double bb = block_reduce(acc, sh);
+        s.alfa = alfa;
+        s.oldb = s.beta;
+        s.beta = sqrt(fmax(bb, 0.0));
+        const double oldeps = s.epsln;
+        const double delta = s.cs * s.dbar + s.sn * alfa;
+        const double gbar = s.sn * s.dbar - s.cs * alfa;
+        s.epsln = s.sn * s.beta;
+        s.dbar = -s.cs * s.beta;
+        const double gamma = fmax(sqrt(gbar * gbar + s.beta * s.beta), 1e-300);
+        s.cs = gbar / gamma;
+        s.sn = s.beta / gamma;
+        const double phi = s.cs * s.phibar;
+        s.phibar = s.sn * s.phibar;
+        s.oldeps = oldeps;
+        s.delta = delta;
+        s.gamma = gamma;
+        s.phi = phi;
+        s.its += 1;
+        if (s.phibar <= s.tol) {
+            s.done = 1;
+            s.reason = (s.phibar <= abstol) ? 3 : 2;
+        } else if (s.beta == 0.0) {
+            s.done = 1;
+            s.reason = 2;
+        } else if (s.its >= maxit) {
+            s.done = 1;
+            s.reason = -3;
+        }
+        const double denom = 1.0 / gamma;
+        const double vs = (s.beta > 0.0) ? 1.0 / s.beta : 0.0;
+        // own rows: w_k = (v - oldeps w_{k-2} - delta w_{k-1}) / gamma, x += phi w_k
+        if (t < nr) {
+            const double wn = (v[rb + t] - oldeps * wb - delta * wa) * denom;
+            wb = wa;
+            wa = wn;
+            xo += phi * wn;
+        }
+        __syncthreads();   // every read of v and r2 of this iteration is done
+        // shift: r1 <- r2, r2 <- r2_new, v <- z / beta_new (each thread its own entries)
+        for (int i = t; i < N; i += VB) {
+            r1[i] = r2[i];
+            r2[i] = y[i];
+            v[i] = z[i] * vs;
+        }
+        __syncthreads();
+        par ^= 1;
+        if (s.done) break;
+        gemv(ynext);
+        ok = grid_sync(gb, nbk, gen);
+    }
+    if (t < nr) x[rb + t] = xo;
+    if (blockIdx.x == 0 && t == 0) {
+        if (!ok) {
+            s.done = 1;
+            s.reason = -5;
+            dcnt[DC_BARRIER_TIMEOUT] = 1;
+        }
+        *st_out = s;
+        dcnt[DC_MSS_ITERS] += s.its;
+        if (s.reason < 0) dcnt[DC_MSS_CAPPED] += 1;
+    }
+}
+
+static ps_err minres_small_info(Ctx* c, ps_solve_info* info);
 // Solve A x = b for the small dense block A (N x N, row-major) by the recurrence above in one launch.
 // With `info` the final state is read back (a host round trip); without, nothing is.
 static ps_err minres_small(Ctx* c, int64_t N, const double* A, const double* b, double* x, double rtol, int maxit,
@@ -782,6 +973,18 @@ static ps_err minres_small(Ctx* c, int64_t N, const double* A, const double* b, 
     }
     // at most one block per CU, so the whole grid is co-resident for its barriers
     const int ncu = std::max(1, c->n_cu);
+    // as many rows per block as the registers hold: fewer blocks meet at each barrier
+    const int K = (int)((N + VB - 1) / VB);
+    const int rows_lds = std::min(MS_MAXR, (MS_MAXR * MS_MAXK) / std::max(K, 1));
+    if (N <= MS_LDS_N && (N + rows_lds - 1) / rows_lds <= ncu) {
+        const int rows = rows_lds;
+        const unsigned nb = (unsigned)((N + rows - 1) / rows);
+        const size_t lds = sizeof(double) * 5 * (size_t)N;
+        k_ms_solve_lds<<<nb, VB, lds, st>>>((int)N, A, b, x, c->kr, rows, rtol, c->p.ksp_abstol, maxit, c->gbar,
+                                             c->mstate, c->dcnt);
+        PS_LAUNCH(c);
+        return minres_small_info(c, info);
+    }
     int rows = (int)std::max<int64_t>(8, (N + 255) / 256);
     while ((N + rows - 1) / rows > ncu) rows *= 2;
     const unsigned nb = (unsigned)((N + rows - 1) / rows);
@@ -789,6 +992,12 @@ static ps_err minres_small(Ctx* c, int64_t N, const double* A, const double* b, 
     k_ms_solve<<<nb, VB, lds, st>>>(N, A, b, x, set[0], set[1], rows, rtol, c->p.ksp_abstol, maxit, c->gbar,
                                      c->mstate, c->dcnt);
     PS_LAUNCH(c);
+    return minres_small_info(c, info);
+}
+
+// The stresslet solve's outcome, read back only when asked for (a host round trip).
+static ps_err minres_small_info(Ctx* c, ps_solve_info* info) {
+    hipStream_t st = c->stream;
     if (!info) return PS_OK;
     PS_HIP(c, hipMemcpyAsync(c->mstate_h, c->mstate, sizeof(MinresState), hipMemcpyDeviceToHost, st));
     PS_HIP(c, hipStreamSynchronize(st));

@@ -391,21 +391,7 @@ __device__ __forceinline__ void aa_sym_tile(int Nm, int I, int J, const AAConst&
     }
 }
 
-// The pairs that involve the partial last tile come first (npad of them), so their blocks start
-// early and overlap the rest of the launch instead of forming a tail.
-template <class T, bool BOX>
-__global__ __launch_bounds__(PS_BLOCK) void k_aa_sym(int Nm, const int2* __restrict__ pairs, int npad, AAConst k,
-                                                     const typename AAPos<T, BOX>::P* __restrict__ xs,
-                                                     const double* __restrict__ f, T* __restrict__ part) {
-    __shared__ AASmem<T, BOX> sm;
-    const int2 pr = pairs[blockIdx.x];
-    if ((int)blockIdx.x < npad)
-        aa_sym_tile<T, BOX, true>(Nm, pr.x, pr.y, k, xs, f, part, sm);
-    else
-        aa_sym_tile<T, BOX, false>(Nm, pr.x, pr.y, k, xs, f, part, sm);
-}
-
-// Scaled monomer positions for k_aa_sym: [Nm][4] (x, y, z, 0) * inv_L.
+// Scaled monomer positions for the symmetric tile pairs: [Nm][4] (x, y, z, 0) * inv_L.
 template <class T>
 __global__ void k_aa_pos(int Nm, const double* __restrict__ xm, double inv_L, T* __restrict__ xs) {
     const int i = blockIdx.x * PS_BLOCK + threadIdx.x;
@@ -416,7 +402,7 @@ __global__ void k_aa_pos(int Nm, const double* __restrict__ xm, double inv_L, T*
     xs[4 * i + 3] = (T)0;
 }
 
-// Fixed-point monomer positions for the FP32 symmetric kernel in a box: [Nm][4] (x, y, z, 0) with
+// Fixed-point monomer positions for the FP32 symmetric tile pairs in a box: [Nm][4] (x, y, z, 0) with
 // x / L wrapped into [-1/2, 1/2] and scaled by 2^32 (the ends meet: 2^31 wraps to -2^31).
 __global__ void k_aa_pos_fix(int Nm, const double* __restrict__ xm, double inv_L, int* __restrict__ xs) {
     const int i = blockIdx.x * PS_BLOCK + threadIdx.x;
@@ -433,18 +419,26 @@ __global__ void k_aa_pos_fix(int Nm, const double* __restrict__ xm, double inv_L
 
 // ---------------------------------------------------------------------------- AB, monomer rows
 // u_m += stale . sum_c F_c + y12b (e x T_c) - x12g (e.S^e) e + 2 y12g (S^e - (e.S^e) e)
+template <class T>
+struct ABMonoSmem {
+    T cx[PS_BLOCK], cy[PS_BLOCK], cz[PS_BLOCK];
+    T cv[11][PS_BLOCK];
+};
+
+// One block of monomer rows (bx) against one chunk of colloid sources (by).
 template <class T, bool BOX>
-__global__ __launch_bounds__(PS_BLOCK) void k_ab_mono(SysConst s, const T* __restrict__ xm,
-                                                      const double* __restrict__ f,
-                                                      const double* __restrict__ stale, int per_cchunk,
-                                                      T* __restrict__ part) {
-    __shared__ T cx[PS_BLOCK], cy[PS_BLOCK], cz[PS_BLOCK];
-    __shared__ T cv[11][PS_BLOCK];
+__device__ __forceinline__ void ab_mono_block(int bx, int by, const SysConst& s, const T* __restrict__ xm,
+                                              const double* __restrict__ f, const double* __restrict__ stale,
+                                              int per_cchunk, T* __restrict__ part, ABMonoSmem<T>& sm) {
+    T* cx = sm.cx;
+    T* cy = sm.cy;
+    T* cz = sm.cz;
+    T(*cv)[PS_BLOCK] = sm.cv;
     const int Nm = s.Nm;
-    // colloid sources [cb, ce) of this block's chunk (blockIdx.y)
-    const int cb = blockIdx.y * per_cchunk;
+    // colloid sources [cb, ce) of this block's chunk
+    const int cb = by * per_cchunk;
     const int ce = min(s.Nc, cb + per_cchunk);
-    const int i = blockIdx.x * PS_BLOCK + threadIdx.x;
+    const int i = bx * PS_BLOCK + threadIdx.x;
     const bool act = i < Nm;
     T xi = 0, yi = 0, zi = 0;
     if (act) {
@@ -502,21 +496,29 @@ __global__ __launch_bounds__(PS_BLOCK) void k_ab_mono(SysConst s, const T* __res
         ux += (T)stale[0] * Fx + (T)stale[1] * Fy + (T)stale[2] * Fz;
         uy += (T)stale[3] * Fx + (T)stale[4] * Fy + (T)stale[5] * Fz;
         uz += (T)stale[6] * Fx + (T)stale[7] * Fy + (T)stale[8] * Fz;
-        part[((int64_t)blockIdx.y * 3 + 0) * Nm + i] = ux;
-        part[((int64_t)blockIdx.y * 3 + 1) * Nm + i] = uy;
-        part[((int64_t)blockIdx.y * 3 + 2) * Nm + i] = uz;
+        part[((int64_t)by * 3 + 0) * Nm + i] = ux;
+        part[((int64_t)by * 3 + 1) * Nm + i] = uy;
+        part[((int64_t)by * 3 + 2) * Nm + i] = uz;
     }
+}
+
+template <class T, bool BOX>
+__global__ __launch_bounds__(PS_BLOCK) void k_ab_mono(SysConst s, const T* __restrict__ xm,
+                                                      const double* __restrict__ f,
+                                                      const double* __restrict__ stale, int per_cchunk,
+                                                      T* __restrict__ part) {
+    __shared__ ABMonoSmem<T> sm;
+    ab_mono_block<T, BOX>(blockIdx.x, blockIdx.y, s, xm, f, stale, per_cchunk, part, sm);
 }
 
 // ---------------------------------------------------------------------------- AB, colloid rows
 // Omega_c += y12b (f_m x e);  E_c += flatten(G), G_ij = sum_k f_k gt_kij;  also sum_m f_m.
+// One colloid row set (c) against one chunk of monomer sources (ch).
 template <class T, bool BOX>
-__global__ __launch_bounds__(PS_BLOCK) void k_ab_coll(SysConst s, const T* __restrict__ xm,
-                                                      const double* __restrict__ f, int per_chunk,
-                                                      T* __restrict__ part_c, T* __restrict__ part_fs) {
-    __shared__ T sh[4][11];
+__device__ __forceinline__ void ab_coll_block(int c, int ch, const SysConst& s, const T* __restrict__ xm,
+                                              const double* __restrict__ f, int per_chunk, T* __restrict__ part_c,
+                                              T* __restrict__ part_fs, T (*sh)[11]) {
     const int Nm = s.Nm, Nc = s.Nc;
-    const int c = blockIdx.x, ch = blockIdx.y;
     const int g = Nm + c;
     const T xc = xm[3 * g], yc = xm[3 * g + 1], zc = xm[3 * g + 2];
     const T gA = (T)(C9D20 * (3 + 5 * s.beta2));
@@ -576,6 +578,52 @@ __global__ __launch_bounds__(PS_BLOCK) void k_ab_coll(SysConst s, const T* __res
             part_fs[3 * ch + 1] = acc[1];
             part_fs[3 * ch + 2] = acc[2];
         }
+    }
+}
+
+template <class T, bool BOX>
+__global__ __launch_bounds__(PS_BLOCK) void k_ab_coll(SysConst s, const T* __restrict__ xm,
+                                                      const double* __restrict__ f, int per_chunk,
+                                                      T* __restrict__ part_c, T* __restrict__ part_fs) {
+    __shared__ T sh[4][11];
+    ab_coll_block<T, BOX>(blockIdx.x, blockIdx.y, s, xm, f, per_chunk, part_c, part_fs, sh);
+}
+
+// The monomer-monomer tile pairs and, behind them in the same grid, the monomer rows against the
+// colloid sources and the colloid rows against the monomer sources: the small colloid blocks fill
+// the tail of the tile-pair blocks instead of waiting in launches of their own.  Blocks [0, nsym)
+// are tile pairs (the ghost-bead pairs first), then nmono = gx * nchm monomer-row blocks, then
+// Nc * nchc colloid-row blocks.
+struct ApplyGrid {
+    int nsym, npad, gx, nmono, ncoll;
+    int per_cchunk, per_chunk;
+};
+
+template <class T, bool BOX>
+__global__ __launch_bounds__(PS_BLOCK) void k_apply_sym(SysConst s, ApplyGrid G, const int2* __restrict__ pairs,
+                                                        AAConst k, const typename AAPos<T, BOX>::P* __restrict__ xs,
+                                                        const T* __restrict__ xm, const double* __restrict__ f,
+                                                        const double* __restrict__ stale, T* __restrict__ part_aa,
+                                                        T* __restrict__ part_abm, T* __restrict__ part_c,
+                                                        T* __restrict__ part_fs) {
+    constexpr size_t SZ_AA = sizeof(AASmem<T, BOX>), SZ_M = sizeof(ABMonoSmem<T>);
+    __shared__ __align__(16) unsigned char smem[SZ_AA > SZ_M ? SZ_AA : SZ_M];
+    const int b = blockIdx.x;
+    if (b < G.nsym) {
+        const int2 pr = pairs[b];
+        AASmem<T, BOX>& sm = *reinterpret_cast<AASmem<T, BOX>*>(smem);
+        if (b < G.npad)
+            aa_sym_tile<T, BOX, true>(s.Nm, pr.x, pr.y, k, xs, f, part_aa, sm);
+        else
+            aa_sym_tile<T, BOX, false>(s.Nm, pr.x, pr.y, k, xs, f, part_aa, sm);
+    } else if (b < G.nsym + G.nmono) {
+        const int q = b - G.nsym;
+        ab_mono_block<T, BOX>(q % G.gx, q / G.gx, s, xm, f, stale, G.per_cchunk, part_abm,
+                              *reinterpret_cast<ABMonoSmem<T>*>(smem));
+    } else {
+        const int q = b - G.nsym - G.nmono;
+        ab_coll_block<T, BOX>(q % s.Nc, q / s.Nc, s, xm, f, G.per_chunk, part_c, part_fs,
+                              reinterpret_cast<T(*)[11]>(smem));
     }
 }
 
@@ -798,102 +846,113 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
         PS_TRY(c->prof_record(1, &ea0, &ea1));
         PS_HIP(c, hipEventRecord(ea0, st));
     }
-    // The colloid-source rows of the monomers, the colloid rows and the colloid pairs do not depend
-    // on the monomer-monomer tile kernel: on the side stream they run beside it (and fill its tail);
-    // the finalize waits for both.
-    auto side_kernels = [&](hipStream_t ss, bool mono) -> ps_err {
-        if (mono) {
-            dim3 g2((s.Nm + PS_BLOCK - 1) / PS_BLOCK, aw.nchunk_m);
-            if (s.box_active)
-                k_ab_mono<T, true><<<g2, PS_BLOCK, 0, ss>>>(s, xm, f, c->stale, aw.per_chunk_m, (T*)aw.part_abm);
+    const bool box = s.box_active, ch = s.chain_HI;
+    // colloid pairs (FP64, every form)
+    if (s.Nc > 0) {
+        k_bb<<<s.Nc, PS_BLOCK, 0, st>>>(s, c->xm, f, aw.bb);
+        PS_LAUNCH(c);
+    }
+    if (srows) {
+        // the stresslet rows only: the colloid rows against the monomer sources
+        if (s.Nc > 0 && s.Nm > 0) {
+            dim3 g3(s.Nc, aw.nchunk_c);
+            if (box)
+                k_ab_coll<T, true><<<g3, PS_BLOCK, 0, st>>>(s, xm, f, aw.per_chunk_c, (T*)aw.part_c, (T*)aw.part_fs);
             else
-                k_ab_mono<T, false><<<g2, PS_BLOCK, 0, ss>>>(s, xm, f, c->stale, aw.per_chunk_m, (T*)aw.part_abm);
+                k_ab_coll<T, false><<<g3, PS_BLOCK, 0, st>>>(s, xm, f, aw.per_chunk_c, (T*)aw.part_c, (T*)aw.part_fs);
             PS_LAUNCH(c);
         }
-        if (s.Nc > 0) {
-            if (s.Nm > 0) {
+    } else if (s.Nm > 0 && aw.sym) {
+        // one launch: monomer tile pairs, then the monomer-colloid blocks in its tail.  Profiling: the
+        // launch's own dispatch records its start/stop (hipExtLaunchKernel); under rocprofv3 (whose
+        // dispatch interception crashes the process together with hipExtLaunchKernel's events) it is
+        // bracketed by event records.
+        const bool dispatch_ev = c->prof_dispatch_events;
+        if (c->prof_on) PS_TRY(c->prof_record(0, &ek0, &ek1));
+        if (c->prof_on && !dispatch_ev) PS_HIP(c, hipEventRecord(ek0, st));
+        AAConst k{};
+        constexpr bool f32 = sizeof(T) == sizeof(float);
+        // the unit of the separations: box units, or box / 2^32 for the fixed-point FP32 form
+        const double L = box ? (f32 ? s.L[0] / 4294967296.0 : s.L[0]) : 1.0;
+        k.c1 = 3.0 / (4.0 * L);
+        k.c2 = 1.0 / (2.0 * L * L * L);
+        k.c3 = -3.0 / (8.0 * L);
+        k.c4 = -3.0 / (2.0 * L * L * L);
+        k.L = L;
+        k.tb2s = s.two_beta * s.two_beta / (L * L);
+        // ghost beads sit at the origin with f = 0: the clamp keeps every coefficient finite
+        k.clamp = f32 ? (box ? 1.0 : 1e-30) : 1e-200;
+        ApplyGrid G{};
+        G.nsym = aw.npair_full + aw.npair_pad;
+        G.npad = aw.npair_pad;
+        G.gx = (s.Nm + PS_BLOCK - 1) / PS_BLOCK;
+        G.nmono = s.Nc > 0 ? G.gx * aw.nchunk_m : 0;
+        G.ncoll = s.Nc > 0 ? s.Nc * aw.nchunk_c : 0;
+        G.per_cchunk = aw.per_chunk_m;
+        G.per_chunk = aw.per_chunk_c;
+        const unsigned nblk = (unsigned)(G.nsym + G.nmono + G.ncoll);
+        T* pa = (T*)aw.part_aa;
+        T* pm = (T*)aw.part_abm;
+        T* pc = (T*)aw.part_c;
+        T* pf = (T*)aw.part_fs;
+        using PB = const typename AAPos<T, true>::P*;
+        using PN = const typename AAPos<T, false>::P*;
+        if (dispatch_ev) {
+            if (box)
+                hipExtLaunchKernelGGL(k_apply_sym<T, true>, dim3(nblk), dim3(PS_BLOCK), 0, st, ek0, ek1, 0, s, G,
+                                      aw.pairs, k, (PB)aw.xs, xm, f, c->stale, pa, pm, pc, pf);
+            else
+                hipExtLaunchKernelGGL(k_apply_sym<T, false>, dim3(nblk), dim3(PS_BLOCK), 0, st, ek0, ek1, 0, s, G,
+                                      aw.pairs, k, (PN)aw.xs, xm, f, c->stale, pa, pm, pc, pf);
+        } else if (box) {
+            k_apply_sym<T, true><<<nblk, PS_BLOCK, 0, st>>>(s, G, aw.pairs, k, (PB)aw.xs, xm, f, c->stale, pa, pm, pc,
+                                                            pf);
+        } else {
+            k_apply_sym<T, false><<<nblk, PS_BLOCK, 0, st>>>(s, G, aw.pairs, k, (PN)aw.xs, xm, f, c->stale, pa, pm,
+                                                             pc, pf);
+        }
+        PS_LAUNCH(c);
+        if (c->prof_on && !dispatch_ev) PS_HIP(c, hipEventRecord(ek1, st));
+    } else {
+        // the ordered monomer kernel (chain_HI, non-cubic boxes, small systems), then the
+        // monomer-colloid kernels
+        if (s.Nm > 0) {
+            if (c->prof_on) PS_TRY(c->prof_record(0, &ek0, &ek1));
+            if (c->prof_on) PS_HIP(c, hipEventRecord(ek0, st));
+            dim3 g1((s.Nm + PS_BLOCK - 1) / PS_BLOCK, aw.nsplit_aa);
+            if (box && ch)
+                k_aa<T, true, true><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
+            else if (box)
+                k_aa<T, true, false><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
+            else if (ch)
+                k_aa<T, false, true><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
+            else
+                k_aa<T, false, false><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
+            PS_LAUNCH(c);
+            if (c->prof_on) PS_HIP(c, hipEventRecord(ek1, st));
+            if (s.Nc > 0) {
+                dim3 g2((s.Nm + PS_BLOCK - 1) / PS_BLOCK, aw.nchunk_m);
+                if (box)
+                    k_ab_mono<T, true><<<g2, PS_BLOCK, 0, st>>>(s, xm, f, c->stale, aw.per_chunk_m, (T*)aw.part_abm);
+                else
+                    k_ab_mono<T, false><<<g2, PS_BLOCK, 0, st>>>(s, xm, f, c->stale, aw.per_chunk_m, (T*)aw.part_abm);
+                PS_LAUNCH(c);
                 dim3 g3(s.Nc, aw.nchunk_c);
-                if (s.box_active)
-                    k_ab_coll<T, true><<<g3, PS_BLOCK, 0, ss>>>(s, xm, f, aw.per_chunk_c, (T*)aw.part_c,
+                if (box)
+                    k_ab_coll<T, true><<<g3, PS_BLOCK, 0, st>>>(s, xm, f, aw.per_chunk_c, (T*)aw.part_c,
                                                                 (T*)aw.part_fs);
                 else
-                    k_ab_coll<T, false><<<g3, PS_BLOCK, 0, ss>>>(s, xm, f, aw.per_chunk_c, (T*)aw.part_c,
+                    k_ab_coll<T, false><<<g3, PS_BLOCK, 0, st>>>(s, xm, f, aw.per_chunk_c, (T*)aw.part_c,
                                                                  (T*)aw.part_fs);
                 PS_LAUNCH(c);
             }
-            k_bb<<<s.Nc, PS_BLOCK, 0, ss>>>(s, c->xm, f, aw.bb);
-            PS_LAUNCH(c);
         }
-        return PS_OK;
-    };
-    const bool forked = s.Nm > 0 && !srows;
-    if (forked) {
-        PS_HIP(c, hipEventRecord(c->side_fork, st));
-        PS_HIP(c, hipStreamWaitEvent(c->side, c->side_fork, 0));
-        PS_TRY(side_kernels(c->side, true));
-        PS_HIP(c, hipEventRecord(c->side_join, c->side));
-    } else {
-        PS_TRY(side_kernels(st, false));
     }
-    if (s.Nm > 0 && !srows) {
-        dim3 g1((s.Nm + PS_BLOCK - 1) / PS_BLOCK, aw.nsplit_aa);
-        // profiling: the symmetric kernel's own dispatch records its start/stop (hipExtLaunchKernel);
-        // the ordered kernel, and every kernel under rocprofv3 (whose dispatch interception crashes
-        // the process together with hipExtLaunchKernel's events), is bracketed by event records
-        const bool dispatch_ev = aw.sym && c->prof_dispatch_events;
-        if (c->prof_on) PS_TRY(c->prof_record(0, &ek0, &ek1));
-        if (c->prof_on && !dispatch_ev) PS_HIP(c, hipEventRecord(ek0, st));
-        const bool box = s.box_active, ch = s.chain_HI;
-        if (aw.sym) {
-            AAConst k{};
-            constexpr bool f32 = sizeof(T) == sizeof(float);
-            // the unit of the separations: box units, or box / 2^32 for the fixed-point FP32 form
-            const double L = box ? (f32 ? s.L[0] / 4294967296.0 : s.L[0]) : 1.0;
-            k.c1 = 3.0 / (4.0 * L);
-            k.c2 = 1.0 / (2.0 * L * L * L);
-            k.c3 = -3.0 / (8.0 * L);
-            k.c4 = -3.0 / (2.0 * L * L * L);
-            k.L = L;
-            k.tb2s = s.two_beta * s.two_beta / (L * L);
-            // ghost beads sit at the origin with f = 0: the clamp keeps every coefficient finite
-            k.clamp = f32 ? (box ? 1.0 : 1e-30) : 1e-200;
-            const void* xs = aw.xs;
-            const int nblk = aw.npair_full + aw.npair_pad;
-            const int2* pairs = aw.pairs;
-            const int npad = aw.npair_pad;
-            T* part = (T*)aw.part_aa;
-            using PB = const typename AAPos<T, true>::P*;
-            using PN = const typename AAPos<T, false>::P*;
-            if (dispatch_ev) {
-                if (box)
-                    hipExtLaunchKernelGGL(k_aa_sym<T, true>, dim3(nblk), dim3(PS_BLOCK), 0, st, ek0, ek1, 0, s.Nm,
-                                          pairs, npad, k, (PB)xs, f, part);
-                else
-                    hipExtLaunchKernelGGL(k_aa_sym<T, false>, dim3(nblk), dim3(PS_BLOCK), 0, st, ek0, ek1, 0, s.Nm,
-                                          pairs, npad, k, (PN)xs, f, part);
-            } else if (box) {
-                k_aa_sym<T, true><<<nblk, PS_BLOCK, 0, st>>>(s.Nm, pairs, npad, k, (PB)xs, f, part);
-            } else {
-                k_aa_sym<T, false><<<nblk, PS_BLOCK, 0, st>>>(s.Nm, pairs, npad, k, (PN)xs, f, part);
-            }
-            PS_LAUNCH(c);
-        } else if (box && ch)
-            k_aa<T, true, true><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
-        else if (box)
-            k_aa<T, true, false><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
-        else if (ch)
-            k_aa<T, false, true><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
-        else
-            k_aa<T, false, false><<<g1, PS_BLOCK, 0, st>>>(s, xm, f, c->chain, aw.per_split_aa, (T*)aw.part_aa);
-        PS_LAUNCH(c);
-        if (c->prof_on && !dispatch_ev) PS_HIP(c, hipEventRecord(ek1, st));
-    }
-    if (forked) PS_HIP(c, hipStreamWaitEvent(st, c->side_join, 0));
     const int64_t r_begin = srows ? s.nu : 0;
     if (s.n - r_begin <= 0) return PS_OK;
     dim3 g4((unsigned)((s.n - r_begin + PS_BLOCK - 1) / PS_BLOCK));
     k_finalize<T><<<g4, PS_BLOCK, 0, st>>>(s, f, u, (const T*)aw.part_aa, s.Nm > 0 ? aw.nsplit_aa : 0,
-                                           (const T*)aw.part_abm, s.Nm > 0 ? aw.nchunk_m : 0,
+                                           (const T*)aw.part_abm, (s.Nm > 0 && s.Nc > 0) ? aw.nchunk_m : 0,
                                            (const T*)aw.part_c, (const T*)aw.part_fs,
                                            (s.Nm > 0 && s.Nc > 0) ? aw.nchunk_c : 0, aw.bb, c->stale,
                                            srows ? 0 : saddle, r_begin);

@@ -90,9 +90,6 @@ struct Ctx {
     int Np = 0, Nm = 0, Nc = 0;
     int64_t n = 0, nu = 0, ns = 0;
     hipStream_t stream = nullptr;
-    hipStream_t side = nullptr;     // second stream: the colloid-row and carried-tile kernels of an apply
-                                    // run on it beside the monomer-monomer tile kernel (default priority)
-    hipEvent_t side_fork = nullptr, side_join = nullptr;
     int device = 0;
     int n_cu = 0;
     std::string err;

@@ -61,6 +61,10 @@ __device__ inline bool tql2_wave(int m, double& dv, double& ev, double* Z) {
                 double s = 1.0, c = 1.0, p = 0.0;
                 bool underflow = false;
                 double dip1 = tql2_rl(dv, mm);   // d[i + 1] before this sweep changes it
+                // column i + 1 of Z, carried in a register from one rotation to the next: each rotation
+                // reads only column i, which this sweep has not written yet
+                double zc = lane < m ? Z[lane + mm * m] : 0.0;
+                int ilast = mm;
                 for (int i = mm - 1; i >= l; i--) {
                     const double ei = tql2_rl(ev, i), di = tql2_rl(dv, i);
                     const double f = s * ei, b = c * ei;
@@ -74,6 +78,7 @@ __device__ inline bool tql2_wave(int m, double& dv, double& ev, double* Z) {
                         underflow = true;
                         break;
                     }
+                    ilast = i;
                     const double y = tql2_rsqrt(t);
                     r = t * y;
                     s = f * y;
@@ -88,12 +93,12 @@ __device__ inline bool tql2_wave(int m, double& dv, double& ev, double* Z) {
                     g = c * rr - b;
                     dip1 = di;
                     if (lane < m) {
-                        const double zf = Z[lane + (i + 1) * m];
                         const double zi = Z[lane + i * m];
-                        Z[lane + (i + 1) * m] = s * zi + c * zf;
-                        Z[lane + i * m] = c * zi - s * zf;
+                        Z[lane + (i + 1) * m] = s * zi + c * zc;
+                        zc = c * zi - s * zc;
                     }
                 }
+                if (lane < m) Z[lane + ilast * m] = zc;   // the column the last rotation left in zc
                 if (underflow) continue;
                 if (lane == l) {
                     dv -= p;
