@@ -41,8 +41,9 @@ PEAK_FP32_VALU = 78.6e12
 # HBM bytes per launch of the fused apply kernel, from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE
 # of `bench.py --profile-form <form> --steps 2 --warmup 0 --no-cpu --calibrate-copy 256`, each scaled by
 # what the counter reads for the 256 MiB calibration copy; keyed by (workload, tiles).  None until
-# measured for that kernel.
-TRAFFIC_APPLY = {}
+# measured for that kernel.  profiles/k_apply_sym_hbm_traffic_n20000.md: FP32, 9,107.5 KiB fetched and
+# 19,761.0 KiB written per launch (FETCH_SIZE 0.50x, WRITE_SIZE 1.00x of the calibration copy).
+TRAFFIC_APPLY = {("c3", "fp32"): (9107.5 + 19761.0) * 1024}
 
 
 def ALGO_BYTES(Nm, Nc, elt):
@@ -439,6 +440,9 @@ def main():
                          "(default c5dense, N = 8,000) with STORAGE; prints its own JSON line")
     ap.add_argument("--profile-form", default=None,
                     help="only run warmup + timed steps of this form (for a profiler), no JSON line")
+    ap.add_argument("--profile-events", action="store_true",
+                    help="with --profile-form: time the apply kernel's launches with the library's events over "
+                         "the timed steps (the measurement the JSON line's roofline uses) and log the average")
     ap.add_argument("--arrow", action="store_true",
                     help="instead: the mm_HI=False arrowhead regime on the reference's published case; prints "
                          "its own JSON line")
@@ -497,10 +501,13 @@ def main():
     if args.minres:
         forms["fp64/minres"] = dict(precision=0, solver=0)
     if args.profile_form:
-        ctx = make_ctx(Context, pd, x0, **forms[args.profile_form])
-        dt, prof, per = timed_run(ctx, args.steps, args.warmup, args.profile_form, profile=False)
+        ctx = make_ctx(Context, pd, x0, graph=0, **forms[args.profile_form])
+        dt, prof, per = timed_run(ctx, args.steps, args.warmup, args.profile_form, profile=args.profile_events)
         log(f"{args.profile_form}: {Np * args.steps / dt:.4g} bead-steps/s, {1e3 * dt / args.steps:.2f} ms/step, "
             f"per step {per}")
+        if prof:
+            log(f"{args.profile_form}: apply kernel {prof['aa_launches']} launches, average "
+                f"{prof['aa_ms'] / max(prof['aa_launches'], 1) * 1e3:.2f} us (library events)")
         ctx.close()
         return
 
@@ -525,8 +532,8 @@ def main():
                              graph_ms_per_step=modes[1]["ms_per_step"])
         # kernel timing in a separate run (the library records events around the launches, which
         # turns graph replay off): the fused monomer apply kernel, and each whole apply
-        ctx = make_ctx(Context, pd, x0, **over)
-        _, prof, _ = timed_run(ctx, 3, args.warmup, f"{form} kernel timing", profile=True)
+        ctx = make_ctx(Context, pd, x0, graph=0, **over)
+        _, prof, _ = timed_run(ctx, args.steps, args.warmup, f"{form} kernel timing", profile=True)
         ctx.close()
         results[form]["apply_kernel_ms_avg"] = prof["aa_ms"] / max(prof["aa_launches"], 1)
         results[form]["apply_kernel_launches"] = prof["aa_launches"]

@@ -863,13 +863,12 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
             PS_LAUNCH(c);
         }
     } else if (s.Nm > 0 && aw.sym) {
-        // one launch: monomer tile pairs, then the monomer-colloid blocks in its tail.  Profiling: the
-        // launch's own dispatch records its start/stop (hipExtLaunchKernel); under rocprofv3 (whose
-        // dispatch interception crashes the process together with hipExtLaunchKernel's events) it is
-        // bracketed by event records.
-        const bool dispatch_ev = c->prof_dispatch_events;
-        if (c->prof_on) PS_TRY(c->prof_record(0, &ek0, &ek1));
-        if (c->prof_on && !dispatch_ev) PS_HIP(c, hipEventRecord(ek0, st));
+        // one launch: monomer tile pairs, then the monomer-colloid blocks in its tail.  Profiling:
+        // the launch is bracketed by event records on the library's stream.
+        if (c->prof_on) {
+            PS_TRY(c->prof_record(0, &ek0, &ek1));
+            PS_HIP(c, hipEventRecord(ek0, st));
+        }
         AAConst k{};
         constexpr bool f32 = sizeof(T) == sizeof(float);
         // the unit of the separations: box units, or box / 2^32 for the fixed-point FP32 form
@@ -897,22 +896,14 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
         T* pf = (T*)aw.part_fs;
         using PB = const typename AAPos<T, true>::P*;
         using PN = const typename AAPos<T, false>::P*;
-        if (dispatch_ev) {
-            if (box)
-                hipExtLaunchKernelGGL(k_apply_sym<T, true>, dim3(nblk), dim3(PS_BLOCK), 0, st, ek0, ek1, 0, s, G,
-                                      aw.pairs, k, (PB)aw.xs, xm, f, c->stale, pa, pm, pc, pf);
-            else
-                hipExtLaunchKernelGGL(k_apply_sym<T, false>, dim3(nblk), dim3(PS_BLOCK), 0, st, ek0, ek1, 0, s, G,
-                                      aw.pairs, k, (PN)aw.xs, xm, f, c->stale, pa, pm, pc, pf);
-        } else if (box) {
+        if (box)
             k_apply_sym<T, true><<<nblk, PS_BLOCK, 0, st>>>(s, G, aw.pairs, k, (PB)aw.xs, xm, f, c->stale, pa, pm, pc,
                                                             pf);
-        } else {
+        else
             k_apply_sym<T, false><<<nblk, PS_BLOCK, 0, st>>>(s, G, aw.pairs, k, (PN)aw.xs, xm, f, c->stale, pa, pm,
                                                              pc, pf);
-        }
         PS_LAUNCH(c);
-        if (c->prof_on && !dispatch_ev) PS_HIP(c, hipEventRecord(ek1, st));
+        if (c->prof_on) PS_HIP(c, hipEventRecord(ek1, st));
     } else {
         // the ordered monomer kernel (chain_HI, non-cubic boxes, small systems), then the
         // monomer-colloid kernels
