@@ -178,6 +178,17 @@ ps_err ps_stale_tile(void* ctx, double* tile9);
 /* u = M f (n entries), FP64 or the context's precision.  ~ the mobility block of MatMult(A). */
 ps_err ps_mob_mult(void* ctx, const double* f, double* u);
 ps_err ps_mob_mult_dev(void* ctx, const double* f_dev, double* u_dev);
+/* u = M f with no context and no allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  M is
+ * the FP64 grand mobility mob() would define at the positions x (3 Np, device), with the parameters *p
+ * (mm_HI = 1; box, chain_HI, beta, Nm, Nc, Nmono_per_chain are read); f and u (n = 3 Nm + 11 Nc,
+ * device) are the caller's; the work is ordered on `stream` (a hipStream_t, NULL = the null stream).
+ * Partial sums, the carried tile and the chain ids live in one 16 MiB device array per GPU, held until
+ * the process ends: systems that need more (about Nm > 300,000) are rejected with PS_ERR_ARG.  Calls on
+ * one GPU share that array, so calls that overlap in time on one GPU race; errors are read with
+ * ps_last_error(NULL). */
+ps_err ps_mob_mult_noalloc(const ps_params* p, const double* x_dev, const double* f_dev, double* u_dev,
+                           void* stream);
 /* MATOP_MULT of the saddle operator A = [[M, B], [B^T, 0]], B = [-I_{3Nm+6Nc}; 0]
  * (src/matfree_A.cpp:38-80, src/arrays.cpp:349-408): y = A x, ns entries. */
 ps_err ps_saddle_mult(void* ctx, const double* x, double* y);

@@ -422,6 +422,9 @@ thread_local std::string g_create_error;
 
 }  // namespace
 
+// Errors of the entry points that have no context (ps_last_error(NULL) reads them).
+void ps_set_context_free_error(const std::string& msg) { g_create_error = msg; }
+
 ps_err Ctx::prof_harvest() {
     Ctx* c = this;
     if (ev_used == 0) return PS_OK;

@@ -22,6 +22,8 @@
 #include <hip/hip_ext.h>
 
 #include <type_traits>
+#include <algorithm>
+#include <mutex>
 
 namespace {
 
@@ -1087,4 +1089,177 @@ ps_err saddle_apply(Ctx* c, const double* x, double* y, int prec) {
     if (c->arrow) return arrow_apply(c, x, y, 1);
     if (prec == PS_PREC_MIXED && c->xm32) return launch_apply<float>(c, c->xm32, c->aw32, x, y, 1);
     return launch_apply<double>(c, c->xm, c->aw64, x, y, 1);
+}
+
+// ---------------------------------------------------------------------------- no-allocation apply
+// u = M f at positions x with no context and no allocation: the partial sums, the carried tile and
+// the chain ids live in one file-scope device array (each GPU the library runs on holds its own copy
+// until the process ends).  Host code never uses the array's name as an address: its device address
+// is looked up once per device and kept.
+namespace {
+
+constexpr size_t NOALLOC_DOUBLES = 2u << 20;   // 16 MiB
+__device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEV = 64;
+double* g_scratch_ptr[NOALLOC_MAX_DEV] = {};
+std::mutex g_scratch_mu;
+
+// stale[9..17] starts as the identity (the mob_a fill_self leaves, src/mob.cpp:65-86) for a system
+// with no processed monomer pair; the colloid-row partials start at zero; chain ids for chain_HI.
+__global__ void k_noalloc_init(double* __restrict__ st, int* __restrict__ chain, int Nm, int nmono,
+                               double* __restrict__ zero, int64_t nzero) {
+    const int64_t t = (int64_t)blockIdx.x * PS_BLOCK + threadIdx.x;
+    if (t < 18) st[t] = (t >= 9 && ((t - 9) % 4 == 0)) ? 1.0 : 0.0;
+    if (t < Nm) chain[t] = nmono > 0 ? (int)(t / nmono) : 0;
+    if (t < nzero) zero[t] = 0.0;
+}
+
+}  // namespace
+
+extern "C" ps_err ps_mob_mult_noalloc(const ps_params* p, const double* x, const double* f, double* u,
+                                      void* stream) {
+    if (!p || !x || !f || !u) {
+        ps_set_context_free_error("ps_mob_mult_noalloc: null argument");
+        return PS_ERR_ARG;
+    }
+    if (!p->mm_HI || p->Np != p->Nm + p->Nc || p->Np < 2 || p->Nm < 0 || p->Nc < 0 || !(p->beta > 0)) {
+        ps_set_context_free_error("ps_mob_mult_noalloc: needs mm_HI = 1, Np = Nm + Nc >= 2 and beta > 0");
+        return PS_ERR_ARG;
+    }
+    if (p->box_active && (p->box[0] <= 0 || p->box[1] <= 0 || p->box[2] <= 0)) {
+        ps_set_context_free_error("ps_mob_mult_noalloc: box lengths must be positive");
+        return PS_ERR_ARG;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEV) {
+        ps_set_context_free_error("ps_mob_mult_noalloc: no usable current device");
+        return PS_ERR_NODEV;
+    }
+    double* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {   // first call on this device, with it current
+            void* ptr = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !ptr) {
+                ps_set_context_free_error(std::string("ps_mob_mult_noalloc: hipGetSymbolAddress: ") +
+                                          hipGetErrorString(e));
+                return PS_ERR_HIP;
+            }
+            g_scratch_ptr[dev] = static_cast<double*>(ptr);
+        }
+        base = g_scratch_ptr[dev];
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipDeviceProp_t prop;
+    int ncu = 256;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) ncu = prop.multiProcessorCount;
+    SysConst s{};
+    s.Np = p->Np;
+    s.Nm = p->Nm;
+    s.Nc = p->Nc;
+    s.chain_HI = p->chain_HI;
+    s.box_active = p->box_active;
+    for (int d = 0; d < 3; d++) {
+        s.L[d] = p->box_active ? p->box[d] : 0.0;
+        s.Linv[d] = p->box_active ? 1.0 / p->box[d] : 0.0;
+    }
+    s.beta = p->beta;
+    s.beta2 = p->beta * p->beta;
+    s.beta_inv = 1.0 / p->beta;
+    s.two_beta = 2.0 * p->beta;
+    s.n = 3 * (int64_t)s.Nm + 11 * (int64_t)s.Nc;
+    s.nu = 3 * (int64_t)s.Nm + 6 * (int64_t)s.Nc;
+    s.ns = s.n + s.nu;
+    s.nm3 = 3 * (int64_t)s.Nm;
+    s.nc3 = 3 * (int64_t)s.Nc;
+    const int Nm = s.Nm, Nc = s.Nc;
+    // the last processed monomer pair of mob() (src/mob.cpp:845-897): chain ids i / Nmono_per_chain
+    const int nmono = p->Nmono_per_chain > 0 ? p->Nmono_per_chain : Nm;
+    int si = -1, sj = -1;
+    for (int i = Nm - 2; i >= 0 && si < 0; i--)
+        for (int j = Nm - 1; j > i; j--) {
+            if (p->chain_HI && i / nmono == j / nmono) continue;
+            si = i;
+            sj = j;
+            break;
+        }
+    // splits: the ordered monomer kernel over source splits of whole 256-bead tiles, the colloid
+    // chunks as in a context; all partials in the scratch array
+    const int ntb = (Nm + PS_BLOCK - 1) / PS_BLOCK;
+    const int target = 4 * ncu;
+    int nchm = Nc > 0 ? std::max(1, std::min((Nc + 31) / 32, ntb > 0 ? (target + ntb - 1) / ntb : 1)) : 1;
+    const int per_chunk_m = Nc > 0 ? (Nc + nchm - 1) / nchm : 1;
+    nchm = Nc > 0 ? (Nc + per_chunk_m - 1) / per_chunk_m : 1;
+    int nchc = Nc > 0 ? (2 * ncu + Nc - 1) / Nc : 1;
+    nchc = std::max(1, std::min(nchc, std::max(1, Nm / PS_BLOCK)));
+    const int per_chunk_c = Nm > 0 ? (Nm + nchc - 1) / nchc : 0;
+    nchc = Nm > 0 ? (Nm + per_chunk_c - 1) / per_chunk_c : 1;
+    auto al = [](size_t v) { return (v + 31) / 32 * 32; };
+    const size_t off_chain = 32, n_chain = al(((size_t)std::max(Nm, 1) + 1) / 2);
+    const size_t off_c = off_chain + n_chain, n_c = al((size_t)nchc * std::max(Nc, 1) * 12);
+    const size_t off_fs = off_c + n_c, n_fs = al((size_t)nchc * 3);
+    const size_t off_bb = off_fs + n_fs, n_bb = al((size_t)11 * std::max(Nc, 1));
+    const size_t off_abm = off_bb + n_bb, n_abm = al((size_t)nchm * 3 * std::max(Nm, 1));
+    const size_t off_aa = off_abm + n_abm;
+    if (off_aa + 3 * (size_t)std::max(Nm, 1) > NOALLOC_DOUBLES) {
+        ps_set_context_free_error("ps_mob_mult_noalloc: the system needs more than the " +
+                                  std::to_string(NOALLOC_DOUBLES * 8 >> 20) + " MiB scratch array (Nm = " +
+                                  std::to_string(Nm) + ", Nc = " + std::to_string(Nc) + ")");
+        return PS_ERR_ARG;
+    }
+    const size_t room = (NOALLOC_DOUBLES - off_aa) / (3 * (size_t)std::max(Nm, 1));
+    int nsplit = ntb > 0 ? (target + ntb - 1) / ntb : 1;
+    nsplit = std::max(1, std::min({nsplit, std::max(1, Nm / 64), (int)std::min<size_t>(room, 1 << 20)}));
+    int per_split = Nm > 0 ? (Nm + nsplit - 1) / nsplit : 0;
+    per_split = ((per_split + PS_BLOCK - 1) / PS_BLOCK) * PS_BLOCK;
+    nsplit = Nm > 0 ? (Nm + per_split - 1) / per_split : 1;
+    double* stale = base;
+    int* chain = reinterpret_cast<int*>(base + off_chain);
+    double* part_c = base + off_c;
+    double* part_fs = base + off_fs;
+    double* bb = base + off_bb;
+    double* part_abm = base + off_abm;
+    double* part_aa = base + off_aa;
+    const int64_t nzero = (int64_t)(off_abm - off_c);   // colloid partials and bb
+    const int64_t ninit = std::max<int64_t>({18, (int64_t)Nm, nzero});
+    k_noalloc_init<<<(unsigned)((ninit + PS_BLOCK - 1) / PS_BLOCK), PS_BLOCK, 0, st>>>(stale, chain, Nm, nmono,
+                                                                                      part_c, nzero);
+    k_stale<<<1, 64, 0, st>>>(s, x, si, sj, stale);
+    if (Nc > 0) k_bb<<<Nc, PS_BLOCK, 0, st>>>(s, x, f, bb);
+    const bool box = s.box_active, ch = s.chain_HI;
+    if (Nm > 0) {
+        dim3 g1(ntb, nsplit);
+        if (box && ch)
+            k_aa<double, true, true><<<g1, PS_BLOCK, 0, st>>>(s, x, f, chain, per_split, part_aa);
+        else if (box)
+            k_aa<double, true, false><<<g1, PS_BLOCK, 0, st>>>(s, x, f, chain, per_split, part_aa);
+        else if (ch)
+            k_aa<double, false, true><<<g1, PS_BLOCK, 0, st>>>(s, x, f, chain, per_split, part_aa);
+        else
+            k_aa<double, false, false><<<g1, PS_BLOCK, 0, st>>>(s, x, f, chain, per_split, part_aa);
+        if (Nc > 0) {
+            dim3 g2(ntb, nchm);
+            if (box)
+                k_ab_mono<double, true><<<g2, PS_BLOCK, 0, st>>>(s, x, f, stale, per_chunk_m, part_abm);
+            else
+                k_ab_mono<double, false><<<g2, PS_BLOCK, 0, st>>>(s, x, f, stale, per_chunk_m, part_abm);
+            dim3 g3(Nc, nchc);
+            if (box)
+                k_ab_coll<double, true><<<g3, PS_BLOCK, 0, st>>>(s, x, f, per_chunk_c, part_c, part_fs);
+            else
+                k_ab_coll<double, false><<<g3, PS_BLOCK, 0, st>>>(s, x, f, per_chunk_c, part_c, part_fs);
+        }
+    }
+    dim3 g4((unsigned)((s.n + PS_BLOCK - 1) / PS_BLOCK));
+    k_finalize<double><<<g4, PS_BLOCK, 0, st>>>(s, f, u, part_aa, Nm > 0 ? nsplit : 0, part_abm,
+                                                (Nm > 0 && Nc > 0) ? nchm : 0, part_c, part_fs,
+                                                (Nm > 0 && Nc > 0) ? nchc : 0, bb, stale, 0, 0);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        ps_set_context_free_error(std::string("ps_mob_mult_noalloc: kernel launch: ") + hipGetErrorString(e));
+        return PS_ERR_HIP;
+    }
+    return PS_OK;
 }

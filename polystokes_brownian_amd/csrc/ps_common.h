@@ -258,6 +258,7 @@ void pairs_free(Ctx* c);
 ps_err check_dist_dev(Ctx* c);
 ps_err rhs_dev(Ctx* c, int drift);
 ps_err vlist_copy(Ctx* c, int32_t* rows, int32_t* partners, int64_t* pidx, int64_t cap, int64_t* n_pairs);
+void ps_set_context_free_error(const std::string& msg);           // for ps_last_error(NULL)
 ps_err vlist_count(Ctx* c, int64_t* n_pairs);                      // pair count of the last check_dist (syncs)
 ps_err run_steps(Ctx* c, int nsteps);
 ps_err dense_build(Ctx* c, int storage);

@@ -51,6 +51,7 @@ def lib():
         L.ps_stale_tile.argtypes = [vp, dp]
         for name in ("ps_mob_mult", "ps_mob_mult_dev", "ps_saddle_mult", "ps_saddle_mult_dev"):
             getattr(L, name).argtypes = [vp, C.c_void_p, C.c_void_p]
+        L.ps_mob_mult_noalloc.argtypes = [C.POINTER(PsParams), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.ps_saddle_diag.argtypes = [vp, dp]
         L.ps_saddle_solve.argtypes = [vp, C.c_void_p, C.c_void_p, ip, C.POINTER(PsSolveInfo)]
         L.ps_saddle_solve_dev.argtypes = [vp, C.c_void_p, C.c_void_p, ip, C.POINTER(PsSolveInfo)]
@@ -323,6 +324,16 @@ class Context:
         p = PsProfile()
         self._check(lib().ps_profile_end(self.h, C.byref(p)))
         return {k: getattr(p, k) for k, _ in PsProfile._fields_}
+
+
+def mob_mult_noalloc(params, x_ptr, f_ptr, u_ptr, stream=0):
+    """ps_mob_mult_noalloc: u = M f at positions x with no context and no allocation; the arguments
+    are device addresses (ints) of 3 Np, n and n doubles and a hipStream_t handle (0: null stream)."""
+    if isinstance(params, dict):
+        params = default_params(**params)
+    rc = lib().ps_mob_mult_noalloc(C.byref(params), x_ptr, f_ptr, u_ptr, stream or None)
+    if rc != PS_OK:
+        raise PsError(rc, lib().ps_last_error(None).decode())
 
 
 class Box:
