@@ -97,8 +97,10 @@ typedef struct ps_params {
 
 /* Monomer-monomer tile kernel.  ORDERED evaluates every ordered pair (target-stationary, source
  * splits); SYMMETRIC evaluates every unordered pair once and applies the tile both ways (cubic or
- * no box, chain_HI off; otherwise ORDERED is used).  AUTO picks SYMMETRIC when its tile-pair grid
- * fills the device.  Both give the same operator to rounding. */
+ * no box, chain_HI off; otherwise ORDERED is used).  AUTO, where both apply, times each on
+ * synthetic positions of the system's size at ps_create and keeps the faster (the outcome is
+ * cached per size class for the life of the process; ps_kernel_choice reports it).  Both give the
+ * same operator to rounding. */
 enum { PS_AA_AUTO = 0, PS_AA_ORDERED = 1, PS_AA_SYMMETRIC = 2 };
 
 /* Saddle-solve algorithms.  Both solve the same system A x = b.
@@ -327,6 +329,11 @@ ps_err ps_batch_op(void* batch, int op, const double* in, double* out, int* iter
  * it ran with; either output (R entries) may be null.  No reference counterpart: the reference
  * runs one system per process. */
 ps_err ps_batch_times(void* batch, double* us, int* team);
+/* The monomer-monomer kernel a context runs (PS_AA_ORDERED or PS_AA_SYMMETRIC) for its FP64
+ * applies (kernel64) and its FP32 applies (kernel32, 0 when the context has none), and, where
+ * PS_AA_AUTO timed them, the chosen kernel's measured milliseconds per apply (ms64, ms32; else 0).
+ * Any output may be null.  No reference counterpart. */
+ps_err ps_kernel_choice(void* ctx, int* kernel64, int* kernel32, float* ms64, float* ms32);
 /* Workgroups of the last step's launch (nwg_last) and the co-resident maximum the schedule packs
  * replicas onto (nwg_max); either pointer may be null.  No reference counterpart. */
 ps_err ps_batch_workgroups(void* batch, int* nwg_last, int* nwg_max);

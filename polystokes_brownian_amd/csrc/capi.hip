@@ -726,6 +726,17 @@ ps_err ps_get_vectors(void* h, double* up, double* vdn, double* vdc, double* vb,
     return PS_OK;
 }
 
+ps_err ps_kernel_choice(void* h, int* kernel64, int* kernel32, float* ms64, float* ms32) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    const bool has32 = c->p.precision == PS_PREC_MIXED;
+    if (kernel64) *kernel64 = c->aw64.sym ? PS_AA_SYMMETRIC : PS_AA_ORDERED;
+    if (kernel32) *kernel32 = has32 ? (c->aw32.sym ? PS_AA_SYMMETRIC : PS_AA_ORDERED) : 0;
+    if (ms64) *ms64 = c->aa_tuned_ms[0];
+    if (ms32) *ms32 = has32 ? c->aa_tuned_ms[1] : 0.f;
+    return PS_OK;
+}
+
 ps_err ps_get_counters(void* h, ps_counters* out) {
     Ctx* c = as_ctx(h);
     if (!c || !out) return PS_ERR_ARG;

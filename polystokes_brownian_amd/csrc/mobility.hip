@@ -23,7 +23,9 @@
 
 #include <type_traits>
 #include <algorithm>
+#include <map>
 #include <mutex>
+#include <tuple>
 
 namespace {
 
@@ -956,7 +958,7 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
     return PS_OK;
 }
 
-ps_err alloc_work(Ctx* c, ApplyWork& aw, size_t elt) {
+ps_err alloc_work(Ctx* c, ApplyWork& aw, size_t elt, bool sym) {
     const SysConst& s = c->sc;
     const int ntb = (s.Nm + PS_BLOCK - 1) / PS_BLOCK;
     const int target = 4 * (c->n_cu > 0 ? c->n_cu : 256);
@@ -971,13 +973,7 @@ ps_err alloc_work(Ctx* c, ApplyWork& aw, size_t elt) {
     nchunk = std::max(1, std::min(nchunk, std::max(1, s.Nm / PS_BLOCK)));
     aw.per_chunk_c = s.Nm > 0 ? (s.Nm + nchunk - 1) / nchunk : 0;
     aw.nchunk_c = s.Nm > 0 ? (s.Nm + aw.per_chunk_c - 1) / aw.per_chunk_c : 1;
-    // symmetric AA kernel: cubic box (or none), no chain_HI, and (AUTO) at least one tile pair per CU
-    const bool cubic = !s.box_active || (s.L[0] == s.L[1] && s.L[1] == s.L[2]);
-    const int npairs_all = ntb * (ntb + 1) / 2;
-    const int ncu = c->n_cu > 0 ? c->n_cu : 256;
-    const int sel = c->p.aa_kernel;
-    aw.sym = s.Nm > 0 && cubic && !s.chain_HI &&
-             (sel == PS_AA_SYMMETRIC || (sel == PS_AA_AUTO && npairs_all >= ncu));
+    aw.sym = sym;
     if (aw.sym) {
         aw.ntile = ntb;
         aw.nsplit_aa = ntb;
@@ -1023,12 +1019,136 @@ void free_work(ApplyWork& aw) {
 
 }  // namespace
 
-ps_err apply_alloc(Ctx* c) {
-    PS_TRY(alloc_work(c, c->aw64, sizeof(double)));
-    if (c->p.precision == PS_PREC_MIXED) {
-        PS_TRY(alloc_work(c, c->aw32, sizeof(float)));
-        PS_HIP(c, hipMalloc(&c->xm32, sizeof(float) * 3 * (size_t)c->Np));
+// ---------------------------------------------------------------------------- kernel choice
+// PS_AA_AUTO: where both monomer kernels apply (a cubic box or none, no chain_HI) the choice is
+// measured, not ruled: at create, each kernel runs a few applies on synthetic positions of the
+// system's size and density (uniform in the box, or in a cube of the box's density without one) and
+// the faster is kept.  The outcome is cached per size class (monomers, colloids, precision, box, CUs)
+// for the life of the process, so later contexts of the same size do not time again.
+namespace {
+
+struct AATuneKey {
+    int Nm, Nc, elt, box, ncu;
+    bool operator<(const AATuneKey& o) const {
+        return std::tie(Nm, Nc, elt, box, ncu) < std::tie(o.Nm, o.Nc, o.elt, o.box, o.ncu);
     }
+};
+std::map<AATuneKey, std::pair<bool, float>> g_aa_tune;   // -> (symmetric faster, its ms per apply)
+std::mutex g_aa_tune_mu;
+
+// synthetic monomer and colloid positions of the system's size, seeded (the apply times do not depend
+// on where the beads are, only on how many pairs fall in the short-range branch)
+std::vector<double> tune_positions(const Ctx* c) {
+    const SysConst& s = c->sc;
+    // no box: a cube with about one bead per (2 radii)^3
+    const double side = s.box_active ? s.L[0] : 2.0 * std::cbrt((double)std::max(1, s.Np));
+    std::vector<double> x(3 * (size_t)s.Np);
+    uint64_t st = 0x9e3779b97f4a7c15ull;
+    for (auto& v : x) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        v = ((double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5) * side;
+    }
+    return x;
+}
+
+template <class T>
+ps_err time_apply(Ctx* c, const T* xm, ApplyWork& aw, int reps, float* ms) {
+    const unsigned gm = (unsigned)((c->Nm + PS_BLOCK - 1) / PS_BLOCK);
+    if (aw.sym && sizeof(T) == sizeof(float) && c->sc.box_active)
+        k_aa_pos_fix<<<gm, PS_BLOCK, 0, c->stream>>>(c->Nm, c->xm, aw.inv_L, (int*)aw.xs);
+    else if (aw.sym)
+        k_aa_pos<T><<<gm, PS_BLOCK, 0, c->stream>>>(c->Nm, c->xm, aw.inv_L, (T*)aw.xs);
+    PS_LAUNCH(c);
+    hipEvent_t e0, e1;
+    PS_HIP(c, hipEventCreate(&e0));
+    PS_HIP(c, hipEventCreate(&e1));
+    ps_err rc = launch_apply<T>(c, xm, aw, c->io_a, c->io_b, 0);   // warmup
+    if (rc == PS_OK) rc = hipEventRecord(e0, c->stream) == hipSuccess ? PS_OK : PS_ERR_HIP;
+    for (int k = 0; k < reps && rc == PS_OK; k++) rc = launch_apply<T>(c, xm, aw, c->io_a, c->io_b, 0);
+    if (rc == PS_OK) rc = hipEventRecord(e1, c->stream) == hipSuccess ? PS_OK : PS_ERR_HIP;
+    if (rc == PS_OK) rc = hipEventSynchronize(e1) == hipSuccess ? PS_OK : PS_ERR_HIP;
+    float t = 0.f;
+    if (rc == PS_OK) rc = hipEventElapsedTime(&t, e0, e1) == hipSuccess ? PS_OK : PS_ERR_HIP;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == PS_ERR_HIP) c->set_error("apply timing for the kernel choice failed");
+    *ms = t / reps;
+    return rc;
+}
+
+// aw <- the faster layout for precision T (both allocated and timed, the slower freed)
+template <class T>
+ps_err choose_kernel(Ctx* c, ApplyWork& aw, const T* xm) {
+    const AATuneKey key{c->Nm, c->Nc, (int)sizeof(T), c->sc.box_active, c->n_cu};
+    {
+        std::lock_guard<std::mutex> lk(g_aa_tune_mu);
+        auto it = g_aa_tune.find(key);
+        if (it != g_aa_tune.end()) {
+            c->aa_tuned_ms[sizeof(T) == sizeof(float)] = it->second.second;
+            return alloc_work(c, aw, sizeof(T), it->second.first);
+        }
+    }
+    ApplyWork ord, sym;
+    PS_TRY(alloc_work(c, ord, sizeof(T), false));
+    ps_err rc = alloc_work(c, sym, sizeof(T), true);
+    float t_ord = 0.f, t_sym = 0.f;
+    const int64_t cnt0 = c->cnt.applies;
+    if (rc == PS_OK) rc = time_apply<T>(c, xm, ord, 3, &t_ord);
+    if (rc == PS_OK) rc = time_apply<T>(c, xm, sym, 3, &t_sym);
+    c->cnt.applies = cnt0;
+    if (rc != PS_OK) {
+        free_work(ord);
+        free_work(sym);
+        return rc;
+    }
+    const bool use_sym = t_sym <= t_ord;
+    if (use_sym) {
+        free_work(ord);
+        aw = sym;
+    } else {
+        free_work(sym);
+        aw = ord;
+    }
+    c->aa_tuned_ms[sizeof(T) == sizeof(float)] = use_sym ? t_sym : t_ord;
+    std::lock_guard<std::mutex> lk(g_aa_tune_mu);
+    g_aa_tune[key] = {use_sym, use_sym ? t_sym : t_ord};
+    return PS_OK;
+}
+
+}  // namespace
+
+ps_err apply_alloc(Ctx* c) {
+    const SysConst& s = c->sc;
+    const bool cubic = !s.box_active || (s.L[0] == s.L[1] && s.L[1] == s.L[2]);
+    // the symmetric kernel needs a cubic box (or none) and no same-chain skip
+    const bool eligible = s.Nm > 0 && cubic && !s.chain_HI;
+    const int sel = c->p.aa_kernel;
+    const bool mixed = c->p.precision == PS_PREC_MIXED;
+    if (mixed) PS_HIP(c, hipMalloc(&c->xm32, sizeof(float) * 3 * (size_t)c->Np));
+    if (!eligible || sel != PS_AA_AUTO) {
+        const bool sym = eligible && sel == PS_AA_SYMMETRIC;
+        PS_TRY(alloc_work(c, c->aw64, sizeof(double), sym));
+        if (mixed) PS_TRY(alloc_work(c, c->aw32, sizeof(float), sym));
+        return PS_OK;
+    }
+    // measured choice on synthetic positions (c->xm is overwritten by the first mob())
+    const std::vector<double> xt = tune_positions(c);
+    PS_HIP(c, hipMemcpy(c->xm, xt.data(), sizeof(double) * xt.size(), hipMemcpyHostToDevice));
+    std::vector<double> ft(c->n);
+    uint64_t st = 12345;
+    for (auto& v : ft) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        v = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+    PS_HIP(c, hipMemcpy(c->io_a, ft.data(), sizeof(double) * ft.size(), hipMemcpyHostToDevice));
+    PS_TRY(choose_kernel<double>(c, c->aw64, c->xm));
+    if (mixed) {
+        const int64_t n3 = 3 * (int64_t)c->Np;
+        k_to_f32<<<(unsigned)((n3 + PS_BLOCK - 1) / PS_BLOCK), PS_BLOCK, 0, c->stream>>>(c->xm, c->xm32, n3);
+        PS_LAUNCH(c);
+        PS_TRY(choose_kernel<float>(c, c->aw32, c->xm32));
+    }
+    PS_HIP(c, hipStreamSynchronize(c->stream));
     return PS_OK;
 }
 

@@ -113,6 +113,7 @@ struct Ctx {
     bool have_pd = false, have_mob = false, have_x = false;
     // ---- apply
     ApplyWork aw64, aw32;
+    float aa_tuned_ms[2] = {0.f, 0.f};   // PS_AA_AUTO: the measured ms per apply of the chosen kernel (FP64, FP32)
     // ---- Krylov work
     double *kr = nullptr;           // 9 * ns scratch vectors for MINRES
     double *dinv = nullptr;         // ns

@@ -93,6 +93,8 @@ def lib():
         L.ps_batch_config.argtypes = [vp, ip, ip, ip]
         L.ps_batch_phase_times.argtypes = [vp, dp]
         L.ps_batch_workgroups.argtypes = [vp, C.POINTER(ip), C.POINTER(ip)]
+        L.ps_kernel_choice.argtypes = [vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(C.c_float),
+                                       C.POINTER(C.c_float)]
         L.ps_batch_plan.argtypes = [ip, dp, C.POINTER(ip), ip, C.POINTER(ip), C.POINTER(ip), dp, dp]
         _lib = L
     return _lib
@@ -180,6 +182,16 @@ class Context:
         c = PsCounters()
         self._check(lib().ps_get_counters(self.h, C.byref(c)))
         return {k: getattr(c, k) for k, _ in PsCounters._fields_}
+
+    def kernel_choice(self):
+        """ps_kernel_choice: {"fp64": "ordered"|"symmetric", "fp32": ... or None, "ms64", "ms32"} --
+        the monomer-monomer kernel each apply precision runs and, under aa_kernel "auto", the
+        chosen kernel's measured milliseconds per apply (0 where nothing was timed)."""
+        k64, k32 = C.c_int(), C.c_int()
+        t64, t32 = C.c_float(), C.c_float()
+        self._check(lib().ps_kernel_choice(self.h, C.byref(k64), C.byref(k32), C.byref(t64), C.byref(t32)))
+        name = {1: "ordered", 2: "symmetric"}
+        return {"fp64": name[k64.value], "fp32": name.get(k32.value), "ms64": t64.value, "ms32": t32.value}
 
     def check_dist(self):
         n = C.c_int64()

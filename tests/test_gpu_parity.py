@@ -156,8 +156,35 @@ def test_symmetric_aa_kernel(name):
             assert rel(u, uo) < (1e-12 if prec == 0 else 1e-5)
             assert np.array_equal(u, g.mob_mult(f))   # deterministic
             us[(kern, prec)] = u
+            # chain_HI systems run the ordered kernel whatever is asked
+            want = "symmetric" if kern == 2 and not pd.get("chain_HI") else "ordered"
+            assert g.kernel_choice()["fp64"] == want
             g.close()
     assert rel(us[(2, 0)], us[(1, 0)]) < 1e-13
+    # aa_kernel "auto": one of the two kernels, timed at create, and the same operator as that kernel
+    for prec in (0, 1):
+        g = Context(dict(pd, aa_kernel=0, precision=prec))
+        kc = g.kernel_choice()
+        print("case", name, "auto precision", prec, kc, flush=True)
+        if pd.get("chain_HI"):   # only the ordered kernel applies: nothing timed
+            assert kc["fp64"] == "ordered" and kc["ms64"] == 0
+        else:
+            assert kc["fp64"] in ("ordered", "symmetric") and kc["ms64"] > 0
+            assert (kc["fp32"] in ("ordered", "symmetric") and kc["ms32"] > 0) if prec == 1 else kc["fp32"] is None
+        g.initial_configuration(x0)
+        g.check_dist()
+        g.mob()
+        u = g.mob_mult(f)
+        k = {"ordered": 1, "symmetric": 2}[kc["fp64" if prec == 0 else "fp32"]]
+        if prec == 0:
+            assert rel(u, us[(k, 0)]) < 1e-13
+        else:
+            assert np.array_equal(u, us[(k, 1)])
+        # the second context of the same size class takes the cached choice
+        g2 = Context(dict(pd, aa_kernel=0, precision=prec))
+        assert g2.kernel_choice() == kc
+        g2.close()
+        g.close()
 
 
 @pytest.mark.parametrize("name", ["c1", "small_chainHI"])
