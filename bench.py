@@ -87,8 +87,49 @@ def timed_run(ctx, steps, warmup, label, profile=True):
     prof = ctx.profile_end() if profile else None
     c1 = ctx.counters()
     per = {k: (c1[k] - c0[k]) / steps for k in ("applies", "minres_iterations", "lanczos_iterations",
-                                                 "saddle_solves", "mob_calls")}
+                                                 "saddle_solves", "mob_calls", "stresslet_iterations")}
     return dt, prof, per
+
+
+def lub_effect(Context, pd, x0, over):
+    """The lubricated form against the same form without lubrication, one step each from x0 with the
+    same noise: colloid pairs inside the cutoff at x0 and the relative change of the step's velocities
+    (all beads, and the colloids' U, W)."""
+    vs = {}
+    for lub in (0, 1):
+        c = make_ctx(Context, pd, x0, **dict(over, lub=lub))
+        c.run(1)
+        vs[lub] = c.vectors()
+        c.close()
+    c = make_ctx(Context, pd, x0, **over)
+    c.check_dist()
+    c.mob()
+    npairs = c.lub_build(pd.get("lub_rcut", 4.0))
+    c.close()
+    nm3 = 3 * pd["Nm"]
+    a, b = vs[1]["v_det_n"], vs[0]["v_det_n"]
+    return {"colloid_pairs_within_cutoff": int(npairs),
+            "first_solve_colloid_vel_rel_change": rel(a[nm3:], b[nm3:]),
+            "first_solve_vel_rel_change": rel(a, b),
+            "step_vel_rel_change": rel(vs[1]["up"], vs[0]["up"]),
+            "note": "first solve: the deterministic predictor solve at x0; the monomer rows respond to the sum "
+                    "of all colloid forces through the carried translation tile of mob()"}
+
+
+def mixed_vs_gpu_fp64(Context, pd, x0, forms):
+    """Without the oracle (sizes it cannot finish in minutes): one step of each form from x0 with the
+    same noise, the mixed form's velocity and displacement against the GPU FP64 form's."""
+    out = {}
+    for form in ("mixed/block", "fp64/block"):
+        c = make_ctx(Context, pd, x0, **forms[form])
+        c.run(1)
+        v = c.vectors()
+        out[form] = (v["up"], c.state()[0], v["v_det_n"])
+        c.close()
+    (um, xm, vm), (ud, xd, vd) = out["mixed/block"], out["fp64/block"]
+    return {"vel_rel_err": rel(um, ud), "disp_rel_err": rel(xm - x0, xd - x0),
+            "first_solve_vel_rel_err": rel(vm, vd),
+            "reference": "the GPU FP64 block form (the oracle is not run at this size)"}
 
 
 def one_step_up(Context, pd, x0, **over):
@@ -433,6 +474,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle: no CPU baseline, no errors vs the oracle")
     ap.add_argument("--minres", action="store_true",
                     help="also time the reference algorithm (Jacobi MINRES on the saddle system) for one step")
+    ap.add_argument("--lub", action="store_true",
+                    help="also time the mixed block form with lubrication in the saddle (lub = 1), and its effect "
+                         "on the step's velocities")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (a plain copy of known size, to scale HBM counters)")
     ap.add_argument("--dense", default=None, metavar="STORAGE", choices=["bf16", "f16"],
@@ -500,6 +544,10 @@ def main():
     }
     if args.minres:
         forms["fp64/minres"] = dict(precision=0, solver=0)
+    if args.lub:
+        # lubrication in the saddle (opt-in physics, not the reference's system): eager only, its
+        # colloid saddle solves poll on the host
+        forms["mixed/block+lub"] = dict(precision=1, solver=1, ir_maxit=0, lub=1)
     if args.profile_form:
         ctx = make_ctx(Context, pd, x0, graph=0, **forms[args.profile_form])
         dt, prof, per = timed_run(ctx, args.steps, args.warmup, args.profile_form, profile=args.profile_events)
@@ -520,6 +568,16 @@ def main():
             dt, _, per = timed_run(ctx, 1, 0, form, profile=False)
             ctx.close()
             results[form] = dict(value=Np / dt, ms_per_step=1e3 * dt, per_step=per)
+            continue
+        if form.endswith("+lub"):
+            ctx = make_ctx(Context, pd, x0, graph=0, **over)
+            dt, _, per = timed_run(ctx, args.steps, args.warmup, form, profile=False)
+            ctx.close()
+            results[form] = dict(value=Np * args.steps / dt, ms_per_step=1e3 * dt / args.steps, per_step=per,
+                                 graph=0)
+            results[form]["effect_vs_no_lubrication"] = lub_effect(Context, pd, x0, over)
+            log(f"{form}: {results[form]['value']:.4g} bead-steps/s, {results[form]['ms_per_step']:.2f} ms/step, "
+                f"per step {per}, {results[form]['effect_vs_no_lubrication']}")
             continue
         modes = {}
         for graph in (0, 1):
@@ -549,7 +607,7 @@ def main():
         xi = np.random.default_rng(7).normal(size=3 * Nm + 11 * Nc)
         ref = oracle_reference(pd, x0, w, xi)
         for form, over in forms.items():
-            if form == "fp64/minres":
+            if form == "fp64/minres" or form.endswith("+lub"):
                 continue
             errs[form] = errors_vs_oracle(Context, pd, x0, w, over, ref, xi)
             log(f"{form} vs oracle: {errs[form]}")
@@ -575,6 +633,10 @@ def main():
                           f"times the {ref['applies']} applies of one block-algorithm step (the step's other work "
                           f"is under 2% of it on the host)"}
         log(f"cpu baseline {cpu}; one core {cpu1}")
+    errs_gpu = None
+    if args.no_cpu:
+        errs_gpu = mixed_vs_gpu_fp64(Context, pd, x0, forms)
+        log(f"mixed/block vs GPU FP64: {errs_gpu}")
 
     head = results["mixed/block"]
     f64 = results["fp64/block"]
@@ -609,7 +671,8 @@ def main():
                                f"tiles, FP64 recurrences and stresslet solves), exact block-eliminated saddle "
                                f"solves, Lanczos m<={pd['sqrt_kmax']} rtol {pd['sqrt_rtol']}",
                    "global_batch": Np, "seq_len": 1, "parallelism": "single-gpu"},
-        "errors_vs_fp64_oracle": errs,
+        "errors_vs_fp64_oracle": errs if errs else None,
+        "errors_mixed_vs_gpu_fp64": errs_gpu,
         "fp64_value": f64["value"],
         "mixed_over_fp64": head["value"] / f64["value"],
         "forms": results,

@@ -90,9 +90,18 @@ typedef struct ps_params {
     int solver;                   /* PS_SOLVER_MINRES (reference algorithm) or PS_SOLVER_BLOCK */
     int aa_kernel;                /* monomer-monomer kernel: PS_AA_AUTO, PS_AA_ORDERED, PS_AA_SYMMETRIC */
     int graph;                    /* ps_run: 1 (default) replays each step as a HIP graph where the step
-                                     has no host round trip (mm_HI=False, one colloid, PS_SOLVER_BLOCK,
-                                     kT > 0, no mono_ev, no record_forces, profiling off); 0 launches
-                                     every kernel */
+                                     has no host round trip: PS_SOLVER_BLOCK, kT > 0, profiling off,
+                                     lubrication off, and either mm_HI (FP64, or the mixed form with
+                                     ir_maxit = 0) or mm_HI=False with one colloid, no mono_ev and no
+                                     record_forces; 0 launches every kernel */
+    /* Lubrication in the saddle (no reference counterpart; src/saddle.cpp:53 leaves it as a TODO).
+     * lub = 1 puts -R_lub, the pair lubrication resistance between colloid velocities and colloid
+     * forces and torques (the ps_lub_* operator with no strain), in the saddle's lower-right block:
+     * the constraint rows become -f_u - R_lub u = b_lambda.  Off by default (the reference's system).
+     * Needs mm_HI and the three tables (ps_lub_set_table) before the first ps_mob; every ps_mob
+     * rebuilds the colloid pairs closer than lub_rcut (default 4.0, at most the last table node). */
+    int lub;
+    double lub_rcut;
 } ps_params;
 
 /* Monomer-monomer tile kernel.  ORDERED evaluates every ordered pair (target-stationary, source
@@ -131,8 +140,9 @@ typedef struct ps_counters {
     int64_t mob_calls, applies, minres_iterations, lanczos_iterations, saddle_solves, sqrt_calls;
     int64_t nvlist;      /* Verlet pairs found by the last check_dist */
     int64_t lanczos_nneg_events, lanczos_nneg_count;
-    int64_t stresslet_iterations;  /* PS_SOLVER_BLOCK: MINRES iterations of the 5Nc x 5Nc stresslet solves */
-    int64_t stresslet_capped;      /* PS_SOLVER_BLOCK: stresslet solves stopped at their cap (10000) */
+    int64_t stresslet_iterations;  /* PS_SOLVER_BLOCK: MINRES iterations of the 5Nc x 5Nc stresslet solves
+                                      (with lubrication: of the 17Nc colloid saddle solves) */
+    int64_t stresslet_capped;      /* PS_SOLVER_BLOCK: those solves stopped at their cap (10000) */
 } ps_counters;
 
 /* Fill *p with the reference defaults (src/bindings.cpp:56-69; record_forces=true as bound). */
@@ -259,7 +269,7 @@ ps_err ps_format_time(double t, char* buf, int len);
  * rows, r first (which 0: r2babc, 11 columns; 1: r2bgh, 7; 2: r2bm, 4) and interpolated by monotone
  * cubic Hermite (Fritsch-Carlson) in r; below the first node the first row is used.  ps_lub_build
  * lists the colloid pairs with 0 < r < rcut (rcut at most the last node) at the positions of the
- * last ps_mob.  ps_lub_apply maps uw = [U_c (3Nc); W_c (3Nc)] and a strain einf shared by all colloids
+ * last ps_mob, through a cell list over the colloids.  ps_lub_apply maps uw = [U_c (3Nc); W_c (3Nc)] and a strain einf shared by all colloids
  * (5 values flattened as the reference's EV, src/mob.cpp:300-305; NULL for none) to
  * fts = [F_c (3Nc); T_c (3Nc); S_c (5Nc, flattened as SV)] in FP64: (U - U_inf, W - W_inf, -E_inf)
  * -> (F, T, S) exerted on the fluid.  ps_lub_interp / ps_lub_slopes evaluate the interpolant and

@@ -32,6 +32,7 @@ class PsParams(C.Structure):
         ("force_sel", C.c_int), ("hs_kpref", C.c_double), ("hs_kmode", C.c_int), ("hs_exp_k", C.c_double),
         ("precision", C.c_int), ("ir_rtol", C.c_double), ("ir_maxit", C.c_int),
         ("solver", C.c_int), ("aa_kernel", C.c_int), ("graph", C.c_int),
+        ("lub", C.c_int), ("lub_rcut", C.c_double),
     ]
 
 

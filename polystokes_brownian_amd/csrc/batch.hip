@@ -1556,6 +1556,7 @@ ps_err ps_batch_create(const ps_params* p, int R, const int* Nc, const int* Npol
     };
     if (!p->mm_HI) return bad("batched replicas: full monomer hydrodynamics (mm_HI) only");
     if (p->box_active) return bad("batched replicas: unbounded systems only (no periodic box)");
+    if (p->lub) return bad("batched replicas: no lubrication (a single context's saddle offers it)");
     if (p->sqrt_kmax < 1 || p->sqrt_kmax > BATCH_KMAX) return bad("batched replicas: sqrt_kmax must be in 1..48");
     Batch* b = new Batch();
     b->R = R;

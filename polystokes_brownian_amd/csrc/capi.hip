@@ -296,6 +296,8 @@ ps_err fail(Ctx* c, ps_err code, const std::string& msg) {
 // no WCA grid from the host.
 bool step_graphable(const Ctx* c) {
     if (!c->p.graph || c->p.solver != PS_SOLVER_BLOCK || !(c->p.kT > 0) || c->prof_on) return false;
+    // the lubricated block solve polls its colloid MINRES on the host
+    if (c->p.lub) return false;
     if (c->arrow) return c->Nc == 1 && c->Nm > 0 && !c->p.mono_ev && !c->p.record_forces;
     // FP64 refinement of the mixed form reads its residual norms on the host
     return !(c->p.precision == PS_PREC_MIXED && c->p.ir_maxit > 0);
@@ -520,6 +522,8 @@ void ps_params_default(ps_params* p) {
     p->solver = PS_SOLVER_MINRES;
     p->aa_kernel = PS_AA_AUTO;
     p->graph = 1;
+    p->lub = 0;
+    p->lub_rcut = 4.0;
 }
 
 const char* ps_last_error(const void* ctx) {
@@ -551,6 +555,10 @@ ps_err ps_create(const ps_params* p, void** out) {
     if (p->precision < PS_PREC_FP64 || p->precision > PS_PREC_MIXED || p->solver < PS_SOLVER_MINRES ||
         p->solver > PS_SOLVER_BLOCK || p->aa_kernel < PS_AA_AUTO || p->aa_kernel > PS_AA_SYMMETRIC) {
         g_create_error = "precision, solver or aa_kernel out of range";
+        return PS_ERR_ARG;
+    }
+    if (p->lub != 0 && (p->lub != 1 || !p->mm_HI || !(p->lub_rcut > 0.0))) {
+        g_create_error = "lub must be 0 or 1; lubrication needs mm_HI and lub_rcut > 0";
         return PS_ERR_ARG;
     }
     int ndev = 0;
@@ -651,6 +659,7 @@ ps_err ps_create(const ps_params* p, void** out) {
     if (krylov_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
     if (pairs_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
     if (c->arrow && arrow_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
+    if (c->p.lub && lub_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
     for (double* v : {c->up, c->udiff, c->v_det_n, c->v_det_c, c->v_brown, c->fext, c->drift_v})
         (void)hipMemsetAsync(v, 0, sizeof(double) * (size_t)c->nu, c->stream);
     for (double* v : {c->X, c->Xdet, c->Xd})
@@ -749,8 +758,8 @@ ps_err ps_get_counters(void* h, ps_counters* out) {
     out->lanczos_iterations += c->dcnt_h[DC_LANCZOS_ITERS];
     out->lanczos_nneg_events += c->dcnt_h[DC_NNEG_EVENTS];
     out->lanczos_nneg_count += c->dcnt_h[DC_NNEG_COUNT];
-    out->stresslet_iterations = c->dcnt_h[DC_MSS_ITERS];
-    out->stresslet_capped = c->dcnt_h[DC_MSS_CAPPED];
+    out->stresslet_iterations += c->dcnt_h[DC_MSS_ITERS];
+    out->stresslet_capped += c->dcnt_h[DC_MSS_CAPPED];
     return PS_OK;
 }
 

@@ -1015,11 +1015,34 @@ static ps_err minres_small_info(Ctx* c, ps_solve_info* info) {
     return s.reason > 0 ? PS_OK : PS_ERR_NOCONV;
 }
 
-// x = [-b_lambda (nu) ; 0 (5Nc)]  (f_u from the constraint rows B^T f = b_lambda)
-__global__ void k_bs_init(double* __restrict__ x, const double* __restrict__ b, int64_t n, int64_t nu) {
+// x = [-b_lambda (nu) ; 0 (5Nc)]  (f_u from the constraint rows B^T f = b_lambda); entries from
+// `upto` on are zero (the lubricated solve fixes only the monomer forces, upto = 3Nm)
+__global__ void k_bs_init(double* __restrict__ x, const double* __restrict__ b, int64_t n, int64_t upto) {
     const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
     if (i >= n) return;
-    x[i] = (i < nu) ? -b[n + i] : 0.0;
+    x[i] = (i < upto) ? -b[n + i] : 0.0;
+}
+
+// Colloid saddle of the lubricated block solve: its right-hand side [b_c - y_c (11Nc); b_lambda,c (6Nc)]
+// and Jacobi inverse diagonal (self blocks 1, 4/3, 5/9; 1 on the velocity rows, whose diagonal is
+// -R_ii: any positive diagonal preconditions MINRES)
+__global__ void k_cs_rhs(const double* __restrict__ b, const double* __restrict__ y, int64_t n, int64_t nm3,
+                         int Nc, double* __restrict__ rhs, double* __restrict__ dinv) {
+    const int64_t k = (int64_t)blockIdx.x * VB + threadIdx.x;
+    const int64_t nc3 = 3 * (int64_t)Nc, nc6 = 6 * (int64_t)Nc, nc11 = 11 * (int64_t)Nc;
+    if (k >= nc11 + nc6) return;
+    if (k < nc11) {
+        rhs[k] = b[nm3 + k] - y[nm3 + k];
+        dinv[k] = k < nc3 ? 1.0 : k < nc6 ? 4.0 / 3.0 : 5.0 / 9.0;
+    } else {
+        rhs[k] = b[n + nm3 + (k - nc11)];
+        dinv[k] = 1.0;
+    }
+}
+// the colloid forces of the solution into x
+__global__ void k_cs_scatter(const double* __restrict__ v, double* __restrict__ x, int64_t nm3, int64_t nc11) {
+    const int64_t k = (int64_t)blockIdx.x * VB + threadIdx.x;
+    if (k < nc11) x[nm3 + k] = v[k];
 }
 // rs = b_s - y_s
 __global__ void k_bs_rs(double* __restrict__ rs, const double* __restrict__ b, const double* __restrict__ y,
@@ -1045,6 +1068,38 @@ static ps_err block_solve_once(Ctx* c, const double* b, double* x, int prec, ps_
     if (c->arrow && c->Nc == 1 && c->sc.nm3 > 0) {
         PS_TRY(arrow_block_solve(c, b, x, uout));
         if (u_done) *u_done = uout != nullptr;
+        c->cnt.saddle_solves++;
+        return PS_OK;
+    }
+    if (c->p.lub && c->Nc > 0) {
+        // Lubrication couples the colloid forces to the colloid velocities (-f_u - R u = b_lambda), so
+        // only the monomer forces are fixed: f_m = -b_lambda,m.  With y = M [f_m; 0], the colloid
+        // unknowns z = (F, T, S) and w = (U, W) solve the symmetric colloid saddle
+        //   [[M_cc, -E], [-E^T, -R]] [z; w] = [b_c - y_c; b_lambda,c]
+        // (E: the velocity rows of z), by MINRES on the operator (colloid pairs on the fly); then
+        // u = (M f)_u - b_u as without lubrication.
+        const int64_t nm3 = c->sc.nm3, nc11 = 11 * (int64_t)c->Nc, N = 17 * (int64_t)c->Nc;
+        const size_t ncb = (size_t)std::max(c->Nc, 1);
+        double* rhs = c->cs_buf;
+        double* v = c->cs_buf + 17 * ncb;
+        double* dinv = c->cs_buf + 34 * ncb;
+        k_bs_init<<<eblocks(n), VB, 0, st>>>(x, b, n, nm3);
+        PS_LAUNCH(c);
+        PS_TRY(mob_apply_colloid_rows(c, x, c->bs_y));
+        k_cs_rhs<<<eblocks(N), VB, 0, st>>>(b, c->bs_y, n, nm3, c->Nc, rhs, dinv);
+        PS_LAUNCH(c);
+        auto op = [c](const double* in, double* out) { return colloid_saddle_apply(c, in, out); };
+        ps_solve_info si{};
+        ps_err r = minres_core(c, N, dinv, op, rhs, v, 0, 1e-12, 10000, &si);
+        if (r != PS_OK && r != PS_ERR_NOCONV) return r;
+        c->cnt.stresslet_iterations += si.iterations;
+        if (r == PS_ERR_NOCONV) c->cnt.stresslet_capped++;
+        if (mss_info) mss_info->iterations += si.iterations;
+        k_cs_scatter<<<eblocks(nc11), VB, 0, st>>>(v, x, nm3, nc11);
+        PS_LAUNCH(c);
+        PS_TRY(mob_apply(c, x, c->bs_y, prec));
+        k_bs_u<<<eblocks(nu), VB, 0, st>>>(x, c->bs_y, b, n, nu);
+        PS_LAUNCH(c);
         c->cnt.saddle_solves++;
         return PS_OK;
     }

@@ -132,27 +132,182 @@ __device__ __forceinline__ double lub_geom(const SysConst& s, const double* x, i
     return r;
 }
 
-// Partners of colloid i (both directions): count, then fill in increasing order.
-__global__ void k_lub_count(SysConst s, const double* __restrict__ xm, double rcut, int* __restrict__ cnt) {
-    const int i = blockIdx.x * LT + threadIdx.x;
-    if (i >= s.Nc) return;
-    int c = 0;
-    double e[3];
-    for (int j = 0; j < s.Nc; j++) {
-        const double r = j != i ? lub_geom(s, xm, s.Nm + i, s.Nm + j, e) : 0.0;
-        if (r > 0.0 && r < rcut) c++;
+// Colloid cell list.  Cells are at least rcut wide, so every partner of a colloid lies in its own
+// cell or one of the 26 around it.  Periodic box: floor(L / rcut) cells per axis, wrapped (an axis of
+// one or two cells visits each of its cells once).  No box: the colloids' bounding box, cells of side
+// rcut.  Either way the cell side grows until the grid has at most `cap` cells.
+struct LubGrid {
+    double org[3], h[3];
+    int dims[3];
+    int periodic;
+};
+
+__global__ __launch_bounds__(LT) void k_lub_grid(SysConst s, const double* __restrict__ xm, double rcut, int cap,
+                                                  LubGrid* __restrict__ g) {
+    __shared__ double lo[3][LT], hi[3][LT];
+    const int t = threadIdx.x;
+    for (int k = 0; k < 3; k++) {
+        lo[k][t] = 1e300;
+        hi[k][t] = -1e300;
     }
-    cnt[i] = c;
+    if (!s.box_active)
+        for (int i = t; i < s.Nc; i += LT)
+            for (int k = 0; k < 3; k++) {
+                const double v = xm[3 * (s.Nm + i) + k];
+                lo[k][t] = fmin(lo[k][t], v);
+                hi[k][t] = fmax(hi[k][t], v);
+            }
+    __syncthreads();
+    for (int w = LT / 2; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < 3; k++) {
+                lo[k][t] = fmin(lo[k][t], lo[k][t + w]);
+                hi[k][t] = fmax(hi[k][t], hi[k][t + w]);
+            }
+        __syncthreads();
+    }
+    if (t != 0) return;
+    LubGrid G{};
+    G.periodic = s.box_active;
+    double ext[3];
+    for (int k = 0; k < 3; k++) {
+        ext[k] = s.box_active ? s.L[k] : fmax(hi[k][0] - lo[k][0], 0.0);
+        G.org[k] = s.box_active ? 0.0 : lo[k][0];
+        if (!(ext[k] < 1e300)) ext[k] = 0.0;   // non-finite positions: one cell on this axis
+    }
+    double side = rcut;
+    for (;;) {
+        long long tot = 1;
+        for (int k = 0; k < 3; k++) {
+            // periodic: whole cells of at least `side`; open: enough cells of `side` to cover the extent
+            G.dims[k] = s.box_active ? max(1, (int)floor(ext[k] / side)) : (int)floor(ext[k] / side) + 1;
+            tot *= G.dims[k];
+        }
+        if (tot <= cap) break;
+        side *= 1.25;
+    }
+    for (int k = 0; k < 3; k++) G.h[k] = s.box_active ? ext[k] / G.dims[k] : side;
+    *g = G;
 }
-__global__ void k_lub_fill(SysConst s, const double* __restrict__ xm, double rcut, const int* __restrict__ off,
-                           int* __restrict__ part) {
+
+__device__ __forceinline__ int lub_cell_axis(const LubGrid& G, const double* x, int k) {
+    if (G.periodic) {
+        const double f = x[k] / (G.h[k] * G.dims[k]);
+        const int c = (int)floor((f - floor(f)) * G.dims[k]);
+        return min(max(c, 0), G.dims[k] - 1);
+    }
+    return min(max((int)floor((x[k] - G.org[k]) / G.h[k]), 0), G.dims[k] - 1);
+}
+
+__global__ void k_lub_bin(SysConst s, const double* __restrict__ xm, const LubGrid* __restrict__ gp,
+                          int* __restrict__ cell_of, int* __restrict__ cell_cnt) {
     const int i = blockIdx.x * LT + threadIdx.x;
     if (i >= s.Nc) return;
-    int o = off[i];
+    const LubGrid G = *gp;
+    const double* x = xm + 3 * (s.Nm + i);
+    const int c = (lub_cell_axis(G, x, 2) * G.dims[1] + lub_cell_axis(G, x, 1)) * G.dims[0] + lub_cell_axis(G, x, 0);
+    cell_of[i] = c;
+    atomicAdd(&cell_cnt[c], 1);
+}
+
+// Exclusive scan of the cell counts into start[0..ncell] (one block), and a copy into the fill cursors.
+__global__ __launch_bounds__(LT) void k_lub_scan(const LubGrid* __restrict__ gp, const int* __restrict__ cnt,
+                                                 int* __restrict__ start, int* __restrict__ cursor) {
+    __shared__ int part[LT];
+    const LubGrid G = *gp;
+    const int ncell = G.dims[0] * G.dims[1] * G.dims[2];
+    const int t = threadIdx.x;
+    const int per = (ncell + LT - 1) / LT;
+    const int a = min(t * per, ncell), b = min(a + per, ncell);
+    int sum = 0;
+    for (int k = a; k < b; k++) sum += cnt[k];
+    part[t] = sum;
+    __syncthreads();
+    if (t == 0) {
+        int run = 0;
+        for (int k = 0; k < LT; k++) {
+            const int v = part[k];
+            part[k] = run;
+            run += v;
+        }
+        start[ncell] = run;
+    }
+    __syncthreads();
+    int run = part[t];
+    for (int k = a; k < b; k++) {
+        start[k] = run;
+        cursor[k] = run;
+        run += cnt[k];
+    }
+}
+
+__global__ void k_zero_cells(int* __restrict__ a, int n) {
+    const int i = blockIdx.x * LT + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+__global__ void k_lub_scatter(int Nc, const int* __restrict__ cell_of, int* __restrict__ cursor,
+                              int* __restrict__ mem) {
+    const int i = blockIdx.x * LT + threadIdx.x;
+    if (i >= Nc) return;
+    mem[atomicAdd(&cursor[cell_of[i]], 1)] = i;   // order within a cell is irrelevant: partners are sorted
+}
+
+// Partners of colloid i (both directions) from the 27 cells around its own: count, then fill and sort
+// ascending (the same list, in the same order, as a scan over every colloid).
+template <bool FILL>
+__global__ void k_lub_pairs(SysConst s, const double* __restrict__ xm, double rcut, const LubGrid* __restrict__ gp,
+                            const int* __restrict__ cell_of, const int* __restrict__ start,
+                            const int* __restrict__ mem, int* __restrict__ cnt_or_off, int* __restrict__ part) {
+    const int i = blockIdx.x * LT + threadIdx.x;
+    if (i >= s.Nc) return;
+    const LubGrid G = *gp;
+    const int ci = cell_of[i];
+    const int cx = ci % G.dims[0], cy = (ci / G.dims[0]) % G.dims[1], cz = ci / (G.dims[0] * G.dims[1]);
+    const int c0[3] = {cx, cy, cz};
+    int lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {
+        // periodic axes of 1 or 2 cells visit each cell once; open axes stop at the grid's edge
+        if (G.periodic) {
+            lo[k] = G.dims[k] >= 3 ? -1 : 0;
+            hi[k] = G.dims[k] >= 3 ? 1 : G.dims[k] - 1;
+        } else {
+            lo[k] = c0[k] > 0 ? -1 : 0;
+            hi[k] = c0[k] < G.dims[k] - 1 ? 1 : 0;
+        }
+    }
+    int o = FILL ? cnt_or_off[i] : 0;
+    const int o0 = o;
     double e[3];
-    for (int j = 0; j < s.Nc; j++) {
-        const double r = j != i ? lub_geom(s, xm, s.Nm + i, s.Nm + j, e) : 0.0;
-        if (r > 0.0 && r < rcut) part[o++] = j;
+    for (int dz = lo[2]; dz <= hi[2]; dz++)
+        for (int dy = lo[1]; dy <= hi[1]; dy++)
+            for (int dx = lo[0]; dx <= hi[0]; dx++) {
+                int q[3] = {c0[0] + dx, c0[1] + dy, c0[2] + dz};
+                if (G.periodic)
+                    for (int k = 0; k < 3; k++) q[k] = (q[k] + G.dims[k]) % G.dims[k];
+                const int cq = (q[2] * G.dims[1] + q[1]) * G.dims[0] + q[0];
+                for (int t = start[cq]; t < start[cq + 1]; t++) {
+                    const int j = mem[t];
+                    if (j == i) continue;
+                    const double r = lub_geom(s, xm, s.Nm + i, s.Nm + j, e);
+                    if (r > 0.0 && r < rcut) {
+                        if (FILL) part[o] = j;
+                        o++;
+                    }
+                }
+            }
+    if (!FILL) {
+        cnt_or_off[i] = o;
+        return;
+    }
+    for (int a = o0 + 1; a < o; a++) {   // insertion sort of the (short) partner list
+        const int v = part[a];
+        int b = a - 1;
+        while (b >= o0 && part[b] > v) {
+            part[b + 1] = part[b];
+            b--;
+        }
+        part[b + 1] = v;
     }
 }
 
@@ -287,30 +442,51 @@ ps_err lub_set_table(Ctx* c, int which, const double* rows, int nrows, int ncols
     return PS_OK;
 }
 
-ps_err lub_build(Ctx* c, double rcut, int64_t* npairs) {
-    if (!c->have_mob) {
-        c->set_error("lub_build: call ps_mob first (the pairs use the mob() positions)");
-        return PS_ERR_STATE;
-    }
+// The colloid pairs closer than rcut at the positions of the last mob(), through the cell list.  One
+// host round trip (the partner counts size the list).
+ps_err lub_pairs(Ctx* c, double rcut, int64_t* npairs) {
     for (int w = 0; w < 3; w++)
         if (!c->lub_tab[w]) {
-            c->set_error("lub_build: load the three tables with ps_lub_set_table first");
+            c->set_error("lubrication: load the three tables with ps_lub_set_table first");
             return PS_ERR_STATE;
         }
     const double rmax = std::min(c->lub_rlast[0], std::min(c->lub_rlast[1], c->lub_rlast[2]));
     if (!(rcut > 0.0 && rcut <= rmax)) {
-        c->set_error("lub_build: 0 < rcut <= the last tabulated distance");
+        c->set_error("lubrication: 0 < rcut <= the last tabulated distance");
         return PS_ERR_ARG;
     }
     const int Nc = c->Nc;
-    if (!c->lub_off) PS_HIP(c, hipMalloc(&c->lub_off, sizeof(int) * (Nc + 1)));
+    const int cap = 2 * Nc + 64;   // cells
+    hipStream_t st = c->stream;
+    if (!c->lub_off) {
+        PS_HIP(c, hipMalloc(&c->lub_off, sizeof(int) * (Nc + 1)));
+        PS_HIP(c, hipMalloc(&c->lub_grid, sizeof(LubGrid)));
+        PS_HIP(c, hipMalloc(&c->lub_cell_of, sizeof(int) * std::max(Nc, 1)));
+        PS_HIP(c, hipMalloc(&c->lub_cell_mem, sizeof(int) * std::max(Nc, 1)));
+        PS_HIP(c, hipMalloc(&c->lub_cells, sizeof(int) * (3 * (size_t)cap + 1)));
+    }
+    int* cell_cnt = c->lub_cells;
+    int* cell_start = c->lub_cells + cap;
+    int* cell_cur = c->lub_cells + 2 * cap + 1;
     std::vector<int> cnt(std::max(Nc, 1)), off(Nc + 1, 0);
     const unsigned g = (unsigned)((Nc + LT - 1) / LT);
     if (Nc > 0) {
-        k_lub_count<<<g, LT, 0, c->stream>>>(c->sc, c->xm, rcut, c->lub_off);
+        LubGrid* G = static_cast<LubGrid*>(c->lub_grid);
+        k_lub_grid<<<1, LT, 0, st>>>(c->sc, c->xm, rcut, cap, G);
         PS_LAUNCH(c);
-        PS_HIP(c, hipMemcpyAsync(cnt.data(), c->lub_off, sizeof(int) * Nc, hipMemcpyDeviceToHost, c->stream));
-        PS_HIP(c, hipStreamSynchronize(c->stream));
+        k_zero_cells<<<(cap + LT - 1) / LT, LT, 0, st>>>(cell_cnt, cap);
+        PS_LAUNCH(c);
+        k_lub_bin<<<g, LT, 0, st>>>(c->sc, c->xm, G, c->lub_cell_of, cell_cnt);
+        PS_LAUNCH(c);
+        k_lub_scan<<<1, LT, 0, st>>>(G, cell_cnt, cell_start, cell_cur);
+        PS_LAUNCH(c);
+        k_lub_scatter<<<g, LT, 0, st>>>(Nc, c->lub_cell_of, cell_cur, c->lub_cell_mem);
+        PS_LAUNCH(c);
+        k_lub_pairs<false><<<g, LT, 0, st>>>(c->sc, c->xm, rcut, G, c->lub_cell_of, cell_start, c->lub_cell_mem,
+                                               c->lub_off, nullptr);
+        PS_LAUNCH(c);
+        PS_HIP(c, hipMemcpyAsync(cnt.data(), c->lub_off, sizeof(int) * Nc, hipMemcpyDeviceToHost, st));
+        PS_HIP(c, hipStreamSynchronize(st));
     }
     for (int i = 0; i < Nc; i++) off[i + 1] = off[i] + cnt[i];
     const int total = off[Nc];
@@ -320,15 +496,26 @@ ps_err lub_build(Ctx* c, double rcut, int64_t* npairs) {
         c->lub_cap = std::max(total, 16);
         PS_HIP(c, hipMalloc(&c->lub_part, sizeof(int) * c->lub_cap));
     }
-    PS_HIP(c, hipMemcpy(c->lub_off, off.data(), sizeof(int) * (Nc + 1), hipMemcpyHostToDevice));
+    PS_HIP(c, hipMemcpyAsync(c->lub_off, off.data(), sizeof(int) * (Nc + 1), hipMemcpyHostToDevice, st));
     if (total > 0) {
-        k_lub_fill<<<g, LT, 0, c->stream>>>(c->sc, c->xm, rcut, c->lub_off, c->lub_part);
+        k_lub_pairs<true><<<g, LT, 0, st>>>(c->sc, c->xm, rcut, static_cast<LubGrid*>(c->lub_grid), c->lub_cell_of,
+                                              cell_start, c->lub_cell_mem, c->lub_off, c->lub_part);
         PS_LAUNCH(c);
     }
+    // off.data() is read by the copy above before the vector goes out of scope
+    PS_HIP(c, hipStreamSynchronize(st));
     c->lub_rcut = rcut;
     c->lub_built = true;
     if (npairs) *npairs = total / 2;
     return PS_OK;
+}
+
+ps_err lub_build(Ctx* c, double rcut, int64_t* npairs) {
+    if (!c->have_mob) {
+        c->set_error("lub_build: call ps_mob first (the pairs use the mob() positions)");
+        return PS_ERR_STATE;
+    }
+    return lub_pairs(c, rcut, npairs);
 }
 
 ps_err lub_apply_dev(Ctx* c, const double* uw, const double* einf, double* fts) {
@@ -351,14 +538,48 @@ ps_err lub_apply_dev(Ctx* c, const double* uw, const double* einf, double* fts) 
     return PS_OK;
 }
 
+ps_err lub_alloc(Ctx* c) {
+    // colloid saddle of the lubricated block solve: b, x, inverse diagonal (17 Nc each), the lubrication
+    // output and the colloid-source partial rows (11 Nc each)
+    const size_t nc = (size_t)std::max(c->Nc, 1);
+    PS_HIP(c, hipMalloc(&c->cs_buf, sizeof(double) * 73 * nc));
+    return PS_OK;
+}
+
+namespace {
+// y[n + nm3 + k] -= fts[k], k < 6 Nc: the lower-right block -R_lub acting on the colloid velocities
+__global__ void k_lub_sub(int64_t nc6, const double* __restrict__ fts, double* __restrict__ y) {
+    const int64_t k = (int64_t)blockIdx.x * LT + threadIdx.x;
+    if (k < nc6) y[k] -= fts[k];
+}
+}  // namespace
+
+ps_err lub_saddle_term(Ctx* c, const double* x, double* y) {
+    if (!c->lub_built) {
+        c->set_error("lubrication: no pair list (ps_mob builds it)");
+        return PS_ERR_STATE;
+    }
+    const int64_t off = c->n + c->sc.nm3, nc6 = 6 * (int64_t)c->Nc;
+    double* fts = c->cs_buf + 51 * (size_t)std::max(c->Nc, 1);
+    PS_TRY(lub_apply_dev(c, x + off, nullptr, fts));
+    if (nc6 > 0) {
+        k_lub_sub<<<(unsigned)((nc6 + LT - 1) / LT), LT, 0, c->stream>>>(nc6, fts, y + off);
+        PS_LAUNCH(c);
+    }
+    return PS_OK;
+}
+
 void lub_free(Ctx* c) {
     for (int w = 0; w < 3; w++) {
         if (c->lub_tab[w]) (void)hipFree(c->lub_tab[w]);
         c->lub_tab[w] = nullptr;
     }
-    if (c->lub_off) (void)hipFree(c->lub_off);
-    if (c->lub_part) (void)hipFree(c->lub_part);
-    c->lub_off = c->lub_part = nullptr;
+    for (void* q : {(void*)c->lub_off, (void*)c->lub_part, c->lub_grid, (void*)c->lub_cell_of, (void*)c->lub_cell_mem,
+                    (void*)c->lub_cells, (void*)c->cs_buf})
+        if (q) (void)hipFree(q);
+    c->lub_off = c->lub_part = c->lub_cell_of = c->lub_cell_mem = c->lub_cells = nullptr;
+    c->lub_grid = nullptr;
+    c->cs_buf = nullptr;
     c->lub_built = false;
 }
 

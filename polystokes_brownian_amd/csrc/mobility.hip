@@ -632,15 +632,16 @@ __global__ __launch_bounds__(PS_BLOCK) void k_apply_sym(SysConst s, ApplyGrid G,
 }
 
 // ---------------------------------------------------------------------------- BB (FP64)
-// Colloid rows from colloid sources: one block per colloid, threads over partners.
+// Colloid rows from colloid sources: one block per colloid, threads over partners.  fc is the colloid
+// segment of a force vector: F (3Nc), T (3Nc), S (5Nc).
 __global__ __launch_bounds__(PS_BLOCK) void k_bb(SysConst s, const double* __restrict__ xm,
-                                                 const double* __restrict__ f, double* __restrict__ bb) {
+                                                 const double* __restrict__ fc, double* __restrict__ bb) {
     __shared__ double sh[4][11];
     const int Nm = s.Nm, Nc = s.Nc;
     const int c = blockIdx.x;
     double acc[11];
     for (int k = 0; k < 11; k++) acc[k] = 0.0;
-    const int64_t nm3 = s.nm3, nc3 = s.nc3, nu = s.nu;
+    const int64_t nc3 = s.nc3;
     for (int o = threadIdx.x; o < Nc; o += PS_BLOCK) {
         if (o == c) continue;
         const bool role1 = c < o;   // pair (min, max); c is particle "1" when it is the smaller index
@@ -650,9 +651,9 @@ __global__ __launch_bounds__(PS_BLOCK) void k_bb(SysConst s, const double* __res
         BBT T;
         tile_bb(dr_inv, e, T);
         double Fo[3], To[3], So[5];
-        for (int k = 0; k < 3; k++) Fo[k] = f[nm3 + 3 * o + k];
-        for (int k = 0; k < 3; k++) To[k] = f[nm3 + nc3 + 3 * o + k];
-        for (int k = 0; k < 5; k++) So[k] = f[nu + 5 * o + k];
+        for (int k = 0; k < 3; k++) Fo[k] = fc[3 * o + k];
+        for (int k = 0; k < 3; k++) To[k] = fc[nc3 + 3 * o + k];
+        for (int k = 0; k < 5; k++) So[k] = fc[2 * nc3 + 5 * o + k];
         if (role1) {
             for (int i = 0; i < 3; i++) {
                 double u = 0.0, w = 0.0;
@@ -839,13 +840,14 @@ __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, in
     if (i < n) b[i] = (float)a[i];
 }
 
-// saddle: 0 = u = M f, 1 = saddle operator, 2 = only the stresslet rows [nu, n) of M f.
+// saddle: 0 = u = M f, 1 = saddle operator, 2 = only the stresslet rows [nu, n) of M f, 3 = only the
+// colloid rows [nm3, n) of M f.
 template <class T>
 ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double* u, int saddle) {
     const SysConst& s = c->sc;
     hipStream_t st = c->stream;
     hipEvent_t ea0 = nullptr, ea1 = nullptr, ek0 = nullptr, ek1 = nullptr;
-    const bool srows = saddle == 2;
+    const bool srows = saddle == 2 || saddle == 3;
     if (c->prof_on && !srows) {
         PS_TRY(c->prof_record(1, &ea0, &ea1));
         PS_HIP(c, hipEventRecord(ea0, st));
@@ -853,7 +855,7 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
     const bool box = s.box_active, ch = s.chain_HI;
     // colloid pairs (FP64, every form)
     if (s.Nc > 0) {
-        k_bb<<<s.Nc, PS_BLOCK, 0, st>>>(s, c->xm, f, aw.bb);
+        k_bb<<<s.Nc, PS_BLOCK, 0, st>>>(s, c->xm, f + s.nm3, aw.bb);
         PS_LAUNCH(c);
     }
     if (srows) {
@@ -943,7 +945,7 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
             }
         }
     }
-    const int64_t r_begin = srows ? s.nu : 0;
+    const int64_t r_begin = saddle == 2 ? s.nu : saddle == 3 ? s.nm3 : 0;
     if (s.n - r_begin <= 0) return PS_OK;
     dim3 g4((unsigned)((s.n - r_begin + PS_BLOCK - 1) / PS_BLOCK));
     k_finalize<T><<<g4, PS_BLOCK, 0, st>>>(s, f, u, (const T*)aw.part_aa, s.Nm > 0 ? aw.nsplit_aa : 0,
@@ -1186,10 +1188,12 @@ ps_err mob_prepare(Ctx* c) {
         k_aa_pos<float><<<gm, PS_BLOCK, 0, c->stream>>>(c->Nm, c->xm, c->aw32.inv_L, (float*)c->aw32.xs);
         PS_LAUNCH(c);
     }
-    if (c->Mss && c->Nc > 0) {
+    if (c->Mss && c->Nc > 0 && !c->p.lub) {   // the lubricated block solve applies M_cc on the fly
         k_mss<<<c->Nc, PS_BLOCK, 0, c->stream>>>(c->sc, c->xm, c->Mss);
         PS_LAUNCH(c);
     }
+    // lubrication: the colloid pairs at these positions (one host round trip)
+    if (c->p.lub) PS_TRY(lub_pairs(c, c->p.lub_rcut, nullptr));
     c->cnt.mob_calls++;
     return PS_OK;
 }
@@ -1205,10 +1209,65 @@ ps_err mob_apply_stresslet_rows(Ctx* c, const double* f, double* u) {
     return launch_apply<double>(c, c->xm, c->aw64, f, u, 2);
 }
 
+ps_err mob_apply_colloid_rows(Ctx* c, const double* f, double* u) {
+    return launch_apply<double>(c, c->xm, c->aw64, f, u, 3);
+}
+
 ps_err saddle_apply(Ctx* c, const double* x, double* y, int prec) {
     if (c->arrow) return arrow_apply(c, x, y, 1);
-    if (prec == PS_PREC_MIXED && c->xm32) return launch_apply<float>(c, c->xm32, c->aw32, x, y, 1);
-    return launch_apply<double>(c, c->xm, c->aw64, x, y, 1);
+    if (prec == PS_PREC_MIXED && c->xm32) PS_TRY(launch_apply<float>(c, c->xm32, c->aw32, x, y, 1));
+    else PS_TRY(launch_apply<double>(c, c->xm, c->aw64, x, y, 1));
+    // lubrication: the lower-right block -R_lub on the colloid velocities (FP64 in every form)
+    if (c->p.lub) PS_TRY(lub_saddle_term(c, x, y));
+    return PS_OK;
+}
+
+namespace {
+// The colloid saddle of the lubricated block solve, v = [z (11Nc: F, T, S); w (6Nc: U, W)]:
+//   y_z = M_cc z - [w; 0],  y_w = -z_FT - R_lub w,
+// with M_cc the colloid-colloid block of M (self blocks of fill_self, src/mob.cpp:600-612, and the
+// BB tiles in bb) and fts = R_lub w.
+__global__ void k_cs_combine(int Nc, const double* __restrict__ v, const double* __restrict__ bb,
+                             const double* __restrict__ fts, double* __restrict__ y) {
+    const int64_t r = (int64_t)blockIdx.x * PS_BLOCK + threadIdx.x;
+    const int64_t nc3 = 3 * (int64_t)Nc, nc6 = 6 * (int64_t)Nc, nc11 = 11 * (int64_t)Nc;
+    if (r >= nc11 + nc6) return;
+    const double* w = v + nc11;
+    double out;
+    if (r < nc3) {
+        const int c = (int)(r / 3), comp = (int)(r % 3);
+        out = v[r] + bb[11 * (int64_t)c + comp] - w[r];
+    } else if (r < nc6) {
+        const int c = (int)((r - nc3) / 3), comp = (int)((r - nc3) % 3);
+        out = 0.75 * v[r] + bb[11 * (int64_t)c + 3 + comp] - w[r];
+    } else if (r < nc11) {
+        const int c = (int)((r - nc6) / 5), comp = (int)((r - nc6) % 5);
+        const double* S = v + nc6 + 5 * (int64_t)c;
+        double self = (9. / 5.) * S[comp];
+        if (comp == 0) self += (9. / 10.) * S[4];
+        if (comp == 4) self += (9. / 10.) * S[0];
+        out = self + bb[11 * (int64_t)c + 6 + comp];
+    } else {
+        const int64_t k = r - nc11;
+        out = -v[k] - fts[k];
+    }
+    y[r] = out;
+}
+}  // namespace
+
+ps_err colloid_saddle_apply(Ctx* c, const double* v, double* y) {
+    const int Nc = c->Nc;
+    const size_t ncb = (size_t)std::max(Nc, 1);
+    double* fts = c->cs_buf + 51 * ncb;
+    double* bb = c->cs_buf + 62 * ncb;
+    hipStream_t st = c->stream;
+    k_bb<<<Nc, PS_BLOCK, 0, st>>>(c->sc, c->xm, v, bb);
+    PS_LAUNCH(c);
+    PS_TRY(lub_apply_dev(c, v + 11 * (int64_t)Nc, nullptr, fts));
+    const int64_t N = 17 * (int64_t)Nc;
+    k_cs_combine<<<(unsigned)((N + PS_BLOCK - 1) / PS_BLOCK), PS_BLOCK, 0, st>>>(Nc, v, bb, fts, y);
+    PS_LAUNCH(c);
+    return PS_OK;
 }
 
 // ---------------------------------------------------------------------------- no-allocation apply
@@ -1347,7 +1406,7 @@ extern "C" ps_err ps_mob_mult_noalloc(const ps_params* p, const double* x, const
     k_noalloc_init<<<(unsigned)((ninit + PS_BLOCK - 1) / PS_BLOCK), PS_BLOCK, 0, st>>>(stale, chain, Nm, nmono,
                                                                                       part_c, nzero);
     k_stale<<<1, 64, 0, st>>>(s, x, si, sj, stale);
-    if (Nc > 0) k_bb<<<Nc, PS_BLOCK, 0, st>>>(s, x, f, bb);
+    if (Nc > 0) k_bb<<<Nc, PS_BLOCK, 0, st>>>(s, x, f + s.nm3, bb);
     const bool box = s.box_active, ch = s.chain_HI;
     if (Nm > 0) {
         dim3 g1(ntb, nsplit);

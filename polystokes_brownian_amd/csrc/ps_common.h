@@ -218,6 +218,12 @@ struct Ctx {
     int lub_cap = 0;
     double lub_rcut = 0.0;
     bool lub_built = false;
+    void* lub_grid = nullptr;       // cell-list geometry (device)
+    int* lub_cell_of = nullptr;     // [Nc] cell of each colloid
+    int* lub_cell_mem = nullptr;    // [Nc] colloids by cell
+    int* lub_cells = nullptr;       // counts, starts, cursors of the cells
+    // lubricated block solve: the colloid saddle (17 Nc unknowns) and its work vectors
+    double* cs_buf = nullptr;
 };
 
 // ----------------------------------------------------------------------------- device helpers
@@ -246,6 +252,7 @@ ps_err mob_prepare(Ctx* c);                                        // stale tile
 ps_err mob_apply(Ctx* c, const double* f, double* u, int prec);    // u = M f (device pointers)
 ps_err saddle_apply(Ctx* c, const double* x, double* y, int prec); // y = A x (device pointers)
 ps_err mob_apply_stresslet_rows(Ctx* c, const double* f, double* u); // u[nu:n] = (M f)[nu:n], FP64
+ps_err mob_apply_colloid_rows(Ctx* c, const double* f, double* u);   // u[nm3:n] = (M f)[nm3:n], FP64
 ps_err apply_alloc(Ctx* c);
 void apply_free(Ctx* c);
 ps_err krylov_alloc(Ctx* c);
@@ -281,6 +288,10 @@ ps_err arrow_wca(Ctx* c);                                          // monomer WC
 ps_err scan_ints(Ctx* c, const int* in, int* out, int n);          // out[n] = total, out[n+1] = max
 ps_err lub_set_table(Ctx* c, int which, const double* rows, int nrows, int ncols);
 ps_err lub_build(Ctx* c, double rcut, int64_t* npairs);
+ps_err lub_pairs(Ctx* c, double rcut, int64_t* npairs);   // the pair list, without the mob() check
+ps_err lub_alloc(Ctx* c);                                  // work buffers of the lubricated saddle
+ps_err colloid_saddle_apply(Ctx* c, const double* v, double* y);   // the lubricated colloid saddle, 17 Nc
+ps_err lub_saddle_term(Ctx* c, const double* x, double* y);   // y[n+nm3 : n+nu] -= R_lub x[n+nm3 : n+nu]
 ps_err lub_apply_dev(Ctx* c, const double* uw, const double* einf, double* fts);
 void lub_free(Ctx* c);
 ps_err lub_interp_host(const double* rows, int nrows, int ncols, double r, double* out);

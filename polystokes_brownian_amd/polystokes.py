@@ -135,6 +135,9 @@ class Context:
         self.n = 3 * self.Nm + 11 * self.Nc
         self.nu = 3 * self.Nm + 6 * self.Nc
         self.ns = self.n + self.nu
+        if params.lub:
+            # lubrication in the saddle reads the bundled two-sphere tables
+            self.lub_set_tables()
 
     def _check(self, rc, allow_noconv=False):
         if rc == PS_OK or (allow_noconv and rc == PS_ERR_NOCONV):
