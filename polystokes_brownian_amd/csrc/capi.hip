@@ -230,7 +230,7 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s, double* noise_fixed = nullptr) {
         if (!noise_fixed) PS_TRY(pipe->release(s));
         k_probe<<<eb(n3), TB, 0, st>>>(c->x, c->fext, eps, n3);
         PS_LAUNCH(c);
-        PS_TRY(check_dist_dev(c));
+        PS_TRY(probe_geometry_dev(c));
         PS_TRY(mob_prepare(c));
         PS_TRY(rhs_dev(c, 1));
         PS_TRY(solve(c, c->Xd, 0, c->drift_v));
@@ -240,7 +240,7 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s, double* noise_fixed = nullptr) {
         }
         k_probe<<<eb(n3), TB, 0, st>>>(c->x, c->fext, -2.0 * eps, n3);
         PS_LAUNCH(c);
-        PS_TRY(check_dist_dev(c));
+        PS_TRY(probe_geometry_dev(c));
         PS_TRY(mob_prepare(c));
         PS_TRY(rhs_dev(c, 1));
         PS_TRY(solve(c, c->Xd, 1, c->udiff));

@@ -24,6 +24,7 @@ namespace {
 constexpr int VB = PS_BLOCK;   // threads per vector block
 constexpr int VPT = 4;         // elements per thread
 constexpr int VCHUNK = VB * VPT;
+constexpr int MS_NSYS = 4;     // stresslet systems one launch of the wide MINRES can carry
 
 inline int nblocks(int64_t n) { return (int)std::max<int64_t>(1, (n + VCHUNK - 1) / VCHUNK); }
 
@@ -338,7 +339,7 @@ ps_err krylov_alloc(Ctx* c) {
     PS_HIP(c, hipMalloc(&c->dinv, sizeof(double) * (size_t)ns));
     c->red_cap = std::max(8192, 2 * nblocks(ns) * 66);
     PS_HIP(c, hipMalloc(&c->red, sizeof(double) * (size_t)c->red_cap));
-    PS_HIP(c, hipMalloc(&c->mstate, sizeof(MinresState) * 2));
+    PS_HIP(c, hipMalloc(&c->mstate, sizeof(MinresState) * MS_NSYS));
     PS_HIP(c, hipHostMalloc(&c->mstate_h, sizeof(MinresState) * 2, hipHostMallocDefault));
     c->kmax = std::max(1, std::min<int>(c->p.sqrt_kmax > 0 ? c->p.sqrt_kmax : 64, 64));
     if ((int64_t)c->kmax > n) c->kmax = (int)std::max<int64_t>(1, n);
@@ -766,65 +767,128 @@ __global__ __launch_bounds__(VB) void k_ms_solve(int64_t N, const double* __rest
     }
 }
 
-// The same recurrence for a small block (5Nc <= MS_LDS_N): every block keeps r1, r2 and v of the
-// whole vector in LDS (computed redundantly, so identical in every block) and its rows of A in
-// registers (rows x ceil(N / 256) <= 32 doubles a thread).  The only data crossing blocks is
-// y = A v: each block writes its rows and, after the barrier, reads the whole vector once.
-constexpr int MS_LDS_N = 1024, MS_MAXR = 8, MS_MAXK = 4;
+// The same recurrence for blocks of up to 2048 unknowns, several independent systems in one launch
+// (MS_NSYS at most).  Each system gets nbs blocks of 1024 threads; a block keeps r1, r2, v, z and y
+// of its system's whole vector in LDS (computed redundantly, so identical in every block of the
+// system) and R = 1024 / TPR of its rows of A in registers: TPR threads per row, 32 columns a
+// thread.  The only data crossing blocks is y = A v: each block writes its rows and, after the
+// grid-wide barrier, reads its system's whole vector once.  The barrier also tells every block
+// whether any system is still iterating, so all blocks leave together once the last one converges;
+// a system solved earlier sits out the remaining barriers.  With one system this is the single
+// solve; with several, one launch costs the slowest system's iterations instead of their sum.
+constexpr int MW = 1024, MW_COLS = 32;
 
-__global__ __launch_bounds__(VB) void k_ms_solve_lds(int N, const double* __restrict__ A, const double* __restrict__ b,
-                                                     double* __restrict__ x, double* __restrict__ ybuf, int rows,
-                                                     double rtol, double abstol, int maxit, unsigned* gb,
-                                                     MinresState* __restrict__ st_out, int64_t* __restrict__ dcnt) {
+struct MsSys {
+    const double* A;    // N x N, row-major
+    const double* b;    // N
+    double* x;          // N
+    double* ybuf;       // [2][N]: y = A v, double-buffered across the barrier
+    MinresState* st;    // final state
+};
+struct MsBatch {
+    MsSys s[MS_NSYS];
+};
+
+// Deterministic sum over the 16 waves of a 1024-thread block (fixed tree), broadcast to all threads.
+__device__ __forceinline__ double block_reduce_w(double v, double* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) r[k] = sh[2 * k] + sh[2 * k + 1];
+    return ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+}
+
+// grid_sync that also counts the blocks still iterating: each arrival adds 1 to the low half of the
+// arrival word and `active` to the high half; the last one publishes the count in gb[3] before
+// releasing the generation.  `any` is set to whether any block arrived active.
+__device__ bool grid_sync_any(unsigned* gb, unsigned nblocks, unsigned& gen, bool active, bool& any) {
+    __shared__ int ok_sh, any_sh;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const unsigned target = gen + 1;
+        const unsigned add = 1u + (active ? 0x10000u : 0u);
+        const unsigned old = __hip_atomic_fetch_add(&gb[0], add, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if ((old & 0xffffu) == nblocks - 1) {
+            const unsigned nact = (old >> 16) + (active ? 1u : 0u);
+            __hip_atomic_store(&gb[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&gb[3], nact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&gb[1], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const uint64_t t0 = wall_clock64();
+            while ((int)(__hip_atomic_load(&gb[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+                if (__hip_atomic_load(&gb[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    ok = 0;
+                    break;
+                }
+                if (wall_clock64() - t0 > GRID_WAIT_TICKS) {
+                    __hip_atomic_store(&gb[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        ok_sh = ok;
+        any_sh = __hip_atomic_load(&gb[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+    }
+    __syncthreads();
+    gen++;
+    any = any_sh != 0;
+    return ok_sh != 0;
+}
+
+template <int TPR>
+__global__ __launch_bounds__(MW) void k_ms_wide(int N, MsBatch bt, int nbs, double rtol, double abstol, int maxit,
+                                                unsigned* gb, int64_t* __restrict__ dcnt) {
+    constexpr int R = MW / TPR;   // rows per block
     extern __shared__ double lds[];
     double* r1 = lds;           // [N]
     double* r2 = lds + N;       // [N]
     double* v = lds + 2 * N;    // [N]
-    double* z = lds + 3 * N;    // [N]: r2_new, then z = P^-1 r2_new, then v_{k+1}
-    double* y = lds + 4 * N;    // [N]: A v_k, then y - f r1
-    __shared__ double sh[4];
-    __shared__ double shr[MS_MAXR][4];
+    double* z = lds + 3 * N;    // [N]: z = P^-1 r2_new, then v_{k+1} before scaling
+    double* y = lds + 4 * N;    // [N]: A v_k - f r1, then r2_new
+    __shared__ double sh[16];
     if (__hip_atomic_load(&gb[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     unsigned gen = __hip_atomic_load(&gb[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned nbk = gridDim.x;
-    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-    const int rb = blockIdx.x * rows;
-    const int nr = min(rows, N - rb);
+    const int sys = blockIdx.x / nbs, bs = blockIdx.x - sys * nbs;
+    const MsSys S = bt.s[sys];
+    const int t = threadIdx.x, q = t / TPR, tl = t % TPR;
+    const int rb = bs * R;
+    const int nr = max(0, min(R, N - rb));
     const int ncb = N / 5;
-    // this block's rows of A, kept in registers for the whole solve
-    double Ac[MS_MAXR][MS_MAXK];
+    // this block's rows of A, kept in registers for the whole solve (thread (q, tl): row rb + q,
+    // columns tl + k TPR)
+    double Ac[MW_COLS];
 #pragma unroll
-    for (int q = 0; q < MS_MAXR; q++)
-#pragma unroll
-        for (int k = 0; k < MS_MAXK; k++) {
-            const int j = t + k * VB;
-            Ac[q][k] = (q < nr && j < N) ? A[(int64_t)(rb + q) * N + j] : 0.0;
-        }
-    // y_own = A v for this block's rows, into ybuf
+    for (int k = 0; k < MW_COLS; k++) {
+        const int j = tl + k * TPR;
+        Ac[k] = (q < nr && j < N) ? S.A[(int64_t)(rb + q) * N + j] : 0.0;
+    }
     auto gemv = [&](double* __restrict__ yout) {
-        double pq[MS_MAXR];
+        double a = 0.0;
 #pragma unroll
-        for (int q = 0; q < MS_MAXR; q++) {
-            double a = 0.0;
-#pragma unroll
-            for (int k = 0; k < MS_MAXK; k++) {
-                const int j = t + k * VB;
-                if (j < N) a += Ac[q][k] * v[j];
-            }
-            pq[q] = wave_sum(a);
+        for (int k = 0; k < MW_COLS; k++) {
+            const int j = tl + k * TPR;
+            if (j < N) a = __builtin_fma(Ac[k], v[j], a);
         }
-        if (lane == 0)
 #pragma unroll
-            for (int q = 0; q < MS_MAXR; q++) shr[q][wid] = pq[q];
-        __syncthreads();
-        if (t < nr) yout[rb + t] = (shr[t][0] + shr[t][1]) + (shr[t][2] + shr[t][3]);
+        for (int off = TPR / 2; off > 0; off >>= 1) a += __shfl_xor(a, off, TPR);
+        if (tl == 0 && q < nr) yout[rb + q] = a;
     };
     // start (cold): x = 0, r1 = r2 = b, z = P^-1 b, beta1^2 = b . z
     double acc = 0.0;
-    for (int cb = t; cb < ncb; cb += VB) {
+    for (int cb = t; cb < ncb; cb += MW) {
         double r[5], zz[5];
 #pragma unroll
-        for (int k = 0; k < 5; k++) r[k] = b[5 * cb + k];
+        for (int k = 0; k < 5; k++) r[k] = S.b[5 * cb + k];
         ms_prec5(r, zz);
 #pragma unroll
         for (int k = 0; k < 5; k++) {
@@ -834,7 +898,7 @@ __global__ __launch_bounds__(VB) void k_ms_solve_lds(int N, const double* __rest
             acc += r[k] * zz[k];
         }
     }
-    const double b1 = block_reduce(acc, sh);
+    const double b1 = block_reduce_w(acc, sh);
     MinresState s{};
     s.beta1 = b1 > 0.0 ? sqrt(b1) : 0.0;
     s.tol = fmax(rtol * sqrt(fmax(b1, 0.0)), abstol);
@@ -849,111 +913,139 @@ __global__ __launch_bounds__(VB) void k_ms_solve_lds(int N, const double* __rest
         s.reason = 2;
     }
     const double sc0 = s.done ? 0.0 : 1.0 / s.beta;
-    for (int i = t; i < N; i += VB) v[i] = z[i] * sc0;
-    // own rows: w_{k-1}, w_{k-2}, x (thread q holds row rb + q)
+    for (int i = t; i < N; i += MW) v[i] = z[i] * sc0;
+    // own rows: w_{k-1}, w_{k-2}, x (thread t < nr holds row rb + t)
     double wa = 0.0, wb = 0.0, xo = 0.0;
     __syncthreads();
     int par = 0;
-    bool ok = true;
-    if (!s.done) {
-        gemv(ybuf);
-        ok = grid_sync(gb, nbk, gen);
-    }
-    while (ok && !s.done) {
-        const double* yk = ybuf + (par ? N : 0);
-        double* ynext = ybuf + (par ? 0 : N);
-        // y -= (beta/oldb) r1 (from the 2nd iteration); alfa = v . y
-        const bool sub = s.its >= 1;
-        const double f = sub ? s.beta / s.oldb : 0.0;
-        acc = 0.0;
-        for (int i = t; i < N; i += VB) {
-            double ya = yk[i];
-            if (sub) ya -= f * r1[i];
-            y[i] = ya;
-            acc += v[i] * ya;
-        }
-        const double alfa = block_reduce(acc, sh);
-        // r2_new = y - (alfa/beta) r2; z = P^-1 r2_new per colloid block; beta_new^2 = r2_new . z
-        const double f2 = alfa / s.beta;
-        acc = 0.0;
-        for (int cb = t; cb < ncb; cb += VB) {
-            double r[5], zz[5];
-#pragma unroll
-            for (int k = 0; k < 5; k++) {
-                const int i = 5 * cb + k;
-                r[k] = y[i] - f2 * r2[i];
+    if (!s.done) gemv(S.ybuf);
+    bool any = true;
+    bool ok = grid_sync_any(gb, nbk, gen, !s.done, any);
+    while (ok && any) {
+        if (!s.done) {
+            const double* yk = S.ybuf + (par ? N : 0);
+            double* ynext = S.ybuf + (par ? 0 : N);
+            // y -= (beta/oldb) r1 (from the 2nd iteration); alfa = v . y
+            const bool sub = s.its >= 1;
+            const double f = sub ? s.beta / s.oldb : 0.0;
+            acc = 0.0;
+            for (int i = t; i < N; i += MW) {
+                double ya = yk[i];
+                if (sub) ya -= f * r1[i];
+                y[i] = ya;
+                acc += v[i] * ya;
             }
-            ms_prec5(r, zz);
+            const double alfa = block_reduce_w(acc, sh);
+            // r2_new = y - (alfa/beta) r2; z = P^-1 r2_new per colloid block; beta_new^2 = r2_new . z
+            const double f2 = alfa / s.beta;
+            acc = 0.0;
+            for (int cb = t; cb < ncb; cb += MW) {
+                double r[5], zz[5];
 #pragma unroll
-            for (int k = 0; k < 5; k++) {
-                const int i = 5 * cb + k;
-                y[i] = r[k];   // r2_new (y' is dead once read)
-                z[i] = zz[k];
-                acc += r[k] * zz[k];
+                for (int k = 0; k < 5; k++) {
+                    const int i = 5 * cb + k;
+                    r[k] = y[i] - f2 * r2[i];
+                }
+                ms_prec5(r, zz);
+#pragma unroll
+                for (int k = 0; k < 5; k++) {
+                    const int i = 5 * cb + k;
+                    y[i] = r[k];   // r2_new (y' is dead once read)
+                    z[i] = zz[k];
+                    acc += r[k] * zz[k];
+                }
             }
+            const double bb = block_reduce_w(acc, sh);
+            s.alfa = alfa;
+            s.oldb = s.beta;
+            s.beta = sqrt(fmax(bb, 0.0));
+            const double oldeps = s.epsln;
+            const double delta = s.cs * s.dbar + s.sn * alfa;
+            const double gbar = s.sn * s.dbar - s.cs * alfa;
+            s.epsln = s.sn * s.beta;
+            s.dbar = -s.cs * s.beta;
+            const double gamma = fmax(sqrt(gbar * gbar + s.beta * s.beta), 1e-300);
+            s.cs = gbar / gamma;
+            s.sn = s.beta / gamma;
+            const double phi = s.cs * s.phibar;
+            s.phibar = s.sn * s.phibar;
+            s.oldeps = oldeps;
+            s.delta = delta;
+            s.gamma = gamma;
+            s.phi = phi;
+            s.its += 1;
+            if (s.phibar <= s.tol) {
+                s.done = 1;
+                s.reason = (s.phibar <= abstol) ? 3 : 2;
+            } else if (s.beta == 0.0) {
+                s.done = 1;
+                s.reason = 2;
+            } else if (s.its >= maxit) {
+                s.done = 1;
+                s.reason = -3;
+            }
+            const double denom = 1.0 / gamma;
+            const double vs = (s.beta > 0.0) ? 1.0 / s.beta : 0.0;
+            // own rows: w_k = (v - oldeps w_{k-2} - delta w_{k-1}) / gamma, x += phi w_k
+            if (t < nr) {
+                const double wn = (v[rb + t] - oldeps * wb - delta * wa) * denom;
+                wb = wa;
+                wa = wn;
+                xo += phi * wn;
+            }
+            __syncthreads();   // every read of v and r2 of this iteration is done
+            // shift: r1 <- r2, r2 <- r2_new, v <- z / beta_new (each thread its own entries)
+            for (int i = t; i < N; i += MW) {
+                r1[i] = r2[i];
+                r2[i] = y[i];
+                v[i] = z[i] * vs;
+            }
+            __syncthreads();
+            par ^= 1;
+            if (!s.done) gemv(ynext);
         }
-        const double bb = block_reduce(acc, sh);
-        s.alfa = alfa;
-        s.oldb = s.beta;
-        s.beta = sqrt(fmax(bb, 0.0));
-        const double oldeps = s.epsln;
-        const double delta = s.cs * s.dbar + s.sn * alfa;
-        const double gbar = s.sn * s.dbar - s.cs * alfa;
-        s.epsln = s.sn * s.beta;
-        s.dbar = -s.cs * s.beta;
-        const double gamma = fmax(sqrt(gbar * gbar + s.beta * s.beta), 1e-300);
-        s.cs = gbar / gamma;
-        s.sn = s.beta / gamma;
-        const double phi = s.cs * s.phibar;
-        s.phibar = s.sn * s.phibar;
-        s.oldeps = oldeps;
-        s.delta = delta;
-        s.gamma = gamma;
-        s.phi = phi;
-        s.its += 1;
-        if (s.phibar <= s.tol) {
-            s.done = 1;
-            s.reason = (s.phibar <= abstol) ? 3 : 2;
-        } else if (s.beta == 0.0) {
-            s.done = 1;
-            s.reason = 2;
-        } else if (s.its >= maxit) {
-            s.done = 1;
-            s.reason = -3;
-        }
-        const double denom = 1.0 / gamma;
-        const double vs = (s.beta > 0.0) ? 1.0 / s.beta : 0.0;
-        // own rows: w_k = (v - oldeps w_{k-2} - delta w_{k-1}) / gamma, x += phi w_k
-        if (t < nr) {
-            const double wn = (v[rb + t] - oldeps * wb - delta * wa) * denom;
-            wb = wa;
-            wa = wn;
-            xo += phi * wn;
-        }
-        __syncthreads();   // every read of v and r2 of this iteration is done
-        // shift: r1 <- r2, r2 <- r2_new, v <- z / beta_new (each thread its own entries)
-        for (int i = t; i < N; i += VB) {
-            r1[i] = r2[i];
-            r2[i] = y[i];
-            v[i] = z[i] * vs;
-        }
-        __syncthreads();
-        par ^= 1;
-        if (s.done) break;
-        gemv(ynext);
-        ok = grid_sync(gb, nbk, gen);
+        ok = grid_sync_any(gb, nbk, gen, !s.done, any);
     }
-    if (t < nr) x[rb + t] = xo;
-    if (blockIdx.x == 0 && t == 0) {
+    if (t < nr) S.x[rb + t] = xo;
+    if (bs == 0 && t == 0) {
         if (!ok) {
             s.done = 1;
-            s.reason = -5;
+            s.reason = -5;   // KSP_DIVERGED_BREAKDOWN's sign: the wait ran out of time
             dcnt[DC_BARRIER_TIMEOUT] = 1;
         }
-        *st_out = s;
-        dcnt[DC_MSS_ITERS] += s.its;
-        if (s.reason < 0) dcnt[DC_MSS_CAPPED] += 1;
+        *S.st = s;
+        atomicAdd((unsigned long long*)&dcnt[DC_MSS_ITERS], (unsigned long long)s.its);
+        if (s.reason < 0) atomicAdd((unsigned long long*)&dcnt[DC_MSS_CAPPED], 1ull);
     }
+}
+
+// Launch shape of k_ms_wide for N unknowns: threads per row (32 or 64), rows per block, blocks per
+// system; false when a system of this size does not fit (more than 2048 unknowns).
+inline bool ms_wide_shape(int64_t N, int* tpr, int* nbs) {
+    if (N <= 0 || N > 2 * MW) return false;
+    *tpr = N <= MW ? 32 : 64;
+    const int R = MW / *tpr;
+    *nbs = (int)((N + R - 1) / R);
+    return true;
+}
+
+// Launch k_ms_wide over `ns` systems of N unknowns each (at most one block per CU in all, so the whole
+// grid is co-resident for its barriers).
+static ps_err ms_wide_launch(Ctx* c, int64_t N, const MsBatch& bt, int ns, double rtol, int maxit) {
+    int tpr = 0, nbs = 0;
+    if (ns < 1 || ns > MS_NSYS || !ms_wide_shape(N, &tpr, &nbs) || (int64_t)nbs * ns > std::max(1, c->n_cu)) {
+        c->set_error("stresslet MINRES: " + std::to_string(ns) + " systems of " + std::to_string(N) +
+                     " unknowns do not fit one block per CU");
+        return PS_ERR_ARG;
+    }
+    const size_t lds = sizeof(double) * 5 * (size_t)N;
+    const unsigned nb = (unsigned)(nbs * ns);
+    if (tpr == 32)
+        k_ms_wide<32><<<nb, MW, lds, c->stream>>>((int)N, bt, nbs, rtol, c->p.ksp_abstol, maxit, c->gbar, c->dcnt);
+    else
+        k_ms_wide<64><<<nb, MW, lds, c->stream>>>((int)N, bt, nbs, rtol, c->p.ksp_abstol, maxit, c->gbar, c->dcnt);
+    PS_LAUNCH(c);
+    return PS_OK;
 }
 
 static ps_err minres_small_info(Ctx* c, ps_solve_info* info);
@@ -973,16 +1065,11 @@ static ps_err minres_small(Ctx* c, int64_t N, const double* A, const double* b, 
     }
     // at most one block per CU, so the whole grid is co-resident for its barriers
     const int ncu = std::max(1, c->n_cu);
-    // as many rows per block as the registers hold: fewer blocks meet at each barrier
-    const int K = (int)((N + VB - 1) / VB);
-    const int rows_lds = std::min(MS_MAXR, (MS_MAXR * MS_MAXK) / std::max(K, 1));
-    if (N <= MS_LDS_N && (N + rows_lds - 1) / rows_lds <= ncu) {
-        const int rows = rows_lds;
-        const unsigned nb = (unsigned)((N + rows - 1) / rows);
-        const size_t lds = sizeof(double) * 5 * (size_t)N;
-        k_ms_solve_lds<<<nb, VB, lds, st>>>((int)N, A, b, x, c->kr, rows, rtol, c->p.ksp_abstol, maxit, c->gbar,
-                                             c->mstate, c->dcnt);
-        PS_LAUNCH(c);
+    int tpr = 0, nbs = 0;
+    if (ms_wide_shape(N, &tpr, &nbs) && nbs <= ncu) {
+        MsBatch bt{};
+        bt.s[0] = MsSys{A, b, x, c->kr, c->mstate};
+        PS_TRY(ms_wide_launch(c, N, bt, 1, rtol, maxit));
         return minres_small_info(c, info);
     }
     int rows = (int)std::max<int64_t>(8, (N + 255) / 256);
@@ -1050,11 +1137,14 @@ __global__ void k_bs_rs(double* __restrict__ rs, const double* __restrict__ b, c
     const int64_t k = (int64_t)blockIdx.x * VB + threadIdx.x;
     if (k < n5) rs[k] = b[nu + k] - y[nu + k];
 }
-// u = (M f)_u - b_u
+// u = (M f)_u - b_u, into x[n:] and, when given, uout
 __global__ void k_bs_u(double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ b,
-                       int64_t n, int64_t nu) {
+                       int64_t n, int64_t nu, double* __restrict__ uout) {
     const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
-    if (i < nu) x[n + i] = y[i] - b[i];
+    if (i >= nu) return;
+    const double u = y[i] - b[i];
+    x[n + i] = u;
+    if (uout) uout[i] = u;
 }
 
 // Exact block elimination of A x = b with B = [-I_nu; 0]:
@@ -1098,7 +1188,7 @@ static ps_err block_solve_once(Ctx* c, const double* b, double* x, int prec, ps_
         k_cs_scatter<<<eblocks(nc11), VB, 0, st>>>(v, x, nm3, nc11);
         PS_LAUNCH(c);
         PS_TRY(mob_apply(c, x, c->bs_y, prec));
-        k_bs_u<<<eblocks(nu), VB, 0, st>>>(x, c->bs_y, b, n, nu);
+        k_bs_u<<<eblocks(nu), VB, 0, st>>>(x, c->bs_y, b, n, nu, nullptr);
         PS_LAUNCH(c);
         c->cnt.saddle_solves++;
         return PS_OK;
@@ -1120,10 +1210,55 @@ static ps_err block_solve_once(Ctx* c, const double* b, double* x, int prec, ps_
         if (mss_info) mss_info->iterations += si.iterations;
     }
     PS_TRY(mob_apply(c, x, c->bs_y, prec));
-    k_bs_u<<<eblocks(nu), VB, 0, st>>>(x, c->bs_y, b, n, nu);
+    k_bs_u<<<eblocks(nu), VB, 0, st>>>(x, c->bs_y, b, n, nu, nullptr);
     PS_LAUNCH(c);
     c->cnt.saddle_solves++;
     return PS_OK;
+}
+
+// The block elimination in stages, for solves that share one launch of the stresslet MINRES (the
+// joint step).  Stage 1 at the current mob() state: x = [-b_lambda; 0], rs = b_s - (M [f_u; 0])_s.
+ps_err block_stage1(Ctx* c, const double* b, double* x, double* rs) {
+    const int64_t n = c->n, nu = c->nu, n5 = 5 * (int64_t)c->Nc;
+    hipStream_t st = c->stream;
+    k_bs_init<<<eblocks(n), VB, 0, st>>>(x, b, n, nu);
+    PS_LAUNCH(c);
+    PS_TRY(mob_apply_stresslet_rows(c, x, c->bs_y));
+    k_bs_rs<<<eblocks(n5), VB, 0, st>>>(rs, b, c->bs_y, nu, n5);
+    PS_LAUNCH(c);
+    return PS_OK;
+}
+
+// Stage 3 at the current mob() state, once x's stresslet forces are solved: u = (M f)_u - b_u into
+// x[n:] and, when given, uout.
+ps_err block_stage3(Ctx* c, const double* b, double* x, int prec, double* uout) {
+    const int64_t n = c->n, nu = c->nu;
+    PS_TRY(mob_apply(c, x, c->bs_y, prec));
+    k_bs_u<<<eblocks(nu), VB, 0, c->stream>>>(x, c->bs_y, b, n, nu, uout);
+    PS_LAUNCH(c);
+    c->cnt.saddle_solves++;
+    return PS_OK;
+}
+
+// Whether `nsys` stresslet systems fit one launch of k_ms_wide on this device.
+bool mss_joint_fits(const Ctx* c, int nsys) {
+    int tpr = 0, nbs = 0;
+    const int64_t N = 5 * (int64_t)c->Nc;
+    return c->Nc > 0 && nsys >= 1 && nsys <= MS_NSYS && ms_wide_shape(N, &tpr, &nbs) &&
+           (int64_t)nbs * nsys <= std::max(1, c->n_cu) && 2 * N * nsys <= 9 * c->ns;
+}
+
+// The stresslet solves M_k x_k = rs_k of `nsys` block solves in one launch (rtol 1e-12, cap 10000).
+ps_err mss_joint(Ctx* c, int nsys, const double* const* A, const double* const* rs, double* const* x) {
+    const int64_t N = 5 * (int64_t)c->Nc;
+    if (!mss_joint_fits(c, nsys)) {
+        c->set_error("joint stresslet MINRES: " + std::to_string(nsys) + " systems of " + std::to_string(N) +
+                     " unknowns do not fit one launch");
+        return PS_ERR_ARG;
+    }
+    MsBatch bt{};
+    for (int k = 0; k < nsys; k++) bt.s[k] = MsSys{A[k], rs[k], x[k], c->kr + 2 * N * k, c->mstate + k};
+    return ms_wide_launch(c, N, bt, nsys, 1e-12, 10000);
 }
 
 
@@ -1354,11 +1489,17 @@ __global__ __launch_bounds__(VB) void k_lz_final(int64_t n, const double* __rest
 // stopping test picks the m the sequential test would have stopped at, so the result equals
 // stopping there; vectors built past it are never used.  With `info` the outcome is read back.
 ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* out, ps_sqrt_info* info) {
-    const int64_t n = c->n;
     if (c->arrow) {
         if (info) *info = ps_sqrt_info{};
         return arrow_noise(c, xi, out);
     }
+    PS_TRY(sqrt_recurrence_dev(c, xi));
+    return sqrt_finish_dev(c, out, info);
+}
+
+// The Lanczos recurrence of sqrt_apply_dev: kmax vectors into Q, the scalars into lz.
+ps_err sqrt_recurrence_dev(Ctx* c, const double* xi) {
+    const int64_t n = c->n;
     const unsigned nch = (unsigned)((n + LZ_CH - 1) / LZ_CH);
     const unsigned ne = eblocks(n);
     const int nbd = nblocks(n);
@@ -1390,6 +1531,15 @@ ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* out, ps_sqrt_info* info)
                                      c->lz + LZ_BETA2 + j);
         PS_LAUNCH(c);
     }
+    return PS_OK;
+}
+
+// The square-root weights of every m, the stopping test and out = Q g (reads only Q and lz).
+ps_err sqrt_finish_dev(Ctx* c, double* out, ps_sqrt_info* info) {
+    const int64_t n = c->n;
+    const unsigned ne = eblocks(n);
+    hipStream_t st = c->stream;
+    const int kmax = c->kmax;
     const double rtol = c->p.sqrt_rtol;
     double* negs = c->lzg + LZ_G * 64;
     k_lz_ritz<<<kmax, 64, 0, st>>>(c->lz, kmax, rtol > 0.0 ? 1 : 0, c->lzg, negs);

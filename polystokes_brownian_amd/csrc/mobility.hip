@@ -769,7 +769,9 @@ __global__ __launch_bounds__(PS_BLOCK) void k_finalize(SysConst s, const double*
 // ---------------------------------------------------------------------------- carried tile
 // stale[0..8]: the mob_a the AB pairs of this mob() call see; stale[9..17]: what is left for the
 // next call (the last BB pair's mob_a, or this call's tile when there is no BB pair).
-__global__ void k_stale(SysConst s, const double* __restrict__ xm, int si, int sj, double* __restrict__ st) {
+// `prev` is the previous mob() call's tile pair (st itself when the state has one slot).
+__global__ void k_stale(SysConst s, const double* __restrict__ xm, int si, int sj, const double* prev,
+                        double* st) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double cur[9];
     if (si >= 0) {
@@ -790,7 +792,7 @@ __global__ void k_stale(SysConst s, const double* __restrict__ xm, int si, int s
                 cur[3 * i + j] = (i == j) ? x12a * ee + y12a * (1. - ee) : x12a * ee - y12a * ee;
             }
     } else {
-        for (int k = 0; k < 9; k++) cur[k] = st[9 + k];
+        for (int k = 0; k < 9; k++) cur[k] = prev[9 + k];
     }
     for (int k = 0; k < 9; k++) st[k] = cur[k];
     if (s.Nc >= 2) {
@@ -1169,8 +1171,10 @@ ps_err mob_prepare(Ctx* c) {
     }
     PS_HIP(c, hipMemcpyAsync(c->xm, c->xpd, sizeof(double) * 3 * (size_t)c->Np, hipMemcpyDeviceToDevice,
                              c->stream));
-    k_stale<<<1, 64, 0, c->stream>>>(c->sc, c->xm, c->stale_i, c->stale_j, c->stale);
+    k_stale<<<1, 64, 0, c->stream>>>(c->sc, c->xm, c->stale_i, c->stale_j, c->stale_prev ? c->stale_prev : c->stale,
+                                      c->stale);
     PS_LAUNCH(c);
+    c->stale_prev = c->stale;
     if (c->xm32) {
         const int64_t n3 = 3 * (int64_t)c->Np;
         k_to_f32<<<(unsigned)((n3 + PS_BLOCK - 1) / PS_BLOCK), PS_BLOCK, 0, c->stream>>>(c->xm, c->xm32, n3);
@@ -1196,6 +1200,57 @@ ps_err mob_prepare(Ctx* c) {
     if (c->p.lub) PS_TRY(lub_pairs(c, c->p.lub_rcut, nullptr));
     c->cnt.mob_calls++;
     return PS_OK;
+}
+
+// ---------------------------------------------------------------------------- mob() slots
+// The joint step keeps the mob() states of the predictor point and of both RFD probe points alive
+// at once.  Slot 0 is the context's own buffers; slots 1 and 2 get their own copies of everything a
+// mob() call writes: the positions snapshot (FP64 and FP32), the kernel-layout positions, the carried
+// tile pair and the stresslet block.
+ps_err mob_slots_alloc(Ctx* c) {
+    Ctx::MobSlot& s0 = c->mslot[0];
+    s0.xm = c->xm;
+    s0.xm32 = c->xm32;
+    s0.xs64 = c->aw64.sym ? c->aw64.xs : nullptr;
+    s0.xs32 = c->aw32.sym ? c->aw32.xs : nullptr;
+    s0.stale = c->stale;
+    s0.Mss = c->Mss;
+    const size_t n3 = 3 * (size_t)c->Np, nm4 = 4 * (size_t)std::max(c->Nm, 1), n5 = 5 * (size_t)std::max(c->Nc, 1);
+    for (int k = 1; k < 3; k++) {
+        Ctx::MobSlot& m = c->mslot[k];
+        PS_HIP(c, hipMalloc(&m.xm, sizeof(double) * n3));
+        if (s0.xm32) PS_HIP(c, hipMalloc(&m.xm32, sizeof(float) * n3));
+        if (s0.xs64) PS_HIP(c, hipMalloc(&m.xs64, sizeof(double) * nm4));
+        if (s0.xs32) PS_HIP(c, hipMalloc(&m.xs32, sizeof(float) * nm4));
+        PS_HIP(c, hipMalloc(&m.stale, sizeof(double) * 18));
+        PS_HIP(c, hipMemcpy(m.stale, s0.stale, sizeof(double) * 18, hipMemcpyDeviceToDevice));
+        if (s0.Mss) PS_HIP(c, hipMalloc(&m.Mss, sizeof(double) * n5 * n5));
+    }
+    c->nmslot = 3;
+    return PS_OK;
+}
+
+void mob_use_slot(Ctx* c, int k) {
+    if (k < 0 || k >= c->nmslot) return;
+    const Ctx::MobSlot& m = c->mslot[k];
+    c->xm = m.xm;
+    c->xm32 = m.xm32;
+    if (c->aw64.sym) c->aw64.xs = m.xs64;
+    if (c->aw32.sym) c->aw32.xs = m.xs32;
+    c->stale = m.stale;
+    c->Mss = m.Mss;
+}
+
+void mob_slots_free(Ctx* c) {
+    if (c->nmslot > 0) mob_use_slot(c, 0);
+    for (int k = 1; k < 3; k++) {
+        Ctx::MobSlot& m = c->mslot[k];
+        for (void* p : {(void*)m.xm, (void*)m.xm32, m.xs64, m.xs32, (void*)m.stale, (void*)m.Mss})
+            if (p) (void)hipFree(p);
+        m = Ctx::MobSlot();
+    }
+    c->nmslot = 0;
+    c->stale_prev = c->stale;
 }
 
 ps_err mob_apply(Ctx* c, const double* f, double* u, int prec) {
@@ -1405,7 +1460,7 @@ extern "C" ps_err ps_mob_mult_noalloc(const ps_params* p, const double* x, const
     const int64_t ninit = std::max<int64_t>({18, (int64_t)Nm, nzero});
     k_noalloc_init<<<(unsigned)((ninit + PS_BLOCK - 1) / PS_BLOCK), PS_BLOCK, 0, st>>>(stale, chain, Nm, nmono,
                                                                                       part_c, nzero);
-    k_stale<<<1, 64, 0, st>>>(s, x, si, sj, stale);
+    k_stale<<<1, 64, 0, st>>>(s, x, si, sj, stale, stale);
     if (Nc > 0) k_bb<<<Nc, PS_BLOCK, 0, st>>>(s, x, f + s.nm3, bb);
     const bool box = s.box_active, ch = s.chain_HI;
     if (Nm > 0) {

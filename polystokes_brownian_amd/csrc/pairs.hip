@@ -58,12 +58,16 @@ __device__ __forceinline__ int cell_coord(const PairConst& pc, double x, int d) 
     return ci;
 }
 
-__global__ void k_bin(PairConst pc, const double* __restrict__ xs, int* __restrict__ cell_of,
+// Bin particle p by the positions x, which become this check_dist's geometry snapshot xpd.
+__global__ void k_bin(PairConst pc, const double* __restrict__ x, double* __restrict__ xpd, int* __restrict__ cell_of,
                       int* __restrict__ cell_count) {
     const int p = blockIdx.x * TB + threadIdx.x;
     if (p >= pc.s.Np) return;
-    const int cx = cell_coord(pc, xs[3 * p], 0), cy = cell_coord(pc, xs[3 * p + 1], 1),
-              cz = cell_coord(pc, xs[3 * p + 2], 2);
+    const double px = x[3 * p], py = x[3 * p + 1], pz = x[3 * p + 2];
+    xpd[3 * p] = px;
+    xpd[3 * p + 1] = py;
+    xpd[3 * p + 2] = pz;
+    const int cx = cell_coord(pc, px, 0), cy = cell_coord(pc, py, 1), cz = cell_coord(pc, pz, 2);
     const int cell = cx + pc.ncell[0] * (cy + pc.ncell[1] * cz);
     cell_of[p] = cell;
     atomicAdd(&cell_count[cell], 1);
@@ -86,38 +90,70 @@ __global__ void k_scatter(int Np, const double* __restrict__ xs, const int* __re
 }
 
 // Single-block exclusive scan of n ints: out[0..n] with out[n] = total, and out[n + 1] = max(in).
-// Each of the 1024 threads scans a contiguous chunk; the chunk sums are scanned within each wave by
-// shuffles and across the 16 waves through LDS (one barrier).
-__global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ in, int* __restrict__ out, int n) {
+// The 1024 threads walk the array in tiles of 4096: each thread loads four consecutive ints (one
+// 16-byte load, coalesced across the block), the thread sums are scanned within each wave by
+// shuffles and across the 16 waves through LDS, and the next tile's load is in flight meanwhile.
+// With `clear` the input is zeroed once read (the cell counts double as the scatter's cursors).
+constexpr int SCAN_T = 1024, SCAN_TILE = 4 * SCAN_T;
+
+__device__ __forceinline__ int4 scan_load4(const int* __restrict__ in, int n, int i0) {
+    if (i0 + 3 < n) return *reinterpret_cast<const int4*>(in + i0);
+    int4 v = {0, 0, 0, 0};
+    if (i0 < n) v.x = in[i0];
+    if (i0 + 1 < n) v.y = in[i0 + 1];
+    if (i0 + 2 < n) v.z = in[i0 + 2];
+    return v;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan(int* __restrict__ in, int* __restrict__ out, int n, int clear) {
     __shared__ int wsum[16], wmax[16];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int per = (n + 1023) / 1024;
-    const int b = min(t * per, n), e = min(b + per, n);
-    int s = 0, mx = 0;
-    for (int i = b; i < e; i++) {
-        const int v = in[i];
-        s += v;
-        mx = max(mx, v);
-    }
-    int inc = s;   // inclusive scan of the chunk sums within the wave
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += y;
+    int carry = 0, mx = 0;
+    int4 cur = scan_load4(in, n, 4 * t);
+    for (int base = 0; base < n; base += SCAN_TILE) {
+        const int i0 = base + 4 * t;
+        const int4 nxt = base + SCAN_TILE < n ? scan_load4(in, n, i0 + SCAN_TILE) : int4{0, 0, 0, 0};
+        const int s = (cur.x + cur.y) + (cur.z + cur.w);
+        mx = max(mx, max(max(cur.x, cur.y), max(cur.z, cur.w)));
+        int inc = s;   // inclusive scan of the thread sums within the wave
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        int before = 0, tot = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int v = wsum[k];
+            before += k < w ? v : 0;
+            tot += v;
+        }
+        int4 e;
+        e.x = carry + before + inc - s;
+        e.y = e.x + cur.x;
+        e.z = e.y + cur.y;
+        e.w = e.z + cur.z;
+        if (i0 + 3 < n) {
+            *reinterpret_cast<int4*>(out + i0) = e;
+            if (clear) *reinterpret_cast<int4*>(in + i0) = int4{0, 0, 0, 0};
+        } else {
+            const int ev[4] = {e.x, e.y, e.z, e.w};
+            for (int k = 0; k < 4; k++)
+                if (i0 + k < n) {
+                    out[i0 + k] = ev[k];
+                    if (clear) in[i0 + k] = 0;
+                }
+        }
+        carry += tot;
+        __syncthreads();   // wsum is rewritten by the next tile
+        cur = nxt;
     }
     for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
-    if (lane == 63) wsum[w] = inc;
     if (lane == 0) wmax[w] = mx;
     __syncthreads();
-    int base = 0;
-    for (int k = 0; k < w; k++) base += wsum[k];
-    int run = base + inc - s;
-    for (int i = b; i < e; i++) {
-        const int v = in[i];
-        out[i] = run;
-        run += v;
-    }
-    if (t == 1023) {
-        out[n] = base + inc;
+    if (t == 0) {
+        out[n] = carry;
         int m = 0;
         for (int k = 0; k < 16; k++) m = max(m, wmax[k]);
         out[n + 1] = m;
@@ -489,7 +525,7 @@ void pairs_free(Ctx* c) {
 }
 
 ps_err scan_ints(Ctx* c, const int* in, int* out, int n) {
-    k_scan<<<1, 1024, 0, c->stream>>>(in, out, n);
+    k_scan<<<1, SCAN_T, 0, c->stream>>>(const_cast<int*>(in), out, n, 0);
     PS_LAUNCH(c);
     return PS_OK;
 }
@@ -504,20 +540,20 @@ ps_err check_dist_dev(Ctx* c) {
     const int Np = c->Np;
     hipStream_t st = c->stream;
     PairConst pc = make_pc(c);
-    // geometry of this check_dist = current positions
-    PS_HIP(c, hipMemcpyAsync(c->xpd, c->x, sizeof(double) * 3 * (size_t)Np, hipMemcpyDeviceToDevice, st));
+    // geometry of this check_dist = current positions (copied by k_bin in cell mode)
     const unsigned gp = (unsigned)((Np + TB - 1) / TB);
     const double* xsort = c->xpd;
+    if (c->cell_mode != 1)
+        PS_HIP(c, hipMemcpyAsync(c->xpd, c->x, sizeof(double) * 3 * (size_t)Np, hipMemcpyDeviceToDevice, st));
     if (c->cell_mode == 1) {
         const int ncells = c->ncell[0] * c->ncell[1] * c->ncell[2];
         const unsigned gc = (unsigned)((ncells + TB - 1) / TB);
         k_zero_int<<<gc, TB, 0, st>>>(c->cell_count, ncells);
         PS_LAUNCH(c);
-        k_bin<<<gp, TB, 0, st>>>(pc, c->xpd, c->cell_of, c->cell_count);
+        k_bin<<<gp, TB, 0, st>>>(pc, c->x, c->xpd, c->cell_of, c->cell_count);
         PS_LAUNCH(c);
-        k_scan<<<1, 1024, 0, st>>>(c->cell_count, c->cell_start, ncells);
-        PS_LAUNCH(c);
-        k_zero_int<<<gc, TB, 0, st>>>(c->cell_count, ncells);
+        // the scan clears the counts behind it: they are the scatter's cursors next
+        k_scan<<<1, SCAN_T, 0, st>>>(c->cell_count, c->cell_start, ncells, 1);
         PS_LAUNCH(c);
         k_scatter<<<gp, TB, 0, st>>>(Np, c->xpd, c->cell_of, c->cell_start, c->cell_count, c->cell_perm, c->cell_xs);
         PS_LAUNCH(c);
@@ -526,7 +562,7 @@ ps_err check_dist_dev(Ctx* c) {
     const unsigned gg = (unsigned)(((int64_t)Np * GL + TB - 1) / TB);
     k_nb_count<<<gg, TB, 0, st>>>(pc, c->xpd, c->cell_of, c->cell_start, c->cell_perm, xsort, c->nb_count);
     PS_LAUNCH(c);
-    k_scan<<<1, 1024, 0, st>>>(c->nb_count, c->nb_off, Np);
+    k_scan<<<1, SCAN_T, 0, st>>>(c->nb_count, c->nb_off, Np, 0);
     PS_LAUNCH(c);
     // LDS per block: a cursor and VL_ROW_CAP entries for each of its 8 groups
     const int gpb = TB / GL;
@@ -536,6 +572,17 @@ ps_err check_dist_dev(Ctx* c) {
                                                                         c->nb_part, c->nb_cap,
                                                                         c->dcnt + DC_VLIST_OVERFLOW);
     PS_LAUNCH(c);
+    c->have_pd = true;
+    return PS_OK;
+}
+
+// check_dist at an RFD probe point (src/drift.cpp:38,51): mob() there reads only the pair geometry,
+// i.e. the positions, and RHS(true) uses no Verlet pairs, so the list built at a probe is never read
+// (the corrector's check_dist replaces it before the next force evaluation).  Only the geometry
+// snapshot is taken; the arrowhead regime keeps its full check_dist, whose AB list its mob() reads.
+ps_err probe_geometry_dev(Ctx* c) {
+    if (c->arrow) return check_dist_dev(c);
+    PS_HIP(c, hipMemcpyAsync(c->xpd, c->x, sizeof(double) * 3 * (size_t)c->Np, hipMemcpyDeviceToDevice, c->stream));
     c->have_pd = true;
     return PS_OK;
 }
@@ -553,6 +600,13 @@ ps_err vlist_count(Ctx* c, int64_t* n_pairs) {
     c->nvlist = c->hflag[0] / 2;
     c->cnt.nvlist = c->nvlist;
     if (n_pairs) *n_pairs = c->nvlist;
+    return PS_OK;
+}
+
+// RHS(true) (src/rhs.cpp:175-184) into `out` (ns): zero force rows, the RFD noise in the constraint rows.
+ps_err rhs_drift_dev(Ctx* c, double* out) {
+    k_rhs_drift<<<(unsigned)((c->sc.ns + TB - 1) / TB), TB, 0, c->stream>>>(c->sc, c->fext, out);
+    PS_LAUNCH(c);
     return PS_OK;
 }
 

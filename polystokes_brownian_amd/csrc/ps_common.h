@@ -224,6 +224,26 @@ struct Ctx {
     int* lub_cells = nullptr;       // counts, starts, cursors of the cells
     // lubricated block solve: the colloid saddle (17 Nc unknowns) and its work vectors
     double* cs_buf = nullptr;
+    // ---- joint step (one_step_joint): the mob() states at x, x + eps w and x - eps w held at once.
+    //      Slot 0 is the context's own buffers; slots 1 and 2 serve the two RFD probes.  The current
+    //      slot's buffers are the ones in xm, xm32, aw64.xs, aw32.xs, stale and Mss.
+    struct MobSlot {
+        double* xm = nullptr;
+        float* xm32 = nullptr;
+        void* xs64 = nullptr;
+        void* xs32 = nullptr;
+        double* stale = nullptr;
+        double* Mss = nullptr;
+    };
+    MobSlot mslot[3];
+    int nmslot = 0;                 // 0: one mob() state (no joint step); 3: slots in use
+    double* stale_prev = nullptr;   // the carried tile of the last mob() call (the next one reads it)
+    hipStream_t side = nullptr;     // second stream of the joint step (default priority)
+    hipEvent_t jev[2] = {nullptr, nullptr};   // fork and join of the side stream
+    double* rhs_s = nullptr;        // ns: the slip solve's right-hand side
+    double* rhs_d = nullptr;        // ns: the RFD probes' right-hand side
+    double* Xd2 = nullptr;          // ns: the second probe's solution
+    double* bsr4 = nullptr;         // [4][5Nc]: stresslet right-hand sides of the four joint systems
 };
 
 // ----------------------------------------------------------------------------- device helpers
@@ -249,6 +269,15 @@ __device__ __forceinline__ float ps_rint<float>(float v) { return __builtin_rint
 
 // Host entry points implemented in the translation units.
 ps_err mob_prepare(Ctx* c);                                        // stale tile + snapshots
+ps_err mob_slots_alloc(Ctx* c);                                    // slots 1, 2 of the joint step
+void mob_slots_free(Ctx* c);
+void mob_use_slot(Ctx* c, int k);                                  // make slot k the current mob() state
+ps_err block_stage1(Ctx* c, const double* b, double* x, double* rs);
+ps_err block_stage3(Ctx* c, const double* b, double* x, int prec, double* uout);
+bool mss_joint_fits(const Ctx* c, int nsys);
+ps_err mss_joint(Ctx* c, int nsys, const double* const* A, const double* const* rs, double* const* x);
+ps_err sqrt_recurrence_dev(Ctx* c, const double* xi);
+ps_err sqrt_finish_dev(Ctx* c, double* out, ps_sqrt_info* info);
 ps_err mob_apply(Ctx* c, const double* f, double* u, int prec);    // u = M f (device pointers)
 ps_err saddle_apply(Ctx* c, const double* x, double* y, int prec); // y = A x (device pointers)
 ps_err mob_apply_stresslet_rows(Ctx* c, const double* f, double* u); // u[nu:n] = (M f)[nu:n], FP64
@@ -264,7 +293,9 @@ ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* w, ps_sqrt_info* info);
 ps_err pairs_alloc(Ctx* c);
 void pairs_free(Ctx* c);
 ps_err check_dist_dev(Ctx* c);
+ps_err probe_geometry_dev(Ctx* c);
 ps_err rhs_dev(Ctx* c, int drift);
+ps_err rhs_drift_dev(Ctx* c, double* out);
 ps_err vlist_copy(Ctx* c, int32_t* rows, int32_t* partners, int64_t* pidx, int64_t cap, int64_t* n_pairs);
 void ps_set_context_free_error(const std::string& msg);           // for ps_last_error(NULL)
 ps_err vlist_count(Ctx* c, int64_t* n_pairs);                      // pair count of the last check_dist (syncs)
