@@ -197,7 +197,9 @@ __global__ void k_zero(double* __restrict__ a, int64_t n) {
 // One iteration of the run() loop; `pipe` supplies this step's noise (step index s), or is null
 // when kT <= 0.  With `noise_fixed` (graph capture) the noise is read from that pinned slot and the
 // pipe is not touched.
+ps_err one_step_joint(Ctx* c, NoisePipe* pipe, int s, double* noise_fixed);
 ps_err one_step(Ctx* c, NoisePipe* pipe, int s, double* noise_fixed = nullptr) {
+    if (c->nmslot > 0) return one_step_joint(c, pipe, s, noise_fixed);
     const int64_t n = c->n, nu = c->nu, n3 = 3 * (int64_t)c->Np, nc5 = 5 * (int64_t)c->Nc;
     hipStream_t st = c->stream;
     const double dt = c->p.dt, kT = c->p.kT, eps = 1.0e-3;
@@ -279,6 +281,168 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s, double* noise_fixed = nullptr) {
     }
     c->have_mob = true;
     return PS_OK;
+}
+
+// rhs_s = rhs with a * w added to its first n entries (the slip solve's right-hand side)
+__global__ void k_slip_rhs(double* __restrict__ rhs_s, const double* __restrict__ rhs, const double* __restrict__ w,
+                           double a, int64_t n, int64_t ns) {
+    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i < ns) rhs_s[i] = i < n ? rhs[i] + a * w[i] : rhs[i];
+}
+
+// Whether the step runs as one_step_joint: full monomer HI, block-eliminated solves with no
+// refinement sweep, thermal noise, colloids whose four stresslet systems fit one launch, and no
+// lubrication.  (record_forces needs nothing here: with thermal noise the forces it copies into fext
+// are overwritten by the RFD normals in the same step, src/vel.cpp:83-85, src/drift.cpp:9-15.)
+bool joint_step_ok(const Ctx* c) {
+    if (c->arrow || c->p.lub || c->p.solver != PS_SOLVER_BLOCK || !(c->p.kT > 0)) return false;
+    if (c->p.precision == PS_PREC_MIXED && c->p.ir_maxit > 0) return false;
+    if (c->Nm <= 0 || c->Nc <= 0) return false;
+    // PS_JOINT_STEP=0 keeps the sequential order (one_step), e.g. to compare the two
+    const char* env = std::getenv("PS_JOINT_STEP");
+    if (env && std::strcmp(env, "0") == 0) return false;
+    return mss_joint_fits(c, 4);
+}
+
+// One iteration of the run() loop with the same operations as one_step, reordered so that the
+// stresslet solves of the predictor, the slip solve and both RFD probes share one launch:
+//   - stage 1 of the predictor's solve (x = [-b_lambda; 0], the stresslet right-hand side) at x;
+//   - the Lanczos recurrence at x;
+//   - on the side stream, while the main one solves the tridiagonal square roots: both probes'
+//     displacement, geometry, mob() (slots 1 and 2) and stage 1 of their solves;
+//   - stage 1 of the slip solve at x;
+//   - the four stresslet MINRES solves in one launch: its cost is the slowest one's iterations, not
+//     their sum;
+//   - stage 3 (the full apply and the velocities) of the four solves;
+//   - the integrator and the corrector, as in one_step.
+// The mob() calls run in the reference's order (x, x + eps W, x - eps W, corrector), so the carried
+// tile passes from each to the next exactly as there, and each solve is the same block elimination
+// as one_step's; only their order differs.
+ps_err one_step_joint(Ctx* c, NoisePipe* pipe, int s, double* noise_fixed) {
+    const int64_t n = c->n, nu = c->nu, ns = c->ns, n3 = 3 * (int64_t)c->Np, nc5 = 5 * (int64_t)c->Nc;
+    hipStream_t A = c->stream, B = c->side;
+    const double dt = c->p.dt, kT = c->p.kT, eps = 1.0e-3;
+    const int prec = c->p.precision;
+    // every exit leaves slot 0 current and the main stream in c->stream
+    struct Restore {
+        Ctx* c;
+        hipStream_t a;
+        ~Restore() {
+            c->stream = a;
+            mob_use_slot(c, 0);
+        }
+    } restore{c, A};
+    c->t += dt;
+    PS_TRY(vec_copy(c, c->x_n, c->x, n3));
+    // ---- predictor point x: forces, mob() into slot 0, stage 1 of the deterministic solve
+    PS_TRY(check_dist_dev(c));
+    PS_TRY(rhs_dev(c, 0));
+    mob_use_slot(c, 0);
+    PS_TRY(mob_prepare(c));
+    double* bs0 = c->bsr4;
+    double* bs1 = c->bsr4 + nc5;
+    double* bs2 = c->bsr4 + 2 * nc5;
+    double* bs3 = c->bsr4 + 3 * nc5;
+    PS_TRY(block_stage1(c, c->rhs, c->Xdet, bs0));
+    // ---- noise: the slip normals, then the RFD normals (the reference's draw order)
+    double* noise = noise_fixed ? noise_fixed : pipe->take(s);
+    PS_HIP(c, hipMemcpyAsync(c->xi, noise, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, A));
+    PS_HIP(c, hipMemcpyAsync(c->fext, noise + n, sizeof(double) * (size_t)nu, hipMemcpyHostToDevice, A));
+    if (!noise_fixed) PS_TRY(pipe->release(s));
+    PS_TRY(sqrt_recurrence_dev(c, c->xi));
+    // ---- fork: the probes on the side stream
+    PS_HIP(c, hipEventRecord(c->jev[0], A));
+    PS_HIP(c, hipStreamWaitEvent(B, c->jev[0], 0));
+    c->stream = B;
+    k_probe<<<eb(n3), TB, 0, B>>>(c->x, c->fext, eps, n3);
+    PS_LAUNCH(c);
+    PS_TRY(probe_geometry_dev(c));
+    mob_use_slot(c, 1);
+    PS_TRY(mob_prepare(c));
+    PS_TRY(rhs_drift_dev(c, c->rhs_d));
+    PS_TRY(block_stage1(c, c->rhs_d, c->Xd, bs2));
+    k_probe<<<eb(n3), TB, 0, B>>>(c->x, c->fext, -2.0 * eps, n3);
+    PS_LAUNCH(c);
+    PS_TRY(probe_geometry_dev(c));
+    mob_use_slot(c, 2);
+    PS_TRY(mob_prepare(c));
+    PS_TRY(block_stage1(c, c->rhs_d, c->Xd2, bs3));
+    PS_HIP(c, hipEventRecord(c->jev[1], B));
+    // ---- main stream: square-root weights, W, the slip right-hand side; join; slip stage 1
+    c->stream = A;
+    mob_use_slot(c, 0);
+    PS_TRY(sqrt_finish_dev(c, c->W, nullptr));
+    k_slip_rhs<<<eb(ns), TB, 0, A>>>(c->rhs_s, c->rhs, c->W, std::sqrt(2 * kT / dt), n, ns);
+    PS_LAUNCH(c);
+    PS_HIP(c, hipStreamWaitEvent(A, c->jev[1], 0));
+    PS_TRY(block_stage1(c, c->rhs_s, c->X, bs1));
+    // ---- the four stresslet solves in one launch
+    {
+        const double* Ms[4] = {c->mslot[0].Mss, c->mslot[0].Mss, c->mslot[1].Mss, c->mslot[2].Mss};
+        const double* rs[4] = {bs0, bs1, bs2, bs3};
+        double* xs[4] = {c->Xdet + nu, c->X + nu, c->Xd + nu, c->Xd2 + nu};
+        PS_TRY(mss_joint(c, 4, Ms, rs, xs));
+    }
+    // ---- stage 3: velocities of the predictor, the slip solve and both probes
+    PS_TRY(block_stage3(c, c->rhs, c->Xdet, prec, c->v_det_n));
+    PS_TRY(block_stage3(c, c->rhs_s, c->X, prec, c->up));
+    k_sub<<<eb(nu), TB, 0, A>>>(c->v_brown, c->up, c->v_det_n, nu);
+    PS_LAUNCH(c);
+    mob_use_slot(c, 1);
+    PS_TRY(block_stage3(c, c->rhs_d, c->Xd, prec, c->drift_v));
+    k_sb<<<eb(nc5), TB, 0, A>>>(c->sb, c->Xd, nu, nc5, 0);
+    PS_LAUNCH(c);
+    mob_use_slot(c, 2);
+    PS_TRY(block_stage3(c, c->rhs_d, c->Xd2, prec, c->udiff));
+    k_sub<<<eb(nu), TB, 0, A>>>(c->udiff, c->udiff, c->drift_v, nu);
+    PS_LAUNCH(c);
+    k_sb<<<eb(nc5), TB, 0, A>>>(c->sb, c->Xd2, nu, nc5, 1);
+    PS_LAUNCH(c);
+    mob_use_slot(c, 0);
+    k_step<<<eb(c->Np), TB, 0, A>>>(c->sc, c->x, c->fext, c->udiff, c->up, dt, kT);
+    PS_LAUNCH(c);
+    // ---- corrector (slot 0)
+    PS_TRY(check_dist_dev(c));
+    PS_TRY(rhs_dev(c, 0));
+    PS_TRY(mob_prepare(c));
+    PS_TRY(solve(c, c->Xdet, 0, c->v_det_c));
+    k_trapz<<<eb(nu), TB, 0, A>>>(c->up, c->v_det_n, c->v_det_c, c->v_brown, nu);
+    PS_LAUNCH(c);
+    k_step_from<<<eb(c->Np), TB, 0, A>>>(c->sc, c->x, c->x_n, c->udiff, c->up, dt, kT);
+    PS_LAUNCH(c);
+    k_quat<<<eb(c->Nc), TB, 0, A>>>(c->Nc, 3 * (int64_t)c->Np, c->q, c->up, dt);
+    PS_LAUNCH(c);
+    c->have_mob = true;
+    return PS_OK;
+}
+
+// Allocate what the joint step needs (once, at create, when joint_step_ok holds).
+ps_err joint_alloc(Ctx* c) {
+    const size_t ns = (size_t)c->ns, nc5 = 5 * (size_t)c->Nc;
+    PS_TRY(mob_slots_alloc(c));
+    PS_HIP(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    for (hipEvent_t& e : c->jev) PS_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    PS_HIP(c, hipMalloc(&c->rhs_s, sizeof(double) * ns));
+    PS_HIP(c, hipMalloc(&c->rhs_d, sizeof(double) * ns));
+    PS_HIP(c, hipMalloc(&c->Xd2, sizeof(double) * ns));
+    PS_HIP(c, hipMalloc(&c->bsr4, sizeof(double) * 4 * nc5));
+    PS_HIP(c, hipMemsetAsync(c->Xd2, 0, sizeof(double) * ns, c->stream));
+    return PS_OK;
+}
+
+void joint_free(Ctx* c) {
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    mob_slots_free(c);
+    for (void* p : {(void*)c->rhs_s, (void*)c->rhs_d, (void*)c->Xd2, (void*)c->bsr4})
+        if (p) (void)hipFree(p);
+    c->rhs_s = c->rhs_d = c->Xd2 = c->bsr4 = nullptr;
+    for (hipEvent_t& e : c->jev)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+    if (c->side) (void)hipStreamDestroy(c->side);
+    c->side = nullptr;
 }
 
 Ctx* as_ctx(void* h) { return static_cast<Ctx*>(h); }
@@ -399,6 +563,7 @@ void destroy_ctx(Ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     c->ev_pool.clear();
+    joint_free(c);
     apply_free(c);
     krylov_free(c);
     pairs_free(c);
@@ -660,6 +825,7 @@ ps_err ps_create(const ps_params* p, void** out) {
     if (pairs_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
     if (c->arrow && arrow_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
     if (c->p.lub && lub_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
+    if (joint_step_ok(c) && joint_alloc(c) != PS_OK) return bail(PS_ERR_ALLOC);
     for (double* v : {c->up, c->udiff, c->v_det_n, c->v_det_c, c->v_brown, c->fext, c->drift_v})
         (void)hipMemsetAsync(v, 0, sizeof(double) * (size_t)c->nu, c->stream);
     for (double* v : {c->X, c->Xdet, c->Xd})
