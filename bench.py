@@ -35,8 +35,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}
-# FP32 VALU peak without packed (dual-issue) FP32 instructions, which this build does not emit:
-# 256 CUs x 64 FMA lanes x 2 flop x 2.4 GHz.
+# FP32 fused multiply-adds at one wave64 instruction per CU per cycle (256 CUs x 64 lanes x 2 flop x
+# 2.4 GHz): half the published FP32 figure, which counts two VALU instructions issued per CU per cycle
+# (gfx950 issues two independent FP32 instructions in one cycle; packed FP32 is not used here).
 PEAK_FP32_VALU = 78.6e12
 # HBM bytes per launch of the fused apply kernel, from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE
 # of `bench.py --profile-form <form> --steps 2 --warmup 0 --no-cpu --calibrate-copy 256`, each scaled by
@@ -44,6 +45,34 @@ PEAK_FP32_VALU = 78.6e12
 # measured for that kernel.  profiles/k_apply_sym_hbm_traffic_n20000.md: FP32, 9,107.5 KiB fetched and
 # 19,761.0 KiB written per launch (FETCH_SIZE 0.50x, WRITE_SIZE 1.00x of the calibration copy).
 TRAFFIC_APPLY = {("c3", "fp32"): (9107.5 + 19761.0) * 1024}
+# VALU issue of the fused apply kernel from rocprofv3 --pmc SQ_INSTS_VALU SQ_ACTIVE_INST_VALU2 ... of
+# `bench.py --profile-form mixed/block --steps 1 --warmup 0 --no-cpu` (profiles/k_apply_sym_valu_issue_n20000.md):
+# instructions per launch, and the share issued two per SIMD quad-cycle (gfx950 dual issue).
+VALU_ISSUE = {("c3", "fp32"): {"instructions_per_launch": 137.19e6, "dual_issued_fraction": 0.652,
+                               "instructions_per_issue_quad_cycle": 1.484}}
+
+
+def measured_peaks():
+    """VALU issue rates measured on this GPU by the probe library (polystokes_brownian_amd/libpsprobe.so,
+    csrc/peak_probe.hip): wave64 instructions per CU per shader cycle of independent chains of one
+    instruction form, and the shader clock read in the kernel."""
+    import ctypes as C
+    path = os.path.join(ROOT, "polystokes_brownian_amd", "libpsprobe.so")
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    L.psp_valu_rate.argtypes = [C.c_int] + [C.POINTER(C.c_double)] * 3
+    names = {0: "v_fmac_f32", 1: "v_fma_f32", 2: "v_fmac_f32+v_mul_f32", 3: "v_fma_f64",
+             4: "v_mov_b32_dpp wave_rol:1", 5: "v_rsq_f32"}
+    out = {}
+    for form, name in names.items():
+        ms, ghz, ipc = C.c_double(), C.c_double(), C.c_double()
+        rc = L.psp_valu_rate(form, C.byref(ms), C.byref(ghz), C.byref(ipc))
+        if rc != 0:
+            return {"error": f"psp_valu_rate({form}) returned {rc}"}
+        out[name] = {"instr_per_cu_cycle": round(ipc.value, 3), "shader_ghz": round(ghz.value, 3)}
+    log(f"VALU issue probe: {out}")
+    return out
 
 
 def ALGO_BYTES(Nm, Nc, elt):
@@ -407,7 +436,7 @@ def batch_bench(args, configs):
                      "frac": achieved / PEAK["fp32"], "traffic": None,
                      "kernel": "k_batch_step (the whole step of every replica; VALU-bound tile generation)",
                      "flops_per_launch": flops, "avg_launch_ms": head["ms_per_step"],
-                     "frac_of_fp32_valu_peak": achieved / PEAK_FP32_VALU},
+                     "frac_of_single_issue_fp32_fma_rate": achieved / PEAK_FP32_VALU},
         "cpu_baseline": cpu,
     }
 
@@ -497,6 +526,11 @@ def main():
                     help="instead: the batched-replica workload (256 replicas); prints its own JSON line")
     ap.add_argument("--batch-replicas", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--peaks", action="store_true",
+                    help="instead: only the VALU issue-rate probe (instructions per CU per cycle, shader clock)")
+    ap.add_argument("--no-batch", action="store_true", help="omit the batched replicas from the default line")
+    ap.add_argument("--no-c5", action="store_true", help="omit the 100,000-bead workload from the default line")
+    ap.add_argument("--c5-steps", type=int, default=2, help="timed steps of the 100,000-bead workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the solve is not sharded across GPUs",
@@ -518,6 +552,9 @@ def main():
         log(f"calibration copy: {nel * 8} bytes read and written per dispatch, 2 dispatches")
         del src, dst
 
+    if args.peaks:
+        print(json.dumps({"valu_issue_probe": measured_peaks()}), flush=True)
+        return
     if args.dense:
         return dense_bench(args, configs, Context)
     if args.batch:
@@ -529,12 +566,33 @@ def main():
             print(json.dumps(line), flush=True)
         return
 
-    w = configs.workload(args.workload)
+    out = workload_line(args, args.workload, args.steps, args.warmup, cpu=not args.no_cpu, graphs=(0, 1),
+                        minres=args.minres, lub=args.lub, profile_form=args.profile_form,
+                        profile_events=args.profile_events)
+    if out is None:
+        return
+    if args.workload == "c3" and not args.no_c5:
+        # the large matrix-free workload (100,000 beads): GPU forms only, errors against the GPU FP64 form
+        out["large_matrix_free_c5"] = workload_line(args, "c5", args.c5_steps, 1, cpu=False, graphs=(0,))
+    if args.workload == "c3" and not args.no_batch:
+        out["batched_replicas_c4"] = batch_bench(args, configs)
+    if not args.no_arrow:
+        out["arrowhead_published_case"] = arrow_bench(args, configs, Context, args.arrow_steps)
+    out["valu_issue_probe"] = measured_peaks()
+    print(json.dumps(out), flush=True)
+
+
+def workload_line(args, wname, steps, warmup, cpu, graphs, minres=False, lub=False, profile_form=None,
+                  profile_events=False):
+    """The timed forms of one single-system workload and their JSON object (None with profile_form)."""
+    from polystokes_brownian_amd import configs
+    from polystokes_brownian_amd.polystokes import Context
+    w = configs.workload(wname)
     pd = configs.params_dict(w)
     t0 = time.perf_counter()
     x0 = configs.positions(w)
     Np, Nm, Nc = w["Np"], w["Nm"], w["Nc"]
-    log(f"workload {args.workload}: Np={Np} (Nc={Nc}, Nm={Nm}), configuration {time.perf_counter() - t0:.1f}s")
+    log(f"workload {wname}: Np={Np} (Nc={Nc}, Nm={Nm}), configuration {time.perf_counter() - t0:.1f}s")
 
     # mixed: every mobility apply on FP32 tiles (fixed-point separations), FP64 recurrences, stresslet
     # solves and colloid pairs, no refinement sweep (the block elimination is exact up to the apply)
@@ -542,22 +600,22 @@ def main():
         "mixed/block": dict(precision=1, solver=1, ir_maxit=0),
         "fp64/block": dict(precision=0, solver=1),
     }
-    if args.minres:
+    if minres:
         forms["fp64/minres"] = dict(precision=0, solver=0)
-    if args.lub:
+    if lub:
         # lubrication in the saddle (opt-in physics, not the reference's system): eager only, its
         # colloid saddle solves poll on the host
         forms["mixed/block+lub"] = dict(precision=1, solver=1, ir_maxit=0, lub=1)
-    if args.profile_form:
-        ctx = make_ctx(Context, pd, x0, graph=0, **forms[args.profile_form])
-        dt, prof, per = timed_run(ctx, args.steps, args.warmup, args.profile_form, profile=args.profile_events)
-        log(f"{args.profile_form}: {Np * args.steps / dt:.4g} bead-steps/s, {1e3 * dt / args.steps:.2f} ms/step, "
+    if profile_form:
+        ctx = make_ctx(Context, pd, x0, graph=0, **forms[profile_form])
+        dt, prof, per = timed_run(ctx, steps, warmup, profile_form, profile=profile_events)
+        log(f"{profile_form}: {Np * steps / dt:.4g} bead-steps/s, {1e3 * dt / steps:.2f} ms/step, "
             f"per step {per}")
         if prof:
-            log(f"{args.profile_form}: apply kernel {prof['aa_launches']} launches, average "
+            log(f"{profile_form}: apply kernel {prof['aa_launches']} launches, average "
                 f"{prof['aa_ms'] / max(prof['aa_launches'], 1) * 1e3:.2f} us (library events)")
         ctx.close()
-        return
+        return None
 
     # timed steps: each form eagerly (graph 0) and as replayed step graphs (graph 1); the faster is
     # the form's figure, both are reported
@@ -571,39 +629,40 @@ def main():
             continue
         if form.endswith("+lub"):
             ctx = make_ctx(Context, pd, x0, graph=0, **over)
-            dt, _, per = timed_run(ctx, args.steps, args.warmup, form, profile=False)
+            dt, _, per = timed_run(ctx, steps, warmup, form, profile=False)
             ctx.close()
-            results[form] = dict(value=Np * args.steps / dt, ms_per_step=1e3 * dt / args.steps, per_step=per,
+            results[form] = dict(value=Np * steps / dt, ms_per_step=1e3 * dt / steps, per_step=per,
                                  graph=0)
             results[form]["effect_vs_no_lubrication"] = lub_effect(Context, pd, x0, over)
             log(f"{form}: {results[form]['value']:.4g} bead-steps/s, {results[form]['ms_per_step']:.2f} ms/step, "
                 f"per step {per}, {results[form]['effect_vs_no_lubrication']}")
             continue
         modes = {}
-        for graph in (0, 1):
+        for graph in graphs:
             ctx = make_ctx(Context, pd, x0, graph=graph, **over)
-            dt, _, per = timed_run(ctx, args.steps, args.warmup, f"{form} graph={graph}", profile=False)
+            dt, _, per = timed_run(ctx, steps, warmup, f"{form} graph={graph}", profile=False)
             ctx.close()
-            modes[graph] = dict(value=Np * args.steps / dt, ms_per_step=1e3 * dt / args.steps, per_step=per)
+            modes[graph] = dict(value=Np * steps / dt, ms_per_step=1e3 * dt / steps, per_step=per)
         best = max(modes, key=lambda g: modes[g]["value"])
-        results[form] = dict(modes[best], graph=best, eager_ms_per_step=modes[0]["ms_per_step"],
-                             graph_ms_per_step=modes[1]["ms_per_step"])
+        results[form] = dict(modes[best], graph=best)
+        for g in modes:
+            results[form]["graph_ms_per_step" if g else "eager_ms_per_step"] = modes[g]["ms_per_step"]
         # kernel timing in a separate run (the library records events around the launches, which
         # turns graph replay off): the fused monomer apply kernel, and each whole apply
         ctx = make_ctx(Context, pd, x0, graph=0, **over)
-        _, prof, _ = timed_run(ctx, args.steps, args.warmup, f"{form} kernel timing", profile=True)
+        _, prof, _ = timed_run(ctx, steps, warmup, f"{form} kernel timing", profile=True)
         results[form]["monomer_kernel"] = ctx.kernel_choice()
         ctx.close()
         results[form]["apply_kernel_ms_avg"] = prof["aa_ms"] / max(prof["aa_launches"], 1)
         results[form]["apply_kernel_launches"] = prof["aa_launches"]
         results[form]["apply_ms_avg"] = prof["apply_ms"] / max(prof["apply_launches"], 1)
+        timing = ", ".join(f"graph={g} {m['ms_per_step']:.2f}" for g, m in modes.items())
         log(f"{form}: {results[form]['value']:.4g} bead-steps/s, {results[form]['ms_per_step']:.2f} ms/step "
-            f"(eager {modes[0]['ms_per_step']:.2f}, graph {modes[1]['ms_per_step']:.2f}), per step {per}, "
-            f"apply kernel {results[form]['apply_kernel_ms_avg'] * 1e3:.1f} us")
+            f"({timing}), per step {per}, apply kernel {results[form]['apply_kernel_ms_avg'] * 1e3:.1f} us")
 
     # accuracy against the FP64 oracle, and the CPU baseline (the oracle's step on the host cores)
     errs, cpu, cpu1 = {}, None, None
-    if not args.no_cpu:
+    if cpu:
         xi = np.random.default_rng(7).normal(size=3 * Nm + 11 * Nc)
         ref = oracle_reference(pd, x0, w, xi)
         for form, over in forms.items():
@@ -634,7 +693,7 @@ def main():
                           f"is under 2% of it on the host)"}
         log(f"cpu baseline {cpu}; one core {cpu1}")
     errs_gpu = None
-    if args.no_cpu:
+    if not cpu:
         errs_gpu = mixed_vs_gpu_fp64(Context, pd, x0, forms)
         log(f"mixed/block vs GPU FP64: {errs_gpu}")
 
@@ -651,7 +710,7 @@ def main():
     flops64 = 2.0 * (n_mm * n_mm + (n_mc if sym64 else 0))
     ach = flops / (head["apply_kernel_ms_avg"] * 1e-3)
     ach64 = flops64 / (f64["apply_kernel_ms_avg"] * 1e-3)
-    traffic = TRAFFIC_APPLY.get((args.workload, "fp32")) if sym32 else None
+    traffic = TRAFFIC_APPLY.get((wname, "fp32")) if sym32 else None
     kname = ("k_apply_sym<float, true> (monomer-monomer tile pairs on FP32 tiles with fixed-point separations, "
              "the monomer-colloid blocks in the same launch); VALU-bound tile generation" if sym32 else
              "k_aa<float> (ordered monomer-monomer tiles on FP32); VALU-bound tile generation")
@@ -659,14 +718,15 @@ def main():
             "frac": ach / PEAK["fp32"], "traffic": traffic, "kernel": kname,
             "kernel_choice": head["monomer_kernel"],
             "flops_per_launch": flops, "avg_launch_ms": head["apply_kernel_ms_avg"],
-            "frac_of_fp32_valu_peak_unpacked": ach / PEAK_FP32_VALU,
+            "frac_of_single_issue_fp32_fma_rate": ach / PEAK_FP32_VALU,
+            "valu_issue": VALU_ISSUE.get((wname, "fp32")) if sym32 else None,
             "traffic_algorithmic_bytes": ALGO_BYTES(Nm, Nc, 4)}
     out = {
         "metric": "bead-steps/s at N=20k: mixed-prec vs FP64 GPU at 1e-6 rel err; % roofline",
-        "value": head["value"], "unit": "bead-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "value": head["value"], "unit": "bead-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32+fp64", "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {Np} beads ({Nc} probes + {w['Npoly']} chains x 20), "
+        "config": {"workload": f"{wname}: {Np} beads ({Nc} probes + {w['Npoly']} chains x 20), "
                                f"box {w['L']:.1f}, full mm-HI, mixed precision (every mobility apply on FP32 "
                                f"tiles, FP64 recurrences and stresslet solves), exact block-eliminated saddle "
                                f"solves, Lanczos m<={pd['sqrt_kmax']} rtol {pd['sqrt_rtol']}",
@@ -679,7 +739,7 @@ def main():
         "roofline": roof,
         "roofline_fp64": {"bound": "valu", "achieved": ach64 / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
                           "frac": ach64 / PEAK["fp64"], "avg_launch_ms": f64["apply_kernel_ms_avg"],
-                          "traffic": TRAFFIC_APPLY.get((args.workload, "fp64"))},
+                          "traffic": TRAFFIC_APPLY.get((wname, "fp64"))},
         "cpu_baseline": cpu,
         "cpu_baseline_one_core": cpu1,
     }
@@ -688,9 +748,9 @@ def main():
             results["fp64/minres"],
             note="Jacobi MINRES on the saddle system (rtol 1e-6, cap 10000 iterations per solve) does not "
                  "converge at this size; one step")
-    if not args.no_arrow:
-        out["arrowhead_published_case"] = arrow_bench(args, configs, Context, args.arrow_steps)
-    print(json.dumps(out), flush=True)
+    return out
+
+
 
 
 if __name__ == "__main__":
