@@ -316,7 +316,8 @@ def arrow_bench(args, configs, Context, steps):
 
 def batch_bench(args, configs):
     """Batched replicas (256 unbounded systems, N_r log-uniform in [100, 3000], 10% probes, 20-bead
-    chains, C3 parameters, seeds 1000 + r): one persistent kernel per step, replicas far above the
+    chains, C3 parameters, seeds 1000 + r): one persistent kernel per step, replayed as a captured HIP
+    graph (noise copy, team reset, step kernel), replicas far above the
     mean work on teams of workgroups, the packing re-tuned from measured per-replica times.  Forms:
     mixed (FP32 monomer tiles in the 32-iteration Lanczos square root, FP64 solves and recurrences;
     headline) and FP64.  value = aggregate bead-steps/s = sum_r N_r x steps / (sum of the timed
@@ -369,12 +370,14 @@ def batch_bench(args, configs):
             its = max(its, int(st[:, 0].max()))
             unconv += int(st[:, 3].sum())
         us, team = b.times()
+        ginfo = b.graph_info()
         b.close()
         res[form] = dict(value=float(N.sum()) * steps / dt, ms_per_step=1e3 * dt / steps,
                          lanczos_iterations_max=int(st[:, 1].max()), minres_iterations_max=its,
                          unconverged_solves=unconv, teams=int((team > 1).sum()),
                          team_workgroups=int(team[team > 1].sum()), team_max=int(team.max()),
-                         replica_ms_max=float(us.max() / 1e3), replica_ms_mean=float(us.mean() / 1e3))
+                         replica_ms_max=float(us.max() / 1e3), replica_ms_mean=float(us.mean() / 1e3),
+                         step_graphs=ginfo)
         log(f"batch {form}: {res[form]['value']:.4g} bead-steps/s, {res[form]['ms_per_step']:.2f} ms/step, "
             f"{res[form]['teams']} teams, replica ms max {res[form]['replica_ms_max']:.1f}")
     errs = np.array([rel(a, c) for a, c in zip(ups["mixed"], ups["fp64"])])

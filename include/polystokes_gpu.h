@@ -356,6 +356,12 @@ ps_err ps_batch_phase_times(void* batch, double* us5);
  * 16 colloid rows); with autotune
  * the packing is redone before every step from the measured per-replica times. */
 ps_err ps_batch_config(void* batch, int team_max, int min_blocks, int autotune);
+/* With ps_params.graph = 1 (the default) each step of ps_batch_run is a HIP graph per noise slot --
+ * the noise copy, the team reset and the cooperative step kernel -- captured once and replayed (the
+ * packing lives in device memory, so a re-tuned schedule keeps the graph; a new trap stiffness
+ * recaptures it).  Out (each may be null): graph launches so far, captures, and 1 when a capture
+ * failed and the steps run eagerly.  No reference counterpart. */
+ps_err ps_batch_graph_info(void* batch, int64_t* replays, int64_t* captures, int* failed);
 /* The step schedule's planner, without a GPU: lone-workgroup costs and team caps (>= 1) of R
  * replicas on W co-resident workgroups.  Out (each may be null): the team size of each replica, the
  * workgroup of its first rank, each workgroup's planned load (W) and the makespan target T.  No

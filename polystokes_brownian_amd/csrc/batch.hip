@@ -1317,6 +1317,47 @@ ps_err batch_launch_step(hipStream_t st, const BConst& B, const BRep* reps, cons
     k_batch_step<<<nwg, BW, 0, st>>>(a);
     return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_HIP;
 }
+namespace {
+// Clears the team fork-join blocks before a step (a kernel rather than a memset node, so that a captured
+// step replays it exactly).
+__global__ void k_ctl_reset(BTeamCtl* __restrict__ ctl, int n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int64_t W = sizeof(BTeamCtl) / sizeof(int);
+    if (i < n * W) reinterpret_cast<int*>(ctl)[i] = 0;
+}
+ps_err batch_launch_reset(hipStream_t st, BTeamCtl* ctl, int n) {
+    const int64_t words = (int64_t)n * (int64_t)(sizeof(BTeamCtl) / sizeof(int));
+    k_ctl_reset<<<(unsigned)((words + 255) / 256), 256, 0, st>>>(ctl, n);
+    return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_HIP;
+}
+// Marks the step kernel's node of a captured graph cooperative (co-resident workgroups, which the team
+// waits rely on) where the capture did not already.
+hipError_t batch_mark_cooperative(hipGraph_t g) {
+    size_t nn = 0;
+    hipError_t e = hipGraphGetNodes(g, nullptr, &nn);
+    if (e != hipSuccess) return e;
+    std::vector<hipGraphNode_t> nodes(nn);
+    if ((e = hipGraphGetNodes(g, nodes.data(), &nn)) != hipSuccess) return e;
+    int found = 0;
+    for (hipGraphNode_t node : nodes) {
+        hipGraphNodeType t;
+        if ((e = hipGraphNodeGetType(node, &t)) != hipSuccess) return e;
+        if (t != hipGraphNodeTypeKernel) continue;
+        hipKernelNodeParams kp{};
+        if ((e = hipGraphKernelNodeGetParams(node, &kp)) != hipSuccess) return e;
+        if (kp.func != reinterpret_cast<void*>(k_batch_step)) continue;
+        found++;
+        hipKernelNodeAttrValue v{};
+        if ((e = hipGraphKernelNodeGetAttribute(node, hipKernelNodeAttributeCooperative, &v)) != hipSuccess) return e;
+        if (!v.cooperative) {
+            v.cooperative = 1;
+            if ((e = hipGraphKernelNodeSetAttribute(node, hipKernelNodeAttributeCooperative, &v)) != hipSuccess)
+                return e;
+        }
+    }
+    return found == 1 ? hipSuccess : hipErrorInvalidValue;
+}
+}  // namespace
 int batch_step_occupancy() {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_step, BW, 0) != hipSuccess) return 0;
@@ -1376,6 +1417,12 @@ struct Batch {
     double* noise_h[2] = {nullptr, nullptr};
     int64_t noise_n = 0;
     hipEvent_t copied[2] = {nullptr, nullptr};
+    // the step as a HIP graph per noise slot (noise copy, team reset, the step kernel), captured once
+    // and replayed; recaptured when the trap stiffness the kernel takes changes
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    double gktrap[2] = {0.0, 0.0};
+    bool graph_failed = false;
+    int64_t graph_replays = 0, graph_captures = 0;
     std::vector<std::unique_ptr<NormalStream>> rng;
     hipStream_t stream = nullptr;
     double t = 0.0, ktrap_cur = 0.0;
@@ -1421,6 +1468,8 @@ void batch_free(Batch* b) {
         if (h) (void)hipHostFree(h);
     for (hipEvent_t e : b->copied)
         if (e) (void)hipEventDestroy(e);
+    for (hipGraphExec_t g : b->gexec)
+        if (g) (void)hipGraphExecDestroy(g);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -1515,12 +1564,15 @@ void build_schedule(Batch* b) {
     b->team_of = g;
     b->nteams = 0;
     for (int r : ord) b->nteams += g[r] > 1;
-    b->nwg = (int)lists.size();
+    b->nwg = std::min((int)lists.size(), W);
+    // every one of the W launched workgroups has an entry (the unused ones an empty solo range) and the
+    // team and solo lists sit at fixed offsets, so the launch's arguments do not depend on the packing
+    // (a captured step stays valid when the schedule is re-tuned)
     std::vector<int>& S = b->sched;
-    S.assign(5 * (size_t)b->nwg + R, 0);
-    b->team_off = 5 * b->nwg;
-    b->solo_off = b->team_off + b->nteams;
-    for (int i = 0; i < b->nwg; i++) S[5 * i] = -1;
+    S.assign(5 * (size_t)W + 2 * (size_t)R, 0);
+    b->team_off = 5 * W;
+    b->solo_off = b->team_off + R;
+    for (int i = 0; i < W; i++) S[5 * i] = -1;
     int w = 0, k = 0;
     for (int r : ord) {
         if (g[r] <= 1) continue;
@@ -1533,9 +1585,10 @@ void build_schedule(Batch* b) {
         k++;
     }
     int pos = 0;
-    for (int i = 0; i < b->nwg; i++) {
+    for (int i = 0; i < W; i++) {
         S[5 * i + 3] = pos;
-        for (int r : lists[i]) S[b->solo_off + pos++] = r;
+        if (i < b->nwg)
+            for (int r : lists[i]) S[b->solo_off + pos++] = r;
         S[5 * i + 4] = pos;
     }
     S.resize(b->solo_off + pos);
@@ -1816,6 +1869,41 @@ ps_err batch_schedule_step(Batch* b, bool measured) {
 }
 }  // namespace
 
+// Captures step graph k (the noise copy from pinned slot k, the team reset and the step kernel, with
+// the launch arguments of this step: none of them depends on the packing, which lives in d_sched).
+// A capture that fails leaves graphs off for this batch; its steps then launch eagerly.
+static void batch_capture_step(Batch* b, int k, const BSchedDev& sd) {
+    if (b->gexec[k]) {
+        (void)hipGraphExecDestroy(b->gexec[k]);
+        b->gexec[k] = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        bool ok = true;
+        if (b->p.kT > 0)
+            ok = hipMemcpyAsync(b->noise_d, b->noise_h[k], sizeof(double) * (size_t)b->noise_n, hipMemcpyHostToDevice,
+                                b->stream) == hipSuccess;
+        if (ok) ok = batch_launch_reset(b->stream, b->d_ctl, b->nwg_max) == PS_OK;
+        if (ok)
+            ok = batch_launch_step(b->stream, b->B, b->d_reps, sd, b->nwg_max, b->coop, b->pool, b->noise_d,
+                                   b->ktrap_cur, b->d_stats) == PS_OK;
+        e = hipStreamEndCapture(b->stream, &g);
+        if (e == hipSuccess && !ok) e = hipErrorUnknown;
+    }
+    if (e == hipSuccess && b->coop) e = batch_mark_cooperative(g);
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->gexec[k], g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        b->gexec[k] = nullptr;
+        b->graph_failed = true;
+        return;
+    }
+    b->gktrap[k] = b->ktrap_cur;
+    b->graph_captures++;
+}
+
 // nsteps of every replica: one persistent kernel per step; a host thread draws step k + 1's noise
 // while the GPU runs step k (two pinned slots; a slot is refilled only after its copy completed).
 ps_err ps_batch_run(void* h, int nsteps) {
@@ -1854,44 +1942,62 @@ ps_err ps_batch_run(void* h, int nsteps) {
         });
     }
     ps_err rc = PS_OK;
+    const bool use_graph = b->p.graph != 0;
     for (int s = 0; s < nsteps && rc == PS_OK; s++) {
-        const int k = s & 1;
+        const int k = noisy ? (s & 1) : 0;
         b->t += b->p.dt;
         // trapping_forces: weaken_trap replaces ktrap permanently after tstart + trun
         if (b->B.trap_on && b->t > b->p.tstart + b->p.trun && b->p.weaken_trap >= 0) b->ktrap_cur = b->p.weaken_trap;
         if (noisy) {
-            {
-                std::unique_lock<std::mutex> lk(mu);
-                cv.wait(lk, [&] { return filled > s; });
-            }
-            hipError_t e = hipMemcpyAsync(b->noise_d, b->noise_h[k], sizeof(double) * (size_t)b->noise_n,
-                                          hipMemcpyHostToDevice, b->stream);
-            if (e == hipSuccess) e = hipEventRecord(b->copied[k], b->stream);
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return filled > s; });
+        }
+        rc = batch_schedule_step(b, s > 0 || b->steps_run > 0);
+        const BSchedDev sd{b->d_sched, b->team_off, b->solo_off, b->d_ctl, b->d_err, b->d_times, b->d_phases};
+        if (rc == PS_OK && use_graph && !b->graph_failed && (!b->gexec[k] || b->gktrap[k] != b->ktrap_cur))
+            batch_capture_step(b, k, sd);
+        if (rc == PS_OK && use_graph && !b->graph_failed && b->gexec[k]) {
+            const hipError_t e = hipGraphLaunch(b->gexec[k], b->stream);
             if (e != hipSuccess) {
-                b->set_error(std::string("noise copy: ") + hipGetErrorString(e));
+                b->set_error(std::string("batched step graph: ") + hipGetErrorString(e));
                 rc = PS_ERR_HIP;
+            } else {
+                b->graph_replays++;
+            }
+        } else if (rc == PS_OK) {
+            if (noisy) {
+                const hipError_t e = hipMemcpyAsync(b->noise_d, b->noise_h[k], sizeof(double) * (size_t)b->noise_n,
+                                                    hipMemcpyHostToDevice, b->stream);
+                if (e != hipSuccess) {
+                    b->set_error(std::string("noise copy: ") + hipGetErrorString(e));
+                    rc = PS_ERR_HIP;
+                }
+            }
+            if (rc == PS_OK && b->nteams > 0) {
+                rc = batch_launch_reset(b->stream, b->d_ctl, b->nwg_max);
+                if (rc != PS_OK) b->set_error("team reset: kernel launch failed");
+            }
+            if (rc == PS_OK) {
+                rc = batch_launch_step(b->stream, b->B, b->d_reps, sd, b->nwg_max, b->nteams > 0, b->pool,
+                                       b->noise_d, b->ktrap_cur, b->d_stats);
+                if (rc != PS_OK) b->set_error("batched step: kernel launch failed");
+            }
+        }
+        if (noisy) {
+            // the slot's copy is part of this step's work; the producer refills it only after it
+            if (rc == PS_OK) {
+                const hipError_t e = hipEventRecord(b->copied[k], b->stream);
+                if (e != hipSuccess) {
+                    b->set_error(std::string("noise event: ") + hipGetErrorString(e));
+                    rc = PS_ERR_HIP;
+                }
             }
             {
                 std::lock_guard<std::mutex> lk(mu);
                 consumed = s + 1;
             }
             cv.notify_all();
-            if (rc != PS_OK) break;
         }
-        rc = batch_schedule_step(b, s > 0 || b->steps_run > 0);
-        if (rc != PS_OK) break;
-        if (b->nteams > 0) {
-            hipError_t e = hipMemsetAsync(b->d_ctl, 0, sizeof(BTeamCtl) * (size_t)b->nteams, b->stream);
-            if (e != hipSuccess) {
-                b->set_error(std::string("team reset: ") + hipGetErrorString(e));
-                rc = PS_ERR_HIP;
-                break;
-            }
-        }
-        const BSchedDev sd{b->d_sched, b->team_off, b->solo_off, b->d_ctl, b->d_err, b->d_times, b->d_phases};
-        rc = batch_launch_step(b->stream, b->B, b->d_reps, sd, b->nwg, b->nteams > 0, b->pool, b->noise_d,
-                               b->ktrap_cur, b->d_stats);
-        if (rc != PS_OK) b->set_error("batched step: kernel launch failed");
         b->steps_run++;
     }
     if (noisy) {
@@ -1966,6 +2072,15 @@ ps_err ps_batch_config(void* h, int team_max, int min_blocks, int autotune) {
     b->min_blocks = min_blocks;
     b->autotune = autotune != 0;
     b->resched = true;   // re-pack before the next step
+    return PS_OK;
+}
+
+ps_err ps_batch_graph_info(void* h, int64_t* replays, int64_t* captures, int* failed) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b) return PS_ERR_ARG;
+    if (replays) *replays = b->graph_replays;
+    if (captures) *captures = b->graph_captures;
+    if (failed) *failed = b->graph_failed ? 1 : 0;
     return PS_OK;
 }
 

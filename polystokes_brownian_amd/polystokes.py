@@ -91,6 +91,7 @@ def lib():
         L.ps_batch_op.argtypes = [vp, ip, dp, dp, C.POINTER(ip)]
         L.ps_batch_times.argtypes = [vp, dp, C.POINTER(ip)]
         L.ps_batch_config.argtypes = [vp, ip, ip, ip]
+        L.ps_batch_graph_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(ip)]
         L.ps_batch_phase_times.argtypes = [vp, dp]
         L.ps_batch_workgroups.argtypes = [vp, C.POINTER(ip), C.POINTER(ip)]
         L.ps_kernel_choice.argtypes = [vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(C.c_float),
@@ -562,6 +563,12 @@ class Batch:
     def config(self, team_max=16, min_blocks=2, autotune=True):
         """ps_batch_config: the step schedule's team limits and autotuning."""
         self._check(lib().ps_batch_config(self.h, int(team_max), int(min_blocks), int(bool(autotune))))
+
+    def graph_info(self):
+        """ps_batch_graph_info: {"replays", "captures", "failed"} of the step graphs."""
+        rep, cap, fail = C.c_int64(), C.c_int64(), C.c_int()
+        self._check(lib().ps_batch_graph_info(self.h, C.byref(rep), C.byref(cap), C.byref(fail)))
+        return {"replays": rep.value, "captures": cap.value, "failed": bool(fail.value)}
 
     def op(self, op, inputs=None):
         """ps_batch_op: a list of per-replica input vectors in, per-replica outputs (and the

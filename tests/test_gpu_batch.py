@@ -139,6 +139,37 @@ def test_teams_are_bit_identical_to_lone_workgroups_and_match_the_oracle():
     bs.close()
 
 
+def test_step_graphs_replay_the_eager_steps():
+    """Each step captured as a HIP graph per noise slot (the noise copy, the team reset, the
+    cooperative step kernel) and replayed gives the same bits as launching them, over three steps
+    in two runs (the second run starts from the prefetched noise), with teams and autotuning."""
+    params, reps = make(TEAM_SIZES)
+    bg = Batch(dict(params, graph=1), reps)
+    be = Batch(dict(params, graph=0), reps)
+    for b in (bg, be):
+        b.config(team_max=4, min_blocks=1, autotune=True)
+        b.run(1)
+        b.run(2)
+    info = bg.graph_info()
+    print("graph info", info, "eager", be.graph_info(), flush=True)
+    assert not info["failed"], info
+    assert info["replays"] == 3 and info["captures"] == 2, info
+    assert be.graph_info()["replays"] == 0
+    _, team = bg.times()
+    assert team.max() > 1, team
+    for r in range(len(reps)):
+        xg, qg, tg = bg.state(r)
+        xe, qe, te = be.state(r)
+        assert np.array_equal(xg, xe), r
+        assert np.array_equal(qg, qe), r
+        assert tg == te
+        vg, ve = bg.vectors(r), be.vectors(r)
+        for k in vg:
+            assert np.array_equal(vg[k], ve[k]), (r, k)
+    bg.close()
+    be.close()
+
+
 def test_chain_hi_replicas_match_the_oracle():
     """chain_HI (no hydrodynamics between beads of the same chain): the monomer blocks run one way
     with the same-chain skip; the apply and two steps of every replica against the oracle."""
