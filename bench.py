@@ -569,14 +569,14 @@ def main():
             print(json.dumps(line), flush=True)
         return
 
-    out = workload_line(args, args.workload, args.steps, args.warmup, cpu=not args.no_cpu, graphs=(0, 1),
+    out = workload_line(args, args.workload, args.steps, args.warmup, with_cpu=not args.no_cpu, graphs=(0, 1),
                         minres=args.minres, lub=args.lub, profile_form=args.profile_form,
                         profile_events=args.profile_events)
     if out is None:
         return
     if args.workload == "c3" and not args.no_c5:
         # the large matrix-free workload (100,000 beads): GPU forms only, errors against the GPU FP64 form
-        out["large_matrix_free_c5"] = workload_line(args, "c5", args.c5_steps, 1, cpu=False, graphs=(0,))
+        out["large_matrix_free_c5"] = workload_line(args, "c5", args.c5_steps, 1, with_cpu=False, graphs=(0,))
     if args.workload == "c3" and not args.no_batch:
         out["batched_replicas_c4"] = batch_bench(args, configs)
     if not args.no_arrow:
@@ -585,7 +585,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def workload_line(args, wname, steps, warmup, cpu, graphs, minres=False, lub=False, profile_form=None,
+def workload_line(args, wname, steps, warmup, with_cpu, graphs, minres=False, lub=False, profile_form=None,
                   profile_events=False):
     """The timed forms of one single-system workload and their JSON object (None with profile_form)."""
     from polystokes_brownian_amd import configs
@@ -665,7 +665,7 @@ def workload_line(args, wname, steps, warmup, cpu, graphs, minres=False, lub=Fal
 
     # accuracy against the FP64 oracle, and the CPU baseline (the oracle's step on the host cores)
     errs, cpu, cpu1 = {}, None, None
-    if cpu:
+    if with_cpu:
         xi = np.random.default_rng(7).normal(size=3 * Nm + 11 * Nc)
         ref = oracle_reference(pd, x0, w, xi)
         for form, over in forms.items():
@@ -696,7 +696,7 @@ def workload_line(args, wname, steps, warmup, cpu, graphs, minres=False, lub=Fal
                           f"is under 2% of it on the host)"}
         log(f"cpu baseline {cpu}; one core {cpu1}")
     errs_gpu = None
-    if not cpu:
+    if not with_cpu:
         errs_gpu = mixed_vs_gpu_fp64(Context, pd, x0, forms)
         log(f"mixed/block vs GPU FP64: {errs_gpu}")
 

@@ -277,73 +277,6 @@ __device__ void b_stale(const BConst& B, const BRep& R, const double* xm, double
         for (int k = 0; k < 9; k++) st[9 + k] = cur[k];
     }
 }
-// Colloid-colloid coupling in contracted form: the FTS tile of mobility(..., AB=false, AA=false)
-// (src/mob.cpp:96-356) applied to one source's [F T S] without forming the 11 x 11 tile, for
-// e = (x_lo - x_hi) / r of the pair's lower and higher colloid index.  With S the symmetric
-// traceless matrix of the stresslet (S_22 = -S_0 - S_4), a = S e, s = e.S e and w = T x e, the
-// role expressions of src/mob.cpp:1042-1143 reduce to (sg = +1 when the target is the lower index,
-// -1 when it is the higher; flat(X) = [X00 - X22, 2X01, 2X02, 2X12, X11 - X22]):
-//   U += (x12a - y12a)(e.F) e + y12a F - y12b w + sg [(-x12g - 2 y12g) s e + 2 y12g a]
-//   W += y12b (F x e) + (x12c + y12c)(e.T) e + y12c (1 - 2 e_i^2) T_i + 2 y12h (e x a)
-//   Sr += -sg flat(p e e^T + y12g (e F^T + F e^T)) + flat(y12h (e w^T + w e^T))
-//         + flat(A e e^T + B (e a^T + a e^T) + z12m S),
-// p = -(x12g + 2 y12g)(e.F), A = s (3/2 x12m - 2 y12m + 1/2 z12m), B = y12m - z12m (the '+' of
-// mob_c is the (x12c + y12c) off-diagonal term).
-__device__ __forceinline__ void cc_mS(const double e[3], double ri3, double ri5, const double S[5], double out[5]) {
-    const double x12m = -C9D2 * ri3 + C54D5 * ri5, y12m = C9D4 * ri3 - C36D5 * ri5, z12m = C9D5 * ri5;
-    const double S22 = -S[0] - S[4];
-    const double a0 = S[0] * e[0] + S[1] * e[1] + S[2] * e[2];
-    const double a1 = S[1] * e[0] + S[4] * e[1] + S[3] * e[2];
-    const double a2 = S[2] * e[0] + S[3] * e[1] + S22 * e[2];
-    const double s = e[0] * a0 + e[1] * a1 + e[2] * a2;
-    const double A = s * (C3D2 * x12m - 2.0 * y12m + C1D2 * z12m), Bc = y12m - z12m;
-    out[0] = A * (e[0] * e[0] - e[2] * e[2]) + 2.0 * Bc * (e[0] * a0 - e[2] * a2) + z12m * (S[0] - S22);
-    out[1] = 2.0 * (A * e[0] * e[1] + Bc * (e[0] * a1 + e[1] * a0) + z12m * S[1]);
-    out[2] = 2.0 * (A * e[0] * e[2] + Bc * (e[0] * a2 + e[2] * a0) + z12m * S[2]);
-    out[3] = 2.0 * (A * e[1] * e[2] + Bc * (e[1] * a2 + e[2] * a1) + z12m * S[3]);
-    out[4] = A * (e[1] * e[1] - e[2] * e[2]) + 2.0 * Bc * (e[1] * a1 - e[2] * a2) + z12m * (S[4] - S22);
-}
-template <bool SROWS>
-__device__ __forceinline__ void cc_couple(const double e[3], double ri, double sg, const double F[3], const double Tq[3],
-                                          const double S[5], double U[3], double W[3], double Sr[5]) {
-    const double ri2 = ri * ri, ri3 = ri2 * ri, ri4 = ri2 * ri2, ri5 = ri4 * ri;
-    const double y12g = C6D5 * ri4, x12g = C9D4 * ri2 - C18D5 * ri4, y12h = -C9D8 * ri3, y12b = -C3D4 * ri2;
-    const double S22 = -S[0] - S[4];
-    const double a0 = S[0] * e[0] + S[1] * e[1] + S[2] * e[2];
-    const double a1 = S[1] * e[0] + S[4] * e[1] + S[3] * e[2];
-    const double a2 = S[2] * e[0] + S[3] * e[1] + S22 * e[2];
-    const double w0 = Tq[1] * e[2] - Tq[2] * e[1], w1 = Tq[2] * e[0] - Tq[0] * e[2], w2 = Tq[0] * e[1] - Tq[1] * e[0];
-    const double eF = e[0] * F[0] + e[1] * F[1] + e[2] * F[2];
-    if (!SROWS) {
-        const double x12a = C3D2 * ri - ri3, y12a = C3D4 * ri + C1D2 * ri3;
-        const double x12c = C3D4 * ri3, y12c = -C3D8 * ri3;
-        const double s = e[0] * a0 + e[1] * a1 + e[2] * a2;
-        const double gs = sg * (-x12g * s - 2.0 * y12g * s), g2 = sg * 2.0 * y12g;
-        const double ca = (x12a - y12a) * eF;
-        U[0] += ca * e[0] + y12a * F[0] - y12b * w0 + (gs * e[0] + g2 * a0);
-        U[1] += ca * e[1] + y12a * F[1] - y12b * w1 + (gs * e[1] + g2 * a1);
-        U[2] += ca * e[2] + y12a * F[2] - y12b * w2 + (gs * e[2] + g2 * a2);
-        const double eT = e[0] * Tq[0] + e[1] * Tq[1] + e[2] * Tq[2];
-        const double cc = (x12c + y12c) * eT, h2 = 2.0 * y12h;
-        W[0] += y12b * (F[1] * e[2] - F[2] * e[1]) + cc * e[0] + y12c * (1.0 - 2.0 * e[0] * e[0]) * Tq[0] +
-                h2 * (e[1] * a2 - e[2] * a1);
-        W[1] += y12b * (F[2] * e[0] - F[0] * e[2]) + cc * e[1] + y12c * (1.0 - 2.0 * e[1] * e[1]) * Tq[1] +
-                h2 * (e[2] * a0 - e[0] * a2);
-        W[2] += y12b * (F[0] * e[1] - F[1] * e[0]) + cc * e[2] + y12c * (1.0 - 2.0 * e[2] * e[2]) * Tq[2] +
-                h2 * (e[0] * a1 - e[1] * a0);
-    }
-    const double p = -(x12g + 2.0 * y12g) * eF, ms = -sg, h = 2.0 * y12h;
-    double m[5];
-    cc_mS(e, ri3, ri5, S, m);
-    Sr[0] += ms * (p * (e[0] * e[0] - e[2] * e[2]) + 2.0 * y12g * (e[0] * F[0] - e[2] * F[2])) +
-             h * (e[0] * w0 - e[2] * w2) + m[0];
-    Sr[1] += ms * (2.0 * (p * e[0] * e[1] + y12g * (e[0] * F[1] + e[1] * F[0]))) + h * (e[0] * w1 + e[1] * w0) + m[1];
-    Sr[2] += ms * (2.0 * (p * e[0] * e[2] + y12g * (e[0] * F[2] + e[2] * F[0]))) + h * (e[0] * w2 + e[2] * w0) + m[2];
-    Sr[3] += ms * (2.0 * (p * e[1] * e[2] + y12g * (e[1] * F[2] + e[2] * F[1]))) + h * (e[1] * w2 + e[2] * w1) + m[3];
-    Sr[4] += ms * (p * (e[1] * e[1] - e[2] * e[2]) + 2.0 * y12g * (e[1] * F[1] - e[2] * F[2])) +
-             h * (e[1] * w1 - e[2] * w2) + m[4];
-}
-
 // ---------------------------------------------------------------------------------- apply
 // Fused multiply-add, written out where the tiles want it (this file is built without contraction,
 // which the pair geometry and the integrator need).

@@ -595,13 +595,56 @@ __global__ __launch_bounds__(PS_BLOCK) void k_ab_coll(SysConst s, const T* __res
 
 // The monomer-monomer tile pairs and, behind them in the same grid, the monomer rows against the
 // colloid sources and the colloid rows against the monomer sources: the small colloid blocks fill
-// the tail of the tile-pair blocks instead of waiting in launches of their own.  Blocks [0, nsym)
-// are tile pairs (the ghost-bead pairs first), then nmono = gx * nchm monomer-row blocks, then
-// Nc * nchc colloid-row blocks.
+// the tail of the tile-pair blocks instead of waiting in launches of their own.  Blocks [0, nbb)
+// are the colloid-pair rows (one per colloid), then nsym tile pairs (the ghost-bead pairs first),
+// then nmono = gx * nchm monomer-row blocks, then Nc * nchc colloid-row blocks.
 struct ApplyGrid {
-    int nsym, npad, gx, nmono, ncoll;
+    int nsym, npad, gx, nmono, ncoll, nbb;
     int per_cchunk, per_chunk;
 };
+
+// Colloid rows from colloid sources for one target colloid c (the k_bb result), with the pair tile
+// applied in contracted form (cc_couple) instead of forming the 11 x 11 tile.  The translation,
+// rotation and stresslet rows are summed in three passes over the partners (the pair geometry is
+// recomputed in each), which keeps the registers below those of the tile pairs sharing the launch.
+// fc is the colloid segment of the force vector (F 3Nc, T 3Nc, S 5Nc); the FP64 positions give the
+// pair geometry.
+template <int PART>
+__device__ __forceinline__ void bb_pass(int c, const SysConst& s, const double* __restrict__ xm,
+                                        const double* __restrict__ fc, double (&acc)[11]) {
+    const int Nm = s.Nm, Nc = s.Nc;
+    const int64_t nc3 = s.nc3;
+    for (int o = threadIdx.x; o < Nc; o += PS_BLOCK) {
+        if (o == c) continue;
+        // e = (x_lo - x_hi) / r of the pair's lower and higher index; sg = +1 when c is the lower
+        const bool lo = c < o;
+        double e[3], dr, dr_inv;
+        pair_geom_dev(s, xm, Nm + (lo ? c : o), Nm + (lo ? o : c), e, dr, dr_inv);
+        const double F[3] = {fc[3 * o], fc[3 * o + 1], fc[3 * o + 2]};
+        const double Tq[3] = {fc[nc3 + 3 * o], fc[nc3 + 3 * o + 1], fc[nc3 + 3 * o + 2]};
+        const double S[5] = {fc[2 * nc3 + 5 * o], fc[2 * nc3 + 5 * o + 1], fc[2 * nc3 + 5 * o + 2],
+                             fc[2 * nc3 + 5 * o + 3], fc[2 * nc3 + 5 * o + 4]};
+        double U[3] = {0.0, 0.0, 0.0}, W[3] = {0.0, 0.0, 0.0}, Sr[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        cc_couple<PART == 2>(e, dr_inv, lo ? 1.0 : -1.0, F, Tq, S, U, W, Sr);
+        if (PART == 0)
+            for (int k = 0; k < 3; k++) acc[k] += U[k];
+        if (PART == 1)
+            for (int k = 0; k < 3; k++) acc[3 + k] += W[k];
+        if (PART == 2)
+            for (int k = 0; k < 5; k++) acc[6 + k] += Sr[k];
+    }
+}
+__device__ __forceinline__ void bb_block(int c, const SysConst& s, const double* __restrict__ xm,
+                                         const double* __restrict__ fc, double* __restrict__ bb, double (*sh)[11]) {
+    double acc[11];
+    for (int k = 0; k < 11; k++) acc[k] = 0.0;
+    bb_pass<0>(c, s, xm, fc, acc);
+    bb_pass<1>(c, s, xm, fc, acc);
+    bb_pass<2>(c, s, xm, fc, acc);
+    block_sum<double, 11>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 11; k++) bb[(int64_t)c * 11 + k] = acc[k];
+}
 
 template <class T, bool BOX>
 __global__ __launch_bounds__(PS_BLOCK) void k_apply_sym(SysConst s, ApplyGrid G, const int2* __restrict__ pairs,
@@ -609,10 +652,16 @@ __global__ __launch_bounds__(PS_BLOCK) void k_apply_sym(SysConst s, ApplyGrid G,
                                                         const T* __restrict__ xm, const double* __restrict__ f,
                                                         const double* __restrict__ stale, T* __restrict__ part_aa,
                                                         T* __restrict__ part_abm, T* __restrict__ part_c,
-                                                        T* __restrict__ part_fs) {
+                                                        T* __restrict__ part_fs, const double* __restrict__ xm64,
+                                                        double* __restrict__ bb) {
     constexpr size_t SZ_AA = sizeof(AASmem<T, BOX>), SZ_M = sizeof(ABMonoSmem<T>);
     __shared__ __align__(16) unsigned char smem[SZ_AA > SZ_M ? SZ_AA : SZ_M];
-    const int b = blockIdx.x;
+    // the colloid-pair blocks come first: short and latency-bound, they run beside the first tile pairs
+    if ((int)blockIdx.x < G.nbb) {
+        bb_block(blockIdx.x, s, xm64, f + s.nm3, bb, reinterpret_cast<double(*)[11]>(smem));
+        return;
+    }
+    const int b = blockIdx.x - G.nbb;
     if (b < G.nsym) {
         const int2 pr = pairs[b];
         AASmem<T, BOX>& sm = *reinterpret_cast<AASmem<T, BOX>*>(smem);
@@ -855,8 +904,9 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
         PS_HIP(c, hipEventRecord(ea0, st));
     }
     const bool box = s.box_active, ch = s.chain_HI;
-    // colloid pairs (FP64, every form)
-    if (s.Nc > 0) {
+    // colloid pairs (FP64, every form): blocks of the fused apply launch, else a launch of their own
+    const bool fused_bb = !srows && s.Nm > 0 && aw.sym;
+    if (s.Nc > 0 && !fused_bb) {
         k_bb<<<s.Nc, PS_BLOCK, 0, st>>>(s, c->xm, f + s.nm3, aw.bb);
         PS_LAUNCH(c);
     }
@@ -895,9 +945,10 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
         G.gx = (s.Nm + PS_BLOCK - 1) / PS_BLOCK;
         G.nmono = s.Nc > 0 ? G.gx * aw.nchunk_m : 0;
         G.ncoll = s.Nc > 0 ? s.Nc * aw.nchunk_c : 0;
+        G.nbb = s.Nc > 0 ? s.Nc : 0;
         G.per_cchunk = aw.per_chunk_m;
         G.per_chunk = aw.per_chunk_c;
-        const unsigned nblk = (unsigned)(G.nsym + G.nmono + G.ncoll);
+        const unsigned nblk = (unsigned)(G.nsym + G.nmono + G.ncoll + G.nbb);
         T* pa = (T*)aw.part_aa;
         T* pm = (T*)aw.part_abm;
         T* pc = (T*)aw.part_c;
@@ -906,10 +957,10 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
         using PN = const typename AAPos<T, false>::P*;
         if (box)
             k_apply_sym<T, true><<<nblk, PS_BLOCK, 0, st>>>(s, G, aw.pairs, k, (PB)aw.xs, xm, f, c->stale, pa, pm, pc,
-                                                            pf);
+                                                            pf, c->xm, aw.bb);
         else
             k_apply_sym<T, false><<<nblk, PS_BLOCK, 0, st>>>(s, G, aw.pairs, k, (PN)aw.xs, xm, f, c->stale, pa, pm,
-                                                             pc, pf);
+                                                             pc, pf, c->xm, aw.bb);
         PS_LAUNCH(c);
         if (c->prof_on) PS_HIP(c, hipEventRecord(ek1, st));
     } else {
