@@ -120,6 +120,28 @@ def timed_run(ctx, steps, warmup, label, profile=True):
     return dt, prof, per
 
 
+def fixed_state_timing(Context, pd, x0, over, steps, label):
+    """Steps that each start from the same state x0 (restored outside the timed region, the noise
+    stream running on), one ps_run(1) per step: the mixed and FP64 forms then do the same work per
+    step (the same stresslet iterations), so their ratio is not blurred by two trajectories that part
+    ways after a few steps."""
+    ctx = make_ctx(Context, pd, x0, **over)
+    ctx.run(1)   # warmup
+    dt, its = 0.0, 0.0
+    for _ in range(steps):
+        ctx.initial_configuration(x0)
+        ctx.state()
+        c0 = ctx.counters()
+        t0 = time.perf_counter()
+        ctx.run(1)
+        ctx.state()
+        dt += time.perf_counter() - t0
+        its += ctx.counters()["stresslet_iterations"] - c0["stresslet_iterations"]
+    ctx.close()
+    log(f"{label} from x0: {1e3 * dt / steps:.2f} ms/step, {its / steps:.1f} stresslet iterations per step")
+    return {"ms_per_step": 1e3 * dt / steps, "stresslet_iterations": its / steps, "steps": steps}
+
+
 def lub_effect(Context, pd, x0, over):
     """The lubricated form against the same form without lubrication, one step each from x0 with the
     same noise: colloid pairs inside the cutoff at x0 and the relative change of the step's velocities
@@ -663,6 +685,14 @@ def workload_line(args, wname, steps, warmup, with_cpu, graphs, minres=False, lu
         log(f"{form}: {results[form]['value']:.4g} bead-steps/s, {results[form]['ms_per_step']:.2f} ms/step "
             f"({timing}), per step {per}, apply kernel {results[form]['apply_kernel_ms_avg'] * 1e3:.1f} us")
 
+    # both forms on steps that each start from x0 (the same work per step)
+    fixed = None
+    if "mixed/block" in forms and "fp64/block" in forms:
+        fm = fixed_state_timing(Context, pd, x0, forms["mixed/block"], steps, "mixed/block")
+        ff = fixed_state_timing(Context, pd, x0, forms["fp64/block"], steps, "fp64/block")
+        fixed = {"mixed/block": fm, "fp64/block": ff, "mixed_over_fp64": ff["ms_per_step"] / fm["ms_per_step"],
+                 "note": "each step from the initial configuration (restored outside the timed region), one "
+                         "ps_run(1) per step, launched kernel by kernel"}
     # accuracy against the FP64 oracle, and the CPU baseline (the oracle's step on the host cores)
     errs, cpu, cpu1 = {}, None, None
     if with_cpu:
@@ -738,6 +768,7 @@ def workload_line(args, wname, steps, warmup, with_cpu, graphs, minres=False, lu
         "errors_mixed_vs_gpu_fp64": errs_gpu,
         "fp64_value": f64["value"],
         "mixed_over_fp64": head["value"] / f64["value"],
+        "from_fixed_state": fixed,
         "forms": results,
         "roofline": roof,
         "roofline_fp64": {"bound": "valu", "achieved": ach64 / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
