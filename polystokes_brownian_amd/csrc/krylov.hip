@@ -1349,20 +1349,6 @@ static ps_err saddle_solve_into(Ctx* c, const double* b, double* x, int warm, ps
     return rc;
 }
 
-// beta_j^2 = sum of the |w|^2 partials (every block alike; block 0 stores it); q_{j+1} = w / beta_j
-// when q is given.  No host round trip per iteration.
-__global__ __launch_bounds__(VB) void k_lz_next(int64_t n, double* __restrict__ q, const double* __restrict__ w,
-                                                const double* __restrict__ red2, int nb2,
-                                                double* __restrict__ norm2_out) {
-    __shared__ double sh[4];
-    const double b2 = sum_partials(red2, nb2, sh);
-    if (blockIdx.x == 0 && threadIdx.x == 0) *norm2_out = b2;
-    if (!q) return;
-    const double sc = 1.0 / sqrt(fmax(b2, 0.0));
-    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
-    if (i < n) q[i] = w[i] * sc;
-}
-
 // Lanczos scalars in c->lz: alpha_j at [j], beta_j^2 at [LZ_BETA2 + j], |xi|^2 at [LZ_XI2].
 constexpr int LZ_BETA2 = 64, LZ_XI2 = 3 * 64 + 1;
 constexpr int LZ_G = 64;   // row stride of the weights table c->lzg (one row per m)
@@ -1515,11 +1501,20 @@ ps_err sqrt_recurrence_dev(Ctx* c, const double* xi) {
     PS_LAUNCH(c);
     k_lz_start<<<ne, VB, 0, st>>>(n, xi, c->Q, c->lz);
     PS_LAUNCH(c);
+    // w alternates between two buffers: iteration j applies M to the previous iteration's w, which is
+    // orthogonalised but not yet normalised.  Its finalize sums |w|^2 into beta_{j-1}^2, stores
+    // q_j = w / beta_{j-1} and scales M w by 1 / beta_{j-1} (M is linear), so no launch of its own
+    // normalises q_j; the last iteration's beta is never needed.
+    double* wbuf[2] = {c->kr, c->kr + n};
     for (int j = 0; j < kmax; j++) {
         // w = M q_j; two classical Gram-Schmidt passes against q_0..q_j (the first one also yields
-        // alpha_j = q_j . M q_j); beta_j = |w|; q_{j+1} = w / beta_j.
-        const double* qj = c->Q + (int64_t)j * n;
-        PS_TRY(mob_apply(c, qj, w, prec));
+        // alpha_j = q_j . M q_j); |w|^2 partials for beta_j.
+        w = wbuf[j & 1];
+        if (j == 0)
+            PS_TRY(mob_apply_lanczos(c, c->Q, w, nullptr, nullptr, 0, nullptr, prec));
+        else
+            PS_TRY(mob_apply_lanczos(c, wbuf[(j - 1) & 1], w, c->Q + (int64_t)j * n, red2, (int)ne,
+                                     c->lz + LZ_BETA2 + j - 1, prec));
         for (int pass = 0; pass < 2; pass++) {
             k_lz_dots<<<dim3(nch, j + 1), VB, 0, st>>>(n, c->Q, w, c->red);
             PS_LAUNCH(c);
@@ -1527,9 +1522,6 @@ ps_err sqrt_recurrence_dev(Ctx* c, const double* xi) {
                                         pass == 1 ? red2 : nullptr);
             PS_LAUNCH(c);
         }
-        k_lz_next<<<ne, VB, 0, st>>>(n, j + 1 < kmax ? c->Q + (int64_t)(j + 1) * n : nullptr, w, red2, (int)ne,
-                                     c->lz + LZ_BETA2 + j);
-        PS_LAUNCH(c);
     }
     return PS_OK;
 }

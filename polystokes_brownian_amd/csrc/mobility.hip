@@ -751,50 +751,55 @@ __global__ __launch_bounds__(PS_BLOCK) void k_bb(SysConst s, const double* __res
 }
 
 // ---------------------------------------------------------------------------- finalize
-// mode 0: u = M f;  mode 1: saddle y = A x with x = [f ; lambda] (lambda = f + n);
-// rows start at r_begin (the stresslet-rows-only apply starts at nu).
+// Row r of u = M f from the self blocks (fill_self, src/mob.cpp:539-633) and the partial sums the
+// apply kernels left: the monomer tile partials (nsplit per target), the monomer-colloid chunks, the
+// colloid rows' chunks and the colloid pairs (bb).  Every partial is added in a fixed order.
 template <class T>
-__global__ __launch_bounds__(PS_BLOCK) void k_finalize(SysConst s, const double* __restrict__ f,
-                                                       double* __restrict__ u, const T* __restrict__ part_aa,
-                                                       int nsplit, const T* __restrict__ part_abm, int nchunk_m,
-                                                       const T* __restrict__ part_c,
-                                                       const T* __restrict__ part_fs, int nchunk,
-                                                       const double* __restrict__ bb,
-                                                       const double* __restrict__ stale, int saddle,
-                                                       int64_t r_begin) {
-    const int64_t r = r_begin + (int64_t)blockIdx.x * PS_BLOCK + threadIdx.x;
-    const int64_t n = s.n, nm3 = s.nm3, nc3 = s.nc3, nu = s.nu;
+struct FinArgs {
+    const T* part_aa;
+    int nsplit;
+    const T* part_abm;
+    int nchunk_m;
+    const T* part_c;
+    const T* part_fs;
+    int nchunk;
+    const double* bb;
+    const double* stale;
+};
+
+template <class T>
+__device__ __forceinline__ double finalize_row(const SysConst& s, const double* __restrict__ f, const FinArgs<T>& a,
+                                               int64_t r) {
+    const int64_t nm3 = s.nm3, nc3 = s.nc3, nu = s.nu;
     const int Nm = s.Nm, Nc = s.Nc;
-    if (r >= n) return;
-    double v;
     if (r < nm3) {
         const int i = (int)(r / 3), comp = (int)(r - 3 * (int64_t)i);
         double acc = 0.0;
         // the loads are independent: unrolled so that several are in flight (the sum order is fixed)
 #pragma unroll 8
-        for (int sp = 0; sp < nsplit; sp++) acc += (double)part_aa[((int64_t)sp * 3 + comp) * Nm + i];
-        for (int cm = 0; cm < nchunk_m; cm++) acc += (double)part_abm[((int64_t)cm * 3 + comp) * Nm + i];
-        v = s.beta_inv * f[r] + acc;
+        for (int sp = 0; sp < a.nsplit; sp++) acc += (double)a.part_aa[((int64_t)sp * 3 + comp) * Nm + i];
+        for (int cm = 0; cm < a.nchunk_m; cm++) acc += (double)a.part_abm[((int64_t)cm * 3 + comp) * Nm + i];
+        return s.beta_inv * f[r] + acc;
     } else if (r < nm3 + nc3) {
         const int64_t q = r - nm3;
         const int c = (int)(q / 3), comp = (int)(q - 3 * (int64_t)c);
         double fs[3] = {0.0, 0.0, 0.0};
         double acc = 0.0;
-        for (int ch = 0; ch < nchunk; ch++) {
-            fs[0] += (double)part_fs[3 * ch];
-            fs[1] += (double)part_fs[3 * ch + 1];
-            fs[2] += (double)part_fs[3 * ch + 2];
+        for (int ch = 0; ch < a.nchunk; ch++) {
+            fs[0] += (double)a.part_fs[3 * ch];
+            fs[1] += (double)a.part_fs[3 * ch + 1];
+            fs[2] += (double)a.part_fs[3 * ch + 2];
         }
-        for (int ch = 0; ch < nchunk; ch++) acc += (double)part_c[((int64_t)ch * Nc + c) * 12 + comp];
+        for (int ch = 0; ch < a.nchunk; ch++) acc += (double)a.part_c[((int64_t)ch * Nc + c) * 12 + comp];
         // transposed carried tile: U_c[comp] += sum_jj stale[jj][comp] * sum_m f_m[jj]
-        acc += stale[0 * 3 + comp] * fs[0] + stale[1 * 3 + comp] * fs[1] + stale[2 * 3 + comp] * fs[2];
-        v = 1.0 * f[r] + acc + bb[(int64_t)c * 11 + comp];
+        acc += a.stale[0 * 3 + comp] * fs[0] + a.stale[1 * 3 + comp] * fs[1] + a.stale[2 * 3 + comp] * fs[2];
+        return 1.0 * f[r] + acc + a.bb[(int64_t)c * 11 + comp];
     } else if (r < nu) {
         const int64_t q = r - nm3 - nc3;
         const int c = (int)(q / 3), comp = (int)(q - 3 * (int64_t)c);
         double acc = 0.0;
-        for (int ch = 0; ch < nchunk; ch++) acc += (double)part_c[((int64_t)ch * Nc + c) * 12 + 3 + comp];
-        v = (3. / 4.) * f[r] + acc + bb[(int64_t)c * 11 + 3 + comp];
+        for (int ch = 0; ch < a.nchunk; ch++) acc += (double)a.part_c[((int64_t)ch * Nc + c) * 12 + 3 + comp];
+        return (3. / 4.) * f[r] + acc + a.bb[(int64_t)c * 11 + 3 + comp];
     } else {
         const int64_t q = r - nu;
         const int c = (int)(q / 5), comp = (int)(q - 5 * (int64_t)c);
@@ -803,9 +808,21 @@ __global__ __launch_bounds__(PS_BLOCK) void k_finalize(SysConst s, const double*
         if (comp == 0) self += (9. / 10.) * S[4];
         if (comp == 4) self += (9. / 10.) * S[0];
         double acc = 0.0;
-        for (int ch = 0; ch < nchunk; ch++) acc += (double)part_c[((int64_t)ch * Nc + c) * 12 + 6 + comp];
-        v = self + acc + bb[(int64_t)c * 11 + 6 + comp];
+        for (int ch = 0; ch < a.nchunk; ch++) acc += (double)a.part_c[((int64_t)ch * Nc + c) * 12 + 6 + comp];
+        return self + acc + a.bb[(int64_t)c * 11 + 6 + comp];
     }
+}
+
+// mode 0: u = M f;  mode 1: saddle y = A x with x = [f ; lambda] (lambda = f + n);
+// rows start at r_begin (the stresslet-rows-only apply starts at nu).
+template <class T>
+__global__ __launch_bounds__(PS_BLOCK) void k_finalize(SysConst s, const double* __restrict__ f,
+                                                       double* __restrict__ u, FinArgs<T> a, int saddle,
+                                                       int64_t r_begin) {
+    const int64_t r = r_begin + (int64_t)blockIdx.x * PS_BLOCK + threadIdx.x;
+    const int64_t n = s.n, nu = s.nu;
+    if (r >= n) return;
+    double v = finalize_row<T>(s, f, a, r);
     if (saddle) {
         if (r < nu) {
             v -= f[n + r];
@@ -813,6 +830,34 @@ __global__ __launch_bounds__(PS_BLOCK) void k_finalize(SysConst s, const double*
         }
     }
     u[r] = v;
+}
+
+// The Lanczos iteration's finalize: f is the previous iteration's orthogonalised vector w, not yet
+// normalised.  beta^2 = the sum of the nb2 block partials of |w|^2 (every block alike, block 0 stores
+// it); y = (M f) / beta = M q and q = f / beta is stored as the basis vector.  With red2 null, f is
+// already q and nothing is scaled.
+template <class T>
+__global__ __launch_bounds__(PS_BLOCK) void k_finalize_lz(SysConst s, const double* __restrict__ f,
+                                                          double* __restrict__ y, FinArgs<T> a,
+                                                          const double* __restrict__ red2, int nb2,
+                                                          double* __restrict__ beta2_out, double* __restrict__ qout) {
+    __shared__ double sh[4];
+    double sc = 1.0;
+    if (red2) {
+        // every thread of every block forms the same sum: wave sums, then the four in a fixed order
+        double v = 0.0;
+        for (int i = threadIdx.x; i < nb2; i += PS_BLOCK) v += red2[i];
+        v = wave_sum(v);
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+        __syncthreads();
+        const double b2 = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+        if (blockIdx.x == 0 && threadIdx.x == 0 && beta2_out) *beta2_out = b2;
+        sc = 1.0 / sqrt(fmax(b2, 0.0));
+    }
+    const int64_t r = (int64_t)blockIdx.x * PS_BLOCK + threadIdx.x;
+    if (r >= s.n) return;
+    y[r] = sc * finalize_row<T>(s, f, a, r);
+    if (qout) qout[r] = sc * f[r];
 }
 
 // ---------------------------------------------------------------------------- carried tile
@@ -891,8 +936,16 @@ __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, in
     if (i < n) b[i] = (float)a[i];
 }
 
+template <class T>
+FinArgs<T> fin_args(const Ctx* c, const ApplyWork& aw) {
+    const SysConst& s = c->sc;
+    return FinArgs<T>{(const T*)aw.part_aa, s.Nm > 0 ? aw.nsplit_aa : 0, (const T*)aw.part_abm,
+                      (s.Nm > 0 && s.Nc > 0) ? aw.nchunk_m : 0, (const T*)aw.part_c, (const T*)aw.part_fs,
+                      (s.Nm > 0 && s.Nc > 0) ? aw.nchunk_c : 0, aw.bb, c->stale};
+}
+
 // saddle: 0 = u = M f, 1 = saddle operator, 2 = only the stresslet rows [nu, n) of M f, 3 = only the
-// colloid rows [nm3, n) of M f.
+// colloid rows [nm3, n) of M f, 4 = the apply kernels only (the partial sums; the caller finalizes).
 template <class T>
 ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double* u, int saddle) {
     const SysConst& s = c->sc;
@@ -998,14 +1051,15 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
             }
         }
     }
+    if (saddle == 4) {
+        if (c->prof_on) PS_HIP(c, hipEventRecord(ea1, st));
+        c->cnt.applies++;
+        return PS_OK;
+    }
     const int64_t r_begin = saddle == 2 ? s.nu : saddle == 3 ? s.nm3 : 0;
     if (s.n - r_begin <= 0) return PS_OK;
     dim3 g4((unsigned)((s.n - r_begin + PS_BLOCK - 1) / PS_BLOCK));
-    k_finalize<T><<<g4, PS_BLOCK, 0, st>>>(s, f, u, (const T*)aw.part_aa, s.Nm > 0 ? aw.nsplit_aa : 0,
-                                           (const T*)aw.part_abm, (s.Nm > 0 && s.Nc > 0) ? aw.nchunk_m : 0,
-                                           (const T*)aw.part_c, (const T*)aw.part_fs,
-                                           (s.Nm > 0 && s.Nc > 0) ? aw.nchunk_c : 0, aw.bb, c->stale,
-                                           srows ? 0 : saddle, r_begin);
+    k_finalize<T><<<g4, PS_BLOCK, 0, st>>>(s, f, u, fin_args<T>(c, aw), srows ? 0 : saddle, r_begin);
     PS_LAUNCH(c);
     if (srows) return PS_OK;
     if (c->prof_on) PS_HIP(c, hipEventRecord(ea1, st));
@@ -1310,6 +1364,24 @@ ps_err mob_apply(Ctx* c, const double* f, double* u, int prec) {
     return launch_apply<double>(c, c->xm, c->aw64, f, u, 0);
 }
 
+// The Lanczos iteration's apply: y = M (f / beta) and q = f / beta with beta^2 the sum of the nb2
+// partials in red2 (stored to *beta2_out), or y = M f when red2 is null (k_finalize_lz).
+template <class T>
+static ps_err apply_lanczos(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double* y, double* qout,
+                            const double* red2, int nb2, double* beta2_out) {
+    PS_TRY(launch_apply<T>(c, xm, aw, f, nullptr, 4));
+    const unsigned g = (unsigned)((c->n + PS_BLOCK - 1) / PS_BLOCK);
+    k_finalize_lz<T><<<g, PS_BLOCK, 0, c->stream>>>(c->sc, f, y, fin_args<T>(c, aw), red2, nb2, beta2_out, qout);
+    PS_LAUNCH(c);
+    return PS_OK;
+}
+ps_err mob_apply_lanczos(Ctx* c, const double* f, double* y, double* qout, const double* red2, int nb2,
+                         double* beta2_out, int prec) {
+    if (prec == PS_PREC_MIXED && c->xm32)
+        return apply_lanczos<float>(c, c->xm32, c->aw32, f, y, qout, red2, nb2, beta2_out);
+    return apply_lanczos<double>(c, c->xm, c->aw64, f, y, qout, red2, nb2, beta2_out);
+}
+
 ps_err mob_apply_stresslet_rows(Ctx* c, const double* f, double* u) {
     if (c->arrow) return arrow_apply(c, f, u, 0);   // every row: O(N) anyway
     return launch_apply<double>(c, c->xm, c->aw64, f, u, 2);
@@ -1538,9 +1610,9 @@ extern "C" ps_err ps_mob_mult_noalloc(const ps_params* p, const double* x, const
         }
     }
     dim3 g4((unsigned)((s.n + PS_BLOCK - 1) / PS_BLOCK));
-    k_finalize<double><<<g4, PS_BLOCK, 0, st>>>(s, f, u, part_aa, Nm > 0 ? nsplit : 0, part_abm,
-                                                (Nm > 0 && Nc > 0) ? nchm : 0, part_c, part_fs,
-                                                (Nm > 0 && Nc > 0) ? nchc : 0, bb, stale, 0, 0);
+    const FinArgs<double> fa{part_aa, Nm > 0 ? nsplit : 0, part_abm, (Nm > 0 && Nc > 0) ? nchm : 0, part_c, part_fs,
+                             (Nm > 0 && Nc > 0) ? nchc : 0, bb, stale};
+    k_finalize<double><<<g4, PS_BLOCK, 0, st>>>(s, f, u, fa, 0, 0);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         ps_set_context_free_error(std::string("ps_mob_mult_noalloc: kernel launch: ") + hipGetErrorString(e));

@@ -279,6 +279,10 @@ ps_err mss_joint(Ctx* c, int nsys, const double* const* A, const double* const* 
 ps_err sqrt_recurrence_dev(Ctx* c, const double* xi);
 ps_err sqrt_finish_dev(Ctx* c, double* out, ps_sqrt_info* info);
 ps_err mob_apply(Ctx* c, const double* f, double* u, int prec);    // u = M f (device pointers)
+// The Lanczos iteration's apply: y = M (f / beta), q = f / beta, beta^2 = the sum of nb2 partials in
+// red2 (stored to *beta2_out); with red2 null, y = M f and q is not written.
+ps_err mob_apply_lanczos(Ctx* c, const double* f, double* y, double* qout, const double* red2, int nb2,
+                         double* beta2_out, int prec);
 ps_err saddle_apply(Ctx* c, const double* x, double* y, int prec); // y = A x (device pointers)
 ps_err mob_apply_stresslet_rows(Ctx* c, const double* f, double* u); // u[nu:n] = (M f)[nu:n], FP64
 ps_err mob_apply_colloid_rows(Ctx* c, const double* f, double* u);   // u[nm3:n] = (M f)[nm3:n], FP64
