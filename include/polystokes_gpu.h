@@ -344,6 +344,11 @@ ps_err ps_batch_times(void* batch, double* us, int* team);
  * PS_AA_AUTO timed them, the chosen kernel's measured milliseconds per apply (ms64, ms32; else 0).
  * Any output may be null.  No reference counterpart. */
 ps_err ps_kernel_choice(void* ctx, int* kernel64, int* kernel32, float* ms64, float* ms32);
+/* The tile shape PS_AA_AUTO chose for the ordered kernel: its source split in beads (a multiple of the
+ * 256-bead tile; each block applies one 256-target tile to one split), for the FP64 and the FP32
+ * applies; 0 where the symmetric kernel (256-bead tile pairs) runs or the context has no FP32 applies.
+ * Either output may be null.  No reference counterpart. */
+ps_err ps_kernel_shape(void* ctx, int* split64, int* split32);
 /* Workgroups of the last step's launch (nwg_last) and the co-resident maximum the schedule packs
  * replicas onto (nwg_max); either pointer may be null.  No reference counterpart. */
 ps_err ps_batch_workgroups(void* batch, int* nwg_last, int* nwg_max);

@@ -912,6 +912,15 @@ ps_err ps_kernel_choice(void* h, int* kernel64, int* kernel32, float* ms64, floa
     return PS_OK;
 }
 
+ps_err ps_kernel_shape(void* h, int* split64, int* split32) {
+    Ctx* c = as_ctx(h);
+    if (!c) return PS_ERR_ARG;
+    const bool has32 = c->p.precision == PS_PREC_MIXED;
+    if (split64) *split64 = c->aw64.sym ? 0 : c->aw64.per_split_aa;
+    if (split32) *split32 = has32 && !c->aw32.sym ? c->aw32.per_split_aa : 0;
+    return PS_OK;
+}
+
 ps_err ps_get_counters(void* h, ps_counters* out) {
     Ctx* c = as_ctx(h);
     if (!c || !out) return PS_ERR_ARG;

@@ -1067,16 +1067,25 @@ ps_err launch_apply(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double*
     return PS_OK;
 }
 
-ps_err alloc_work(Ctx* c, ApplyWork& aw, size_t elt, bool sym) {
+// The ordered kernel's default source split, in 256-bead tiles: about 4 blocks per CU.
+int default_split_tiles(const Ctx* c) {
     const SysConst& s = c->sc;
     const int ntb = (s.Nm + PS_BLOCK - 1) / PS_BLOCK;
     const int target = 4 * (c->n_cu > 0 ? c->n_cu : 256);
     int nsplit = ntb > 0 ? (target + ntb - 1) / ntb : 1;
     nsplit = std::max(1, std::min(nsplit, std::max(1, s.Nm / 64)));
-    aw.nsplit_aa = nsplit;
+    const int per = s.Nm > 0 ? (s.Nm + nsplit - 1) / nsplit : 0;
+    return std::max(1, (per + PS_BLOCK - 1) / PS_BLOCK);
+}
+
+// split_tiles: the ordered kernel's source split in whole 256-bead tiles (0: the default).
+ps_err alloc_work(Ctx* c, ApplyWork& aw, size_t elt, bool sym, int split_tiles = 0) {
+    const SysConst& s = c->sc;
+    const int ntb = (s.Nm + PS_BLOCK - 1) / PS_BLOCK;
     // source splits are whole 256-bead tiles (k_aa relies on tile-aligned sources)
-    aw.per_split_aa = s.Nm > 0 ? (s.Nm + nsplit - 1) / nsplit : 0;
-    aw.per_split_aa = ((aw.per_split_aa + PS_BLOCK - 1) / PS_BLOCK) * PS_BLOCK;
+    const int target = 4 * (c->n_cu > 0 ? c->n_cu : 256);
+    const int tiles = std::max(1, std::min(split_tiles > 0 ? split_tiles : default_split_tiles(c), std::max(ntb, 1)));
+    aw.per_split_aa = s.Nm > 0 ? tiles * PS_BLOCK : 0;
     aw.nsplit_aa = s.Nm > 0 ? (s.Nm + aw.per_split_aa - 1) / aw.per_split_aa : 1;
     int nchunk = s.Nc > 0 ? (2 * (c->n_cu > 0 ? c->n_cu : 256) + s.Nc - 1) / s.Nc : 1;
     nchunk = std::max(1, std::min(nchunk, std::max(1, s.Nm / PS_BLOCK)));
@@ -1142,7 +1151,12 @@ struct AATuneKey {
         return std::tie(Nm, Nc, elt, box, ncu) < std::tie(o.Nm, o.Nc, o.elt, o.box, o.ncu);
     }
 };
-std::map<AATuneKey, std::pair<bool, float>> g_aa_tune;   // -> (symmetric faster, its ms per apply)
+struct AATuneVal {
+    bool sym;          // the symmetric kernel is the faster
+    int split_tiles;   // else the ordered kernel's fastest source split (256-bead tiles)
+    float ms;          // the chosen layout's ms per apply
+};
+std::map<AATuneKey, AATuneVal> g_aa_tune;
 std::mutex g_aa_tune_mu;
 
 // synthetic monomer and colloid positions of the system's size, seeded (the apply times do not depend
@@ -1185,42 +1199,61 @@ ps_err time_apply(Ctx* c, const T* xm, ApplyWork& aw, int reps, float* ms) {
     return rc;
 }
 
-// aw <- the faster layout for precision T (both allocated and timed, the slower freed)
+// aw <- the fastest layout for precision T: the symmetric kernel (when `sym_ok`) and the ordered kernel
+// at source splits of half, one and two times its default length (the tile shape of its blocks), each
+// allocated and timed on the synthetic system, the slower ones freed.  The choice is cached per size
+// class for the life of the process.
 template <class T>
-ps_err choose_kernel(Ctx* c, ApplyWork& aw, const T* xm) {
-    const AATuneKey key{c->Nm, c->Nc, (int)sizeof(T), c->sc.box_active, c->n_cu};
+ps_err choose_kernel(Ctx* c, ApplyWork& aw, const T* xm, bool sym_ok) {
+    const AATuneKey key{c->Nm, c->Nc, (int)sizeof(T), c->sc.box_active | (sym_ok ? 0 : 2), c->n_cu};
     {
         std::lock_guard<std::mutex> lk(g_aa_tune_mu);
         auto it = g_aa_tune.find(key);
         if (it != g_aa_tune.end()) {
-            c->aa_tuned_ms[sizeof(T) == sizeof(float)] = it->second.second;
-            return alloc_work(c, aw, sizeof(T), it->second.first);
+            c->aa_tuned_ms[sizeof(T) == sizeof(float)] = it->second.ms;
+            return alloc_work(c, aw, sizeof(T), it->second.sym, it->second.split_tiles);
         }
     }
-    ApplyWork ord, sym;
-    PS_TRY(alloc_work(c, ord, sizeof(T), false));
-    ps_err rc = alloc_work(c, sym, sizeof(T), true);
-    float t_ord = 0.f, t_sym = 0.f;
+    const int ntb = std::max(1, (c->Nm + PS_BLOCK - 1) / PS_BLOCK);
+    const int d = default_split_tiles(c);
+    std::vector<int> splits;
+    for (int t : {d, std::max(1, d / 2), std::min(ntb, 2 * d)})
+        if (std::find(splits.begin(), splits.end(), t) == splits.end()) splits.push_back(t);
+    ApplyWork best{};
+    AATuneVal bv{false, 0, 0.f};
+    bool have = false;
+    ps_err rc = PS_OK;
     const int64_t cnt0 = c->cnt.applies;
-    if (rc == PS_OK) rc = time_apply<T>(c, xm, ord, 3, &t_ord);
-    if (rc == PS_OK) rc = time_apply<T>(c, xm, sym, 3, &t_sym);
+    auto consider = [&](bool sym, int split) {
+        if (rc != PS_OK) return;
+        ApplyWork w{};
+        rc = alloc_work(c, w, sizeof(T), sym, split);
+        float t = 0.f;
+        if (rc == PS_OK) rc = time_apply<T>(c, xm, w, 3, &t);
+        if (rc != PS_OK) {
+            free_work(w);
+            return;
+        }
+        if (!have || t < bv.ms) {
+            if (have) free_work(best);
+            best = w;
+            bv = AATuneVal{sym, sym ? 0 : split, t};
+            have = true;
+        } else {
+            free_work(w);
+        }
+    };
+    for (int t : splits) consider(false, t);
+    if (sym_ok) consider(true, 0);
     c->cnt.applies = cnt0;
     if (rc != PS_OK) {
-        free_work(ord);
-        free_work(sym);
+        if (have) free_work(best);
         return rc;
     }
-    const bool use_sym = t_sym <= t_ord;
-    if (use_sym) {
-        free_work(ord);
-        aw = sym;
-    } else {
-        free_work(sym);
-        aw = ord;
-    }
-    c->aa_tuned_ms[sizeof(T) == sizeof(float)] = use_sym ? t_sym : t_ord;
+    aw = best;
+    c->aa_tuned_ms[sizeof(T) == sizeof(float)] = bv.ms;
     std::lock_guard<std::mutex> lk(g_aa_tune_mu);
-    g_aa_tune[key] = {use_sym, use_sym ? t_sym : t_ord};
+    g_aa_tune[key] = bv;
     return PS_OK;
 }
 
@@ -1234,7 +1267,7 @@ ps_err apply_alloc(Ctx* c) {
     const int sel = c->p.aa_kernel;
     const bool mixed = c->p.precision == PS_PREC_MIXED;
     if (mixed) PS_HIP(c, hipMalloc(&c->xm32, sizeof(float) * 3 * (size_t)c->Np));
-    if (!eligible || sel != PS_AA_AUTO) {
+    if (c->arrow || s.Nm <= 0 || sel != PS_AA_AUTO) {   // the arrowhead regime applies no tiles
         const bool sym = eligible && sel == PS_AA_SYMMETRIC;
         PS_TRY(alloc_work(c, c->aw64, sizeof(double), sym));
         if (mixed) PS_TRY(alloc_work(c, c->aw32, sizeof(float), sym));
@@ -1250,12 +1283,12 @@ ps_err apply_alloc(Ctx* c) {
         v = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
     }
     PS_HIP(c, hipMemcpy(c->io_a, ft.data(), sizeof(double) * ft.size(), hipMemcpyHostToDevice));
-    PS_TRY(choose_kernel<double>(c, c->aw64, c->xm));
+    PS_TRY(choose_kernel<double>(c, c->aw64, c->xm, eligible));
     if (mixed) {
         const int64_t n3 = 3 * (int64_t)c->Np;
         k_to_f32<<<(unsigned)((n3 + PS_BLOCK - 1) / PS_BLOCK), PS_BLOCK, 0, c->stream>>>(c->xm, c->xm32, n3);
         PS_LAUNCH(c);
-        PS_TRY(choose_kernel<float>(c, c->aw32, c->xm32));
+        PS_TRY(choose_kernel<float>(c, c->aw32, c->xm32, eligible));
     }
     PS_HIP(c, hipStreamSynchronize(c->stream));
     return PS_OK;

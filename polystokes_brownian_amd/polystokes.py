@@ -91,6 +91,7 @@ def lib():
         L.ps_batch_op.argtypes = [vp, ip, dp, dp, C.POINTER(ip)]
         L.ps_batch_times.argtypes = [vp, dp, C.POINTER(ip)]
         L.ps_batch_config.argtypes = [vp, ip, ip, ip]
+        L.ps_kernel_shape.argtypes = [vp, C.POINTER(ip), C.POINTER(ip)]
         L.ps_batch_graph_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(ip)]
         L.ps_batch_phase_times.argtypes = [vp, dp]
         L.ps_batch_workgroups.argtypes = [vp, C.POINTER(ip), C.POINTER(ip)]
@@ -194,8 +195,11 @@ class Context:
         k64, k32 = C.c_int(), C.c_int()
         t64, t32 = C.c_float(), C.c_float()
         self._check(lib().ps_kernel_choice(self.h, C.byref(k64), C.byref(k32), C.byref(t64), C.byref(t32)))
+        s64, s32 = C.c_int(), C.c_int()
+        self._check(lib().ps_kernel_shape(self.h, C.byref(s64), C.byref(s32)))
         name = {1: "ordered", 2: "symmetric"}
-        return {"fp64": name[k64.value], "fp32": name.get(k32.value), "ms64": t64.value, "ms32": t32.value}
+        return {"fp64": name[k64.value], "fp32": name.get(k32.value), "ms64": t64.value, "ms32": t32.value,
+                "split64": s64.value, "split32": s32.value}
 
     def check_dist(self):
         n = C.c_int64()
