@@ -14,6 +14,7 @@
 // modified Gram-Schmidt, tridiagonal eigendecomposition on the host (m <= 64), negative Ritz
 // values floored, relative-change stopping test.
 #include "ps_common.h"
+#include "lzdots.h"
 #include "tql2.h"
 
 #include <algorithm>
@@ -280,53 +281,51 @@ __global__ void k_recip(double* __restrict__ d, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------- Lanczos kernels
-// Classical Gram-Schmidt pass against Q_0..Q_{m-1}.  Dots: block (chunk, q) writes the partial of
-// Q_q . w over its chunk of LZ_CH elements to red[q * nchunk + chunk].
-constexpr int LZ_CH = VB * 8;
-__global__ __launch_bounds__(VB) void k_lz_dots(int64_t n, const double* __restrict__ Q,
-                                                const double* __restrict__ w, double* __restrict__ red) {
-    __shared__ double sh[4];
-    const int q = blockIdx.y;
-    const int64_t base = (int64_t)blockIdx.x * LZ_CH;
-    const double* Qq = Q + (int64_t)q * n;
-    double acc = 0.0;
-#pragma unroll
-    for (int k = 0; k < LZ_CH / VB; k++) {
-        const int64_t i = base + k * VB + threadIdx.x;
-        if (i < n) acc += Qq[i] * w[i];
-    }
-    acc = block_reduce(acc, sh);
-    if (threadIdx.x == 0) red[(int64_t)q * gridDim.x + blockIdx.x] = acc;
-}
-// h_q = sum of the nchunk partials of q (fixed order, every block alike); w -= sum_q h_q Q_q.
-// Pass 0 stores h_{m-1} = q_j . (M q_j) as the Lanczos alpha_j; pass 1 leaves per-block partials
-// of |w|^2 in red2.
-__global__ __launch_bounds__(VB) void k_lz_sub(int64_t n, const double* __restrict__ Q, int m,
-                                               double* __restrict__ w, const double* __restrict__ red,
-                                               int nchunk, double* __restrict__ alpha_out,
-                                               double* __restrict__ red2) {
-    __shared__ double sh[4];
+// The two Gram-Schmidt passes of a Lanczos iteration over rows of 256 (block b owns rows 256 b ..),
+// with the block partials of the Lanczos finalize (red1) or of the previous launch (red2):
+//   k_lz_sub_dots: h = the sums of red1 (every block alike); alpha_j = h_{m-1}; w -= sum_k h_k Q_k;
+//                  then the second pass's partial dots Q_k . w into red2;
+//   k_lz_sub_norm: h from red2; w -= sum_k h_k Q_k; the block partials of |w|^2 into red3, which the
+//                  next iteration's finalize sums into beta^2.
+__global__ __launch_bounds__(VB) void k_lz_sub_dots(int64_t n, const double* __restrict__ Q, int m,
+                                                    double* __restrict__ w, const double* __restrict__ red1, int nb,
+                                                    double* __restrict__ alpha_out, double* __restrict__ red2) {
     __shared__ double h[64];
-    if (threadIdx.x < m) {
-        const double* r = red + (int64_t)threadIdx.x * nchunk;
-        double v = 0.0;
+    __shared__ double lds[LZD_CH * LZD_T];
+    lzd_sum(red1, nb, m, h);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *alpha_out = h[m - 1];
+    const int64_t r = (int64_t)blockIdx.x * VB + threadIdx.x;
+    const bool in = r < n;
+    const int64_t rc = in ? r : 0;
+    double wi = in ? w[r] : 0.0;
 #pragma unroll 8
-        for (int c = 0; c < nchunk; c++) v += r[c];
-        h[threadIdx.x] = v;
+    for (int q = 0; q < m; q++) wi -= h[q] * Q[(int64_t)q * n + rc];
+    if (in) w[r] = wi;
+    for (int q0 = 0; q0 < m; q0 += LZD_CH) {
+        const int mc = min(LZD_CH, m - q0);
+#pragma unroll 8
+        for (int k = 0; k < mc; k++) lds[k * LZD_T + threadIdx.x] = in ? Q[(int64_t)(q0 + k) * n + rc] * wi : 0.0;
+        __syncthreads();
+        lzd_store(lds, mc, red2 + (int64_t)q0 * nb, nb);
+        __syncthreads();
     }
+}
+__global__ __launch_bounds__(VB) void k_lz_sub_norm(int64_t n, const double* __restrict__ Q, int m,
+                                                    double* __restrict__ w, const double* __restrict__ red2, int nb,
+                                                    double* __restrict__ red3) {
+    __shared__ double h[64];
+    __shared__ double lds[LZD_T];
+    lzd_sum(red2, nb, m, h);
+    const int64_t r = (int64_t)blockIdx.x * VB + threadIdx.x;
+    const bool in = r < n;
+    const int64_t rc = in ? r : 0;
+    double wi = in ? w[r] : 0.0;
+#pragma unroll 8
+    for (int q = 0; q < m; q++) wi -= h[q] * Q[(int64_t)q * n + rc];
+    if (in) w[r] = wi;
+    lds[threadIdx.x] = in ? wi * wi : 0.0;
     __syncthreads();
-    if (alpha_out && blockIdx.x == 0 && threadIdx.x == 0) *alpha_out = h[m - 1];
-    const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
-    double wi = 0.0;
-    if (i < n) {
-        wi = w[i];
-        for (int q = 0; q < m; q++) wi -= h[q] * Q[(int64_t)q * n + i];
-        w[i] = wi;
-    }
-    if (red2) {
-        const double acc = block_reduce(wi * wi, sh);
-        if (threadIdx.x == 0) red2[blockIdx.x] = acc;
-    }
+    lzd_store(lds, 1, red3, nb);
 }
 inline unsigned eblocks(int64_t n) { return (unsigned)((n + VB - 1) / VB); }
 
@@ -348,6 +347,7 @@ ps_err krylov_alloc(Ctx* c) {
     PS_HIP(c, hipMemset(c->lz, 0, sizeof(double) * (3 * 64 + 8)));
     PS_HIP(c, hipHostMalloc(&c->lz_h, sizeof(double) * (3 * 64 + 8), hipHostMallocDefault));
     PS_HIP(c, hipMalloc(&c->lzg, sizeof(double) * 65 * 64));
+    PS_HIP(c, hipMalloc(&c->lzred, sizeof(double) * 129 * (size_t)eblocks(n)));
     PS_HIP(c, hipMalloc(&c->lzinfo, sizeof(double) * 4));
     PS_HIP(c, hipMemset(c->lzinfo, 0, sizeof(double) * 4));
     PS_HIP(c, hipMalloc(&c->dcnt, sizeof(int64_t) * DC_N));
@@ -374,8 +374,9 @@ ps_err krylov_alloc(Ctx* c) {
 void krylov_free(Ctx* c) {
     for (void* p : {(void*)c->kr, (void*)c->dinv, (void*)c->red, (void*)c->mstate, (void*)c->Q, (void*)c->lz,
                     (void*)c->lzg, (void*)c->lzinfo, (void*)c->dcnt, (void*)c->gbar, (void*)c->ir_r, (void*)c->ir_d,
-                    (void*)c->Mss, (void*)c->bs_y, (void*)c->bs_r})
+                    (void*)c->Mss, (void*)c->bs_y, (void*)c->bs_r, (void*)c->lzred})
         if (p) (void)hipFree(p);
+    c->lzred = nullptr;
     c->Mss = c->bs_y = c->bs_r = nullptr;
     c->lzg = c->lzinfo = nullptr;
     c->dcnt = nullptr;
@@ -1486,10 +1487,8 @@ ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* out, ps_sqrt_info* info)
 // The Lanczos recurrence of sqrt_apply_dev: kmax vectors into Q, the scalars into lz.
 ps_err sqrt_recurrence_dev(Ctx* c, const double* xi) {
     const int64_t n = c->n;
-    const unsigned nch = (unsigned)((n + LZ_CH - 1) / LZ_CH);
     const unsigned ne = eblocks(n);
     const int nbd = nblocks(n);
-    double* red2 = c->red + (size_t)nch * 64;   // after the dot partials of up to 64 vectors
     hipStream_t st = c->stream;
     const int kmax = c->kmax;
     const int prec = c->p.precision;
@@ -1503,25 +1502,25 @@ ps_err sqrt_recurrence_dev(Ctx* c, const double* xi) {
     PS_LAUNCH(c);
     // w alternates between two buffers: iteration j applies M to the previous iteration's w, which is
     // orthogonalised but not yet normalised.  Its finalize sums |w|^2 into beta_{j-1}^2, stores
-    // q_j = w / beta_{j-1} and scales M w by 1 / beta_{j-1} (M is linear), so no launch of its own
-    // normalises q_j; the last iteration's beta is never needed.
+    // q_j = w / beta_{j-1}, scales M w by 1 / beta_{j-1} (M is linear) and leaves the first
+    // Gram-Schmidt pass's partial dots; the two subtract launches leave the second pass's dots and the
+    // partials of |w|^2.  The last iteration's beta is never needed.
+    const int nb = (int)ne;
+    double* red1 = c->lzred;
+    double* red2 = c->lzred + 64 * (size_t)nb;
+    double* red3 = c->lzred + 128 * (size_t)nb;
     double* wbuf[2] = {c->kr, c->kr + n};
     for (int j = 0; j < kmax; j++) {
-        // w = M q_j; two classical Gram-Schmidt passes against q_0..q_j (the first one also yields
-        // alpha_j = q_j . M q_j); |w|^2 partials for beta_j.
         w = wbuf[j & 1];
         if (j == 0)
-            PS_TRY(mob_apply_lanczos(c, c->Q, w, nullptr, nullptr, 0, nullptr, prec));
+            PS_TRY(mob_apply_lanczos(c, c->Q, w, nullptr, nullptr, nb, nullptr, 1, red1, prec));
         else
-            PS_TRY(mob_apply_lanczos(c, wbuf[(j - 1) & 1], w, c->Q + (int64_t)j * n, red2, (int)ne,
-                                     c->lz + LZ_BETA2 + j - 1, prec));
-        for (int pass = 0; pass < 2; pass++) {
-            k_lz_dots<<<dim3(nch, j + 1), VB, 0, st>>>(n, c->Q, w, c->red);
-            PS_LAUNCH(c);
-            k_lz_sub<<<ne, VB, 0, st>>>(n, c->Q, j + 1, w, c->red, nch, pass == 0 ? c->lz + j : nullptr,
-                                        pass == 1 ? red2 : nullptr);
-            PS_LAUNCH(c);
-        }
+            PS_TRY(mob_apply_lanczos(c, wbuf[(j - 1) & 1], w, c->Q + (int64_t)j * n, red3, nb,
+                                     c->lz + LZ_BETA2 + j - 1, j + 1, red1, prec));
+        k_lz_sub_dots<<<ne, VB, 0, st>>>(n, c->Q, j + 1, w, red1, nb, c->lz + j, red2);
+        PS_LAUNCH(c);
+        k_lz_sub_norm<<<ne, VB, 0, st>>>(n, c->Q, j + 1, w, red2, nb, red3);
+        PS_LAUNCH(c);
     }
     return PS_OK;
 }

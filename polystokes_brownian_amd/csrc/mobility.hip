@@ -18,6 +18,7 @@
 // src/mob.cpp:220-227), so no 3x5 or 5x5 block is ever materialised for AB pairs.
 #include "ps_common.h"
 #include "tiles.h"
+#include "lzdots.h"
 
 #include <hip/hip_ext.h>
 
@@ -833,20 +834,25 @@ __global__ __launch_bounds__(PS_BLOCK) void k_finalize(SysConst s, const double*
 }
 
 // The Lanczos iteration's finalize: f is the previous iteration's orthogonalised vector w, not yet
-// normalised.  beta^2 = the sum of the nb2 block partials of |w|^2 (every block alike, block 0 stores
-// it); y = (M f) / beta = M q and q = f / beta is stored as the basis vector.  With red2 null, f is
-// already q and nothing is scaled.
+// normalised.  beta^2 = the sum of the nb block partials of |w|^2 in red3 (every block alike, block 0
+// stores it); y = (M f) / beta = M q and q = f / beta is stored as the basis vector.  With red3 null,
+// f is already q and nothing is scaled.  The first Gram-Schmidt pass's partial dots follow in the same
+// launch: red1[k * nb + block] = the block's sum of Q_k[r] y[r] for k < m, with Q_{m-1} = q.
 template <class T>
 __global__ __launch_bounds__(PS_BLOCK) void k_finalize_lz(SysConst s, const double* __restrict__ f,
                                                           double* __restrict__ y, FinArgs<T> a,
-                                                          const double* __restrict__ red2, int nb2,
-                                                          double* __restrict__ beta2_out, double* __restrict__ qout) {
+                                                          const double* __restrict__ red3, int nb,
+                                                          double* __restrict__ beta2_out, double* __restrict__ qout,
+                                                          const double* __restrict__ Q, int m,
+                                                          double* __restrict__ red1) {
     __shared__ double sh[4];
+    __shared__ double lds[LZD_CH * LZD_T];
+    const int64_t n = s.n;
     double sc = 1.0;
-    if (red2) {
+    if (red3) {
         // every thread of every block forms the same sum: wave sums, then the four in a fixed order
         double v = 0.0;
-        for (int i = threadIdx.x; i < nb2; i += PS_BLOCK) v += red2[i];
+        for (int i = threadIdx.x; i < nb; i += PS_BLOCK) v += red3[i];
         v = wave_sum(v);
         if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
         __syncthreads();
@@ -855,9 +861,27 @@ __global__ __launch_bounds__(PS_BLOCK) void k_finalize_lz(SysConst s, const doub
         sc = 1.0 / sqrt(fmax(b2, 0.0));
     }
     const int64_t r = (int64_t)blockIdx.x * PS_BLOCK + threadIdx.x;
-    if (r >= s.n) return;
-    y[r] = sc * finalize_row<T>(s, f, a, r);
-    if (qout) qout[r] = sc * f[r];
+    const bool in = r < n;
+    double yr = 0.0, qr = 0.0;
+    if (in) {
+        yr = sc * finalize_row<T>(s, f, a, r);
+        y[r] = yr;
+        qr = sc * f[r];
+        if (qout) qout[r] = qr;
+    }
+    const int64_t rc = in ? r : 0;
+    for (int q0 = 0; q0 < m; q0 += LZD_CH) {
+        const int mc = min(LZD_CH, m - q0);
+#pragma unroll 8
+        for (int k = 0; k < mc; k++) {
+            const int q = q0 + k;
+            const double qv = q == m - 1 ? qr : Q[(int64_t)q * n + rc];
+            lds[k * LZD_T + threadIdx.x] = in ? qv * yr : 0.0;
+        }
+        __syncthreads();
+        lzd_store(lds, mc, red1 + (int64_t)q0 * nb, nb);
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------- carried tile
@@ -1397,22 +1421,24 @@ ps_err mob_apply(Ctx* c, const double* f, double* u, int prec) {
     return launch_apply<double>(c, c->xm, c->aw64, f, u, 0);
 }
 
-// The Lanczos iteration's apply: y = M (f / beta) and q = f / beta with beta^2 the sum of the nb2
-// partials in red2 (stored to *beta2_out), or y = M f when red2 is null (k_finalize_lz).
+// The Lanczos iteration's apply: y = M (f / beta) and q = f / beta with beta^2 the sum of the nb
+// partials in red3 (stored to *beta2_out), or y = M f when red3 is null, then the first Gram-Schmidt
+// pass's partial dots against Q_0..Q_{m-1} into red1 (k_finalize_lz; nb = ceil(n / 256)).
 template <class T>
 static ps_err apply_lanczos(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double* y, double* qout,
-                            const double* red2, int nb2, double* beta2_out) {
+                            const double* red3, int nb, double* beta2_out, int m, double* red1) {
     PS_TRY(launch_apply<T>(c, xm, aw, f, nullptr, 4));
     const unsigned g = (unsigned)((c->n + PS_BLOCK - 1) / PS_BLOCK);
-    k_finalize_lz<T><<<g, PS_BLOCK, 0, c->stream>>>(c->sc, f, y, fin_args<T>(c, aw), red2, nb2, beta2_out, qout);
+    k_finalize_lz<T><<<g, PS_BLOCK, 0, c->stream>>>(c->sc, f, y, fin_args<T>(c, aw), red3, nb, beta2_out, qout, c->Q,
+                                                    m, red1);
     PS_LAUNCH(c);
     return PS_OK;
 }
-ps_err mob_apply_lanczos(Ctx* c, const double* f, double* y, double* qout, const double* red2, int nb2,
-                         double* beta2_out, int prec) {
+ps_err mob_apply_lanczos(Ctx* c, const double* f, double* y, double* qout, const double* red3, int nb,
+                         double* beta2_out, int m, double* red1, int prec) {
     if (prec == PS_PREC_MIXED && c->xm32)
-        return apply_lanczos<float>(c, c->xm32, c->aw32, f, y, qout, red2, nb2, beta2_out);
-    return apply_lanczos<double>(c, c->xm, c->aw64, f, y, qout, red2, nb2, beta2_out);
+        return apply_lanczos<float>(c, c->xm32, c->aw32, f, y, qout, red3, nb, beta2_out, m, red1);
+    return apply_lanczos<double>(c, c->xm, c->aw64, f, y, qout, red3, nb, beta2_out, m, red1);
 }
 
 ps_err mob_apply_stresslet_rows(Ctx* c, const double* f, double* u) {

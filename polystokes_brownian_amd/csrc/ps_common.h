@@ -126,6 +126,7 @@ struct Ctx {
     double* lz_h = nullptr;         // pinned host mirror
     double *ir_r = nullptr, *ir_d = nullptr;   // refinement residual / correction (ns)
     int kmax = 0;
+    double* lzred = nullptr;        // Lanczos Gram-Schmidt partials: [64][nb] per pass (two), then [nb] |w|^2
     double* lzg = nullptr;          // Lanczos square root weights of every m (64 x 64), then per m the
                                     // negative Ritz value count (64)
     double* lzinfo = nullptr;       // [4]: m, negative Ritz values, relative change, beta0 of the last one
@@ -279,10 +280,11 @@ ps_err mss_joint(Ctx* c, int nsys, const double* const* A, const double* const* 
 ps_err sqrt_recurrence_dev(Ctx* c, const double* xi);
 ps_err sqrt_finish_dev(Ctx* c, double* out, ps_sqrt_info* info);
 ps_err mob_apply(Ctx* c, const double* f, double* u, int prec);    // u = M f (device pointers)
-// The Lanczos iteration's apply: y = M (f / beta), q = f / beta, beta^2 = the sum of nb2 partials in
-// red2 (stored to *beta2_out); with red2 null, y = M f and q is not written.
-ps_err mob_apply_lanczos(Ctx* c, const double* f, double* y, double* qout, const double* red2, int nb2,
-                         double* beta2_out, int prec);
+// The Lanczos iteration's apply: y = M (f / beta), q = f / beta, beta^2 = the sum of the nb partials in
+// red3 (stored to *beta2_out); with red3 null, y = M f and q is not written.  Then the partial dots
+// red1[k nb + block] of Q_k . y for k < m (Q_{m-1} = q), nb = ceil(n / 256).
+ps_err mob_apply_lanczos(Ctx* c, const double* f, double* y, double* qout, const double* red3, int nb,
+                         double* beta2_out, int m, double* red1, int prec);
 ps_err saddle_apply(Ctx* c, const double* x, double* y, int prec); // y = A x (device pointers)
 ps_err mob_apply_stresslet_rows(Ctx* c, const double* f, double* u); // u[nu:n] = (M f)[nu:n], FP64
 ps_err mob_apply_colloid_rows(Ctx* c, const double* f, double* u);   // u[nm3:n] = (M f)[nm3:n], FP64
