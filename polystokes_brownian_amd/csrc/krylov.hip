@@ -26,6 +26,7 @@ constexpr int VB = PS_BLOCK;   // threads per vector block
 constexpr int VPT = 4;         // elements per thread
 constexpr int VCHUNK = VB * VPT;
 constexpr int MS_NSYS = 4;     // stresslet systems one launch of the wide MINRES can carry
+constexpr int GB_SYS = 32;     // unsigned words between two systems' barrier words (128 bytes)
 
 inline int nblocks(int64_t n) { return (int)std::max<int64_t>(1, (n + VCHUNK - 1) / VCHUNK); }
 
@@ -353,8 +354,8 @@ ps_err krylov_alloc(Ctx* c) {
     PS_HIP(c, hipMalloc(&c->dcnt, sizeof(int64_t) * DC_N));
     PS_HIP(c, hipMemset(c->dcnt, 0, sizeof(int64_t) * DC_N));
     PS_HIP(c, hipHostMalloc(&c->dcnt_h, sizeof(int64_t) * DC_N, hipHostMallocDefault));
-    PS_HIP(c, hipMalloc(&c->gbar, sizeof(unsigned) * 4));
-    PS_HIP(c, hipMemset(c->gbar, 0, sizeof(unsigned) * 4));
+    PS_HIP(c, hipMalloc(&c->gbar, sizeof(unsigned) * GB_SYS * (1 + MS_NSYS)));
+    PS_HIP(c, hipMemset(c->gbar, 0, sizeof(unsigned) * GB_SYS * (1 + MS_NSYS)));
     PS_HIP(c, hipMalloc(&c->ir_r, sizeof(double) * (size_t)ns));
     PS_HIP(c, hipMalloc(&c->ir_d, sizeof(double) * (size_t)ns));
     if (!c->arrow) {   // the arrowhead diagonal is taken at the first MINRES solve (arrow.hip)
@@ -547,29 +548,31 @@ struct MsSet {
     double *r1, *r2, *y, *v, *wa, *wb;   // wa = w_{k-1}, wb = w_{k-2}; y = A v
 };
 
-// Barrier over the whole grid (every block co-resident: the grid is at most one block per CU).  Each
-// wave waits for its own stores, the block meets, one thread arrives with an agent-scope release and
-// waits for the generation to move on, bounded by the wall clock; a timeout sets the sticky error
-// word, which ends every other block's wait at once.  Returns false after a timeout.
-__device__ bool grid_sync(unsigned* gb, unsigned nblocks, unsigned& gen) {
+// Barrier over nblocks co-resident blocks (the grid is at most one block per CU): the whole grid, or
+// the blocks of one system.  bw[0] counts arrivals, bw[1] is the generation, *err the sticky error
+// word shared by every barrier of the context.  Each wave waits for its own stores, the block meets,
+// one thread arrives with an agent-scope release and waits for the generation to move on, bounded by
+// the wall clock; a timeout sets the error word, which ends every other block's wait at once.
+// Returns false after a timeout.
+__device__ bool grid_sync(unsigned* bw, unsigned* err, unsigned nblocks, unsigned& gen) {
     __shared__ int ok_sh;
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
         const unsigned target = gen + 1;
-        if (__hip_atomic_fetch_add(&gb[0], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == nblocks - 1) {
-            __hip_atomic_store(&gb[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&gb[1], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (__hip_atomic_fetch_add(&bw[0], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == nblocks - 1) {
+            __hip_atomic_store(&bw[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bw[1], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const uint64_t t0 = wall_clock64();
-            while ((int)(__hip_atomic_load(&gb[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
-                if (__hip_atomic_load(&gb[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            while ((int)(__hip_atomic_load(&bw[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
                     ok = 0;
                     break;
                 }
                 if (wall_clock64() - t0 > GRID_WAIT_TICKS) {
-                    __hip_atomic_store(&gb[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     ok = 0;
                     break;
                 }
@@ -667,7 +670,7 @@ __global__ __launch_bounds__(VB) void k_ms_solve(int64_t N, const double* __rest
     bool ok = true;
     if (!s.done) {
         ms_gemv_rows(N, A, V, rb, re, set0.y);
-        ok = grid_sync(gb, nbk, gen);
+        ok = grid_sync(gb, gb + 2, nbk, gen);
     }
     int par = 0;
     while (ok && !s.done) {
@@ -754,7 +757,7 @@ __global__ __launch_bounds__(VB) void k_ms_solve(int64_t N, const double* __rest
         par ^= 1;
         if (s.done) break;
         ms_gemv_rows(N, A, V, rb, re, nxt.y);
-        ok = grid_sync(gb, nbk, gen);
+        ok = grid_sync(gb, gb + 2, nbk, gen);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (!ok) {
@@ -773,10 +776,11 @@ __global__ __launch_bounds__(VB) void k_ms_solve(int64_t N, const double* __rest
 // of its system's whole vector in LDS (computed redundantly, so identical in every block of the
 // system) and R = 1024 / TPR of its rows of A in registers: TPR threads per row, 32 columns a
 // thread.  The only data crossing blocks is y = A v: each block writes its rows and, after the
-// grid-wide barrier, reads its system's whole vector once.  The barrier also tells every block
-// whether any system is still iterating, so all blocks leave together once the last one converges;
-// a system solved earlier sits out the remaining barriers.  With one system this is the single
-// solve; with several, one launch costs the slowest system's iterations instead of their sum.
+// barrier among its system's blocks (each system its own barrier words, GB_SYS words apart from
+// gb + GB_SYS), reads its system's whole vector once.  Every block of a system runs the same
+// recurrence on the same data, so all of them see convergence at the same iteration and leave
+// together; a system does not wait for the others.  With one system this is the single solve; with
+// several, one launch costs the slowest system's iterations instead of their sum.
 constexpr int MW = 1024, MW_COLS = 32;
 
 struct MsSys {
@@ -803,48 +807,6 @@ __device__ __forceinline__ double block_reduce_w(double v, double* sh) {
     return ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
 }
 
-// grid_sync that also counts the blocks still iterating: each arrival adds 1 to the low half of the
-// arrival word and `active` to the high half; the last one publishes the count in gb[3] before
-// releasing the generation.  `any` is set to whether any block arrived active.
-__device__ bool grid_sync_any(unsigned* gb, unsigned nblocks, unsigned& gen, bool active, bool& any) {
-    __shared__ int ok_sh, any_sh;
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int ok = 1;
-        const unsigned target = gen + 1;
-        const unsigned add = 1u + (active ? 0x10000u : 0u);
-        const unsigned old = __hip_atomic_fetch_add(&gb[0], add, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        if ((old & 0xffffu) == nblocks - 1) {
-            const unsigned nact = (old >> 16) + (active ? 1u : 0u);
-            __hip_atomic_store(&gb[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&gb[3], nact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&gb[1], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            const uint64_t t0 = wall_clock64();
-            while ((int)(__hip_atomic_load(&gb[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
-                if (__hip_atomic_load(&gb[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-                    ok = 0;
-                    break;
-                }
-                if (wall_clock64() - t0 > GRID_WAIT_TICKS) {
-                    __hip_atomic_store(&gb[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    ok = 0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        ok_sh = ok;
-        any_sh = __hip_atomic_load(&gb[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
-    }
-    __syncthreads();
-    gen++;
-    any = any_sh != 0;
-    return ok_sh != 0;
-}
-
 template <int TPR>
 __global__ __launch_bounds__(MW) void k_ms_wide(int N, MsBatch bt, int nbs, double rtol, double abstol, int maxit,
                                                 unsigned* gb, int64_t* __restrict__ dcnt) {
@@ -857,9 +819,9 @@ __global__ __launch_bounds__(MW) void k_ms_wide(int N, MsBatch bt, int nbs, doub
     double* y = lds + 4 * N;    // [N]: A v_k - f r1, then r2_new
     __shared__ double sh[16];
     if (__hip_atomic_load(&gb[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
-    unsigned gen = __hip_atomic_load(&gb[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned nbk = gridDim.x;
     const int sys = blockIdx.x / nbs, bs = blockIdx.x - sys * nbs;
+    unsigned* bw = gb + GB_SYS * (1 + sys);
+    unsigned gen = __hip_atomic_load(&bw[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const MsSys S = bt.s[sys];
     const int t = threadIdx.x, q = t / TPR, tl = t % TPR;
     const int rb = bs * R;
@@ -919,93 +881,95 @@ __global__ __launch_bounds__(MW) void k_ms_wide(int N, MsBatch bt, int nbs, doub
     double wa = 0.0, wb = 0.0, xo = 0.0;
     __syncthreads();
     int par = 0;
-    if (!s.done) gemv(S.ybuf);
-    bool any = true;
-    bool ok = grid_sync_any(gb, nbk, gen, !s.done, any);
-    while (ok && any) {
-        if (!s.done) {
-            const double* yk = S.ybuf + (par ? N : 0);
-            double* ynext = S.ybuf + (par ? 0 : N);
-            // y -= (beta/oldb) r1 (from the 2nd iteration); alfa = v . y
-            const bool sub = s.its >= 1;
-            const double f = sub ? s.beta / s.oldb : 0.0;
-            acc = 0.0;
-            for (int i = t; i < N; i += MW) {
-                double ya = yk[i];
-                if (sub) ya -= f * r1[i];
-                y[i] = ya;
-                acc += v[i] * ya;
-            }
-            const double alfa = block_reduce_w(acc, sh);
-            // r2_new = y - (alfa/beta) r2; z = P^-1 r2_new per colloid block; beta_new^2 = r2_new . z
-            const double f2 = alfa / s.beta;
-            acc = 0.0;
-            for (int cb = t; cb < ncb; cb += MW) {
-                double r[5], zz[5];
-#pragma unroll
-                for (int k = 0; k < 5; k++) {
-                    const int i = 5 * cb + k;
-                    r[k] = y[i] - f2 * r2[i];
-                }
-                ms_prec5(r, zz);
-#pragma unroll
-                for (int k = 0; k < 5; k++) {
-                    const int i = 5 * cb + k;
-                    y[i] = r[k];   // r2_new (y' is dead once read)
-                    z[i] = zz[k];
-                    acc += r[k] * zz[k];
-                }
-            }
-            const double bb = block_reduce_w(acc, sh);
-            s.alfa = alfa;
-            s.oldb = s.beta;
-            s.beta = sqrt(fmax(bb, 0.0));
-            const double oldeps = s.epsln;
-            const double delta = s.cs * s.dbar + s.sn * alfa;
-            const double gbar = s.sn * s.dbar - s.cs * alfa;
-            s.epsln = s.sn * s.beta;
-            s.dbar = -s.cs * s.beta;
-            const double gamma = fmax(sqrt(gbar * gbar + s.beta * s.beta), 1e-300);
-            s.cs = gbar / gamma;
-            s.sn = s.beta / gamma;
-            const double phi = s.cs * s.phibar;
-            s.phibar = s.sn * s.phibar;
-            s.oldeps = oldeps;
-            s.delta = delta;
-            s.gamma = gamma;
-            s.phi = phi;
-            s.its += 1;
-            if (s.phibar <= s.tol) {
-                s.done = 1;
-                s.reason = (s.phibar <= abstol) ? 3 : 2;
-            } else if (s.beta == 0.0) {
-                s.done = 1;
-                s.reason = 2;
-            } else if (s.its >= maxit) {
-                s.done = 1;
-                s.reason = -3;
-            }
-            const double denom = 1.0 / gamma;
-            const double vs = (s.beta > 0.0) ? 1.0 / s.beta : 0.0;
-            // own rows: w_k = (v - oldeps w_{k-2} - delta w_{k-1}) / gamma, x += phi w_k
-            if (t < nr) {
-                const double wn = (v[rb + t] - oldeps * wb - delta * wa) * denom;
-                wb = wa;
-                wa = wn;
-                xo += phi * wn;
-            }
-            __syncthreads();   // every read of v and r2 of this iteration is done
-            // shift: r1 <- r2, r2 <- r2_new, v <- z / beta_new (each thread its own entries)
-            for (int i = t; i < N; i += MW) {
-                r1[i] = r2[i];
-                r2[i] = y[i];
-                v[i] = z[i] * vs;
-            }
-            __syncthreads();
-            par ^= 1;
-            if (!s.done) gemv(ynext);
+    bool ok = true;
+    if (!s.done) {
+        gemv(S.ybuf);
+        ok = grid_sync(bw, gb + 2, (unsigned)nbs, gen);
+    }
+    while (ok && !s.done) {
+        const double* yk = S.ybuf + (par ? N : 0);
+        double* ynext = S.ybuf + (par ? 0 : N);
+        // y -= (beta/oldb) r1 (from the 2nd iteration); alfa = v . y
+        const bool sub = s.its >= 1;
+        const double f = sub ? s.beta / s.oldb : 0.0;
+        acc = 0.0;
+        for (int i = t; i < N; i += MW) {
+            double ya = yk[i];
+            if (sub) ya -= f * r1[i];
+            y[i] = ya;
+            acc += v[i] * ya;
         }
-        ok = grid_sync_any(gb, nbk, gen, !s.done, any);
+        const double alfa = block_reduce_w(acc, sh);
+        // r2_new = y - (alfa/beta) r2; z = P^-1 r2_new per colloid block; beta_new^2 = r2_new . z
+        const double f2 = alfa / s.beta;
+        acc = 0.0;
+        for (int cb = t; cb < ncb; cb += MW) {
+            double r[5], zz[5];
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                const int i = 5 * cb + k;
+                r[k] = y[i] - f2 * r2[i];
+            }
+            ms_prec5(r, zz);
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                const int i = 5 * cb + k;
+                y[i] = r[k];   // r2_new (y' is dead once read)
+                z[i] = zz[k];
+                acc += r[k] * zz[k];
+            }
+        }
+        const double bb = block_reduce_w(acc, sh);
+        s.alfa = alfa;
+        s.oldb = s.beta;
+        s.beta = sqrt(fmax(bb, 0.0));
+        const double oldeps = s.epsln;
+        const double delta = s.cs * s.dbar + s.sn * alfa;
+        const double gbar = s.sn * s.dbar - s.cs * alfa;
+        s.epsln = s.sn * s.beta;
+        s.dbar = -s.cs * s.beta;
+        const double gamma = fmax(sqrt(gbar * gbar + s.beta * s.beta), 1e-300);
+        s.cs = gbar / gamma;
+        s.sn = s.beta / gamma;
+        const double phi = s.cs * s.phibar;
+        s.phibar = s.sn * s.phibar;
+        s.oldeps = oldeps;
+        s.delta = delta;
+        s.gamma = gamma;
+        s.phi = phi;
+        s.its += 1;
+        if (s.phibar <= s.tol) {
+            s.done = 1;
+            s.reason = (s.phibar <= abstol) ? 3 : 2;
+        } else if (s.beta == 0.0) {
+            s.done = 1;
+            s.reason = 2;
+        } else if (s.its >= maxit) {
+            s.done = 1;
+            s.reason = -3;
+        }
+        const double denom = 1.0 / gamma;
+        const double vs = (s.beta > 0.0) ? 1.0 / s.beta : 0.0;
+        // own rows: w_k = (v - oldeps w_{k-2} - delta w_{k-1}) / gamma, x += phi w_k
+        if (t < nr) {
+            const double wn = (v[rb + t] - oldeps * wb - delta * wa) * denom;
+            wb = wa;
+            wa = wn;
+            xo += phi * wn;
+        }
+        __syncthreads();   // every read of v and r2 of this iteration is done
+        // shift: r1 <- r2, r2 <- r2_new, v <- z / beta_new (each thread its own entries)
+        for (int i = t; i < N; i += MW) {
+            r1[i] = r2[i];
+            r2[i] = y[i];
+            v[i] = z[i] * vs;
+        }
+        __syncthreads();
+        par ^= 1;
+        if (!s.done) {
+            gemv(ynext);
+            ok = grid_sync(bw, gb + 2, (unsigned)nbs, gen);
+        }
     }
     if (t < nr) S.x[rb + t] = xo;
     if (bs == 0 && t == 0) {
@@ -1396,7 +1360,7 @@ __global__ __launch_bounds__(64) void k_lz_ritz(const double* __restrict__ lz, i
     }
     double dv = lane < m ? lz[lane] : 0.0;
     double ev = (lane < m - 1) ? sqrt(fmax(lz[LZ_BETA2 + lane], 0.0)) : 0.0;
-    tql2_wave(m, dv, ev, Z);
+    tql2_wave<64>(m, dv, ev, Z);
     if (lane < m) d[lane] = dv;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     double gi = 0.0;
@@ -1404,7 +1368,7 @@ __global__ __launch_bounds__(64) void k_lz_ritz(const double* __restrict__ lz, i
         for (int jj = 0; jj < m; jj++) {
             double th = d[jj];
             if (th < 0.0) th = 0.0;
-            gi += Z[lane + jj * m] * sqrt(th) * Z[0 + jj * m];
+            gi += Z[lane + jj * 64] * sqrt(th) * Z[0 + jj * 64];
         }
     int nneg = 0;
     for (int jj = 0; jj < m; jj++)

@@ -133,7 +133,8 @@ struct Ctx {
     // ---- device-side counters and flags (DC_*), read by ps_get_counters and after ps_run
     int64_t* dcnt = nullptr;
     int64_t* dcnt_h = nullptr;      // pinned
-    unsigned* gbar = nullptr;       // grid barrier of the persistent stresslet MINRES: count, generation, error
+    unsigned* gbar = nullptr;       // grid barriers of the persistent MINRES kernels: [0] count, [1] generation,
+                                    // [2] error; per wide-MINRES system s, count and generation at 32 (1 + s)
     // ---- block-elimination solver (PS_SOLVER_BLOCK)
     double* Mss = nullptr;          // 5Nc x 5Nc stresslet block, rebuilt by mob()
     double* bs_y = nullptr;         // n

@@ -7,10 +7,11 @@
 //
 // Layout: lane k holds d[k] and e[k] in registers and row k of Z (column-major m x m in LDS,
 // Z[k + col m]).  The QL sweep is one sequential chain of plane rotations; every lane runs it on
-// scalars read from the owning lanes (readlane, no LDS round trip and no fence per rotation) and
-// applies each rotation to its own row of Z.  A rotation's radius comes from a reciprocal square
-// root refined by two Newton steps (one chain of multiply-adds instead of a square root and two
-// divisions).  m <= 64.
+// scalars read from the owning lanes (readlane, no LDS round trip and no fence per rotation), and
+// lane i keeps rotation i's cosine and sine.  After the sweep each lane applies the sweep's rotations
+// to its own row of Z in the same order, so the chain never waits for LDS and the row updates'
+// loads do not depend on it.  A rotation's radius comes from a reciprocal square root refined by two
+// Newton steps (one chain of multiply-adds instead of a square root and two divisions).  m <= 64.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -35,12 +36,22 @@ __device__ __forceinline__ double tql2_rsqrt(double t) {
 
 // Eigenvalues of T (diagonal dv, off-diagonal ev of lane k; ev of lane m-1 unused) into dv,
 // eigenvectors into the columns of Z.  Returns false when an eigenvalue needs more than 200 QL
-// iterations (T is then left partly reduced).
+// iterations (T is then left partly reduced).  LDZ = 0: Z is m x m (column stride m), lanes >= m
+// idle in the row updates.  LDZ = 64: column stride 64 (Z holds 64 m doubles), every lane updates its
+// own row and rows >= m carry values no one reads, so the row updates need no lane mask.
+template <int LDZ = 0>
 __device__ inline bool tql2_wave(int m, double& dv, double& ev, double* Z) {
+    static_assert(LDZ == 0 || LDZ == 64, "column stride m or 64");
     const int lane = threadIdx.x & 63;
-    for (int k = lane; k < m * m; k += 64) Z[k] = 0.0;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (lane < m) Z[lane + lane * m] = 1.0;
+    const int ld = LDZ ? LDZ : m;
+    const bool own = LDZ ? true : lane < m;
+    if (LDZ) {
+        for (int col = 0; col < m; col++) Z[lane + col * ld] = lane == col ? 1.0 : 0.0;
+    } else {
+        for (int k = lane; k < m * m; k += 64) Z[k] = 0.0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (lane < m) Z[lane + lane * m] = 1.0;
+    }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (lane >= m - 1) ev = 0.0;
     for (int l = 0; l < m; l++) {
@@ -55,50 +66,115 @@ __device__ inline bool tql2_wave(int m, double& dv, double& ev, double* Z) {
             if (mm != l) {
                 if (iter++ == 200) return false;
                 const double dl = tql2_rl(dv, l), el = tql2_rl(ev, l);
-                double g = (tql2_rl(dv, l + 1) - dl) / (2.0 * el);
-                double r = sqrt(g * g + 1.0);
-                g = tql2_rl(dv, mm) - dl + el / (g + (g >= 0 ? fabs(r) : -fabs(r)));
-                double s = 1.0, c = 1.0, p = 0.0;
+                double g0 = (tql2_rl(dv, l + 1) - dl) / (2.0 * el);
+                const double r0 = sqrt(g0 * g0 + 1.0);
+                const double dmm = tql2_rl(dv, mm);   // d[mm] before this sweep changes it
+                g0 = dmm - dl + el / (g0 + (g0 >= 0 ? fabs(r0) : -fabs(r0)));
+                const double dv0 = dv, ev0 = ev;
+                double cr = 1.0, sr = 0.0;   // lane i: rotation i of this sweep
+                double p = 0.0, g = g0;
+                int ilast = l;
                 bool underflow = false;
-                double dip1 = tql2_rl(dv, mm);   // d[i + 1] before this sweep changes it
-                // column i + 1 of Z, carried in a register from one rotation to the next: each rotation
-                // reads only column i, which this sweep has not written yet
-                double zc = lane < m ? Z[lane + mm * m] : 0.0;
-                int ilast = mm;
-                for (int i = mm - 1; i >= l; i--) {
-                    const double ei = tql2_rl(ev, i), di = tql2_rl(dv, i);
-                    const double f = s * ei, b = c * ei;
-                    const double t = __builtin_fma(f, f, g * g);
-                    if (t == 0.0) {   // both zero: the rotation is the identity, deflate here
+                // The sweep without the test for a vanishing rotation (t = 0), which only records
+                // whether one occurred: then d and e are restored and the sweep is run again with the
+                // test, deflating at the first such rotation.  d[i] and e[i] are read from lane i,
+                // which this sweep writes only at rotation i - 1.
+                {
+                    double s = 1.0, c = 1.0, dip1 = dmm;
+                    bool zero = false;
+                    for (int i = mm - 1; i >= l; i--) {
+                        const double ei = tql2_rl(ev, i), di = tql2_rl(dv, i);
+                        const double f = s * ei, b = c * ei;
+                        const double t = __builtin_fma(f, f, g * g);
+                        zero = zero || t == 0.0;
+                        const double y = tql2_rsqrt(t);
+                        const double r = t * y;
+                        s = f * y;
+                        c = g * y;
+                        g = dip1 - p;
+                        const double rr = (di - g) * s + 2.0 * c * b;
+                        p = s * rr;
                         if (lane == i + 1) {
-                            ev = 0.0;
-                            dv = dip1 - p;
+                            ev = r;
+                            dv = g + p;
                         }
-                        if (lane == mm) ev = 0.0;
-                        underflow = true;
-                        break;
+                        if (lane == i) {
+                            cr = c;
+                            sr = s;
+                        }
+                        g = c * rr - b;
+                        dip1 = di;
                     }
-                    ilast = i;
-                    const double y = tql2_rsqrt(t);
-                    r = t * y;
-                    s = f * y;
-                    c = g * y;
-                    g = dip1 - p;
-                    const double rr = (di - g) * s + 2.0 * c * b;
-                    p = s * rr;
-                    if (lane == i + 1) {
-                        ev = r;
-                        dv = g + p;
-                    }
-                    g = c * rr - b;
-                    dip1 = di;
-                    if (lane < m) {
-                        const double zi = Z[lane + i * m];
-                        Z[lane + (i + 1) * m] = s * zi + c * zc;
-                        zc = c * zi - s * zc;
+                    // t is the same in every lane: a uniform branch
+                    if (__builtin_amdgcn_readfirstlane((int)zero)) {
+                        dv = dv0;
+                        ev = ev0;
+                        s = 1.0;
+                        c = 1.0;
+                        p = 0.0;
+                        g = g0;
+                        dip1 = dmm;
+                        ilast = mm;
+                        for (int i = mm - 1; i >= l; i--) {
+                            const double ei = tql2_rl(ev, i), di = tql2_rl(dv, i);
+                            const double f = s * ei, b = c * ei;
+                            const double t = __builtin_fma(f, f, g * g);
+                            if (__builtin_amdgcn_readfirstlane((int)(t == 0.0))) {
+                                // both zero: the rotation is the identity, deflate here
+                                if (lane == i + 1) {
+                                    ev = 0.0;
+                                    dv = dip1 - p;
+                                }
+                                if (lane == mm) ev = 0.0;
+                                underflow = true;
+                                break;
+                            }
+                            ilast = i;
+                            const double y = tql2_rsqrt(t);
+                            const double r = t * y;
+                            s = f * y;
+                            c = g * y;
+                            g = dip1 - p;
+                            const double rr = (di - g) * s + 2.0 * c * b;
+                            p = s * rr;
+                            if (lane == i + 1) {
+                                ev = r;
+                                dv = g + p;
+                            }
+                            if (lane == i) {
+                                cr = c;
+                                sr = s;
+                            }
+                            g = c * rr - b;
+                            dip1 = di;
+                        }
                     }
                 }
-                if (lane < m) Z[lane + ilast * m] = zc;   // the column the last rotation left in zc
+                // the rotations mm - 1 .. ilast on each lane's row of Z: column i + 1 is carried in zc
+                // from one rotation to the next (each reads column i, not yet written by this pass);
+                // column i - 1 is loaded one rotation ahead, into za and zb in turn
+                {
+                    double zc = own ? Z[lane + mm * ld] : 0.0;
+                    double za = (own && ilast < mm) ? Z[lane + (mm - 1) * ld] : 0.0, zb;
+                    int i = mm - 1;
+                    while (i >= ilast) {
+                        zb = own ? Z[lane + max(i - 1, ilast) * ld] : 0.0;
+                        {
+                            const double ci = tql2_rl(cr, i), si = tql2_rl(sr, i);
+                            if (own) Z[lane + (i + 1) * ld] = si * za + ci * zc;
+                            zc = ci * za - si * zc;
+                        }
+                        if (--i < ilast) break;
+                        za = own ? Z[lane + max(i - 1, ilast) * ld] : 0.0;
+                        {
+                            const double ci = tql2_rl(cr, i), si = tql2_rl(sr, i);
+                            if (own) Z[lane + (i + 1) * ld] = si * zb + ci * zc;
+                            zc = ci * zb - si * zc;
+                        }
+                        --i;
+                    }
+                    if (own) Z[lane + ilast * ld] = zc;   // the column the last rotation left in zc
+                }
                 if (underflow) continue;
                 if (lane == l) {
                     dv -= p;
