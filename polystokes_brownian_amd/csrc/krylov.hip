@@ -782,6 +782,10 @@ __global__ __launch_bounds__(VB) void k_ms_solve(int64_t N, const double* __rest
 // together; a system does not wait for the others.  With one system this is the single solve; with
 // several, one launch costs the slowest system's iterations instead of their sum.
 constexpr int MW = 1024, MW_COLS = 32;
+// Of a thread's MW_COLS columns of A, the last ms_ldsc(TPR) live in LDS, which leaves the registers
+// room for the recurrence without spills (the LDS then holds at most 4 N + TPR MW_COLS + that many
+// times MW doubles: 152 KiB at N = 2048).
+constexpr int ms_ldsc(int tpr) { return tpr == 32 ? 10 : 8; }
 
 struct MsSys {
     const double* A;    // N x N, row-major
@@ -795,29 +799,49 @@ struct MsBatch {
 };
 
 // Deterministic sum over the 16 waves of a 1024-thread block (fixed tree), broadcast to all threads.
-__device__ __forceinline__ double block_reduce_w(double v, double* sh) {
+// sh holds two slots of 16; consecutive calls alternate slots (`slot`), so one block barrier per call
+// suffices: a slot is written again only after the next call's barrier, which every reader of its
+// previous contents has passed.
+__device__ __forceinline__ double block_reduce_w(double v, double (*sh)[16], int slot) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     v = wave_sum(v);
-    __syncthreads();
-    if (lane == 0) sh[wid] = v;
+    if (lane == 0) sh[slot][wid] = v;
     __syncthreads();
     double r[8];
 #pragma unroll
-    for (int k = 0; k < 8; k++) r[k] = sh[2 * k] + sh[2 * k + 1];
+    for (int k = 0; k < 8; k++) r[k] = sh[slot][2 * k] + sh[slot][2 * k + 1];
     return ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+}
+
+// Entry k = i mod 5 of the stresslet self-block preconditioner z = P^-1 r (ms_prec5), one entry per
+// thread: rp is the entry it couples to (i + 4 for k = 0, i - 4 for k = 4; unused otherwise).
+__device__ __forceinline__ double ms_prec_el(int k, double r, double rp) {
+    const double pa = 1.8 / 2.43, pb = -0.9 / 2.43;
+    if (k == 0) return pa * r + pb * rp;
+    if (k == 4) return pb * rp + pa * r;
+    return r * (5.0 / 9.0);
 }
 
 template <int TPR>
 __global__ __launch_bounds__(MW) void k_ms_wide(int N, MsBatch bt, int nbs, double rtol, double abstol, int maxit,
                                                 unsigned* gb, int64_t* __restrict__ dcnt) {
-    constexpr int R = MW / TPR;   // rows per block
+    constexpr int R = MW / TPR;             // rows per block
+    constexpr int NV = TPR * MW_COLS;       // columns the gemv threads cover (v is zero beyond N)
+    constexpr int NE = NV / MW;             // vector entries per thread (t, t + MW, ...)
     extern __shared__ double lds[];
+    // residuals r1 (previous) and r2 (current) live in two buffers that trade places every iteration:
+    // the new residual is written over r1, which the iteration has read by then
     double* r1 = lds;           // [N]
     double* r2 = lds + N;       // [N]
-    double* v = lds + 2 * N;    // [N]
-    double* z = lds + 3 * N;    // [N]: z = P^-1 r2_new, then v_{k+1} before scaling
-    double* y = lds + 4 * N;    // [N]: A v_k - f r1, then r2_new
-    __shared__ double sh[16];
+    double* z = lds + 2 * N;    // [N]: P^-1 r2_new
+    double* y = lds + 3 * N;    // [N]: A v_k - f r1
+    double* v = lds + 4 * N;    // [NV]: v_k
+    double* Al = v + NV;        // [ms_ldsc(TPR)][MW]: the last columns of each thread's row of A
+    __shared__ double sh[2][16];
+    // the recurrence's scalars, the same in every thread: two slots, the current one read by all
+    // threads and the next one written by thread 0 (every thread computes it; only registers for the
+    // span of the update)
+    __shared__ MinresState ss[2];
     if (__hip_atomic_load(&gb[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     const int sys = blockIdx.x / nbs, bs = blockIdx.x - sys * nbs;
     unsigned* bw = gb + GB_SYS * (1 + sys);
@@ -826,100 +850,134 @@ __global__ __launch_bounds__(MW) void k_ms_wide(int N, MsBatch bt, int nbs, doub
     const int t = threadIdx.x, q = t / TPR, tl = t % TPR;
     const int rb = bs * R;
     const int nr = max(0, min(R, N - rb));
-    const int ncb = N / 5;
-    // this block's rows of A, kept in registers for the whole solve (thread (q, tl): row rb + q,
-    // columns tl + k TPR)
-    double Ac[MW_COLS];
+    // this block's rows of A, kept on chip for the whole solve (thread (q, tl): row rb + q, columns
+    // tl + k TPR; k < MW_COLS - ms_ldsc(TPR) in registers, the rest in LDS)
+    constexpr int KR = MW_COLS - ms_ldsc(TPR);
+    double Ac[KR];
 #pragma unroll
     for (int k = 0; k < MW_COLS; k++) {
         const int j = tl + k * TPR;
-        Ac[k] = (q < nr && j < N) ? S.A[(int64_t)(rb + q) * N + j] : 0.0;
+        const double a = (q < nr && j < N) ? S.A[(int64_t)(rb + q) * N + j] : 0.0;
+        if (k < KR)
+            Ac[k] = a;
+        else
+            Al[(k - KR) * MW + t] = a;
     }
+    // y rows = A v: no bounds test per column (v is zero past N, as are the padded entries of A)
     auto gemv = [&](double* __restrict__ yout) {
         double a = 0.0;
 #pragma unroll
-        for (int k = 0; k < MW_COLS; k++) {
-            const int j = tl + k * TPR;
-            if (j < N) a = __builtin_fma(Ac[k], v[j], a);
-        }
+        for (int k = 0; k < KR; k++) a = __builtin_fma(Ac[k], v[tl + k * TPR], a);
+#pragma unroll
+        for (int k = KR; k < MW_COLS; k++) a = __builtin_fma(Al[(k - KR) * MW + t], v[tl + k * TPR], a);
 #pragma unroll
         for (int off = TPR / 2; off > 0; off >>= 1) a += __shfl_xor(a, off, TPR);
         if (tl == 0 && q < nr) yout[rb + q] = a;
     };
+    // each thread owns entries i = t + e MW (e < NE, i < N) of the vectors; the partner of i is the
+    // entry its preconditioner row couples to
+    auto partner = [](int i) {
+        const int k = i % 5;
+        return k == 0 ? i + 4 : (k == 4 ? i - 4 : i);
+    };
     // start (cold): x = 0, r1 = r2 = b, z = P^-1 b, beta1^2 = b . z
     double acc = 0.0;
-    for (int cb = t; cb < ncb; cb += MW) {
-        double r[5], zz[5];
+    double zs[NE];
 #pragma unroll
-        for (int k = 0; k < 5; k++) r[k] = S.b[5 * cb + k];
-        ms_prec5(r, zz);
-#pragma unroll
-        for (int k = 0; k < 5; k++) {
-            r1[5 * cb + k] = r[k];
-            r2[5 * cb + k] = r[k];
-            z[5 * cb + k] = zz[k];
-            acc += r[k] * zz[k];
+    for (int e = 0; e < NE; e++) {
+        const int i = t + e * MW;
+        zs[e] = 0.0;
+        if (i < N) {
+            const double r = S.b[i];
+            zs[e] = ms_prec_el(i % 5, r, S.b[partner(i)]);
+            r1[i] = r;
+            r2[i] = r;
+            acc += r * zs[e];
+        } else {
+            v[i] = 0.0;
         }
     }
-    const double b1 = block_reduce_w(acc, sh);
-    MinresState s{};
-    s.beta1 = b1 > 0.0 ? sqrt(b1) : 0.0;
-    s.tol = fmax(rtol * sqrt(fmax(b1, 0.0)), abstol);
-    s.beta = s.beta1;
-    s.phibar = s.beta1;
-    s.cs = -1.0;
-    if (b1 <= 0.0) {
-        s.done = 1;
-        s.reason = 3;
-    } else if (s.phibar <= s.tol) {
-        s.done = 1;
-        s.reason = 2;
+    int slot = 0;
+    const double b1 = block_reduce_w(acc, sh, slot);
+    slot ^= 1;
+    int sp = 0;
+    bool done;
+    {
+        MinresState s{};
+        s.beta1 = b1 > 0.0 ? sqrt(b1) : 0.0;
+        s.tol = fmax(rtol * sqrt(fmax(b1, 0.0)), abstol);
+        s.beta = s.beta1;
+        s.phibar = s.beta1;
+        s.cs = -1.0;
+        if (b1 <= 0.0) {
+            s.done = 1;
+            s.reason = 3;
+        } else if (s.phibar <= s.tol) {
+            s.done = 1;
+            s.reason = 2;
+        }
+        done = s.done != 0;
+        const double sc0 = done ? 0.0 : 1.0 / s.beta;
+#pragma unroll
+        for (int e = 0; e < NE; e++) {
+            const int i = t + e * MW;
+            if (i < N) v[i] = zs[e] * sc0;
+        }
+        if (t == 0) ss[0] = s;
     }
-    const double sc0 = s.done ? 0.0 : 1.0 / s.beta;
-    for (int i = t; i < N; i += MW) v[i] = z[i] * sc0;
     // own rows: w_{k-1}, w_{k-2}, x (thread t < nr holds row rb + t)
     double wa = 0.0, wb = 0.0, xo = 0.0;
     __syncthreads();
     int par = 0;
     bool ok = true;
-    if (!s.done) {
+    if (!done) {
         gemv(S.ybuf);
         ok = grid_sync(bw, gb + 2, (unsigned)nbs, gen);
     }
-    while (ok && !s.done) {
+    while (ok && !done) {
+        const MinresState& cur = ss[sp];
         const double* yk = S.ybuf + (par ? N : 0);
         double* ynext = S.ybuf + (par ? 0 : N);
-        // y -= (beta/oldb) r1 (from the 2nd iteration); alfa = v . y
-        const bool sub = s.its >= 1;
-        const double f = sub ? s.beta / s.oldb : 0.0;
+        // y -= (beta/oldb) r1 (from the 2nd iteration); alfa = v . y; this thread's row of v_k kept for
+        // the w update
+        const bool sub = cur.its >= 1;
+        const double f = sub ? cur.beta / cur.oldb : 0.0;
+        const double vown = t < nr ? v[rb + t] : 0.0;
         acc = 0.0;
-        for (int i = t; i < N; i += MW) {
-            double ya = yk[i];
-            if (sub) ya -= f * r1[i];
-            y[i] = ya;
-            acc += v[i] * ya;
-        }
-        const double alfa = block_reduce_w(acc, sh);
-        // r2_new = y - (alfa/beta) r2; z = P^-1 r2_new per colloid block; beta_new^2 = r2_new . z
-        const double f2 = alfa / s.beta;
-        acc = 0.0;
-        for (int cb = t; cb < ncb; cb += MW) {
-            double r[5], zz[5];
 #pragma unroll
-            for (int k = 0; k < 5; k++) {
-                const int i = 5 * cb + k;
-                r[k] = y[i] - f2 * r2[i];
-            }
-            ms_prec5(r, zz);
-#pragma unroll
-            for (int k = 0; k < 5; k++) {
-                const int i = 5 * cb + k;
-                y[i] = r[k];   // r2_new (y' is dead once read)
-                z[i] = zz[k];
-                acc += r[k] * zz[k];
+        for (int e = 0; e < NE; e++) {
+            const int i = t + e * MW;
+            if (i < N) {
+                double ya = yk[i];
+                if (sub) ya -= f * r1[i];
+                y[i] = ya;
+                acc += v[i] * ya;
             }
         }
-        const double bb = block_reduce_w(acc, sh);
+        const double alfa = block_reduce_w(acc, sh, slot);
+        slot ^= 1;
+        // r2_new = y - (alfa/beta) r2 (over r1); z = P^-1 r2_new; beta_new^2 = r2_new . z
+        const double f2 = alfa / cur.beta;
+        acc = 0.0;
+#pragma unroll
+        for (int e = 0; e < NE; e++) {
+            const int i = t + e * MW;
+            if (i < N) {
+                const int pi = partner(i);
+                const double r = y[i] - f2 * r2[i];
+                zs[e] = ms_prec_el(i % 5, r, y[pi] - f2 * r2[pi]);
+                r1[i] = r;
+                acc += r * zs[e];
+            }
+        }
+        const double bb = block_reduce_w(acc, sh, slot);
+        slot ^= 1;
+        {
+            double* tmp = r1;   // r1 <- r2, r2 <- r2_new
+            r1 = r2;
+            r2 = tmp;
+        }
+        MinresState s = cur;
         s.alfa = alfa;
         s.oldb = s.beta;
         s.beta = sqrt(fmax(bb, 0.0));
@@ -948,31 +1006,36 @@ __global__ __launch_bounds__(MW) void k_ms_wide(int N, MsBatch bt, int nbs, doub
             s.done = 1;
             s.reason = -3;
         }
+        done = s.done != 0;
+        // the other slot was last read before this iteration's first reduction barrier; it is read
+        // next after the barrier ahead of the gemv (or, once done, by this thread at the end)
+        if (t == 0) ss[sp ^ 1] = s;
+        sp ^= 1;
         const double denom = 1.0 / gamma;
         const double vs = (s.beta > 0.0) ? 1.0 / s.beta : 0.0;
         // own rows: w_k = (v - oldeps w_{k-2} - delta w_{k-1}) / gamma, x += phi w_k
         if (t < nr) {
-            const double wn = (v[rb + t] - oldeps * wb - delta * wa) * denom;
+            const double wn = (vown - oldeps * wb - delta * wa) * denom;
             wb = wa;
             wa = wn;
             xo += phi * wn;
         }
-        __syncthreads();   // every read of v and r2 of this iteration is done
-        // shift: r1 <- r2, r2 <- r2_new, v <- z / beta_new (each thread its own entries)
-        for (int i = t; i < N; i += MW) {
-            r1[i] = r2[i];
-            r2[i] = y[i];
-            v[i] = z[i] * vs;
-        }
-        __syncthreads();
         par ^= 1;
-        if (!s.done) {
+        if (!done) {
+            // v_{k+1} = z / beta_new: every read of v_k was before the last reduction's barrier
+#pragma unroll
+            for (int e = 0; e < NE; e++) {
+                const int i = t + e * MW;
+                if (i < N) v[i] = zs[e] * vs;
+            }
+            __syncthreads();
             gemv(ynext);
             ok = grid_sync(bw, gb + 2, (unsigned)nbs, gen);
         }
     }
     if (t < nr) S.x[rb + t] = xo;
     if (bs == 0 && t == 0) {
+        MinresState s = ss[sp];
         if (!ok) {
             s.done = 1;
             s.reason = -5;   // KSP_DIVERGED_BREAKDOWN's sign: the wait ran out of time
@@ -1003,7 +1066,7 @@ static ps_err ms_wide_launch(Ctx* c, int64_t N, const MsBatch& bt, int ns, doubl
                      " unknowns do not fit one block per CU");
         return PS_ERR_ARG;
     }
-    const size_t lds = sizeof(double) * 5 * (size_t)N;
+    const size_t lds = sizeof(double) * (4 * (size_t)N + (size_t)tpr * MW_COLS + (size_t)ms_ldsc(tpr) * MW);
     const unsigned nb = (unsigned)(nbs * ns);
     if (tpr == 32)
         k_ms_wide<32><<<nb, MW, lds, c->stream>>>((int)N, bt, nbs, rtol, c->p.ksp_abstol, maxit, c->gbar, c->dcnt);
