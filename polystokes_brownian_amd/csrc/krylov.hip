@@ -26,7 +26,7 @@ constexpr int VB = PS_BLOCK;   // threads per vector block
 constexpr int VPT = 4;         // elements per thread
 constexpr int VCHUNK = VB * VPT;
 constexpr int MS_NSYS = 4;     // stresslet systems one launch of the wide MINRES can carry
-constexpr int GB_SYS = 32;     // unsigned words between two systems' barrier words (128 bytes)
+constexpr int GB_SYS = 128;    // unsigned words per system's barrier flags (a flag per block, nbs <= 128)
 
 inline int nblocks(int64_t n) { return (int)std::max<int64_t>(1, (n + VCHUNK - 1) / VCHUNK); }
 
@@ -587,6 +587,46 @@ __device__ bool grid_sync(unsigned* bw, unsigned* err, unsigned nblocks, unsigne
     return ok_sh != 0;
 }
 
+// Barrier among the nblocks (<= GB_SYS) blocks of one system: block `me` publishes the generation it
+// has reached in its own word flags[me] (agent-scope release), and wave 0 waits until every word of the
+// system has reached it, one lane per word, bounded by the wall clock; a timeout sets the sticky error
+// word, which ends every other block's wait at once.  No read-modify-write: no block waits on another's
+// atomic.  Returns false after a timeout.
+__device__ bool flag_sync(unsigned* flags, unsigned* err, int nblocks, int me, unsigned& gen) {
+    __shared__ int ok_sh;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const unsigned target = gen + 1;
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        if (lane == 0) __hip_atomic_store(&flags[me], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        int ok = 1;
+        const uint64_t t0 = wall_clock64();
+        while (true) {
+            bool reached = true;
+            for (int b = lane; b < nblocks; b += 64)
+                reached = reached &&
+                          (int)(__hip_atomic_load(&flags[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) >= 0;
+            if (__all(reached)) break;
+            if (__any(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+                ok = 0;
+                break;
+            }
+            if (wall_clock64() - t0 > GRID_WAIT_TICKS) {
+                if (lane == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (lane == 0) ok_sh = ok;
+    }
+    __syncthreads();
+    gen++;
+    return ok_sh != 0;
+}
+
 __device__ __forceinline__ void ms_prec5(const double (&r)[5], double (&z)[5]) {
     const double pa = 1.8 / 2.43, pb = -0.9 / 2.43;
     z[0] = pa * r[0] + pb * r[4];
@@ -774,10 +814,10 @@ __global__ __launch_bounds__(VB) void k_ms_solve(int64_t N, const double* __rest
 // The same recurrence for blocks of up to 2048 unknowns, several independent systems in one launch
 // (MS_NSYS at most).  Each system gets nbs blocks of 1024 threads; a block keeps r1, r2, v, z and y
 // of its system's whole vector in LDS (computed redundantly, so identical in every block of the
-// system) and R = 1024 / TPR of its rows of A in registers: TPR threads per row, 32 columns a
-// thread.  The only data crossing blocks is y = A v: each block writes its rows and, after the
-// barrier among its system's blocks (each system its own barrier words, GB_SYS words apart from
-// gb + GB_SYS), reads its system's whole vector once.  Every block of a system runs the same
+// system) and R = 1024 / TPR of its rows of A on chip (registers, the last columns in LDS): TPR
+// threads per row, 32 columns a thread.  The only data crossing blocks is y = A v: each block writes
+// its rows and, after the barrier among its system's blocks (flag_sync over the system's GB_SYS flag
+// words from gb + GB_SYS (1 + system)), reads its system's whole vector once.  Every block of a system runs the same
 // recurrence on the same data, so all of them see convergence at the same iteration and leave
 // together; a system does not wait for the others.  With one system this is the single solve; with
 // several, one launch costs the slowest system's iterations instead of their sum.
@@ -804,7 +844,7 @@ struct MsBatch {
 // previous contents has passed.
 __device__ __forceinline__ double block_reduce_w(double v, double (*sh)[16], int slot) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    v = wave_sum(v);
+    v = lzd_wave_sum(v);
     if (lane == 0) sh[slot][wid] = v;
     __syncthreads();
     double r[8];
@@ -820,6 +860,19 @@ __device__ __forceinline__ double ms_prec_el(int k, double r, double rp) {
     if (k == 0) return pa * r + pb * rp;
     if (k == 4) return pb * rp + pa * r;
     return r * (5.0 / 9.0);
+}
+
+// Sum over each group of TPR consecutive lanes (32: the two halves of the wave; 64: the wave), the
+// same bits in every lane of the group: quad and row sums by DPP, then the rows' sums by lane reads.
+template <int TPR>
+__device__ __forceinline__ double group_sum(double a) {
+    a += lzd_dpp<0xB1>(a);    // quad_perm [1, 0, 3, 2]
+    a += lzd_dpp<0x4E>(a);    // quad_perm [2, 3, 0, 1]
+    a += lzd_dpp<0x124>(a);   // row_ror:4
+    a += lzd_dpp<0x128>(a);   // row_ror:8
+    const double s0 = lzd_lane(a, 0), s1 = lzd_lane(a, 16), s2 = lzd_lane(a, 32), s3 = lzd_lane(a, 48);
+    if constexpr (TPR == 64) return (s0 + s1) + (s2 + s3);
+    return (threadIdx.x & 32) ? s2 + s3 : s0 + s1;
 }
 
 template <int TPR>
@@ -844,8 +897,8 @@ __global__ __launch_bounds__(MW) void k_ms_wide(int N, MsBatch bt, int nbs, doub
     __shared__ MinresState ss[2];
     if (__hip_atomic_load(&gb[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     const int sys = blockIdx.x / nbs, bs = blockIdx.x - sys * nbs;
-    unsigned* bw = gb + GB_SYS * (1 + sys);
-    unsigned gen = __hip_atomic_load(&bw[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned* flags = gb + GB_SYS * (1 + sys);
+    unsigned gen = __hip_atomic_load(&flags[bs], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const MsSys S = bt.s[sys];
     const int t = threadIdx.x, q = t / TPR, tl = t % TPR;
     const int rb = bs * R;
@@ -870,8 +923,7 @@ __global__ __launch_bounds__(MW) void k_ms_wide(int N, MsBatch bt, int nbs, doub
         for (int k = 0; k < KR; k++) a = __builtin_fma(Ac[k], v[tl + k * TPR], a);
 #pragma unroll
         for (int k = KR; k < MW_COLS; k++) a = __builtin_fma(Al[(k - KR) * MW + t], v[tl + k * TPR], a);
-#pragma unroll
-        for (int off = TPR / 2; off > 0; off >>= 1) a += __shfl_xor(a, off, TPR);
+        a = group_sum<TPR>(a);
         if (tl == 0 && q < nr) yout[rb + q] = a;
     };
     // each thread owns entries i = t + e MW (e < NE, i < N) of the vectors; the partner of i is the
@@ -932,7 +984,7 @@ __global__ __launch_bounds__(MW) void k_ms_wide(int N, MsBatch bt, int nbs, doub
     bool ok = true;
     if (!done) {
         gemv(S.ybuf);
-        ok = grid_sync(bw, gb + 2, (unsigned)nbs, gen);
+        ok = flag_sync(flags, gb + 2, nbs, bs, gen);
     }
     while (ok && !done) {
         const MinresState& cur = ss[sp];
@@ -1030,7 +1082,7 @@ __global__ __launch_bounds__(MW) void k_ms_wide(int N, MsBatch bt, int nbs, doub
             }
             __syncthreads();
             gemv(ynext);
-            ok = grid_sync(bw, gb + 2, (unsigned)nbs, gen);
+            ok = flag_sync(flags, gb + 2, nbs, bs, gen);
         }
     }
     if (t < nr) S.x[rb + t] = xo;
@@ -1049,6 +1101,7 @@ __global__ __launch_bounds__(MW) void k_ms_wide(int N, MsBatch bt, int nbs, doub
 
 // Launch shape of k_ms_wide for N unknowns: threads per row (32 or 64), rows per block, blocks per
 // system; false when a system of this size does not fit (more than 2048 unknowns).
+static_assert(2 * MW / (MW / 64) <= GB_SYS, "a system's blocks must fit its barrier flags");
 inline bool ms_wide_shape(int64_t N, int* tpr, int* nbs) {
     if (N <= 0 || N > 2 * MW) return false;
     *tpr = N <= MW ? 32 : 64;

@@ -134,7 +134,7 @@ struct Ctx {
     int64_t* dcnt = nullptr;
     int64_t* dcnt_h = nullptr;      // pinned
     unsigned* gbar = nullptr;       // grid barriers of the persistent MINRES kernels: [0] count, [1] generation,
-                                    // [2] error; per wide-MINRES system s, count and generation at 32 (1 + s)
+                                    // [2] error; per wide-MINRES system s, a flag per block at 128 (1 + s)
     // ---- block-elimination solver (PS_SOLVER_BLOCK)
     double* Mss = nullptr;          // 5Nc x 5Nc stresslet block, rebuilt by mob()
     double* bs_y = nullptr;         // n
