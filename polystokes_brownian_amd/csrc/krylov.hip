@@ -282,51 +282,43 @@ __global__ void k_recip(double* __restrict__ d, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------- Lanczos kernels
-// The two Gram-Schmidt passes of a Lanczos iteration over rows of 256 (block b owns rows 256 b ..),
+// The two Gram-Schmidt passes of a Lanczos iteration over rows of LZB (block b owns rows LZB b ..),
 // with the block partials of the Lanczos finalize (red1) or of the previous launch (red2):
 //   k_lz_sub_dots: h = the sums of red1 (every block alike); alpha_j = h_{m-1}; w -= sum_k h_k Q_k;
 //                  then the second pass's partial dots Q_k . w into red2;
 //   k_lz_sub_norm: h from red2; w -= sum_k h_k Q_k; the block partials of |w|^2 into red3, which the
 //                  next iteration's finalize sums into beta^2.
-__global__ __launch_bounds__(VB) void k_lz_sub_dots(int64_t n, const double* __restrict__ Q, int m,
-                                                    double* __restrict__ w, const double* __restrict__ red1, int nb,
-                                                    double* __restrict__ alpha_out, double* __restrict__ red2) {
+__global__ __launch_bounds__(LZB) void k_lz_sub_dots(int64_t n, const double* __restrict__ Q, int m,
+                                                     double* __restrict__ w, const double* __restrict__ red1, int nb,
+                                                     double* __restrict__ alpha_out, double* __restrict__ red2) {
     __shared__ double h[64];
-    __shared__ double lds[LZD_CH * LZD_T];
+    __shared__ double sh[64][LZW];
     lzd_sum(red1, nb, m, h);
     if (blockIdx.x == 0 && threadIdx.x == 0) *alpha_out = h[m - 1];
-    const int64_t r = (int64_t)blockIdx.x * VB + threadIdx.x;
+    const int64_t r = (int64_t)blockIdx.x * LZB + threadIdx.x;
     const bool in = r < n;
     const int64_t rc = in ? r : 0;
     double wi = in ? w[r] : 0.0;
 #pragma unroll 8
     for (int q = 0; q < m; q++) wi -= h[q] * Q[(int64_t)q * n + rc];
     if (in) w[r] = wi;
-    for (int q0 = 0; q0 < m; q0 += LZD_CH) {
-        const int mc = min(LZD_CH, m - q0);
-#pragma unroll 8
-        for (int k = 0; k < mc; k++) lds[k * LZD_T + threadIdx.x] = in ? Q[(int64_t)(q0 + k) * n + rc] * wi : 0.0;
-        __syncthreads();
-        lzd_store(lds, mc, red2 + (int64_t)q0 * nb, nb);
-        __syncthreads();
-    }
+    lzd_block_dots(
+        m, [&](int q) { return in ? Q[(int64_t)q * n + rc] * wi : 0.0; }, sh, red2, nb);
 }
-__global__ __launch_bounds__(VB) void k_lz_sub_norm(int64_t n, const double* __restrict__ Q, int m,
-                                                    double* __restrict__ w, const double* __restrict__ red2, int nb,
-                                                    double* __restrict__ red3) {
+__global__ __launch_bounds__(LZB) void k_lz_sub_norm(int64_t n, const double* __restrict__ Q, int m,
+                                                     double* __restrict__ w, const double* __restrict__ red2, int nb,
+                                                     double* __restrict__ red3) {
     __shared__ double h[64];
-    __shared__ double lds[LZD_T];
+    __shared__ double sh[1][LZW];
     lzd_sum(red2, nb, m, h);
-    const int64_t r = (int64_t)blockIdx.x * VB + threadIdx.x;
+    const int64_t r = (int64_t)blockIdx.x * LZB + threadIdx.x;
     const bool in = r < n;
     const int64_t rc = in ? r : 0;
     double wi = in ? w[r] : 0.0;
 #pragma unroll 8
     for (int q = 0; q < m; q++) wi -= h[q] * Q[(int64_t)q * n + rc];
     if (in) w[r] = wi;
-    lds[threadIdx.x] = in ? wi * wi : 0.0;
-    __syncthreads();
-    lzd_store(lds, 1, red3, nb);
+    lzd_block_dots(1, [&](int) { return wi * wi; }, sh, red3, nb);
 }
 inline unsigned eblocks(int64_t n) { return (unsigned)((n + VB - 1) / VB); }
 
@@ -348,7 +340,7 @@ ps_err krylov_alloc(Ctx* c) {
     PS_HIP(c, hipMemset(c->lz, 0, sizeof(double) * (3 * 64 + 8)));
     PS_HIP(c, hipHostMalloc(&c->lz_h, sizeof(double) * (3 * 64 + 8), hipHostMallocDefault));
     PS_HIP(c, hipMalloc(&c->lzg, sizeof(double) * 65 * 64));
-    PS_HIP(c, hipMalloc(&c->lzred, sizeof(double) * 129 * (size_t)eblocks(n)));
+    PS_HIP(c, hipMalloc(&c->lzred, sizeof(double) * 129 * (size_t)lzd_blocks(n)));
     PS_HIP(c, hipMalloc(&c->lzinfo, sizeof(double) * 4));
     PS_HIP(c, hipMemset(c->lzinfo, 0, sizeof(double) * 4));
     PS_HIP(c, hipMalloc(&c->dcnt, sizeof(int64_t) * DC_N));
@@ -812,19 +804,20 @@ __global__ __launch_bounds__(VB) void k_ms_solve(int64_t N, const double* __rest
 }
 
 // The same recurrence for blocks of up to 2048 unknowns, several independent systems in one launch
-// (MS_NSYS at most).  Each system gets nbs blocks of 1024 threads; a block keeps r1, r2, v, z and y
-// of its system's whole vector in LDS (computed redundantly, so identical in every block of the
-// system) and R = 1024 / TPR of its rows of A on chip (registers, the last columns in LDS): TPR
-// threads per row, 32 columns a thread.  The only data crossing blocks is y = A v: each block writes
-// its rows and, after the barrier among its system's blocks (flag_sync over the system's GB_SYS flag
-// words from gb + GB_SYS (1 + system)), reads its system's whole vector once.  Every block of a system runs the same
+// (MS_NSYS at most).  Each system gets nbs blocks of 1024 threads; a block keeps r1, r2, v and y of
+// its system's whole vector in LDS and z = P^-1 r2 in registers (computed redundantly, so identical in
+// every block of the system) and R = 1024 / TPR of its rows of A on chip (registers, the last columns
+// in LDS): TPR threads per row, 32 columns a thread.  The only data crossing blocks is y = A v: each
+// block writes its rows and, after the barrier among its system's blocks (flag_sync over the system's
+// GB_SYS flag words from gb + GB_SYS (1 + system)), reads its system's whole vector once.  Every block
+// of a system runs the same
 // recurrence on the same data, so all of them see convergence at the same iteration and leave
 // together; a system does not wait for the others.  With one system this is the single solve; with
 // several, one launch costs the slowest system's iterations instead of their sum.
 constexpr int MW = 1024, MW_COLS = 32;
 // Of a thread's MW_COLS columns of A, the last ms_ldsc(TPR) live in LDS, which leaves the registers
-// room for the recurrence without spills (the LDS then holds at most 4 N + TPR MW_COLS + that many
-// times MW doubles: 152 KiB at N = 2048).
+// room for the recurrence without spills (the LDS then holds at most 3 N + TPR MW_COLS + that many
+// times MW doubles: 136 KiB at N = 2048).
 constexpr int ms_ldsc(int tpr) { return tpr == 32 ? 10 : 8; }
 
 struct MsSys {
@@ -886,9 +879,8 @@ __global__ __launch_bounds__(MW) void k_ms_wide(int N, MsBatch bt, int nbs, doub
     // the new residual is written over r1, which the iteration has read by then
     double* r1 = lds;           // [N]
     double* r2 = lds + N;       // [N]
-    double* z = lds + 2 * N;    // [N]: P^-1 r2_new
-    double* y = lds + 3 * N;    // [N]: A v_k - f r1
-    double* v = lds + 4 * N;    // [NV]: v_k
+    double* y = lds + 2 * N;    // [N]: A v_k - f r1
+    double* v = lds + 3 * N;    // [NV]: v_k
     double* Al = v + NV;        // [ms_ldsc(TPR)][MW]: the last columns of each thread's row of A
     __shared__ double sh[2][16];
     // the recurrence's scalars, the same in every thread: two slots, the current one read by all
@@ -1119,7 +1111,7 @@ static ps_err ms_wide_launch(Ctx* c, int64_t N, const MsBatch& bt, int ns, doubl
                      " unknowns do not fit one block per CU");
         return PS_ERR_ARG;
     }
-    const size_t lds = sizeof(double) * (4 * (size_t)N + (size_t)tpr * MW_COLS + (size_t)ms_ldsc(tpr) * MW);
+    const size_t lds = sizeof(double) * (3 * (size_t)N + (size_t)tpr * MW_COLS + (size_t)ms_ldsc(tpr) * MW);
     const unsigned nb = (unsigned)(nbs * ns);
     if (tpr == 32)
         k_ms_wide<32><<<nb, MW, lds, c->stream>>>((int)N, bt, nbs, rtol, c->p.ksp_abstol, maxit, c->gbar, c->dcnt);
@@ -1585,7 +1577,7 @@ ps_err sqrt_recurrence_dev(Ctx* c, const double* xi) {
     // q_j = w / beta_{j-1}, scales M w by 1 / beta_{j-1} (M is linear) and leaves the first
     // Gram-Schmidt pass's partial dots; the two subtract launches leave the second pass's dots and the
     // partials of |w|^2.  The last iteration's beta is never needed.
-    const int nb = (int)ne;
+    const int nb = (int)lzd_blocks(n);
     double* red1 = c->lzred;
     double* red2 = c->lzred + 64 * (size_t)nb;
     double* red3 = c->lzred + 128 * (size_t)nb;
@@ -1597,9 +1589,9 @@ ps_err sqrt_recurrence_dev(Ctx* c, const double* xi) {
         else
             PS_TRY(mob_apply_lanczos(c, wbuf[(j - 1) & 1], w, c->Q + (int64_t)j * n, red3, nb,
                                      c->lz + LZ_BETA2 + j - 1, j + 1, red1, prec));
-        k_lz_sub_dots<<<ne, VB, 0, st>>>(n, c->Q, j + 1, w, red1, nb, c->lz + j, red2);
+        k_lz_sub_dots<<<nb, LZB, 0, st>>>(n, c->Q, j + 1, w, red1, nb, c->lz + j, red2);
         PS_LAUNCH(c);
-        k_lz_sub_norm<<<ne, VB, 0, st>>>(n, c->Q, j + 1, w, red2, nb, red3);
+        k_lz_sub_norm<<<nb, LZB, 0, st>>>(n, c->Q, j + 1, w, red2, nb, red3);
         PS_LAUNCH(c);
     }
     return PS_OK;

@@ -1,19 +1,25 @@
 // lzdots.h -- block partials of many dot products for the Lanczos Gram-Schmidt passes (device code),
 // shared by the Lanczos finalize (mobility.hip) and the subtract kernels (krylov.hip).
 //
-// A block of 256 threads owns 256 rows of the vectors.  Each thread leaves its row's contributions
-// c_q (q < m, at most LZD_CH at a time) in LDS (lds[q * 256 + thread]); lzd_store reduces them to one
-// partial per q and block, red[q * nb + block], with a fixed order (the four 64-row quarters, then the
-// wave by row DPP moves and lane reads); lzd_sum gives every block the same h_q = sum over the nb
-// partials.
+// The Lanczos vector kernels run blocks of LZB threads, one row of the vectors per thread, so a vector
+// of n rows leaves nb = ceil(n / LZB) block partials per dot product.  lzd_block_dots forms one partial
+// per dot and block in a fixed order (DPP sums within each wave, then the wave sums in a fixed tree);
+// lzd_sum gives every block the same h_q = sum over the nb partials.
 #pragma once
 #include "ps_common.h"
 
 #include <hip/hip_runtime.h>
 
-constexpr int LZD_T = 256;     // threads (rows) per block
-constexpr int LZD_CH = 32;     // basis vectors per pass over the LDS rows (32 x 256 doubles = 64 KiB)
-static_assert(LZD_T == PS_BLOCK, "the Lanczos dot kernels run one row per thread in blocks of PS_BLOCK");
+// PS_LZB: 256, 512 or 1024.  256 measured fastest at N = 20,000 (per Lanczos iteration, finalize plus
+// both subtract kernels: 34.5 us at 256, 36.2 us at 512, 48 us at 1024): with fewer, larger blocks the
+// finalize's reads of the apply partials have too few rows in flight.
+#ifndef PS_LZB
+#define PS_LZB 256
+#endif
+constexpr int LZB = PS_LZB;      // threads (rows) per block of the Lanczos vector kernels
+constexpr int LZW = LZB / 64;    // waves per block
+static_assert(LZW == 4 || LZW == 8 || LZW == 16, "256, 512 or 1024 threads per block");
+__host__ __device__ inline int64_t lzd_blocks(int64_t n) { return (n + LZB - 1) / LZB; }
 
 template <int CTRL>
 __device__ __forceinline__ double lzd_dpp(double v) {
@@ -28,49 +34,80 @@ __device__ __forceinline__ double lzd_lane(double v, int lane) {
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
     return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
-// Sum over the 64 lanes, the same bits in every lane: quad and row sums by DPP (every lane of a row of
-// 16 ends with a sum of the row), then the sums held by lanes 0, 16, 32 and 48 in a fixed order.
-__device__ __forceinline__ double lzd_wave_sum(double v) {
+// Sum over each row of 16 lanes, left in every lane of the row: quad sums, then the row's quads.
+__device__ __forceinline__ double lzd_row_sum(double v) {
     v += lzd_dpp<0xB1>(v);    // quad_perm [1, 0, 3, 2]
     v += lzd_dpp<0x4E>(v);    // quad_perm [2, 3, 0, 1]
     v += lzd_dpp<0x124>(v);   // row_ror:4
     v += lzd_dpp<0x128>(v);   // row_ror:8
+    return v;
+}
+// Sum over the 64 lanes, the same bits in every lane: row sums by DPP, then the sums held by lanes 0,
+// 16, 32 and 48 in a fixed order.  Every lane of the wave must be active.
+__device__ __forceinline__ double lzd_wave_sum(double v) {
+    v = lzd_row_sum(v);
     return (lzd_lane(v, 0) + lzd_lane(v, 16)) + (lzd_lane(v, 32) + lzd_lane(v, 48));
 }
 
-// red[q * nb + blockIdx.x] = the block's sum of lds[q * 256 + t] over its 256 threads, q < m.  The
-// caller has filled lds and synchronised the block.
-__device__ __forceinline__ void lzd_store(const double* lds, int m, double* __restrict__ red, int nb) {
+// The block partials of m <= 64 dot products: contrib(q) is this thread's row's term of dot q (zero
+// for rows past n).  Per q every wave sums its 64 rows and lane 0 leaves the sum in sh[q][wave]; after
+// a block barrier thread q adds the LZW wave sums in a fixed tree into red[q * nb + block].  Every
+// thread of the block calls it.  sh is not written again by this call after the barrier.
+template <class F>
+__device__ __forceinline__ void lzd_block_dots(int m, F&& contrib, double (*sh)[LZW], double* __restrict__ red,
+                                               int nb) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int q = wid; q < m; q += LZD_T / 64) {
-        const double* row = lds + q * LZD_T;
-        const double v = lzd_wave_sum((row[lane] + row[lane + 64]) + (row[lane + 128] + row[lane + 192]));
-        if (lane == 0) red[(int64_t)q * nb + blockIdx.x] = v;
+    // eight terms at a time, all loaded before any is summed (the index clamped to m - 1, so that the
+    // loads need no branch; the padded terms are not summed)
+    for (int q0 = 0; q0 < m; q0 += 8) {
+        double c[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) c[k] = contrib(min(q0 + k, m - 1));
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (q0 + k < m) {
+                const double s = lzd_wave_sum(c[k]);
+                if (lane == 0) sh[q0 + k][wid] = s;
+            }
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < m) {
+        double r[LZW];
+#pragma unroll
+        for (int k = 0; k < LZW; k++) r[k] = sh[threadIdx.x][k];
+#pragma unroll
+        for (int w = LZW / 2; w > 0; w >>= 1)
+#pragma unroll
+            for (int k = 0; k < w; k++) r[k] = r[2 * k] + r[2 * k + 1];
+        red[(int64_t)threadIdx.x * nb + blockIdx.x] = r[0];
     }
 }
 
-// h[q] = sum over b < nb of red[q * nb + b] for q < m (h in LDS, the same bits in every block): four
-// threads per q, each with four interleaved accumulators (so that the loads are in flight together),
-// combined in a fixed order.  Ends with a block barrier.
+// h[q] = sum over b < nb of red[q * nb + b] for q < m (h in LDS, the same bits in every block): the
+// 16 threads of one DPP row per q (LZB / 16 q at a time), each with four interleaved accumulators (so
+// that its loads are in flight together), combined within the row.  Every thread of the block calls it; ends with a block
+// barrier.
 __device__ __forceinline__ void lzd_sum(const double* __restrict__ red, int nb, int m, double* h) {
-    const int t = threadIdx.x, q = t >> 2, s = t & 3;
-    double v = 0.0;
-    if (q < m) {
-        const double* rq = red + (int64_t)q * nb;
+    const int t = threadIdx.x, s = t & 15;
+    for (int q0 = 0; q0 < m; q0 += LZB / 16) {
+        const int q = q0 + (t >> 4);
         double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        int b = s;
-        for (; b + 12 < nb; b += 16) {
-            a0 += rq[b];
-            a1 += rq[b + 4];
-            a2 += rq[b + 8];
-            a3 += rq[b + 12];
+        if (q < m) {
+            const double* rq = red + (int64_t)q * nb;
+            int b = s;
+            for (; b + 48 < nb; b += 64) {
+                a0 += rq[b];
+                a1 += rq[b + 16];
+                a2 += rq[b + 32];
+                a3 += rq[b + 48];
+            }
+            if (b < nb) a0 += rq[b];
+            if (b + 16 < nb) a1 += rq[b + 16];
+            if (b + 32 < nb) a2 += rq[b + 32];
         }
-        for (; b < nb; b += 4) a0 += rq[b];
-        v = (a0 + a1) + (a2 + a3);
+        const double v = lzd_row_sum((a0 + a1) + (a2 + a3));
+        if (s == 0 && q < m) h[q] = v;
     }
-    // the four threads of a q are consecutive lanes: (v0 + v1) + (v2 + v3) in each of them
-    v += __shfl_xor(v, 1, 64);
-    v += __shfl_xor(v, 2, 64);
-    if (s == 0 && q < m) h[q] = v;
     __syncthreads();
 }

@@ -839,28 +839,30 @@ __global__ __launch_bounds__(PS_BLOCK) void k_finalize(SysConst s, const double*
 // f is already q and nothing is scaled.  The first Gram-Schmidt pass's partial dots follow in the same
 // launch: red1[k * nb + block] = the block's sum of Q_k[r] y[r] for k < m, with Q_{m-1} = q.
 template <class T>
-__global__ __launch_bounds__(PS_BLOCK) void k_finalize_lz(SysConst s, const double* __restrict__ f,
-                                                          double* __restrict__ y, FinArgs<T> a,
-                                                          const double* __restrict__ red3, int nb,
-                                                          double* __restrict__ beta2_out, double* __restrict__ qout,
-                                                          const double* __restrict__ Q, int m,
-                                                          double* __restrict__ red1) {
-    __shared__ double sh[4];
-    __shared__ double lds[LZD_CH * LZD_T];
+__global__ __launch_bounds__(LZB) void k_finalize_lz(SysConst s, const double* __restrict__ f,
+                                                     double* __restrict__ y, FinArgs<T> a,
+                                                     const double* __restrict__ red3, int nb,
+                                                     double* __restrict__ beta2_out, double* __restrict__ qout,
+                                                     const double* __restrict__ Q, int m,
+                                                     double* __restrict__ red1) {
+    __shared__ double sh[64][LZW];
+    __shared__ double b2s;
     const int64_t n = s.n;
     double sc = 1.0;
     if (red3) {
-        // every thread of every block forms the same sum: wave sums, then the four in a fixed order
-        double v = 0.0;
-        for (int i = threadIdx.x; i < nb; i += PS_BLOCK) v += red3[i];
-        v = wave_sum(v);
-        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+        // wave 0 of every block forms the same sum of the nb partials
+        if (threadIdx.x < 64) {
+            double v = 0.0;
+            for (int i = threadIdx.x; i < nb; i += 64) v += red3[i];
+            v = lzd_wave_sum(v);
+            if (threadIdx.x == 0) b2s = v;
+        }
         __syncthreads();
-        const double b2 = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+        const double b2 = b2s;
         if (blockIdx.x == 0 && threadIdx.x == 0 && beta2_out) *beta2_out = b2;
         sc = 1.0 / sqrt(fmax(b2, 0.0));
     }
-    const int64_t r = (int64_t)blockIdx.x * PS_BLOCK + threadIdx.x;
+    const int64_t r = (int64_t)blockIdx.x * LZB + threadIdx.x;
     const bool in = r < n;
     double yr = 0.0, qr = 0.0;
     if (in) {
@@ -870,18 +872,13 @@ __global__ __launch_bounds__(PS_BLOCK) void k_finalize_lz(SysConst s, const doub
         if (qout) qout[r] = qr;
     }
     const int64_t rc = in ? r : 0;
-    for (int q0 = 0; q0 < m; q0 += LZD_CH) {
-        const int mc = min(LZD_CH, m - q0);
-#pragma unroll 8
-        for (int k = 0; k < mc; k++) {
-            const int q = q0 + k;
+    lzd_block_dots(
+        m,
+        [&](int q) {
             const double qv = q == m - 1 ? qr : Q[(int64_t)q * n + rc];
-            lds[k * LZD_T + threadIdx.x] = in ? qv * yr : 0.0;
-        }
-        __syncthreads();
-        lzd_store(lds, mc, red1 + (int64_t)q0 * nb, nb);
-        __syncthreads();
-    }
+            return in ? qv * yr : 0.0;
+        },
+        sh, red1, nb);
 }
 
 // ---------------------------------------------------------------------------- carried tile
@@ -1423,14 +1420,14 @@ ps_err mob_apply(Ctx* c, const double* f, double* u, int prec) {
 
 // The Lanczos iteration's apply: y = M (f / beta) and q = f / beta with beta^2 the sum of the nb
 // partials in red3 (stored to *beta2_out), or y = M f when red3 is null, then the first Gram-Schmidt
-// pass's partial dots against Q_0..Q_{m-1} into red1 (k_finalize_lz; nb = ceil(n / 256)).
+// pass's partial dots against Q_0..Q_{m-1} into red1 (k_finalize_lz; nb = ceil(n / LZB)).
 template <class T>
 static ps_err apply_lanczos(Ctx* c, const T* xm, ApplyWork& aw, const double* f, double* y, double* qout,
                             const double* red3, int nb, double* beta2_out, int m, double* red1) {
     PS_TRY(launch_apply<T>(c, xm, aw, f, nullptr, 4));
-    const unsigned g = (unsigned)((c->n + PS_BLOCK - 1) / PS_BLOCK);
-    k_finalize_lz<T><<<g, PS_BLOCK, 0, c->stream>>>(c->sc, f, y, fin_args<T>(c, aw), red3, nb, beta2_out, qout, c->Q,
-                                                    m, red1);
+    const unsigned g = (unsigned)lzd_blocks(c->n);
+    k_finalize_lz<T><<<g, LZB, 0, c->stream>>>(c->sc, f, y, fin_args<T>(c, aw), red3, nb, beta2_out, qout, c->Q, m,
+                                               red1);
     PS_LAUNCH(c);
     return PS_OK;
 }
