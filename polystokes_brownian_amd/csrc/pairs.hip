@@ -160,8 +160,11 @@ __global__ __launch_bounds__(SCAN_T) void k_scan(int* __restrict__ in, int* __re
     }
 }
 
-// A group of GL lanes serves one particle: in cell mode lane l < 27 walks stencil cell l (the 27
-// cells are distinct when every axis has >= 3 cells); otherwise the lanes stride over all particles.
+// A group of GL lanes serves one particle.  In cell mode lane l < 27 finds the range of stencil cell
+// l in the cell-sorted order (the 27 cells are distinct when every axis has >= 3 cells); the group
+// then walks the concatenation of the 27 ranges, candidate k by lane k mod GL, so that the lanes
+// share the work however the particles crowd into cells (chains fill a few cells densely).
+// Otherwise the lanes stride over all particles.  Every lane of the group calls it.
 constexpr int GL = 32;
 
 template <class F>
@@ -173,15 +176,36 @@ __device__ __forceinline__ void visit_candidates(const PairConst& pc, const doub
             if (q != p) fn(q, xs + 3 * q);
         return;
     }
-    if (lane >= 27) return;
-    const int c = cell_of[p];
-    const int nx = pc.ncell[0], ny = pc.ncell[1], nz = pc.ncell[2];
-    const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-    const int ox = lane % 3 - 1, oy = (lane / 3) % 3 - 1, oz = lane / 9 - 1;
-    const int ax = (cx + ox + nx) % nx, ay = (cy + oy + ny) % ny, az = (cz + oz + nz) % nz;
-    const int cc = ax + nx * (ay + ny * az);
-    const int t1 = cell_start[cc + 1];
-    for (int t = cell_start[cc]; t < t1; t++) {
+    // per group: each stencil cell's first sorted index and the inclusive prefix of the counts
+    __shared__ int rng[TB / GL][2][GL];
+    const int g = (threadIdx.x % TB) / GL;
+    int st = 0, cnt = 0;
+    if (lane < 27) {
+        const int c = cell_of[p];
+        const int nx = pc.ncell[0], ny = pc.ncell[1], nz = pc.ncell[2];
+        const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+        const int ox = lane % 3 - 1, oy = (lane / 3) % 3 - 1, oz = lane / 9 - 1;
+        const int ax = (cx + ox + nx) % nx, ay = (cy + oy + ny) % ny, az = (cz + oz + nz) % nz;
+        const int cc = ax + nx * (ay + ny * az);
+        st = cell_start[cc];
+        cnt = cell_start[cc + 1] - st;
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < GL; o <<= 1) {
+        const int v = __shfl_up(incl, o, GL);
+        if (lane >= o) incl += v;
+    }
+    rng[g][0][lane] = st;
+    rng[g][1][lane] = incl;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int total = rng[g][1][GL - 1];
+    int r = 0;   // the stencil cell of this lane's current candidate (candidates come in cell order)
+    for (int k = lane; k < total; k += GL) {
+        while (rng[g][1][r] <= k) r++;
+        const int t = rng[g][0][r] + k - (r ? rng[g][1][r - 1] : 0);
         const int q = perm[t];
         if (q != p) fn(q, xsort + 3 * t);
     }
