@@ -23,6 +23,7 @@
 #include "tiles.h"
 #include "batch.h"
 #include "tql2.h"
+#include "dpp.h"
 
 #pragma clang fp contract(off)
 
@@ -33,10 +34,8 @@ constexpr int NW = BW / 64;
 constexpr int BCCH = 512;   // colloids staged in LDS per pass of the monomer rows
 constexpr int FTILE = 1024;   // force partners per LDS tile
 
-__device__ __forceinline__ double wsum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+// Wave sum, the same bits in every lane (DPP row sums and four lane reads); every lane active.
+__device__ __forceinline__ double wsum(double v) { return wave_sum64(v); }
 // Workgroup sum; every thread gets the same value (fixed order).  red: LDS, NW doubles.
 __device__ double wg_sum(double v, double* red) {
     v = wsum(v);

@@ -837,7 +837,7 @@ struct MsBatch {
 // previous contents has passed.
 __device__ __forceinline__ double block_reduce_w(double v, double (*sh)[16], int slot) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    v = lzd_wave_sum(v);
+    v = wave_sum64(v);
     if (lane == 0) sh[slot][wid] = v;
     __syncthreads();
     double r[8];
@@ -859,11 +859,11 @@ __device__ __forceinline__ double ms_prec_el(int k, double r, double rp) {
 // same bits in every lane of the group: quad and row sums by DPP, then the rows' sums by lane reads.
 template <int TPR>
 __device__ __forceinline__ double group_sum(double a) {
-    a += lzd_dpp<0xB1>(a);    // quad_perm [1, 0, 3, 2]
-    a += lzd_dpp<0x4E>(a);    // quad_perm [2, 3, 0, 1]
-    a += lzd_dpp<0x124>(a);   // row_ror:4
-    a += lzd_dpp<0x128>(a);   // row_ror:8
-    const double s0 = lzd_lane(a, 0), s1 = lzd_lane(a, 16), s2 = lzd_lane(a, 32), s3 = lzd_lane(a, 48);
+    a += dpp_mov<0xB1>(a);    // quad_perm [1, 0, 3, 2]
+    a += dpp_mov<0x4E>(a);    // quad_perm [2, 3, 0, 1]
+    a += dpp_mov<0x124>(a);   // row_ror:4
+    a += dpp_mov<0x128>(a);   // row_ror:8
+    const double s0 = lane_read(a, 0), s1 = lane_read(a, 16), s2 = lane_read(a, 32), s3 = lane_read(a, 48);
     if constexpr (TPR == 64) return (s0 + s1) + (s2 + s3);
     return (threadIdx.x & 32) ? s2 + s3 : s0 + s1;
 }

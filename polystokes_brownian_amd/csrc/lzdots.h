@@ -7,6 +7,7 @@
 // lzd_sum gives every block the same h_q = sum over the nb partials.
 #pragma once
 #include "ps_common.h"
+#include "dpp.h"
 
 #include <hip/hip_runtime.h>
 
@@ -20,34 +21,6 @@ constexpr int LZB = PS_LZB;      // threads (rows) per block of the Lanczos vect
 constexpr int LZW = LZB / 64;    // waves per block
 static_assert(LZW == 4 || LZW == 8 || LZW == 16, "256, 512 or 1024 threads per block");
 __host__ __device__ inline int64_t lzd_blocks(int64_t n) { return (n + LZB - 1) / LZB; }
-
-template <int CTRL>
-__device__ __forceinline__ double lzd_dpp(double v) {
-    const long long b = __builtin_bit_cast(long long, v);
-    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
-    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ double lzd_lane(double v, int lane) {
-    const long long b = __builtin_bit_cast(long long, v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
-}
-// Sum over each row of 16 lanes, left in every lane of the row: quad sums, then the row's quads.
-__device__ __forceinline__ double lzd_row_sum(double v) {
-    v += lzd_dpp<0xB1>(v);    // quad_perm [1, 0, 3, 2]
-    v += lzd_dpp<0x4E>(v);    // quad_perm [2, 3, 0, 1]
-    v += lzd_dpp<0x124>(v);   // row_ror:4
-    v += lzd_dpp<0x128>(v);   // row_ror:8
-    return v;
-}
-// Sum over the 64 lanes, the same bits in every lane: row sums by DPP, then the sums held by lanes 0,
-// 16, 32 and 48 in a fixed order.  Every lane of the wave must be active.
-__device__ __forceinline__ double lzd_wave_sum(double v) {
-    v = lzd_row_sum(v);
-    return (lzd_lane(v, 0) + lzd_lane(v, 16)) + (lzd_lane(v, 32) + lzd_lane(v, 48));
-}
 
 // The block partials of m <= 64 dot products: contrib(q) is this thread's row's term of dot q (zero
 // for rows past n).  Per q every wave sums its 64 rows and lane 0 leaves the sum in sh[q][wave]; after
@@ -66,7 +39,7 @@ __device__ __forceinline__ void lzd_block_dots(int m, F&& contrib, double (*sh)[
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             if (q0 + k < m) {
-                const double s = lzd_wave_sum(c[k]);
+                const double s = wave_sum64(c[k]);
                 if (lane == 0) sh[q0 + k][wid] = s;
             }
         }
@@ -106,7 +79,7 @@ __device__ __forceinline__ void lzd_sum(const double* __restrict__ red, int nb, 
             if (b + 16 < nb) a1 += rq[b + 16];
             if (b + 32 < nb) a2 += rq[b + 32];
         }
-        const double v = lzd_row_sum((a0 + a1) + (a2 + a3));
+        const double v = row_sum16((a0 + a1) + (a2 + a3));
         if (s == 0 && q < m) h[q] = v;
     }
     __syncthreads();

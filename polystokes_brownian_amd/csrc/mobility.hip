@@ -854,7 +854,7 @@ __global__ __launch_bounds__(LZB) void k_finalize_lz(SysConst s, const double* _
         if (threadIdx.x < 64) {
             double v = 0.0;
             for (int i = threadIdx.x; i < nb; i += 64) v += red3[i];
-            v = lzd_wave_sum(v);
+            v = wave_sum64(v);
             if (threadIdx.x == 0) b2s = v;
         }
         __syncthreads();
