@@ -19,6 +19,7 @@
 #include "ps_common.h"
 #include "tiles.h"
 #include "lzdots.h"
+#include "dpp.h"
 
 #include <hip/hip_ext.h>
 
@@ -39,18 +40,13 @@ __device__ __forceinline__ void min_img(const SysConst& s, T& dx, T& dy, T& dz) 
     dz -= (T)s.L[2] * ps_rint<T>(dz * (T)s.Linv[2]);
 }
 
-template <class T>
-__device__ __forceinline__ T wave_sum(T v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
 // Block-wide sum of NV values per thread (256 threads = 4 waves); result valid in thread 0.
 template <class T, int NV>
 __device__ __forceinline__ void block_sum(T (&v)[NV], T (*sh)[NV]) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
+    for (int k = 0; k < NV; k++) v[k] = wave_sum64(v[k]);
     if (lane == 0)
 #pragma unroll
         for (int k = 0; k < NV; k++) sh[wid][k] = v[k];
