@@ -881,9 +881,8 @@ __global__ __launch_bounds__(LZB) void k_finalize_lz(SysConst s, const double* _
 // stale[0..8]: the mob_a the AB pairs of this mob() call see; stale[9..17]: what is left for the
 // next call (the last BB pair's mob_a, or this call's tile when there is no BB pair).
 // `prev` is the previous mob() call's tile pair (st itself when the state has one slot).
-__global__ void k_stale(SysConst s, const double* __restrict__ xm, int si, int sj, const double* prev,
-                        double* st) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ void stale_tile(const SysConst& s, const double* __restrict__ xm, int si, int sj, const double* prev,
+                           double* st) {
     double cur[9];
     if (si >= 0) {
         double e[3], dr, dr_inv;
@@ -919,6 +918,55 @@ __global__ void k_stale(SysConst s, const double* __restrict__ xm, int si, int s
     } else {
         for (int k = 0; k < 9; k++) st[9 + k] = cur[k];
     }
+}
+
+__global__ void k_stale(SysConst s, const double* __restrict__ xm, int si, int sj, const double* prev,
+                        double* st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    stale_tile(s, xm, si, sj, prev, st);
+}
+
+// mob()'s position snapshot in one launch: xm = xpd (3 Np), its FP32 copy (mixed form), the scaled
+// monomer positions of the FP64 and FP32 symmetric tile pairs (as k_aa_pos / k_aa_pos_fix), and, in
+// thread 0, the carried tile (as k_stale, from the same positions).
+__global__ __launch_bounds__(PS_BLOCK) void k_mob_pos(SysConst s, const double* __restrict__ xpd, double* __restrict__ xm,
+                                                      float* __restrict__ xm32, double* __restrict__ xs64,
+                                                      double inv_L64, void* __restrict__ xs32, int xs32_fix,
+                                                      double inv_L32, int si, int sj, const double* prev,
+                                                      double* st) {
+    const int64_t g = (int64_t)blockIdx.x * PS_BLOCK + threadIdx.x;
+    const int64_t n3 = 3 * (int64_t)s.Np;
+    if (g < n3) {
+        const double v = xpd[g];
+        xm[g] = v;
+        if (xm32) xm32[g] = (float)v;
+    }
+    if (g < s.Nm) {
+        const int i = (int)g;
+        const double p[3] = {xpd[3 * g], xpd[3 * g + 1], xpd[3 * g + 2]};
+        if (xs64) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) xs64[4 * g + d] = p[d] * inv_L64;
+            xs64[4 * g + 3] = 0.0;
+        }
+        if (xs32 && xs32_fix) {
+            int* xf = static_cast<int*>(xs32);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                double t = p[d] * inv_L32;
+                t -= __builtin_rint(t);
+                const long long q = __builtin_llrint(t * 4294967296.0);
+                xf[4 * i + d] = (int)(unsigned)(unsigned long long)q;
+            }
+            xf[4 * i + 3] = 0;
+        } else if (xs32) {
+            float* xf = static_cast<float*>(xs32);
+#pragma unroll
+            for (int d = 0; d < 3; d++) xf[4 * i + d] = (float)(p[d] * inv_L32);
+            xf[4 * i + 3] = 0.0f;
+        }
+    }
+    if (g == 0) stale_tile(s, xpd, si, sj, prev, st);
 }
 
 // Dense stresslet-stresslet block M_ss (5Nc x 5Nc, row-major): self blocks (fill_self) and the
@@ -1324,29 +1372,19 @@ ps_err mob_prepare(Ctx* c) {
         c->cnt.mob_calls++;
         return arrow_mob(c);
     }
-    PS_HIP(c, hipMemcpyAsync(c->xm, c->xpd, sizeof(double) * 3 * (size_t)c->Np, hipMemcpyDeviceToDevice,
-                             c->stream));
-    k_stale<<<1, 64, 0, c->stream>>>(c->sc, c->xm, c->stale_i, c->stale_j, c->stale_prev ? c->stale_prev : c->stale,
-                                      c->stale);
-    PS_LAUNCH(c);
-    c->stale_prev = c->stale;
-    if (c->xm32) {
+    {
+        // xm, its FP32 copy, the tile pairs' scaled positions and the carried tile in one launch
         const int64_t n3 = 3 * (int64_t)c->Np;
-        k_to_f32<<<(unsigned)((n3 + PS_BLOCK - 1) / PS_BLOCK), PS_BLOCK, 0, c->stream>>>(c->xm, c->xm32, n3);
+        const unsigned gp = (unsigned)((n3 + PS_BLOCK - 1) / PS_BLOCK);
+        const bool fix32 = c->aw32.sym && c->sc.box_active;
+        k_mob_pos<<<gp, PS_BLOCK, 0, c->stream>>>(c->sc, c->xpd, c->xm, c->xm32,
+                                                  c->aw64.sym ? (double*)c->aw64.xs : nullptr, c->aw64.inv_L,
+                                                  c->aw32.sym ? c->aw32.xs : nullptr, fix32 ? 1 : 0, c->aw32.inv_L,
+                                                  c->stale_i, c->stale_j, c->stale_prev ? c->stale_prev : c->stale,
+                                                  c->stale);
         PS_LAUNCH(c);
     }
-    const unsigned gm = (unsigned)((c->Nm + PS_BLOCK - 1) / PS_BLOCK);
-    if (c->aw64.sym) {
-        k_aa_pos<double><<<gm, PS_BLOCK, 0, c->stream>>>(c->Nm, c->xm, c->aw64.inv_L, (double*)c->aw64.xs);
-        PS_LAUNCH(c);
-    }
-    if (c->aw32.sym && c->sc.box_active) {
-        k_aa_pos_fix<<<gm, PS_BLOCK, 0, c->stream>>>(c->Nm, c->xm, c->aw32.inv_L, (int*)c->aw32.xs);
-        PS_LAUNCH(c);
-    } else if (c->aw32.sym) {
-        k_aa_pos<float><<<gm, PS_BLOCK, 0, c->stream>>>(c->Nm, c->xm, c->aw32.inv_L, (float*)c->aw32.xs);
-        PS_LAUNCH(c);
-    }
+    c->stale_prev = c->stale;
     if (c->Mss && c->Nc > 0 && !c->p.lub) {   // the lubricated block solve applies M_cc on the fly
         k_mss<<<c->Nc, PS_BLOCK, 0, c->stream>>>(c->sc, c->xm, c->Mss);
         PS_LAUNCH(c);
