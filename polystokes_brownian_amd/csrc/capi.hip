@@ -72,9 +72,25 @@ __global__ void k_step_from(SysConst s, double* __restrict__ x, const double* __
 }
 
 // x += a * w over the first n3 entries (the RFD probe displacements, src/drift.cpp:33,48).
-__global__ void k_probe(double* __restrict__ x, const double* __restrict__ w, double a, int64_t n3) {
+// x += a w; with xpd, the probe's geometry snapshot (xpd = x) in the same pass
+__global__ void k_probe(double* __restrict__ x, const double* __restrict__ w, double a, int64_t n3,
+                        double* __restrict__ xpd) {
     const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
-    if (i < n3) x[i] += a * w[i];
+    if (i >= n3) return;
+    const double v = x[i] + a * w[i];
+    x[i] = v;
+    if (xpd) xpd[i] = v;
+}
+
+// Move to an RFD probe point and take the geometry its mob() reads (the arrowhead regime runs its
+// full check_dist there instead).
+ps_err probe_move(Ctx* c, double a) {
+    const int64_t n3 = 3 * (int64_t)c->Np;
+    k_probe<<<eb(n3), TB, 0, c->stream>>>(c->x, c->fext, a, n3, c->arrow ? nullptr : c->xpd);
+    PS_LAUNCH(c);
+    if (c->arrow) return probe_geometry_dev(c);
+    c->have_pd = true;
+    return PS_OK;
 }
 
 // step_quaternion (src/traj.cpp:64-94, src/quat.cpp:4-54): q <- normalize(q + dt/2 q (x) (0, omega)).
@@ -200,13 +216,12 @@ __global__ void k_zero(double* __restrict__ a, int64_t n) {
 ps_err one_step_joint(Ctx* c, NoisePipe* pipe, int s, double* noise_fixed);
 ps_err one_step(Ctx* c, NoisePipe* pipe, int s, double* noise_fixed = nullptr) {
     if (c->nmslot > 0) return one_step_joint(c, pipe, s, noise_fixed);
-    const int64_t n = c->n, nu = c->nu, n3 = 3 * (int64_t)c->Np, nc5 = 5 * (int64_t)c->Nc;
+    const int64_t n = c->n, nu = c->nu, nc5 = 5 * (int64_t)c->Nc;
     hipStream_t st = c->stream;
     const double dt = c->p.dt, kT = c->p.kT, eps = 1.0e-3;
     c->t += dt;
-    PS_TRY(vec_copy(c, c->x_n, c->x, n3));
-    // ---- predictor
-    PS_TRY(check_dist_dev(c));
+    // ---- predictor (x_n = x is saved by the same pass that bins the positions)
+    PS_TRY(check_dist_dev(c, c->x_n));
     PS_TRY(rhs_dev(c, 0));
     c->ar_gram_next = c->arrow;
     PS_TRY(mob_prepare(c));
@@ -230,9 +245,7 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s, double* noise_fixed = nullptr) {
         // drift(): RFD probes at x +/- eps W
         PS_HIP(c, hipMemcpyAsync(c->fext, noise + n, sizeof(double) * (size_t)nu, hipMemcpyHostToDevice, st));
         if (!noise_fixed) PS_TRY(pipe->release(s));
-        k_probe<<<eb(n3), TB, 0, st>>>(c->x, c->fext, eps, n3);
-        PS_LAUNCH(c);
-        PS_TRY(probe_geometry_dev(c));
+        PS_TRY(probe_move(c, eps));
         PS_TRY(mob_prepare(c));
         PS_TRY(rhs_dev(c, 1));
         PS_TRY(solve(c, c->Xd, 0, c->drift_v));
@@ -240,9 +253,7 @@ ps_err one_step(Ctx* c, NoisePipe* pipe, int s, double* noise_fixed = nullptr) {
             k_sb<<<eb(nc5), TB, 0, st>>>(c->sb, c->Xd, nu, nc5, 0);
             PS_LAUNCH(c);
         }
-        k_probe<<<eb(n3), TB, 0, st>>>(c->x, c->fext, -2.0 * eps, n3);
-        PS_LAUNCH(c);
-        PS_TRY(probe_geometry_dev(c));
+        PS_TRY(probe_move(c, -2.0 * eps));
         PS_TRY(mob_prepare(c));
         PS_TRY(rhs_dev(c, 1));
         PS_TRY(solve(c, c->Xd, 1, c->udiff));
@@ -319,7 +330,7 @@ bool joint_step_ok(const Ctx* c) {
 // tile passes from each to the next exactly as there, and each solve is the same block elimination
 // as one_step's; only their order differs.
 ps_err one_step_joint(Ctx* c, NoisePipe* pipe, int s, double* noise_fixed) {
-    const int64_t n = c->n, nu = c->nu, ns = c->ns, n3 = 3 * (int64_t)c->Np, nc5 = 5 * (int64_t)c->Nc;
+    const int64_t n = c->n, nu = c->nu, ns = c->ns, nc5 = 5 * (int64_t)c->Nc;
     hipStream_t A = c->stream, B = c->side;
     const double dt = c->p.dt, kT = c->p.kT, eps = 1.0e-3;
     const int prec = c->p.precision;
@@ -333,9 +344,9 @@ ps_err one_step_joint(Ctx* c, NoisePipe* pipe, int s, double* noise_fixed) {
         }
     } restore{c, A};
     c->t += dt;
-    PS_TRY(vec_copy(c, c->x_n, c->x, n3));
-    // ---- predictor point x: forces, mob() into slot 0, stage 1 of the deterministic solve
-    PS_TRY(check_dist_dev(c));
+    // ---- predictor point x: forces (x_n = x saved by the binning pass), mob() into slot 0, stage 1
+    // of the deterministic solve
+    PS_TRY(check_dist_dev(c, c->x_n));
     PS_TRY(rhs_dev(c, 0));
     mob_use_slot(c, 0);
     PS_TRY(mob_prepare(c));
@@ -346,24 +357,19 @@ ps_err one_step_joint(Ctx* c, NoisePipe* pipe, int s, double* noise_fixed) {
     PS_TRY(block_stage1(c, c->rhs, c->Xdet, bs0));
     // ---- noise: the slip normals, then the RFD normals (the reference's draw order)
     double* noise = noise_fixed ? noise_fixed : pipe->take(s);
-    PS_HIP(c, hipMemcpyAsync(c->xi, noise, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, A));
-    PS_HIP(c, hipMemcpyAsync(c->fext, noise + n, sizeof(double) * (size_t)nu, hipMemcpyHostToDevice, A));
+    PS_HIP(c, hipMemcpyAsync(c->xi, noise, sizeof(double) * (size_t)(n + nu), hipMemcpyHostToDevice, A));
     if (!noise_fixed) PS_TRY(pipe->release(s));
     PS_TRY(sqrt_recurrence_dev(c, c->xi));
     // ---- fork: the probes on the side stream
     PS_HIP(c, hipEventRecord(c->jev[0], A));
     PS_HIP(c, hipStreamWaitEvent(B, c->jev[0], 0));
     c->stream = B;
-    k_probe<<<eb(n3), TB, 0, B>>>(c->x, c->fext, eps, n3);
-    PS_LAUNCH(c);
-    PS_TRY(probe_geometry_dev(c));
+    PS_TRY(probe_move(c, eps));
     mob_use_slot(c, 1);
     PS_TRY(mob_prepare(c));
     PS_TRY(rhs_drift_dev(c, c->rhs_d));
     PS_TRY(block_stage1(c, c->rhs_d, c->Xd, bs2));
-    k_probe<<<eb(n3), TB, 0, B>>>(c->x, c->fext, -2.0 * eps, n3);
-    PS_LAUNCH(c);
-    PS_TRY(probe_geometry_dev(c));
+    PS_TRY(probe_move(c, -2.0 * eps));
     mob_use_slot(c, 2);
     PS_TRY(mob_prepare(c));
     PS_TRY(block_stage1(c, c->rhs_d, c->Xd2, bs3));
@@ -573,7 +579,7 @@ void destroy_ctx(Ctx* c) {
     if (c->dense_io) (void)hipFree(c->dense_io);
     for (void* p : {(void*)c->x, (void*)c->xpd, (void*)c->xm, (void*)c->xpos0, (void*)c->q, (void*)c->x_n,
                     (void*)c->chain, (void*)c->up, (void*)c->udiff, (void*)c->v_det_n, (void*)c->v_det_c,
-                    (void*)c->v_brown, (void*)c->fext, (void*)c->drift_v, (void*)c->sb, (void*)c->rhs,
+                    (void*)c->v_brown, (void*)c->drift_v, (void*)c->sb, (void*)c->rhs,
                     (void*)c->X, (void*)c->Xdet, (void*)c->Xd, (void*)c->W, (void*)c->xi, (void*)c->stale,
                     (void*)c->io_a, (void*)c->io_b})
         if (p) (void)hipFree(p);
@@ -778,14 +784,17 @@ ps_err ps_create(const ps_params* p, void** out) {
     bool ok = dalloc(&c->x, n3) && dalloc(&c->xpd, n3) && dalloc(&c->xm, n3) && dalloc(&c->xpos0, 3 * (size_t)c->Nc) &&
               dalloc(&c->q, 4 * (size_t)c->Nc) && dalloc(&c->x_n, n3) && dalloc(&c->up, c->nu) &&
               dalloc(&c->udiff, c->nu) && dalloc(&c->v_det_n, c->nu) && dalloc(&c->v_det_c, c->nu) &&
-              dalloc(&c->v_brown, c->nu) && dalloc(&c->fext, c->nu) && dalloc(&c->drift_v, c->nu) &&
+              dalloc(&c->v_brown, c->nu) && dalloc(&c->drift_v, c->nu) &&
               dalloc(&c->sb, 5 * (size_t)c->Nc) && dalloc(&c->rhs, c->ns) && dalloc(&c->X, c->ns) &&
-              dalloc(&c->Xdet, c->ns) && dalloc(&c->Xd, c->ns) && dalloc(&c->W, c->n) && dalloc(&c->xi, c->n) &&
+              dalloc(&c->Xdet, c->ns) && dalloc(&c->Xd, c->ns) && dalloc(&c->W, c->n) && dalloc(&c->xi, c->n + c->nu) &&
               dalloc(&c->stale, 18) && dalloc(&c->io_a, c->ns) && dalloc(&c->io_b, c->ns);
     if (!ok) {
         c->err = "device allocation failed";
         return bail(PS_ERR_ALLOC);
     }
+    // one step's noise as drawn: the slip normals (n), then the RFD normals (nu), so that one copy
+    // moves both
+    c->fext = c->xi + c->n;
     if (hipEventCreateWithFlags(&c->noise_ev[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->noise_ev[1], hipEventDisableTiming) != hipSuccess) {
         c->err = "event creation failed";

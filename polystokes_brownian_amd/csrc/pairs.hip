@@ -59,14 +59,19 @@ __device__ __forceinline__ int cell_coord(const PairConst& pc, double x, int d) 
 }
 
 // Bin particle p by the positions x, which become this check_dist's geometry snapshot xpd.
-__global__ void k_bin(PairConst pc, const double* __restrict__ x, double* __restrict__ xpd, int* __restrict__ cell_of,
-                      int* __restrict__ cell_count) {
+__global__ void k_bin(PairConst pc, const double* __restrict__ x, double* __restrict__ xpd,
+                      double* __restrict__ xsave, int* __restrict__ cell_of, int* __restrict__ cell_count) {
     const int p = blockIdx.x * TB + threadIdx.x;
     if (p >= pc.s.Np) return;
     const double px = x[3 * p], py = x[3 * p + 1], pz = x[3 * p + 2];
     xpd[3 * p] = px;
     xpd[3 * p + 1] = py;
     xpd[3 * p + 2] = pz;
+    if (xsave) {
+        xsave[3 * p] = px;
+        xsave[3 * p + 1] = py;
+        xsave[3 * p + 2] = pz;
+    }
     const int cx = cell_coord(pc, px, 0), cy = cell_coord(pc, py, 1), cz = cell_coord(pc, pz, 2);
     const int cell = cx + pc.ncell[0] * (cy + pc.ncell[1] * cz);
     cell_of[p] = cell;
@@ -74,15 +79,16 @@ __global__ void k_bin(PairConst pc, const double* __restrict__ x, double* __rest
 }
 
 // Scatter particles into cell order: perm[slot] = p and a cell-ordered copy of the positions, so a
-// cell's candidates are read contiguously.  The in-cell order depends on atomics; rows are sorted by
-// partner afterwards, so the pair lists do not.
+// cell's candidates are read contiguously.  The cell counts are the cursors, counted down, so they
+// are all zero again afterwards, ready for the next binning.  The in-cell order depends on atomics;
+// rows are sorted by partner afterwards, so the pair lists do not.
 __global__ void k_scatter(int Np, const double* __restrict__ xs, const int* __restrict__ cell_of,
                           const int* __restrict__ cell_start, int* __restrict__ cursor, int* __restrict__ perm,
                           double* __restrict__ xsort) {
     const int p = blockIdx.x * TB + threadIdx.x;
     if (p >= Np) return;
     const int c = cell_of[p];
-    const int slot = cell_start[c] + atomicAdd(&cursor[c], 1);
+    const int slot = cell_start[c] + atomicSub(&cursor[c], 1) - 1;
     perm[slot] = p;
     xsort[3 * slot] = xs[3 * p];
     xsort[3 * slot + 1] = xs[3 * p + 1];
@@ -288,11 +294,6 @@ __global__ __launch_bounds__(TB) void k_nb_fill(PairConst pc, const double* __re
     }
 }
 
-__global__ void k_zero_int(int* __restrict__ a, int n) {
-    const int i = blockIdx.x * TB + threadIdx.x;
-    if (i < n) a[i] = 0;
-}
-
 // ---------------------------------------------------------------------------- forces
 struct ForceConst {
     SysConst s;
@@ -483,6 +484,8 @@ ps_err pairs_alloc(Ctx* c) {
     PS_HIP(c, hipMalloc(&c->cell_of, sizeof(int) * std::max(Np, 1)));
     PS_HIP(c, hipMalloc(&c->cell_start, sizeof(int) * (ncells + 2)));
     PS_HIP(c, hipMalloc(&c->cell_count, sizeof(int) * ncells));
+    // zero here once: every check_dist leaves the counts at zero (the scatter counts them down)
+    PS_HIP(c, hipMemset(c->cell_count, 0, sizeof(int) * ncells));
     PS_HIP(c, hipMalloc(&c->cell_perm, sizeof(int) * std::max(Np, 1)));
     PS_HIP(c, hipMalloc(&c->cell_xs, sizeof(double) * 3 * std::max(Np, 1)));
     PS_HIP(c, hipMalloc(&c->nb_count, sizeof(int) * std::max(Np, 1)));
@@ -559,25 +562,27 @@ ps_err scan_ints(Ctx* c, const int* in, int* out, int n) {
 // DC_VLIST_OVERFLOW flag, which the entry points report as an error.
 constexpr int VL_ROW_CAP = 511;
 
-ps_err check_dist_dev(Ctx* c) {
-    if (c->arrow) return arrow_check_dist(c);
+ps_err check_dist_dev(Ctx* c, double* xsave) {
+    if (c->arrow) {
+        if (xsave) PS_TRY(vec_copy(c, xsave, c->x, 3 * (int64_t)c->Np));
+        return arrow_check_dist(c);
+    }
     const int Np = c->Np;
     hipStream_t st = c->stream;
     PairConst pc = make_pc(c);
     // geometry of this check_dist = current positions (copied by k_bin in cell mode)
     const unsigned gp = (unsigned)((Np + TB - 1) / TB);
     const double* xsort = c->xpd;
-    if (c->cell_mode != 1)
+    if (c->cell_mode != 1) {
         PS_HIP(c, hipMemcpyAsync(c->xpd, c->x, sizeof(double) * 3 * (size_t)Np, hipMemcpyDeviceToDevice, st));
+        if (xsave) PS_TRY(vec_copy(c, xsave, c->x, 3 * (int64_t)Np));
+    }
     if (c->cell_mode == 1) {
         const int ncells = c->ncell[0] * c->ncell[1] * c->ncell[2];
-        const unsigned gc = (unsigned)((ncells + TB - 1) / TB);
-        k_zero_int<<<gc, TB, 0, st>>>(c->cell_count, ncells);
+        // the counts are zero on entry (k_scatter leaves them so)
+        k_bin<<<gp, TB, 0, st>>>(pc, c->x, c->xpd, xsave, c->cell_of, c->cell_count);
         PS_LAUNCH(c);
-        k_bin<<<gp, TB, 0, st>>>(pc, c->x, c->xpd, c->cell_of, c->cell_count);
-        PS_LAUNCH(c);
-        // the scan clears the counts behind it: they are the scatter's cursors next
-        k_scan<<<1, SCAN_T, 0, st>>>(c->cell_count, c->cell_start, ncells, 1);
+        k_scan<<<1, SCAN_T, 0, st>>>(c->cell_count, c->cell_start, ncells, 0);
         PS_LAUNCH(c);
         k_scatter<<<gp, TB, 0, st>>>(Np, c->xpd, c->cell_of, c->cell_start, c->cell_count, c->cell_perm, c->cell_xs);
         PS_LAUNCH(c);

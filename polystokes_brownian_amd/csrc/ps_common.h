@@ -299,7 +299,8 @@ ps_err saddle_solve_dev(Ctx* c, const double* b, double* x, int warm, ps_solve_i
 ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* w, ps_sqrt_info* info);
 ps_err pairs_alloc(Ctx* c);
 void pairs_free(Ctx* c);
-ps_err check_dist_dev(Ctx* c);
+// check_dist on the device; with xsave, the positions it reads are also copied there
+ps_err check_dist_dev(Ctx* c, double* xsave = nullptr);
 ps_err probe_geometry_dev(Ctx* c);
 ps_err rhs_dev(Ctx* c, int drift);
 ps_err rhs_drift_dev(Ctx* c, double* out);
