@@ -148,9 +148,9 @@ __global__ __launch_bounds__(PS_BLOCK) void k_aa(SysConst s, const T* __restrict
         }
     }
     if (act) {
-        part[((int64_t)blockIdx.y * 3 + 0) * Nm + i] = ux;
-        part[((int64_t)blockIdx.y * 3 + 1) * Nm + i] = uy;
-        part[((int64_t)blockIdx.y * 3 + 2) * Nm + i] = uz;
+        part[((int64_t)blockIdx.y * Nm + i) * 3 + 0] = ux;
+        part[((int64_t)blockIdx.y * Nm + i) * 3 + 1] = uy;
+        part[((int64_t)blockIdx.y * Nm + i) * 3 + 2] = uz;
     }
 }
 
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(PS_BLOCK) void k_aa(SysConst s, const T* __restrict
 // accumulator per source at the end of each window; the waves visit the windows in staggered
 // phases (window (w + p) mod 4 in phase p, a barrier between phases), so no two waves touch one
 // window at once and every accumulator sums in a fixed order.
-// Diagonal blocks (I == J) run the ordered loop.  Output: part[(J 3 + c) Nm + i] is the
+// Diagonal blocks (I == J) run the ordered loop.  Output: part[(J Nm + i) 3 + c] is the
 // contribution of source tile J to target i, written exactly once per (J, i).
 //
 // Positions.  FP64, and FP32 without a box: x / L in the compute type (L = 1 without a box), so the
@@ -322,9 +322,9 @@ __device__ __forceinline__ void aa_sym_tile(int Nm, int I, int J, const AAConst&
             }
         }
         if (!PAD || i < Nm) {
-            part[((int64_t)J * 3 + 0) * Nm + i] = ux;
-            part[((int64_t)J * 3 + 1) * Nm + i] = uy;
-            part[((int64_t)J * 3 + 2) * Nm + i] = uz;
+            part[((int64_t)J * Nm + i) * 3 + 0] = ux;
+            part[((int64_t)J * Nm + i) * 3 + 1] = uy;
+            part[((int64_t)J * Nm + i) * 3 + 2] = uz;
         }
         return;
     }
@@ -381,14 +381,14 @@ __device__ __forceinline__ void aa_sym_tile(int Nm, int I, int J, const AAConst&
         __syncthreads();
     }
     if (!PAD || i < Nm) {
-        part[((int64_t)J * 3 + 0) * Nm + i] = ux;
-        part[((int64_t)J * 3 + 1) * Nm + i] = uy;
-        part[((int64_t)J * 3 + 2) * Nm + i] = uz;
+        part[((int64_t)J * Nm + i) * 3 + 0] = ux;
+        part[((int64_t)J * Nm + i) * 3 + 1] = uy;
+        part[((int64_t)J * Nm + i) * 3 + 2] = uz;
     }
     const int j = J * PS_BLOCK + t;
     if (!PAD || j < Nm) {
 #pragma unroll
-        for (int c = 0; c < 3; c++) part[((int64_t)I * 3 + c) * Nm + j] = A[c][t];
+        for (int c = 0; c < 3; c++) part[((int64_t)I * Nm + j) * 3 + c] = A[c][t];
     }
 }
 
@@ -497,9 +497,9 @@ __device__ __forceinline__ void ab_mono_block(int bx, int by, const SysConst& s,
         ux += (T)stale[0] * Fx + (T)stale[1] * Fy + (T)stale[2] * Fz;
         uy += (T)stale[3] * Fx + (T)stale[4] * Fy + (T)stale[5] * Fz;
         uz += (T)stale[6] * Fx + (T)stale[7] * Fy + (T)stale[8] * Fz;
-        part[((int64_t)by * 3 + 0) * Nm + i] = ux;
-        part[((int64_t)by * 3 + 1) * Nm + i] = uy;
-        part[((int64_t)by * 3 + 2) * Nm + i] = uz;
+        part[((int64_t)by * Nm + i) * 3 + 0] = ux;
+        part[((int64_t)by * Nm + i) * 3 + 1] = uy;
+        part[((int64_t)by * Nm + i) * 3 + 2] = uz;
     }
 }
 
@@ -774,8 +774,8 @@ __device__ __forceinline__ double finalize_row(const SysConst& s, const double* 
         double acc = 0.0;
         // the loads are independent: unrolled so that several are in flight (the sum order is fixed)
 #pragma unroll 8
-        for (int sp = 0; sp < a.nsplit; sp++) acc += (double)a.part_aa[((int64_t)sp * 3 + comp) * Nm + i];
-        for (int cm = 0; cm < a.nchunk_m; cm++) acc += (double)a.part_abm[((int64_t)cm * 3 + comp) * Nm + i];
+        for (int sp = 0; sp < a.nsplit; sp++) acc += (double)a.part_aa[((int64_t)sp * Nm + i) * 3 + comp];
+        for (int cm = 0; cm < a.nchunk_m; cm++) acc += (double)a.part_abm[((int64_t)cm * Nm + i) * 3 + comp];
         return s.beta_inv * f[r] + acc;
     } else if (r < nm3 + nc3) {
         const int64_t q = r - nm3;

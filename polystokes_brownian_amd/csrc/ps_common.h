@@ -76,8 +76,8 @@ struct ApplyWork {
     double inv_L = 1.0;
     int nsplit_aa = 1, nchunk_c = 1, nchunk_m = 1;
     int per_split_aa = 0, per_chunk_c = 0, per_chunk_m = 0;
-    void* part_aa = nullptr;     // [nsplit][3][Nm]  (T)
-    void* part_abm = nullptr;    // [nchunk_m][3][Nm] (T), colloid-source chunks
+    void* part_aa = nullptr;     // [nsplit][Nm][3]  (T): a monomer row r = 3 i + c at [sp][r]
+    void* part_abm = nullptr;    // [nchunk_m][Nm][3] (T), colloid-source chunks
     void* part_c = nullptr;      // [nchunk][Nc][12] (T) (11 rows + monomer-force sum slot reuse)
     void* part_fs = nullptr;     // [nchunk][3]      (T) sum of monomer forces per chunk
     double* bb = nullptr;        // [Nc][11]
