@@ -302,8 +302,11 @@ __global__ __launch_bounds__(LZB) void k_lz_sub_dots(int64_t n, const double* __
 #pragma unroll 8
     for (int q = 0; q < m; q++) wi -= h[q] * Q[(int64_t)q * n + rc];
     if (in) w[r] = wi;
+    // rows past n read row 0 of Q (finite) times zero; no branch around the load, so the eight of a
+    // batch are in flight together
+    const double wz = in ? wi : 0.0;
     lzd_block_dots(
-        m, [&](int q) { return in ? Q[(int64_t)q * n + rc] * wi : 0.0; }, sh, red2, nb);
+        m, [&](int q) { return Q[(int64_t)q * n + rc] * wz; }, sh, red2, nb);
 }
 __global__ __launch_bounds__(LZB) void k_lz_sub_norm(int64_t n, const double* __restrict__ Q, int m,
                                                      double* __restrict__ w, const double* __restrict__ red2, int nb,

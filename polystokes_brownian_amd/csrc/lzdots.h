@@ -31,11 +31,16 @@ __device__ __forceinline__ void lzd_block_dots(int m, F&& contrib, double (*sh)[
                                                int nb) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     // eight terms at a time, all loaded before any is summed (the index clamped to m - 1, so that the
-    // loads need no branch; the padded terms are not summed)
-    for (int q0 = 0; q0 < m; q0 += 8) {
-        double c[8];
+    // loads need no branch; the padded terms are not summed), the next eight loaded while these are
+    // summed
+    double c[8], nx[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) c[k] = contrib(min(q0 + k, m - 1));
+    for (int k = 0; k < 8; k++) c[k] = contrib(min(k, m - 1));
+    for (int q0 = 0; q0 < m; q0 += 8) {
+        if (q0 + 8 < m) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) nx[k] = contrib(min(q0 + 8 + k, m - 1));
+        }
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             if (q0 + k < m) {
@@ -43,6 +48,8 @@ __device__ __forceinline__ void lzd_block_dots(int m, F&& contrib, double (*sh)[
                 if (lane == 0) sh[q0 + k][wid] = s;
             }
         }
+#pragma unroll
+        for (int k = 0; k < 8; k++) c[k] = nx[k];
     }
     __syncthreads();
     if ((int)threadIdx.x < m) {

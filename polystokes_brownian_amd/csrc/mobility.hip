@@ -868,13 +868,11 @@ __global__ __launch_bounds__(LZB) void k_finalize_lz(SysConst s, const double* _
         if (qout) qout[r] = qr;
     }
     const int64_t rc = in ? r : 0;
+    // Q_{m-1} is q, which this thread has just stored for its row; rows past n read row 0 of Q (a
+    // finite value) times yr = 0.  No branch around the load, so the eight of a batch are in flight
+    // together.
     lzd_block_dots(
-        m,
-        [&](int q) {
-            const double qv = q == m - 1 ? qr : Q[(int64_t)q * n + rc];
-            return in ? qv * yr : 0.0;
-        },
-        sh, red1, nb);
+        m, [&](int q) { return Q[(int64_t)q * n + rc] * yr; }, sh, red1, nb);
 }
 
 // ---------------------------------------------------------------------------- carried tile
