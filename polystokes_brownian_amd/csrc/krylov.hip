@@ -321,7 +321,8 @@ __global__ __launch_bounds__(LZB) void k_lz_sub_norm(int64_t n, const double* __
 #pragma unroll 8
     for (int q = 0; q < m; q++) wi -= h[q] * Q[(int64_t)q * n + rc];
     if (in) w[r] = wi;
-    lzd_block_dots(1, [&](int) { return wi * wi; }, sh, red3, nb);
+    const double wz = in ? wi : 0.0;   // rows past n hold -sum_q h_q Q_q[0], not part of w
+    lzd_block_dots(1, [&](int) { return wz * wz; }, sh, red3, nb);
 }
 inline unsigned eblocks(int64_t n) { return (unsigned)((n + VB - 1) / VB); }
 
