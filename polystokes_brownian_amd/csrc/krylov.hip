@@ -1448,13 +1448,15 @@ __device__ __forceinline__ int lz_mlim(const double* __restrict__ lz, int kmax, 
 }
 
 // The square-root weights g_m = beta0 U diag(sqrt(max(theta, 0))) U^T e1 of T_m = U diag(theta) U^T
-// (src/slip_vel.cpp:427-459) for m = blockIdx.x + 1, one wave each, all m in parallel: the
-// recurrence ran ahead to kmax, and the stopping test (k_lz_final) needs the weights of every m up
-// to the one where it stops.  Without the test only the last m is solved.
-__global__ __launch_bounds__(64) void k_lz_ritz(const double* __restrict__ lz, int kmax, int every_m,
-                                               double* __restrict__ G, double* __restrict__ negs) {
-    __shared__ double d[64], Z[64 * 64];
-    const int lane = threadIdx.x;
+// (src/slip_vel.cpp:427-459) for m = blockIdx.x + 1, one block of two waves each (tql2_pair: the QL
+// sweeps on one wave, the eigenvector rows on the other), all m in parallel: the recurrence ran
+// ahead to kmax, and the stopping test (k_lz_final) needs the weights of every m up to the one where
+// it stops.  Without the test only the last m is solved.
+__global__ __launch_bounds__(128) void k_lz_ritz(const double* __restrict__ lz, int kmax, int every_m,
+                                                double* __restrict__ G, double* __restrict__ negs) {
+    __shared__ Tql2Pair sh;
+    __shared__ double Z[64 * 64];
+    const int lane = threadIdx.x & 63;
     const int m = blockIdx.x + 1;
     const double beta0 = sqrt(fmax(lz[LZ_XI2], 0.0));
     if (!(beta0 > 0.0)) return;
@@ -1466,25 +1468,24 @@ __global__ __launch_bounds__(64) void k_lz_ritz(const double* __restrict__ lz, i
         if (i < m - 1) finite = finite && isfinite(lz[LZ_BETA2 + i]);
     }
     if (!finite) {   // a broken recurrence: no QL sweeps over NaNs, the weights say so instead
-        if (lane < m) G[(m - 1) * LZ_G + lane] = __builtin_nan("");
-        if (lane == 0) negs[m - 1] = 0.0;
+        if (threadIdx.x < m) G[(m - 1) * LZ_G + lane] = __builtin_nan("");
+        if (threadIdx.x == 0) negs[m - 1] = 0.0;
         return;
     }
-    double dv = lane < m ? lz[lane] : 0.0;
-    double ev = (lane < m - 1) ? sqrt(fmax(lz[LZ_BETA2 + lane], 0.0)) : 0.0;
-    tql2_wave<64>(m, dv, ev, Z);
-    if (lane < m) d[lane] = dv;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const double dv = lane < m ? lz[lane] : 0.0;
+    const double ev = (lane < m - 1) ? sqrt(fmax(lz[LZ_BETA2 + lane], 0.0)) : 0.0;
+    tql2_pair(m, dv, ev, sh, Z);
+    if (threadIdx.x >= 64) return;
     double gi = 0.0;
     if (lane < m)
         for (int jj = 0; jj < m; jj++) {
-            double th = d[jj];
+            double th = sh.de[jj].x;
             if (th < 0.0) th = 0.0;
             gi += Z[lane + jj * 64] * sqrt(th) * Z[0 + jj * 64];
         }
     int nneg = 0;
     for (int jj = 0; jj < m; jj++)
-        if (d[jj] < 0.0) nneg++;
+        if (sh.de[jj].x < 0.0) nneg++;
     if (lane < m) G[(m - 1) * LZ_G + lane] = beta0 * gi;
     if (lane == 0) negs[m - 1] = (double)nneg;
 }
@@ -1609,7 +1610,7 @@ ps_err sqrt_finish_dev(Ctx* c, double* out, ps_sqrt_info* info) {
     const int kmax = c->kmax;
     const double rtol = c->p.sqrt_rtol;
     double* negs = c->lzg + LZ_G * 64;
-    k_lz_ritz<<<kmax, 64, 0, st>>>(c->lz, kmax, rtol > 0.0 ? 1 : 0, c->lzg, negs);
+    k_lz_ritz<<<kmax, 128, 0, st>>>(c->lz, kmax, rtol > 0.0 ? 1 : 0, c->lzg, negs);
     PS_LAUNCH(c);
     k_lz_final<<<ne, VB, 0, st>>>(n, c->Q, c->lz, kmax, rtol, c->lzg, negs, out, c->lzinfo, c->dcnt);
     PS_LAUNCH(c);
