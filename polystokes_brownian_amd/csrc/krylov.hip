@@ -1440,11 +1440,13 @@ __global__ void k_lz_start(int64_t n, const double* __restrict__ xi, double* __r
 }
 
 // The number of Lanczos vectors before the stopping test: the first j with beta_j <= 1e-12 beta_0
-// (an invariant subspace, src/slip_vel.cpp:417) gives m = j + 1, else m = kmax.
+// (an invariant subspace, src/slip_vel.cpp:417) gives m = j + 1, else m = kmax (<= 64).  Lane j
+// tests beta_j; every lane of the calling wave must be active.
 __device__ __forceinline__ int lz_mlim(const double* __restrict__ lz, int kmax, double beta0) {
-    for (int j = 0; j < kmax - 1; j++)
-        if (sqrt(fmax(lz[LZ_BETA2 + j], 0.0)) <= 1e-12 * beta0) return j + 1;
-    return kmax;
+    const int j = threadIdx.x & 63;
+    const bool stop = j < kmax - 1 && sqrt(fmax(lz[LZ_BETA2 + j], 0.0)) <= 1e-12 * beta0;
+    const unsigned long long mask = __ballot(stop);
+    return mask ? (int)__builtin_ctzll(mask) + 1 : kmax;
 }
 
 // The square-root weights g_m = beta0 U diag(sqrt(max(theta, 0))) U^T e1 of T_m = U diag(theta) U^T
@@ -1462,12 +1464,8 @@ __global__ __launch_bounds__(128) void k_lz_ritz(const double* __restrict__ lz, 
     if (!(beta0 > 0.0)) return;
     const int mlim = lz_mlim(lz, kmax, beta0);
     if (m > mlim || (!every_m && m != mlim)) return;
-    bool finite = true;
-    for (int i = 0; i < m; i++) {
-        finite = finite && isfinite(lz[i]);
-        if (i < m - 1) finite = finite && isfinite(lz[LZ_BETA2 + i]);
-    }
-    if (!finite) {   // a broken recurrence: no QL sweeps over NaNs, the weights say so instead
+    const bool bad = (lane < m && !isfinite(lz[lane])) || (lane < m - 1 && !isfinite(lz[LZ_BETA2 + lane]));
+    if (__ballot(bad)) {   // a broken recurrence: no QL sweeps over NaNs, the weights say so instead
         if (threadIdx.x < m) G[(m - 1) * LZ_G + lane] = __builtin_nan("");
         if (threadIdx.x == 0) negs[m - 1] = 0.0;
         return;
@@ -1492,34 +1490,43 @@ __global__ __launch_bounds__(128) void k_lz_ritz(const double* __restrict__ lz, 
 
 // Stopping test and recombination.  m* is the first m >= 2 whose relative change
 // |g_m - [g_{m-1}; 0]| / |g_m| is at most rtol (rtol > 0), else the m of lz_mlim; out = Q g_{m*}.
-// Every block finds m* itself (wave 0, a few hundred flops); block 0 records it, the negative Ritz
-// values and the last relative change in info and adds them to the device counters.
+// Every block finds m* itself (the weights staged in LDS, one lane per m, the first m that passes
+// by ballot); block 0 records it, the negative Ritz values and the last relative change in info and
+// adds them to the device counters.
 __global__ __launch_bounds__(VB) void k_lz_final(int64_t n, const double* __restrict__ Q, const double* __restrict__ lz,
                                                  int kmax, double rtol, const double* __restrict__ G,
                                                  const double* __restrict__ negs, double* __restrict__ out,
                                                  double* __restrict__ info, int64_t* __restrict__ dcnt) {
     __shared__ double sg[64];
+    __shared__ double sG[64 * 65];   // rows m - 1 of G, stride 65 (lanes reading one row each spread over the banks)
     __shared__ int sm;
+    const double beta0 = sqrt(fmax(lz[LZ_XI2], 0.0));
+    const int mlim = beta0 > 0.0 ? lz_mlim(lz, kmax, beta0) : 0;
+    const bool test = rtol > 0.0 && mlim >= 2;
+    if (test) {
+        for (int k = threadIdx.x; k < mlim * LZ_G; k += VB) sG[(k / LZ_G) * 65 + k % LZ_G] = G[k];
+        __syncthreads();
+    }
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
-        const double beta0 = sqrt(fmax(lz[LZ_XI2], 0.0));
-        int mstar = 0;
+        int mstar = mlim;
         double relc = 0.0;
-        if (beta0 > 0.0) {
-            const int mlim = lz_mlim(lz, kmax, beta0);
-            mstar = mlim;
-            if (rtol > 0.0)
-                for (int m = 2; m <= mlim; m++) {
-                    const double gi = lane < m ? G[(m - 1) * LZ_G + lane] : 0.0;
-                    const double gp = lane < m - 1 ? G[(m - 2) * LZ_G + lane] : 0.0;
-                    const double dn = wave_sum((gi - gp) * (gi - gp));
-                    const double gn = wave_sum(gi * gi);
-                    relc = sqrt(dn / gn);
-                    if (relc <= rtol) {
-                        mstar = m;
-                        break;
-                    }
+        if (test) {
+            // lane j: the relative change at m = j + 2; m* is the first m that passes
+            const int m = lane + 2;
+            double dn = 0.0, gn = 0.0;
+            if (m <= mlim)
+                for (int k = 0; k < m; k++) {
+                    const double gi = sG[(m - 1) * 65 + k];
+                    const double gp = k < m - 1 ? sG[(m - 2) * 65 + k] : 0.0;
+                    dn += (gi - gp) * (gi - gp);
+                    gn += gi * gi;
                 }
+            const double rc = m <= mlim ? sqrt(dn / gn) : 0.0;
+            const unsigned long long pass = __ballot(m <= mlim && rc <= rtol);
+            const int j = pass ? (int)__builtin_ctzll(pass) : mlim - 2;
+            mstar = j + 2;
+            relc = __shfl(rc, j, 64);
         }
         sg[lane] = (lane < mstar) ? G[(mstar - 1) * LZ_G + lane] : 0.0;
         if (lane == 0) {
