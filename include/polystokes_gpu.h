@@ -216,6 +216,15 @@ ps_err ps_saddle_solve_dev(void* ctx, const double* b_dev, double* x_dev, int wa
  * two-pass full reorthogonalisation and negative Ritz values floored (src/slip_vel.cpp:351-477). */
 ps_err ps_sqrt_apply(void* ctx, const double* xi, double* w, ps_sqrt_info* info);
 ps_err ps_sqrt_apply_dev(void* ctx, const double* xi_dev, double* w_dev, ps_sqrt_info* info);
+/* The square-root weights g = beta0 U diag(sqrt(max(theta, 0))) U^T e1 of one symmetric tridiagonal
+ * T = U diag(theta) U^T (diagonal alpha[m], off-diagonal beta[m - 1]; 1 <= m <= 64), computed on the
+ * device by the eigensolver ps_sqrt_apply runs after its recurrence, from host arrays (the current
+ * device; a blocking call).  ~ the LAPACK dsyev step of schur_sqrt_lanczos (src/slip_vel.cpp:427-459).
+ * As in the recurrence, beta enters squared (its sign is dropped) and T is cut after the first
+ * beta_j <= 1e-12 beta0: *m_used is the order solved, g[m_used .. m) are zero, *nneg counts the
+ * negative eigenvalues floored to zero. */
+ps_err ps_tridiag_sqrt(int m, const double* alpha, const double* beta, double beta0, double* g, int* m_used,
+                       int* nneg);
 /* RHS(drift) (src/rhs.cpp:160-260): rebuild the saddle right-hand side on the device from the
  * current Verlet list; rhs (ns entries) is copied back when non-NULL. */
 ps_err ps_rhs(void* ctx, int drift, double* rhs);

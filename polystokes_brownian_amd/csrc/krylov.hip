@@ -1554,6 +1554,52 @@ __global__ __launch_bounds__(VB) void k_lz_final(int64_t n, const double* __rest
     out[i] = acc;
 }
 
+// One tridiagonal's weights on the device from host arrays: k_lz_ritz on a recurrence record that
+// holds only alpha, beta^2 and beta0^2 (the solve the Lanczos square root runs after its recurrence).
+extern "C" ps_err ps_tridiag_sqrt(int m, const double* alpha, const double* beta, double beta0, double* g,
+                                  int* m_used, int* nneg) {
+    if (m < 1 || m > 64 || !alpha || !g || !m_used || !nneg || (m > 1 && !beta) || !(beta0 >= 0.0)) {
+        ps_set_context_free_error("ps_tridiag_sqrt: m must be in 1..64, alpha, beta (m - 1 entries), g, m_used "
+                                  "and nneg given, beta0 >= 0");
+        return PS_ERR_ARG;
+    }
+    double lz[3 * 64 + 8] = {};
+    for (int i = 0; i < m; i++) lz[i] = alpha[i];
+    for (int i = 0; i + 1 < m; i++) lz[LZ_BETA2 + i] = beta[i] * beta[i];
+    lz[LZ_XI2] = beta0 * beta0;
+    int ml = m;   // lz_mlim's cut, on the host
+    for (int j = 0; j + 1 < m; j++)
+        if (sqrt(fmax(lz[LZ_BETA2 + j], 0.0)) <= 1e-12 * beta0) {
+            ml = j + 1;
+            break;
+        }
+    double *dlz = nullptr, *dg = nullptr;
+    if (hipMalloc(&dlz, sizeof(lz)) != hipSuccess || hipMalloc(&dg, sizeof(double) * (LZ_G + 1) * 64) != hipSuccess) {
+        if (dlz) (void)hipFree(dlz);
+        ps_set_context_free_error("ps_tridiag_sqrt: device allocation failed");
+        return PS_ERR_ALLOC;
+    }
+    double row[64], nn = 0.0;
+    hipError_t e = hipMemcpy(dlz, lz, sizeof(lz), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(dg, 0, sizeof(double) * (LZ_G + 1) * 64);
+    if (e == hipSuccess) {
+        k_lz_ritz<<<m, 128>>>(dlz, m, 0, dg, dg + LZ_G * 64);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(row, dg + (ml - 1) * LZ_G, sizeof(double) * 64, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&nn, dg + LZ_G * 64 + (ml - 1), sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(dlz);
+    (void)hipFree(dg);
+    if (e != hipSuccess) {
+        ps_set_context_free_error(std::string("ps_tridiag_sqrt: ") + hipGetErrorString(e));
+        return PS_ERR_HIP;
+    }
+    for (int i = 0; i < m; i++) g[i] = i < ml ? row[i] : 0.0;
+    *m_used = ml;
+    *nneg = (int)nn;
+    return PS_OK;
+}
+
 // w = M^{1/2} xi by Lanczos (schur_sqrt_lanczos, src/slip_vel.cpp:351-477, applied to the whole grand
 // mobility) with no host round trip: the recurrence runs kmax iterations on the device (two
 // classical Gram-Schmidt passes each), the weights of every m come from one-wave QL solves, and the

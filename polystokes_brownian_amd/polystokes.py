@@ -52,6 +52,7 @@ def lib():
         for name in ("ps_mob_mult", "ps_mob_mult_dev", "ps_saddle_mult", "ps_saddle_mult_dev"):
             getattr(L, name).argtypes = [vp, C.c_void_p, C.c_void_p]
         L.ps_mob_mult_noalloc.argtypes = [C.POINTER(PsParams), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ps_tridiag_sqrt.argtypes = [C.c_int, dp, dp, C.c_double, dp, C.POINTER(ip), C.POINTER(ip)]
         L.ps_saddle_diag.argtypes = [vp, dp]
         L.ps_saddle_solve.argtypes = [vp, C.c_void_p, C.c_void_p, ip, C.POINTER(PsSolveInfo)]
         L.ps_saddle_solve_dev.argtypes = [vp, C.c_void_p, C.c_void_p, ip, C.POINTER(PsSolveInfo)]
@@ -354,6 +355,19 @@ def mob_mult_noalloc(params, x_ptr, f_ptr, u_ptr, stream=0):
     rc = lib().ps_mob_mult_noalloc(C.byref(params), x_ptr, f_ptr, u_ptr, stream or None)
     if rc != PS_OK:
         raise PsError(rc, lib().ps_last_error(None).decode())
+
+
+def tridiag_sqrt(alpha, beta, beta0):
+    """ps_tridiag_sqrt: the square-root weights beta0 U sqrt(max(theta, 0)) U^T e1 of the tridiagonal
+    (alpha, beta) on the device.  Returns (g, m_used, nneg)."""
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    b = np.ascontiguousarray(beta if len(beta) else np.zeros(1), dtype=np.float64)
+    g = np.zeros(len(a))
+    mu, nn = C.c_int(0), C.c_int(0)
+    rc = lib().ps_tridiag_sqrt(len(a), _p(a), _p(b), float(beta0), _p(g), C.byref(mu), C.byref(nn))
+    if rc != PS_OK:
+        raise PsError(rc, lib().ps_last_error(None).decode())
+    return g, mu.value, nn.value
 
 
 class Box:
