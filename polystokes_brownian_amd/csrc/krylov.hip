@@ -1602,7 +1602,7 @@ extern "C" ps_err ps_tridiag_sqrt(int m, const double* alpha, const double* beta
 
 // w = M^{1/2} xi by Lanczos (schur_sqrt_lanczos, src/slip_vel.cpp:351-477, applied to the whole grand
 // mobility) with no host round trip: the recurrence runs kmax iterations on the device (two
-// classical Gram-Schmidt passes each), the weights of every m come from one-wave QL solves, and the
+// classical Gram-Schmidt passes each), the weights of every m come from two-wave QL solves, and the
 // stopping test picks the m the sequential test would have stopped at, so the result equals
 // stopping there; vectors built past it are never used.  With `info` the outcome is read back.
 ps_err sqrt_apply_dev(Ctx* c, const double* xi, double* out, ps_sqrt_info* info) {
